@@ -1,0 +1,391 @@
+"""Seeded synthetic cases: polyMesh generators, initial fields, boundary
+conditions and OpenFOAM-syntax dictionaries for the five workloads the
+implicit coupled pseudo-step is measured on.
+
+Meshes are written the way OpenFOAM's blockMesh writes them
+(owner < neighbour, internal faces in upper-triangular order, boundary
+faces grouped per patch, face normals pointing out of the owner), which is
+the addressing ICSFoam's block matrices rely on
+(src/blockFvMatrix/blockFvMatrix.C:220-234).
+
+Everything here is input generation; no solver arithmetic lives here.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+# patch kinds understood by the library (mirrors OpenFOAM polyPatch types)
+PATCH_PATCH, PATCH_WALL, PATCH_EMPTY = 0, 1, 2
+# boundary-condition kinds (OpenFOAM fvPatchField type names)
+BC_ZERO_GRADIENT, BC_FIXED_VALUE, BC_SLIP = 0, 1, 2
+
+R_GAS = 287.0
+GAMMA = 1.4
+
+
+@dataclasses.dataclass
+class Patch:
+    name: str
+    kind: int
+    start: int
+    size: int
+
+
+@dataclasses.dataclass
+class PolyMesh:
+    points: np.ndarray          # float64 [nPoints, 3]
+    face_off: np.ndarray        # int32 [nFaces+1]
+    face_verts: np.ndarray      # int32 [face_off[-1]]
+    owner: np.ndarray           # int32 [nFaces]
+    neighbour: np.ndarray       # int32 [nInternalFaces]
+    patches: List[Patch]
+    n_cells: int
+
+    @property
+    def n_faces(self) -> int:
+        return int(self.owner.shape[0])
+
+    @property
+    def n_internal(self) -> int:
+        return int(self.neighbour.shape[0])
+
+    def patch_arrays(self):
+        start = np.array([p.start for p in self.patches], np.int32)
+        size = np.array([p.size for p in self.patches], np.int32)
+        kind = np.array([p.kind for p in self.patches], np.int32)
+        return start, size, kind
+
+
+@dataclasses.dataclass
+class BoundarySpec:
+    """Per-patch p, U, T conditions. Fixed values are per patch face."""
+    p_type: int
+    U_type: int
+    T_type: int
+    p_val: Optional[np.ndarray] = None   # [size]
+    U_val: Optional[np.ndarray] = None   # [size, 3]
+    T_val: Optional[np.ndarray] = None   # [size]
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    mesh: PolyMesh
+    rho: np.ndarray     # [nC]
+    U: np.ndarray       # [nC, 3]
+    T: np.ndarray       # [nC]
+    bcs: List[BoundarySpec]
+    thermo: str
+    fvSchemes: str
+    fvSolution: str
+    cell_order: Optional[np.ndarray] = None   # new -> original cell index
+
+
+# --------------------------------------------------------------------------
+# structured hex blocks
+# --------------------------------------------------------------------------
+
+def _hex_topology(nx: int, ny: int, nz: int, patch_faces):
+    """Faces of an nx*ny*nz block in blockMesh order.
+
+    patch_faces: list of (name, kind, sides) where sides is a list of
+    '-x', '+x', '-y', '+y', '-z', '+z'.
+    Returns face_verts [nF,4], owner, neighbour, patches.
+    """
+    NX, NY = nx + 1, ny + 1
+
+    def pid(i, j, k):
+        return i + NX * (j + NY * k)
+
+    def cid(i, j, k):
+        return i + nx * (j + ny * k)
+
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    i = i.ravel(); j = j.ravel(); k = k.ravel()
+    c = cid(i, j, k)
+
+    def quad_x(ii, jj, kk):   # normal +x at x-index ii
+        return np.stack([pid(ii, jj, kk), pid(ii, jj + 1, kk), pid(ii, jj + 1, kk + 1), pid(ii, jj, kk + 1)], 1)
+
+    def quad_y(ii, jj, kk):   # normal +y at y-index jj
+        return np.stack([pid(ii, jj, kk), pid(ii, jj, kk + 1), pid(ii + 1, jj, kk + 1), pid(ii + 1, jj, kk)], 1)
+
+    def quad_z(ii, jj, kk):   # normal +z at z-index kk
+        return np.stack([pid(ii, jj, kk), pid(ii + 1, jj, kk), pid(ii + 1, jj + 1, kk), pid(ii, jj + 1, kk)], 1)
+
+    own_l, nei_l, fv_l = [], [], []
+    m = i < nx - 1
+    own_l.append(c[m]); nei_l.append(c[m] + 1); fv_l.append(quad_x(i[m] + 1, j[m], k[m]))
+    m = j < ny - 1
+    own_l.append(c[m]); nei_l.append(c[m] + nx); fv_l.append(quad_y(i[m], j[m] + 1, k[m]))
+    m = k < nz - 1
+    own_l.append(c[m]); nei_l.append(c[m] + nx * ny); fv_l.append(quad_z(i[m], j[m], k[m] + 1))
+    own = np.concatenate(own_l); nei = np.concatenate(nei_l); fv = np.concatenate(fv_l)
+    order = np.lexsort((nei, own))
+    own, nei, fv = own[order], nei[order], fv[order]
+
+    def side(s):
+        if s == "-x":
+            m = i == 0
+            q = quad_x(i[m], j[m], k[m])[:, [0, 3, 2, 1]]
+        elif s == "+x":
+            m = i == nx - 1
+            q = quad_x(i[m] + 1, j[m], k[m])
+        elif s == "-y":
+            m = j == 0
+            q = quad_y(i[m], j[m], k[m])[:, [0, 3, 2, 1]]
+        elif s == "+y":
+            m = j == ny - 1
+            q = quad_y(i[m], j[m] + 1, k[m])
+        elif s == "-z":
+            m = k == 0
+            q = quad_z(i[m], j[m], k[m])[:, [0, 3, 2, 1]]
+        elif s == "+z":
+            m = k == nz - 1
+            q = quad_z(i[m], j[m], k[m] + 1)
+        else:
+            raise ValueError(s)
+        return c[m], q
+
+    b_own, b_fv, patches = [], [], []
+    start = own.shape[0]
+    for name, kind, sides in patch_faces:
+        size = 0
+        for s in sides:
+            o, q = side(s)
+            b_own.append(o); b_fv.append(q); size += o.shape[0]
+        patches.append(Patch(name, kind, start, size))
+        start += size
+    owner = np.concatenate([own] + b_own).astype(np.int32)
+    face_verts = np.concatenate([fv] + b_fv).astype(np.int32)
+    return face_verts, owner, nei.astype(np.int32), patches
+
+
+def _mesh_from_quads(points, quads, owner, neighbour, patches, n_cells) -> PolyMesh:
+    nF = quads.shape[0]
+    face_off = (np.arange(nF + 1, dtype=np.int64) * 4).astype(np.int32)
+    return PolyMesh(points=np.ascontiguousarray(points, np.float64), face_off=face_off,
+                    face_verts=np.ascontiguousarray(quads.ravel(), np.int32),
+                    owner=owner, neighbour=neighbour, patches=patches, n_cells=n_cells)
+
+
+def hex_box(nx: int, ny: int, nz: int, lengths=(1.0, 1.0, 1.0), jitter: float = 0.0,
+            seed: int = 0) -> PolyMesh:
+    """Box [0,Lx]x[0,Ly]x[0,Lz]; interior nodes jittered by U(-jitter,jitter)*h
+    along each axis in which they are interior (boundary nodes stay on their
+    planes)."""
+    Lx, Ly, Lz = lengths
+    hx, hy, hz = Lx / nx, Ly / ny, Lz / nz
+    kk, jj, ii = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    pts = np.stack([ii.ravel() * hx, jj.ravel() * hy, kk.ravel() * hz], 1).astype(np.float64)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        d = rng.uniform(-jitter, jitter, size=pts.shape)
+        inner = np.stack([(ii.ravel() > 0) & (ii.ravel() < nx),
+                          (jj.ravel() > 0) & (jj.ravel() < ny),
+                          (kk.ravel() > 0) & (kk.ravel() < nz)], 1)
+        pts += d * np.array([hx, hy, hz]) * inner
+    fv, own, nei, patches = _hex_topology(
+        nx, ny, nz,
+        [("inlet", PATCH_PATCH, ["-x"]), ("outlet", PATCH_PATCH, ["+x"]),
+         ("walls", PATCH_WALL, ["-y", "+y", "-z", "+z"])])
+    return _mesh_from_quads(pts, fv, own, nei, patches, nx * ny * nz)
+
+
+def bump_channel(nx: int = 200, ny: int = 50, height: float = 0.1) -> PolyMesh:
+    """2-D channel x in [-1.5,1.5], y in [0,1], circular-arc bump of the given
+    height on x in [-0.5,0.5] (geometry of the circularArcBump tutorial,
+    uniform spacing), one cell thick in z in [-0.1, 0.1]."""
+    chord = 1.0
+    Rarc = (0.25 * chord * chord + height * height) / (2.0 * height)
+    x = np.linspace(-1.5, 1.5, nx + 1)
+    yb = np.where(np.abs(x) < 0.5 * chord,
+                  np.sqrt(np.maximum(Rarc * Rarc - x * x, 0.0)) - (Rarc - height), 0.0)
+    kk, jj, ii = np.meshgrid(np.arange(2), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    X = x[ii]
+    Y = yb[ii] + (1.0 - yb[ii]) * jj / ny
+    Z = np.where(kk == 0, -0.1, 0.1)
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float64)
+    fv, own, nei, patches = _hex_topology(
+        nx, ny, 1,
+        [("inlet", PATCH_PATCH, ["-x"]), ("outlet", PATCH_PATCH, ["+x"]),
+         ("lowerWall", PATCH_WALL, ["-y"]), ("upperWall", PATCH_WALL, ["+y"]),
+         ("frontAndBack", PATCH_EMPTY, ["-z", "+z"])])
+    return _mesh_from_quads(pts, fv, own, nei, patches, nx * ny)
+
+
+def approx_cell_centres(mesh: PolyMesh) -> np.ndarray:
+    """Average of each cell's face-vertex averages (input generation only)."""
+    nF = mesh.n_faces
+    fc = mesh.points[mesh.face_verts.reshape(nF, -1)].mean(axis=1) if \
+        np.all(np.diff(mesh.face_off) == 4) else None
+    if fc is None:
+        raise ValueError("approx_cell_centres expects quad faces")
+    acc = np.zeros((mesh.n_cells, 3)); cnt = np.zeros(mesh.n_cells)
+    np.add.at(acc, mesh.owner, fc); np.add.at(cnt, mesh.owner, 1)
+    np.add.at(acc, mesh.neighbour, fc[:mesh.n_internal]); np.add.at(cnt, mesh.neighbour, 1)
+    return acc / cnt[:, None]
+
+
+def face_centres_of(mesh: PolyMesh, start: int, size: int) -> np.ndarray:
+    fv = mesh.face_verts.reshape(mesh.n_faces, -1)[start:start + size]
+    return mesh.points[fv].mean(axis=1)
+
+
+# --------------------------------------------------------------------------
+# dictionaries (OpenFOAM syntax, the keyword subset the library reads)
+# --------------------------------------------------------------------------
+
+THERMO = """thermoType { type hePsiThermo; mixture pureMixture; transport const;
+ thermo hConst; equationOfState perfectGas; specie specie; energy sensibleInternalEnergy; }
+mixture { specie { molWeight 28.966; } thermodynamics { Cp %r; Hf 0; Tref 0; }
+ transport { mu %r; Pr %r; } }
+R %r; gamma %r;
+"""
+
+
+def thermo_dict(mu: float = 0.0, Pr: float = 0.72) -> str:
+    return THERMO % (GAMMA * R_GAS / (GAMMA - 1.0), mu, Pr, R_GAS, GAMMA)
+
+
+def schemes_dict(flux: str = "AUSMPlusUp", limiter: str = "cellLimited<Venkatakrishnan> Gauss linear 1",
+                 recon: str = "linearUpwind limitedGrad", lowMach: bool = True,
+                 entropyFix: float = 0.05) -> str:
+    return f"""convectiveFluxScheme {{ fluxScheme {flux}; lowMachAusm {'true' if lowMach else 'false'}; entropyFixCoeff {entropyFix}; }}
+viscousFluxScheme {{ LaxFriedrichJacobian true; }}
+ddtSchemes {{ default dualTime rPseudoDeltaT steadyState; }}
+gradSchemes {{ default Gauss linear; limitedGrad {limiter}; }}
+divSchemes {{ default none; div(tauMC) Gauss linear; }}
+laplacianSchemes {{ default Gauss linear corrected; }}
+interpolationSchemes {{ default linear; reconstruct(rho) {recon}; reconstruct(U) {recon}; reconstruct(T) {recon}; }}
+snGradSchemes {{ default corrected; }}
+"""
+
+
+def solution_dict(solver: str = "GMRES", precond: str = "LUSGS", nDirections: int = 30,
+                  maxIter: int = 1, tol: float = 1e-12, relTol: float = 1e-2,
+                  co: float = 50.0, coMin: Optional[float] = None, coMax: Optional[float] = None,
+                  precision: str = "double", nPseudoCorr: int = 1, innerDirections: int = 30,
+                  orthogonalisation: str = "MGS") -> str:
+    coMin = co if coMin is None else coMin
+    coMax = co if coMax is None else coMax
+    return f"""flowSolver {{ solver {solver}; {solver} {{ preconditioner {precond}; nDirections {nDirections};
+ maxIter {maxIter}; minIter 0; tolerance {tol}; relTol {relTol}; precision {precision};
+ innerDirections {innerDirections}; orthogonalisation {orthogonalisation}; }} }}
+pseudoTime {{ nPseudoCorr {nPseudoCorr}; pseudoTol 1e-12; pseudoTolRel 1e-12; pseudoCoNum {co};
+ pseudoCoNumMin {coMin}; pseudoCoNumMax {coMax}; localTimestepping true; localTimesteppingBounding true; }}
+"""
+
+
+# --------------------------------------------------------------------------
+# seeded fields
+# --------------------------------------------------------------------------
+
+def fourier_noise(xy: np.ndarray, seed: int, n_modes: int = 8) -> np.ndarray:
+    """sum_k sin(k.x + phi_k)/n_modes with seeded wave vectors and phases."""
+    rng = np.random.default_rng(seed)
+    kvec = rng.uniform(1.0, 3.0 * 2.0 * math.pi, size=(n_modes, xy.shape[1]))
+    phi = rng.uniform(0.0, 2.0 * math.pi, size=n_modes)
+    return np.sin(xy @ kvec.T + phi).sum(axis=1) / n_modes
+
+
+def _state_from_mach(M, p, T):
+    c = np.sqrt(GAMMA * R_GAS * T)
+    rho = p / (R_GAS * T)
+    return rho, M * c
+
+
+def case_bump(nx: int = 200, ny: int = 50, seed: int = 1, flux: str = "AUSMPlusUp",
+              co: float = 50.0, nDirections: int = 30, precision: str = "double",
+              solver: str = "GMRES", precond: str = "LUSGS") -> Case:
+    """Inviscid transonic bump channel: M=0.675, p=1e5 Pa, T=300 K, plus a
+    1% seeded 8-mode Fourier perturbation on rho, U and T."""
+    mesh = bump_channel(nx, ny)
+    C = approx_cell_centres(mesh)
+    s = 0.01 * fourier_noise(C[:, :2], seed)
+    p0, T0, M0 = 1e5, 300.0, 0.675
+    rho0, U0 = _state_from_mach(M0, p0, T0)
+    rho = rho0 * (1 + s)
+    T = T0 * (1 + s)
+    U = np.zeros((mesh.n_cells, 3)); U[:, 0] = U0 * (1 + s)
+    bcs = _bump_bcs(mesh, p0, U0, T0)
+    return Case("bump", mesh, rho, U, T, bcs, thermo_dict(0.0),
+                schemes_dict(flux), solution_dict(solver=solver, precond=precond,
+                                                  nDirections=nDirections, co=co,
+                                                  precision=precision))
+
+
+def _bump_bcs(mesh, p0, U0, T0):
+    bcs = []
+    for p in mesh.patches:
+        if p.name == "inlet":
+            Uv = np.zeros((p.size, 3)); Uv[:, 0] = U0
+            bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_FIXED_VALUE, BC_FIXED_VALUE,
+                                    None, Uv, np.full(p.size, T0)))
+        elif p.name == "outlet":
+            bcs.append(BoundarySpec(BC_FIXED_VALUE, BC_ZERO_GRADIENT, BC_ZERO_GRADIENT,
+                                    np.full(p.size, p0), None, None))
+        elif p.kind == PATCH_WALL:
+            bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_SLIP, BC_ZERO_GRADIENT))
+        else:
+            bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_ZERO_GRADIENT, BC_ZERO_GRADIENT))
+    return bcs
+
+
+def case_box(n=(32, 16, 16), lengths=(2.0, 1.0, 1.0), jitter: float = 0.1, seed: int = 7,
+             flux: str = "ROE", mu: float = 1.8e-5, mach=(0.5, 0.9), p_jump: float = 2.0,
+             noise: float = 0.02, co: float = 50.0, nDirections: int = 30,
+             precision: str = "double", precond: str = "LUSGS", solver: str = "GMRES",
+             innerDirections: int = 30, orthogonalisation: str = "MGS",
+             name: str = "box") -> Case:
+    """Jittered hex box with a linear Mach ramp along x, 2% smooth noise and
+    an optional planar tanh pressure jump (ratio p_jump) at mid-length."""
+    nx, ny, nz = n
+    mesh = hex_box(nx, ny, nz, lengths, jitter, seed)
+    C = approx_cell_centres(mesh)
+    Lx = lengths[0]
+    xi = C[:, 0] / Lx
+    M = mach[0] + (mach[1] - mach[0]) * xi
+    s = noise * fourier_noise(C / np.array(lengths), seed + 1)
+    T0 = 300.0
+    p = 1e5 * np.ones(mesh.n_cells)
+    if p_jump != 1.0:
+        delta = 0.05 * Lx
+        p = 1e5 * (1.0 + 0.5 * (p_jump - 1.0) * (np.tanh((C[:, 0] - 0.5 * Lx) / delta) + 1.0))
+    T = T0 * (1 + s)
+    rho = p / (R_GAS * T)
+    c = np.sqrt(GAMMA * R_GAS * T)
+    U = np.zeros((mesh.n_cells, 3)); U[:, 0] = M * c * (1 + s)
+    U[:, 1] = 0.05 * M * c * s
+    rho0, U0 = _state_from_mach(mach[0], 1e5, T0)
+    bcs = []
+    for pt in mesh.patches:
+        if pt.name == "inlet":
+            Uv = np.zeros((pt.size, 3)); Uv[:, 0] = U0
+            bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_FIXED_VALUE, BC_FIXED_VALUE,
+                                    None, Uv, np.full(pt.size, T0)))
+        elif pt.name == "outlet":
+            bcs.append(BoundarySpec(BC_FIXED_VALUE, BC_ZERO_GRADIENT, BC_ZERO_GRADIENT,
+                                    np.full(pt.size, 1e5 * p_jump), None, None))
+        else:
+            bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_SLIP, BC_ZERO_GRADIENT))
+    return Case(name, mesh, rho, U, T, bcs, thermo_dict(mu),
+                schemes_dict(flux), solution_dict(solver=solver, precond=precond,
+                                                  nDirections=nDirections, co=co,
+                                                  precision=precision,
+                                                  innerDirections=innerDirections,
+                                                  orthogonalisation=orthogonalisation))
+
+
+def permute_case(case: Case, new_to_old: np.ndarray, renumber) -> Case:
+    """Apply a cell permutation (new -> old) through the supplied mesh
+    renumbering routine and carry the cell fields along."""
+    mesh = renumber(case.mesh, new_to_old)
+    order = new_to_old if case.cell_order is None else case.cell_order[new_to_old]
+    return dataclasses.replace(case, mesh=mesh, rho=case.rho[new_to_old],
+                               U=case.U[new_to_old], T=case.T[new_to_old],
+                               cell_order=order)

@@ -1,0 +1,77 @@
+// Launch interfaces of the HIP kernels (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <vector>
+
+#include "icsf_views.h"
+
+namespace icsf {
+
+constexpr int KMAX = 64;  // largest nDirections supported by the device GMRES
+
+// Sweep levels of the LU-SGS dependency graph (cells grouped by level).
+struct Levels {
+    int nFwd = 0, nBwd = 0;
+    std::vector<int> fwdPtrHost, bwdPtrHost;  // [n+1] offsets into the cell lists
+    int* fwdCells = nullptr;                  // device, [nC]
+    int* bwdCells = nullptr;                  // device, [nC]
+    bool fwdSkip = true;                      // level 0 has no lower neighbours
+};
+
+// Scratch for two-stage reductions.
+struct Work {
+    double* part = nullptr;  // [1024][64]
+};
+
+struct GmresScalars;
+
+struct GmresArgs {
+    const MeshView* m;
+    const Levels* lev;
+    Work* wk;
+    GmresScalars* g;  // device
+    int nDirs;
+    int precond;      // 0 LUSGS, 1 Jacobi, 2 none
+    int orth;         // 0 MGS (reference order), 1 CGS
+    int blkFp32;
+    const void* blk;
+    const void* dg;
+    const double* rD;
+    const void* dinv;
+    void* r;          // residual / work vector, [5][nC]
+    void* V;          // Krylov basis, nDirs vectors of [5][nC]
+};
+
+int reduceBlocks(int nC);
+
+void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const double* dW, int mode,
+                  const double* rho0, const double* U0, const double* T0, hipStream_t st);
+void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st);
+void launchFaceFlux(const MeshView& m, const Phys& ph, const StateView& s, double* phi, double* phiU, double* phiE,
+                    hipStream_t st);
+void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
+                    double* rD, double* dg64, int* err, hipStream_t st);
+
+void launchSpmv(const MeshView& m, int blkFp32, int vecFp32, const void* blk, const void* dg, const void* x,
+                const double* scal, void* vout, void* y, hipStream_t st);
+void launchLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp32, const void* blk, const double* rD,
+                 void* r, hipStream_t st);
+void launchJacobiSetup(int nC, int blkFp32, const void* dg, void* dinv, hipStream_t st);
+void launchJacobiApply(int nC, int blkFp32, int vecFp32, const void* dinv, void* r, hipStream_t st);
+
+// GMRES building blocks (icsf_krylov.hip)
+size_t gmresScalarsBytes();
+void gmresPrepare(const MeshView& m, Work& wk, GmresScalars* g, int vecFp32, const void* W, const void* b,
+                  void* tmp1, void* tmp2, int blkFp32, const void* blk, const void* dg, hipStream_t st);
+void gmresCycle(const GmresArgs& a, int vecFp32, hipStream_t st);
+void gmresUpdateAndResidual(const MeshView& m, Work& wk, GmresScalars* g, int vecFp32, int blkFp32,
+                            const void* blk, const void* dg, int nDirs, const void* V, void* dW, const void* b,
+                            void* r, hipStream_t st);
+void gmresReadResiduals(const GmresScalars* g, double* res10, hipStream_t st);
+const double* gmresResidualsDevice(const GmresScalars* g);
+
+void launchCopy(size_t n, int fp32, const void* a, void* b, hipStream_t st);
+void launchConvert(size_t n, int fromFp32, const void* a, void* b, hipStream_t st);
+void launchZeroComponent(int nC, int comp, int fp32, void* x, hipStream_t st);
+
+}  // namespace icsf

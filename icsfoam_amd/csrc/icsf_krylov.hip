@@ -1,0 +1,732 @@
+// Linear-algebra kernels of the coupled solve:
+//   block SpMV        coupledMatrix::matrixMul -> blockFvMatrix::Amul
+//                     (coupledMatrix.C:66-123, blockFvMatrix.C:331-381)
+//   LU-SGS            lusgs::precondition (lusgs.C:220-382), level-scheduled
+//   block-Jacobi      Jacobi::precondition / JacobiSmoother (Jacobi.C:55-132)
+//   GMRES pieces      gmres::solveDelta (gmres.C:772-1108): reductions,
+//                     orthogonalisation, Givens, back-substitution, update.
+//
+// Vectors are SoA [5][nC] in [rho, rhoE, rhoUx, rhoUy, rhoUz] order.
+// Off-diagonal 5x5 blocks live in a sliced-ELL layout (64-row slices, one
+// wavefront per slice, entry-major inside a slot) so every block load of a
+// wavefront is one contiguous 64-element run.
+#include "icsf_kernels.h"
+#include "icsf_math.h"
+#include "icsf_views.h"
+
+namespace icsf {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ double ld(const T* p, size_t i) { return (double)p[i]; }
+
+// ---------------------------------------------------------------- SpMV ----
+// Per output component three partial sums are kept (scalar-scalar,
+// scalar-scalar', scalar/vector-vector parts) and combined in the order
+// coupledMatrix::matrixMul adds its sub-block products.
+template <typename TB, typename TV>
+__global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__ blk, const TB* __restrict__ dg,
+                                              const TV* __restrict__ x, const double* __restrict__ scal,
+                                              TV* __restrict__ vout, TV* __restrict__ y) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const size_t nC = m.nC;
+    typedef TV A;
+    const A beta = scal ? (A)(*scal) : (A)1;
+    auto X = [&](int k, int j) -> A {
+        const A v = x[(size_t)k * nC + j];
+        return scal ? v / beta : v;
+    };
+    A xc[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) xc[k] = X(k, c);
+    if (vout)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) vout[(size_t)k * nC + c] = xc[k];
+    A acc[5][3];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const A b0 = (A)dg[diagOff(c, 5 * i)], b1 = (A)dg[diagOff(c, 5 * i + 1)];
+        const A b2 = (A)dg[diagOff(c, 5 * i + 2)], b3 = (A)dg[diagOff(c, 5 * i + 3)],
+                b4 = (A)dg[diagOff(c, 5 * i + 4)];
+        acc[i][0] = b0 * xc[0];
+        acc[i][1] = b1 * xc[1];
+        acc[i][2] = b2 * xc[2] + b3 * xc[3] + b4 * xc[4];
+    }
+    const int sl = c >> 6, lane = c & 63;
+    const int len = m.rowLen[c];
+    const size_t base = (size_t)m.slicePtr[sl];
+    for (int k = 0; k < len; ++k) {
+        const size_t slot = (base + k) * 64 + lane;
+        const int j = m.col[slot];
+        A xj[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) xj[q] = X(q, j);
+        const size_t bb = (base + k) * NB * 64 + lane;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const A b0 = (A)blk[bb + (5 * i) * 64], b1 = (A)blk[bb + (5 * i + 1) * 64];
+            const A b2 = (A)blk[bb + (5 * i + 2) * 64], b3 = (A)blk[bb + (5 * i + 3) * 64],
+                    b4 = (A)blk[bb + (5 * i + 4) * 64];
+            acc[i][0] += b0 * xj[0];
+            acc[i][1] += b1 * xj[1];
+            const A d = b2 * xj[2] + b3 * xj[3] + b4 * xj[4];
+            acc[i][2] += d;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        A r = (A)0 + acc[i][0];
+        r = r + acc[i][1];
+        r = r + acc[i][2];
+        y[(size_t)i * nC + c] = r;
+    }
+}
+
+// ------------------------------------------------------------- LU-SGS -----
+// Forward sweep of one level: r[c] -= sum over lower neighbours p (in
+// increasing p) of L_cp * (rD[p] r[p]).
+template <typename TB, typename TV>
+__global__ void __launch_bounds__(256) k_lusgs_fwd(MeshView m, const TB* __restrict__ blk,
+                                                   const double* __restrict__ rD, TV* __restrict__ r,
+                                                   const int* __restrict__ cells, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int c = cells[t];
+    const int nl = m.nLower[c];
+    if (nl == 0) return;
+    const size_t nC = m.nC;
+    typedef TV A;
+    A acc[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) acc[i] = r[(size_t)i * nC + c];
+    const int sl = c >> 6, lane = c & 63;
+    const size_t base = (size_t)m.slicePtr[sl];
+    for (int k = 0; k < nl; ++k) {
+        const size_t slot = (base + k) * 64 + lane;
+        const int p = m.col[slot];
+        const A rd = (A)rD[p];
+        A d[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) d[q] = rd * r[(size_t)q * nC + p];
+        const size_t bb = (base + k) * NB * 64 + lane;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const A b0 = (A)blk[bb + (5 * i) * 64], b1 = (A)blk[bb + (5 * i + 1) * 64];
+            const A b2 = (A)blk[bb + (5 * i + 2) * 64], b3 = (A)blk[bb + (5 * i + 3) * 64],
+                    b4 = (A)blk[bb + (5 * i + 4) * 64];
+            acc[i] -= b0 * d[0];
+            acc[i] -= b1 * d[1];
+            const A dd = b2 * d[2] + b3 * d[3] + b4 * d[4];
+            acc[i] -= dd;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) r[(size_t)i * nC + c] = acc[i];
+}
+
+// Backward sweep of one level: r[c] = rD[c] (r[c] - sum over upper
+// neighbours q (decreasing q) of U_cq r[q]), r[q] already final.
+template <typename TB, typename TV>
+__global__ void __launch_bounds__(256) k_lusgs_bwd(MeshView m, const TB* __restrict__ blk,
+                                                   const double* __restrict__ rD, TV* __restrict__ r,
+                                                   const int* __restrict__ cells, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int c = cells[t];
+    const size_t nC = m.nC;
+    typedef TV A;
+    A acc[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) acc[i] = r[(size_t)i * nC + c];
+    const int sl = c >> 6, lane = c & 63;
+    const size_t base = (size_t)m.slicePtr[sl];
+    const int nl = m.nLower[c], len = m.rowLen[c];
+    for (int k = len - 1; k >= nl; --k) {
+        const size_t slot = (base + k) * 64 + lane;
+        const int q = m.col[slot];
+        A d[5];
+#pragma unroll
+        for (int z = 0; z < 5; ++z) d[z] = r[(size_t)z * nC + q];
+        const size_t bb = (base + k) * NB * 64 + lane;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const A b0 = (A)blk[bb + (5 * i) * 64], b1 = (A)blk[bb + (5 * i + 1) * 64];
+            const A b2 = (A)blk[bb + (5 * i + 2) * 64], b3 = (A)blk[bb + (5 * i + 3) * 64],
+                    b4 = (A)blk[bb + (5 * i + 4) * 64];
+            acc[i] -= b0 * d[0];
+            acc[i] -= b1 * d[1];
+            const A dd = b2 * d[2] + b3 * d[3] + b4 * d[4];
+            acc[i] -= dd;
+        }
+    }
+    const A rd = (A)rD[c];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) r[(size_t)i * nC + c] = rd * acc[i];
+}
+
+// ------------------------------------------------------- block-Jacobi -----
+// 5x5 inverse of each diagonal block by LU with partial pivoting.
+template <typename TB>
+__global__ void k_jacobi_setup(int nC, const TB* __restrict__ dg, TB* __restrict__ dinv) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    double a[5][5];
+    int piv[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        piv[i] = i;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) a[i][j] = (double)dg[diagOff(c, 5 * i + j)];
+    }
+    for (int col = 0; col < 5; ++col) {
+        int p = col;
+        for (int r = col + 1; r < 5; ++r)
+            if (fabs(a[r][col]) > fabs(a[p][col])) p = r;
+        if (p != col) {
+            for (int q = 0; q < 5; ++q) { double t = a[p][q]; a[p][q] = a[col][q]; a[col][q] = t; }
+            int t = piv[p]; piv[p] = piv[col]; piv[col] = t;
+        }
+        for (int r = col + 1; r < 5; ++r) {
+            a[r][col] /= a[col][col];
+            for (int q = col + 1; q < 5; ++q) a[r][q] -= a[r][col] * a[col][q];
+        }
+    }
+    for (int e = 0; e < 5; ++e) {
+        double yv[5];
+        for (int i = 0; i < 5; ++i) {
+            double s = piv[i] == e ? 1.0 : 0.0;
+            for (int q = 0; q < i; ++q) s -= a[i][q] * yv[q];
+            yv[i] = s;
+        }
+        for (int i = 4; i >= 0; --i) {
+            double s = yv[i];
+            for (int q = i + 1; q < 5; ++q) s -= a[i][q] * yv[q];
+            yv[i] = s / a[i][i];
+        }
+        for (int i = 0; i < 5; ++i) dinv[diagOff(c, 5 * i + e)] = (TB)yv[i];
+    }
+}
+
+template <typename TB, typename TV>
+__global__ void k_jacobi_apply(int nC, const TB* __restrict__ dinv, TV* __restrict__ r) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    typedef TV A;
+    A v[5], o[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) { v[i] = r[(size_t)i * nC + c]; o[i] = 0; }
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) o[i] += (A)dinv[diagOff(c, 5 * i + j)] * v[j];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) r[(size_t)i * nC + c] = o[i];
+}
+
+// ---------------------------------------------------------- reductions ----
+__device__ __forceinline__ double waveSum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Block-level sum of NV values per thread -> out[blockIdx.x*NV + k].
+template <int NV>
+__device__ __forceinline__ void blockReduceStore(double (&v)[NV], double* out) {
+    __shared__ double sh[4][NV > 0 ? NV : 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double s = waveSum(v[k]);
+        if (lane == 0) sh[wv][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        out[(size_t)blockIdx.x * NV + k] = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
+    }
+}
+
+// Final reduction of [nb][nv] partials with one 256-thread block; out[k].
+__global__ void k_reduce_final(const double* __restrict__ part, int nb, int nv, double* __restrict__ out) {
+    __shared__ double sh[4];
+    for (int k = 0; k < nv; ++k) {
+        double v = 0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) v += part[(size_t)b * nv + k];
+        v = waveSum(v);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) out[k] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+        __syncthreads();
+    }
+}
+
+// sums of the 5 components of x (for the average) -> partials
+template <typename TV>
+__global__ void __launch_bounds__(256) k_sum5(int nC, const TV* __restrict__ x, double* __restrict__ part) {
+    double v[5] = {0, 0, 0, 0, 0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nC; c += gridDim.x * blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) v[k] += (double)x[(size_t)k * nC + c];
+    blockReduceStore<5>(v, part);
+}
+
+// norm-factor partials (gmres.C:839-850):
+// [0..1] sum |t_s|+|b_s|, [2] sum |t_v|+|b_v|, [3..4] sum |b_s|, [5..7] sum |b_v,d|
+template <typename TV>
+__global__ void __launch_bounds__(256) k_normfactor(int nC, const TV* __restrict__ t, const TV* __restrict__ b,
+                                                    double* __restrict__ part) {
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nC; c += gridDim.x * blockDim.x) {
+        double tv[5], bv[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) { tv[k] = (double)t[(size_t)k * nC + c]; bv[k] = (double)b[(size_t)k * nC + c]; }
+        v[0] += fabs(tv[0]) + fabs(bv[0]);
+        v[1] += fabs(tv[1]) + fabs(bv[1]);
+        v[2] += sqrt(tv[2] * tv[2] + tv[3] * tv[3] + tv[4] * tv[4]) + sqrt(bv[2] * bv[2] + bv[3] * bv[3] + bv[4] * bv[4]);
+        v[3] += fabs(bv[0]);
+        v[4] += fabs(bv[1]);
+        v[5] += fabs(bv[2]);
+        v[6] += fabs(bv[3]);
+        v[7] += fabs(bv[4]);
+    }
+    blockReduceStore<8>(v, part);
+}
+
+// final residual partials of r = b - A dW: [0..4] sum |r_k|
+template <typename TV>
+__global__ void __launch_bounds__(256) k_resid_l1(int nC, const TV* __restrict__ b, TV* __restrict__ Ax,
+                                                  double* __restrict__ part) {
+    double v[5] = {0, 0, 0, 0, 0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nC; c += gridDim.x * blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const TV r = b[(size_t)k * nC + c] - Ax[(size_t)k * nC + c];
+            Ax[(size_t)k * nC + c] = r;
+            v[k] += fabs((double)r);
+        }
+    blockReduceStore<5>(v, part);
+}
+
+template <typename TV>
+__global__ void __launch_bounds__(256) k_norm2(int nC, const TV* __restrict__ x, double* __restrict__ part) {
+    double v[1] = {0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nC; c += gridDim.x * blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const double q = (double)x[(size_t)k * nC + c];
+            v[0] += q * q;
+        }
+    blockReduceStore<1>(v, part);
+}
+
+// CGS projection partials: <w, V_j> for j < nv (nv <= 64, chunked by 16)
+template <typename TV, int CH>
+__global__ void __launch_bounds__(256) k_multidot(int nC, const TV* __restrict__ w, const TV* __restrict__ Vb,
+                                                  size_t vstride, int j0, int nv, double* __restrict__ part) {
+    double v[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) v[k] = 0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nC; c += gridDim.x * blockDim.x) {
+        TV wv[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) wv[q] = w[(size_t)q * nC + c];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            if (k < nv) {
+                const TV* vj = Vb + (size_t)(j0 + k) * vstride;
+                double s = 0;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) s += (double)(wv[q] * vj[(size_t)q * nC + c]);
+                v[k] += s;
+            }
+        }
+    }
+    blockReduceStore<CH>(v, part);
+}
+
+// w -= sum_j h_j V_j (j < nv), then ||w||^2 partials
+template <typename TV>
+__global__ void __launch_bounds__(256) k_cgs_update(int nC, TV* __restrict__ w, const TV* __restrict__ Vb,
+                                                    size_t vstride, int nv, const double* __restrict__ h,
+                                                    double* __restrict__ part) {
+    double v[1] = {0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nC; c += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            TV a = w[(size_t)q * nC + c];
+            for (int j = 0; j < nv; ++j) a -= (TV)h[j] * Vb[(size_t)j * vstride + (size_t)q * nC + c];
+            w[(size_t)q * nC + c] = a;
+            v[0] += (double)a * (double)a;
+        }
+    }
+    blockReduceStore<1>(v, part);
+}
+
+// MGS step: (optionally) w -= h * V_prev, then partials of <w, V_next>
+// (5 per-variable partial sums, combined in the reference's order on reduce)
+template <typename TV>
+__global__ void __launch_bounds__(256) k_mgs_step(int nC, TV* __restrict__ w, const TV* __restrict__ Vprev,
+                                                  const double* __restrict__ hprev, const TV* __restrict__ Vnext,
+                                                  int wantNorm, double* __restrict__ part) {
+    double v[3] = {0, 0, 0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nC; c += gridDim.x * blockDim.x) {
+        TV a[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            a[q] = w[(size_t)q * nC + c];
+            if (Vprev) {
+                a[q] -= (TV)(*hprev) * Vprev[(size_t)q * nC + c];
+                w[(size_t)q * nC + c] = a[q];
+            }
+        }
+        if (Vnext) {
+            v[0] += (double)(a[0] * Vnext[c]);
+            v[1] += (double)(a[1] * Vnext[(size_t)nC + c]);
+            v[2] += (double)(a[2] * Vnext[2 * (size_t)nC + c] + a[3] * Vnext[3 * (size_t)nC + c] +
+                             a[4] * Vnext[4 * (size_t)nC + c]);
+        }
+        if (wantNorm) {
+            v[0] += (double)(a[0] * a[0]);
+            v[1] += (double)(a[1] * a[1]);
+            v[2] += (double)(a[2] * a[2] + a[3] * a[3] + a[4] * a[4]);
+        }
+    }
+    blockReduceStore<3>(v, part);
+}
+
+// x = y - avg (component-wise); dW = 0 afterwards is done by memset.
+template <typename TV>
+__global__ void k_sub_avg(int nC, const TV* __restrict__ W, const double* __restrict__ sums, TV* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) out[(size_t)k * nC + c] = W[(size_t)k * nC + c] - (TV)(sums[k] / nC);
+}
+
+template <typename TV>
+__global__ void k_copy(size_t n, const TV* __restrict__ a, TV* __restrict__ b) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+template <typename TA, typename TB2>
+__global__ void k_convert(size_t n, const TA* __restrict__ a, TB2* __restrict__ b) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (TB2)a[i];
+}
+
+// dW += sum_i y_i V_i (gmres.C:1050-1059), sequential in i per entry
+template <typename TV, typename TX>
+__global__ void __launch_bounds__(256) k_update_sol(int nC, TX* __restrict__ dW, const TV* __restrict__ Vb,
+                                                    size_t vstride, int nv, const double* __restrict__ yh) {
+    const size_t n = (size_t)5 * nC;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    TX a = dW[i];
+    for (int j = 0; j < nv; ++j) a += (TX)yh[j] * (TX)Vb[(size_t)j * vstride + i];
+    dW[i] = a;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ GMRES state --
+// Scalars of one GMRES(m) cycle kept on the device (no host round trips).
+struct GmresScalars {
+    double H[KMAX][KMAX];
+    double cs[KMAX], sn[KMAX], bh[KMAX + 1], yh[KMAX];
+    double beta;
+    double sNF[2], vNF;
+    double res[10];   // sInit[2] vInit[3] sFinal[2] vFinal[3]
+    double red[64];   // scratch reduction results
+    double hcol[KMAX];  // contiguous copy of the current Hessenberg column
+};
+
+namespace {
+
+__device__ void givensRot(double h, double beta, double& c, double& s) {
+    if (beta == 0) { c = 1; s = 0; }
+    else if (fabs(beta) > fabs(h)) {
+        const double tau = -h / beta;
+        s = 1.0 / sqrt(1.0 + tau * tau);
+        c = s * tau;
+    } else {
+        const double tau = -beta / h;
+        c = 1.0 / sqrt(1.0 + tau * tau);
+        s = c * tau;
+    }
+}
+
+// Norm factors and initial residuals from the k_normfactor reduction.
+__global__ void k_gm_init(GmresScalars* g) {
+    if (threadIdx.x != 0) return;
+    const double* r = g->red;
+    g->sNF[0] = r[0] + kVSMALL;
+    g->sNF[1] = r[1] + kVSMALL;
+    g->vNF = r[2] + kVSMALL;
+    g->res[0] = r[3] / g->sNF[0];
+    g->res[1] = r[4] / g->sNF[1];
+    for (int d = 0; d < 3; ++d) g->res[2 + d] = r[5 + d] / g->vNF;
+    for (int k = 0; k < 5; ++k) g->res[5 + k] = g->res[k];
+}
+
+// beta = ||r|| at the start of a cycle
+__global__ void k_gm_beta0(GmresScalars* g, int nDirs) {
+    if (threadIdx.x != 0) return;
+    g->beta = sqrt(g->red[0]);
+    for (int i = 0; i <= nDirs; ++i) g->bh[i] = 0;
+    g->bh[0] = g->beta;
+}
+
+// store the CGS projections as column i of H
+__global__ void k_gm_store_col(GmresScalars* g, int i, int j0, int nv) {
+    const int k = threadIdx.x;
+    if (k < nv) {
+        g->H[j0 + k][i] = g->red[k];
+        g->hcol[j0 + k] = g->red[k];
+    }
+}
+
+// MGS: H[j][i] from the 3 per-variable partial sums
+__global__ void k_gm_store_h(GmresScalars* g, int i, int j) {
+    if (threadIdx.x != 0) return;
+    g->H[j][i] = (0.0 + g->red[0] + g->red[1]) + g->red[2];
+}
+
+// apply previous rotations, new rotation, update bh (gmres.C:999-1027)
+__global__ void k_gm_givens(GmresScalars* g, int i, int mgsNorm) {
+    if (threadIdx.x != 0) return;
+    double beta2 = mgsNorm ? ((0.0 + g->red[0] + g->red[1]) + g->red[2]) : g->red[0];
+    const double beta = sqrt(beta2);
+    for (int j = 0; j < i; ++j) {
+        const double Hji = g->H[j][i];
+        g->H[j][i] = g->cs[j] * Hji - g->sn[j] * g->H[j + 1][i];
+        g->H[j + 1][i] = g->sn[j] * Hji + g->cs[j] * g->H[j + 1][i];
+    }
+    givensRot(g->H[i][i], beta, g->cs[i], g->sn[i]);
+    const double bhi = g->bh[i];
+    g->bh[i] = g->cs[i] * bhi - g->sn[i] * g->bh[i + 1];
+    g->bh[i + 1] = g->sn[i] * bhi + g->cs[i] * g->bh[i + 1];
+    g->H[i][i] = g->cs[i] * g->H[i][i] - g->sn[i] * beta;
+    g->beta = beta;
+}
+
+__global__ void k_gm_backsub(GmresScalars* g, int n) {
+    if (threadIdx.x != 0) return;
+    for (int i = n - 1; i >= 0; --i) {
+        double s = g->bh[i];
+        for (int j = i + 1; j < n; ++j) s -= g->H[i][j] * g->yh[j];
+        double hd = g->H[i][i];
+        hd = hd >= 0 ? hd + kVSMALL : hd - kVSMALL;
+        g->yh[i] = s / hd;
+    }
+}
+
+__global__ void k_gm_final(GmresScalars* g, int emptyDir) {
+    if (threadIdx.x != 0) return;
+    const double* r = g->red;
+    g->res[5] = r[0] / g->sNF[0];
+    g->res[6] = r[1] / g->sNF[1];
+    for (int d = 0; d < 3; ++d) g->res[7 + d] = (d == emptyDir) ? 0.0 : r[2 + d] / g->vNF;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- host side --
+static inline int nb(size_t n, int b) { return (int)((n + b - 1) / b); }
+
+int reduceBlocks(int nC) {
+    int n = nb(nC, 256);
+    return n < 1024 ? (n < 1 ? 1 : n) : 1024;
+}
+
+template <typename TB, typename TV>
+void spmv(const MeshView& m, const void* blk, const void* dg, const void* x, const double* scal, void* vout, void* y,
+          hipStream_t st) {
+    k_spmv<TB, TV><<<nb(m.nC, 256), 256, 0, st>>>(m, (const TB*)blk, (const TB*)dg, (const TV*)x, scal, (TV*)vout,
+                                                  (TV*)y);
+}
+
+void launchSpmv(const MeshView& m, int blkFp32, int vecFp32, const void* blk, const void* dg, const void* x,
+                const double* scal, void* vout, void* y, hipStream_t st) {
+    if (blkFp32 && vecFp32) spmv<float, float>(m, blk, dg, x, scal, vout, y, st);
+    else if (blkFp32) spmv<float, double>(m, blk, dg, x, scal, vout, y, st);
+    else if (vecFp32) spmv<double, float>(m, blk, dg, x, scal, vout, y, st);
+    else spmv<double, double>(m, blk, dg, x, scal, vout, y, st);
+}
+
+void launchLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp32, const void* blk, const double* rD,
+                 void* r, hipStream_t st) {
+    for (int l = 0; l < L.nFwd; ++l) {
+        const int s0 = L.fwdPtrHost[l], n = L.fwdPtrHost[l + 1] - s0;
+        if (L.fwdSkip && l == 0) continue;  // first level has no lower neighbours
+        if (blkFp32 && vecFp32)
+            k_lusgs_fwd<float, float><<<nb(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (float*)r, L.fwdCells + s0, n);
+        else if (blkFp32)
+            k_lusgs_fwd<float, double><<<nb(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (double*)r, L.fwdCells + s0, n);
+        else if (vecFp32)
+            k_lusgs_fwd<double, float><<<nb(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (float*)r, L.fwdCells + s0, n);
+        else
+            k_lusgs_fwd<double, double><<<nb(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (double*)r, L.fwdCells + s0, n);
+    }
+    for (int l = 0; l < L.nBwd; ++l) {
+        const int s0 = L.bwdPtrHost[l], n = L.bwdPtrHost[l + 1] - s0;
+        if (blkFp32 && vecFp32)
+            k_lusgs_bwd<float, float><<<nb(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (float*)r, L.bwdCells + s0, n);
+        else if (blkFp32)
+            k_lusgs_bwd<float, double><<<nb(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (double*)r, L.bwdCells + s0, n);
+        else if (vecFp32)
+            k_lusgs_bwd<double, float><<<nb(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (float*)r, L.bwdCells + s0, n);
+        else
+            k_lusgs_bwd<double, double><<<nb(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (double*)r, L.bwdCells + s0, n);
+    }
+}
+
+void launchJacobiSetup(int nC, int blkFp32, const void* dg, void* dinv, hipStream_t st) {
+    if (blkFp32) k_jacobi_setup<float><<<nb(nC, 256), 256, 0, st>>>(nC, (const float*)dg, (float*)dinv);
+    else k_jacobi_setup<double><<<nb(nC, 256), 256, 0, st>>>(nC, (const double*)dg, (double*)dinv);
+}
+
+void launchJacobiApply(int nC, int blkFp32, int vecFp32, const void* dinv, void* r, hipStream_t st) {
+    if (blkFp32 && vecFp32) k_jacobi_apply<float, float><<<nb(nC, 256), 256, 0, st>>>(nC, (const float*)dinv, (float*)r);
+    else if (blkFp32) k_jacobi_apply<float, double><<<nb(nC, 256), 256, 0, st>>>(nC, (const float*)dinv, (double*)r);
+    else if (vecFp32) k_jacobi_apply<double, float><<<nb(nC, 256), 256, 0, st>>>(nC, (const double*)dinv, (float*)r);
+    else k_jacobi_apply<double, double><<<nb(nC, 256), 256, 0, st>>>(nC, (const double*)dinv, (double*)r);
+}
+
+// --------------------------------------------------------- GMRES driver --
+template <typename TV>
+void gmresCycleT(const GmresArgs& a, hipStream_t st) {
+    const MeshView& m = *a.m;
+    const int nC = m.nC;
+    const int nbk = reduceBlocks(nC);
+    const size_t vs = (size_t)5 * nC;
+    TV* r = (TV*)a.r;
+    TV* V = (TV*)a.V;
+    GmresScalars* g = a.g;
+    double* red = g->red;
+    auto precond = [&](TV* v) {
+        if (a.precond == 0) launchLusgs(m, *a.lev, a.blkFp32, sizeof(TV) == 4, a.blk, a.rD, v, st);
+        else if (a.precond == 1) launchJacobiApply(nC, a.blkFp32, sizeof(TV) == 4, a.dinv, v, st);
+    };
+    // r <- P^-1 r; beta = ||r||
+    precond(r);
+    k_norm2<TV><<<nbk, 256, 0, st>>>(nC, r, a.wk->part);
+    k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 1, red);
+    k_gm_beta0<<<1, 64, 0, st>>>(g, a.nDirs);
+    for (int i = 0; i < a.nDirs; ++i) {
+        TV* vi = V + (size_t)i * vs;
+        // v_i = r / beta ; r = A v_i
+        launchSpmv(m, a.blkFp32, sizeof(TV) == 4, a.blk, a.dg, r, &g->beta, vi, r, st);
+        precond(r);
+        if (a.orth == 0) {
+            // modified Gram-Schmidt, reference order
+            for (int j = 0; j <= i + 1; ++j) {
+                const TV* vprev = j > 0 ? V + (size_t)(j - 1) * vs : nullptr;
+                const TV* vnext = j <= i ? V + (size_t)j * vs : nullptr;
+                const double* hprev = j > 0 ? &g->H[j - 1][i] : nullptr;
+                k_mgs_step<TV><<<nbk, 256, 0, st>>>(nC, r, vprev, hprev, vnext, j == i + 1, a.wk->part);
+                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 3, red);
+                if (j <= i) k_gm_store_h<<<1, 64, 0, st>>>(g, i, j);
+            }
+            k_gm_givens<<<1, 64, 0, st>>>(g, i, 1);
+        } else {
+            // classical Gram-Schmidt: one projection pass + one update pass
+            for (int j0 = 0; j0 <= i; j0 += 16) {
+                const int nv = (i + 1 - j0) < 16 ? (i + 1 - j0) : 16;
+                k_multidot<TV, 16><<<nbk, 256, 0, st>>>(nC, r, V, vs, j0, nv, a.wk->part);
+                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 16, red);
+                k_gm_store_col<<<1, 64, 0, st>>>(g, i, j0, nv);
+            }
+            k_cgs_update<TV><<<nbk, 256, 0, st>>>(nC, r, V, vs, i + 1, g->hcol, a.wk->part);
+            k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 1, red);
+            k_gm_givens<<<1, 64, 0, st>>>(g, i, 0);
+        }
+    }
+    k_gm_backsub<<<1, 64, 0, st>>>(g, a.nDirs);
+}
+
+}  // namespace icsf
+
+namespace icsf {
+
+namespace {
+template <typename TV>
+__global__ void k_zero_comp(int nC, int comp, TV* x) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nC) x[(size_t)comp * nC + c] = (TV)0;
+}
+}  // namespace
+
+size_t gmresScalarsBytes() { return sizeof(GmresScalars); }
+
+void gmresCycle(const GmresArgs& a, int vecFp32, hipStream_t st) {
+    if (vecFp32) gmresCycleT<float>(a, st);
+    else gmresCycleT<double>(a, st);
+}
+
+template <typename TV>
+static void prepareT(const MeshView& m, Work& wk, GmresScalars* g, const void* W, const void* b, void* tmp1,
+                     void* tmp2, int blkFp32, const void* blk, const void* dg, hipStream_t st) {
+    const int nC = m.nC;
+    const int nbk = reduceBlocks(nC);
+    k_sum5<TV><<<nbk, 256, 0, st>>>(nC, (const TV*)W, wk.part);
+    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 5, g->red);
+    k_sub_avg<TV><<<nb(nC, 256), 256, 0, st>>>(nC, (const TV*)W, g->red, (TV*)tmp1);
+    launchSpmv(m, blkFp32, sizeof(TV) == 4, blk, dg, tmp1, nullptr, nullptr, tmp2, st);
+    k_normfactor<TV><<<nbk, 256, 0, st>>>(nC, (const TV*)tmp2, (const TV*)b, wk.part);
+    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 8, g->red);
+    k_gm_init<<<1, 64, 0, st>>>(g);
+}
+
+void gmresPrepare(const MeshView& m, Work& wk, GmresScalars* g, int vecFp32, const void* W, const void* b,
+                  void* tmp1, void* tmp2, int blkFp32, const void* blk, const void* dg, hipStream_t st) {
+    if (vecFp32) prepareT<float>(m, wk, g, W, b, tmp1, tmp2, blkFp32, blk, dg, st);
+    else prepareT<double>(m, wk, g, W, b, tmp1, tmp2, blkFp32, blk, dg, st);
+}
+
+template <typename TV>
+static void updResT(const MeshView& m, Work& wk, GmresScalars* g, int blkFp32, const void* blk, const void* dg,
+                    int nDirs, const void* V, void* dW, const void* b, void* r, hipStream_t st) {
+    const int nC = m.nC;
+    const int nbk = reduceBlocks(nC);
+    const size_t vs = (size_t)5 * nC;
+    k_update_sol<TV, TV><<<nb(vs, 256), 256, 0, st>>>(nC, (TV*)dW, (const TV*)V, vs, nDirs, g->yh);
+    launchSpmv(m, blkFp32, sizeof(TV) == 4, blk, dg, dW, nullptr, nullptr, r, st);
+    k_resid_l1<TV><<<nbk, 256, 0, st>>>(nC, (const TV*)b, (TV*)r, wk.part);
+    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 5, g->red);
+    k_gm_final<<<1, 64, 0, st>>>(g, m.emptyDir);
+}
+
+void gmresUpdateAndResidual(const MeshView& m, Work& wk, GmresScalars* g, int vecFp32, int blkFp32,
+                            const void* blk, const void* dg, int nDirs, const void* V, void* dW, const void* b,
+                            void* r, hipStream_t st) {
+    if (vecFp32) updResT<float>(m, wk, g, blkFp32, blk, dg, nDirs, V, dW, b, r, st);
+    else updResT<double>(m, wk, g, blkFp32, blk, dg, nDirs, V, dW, b, r, st);
+}
+
+void gmresReadResiduals(const GmresScalars* g, double* res10, hipStream_t st) {
+    (void)hipMemcpyAsync(res10, g->res, sizeof(double) * 10, hipMemcpyDeviceToHost, st);
+    (void)hipStreamSynchronize(st);
+}
+
+const double* gmresResidualsDevice(const GmresScalars* g) { return g->res; }
+
+void launchCopy(size_t n, int fp32, const void* a, void* b, hipStream_t st) {
+    if (fp32) k_copy<float><<<nb(n, 256), 256, 0, st>>>(n, (const float*)a, (float*)b);
+    else k_copy<double><<<nb(n, 256), 256, 0, st>>>(n, (const double*)a, (double*)b);
+}
+
+void launchConvert(size_t n, int fromFp32, const void* a, void* b, hipStream_t st) {
+    if (fromFp32) k_convert<float, double><<<nb(n, 256), 256, 0, st>>>(n, (const float*)a, (double*)b);
+    else k_convert<double, float><<<nb(n, 256), 256, 0, st>>>(n, (const double*)a, (float*)b);
+}
+
+void launchZeroComponent(int nC, int comp, int fp32, void* x, hipStream_t st) {
+    if (fp32) k_zero_comp<float><<<nb(nC, 256), 256, 0, st>>>(nC, comp, (float*)x);
+    else k_zero_comp<double><<<nb(nC, 256), 256, 0, st>>>(nC, comp, (double*)x);
+}
+
+}  // namespace icsf

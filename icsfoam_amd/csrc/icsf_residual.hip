@@ -1,0 +1,631 @@
+// Residual and Jacobian assembly kernels: one thread per cell, pulling the
+// cell's faces from a face-ordered cell->face list, so every cell sums its
+// face contributions in the same order as OpenFOAM's face loops
+// (surfaceIntegrate, gaussGrad, LduMatrix::negSumDiag) and no atomics are
+// needed.  Faces are evaluated once per adjacent cell with identical code
+// and inputs, hence identical bits on both sides.
+#include "icsf_flux.h"
+#include "icsf_kernels.h"
+#include "icsf_views.h"
+
+namespace icsf {
+
+namespace {
+
+__device__ __forceinline__ V3 cellU(const double* fld, int c) {
+    return V3{fld[c * NFLD + F_UX], fld[c * NFLD + F_UY], fld[c * NFLD + F_UZ]};
+}
+
+// Primitive/derived cell fields from conserved W after an increment
+// (updateFields.H:7-78), or from (rho, U, T) at initialisation.
+__global__ void k_update(MeshView m, Phys ph, StateView s, const double* __restrict__ dW, int mode,
+                         const double* __restrict__ rho0, const double* __restrict__ U0,
+                         const double* __restrict__ T0) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const int nC = m.nC;
+    double rho, e, T, p;
+    V3 U;
+    if (mode == 0) {
+        // initialisation from rho, U, T (createFields.H)
+        rho = rho0[c];
+        U = V3{U0[3 * c], U0[3 * c + 1], U0[3 * c + 2]};
+        T = T0[c];
+        e = ph.Cv * T;
+        const double psi = 1.0 / (ph.R * T);
+        p = rho / psi;
+    } else {
+        rho = s.W[c];
+        V3 rU = V3{s.W[2 * (size_t)nC + c], s.W[3 * (size_t)nC + c], s.W[4 * (size_t)nC + c]};
+        double rE = s.W[(size_t)nC + c];
+        if (dW) {
+            rho += dW[c];
+            rU = V3{rU.x + dW[2 * (size_t)nC + c], rU.y + dW[3 * (size_t)nC + c], rU.z + dW[4 * (size_t)nC + c]};
+            rE += dW[(size_t)nC + c];
+        }
+        rho = fmax(rho, ph.rhoMin);
+        U = rU / rho;
+        e = rE / rho - 0.5 * magSqr(U);
+        e = fmax(e, ph.Cv * ph.TMin);
+        if (ph.TMax < kGREAT) e = fmin(e, ph.Cv * ph.TMax);
+        T = e / ph.Cv;
+        const double psi = 1.0 / (ph.R * T);
+        p = rho / psi;
+    }
+    const double rE = rho * (e + 0.5 * magSqr(U));
+    s.W[c] = rho;
+    s.W[(size_t)nC + c] = rE;
+    s.W[2 * (size_t)nC + c] = rho * U.x;
+    s.W[3 * (size_t)nC + c] = rho * U.y;
+    s.W[4 * (size_t)nC + c] = rho * U.z;
+    const double E = ph.Cv * T + 0.5 * magSqr(U);
+    const double H = fmax(E, kSMALL) + fmax(p / rho, kSMALL);
+    const double g = ph.gamma;
+    double* f = s.fld + (size_t)c * NFLD;
+    f[F_RHO] = rho;
+    f[F_P] = p;
+    f[F_UX] = U.x;
+    f[F_UY] = U.y;
+    f[F_UZ] = U.z;
+    f[F_E] = E;
+    f[F_H] = H;
+    f[F_CS] = sqrt(2.0 * (g - 1.0) / (g + 1.0) * H);
+    s.T[c] = T;
+    s.ecalc[c] = rE / rho - 0.5 * magSqr(U);
+}
+
+// Boundary values of p, U, T (fixedValue / zeroGradient / slip) and the
+// derived reconstruction fields on every non-empty boundary face.
+__global__ void k_boundary(MeshView m, Phys ph, StateView s) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nB) return;
+    if (m.bKind[b] == PK_EMPTY) return;
+    const int f = m.nI + b;
+    const int c = m.own[f];
+    const double* fc = s.fld + (size_t)c * NFLD;
+    const double pB = m.bPT[b] == BC_FIXED ? m.bPv[b] : fc[F_P];
+    const double TB = m.bTT[b] == BC_FIXED ? m.bTv[b] : s.T[c];
+    const V3 Uc = cellU(s.fld, c);
+    V3 UB;
+    if (m.bUT[b] == BC_FIXED) UB = ld3(m.bUv + 3 * (size_t)b);
+    else if (m.bUT[b] == BC_SLIP) {
+        const V3 n = ld3(m.Sf + 3 * (size_t)f) / m.magSf[f];
+        UB = (Uc + reflect(n, Uc)) / 2.0;
+    } else UB = Uc;
+    const double psi = 1.0 / (ph.R * TB);
+    const double rhoB = psi * pB;
+    const double E = ph.Cv * TB + 0.5 * magSqr(UB);
+    const double H = fmax(E, kSMALL) + fmax(pB / rhoB, kSMALL);
+    const double g = ph.gamma;
+    double* o = s.bfld + (size_t)b * NFLD;
+    o[F_RHO] = rhoB;
+    o[F_P] = pB;
+    o[F_UX] = UB.x;
+    o[F_UY] = UB.y;
+    o[F_UZ] = UB.z;
+    o[F_E] = E;
+    o[F_H] = H;
+    o[F_CS] = sqrt(2.0 * (g - 1.0) / (g + 1.0) * H);
+    s.bT[b] = TB;
+}
+
+__device__ __forceinline__ void limitFace(double& lim, double maxD, double minD, double ext, int limiter) {
+    double r = 1;
+    if (ext > kSMALL) r = maxD / ext;
+    else if (ext < -kSMALL) r = minD / ext;
+    double l;
+    if (limiter == 1) l = (sqr(r) + 2 * r) / (sqr(r) + r + 2);
+    else l = fmin(r, 1.0);
+    lim = fmin(lim, l);
+}
+
+// Gauss linear gradients with cellLimited<limiter> (k = 1) of the NG
+// reconstruction fields; unlimited grad(U) and grad(ecalc) when viscous.
+template <int NG>
+__global__ void __launch_bounds__(256) k_grad(MeshView m, Phys ph, StateView s) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    double gx[NG], gy[NG], gz[NG], mx[NG], mn[NG], vc[NG];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) {
+        vc[k] = s.fld[(size_t)c * NFLD + k];
+        gx[k] = gy[k] = gz[k] = 0.0;
+        mx[k] = mn[k] = vc[k];
+    }
+    const bool visc = ph.viscous;
+    V3 gex{0, 0, 0};
+    const double ecc = visc ? s.ecalc[c] : 0.0;
+    const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
+    for (int q = e0; q < e1; ++q) {
+        const int f = m.cf[q];
+        const V3 S = ld3(m.Sf + 3 * (size_t)f);
+        if (f < m.nI) {
+            const int o = m.own[f], n = m.nei[f];
+            const double w = m.w[f];
+            const bool isOwn = (o == c);
+            const int other = isOwn ? n : o;
+#pragma unroll
+            for (int k = 0; k < NG; ++k) {
+                const double vo = s.fld[(size_t)o * NFLD + k], vn = s.fld[(size_t)n * NFLD + k];
+                const double vf = w * (vo - vn) + vn;
+                const V3 t = S * vf;
+                if (isOwn) { gx[k] = gx[k] + t.x; gy[k] = gy[k] + t.y; gz[k] = gz[k] + t.z; }
+                else { gx[k] = gx[k] - t.x; gy[k] = gy[k] - t.y; gz[k] = gz[k] - t.z; }
+                const double vOther = s.fld[(size_t)other * NFLD + k];
+                mx[k] = fmax(mx[k], vOther);
+                mn[k] = fmin(mn[k], vOther);
+            }
+            if (visc) {
+                const double eo = s.ecalc[o], en = s.ecalc[n];
+                const double vf = w * (eo - en) + en;
+                const V3 t = S * vf;
+                gex = isOwn ? gex + t : gex - t;
+            }
+        } else {
+            const int b = f - m.nI;
+#pragma unroll
+            for (int k = 0; k < NG; ++k) {
+                const double vb = s.bfld[(size_t)b * NFLD + k];
+                const V3 t = S * vb;
+                gx[k] = gx[k] + t.x; gy[k] = gy[k] + t.y; gz[k] = gz[k] + t.z;
+                mx[k] = fmax(mx[k], vb);
+                mn[k] = fmin(mn[k], vb);
+            }
+            if (visc) {
+                const double* bf = s.bfld + (size_t)b * NFLD;
+                const V3 UB = V3{bf[F_UX], bf[F_UY], bf[F_UZ]};
+                const double rhoB = bf[F_RHO];
+                const double eB = ph.Cv * s.bT[b];
+                const double rhoEB = rhoB * (eB + 0.5 * magSqr(UB));
+                const double ebv = rhoEB / rhoB - 0.5 * magSqr(UB);
+                gex = gex + S * ebv;
+            }
+        }
+    }
+    const double Vc = m.V[c];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) { gx[k] = gx[k] / Vc; gy[k] = gy[k] / Vc; gz[k] = gz[k] / Vc; }
+    if (visc) {
+        // grad(U)_ij = d U_j / d x_i; column j is the gradient of U_j
+        double* g9 = s.gU + (size_t)c * 9;
+        g9[0] = gx[F_UX]; g9[1] = gx[F_UY]; g9[2] = gx[F_UZ];
+        g9[3] = gy[F_UX]; g9[4] = gy[F_UY]; g9[5] = gy[F_UZ];
+        g9[6] = gz[F_UX]; g9[7] = gz[F_UY]; g9[8] = gz[F_UZ];
+        gex = gex / Vc;
+        s.ge[3 * (size_t)c] = gex.x; s.ge[3 * (size_t)c + 1] = gex.y; s.ge[3 * (size_t)c + 2] = gex.z;
+    }
+    if (ph.recon == 1 && ph.limiter != 0) {
+        double lim[NG];
+#pragma unroll
+        for (int k = 0; k < NG; ++k) { mx[k] -= vc[k]; mn[k] -= vc[k]; lim[k] = 1.0; }
+        const V3 Cc = ld3(m.C + 3 * (size_t)c);
+        for (int q = e0; q < e1; ++q) {
+            const int f = m.cf[q];
+            const V3 d = ld3(m.Cf + 3 * (size_t)f) - Cc;
+#pragma unroll
+            for (int k = 0; k < NG; ++k)
+                limitFace(lim[k], mx[k], mn[k], dot(d, V3{gx[k], gy[k], gz[k]}), ph.limiter);
+        }
+#pragma unroll
+        for (int k = 0; k < NG; ++k) { gx[k] = gx[k] * lim[k]; gy[k] = gy[k] * lim[k]; gz[k] = gz[k] * lim[k]; }
+    }
+    double* go = s.grd + (size_t)c * NFLD * 3;
+#pragma unroll
+    for (int k = 0; k < NG; ++k) { go[3 * k] = gx[k]; go[3 * k + 1] = gy[k]; go[3 * k + 2] = gz[k]; }
+}
+
+// L (owner-side) and R (neighbour-side) reconstructed value of field k on
+// internal face f (linearUpwind with pos_/neg_ weights).
+__device__ __forceinline__ void recon(const MeshView& m, const StateView& s, int recon, int f, int o, int n, int k,
+                                      double& L, double& R) {
+    const double vo = s.fld[(size_t)o * NFLD + k], vn = s.fld[(size_t)n * NFLD + k];
+    double l = 1.0 * (vo - vn) + vn;
+    double r = 0.0 * (vo - vn) + vn;
+    if (recon == 1) {
+        const V3 fc = ld3(m.Cf + 3 * (size_t)f);
+        const double* go = s.grd + ((size_t)o * NFLD + k) * 3;
+        const double* gn = s.grd + ((size_t)n * NFLD + k) * 3;
+        l += dot(fc - ld3(m.C + 3 * (size_t)o), V3{go[0], go[1], go[2]});
+        r += dot(fc - ld3(m.C + 3 * (size_t)n), V3{gn[0], gn[1], gn[2]});
+    }
+    L = l;
+    R = r;
+}
+
+__device__ __forceinline__ double soundSpeed(const Phys& ph, double T) {
+    const double psi = 1.0 / (ph.R * T);
+    return sqrt(ph.gamma / psi);
+}
+
+// Convective flux of face f (internal or boundary) -> (rho, rhoE, rhoU).
+__device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, const StateView& s, int f, double& fr,
+                                         double& fe, V3& fu) {
+    const V3 S = ld3(m.Sf + 3 * (size_t)f);
+    const double ms = m.magSf[f];
+    double rL, rR, pL, pR, uxL, uxR, uyL, uyR, uzL, uzR, HL, HR, xL, xR;
+    if (f < m.nI) {
+        const int o = m.own[f], n = m.nei[f];
+        recon(m, s, ph.recon, f, o, n, F_UX, uxL, uxR);
+        recon(m, s, ph.recon, f, o, n, F_UY, uyL, uyR);
+        recon(m, s, ph.recon, f, o, n, F_UZ, uzL, uzR);
+        recon(m, s, ph.recon, f, o, n, F_P, pL, pR);
+        recon(m, s, ph.recon, f, o, n, F_RHO, rL, rR);
+        recon(m, s, ph.recon, f, o, n, F_H, HL, HR);
+        recon(m, s, ph.recon, f, o, n, ph.flux == FLUX_AUSM ? F_CS : F_E, xL, xR);
+    } else {
+        const double* bf = s.bfld + (size_t)(f - m.nI) * NFLD;
+        rL = rR = bf[F_RHO];
+        pL = pR = bf[F_P];
+        uxL = uxR = bf[F_UX];
+        uyL = uyR = bf[F_UY];
+        uzL = uzR = bf[F_UZ];
+        HL = HR = bf[F_H];
+        xL = xR = bf[ph.flux == FLUX_AUSM ? F_CS : F_E];
+    }
+    const V3 UL{uxL, uyL, uzL}, UR{uxR, uyR, uzR};
+    if (ph.flux == FLUX_AUSM)
+        ausmFace(ph.lowMach, S, ms, UL, UR, xL, xR, pL, pR, rL, rR, HL, HR, fr, fu, fe);
+    else
+        roeFace(ph.gamma, ph.entropyFix, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, HL, HR, fr, fu, fe);
+}
+
+// Residual b = R*V (residualsUpdate.H) and rPseudoDeltaT
+// (setCoAndDeltaT.H:57-141) for each cell.
+__global__ void __launch_bounds__(256) k_residual(MeshView m, Phys ph, StateView s) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    double ar = 0, ae = 0;
+    V3 au{0, 0, 0};
+    V3 aLap{0, 0, 0}, aTau{0, 0, 0};
+    double aSig = 0, aLapE = 0;
+    double rdt = 0.0;
+    const bool visc = ph.viscous;
+    const double mu = ph.mu, alpha = ph.alpha;
+    const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
+    for (int q = e0; q < e1; ++q) {
+        const int f = m.cf[q];
+        double fr, fe;
+        V3 fu;
+        faceFlux(m, ph, s, f, fr, fe, fu);
+        const V3 S = ld3(m.Sf + 3 * (size_t)f);
+        const double ms = m.magSf[f];
+        if (f < m.nI) {
+            const int o = m.own[f], n = m.nei[f];
+            const bool isOwn = (o == c);
+            if (isOwn) { ar += fr; ae += fe; au = au + fu; }
+            else { ar -= fr; ae -= fe; au = au - fu; }
+            // local time step: frdt = nonOrthDeltaCoeffs*lambda
+            const double w = m.w[f];
+            const double co = soundSpeed(ph, s.T[o]), cn = soundSpeed(ph, s.T[n]);
+            const double cf = w * (co - cn) + cn;
+            const V3 Uo = cellU(s.fld, o), Un = cellU(s.fld, n);
+            const V3 Uf = w * (Uo - Un) + Un;
+            const double lam = cf + fabs(dot(Uf, S / ms));
+            rdt = fmax(rdt, m.nodc[f] * lam);
+            if (visc) {
+                const double* g9o = s.gU + (size_t)o * 9;
+                const double* g9n = s.gU + (size_t)n * 9;
+                T9 gUo, gUn;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) { gUo.t[i] = g9o[i]; gUn.t[i] = g9n[i]; }
+                const T9 gUf = w * (gUo - gUn) + gUn;
+                const T9 tauO = mu * dev2(transposeT(gUo));
+                const T9 tauN = mu * dev2(transposeT(gUn));
+                const T9 tauf = w * (tauO - tauN) + tauN;
+                const double muf = w * (mu - mu) + mu;
+                const double alf = w * (alpha - alpha) + alpha;
+                const V3 k3 = ld3(m.corr + 3 * (size_t)f);
+                const V3 sn = m.nodc[f] * (Un - Uo) + vdott(k3, gUf);
+                const V3 lap = (muf * sn) * ms;
+                const V3 dt = vdott(S, tauf);
+                const V3 sig = tdotv(muf * gUf + tauf, Uf);
+                const double sdu = dot(sig, S);
+                const V3 geO = ld3(s.ge + 3 * (size_t)o), geN = ld3(s.ge + 3 * (size_t)n);
+                const V3 gef = w * (geO - geN) + geN;
+                const double sne = m.nodc[f] * (s.ecalc[n] - s.ecalc[o]) + dot(k3, gef);
+                const double lapE = (alf * sne) * ms;
+                if (isOwn) { aLap = aLap + lap; aTau = aTau + dt; aSig += sdu; aLapE += lapE; }
+                else { aLap = aLap - lap; aTau = aTau - dt; aSig -= sdu; aLapE -= lapE; }
+            }
+        } else {
+            const int b = f - m.nI;
+            ar += fr; ae += fe; au = au + fu;
+            if (m.bKind[b] == PK_WALL) {
+                const double pl = 0.5 * m.nodc[f] * (soundSpeed(ph, s.T[c]) + fabs(dot(cellU(s.fld, c), S / ms)));
+                rdt = fmax(rdt, pl);
+            }
+            if (visc) {
+                const double* bf = s.bfld + (size_t)b * NFLD;
+                const V3 n = S / ms;
+                const V3 Uc = cellU(s.fld, c);
+                const V3 UB{bf[F_UX], bf[F_UY], bf[F_UZ]};
+                V3 snU;
+                const int ut = m.bUT[b];
+                if (ut == BC_ZG) snU = V3{0, 0, 0};
+                else if (ut == BC_SLIP) snU = (reflect(n, Uc) - Uc) * (m.dc[f] / 2.0);
+                else snU = m.dc[f] * (UB - Uc);
+                T9 gUc;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) gUc.t[i] = s.gU[(size_t)c * 9 + i];
+                const T9 gUb = gUc + outer(n, snU - vdott(n, gUc));
+                const T9 taub = mu * dev2(transposeT(gUb));
+                const V3 lap = (mu * snU) * ms;
+                const V3 dt = vdott(S, taub);
+                const V3 sig = tdotv(mu * gUb + taub, UB);
+                const double sdu = dot(sig, S);
+                const double rhoB = bf[F_RHO];
+                const double rhoEB = rhoB * (ph.Cv * s.bT[b] + 0.5 * magSqr(UB));
+                const double eb = rhoEB / rhoB - 0.5 * magSqr(UB);
+                const double lapE = (alpha * (m.dc[f] * (eb - s.ecalc[c]))) * ms;
+                aLap = aLap + lap; aTau = aTau + dt; aSig += sdu; aLapE += lapE;
+            }
+        }
+    }
+    const double Vc = m.V[c];
+    const int nC = m.nC;
+    double Rr = -(ar / Vc);
+    double Re = -(ae / Vc);
+    V3 Ru = V3{-(au.x / Vc), -(au.y / Vc), -(au.z / Vc)};
+    if (visc) {
+        Ru = Ru + aLap / Vc;
+        Ru = Ru + aTau / Vc;
+        Re += aSig / Vc;
+        Re += aLapE / Vc;
+    }
+    s.b[c] = Rr * Vc;
+    s.b[(size_t)nC + c] = Re * Vc;
+    s.b[2 * (size_t)nC + c] = Ru.x * Vc;
+    s.b[3 * (size_t)nC + c] = Ru.y * Vc;
+    s.b[4 * (size_t)nC + c] = Ru.z * Vc;
+    s.rDT[c] = rdt / s.pseudoCo[c];
+}
+
+// Face fluxes of every face (convectiveFluxScheme::calcFlux output), for
+// parity checks; fluxes of empty faces are zero.
+__global__ void k_face_flux(MeshView m, Phys ph, StateView s, double* phi, double* phiU, double* phiE) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nF) return;
+    if (f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY) {
+        phi[f] = 0; phiE[f] = 0;
+        phiU[3 * (size_t)f] = phiU[3 * (size_t)f + 1] = phiU[3 * (size_t)f + 2] = 0;
+        return;
+    }
+    double fr, fe;
+    V3 fu;
+    faceFlux(m, ph, s, f, fr, fe, fu);
+    phi[f] = fr;
+    phiE[f] = fe;
+    phiU[3 * (size_t)f] = fu.x;
+    phiU[3 * (size_t)f + 1] = fu.y;
+    phiU[3 * (size_t)f + 2] = fu.z;
+}
+
+// Row assembly of the coupled Jacobian: off-diagonal 5x5 blocks of cell c
+// into its sliced-ELL slots, the diagonal block, and the LU-SGS scalar
+// reciprocal diagonal.  Reference: convectiveFluxScheme::addFluxTerms,
+// addDissipationJacobian, addBoundaryTerms, addTemporalTerms
+// (convectiveFluxScheme.C:219-546), viscousFluxScheme::addFluxTerms
+// (viscousFluxScheme.C:220-247), blockFvMatrix::insertBlock /
+// insertDissipationBlock (blockFvMatrix.C:212-326), lusgs::lusgs
+// (lusgs.C:50-125).
+template <typename TB>
+__global__ void __launch_bounds__(256) k_jacobian(MeshView m, Phys ph, StateView s, TB* __restrict__ blk,
+                                                  TB* __restrict__ dg, double* __restrict__ rD,
+                                                  double* __restrict__ dgOut64, int* __restrict__ err) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const double g = ph.gamma;
+    double D[NB];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) D[e] = 0.0;
+    double diss = 0.0, vis = 0.0;
+    const bool visc = ph.viscous;
+    const double mu = ph.mu, alpha = ph.alpha;
+    const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
+    const int sl = c >> 6, lane = c & 63;
+    const size_t rowBase = (size_t)m.slicePtr[sl] * 64 + lane;
+    int slot = 0;
+    // internal faces: off-diagonal blocks + convective part of the diagonal
+    for (int q = e0; q < e1; ++q) {
+        const int f = m.cf[q];
+        if (f >= m.nI) continue;
+        const int o = m.own[f], n = m.nei[f];
+        const bool isOwn = (o == c);
+        const V3 S = ld3(m.Sf + 3 * (size_t)f);
+        const double ms = m.magSf[f];
+        const V3 nrm = S / ms;
+        double uxL, uxR, uyL, uyR, uzL, uzR, EL, ER;
+        recon(m, s, ph.recon, f, o, n, F_UX, uxL, uxR);
+        recon(m, s, ph.recon, f, o, n, F_UY, uyL, uyR);
+        recon(m, s, ph.recon, f, o, n, F_UZ, uzL, uzR);
+        recon(m, s, ph.recon, f, o, n, F_E, EL, ER);
+        // own side needs: block = upper (A_R), diagonal -= lower (A_L);
+        // neighbour side: block = lower (A_L), diagonal -= upper (A_R)
+        double A[NB];
+        fluxJacobian(g, isOwn ? V3{uxR, uyR, uzR} : V3{uxL, uyL, uzL}, isOwn ? ER : EL, nrm, A);
+        const double hb = isOwn ? 0.5 * ms : -0.5 * ms;
+        double Bk[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) Bk[e] = hb * A[e];
+        fluxJacobian(g, isOwn ? V3{uxL, uyL, uzL} : V3{uxR, uyR, uzR}, isOwn ? EL : ER, nrm, A);
+        const double hd = isOwn ? -0.5 * ms : 0.5 * ms;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) D[e] -= hd * A[e];
+        // dissipation (insertDissipationBlock, then operator-=)
+        const double w = m.w[f];
+        const double co = soundSpeed(ph, s.T[o]), cn = soundSpeed(ph, s.T[n]);
+        const double cfc = w * (co - cn) + cn;
+        const V3 Uo = cellU(s.fld, o), Un = cellU(s.fld, n);
+        const V3 Uf = w * (Uo - Un) + Un;
+        const double lam = cfc + fabs(dot(Uf, S / ms));
+        const double v = 0.5 * ms * lam;
+        Bk[0] = Bk[0] - v; Bk[6] = Bk[6] - v; Bk[12] = Bk[12] - v; Bk[18] = Bk[18] - v; Bk[24] = Bk[24] - v;
+        diss -= v;
+        if (visc) {
+            const double rf = w * (s.fld[(size_t)o * NFLD + F_RHO] - s.fld[(size_t)n * NFLD + F_RHO]) +
+                              s.fld[(size_t)n * NFLD + F_RHO];
+            const double muf = w * (mu - mu) + mu;
+            const double alf = w * (alpha - alpha) + alpha;
+            const double lv = (muf + alf) / rf;
+            const double sv = 0.5 * lv * ms * m.dc[f];
+            Bk[0] = Bk[0] - sv; Bk[6] = Bk[6] - sv; Bk[12] = Bk[12] - sv; Bk[18] = Bk[18] - sv; Bk[24] = Bk[24] - sv;
+            vis -= sv;
+        }
+        const size_t sbase = rowBase + (size_t)slot * 64;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) blk[((sbase >> 6) * NB + e) * 64 + (sbase & 63)] = (TB)Bk[e];
+        ++slot;
+    }
+    // boundary dissipation (all non-empty patches) follows the internal faces
+    for (int q = e0; q < e1; ++q) {
+        const int f = m.cf[q];
+        if (f < m.nI) continue;
+        const int b = f - m.nI;
+        const V3 S = ld3(m.Sf + 3 * (size_t)f);
+        const double ms = m.magSf[f];
+        const double* bf = s.bfld + (size_t)b * NFLD;
+        const double lamB = soundSpeed(ph, s.bT[b]) + fabs(dot(V3{bf[F_UX], bf[F_UY], bf[F_UZ]}, S / ms));
+        diss -= 0.5 * ms * lamB;
+    }
+    D[0] = D[0] - diss; D[6] = D[6] - diss; D[12] = D[12] - diss; D[18] = D[18] - diss; D[24] = D[24] - diss;
+    // boundary terms (addBoundaryTerms), patch/face order
+    const double Cv = ph.Cv;
+    const double rhoI = s.fld[(size_t)c * NFLD + F_RHO];
+    const V3 UI = cellU(s.fld, c);
+    for (int q = e0; q < e1; ++q) {
+        const int f = m.cf[q];
+        if (f < m.nI) continue;
+        const int b = f - m.nI;
+        const V3 Sf = ld3(m.Sf + 3 * (size_t)f);
+        const V3 n = Sf / m.magSf[f];
+        const double pVIC = m.bPT[b] == BC_FIXED ? 0.0 : 1.0;
+        const double tVIC = m.bTT[b] == BC_FIXED ? 0.0 : 1.0;
+        V3 uVIC;
+        if (m.bUT[b] == BC_FIXED) uVIC = V3{0, 0, 0};
+        else if (m.bUT[b] == BC_SLIP) uVIC = V3{1 - fabs(n.x), 1 - fabs(n.y), 1 - fabs(n.z)};
+        else uVIC = V3{1, 1, 1};
+        const double* bf = s.bfld + (size_t)b * NFLD;
+        const double rhoB = bf[F_RHO], pB = bf[F_P], TB_ = s.bT[b];
+        const V3 UB{bf[F_UX], bf[F_UY], bf[F_UZ]};
+        const double UrelSf = dot(UB, Sf);
+        const double rhoEB = rhoB * (Cv * TB_ + 0.5 * magSqr(UB));
+        const V3 SfV{Sf.x * uVIC.x, Sf.y * uVIC.y, Sf.z * uVIC.z};
+        const double dCdp = rhoB / pB * UrelSf * pVIC;
+        const V3 dCdU = rhoB * SfV;
+        const double dCdT = -rhoB / TB_ * UrelSf * tVIC;
+        const V3 dMdp = (rhoB / pB * UB * UrelSf + Sf) * pVIC;
+        T9 dMdU = outer(rhoB * UB, SfV);
+        const V3 dMdUDiag = rhoB * UrelSf * uVIC;
+        const V3 dMdT = -rhoB / TB_ * UB * UrelSf * tVIC;
+        const double dEdp = (rhoEB / pB * UrelSf + dot(UB, Sf)) * pVIC;
+        const V3 UBV{UB.x * uVIC.x, UB.y * uVIC.y, UB.z * uVIC.z};
+        const V3 dEdU = SfV * (rhoEB + pB) + rhoB * UrelSf * UBV;
+        const double dEdT = UrelSf * (rhoB * Cv - rhoEB / TB_) * tVIC;
+        dMdU.t[0] += dMdUDiag.x; dMdU.t[4] += dMdUDiag.y; dMdU.t[8] += dMdUDiag.z;
+        const double rhoEI = rhoI * (Cv * s.T[c] + 0.5 * magSqr(UI));
+        const double dPdRho = 0.5 * (g - 1) * magSqr(UI);
+        const V3 dUdRho{-UI.x / rhoI, -UI.y / rhoI, -UI.z / rhoI};
+        const double dTdRho = -1.0 / (Cv * rhoI) * (rhoEI / rhoI - magSqr(UI));
+        const V3 dPdRhoU = -(g - 1) * UI;
+        const double dUdRhoU = 1.0 / rhoI;
+        const V3 dTdRhoU{-UI.x / (Cv * rhoI), -UI.y / (Cv * rhoI), -UI.z / (Cv * rhoI)};
+        const double dPdRhoE = g - 1;
+        const double dTdRhoE = 1.0 / (Cv * rhoI);
+        // continuity row
+        D[0] += dCdp * dPdRho; D[0] += dot(dCdU, dUdRho); D[0] += dCdT * dTdRho;
+        {
+            const V3 t1 = dCdp * dPdRhoU, t2 = dCdU * dUdRhoU, t3 = dCdT * dTdRhoU;
+            D[2] += t1.x; D[3] += t1.y; D[4] += t1.z;
+            D[2] += t2.x; D[3] += t2.y; D[4] += t2.z;
+            D[2] += t3.x; D[3] += t3.y; D[4] += t3.z;
+        }
+        D[1] += dCdp * dPdRhoE; D[1] += 0.0; D[1] += dCdT * dTdRhoE;
+        // momentum rows
+        {
+            const V3 a = dMdp * dPdRho, bb = tdotv(dMdU, dUdRho), cc = dMdT * dTdRho;
+            D[10] += a.x; D[15] += a.y; D[20] += a.z;
+            D[10] += bb.x; D[15] += bb.y; D[20] += bb.z;
+            D[10] += cc.x; D[15] += cc.y; D[20] += cc.z;
+            const T9 A1 = outer(dMdp, dPdRhoU), B1 = dMdU * dUdRhoU, C1 = outer(dMdT, dTdRhoU);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) D[5 * (2 + i) + 2 + j] += A1.t[3 * i + j];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) D[5 * (2 + i) + 2 + j] += B1.t[3 * i + j];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) D[5 * (2 + i) + 2 + j] += C1.t[3 * i + j];
+            const V3 e1v = dMdp * dPdRhoE, e3v = dMdT * dTdRhoE;
+            D[11] += e1v.x; D[16] += e1v.y; D[21] += e1v.z;
+            D[11] += 0.0; D[16] += 0.0; D[21] += 0.0;
+            D[11] += e3v.x; D[16] += e3v.y; D[21] += e3v.z;
+        }
+        // energy row
+        D[5] += dEdp * dPdRho; D[5] += dot(dEdU, dUdRho); D[5] += dEdT * dTdRho;
+        {
+            const V3 f1 = dEdp * dPdRhoU, f2 = dEdU * dUdRhoU, f3 = dEdT * dTdRhoU;
+            D[7] += f1.x; D[8] += f1.y; D[9] += f1.z;
+            D[7] += f2.x; D[8] += f2.y; D[9] += f2.z;
+            D[7] += f3.x; D[8] += f3.y; D[9] += f3.z;
+        }
+        D[6] += dEdp * dPdRhoE; D[6] += 0.0; D[6] += dEdT * dTdRhoE;
+    }
+    // temporal term: ddtCoeff = (rDT*V)/V, diag += ddtCoeff*V
+    {
+        const double Vc = m.V[c];
+        const double ddtCoeff = (s.rDT[c] * Vc) / Vc;
+        const double dcv = ddtCoeff * Vc;
+        D[0] += dcv; D[12] += dcv; D[18] += dcv; D[24] += dcv; D[6] += dcv;
+    }
+    if (visc) { D[0] = D[0] - vis; D[12] = D[12] - vis; D[18] = D[18] - vis; D[24] = D[24] - vis; D[6] = D[6] - vis; }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) dg[diagOff(c, e)] = (TB)D[e];
+    if (dgOut64)
+#pragma unroll
+        for (int e = 0; e < NB; ++e) dgOut64[diagOff(c, e)] = D[e];
+    // LU-SGS scalar diagonal
+    double r = kGREAT;
+    r = 1.0 / fmax(1.0 / r, fabs(D[0]));
+    r = 1.0 / fmax(1.0 / r, fabs(D[6]));
+    r = 1.0 / fmax(1.0 / r, fabs(D[12]));
+    r = 1.0 / fmax(1.0 / r, fabs(D[18]));
+    r = 1.0 / fmax(1.0 / r, fabs(D[24]));
+    rD[c] = r;
+    if (!(r >= kVSMALL)) atomicOr(err, 1);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers --
+static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const double* dW, int mode,
+                  const double* rho0, const double* U0, const double* T0, hipStream_t st) {
+    k_update<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, dW, mode, rho0, U0, T0);
+}
+
+void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st) {
+    if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
+    if (ph.flux == FLUX_AUSM) k_grad<8><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
+    else k_grad<7><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
+    k_residual<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
+}
+
+void launchFaceFlux(const MeshView& m, const Phys& ph, const StateView& s, double* phi, double* phiU, double* phiE,
+                    hipStream_t st) {
+    k_face_flux<<<nblk(m.nF, 256), 256, 0, st>>>(m, ph, s, phi, phiU, phiE);
+}
+
+void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
+                    double* rD, double* dg64, int* err, hipStream_t st) {
+    if (fp32)
+        k_jacobian<float><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, dg64, err);
+    else
+        k_jacobian<double><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, dg64, err);
+}
+
+}  // namespace icsf

@@ -1,0 +1,993 @@
+// Host driver and C ABI of the implicit coupled pseudo-time step.
+//
+// Mirrors dbnsFoam's per-step sequence (applications/solvers/dbnsFoam:
+// outerLoop.H, setCoAndDeltaT.H, boundLocalTimeStep.H, updateFields.H) with
+// all fields, the block Jacobian and the Krylov basis resident on the GPU;
+// the only host<->device traffic inside a step is kernel launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "icsf.h"
+#include "icsf_dict.h"
+#include "icsf_kernels.h"
+#include "icsf_math.h"
+#include "icsf_mesh.h"
+
+using namespace icsf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(ICSF_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int checkDevice() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+        return fail(ICSF_ERR_NODEVICE, "no HIP device is available; the coupled solver runs only on the GPU");
+    return ICSF_OK;
+}
+
+struct DevBuf {
+    std::vector<void*> ptrs;
+    template <class T>
+    int alloc(T*& p, size_t n) {
+        p = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(ICSF_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        ptrs.push_back(p);
+        return ICSF_OK;
+    }
+    template <class T>
+    int upload(T*& p, const T* h, size_t n, hipStream_t st = nullptr) {
+        int rc = alloc(p, n);
+        if (rc) return rc;
+        if (n) HIPCHK(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice));
+        (void)st;
+        return ICSF_OK;
+    }
+    ~DevBuf() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+#define RC(x)                   \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_) return rc_;    \
+    } while (0)
+
+}  // namespace
+
+// ------------------------------------------------------------------ mesh --
+struct icsf_mesh {
+    HostMesh h;
+    SellLayout sell;
+    Levels lev;
+    DevBuf buf;
+    MeshView view{};
+};
+
+// ------------------------------------------------------------------ case --
+enum { NTIMER = 6 };
+
+struct icsf_case {
+    icsf_mesh* mesh = nullptr;
+    hipStream_t st = nullptr;
+    Phys ph{};
+    // solution controls
+    int solver = 0;      // 0 GMRES (left, reference), 1 FGMRES (right, outer)
+    int precond = 0;     // 0 LUSGS, 1 Jacobi, 2 none
+    int nDirs = 30, maxIter = 1, minIter = 0, innerDirs = 30, orth = 0, mixed = 0;
+    double tol = 1e-12, relTol = 1e-2;
+    double co = 1, coMin = 0.1, coMax = 25, coIncr = 1.25, coDecr = 0.1, lowerBound = 0.95;
+    int localTS = 1, bounding = 1;
+    // boundary conditions (per boundary face)
+    std::vector<int> bPT, bUT, bTT;
+    std::vector<double> bPv, bUv, bTv;
+    int *d_bPT = nullptr, *d_bUT = nullptr, *d_bTT = nullptr;
+    double *d_bPv = nullptr, *d_bUv = nullptr, *d_bTv = nullptr;
+    bool bcDirty = true;
+    MeshView mv{};
+    StateView sv{};
+    // Jacobian
+    void *blk = nullptr, *dg = nullptr, *dinv = nullptr;
+    double* rD = nullptr;
+    int* derr = nullptr;
+    int blkFp32 = 0;
+    bool haveResidual = false, assembled = false, haveFields = false;
+    // Krylov work (FP64)
+    double *V = nullptr, *r = nullptr, *t1 = nullptr, *t2 = nullptr, *dW = nullptr, *Z = nullptr;
+    double* dres = nullptr;  // [2][5] init / previous initial residuals
+    double* fac = nullptr;
+    // FP32 inner solve
+    float *V32 = nullptr, *r32 = nullptr;
+    GmresScalars* g = nullptr;
+    GmresScalars* g2 = nullptr;
+    Work wk;
+    // pseudo-time
+    int corr = 0;
+    bool haveInit = false, havePrev = false;
+    // timing
+    bool timing = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
+    double tms[NTIMER] = {0};
+    long long tcnt[NTIMER] = {0};
+    DevBuf buf;
+    ~icsf_case() {
+        for (auto& e : events) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace {
+
+struct Timed {
+    icsf_case* c;
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(icsf_case* cc, int k) : c(cc), cls(k) {
+        if (c->timing) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, c->st);
+        }
+    }
+    ~Timed() {
+        if (c->timing) {
+            (void)hipEventRecord(b, c->st);
+            c->events.push_back({cls, {a, b}});
+        }
+    }
+};
+
+void resolveEvents(icsf_case* c) {
+    if (c->events.empty()) return;
+    (void)hipStreamSynchronize(c->st);
+    for (auto& e : c->events) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e.second.first, e.second.second);
+        c->tms[e.first] += ms;
+        c->tcnt[e.first] += 1;
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
+    }
+    c->events.clear();
+}
+
+// ---- device kernels local to the driver --------------------------------
+__global__ void k_ser(int nC, double* co, const double* dres, double coIncr, double coDecr, double coMin,
+                      double coMax) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    const double* ini = dres;      // latest initial residuals
+    const double* prv = dres + 5;  // previous
+    const double nI = sqrt(ini[0] * ini[0] + ini[1] * ini[1] + (ini[2] * ini[2] + ini[3] * ini[3] + ini[4] * ini[4]));
+    const double nP = sqrt(prv[0] * prv[0] + prv[1] * prv[1] + (prv[2] * prv[2] + prv[3] * prv[3] + prv[4] * prv[4]));
+    double ratio = nP / nI;
+    ratio = fmax(fmin(ratio, coIncr), coDecr);
+    double v = co[c] * ratio;
+    v = fmax(fmin(v, coMax), coMin);
+    co[c] = v;
+}
+
+__global__ void k_copy5(const double* a, double* b) {
+    if (threadIdx.x < 5) b[threadIdx.x] = a[threadIdx.x];
+}
+
+// boundLocalTimeStep.H:174-271 evaluated before the state update (as in
+// outerLoop.H:99), so the state compared is the pre-solve state.
+__global__ void k_bound_flag(int nC, double lb, const double* W, double* bad) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    const double rho = W[c], rE = W[(size_t)nC + c];
+    const double ux = W[2 * (size_t)nC + c] / rho, uy = W[3 * (size_t)nC + c] / rho, uz = W[4 * (size_t)nC + c] / rho;
+    const double rhoMin = lb * rho;
+    const double eMin = lb * (rE / rho - 0.5 * (ux * ux + uy * uy + uz * uz));
+    const double eT = rE / rho - 0.5 * (ux * ux + uy * uy + uz * uz);
+    bad[c] = (rho < rhoMin || eT < eMin || eT < 1.0e-15) ? 1.0 : 0.0;
+}
+
+__global__ void k_bound_apply(int nC, int nI, const int* cfOff, const int* cf, const int* own, const int* nei,
+                              const double* bad, double* co) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    double f = 1.0;
+    for (int q = cfOff[c]; q < cfOff[c + 1]; ++q) {
+        const int fc = cf[q];
+        if (fc >= nI) continue;
+        const int other = own[fc] == c ? nei[fc] : own[fc];
+        if (bad[c] != 0.0) f = fmin(0.5, f);
+        if (bad[other] != 0.0) f = fmin(0.75, f);
+    }
+    co[c] *= f;
+}
+
+__global__ void k_fill(int n, double v, double* x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+int nblk(size_t n, int b) { return (int)((n + b - 1) / b); }
+
+// ---- settings -----------------------------------------------------------
+int readSettings(icsf_case* c, const char* thermo, const char* schemes, const char* solution) {
+    Dict td, sd, fd;
+    std::string err;
+    if (!(err = parseDict(thermo ? thermo : "", td)).empty()) return fail(ICSF_ERR_ARG, "thermophysicalProperties: " + err);
+    if (!(err = parseDict(schemes ? schemes : "", sd)).empty()) return fail(ICSF_ERR_ARG, "fvSchemes: " + err);
+    if (!(err = parseDict(solution ? solution : "", fd)).empty()) return fail(ICSF_ERR_ARG, "fvSolution: " + err);
+    Phys& ph = c->ph;
+    // thermophysical model: hePsiThermo / hConst / perfectGas, e = Cv T
+    double R = td.scalar("R", -1);
+    const double molW = td.scalar("mixture/specie/molWeight", 28.966);
+    if (R <= 0) R = 8314.47 / molW;
+    double gamma = td.scalar("gamma", -1);
+    const double Cp = td.scalar("mixture/thermodynamics/Cp", 1005.0);
+    if (gamma <= 0) gamma = Cp / (Cp - R);
+    ph.R = R;
+    ph.gamma = gamma;
+    ph.Cv = R / (gamma - 1.0);
+    ph.mu = td.scalar("mixture/transport/mu", 0.0);
+    const double Pr = td.scalar("mixture/transport/Pr", 1.0);
+    ph.alpha = gamma * ph.mu / Pr;  // alphahe = Cp/Cv * kappa/Cp
+    ph.viscous = ph.mu > 0 ? 1 : 0;
+    // fvSchemes
+    const std::string fs = sd.word("convectiveFluxScheme/fluxScheme", "");
+    if (fs == "AUSMPlusUp") ph.flux = FLUX_AUSM;
+    else if (fs == "ROE") ph.flux = FLUX_ROE;
+    else
+        return fail(ICSF_ERR_KEYWORD, "Unknown convectiveFluxScheme type '" + fs +
+                                          "'; valid types are AUSMPlusUp ROE");
+    ph.lowMach = sd.sw("convectiveFluxScheme/lowMachAusm", true) ? 1 : 0;
+    ph.entropyFix = sd.scalar("convectiveFluxScheme/entropyFixCoeff", 0.05);
+    const auto* ddt = sd.find("ddtSchemes/default");
+    if (!ddt || ddt->empty() || ddt->back() != "steadyState" || (*ddt)[0] != "dualTime")
+        return fail(ICSF_ERR_KEYWORD, "ddtSchemes default must be 'dualTime rPseudoDeltaT steadyState'");
+    // reconstruction: all three reconstruct(...) entries must agree
+    std::string recon;
+    for (const char* k : {"reconstruct(rho)", "reconstruct(U)", "reconstruct(T)"}) {
+        const auto* v = sd.find(std::string("interpolationSchemes/") + k);
+        std::string s;
+        if (v) for (size_t i = 0; i < v->size(); ++i) s += (i ? " " : "") + (*v)[i];
+        if (recon.empty()) recon = s;
+        else if (s != recon) return fail(ICSF_ERR_KEYWORD, "reconstruct(rho|U|T) schemes must be identical");
+    }
+    ph.limiter = 0;
+    if (recon == "upwind") ph.recon = 0;
+    else if (recon.rfind("linearUpwind", 0) == 0) {
+        ph.recon = 1;
+        std::string gname = recon.size() > 13 ? recon.substr(13) : "default";
+        const auto* gs = sd.find("gradSchemes/" + gname);
+        if (!gs) gs = sd.find("gradSchemes/default");
+        std::string g;
+        if (gs) for (size_t i = 0; i < gs->size(); ++i) g += (i ? " " : "") + (*gs)[i];
+        if (g == "Gauss linear") ph.limiter = 0;
+        else if (g == "cellLimited<Venkatakrishnan> Gauss linear 1") ph.limiter = 1;
+        else if (g == "cellLimited<minmod> Gauss linear 1" || g == "cellLimited Gauss linear 1") ph.limiter = 2;
+        else return fail(ICSF_ERR_KEYWORD, "unsupported gradScheme '" + g + "'");
+    } else return fail(ICSF_ERR_KEYWORD, "unsupported reconstruction scheme '" + recon + "'");
+    // fvSolution
+    const std::string sv = fd.word("flowSolver/solver", "GMRES");
+    if (sv == "GMRES") c->solver = 0;
+    else if (sv == "FGMRES") c->solver = 1;
+    else return fail(ICSF_ERR_KEYWORD, "Unknown coupled solver '" + sv + "'; valid solvers are GMRES FGMRES");
+    const std::string base = "flowSolver/" + sv + "/";
+    const std::string pc = fd.word(base + "preconditioner", "LUSGS");
+    if (pc == "LUSGS") c->precond = 0;
+    else if (pc == "Jacobi") c->precond = 1;
+    else if (pc == "none") c->precond = 2;
+    else return fail(ICSF_ERR_KEYWORD, "Unknown preconditioner '" + pc + "'; valid are LUSGS Jacobi none");
+    c->nDirs = (int)fd.scalar(base + "nDirections", 30);
+    c->maxIter = (int)fd.scalar(base + "maxIter", 1000);
+    c->minIter = (int)fd.scalar(base + "minIter", 0);
+    c->tol = fd.scalar(base + "tolerance", 1e-12);
+    c->relTol = fd.scalar(base + "relTol", 1e-2);
+    c->innerDirs = (int)fd.scalar(base + "innerDirections", 30);
+    const std::string prec = fd.word(base + "precision", "double");
+    if (prec == "double") c->mixed = 0;
+    else if (prec == "mixed") c->mixed = 1;
+    else return fail(ICSF_ERR_KEYWORD, "precision must be double or mixed");
+    const std::string orth = fd.word(base + "orthogonalisation", "MGS");
+    if (orth == "MGS") c->orth = 0;
+    else if (orth == "CGS") c->orth = 1;
+    else return fail(ICSF_ERR_KEYWORD, "orthogonalisation must be MGS or CGS");
+    if (c->nDirs < 1 || c->nDirs > KMAX || c->innerDirs < 1 || c->innerDirs > KMAX)
+        return fail(ICSF_ERR_ARG, "nDirections must be in [1, 64]");
+    if (c->mixed && c->solver != 1)
+        return fail(ICSF_ERR_KEYWORD, "precision mixed requires solver FGMRES (FP64 outer, FP32 inner)");
+    c->co = fd.scalar("pseudoTime/pseudoCoNum", 1.0);
+    c->coMin = fd.scalar("pseudoTime/pseudoCoNumMin", 0.1);
+    c->coMax = fd.scalar("pseudoTime/pseudoCoNumMax", 25.0);
+    c->coIncr = fd.scalar("pseudoTime/pseudoCoNumMaxIncreaseFactor", 1.25);
+    c->coDecr = fd.scalar("pseudoTime/pseudoCoNumMinDecreaseFactor", 0.1);
+    c->localTS = fd.sw("pseudoTime/localTimestepping", true) ? 1 : 0;
+    if (!c->localTS) return fail(ICSF_ERR_KEYWORD, "only localTimestepping true is supported");
+    c->bounding = fd.sw("pseudoTime/localTimesteppingBounding", true) ? 1 : 0;
+    double lb = fd.scalar("pseudoTime/localTimesteppingLowerBound", 0.95);
+    lb = lb > 0 ? lb : 0.0;
+    lb = lb < 0.99 ? lb : 0.99;
+    c->lowerBound = lb;
+    ph.rhoMin = fd.scalar("pseudoTime/rhoMin", -kGREAT);
+    ph.TMin = fd.scalar("pseudoTime/TMin", kSMALL);
+    ph.TMax = fd.scalar("pseudoTime/TMax", kGREAT);
+    c->blkFp32 = c->mixed;
+    return ICSF_OK;
+}
+
+int uploadBC(icsf_case* c) {
+    if (!c->bcDirty) return ICSF_OK;
+    const int nB = c->mesh->h.nFaces - c->mesh->h.nInternal;
+    const size_t n = (size_t)std::max(nB, 1);
+    HIPCHK(hipMemcpy(c->d_bPT, c->bPT.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_bUT, c->bUT.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_bTT, c->bTT.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_bPv, c->bPv.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_bUv, c->bUv.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_bTv, c->bTv.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    c->bcDirty = false;
+    return ICSF_OK;
+}
+
+int residualStage(icsf_case* c) {
+    RC(uploadBC(c));
+    {
+        Timed t(c, 0);
+        launchResidual(c->mv, c->ph, c->sv, c->st);
+    }
+    c->haveResidual = true;
+    return ICSF_OK;
+}
+
+int jacobianStage(icsf_case* c) {
+    {
+        Timed t(c, 1);
+        launchJacobian(c->mv, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, nullptr, c->derr, c->st);
+    }
+    if (c->precond == 1) launchJacobiSetup(c->mv.nC, c->blkFp32, c->dg, c->dinv, c->st);
+    c->assembled = true;
+    return ICSF_OK;
+}
+
+void applyPrecond(icsf_case* c, int vecFp32, void* v) {
+    Timed t(c, 3);
+    if (c->precond == 0) launchLusgs(c->mv, c->mesh->lev, c->blkFp32, vecFp32, c->blk, c->rD, v, c->st);
+    else if (c->precond == 1) launchJacobiApply(c->mv.nC, c->blkFp32, vecFp32, c->dinv, v, c->st);
+}
+
+// Left-preconditioned GMRES(m) cycles of gmres::solveDelta on the FP64
+// vectors (reference algorithm); returns cycles run.
+int gmresSolve(icsf_case* c, const double* W, const double* b, int& nIter) {
+    const MeshView& m = c->mv;
+    const size_t vs = (size_t)5 * m.nC;
+    {
+        Timed t(c, 5);
+        gmresPrepare(m, c->wk, c->g, 0, W, b, c->t1, c->t2, c->blkFp32, c->blk, c->dg, c->st);
+        HIPCHK(hipMemsetAsync(c->dW, 0, vs * sizeof(double), c->st));
+        launchCopy(vs, 0, b, c->r, c->st);
+    }
+    nIter = 0;
+    for (;;) {
+        GmresArgs a{&c->mv, &c->mesh->lev, &c->wk, c->g, c->nDirs, c->precond, c->orth, c->blkFp32,
+                    c->blk, c->dg, c->rD, c->dinv, c->r, c->V};
+        {
+            Timed t(c, 4);
+            gmresCycle(a, 0, c->st);
+        }
+        {
+            Timed t(c, 5);
+            gmresUpdateAndResidual(m, c->wk, c->g, 0, c->blkFp32, c->blk, c->dg, c->nDirs, c->V, c->dW, b, c->r,
+                                   c->st);
+        }
+        ++nIter;
+        if (nIter < c->minIter) continue;
+        if (nIter >= c->maxIter) break;
+        double res[10];
+        gmresReadResiduals(c->g, res, c->st);
+        double mx = -1e300, rel = -1e300;
+        for (int k = 5; k < 10; ++k) mx = std::max(mx, res[k]);
+        for (int k = 0; k < 5; ++k) {
+            if (k >= 2 && (k - 2) == m.emptyDir) continue;
+            rel = std::max(rel, res[5 + k] / (res[k] + kROOTVSMALL));
+        }
+        if (mx < c->tol || rel < c->relTol) break;
+    }
+    if (m.emptyDir >= 0) launchZeroComponent(m.nC, 2 + m.emptyDir, 0, c->dW, c->st);
+    return ICSF_OK;
+}
+
+// Flexible GMRES: FP64 outer Arnoldi on A with right preconditioning by an
+// FP32 inner solve (left-preconditioned GMRES(innerDirections) with FP32
+// LU-SGS on the FP32 Jacobian), or by the preconditioner alone when
+// innerDirections == 0.  The true residual b - A dW is evaluated in FP64.
+int fgmresSolve(icsf_case* c, const double* W, const double* b, int& nIter);
+
+int solveStage(icsf_case* c, const double* W, const double* b, int& nIter) {
+    if (c->solver == 1) return fgmresSolve(c, W, b, nIter);
+    return gmresSolve(c, W, b, nIter);
+}
+
+int pseudoStep(icsf_case* c) {
+    const MeshView& m = c->mv;
+    c->corr++;
+    if (c->haveInit) {
+        if (c->corr != 1 && c->havePrev)
+            k_ser<<<nblk(m.nC, 256), 256, 0, c->st>>>(m.nC, c->sv.pseudoCo, c->dres, c->coIncr, c->coDecr, c->coMin,
+                                                      c->coMax);
+        k_copy5<<<1, 64, 0, c->st>>>(c->dres, c->dres + 5);
+        c->havePrev = true;
+    }
+    RC(residualStage(c));
+    RC(jacobianStage(c));
+    int nIter = 0;
+    RC(solveStage(c, c->sv.W, c->sv.b, nIter));
+    k_copy5<<<1, 64, 0, c->st>>>(gmresResidualsDevice(c->g), c->dres);
+    c->haveInit = true;
+    if (c->localTS && c->bounding) {
+        k_bound_flag<<<nblk(m.nC, 256), 256, 0, c->st>>>(m.nC, c->lowerBound, c->sv.W, c->fac);
+        k_bound_apply<<<nblk(m.nC, 256), 256, 0, c->st>>>(m.nC, m.nI, m.cfOff, m.cf, m.own, m.nei, c->fac,
+                                                          c->sv.pseudoCo);
+    }
+    {
+        Timed t(c, 5);
+        launchUpdate(m, c->ph, c->sv, c->dW, 1, nullptr, nullptr, nullptr, c->st);
+    }
+    return ICSF_OK;
+}
+
+}  // namespace
+
+// FGMRES (outer FP64, inner FP32) -----------------------------------------
+namespace {
+int fgmresSolve(icsf_case* c, const double* W, const double* b, int& nIter) {
+    return fail(ICSF_ERR_STATE, "FGMRES is not available in this build");
+}
+}  // namespace
+
+// ================================================================ C ABI ===
+extern "C" {
+
+const char* icsf_last_error(void) { return g_err.c_str(); }
+int icsf_version(void) { return 100; }
+
+int icsf_rcm_order(int nCells, int nInternalFaces, const int* owner, const int* neighbour, int* newToOld) {
+    if (nCells <= 0 || nInternalFaces < 0 || !newToOld) return fail(ICSF_ERR_ARG, "icsf_rcm_order: bad arguments");
+    std::vector<int> o;
+    rcmOrder(nCells, nInternalFaces, owner, neighbour, o);
+    std::copy(o.begin(), o.end(), newToOld);
+    return ICSF_OK;
+}
+
+int icsf_colour_cells(int nCells, int nInternalFaces, const int* owner, const int* neighbour, int* colour,
+                      int* nColours) {
+    if (nCells <= 0 || !colour) return fail(ICSF_ERR_ARG, "icsf_colour_cells: bad arguments");
+    std::vector<int> col;
+    const int n = greedyColour(nCells, nInternalFaces, owner, neighbour, col);
+    std::copy(col.begin(), col.end(), colour);
+    if (nColours) *nColours = n;
+    return ICSF_OK;
+}
+
+int icsf_colour_major_order(int nCells, const int* colour, int nColours, int* newToOld) {
+    if (nCells <= 0 || !colour || !newToOld) return fail(ICSF_ERR_ARG, "icsf_colour_major_order: bad arguments");
+    std::vector<int> col(colour, colour + nCells), o;
+    colourMajorOrder(col, nColours, o);
+    if ((int)o.size() != nCells) return fail(ICSF_ERR_ARG, "colours out of range");
+    std::copy(o.begin(), o.end(), newToOld);
+    return ICSF_OK;
+}
+
+int icsf_renumber_mesh(int nCells, int nFaces, int nInternalFaces, const int* faceOffsets, const int* faceVerts,
+                       const int* owner, const int* neighbour, const int* newToOld, int* faceVertsOut,
+                       int* ownerOut, int* neighbourOut, int* faceNewToOld) {
+    if (nCells <= 0 || nFaces <= 0 || nInternalFaces < 0) return fail(ICSF_ERR_ARG, "icsf_renumber_mesh: bad sizes");
+    std::vector<char> seen(nCells, 0);
+    for (int i = 0; i < nCells; ++i) {
+        if (newToOld[i] < 0 || newToOld[i] >= nCells || seen[newToOld[i]])
+            return fail(ICSF_ERR_ARG, "newToOld is not a permutation");
+        seen[newToOld[i]] = 1;
+    }
+    std::vector<int> fv, ow, nb, fm;
+    renumberMesh(nCells, nFaces, nInternalFaces, faceOffsets, faceVerts, owner, neighbour, newToOld, fv, ow, nb, fm);
+    std::copy(fv.begin(), fv.end(), faceVertsOut);
+    std::copy(ow.begin(), ow.end(), ownerOut);
+    std::copy(nb.begin(), nb.end(), neighbourOut);
+    if (faceNewToOld) std::copy(fm.begin(), fm.end(), faceNewToOld);
+    return ICSF_OK;
+}
+
+int icsf_sweep_levels(int nCells, int nInternalFaces, const int* owner, const int* neighbour, int* fwdLevel,
+                      int* bwdLevel, int* nFwd, int* nBwd) {
+    if (nCells <= 0) return fail(ICSF_ERR_ARG, "icsf_sweep_levels: bad sizes");
+    std::vector<int> f, b;
+    int nf = 0, nbw = 0;
+    sweepLevels(nCells, nInternalFaces, owner, neighbour, f, b, nf, nbw);
+    if (fwdLevel) std::copy(f.begin(), f.end(), fwdLevel);
+    if (bwdLevel) std::copy(b.begin(), b.end(), bwdLevel);
+    if (nFwd) *nFwd = nf;
+    if (nBwd) *nBwd = nbw;
+    return ICSF_OK;
+}
+
+int icsf_mesh_geometry(int nPoints, const double* points, int nFaces, const int* faceOffsets, const int* faceVerts,
+                       int nInternalFaces, const int* owner, const int* neighbour, double* C, double* V, double* Sf,
+                       double* Cf, double* weights, double* deltaCoeffs, double* nonOrthDeltaCoeffs,
+                       double* corrVecs) {
+    HostMesh h;
+    const int zero = 0, nb = nFaces - nInternalFaces;
+    std::string err = buildHostMesh(h, nPoints, points, nFaces, faceOffsets, faceVerts, nInternalFaces, owner,
+                                    neighbour, 1, &nInternalFaces, &nb, &zero);
+    if (!err.empty()) return fail(ICSF_ERR_ARG, err);
+    if (C) std::copy(h.C.begin(), h.C.end(), C);
+    if (V) std::copy(h.V.begin(), h.V.end(), V);
+    if (Sf) std::copy(h.Sf.begin(), h.Sf.end(), Sf);
+    if (Cf) std::copy(h.Cf.begin(), h.Cf.end(), Cf);
+    if (weights) std::copy(h.weights.begin(), h.weights.end(), weights);
+    if (deltaCoeffs) std::copy(h.deltaCoeffs.begin(), h.deltaCoeffs.end(), deltaCoeffs);
+    if (nonOrthDeltaCoeffs) std::copy(h.nonOrthDeltaCoeffs.begin(), h.nonOrthDeltaCoeffs.end(), nonOrthDeltaCoeffs);
+    if (corrVecs) std::copy(h.corrVecs.begin(), h.corrVecs.end(), corrVecs);
+    return ICSF_OK;
+}
+
+int icsf_mesh_create(int nPoints, const double* points, int nFaces, const int* faceOffsets, const int* faceVerts,
+                     int nInternalFaces, const int* owner, const int* neighbour, int nPatches,
+                     const int* patchStart, const int* patchSize, const int* patchKind, icsf_mesh** out) {
+    if (!out) return fail(ICSF_ERR_ARG, "icsf_mesh_create: out is NULL");
+    *out = nullptr;
+    RC(checkDevice());
+    std::unique_ptr<icsf_mesh> M(new icsf_mesh);
+    std::string err = buildHostMesh(M->h, nPoints, points, nFaces, faceOffsets, faceVerts, nInternalFaces, owner,
+                                    neighbour, nPatches, patchStart, patchSize, patchKind);
+    if (!err.empty()) return fail(ICSF_ERR_ARG, err);
+    HostMesh& h = M->h;
+    buildSell(h.nCells, h.nInternal, h.owner.data(), h.neighbour.data(), M->sell);
+    // sweep levels -> cell lists
+    std::vector<int> fl, bl;
+    int nf = 0, nbw = 0;
+    sweepLevels(h.nCells, h.nInternal, h.owner.data(), h.neighbour.data(), fl, bl, nf, nbw);
+    Levels& L = M->lev;
+    L.nFwd = nf;
+    L.nBwd = nbw;
+    auto bucket = [&](const std::vector<int>& lev, int n, std::vector<int>& ptr, std::vector<int>& cells) {
+        ptr.assign(n + 1, 0);
+        for (int c = 0; c < h.nCells; ++c) ptr[lev[c] + 1]++;
+        for (int l = 0; l < n; ++l) ptr[l + 1] += ptr[l];
+        cells.assign(h.nCells, 0);
+        std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+        for (int c = 0; c < h.nCells; ++c) cells[pos[lev[c]]++] = c;
+    };
+    std::vector<int> fc, bc;
+    bucket(fl, nf, L.fwdPtrHost, fc);
+    bucket(bl, nbw, L.bwdPtrHost, bc);
+    DevBuf& B = M->buf;
+    MeshView& v = M->view;
+    v.nC = h.nCells;
+    v.nF = h.nFaces;
+    v.nI = h.nInternal;
+    v.nB = h.nFaces - h.nInternal;
+    int *own, *nei, *cfOff, *cf, *bKind, *slicePtr, *col, *rowLen, *nLower;
+    double *Sf, *magSf, *Cf, *w, *dc, *nodc, *corr, *C, *V;
+    RC(B.upload(own, h.owner.data(), h.owner.size()));
+    RC(B.upload(nei, h.neighbour.data(), h.neighbour.size()));
+    RC(B.upload(Sf, h.Sf.data(), h.Sf.size()));
+    RC(B.upload(magSf, h.magSf.data(), h.magSf.size()));
+    RC(B.upload(Cf, h.Cf.data(), h.Cf.size()));
+    RC(B.upload(w, h.weights.data(), h.weights.size()));
+    RC(B.upload(dc, h.deltaCoeffs.data(), h.deltaCoeffs.size()));
+    RC(B.upload(nodc, h.nonOrthDeltaCoeffs.data(), h.nonOrthDeltaCoeffs.size()));
+    RC(B.upload(corr, h.corrVecs.data(), h.corrVecs.size()));
+    RC(B.upload(C, h.C.data(), h.C.size()));
+    RC(B.upload(V, h.V.data(), h.V.size()));
+    RC(B.upload(cfOff, h.cellFaceOff.data(), h.cellFaceOff.size()));
+    RC(B.upload(cf, h.cellFaces.data(), h.cellFaces.size()));
+    std::vector<int> kinds(std::max(v.nB, 1), 0);
+    for (size_t p = 0; p < h.patchStart.size(); ++p)
+        for (int f = h.patchStart[p]; f < h.patchStart[p] + h.patchSize[p]; ++f) kinds[f - h.nInternal] = h.patchKind[p];
+    RC(B.upload(bKind, kinds.data(), kinds.size()));
+    RC(B.upload(slicePtr, M->sell.slicePtr.data(), M->sell.slicePtr.size()));
+    RC(B.upload(col, M->sell.col.data(), M->sell.col.size()));
+    RC(B.upload(rowLen, M->sell.rowLen.data(), M->sell.rowLen.size()));
+    RC(B.upload(nLower, M->sell.nLower.data(), M->sell.nLower.size()));
+    RC(B.upload(L.fwdCells, fc.data(), fc.size()));
+    RC(B.upload(L.bwdCells, bc.data(), bc.size()));
+    v.own = own; v.nei = nei; v.Sf = Sf; v.magSf = magSf; v.Cf = Cf; v.w = w; v.dc = dc; v.nodc = nodc;
+    v.corr = corr; v.C = C; v.V = V; v.cfOff = cfOff; v.cf = cf; v.bKind = bKind;
+    v.nSlices = M->sell.nSlices; v.slicePtr = slicePtr; v.col = col; v.rowLen = rowLen; v.nLower = nLower;
+    v.emptyDir = h.emptyDir;
+    *out = M.release();
+    return ICSF_OK;
+}
+
+int icsf_mesh_destroy(icsf_mesh* mesh) {
+    delete mesh;
+    return ICSF_OK;
+}
+
+int icsf_mesh_info(const icsf_mesh* m, int* info) {
+    if (!m || !info) return fail(ICSF_ERR_ARG, "icsf_mesh_info: NULL argument");
+    info[0] = m->h.nCells;
+    info[1] = m->h.nFaces;
+    info[2] = m->h.nInternal;
+    info[3] = m->lev.nFwd;
+    info[4] = m->lev.nBwd;
+    info[5] = m->sell.nSlices;
+    info[6] = m->sell.slicePtr.empty() ? 0 : m->sell.slicePtr.back() * SLICE;
+    info[7] = m->h.emptyDir;
+    return ICSF_OK;
+}
+
+int icsf_case_create(icsf_mesh* mesh, const char* thermo, const char* schemes, const char* solution,
+                     icsf_case** out) {
+    if (!mesh || !out) return fail(ICSF_ERR_ARG, "icsf_case_create: NULL argument");
+    *out = nullptr;
+    RC(checkDevice());
+    std::unique_ptr<icsf_case> C(new icsf_case);
+    C->mesh = mesh;
+    RC(readSettings(C.get(), thermo, schemes, solution));
+    HIPCHK(hipStreamCreate(&C->st));
+    const HostMesh& h = mesh->h;
+    const int nC = h.nCells, nB = h.nFaces - h.nInternal;
+    const size_t nBs = (size_t)std::max(nB, 1);
+    C->bPT.assign(nBs, BC_ZG);
+    C->bUT.assign(nBs, BC_ZG);
+    C->bTT.assign(nBs, BC_ZG);
+    C->bPv.assign(nBs, 0.0);
+    C->bUv.assign(3 * nBs, 0.0);
+    C->bTv.assign(nBs, 0.0);
+    DevBuf& B = C->buf;
+    RC(B.alloc(C->d_bPT, nBs)); RC(B.alloc(C->d_bUT, nBs)); RC(B.alloc(C->d_bTT, nBs));
+    RC(B.alloc(C->d_bPv, nBs)); RC(B.alloc(C->d_bUv, 3 * nBs)); RC(B.alloc(C->d_bTv, nBs));
+    C->mv = mesh->view;
+    C->mv.bPT = C->d_bPT; C->mv.bUT = C->d_bUT; C->mv.bTT = C->d_bTT;
+    C->mv.bPv = C->d_bPv; C->mv.bUv = C->d_bUv; C->mv.bTv = C->d_bTv;
+    StateView& s = C->sv;
+    RC(B.alloc(s.W, (size_t)5 * nC));
+    RC(B.alloc(s.fld, (size_t)NFLD * nC));
+    RC(B.alloc(s.T, (size_t)nC));
+    RC(B.alloc(s.ecalc, (size_t)nC));
+    RC(B.alloc(s.bfld, (size_t)NFLD * nBs));
+    RC(B.alloc(s.bT, nBs));
+    RC(B.alloc(s.grd, (size_t)NFLD * 3 * nC));
+    RC(B.alloc(s.gU, (size_t)9 * nC));
+    RC(B.alloc(s.ge, (size_t)3 * nC));
+    RC(B.alloc(s.pseudoCo, (size_t)nC));
+    RC(B.alloc(s.rDT, (size_t)nC));
+    RC(B.alloc(s.b, (size_t)5 * nC));
+    HIPCHK(hipMemset(s.bfld, 0, NFLD * nBs * sizeof(double)));
+    HIPCHK(hipMemset(s.bT, 0, nBs * sizeof(double)));
+    k_fill<<<nblk(nC, 256), 256, 0, C->st>>>(nC, C->co, s.pseudoCo);
+    // Jacobian storage
+    const size_t slots = (size_t)mesh->sell.slicePtr.back() * SLICE;
+    const size_t dgN = (size_t)mesh->sell.nSlices * SLICE * NB;
+    const size_t eb = C->blkFp32 ? 4 : 8;
+    {
+        char* p;
+        RC(B.alloc(p, slots * NB * eb)); C->blk = p;
+        HIPCHK(hipMemset(C->blk, 0, slots * NB * eb));
+        RC(B.alloc(p, dgN * eb)); C->dg = p;
+        HIPCHK(hipMemset(C->dg, 0, dgN * eb));
+        if (C->precond == 1) { RC(B.alloc(p, dgN * eb)); C->dinv = p; }
+    }
+    RC(B.alloc(C->rD, (size_t)nC));
+    RC(B.alloc(C->derr, 1));
+    HIPCHK(hipMemset(C->derr, 0, sizeof(int)));
+    // Krylov storage
+    const size_t vs = (size_t)5 * nC;
+    const int nOuter = C->nDirs;
+    RC(B.alloc(C->V, vs * (size_t)(nOuter + 1)));
+    RC(B.alloc(C->r, vs));
+    RC(B.alloc(C->t1, vs));
+    RC(B.alloc(C->t2, vs));
+    RC(B.alloc(C->dW, vs));
+    RC(B.alloc(C->dres, 10));
+    RC(B.alloc(C->fac, (size_t)nC));
+    if (C->solver == 1) {
+        RC(B.alloc(C->Z, vs * (size_t)nOuter));
+        if (C->mixed) {
+            RC(B.alloc(C->V32, vs * (size_t)(C->innerDirs + 1)));
+            RC(B.alloc(C->r32, vs));
+        }
+    }
+    {
+        char* p;
+        RC(B.alloc(p, gmresScalarsBytes())); C->g = (GmresScalars*)p;
+        RC(B.alloc(p, gmresScalarsBytes())); C->g2 = (GmresScalars*)p;
+        HIPCHK(hipMemset(C->g, 0, gmresScalarsBytes()));
+        HIPCHK(hipMemset(C->g2, 0, gmresScalarsBytes()));
+    }
+    RC(B.alloc(C->wk.part, (size_t)1024 * 64));
+    HIPCHK(hipStreamSynchronize(C->st));
+    *out = C.release();
+    return ICSF_OK;
+}
+
+int icsf_case_destroy(icsf_case* c) {
+    if (c && c->st) (void)hipStreamSynchronize(c->st);
+    delete c;
+    return ICSF_OK;
+}
+
+int icsf_set_boundary(icsf_case* c, int patch, int pType, int UType, int TType, const double* pValue,
+                      const double* UValue, const double* TValue) {
+    if (!c) return fail(ICSF_ERR_ARG, "NULL case");
+    const HostMesh& h = c->mesh->h;
+    if (patch < 0 || patch >= (int)h.patchStart.size()) return fail(ICSF_ERR_ARG, "patch index out of range");
+    auto ok = [](int t, bool vec) { return t == BC_ZG || t == BC_FIXED || (vec && t == BC_SLIP); };
+    if (!ok(pType, false) || !ok(UType, true) || !ok(TType, false))
+        return fail(ICSF_ERR_KEYWORD, "unsupported boundary-condition type");
+    if ((pType == BC_FIXED && !pValue) || (UType == BC_FIXED && !UValue) || (TType == BC_FIXED && !TValue))
+        return fail(ICSF_ERR_ARG, "fixedValue condition without values");
+    const int s = h.patchStart[patch] - h.nInternal, n = h.patchSize[patch];
+    for (int i = 0; i < n; ++i) {
+        c->bPT[s + i] = pType;
+        c->bUT[s + i] = UType;
+        c->bTT[s + i] = TType;
+        if (pType == BC_FIXED) c->bPv[s + i] = pValue[i];
+        if (UType == BC_FIXED) for (int d = 0; d < 3; ++d) c->bUv[3 * (s + i) + d] = UValue[3 * i + d];
+        if (TType == BC_FIXED) {
+            if (!(TValue[i] > 0)) return fail(ICSF_ERR_NONFINITE, "fixed temperature must be positive");
+            c->bTv[s + i] = TValue[i];
+        }
+    }
+    c->bcDirty = true;
+    return ICSF_OK;
+}
+
+int icsf_set_fields(icsf_case* c, const double* rho, const double* U, const double* T) {
+    if (!c || !rho || !U || !T) return fail(ICSF_ERR_ARG, "icsf_set_fields: NULL argument");
+    const int nC = c->mv.nC;
+    for (int i = 0; i < nC; ++i)
+        if (!(rho[i] > 0) || !(T[i] > 0) || !std::isfinite(U[3 * i]) || !std::isfinite(U[3 * i + 1]) ||
+            !std::isfinite(U[3 * i + 2]))
+            return fail(ICSF_ERR_NONFINITE, "initial state must have finite U and positive rho, T");
+    double *dr, *du, *dt;
+    DevBuf tmp;
+    RC(tmp.upload(dr, rho, nC));
+    RC(tmp.upload(du, U, (size_t)3 * nC));
+    RC(tmp.upload(dt, T, nC));
+    launchUpdate(c->mv, c->ph, c->sv, nullptr, 0, dr, du, dt, c->st);
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->haveFields = true;
+    c->haveResidual = c->assembled = false;
+    return ICSF_OK;
+}
+
+int icsf_get_fields(icsf_case* c, double* rho, double* U, double* T, double* p, double* W) {
+    if (!c) return fail(ICSF_ERR_ARG, "NULL case");
+    const int nC = c->mv.nC;
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<double> fld((size_t)NFLD * nC);
+    HIPCHK(hipMemcpy(fld.data(), c->sv.fld, fld.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (rho) for (int i = 0; i < nC; ++i) rho[i] = fld[(size_t)i * NFLD + F_RHO];
+    if (p) for (int i = 0; i < nC; ++i) p[i] = fld[(size_t)i * NFLD + F_P];
+    if (U)
+        for (int i = 0; i < nC; ++i)
+            for (int d = 0; d < 3; ++d) U[3 * i + d] = fld[(size_t)i * NFLD + F_UX + d];
+    if (T) HIPCHK(hipMemcpy(T, c->sv.T, (size_t)nC * sizeof(double), hipMemcpyDeviceToHost));
+    if (W) HIPCHK(hipMemcpy(W, c->sv.W, (size_t)5 * nC * sizeof(double), hipMemcpyDeviceToHost));
+    return ICSF_OK;
+}
+
+int icsf_calc_flux(icsf_case* c, double* phi, double* phiUp, double* phiEp) {
+    if (!c || !phi || !phiUp || !phiEp) return fail(ICSF_ERR_ARG, "icsf_calc_flux: NULL argument");
+    if (!c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
+    RC(residualStage(c));
+    const int nF = c->mv.nF;
+    double *a, *b, *d;
+    DevBuf tmp;
+    RC(tmp.alloc(a, nF)); RC(tmp.alloc(b, (size_t)3 * nF)); RC(tmp.alloc(d, nF));
+    launchFaceFlux(c->mv, c->ph, c->sv, a, b, d, c->st);
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(phi, a, nF * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(phiUp, b, 3 * (size_t)nF * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(phiEp, d, nF * sizeof(double), hipMemcpyDeviceToHost));
+    return ICSF_OK;
+}
+
+int icsf_residual(icsf_case* c, double* b, double* rDT) {
+    if (!c) return fail(ICSF_ERR_ARG, "NULL case");
+    if (!c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
+    RC(residualStage(c));
+    HIPCHK(hipStreamSynchronize(c->st));
+    const int nC = c->mv.nC;
+    if (b) HIPCHK(hipMemcpy(b, c->sv.b, (size_t)5 * nC * sizeof(double), hipMemcpyDeviceToHost));
+    if (rDT) HIPCHK(hipMemcpy(rDT, c->sv.rDT, (size_t)nC * sizeof(double), hipMemcpyDeviceToHost));
+    return ICSF_OK;
+}
+
+int icsf_assemble_jacobian(icsf_case* c) {
+    if (!c) return fail(ICSF_ERR_ARG, "NULL case");
+    if (!c->haveResidual) return fail(ICSF_ERR_STATE, "icsf_residual must run before the Jacobian assembly");
+    RC(jacobianStage(c));
+    int e = 0;
+    HIPCHK(hipMemcpyAsync(&e, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (e) {
+        HIPCHK(hipMemset(c->derr, 0, sizeof(int)));
+        return fail(ICSF_ERR_ZERO_DIAG, "All diagonals of coupledMatrix are zero.");
+    }
+    return ICSF_OK;
+}
+
+int icsf_get_jacobian(icsf_case* c, double* diag, double* upper, double* lower) {
+    if (!c) return fail(ICSF_ERR_ARG, "NULL case");
+    if (!c->assembled) return fail(ICSF_ERR_STATE, "Jacobian not assembled");
+    const icsf_mesh* M = c->mesh;
+    const size_t slots = (size_t)M->sell.slicePtr.back() * SLICE;
+    const size_t dgN = (size_t)M->sell.nSlices * SLICE * NB;
+    const size_t eb = c->blkFp32 ? 4 : 8;
+    std::vector<char> hb(slots * NB * eb), hd(dgN * eb);
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(hb.data(), c->blk, hb.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hd.data(), c->dg, hd.size(), hipMemcpyDeviceToHost));
+    auto get = [&](const std::vector<char>& v, size_t i) -> double {
+        return c->blkFp32 ? (double)((const float*)v.data())[i] : ((const double*)v.data())[i];
+    };
+    const int nC = M->h.nCells, nI = M->h.nInternal;
+    if (diag)
+        for (int r = 0; r < nC; ++r)
+            for (int e = 0; e < NB; ++e) diag[(size_t)r * NB + e] = get(hd, diagOff(r, e));
+    for (int f = 0; f < nI; ++f)
+        for (int e = 0; e < NB; ++e) {
+            if (upper) upper[(size_t)f * NB + e] = get(hb, sellOff(M->sell.slotUpper[f], e));
+            if (lower) lower[(size_t)f * NB + e] = get(hb, sellOff(M->sell.slotLower[f], e));
+        }
+    return ICSF_OK;
+}
+
+int icsf_set_jacobian(icsf_case* c, const double* diag, const double* upper, const double* lower) {
+    if (!c || !diag || !upper || !lower) return fail(ICSF_ERR_ARG, "icsf_set_jacobian: NULL argument");
+    const icsf_mesh* M = c->mesh;
+    const size_t slots = (size_t)M->sell.slicePtr.back() * SLICE;
+    const size_t dgN = (size_t)M->sell.nSlices * SLICE * NB;
+    const int nC = M->h.nCells, nI = M->h.nInternal;
+    std::vector<double> hb(slots * NB, 0.0), hd(dgN, 0.0), rD(nC);
+    for (int r = 0; r < nC; ++r)
+        for (int e = 0; e < NB; ++e) hd[diagOff(r, e)] = diag[(size_t)r * NB + e];
+    for (int f = 0; f < nI; ++f)
+        for (int e = 0; e < NB; ++e) {
+            hb[sellOff(M->sell.slotUpper[f], e)] = upper[(size_t)f * NB + e];
+            hb[sellOff(M->sell.slotLower[f], e)] = lower[(size_t)f * NB + e];
+        }
+    for (int r = 0; r < nC; ++r) {
+        const double* D = diag + (size_t)r * NB;
+        double x = kGREAT;
+        for (int e : {0, 6, 12, 18, 24}) x = 1.0 / std::max(1.0 / x, std::fabs(D[e]));
+        if (!(x >= kVSMALL)) return fail(ICSF_ERR_ZERO_DIAG, "All diagonals of coupledMatrix are zero.");
+        rD[r] = x;
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->blkFp32) {
+        std::vector<float> fb(hb.begin(), hb.end()), fd(hd.begin(), hd.end());
+        HIPCHK(hipMemcpy(c->blk, fb.data(), fb.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->dg, fd.data(), fd.size() * 4, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(hipMemcpy(c->blk, hb.data(), hb.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->dg, hd.data(), hd.size() * 8, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(c->rD, rD.data(), (size_t)nC * 8, hipMemcpyHostToDevice));
+    if (c->precond == 1) launchJacobiSetup(nC, c->blkFp32, c->dg, c->dinv, c->st);
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->assembled = true;
+    return ICSF_OK;
+}
+
+int icsf_matrix_mul(icsf_case* c, const double* x, double* y) {
+    if (!c || !x || !y) return fail(ICSF_ERR_ARG, "icsf_matrix_mul: NULL argument");
+    if (!c->assembled) return fail(ICSF_ERR_STATE, "Jacobian not assembled");
+    const size_t vs = (size_t)5 * c->mv.nC;
+    HIPCHK(hipMemcpyAsync(c->t1, x, vs * 8, hipMemcpyHostToDevice, c->st));
+    launchSpmv(c->mv, c->blkFp32, 0, c->blk, c->dg, c->t1, nullptr, nullptr, c->t2, c->st);
+    HIPCHK(hipMemcpyAsync(y, c->t2, vs * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return ICSF_OK;
+}
+
+int icsf_precondition(icsf_case* c, double* r) {
+    if (!c || !r) return fail(ICSF_ERR_ARG, "icsf_precondition: NULL argument");
+    if (!c->assembled) return fail(ICSF_ERR_STATE, "Jacobian not assembled");
+    const size_t vs = (size_t)5 * c->mv.nC;
+    HIPCHK(hipMemcpyAsync(c->t1, r, vs * 8, hipMemcpyHostToDevice, c->st));
+    applyPrecond(c, 0, c->t1);
+    HIPCHK(hipMemcpyAsync(r, c->t1, vs * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return ICSF_OK;
+}
+
+int icsf_solve_for_incr(icsf_case* c, const double* W, const double* b, double* dW, double* residuals,
+                        int* nIter) {
+    if (!c) return fail(ICSF_ERR_ARG, "NULL case");
+    if (!c->assembled) return fail(ICSF_ERR_STATE, "Jacobian not assembled");
+    const size_t vs = (size_t)5 * c->mv.nC;
+    DevBuf tmp;
+    const double* dWsrc = c->sv.W;
+    const double* db = c->sv.b;
+    if (W) {
+        double* p;
+        RC(tmp.upload(p, W, vs));
+        dWsrc = p;
+    }
+    if (b) {
+        double* p;
+        RC(tmp.upload(p, b, vs));
+        db = p;
+    }
+    int it = 0;
+    RC(solveStage(c, dWsrc, db, it));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (dW) HIPCHK(hipMemcpy(dW, c->dW, vs * 8, hipMemcpyDeviceToHost));
+    if (residuals) gmresReadResiduals(c->g, residuals, c->st);
+    if (nIter) *nIter = it;
+    return ICSF_OK;
+}
+
+int icsf_pseudo_steps(icsf_case* c, int nSteps, double* residuals, int* nIter) {
+    if (!c || nSteps < 0) return fail(ICSF_ERR_ARG, "icsf_pseudo_steps: bad arguments");
+    if (!c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
+    for (int i = 0; i < nSteps; ++i) {
+        RC(pseudoStep(c));
+        if (residuals) gmresReadResiduals(c->g, residuals + 10 * (size_t)i, c->st);
+        if (nIter) nIter[i] = 1;
+    }
+    int e = 0;
+    HIPCHK(hipMemcpyAsync(&e, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (e) {
+        HIPCHK(hipMemset(c->derr, 0, sizeof(int)));
+        return fail(ICSF_ERR_ZERO_DIAG, "All diagonals of coupledMatrix are zero.");
+    }
+    resolveEvents(c);
+    return ICSF_OK;
+}
+
+int icsf_get_pseudo_co(icsf_case* c, double* co) {
+    if (!c || !co) return fail(ICSF_ERR_ARG, "NULL argument");
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(co, c->sv.pseudoCo, (size_t)c->mv.nC * 8, hipMemcpyDeviceToHost));
+    return ICSF_OK;
+}
+
+int icsf_synchronize(icsf_case* c) {
+    if (!c) return fail(ICSF_ERR_ARG, "NULL case");
+    HIPCHK(hipStreamSynchronize(c->st));
+    return ICSF_OK;
+}
+
+int icsf_timing_enable(icsf_case* c, int on) {
+    if (!c) return fail(ICSF_ERR_ARG, "NULL case");
+    resolveEvents(c);
+    c->timing = on != 0;
+    for (int k = 0; k < NTIMER; ++k) { c->tms[k] = 0; c->tcnt[k] = 0; }
+    return ICSF_OK;
+}
+
+int icsf_timing_read(icsf_case* c, int n, double* ms, long long* count) {
+    if (!c) return fail(ICSF_ERR_ARG, "NULL case");
+    resolveEvents(c);
+    for (int k = 0; k < n && k < NTIMER; ++k) {
+        if (ms) ms[k] = c->tms[k];
+        if (count) count[k] = c->tcnt[k];
+    }
+    return ICSF_OK;
+}
+
+}  // extern "C"

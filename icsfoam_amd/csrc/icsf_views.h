@@ -1,0 +1,78 @@
+// Device-side views of the mesh, physical settings and per-case buffers.
+// All pointers are device pointers owned by the library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace icsf {
+
+// reconstructed cell fields, AoS [nCells][NFLD]
+enum { F_RHO = 0, F_P = 1, F_UX = 2, F_UY = 3, F_UZ = 4, F_E = 5, F_H = 6, F_CS = 7, NFLD = 8 };
+enum { FLUX_AUSM = 0, FLUX_ROE = 1 };
+enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2 };
+enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2 };
+constexpr int NB = 25;  // entries of a 5x5 block
+
+struct MeshView {
+    int nC, nF, nI, nB;
+    const int* own;       // [nF]
+    const int* nei;       // [nI]
+    const double* Sf;     // [nF][3]
+    const double* magSf;  // [nF]
+    const double* Cf;     // [nF][3]
+    const double* w;      // [nF]
+    const double* dc;     // [nF]
+    const double* nodc;   // [nF]
+    const double* corr;   // [nF][3]
+    const double* C;      // [nC][3]
+    const double* V;      // [nC]
+    const int* cfOff;     // [nC+1] cell -> faces (sorted by face, no empty faces)
+    const int* cf;
+    const int* bKind;     // [nB] patch kind of each boundary face
+    const int* bPT;       // [nB] p boundary-condition type
+    const int* bUT;       // [nB] U boundary-condition type
+    const int* bTT;       // [nB] T boundary-condition type
+    const double* bPv;    // [nB] fixed p
+    const double* bUv;    // [nB][3] fixed U
+    const double* bTv;    // [nB] fixed T
+    // sliced-ELL off-diagonal layout
+    int nSlices;
+    const int* slicePtr;  // [nSlices+1]
+    const int* col;       // [slicePtr[nSlices]*64]
+    const int* rowLen;    // [nC]
+    const int* nLower;    // [nC]
+    int emptyDir;
+};
+
+struct Phys {
+    double R, gamma, Cv, mu, alpha, entropyFix;
+    int flux, lowMach, recon, limiter, viscous;
+    double rhoMin, TMin, TMax;
+};
+
+struct StateView {
+    double* W;        // [5][nC] conserved, order rho, rhoE, rhoUx, rhoUy, rhoUz
+    double* fld;      // [nC][NFLD]
+    double* T;        // [nC]
+    double* ecalc;    // [nC]  rhoE/rho - |U|^2/2
+    double* bfld;     // [nB][NFLD]
+    double* bT;       // [nB]
+    double* grd;      // [nC][NFLD][3] limited gradients
+    double* gU;       // [nC][9] Gauss grad(U) (viscous)
+    double* ge;       // [nC][3] Gauss grad(ecalc) (viscous)
+    double* pseudoCo; // [nC]
+    double* rDT;      // [nC]
+    double* b;        // [5][nC]
+};
+
+// Block storage offset in the sliced-ELL layout: slot index (as stored in
+// col[]) and block entry e.
+__host__ __device__ __forceinline__ size_t sellOff(size_t slot, int e) {
+    return ((slot >> 6) * NB + (size_t)e) * 64 + (slot & 63);
+}
+// Diagonal block of row r, entry e.
+__host__ __device__ __forceinline__ size_t diagOff(int r, int e) {
+    return ((size_t)(r >> 6) * NB + (size_t)e) * 64 + (r & 63);
+}
+
+}  // namespace icsf

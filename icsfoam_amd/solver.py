@@ -1,0 +1,156 @@
+"""Host-side mirror of the reference's solver interfaces over libicsf.
+
+  fvMesh(polyMesh)                         fvMesh (geometry, lduAddressing)
+  dbnsCase(mesh, thermo, schemes, sol)     createFields.H + runtime selection of
+                                           convectiveFluxScheme / viscousFluxScheme /
+                                           coupledMatrix::solver / ::preconditioner
+  .calcFlux()                              convectiveFluxScheme::calcFlux
+  .residual()                              residualsUpdate.H (+ setCoAndDeltaT.H)
+  .createJacobian()                        createConvectiveJacobian + createViscousJacobian
+  .matrixMul(x) / .precondition(r)         coupledMatrix::matrixMul / preconditioner
+  .solveForIncr()                          coupledMatrix::solveForIncr
+  .pseudo_steps(n)                         dbnsFoam pseudo-time loop (outerLoop + updateFields)
+
+Vectors use the reference's unknown order [rho, rhoE, rhoUx, rhoUy, rhoUz]
+laid out [5][nCells].
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from ._lib import as_f64, as_i32, check, lib, ptr
+from .cases import BC_FIXED_VALUE, Case, PolyMesh
+
+
+class fvMesh:
+    def __init__(self, mesh: PolyMesh):
+        self.poly = mesh
+        start, size, kind = mesh.patch_arrays()
+        h = C.c_void_p()
+        check(lib().icsf_mesh_create(mesh.points.shape[0], ptr(as_f64(mesh.points)), mesh.n_faces,
+                                     ptr(as_i32(mesh.face_off)), ptr(as_i32(mesh.face_verts)), mesh.n_internal,
+                                     ptr(as_i32(mesh.owner)), ptr(as_i32(mesh.neighbour)), len(mesh.patches),
+                                     ptr(start), ptr(size), ptr(kind), C.byref(h)))
+        self._h = h
+        info = np.zeros(8, np.int32)
+        check(lib().icsf_mesh_info(h, ptr(info)))
+        self.nCells, self.nFaces, self.nInternalFaces = int(info[0]), int(info[1]), int(info[2])
+        self.nFwdLevels, self.nBwdLevels = int(info[3]), int(info[4])
+        self.nSlices, self.sellSlots, self.emptyDir = int(info[5]), int(info[6]), int(info[7])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().icsf_mesh_destroy(h)
+            self._h = None
+
+
+class dbnsCase:
+    """Fields + schemes + solution controls of one dbnsFoam case on the GPU."""
+
+    def __init__(self, mesh: fvMesh, thermo: str, fvSchemes: str, fvSolution: str):
+        self.mesh = mesh
+        h = C.c_void_p()
+        check(lib().icsf_case_create(mesh._h, thermo.encode(), fvSchemes.encode(), fvSolution.encode(),
+                                     C.byref(h)))
+        self._h = h
+        self.nCells = mesh.nCells
+
+    @classmethod
+    def from_case(cls, case: Case, mesh: Optional[fvMesh] = None) -> "dbnsCase":
+        mesh = mesh or fvMesh(case.mesh)
+        self = cls(mesh, case.thermo, case.fvSchemes, case.fvSolution)
+        for i, bc in enumerate(case.bcs):
+            self.set_boundary(i, bc.p_type, bc.U_type, bc.T_type, bc.p_val, bc.U_val, bc.T_val)
+        self.set_fields(case.rho, case.U, case.T)
+        return self
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().icsf_case_destroy(h)
+            self._h = None
+
+    # ---- state -------------------------------------------------------
+    def set_boundary(self, patch, p_type, U_type, T_type, p_val=None, U_val=None, T_val=None):
+        pv = as_f64(p_val) if p_type == BC_FIXED_VALUE else None
+        uv = as_f64(U_val) if U_type == BC_FIXED_VALUE else None
+        tv = as_f64(T_val) if T_type == BC_FIXED_VALUE else None
+        check(lib().icsf_set_boundary(self._h, patch, p_type, U_type, T_type, ptr(pv), ptr(uv), ptr(tv)))
+
+    def set_fields(self, rho, U, T):
+        check(lib().icsf_set_fields(self._h, ptr(as_f64(rho)), ptr(as_f64(U)), ptr(as_f64(T))))
+
+    def fields(self):
+        n = self.nCells
+        rho, U, T, p, W = np.empty(n), np.empty((n, 3)), np.empty(n), np.empty(n), np.empty((5, n))
+        check(lib().icsf_get_fields(self._h, ptr(rho), ptr(U), ptr(T), ptr(p), ptr(W)))
+        return dict(rho=rho, U=U, T=T, p=p, W=W)
+
+    # ---- flux / residual / Jacobian ----------------------------------
+    def calcFlux(self):
+        nF = self.mesh.nFaces
+        phi, phiUp, phiEp = np.empty(nF), np.empty((nF, 3)), np.empty(nF)
+        check(lib().icsf_calc_flux(self._h, ptr(phi), ptr(phiUp), ptr(phiEp)))
+        return phi, phiUp, phiEp
+
+    def residual(self):
+        b, rdt = np.empty((5, self.nCells)), np.empty(self.nCells)
+        check(lib().icsf_residual(self._h, ptr(b), ptr(rdt)))
+        return b, rdt
+
+    def createJacobian(self):
+        check(lib().icsf_assemble_jacobian(self._h))
+
+    def jacobian(self):
+        nC, nI = self.nCells, self.mesh.nInternalFaces
+        d, u, l = np.empty((nC, 5, 5)), np.empty((nI, 5, 5)), np.empty((nI, 5, 5))
+        check(lib().icsf_get_jacobian(self._h, ptr(d), ptr(u), ptr(l)))
+        return d, u, l
+
+    def set_jacobian(self, diag, upper, lower):
+        check(lib().icsf_set_jacobian(self._h, ptr(as_f64(diag)), ptr(as_f64(upper)), ptr(as_f64(lower))))
+
+    def matrixMul(self, x):
+        y = np.empty((5, self.nCells))
+        check(lib().icsf_matrix_mul(self._h, ptr(as_f64(x)), ptr(y)))
+        return y
+
+    def precondition(self, r):
+        r = np.array(r, dtype=np.float64, order="C", copy=True)
+        check(lib().icsf_precondition(self._h, ptr(r)))
+        return r
+
+    def solveForIncr(self, W=None, b=None):
+        dW = np.empty((5, self.nCells))
+        res = np.empty(10)
+        it = np.zeros(1, np.int32)
+        check(lib().icsf_solve_for_incr(self._h, ptr(None if W is None else as_f64(W)),
+                                        ptr(None if b is None else as_f64(b)), ptr(dW), ptr(res), ptr(it)))
+        return dW, res, int(it[0])
+
+    # ---- pseudo-time loop ----------------------------------------------
+    def pseudo_steps(self, n: int, residuals: bool = True):
+        res = np.empty((n, 10)) if residuals else None
+        check(lib().icsf_pseudo_steps(self._h, n, ptr(res), None))
+        return res
+
+    def pseudoCo(self):
+        co = np.empty(self.nCells)
+        check(lib().icsf_get_pseudo_co(self._h, ptr(co)))
+        return co
+
+    def synchronize(self):
+        check(lib().icsf_synchronize(self._h))
+
+    def timing(self, on: bool = True):
+        check(lib().icsf_timing_enable(self._h, 1 if on else 0))
+
+    def timing_read(self):
+        ms = np.zeros(6)
+        cnt = np.zeros(6, np.int64)
+        check(lib().icsf_timing_read(self._h, 6, ptr(ms), ptr(cnt)))
+        return ms, cnt

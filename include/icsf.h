@@ -1,0 +1,205 @@
+/*
+ * icsf.h - C ABI of the MI355X implicit coupled pseudo-time step.
+ *
+ * Drop-in boundary for ICSFoam's hot path: the convective/viscous flux
+ * schemes, the blockFvMatrix/coupledMatrix Jacobian and the coupledMatrix
+ * GMRES/LU-SGS solve, driven by the dbnsFoam pseudo-time loop.  Every entry
+ * point takes plain pointers and sizes; host arrays are caller-owned, device
+ * buffers are owned by the library and persist across pseudo-steps.
+ *
+ * Unknown order at this interface: [rho, rhoE, rhoUx, rhoUy, rhoUz] per cell,
+ * vectors laid out [5][nCells] (coupledMatrix with nScal = 2, nVect = 1,
+ * applications/solvers/dbnsFoam/outerLoop.H:53).  5x5 blocks are row-major in
+ * that order.  Cell-wise vector fields (U) are [nCells][3].
+ *
+ * Return codes: 0 on success, negative on failure (icsf_last_error() gives
+ * the text).  Failure codes mirror the reference's FatalError sites.
+ * Reference paths are relative to the ICSFoam source tree.
+ */
+#ifndef ICSF_H
+#define ICSF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ICSF_OK 0
+#define ICSF_ERR_ARG (-1)        /* invalid argument / size */
+#define ICSF_ERR_KEYWORD (-2)    /* unknown scheme/solver keyword: newConvectiveFluxScheme.C:51-60,
+                                    coupledMatrixSolver.C:53-61, coupledMatrixPreconditioner.C:49 */
+#define ICSF_ERR_ZERO_DIAG (-3)  /* all diagonals zero: lusgs.C:119-123 */
+#define ICSF_ERR_NONFINITE (-4)  /* non-finite state or residual */
+#define ICSF_ERR_HIP (-5)        /* HIP runtime failure */
+#define ICSF_ERR_STATE (-6)      /* call-order error (e.g. solve before assemble) */
+#define ICSF_ERR_NODEVICE (-7)   /* no HIP device available */
+
+/* patch kinds (OpenFOAM polyPatch types) */
+#define ICSF_PATCH 0
+#define ICSF_WALL 1
+#define ICSF_EMPTY 2
+/* boundary-condition kinds (OpenFOAM fvPatchField types) */
+#define ICSF_BC_ZERO_GRADIENT 0
+#define ICSF_BC_FIXED_VALUE 1
+#define ICSF_BC_SLIP 2
+
+typedef struct icsf_mesh icsf_mesh;
+typedef struct icsf_case icsf_case;
+
+/* Text of the last error on the calling thread. */
+const char* icsf_last_error(void);
+/* ABI version (major*100 + minor). */
+int icsf_version(void);
+
+/* ---------------------------------------------------------------------
+ * Host-side mesh preprocessing (no GPU use).  Integer work; results are
+ * deterministic and reproducible bit-for-bit.
+ * ------------------------------------------------------------------- */
+
+/* Reverse Cuthill-McKee cell order (new -> old).  Replaces OpenFOAM
+ * renumberMesh's CuthillMcKee renumbering built on
+ * meshTools::bandCompression, which ICSFoam meshes are prepared with;
+ * ICSFoam itself runs on file order (src/blockFvMatrix/blockFvMatrix.C:220-234
+ * consumes the resulting lduAddressing). */
+int icsf_rcm_order(int nCells, int nInternalFaces, const int* owner, const int* neighbour, int* newToOld);
+
+/* Greedy cell colouring in index order; returns the colour of each cell.
+ * No counterpart in the reference (sweeps there are sequential,
+ * src/coupledMatrix/preconditioners/lusgs/lusgs.C:233-381); used to build a
+ * numbering in which the sequential LU-SGS sweep has few dependency levels. */
+int icsf_colour_cells(int nCells, int nInternalFaces, const int* owner, const int* neighbour, int* colour,
+                      int* nColours);
+
+/* Stable partition of cells by colour (new -> old). */
+int icsf_colour_major_order(int nCells, const int* colour, int nColours, int* newToOld);
+
+/* Apply a cell permutation to a polyMesh: internal faces re-oriented so
+ * owner < neighbour (flipped faces reversed), re-sorted upper-triangular;
+ * boundary faces keep their order.  faceNewToOld[f] = old face, or
+ * -(old+1) when the face was flipped.  faceVertsOut has the same length as
+ * faceVerts; face offsets of the output follow the new face order. */
+int icsf_renumber_mesh(int nCells, int nFaces, int nInternalFaces, const int* faceOffsets, const int* faceVerts,
+                       const int* owner, const int* neighbour, const int* newToOld, int* faceVertsOut,
+                       int* ownerOut, int* neighbourOut, int* faceNewToOld);
+
+/* Dependency levels of the forward (lower) and backward (upper) LU-SGS
+ * sweeps in the current numbering (lusgs.C:233-381 visits cells in index
+ * order; cells of one level are independent). */
+int icsf_sweep_levels(int nCells, int nInternalFaces, const int* owner, const int* neighbour, int* fwdLevel,
+                      int* bwdLevel, int* nFwd, int* nBwd);
+
+/* OpenFOAM mesh geometry on the host (primitiveMesh face/cell centres,
+ * areas, volumes; surfaceInterpolation weights, deltaCoeffs,
+ * nonOrthDeltaCoeffs, nonOrthCorrectionVectors).  Any output may be NULL. */
+int icsf_mesh_geometry(int nPoints, const double* points, int nFaces, const int* faceOffsets, const int* faceVerts,
+                       int nInternalFaces, const int* owner, const int* neighbour, double* C, double* V, double* Sf,
+                       double* Cf, double* weights, double* deltaCoeffs, double* nonOrthDeltaCoeffs,
+                       double* corrVecs);
+
+/* ---------------------------------------------------------------------
+ * Device mesh (fvMesh replacement for the path).
+ * ------------------------------------------------------------------- */
+
+/* Upload a polyMesh (owner < neighbour, upper-triangular internal faces,
+ * patches tiling the boundary faces in order).  Computes the geometry,
+ * the face-ordered cell->face pull lists, the sliced-ELL block layout and
+ * the LU-SGS sweep levels. */
+int icsf_mesh_create(int nPoints, const double* points, int nFaces, const int* faceOffsets, const int* faceVerts,
+                     int nInternalFaces, const int* owner, const int* neighbour, int nPatches,
+                     const int* patchStart, const int* patchSize, const int* patchKind, icsf_mesh** out);
+int icsf_mesh_destroy(icsf_mesh* mesh);
+/* info[0..7] = nCells, nFaces, nInternalFaces, nFwdLevels, nBwdLevels,
+ * nSlices, sellSlots, emptyDirection (-1 if 3-D) */
+int icsf_mesh_info(const icsf_mesh* mesh, int* info);
+
+/* ---------------------------------------------------------------------
+ * Case: fields, thermophysical model, schemes and solution controls.
+ * ------------------------------------------------------------------- */
+
+/* Dictionaries in OpenFOAM syntax: thermophysicalProperties (hePsiThermo,
+ * hConst, perfectGas, const transport; keywords R, gamma, mu, Pr), fvSchemes
+ * (convectiveFluxScheme{fluxScheme AUSMPlusUp|ROE; lowMachAusm;
+ * entropyFixCoeff}, ddtSchemes, gradSchemes, interpolationSchemes
+ * reconstruct(rho|U|T)) and fvSolution (flowSolver{solver GMRES|FGMRES;
+ * <solver>{preconditioner LUSGS|Jacobi|none; nDirections; maxIter; minIter;
+ * tolerance; relTol; precision double|mixed; innerDirections;
+ * orthogonalisation MGS|CGS}}, pseudoTime{...}).
+ * Replaces convectiveFluxScheme::New (newConvectiveFluxScheme.C:39-64),
+ * viscousFluxScheme construction (viscousFluxScheme.H:105-113),
+ * coupledMatrix::solver::New (coupledMatrixSolver.C:41-67) and
+ * coupledMatrix::preconditioner::New (coupledMatrixPreconditioner.C:41-64). */
+int icsf_case_create(icsf_mesh* mesh, const char* thermophysicalProperties, const char* fvSchemes,
+                     const char* fvSolution, icsf_case** out);
+int icsf_case_destroy(icsf_case* c);
+
+/* Boundary conditions of p, U, T on one patch; fixed values per patch face
+ * (U as [size][3]); pointers may be NULL for non-fixed conditions. */
+int icsf_set_boundary(icsf_case* c, int patch, int pType, int UType, int TType, const double* pValue,
+                      const double* UValue, const double* TValue);
+
+/* Initial state from rho, U, T (createFields.H: p = rho/psi, rhoU, rhoE). */
+int icsf_set_fields(icsf_case* c, const double* rho, const double* U, const double* T);
+/* Current state; W is the conserved vector [5][nCells]. Any may be NULL. */
+int icsf_get_fields(icsf_case* c, double* rho, double* U, double* T, double* p, double* W);
+
+/* Face fluxes of every face (phi [nFaces], phiUp [nFaces][3], phiEp
+ * [nFaces]); replaces convectiveFluxScheme::calcFlux
+ * (convectiveFluxScheme.H:219-224). */
+int icsf_calc_flux(icsf_case* c, double* phi, double* phiUp, double* phiEp);
+
+/* Residual source b = R*V ([5][nCells]) of residualsUpdate.H:1-183 and the
+ * local rPseudoDeltaT of setCoAndDeltaT.H:57-141.  Either may be NULL. */
+int icsf_residual(icsf_case* c, double* b, double* rPseudoDeltaT);
+
+/* Assemble the coupled Jacobian for the current state (after
+ * icsf_residual): convectiveFluxScheme::createConvectiveJacobian
+ * (convectiveFluxScheme.C:537-546) and viscousFluxScheme::
+ * createViscousJacobian (viscousFluxScheme.C:265-273) into the block
+ * matrix (blockFvMatrix.C:212-326). */
+int icsf_assemble_jacobian(icsf_case* c);
+
+/* LDU view of the assembled matrix: diag [nCells][25], upper/lower
+ * [nInternalFaces][25] (blockFvMatrix diag/upper/lower).  Any may be NULL. */
+int icsf_get_jacobian(icsf_case* c, double* diag, double* upper, double* lower);
+/* Replace the assembled matrix by given LDU blocks (same layout). */
+int icsf_set_jacobian(icsf_case* c, const double* diag, const double* upper, const double* lower);
+
+/* y = A x ([5][nCells]); coupledMatrix::matrixMul (coupledMatrix.C:66-123). */
+int icsf_matrix_mul(icsf_case* c, const double* x, double* y);
+
+/* r <- P^-1 r with the configured preconditioner; lusgs::precondition
+ * (lusgs.C:220-382) / Jacobi::precondition (Jacobi.C:55-132). */
+int icsf_precondition(icsf_case* c, double* r);
+
+/* Solve A dW = b for the increment: coupledMatrix::solveForIncr
+ * (coupledMatrix.C:297-385) -> gmres::solveDelta (gmres.C:772-1108).
+ * W: current state [5][nCells] (NULL = the case state), b: source (NULL =
+ * the last residual), dW out.  residuals[10] = sInit[2], vInit[3],
+ * sFinal[2], vFinal[3] (residualsIO), nIter = cycles run. */
+int icsf_solve_for_incr(icsf_case* c, const double* W, const double* b, double* dW, double* residuals,
+                        int* nIter);
+
+/* nSteps steady pseudo-time steps of dbnsFoam (dbnsFoam.C:110-124:
+ * outerLoop.H + updateFields.H, SER of setCoAndDeltaT.H:3-37, bounding of
+ * boundLocalTimeStep.H).  residuals: [nSteps][10] (may be NULL), nIter:
+ * [nSteps] (may be NULL).  Residuals are read back only when requested. */
+int icsf_pseudo_steps(icsf_case* c, int nSteps, double* residuals, int* nIter);
+
+/* Pseudo-Courant field [nCells] (pseudoCoField). */
+int icsf_get_pseudo_co(icsf_case* c, double* co);
+
+/* Wait for all work queued on the case's stream. */
+int icsf_synchronize(icsf_case* c);
+
+/* Kernel timing with HIP events recorded on the launch stream.
+ * icsf_timing_enable(c, 1) starts accumulating; icsf_timing_read returns,
+ * per kernel class k < n, total milliseconds and launch count.
+ * Classes: 0 residual, 1 jacobian, 2 spmv, 3 lusgs, 4 orthogonalisation,
+ * 5 update/other. */
+int icsf_timing_enable(icsf_case* c, int on);
+int icsf_timing_read(icsf_case* c, int n, double* ms, long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ICSF_H */

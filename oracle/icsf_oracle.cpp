@@ -1,0 +1,1884 @@
+// ORACLE — test infrastructure only.
+//
+// A CPU, FP64, single-threaded restatement of ICSFoam's implicit coupled
+// pseudo-time step, written to follow the reference's own operation order
+// (OpenFOAM-style face loops, blockFvMatrix LDU sub-blocks, left-
+// preconditioned MGS GMRES, sequential scalar-diagonal LU-SGS).  It is the
+// checker for the HIP path and the timed CPU baseline; nothing in the
+// product links it.  Build: oracle/build.py (g++ -O2 -ffp-contract=off).
+//
+// Reference files followed (paths relative to the ICSFoam tree):
+//   applications/solvers/dbnsFoam/{outerLoop.H,residualsUpdate.H,
+//       setCoAndDeltaT.H,boundLocalTimeStep.H,updateFields.H,dbnsFoam.C}
+//   src/convectiveFluxSchemes/{convectiveFluxScheme/convectiveFluxScheme.C,
+//       ausmPlusUp/ausmPlusUpFluxScheme.C,roe/roeFluxScheme.C}
+//   src/viscousFluxScheme/viscousFluxScheme.C
+//   src/blockFvMatrix/{blockFvMatrix.C,blockFvOperatorsTemplates.C}
+//   src/coupledMatrix/{coupledMatrix.C,coupledMatrixSolver.C,
+//       solvers/gmres/gmres.C,preconditioners/lusgs/lusgs.C,
+//       smoothers/Jacobi/JacobiSmoother.C,preconditioners/Jacobi/Jacobi.C}
+// OpenFOAM-v2112 operators the reference calls (restated from their
+// published definitions; not vendored in the reference): primitiveMesh
+// geometry, surfaceInterpolation weights/deltaCoeffs, gaussGrad,
+// cellLimitedGrad<Venkatakrishnan>, linearUpwind, correctedSnGrad,
+// gaussLaplacianScheme, surfaceIntegrate, meshTools::bandCompression.
+//
+// Unknown ordering everywhere at this interface: [rho, rhoE, rhoUx, rhoUy, rhoUz]
+// (coupledMatrix with nScal=2, nVect=1; outerLoop.H:53).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+const double SMALL = 1.0e-15;
+const double VSMALL = 1.0e-300;
+const double ROOTVSMALL = 1.0e-150;
+const double GREAT = 1.0e15;
+
+typedef std::vector<double> vec;
+
+inline double sqr(double x) { return x * x; }
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 v3(const double* p) { return V3{p[0], p[1], p[2]}; }
+inline V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(double s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+inline V3 operator*(V3 a, double s) { return V3{a.x * s, a.y * s, a.z * s}; }
+inline V3 operator/(V3 a, double s) { return V3{a.x / s, a.y / s, a.z / s}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline double magSqr(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+inline double mag(V3 a) { return std::sqrt(magSqr(a)); }
+inline V3 cross(V3 a, V3 b) { return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// 3x3 tensor, row-major xx xy xz yx yy yz zx zy zz
+struct T9 {
+    double t[9];
+};
+inline T9 outer(V3 a, V3 b) {
+    return T9{{a.x * b.x, a.x * b.y, a.x * b.z, a.y * b.x, a.y * b.y, a.y * b.z, a.z * b.x, a.z * b.y, a.z * b.z}};
+}
+inline T9 operator+(const T9& a, const T9& b) {
+    T9 r;
+    for (int i = 0; i < 9; ++i) r.t[i] = a.t[i] + b.t[i];
+    return r;
+}
+inline T9 operator-(const T9& a, const T9& b) {
+    T9 r;
+    for (int i = 0; i < 9; ++i) r.t[i] = a.t[i] - b.t[i];
+    return r;
+}
+inline T9 operator*(double s, const T9& a) {
+    T9 r;
+    for (int i = 0; i < 9; ++i) r.t[i] = s * a.t[i];
+    return r;
+}
+inline T9 operator*(const T9& a, double s) {
+    T9 r;
+    for (int i = 0; i < 9; ++i) r.t[i] = a.t[i] * s;
+    return r;
+}
+// tensor & vector
+inline V3 tdotv(const T9& T, V3 v) {
+    return V3{T.t[0] * v.x + T.t[1] * v.y + T.t[2] * v.z, T.t[3] * v.x + T.t[4] * v.y + T.t[5] * v.z,
+              T.t[6] * v.x + T.t[7] * v.y + T.t[8] * v.z};
+}
+// vector & tensor
+inline V3 vdott(V3 v, const T9& T) {
+    return V3{v.x * T.t[0] + v.y * T.t[3] + v.z * T.t[6], v.x * T.t[1] + v.y * T.t[4] + v.z * T.t[7],
+              v.x * T.t[2] + v.y * T.t[5] + v.z * T.t[8]};
+}
+// tensor & tensor
+inline T9 tdott(const T9& A, const T9& B) {
+    T9 r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            r.t[3 * i + j] = A.t[3 * i] * B.t[j] + A.t[3 * i + 1] * B.t[3 + j] + A.t[3 * i + 2] * B.t[6 + j];
+    return r;
+}
+inline T9 transposeT(const T9& A) {
+    return T9{{A.t[0], A.t[3], A.t[6], A.t[1], A.t[4], A.t[7], A.t[2], A.t[5], A.t[8]}};
+}
+inline T9 dev2(const T9& A) {
+    T9 r = A;
+    const double tr = A.t[0] + A.t[4] + A.t[8];
+    const double s = (2.0 / 3.0) * tr;
+    r.t[0] = A.t[0] - s;
+    r.t[4] = A.t[4] - s;
+    r.t[8] = A.t[8] - s;
+    return r;
+}
+
+enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2 };
+enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2 };
+enum { FLUX_AUSM = 0, FLUX_ROE = 1 };
+
+// ---------------------------------------------------------------- mesh ----
+struct Mesh {
+    int nC = 0, nF = 0, nI = 0, nP = 0;
+    vec pts;
+    std::vector<int> fOff, fV, own, nei;
+    std::vector<int> pStart, pSize, pKind;
+    vec Sf, magSf, Cf, C, V, w, dc, nodc, corr;
+    std::vector<std::vector<int>> cells;  // OpenFOAM cells(): owned faces, then neighboured faces
+    int emptyDir = -1;
+    bool isEmptyFace(int f) const {
+        for (size_t p = 0; p < pStart.size(); ++p)
+            if (f >= pStart[p] && f < pStart[p] + pSize[p]) return pKind[p] == PK_EMPTY;
+        return false;
+    }
+    std::vector<int> facePatch;  // boundary face -> patch
+};
+
+// primitiveMesh::makeFaceCentresAndAreas / makeCellCentresAndVols and
+// surfaceInterpolation::makeWeights / makeDeltaCoeffs /
+// makeNonOrthDeltaCoeffs / makeNonOrthCorrectionVectors.
+void makeGeometry(Mesh& m) {
+    m.Sf.assign(3 * m.nF, 0);
+    m.Cf.assign(3 * m.nF, 0);
+    m.magSf.assign(m.nF, 0);
+    for (int f = 0; f < m.nF; ++f) {
+        const int n = m.fOff[f + 1] - m.fOff[f];
+        const int* fv = &m.fV[m.fOff[f]];
+        V3 S, Cc;
+        if (n == 3) {
+            V3 a = v3(&m.pts[3 * fv[0]]), b = v3(&m.pts[3 * fv[1]]), c = v3(&m.pts[3 * fv[2]]);
+            Cc = (1.0 / 3.0) * (a + b + c);
+            S = 0.5 * cross(b - a, c - a);
+        } else {
+            V3 fc = v3(&m.pts[3 * fv[0]]);
+            for (int k = 1; k < n; ++k) fc = fc + v3(&m.pts[3 * fv[k]]);
+            fc = fc / (double)n;
+            V3 sumN{0, 0, 0}, sumAc{0, 0, 0};
+            double sumA = 0;
+            for (int k = 0; k < n; ++k) {
+                V3 tp = v3(&m.pts[3 * fv[k]]);
+                V3 np_ = v3(&m.pts[3 * fv[(k + 1) % n]]);
+                V3 c = tp + np_ + fc;
+                V3 nn = cross(np_ - tp, fc - tp);
+                double a = mag(nn);
+                sumN = sumN + nn;
+                sumA += a;
+                sumAc = sumAc + a * c;
+            }
+            if (sumA < ROOTVSMALL) {
+                Cc = fc;
+                S = V3{0, 0, 0};
+            } else {
+                Cc = (1.0 / 3.0) * sumAc / sumA;
+                S = 0.5 * sumN;
+            }
+        }
+        m.Sf[3 * f] = S.x; m.Sf[3 * f + 1] = S.y; m.Sf[3 * f + 2] = S.z;
+        m.Cf[3 * f] = Cc.x; m.Cf[3 * f + 1] = Cc.y; m.Cf[3 * f + 2] = Cc.z;
+        m.magSf[f] = mag(S);
+    }
+    std::vector<V3> cEst(m.nC, V3{0, 0, 0});
+    std::vector<int> nCF(m.nC, 0);
+    for (int f = 0; f < m.nF; ++f) { cEst[m.own[f]] = cEst[m.own[f]] + v3(&m.Cf[3 * f]); nCF[m.own[f]]++; }
+    for (int f = 0; f < m.nI; ++f) { cEst[m.nei[f]] = cEst[m.nei[f]] + v3(&m.Cf[3 * f]); nCF[m.nei[f]]++; }
+    for (int c = 0; c < m.nC; ++c) cEst[c] = cEst[c] / (double)nCF[c];
+    std::vector<V3> cc(m.nC, V3{0, 0, 0});
+    vec vol(m.nC, 0);
+    for (int f = 0; f < m.nF; ++f) {
+        int o = m.own[f];
+        V3 fc = v3(&m.Cf[3 * f]);
+        double pyr3 = dot(v3(&m.Sf[3 * f]), fc - cEst[o]);
+        V3 pc = (3.0 / 4.0) * fc + (1.0 / 4.0) * cEst[o];
+        cc[o] = cc[o] + pyr3 * pc;
+        vol[o] += pyr3;
+    }
+    for (int f = 0; f < m.nI; ++f) {
+        int o = m.nei[f];
+        V3 fc = v3(&m.Cf[3 * f]);
+        double pyr3 = dot(v3(&m.Sf[3 * f]), cEst[o] - fc);
+        V3 pc = (3.0 / 4.0) * fc + (1.0 / 4.0) * cEst[o];
+        cc[o] = cc[o] + pyr3 * pc;
+        vol[o] += pyr3;
+    }
+    m.C.assign(3 * m.nC, 0);
+    m.V.assign(m.nC, 0);
+    for (int c = 0; c < m.nC; ++c) {
+        V3 r = std::fabs(vol[c]) > VSMALL ? cc[c] / vol[c] : cEst[c];
+        m.C[3 * c] = r.x; m.C[3 * c + 1] = r.y; m.C[3 * c + 2] = r.z;
+        m.V[c] = vol[c] * (1.0 / 3.0);
+    }
+    m.w.assign(m.nF, 1.0);
+    m.dc.assign(m.nF, 0);
+    m.nodc.assign(m.nF, 0);
+    m.corr.assign(3 * m.nF, 0);
+    for (int f = 0; f < m.nI; ++f) {
+        V3 S = v3(&m.Sf[3 * f]), fc = v3(&m.Cf[3 * f]);
+        V3 Co = v3(&m.C[3 * m.own[f]]), Cn = v3(&m.C[3 * m.nei[f]]);
+        double so = std::fabs(dot(S, fc - Co)), sn = std::fabs(dot(S, Cn - fc));
+        m.w[f] = sn / (so + sn);
+        V3 d = Cn - Co;
+        m.dc[f] = 1.0 / mag(d);
+        V3 ua = S / m.magSf[f];
+        m.nodc[f] = 1.0 / std::max(dot(ua, d), 0.05 * mag(d));
+        V3 k = ua - d * m.nodc[f];
+        m.corr[3 * f] = k.x; m.corr[3 * f + 1] = k.y; m.corr[3 * f + 2] = k.z;
+    }
+    for (int f = m.nI; f < m.nF; ++f) {
+        V3 S = v3(&m.Sf[3 * f]), fc = v3(&m.Cf[3 * f]);
+        V3 Co = v3(&m.C[3 * m.own[f]]);
+        V3 n = S / m.magSf[f];
+        V3 d = n * dot(n, fc - Co);
+        m.dc[f] = 1.0 / mag(d);
+        m.nodc[f] = 1.0 / std::max(dot(n, d), 0.05 * mag(d));
+    }
+    // cells(): OpenFOAM primitiveMesh::calcCells order
+    m.cells.assign(m.nC, {});
+    for (int f = 0; f < m.nF; ++f) m.cells[m.own[f]].push_back(f);
+    for (int f = 0; f < m.nI; ++f) m.cells[m.nei[f]].push_back(f);
+    m.facePatch.assign(m.nF - m.nI, -1);
+    for (size_t p = 0; p < m.pStart.size(); ++p)
+        for (int f = m.pStart[p]; f < m.pStart[p] + m.pSize[p]; ++f) m.facePatch[f - m.nI] = (int)p;
+    m.emptyDir = -1;
+    double acc[3] = {0, 0, 0};
+    bool any = false;
+    for (int f = m.nI; f < m.nF; ++f)
+        if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) {
+            any = true;
+            for (int d = 0; d < 3; ++d) acc[d] += std::fabs(m.Sf[3 * f + d]);
+        }
+    if (any) m.emptyDir = (int)(std::max_element(acc, acc + 3) - acc);
+}
+
+// ---------------------------------------------------------------- case ----
+struct Settings {
+    int flux = FLUX_AUSM, lowMach = 1, recon = 1, limiter = 1, viscous = 0, precond = 0;
+    int nDirs = 30, maxIter = 1, minIter = 0, localTS = 1, bounding = 1;
+    double R = 287.0, gamma = 1.4, mu = 0.0, Pr = 0.72, entropyFix = 0.05;
+    double tol = 1e-12, relTol = 1e-2, co = 50, coMin = 50, coMax = 50, coIncr = 1.25, coDecr = 0.1;
+    double lowerBound = 0.95, rhoMin = -GREAT, TMin = SMALL, TMax = GREAT;
+};
+
+struct BC {
+    int pT = BC_ZG, uT = BC_ZG, tT = BC_ZG;
+    vec pv, uv, tv;
+};
+
+// Sub-block of a blockFvMatrix: diag[nC], upper/lower[nI], each ncomp wide.
+struct LduBlock {
+    int ncomp = 1;
+    vec diag, upper, lower;
+    void init(int nC, int nI, int nc) {
+        ncomp = nc;
+        diag.assign((size_t)nC * nc, 0);
+        upper.assign((size_t)nI * nc, 0);
+        lower.assign((size_t)nI * nc, 0);
+    }
+};
+
+struct Case {
+    Mesh* m;
+    Settings s;
+    std::vector<BC> bcs;
+    double Cv = 0;
+    // state (cells)
+    vec rho, U, T, p, e, rhoU, rhoE;
+    // boundary values (per boundary face index f - nI)
+    vec pB, UB, TB, rhoB, eB, rhoUB, rhoEB;
+    // pseudo-time state
+    vec pseudoCo, rDT;
+    bool haveInit = false, havePrev = false;
+    double initS[2], initV[3], prevS[2], prevV[3];
+    int corr = 0;
+    // coupled matrix: sub-blocks SS[i][j], SV[i] (i scalar row, vector col),
+    // VS[j] (vector row, scalar col j), VV
+    LduBlock SS[2][2], SV[2], VS[2], VV;
+    vec source;  // [5][nC]
+    vec rD;      // LU-SGS reciprocal scalar diagonal
+};
+
+inline double gammaOf(const Case& c) { return c.s.gamma; }
+
+// fixedValue / zeroGradient / slip evaluation of p, U, T on the boundary and
+// the derived rho = psi*p, e = Cv*T fields (updateFields.H:80-104).
+void evalBoundary(Case& k) {
+    Mesh& m = *k.m;
+    const int nB = m.nF - m.nI;
+    k.pB.assign(nB, 0); k.UB.assign(3 * nB, 0); k.TB.assign(nB, 0);
+    k.rhoB.assign(nB, 0); k.eB.assign(nB, 0); k.rhoUB.assign(3 * nB, 0); k.rhoEB.assign(nB, 0);
+    for (size_t pi = 0; pi < m.pStart.size(); ++pi) {
+        if (m.pKind[pi] == PK_EMPTY) continue;
+        const BC& bc = k.bcs[pi];
+        for (int i = 0; i < m.pSize[pi]; ++i) {
+            const int f = m.pStart[pi] + i, b = f - m.nI, c = m.own[f];
+            k.pB[b] = bc.pT == BC_FIXED ? bc.pv[i] : k.p[c];
+            k.TB[b] = bc.tT == BC_FIXED ? bc.tv[i] : k.T[c];
+            V3 Uc = v3(&k.U[3 * c]);
+            V3 ub;
+            if (bc.uT == BC_FIXED) ub = v3(&bc.uv[3 * i]);
+            else if (bc.uT == BC_SLIP) {
+                V3 n = v3(&m.Sf[3 * f]) / m.magSf[f];
+                // (U + (I - 2 n n) & U)/2
+                T9 Tr{{1 - 2.0 * n.x * n.x, -2.0 * n.x * n.y, -2.0 * n.x * n.z,
+                       -2.0 * n.y * n.x, 1 - 2.0 * n.y * n.y, -2.0 * n.y * n.z,
+                       -2.0 * n.z * n.x, -2.0 * n.z * n.y, 1 - 2.0 * n.z * n.z}};
+                ub = (Uc + tdotv(Tr, Uc)) / 2.0;
+            } else ub = Uc;
+            k.UB[3 * b] = ub.x; k.UB[3 * b + 1] = ub.y; k.UB[3 * b + 2] = ub.z;
+            const double psi = 1.0 / (k.s.R * k.TB[b]);
+            k.eB[b] = k.Cv * k.TB[b];
+            k.rhoB[b] = psi * k.pB[b];
+            for (int d = 0; d < 3; ++d) k.rhoUB[3 * b + d] = k.rhoB[b] * k.UB[3 * b + d];
+            k.rhoEB[b] = k.rhoB[b] * (k.eB[b] + 0.5 * magSqr(ub));
+        }
+    }
+}
+
+// ------------------------------------------------------ reconstruction ----
+// Gauss linear gradient (gaussGrad::gradf) of a cell field with boundary
+// face values vb (indexed by boundary face).
+void gaussGrad(const Mesh& m, const double* vc, const double* vb, int stride, std::vector<V3>& g) {
+    g.assign(m.nC, V3{0, 0, 0});
+    for (int f = 0; f < m.nI; ++f) {
+        const int o = m.own[f], n = m.nei[f];
+        const double vf = m.w[f] * (vc[o * stride] - vc[n * stride]) + vc[n * stride];
+        V3 S = v3(&m.Sf[3 * f]) * vf;
+        g[o] = g[o] + S;
+        g[n] = g[n] - S;
+    }
+    for (int f = m.nI; f < m.nF; ++f) {
+        const int pi = m.facePatch[f - m.nI];
+        if (m.pKind[pi] == PK_EMPTY) continue;
+        g[m.own[f]] = g[m.own[f]] + v3(&m.Sf[3 * f]) * vb[(f - m.nI) * stride];
+    }
+    for (int c = 0; c < m.nC; ++c) g[c] = g[c] / m.V[c];
+}
+
+inline void limitFace(double& lim, double maxD, double minD, double ext, int limiter) {
+    double r = 1;
+    if (ext > SMALL) r = maxD / ext;
+    else if (ext < -SMALL) r = minD / ext;
+    double l;
+    if (limiter == 1) l = (sqr(r) + 2 * r) / (sqr(r) + r + 2);   // Venkatakrishnan
+    else l = std::min(r, 1.0);                                   // minmod
+    lim = std::min(lim, l);
+}
+
+// cellLimitedGrad<limiter> Gauss linear 1 (k = 1).
+void limitedGrad(const Case& k, const double* vc, const double* vb, int stride, std::vector<V3>& g) {
+    const Mesh& m = *k.m;
+    gaussGrad(m, vc, vb, stride, g);
+    if (k.s.limiter == 0) return;
+    vec mx(m.nC), mn(m.nC);
+    for (int c = 0; c < m.nC; ++c) mx[c] = mn[c] = vc[c * stride];
+    for (int f = 0; f < m.nI; ++f) {
+        const int o = m.own[f], n = m.nei[f];
+        mx[o] = std::max(mx[o], vc[n * stride]); mn[o] = std::min(mn[o], vc[n * stride]);
+        mx[n] = std::max(mx[n], vc[o * stride]); mn[n] = std::min(mn[n], vc[o * stride]);
+    }
+    for (int f = m.nI; f < m.nF; ++f) {
+        if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
+        const int o = m.own[f];
+        const double b = vb[(f - m.nI) * stride];
+        mx[o] = std::max(mx[o], b); mn[o] = std::min(mn[o], b);
+    }
+    for (int c = 0; c < m.nC; ++c) { mx[c] -= vc[c * stride]; mn[c] -= vc[c * stride]; }
+    vec lim(m.nC, 1.0);
+    for (int f = 0; f < m.nI; ++f) {
+        const int o = m.own[f], n = m.nei[f];
+        V3 fc = v3(&m.Cf[3 * f]);
+        limitFace(lim[o], mx[o], mn[o], dot(fc - v3(&m.C[3 * o]), g[o]), k.s.limiter);
+        limitFace(lim[n], mx[n], mn[n], dot(fc - v3(&m.C[3 * n]), g[n]), k.s.limiter);
+    }
+    for (int f = m.nI; f < m.nF; ++f) {
+        if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
+        const int o = m.own[f];
+        limitFace(lim[o], mx[o], mn[o], dot(v3(&m.Cf[3 * f]) - v3(&m.C[3 * o]), g[o]), k.s.limiter);
+    }
+    for (int c = 0; c < m.nC; ++c) g[c] = g[c] * lim[c];
+}
+
+// fvc::interpolate(vf, pos_/neg_, "reconstruct(X)") with linearUpwind (or
+// upwind) weights: returns L (owner side) and R (neighbour side) values on
+// every face; boundary faces carry the patch value on both sides.
+void reconstruct(const Case& k, const double* vc, const double* vb, int stride, vec& L, vec& R) {
+    const Mesh& m = *k.m;
+    std::vector<V3> g;
+    if (k.s.recon == 1) limitedGrad(k, vc, vb, stride, g);
+    L.assign(m.nF, 0);
+    R.assign(m.nF, 0);
+    for (int f = 0; f < m.nI; ++f) {
+        const int o = m.own[f], n = m.nei[f];
+        double l = 1.0 * (vc[o * stride] - vc[n * stride]) + vc[n * stride];
+        double r = 0.0 * (vc[o * stride] - vc[n * stride]) + vc[n * stride];
+        if (k.s.recon == 1) {
+            V3 fc = v3(&m.Cf[3 * f]);
+            l += dot(fc - v3(&m.C[3 * o]), g[o]);
+            r += dot(fc - v3(&m.C[3 * n]), g[n]);
+        }
+        L[f] = l;
+        R[f] = r;
+    }
+    for (int f = m.nI; f < m.nF; ++f) {
+        if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
+        L[f] = R[f] = vb[(f - m.nI) * stride];
+    }
+}
+
+inline double interpLinear(const Mesh& m, int f, const double* vc, int stride) {
+    return m.w[f] * (vc[m.own[f] * stride] - vc[m.nei[f] * stride]) + vc[m.nei[f] * stride];
+}
+
+// Derived cell/boundary fields used by the flux schemes.
+struct Derived {
+    vec E, H, cstar, Eb, Hb, cstarb, c, cb;  // c = sqrt(gamma/psi)
+};
+void derive(const Case& k, Derived& d) {
+    const Mesh& m = *k.m;
+    const double g = k.s.gamma;
+    const int nB = m.nF - m.nI;
+    d.E.resize(m.nC); d.H.resize(m.nC); d.cstar.resize(m.nC); d.c.resize(m.nC);
+    d.Eb.assign(nB, 0); d.Hb.assign(nB, 0); d.cstarb.assign(nB, 0); d.cb.assign(nB, 0);
+    for (int c = 0; c < m.nC; ++c) {
+        d.E[c] = k.Cv * k.T[c] + 0.5 * magSqr(v3(&k.U[3 * c]));
+        d.H[c] = std::max(d.E[c], SMALL) + std::max(k.p[c] / k.rho[c], SMALL);
+        d.cstar[c] = std::sqrt(2.0 * (g - 1.0) / (g + 1.0) * d.H[c]);
+        const double psi = 1.0 / (k.s.R * k.T[c]);
+        d.c[c] = std::sqrt(g / psi);
+    }
+    for (int b = 0; b < nB; ++b) {
+        if (k.TB[b] == 0) continue;  // empty
+        d.Eb[b] = k.Cv * k.TB[b] + 0.5 * magSqr(v3(&k.UB[3 * b]));
+        d.Hb[b] = std::max(d.Eb[b], SMALL) + std::max(k.pB[b] / k.rhoB[b], SMALL);
+        d.cstarb[b] = std::sqrt(2.0 * (g - 1.0) / (g + 1.0) * d.Hb[b]);
+        const double psi = 1.0 / (k.s.R * k.TB[b]);
+        d.cb[b] = std::sqrt(g / psi);
+    }
+}
+
+// ---------------------------------------------------------------- fluxes --
+// AUSM+up (ausmPlusUpFluxScheme.C:73-299) on one face.
+void ausmFace(const Case& k, V3 S, double magS, V3 UL, V3 UR, double cL, double cR, double pL,
+              double pR, double rL, double rR, double HL, double HR, double& phi, V3& phiU, double& phiE) {
+    const double unL = dot(UL, S) / magS;
+    const double unR = dot(UR, S) / magS;
+    const double cl = sqr(cL) / std::max(cL, unL);
+    const double cr = sqr(cR) / std::max(cR, -unR);
+    const double cf = std::min(cl, cr);
+    const double ML = unL / cf;
+    double MpL, PpL;
+    if (std::fabs(ML) < 1.0) {
+        const double p2 = 0.25 * sqr(ML + 1), m2 = -0.25 * sqr(ML - 1);
+        MpL = p2 * (1 - 2 * m2);
+        PpL = p2 * (2 - ML - 3 * ML * m2);
+    } else {
+        MpL = std::max(ML, 0.0);
+        PpL = ML > 0 ? 1.0 : 0.0;
+    }
+    const double MR = unR / cf;
+    double MmR, PmR;
+    if (std::fabs(MR) < 1.0) {
+        const double m2 = -0.25 * sqr(MR - 1), p2 = 0.25 * sqr(MR + 1);
+        MmR = m2 * (1 + 2 * p2);
+        PmR = m2 * (-2 - MR + 3 * MR * p2);
+    } else {
+        MmR = std::min(MR, 0.0);
+        PmR = MR < 0 ? 1.0 : 0.0;
+    }
+    double M12 = MpL + MmR;
+    double p12 = PpL * pL + PmR * pR;
+    const double Mmean = 0.5 * (sqr(unL) + sqr(unR)) / sqr(cf);
+    const double MDiff = -0.25 * std::max(1.0 - Mmean, 0.0) * (pR - pL) / (0.5 * (rL + rR) * sqr(cf));
+    if ((M12 > 0.0 && M12 + MDiff <= 0.0) || (M12 < 0.0 && M12 + MDiff >= 0.0)) M12 += 0.2 * MDiff;
+    else M12 += MDiff;
+    if (k.s.lowMach) p12 += -0.25 * PpL * PmR * (rL + rR) * cf * (unR - unL);
+    const bool up = M12 >= 0;
+    const V3 Uf = up ? UL : UR;
+    const double rhoa = M12 * cf * (up ? rL : rR);
+    const V3 rhoaU = rhoa * Uf;
+    const double rhoah = rhoa * (up ? HL : HR);
+    phi = rhoa * magS;
+    phiU = rhoaU * magS + p12 * S;
+    phiE = rhoah * magS;
+}
+
+struct RoeDiss {
+    double CR, CE, ER, EE;
+    V3 CU, MR, ME, EU;
+    T9 MU;
+};
+
+// roeFluxScheme::getRoeDissipation (roeFluxScheme.C:41-241), one face.
+RoeDiss roeDissipation(double rho, V3 Vel, double c, double gI, double u, V3 n, double eFix) {
+    const double a1 = gI - 1;
+    const double theta = 0.5 * a1 * magSqr(Vel);
+    const double c2 = sqr(c);
+    const double a2 = 1 / (rho * c * std::sqrt(2.0));
+    const double a3 = rho / (c * std::sqrt(2.0));
+    const double a4 = (theta + c2) / a1;
+    const double a5 = 1 - theta / c2;
+    const double a6 = theta / a1;
+    V3 iP11 = (n * a5) - cross(Vel, n) / rho;
+    T9 iP12 = outer(a1 / c2 * n, Vel);
+    iP12.t[1] += n.z / rho; iP12.t[2] -= n.y / rho; iP12.t[3] -= n.z / rho;
+    iP12.t[5] += n.x / rho; iP12.t[6] += n.y / rho; iP12.t[7] -= n.x / rho;
+    V3 iP13 = (-a1) / c2 * n;
+    double iP21 = a2 * (theta - c * u);
+    V3 iP22 = (-a2) * (a1 * Vel - c * n);
+    double iP23 = a1 * a2;
+    double iP31 = a2 * (theta + c * u);
+    V3 iP32 = (-a2) * (a1 * Vel + c * n);
+    double iP33 = a1 * a2;
+    double L1 = std::fabs(u), L2 = std::fabs(u + c), L3 = std::fabs(u - c);
+    const double eps = eFix * std::max(L2, L3);
+    if (L1 < eps) L1 = (sqr(L1) + sqr(eps)) / (2.0 * eps);
+    if (L2 < eps) L2 = (sqr(L2) + sqr(eps)) / (2.0 * eps);
+    if (L3 < eps) L3 = (sqr(L3) + sqr(eps)) / (2.0 * eps);
+    V3 P11 = n;
+    double P12 = a3, P13 = a3;
+    T9 P21 = outer(Vel, n);
+    P21.t[1] -= n.z * rho; P21.t[2] += n.y * rho; P21.t[3] += n.z * rho;
+    P21.t[5] -= n.x * rho; P21.t[6] -= n.y * rho; P21.t[7] += n.x * rho;
+    V3 P22 = a3 * (Vel + c * n);
+    V3 P23 = a3 * (Vel - c * n);
+    V3 P31 = n * a6 + rho * cross(Vel, n);
+    double P32 = a3 * (a4 + c * u);
+    double P33 = a3 * (a4 - c * u);
+    RoeDiss D;
+    V3 P11L = P11 * L1;
+    T9 P21L = P21 * L1;
+    V3 P31L = P31 * L1;
+    D.CR = dot(P11L, iP11) + L2 * P12 * iP21 + L3 * P13 * iP31;
+    D.CU = vdott(P11L, iP12) + (L2 * P12) * iP22 + (L3 * P13) * iP32;
+    D.CE = dot(P11L, iP13) + L2 * P12 * iP23 + L3 * P13 * iP33;
+    D.MR = tdotv(P21L, iP11) + (L2 * P22) * iP21 + (L3 * P23) * iP31;
+    D.MU = tdott(P21L, iP12) + outer(L2 * P22, iP22) + outer(L3 * P23, iP32);
+    D.ME = tdotv(P21L, iP13) + (L2 * P22) * iP23 + (L3 * P23) * iP33;
+    D.ER = dot(P31L, iP11) + L2 * P32 * iP21 + L3 * P33 * iP31;
+    D.EU = vdott(P31L, iP12) + (L2 * P32) * iP22 + (L3 * P33) * iP32;
+    D.EE = dot(P31L, iP13) + L2 * P32 * iP23 + L3 * P33 * iP33;
+    return D;
+}
+
+// roeFluxScheme::calcFlux (roeFluxScheme.C:276-409), one face.
+void roeFace(const Case& k, V3 S, double magS, double rl, double rr, double pl, double pr, V3 Ul, V3 Ur,
+             double El, double Er, double Hl, double Hr, double& phi, V3& phiU, double& phiE) {
+    const V3 n = S / magS;
+    const double coefR = std::sqrt(std::max(VSMALL, rr) / std::max(VSMALL, rl));
+    const double rhoRoe = coefR * rl;
+    const V3 URoe = (coefR * Ur + Ul) / (coefR + 1.0);
+    const double HRoe = (coefR * Hr + Hl) / (coefR + 1.0);
+    const double gI = k.s.gamma;
+    const double cRoe = std::sqrt(std::fabs((gI - 1.0) * (HRoe - 0.5 * magSqr(URoe))));
+    const double uMl = dot(Ul, n), uMr = dot(Ur, n);
+    const double uProj = dot(URoe, n);
+    const V3 rUl = rl * Ul, rUr = rr * Ur;
+    const double rEl = rl * El, rEr = rr * Er;
+    RoeDiss D = roeDissipation(rhoRoe, URoe, cRoe, gI, uProj, n, k.s.entropyFix);
+    const double dR = rr - rl;
+    const V3 dRU = rUr - rUl;
+    const double dRE = rEr - rEl;
+    phi = -0.5 * magS * (D.CR * dR + dot(D.CU, dRU) + D.CE * dRE);
+    phiU = -0.5 * magS * (D.MR * dR + tdotv(D.MU, dRU) + D.ME * dRE);
+    phiE = -0.5 * magS * (D.ER * dR + dot(D.EU, dRU) + D.EE * dRE);
+    const double rnl = rl * uMl, rnr = rr * uMr;
+    phi += 0.5 * magS * (rnl + rnr);
+    phiU = phiU + 0.5 * magS * (rnl * Ul + rnr * Ur + n * (pl + pr));
+    phiE += 0.5 * magS * (rnl * Hl + rnr * Hr);
+}
+
+// convectiveFluxScheme::calcFlux for the selected scheme on all faces.
+void calcFlux(Case& k, vec& phi, vec& phiU, vec& phiE) {
+    Mesh& m = *k.m;
+    Derived d;
+    derive(k, d);
+    const int nB = m.nF - m.nI;
+    vec Ux(m.nC), Uy(m.nC), Uz(m.nC), Ubx(nB), Uby(nB), Ubz(nB);
+    for (int c = 0; c < m.nC; ++c) { Ux[c] = k.U[3 * c]; Uy[c] = k.U[3 * c + 1]; Uz[c] = k.U[3 * c + 2]; }
+    for (int b = 0; b < nB; ++b) { Ubx[b] = k.UB[3 * b]; Uby[b] = k.UB[3 * b + 1]; Ubz[b] = k.UB[3 * b + 2]; }
+    vec UxL, UxR, UyL, UyR, UzL, UzR, pL, pR, rL, rR, HL, HR, cL, cR, EL, ER;
+    reconstruct(k, Ux.data(), Ubx.data(), 1, UxL, UxR);
+    reconstruct(k, Uy.data(), Uby.data(), 1, UyL, UyR);
+    reconstruct(k, Uz.data(), Ubz.data(), 1, UzL, UzR);
+    reconstruct(k, k.p.data(), k.pB.data(), 1, pL, pR);
+    reconstruct(k, k.rho.data(), k.rhoB.data(), 1, rL, rR);
+    reconstruct(k, d.H.data(), d.Hb.data(), 1, HL, HR);
+    if (k.s.flux == FLUX_AUSM) reconstruct(k, d.cstar.data(), d.cstarb.data(), 1, cL, cR);
+    else reconstruct(k, d.E.data(), d.Eb.data(), 1, EL, ER);
+    phi.assign(m.nF, 0);
+    phiU.assign(3 * m.nF, 0);
+    phiE.assign(m.nF, 0);
+    for (int f = 0; f < m.nF; ++f) {
+        if (f >= m.nI && m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
+        V3 S = v3(&m.Sf[3 * f]);
+        V3 UL{UxL[f], UyL[f], UzL[f]}, UR{UxR[f], UyR[f], UzR[f]};
+        double ph, pe;
+        V3 pu;
+        if (k.s.flux == FLUX_AUSM)
+            ausmFace(k, S, m.magSf[f], UL, UR, cL[f], cR[f], pL[f], pR[f], rL[f], rR[f], HL[f], HR[f], ph, pu, pe);
+        else
+            roeFace(k, S, m.magSf[f], rL[f], rR[f], pL[f], pR[f], UL, UR, EL[f], ER[f], HL[f], HR[f], ph, pu, pe);
+        phi[f] = ph;
+        phiU[3 * f] = pu.x; phiU[3 * f + 1] = pu.y; phiU[3 * f + 2] = pu.z;
+        phiE[f] = pe;
+    }
+}
+
+// fvc::surfaceIntegrate of one scalar surface field (internal+boundary),
+// divided by V.
+void surfaceIntegrate(const Mesh& m, const double* sf, int stride, int comp, vec& out) {
+    out.assign(m.nC, 0);
+    for (int f = 0; f < m.nI; ++f) {
+        out[m.own[f]] += sf[f * stride + comp];
+        out[m.nei[f]] -= sf[f * stride + comp];
+    }
+    for (int f = m.nI; f < m.nF; ++f) {
+        if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
+        out[m.own[f]] += sf[f * stride + comp];
+    }
+    for (int c = 0; c < m.nC; ++c) out[c] /= m.V[c];
+}
+
+// Viscous residual terms (residualsUpdate.H:16-43), added to R (per-volume).
+void viscousResidual(Case& k, vec& RU /*3nC*/, vec& RE) {
+    Mesh& m = *k.m;
+    const int nB = m.nF - m.nI;
+    const double mu = k.s.mu;
+    const double alpha = k.s.gamma * mu / k.s.Pr;  // alphahe = Cp/Cv * kappa/Cp
+    // grad(U) Gauss linear: tensor gU_ij = d U_j / d x_i
+    std::vector<T9> gU(m.nC, T9{{0}});
+    for (int f = 0; f < m.nI; ++f) {
+        const int o = m.own[f], n = m.nei[f];
+        V3 Uf = m.w[f] * (v3(&k.U[3 * o]) - v3(&k.U[3 * n])) + v3(&k.U[3 * n]);
+        T9 t = outer(v3(&m.Sf[3 * f]), Uf);
+        gU[o] = gU[o] + t;
+        gU[n] = gU[n] - t;
+    }
+    for (int f = m.nI; f < m.nF; ++f) {
+        if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
+        T9 t = outer(v3(&m.Sf[3 * f]), v3(&k.UB[3 * (f - m.nI)]));
+        gU[m.own[f]] = gU[m.own[f]] + t;
+    }
+    for (int c = 0; c < m.nC; ++c)
+        for (int i = 0; i < 9; ++i) gU[c].t[i] /= m.V[c];
+    // boundary snGrad of U and gradient boundary values
+    std::vector<V3> snU(nB, V3{0, 0, 0});
+    std::vector<T9> gUb(nB, T9{{0}});
+    for (size_t pi = 0; pi < m.pStart.size(); ++pi) {
+        if (m.pKind[pi] == PK_EMPTY) continue;
+        for (int i = 0; i < m.pSize[pi]; ++i) {
+            const int f = m.pStart[pi] + i, b = f - m.nI, c = m.own[f];
+            V3 n = v3(&m.Sf[3 * f]) / m.magSf[f];
+            V3 Uc = v3(&k.U[3 * c]);
+            const int ut = k.bcs[pi].uT;
+            if (ut == BC_ZG) snU[b] = V3{0, 0, 0};
+            else if (ut == BC_SLIP) {
+                T9 Tr{{1 - 2.0 * n.x * n.x, -2.0 * n.x * n.y, -2.0 * n.x * n.z,
+                       -2.0 * n.y * n.x, 1 - 2.0 * n.y * n.y, -2.0 * n.y * n.z,
+                       -2.0 * n.z * n.x, -2.0 * n.z * n.y, 1 - 2.0 * n.z * n.z}};
+                snU[b] = (tdotv(Tr, Uc) - Uc) * (m.dc[f] / 2.0);
+            } else snU[b] = m.dc[f] * (v3(&k.UB[3 * b]) - Uc);
+            gUb[b] = gU[c] + outer(n, snU[b] - vdott(n, gU[c]));
+        }
+    }
+    std::vector<T9> tau(m.nC), taub(nB, T9{{0}});
+    for (int c = 0; c < m.nC; ++c) tau[c] = mu * dev2(transposeT(gU[c]));
+    for (int b = 0; b < nB; ++b) taub[b] = mu * dev2(transposeT(gUb[b]));
+    // eCalc and its Gauss gradient
+    vec ec(m.nC), eb(nB, 0);
+    for (int c = 0; c < m.nC; ++c) ec[c] = k.rhoE[c] / k.rho[c] - 0.5 * magSqr(v3(&k.U[3 * c]));
+    for (int b = 0; b < nB; ++b)
+        if (k.rhoB[b] != 0) eb[b] = k.rhoEB[b] / k.rhoB[b] - 0.5 * magSqr(v3(&k.UB[3 * b]));
+    std::vector<V3> ge;
+    gaussGrad(m, ec.data(), eb.data(), 1, ge);
+    // surface fields
+    vec lapU(3 * m.nF, 0), divTau(3 * m.nF, 0), sDotU(m.nF, 0), lapE(m.nF, 0);
+    for (int f = 0; f < m.nI; ++f) {
+        const int o = m.own[f], n = m.nei[f];
+        const double w = m.w[f];
+        V3 S = v3(&m.Sf[3 * f]);
+        V3 k3 = v3(&m.corr[3 * f]);
+        T9 gUf = w * (gU[o] - gU[n]) + gU[n];
+        T9 tauf = w * (tau[o] - tau[n]) + tau[n];
+        V3 Uf = w * (v3(&k.U[3 * o]) - v3(&k.U[3 * n])) + v3(&k.U[3 * n]);
+        const double muf = w * (mu - mu) + mu;
+        const double alf = w * (alpha - alpha) + alpha;
+        V3 sn = m.nodc[f] * (v3(&k.U[3 * n]) - v3(&k.U[3 * o])) + vdott(k3, gUf);
+        V3 l = (muf * sn) * m.magSf[f];
+        lapU[3 * f] = l.x; lapU[3 * f + 1] = l.y; lapU[3 * f + 2] = l.z;
+        V3 dt = vdott(S, tauf);
+        divTau[3 * f] = dt.x; divTau[3 * f + 1] = dt.y; divTau[3 * f + 2] = dt.z;
+        V3 sig = tdotv(muf * gUf + tauf, Uf);
+        sDotU[f] = dot(sig, S);
+        V3 gef = w * (ge[o] - ge[n]) + ge[n];
+        const double sne = m.nodc[f] * (ec[n] - ec[o]) + dot(k3, gef);
+        lapE[f] = (alf * sne) * m.magSf[f];
+    }
+    for (int f = m.nI; f < m.nF; ++f) {
+        if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
+        const int b = f - m.nI, c = m.own[f];
+        V3 S = v3(&m.Sf[3 * f]);
+        V3 l = (mu * snU[b]) * m.magSf[f];
+        lapU[3 * f] = l.x; lapU[3 * f + 1] = l.y; lapU[3 * f + 2] = l.z;
+        V3 dt = vdott(S, taub[b]);
+        divTau[3 * f] = dt.x; divTau[3 * f + 1] = dt.y; divTau[3 * f + 2] = dt.z;
+        V3 sig = tdotv(mu * gUb[b] + taub[b], v3(&k.UB[3 * b]));
+        sDotU[f] = dot(sig, S);
+        const double sne = m.dc[f] * (eb[b] - ec[c]);
+        lapE[f] = (alpha * sne) * m.magSf[f];
+    }
+    vec t;
+    for (int d = 0; d < 3; ++d) {
+        surfaceIntegrate(m, lapU.data(), 3, d, t);
+        for (int c = 0; c < m.nC; ++c) RU[3 * c + d] += t[c];
+    }
+    for (int d = 0; d < 3; ++d) {
+        surfaceIntegrate(m, divTau.data(), 3, d, t);
+        for (int c = 0; c < m.nC; ++c) RU[3 * c + d] += t[c];
+    }
+    surfaceIntegrate(m, sDotU.data(), 1, 0, t);
+    for (int c = 0; c < m.nC; ++c) RE[c] += t[c];
+    surfaceIntegrate(m, lapE.data(), 1, 0, t);
+    for (int c = 0; c < m.nC; ++c) RE[c] += t[c];
+}
+
+// residualsUpdate.H: b = R*V in [rho, rhoE, rhoU] order.
+void residual(Case& k, vec& b) {
+    Mesh& m = *k.m;
+    vec phi, phiU, phiE;
+    calcFlux(k, phi, phiU, phiE);
+    vec Rr, RU(3 * m.nC), RE, t;
+    surfaceIntegrate(m, phi.data(), 1, 0, Rr);
+    for (int c = 0; c < m.nC; ++c) Rr[c] = -Rr[c];
+    for (int d = 0; d < 3; ++d) {
+        surfaceIntegrate(m, phiU.data(), 3, d, t);
+        for (int c = 0; c < m.nC; ++c) RU[3 * c + d] = -t[c];
+    }
+    surfaceIntegrate(m, phiE.data(), 1, 0, RE);
+    for (int c = 0; c < m.nC; ++c) RE[c] = -RE[c];
+    if (k.s.viscous) viscousResidual(k, RU, RE);
+    const int nC = m.nC;
+    b.assign(5 * (size_t)nC, 0);
+    for (int c = 0; c < nC; ++c) {
+        b[c] = Rr[c] * m.V[c];
+        b[nC + c] = RE[c] * m.V[c];
+        for (int d = 0; d < 3; ++d) b[(2 + d) * (size_t)nC + c] = RU[3 * c + d] * m.V[c];
+    }
+}
+
+// setCoAndDeltaT.H:39-142 (local time stepping): rPseudoDeltaT.
+void localDeltaT(Case& k) {
+    Mesh& m = *k.m;
+    Derived d;
+    derive(k, d);
+    k.rDT.assign(m.nC, 0.0);
+    for (int f = 0; f < m.nI; ++f) {
+        const double cf = interpLinear(m, f, d.c.data(), 1);
+        V3 Uf = m.w[f] * (v3(&k.U[3 * m.own[f]]) - v3(&k.U[3 * m.nei[f]])) + v3(&k.U[3 * m.nei[f]]);
+        const double lam = cf + std::fabs(dot(Uf, v3(&m.Sf[3 * f]) / m.magSf[f]));
+        const double frdt = m.nodc[f] * lam;
+        k.rDT[m.own[f]] = std::max(k.rDT[m.own[f]], frdt);
+        k.rDT[m.nei[f]] = std::max(k.rDT[m.nei[f]], frdt);
+    }
+    for (size_t pi = 0; pi < m.pStart.size(); ++pi) {
+        if (m.pKind[pi] != PK_WALL) continue;
+        for (int f = m.pStart[pi]; f < m.pStart[pi] + m.pSize[pi]; ++f) {
+            const int c = m.own[f];
+            const double pl = 0.5 * m.nodc[f] * (d.c[c] + std::fabs(dot(v3(&k.U[3 * c]), v3(&m.Sf[3 * f]) / m.magSf[f])));
+            k.rDT[c] = std::max(k.rDT[c], pl);
+        }
+    }
+    for (int c = 0; c < m.nC; ++c) k.rDT[c] /= k.pseudoCo[c];
+}
+
+// ------------------------------------------------------------- Jacobian ---
+// addFluxTerms / addDissipationJacobian / addBoundaryTerms /
+// addTemporalTerms (convectiveFluxScheme.C:47-546), viscous LF term
+// (viscousFluxScheme.C:220-247).  Block layout follows blockFvMatrix:
+// insertBlock (blockFvMatrix.C:212-268) and insertDissipationBlock (:272-326)
+// build a temporary matrix whose diagonal comes from negSumDiag, then add /
+// subtract it.
+void addBlockTo(LduBlock& A, const LduBlock& B, double sign) {
+    for (size_t i = 0; i < A.diag.size(); ++i) A.diag[i] += sign * B.diag[i];
+    for (size_t i = 0; i < A.upper.size(); ++i) A.upper[i] += sign * B.upper[i];
+    for (size_t i = 0; i < A.lower.size(); ++i) A.lower[i] += sign * B.lower[i];
+}
+void negSumDiag(const Mesh& m, LduBlock& B) {
+    const int nc = B.ncomp;
+    for (int f = 0; f < m.nI; ++f)
+        for (int q = 0; q < nc; ++q) {
+            B.diag[m.own[f] * nc + q] -= B.lower[f * nc + q];
+            B.diag[m.nei[f] * nc + q] -= B.upper[f * nc + q];
+        }
+}
+
+void assemble(Case& k) {
+    Mesh& m = *k.m;
+    const int nC = m.nC, nI = m.nI, nB = m.nF - m.nI;
+    const double g = k.s.gamma;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) k.SS[i][j].init(nC, nI, 1);
+    for (int i = 0; i < 2; ++i) { k.SV[i].init(nC, nI, 3); k.VS[i].init(nC, nI, 3); }
+    k.VV.init(nC, nI, 9);
+    Derived d;
+    derive(k, d);
+    // left/right states for the Jacobian: U (reconstruct(U)) and E (reconstruct(T))
+    vec Ux(nC), Uy(nC), Uz(nC), Ubx(nB), Uby(nB), Ubz(nB);
+    for (int c = 0; c < nC; ++c) { Ux[c] = k.U[3 * c]; Uy[c] = k.U[3 * c + 1]; Uz[c] = k.U[3 * c + 2]; }
+    for (int b = 0; b < nB; ++b) { Ubx[b] = k.UB[3 * b]; Uby[b] = k.UB[3 * b + 1]; Ubz[b] = k.UB[3 * b + 2]; }
+    vec UxL, UxR, UyL, UyR, UzL, UzR, EL, ER;
+    reconstruct(k, Ux.data(), Ubx.data(), 1, UxL, UxR);
+    reconstruct(k, Uy.data(), Uby.data(), 1, UyL, UyR);
+    reconstruct(k, Uz.data(), Ubz.data(), 1, UzL, UzR);
+    reconstruct(k, d.E.data(), d.Eb.data(), 1, EL, ER);
+    // insertBlock for each sub-block: temporaries with negSumDiag then +=
+    LduBlock tS, tV, tT;
+    auto insert = [&](LduBlock& target, int nc, auto leftFn, auto rightFn) {
+        LduBlock mx;
+        mx.init(nC, nI, nc);
+        double lv[9], rv[9];
+        for (int f = 0; f < nI; ++f) {
+            leftFn(f, lv);
+            rightFn(f, rv);
+            const double ms = m.magSf[f];
+            for (int q = 0; q < nc; ++q) {
+                mx.upper[f * nc + q] = 0.5 * ms * rv[q];
+                mx.lower[f * nc + q] = -0.5 * ms * lv[q];
+            }
+        }
+        negSumDiag(m, mx);
+        addBlockTo(target, mx, 1.0);
+    };
+    auto state = [&](int f, bool left, V3& U, double& E) {
+        if (left) { U = V3{UxL[f], UyL[f], UzL[f]}; E = EL[f]; }
+        else { U = V3{UxR[f], UyR[f], UzR[f]}; E = ER[f]; }
+    };
+    auto nrm = [&](int f) { return v3(&m.Sf[3 * f]) / m.magSf[f]; };
+    const double gI = g;  // linear interpolation of a uniform gamma
+    auto theta = [&](V3 U) { return 0.5 * (gI - 1) * magSqr(U); };
+    // dContByRhoU: n, n
+    insert(k.SV[0], 3, [&](int f, double* o) { V3 n = nrm(f); o[0] = n.x; o[1] = n.y; o[2] = n.z; },
+           [&](int f, double* o) { V3 n = nrm(f); o[0] = n.x; o[1] = n.y; o[2] = n.z; });
+    auto momByRho = [&](bool left) {
+        return [&, left](int f, double* o) {
+            V3 U; double E; state(f, left, U, E);
+            V3 n = nrm(f);
+            V3 r = (n * theta(U)) - (U * dot(U, n));
+            o[0] = r.x; o[1] = r.y; o[2] = r.z;
+        };
+    };
+    insert(k.VS[0], 3, momByRho(true), momByRho(false));
+    auto momByRhoU = [&](bool left) {
+        return [&, left](int f, double* o) {
+            V3 U; double E; state(f, left, U, E);
+            V3 n = nrm(f);
+            const double a2 = gI - 1;
+            T9 r = outer(U, n) - outer(a2 * n, U);
+            const double pu = dot(U, n);
+            r.t[0] += pu; r.t[4] += pu; r.t[8] += pu;
+            for (int q = 0; q < 9; ++q) o[q] = r.t[q];
+        };
+    };
+    insert(k.VV, 9, momByRhoU(true), momByRhoU(false));
+    auto momByRhoE = [&](bool left) {
+        return [&, left](int f, double* o) {
+            V3 n = nrm(f);
+            V3 r = n * (gI - 1);
+            (void)left;
+            o[0] = r.x; o[1] = r.y; o[2] = r.z;
+        };
+    };
+    insert(k.VS[1], 3, momByRhoE(true), momByRhoE(false));
+    auto enByRho = [&](bool left) {
+        return [&, left](int f, double* o) {
+            V3 U; double E; state(f, left, U, E);
+            V3 n = nrm(f);
+            const double th = theta(U);
+            const double a1 = gI * E - th;
+            o[0] = dot(U, n) * (th - a1);
+        };
+    };
+    insert(k.SS[1][0], 1, enByRho(true), enByRho(false));
+    auto enByRhoU = [&](bool left) {
+        return [&, left](int f, double* o) {
+            V3 U; double E; state(f, left, U, E);
+            V3 n = nrm(f);
+            const double th = theta(U);
+            const double a1 = gI * E - th;
+            const double a2 = gI - 1;
+            V3 r = (n * a1) - ((a2 * U) * dot(U, n));
+            o[0] = r.x; o[1] = r.y; o[2] = r.z;
+        };
+    };
+    insert(k.SV[1], 3, enByRhoU(true), enByRhoU(false));
+    auto enByRhoE = [&](bool left) {
+        return [&, left](int f, double* o) {
+            V3 U; double E; state(f, left, U, E);
+            o[0] = gI * dot(U, nrm(f));
+        };
+    };
+    insert(k.SS[1][1], 1, enByRhoE(true), enByRhoE(false));
+    // Dissipation: lambdaConv = interp(c) + |interp(U).n|
+    vec lamI(nI), lamB(nB, 0);
+    for (int f = 0; f < nI; ++f) {
+        const double cf = interpLinear(m, f, d.c.data(), 1);
+        V3 Uf = m.w[f] * (v3(&k.U[3 * m.own[f]]) - v3(&k.U[3 * m.nei[f]])) + v3(&k.U[3 * m.nei[f]]);
+        lamI[f] = cf + std::fabs(dot(Uf, nrm(f)));
+    }
+    for (int f = nI; f < m.nF; ++f) {
+        if (m.pKind[m.facePatch[f - nI]] == PK_EMPTY) continue;
+        lamB[f - nI] = d.cb[f - nI] + std::fabs(dot(v3(&k.UB[3 * (f - nI)]), nrm(f)));
+    }
+    auto dissip = [&](LduBlock& target, int nc) {
+        LduBlock mx;
+        mx.init(nC, nI, nc);
+        for (int f = 0; f < nI; ++f) {
+            const double v = 0.5 * m.magSf[f] * lamI[f];
+            if (nc == 1) { mx.upper[f] = v; mx.lower[f] = v; }
+            else {
+                for (int q = 0; q < 9; ++q) { mx.upper[9 * f + q] = 0; mx.lower[9 * f + q] = 0; }
+                mx.upper[9 * f] = mx.upper[9 * f + 4] = mx.upper[9 * f + 8] = v;
+                mx.lower[9 * f] = mx.lower[9 * f + 4] = mx.lower[9 * f + 8] = v;
+            }
+        }
+        negSumDiag(m, mx);
+        for (int f = nI; f < m.nF; ++f) {
+            if (m.pKind[m.facePatch[f - nI]] == PK_EMPTY) continue;
+            const double v = 0.5 * m.magSf[f] * lamB[f - nI];
+            const int o = m.own[f];
+            if (nc == 1) mx.diag[o] -= v;
+            else { mx.diag[9 * o] -= v; mx.diag[9 * o + 4] -= v; mx.diag[9 * o + 8] -= v; }
+        }
+        addBlockTo(target, mx, -1.0);
+    };
+    dissip(k.SS[0][0], 1);
+    dissip(k.VV, 9);
+    dissip(k.SS[1][1], 1);
+    // addBoundaryTerms (convectiveFluxScheme.C:219-355)
+    for (size_t pi = 0; pi < m.pStart.size(); ++pi) {
+        if (m.pKind[pi] == PK_EMPTY) continue;
+        const BC& bc = k.bcs[pi];
+        for (int i = 0; i < m.pSize[pi]; ++i) {
+            const int f = m.pStart[pi] + i, b = f - nI, c = m.own[f];
+            V3 Sf = v3(&m.Sf[3 * f]);
+            V3 n = Sf / m.magSf[f];
+            const double pVIC = bc.pT == BC_FIXED ? 0.0 : 1.0;
+            const double tVIC = bc.tT == BC_FIXED ? 0.0 : 1.0;
+            V3 uVIC;
+            if (bc.uT == BC_FIXED) uVIC = V3{0, 0, 0};
+            else if (bc.uT == BC_SLIP) uVIC = V3{1 - std::fabs(n.x), 1 - std::fabs(n.y), 1 - std::fabs(n.z)};
+            else uVIC = V3{1, 1, 1};
+            const double rhoB = k.rhoB[b], pB = k.pB[b], TB = k.TB[b];
+            V3 UB = v3(&k.UB[3 * b]);
+            const double UrelSf = dot(UB, Sf);
+            const double rhoEB = rhoB * (k.Cv * TB + 0.5 * magSqr(UB));
+            const double cvB = k.Cv;
+            V3 SfV{Sf.x * uVIC.x, Sf.y * uVIC.y, Sf.z * uVIC.z};
+            const double dCdp = rhoB / pB * UrelSf * pVIC;
+            V3 dCdU = rhoB * SfV;
+            const double dCdT = -rhoB / TB * UrelSf * tVIC;
+            V3 dMdp = (rhoB / pB * UB * UrelSf + Sf) * pVIC;
+            T9 dMdU = outer(rhoB * UB, SfV);
+            V3 dMdUDiag = rhoB * UrelSf * uVIC;
+            V3 dMdT = -rhoB / TB * UB * UrelSf * tVIC;
+            const double dEdp = (rhoEB / pB * UrelSf + dot(UB, Sf)) * pVIC;
+            V3 UBV{UB.x * uVIC.x, UB.y * uVIC.y, UB.z * uVIC.z};
+            V3 dEdU = SfV * (rhoEB + pB) + rhoB * UrelSf * UBV;
+            const double dEdT = UrelSf * (rhoB * cvB - rhoEB / TB) * tVIC;
+            dMdU.t[0] += dMdUDiag.x; dMdU.t[4] += dMdUDiag.y; dMdU.t[8] += dMdUDiag.z;
+            const double rhoI = k.rho[c];
+            V3 UI = v3(&k.U[3 * c]);
+            const double rhoEI = rhoI * (k.Cv * k.T[c] + 0.5 * magSqr(UI));
+            const double gammaI = g, cvI = k.Cv;
+            const double dPdRho = 0.5 * (gammaI - 1) * magSqr(UI);
+            V3 dUdRho = V3{-UI.x / rhoI, -UI.y / rhoI, -UI.z / rhoI};
+            const double dTdRho = -1.0 / (cvI * rhoI) * (rhoEI / rhoI - magSqr(UI));
+            V3 dPdRhoU = -(gammaI - 1) * UI;
+            const double dUdRhoU = 1.0 / rhoI;
+            V3 dTdRhoU = V3{-UI.x / (cvI * rhoI), -UI.y / (cvI * rhoI), -UI.z / (cvI * rhoI)};
+            const double dPdRhoE = gammaI - 1;
+            const double dTdRhoE = 1.0 / (cvI * rhoI);
+            // continuity rows
+            double& cr = k.SS[0][0].diag[c];
+            cr += dCdp * dPdRho; cr += dot(dCdU, dUdRho); cr += dCdT * dTdRho;
+            double* cu = &k.SV[0].diag[3 * c];
+            V3 t1 = dCdp * dPdRhoU, t2 = dCdU * dUdRhoU, t3 = dCdT * dTdRhoU;
+            cu[0] += t1.x; cu[1] += t1.y; cu[2] += t1.z;
+            cu[0] += t2.x; cu[1] += t2.y; cu[2] += t2.z;
+            cu[0] += t3.x; cu[1] += t3.y; cu[2] += t3.z;
+            double& ce = k.SS[0][1].diag[c];
+            ce += dCdp * dPdRhoE; ce += 0.0; ce += dCdT * dTdRhoE;
+            // momentum rows
+            double* mr = &k.VS[0].diag[3 * c];
+            V3 a = dMdp * dPdRho, bb = tdotv(dMdU, dUdRho), cc = dMdT * dTdRho;
+            mr[0] += a.x; mr[1] += a.y; mr[2] += a.z;
+            mr[0] += bb.x; mr[1] += bb.y; mr[2] += bb.z;
+            mr[0] += cc.x; mr[1] += cc.y; mr[2] += cc.z;
+            double* mu_ = &k.VV.diag[9 * c];
+            T9 A = outer(dMdp, dPdRhoU), B = dMdU * dUdRhoU, Cc = outer(dMdT, dTdRhoU);
+            for (int q = 0; q < 9; ++q) mu_[q] += A.t[q];
+            for (int q = 0; q < 9; ++q) mu_[q] += B.t[q];
+            for (int q = 0; q < 9; ++q) mu_[q] += Cc.t[q];
+            double* me = &k.VS[1].diag[3 * c];
+            V3 e1 = dMdp * dPdRhoE, e3 = dMdT * dTdRhoE;
+            me[0] += e1.x; me[1] += e1.y; me[2] += e1.z;
+            me[0] += 0.0; me[1] += 0.0; me[2] += 0.0;
+            me[0] += e3.x; me[1] += e3.y; me[2] += e3.z;
+            // energy rows
+            double& er = k.SS[1][0].diag[c];
+            er += dEdp * dPdRho; er += dot(dEdU, dUdRho); er += dEdT * dTdRho;
+            double* eu = &k.SV[1].diag[3 * c];
+            V3 f1 = dEdp * dPdRhoU, f2 = dEdU * dUdRhoU, f3 = dEdT * dTdRhoU;
+            eu[0] += f1.x; eu[1] += f1.y; eu[2] += f1.z;
+            eu[0] += f2.x; eu[1] += f2.y; eu[2] += f2.z;
+            eu[0] += f3.x; eu[1] += f3.y; eu[2] += f3.z;
+            double& ee = k.SS[1][1].diag[c];
+            ee += dEdp * dPdRhoE; ee += 0.0; ee += dEdT * dTdRhoE;
+        }
+    }
+    // addTemporalTerms: ddtCoeff = max(diag(fvm::ddt))/V with dualTime diag = rDT*V
+    for (int c = 0; c < nC; ++c) {
+        const double ddtCoeff = (k.rDT[c] * m.V[c]) / m.V[c];
+        const double dc = ddtCoeff * m.V[c];
+        k.SS[0][0].diag[c] += dc;
+        k.VV.diag[9 * c] += dc; k.VV.diag[9 * c + 4] += dc; k.VV.diag[9 * c + 8] += dc;
+        k.SS[1][1].diag[c] += dc;
+    }
+    // viscous LF Jacobian: laplacian(0.5*lambdaVisc, 1) subtracted
+    if (k.s.viscous) {
+        const double mu = k.s.mu, alpha = k.s.gamma * mu / k.s.Pr;
+        auto stab = [&](LduBlock& target, int nc) {
+            LduBlock mx;
+            mx.init(nC, nI, nc);
+            for (int f = 0; f < nI; ++f) {
+                const double rf = interpLinear(m, f, k.rho.data(), 1);
+                const double muf = m.w[f] * (mu - mu) + mu;
+                const double alf = m.w[f] * (alpha - alpha) + alpha;
+                const double lv = (muf + alf) / rf;
+                const double s = 0.5 * lv * m.magSf[f] * m.dc[f];
+                if (nc == 1) { mx.upper[f] = s; mx.lower[f] = s; }
+                else {
+                    mx.upper[9 * f] = mx.upper[9 * f + 4] = mx.upper[9 * f + 8] = s;
+                    mx.lower[9 * f] = mx.lower[9 * f + 4] = mx.lower[9 * f + 8] = s;
+                }
+            }
+            negSumDiag(m, mx);
+            addBlockTo(target, mx, -1.0);
+        };
+        stab(k.SS[0][0], 1);
+        stab(k.VV, 9);
+        stab(k.SS[1][1], 1);
+    }
+}
+
+// ---------------------------------------------------------- linear alg ---
+// Vectors at this level: s[0] (rho), s[1] (rhoE): nC each; v: 3*nC interleaved.
+struct CVec {
+    vec s0, s1, v;
+    void init(int nC) { s0.assign(nC, 0); s1.assign(nC, 0); v.assign(3 * (size_t)nC, 0); }
+};
+void toC(const double* x, int nC, CVec& c) {
+    c.init(nC);
+    for (int i = 0; i < nC; ++i) {
+        c.s0[i] = x[i];
+        c.s1[i] = x[nC + i];
+        for (int d = 0; d < 3; ++d) c.v[3 * i + d] = x[(2 + d) * (size_t)nC + i];
+    }
+}
+void fromC(const CVec& c, int nC, double* x) {
+    for (int i = 0; i < nC; ++i) {
+        x[i] = c.s0[i];
+        x[nC + i] = c.s1[i];
+        for (int d = 0; d < 3; ++d) x[(2 + d) * (size_t)nC + i] = c.v[3 * i + d];
+    }
+}
+
+// blockFvMatrix::Amul for (sourceType, blockType) pairs.
+void amulSS(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {
+    y.assign(m.nC, 0);
+    for (int c = 0; c < m.nC; ++c) y[c] = A.diag[c] * x[c];
+    for (int f = 0; f < m.nI; ++f) {
+        y[m.nei[f]] += A.lower[f] * x[m.own[f]];
+        y[m.own[f]] += A.upper[f] * x[m.nei[f]];
+    }
+}
+void amulSV(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {  // scalar row, vector col
+    y.assign(m.nC, 0);
+    for (int c = 0; c < m.nC; ++c) y[c] = dot(v3(&A.diag[3 * c]), v3(&x[3 * c]));
+    for (int f = 0; f < m.nI; ++f) {
+        y[m.nei[f]] += dot(v3(&A.lower[3 * f]), v3(&x[3 * m.own[f]]));
+        y[m.own[f]] += dot(v3(&A.upper[3 * f]), v3(&x[3 * m.nei[f]]));
+    }
+}
+void amulVS(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {  // vector row, scalar col
+    y.assign(3 * (size_t)m.nC, 0);
+    for (int c = 0; c < m.nC; ++c)
+        for (int d = 0; d < 3; ++d) y[3 * c + d] = A.diag[3 * c + d] * x[c];
+    for (int f = 0; f < m.nI; ++f)
+        for (int d = 0; d < 3; ++d) {
+            y[3 * m.nei[f] + d] += A.lower[3 * f + d] * x[m.own[f]];
+            y[3 * m.own[f] + d] += A.upper[3 * f + d] * x[m.nei[f]];
+        }
+}
+void amulVV(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {
+    y.assign(3 * (size_t)m.nC, 0);
+    auto T = [&](const vec& a, int i) { T9 t; for (int q = 0; q < 9; ++q) t.t[q] = a[9 * i + q]; return t; };
+    for (int c = 0; c < m.nC; ++c) {
+        V3 r = tdotv(T(A.diag, c), v3(&x[3 * c]));
+        y[3 * c] = r.x; y[3 * c + 1] = r.y; y[3 * c + 2] = r.z;
+    }
+    for (int f = 0; f < m.nI; ++f) {
+        V3 a = tdotv(T(A.lower, f), v3(&x[3 * m.own[f]]));
+        V3 b = tdotv(T(A.upper, f), v3(&x[3 * m.nei[f]]));
+        for (int d = 0; d < 3; ++d) y[3 * m.nei[f] + d] += (&a.x)[d];
+        for (int d = 0; d < 3; ++d) y[3 * m.own[f] + d] += (&b.x)[d];
+    }
+}
+
+// coupledMatrix::matrixMul (coupledMatrix.C:66-123)
+void matrixMul(const Case& k, const CVec& x, CVec& y) {
+    const Mesh& m = *k.m;
+    y.init(m.nC);
+    vec t;
+    vec* S[2] = {&y.s0, &y.s1};
+    const vec* X[2] = {&x.s0, &x.s1};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            amulSS(m, k.SS[i][j], *X[j], t);
+            for (int c = 0; c < m.nC; ++c) (*S[i])[c] += t[c];
+        }
+    for (int i = 0; i < 2; ++i) {
+        amulSV(m, k.SV[i], x.v, t);
+        for (int c = 0; c < m.nC; ++c) (*S[i])[c] += t[c];
+    }
+    for (int j = 0; j < 2; ++j) {
+        amulVS(m, k.VS[j], *X[j], t);
+        for (size_t c = 0; c < t.size(); ++c) y.v[c] += t[c];
+    }
+    amulVV(m, k.VV, x.v, t);
+    for (size_t c = 0; c < t.size(); ++c) y.v[c] += t[c];
+}
+
+// lusgs::lusgs (lusgs.C:50-125): scalar reciprocal of the max diagonal.
+int lusgsSetup(Case& k) {
+    const Mesh& m = *k.m;
+    k.rD.assign(m.nC, GREAT);
+    for (int c = 0; c < m.nC; ++c) {
+        double r = k.rD[c];
+        r = 1.0 / std::max(1.0 / r, std::fabs(k.SS[0][0].diag[c]));
+        r = 1.0 / std::max(1.0 / r, std::fabs(k.SS[1][1].diag[c]));
+        r = 1.0 / std::max(1.0 / r, std::fabs(k.VV.diag[9 * c]));
+        r = 1.0 / std::max(1.0 / r, std::fabs(k.VV.diag[9 * c + 4]));
+        r = 1.0 / std::max(1.0 / r, std::fabs(k.VV.diag[9 * c + 8]));
+        k.rD[c] = r;
+        if (r < VSMALL) return -3;
+    }
+    return 0;
+}
+
+// lusgs::precondition (lusgs.C:220-382)
+void lusgs(const Case& k, CVec& r) {
+    const Mesh& m = *k.m;
+    const int nC = m.nC;
+    // forward
+    for (int c = 0; c < nC; ++c) {
+        const double rd = k.rD[c];
+        const double dS0 = rd * r.s0[c], dS1 = rd * r.s1[c];
+        const V3 dV = rd * v3(&r.v[3 * c]);
+        const double dS[2] = {dS0, dS1};
+        vec* R[2] = {&r.s0, &r.s1};
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int f : m.cells[c])
+                    if (f < m.nI && m.nei[f] > c) (*R[i])[m.nei[f]] -= k.SS[i][j].lower[f] * dS[j];
+        for (int i = 0; i < 2; ++i)
+            for (int f : m.cells[c])
+                if (f < m.nI && m.nei[f] > c) (*R[i])[m.nei[f]] -= dot(v3(&k.SV[i].lower[3 * f]), dV);
+        for (int j = 0; j < 2; ++j)
+            for (int f : m.cells[c])
+                if (f < m.nI && m.nei[f] > c)
+                    for (int d = 0; d < 3; ++d) r.v[3 * m.nei[f] + d] -= k.VS[j].lower[3 * f + d] * dS[j];
+        for (int f : m.cells[c])
+            if (f < m.nI && m.nei[f] > c) {
+                T9 t; for (int q = 0; q < 9; ++q) t.t[q] = k.VV.lower[9 * f + q];
+                V3 a = tdotv(t, dV);
+                for (int d = 0; d < 3; ++d) r.v[3 * m.nei[f] + d] -= (&a.x)[d];
+            }
+    }
+    // backward
+    for (int c = nC - 1; c >= 0; --c) {
+        const double rd = k.rD[c];
+        const double dS0 = rd * r.s0[c], dS1 = rd * r.s1[c];
+        const V3 dV = rd * v3(&r.v[3 * c]);
+        r.s0[c] = dS0; r.s1[c] = dS1;
+        r.v[3 * c] = dV.x; r.v[3 * c + 1] = dV.y; r.v[3 * c + 2] = dV.z;
+        const double dS[2] = {dS0, dS1};
+        vec* R[2] = {&r.s0, &r.s1};
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int f : m.cells[c])
+                    if (f < m.nI && m.own[f] < c) (*R[i])[m.own[f]] -= k.SS[i][j].upper[f] * dS[j];
+        for (int i = 0; i < 2; ++i)
+            for (int f : m.cells[c])
+                if (f < m.nI && m.own[f] < c) (*R[i])[m.own[f]] -= dot(v3(&k.SV[i].upper[3 * f]), dV);
+        for (int j = 0; j < 2; ++j)
+            for (int f : m.cells[c])
+                if (f < m.nI && m.own[f] < c)
+                    for (int d = 0; d < 3; ++d) r.v[3 * m.own[f] + d] -= k.VS[j].upper[3 * f + d] * dS[j];
+        for (int f : m.cells[c])
+            if (f < m.nI && m.own[f] < c) {
+                T9 t; for (int q = 0; q < 9; ++q) t.t[q] = k.VV.upper[9 * f + q];
+                V3 a = tdotv(t, dV);
+                for (int d = 0; d < 3; ++d) r.v[3 * m.own[f] + d] -= (&a.x)[d];
+            }
+    }
+}
+
+// Block-Jacobi (Jacobi.C:55-132, JacobiSmoother.C:42-203): z = D^-1 r with
+// the 5x5 diagonal block inverted by LU with partial pivoting.
+void diagBlock(const Case& k, int c, double J[5][5]) {
+    J[0][0] = k.SS[0][0].diag[c]; J[0][1] = k.SS[0][1].diag[c];
+    J[1][0] = k.SS[1][0].diag[c]; J[1][1] = k.SS[1][1].diag[c];
+    for (int d = 0; d < 3; ++d) {
+        J[0][2 + d] = k.SV[0].diag[3 * c + d];
+        J[1][2 + d] = k.SV[1].diag[3 * c + d];
+        J[2 + d][0] = k.VS[0].diag[3 * c + d];
+        J[2 + d][1] = k.VS[1].diag[3 * c + d];
+        for (int e = 0; e < 3; ++e) J[2 + d][2 + e] = k.VV.diag[9 * c + 3 * d + e];
+    }
+}
+void invert5(double A[5][5], double inv[5][5]) {
+    int piv[5];
+    double a[5][5];
+    std::memcpy(a, A, sizeof(a));
+    for (int i = 0; i < 5; ++i) piv[i] = i;
+    for (int col = 0; col < 5; ++col) {
+        int p = col;
+        for (int r = col + 1; r < 5; ++r)
+            if (std::fabs(a[r][col]) > std::fabs(a[p][col])) p = r;
+        if (p != col) {
+            for (int q = 0; q < 5; ++q) std::swap(a[p][q], a[col][q]);
+            std::swap(piv[p], piv[col]);
+        }
+        for (int r = col + 1; r < 5; ++r) {
+            a[r][col] /= a[col][col];
+            for (int q = col + 1; q < 5; ++q) a[r][q] -= a[r][col] * a[col][q];
+        }
+    }
+    for (int e = 0; e < 5; ++e) {
+        double y[5];
+        for (int i = 0; i < 5; ++i) {
+            double s = piv[i] == e ? 1.0 : 0.0;
+            for (int q = 0; q < i; ++q) s -= a[i][q] * y[q];
+            y[i] = s;
+        }
+        for (int i = 4; i >= 0; --i) {
+            double s = y[i];
+            for (int q = i + 1; q < 5; ++q) s -= a[i][q] * y[q];
+            y[i] = s / a[i][i];
+        }
+        for (int i = 0; i < 5; ++i) inv[i][e] = y[i];
+    }
+}
+void jacobi(const Case& k, CVec& r) {
+    const Mesh& m = *k.m;
+    for (int c = 0; c < m.nC; ++c) {
+        double J[5][5], Ji[5][5];
+        diagBlock(k, c, J);
+        invert5(J, Ji);
+        double v[5] = {r.s0[c], r.s1[c], r.v[3 * c], r.v[3 * c + 1], r.v[3 * c + 2]}, o[5] = {0, 0, 0, 0, 0};
+        for (int i = 0; i < 5; ++i)
+            for (int j = 0; j < 5; ++j) o[i] += Ji[i][j] * v[j];
+        r.s0[c] = o[0]; r.s1[c] = o[1];
+        for (int d = 0; d < 3; ++d) r.v[3 * c + d] = o[2 + d];
+    }
+}
+
+void precondition(const Case& k, CVec& r) {
+    if (k.s.precond == 0) lusgs(k, r);
+    else jacobi(k, r);
+}
+
+// gmres::givensRotation (gmres.C:44-69)
+void givens(double h, double beta, double& c, double& s) {
+    if (beta == 0) { c = 1; s = 0; }
+    else if (std::fabs(beta) > std::fabs(h)) {
+        double tau = -h / beta;
+        s = 1.0 / std::sqrt(1.0 + sqr(tau));
+        c = s * tau;
+    } else {
+        double tau = -beta / h;
+        c = 1.0 / std::sqrt(1.0 + sqr(tau));
+        s = c * tau;
+    }
+}
+
+struct Res {
+    double sInit[2], sFinal[2], vInit[3], vFinal[3];
+    int nIter;
+};
+
+double vsumMag(const vec& a) { double s = 0; for (double x : a) s += std::fabs(x); return s; }
+
+// gmres::solveDelta (6-arg) (gmres.C:772-1108) with coupledMatrixSolver::stop
+Res gmresSolve(Case& k, const CVec& W, const CVec& b, CVec& dW) {
+    const Mesh& m = *k.m;
+    const int nC = m.nC, N = k.s.nDirs;
+    Res res{};
+    dW = W;
+    double avg0 = 0, avg1 = 0; V3 avgv{0, 0, 0};
+    for (int c = 0; c < nC; ++c) { avg0 += W.s0[c]; avg1 += W.s1[c]; avgv = avgv + v3(&W.v[3 * c]); }
+    avg0 /= nC; avg1 /= nC; avgv = avgv / (double)nC;
+    for (int c = 0; c < nC; ++c) {
+        dW.s0[c] -= avg0; dW.s1[c] -= avg1;
+        dW.v[3 * c] -= avgv.x; dW.v[3 * c + 1] -= avgv.y; dW.v[3 * c + 2] -= avgv.z;
+    }
+    CVec t;
+    matrixMul(k, dW, t);
+    dW.init(nC);
+    double sNF[2], vNF;
+    {
+        double a = 0, bb = 0;
+        for (int c = 0; c < nC; ++c) a += std::fabs(t.s0[c]) + std::fabs(b.s0[c]);
+        for (int c = 0; c < nC; ++c) bb += std::fabs(t.s1[c]) + std::fabs(b.s1[c]);
+        sNF[0] = a + VSMALL; sNF[1] = bb + VSMALL;
+        res.sInit[0] = vsumMag(b.s0) / sNF[0];
+        res.sInit[1] = vsumMag(b.s1) / sNF[1];
+        double v = 0;
+        for (int c = 0; c < nC; ++c) v += mag(v3(&t.v[3 * c])) + mag(v3(&b.v[3 * c]));
+        vNF = v + VSMALL;
+        double cm[3] = {0, 0, 0};
+        for (int c = 0; c < nC; ++c) for (int d = 0; d < 3; ++d) cm[d] += std::fabs(b.v[3 * c + d]);
+        for (int d = 0; d < 3; ++d) res.vInit[d] = cm[d] / vNF;
+        res.sFinal[0] = res.sInit[0]; res.sFinal[1] = res.sInit[1];
+        for (int d = 0; d < 3; ++d) res.vFinal[d] = res.vInit[d];
+    }
+    CVec r = b;
+    std::vector<std::vector<double>> H(N, std::vector<double>(N, 0.0));
+    vec yh(N, 0), bh(N + 1, 0), cs(N, 0), sn(N, 0);
+    std::vector<CVec> Vk(N);
+    auto norm2 = [&](const CVec& x) {
+        double s = 0, a = 0;
+        for (double q : x.s0) a += q * q;
+        s += a; a = 0;
+        for (double q : x.s1) a += q * q;
+        s += a; a = 0;
+        for (int c = 0; c < nC; ++c) a += magSqr(v3(&x.v[3 * c]));
+        s += a;
+        return std::sqrt(s);
+    };
+    res.nIter = 0;
+    while (true) {
+        precondition(k, r);
+        double beta = norm2(r);
+        std::fill(bh.begin(), bh.end(), 0.0);
+        bh[0] = beta;
+        for (int i = 0; i < N; ++i) {
+            CVec& v = Vk[i];
+            v.init(nC);
+            for (int c = 0; c < nC; ++c) { v.s0[c] = r.s0[c] / beta; v.s1[c] = r.s1[c] / beta; }
+            for (size_t q = 0; q < v.v.size(); ++q) v.v[q] = r.v[q] / beta;
+            matrixMul(k, v, r);
+            precondition(k, r);
+            for (int j = 0; j <= i; ++j) {
+                double bb = 0, a = 0;
+                for (int c = 0; c < nC; ++c) a += r.s0[c] * Vk[j].s0[c];
+                bb += a; a = 0;
+                for (int c = 0; c < nC; ++c) a += r.s1[c] * Vk[j].s1[c];
+                bb += a; a = 0;
+                for (int c = 0; c < nC; ++c) a += dot(v3(&r.v[3 * c]), v3(&Vk[j].v[3 * c]));
+                bb += a;
+                H[j][i] = bb;
+                for (int c = 0; c < nC; ++c) { r.s0[c] -= H[j][i] * Vk[j].s0[c]; r.s1[c] -= H[j][i] * Vk[j].s1[c]; }
+                for (size_t q = 0; q < r.v.size(); ++q) r.v[q] -= H[j][i] * Vk[j].v[q];
+            }
+            beta = norm2(r);
+            for (int j = 0; j < i; ++j) {
+                const double Hji = H[j][i];
+                H[j][i] = cs[j] * Hji - sn[j] * H[j + 1][i];
+                H[j + 1][i] = sn[j] * Hji + cs[j] * H[j + 1][i];
+            }
+            givens(H[i][i], beta, cs[i], sn[i]);
+            const double bhi = bh[i];
+            bh[i] = cs[i] * bhi - sn[i] * bh[i + 1];
+            bh[i + 1] = sn[i] * bhi + cs[i] * bh[i + 1];
+            H[i][i] = cs[i] * H[i][i] - sn[i] * beta;
+        }
+        for (int i = N - 1; i >= 0; --i) {
+            double s = bh[i];
+            for (int j = i + 1; j < N; ++j) s -= H[i][j] * yh[j];
+            double hd = H[i][i];
+            // stabilise(H, VSMALL)
+            if (hd >= 0) hd = hd + VSMALL; else hd = hd - VSMALL;
+            yh[i] = s / hd;
+        }
+        for (int i = 0; i < N; ++i) {
+            for (int c = 0; c < nC; ++c) { dW.s0[c] += yh[i] * Vk[i].s0[c]; dW.s1[c] += yh[i] * Vk[i].s1[c]; }
+            for (size_t q = 0; q < dW.v.size(); ++q) dW.v[q] += yh[i] * Vk[i].v[q];
+        }
+        matrixMul(k, dW, r);
+        for (int c = 0; c < nC; ++c) { r.s0[c] = b.s0[c] - r.s0[c]; r.s1[c] = b.s1[c] - r.s1[c]; }
+        for (size_t q = 0; q < r.v.size(); ++q) r.v[q] = b.v[q] - r.v[q];
+        res.sFinal[0] = vsumMag(r.s0) / sNF[0];
+        res.sFinal[1] = vsumMag(r.s1) / sNF[1];
+        double cm[3] = {0, 0, 0};
+        for (int c = 0; c < nC; ++c) for (int d = 0; d < 3; ++d) cm[d] += std::fabs(r.v[3 * c + d]);
+        for (int d = 0; d < 3; ++d) res.vFinal[d] = (d == m.emptyDir) ? 0.0 : cm[d] / vNF;
+        res.nIter++;
+        // stop
+        if (res.nIter < k.s.minIter) continue;
+        double mx = std::max(res.sFinal[0], res.sFinal[1]);
+        for (int d = 0; d < 3; ++d) mx = std::max(mx, res.vFinal[d]);
+        const double RVS = 1.0e-150;
+        double rel = -1e300;
+        rel = std::max(rel, res.sFinal[0] / (res.sInit[0] + RVS));
+        rel = std::max(rel, res.sFinal[1] / (res.sInit[1] + RVS));
+        for (int d = 0; d < 3; ++d)
+            if (d != m.emptyDir) rel = std::max(rel, res.vFinal[d] / (res.vInit[d] + RVS));
+        if (res.nIter >= k.s.maxIter || mx < k.s.tol || rel < k.s.relTol) break;
+    }
+    if (m.emptyDir >= 0)
+        for (int c = 0; c < nC; ++c) dW.v[3 * c + m.emptyDir] = 0.0;
+    return res;
+}
+
+// conserved variables from primitives
+void makeConserved(Case& k) {
+    const int nC = k.m->nC;
+    k.rhoU.resize(3 * nC);
+    k.rhoE.resize(nC);
+    for (int c = 0; c < nC; ++c) {
+        for (int d = 0; d < 3; ++d) k.rhoU[3 * c + d] = k.rho[c] * k.U[3 * c + d];
+        k.rhoE[c] = k.rho[c] * (k.e[c] + 0.5 * magSqr(v3(&k.U[3 * c])));
+    }
+}
+
+// updateFields.H:7-104 (rho bounding with the default rhoMin is inactive
+// for positive densities; the fvc::average branch of bound() is not
+// restated).
+void updateFields(Case& k, const CVec& dW) {
+    const int nC = k.m->nC;
+    for (int c = 0; c < nC; ++c) {
+        k.rho[c] += dW.s0[c];
+        for (int d = 0; d < 3; ++d) k.rhoU[3 * c + d] += dW.v[3 * c + d];
+        k.rhoE[c] += dW.s1[c];
+    }
+    for (int c = 0; c < nC; ++c) {
+        k.rho[c] = std::max(k.rho[c], k.s.rhoMin);
+        for (int d = 0; d < 3; ++d) k.U[3 * c + d] = k.rhoU[3 * c + d] / k.rho[c];
+        k.e[c] = k.rhoE[c] / k.rho[c] - 0.5 * magSqr(v3(&k.U[3 * c]));
+        const double eMin = k.Cv * k.s.TMin;
+        k.e[c] = std::max(k.e[c], eMin);
+        if (k.s.TMax < GREAT) k.e[c] = std::min(k.e[c], k.Cv * k.s.TMax);
+        k.T[c] = k.e[c] / k.Cv;
+        const double psi = 1.0 / (k.s.R * k.T[c]);
+        k.p[c] = k.rho[c] / psi;
+        for (int d = 0; d < 3; ++d) k.rhoU[3 * c + d] = k.rho[c] * k.U[3 * c + d];
+        k.rhoE[c] = k.rho[c] * (k.e[c] + 0.5 * magSqr(v3(&k.U[3 * c])));
+    }
+    evalBoundary(k);
+}
+
+void setFields(Case& k, const double* rho, const double* U, const double* T) {
+    const int nC = k.m->nC;
+    k.rho.assign(rho, rho + nC);
+    k.U.assign(U, U + 3 * nC);
+    k.T.assign(T, T + nC);
+    k.e.resize(nC);
+    k.p.resize(nC);
+    for (int c = 0; c < nC; ++c) {
+        k.e[c] = k.Cv * k.T[c];
+        const double psi = 1.0 / (k.s.R * k.T[c]);
+        k.p[c] = k.rho[c] / psi;
+    }
+    makeConserved(k);
+    evalBoundary(k);
+}
+
+// One dbnsFoam steady pseudo-step: outerLoop.H + updateFields.H, with the
+// SER update of setCoAndDeltaT.H:3-37 and boundLocalTimeStep.H.
+int pseudoStep(Case& k, Res& out) {
+    Mesh& m = *k.m;
+    const int nC = m.nC;
+    k.corr++;
+    // SER
+    if (k.haveInit) {
+        if (k.corr != 1 && k.havePrev) {
+            const double nI = std::sqrt(sqr(k.initS[0]) + sqr(k.initS[1]) + (sqr(k.initV[0]) + sqr(k.initV[1]) + sqr(k.initV[2])));
+            const double nP = std::sqrt(sqr(k.prevS[0]) + sqr(k.prevS[1]) + (sqr(k.prevV[0]) + sqr(k.prevV[1]) + sqr(k.prevV[2])));
+            double ratio = nP / nI;
+            ratio = std::max(std::min(ratio, k.s.coIncr), k.s.coDecr);
+            for (int c = 0; c < nC; ++c) {
+                k.pseudoCo[c] *= ratio;
+                k.pseudoCo[c] = std::max(std::min(k.pseudoCo[c], k.s.coMax), k.s.coMin);
+            }
+        }
+        std::copy(k.initS, k.initS + 2, k.prevS);
+        std::copy(k.initV, k.initV + 3, k.prevV);
+        k.havePrev = true;
+    }
+    vec b;
+    residual(k, b);
+    localDeltaT(k);
+    assemble(k);
+    int rc = k.s.precond == 0 ? lusgsSetup(k) : 0;
+    if (rc) return rc;
+    CVec W, B, dW;
+    {
+        vec w(5 * (size_t)nC);
+        for (int c = 0; c < nC; ++c) {
+            w[c] = k.rho[c];
+            w[nC + c] = k.rhoE[c];
+            for (int d = 0; d < 3; ++d) w[(2 + d) * (size_t)nC + c] = k.rhoU[3 * c + d];
+        }
+        toC(w.data(), nC, W);
+        toC(b.data(), nC, B);
+    }
+    Res r = gmresSolve(k, W, B, dW);
+    out = r;
+    k.haveInit = true;
+    std::copy(r.sInit, r.sInit + 2, k.initS);
+    std::copy(r.vInit, r.vInit + 3, k.initV);
+    // boundLocalTimeStep (evaluated before the update, as the reference does)
+    if (k.s.localTS && k.s.bounding) {
+        vec fac(nC, 1.0);
+        auto bad = [&](int c) {
+            const double rmin = k.s.lowerBound * W.s0[c];
+            const double Uo[3] = {W.v[3 * c] / W.s0[c], W.v[3 * c + 1] / W.s0[c], W.v[3 * c + 2] / W.s0[c]};
+            const double emin = k.s.lowerBound * (W.s1[c] / W.s0[c] - 0.5 * (Uo[0] * Uo[0] + Uo[1] * Uo[1] + Uo[2] * Uo[2]));
+            const double Un[3] = {k.rhoU[3 * c] / k.rho[c], k.rhoU[3 * c + 1] / k.rho[c], k.rhoU[3 * c + 2] / k.rho[c]};
+            const double et = k.rhoE[c] / k.rho[c] - 0.5 * (Un[0] * Un[0] + Un[1] * Un[1] + Un[2] * Un[2]);
+            return k.rho[c] < rmin || et < emin || et < SMALL;
+        };
+        for (int f = 0; f < m.nI; ++f) {
+            const int o = m.own[f], n = m.nei[f];
+            if (bad(o)) { fac[o] = std::min(0.5, fac[o]); fac[n] = std::min(0.75, fac[n]); }
+            if (bad(n)) { fac[n] = std::min(0.5, fac[n]); fac[o] = std::min(0.75, fac[o]); }
+        }
+        for (int c = 0; c < nC; ++c) k.pseudoCo[c] *= fac[c];
+    }
+    updateFields(k, dW);
+    return 0;
+}
+
+}  // namespace
+
+// ======================================================== C interface ====
+extern "C" {
+
+void* orc_mesh_new(int nPoints, const double* pts, int nFaces, const int* fOff, const int* fV, int nInternal,
+                   const int* own, const int* nei, int nPatches, const int* pStart, const int* pSize,
+                   const int* pKind) {
+    Mesh* m = new Mesh;
+    m->nP = nPoints; m->nF = nFaces; m->nI = nInternal;
+    m->pts.assign(pts, pts + 3 * (size_t)nPoints);
+    m->fOff.assign(fOff, fOff + nFaces + 1);
+    m->fV.assign(fV, fV + fOff[nFaces]);
+    m->own.assign(own, own + nFaces);
+    m->nei.assign(nei, nei + nInternal);
+    m->pStart.assign(pStart, pStart + nPatches);
+    m->pSize.assign(pSize, pSize + nPatches);
+    m->pKind.assign(pKind, pKind + nPatches);
+    int nc = 0;
+    for (int f = 0; f < nFaces; ++f) nc = std::max(nc, own[f] + 1);
+    for (int f = 0; f < nInternal; ++f) nc = std::max(nc, nei[f] + 1);
+    m->nC = nc;
+    makeGeometry(*m);
+    return m;
+}
+void orc_mesh_free(void* h) { delete (Mesh*)h; }
+int orc_mesh_ncells(void* h) { return ((Mesh*)h)->nC; }
+
+void orc_mesh_geometry(void* h, double* C, double* V, double* Sf, double* Cf, double* w, double* dc, double* nodc,
+                       double* corr) {
+    Mesh& m = *(Mesh*)h;
+    std::copy(m.C.begin(), m.C.end(), C);
+    std::copy(m.V.begin(), m.V.end(), V);
+    std::copy(m.Sf.begin(), m.Sf.end(), Sf);
+    std::copy(m.Cf.begin(), m.Cf.end(), Cf);
+    std::copy(m.w.begin(), m.w.end(), w);
+    std::copy(m.dc.begin(), m.dc.end(), dc);
+    std::copy(m.nodc.begin(), m.nodc.end(), nodc);
+    std::copy(m.corr.begin(), m.corr.end(), corr);
+}
+
+// meshTools::bandCompression (Cuthill-McKee from the lowest unvisited cell,
+// unvisited neighbours appended by increasing connectivity, stable); the
+// neighbour lists follow primitiveMesh::calcCellCells (internal-face order).
+static std::vector<int> cmOrder(int nC, int nI, const int* own, const int* nei) {
+    std::vector<std::vector<int>> cc(nC);
+    for (int f = 0; f < nI; ++f) { cc[own[f]].push_back(nei[f]); cc[nei[f]].push_back(own[f]); }
+    std::vector<char> vis(nC, 0);
+    std::vector<int> order;
+    std::deque<int> q;
+    for (int s = 0; s < nC; ++s) {
+        if (vis[s]) continue;
+        q.push_back(s);
+        while (!q.empty()) {
+            int c = q.front();
+            q.pop_front();
+            if (vis[c]) continue;
+            vis[c] = 1;
+            order.push_back(c);
+            std::vector<int> cand;
+            for (int x : cc[c]) if (!vis[x]) cand.push_back(x);
+            std::vector<int> idx(cand.size());
+            for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cc[cand[a]].size() < cc[cand[b]].size(); });
+            for (int i : idx) q.push_back(cand[i]);
+        }
+    }
+    return order;
+}
+
+void orc_rcm(int nC, int nI, const int* own, const int* nei, int* newToOld) {
+    std::vector<int> order = cmOrder(nC, nI, own, nei);
+    for (int i = 0; i < nC; ++i) newToOld[i] = order[nC - 1 - i];
+}
+
+// Greedy colouring, cells visited in Cuthill-McKee order.
+int orc_colour(int nC, int nI, const int* own, const int* nei, int* colour) {
+    std::vector<std::vector<int>> cc(nC);
+    for (int f = 0; f < nI; ++f) { cc[own[f]].push_back(nei[f]); cc[nei[f]].push_back(own[f]); }
+    std::vector<int> order = cmOrder(nC, nI, own, nei);
+    for (int c = 0; c < nC; ++c) colour[c] = -1;
+    int nCol = 0;
+    for (int c : order) {
+        std::vector<char> used(nCol + 1, 0);
+        for (int x : cc[c]) if (colour[x] >= 0) used[colour[x]] = 1;
+        int k = 0;
+        while (used[k]) ++k;
+        colour[c] = k;
+        nCol = std::max(nCol, k + 1);
+    }
+    return nCol;
+}
+
+// Cell renumbering with upper-triangular face re-sorting (renumberMesh).
+void orc_renumber(int nC, int nF, int nI, const int* fOff, const int* fV, const int* own, const int* nei,
+                  const int* newToOld, int* fVout, int* ownOut, int* neiOut, int* faceMap) {
+    std::vector<int> o2n(nC);
+    for (int i = 0; i < nC; ++i) o2n[newToOld[i]] = i;
+    struct F { int a, b, f; bool flip; };
+    std::vector<F> fs(nI);
+    for (int f = 0; f < nI; ++f) {
+        int a = o2n[own[f]], b = o2n[nei[f]];
+        bool fl = a > b;
+        if (fl) std::swap(a, b);
+        fs[f] = F{a, b, f, fl};
+    }
+    std::stable_sort(fs.begin(), fs.end(), [](const F& x, const F& y) { return x.a < y.a || (x.a == y.a && x.b < y.b); });
+    size_t pos = 0;
+    for (int k = 0; k < nI; ++k) {
+        const F& q = fs[k];
+        ownOut[k] = q.a; neiOut[k] = q.b;
+        faceMap[k] = q.flip ? -(q.f + 1) : q.f;
+        int s = fOff[q.f], e = fOff[q.f + 1];
+        if (!q.flip) for (int i = s; i < e; ++i) fVout[pos++] = fV[i];
+        else { fVout[pos++] = fV[s]; for (int i = e - 1; i > s; --i) fVout[pos++] = fV[i]; }
+    }
+    for (int f = nI; f < nF; ++f) {
+        ownOut[f] = o2n[own[f]];
+        faceMap[f] = f;
+        for (int i = fOff[f]; i < fOff[f + 1]; ++i) fVout[pos++] = fV[i];
+    }
+}
+
+// iset: flux, lowMach, recon, limiter, viscous, precond, nDirs, maxIter, minIter, localTS, bounding
+// rset: R, gamma, mu, Pr, entropyFix, tol, relTol, co, coMin, coMax, coIncr, coDecr, lowerBound, rhoMin, TMin, TMax
+void* orc_case_new(void* mesh, const int* is, const double* rs) {
+    Case* k = new Case;
+    k->m = (Mesh*)mesh;
+    Settings& s = k->s;
+    s.flux = is[0]; s.lowMach = is[1]; s.recon = is[2]; s.limiter = is[3]; s.viscous = is[4];
+    s.precond = is[5]; s.nDirs = is[6]; s.maxIter = is[7]; s.minIter = is[8]; s.localTS = is[9]; s.bounding = is[10];
+    s.R = rs[0]; s.gamma = rs[1]; s.mu = rs[2]; s.Pr = rs[3]; s.entropyFix = rs[4]; s.tol = rs[5]; s.relTol = rs[6];
+    s.co = rs[7]; s.coMin = rs[8]; s.coMax = rs[9]; s.coIncr = rs[10]; s.coDecr = rs[11]; s.lowerBound = rs[12];
+    s.rhoMin = rs[13]; s.TMin = rs[14]; s.TMax = rs[15];
+    k->Cv = s.R / (s.gamma - 1.0);
+    k->bcs.assign(k->m->pStart.size(), BC());
+    k->pseudoCo.assign(k->m->nC, s.co);
+    return k;
+}
+void orc_case_free(void* h) { delete (Case*)h; }
+
+void orc_set_bc(void* h, int patch, int pT, int uT, int tT, const double* pv, const double* uv, const double* tv) {
+    Case& k = *(Case*)h;
+    BC& b = k.bcs[patch];
+    const int n = k.m->pSize[patch];
+    b.pT = pT; b.uT = uT; b.tT = tT;
+    if (pT == BC_FIXED) b.pv.assign(pv, pv + n);
+    if (uT == BC_FIXED) b.uv.assign(uv, uv + 3 * n);
+    if (tT == BC_FIXED) b.tv.assign(tv, tv + n);
+}
+
+void orc_set_fields(void* h, const double* rho, const double* U, const double* T) {
+    setFields(*(Case*)h, rho, U, T);
+}
+
+void orc_get_fields(void* h, double* rho, double* U, double* T, double* p, double* W) {
+    Case& k = *(Case*)h;
+    const int nC = k.m->nC;
+    std::copy(k.rho.begin(), k.rho.end(), rho);
+    std::copy(k.U.begin(), k.U.end(), U);
+    std::copy(k.T.begin(), k.T.end(), T);
+    std::copy(k.p.begin(), k.p.end(), p);
+    for (int c = 0; c < nC; ++c) {
+        W[c] = k.rho[c];
+        W[nC + c] = k.rhoE[c];
+        for (int d = 0; d < 3; ++d) W[(2 + d) * (size_t)nC + c] = k.rhoU[3 * c + d];
+    }
+}
+
+void orc_calc_flux(void* h, double* phi, double* phiU, double* phiE) {
+    Case& k = *(Case*)h;
+    vec a, b, c;
+    calcFlux(k, a, b, c);
+    std::copy(a.begin(), a.end(), phi);
+    std::copy(b.begin(), b.end(), phiU);
+    std::copy(c.begin(), c.end(), phiE);
+}
+
+void orc_residual(void* h, double* b) {
+    vec r;
+    residual(*(Case*)h, r);
+    std::copy(r.begin(), r.end(), b);
+}
+
+void orc_local_dt(void* h, double* rDT) {
+    Case& k = *(Case*)h;
+    localDeltaT(k);
+    std::copy(k.rDT.begin(), k.rDT.end(), rDT);
+}
+
+// Assemble the coupled Jacobian (needs orc_local_dt first) and return it
+// as 5x5 blocks in [rho, rhoE, rhoU] order: diag[nC*25], upper[nI*25],
+// lower[nI*25].
+void orc_assemble(void* h, double* diag, double* upper, double* lower) {
+    Case& k = *(Case*)h;
+    assemble(k);
+    const Mesh& m = *k.m;
+    auto fill = [&](double* out, int idx, const vec& ss00, const vec& ss01, const vec& ss10, const vec& ss11,
+                    const vec& sv0, const vec& sv1, const vec& vs0, const vec& vs1, const vec& vv) {
+        double* B = out + 25 * (size_t)idx;
+        B[0] = ss00[idx]; B[1] = ss01[idx]; B[5] = ss10[idx]; B[6] = ss11[idx];
+        for (int d = 0; d < 3; ++d) {
+            B[2 + d] = sv0[3 * idx + d];
+            B[5 + 2 + d] = sv1[3 * idx + d];
+            B[5 * (2 + d) + 0] = vs0[3 * idx + d];
+            B[5 * (2 + d) + 1] = vs1[3 * idx + d];
+            for (int e = 0; e < 3; ++e) B[5 * (2 + d) + 2 + e] = vv[9 * idx + 3 * d + e];
+        }
+    };
+    if (diag)
+        for (int c = 0; c < m.nC; ++c)
+            fill(diag, c, k.SS[0][0].diag, k.SS[0][1].diag, k.SS[1][0].diag, k.SS[1][1].diag, k.SV[0].diag,
+                 k.SV[1].diag, k.VS[0].diag, k.VS[1].diag, k.VV.diag);
+    if (upper)
+        for (int f = 0; f < m.nI; ++f)
+            fill(upper, f, k.SS[0][0].upper, k.SS[0][1].upper, k.SS[1][0].upper, k.SS[1][1].upper, k.SV[0].upper,
+                 k.SV[1].upper, k.VS[0].upper, k.VS[1].upper, k.VV.upper);
+    if (lower)
+        for (int f = 0; f < m.nI; ++f)
+            fill(lower, f, k.SS[0][0].lower, k.SS[0][1].lower, k.SS[1][0].lower, k.SS[1][1].lower, k.SV[0].lower,
+                 k.SV[1].lower, k.VS[0].lower, k.VS[1].lower, k.VV.lower);
+}
+
+// Replace the coupled matrix by given 5x5 blocks (same layout as above).
+void orc_set_matrix(void* h, const double* diag, const double* upper, const double* lower) {
+    Case& k = *(Case*)h;
+    const Mesh& m = *k.m;
+    for (int i = 0; i < 2; ++i) for (int j = 0; j < 2; ++j) k.SS[i][j].init(m.nC, m.nI, 1);
+    for (int i = 0; i < 2; ++i) { k.SV[i].init(m.nC, m.nI, 3); k.VS[i].init(m.nC, m.nI, 3); }
+    k.VV.init(m.nC, m.nI, 9);
+    auto put = [&](const double* in, int idx, vec& ss00, vec& ss01, vec& ss10, vec& ss11, vec& sv0, vec& sv1,
+                   vec& vs0, vec& vs1, vec& vv) {
+        const double* B = in + 25 * (size_t)idx;
+        ss00[idx] = B[0]; ss01[idx] = B[1]; ss10[idx] = B[5]; ss11[idx] = B[6];
+        for (int d = 0; d < 3; ++d) {
+            sv0[3 * idx + d] = B[2 + d];
+            sv1[3 * idx + d] = B[7 + d];
+            vs0[3 * idx + d] = B[5 * (2 + d)];
+            vs1[3 * idx + d] = B[5 * (2 + d) + 1];
+            for (int e = 0; e < 3; ++e) vv[9 * idx + 3 * d + e] = B[5 * (2 + d) + 2 + e];
+        }
+    };
+    for (int c = 0; c < m.nC; ++c)
+        put(diag, c, k.SS[0][0].diag, k.SS[0][1].diag, k.SS[1][0].diag, k.SS[1][1].diag, k.SV[0].diag, k.SV[1].diag,
+            k.VS[0].diag, k.VS[1].diag, k.VV.diag);
+    for (int f = 0; f < m.nI; ++f) {
+        put(upper, f, k.SS[0][0].upper, k.SS[0][1].upper, k.SS[1][0].upper, k.SS[1][1].upper, k.SV[0].upper,
+            k.SV[1].upper, k.VS[0].upper, k.VS[1].upper, k.VV.upper);
+        put(lower, f, k.SS[0][0].lower, k.SS[0][1].lower, k.SS[1][0].lower, k.SS[1][1].lower, k.SV[0].lower,
+            k.SV[1].lower, k.VS[0].lower, k.VS[1].lower, k.VV.lower);
+    }
+}
+
+void orc_amul(void* h, const double* x, double* y) {
+    Case& k = *(Case*)h;
+    CVec X, Y;
+    toC(x, k.m->nC, X);
+    matrixMul(k, X, Y);
+    fromC(Y, k.m->nC, y);
+}
+
+int orc_precondition(void* h, double* r) {
+    Case& k = *(Case*)h;
+    if (k.s.precond == 0) {
+        int rc = lusgsSetup(k);
+        if (rc) return rc;
+    }
+    CVec R;
+    toC(r, k.m->nC, R);
+    precondition(k, R);
+    fromC(R, k.m->nC, r);
+    return 0;
+}
+
+// solveForIncr on the current matrix: W and b in [5][nC] layout.
+int orc_solve(void* h, const double* W, const double* b, double* dW, double* res /*10*/, int* nIter) {
+    Case& k = *(Case*)h;
+    if (k.s.precond == 0) {
+        int rc = lusgsSetup(k);
+        if (rc) return rc;
+    }
+    CVec w, bb, dw;
+    toC(W, k.m->nC, w);
+    toC(b, k.m->nC, bb);
+    Res r = gmresSolve(k, w, bb, dw);
+    fromC(dw, k.m->nC, dW);
+    res[0] = r.sInit[0]; res[1] = r.sInit[1];
+    for (int d = 0; d < 3; ++d) res[2 + d] = r.vInit[d];
+    res[5] = r.sFinal[0]; res[6] = r.sFinal[1];
+    for (int d = 0; d < 3; ++d) res[7 + d] = r.vFinal[d];
+    *nIter = r.nIter;
+    return 0;
+}
+
+// n pseudo-steps; res receives 10 residuals per step, nIter per step.
+int orc_pseudo_steps(void* h, int n, double* res, int* nIter) {
+    Case& k = *(Case*)h;
+    for (int i = 0; i < n; ++i) {
+        Res r;
+        int rc = pseudoStep(k, r);
+        if (rc) return rc;
+        if (res) {
+            double* o = res + 10 * (size_t)i;
+            o[0] = r.sInit[0]; o[1] = r.sInit[1];
+            for (int d = 0; d < 3; ++d) o[2 + d] = r.vInit[d];
+            o[5] = r.sFinal[0]; o[6] = r.sFinal[1];
+            for (int d = 0; d < 3; ++d) o[7 + d] = r.vFinal[d];
+        }
+        if (nIter) nIter[i] = r.nIter;
+    }
+    return 0;
+}
+
+void orc_get_pseudo_co(void* h, double* co) {
+    Case& k = *(Case*)h;
+    std::copy(k.pseudoCo.begin(), k.pseudoCo.end(), co);
+}
+
+}  // extern "C"

@@ -1,0 +1,235 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper of oracle/liboracle.so, the CPU FP64 restatement of the
+reference path (see icsf_oracle.cpp for the reference files it follows).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+as the checker / baseline; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_vp, _i = C.c_void_p, C.c_int
+_SIG = {
+    "orc_mesh_new": (_vp, [_i, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp]),
+    "orc_mesh_free": (None, [_vp]),
+    "orc_mesh_ncells": (_i, [_vp]),
+    "orc_mesh_geometry": (None, [_vp] + [_vp] * 8),
+    "orc_rcm": (None, [_i, _i, _vp, _vp, _vp]),
+    "orc_colour": (_i, [_i, _i, _vp, _vp, _vp]),
+    "orc_renumber": (None, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "orc_case_new": (_vp, [_vp, _vp, _vp]),
+    "orc_case_free": (None, [_vp]),
+    "orc_set_bc": (None, [_vp, _i, _i, _i, _i, _vp, _vp, _vp]),
+    "orc_set_fields": (None, [_vp, _vp, _vp, _vp]),
+    "orc_get_fields": (None, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "orc_calc_flux": (None, [_vp, _vp, _vp, _vp]),
+    "orc_residual": (None, [_vp, _vp]),
+    "orc_local_dt": (None, [_vp, _vp]),
+    "orc_assemble": (None, [_vp, _vp, _vp, _vp]),
+    "orc_set_matrix": (None, [_vp, _vp, _vp, _vp]),
+    "orc_amul": (None, [_vp, _vp, _vp]),
+    "orc_precondition": (_i, [_vp, _vp]),
+    "orc_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "orc_pseudo_steps": (_i, [_vp, _i, _vp, _vp]),
+    "orc_get_pseudo_co": (None, [_vp, _vp]),
+}
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            from .build import build
+            build()
+        h = C.CDLL(LIB)
+        for k, (r, a) in _SIG.items():
+            f = getattr(h, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = h
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def parse_settings(thermo: str, schemes: str, solution: str):
+    """Numeric settings from the case dictionaries (simple keyword scan)."""
+    import re
+
+    def num(text, key, default):
+        m = re.search(r"\b" + re.escape(key) + r"\s+([-+0-9.eE]+)\s*;", text)
+        return float(m.group(1)) if m else default
+
+    def word(text, key, default):
+        m = re.search(r"\b" + re.escape(key) + r"\s+([A-Za-z<>]+)", text)
+        return m.group(1) if m else default
+
+    flux = word(schemes, "fluxScheme", "AUSMPlusUp")
+    lowMach = word(schemes, "lowMachAusm", "true") == "true"
+    recon = 0 if re.search(r"reconstruct\(rho\)\s+upwind", schemes) else 1
+    lim = 1 if "Venkatakrishnan" in schemes else (2 if "cellLimited<minmod>" in schemes else 0)
+    mu = num(thermo, "mu", 0.0)
+    precond = {"LUSGS": 0, "Jacobi": 1}[word(solution, "preconditioner", "LUSGS")]
+    iset = np.array([0 if flux == "AUSMPlusUp" else 1, int(lowMach), recon, lim, int(mu > 0), precond,
+                     int(num(solution, "nDirections", 30)), int(num(solution, "maxIter", 1000)),
+                     int(num(solution, "minIter", 0)), 1, 1], np.int32)
+    co = num(solution, "pseudoCoNum", 1.0)
+    rset = np.array([num(thermo, "R", 287.0), num(thermo, "gamma", 1.4), mu, num(thermo, "Pr", 1.0),
+                     num(schemes, "entropyFixCoeff", 0.05), num(solution, "tolerance", 1e-12),
+                     num(solution, "relTol", 1e-2), co, num(solution, "pseudoCoNumMin", 0.1),
+                     num(solution, "pseudoCoNumMax", 25.0), 1.25, 0.1, 0.95, -1e15, 1e-15, 1e15], np.float64)
+    return iset, rset
+
+
+class OracleMesh:
+    def __init__(self, mesh):
+        start, size, kind = mesh.patch_arrays()
+        self._keep = [_f64(mesh.points), _i32(mesh.face_off), _i32(mesh.face_verts), _i32(mesh.owner),
+                      _i32(mesh.neighbour), start, size, kind]
+        k = self._keep
+        self.h = lib().orc_mesh_new(mesh.points.shape[0], _p(k[0]), mesh.n_faces, _p(k[1]), _p(k[2]),
+                                    mesh.n_internal, _p(k[3]), _p(k[4]), len(mesh.patches), _p(start), _p(size),
+                                    _p(kind))
+        self.poly = mesh
+        self.nCells = lib().orc_mesh_ncells(self.h)
+
+    def geometry(self):
+        nC, nF = self.nCells, self.poly.n_faces
+        o = {k: np.empty(s) for k, s in [("C", (nC, 3)), ("V", nC), ("Sf", (nF, 3)), ("Cf", (nF, 3)),
+                                        ("w", nF), ("dc", nF), ("nodc", nF), ("corr", (nF, 3))]}
+        lib().orc_mesh_geometry(self.h, *[_p(o[k]) for k in ["C", "V", "Sf", "Cf", "w", "dc", "nodc", "corr"]])
+        return o
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_free(self.h)
+            self.h = None
+
+
+def rcm(mesh):
+    out = np.empty(mesh.n_cells, np.int32)
+    lib().orc_rcm(mesh.n_cells, mesh.n_internal, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)), _p(out))
+    return out
+
+
+def colour(mesh):
+    out = np.empty(mesh.n_cells, np.int32)
+    n = lib().orc_colour(mesh.n_cells, mesh.n_internal, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)), _p(out))
+    return out, n
+
+
+def renumber(mesh, new_to_old):
+    nF, nI = mesh.n_faces, mesh.n_internal
+    fv = np.empty_like(mesh.face_verts)
+    own = np.empty(nF, np.int32)
+    nei = np.empty(nI, np.int32)
+    fm = np.empty(nF, np.int32)
+    lib().orc_renumber(mesh.n_cells, nF, nI, _p(_i32(mesh.face_off)), _p(_i32(mesh.face_verts)),
+                       _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)), _p(_i32(new_to_old)), _p(fv), _p(own),
+                       _p(nei), _p(fm))
+    return fv, own, nei, fm
+
+
+class OracleCase:
+    def __init__(self, case):
+        self.mesh = OracleMesh(case.mesh)
+        self.iset, self.rset = parse_settings(case.thermo, case.fvSchemes, case.fvSolution)
+        self.h = lib().orc_case_new(self.mesh.h, _p(self.iset), _p(self.rset))
+        self.nCells = self.mesh.nCells
+        self.nInternal = case.mesh.n_internal
+        self.nFaces = case.mesh.n_faces
+        for i, bc in enumerate(case.bcs):
+            pv = _f64(bc.p_val) if bc.p_val is not None else None
+            uv = _f64(bc.U_val) if bc.U_val is not None else None
+            tv = _f64(bc.T_val) if bc.T_val is not None else None
+            self._bc = (pv, uv, tv)
+            lib().orc_set_bc(self.h, i, bc.p_type, bc.U_type, bc.T_type, _p(pv), _p(uv), _p(tv))
+        self.set_fields(case.rho, case.U, case.T)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_case_free(self.h)
+            self.h = None
+
+    def set_fields(self, rho, U, T):
+        lib().orc_set_fields(self.h, _p(_f64(rho)), _p(_f64(U)), _p(_f64(T)))
+
+    def fields(self):
+        n = self.nCells
+        rho, U, T, p, W = np.empty(n), np.empty((n, 3)), np.empty(n), np.empty(n), np.empty((5, n))
+        lib().orc_get_fields(self.h, _p(rho), _p(U), _p(T), _p(p), _p(W))
+        return dict(rho=rho, U=U, T=T, p=p, W=W)
+
+    def calcFlux(self):
+        nF = self.nFaces
+        phi, phiU, phiE = np.empty(nF), np.empty((nF, 3)), np.empty(nF)
+        lib().orc_calc_flux(self.h, _p(phi), _p(phiU), _p(phiE))
+        return phi, phiU, phiE
+
+    def residual(self):
+        b = np.empty((5, self.nCells))
+        lib().orc_residual(self.h, _p(b))
+        rdt = np.empty(self.nCells)
+        lib().orc_local_dt(self.h, _p(rdt))
+        return b, rdt
+
+    def createJacobian(self):
+        nC, nI = self.nCells, self.nInternal
+        d, u, l = np.empty((nC, 5, 5)), np.empty((nI, 5, 5)), np.empty((nI, 5, 5))
+        lib().orc_assemble(self.h, _p(d), _p(u), _p(l))
+        return d, u, l
+
+    def set_matrix(self, d, u, l):
+        lib().orc_set_matrix(self.h, _p(_f64(d)), _p(_f64(u)), _p(_f64(l)))
+
+    def matrixMul(self, x):
+        y = np.empty((5, self.nCells))
+        lib().orc_amul(self.h, _p(_f64(x)), _p(y))
+        return y
+
+    def precondition(self, r):
+        r = np.array(r, dtype=np.float64, order="C", copy=True)
+        rc = lib().orc_precondition(self.h, _p(r))
+        if rc:
+            raise RuntimeError(f"oracle precondition failed ({rc})")
+        return r
+
+    def solveForIncr(self, W, b):
+        dW = np.empty((5, self.nCells))
+        res = np.empty(10)
+        it = np.zeros(1, np.int32)
+        rc = lib().orc_solve(self.h, _p(_f64(W)), _p(_f64(b)), _p(dW), _p(res), _p(it))
+        if rc:
+            raise RuntimeError(f"oracle solve failed ({rc})")
+        return dW, res, int(it[0])
+
+    def pseudo_steps(self, n):
+        res = np.empty((n, 10))
+        it = np.zeros(n, np.int32)
+        rc = lib().orc_pseudo_steps(self.h, n, _p(res), _p(it))
+        if rc:
+            raise RuntimeError(f"oracle pseudo-step failed ({rc})")
+        return res
+
+    def pseudoCo(self):
+        co = np.empty(self.nCells)
+        lib().orc_get_pseudo_co(self.h, _p(co))
+        return co
