@@ -1,0 +1,190 @@
+"""Benchmark: implicit pseudo-steps/s of the coupled solver at 4.1 M cells.
+
+Workload (default): 160^3 hex box (4 096 000 cells, 12 211 200 internal
+faces), node jitter +-10%, random initial cell permutation then RCM then
+colour-major numbering; viscous Roe flux (mu = 1.8e-5, Pr = 0.72), Mach
+0.5 -> 0.9 ramp, planar tanh pressure jump (ratio 2), 2% smooth noise;
+local pseudo-time CFL 50.  Two forms of the same pseudo-step:
+  fp64  : GMRES(30) (left LU-SGS, one cycle) with FP64 Jacobian, factors,
+          vectors and basis - the reference algorithm;
+  mixed : FP64 residual/update and FP64 outer FGMRES(1) around an FP32
+          inner GMRES(30) with FP32 LU-SGS on the FP32 Jacobian.
+`value` is the mixed form's pseudo-steps/s; both forms are reported.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 160]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_case(n: int, form: str, orth: str):
+    from icsfoam_amd import cases
+    from icsfoam_amd.mesh import prepare_case
+
+    if form == "fp64":
+        kw = dict(solver="GMRES", nDirections=30, precision="double")
+    else:
+        kw = dict(solver="FGMRES", nDirections=1, precision="mixed", innerDirections=30)
+    c = cases.case_box(n=(n, n, n), lengths=(1.0, 1.0, 1.0), jitter=0.1, seed=7, flux="ROE", mu=1.8e-5,
+                       mach=(0.5, 0.9), p_jump=2.0, noise=0.02, co=50.0, orthogonalisation=orth, **kw)
+    return c
+
+
+def prepared(n: int):
+    from icsfoam_amd.mesh import prepare_case
+
+    t0 = time.time()
+    base = make_case(n, "fp64", "CGS")
+    log(f"[bench] mesh {n}^3 generated in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    base = prepare_case(base, permute_seed=2024)
+    log(f"[bench] permute + RCM + colour-major in {time.time() - t0:.1f}s")
+    return base
+
+
+def with_form(base, n, form, orth):
+    import dataclasses
+
+    c = make_case(4, form, orth)  # dictionaries only
+    return dataclasses.replace(base, fvSolution=c.fvSolution)
+
+
+def algorithmic_bytes(nC, nI, nB_wall, p, q):
+    """Per-launch algorithmic bytes of the block SpMV: every off-diagonal
+    block (2 per internal face) and diagonal block read once, column index
+    per block slot, x gathered once per cell, y written once."""
+    return (2 * nI + nC) * 25 * p + 2 * nI * 4 + nC * 5 * q * 2
+
+
+def time_form(case, steps, warmup, label):
+    from icsfoam_amd import dbnsCase
+
+    t0 = time.time()
+    g = dbnsCase.from_case(case)
+    log(f"[bench] {label}: case set up in {time.time() - t0:.1f}s")
+    g.pseudo_steps(warmup, residuals=False)
+    g.synchronize()
+    # event timing of the block SpMV launches (class 2) during the timed steps
+    g.timing(1 << 2)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.pseudo_steps(steps, residuals=False)
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    ms, cnt = g.timing_read()
+    g.timing(0)
+    res = g.pseudo_steps(1, residuals=True)[0]
+    log(f"[bench] {label}: {steps} steps in {dt:.3f}s -> {steps / dt:.3f} steps/s; "
+        f"spmv {cnt[2]} launches avg {ms[2] / max(cnt[2], 1):.3f} ms; final res {res[:5]}")
+    # per-class breakdown from one extra profiled step
+    g.timing(0x3F)
+    g.pseudo_steps(1, residuals=False)
+    bms, bcnt = g.timing_read()
+    g.timing(0)
+    del g
+    return dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, spmv_ms=ms[2] / max(cnt[2], 1),
+                spmv_count=int(cnt[2]), breakdown_ms=[round(float(x), 3) for x in bms],
+                breakdown_count=[int(x) for x in bcnt])
+
+
+def cpu_baseline(n_sample: int, n_full: int, steps: int = 2):
+    """Oracle (sequential C++ restatement of the reference path, FP64) on a
+    bounded sample of the same workload; steps/s scaled to the full cell
+    count by the cell ratio (the work per step is linear in cells)."""
+    from oracle.build import build as build_oracle
+    from oracle.oracle import OracleCase
+    from icsfoam_amd.mesh import prepare_case
+
+    if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
+        build_oracle()
+    c = prepare_case(make_case(n_sample, "fp64", "MGS"), permute_seed=2024)
+    o = OracleCase(c)
+    o.pseudo_steps(1)
+    t0 = time.perf_counter()
+    o.pseudo_steps(steps)
+    dt = (time.perf_counter() - t0) / steps
+    scale = (n_sample ** 3) / float(n_full ** 3)
+    return dict(value=scale / dt, unit="pseudo-steps/s", cores=1, kind="port",
+                sample=f"oracle (sequential C++ FP64 restatement) on the {n_sample}^3 box of the same "
+                       f"workload, {steps} timed steps after 1 warm-up: {dt:.2f} s/step, scaled by the cell "
+                       f"ratio {n_sample ** 3}/{n_full ** 3}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=160)
+    ap.add_argument("--orth", default="CGS", choices=["CGS", "MGS"])
+    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the pseudo-step on exactly one GPU; --gpus {args.gpus} is not "
+              "supported (multi-GPU sharding is designed, not built)", file=sys.stderr)
+        sys.exit(2)
+    n = args.n
+    base = prepared(n)
+    nC, nI = base.mesh.n_cells, base.mesh.n_internal
+    nBw = sum(p.size for p in base.mesh.patches if p.kind == 1)
+    r64 = time_form(with_form(base, n, "fp64", args.orth), args.steps, args.warmup, "fp64")
+    rmx = time_form(with_form(base, n, "mixed", args.orth), args.steps, args.warmup, "mixed")
+    # dominant kernel: block SpMV of the mixed form's FP64 outer/FP32 inner
+    # launches; report the FP64-form SpMV (FP64 blocks and vectors) whose
+    # algorithmic bytes are unambiguous.
+    bytes64 = algorithmic_bytes(nC, nI, nBw, 8, 8)
+    ach = bytes64 / (r64["spmv_ms"] * 1e-3)
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline on {args.cpu_sample}^3 ...")
+        cpu = cpu_baseline(args.cpu_sample, n)
+    out = {
+        "metric": "implicit pseudo-steps/s at 4.1M cells: FP32-precond vs FP64; HBM GB/s frac",
+        "value": round(rmx["steps_per_s"], 4),
+        "unit": "pseudo-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(rmx["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 residual/outer + f32 Jacobian/inner",
+        "data": "synthetic (seeded jittered 160^3 hex box, Roe viscous state with tanh pressure jump)",
+        "config": {"workload": f"{n}^3 jittered hex box ({nC} cells, {nI} internal faces), viscous Roe, "
+                               f"local CFL 50, LU-SGS, permuted+RCM+colour-major",
+                   "fp64_steps_per_s": round(r64["steps_per_s"], 4),
+                   "fp64_ms_per_step": round(r64["ms_per_step"], 3),
+                   "mixed_over_fp64": round(rmx["steps_per_s"] / r64["steps_per_s"], 3),
+                   "orthogonalisation": args.orth,
+                   "breakdown_ms_fp64": r64["breakdown_ms"], "breakdown_ms_mixed": rmx["breakdown_ms"]},
+        "roofline": {"bound": "hbm", "kernel": "k_spmv<double,double> (block SpMV, FP64 form)",
+                     "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK, 4), "traffic": None,
+                     "bytes_per_launch": bytes64, "avg_launch_ms": round(r64["spmv_ms"], 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,6 +1,7 @@
 // Launch interfaces of the HIP kernels (host side).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <functional>
 #include <vector>
 
 #include "icsf_views.h"
@@ -39,7 +40,30 @@ struct GmresArgs {
     const double* rD;
     const void* dinv;
     void* r;          // residual / work vector, [5][nC]
+    void* r2;         // second work vector (SpMV output), [5][nC]
     void* V;          // Krylov basis, nDirs vectors of [5][nC]
+    // optional timing marks: mark(kernelClass, begin?1:0)
+    std::function<void(int, int)> mark;
+};
+
+// kernel classes of the timing marks
+enum { TK_RESIDUAL = 0, TK_JACOBIAN = 1, TK_SPMV = 2, TK_PRECOND = 3, TK_ORTH = 4, TK_OTHER = 5 };
+
+// Flexible GMRES (FP64 outer); precond(v, z) writes z = M^-1 v.
+struct FgmresArgs {
+    const MeshView* m;
+    Work* wk;
+    GmresScalars* g;
+    int nDirs;
+    int orth;
+    int blkFp32;
+    const void* blk;
+    const void* dg;
+    void* r;  // [5][nC] FP64
+    void* V;  // nDirs+1 FP64 vectors
+    void* Z;  // nDirs FP64 vectors
+    std::function<void(const double*, double*)> precond;
+    std::function<void(int, int)> mark;
 };
 
 int reduceBlocks(int nC);
@@ -67,11 +91,17 @@ void gmresCycle(const GmresArgs& a, int vecFp32, hipStream_t st);
 void gmresUpdateAndResidual(const MeshView& m, Work& wk, GmresScalars* g, int vecFp32, int blkFp32,
                             const void* blk, const void* dg, int nDirs, const void* V, void* dW, const void* b,
                             void* r, hipStream_t st);
+void fgmresCycle(const FgmresArgs& a, hipStream_t st);
+void fgmresUpdateAndResidual(const MeshView& m, Work& wk, GmresScalars* g, int blkFp32, const void* blk,
+                             const void* dg, int nDirs, const void* Z, void* dW, const void* b, void* r,
+                             hipStream_t st);
+void launchCombine(int nC, int vecFp32, const void* V, int nv, const GmresScalars* g, void* z, hipStream_t st);
 void gmresReadResiduals(const GmresScalars* g, double* res10, hipStream_t st);
 const double* gmresResidualsDevice(const GmresScalars* g);
 
 void launchCopy(size_t n, int fp32, const void* a, void* b, hipStream_t st);
 void launchConvert(size_t n, int fromFp32, const void* a, void* b, hipStream_t st);
+void launchZero(size_t n, int fp32, void* x, hipStream_t st);
 void launchZeroComponent(int nC, int comp, int fp32, void* x, hipStream_t st);
 
 }  // namespace icsf

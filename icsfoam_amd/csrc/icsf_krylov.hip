@@ -603,13 +603,19 @@ void gmresCycleT(const GmresArgs& a, hipStream_t st) {
     const int nC = m.nC;
     const int nbk = reduceBlocks(nC);
     const size_t vs = (size_t)5 * nC;
-    TV* r = (TV*)a.r;
+    TV* r = (TV*)a.r;    // current w
+    TV* r2 = (TV*)a.r2;  // SpMV output (ping-pong with r)
     TV* V = (TV*)a.V;
     GmresScalars* g = a.g;
     double* red = g->red;
+    auto mark = [&](int k, int b) {
+        if (a.mark) a.mark(k, b);
+    };
     auto precond = [&](TV* v) {
+        mark(TK_PRECOND, 1);
         if (a.precond == 0) launchLusgs(m, *a.lev, a.blkFp32, sizeof(TV) == 4, a.blk, a.rD, v, st);
         else if (a.precond == 1) launchJacobiApply(nC, a.blkFp32, sizeof(TV) == 4, a.dinv, v, st);
+        mark(TK_PRECOND, 0);
     };
     // r <- P^-1 r; beta = ||r||
     precond(r);
@@ -618,9 +624,18 @@ void gmresCycleT(const GmresArgs& a, hipStream_t st) {
     k_gm_beta0<<<1, 64, 0, st>>>(g, a.nDirs);
     for (int i = 0; i < a.nDirs; ++i) {
         TV* vi = V + (size_t)i * vs;
-        // v_i = r / beta ; r = A v_i
-        launchSpmv(m, a.blkFp32, sizeof(TV) == 4, a.blk, a.dg, r, &g->beta, vi, r, st);
+        // v_i = r / beta ; w = A v_i (into the other buffer: the SpMV gathers
+        // neighbour entries of its input, so it never runs in place)
+        mark(TK_SPMV, 1);
+        launchSpmv(m, a.blkFp32, sizeof(TV) == 4, a.blk, a.dg, r, &g->beta, vi, r2, st);
+        mark(TK_SPMV, 0);
+        {
+            TV* t = r;
+            r = r2;
+            r2 = t;
+        }
         precond(r);
+        mark(TK_ORTH, 1);
         if (a.orth == 0) {
             // modified Gram-Schmidt, reference order
             for (int j = 0; j <= i + 1; ++j) {
@@ -644,6 +659,7 @@ void gmresCycleT(const GmresArgs& a, hipStream_t st) {
             k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 1, red);
             k_gm_givens<<<1, 64, 0, st>>>(g, i, 0);
         }
+        mark(TK_ORTH, 0);
     }
     k_gm_backsub<<<1, 64, 0, st>>>(g, a.nDirs);
 }
@@ -724,9 +740,103 @@ void launchConvert(size_t n, int fromFp32, const void* a, void* b, hipStream_t s
     else k_convert<double, float><<<nb(n, 256), 256, 0, st>>>(n, (const double*)a, (float*)b);
 }
 
+namespace {
+template <typename T>
+__global__ void k_zero(size_t n, T* x) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (T)0;
+}
+}  // namespace
+
+// Zero fill by a kernel (no memset nodes, so the sequence can be captured).
+void launchZero(size_t n, int fp32, void* x, hipStream_t st) {
+    if (fp32) k_zero<float><<<nb(n, 256), 256, 0, st>>>(n, (float*)x);
+    else k_zero<double><<<nb(n, 256), 256, 0, st>>>(n, (double*)x);
+}
+
 void launchZeroComponent(int nC, int comp, int fp32, void* x, hipStream_t st) {
     if (fp32) k_zero_comp<float><<<nb(nC, 256), 256, 0, st>>>(nC, comp, (float*)x);
     else k_zero_comp<double><<<nb(nC, 256), 256, 0, st>>>(nC, comp, (double*)x);
+}
+
+}  // namespace icsf
+
+// ------------------------------------------------------- flexible GMRES ---
+namespace icsf {
+
+namespace {
+template <typename TV>
+__global__ void k_scale_by(size_t n, const TV* __restrict__ a, const double* __restrict__ beta, TV* __restrict__ b) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i] / (TV)(*beta);
+}
+}  // namespace
+
+// One FGMRES(m) cycle in FP64 (Saad's right-preconditioned flexible GMRES):
+// v_0 = r/||r||; z_j = M_j^-1 v_j (callback); w = A z_j; Gram-Schmidt
+// against v_0..v_j; Givens; back-substitution into g->yh.  Z holds z_j.
+void fgmresCycle(const FgmresArgs& a, hipStream_t st) {
+    const MeshView& m = *a.m;
+    const int nC = m.nC;
+    const int nbk = reduceBlocks(nC);
+    const size_t vs = (size_t)5 * nC;
+    double* r = (double*)a.r;
+    double* V = (double*)a.V;
+    double* Z = (double*)a.Z;
+    GmresScalars* g = a.g;
+    double* red = g->red;
+    k_norm2<double><<<nbk, 256, 0, st>>>(nC, r, a.wk->part);
+    k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 1, red);
+    k_gm_beta0<<<1, 64, 0, st>>>(g, a.nDirs);
+    for (int i = 0; i < a.nDirs; ++i) {
+        double* vi = V + (size_t)i * vs;
+        double* zi = Z + (size_t)i * vs;
+        k_scale_by<double><<<nb(vs, 256), 256, 0, st>>>(vs, r, &g->beta, vi);
+        a.precond(vi, zi);
+        if (a.mark) a.mark(TK_SPMV, 1);
+        launchSpmv(m, a.blkFp32, 0, a.blk, a.dg, zi, nullptr, nullptr, r, st);
+        if (a.mark) a.mark(TK_SPMV, 0);
+        if (a.mark) a.mark(TK_ORTH, 1);
+        if (a.orth == 0) {
+            for (int j = 0; j <= i + 1; ++j) {
+                const double* vprev = j > 0 ? V + (size_t)(j - 1) * vs : nullptr;
+                const double* vnext = j <= i ? V + (size_t)j * vs : nullptr;
+                const double* hprev = j > 0 ? &g->H[j - 1][i] : nullptr;
+                k_mgs_step<double><<<nbk, 256, 0, st>>>(nC, r, vprev, hprev, vnext, j == i + 1, a.wk->part);
+                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 3, red);
+                if (j <= i) k_gm_store_h<<<1, 64, 0, st>>>(g, i, j);
+            }
+            k_gm_givens<<<1, 64, 0, st>>>(g, i, 1);
+        } else {
+            for (int j0 = 0; j0 <= i; j0 += 16) {
+                const int nv = (i + 1 - j0) < 16 ? (i + 1 - j0) : 16;
+                k_multidot<double, 16><<<nbk, 256, 0, st>>>(nC, r, V, vs, j0, nv, a.wk->part);
+                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 16, red);
+                k_gm_store_col<<<1, 64, 0, st>>>(g, i, j0, nv);
+            }
+            k_cgs_update<double><<<nbk, 256, 0, st>>>(nC, r, V, vs, i + 1, g->hcol, a.wk->part);
+            k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 1, red);
+            k_gm_givens<<<1, 64, 0, st>>>(g, i, 0);
+        }
+        if (a.mark) a.mark(TK_ORTH, 0);
+    }
+    k_gm_backsub<<<1, 64, 0, st>>>(g, a.nDirs);
+}
+
+// z = sum_i y_i V_i from zero (inner solve result), FP32 or FP64 vectors.
+void launchCombine(int nC, int vecFp32, const void* V, int nv, const GmresScalars* g, void* z, hipStream_t st) {
+    const size_t vs = (size_t)5 * nC;
+    launchZero(vs, vecFp32, z, st);
+    if (vecFp32)
+        k_update_sol<float, float><<<nb(vs, 256), 256, 0, st>>>(nC, (float*)z, (const float*)V, vs, nv, g->yh);
+    else
+        k_update_sol<double, double><<<nb(vs, 256), 256, 0, st>>>(nC, (double*)z, (const double*)V, vs, nv, g->yh);
+}
+
+void fgmresUpdateAndResidual(const MeshView& m, Work& wk, GmresScalars* g, int blkFp32, const void* blk,
+                             const void* dg, int nDirs, const void* Z, void* dW, const void* b, void* r,
+                             hipStream_t st) {
+    updResT<double>(m, wk, g, blkFp32, blk, dg, nDirs, Z, dW, b, r, st);
 }
 
 }  // namespace icsf

@@ -120,7 +120,7 @@ struct icsf_case {
     double* dres = nullptr;  // [2][5] init / previous initial residuals
     double* fac = nullptr;
     // FP32 inner solve
-    float *V32 = nullptr, *r32 = nullptr;
+    float *V32 = nullptr, *r32 = nullptr, *r32b = nullptr;
     GmresScalars* g = nullptr;
     GmresScalars* g2 = nullptr;
     Work wk;
@@ -129,35 +129,55 @@ struct icsf_case {
     bool haveInit = false, havePrev = false;
     // timing
     bool timing = false;
+    int tmask = 0;
+    std::vector<hipEvent_t> pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
+    std::vector<hipEvent_t> open;
     double tms[NTIMER] = {0};
     long long tcnt[NTIMER] = {0};
     DevBuf buf;
     ~icsf_case() {
         for (auto& e : events) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
+        for (auto& e : pool) (void)hipEventDestroy(e);
+        for (auto& e : open) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
 };
 
 namespace {
 
+hipEvent_t takeEvent(icsf_case* c) {
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// begin/end marks of a kernel class on the case stream (nested per class)
+void markClass(icsf_case* c, int cls, int begin) {
+    if (!c->timing || !((c->tmask >> cls) & 1)) return;
+    if (c->open.size() < NTIMER) c->open.assign(NTIMER, nullptr);
+    if (begin) {
+        hipEvent_t a = takeEvent(c);
+        (void)hipEventRecord(a, c->st);
+        c->open[cls] = a;
+    } else if (c->open[cls]) {
+        hipEvent_t b = takeEvent(c);
+        (void)hipEventRecord(b, c->st);
+        c->events.push_back({cls, {c->open[cls], b}});
+        c->open[cls] = nullptr;
+    }
+}
+
 struct Timed {
     icsf_case* c;
     int cls;
-    hipEvent_t a = nullptr, b = nullptr;
-    Timed(icsf_case* cc, int k) : c(cc), cls(k) {
-        if (c->timing) {
-            (void)hipEventCreate(&a);
-            (void)hipEventCreate(&b);
-            (void)hipEventRecord(a, c->st);
-        }
-    }
-    ~Timed() {
-        if (c->timing) {
-            (void)hipEventRecord(b, c->st);
-            c->events.push_back({cls, {a, b}});
-        }
-    }
+    Timed(icsf_case* cc, int k) : c(cc), cls(k) { markClass(c, cls, 1); }
+    ~Timed() { markClass(c, cls, 0); }
 };
 
 void resolveEvents(icsf_case* c) {
@@ -168,8 +188,8 @@ void resolveEvents(icsf_case* c) {
         (void)hipEventElapsedTime(&ms, e.second.first, e.second.second);
         c->tms[e.first] += ms;
         c->tcnt[e.first] += 1;
-        (void)hipEventDestroy(e.second.first);
-        (void)hipEventDestroy(e.second.second);
+        c->pool.push_back(e.second.first);
+        c->pool.push_back(e.second.second);
     }
     c->events.clear();
 }
@@ -351,7 +371,7 @@ int uploadBC(icsf_case* c) {
 int residualStage(icsf_case* c) {
     RC(uploadBC(c));
     {
-        Timed t(c, 0);
+        Timed t(c, TK_RESIDUAL);
         launchResidual(c->mv, c->ph, c->sv, c->st);
     }
     c->haveResidual = true;
@@ -360,7 +380,7 @@ int residualStage(icsf_case* c) {
 
 int jacobianStage(icsf_case* c) {
     {
-        Timed t(c, 1);
+        Timed t(c, TK_JACOBIAN);
         launchJacobian(c->mv, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, nullptr, c->derr, c->st);
     }
     if (c->precond == 1) launchJacobiSetup(c->mv.nC, c->blkFp32, c->dg, c->dinv, c->st);
@@ -369,7 +389,7 @@ int jacobianStage(icsf_case* c) {
 }
 
 void applyPrecond(icsf_case* c, int vecFp32, void* v) {
-    Timed t(c, 3);
+    Timed t(c, TK_PRECOND);
     if (c->precond == 0) launchLusgs(c->mv, c->mesh->lev, c->blkFp32, vecFp32, c->blk, c->rD, v, c->st);
     else if (c->precond == 1) launchJacobiApply(c->mv.nC, c->blkFp32, vecFp32, c->dinv, v, c->st);
 }
@@ -382,17 +402,15 @@ int gmresSolve(icsf_case* c, const double* W, const double* b, int& nIter) {
     {
         Timed t(c, 5);
         gmresPrepare(m, c->wk, c->g, 0, W, b, c->t1, c->t2, c->blkFp32, c->blk, c->dg, c->st);
-        HIPCHK(hipMemsetAsync(c->dW, 0, vs * sizeof(double), c->st));
+        launchZero(vs, 0, c->dW, c->st);
         launchCopy(vs, 0, b, c->r, c->st);
     }
     nIter = 0;
     for (;;) {
         GmresArgs a{&c->mv, &c->mesh->lev, &c->wk, c->g, c->nDirs, c->precond, c->orth, c->blkFp32,
-                    c->blk, c->dg, c->rD, c->dinv, c->r, c->V};
-        {
-            Timed t(c, 4);
-            gmresCycle(a, 0, c->st);
-        }
+                    c->blk, c->dg, c->rD, c->dinv, c->r, c->t2, c->V};
+        a.mark = [c](int k, int b) { markClass(c, k, b); };
+        gmresCycle(a, 0, c->st);
         {
             Timed t(c, 5);
             gmresUpdateAndResidual(m, c->wk, c->g, 0, c->blkFp32, c->blk, c->dg, c->nDirs, c->V, c->dW, b, c->r,
@@ -459,7 +477,59 @@ int pseudoStep(icsf_case* c) {
 // FGMRES (outer FP64, inner FP32) -----------------------------------------
 namespace {
 int fgmresSolve(icsf_case* c, const double* W, const double* b, int& nIter) {
-    return fail(ICSF_ERR_STATE, "FGMRES is not available in this build");
+    const MeshView& m = c->mv;
+    const size_t vs = (size_t)5 * m.nC;
+    {
+        Timed t(c, 5);
+        gmresPrepare(m, c->wk, c->g, 0, W, b, c->t1, c->t2, c->blkFp32, c->blk, c->dg, c->st);
+        launchZero(vs, 0, c->dW, c->st);
+        launchCopy(vs, 0, b, c->r, c->st);
+    }
+    // z = M^-1 v
+    auto precond = [c, vs](const double* v, double* z) {
+        if (c->mixed) {
+            launchConvert(vs, 0, v, c->r32, c->st);
+            if (c->innerDirs > 0) {
+                GmresArgs a{&c->mv, &c->mesh->lev, &c->wk, c->g2, c->innerDirs, c->precond, c->orth, c->blkFp32,
+                            c->blk, c->dg, c->rD, c->dinv, c->r32, c->r32b, c->V32};
+                a.mark = [c](int k, int b) { markClass(c, k, b); };
+                gmresCycle(a, 1, c->st);
+                Timed t(c, 5);
+                launchCombine(c->mv.nC, 1, c->V32, c->innerDirs, c->g2, c->r32, c->st);
+            } else {
+                applyPrecond(c, 1, c->r32);
+            }
+            launchConvert(vs, 1, c->r32, z, c->st);
+        } else {
+            launchCopy(vs, 0, v, z, c->st);
+            applyPrecond(c, 0, z);
+        }
+    };
+    nIter = 0;
+    for (;;) {
+        FgmresArgs a{&c->mv, &c->wk, c->g, c->nDirs, c->orth, c->blkFp32, c->blk, c->dg, c->r, c->V, c->Z, precond,
+                     [c](int k, int b) { markClass(c, k, b); }};
+        fgmresCycle(a, c->st);
+        {
+            Timed t(c, 5);
+            fgmresUpdateAndResidual(m, c->wk, c->g, c->blkFp32, c->blk, c->dg, c->nDirs, c->Z, c->dW, b, c->r,
+                                    c->st);
+        }
+        ++nIter;
+        if (nIter < c->minIter) continue;
+        if (nIter >= c->maxIter) break;
+        double res[10];
+        gmresReadResiduals(c->g, res, c->st);
+        double mx = -1e300, rel = -1e300;
+        for (int k = 5; k < 10; ++k) mx = std::max(mx, res[k]);
+        for (int k = 0; k < 5; ++k) {
+            if (k >= 2 && (k - 2) == m.emptyDir) continue;
+            rel = std::max(rel, res[5 + k] / (res[k] + kROOTVSMALL));
+        }
+        if (mx < c->tol || rel < c->relTol) break;
+    }
+    if (m.emptyDir >= 0) launchZeroComponent(m.nC, 2 + m.emptyDir, 0, c->dW, c->st);
+    return ICSF_OK;
 }
 }  // namespace
 
@@ -705,6 +775,7 @@ int icsf_case_create(icsf_mesh* mesh, const char* thermo, const char* schemes, c
         if (C->mixed) {
             RC(B.alloc(C->V32, vs * (size_t)(C->innerDirs + 1)));
             RC(B.alloc(C->r32, vs));
+            RC(B.alloc(C->r32b, vs));
         }
     }
     {
@@ -972,10 +1043,11 @@ int icsf_synchronize(icsf_case* c) {
     return ICSF_OK;
 }
 
-int icsf_timing_enable(icsf_case* c, int on) {
+int icsf_timing_enable(icsf_case* c, int mask) {
     if (!c) return fail(ICSF_ERR_ARG, "NULL case");
     resolveEvents(c);
-    c->timing = on != 0;
+    c->timing = mask != 0;
+    c->tmask = mask;
     for (int k = 0; k < NTIMER; ++k) { c->tms[k] = 0; c->tcnt[k] = 0; }
     return ICSF_OK;
 }

@@ -146,8 +146,9 @@ class dbnsCase:
     def synchronize(self):
         check(lib().icsf_synchronize(self._h))
 
-    def timing(self, on: bool = True):
-        check(lib().icsf_timing_enable(self._h, 1 if on else 0))
+    def timing(self, mask: int = 0x3F):
+        """Event timing of kernel classes (bit k = class k; 0 stops)."""
+        check(lib().icsf_timing_enable(self._h, int(mask)))
 
     def timing_read(self):
         ms = np.zeros(6)

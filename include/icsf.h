@@ -190,12 +190,15 @@ int icsf_get_pseudo_co(icsf_case* c, double* co);
 /* Wait for all work queued on the case's stream. */
 int icsf_synchronize(icsf_case* c);
 
-/* Kernel timing with HIP events recorded on the launch stream.
- * icsf_timing_enable(c, 1) starts accumulating; icsf_timing_read returns,
- * per kernel class k < n, total milliseconds and launch count.
- * Classes: 0 residual, 1 jacobian, 2 spmv, 3 lusgs, 4 orthogonalisation,
- * 5 update/other. */
-int icsf_timing_enable(icsf_case* c, int on);
+/* Kernel timing with HIP events recorded on the launch stream around each
+ * launch (or launch group) of the selected kernel classes.
+ * icsf_timing_enable(c, mask) resets and starts accumulating for classes
+ * whose bit is set (mask 0 stops); icsf_timing_read returns, per class
+ * k < n, total milliseconds and event-pair count.  Classes: 0 residual
+ * (boundary + gradient + residual kernels), 1 Jacobian, 2 block SpMV,
+ * 3 preconditioner (all sweep levels of one application), 4
+ * orthogonalisation + Givens of one Arnoldi step, 5 other. */
+int icsf_timing_enable(icsf_case* c, int mask);
 int icsf_timing_read(icsf_case* c, int n, double* ms, long long* count);
 
 #ifdef __cplusplus

@@ -1881,4 +1881,66 @@ void orc_get_pseudo_co(void* h, double* co) {
     std::copy(k.pseudoCo.begin(), k.pseudoCo.end(), co);
 }
 
+
+// Face-level flux for pinning tests: scheme 0 AUSM+up, 1 Roe.
+// in = [Sx,Sy,Sz, rhoL,pL,ULx,ULy,ULz,TL, rhoR,pR,URx,URy,URz,TR]; the
+// reconstructed E, H, c* are formed from each side's state as the cell
+// fields are.  out = [phi, phiUx, phiUy, phiUz, phiE].
+void orc_face_flux(int scheme, int lowMach, double eFix, double R, double gamma, const double* in, double* out) {
+    Case k;
+    k.s.flux = scheme; k.s.lowMach = lowMach; k.s.entropyFix = eFix; k.s.R = R; k.s.gamma = gamma;
+    k.Cv = R / (gamma - 1.0);
+    V3 S{in[0], in[1], in[2]};
+    const double magS = mag(S);
+    double rho[2], p[2], T[2], E[2], H[2], cs[2];
+    V3 U[2];
+    for (int s = 0; s < 2; ++s) {
+        const double* q = in + 3 + 6 * s;
+        rho[s] = q[0]; p[s] = q[1]; U[s] = V3{q[2], q[3], q[4]}; T[s] = q[5];
+        E[s] = k.Cv * T[s] + 0.5 * magSqr(U[s]);
+        H[s] = std::max(E[s], SMALL) + std::max(p[s] / rho[s], SMALL);
+        cs[s] = std::sqrt(2.0 * (gamma - 1.0) / (gamma + 1.0) * H[s]);
+    }
+    double ph, pe;
+    V3 pu;
+    if (scheme == FLUX_AUSM) ausmFace(k, S, magS, U[0], U[1], cs[0], cs[1], p[0], p[1], rho[0], rho[1], H[0], H[1], ph, pu, pe);
+    else roeFace(k, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], E[0], E[1], H[0], H[1], ph, pu, pe);
+    out[0] = ph; out[1] = pu.x; out[2] = pu.y; out[3] = pu.z; out[4] = pe;
+}
+
+// Reconstructed face values (L, R) of one cell field of the current state:
+// 0 rho, 1 p, 2 Ux, 3 Uy, 4 Uz, 5 E, 6 H, 7 c*.
+void orc_reconstruct(void* h, int field, double* L, double* R) {
+    Case& k = *(Case*)h;
+    Mesh& m = *k.m;
+    Derived d;
+    derive(k, d);
+    const int nB = m.nF - m.nI;
+    vec vc(m.nC), vb(nB, 0.0);
+    for (int c = 0; c < m.nC; ++c) {
+        switch (field) {
+            case 0: vc[c] = k.rho[c]; break;
+            case 1: vc[c] = k.p[c]; break;
+            case 2: case 3: case 4: vc[c] = k.U[3 * c + field - 2]; break;
+            case 5: vc[c] = d.E[c]; break;
+            case 6: vc[c] = d.H[c]; break;
+            default: vc[c] = d.cstar[c]; break;
+        }
+    }
+    for (int b = 0; b < nB; ++b) {
+        switch (field) {
+            case 0: vb[b] = k.rhoB[b]; break;
+            case 1: vb[b] = k.pB[b]; break;
+            case 2: case 3: case 4: vb[b] = k.UB[3 * b + field - 2]; break;
+            case 5: vb[b] = d.Eb[b]; break;
+            case 6: vb[b] = d.Hb[b]; break;
+            default: vb[b] = d.cstarb[b]; break;
+        }
+    }
+    vec l, r;
+    reconstruct(k, vc.data(), vb.data(), 1, l, r);
+    std::copy(l.begin(), l.end(), L);
+    std::copy(r.begin(), r.end(), R);
+}
+
 }  // extern "C"

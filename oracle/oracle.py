@@ -39,6 +39,8 @@ _SIG = {
     "orc_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "orc_pseudo_steps": (_i, [_vp, _i, _vp, _vp]),
     "orc_get_pseudo_co": (None, [_vp, _vp]),
+    "orc_face_flux": (None, [_i, _i, C.c_double, C.c_double, C.c_double, _vp, _vp]),
+    "orc_reconstruct": (None, [_vp, _i, _vp, _vp]),
 }
 _lib = None
 
@@ -122,6 +124,14 @@ class OracleMesh:
         if getattr(self, "h", None):
             lib().orc_mesh_free(self.h)
             self.h = None
+
+
+def face_flux(scheme, S, left, right, lowMach=True, eFix=0.05, R=287.0, gamma=1.4):
+    """Flux through one face; left/right = (rho, p, Ux, Uy, Uz, T)."""
+    inp = np.ascontiguousarray(np.concatenate([S, left, right]), np.float64)
+    out = np.empty(5)
+    lib().orc_face_flux({"AUSMPlusUp": 0, "ROE": 1}[scheme], int(lowMach), eFix, R, gamma, _p(inp), _p(out))
+    return out
 
 
 def rcm(mesh):
@@ -228,6 +238,11 @@ class OracleCase:
         if rc:
             raise RuntimeError(f"oracle pseudo-step failed ({rc})")
         return res
+
+    def reconstruct(self, field):
+        L, R = np.empty(self.nFaces), np.empty(self.nFaces)
+        lib().orc_reconstruct(self.h, field, _p(L), _p(R))
+        return L, R
 
     def pseudoCo(self):
         co = np.empty(self.nCells)
