@@ -50,6 +50,7 @@ SIGNATURES = {
     "icsf_colour_major_order": (_i, [_i, _vp, _i, _vp]),
     "icsf_renumber_mesh": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_sweep_levels": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "icsf_block_mesh_hex": (_i, [_i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp]),
     "icsf_mesh_geometry": (_i, [_i, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_mesh_create": (_i, [_i, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "icsf_mesh_destroy": (_i, [_vp]),

@@ -44,6 +44,8 @@ class PolyMesh:
     neighbour: np.ndarray       # int32 [nInternalFaces]
     patches: List[Patch]
     n_cells: int
+    # vertex-average cell centres of a structured block in generation order
+    centre_hint: Optional[np.ndarray] = None
 
     @property
     def n_faces(self) -> int:
@@ -89,8 +91,38 @@ class Case:
 # structured hex blocks
 # --------------------------------------------------------------------------
 
+_SIDES = ["-x", "+x", "-y", "+y", "-z", "+z"]
+
+
 def _hex_topology(nx: int, ny: int, nz: int, patch_faces):
-    """Faces of an nx*ny*nz block in blockMesh order.
+    """Faces of an nx*ny*nz block in blockMesh order (native generator of
+    libicsf, icsf_block_mesh_hex)."""
+    from ._lib import check, lib, ptr
+
+    side_patch = np.full(6, -1, np.int32)
+    for p, (_, _, sides) in enumerate(patch_faces):
+        for sd in sides:
+            side_patch[_SIDES.index(sd)] = p
+    if (side_patch < 0).any():
+        raise ValueError("every side of the block needs a patch")
+    nI = (nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1)
+    nB = 2 * (ny * nz + nx * nz + nx * ny)
+    fv = np.empty((nI + nB, 4), np.int32)
+    own = np.empty(nI + nB, np.int32)
+    nei = np.empty(nI, np.int32)
+    psize = np.empty(len(patch_faces), np.int32)
+    check(lib().icsf_block_mesh_hex(nx, ny, nz, ptr(side_patch), len(patch_faces), ptr(fv), ptr(own), ptr(nei),
+                                    ptr(psize)))
+    patches, start = [], nI
+    for (name, kind, _), sz in zip(patch_faces, psize):
+        patches.append(Patch(name, kind, start, int(sz)))
+        start += int(sz)
+    return fv, own, nei, patches
+
+
+def _hex_topology_numpy(nx: int, ny: int, nz: int, patch_faces):
+    """Faces of an nx*ny*nz block in blockMesh order (numpy statement of the
+    same ordering, used to cross-check the native generator).
 
     patch_faces: list of (name, kind, sides) where sides is a list of
     '-x', '+x', '-y', '+y', '-z', '+z'.
@@ -165,12 +197,18 @@ def _hex_topology(nx: int, ny: int, nz: int, patch_faces):
     return face_verts, owner, nei.astype(np.int32), patches
 
 
-def _mesh_from_quads(points, quads, owner, neighbour, patches, n_cells) -> PolyMesh:
+def _mesh_from_quads(points, quads, owner, neighbour, patches, n_cells, grid=None) -> PolyMesh:
     nF = quads.shape[0]
-    face_off = (np.arange(nF + 1, dtype=np.int64) * 4).astype(np.int32)
+    face_off = np.arange(0, 4 * (nF + 1), 4, dtype=np.int32)
+    hint = None
+    if grid is not None:
+        # grid: points as [nz+1, ny+1, nx+1, 3]; average of the 8 vertices
+        g = grid
+        hint = (g[:-1, :-1, :-1] + g[:-1, :-1, 1:] + g[:-1, 1:, :-1] + g[:-1, 1:, 1:] +
+                g[1:, :-1, :-1] + g[1:, :-1, 1:] + g[1:, 1:, :-1] + g[1:, 1:, 1:]).reshape(-1, 3) / 8.0
     return PolyMesh(points=np.ascontiguousarray(points, np.float64), face_off=face_off,
                     face_verts=np.ascontiguousarray(quads.ravel(), np.int32),
-                    owner=owner, neighbour=neighbour, patches=patches, n_cells=n_cells)
+                    owner=owner, neighbour=neighbour, patches=patches, n_cells=n_cells, centre_hint=hint)
 
 
 def hex_box(nx: int, ny: int, nz: int, lengths=(1.0, 1.0, 1.0), jitter: float = 0.0,
@@ -193,7 +231,7 @@ def hex_box(nx: int, ny: int, nz: int, lengths=(1.0, 1.0, 1.0), jitter: float = 
         nx, ny, nz,
         [("inlet", PATCH_PATCH, ["-x"]), ("outlet", PATCH_PATCH, ["+x"]),
          ("walls", PATCH_WALL, ["-y", "+y", "-z", "+z"])])
-    return _mesh_from_quads(pts, fv, own, nei, patches, nx * ny * nz)
+    return _mesh_from_quads(pts, fv, own, nei, patches, nx * ny * nz, pts.reshape(nz + 1, ny + 1, nx + 1, 3))
 
 
 def bump_channel(nx: int = 200, ny: int = 50, height: float = 0.1) -> PolyMesh:
@@ -215,19 +253,25 @@ def bump_channel(nx: int = 200, ny: int = 50, height: float = 0.1) -> PolyMesh:
         [("inlet", PATCH_PATCH, ["-x"]), ("outlet", PATCH_PATCH, ["+x"]),
          ("lowerWall", PATCH_WALL, ["-y"]), ("upperWall", PATCH_WALL, ["+y"]),
          ("frontAndBack", PATCH_EMPTY, ["-z", "+z"])])
-    return _mesh_from_quads(pts, fv, own, nei, patches, nx * ny)
+    return _mesh_from_quads(pts, fv, own, nei, patches, nx * ny, pts.reshape(2, ny + 1, nx + 1, 3))
 
 
 def approx_cell_centres(mesh: PolyMesh) -> np.ndarray:
-    """Average of each cell's face-vertex averages (input generation only)."""
+    """Approximate cell centres for seeding fields (input generation only):
+    vertex average for generated blocks, else the average of face-vertex
+    averages."""
+    if mesh.centre_hint is not None:
+        return mesh.centre_hint
     nF = mesh.n_faces
     fc = mesh.points[mesh.face_verts.reshape(nF, -1)].mean(axis=1) if \
         np.all(np.diff(mesh.face_off) == 4) else None
     if fc is None:
         raise ValueError("approx_cell_centres expects quad faces")
-    acc = np.zeros((mesh.n_cells, 3)); cnt = np.zeros(mesh.n_cells)
-    np.add.at(acc, mesh.owner, fc); np.add.at(cnt, mesh.owner, 1)
-    np.add.at(acc, mesh.neighbour, fc[:mesh.n_internal]); np.add.at(cnt, mesh.neighbour, 1)
+    n = mesh.n_cells
+    idx = np.concatenate([mesh.owner, mesh.neighbour])
+    cnt = np.bincount(idx, minlength=n).astype(np.float64)
+    acc = np.stack([np.bincount(idx, weights=np.concatenate([fc[:, d], fc[:mesh.n_internal, d]]), minlength=n)
+                    for d in range(3)], 1)
     return acc / cnt[:, None]
 
 

@@ -363,11 +363,26 @@ void renumberMesh(int nCells, int nFaces, int nInternal, const int* faceOff,
         a[f] = o;
         b[f] = n;
     }
-    std::vector<int> order(nInternal);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        return a[x] < a[y] || (a[x] == a[y] && b[x] < b[y]);
-    });
+    // upper-triangular order: bucket by new owner (stable), then order each
+    // bucket by new neighbour (stable insertion sort; buckets are small)
+    std::vector<int> order(nInternal), start(nCells + 1, 0);
+    for (int f = 0; f < nInternal; ++f) start[a[f] + 1]++;
+    for (int c = 0; c < nCells; ++c) start[c + 1] += start[c];
+    {
+        std::vector<int> pos(start.begin(), start.end() - 1);
+        for (int f = 0; f < nInternal; ++f) order[pos[a[f]]++] = f;
+    }
+    for (int c = 0; c < nCells; ++c) {
+        for (int i = start[c] + 1; i < start[c + 1]; ++i) {
+            const int f = order[i];
+            int j = i - 1;
+            while (j >= start[c] && b[order[j]] > b[f]) {
+                order[j + 1] = order[j];
+                --j;
+            }
+            order[j + 1] = f;
+        }
+    }
     ownerOut.assign(nFaces, 0);
     neighbourOut.assign(nInternal, 0);
     faceNewToOld.assign(nFaces, 0);
@@ -457,6 +472,69 @@ void buildSell(int nCells, int nInternal, const int* owner, const int* neighbour
             if (lowerSlot) L.slotLower[f] = (int)slot; else L.slotUpper[f] = (int)slot;
             if (colc < c) L.nLower[c]++;
         }
+    }
+}
+
+}  // namespace icsf
+
+namespace icsf {
+
+// Topology of an nx*ny*nz hexahedral block in blockMesh order: internal
+// faces upper-triangular (+x, +y, +z neighbour of each cell in cell order),
+// then the boundary faces patch by patch; sidePatch[s] gives the patch of
+// side s in (-x, +x, -y, +y, -z, +z), faces of a patch ordered side by
+// side in that order and by cell index within a side.  Face normals point
+// out of the owner.  Point index: i + (nx+1)*(j + (ny+1)*k).
+void hexBlockTopology(int nx, int ny, int nz, const int* sidePatch, int nPatches, int* faceVerts, int* owner,
+                      int* neighbour, int* patchSize) {
+    const long NX = nx + 1, NY = ny + 1;
+    auto pid = [&](long i, long j, long k) { return (int)(i + NX * (j + NY * k)); };
+    auto cid = [&](long i, long j, long k) { return (int)(i + (long)nx * (j + (long)ny * k)); };
+    long f = 0;
+    auto quad = [&](int a0, int a1, int a2, int a3) {
+        int* q = faceVerts + 4 * f;
+        q[0] = a0; q[1] = a1; q[2] = a2; q[3] = a3;
+    };
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j < ny; ++j)
+            for (long i = 0; i < nx; ++i) {
+                const int c = cid(i, j, k);
+                if (i < nx - 1) {
+                    quad(pid(i + 1, j, k), pid(i + 1, j + 1, k), pid(i + 1, j + 1, k + 1), pid(i + 1, j, k + 1));
+                    owner[f] = c; neighbour[f] = c + 1; ++f;
+                }
+                if (j < ny - 1) {
+                    quad(pid(i, j + 1, k), pid(i, j + 1, k + 1), pid(i + 1, j + 1, k + 1), pid(i + 1, j + 1, k));
+                    owner[f] = c; neighbour[f] = c + nx; ++f;
+                }
+                if (k < nz - 1) {
+                    quad(pid(i, j, k + 1), pid(i + 1, j, k + 1), pid(i + 1, j + 1, k + 1), pid(i, j + 1, k + 1));
+                    owner[f] = c; neighbour[f] = c + nx * ny; ++f;
+                }
+            }
+    for (int p = 0; p < nPatches; ++p) {
+        long before = f;
+        for (int s = 0; s < 6; ++s) {
+            if (sidePatch[s] != p) continue;
+            for (long k = 0; k < nz; ++k)
+                for (long j = 0; j < ny; ++j)
+                    for (long i = 0; i < nx; ++i) {
+                        bool on = (s == 0 && i == 0) || (s == 1 && i == nx - 1) || (s == 2 && j == 0) ||
+                                  (s == 3 && j == ny - 1) || (s == 4 && k == 0) || (s == 5 && k == nz - 1);
+                        if (!on) continue;
+                        switch (s) {
+                            case 0: quad(pid(i, j, k), pid(i, j, k + 1), pid(i, j + 1, k + 1), pid(i, j + 1, k)); break;
+                            case 1: quad(pid(i + 1, j, k), pid(i + 1, j + 1, k), pid(i + 1, j + 1, k + 1), pid(i + 1, j, k + 1)); break;
+                            case 2: quad(pid(i, j, k), pid(i + 1, j, k), pid(i + 1, j, k + 1), pid(i, j, k + 1)); break;
+                            case 3: quad(pid(i, j + 1, k), pid(i, j + 1, k + 1), pid(i + 1, j + 1, k + 1), pid(i + 1, j + 1, k)); break;
+                            case 4: quad(pid(i, j, k), pid(i, j + 1, k), pid(i + 1, j + 1, k), pid(i + 1, j, k)); break;
+                            default: quad(pid(i, j, k + 1), pid(i + 1, j, k + 1), pid(i + 1, j + 1, k + 1), pid(i, j + 1, k + 1)); break;
+                        }
+                        owner[f] = cid(i, j, k);
+                        ++f;
+                    }
+        }
+        patchSize[p] = (int)(f - before);
     }
 }
 

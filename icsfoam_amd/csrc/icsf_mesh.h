@@ -76,6 +76,10 @@ void sweepLevels(int nCells, int nInternal, const int* owner, const int* neighbo
                  std::vector<int>& fwdLevel, std::vector<int>& bwdLevel,
                  int& nFwd, int& nBwd);
 
+// blockMesh-style topology of a structured hexahedral block.
+void hexBlockTopology(int nx, int ny, int nz, const int* sidePatch, int nPatches, int* faceVerts, int* owner,
+                      int* neighbour, int* patchSize);
+
 // Sliced-ELL off-diagonal block layout.
 struct SellLayout {
     int nCells = 0, nSlices = 0;

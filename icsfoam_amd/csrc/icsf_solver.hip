@@ -585,6 +585,16 @@ int icsf_renumber_mesh(int nCells, int nFaces, int nInternalFaces, const int* fa
     return ICSF_OK;
 }
 
+int icsf_block_mesh_hex(int nx, int ny, int nz, const int* sidePatch, int nPatches, int* faceVerts, int* owner,
+                        int* neighbour, int* patchSize) {
+    if (nx < 1 || ny < 1 || nz < 1 || nPatches < 1 || !sidePatch || !faceVerts || !owner || !neighbour || !patchSize)
+        return fail(ICSF_ERR_ARG, "icsf_block_mesh_hex: bad arguments");
+    for (int s = 0; s < 6; ++s)
+        if (sidePatch[s] < 0 || sidePatch[s] >= nPatches) return fail(ICSF_ERR_ARG, "sidePatch out of range");
+    hexBlockTopology(nx, ny, nz, sidePatch, nPatches, faceVerts, owner, neighbour, patchSize);
+    return ICSF_OK;
+}
+
 int icsf_sweep_levels(int nCells, int nInternalFaces, const int* owner, const int* neighbour, int* fwdLevel,
                       int* bwdLevel, int* nFwd, int* nBwd) {
     if (nCells <= 0) return fail(ICSF_ERR_ARG, "icsf_sweep_levels: bad sizes");

@@ -51,7 +51,8 @@ def renumber_mesh(mesh: PolyMesh, new_to_old: np.ndarray) -> PolyMesh:
     sizes = np.diff(mesh.face_off)[old]
     off = np.zeros(nF + 1, np.int32)
     np.cumsum(sizes, out=off[1:])
-    return dataclasses.replace(mesh, face_off=off, face_verts=fv, owner=own, neighbour=nei)
+    hint = None if mesh.centre_hint is None else mesh.centre_hint[new_to_old]
+    return dataclasses.replace(mesh, face_off=off, face_verts=fv, owner=own, neighbour=nei, centre_hint=hint)
 
 
 def sweep_levels(mesh: PolyMesh):

@@ -81,6 +81,16 @@ int icsf_renumber_mesh(int nCells, int nFaces, int nInternalFaces, const int* fa
                        const int* owner, const int* neighbour, const int* newToOld, int* faceVertsOut,
                        int* ownerOut, int* neighbourOut, int* faceNewToOld);
 
+/* Topology of an nx*ny*nz hexahedral block in blockMesh order (the
+ * structured-mesh generator of the synthetic cases; replaces running
+ * OpenFOAM blockMesh on a one-block blockMeshDict).  sidePatch[6] assigns
+ * the sides (-x, +x, -y, +y, -z, +z) to patches 0..nPatches-1.  Outputs:
+ * faceVerts [nFaces][4], owner [nFaces], neighbour [nInternalFaces],
+ * patchSize [nPatches]; internal faces first (upper-triangular), then the
+ * patches in order.  Point index of (i,j,k) is i + (nx+1)*(j + (ny+1)*k). */
+int icsf_block_mesh_hex(int nx, int ny, int nz, const int* sidePatch, int nPatches, int* faceVerts, int* owner,
+                        int* neighbour, int* patchSize);
+
 /* Dependency levels of the forward (lower) and backward (upper) LU-SGS
  * sweeps in the current numbering (lusgs.C:233-381 visits cells in index
  * order; cells of one level are independent). */

@@ -442,3 +442,12 @@ def test_gmres_full_space_is_direct_solve():
     dW, res, it = o.solveForIncr(W, b)
     x = np.linalg.solve(A, b.T.ravel())
     assert rel_err(dW.T.ravel(), x) < 1e-9
+
+
+def test_native_block_mesh_matches_numpy_statement():
+    pf = [("inlet", 0, ["-x"]), ("outlet", 0, ["+x"]), ("walls", 1, ["-y", "+y", "-z", "+z"])]
+    a = cases._hex_topology(5, 4, 3, pf)
+    b = cases._hex_topology_numpy(5, 4, 3, pf)
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(x, y)
+    assert [(p.name, p.start, p.size) for p in a[3]] == [(p.name, p.start, p.size) for p in b[3]]
