@@ -52,9 +52,11 @@ def prepared(n: int):
     from icsfoam_amd.mesh import prepare_case
 
     t0 = time.time()
+    log(f"[bench] generating the {n}^3 case ...")
     base = make_case(n, "fp64", "CGS")
     log(f"[bench] mesh {n}^3 generated in {time.time() - t0:.1f}s")
     t0 = time.time()
+    log("[bench] permute + RCM + colour-major ...")
     base = prepare_case(base, permute_seed=2024)
     log(f"[bench] permute + RCM + colour-major in {time.time() - t0:.1f}s")
     return base

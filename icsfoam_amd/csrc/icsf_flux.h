@@ -127,6 +127,39 @@ DF void roeFace(double gI, double eFix, V3 S, double magS, double rl, double rr,
     phiE += 0.5 * magS * (rnl * Hl + rnr * Hr);
 }
 
+// HLLC flux on one face (hllcFluxScheme.C:70-240): Roe-averaged wave
+// speeds, pos(x) = x > 0 and neg(x) = x < 0 switches, all four branch terms
+// evaluated and weighted as in the reference.
+DF void hllcFace(double gI, V3 S, double magS, double rl, double rr, double pl, double pr, V3 Ul, V3 Ur, double cl,
+                 double cr, double El, double Er, double Hl, double Hr, double& phi, V3& phiU, double& phiE) {
+    const V3 n = S / magS;
+    const double coefR = sqrt(fmax(kVSMALL, rr) / fmax(kVSMALL, rl));
+    const V3 uAvg = (coefR * Ur + Ul) / (coefR + 1.0);
+    const double HAvg = (coefR * Hr + Hl) / (coefR + 1.0);
+    const double cAvg = sqrt(fabs((gI - 1.0) * (HAvg - 0.5 * magSqr(uAvg))));
+    const double ul = dot(Ul, n), ur = dot(Ur, n), ua = dot(uAvg, n);
+    const double Sl = fmin(ul - cl, ua - cAvg);
+    const double Sr = fmax(ur + cr, ua + cAvg);
+    const double Sm = (rr * ur * (Sr - ur) - rl * ul * (Sl - ul) + pl - pr) / (rr * (Sr - ur) - rl * (Sl - ul));
+    const double cSl = Sl > 0 ? 1.0 : 0.0, cSr = Sr < 0 ? 1.0 : 0.0, cSm = Sm > 0 ? 1.0 : 0.0;
+    const double cSlm = (1.0 - cSl) * cSm, cSmr = (1.0 - cSm) * (1.0 - cSr);
+    const double frl = Sm / (Sl - Sm) * ((Sl - ul) * rl);
+    const double frr = Sm / (Sr - Sm) * ((Sr - ur) * rr);
+    phi = (cSl * rl * ul + cSlm * frl + cSmr * frr + cSr * rr * ur) * magS;
+    const double psl = rl * (ul - Sl) * (ul - Sm) + pl;
+    const double psr = rr * (ur - Sr) * (ur - Sm) + pr;
+    const V3 rUsl = 1.0 / (Sl - Sm) * ((Sl - ul) * rl * Ul + (psl - pl) * n);
+    const V3 rUsr = 1.0 / (Sr - Sm) * ((Sr - ur) * rr * Ur + (psr - pr) * n);
+    const V3 fUl = Sm * rUsl + psl * n;
+    const V3 fUr = Sm * rUsr + psr * n;
+    phiU = (cSl * (rl * ul * Ul + pl * n) + cSlm * fUl + cSmr * fUr + cSr * (rr * ur * Ur + pr * n)) * magS;
+    const double rEsl = 1.0 / (Sl - Sm) * ((Sl - ul) * (rl * El) - pl * ul + psl * Sm);
+    const double rEsr = 1.0 / (Sr - Sm) * ((Sr - ur) * (rr * Er) - pr * ur + psr * Sm);
+    const double fEl = Sm * (rEsl + psl);
+    const double fEr = Sm * (rEsr + psr);
+    phiE = (cSl * rl * Hl * ul + cSlm * fEl + cSmr * fEr + cSr * rr * Hr * ur) * magS;
+}
+
 // Convective flux Jacobian of one side of a face, rows/cols ordered
 // [rho, rhoE, rhoUx, rhoUy, rhoUz] (convectiveFluxScheme.C:402-464).
 // Entries with no insertBlock in the reference are left at zero.

@@ -69,7 +69,12 @@ __global__ void k_update(MeshView m, Phys ph, StateView s, const double* __restr
     f[F_UZ] = U.z;
     f[F_E] = E;
     f[F_H] = H;
-    f[F_CS] = sqrt(2.0 * (g - 1.0) / (g + 1.0) * H);
+    if (ph.flux == FLUX_HLLC) {
+        const double psi = 1.0 / (ph.R * T);
+        f[F_CS] = fmax(sqrt(g / psi), kVSMALL);
+    } else {
+        f[F_CS] = sqrt(2.0 * (g - 1.0) / (g + 1.0) * H);
+    }
     s.T[c] = T;
     s.ecalc[c] = rE / rho - 0.5 * magSqr(U);
 }
@@ -105,7 +110,8 @@ __global__ void k_boundary(MeshView m, Phys ph, StateView s) {
     o[F_UZ] = UB.z;
     o[F_E] = E;
     o[F_H] = H;
-    o[F_CS] = sqrt(2.0 * (g - 1.0) / (g + 1.0) * H);
+    if (ph.flux == FLUX_HLLC) o[F_CS] = fmax(sqrt(g / psi), kVSMALL);
+    else o[F_CS] = sqrt(2.0 * (g - 1.0) / (g + 1.0) * H);
     s.bT[b] = TB;
 }
 
@@ -242,7 +248,7 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
                                          double& fe, V3& fu) {
     const V3 S = ld3(m.Sf + 3 * (size_t)f);
     const double ms = m.magSf[f];
-    double rL, rR, pL, pR, uxL, uxR, uyL, uyR, uzL, uzR, HL, HR, xL, xR;
+    double rL, rR, pL, pR, uxL, uxR, uyL, uyR, uzL, uzR, HL, HR, xL, xR, eL = 0, eR = 0;
     if (f < m.nI) {
         const int o = m.own[f], n = m.nei[f];
         recon(m, s, ph.recon, f, o, n, F_UX, uxL, uxR);
@@ -251,7 +257,8 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
         recon(m, s, ph.recon, f, o, n, F_P, pL, pR);
         recon(m, s, ph.recon, f, o, n, F_RHO, rL, rR);
         recon(m, s, ph.recon, f, o, n, F_H, HL, HR);
-        recon(m, s, ph.recon, f, o, n, ph.flux == FLUX_AUSM ? F_CS : F_E, xL, xR);
+        recon(m, s, ph.recon, f, o, n, ph.flux == FLUX_ROE ? F_E : F_CS, xL, xR);
+        if (ph.flux == FLUX_HLLC) recon(m, s, ph.recon, f, o, n, F_E, eL, eR);
     } else {
         const double* bf = s.bfld + (size_t)(f - m.nI) * NFLD;
         rL = rR = bf[F_RHO];
@@ -260,13 +267,16 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
         uyL = uyR = bf[F_UY];
         uzL = uzR = bf[F_UZ];
         HL = HR = bf[F_H];
-        xL = xR = bf[ph.flux == FLUX_AUSM ? F_CS : F_E];
+        xL = xR = bf[ph.flux == FLUX_ROE ? F_E : F_CS];
+        eL = eR = bf[F_E];
     }
     const V3 UL{uxL, uyL, uzL}, UR{uxR, uyR, uzR};
     if (ph.flux == FLUX_AUSM)
         ausmFace(ph.lowMach, S, ms, UL, UR, xL, xR, pL, pR, rL, rR, HL, HR, fr, fu, fe);
-    else
+    else if (ph.flux == FLUX_ROE)
         roeFace(ph.gamma, ph.entropyFix, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, HL, HR, fr, fu, fe);
+    else
+        hllcFace(ph.gamma, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, eL, eR, HL, HR, fr, fu, fe);
 }
 
 // Residual b = R*V (residualsUpdate.H) and rPseudoDeltaT
@@ -610,8 +620,8 @@ void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const d
 
 void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st) {
     if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
-    if (ph.flux == FLUX_AUSM) k_grad<8><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
-    else k_grad<7><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
+    if (ph.flux == FLUX_ROE) k_grad<7><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
+    else k_grad<8><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
     k_residual<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
 }
 

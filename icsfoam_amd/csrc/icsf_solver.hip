@@ -275,9 +275,10 @@ int readSettings(icsf_case* c, const char* thermo, const char* schemes, const ch
     const std::string fs = sd.word("convectiveFluxScheme/fluxScheme", "");
     if (fs == "AUSMPlusUp") ph.flux = FLUX_AUSM;
     else if (fs == "ROE") ph.flux = FLUX_ROE;
+    else if (fs == "HLLC") ph.flux = FLUX_HLLC;
     else
         return fail(ICSF_ERR_KEYWORD, "Unknown convectiveFluxScheme type '" + fs +
-                                          "'; valid types are AUSMPlusUp ROE");
+                                          "'; valid types are AUSMPlusUp HLLC ROE");
     ph.lowMach = sd.sw("convectiveFluxScheme/lowMachAusm", true) ? 1 : 0;
     ph.entropyFix = sd.scalar("convectiveFluxScheme/entropyFixCoeff", 0.05);
     const auto* ddt = sd.find("ddtSchemes/default");

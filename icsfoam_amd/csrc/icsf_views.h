@@ -7,8 +7,10 @@
 namespace icsf {
 
 // reconstructed cell fields, AoS [nCells][NFLD]
+// F_CS holds the scheme's reconstructed speed: c* (AUSM+up) or the sound
+// speed max(sqrt(gamma/psi), VSMALL) (HLLC); unused by Roe.
 enum { F_RHO = 0, F_P = 1, F_UX = 2, F_UY = 3, F_UZ = 4, F_E = 5, F_H = 6, F_CS = 7, NFLD = 8 };
-enum { FLUX_AUSM = 0, FLUX_ROE = 1 };
+enum { FLUX_AUSM = 0, FLUX_ROE = 1, FLUX_HLLC = 2 };
 enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2 };
 enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2 };
 constexpr int NB = 25;  // entries of a 5x5 block

@@ -127,7 +127,7 @@ int icsf_mesh_info(const icsf_mesh* mesh, int* info);
 
 /* Dictionaries in OpenFOAM syntax: thermophysicalProperties (hePsiThermo,
  * hConst, perfectGas, const transport; keywords R, gamma, mu, Pr), fvSchemes
- * (convectiveFluxScheme{fluxScheme AUSMPlusUp|ROE; lowMachAusm;
+ * (convectiveFluxScheme{fluxScheme AUSMPlusUp|ROE|HLLC; lowMachAusm;
  * entropyFixCoeff}, ddtSchemes, gradSchemes, interpolationSchemes
  * reconstruct(rho|U|T)) and fvSolution (flowSolver{solver GMRES|FGMRES;
  * <solver>{preconditioner LUSGS|Jacobi|none; nDirections; maxIter; minIter;

@@ -118,7 +118,7 @@ inline T9 dev2(const T9& A) {
 
 enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2 };
 enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2 };
-enum { FLUX_AUSM = 0, FLUX_ROE = 1 };
+enum { FLUX_AUSM = 0, FLUX_ROE = 1, FLUX_HLLC = 2 };
 
 // ---------------------------------------------------------------- mesh ----
 struct Mesh {
@@ -434,6 +434,7 @@ inline double interpLinear(const Mesh& m, int f, const double* vc, int stride) {
 // Derived cell/boundary fields used by the flux schemes.
 struct Derived {
     vec E, H, cstar, Eb, Hb, cstarb, c, cb;  // c = sqrt(gamma/psi)
+    vec chl, chlb;                            // HLLC: max(sqrt(gamma/psi), VSMALL)
 };
 void derive(const Case& k, Derived& d) {
     const Mesh& m = *k.m;
@@ -441,12 +442,14 @@ void derive(const Case& k, Derived& d) {
     const int nB = m.nF - m.nI;
     d.E.resize(m.nC); d.H.resize(m.nC); d.cstar.resize(m.nC); d.c.resize(m.nC);
     d.Eb.assign(nB, 0); d.Hb.assign(nB, 0); d.cstarb.assign(nB, 0); d.cb.assign(nB, 0);
+    d.chl.resize(m.nC); d.chlb.assign(nB, 0);
     for (int c = 0; c < m.nC; ++c) {
         d.E[c] = k.Cv * k.T[c] + 0.5 * magSqr(v3(&k.U[3 * c]));
         d.H[c] = std::max(d.E[c], SMALL) + std::max(k.p[c] / k.rho[c], SMALL);
         d.cstar[c] = std::sqrt(2.0 * (g - 1.0) / (g + 1.0) * d.H[c]);
         const double psi = 1.0 / (k.s.R * k.T[c]);
         d.c[c] = std::sqrt(g / psi);
+        d.chl[c] = std::max(d.c[c], VSMALL);
     }
     for (int b = 0; b < nB; ++b) {
         if (k.TB[b] == 0) continue;  // empty
@@ -455,6 +458,7 @@ void derive(const Case& k, Derived& d) {
         d.cstarb[b] = std::sqrt(2.0 * (g - 1.0) / (g + 1.0) * d.Hb[b]);
         const double psi = 1.0 / (k.s.R * k.TB[b]);
         d.cb[b] = std::sqrt(g / psi);
+        d.chlb[b] = std::max(d.cb[b], VSMALL);
     }
 }
 
@@ -589,6 +593,39 @@ void roeFace(const Case& k, V3 S, double magS, double rl, double rr, double pl, 
     phiE += 0.5 * magS * (rnl * Hl + rnr * Hr);
 }
 
+// hllcFluxScheme::calcFlux (hllcFluxScheme.C:70-240), one face; wave
+// speeds from Roe averages, switches pos(x) = x > 0, neg(x) = x < 0.
+void hllcFace(const Case& k, V3 S, double magS, double rl, double rr, double pl, double pr, V3 Ul, V3 Ur,
+              double cl, double cr, double El, double Er, double Hl, double Hr, double& phi, V3& phiU,
+              double& phiE) {
+    const V3 n = S / magS;
+    const double coefR = std::sqrt(std::max(VSMALL, rr) / std::max(VSMALL, rl));
+    const V3 uAvg = (coefR * Ur + Ul) / (coefR + 1.0);
+    const double HAvg = (coefR * Hr + Hl) / (coefR + 1.0);
+    const double cAvg = std::sqrt(std::fabs((k.s.gamma - 1.0) * (HAvg - 0.5 * magSqr(uAvg))));
+    const double ul = dot(Ul, n), ur = dot(Ur, n), ua = dot(uAvg, n);
+    const double Sl = std::min(ul - cl, ua - cAvg);
+    const double Sr = std::max(ur + cr, ua + cAvg);
+    const double Sm = (rr * ur * (Sr - ur) - rl * ul * (Sl - ul) + pl - pr) / (rr * (Sr - ur) - rl * (Sl - ul));
+    const double cSl = Sl > 0 ? 1.0 : 0.0, cSr = Sr < 0 ? 1.0 : 0.0, cSm = Sm > 0 ? 1.0 : 0.0;
+    const double cSlm = (1.0 - cSl) * cSm, cSmr = (1.0 - cSm) * (1.0 - cSr);
+    const double frl = Sm / (Sl - Sm) * ((Sl - ul) * rl);
+    const double frr = Sm / (Sr - Sm) * ((Sr - ur) * rr);
+    phi = (cSl * rl * ul + cSlm * frl + cSmr * frr + cSr * rr * ur) * magS;
+    const double psl = rl * (ul - Sl) * (ul - Sm) + pl;
+    const double psr = rr * (ur - Sr) * (ur - Sm) + pr;
+    const V3 rUsl = 1.0 / (Sl - Sm) * ((Sl - ul) * rl * Ul + (psl - pl) * n);
+    const V3 rUsr = 1.0 / (Sr - Sm) * ((Sr - ur) * rr * Ur + (psr - pr) * n);
+    const V3 fUl = Sm * rUsl + psl * n;
+    const V3 fUr = Sm * rUsr + psr * n;
+    phiU = (cSl * (rl * ul * Ul + pl * n) + cSlm * fUl + cSmr * fUr + cSr * (rr * ur * Ur + pr * n)) * magS;
+    const double rEsl = 1.0 / (Sl - Sm) * ((Sl - ul) * (rl * El) - pl * ul + psl * Sm);
+    const double rEsr = 1.0 / (Sr - Sm) * ((Sr - ur) * (rr * Er) - pr * ur + psr * Sm);
+    const double fEl = Sm * (rEsl + psl);
+    const double fEr = Sm * (rEsr + psr);
+    phiE = (cSl * rl * Hl * ul + cSlm * fEl + cSmr * fEr + cSr * rr * Hr * ur) * magS;
+}
+
 // convectiveFluxScheme::calcFlux for the selected scheme on all faces.
 void calcFlux(Case& k, vec& phi, vec& phiU, vec& phiE) {
     Mesh& m = *k.m;
@@ -607,6 +644,7 @@ void calcFlux(Case& k, vec& phi, vec& phiU, vec& phiE) {
     reconstruct(k, d.H.data(), d.Hb.data(), 1, HL, HR);
     if (k.s.flux == FLUX_AUSM) reconstruct(k, d.cstar.data(), d.cstarb.data(), 1, cL, cR);
     else reconstruct(k, d.E.data(), d.Eb.data(), 1, EL, ER);
+    if (k.s.flux == FLUX_HLLC) reconstruct(k, d.chl.data(), d.chlb.data(), 1, cL, cR);
     phi.assign(m.nF, 0);
     phiU.assign(3 * m.nF, 0);
     phiE.assign(m.nF, 0);
@@ -618,8 +656,11 @@ void calcFlux(Case& k, vec& phi, vec& phiU, vec& phiE) {
         V3 pu;
         if (k.s.flux == FLUX_AUSM)
             ausmFace(k, S, m.magSf[f], UL, UR, cL[f], cR[f], pL[f], pR[f], rL[f], rR[f], HL[f], HR[f], ph, pu, pe);
-        else
+        else if (k.s.flux == FLUX_ROE)
             roeFace(k, S, m.magSf[f], rL[f], rR[f], pL[f], pR[f], UL, UR, EL[f], ER[f], HL[f], HR[f], ph, pu, pe);
+        else
+            hllcFace(k, S, m.magSf[f], rL[f], rR[f], pL[f], pR[f], UL, UR, cL[f], cR[f], EL[f], ER[f], HL[f], HR[f],
+                     ph, pu, pe);
         phi[f] = ph;
         phiU[3 * f] = pu.x; phiU[3 * f + 1] = pu.y; phiU[3 * f + 2] = pu.z;
         phiE[f] = pe;
@@ -1904,7 +1945,13 @@ void orc_face_flux(int scheme, int lowMach, double eFix, double R, double gamma,
     double ph, pe;
     V3 pu;
     if (scheme == FLUX_AUSM) ausmFace(k, S, magS, U[0], U[1], cs[0], cs[1], p[0], p[1], rho[0], rho[1], H[0], H[1], ph, pu, pe);
-    else roeFace(k, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], E[0], E[1], H[0], H[1], ph, pu, pe);
+    else if (scheme == FLUX_ROE) roeFace(k, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], E[0], E[1], H[0], H[1], ph, pu, pe);
+    else {
+        double ch[2];
+        // sqrt(gamma/psi) with psi = 1/(R T)
+        for (int q = 0; q < 2; ++q) ch[q] = std::max(std::sqrt(gamma / (1.0 / (R * T[q]))), VSMALL);
+        hllcFace(k, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], ch[0], ch[1], E[0], E[1], H[0], H[1], ph, pu, pe);
+    }
     out[0] = ph; out[1] = pu.x; out[2] = pu.y; out[3] = pu.z; out[4] = pe;
 }
 
@@ -1924,7 +1971,7 @@ void orc_reconstruct(void* h, int field, double* L, double* R) {
             case 2: case 3: case 4: vc[c] = k.U[3 * c + field - 2]; break;
             case 5: vc[c] = d.E[c]; break;
             case 6: vc[c] = d.H[c]; break;
-            default: vc[c] = d.cstar[c]; break;
+            default: vc[c] = k.s.flux == FLUX_HLLC ? d.chl[c] : d.cstar[c]; break;
         }
     }
     for (int b = 0; b < nB; ++b) {
@@ -1934,7 +1981,7 @@ void orc_reconstruct(void* h, int field, double* L, double* R) {
             case 2: case 3: case 4: vb[b] = k.UB[3 * b + field - 2]; break;
             case 5: vb[b] = d.Eb[b]; break;
             case 6: vb[b] = d.Hb[b]; break;
-            default: vb[b] = d.cstarb[b]; break;
+            default: vb[b] = k.s.flux == FLUX_HLLC ? d.chlb[b] : d.cstarb[b]; break;
         }
     }
     vec l, r;

@@ -90,7 +90,7 @@ def parse_settings(thermo: str, schemes: str, solution: str):
     lim = 1 if "Venkatakrishnan" in schemes else (2 if "cellLimited<minmod>" in schemes else 0)
     mu = num(thermo, "mu", 0.0)
     precond = {"LUSGS": 0, "Jacobi": 1}[word(solution, "preconditioner", "LUSGS")]
-    iset = np.array([0 if flux == "AUSMPlusUp" else 1, int(lowMach), recon, lim, int(mu > 0), precond,
+    iset = np.array([{"AUSMPlusUp": 0, "ROE": 1, "HLLC": 2}[flux], int(lowMach), recon, lim, int(mu > 0), precond,
                      int(num(solution, "nDirections", 30)), int(num(solution, "maxIter", 1000)),
                      int(num(solution, "minIter", 0)), 1, 1], np.int32)
     co = num(solution, "pseudoCoNum", 1.0)
@@ -130,7 +130,7 @@ def face_flux(scheme, S, left, right, lowMach=True, eFix=0.05, R=287.0, gamma=1.
     """Flux through one face; left/right = (rho, p, Ux, Uy, Uz, T)."""
     inp = np.ascontiguousarray(np.concatenate([S, left, right]), np.float64)
     out = np.empty(5)
-    lib().orc_face_flux({"AUSMPlusUp": 0, "ROE": 1}[scheme], int(lowMach), eFix, R, gamma, _p(inp), _p(out))
+    lib().orc_face_flux({"AUSMPlusUp": 0, "ROE": 1, "HLLC": 2}[scheme], int(lowMach), eFix, R, gamma, _p(inp), _p(out))
     return out
 
 

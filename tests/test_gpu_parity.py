@@ -23,6 +23,8 @@ CASES = {
     "bump_roe": lambda: bump_small(flux="ROE"),
     "box_roe_viscous": lambda: box_small(),
     "box_ausm_viscous": lambda: box_small(flux="AUSMPlusUp"),
+    "bump_hllc": lambda: bump_small(flux="HLLC"),
+    "box_hllc_viscous": lambda: box_small(flux="HLLC"),
 }
 
 

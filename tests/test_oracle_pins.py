@@ -160,7 +160,7 @@ def _oflux(scheme, S, L, Rs):
     return np.array([o[0], o[4], o[1], o[2], o[3]])
 
 
-@pytest.mark.parametrize("scheme", ["AUSMPlusUp", "ROE"])
+@pytest.mark.parametrize("scheme", ["AUSMPlusUp", "ROE", "HLLC"])
 def test_flux_consistency(scheme):
     rng = np.random.default_rng(0)
     for _ in range(50):
@@ -172,7 +172,7 @@ def test_flux_consistency(scheme):
         assert rel_err(f, ref) < 2e-13
 
 
-@pytest.mark.parametrize("scheme", ["AUSMPlusUp", "ROE"])
+@pytest.mark.parametrize("scheme", ["AUSMPlusUp", "ROE", "HLLC"])
 def test_flux_supersonic_upwinding(scheme):
     rng = np.random.default_rng(1)
     for sgn in (1, -1):
@@ -186,7 +186,7 @@ def test_flux_supersonic_upwinding(scheme):
             assert rel_err(f, euler_flux(S, up[0], up[1], up[2])) < 1e-12
 
 
-@pytest.mark.parametrize("scheme", ["AUSMPlusUp", "ROE"])
+@pytest.mark.parametrize("scheme", ["AUSMPlusUp", "ROE", "HLLC"])
 def test_flux_mirror_symmetry_and_rotation(scheme):
     rng = np.random.default_rng(2)
     Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
@@ -249,7 +249,7 @@ def test_limited_reconstruction_stays_within_neighbour_bounds():
 
 
 # ---------------------------------------------------------------- residual --
-@pytest.mark.parametrize("flux", ["AUSMPlusUp", "ROE"])
+@pytest.mark.parametrize("flux", ["AUSMPlusUp", "ROE", "HLLC"])
 def test_freestream_preservation_jittered(flux):
     c = cases.case_box(n=(7, 6, 5), jitter=0.15, flux=flux, mu=1.8e-5, p_jump=1.0, noise=0.0,
                        mach=(0.6, 0.6))
