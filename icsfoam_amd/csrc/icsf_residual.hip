@@ -279,96 +279,143 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
         hllcFace(ph.gamma, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, eL, eR, HL, HR, fr, fu, fe);
 }
 
+// Face terms, one thread per face (each face evaluated once):
+//   FT_CONV..+4  convective flux (rho, rhoE, rhoUx, rhoUy, rhoUz)
+//   FT_LAP..+2   |S| mu_f snGrad(U)            (laplacian(muEff,U))
+//   FT_TAU..+2   Sf & tauMC_f                  (div(tauMC))
+//   FT_SIG       (mu_f grad(U)_f + tauMC_f) & U_f & Sf (div(sigmaDotU & Sf))
+//   FT_LAPE      |S| alpha_f snGrad(e)         (laplacian(alphaEff,e))
+//   FT_LAM       lambda_f = interp(c) + |interp(U).n| (internal) or
+//                c_b + |U_b.n| (boundary)
+//   FT_FRDT      nonOrthDeltaCoeffs*lambda (internal), wall-face
+//                0.5*nonOrthDeltaCoeffs*(c_P + |U_P.n|), else -1
+//   FT_REC..+7   reconstructed Ux,Uy,Uz,E on the owner side, then the
+//                neighbour side (internal faces)
+//   FT_SV        0.5*lambdaVisc*|S|*deltaCoeffs (internal, viscous)
+// Cell kernels then sum these in face order (the reference's face loops).
+__global__ void __launch_bounds__(256) k_face_terms(MeshView m, Phys ph, StateView s) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nF) return;
+    if (f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY) return;
+    double* o = s.ft + (size_t)f * NFT;
+    double fr, fe;
+    V3 fu;
+    faceFlux(m, ph, s, f, fr, fe, fu);
+    o[FT_CONV] = fr;
+    o[FT_CONV + 1] = fe;
+    o[FT_CONV + 2] = fu.x;
+    o[FT_CONV + 3] = fu.y;
+    o[FT_CONV + 4] = fu.z;
+    const V3 S = ld3(m.Sf + 3 * (size_t)f);
+    const double ms = m.magSf[f];
+    const bool visc = ph.viscous;
+    const double mu = ph.mu, alpha = ph.alpha;
+    if (f < m.nI) {
+        const int ow = m.own[f], ne = m.nei[f];
+        const double w = m.w[f];
+        const double co = soundSpeed(ph, s.T[ow]), cn = soundSpeed(ph, s.T[ne]);
+        const double cf = w * (co - cn) + cn;
+        const V3 Uo = cellU(s.fld, ow), Un = cellU(s.fld, ne);
+        const V3 Uf = w * (Uo - Un) + Un;
+        const double lam = cf + fabs(dot(Uf, S / ms));
+        o[FT_LAM] = lam;
+        o[FT_FRDT] = m.nodc[f] * lam;
+        double uxL, uxR, uyL, uyR, uzL, uzR, EL, ER;
+        recon(m, s, ph.recon, f, ow, ne, F_UX, uxL, uxR);
+        recon(m, s, ph.recon, f, ow, ne, F_UY, uyL, uyR);
+        recon(m, s, ph.recon, f, ow, ne, F_UZ, uzL, uzR);
+        recon(m, s, ph.recon, f, ow, ne, F_E, EL, ER);
+        o[FT_REC] = uxL; o[FT_REC + 1] = uyL; o[FT_REC + 2] = uzL; o[FT_REC + 3] = EL;
+        o[FT_REC + 4] = uxR; o[FT_REC + 5] = uyR; o[FT_REC + 6] = uzR; o[FT_REC + 7] = ER;
+        if (visc) {
+            const double* g9o = s.gU + (size_t)ow * 9;
+            const double* g9n = s.gU + (size_t)ne * 9;
+            T9 gUo, gUn;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) { gUo.t[i] = g9o[i]; gUn.t[i] = g9n[i]; }
+            const T9 gUf = w * (gUo - gUn) + gUn;
+            const T9 tauO = mu * dev2(transposeT(gUo));
+            const T9 tauN = mu * dev2(transposeT(gUn));
+            const T9 tauf = w * (tauO - tauN) + tauN;
+            const double muf = w * (mu - mu) + mu;
+            const double alf = w * (alpha - alpha) + alpha;
+            const V3 k3 = ld3(m.corr + 3 * (size_t)f);
+            const V3 sn = m.nodc[f] * (Un - Uo) + vdott(k3, gUf);
+            const V3 lap = (muf * sn) * ms;
+            const V3 dt = vdott(S, tauf);
+            const V3 sig = tdotv(muf * gUf + tauf, Uf);
+            const V3 geO = ld3(s.ge + 3 * (size_t)ow), geN = ld3(s.ge + 3 * (size_t)ne);
+            const V3 gef = w * (geO - geN) + geN;
+            const double sne = m.nodc[f] * (s.ecalc[ne] - s.ecalc[ow]) + dot(k3, gef);
+            o[FT_LAP] = lap.x; o[FT_LAP + 1] = lap.y; o[FT_LAP + 2] = lap.z;
+            o[FT_TAU] = dt.x; o[FT_TAU + 1] = dt.y; o[FT_TAU + 2] = dt.z;
+            o[FT_SIG] = dot(sig, S);
+            o[FT_LAPE] = (alf * sne) * ms;
+            const double rf = w * (s.fld[(size_t)ow * NFLD + F_RHO] - s.fld[(size_t)ne * NFLD + F_RHO]) +
+                              s.fld[(size_t)ne * NFLD + F_RHO];
+            const double lv = (muf + alf) / rf;
+            o[FT_SV] = 0.5 * lv * ms * m.dc[f];
+        }
+    } else {
+        const int b = f - m.nI;
+        const int c = m.own[f];
+        const double* bf = s.bfld + (size_t)b * NFLD;
+        const V3 UB{bf[F_UX], bf[F_UY], bf[F_UZ]};
+        o[FT_LAM] = soundSpeed(ph, s.bT[b]) + fabs(dot(UB, S / ms));
+        o[FT_FRDT] = m.bKind[b] == PK_WALL
+                         ? 0.5 * m.nodc[f] * (soundSpeed(ph, s.T[c]) + fabs(dot(cellU(s.fld, c), S / ms)))
+                         : -1.0;
+        if (visc) {
+            const V3 n = S / ms;
+            const V3 Uc = cellU(s.fld, c);
+            V3 snU;
+            const int ut = m.bUT[b];
+            if (ut == BC_ZG) snU = V3{0, 0, 0};
+            else if (ut == BC_SLIP) snU = (reflect(n, Uc) - Uc) * (m.dc[f] / 2.0);
+            else snU = m.dc[f] * (UB - Uc);
+            T9 gUc;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) gUc.t[i] = s.gU[(size_t)c * 9 + i];
+            const T9 gUb = gUc + outer(n, snU - vdott(n, gUc));
+            const T9 taub = mu * dev2(transposeT(gUb));
+            const V3 lap = (mu * snU) * ms;
+            const V3 dt = vdott(S, taub);
+            const V3 sig = tdotv(mu * gUb + taub, UB);
+            const double rhoB = bf[F_RHO];
+            const double rhoEB = rhoB * (ph.Cv * s.bT[b] + 0.5 * magSqr(UB));
+            const double eb = rhoEB / rhoB - 0.5 * magSqr(UB);
+            o[FT_LAP] = lap.x; o[FT_LAP + 1] = lap.y; o[FT_LAP + 2] = lap.z;
+            o[FT_TAU] = dt.x; o[FT_TAU + 1] = dt.y; o[FT_TAU + 2] = dt.z;
+            o[FT_SIG] = dot(sig, S);
+            o[FT_LAPE] = (alpha * (m.dc[f] * (eb - s.ecalc[c]))) * ms;
+        }
+    }
+}
+
 // Residual b = R*V (residualsUpdate.H) and rPseudoDeltaT
-// (setCoAndDeltaT.H:57-141) for each cell.
-__global__ void __launch_bounds__(256) k_residual(MeshView m, Phys ph, StateView s) {
+// (setCoAndDeltaT.H:57-141) of each cell from the face terms.
+__global__ void __launch_bounds__(256) k_cell_residual(MeshView m, Phys ph, StateView s) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     double ar = 0, ae = 0;
-    V3 au{0, 0, 0};
-    V3 aLap{0, 0, 0}, aTau{0, 0, 0};
-    double aSig = 0, aLapE = 0;
-    double rdt = 0.0;
+    V3 au{0, 0, 0}, aLap{0, 0, 0}, aTau{0, 0, 0};
+    double aSig = 0, aLapE = 0, rdt = 0.0;
     const bool visc = ph.viscous;
-    const double mu = ph.mu, alpha = ph.alpha;
     const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
-        double fr, fe;
-        V3 fu;
-        faceFlux(m, ph, s, f, fr, fe, fu);
-        const V3 S = ld3(m.Sf + 3 * (size_t)f);
-        const double ms = m.magSf[f];
-        if (f < m.nI) {
-            const int o = m.own[f], n = m.nei[f];
-            const bool isOwn = (o == c);
-            if (isOwn) { ar += fr; ae += fe; au = au + fu; }
-            else { ar -= fr; ae -= fe; au = au - fu; }
-            // local time step: frdt = nonOrthDeltaCoeffs*lambda
-            const double w = m.w[f];
-            const double co = soundSpeed(ph, s.T[o]), cn = soundSpeed(ph, s.T[n]);
-            const double cf = w * (co - cn) + cn;
-            const V3 Uo = cellU(s.fld, o), Un = cellU(s.fld, n);
-            const V3 Uf = w * (Uo - Un) + Un;
-            const double lam = cf + fabs(dot(Uf, S / ms));
-            rdt = fmax(rdt, m.nodc[f] * lam);
-            if (visc) {
-                const double* g9o = s.gU + (size_t)o * 9;
-                const double* g9n = s.gU + (size_t)n * 9;
-                T9 gUo, gUn;
-#pragma unroll
-                for (int i = 0; i < 9; ++i) { gUo.t[i] = g9o[i]; gUn.t[i] = g9n[i]; }
-                const T9 gUf = w * (gUo - gUn) + gUn;
-                const T9 tauO = mu * dev2(transposeT(gUo));
-                const T9 tauN = mu * dev2(transposeT(gUn));
-                const T9 tauf = w * (tauO - tauN) + tauN;
-                const double muf = w * (mu - mu) + mu;
-                const double alf = w * (alpha - alpha) + alpha;
-                const V3 k3 = ld3(m.corr + 3 * (size_t)f);
-                const V3 sn = m.nodc[f] * (Un - Uo) + vdott(k3, gUf);
-                const V3 lap = (muf * sn) * ms;
-                const V3 dt = vdott(S, tauf);
-                const V3 sig = tdotv(muf * gUf + tauf, Uf);
-                const double sdu = dot(sig, S);
-                const V3 geO = ld3(s.ge + 3 * (size_t)o), geN = ld3(s.ge + 3 * (size_t)n);
-                const V3 gef = w * (geO - geN) + geN;
-                const double sne = m.nodc[f] * (s.ecalc[n] - s.ecalc[o]) + dot(k3, gef);
-                const double lapE = (alf * sne) * ms;
-                if (isOwn) { aLap = aLap + lap; aTau = aTau + dt; aSig += sdu; aLapE += lapE; }
-                else { aLap = aLap - lap; aTau = aTau - dt; aSig -= sdu; aLapE -= lapE; }
-            }
-        } else {
-            const int b = f - m.nI;
-            ar += fr; ae += fe; au = au + fu;
-            if (m.bKind[b] == PK_WALL) {
-                const double pl = 0.5 * m.nodc[f] * (soundSpeed(ph, s.T[c]) + fabs(dot(cellU(s.fld, c), S / ms)));
-                rdt = fmax(rdt, pl);
-            }
-            if (visc) {
-                const double* bf = s.bfld + (size_t)b * NFLD;
-                const V3 n = S / ms;
-                const V3 Uc = cellU(s.fld, c);
-                const V3 UB{bf[F_UX], bf[F_UY], bf[F_UZ]};
-                V3 snU;
-                const int ut = m.bUT[b];
-                if (ut == BC_ZG) snU = V3{0, 0, 0};
-                else if (ut == BC_SLIP) snU = (reflect(n, Uc) - Uc) * (m.dc[f] / 2.0);
-                else snU = m.dc[f] * (UB - Uc);
-                T9 gUc;
-#pragma unroll
-                for (int i = 0; i < 9; ++i) gUc.t[i] = s.gU[(size_t)c * 9 + i];
-                const T9 gUb = gUc + outer(n, snU - vdott(n, gUc));
-                const T9 taub = mu * dev2(transposeT(gUb));
-                const V3 lap = (mu * snU) * ms;
-                const V3 dt = vdott(S, taub);
-                const V3 sig = tdotv(mu * gUb + taub, UB);
-                const double sdu = dot(sig, S);
-                const double rhoB = bf[F_RHO];
-                const double rhoEB = rhoB * (ph.Cv * s.bT[b] + 0.5 * magSqr(UB));
-                const double eb = rhoEB / rhoB - 0.5 * magSqr(UB);
-                const double lapE = (alpha * (m.dc[f] * (eb - s.ecalc[c]))) * ms;
-                aLap = aLap + lap; aTau = aTau + dt; aSig += sdu; aLapE += lapE;
-            }
+        const double* t = s.ft + (size_t)f * NFT;
+        const double fr = t[FT_CONV], fe = t[FT_CONV + 1];
+        const V3 fu{t[FT_CONV + 2], t[FT_CONV + 3], t[FT_CONV + 4]};
+        const bool plus = f >= m.nI || m.own[f] == c;
+        if (plus) { ar += fr; ae += fe; au = au + fu; }
+        else { ar -= fr; ae -= fe; au = au - fu; }
+        if (t[FT_FRDT] >= 0) rdt = fmax(rdt, t[FT_FRDT]);
+        if (visc) {
+            const V3 lap{t[FT_LAP], t[FT_LAP + 1], t[FT_LAP + 2]};
+            const V3 dt{t[FT_TAU], t[FT_TAU + 1], t[FT_TAU + 2]};
+            if (plus) { aLap = aLap + lap; aTau = aTau + dt; aSig += t[FT_SIG]; aLapE += t[FT_LAPE]; }
+            else { aLap = aLap - lap; aTau = aTau - dt; aSig -= t[FT_SIG]; aLapE -= t[FT_LAPE]; }
         }
     }
     const double Vc = m.V[c];
@@ -390,24 +437,16 @@ __global__ void __launch_bounds__(256) k_residual(MeshView m, Phys ph, StateView
     s.rDT[c] = rdt / s.pseudoCo[c];
 }
 
-// Face fluxes of every face (convectiveFluxScheme::calcFlux output), for
-// parity checks; fluxes of empty faces are zero.
-__global__ void k_face_flux(MeshView m, Phys ph, StateView s, double* phi, double* phiU, double* phiE) {
+// Face fluxes of every face (convectiveFluxScheme::calcFlux output) from
+// the face terms; fluxes of empty faces are zero.
+__global__ void k_face_flux(MeshView m, StateView s, double* phi, double* phiU, double* phiE) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
-    if (f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY) {
-        phi[f] = 0; phiE[f] = 0;
-        phiU[3 * (size_t)f] = phiU[3 * (size_t)f + 1] = phiU[3 * (size_t)f + 2] = 0;
-        return;
-    }
-    double fr, fe;
-    V3 fu;
-    faceFlux(m, ph, s, f, fr, fe, fu);
-    phi[f] = fr;
-    phiE[f] = fe;
-    phiU[3 * (size_t)f] = fu.x;
-    phiU[3 * (size_t)f + 1] = fu.y;
-    phiU[3 * (size_t)f + 2] = fu.z;
+    const bool empty = f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY;
+    const double* t = s.ft + (size_t)f * NFT;
+    phi[f] = empty ? 0.0 : t[FT_CONV];
+    phiE[f] = empty ? 0.0 : t[FT_CONV + 1];
+    for (int d = 0; d < 3; ++d) phiU[3 * (size_t)f + d] = empty ? 0.0 : t[FT_CONV + 2 + d];
 }
 
 // Row assembly of the coupled Jacobian: off-diagonal 5x5 blocks of cell c
@@ -417,11 +456,13 @@ __global__ void k_face_flux(MeshView m, Phys ph, StateView s, double* phi, doubl
 // (convectiveFluxScheme.C:219-546), viscousFluxScheme::addFluxTerms
 // (viscousFluxScheme.C:220-247), blockFvMatrix::insertBlock /
 // insertDissipationBlock (blockFvMatrix.C:212-326), lusgs::lusgs
-// (lusgs.C:50-125).
+// (lusgs.C:50-125).  The diagonal accumulates the convective part, the
+// dissipation and the viscous parts separately and combines them in the
+// order the reference adds its temporary matrices.
 template <typename TB>
-__global__ void __launch_bounds__(256) k_jacobian(MeshView m, Phys ph, StateView s, TB* __restrict__ blk,
-                                                  TB* __restrict__ dg, double* __restrict__ rD,
-                                                  double* __restrict__ dgOut64, int* __restrict__ err) {
+__global__ void __launch_bounds__(256) k_cell_jacobian(MeshView m, Phys ph, StateView s, TB* __restrict__ blk,
+                                                       TB* __restrict__ dg, double* __restrict__ rD,
+                                                       int* __restrict__ err) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const double g = ph.gamma;
@@ -430,72 +471,50 @@ __global__ void __launch_bounds__(256) k_jacobian(MeshView m, Phys ph, StateView
     for (int e = 0; e < NB; ++e) D[e] = 0.0;
     double diss = 0.0, vis = 0.0;
     const bool visc = ph.viscous;
-    const double mu = ph.mu, alpha = ph.alpha;
     const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
     const int sl = c >> 6, lane = c & 63;
-    const size_t rowBase = (size_t)m.slicePtr[sl] * 64 + lane;
+    const size_t rowSlot0 = (size_t)m.slicePtr[sl];
     int slot = 0;
-    // internal faces: off-diagonal blocks + convective part of the diagonal
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
         if (f >= m.nI) continue;
-        const int o = m.own[f], n = m.nei[f];
-        const bool isOwn = (o == c);
+        const bool isOwn = (m.own[f] == c);
+        const double* t = s.ft + (size_t)f * NFT;
         const V3 S = ld3(m.Sf + 3 * (size_t)f);
         const double ms = m.magSf[f];
         const V3 nrm = S / ms;
-        double uxL, uxR, uyL, uyR, uzL, uzR, EL, ER;
-        recon(m, s, ph.recon, f, o, n, F_UX, uxL, uxR);
-        recon(m, s, ph.recon, f, o, n, F_UY, uyL, uyR);
-        recon(m, s, ph.recon, f, o, n, F_UZ, uzL, uzR);
-        recon(m, s, ph.recon, f, o, n, F_E, EL, ER);
-        // own side needs: block = upper (A_R), diagonal -= lower (A_L);
+        // own side: block = upper (A_R), diagonal -= lower (A_L);
         // neighbour side: block = lower (A_L), diagonal -= upper (A_R)
+        const int bs = isOwn ? FT_REC + 4 : FT_REC;
+        const int ds = isOwn ? FT_REC : FT_REC + 4;
         double A[NB];
-        fluxJacobian(g, isOwn ? V3{uxR, uyR, uzR} : V3{uxL, uyL, uzL}, isOwn ? ER : EL, nrm, A);
+        fluxJacobian(g, V3{t[bs], t[bs + 1], t[bs + 2]}, t[bs + 3], nrm, A);
         const double hb = isOwn ? 0.5 * ms : -0.5 * ms;
-        double Bk[NB];
+        const size_t base = ((rowSlot0 + slot) * NB) * 64 + lane;
+        const double v = 0.5 * ms * t[FT_LAM];
+        const double sv = visc ? t[FT_SV] : 0.0;
 #pragma unroll
-        for (int e = 0; e < NB; ++e) Bk[e] = hb * A[e];
-        fluxJacobian(g, isOwn ? V3{uxL, uyL, uzL} : V3{uxR, uyR, uzR}, isOwn ? EL : ER, nrm, A);
+        for (int e = 0; e < NB; ++e) {
+            double x = hb * A[e];
+            if (e == 0 || e == 6 || e == 12 || e == 18 || e == 24) {
+                x = x - v;
+                if (visc) x = x - sv;
+            }
+            blk[base + (size_t)e * 64] = (TB)x;
+        }
+        fluxJacobian(g, V3{t[ds], t[ds + 1], t[ds + 2]}, t[ds + 3], nrm, A);
         const double hd = isOwn ? -0.5 * ms : 0.5 * ms;
 #pragma unroll
         for (int e = 0; e < NB; ++e) D[e] -= hd * A[e];
-        // dissipation (insertDissipationBlock, then operator-=)
-        const double w = m.w[f];
-        const double co = soundSpeed(ph, s.T[o]), cn = soundSpeed(ph, s.T[n]);
-        const double cfc = w * (co - cn) + cn;
-        const V3 Uo = cellU(s.fld, o), Un = cellU(s.fld, n);
-        const V3 Uf = w * (Uo - Un) + Un;
-        const double lam = cfc + fabs(dot(Uf, S / ms));
-        const double v = 0.5 * ms * lam;
-        Bk[0] = Bk[0] - v; Bk[6] = Bk[6] - v; Bk[12] = Bk[12] - v; Bk[18] = Bk[18] - v; Bk[24] = Bk[24] - v;
         diss -= v;
-        if (visc) {
-            const double rf = w * (s.fld[(size_t)o * NFLD + F_RHO] - s.fld[(size_t)n * NFLD + F_RHO]) +
-                              s.fld[(size_t)n * NFLD + F_RHO];
-            const double muf = w * (mu - mu) + mu;
-            const double alf = w * (alpha - alpha) + alpha;
-            const double lv = (muf + alf) / rf;
-            const double sv = 0.5 * lv * ms * m.dc[f];
-            Bk[0] = Bk[0] - sv; Bk[6] = Bk[6] - sv; Bk[12] = Bk[12] - sv; Bk[18] = Bk[18] - sv; Bk[24] = Bk[24] - sv;
-            vis -= sv;
-        }
-        const size_t sbase = rowBase + (size_t)slot * 64;
-#pragma unroll
-        for (int e = 0; e < NB; ++e) blk[((sbase >> 6) * NB + e) * 64 + (sbase & 63)] = (TB)Bk[e];
+        if (visc) vis -= sv;
         ++slot;
     }
     // boundary dissipation (all non-empty patches) follows the internal faces
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
         if (f < m.nI) continue;
-        const int b = f - m.nI;
-        const V3 S = ld3(m.Sf + 3 * (size_t)f);
-        const double ms = m.magSf[f];
-        const double* bf = s.bfld + (size_t)b * NFLD;
-        const double lamB = soundSpeed(ph, s.bT[b]) + fabs(dot(V3{bf[F_UX], bf[F_UY], bf[F_UZ]}, S / ms));
-        diss -= 0.5 * ms * lamB;
+        diss -= 0.5 * m.magSf[f] * s.ft[(size_t)f * NFT + FT_LAM];
     }
     D[0] = D[0] - diss; D[6] = D[6] - diss; D[12] = D[12] - diss; D[18] = D[18] - diss; D[24] = D[24] - diss;
     // boundary terms (addBoundaryTerms), patch/face order
@@ -594,9 +613,6 @@ __global__ void __launch_bounds__(256) k_jacobian(MeshView m, Phys ph, StateView
     if (visc) { D[0] = D[0] - vis; D[12] = D[12] - vis; D[18] = D[18] - vis; D[24] = D[24] - vis; D[6] = D[6] - vis; }
 #pragma unroll
     for (int e = 0; e < NB; ++e) dg[diagOff(c, e)] = (TB)D[e];
-    if (dgOut64)
-#pragma unroll
-        for (int e = 0; e < NB; ++e) dgOut64[diagOff(c, e)] = D[e];
     // LU-SGS scalar diagonal
     double r = kGREAT;
     r = 1.0 / fmax(1.0 / r, fabs(D[0]));
@@ -622,20 +638,23 @@ void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipSt
     if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
     if (ph.flux == FLUX_ROE) k_grad<7><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
     else k_grad<8><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
-    k_residual<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
+    k_face_terms<<<nblk(m.nF, 256), 256, 0, st>>>(m, ph, s);
+    k_cell_residual<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
 }
 
 void launchFaceFlux(const MeshView& m, const Phys& ph, const StateView& s, double* phi, double* phiU, double* phiE,
                     hipStream_t st) {
-    k_face_flux<<<nblk(m.nF, 256), 256, 0, st>>>(m, ph, s, phi, phiU, phiE);
+    (void)ph;
+    k_face_flux<<<nblk(m.nF, 256), 256, 0, st>>>(m, s, phi, phiU, phiE);
 }
 
 void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
                     double* rD, double* dg64, int* err, hipStream_t st) {
+    (void)dg64;
     if (fp32)
-        k_jacobian<float><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, dg64, err);
+        k_cell_jacobian<float><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
     else
-        k_jacobian<double><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, dg64, err);
+        k_cell_jacobian<double><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
 }
 
 }  // namespace icsf

@@ -753,6 +753,8 @@ int icsf_case_create(icsf_mesh* mesh, const char* thermo, const char* schemes, c
     RC(B.alloc(s.pseudoCo, (size_t)nC));
     RC(B.alloc(s.rDT, (size_t)nC));
     RC(B.alloc(s.b, (size_t)5 * nC));
+    RC(B.alloc(s.ft, (size_t)NFT * h.nFaces));
+    HIPCHK(hipMemset(s.ft, 0, (size_t)NFT * h.nFaces * sizeof(double)));
     HIPCHK(hipMemset(s.bfld, 0, NFLD * nBs * sizeof(double)));
     HIPCHK(hipMemset(s.bT, 0, nBs * sizeof(double)));
     k_fill<<<nblk(nC, 256), 256, 0, C->st>>>(nC, C->co, s.pseudoCo);

@@ -15,6 +15,10 @@ enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2 };
 enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2 };
 constexpr int NB = 25;  // entries of a 5x5 block
 
+// per-face terms written by the face kernel, AoS [nFaces][NFT]
+enum { FT_CONV = 0, FT_LAP = 5, FT_TAU = 8, FT_SIG = 11, FT_LAPE = 12, FT_LAM = 13, FT_FRDT = 14, FT_REC = 15,
+       FT_SV = 23, NFT = 24 };
+
 struct MeshView {
     int nC, nF, nI, nB;
     const int* own;       // [nF]
@@ -65,6 +69,7 @@ struct StateView {
     double* pseudoCo; // [nC]
     double* rDT;      // [nC]
     double* b;        // [5][nC]
+    double* ft;       // [nF][NFT] face terms
 };
 
 // Block storage offset in the sliced-ELL layout: slot index (as stored in
