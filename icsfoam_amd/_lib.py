@@ -62,6 +62,9 @@ SIGNATURES = {
     "icsf_get_fields": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_calc_flux": (_i, [_vp, _vp, _vp, _vp]),
     "icsf_residual": (_i, [_vp, _vp, _vp]),
+    "icsf_residual_vjp": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "icsf_residual_vjp_device": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "icsf_residual_device": (_i, [_vp, _vp, _vp]),
     "icsf_assemble_jacobian": (_i, [_vp]),
     "icsf_get_jacobian": (_i, [_vp, _vp, _vp, _vp]),
     "icsf_set_jacobian": (_i, [_vp, _vp, _vp, _vp]),

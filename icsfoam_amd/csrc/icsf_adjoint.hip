@@ -1,0 +1,521 @@
+// Adjoint (vector-Jacobian product) of the inviscid AUSM+up residual
+//     b(W, Sf, Cf) = -(sum_f +-F_f / V) * V,
+// v -> (db/dW)^T v, (db/dSf)^T v, (db/dCf)^T v, with the cell centres,
+// volumes and interpolation weights held fixed.  The reference has no
+// adjoint; the forward chain differentiated here is the one of
+// updateFields.H:7-104 (W -> rho, U, e, T, p and the boundary values),
+// ausmPlusUpFluxScheme.C:73-299 (E, H, c*, reconstruction with
+// cellLimited Gauss gradients, AUSM+up face flux) and residualsUpdate.H:1-14
+// (surfaceIntegrate).
+//
+// The reconstruction is recomputed, not taped: the forward pass re-runs the
+// boundary and gradient kernels (keeping the unlimited gradients and the
+// limiter values), then four reverse kernels pull adjoints without atomics:
+//   k_adj_face   one thread per face: AUSM+up reverse, adjoint of the face
+//                values (owner/neighbour side) and of Sf, Cf (flux part)
+//   k_adj_cell   one thread per cell: adjoint of the limited gradients, of
+//                the limiter (its active face and max/min neighbour) and of
+//                the Gauss gradient sum divided by V
+//   k_adj_gauss  one thread per face: Gauss-gradient face values, boundary
+//                values -> owner primitives, limiter pulls into Cf
+//   k_adj_state  one thread per cell: cell-value adjoints -> E, H, c* ->
+//                rho, U, e, T, p -> W
+// Branches follow the primal value: max(a, b) takes b only if b > a,
+// min(a, b) takes b only if b < a, and the limiter's min over faces keeps
+// the first face that attains it.
+#include "icsf_kernels.h"
+#include "icsf_recon.h"
+#include "icsf_views.h"
+
+namespace icsf {
+
+namespace {
+
+#define DF __device__ __forceinline__
+
+inline int nblk(size_t n, int b) { return (int)((n + b - 1) / b); }
+
+// Fields differentiated for AUSM+up (E is not reconstructed by the scheme).
+DF bool used(int k) { return k != F_E; }
+
+// Reverse of ausmFace(): adjoints of the owner-side (L) and neighbour-side
+// (R) face values, indexed by field, and of Sf.  Recomputes the forward
+// quantities with the same expressions.
+DF void ausmFaceAdj(bool lowMach, V3 S, double magS, const double* L, const double* R, double bphi, V3 bphiU,
+                    double bphiE, double* bL, double* bR, V3& bS) {
+    const V3 UL{L[F_UX], L[F_UY], L[F_UZ]}, UR{R[F_UX], R[F_UY], R[F_UZ]};
+    const double cL = L[F_CS], cR = R[F_CS], pL = L[F_P], pR = R[F_P], rL = L[F_RHO], rR = R[F_RHO];
+    const double HL = L[F_H], HR = R[F_H];
+    // ---- forward
+    const double unL = dot(UL, S) / magS;
+    const double unR = dot(UR, S) / magS;
+    const bool bl = cL < unL;
+    const double dl = bl ? unL : cL;
+    const double cl = sqr(cL) / dl;
+    const bool br = cR < -unR;
+    const double dr = br ? -unR : cR;
+    const double cr = sqr(cR) / dr;
+    const bool bc = cr < cl;
+    const double cf = bc ? cr : cl;
+    const double ML = unL / cf;
+    double MpL, PpL, dMpL, dPpL;
+    if (fabs(ML) < 1.0) {
+        const double p2 = 0.25 * sqr(ML + 1), m2 = -0.25 * sqr(ML - 1);
+        const double dp2 = 0.5 * (ML + 1), dm2 = -0.5 * (ML - 1);
+        MpL = p2 * (1 - 2 * m2);
+        PpL = p2 * (2 - ML - 3 * ML * m2);
+        dMpL = dp2 * (1 - 2 * m2) - 2 * p2 * dm2;
+        dPpL = dp2 * (2 - ML - 3 * ML * m2) + p2 * (-1 - 3 * m2 - 3 * ML * dm2);
+    } else {
+        MpL = fmax(ML, 0.0);
+        PpL = ML > 0 ? 1.0 : 0.0;
+        dMpL = ML < 0.0 ? 0.0 : 1.0;
+        dPpL = 0.0;
+    }
+    const double MR = unR / cf;
+    double MmR, PmR, dMmR, dPmR;
+    if (fabs(MR) < 1.0) {
+        const double m2 = -0.25 * sqr(MR - 1), p2 = 0.25 * sqr(MR + 1);
+        const double dm2 = -0.5 * (MR - 1), dp2 = 0.5 * (MR + 1);
+        MmR = m2 * (1 + 2 * p2);
+        PmR = m2 * (-2 - MR + 3 * MR * p2);
+        dMmR = dm2 * (1 + 2 * p2) + 2 * m2 * dp2;
+        dPmR = dm2 * (-2 - MR + 3 * MR * p2) + m2 * (-1 + 3 * p2 + 3 * MR * dp2);
+    } else {
+        MmR = fmin(MR, 0.0);
+        PmR = MR < 0 ? 1.0 : 0.0;
+        dMmR = 0.0 < MR ? 0.0 : 1.0;
+        dPmR = 0.0;
+    }
+    const double M12 = MpL + MmR;
+    double p12 = PpL * pL + PmR * pR;
+    const double Mmean = 0.5 * (sqr(unL) + sqr(unR)) / sqr(cf);
+    const double omM = 1.0 - Mmean;
+    const double A = fmax(omM, 0.0);
+    const double dA = omM < 0.0 ? 0.0 : 1.0;
+    const double B = 0.5 * (rL + rR) * sqr(cf);
+    const double MDiff = -0.25 * A * (pR - pL) / B;
+    const double kap = ((M12 > 0.0 && M12 + MDiff <= 0.0) || (M12 < 0.0 && M12 + MDiff >= 0.0)) ? 0.2 : 1.0;
+    const double M12f = kap == 0.2 ? M12 + 0.2 * MDiff : M12 + MDiff;
+    const double sr = rL + rR, X = unR - unL;
+    if (lowMach) p12 += -0.25 * PpL * PmR * sr * cf * X;
+    const bool up = M12f >= 0;
+    const double rup = up ? rL : rR, Hup = up ? HL : HR;
+    const V3 Uup = up ? UL : UR;
+    const double rhoa = M12f * cf * rup;
+    // ---- reverse
+    const double rhoa_b = (bphi + dot(bphiU, Uup) + bphiE * Hup) * magS;
+    double magS_b = bphi * rhoa + dot(bphiU, rhoa * Uup) + bphiE * rhoa * Hup;
+    const V3 Uup_b = bphiU * (rhoa * magS);
+    const double Hup_b = bphiE * rhoa * magS;
+    const double p12_b = dot(bphiU, S);
+    bS = p12 * bphiU;
+    const double M12f_b = rhoa_b * cf * rup;
+    double cf_b = rhoa_b * M12f * rup;
+    const double rup_b = rhoa_b * M12f * cf;
+    double PpL_b = 0, PmR_b = 0, rL_b = 0, rR_b = 0, unL_b = 0, unR_b = 0, pL_b = 0, pR_b = 0;
+    if (lowMach) {
+        const double q = -0.25 * p12_b;
+        PpL_b += q * PmR * sr * cf * X;
+        PmR_b += q * PpL * sr * cf * X;
+        const double qs = q * PpL * PmR * cf * X;
+        rL_b += qs;
+        rR_b += qs;
+        cf_b += q * PpL * PmR * sr * X;
+        const double qx = q * PpL * PmR * sr * cf;
+        unR_b += qx;
+        unL_b -= qx;
+    }
+    const double M12_b = M12f_b;
+    const double MDiff_b = kap * M12f_b;
+    const double A_b = MDiff_b * (-0.25 * (pR - pL) / B);
+    const double dp_b = MDiff_b * (-0.25 * A / B);
+    pR_b += dp_b;
+    pL_b -= dp_b;
+    const double B_b = -MDiff_b * MDiff / B;
+    const double Mmean_b = -A_b * dA;
+    rL_b += B_b * 0.5 * sqr(cf);
+    rR_b += B_b * 0.5 * sqr(cf);
+    cf_b += B_b * sr * cf;
+    unL_b += Mmean_b * unL / sqr(cf);
+    unR_b += Mmean_b * unR / sqr(cf);
+    cf_b -= Mmean_b * 2.0 * Mmean / cf;
+    PpL_b += p12_b * pL;
+    pL_b += p12_b * PpL;
+    PmR_b += p12_b * pR;
+    pR_b += p12_b * PmR;
+    const double ML_b = M12_b * dMpL + PpL_b * dPpL;
+    const double MR_b = M12_b * dMmR + PmR_b * dPmR;
+    unR_b += MR_b / cf;
+    cf_b -= MR_b * MR / cf;
+    unL_b += ML_b / cf;
+    cf_b -= ML_b * ML / cf;
+    const double cl_b = bc ? 0.0 : cf_b, cr_b = bc ? cf_b : 0.0;
+    double cL_b = cl_b * 2.0 * cL / dl;
+    const double dl_b = -cl_b * cl / dl;
+    if (bl) unL_b += dl_b;
+    else cL_b += dl_b;
+    double cR_b = cr_b * 2.0 * cR / dr;
+    const double dr_b = -cr_b * cr / dr;
+    if (br) unR_b -= dr_b;
+    else cR_b += dr_b;
+    V3 UL_b = (unL_b / magS) * S, UR_b = (unR_b / magS) * S;
+    bS = bS + (unL_b / magS) * UL + (unR_b / magS) * UR;
+    magS_b -= unL_b * unL / magS + unR_b * unR / magS;
+    double HL_b = 0, HR_b = 0;
+    if (up) { rL_b += rup_b; UL_b = UL_b + Uup_b; HL_b = Hup_b; }
+    else { rR_b += rup_b; UR_b = UR_b + Uup_b; HR_b = Hup_b; }
+    bS = bS + (magS_b / magS) * S;
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k) bL[k] = bR[k] = 0.0;
+    bL[F_UX] = UL_b.x; bL[F_UY] = UL_b.y; bL[F_UZ] = UL_b.z;
+    bR[F_UX] = UR_b.x; bR[F_UY] = UR_b.y; bR[F_UZ] = UR_b.z;
+    bL[F_CS] = cL_b; bR[F_CS] = cR_b;
+    bL[F_P] = pL_b; bR[F_P] = pR_b;
+    bL[F_RHO] = rL_b; bR[F_RHO] = rR_b;
+    bL[F_H] = HL_b; bR[F_H] = HR_b;
+}
+
+// Face pass: adjoint of the face flux from v (b_owner gets -F, b_neighbour
+// +F), AUSM+up reverse, reconstruction reverse into Cf.
+__global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView s, AdjView a,
+                                                  const double* __restrict__ v, double* __restrict__ gS,
+                                                  double* __restrict__ gC) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nF) return;
+    double* lr = a.fLR + (size_t)f * 2 * NFLD;
+    if (f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY) {
+#pragma unroll
+        for (int k = 0; k < 2 * NFLD; ++k) lr[k] = 0.0;
+        for (int d = 0; d < 3; ++d) { gS[3 * (size_t)f + d] = 0.0; gC[3 * (size_t)f + d] = 0.0; }
+        return;
+    }
+    const size_t nC = m.nC;
+    const int o = m.own[f];
+    const bool internal = f < m.nI;
+    const int n = internal ? m.nei[f] : -1;
+    double bf[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) bf[q] = internal ? v[q * nC + n] - v[q * nC + o] : -v[q * nC + o];
+    double L[NFLD], R[NFLD];
+    if (internal) {
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k)
+            if (used(k)) recon(m, s, ph.recon, f, o, n, k, L[k], R[k]);
+            else L[k] = R[k] = 0.0;
+    } else {
+        const double* bv = s.bfld + (size_t)(f - m.nI) * NFLD;
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k) L[k] = R[k] = bv[k];
+    }
+    const V3 S = ld3(m.Sf + 3 * (size_t)f);
+    double bL[NFLD], bR[NFLD];
+    V3 bS;
+    ausmFaceAdj(ph.lowMach, S, m.magSf[f], L, R, bf[0], V3{bf[2], bf[3], bf[4]}, bf[1], bL, bR, bS);
+    V3 bC{0, 0, 0};
+    if (internal) {
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k) { lr[k] = bL[k]; lr[NFLD + k] = bR[k]; }
+        if (ph.recon == 1) {
+#pragma unroll
+            for (int k = 0; k < NFLD; ++k) {
+                if (!used(k)) continue;
+                const V3 go = ld3(s.grd + ((size_t)o * NFLD + k) * 3);
+                const V3 gn = ld3(s.grd + ((size_t)n * NFLD + k) * 3);
+                bC = bC + bL[k] * go + bR[k] * gn;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k) { lr[k] = bL[k] + bR[k]; lr[NFLD + k] = 0.0; }
+    }
+    gS[3 * (size_t)f] = bS.x; gS[3 * (size_t)f + 1] = bS.y; gS[3 * (size_t)f + 2] = bS.z;
+    gC[3 * (size_t)f] = bC.x; gC[3 * (size_t)f + 1] = bC.y; gC[3 * (size_t)f + 2] = bC.z;
+}
+
+// Cell pass: per field, the adjoint of the limited gradient and of the
+// cell value (linearUpwind), then the limiter reverse (recomputing the
+// cell's max/min neighbour and the face that set the limiter), then the
+// Gauss-gradient adjoint divided by V.
+__global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView s, AdjView a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
+    const V3 Cc = ld3(m.C + 3 * (size_t)c);
+    const double Vc = m.V[c];
+    for (int k = 0; k < NFLD; ++k) {
+        const size_t ck = (size_t)c * NFLD + k;
+        if (!used(k)) {
+            a.qc[ck] = 0.0; a.tgt[ck] = -1; a.tval[ck] = 0.0; a.fst[ck] = -1; a.fext[ck] = 0.0;
+            a.G[3 * ck] = a.G[3 * ck + 1] = a.G[3 * ck + 2] = 0.0;
+            continue;
+        }
+        double qd = 0.0;
+        V3 gl{0, 0, 0};
+        for (int q = e0; q < e1; ++q) {
+            const int f = m.cf[q];
+            if (f >= m.nI) continue;
+            const bool isOwn = m.own[f] == c;
+            const double av = a.fLR[(size_t)f * 2 * NFLD + (isOwn ? k : NFLD + k)];
+            qd += av;
+            if (ph.recon == 1) gl = gl + av * (ld3(m.Cf + 3 * (size_t)f) - Cc);
+        }
+        int tgt = -1, fs = -1;
+        double tv = 0.0, fx = 0.0;
+        V3 G{0, 0, 0};
+        if (ph.recon == 1) {
+            const V3 g = ld3(a.gun + 3 * ck);
+            const double lim = a.lim[ck];
+            V3 gb = lim * gl;
+            const double limb = dot(gl, g);
+            if (ph.limiter != 0 && limb != 0.0) {
+                const double vc = s.fld[ck];
+                double mx = vc, mn = vc;
+                int imx = -1, imn = -1;
+                for (int q = e0; q < e1; ++q) {
+                    const int f = m.cf[q];
+                    double vo;
+                    int id;
+                    if (f < m.nI) {
+                        id = m.own[f] == c ? m.nei[f] : m.own[f];
+                        vo = s.fld[(size_t)id * NFLD + k];
+                    } else {
+                        const int b = f - m.nI;
+                        id = m.nC + b;
+                        vo = s.bfld[(size_t)b * NFLD + k];
+                    }
+                    if (mx < vo) { mx = vo; imx = id; }
+                    if (vo < mn) { mn = vo; imn = id; }
+                }
+                const double mxD = mx - vc, mnD = mn - vc;
+                double lm = 1.0, rs = 1.0, es = 0.0;
+                int br = 0;
+                for (int q = e0; q < e1; ++q) {
+                    const int f = m.cf[q];
+                    const V3 d = ld3(m.Cf + 3 * (size_t)f) - Cc;
+                    const double ext = dot(d, g);
+                    double r = 1;
+                    int bq = 0;
+                    if (ext > kSMALL) { r = mxD / ext; bq = 1; }
+                    else if (ext < -kSMALL) { r = mnD / ext; bq = -1; }
+                    double l;
+                    if (ph.limiter == 1) l = (sqr(r) + 2 * r) / (sqr(r) + r + 2);
+                    else l = fmin(r, 1.0);
+                    if (l < lm) { lm = l; fs = f; rs = r; es = ext; br = bq; }
+                }
+                if (fs >= 0 && br != 0) {
+                    double dl;
+                    if (ph.limiter == 1) {
+                        const double den = sqr(rs) + rs + 2;
+                        dl = (-sqr(rs) + 4 * rs + 4) / sqr(den);
+                    } else dl = rs <= 1.0 ? 1.0 : 0.0;
+                    const double rb = limb * dl;
+                    const double numb = rb / es;
+                    fx = -rb * rs / es;
+                    gb = gb + fx * (ld3(m.Cf + 3 * (size_t)fs) - Cc);
+                    const int id = br > 0 ? imx : imn;
+                    if (id >= 0) {
+                        tgt = id;
+                        tv = numb;
+                        qd -= numb;
+                    }
+                } else fs = -1;
+            } else fs = -1;
+            G = gb / Vc;
+        }
+        a.qc[ck] = qd;
+        a.tgt[ck] = tgt;
+        a.tval[ck] = tv;
+        a.fst[ck] = fs;
+        a.fext[ck] = fx;
+        a.G[3 * ck] = G.x; a.G[3 * ck + 1] = G.y; a.G[3 * ck + 2] = G.z;
+    }
+}
+
+// Face pass 2: Gauss-gradient face values (internal faces: adjoint of
+// w*(q_o - q_n) + q_n for the cell pull, Sf part), boundary values
+// (flux + Gauss + limiter max/min) chained through rho_b = p_b/(R T_b),
+// E_b, H_b, c*_b to the owner's p, T, U and to Sf (slip normal), and the
+// limiter's Cf part.
+__global__ void __launch_bounds__(256) k_adj_gauss(MeshView m, Phys ph, StateView s, AdjView a,
+                                                   double* __restrict__ gS, double* __restrict__ gC) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nF) return;
+    double* qf = a.qf + (size_t)f * NFLD;
+    if (f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY) {
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k) qf[k] = 0.0;
+        double* bp = a.bprim + (size_t)(f - m.nI) * 5;
+        for (int q = 0; q < 5; ++q) bp[q] = 0.0;
+        return;
+    }
+    const V3 S = ld3(m.Sf + 3 * (size_t)f);
+    const int o = m.own[f];
+    V3 bS{0, 0, 0}, bC{0, 0, 0};
+    const bool rec = ph.recon == 1;
+    if (f < m.nI) {
+        const int n = m.nei[f];
+        const double w = m.w[f];
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k) {
+            if (!used(k) || !rec) { qf[k] = 0.0; continue; }
+            const size_t ok = (size_t)o * NFLD + k, nk = (size_t)n * NFLD + k;
+            const V3 d = ld3(a.G + 3 * ok) - ld3(a.G + 3 * nk);
+            qf[k] = dot(S, d);
+            const double vo = s.fld[ok], vn = s.fld[nk];
+            bS = bS + (w * (vo - vn) + vn) * d;
+            if (a.fst[ok] == f) bC = bC + a.fext[ok] * ld3(a.gun + 3 * ok);
+            if (a.fst[nk] == f) bC = bC + a.fext[nk] * ld3(a.gun + 3 * nk);
+        }
+    } else {
+        const int b = f - m.nI;
+        const double* bv = s.bfld + (size_t)b * NFLD;
+        const double* lr = a.fLR + (size_t)f * 2 * NFLD;
+        double qb[NFLD];
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k) {
+            qf[k] = 0.0;
+            qb[k] = 0.0;
+            if (!used(k)) continue;
+            qb[k] = lr[k];
+            if (rec) {
+                const size_t ok = (size_t)o * NFLD + k;
+                const V3 Go = ld3(a.G + 3 * ok);
+                qb[k] += dot(S, Go);
+                if (a.tgt[ok] == m.nC + b) qb[k] += a.tval[ok];
+                bS = bS + bv[k] * Go;
+                if (a.fst[ok] == f) bC = bC + a.fext[ok] * ld3(a.gun + 3 * ok);
+            }
+        }
+        // c*_b = sqrt(kappa H_b), H_b = max(E_b, SMALL) + max(p_b/rho_b, SMALL)
+        const double g = ph.gamma;
+        const double kap = 2.0 * (g - 1.0) / (g + 1.0);
+        double Hb = qb[F_H];
+        if (bv[F_CS] > 0.0) Hb += qb[F_CS] * kap / (2.0 * bv[F_CS]);
+        const double TB = s.bT[b];
+        const double pB = bv[F_P], rhoB = bv[F_RHO];
+        const V3 UB{bv[F_UX], bv[F_UY], bv[F_UZ]};
+        const double EB = ph.Cv * TB + 0.5 * magSqr(UB);
+        double Eb = qb[F_E], pb = qb[F_P], rb = qb[F_RHO];
+        V3 Ub{qb[F_UX], qb[F_UY], qb[F_UZ]};
+        if (!(EB < kSMALL)) Eb += Hb;
+        const double pr = pB / rhoB;
+        if (!(pr < kSMALL)) { pb += Hb / rhoB; rb -= Hb * pr / rhoB; }
+        double Tb = ph.Cv * Eb;
+        Ub = Ub + Eb * UB;
+        // rho_b = psi p_b, psi = 1/(R T_b)
+        const double psi = 1.0 / (ph.R * TB);
+        pb += rb * psi;
+        Tb -= rb * pB * psi / TB;
+        double* bp = a.bprim + (size_t)b * 5;
+        bp[0] = m.bPT[b] == BC_FIXED ? 0.0 : pb;
+        bp[1] = m.bTT[b] == BC_FIXED ? 0.0 : Tb;
+        V3 Uc_b{0, 0, 0};
+        const int ut = m.bUT[b];
+        if (ut == BC_ZG) Uc_b = Ub;
+        else if (ut == BC_SLIP) {
+            // U_b = U_c - n (n.U_c), n = Sf/|Sf|
+            const double ms = m.magSf[f];
+            const V3 nn = S / ms;
+            const V3 Uc = cellU(s.fld, o);
+            const double nUb = dot(nn, Ub), nUc = dot(nn, Uc);
+            Uc_b = Ub - nUb * nn;
+            const V3 nb = (-nUc) * Ub - nUb * Uc;
+            bS = bS + (nb - dot(nn, nb) * nn) / ms;
+        }
+        bp[2] = Uc_b.x; bp[3] = Uc_b.y; bp[4] = Uc_b.z;
+    }
+    gS[3 * (size_t)f] += bS.x; gS[3 * (size_t)f + 1] += bS.y; gS[3 * (size_t)f + 2] += bS.z;
+    gC[3 * (size_t)f] += bC.x; gC[3 * (size_t)f + 1] += bC.y; gC[3 * (size_t)f + 2] += bC.z;
+}
+
+// Cell pass 2: total cell-value adjoints (direct + Gauss face values +
+// limiter targets of the neighbours + boundary primitives), then
+// c*, H, E -> rho, p, U, T and the updateFields chain back to W.
+__global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateView s, AdjView a,
+                                                   double* __restrict__ gW) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
+    double q[NFLD];
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k) q[k] = a.qc[(size_t)c * NFLD + k];
+    double pbnd = 0.0, Tbnd = 0.0;
+    V3 Ubnd{0, 0, 0};
+    for (int e = e0; e < e1; ++e) {
+        const int f = m.cf[e];
+        if (f < m.nI) {
+            const bool isOwn = m.own[f] == c;
+            const int j = isOwn ? m.nei[f] : m.own[f];
+            const double wt = isOwn ? m.w[f] : 1.0 - m.w[f];
+            const double* qf = a.qf + (size_t)f * NFLD;
+#pragma unroll
+            for (int k = 0; k < NFLD; ++k) {
+                if (!used(k)) continue;
+                q[k] += wt * qf[k];
+                const size_t jk = (size_t)j * NFLD + k;
+                if (a.tgt[jk] == c) q[k] += a.tval[jk];
+            }
+        } else {
+            const double* bp = a.bprim + (size_t)(f - m.nI) * 5;
+            pbnd += bp[0];
+            Tbnd += bp[1];
+            Ubnd = Ubnd + V3{bp[2], bp[3], bp[4]};
+        }
+    }
+    const size_t nC = m.nC;
+    const double* fl = s.fld + (size_t)c * NFLD;
+    const double rho = fl[F_RHO], p = fl[F_P], H = fl[F_H], cs = fl[F_CS], E = fl[F_E];
+    const V3 U{fl[F_UX], fl[F_UY], fl[F_UZ]};
+    const double T = s.T[c];
+    const double g = ph.gamma;
+    const double kap = 2.0 * (g - 1.0) / (g + 1.0);
+    double Hb = q[F_H];
+    if (cs > 0.0) Hb += q[F_CS] * kap / (2.0 * cs);
+    double rb = q[F_RHO], pb = q[F_P] + pbnd, Eb = q[F_E];
+    V3 Ub = V3{q[F_UX], q[F_UY], q[F_UZ]} + Ubnd;
+    if (!(E < kSMALL)) Eb += Hb;
+    const double pr = p / rho;
+    if (!(pr < kSMALL)) { pb += Hb / rho; rb -= Hb * pr / rho; }
+    double Tb = ph.Cv * Eb + Tbnd;
+    Ub = Ub + Eb * U;
+    // p = rho/psi = rho R T
+    const double psi = 1.0 / (ph.R * T);
+    rb += pb / psi;
+    Tb += pb * p / T;
+    // updateFields: rho = max(W0, rhoMin), U = rhoU/rho,
+    // e = clip(rhoE/rho - |U|^2/2), T = e/Cv
+    const double W0 = a.W0[c], W1 = a.W0[nC + c];
+    const V3 rU{a.W0[2 * nC + c], a.W0[3 * nC + c], a.W0[4 * nC + c]};
+    const bool rhoFree = !(W0 < ph.rhoMin);
+    const double r0 = rhoFree ? W0 : ph.rhoMin;
+    const V3 U0 = rU / r0;
+    const double eU = W1 / r0 - 0.5 * magSqr(U0);
+    bool eFree = !(eU < ph.Cv * ph.TMin);
+    if (ph.TMax < kGREAT && ph.Cv * ph.TMax < eU) eFree = false;
+    const double eb = eFree ? Tb / ph.Cv : 0.0;
+    double rEb = eb / r0;
+    rb -= eb * (W1 / r0) / r0;
+    Ub = Ub - eb * U0;
+    const V3 rUb = Ub / r0;
+    rb -= dot(Ub, U0) / r0;
+    gW[c] = rhoFree ? rb : 0.0;
+    gW[nC + c] = rEb;
+    gW[2 * nC + c] = rUb.x;
+    gW[3 * nC + c] = rUb.y;
+    gW[4 * nC + c] = rUb.z;
+}
+
+#undef DF
+
+}  // namespace
+
+void launchResidualVjp(const MeshView& m, const Phys& ph, const StateView& s, const AdjView& a, const double* v,
+                       double* gW, double* gS, double* gC, hipStream_t st) {
+    k_adj_face<<<nblk(m.nF, 256), 256, 0, st>>>(m, ph, s, a, v, gS, gC);
+    k_adj_cell<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, a);
+    k_adj_gauss<<<nblk(m.nF, 256), 256, 0, st>>>(m, ph, s, a, gS, gC);
+    k_adj_state<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, a, gW);
+}
+
+}  // namespace icsf

@@ -70,6 +70,13 @@ int reduceBlocks(int nC);
 
 void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const double* dW, int mode,
                   const double* rho0, const double* U0, const double* T0, hipStream_t st);
+// Forward part of the residual adjoint: boundary values and gradients,
+// keeping the unlimited gradients and limiter values.
+void launchAdjointForward(const MeshView& m, const Phys& ph, const StateView& s, double* gun, double* limo,
+                          hipStream_t st);
+// Reverse pass: gW = (db/dW)^T v, gS = (db/dSf)^T v, gC = (db/dCf)^T v.
+void launchResidualVjp(const MeshView& m, const Phys& ph, const StateView& s, const AdjView& a, const double* v,
+                       double* gW, double* gS, double* gC, hipStream_t st);
 void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st);
 void launchFaceFlux(const MeshView& m, const Phys& ph, const StateView& s, double* phi, double* phiU, double* phiE,
                     hipStream_t st);

@@ -1,0 +1,44 @@
+// Device helpers shared by the residual, Jacobian and adjoint kernels:
+// cell velocity, the cellLimited face test and the linearUpwind face values.
+#pragma once
+#include "icsf_math.h"
+#include "icsf_views.h"
+
+namespace icsf {
+
+__device__ __forceinline__ V3 cellU(const double* fld, int c) {
+    return V3{fld[c * NFLD + F_UX], fld[c * NFLD + F_UY], fld[c * NFLD + F_UZ]};
+}
+
+// One face's contribution to a cellLimited limiter (k = 1): the face
+// extrapolation ext = (Cf - C).grad, the cell's max/min neighbour
+// differences, Venkatakrishnan (limiter 1) or minmod (limiter 2).
+__device__ __forceinline__ void limitFace(double& lim, double maxD, double minD, double ext, int limiter) {
+    double r = 1;
+    if (ext > kSMALL) r = maxD / ext;
+    else if (ext < -kSMALL) r = minD / ext;
+    double l;
+    if (limiter == 1) l = (sqr(r) + 2 * r) / (sqr(r) + r + 2);
+    else l = fmin(r, 1.0);
+    lim = fmin(lim, l);
+}
+
+// L (owner-side) and R (neighbour-side) reconstructed value of field k on
+// internal face f (linearUpwind with pos_/neg_ weights).
+__device__ __forceinline__ void recon(const MeshView& m, const StateView& s, int recon, int f, int o, int n, int k,
+                                      double& L, double& R) {
+    const double vo = s.fld[(size_t)o * NFLD + k], vn = s.fld[(size_t)n * NFLD + k];
+    double l = 1.0 * (vo - vn) + vn;
+    double r = 0.0 * (vo - vn) + vn;
+    if (recon == 1) {
+        const V3 fc = ld3(m.Cf + 3 * (size_t)f);
+        const double* go = s.grd + ((size_t)o * NFLD + k) * 3;
+        const double* gn = s.grd + ((size_t)n * NFLD + k) * 3;
+        l += dot(fc - ld3(m.C + 3 * (size_t)o), V3{go[0], go[1], go[2]});
+        r += dot(fc - ld3(m.C + 3 * (size_t)n), V3{gn[0], gn[1], gn[2]});
+    }
+    L = l;
+    R = r;
+}
+
+}  // namespace icsf

@@ -6,15 +6,12 @@
 // and inputs, hence identical bits on both sides.
 #include "icsf_flux.h"
 #include "icsf_kernels.h"
+#include "icsf_recon.h"
 #include "icsf_views.h"
 
 namespace icsf {
 
 namespace {
-
-__device__ __forceinline__ V3 cellU(const double* fld, int c) {
-    return V3{fld[c * NFLD + F_UX], fld[c * NFLD + F_UY], fld[c * NFLD + F_UZ]};
-}
 
 // Primitive/derived cell fields from conserved W after an increment
 // (updateFields.H:7-78), or from (rho, U, T) at initialisation.
@@ -115,20 +112,13 @@ __global__ void k_boundary(MeshView m, Phys ph, StateView s) {
     s.bT[b] = TB;
 }
 
-__device__ __forceinline__ void limitFace(double& lim, double maxD, double minD, double ext, int limiter) {
-    double r = 1;
-    if (ext > kSMALL) r = maxD / ext;
-    else if (ext < -kSMALL) r = minD / ext;
-    double l;
-    if (limiter == 1) l = (sqr(r) + 2 * r) / (sqr(r) + r + 2);
-    else l = fmin(r, 1.0);
-    lim = fmin(lim, l);
-}
-
 // Gauss linear gradients with cellLimited<limiter> (k = 1) of the NG
 // reconstruction fields; unlimited grad(U) and grad(ecalc) when viscous.
+// gun/limo (optional, [nC][NFLD][3] and [nC][NFLD]) receive the unlimited
+// gradients and the limiter values for the adjoint pass.
 template <int NG>
-__global__ void __launch_bounds__(256) k_grad(MeshView m, Phys ph, StateView s) {
+__global__ void __launch_bounds__(256) k_grad(MeshView m, Phys ph, StateView s, double* __restrict__ gun,
+                                              double* __restrict__ limo) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     double gx[NG], gy[NG], gz[NG], mx[NG], mn[NG], vc[NG];
@@ -191,6 +181,11 @@ __global__ void __launch_bounds__(256) k_grad(MeshView m, Phys ph, StateView s) 
     const double Vc = m.V[c];
 #pragma unroll
     for (int k = 0; k < NG; ++k) { gx[k] = gx[k] / Vc; gy[k] = gy[k] / Vc; gz[k] = gz[k] / Vc; }
+    if (gun) {
+        double* gq = gun + (size_t)c * NFLD * 3;
+#pragma unroll
+        for (int k = 0; k < NG; ++k) { gq[3 * k] = gx[k]; gq[3 * k + 1] = gy[k]; gq[3 * k + 2] = gz[k]; }
+    }
     if (visc) {
         // grad(U)_ij = d U_j / d x_i; column j is the gradient of U_j
         double* g9 = s.gU + (size_t)c * 9;
@@ -214,28 +209,17 @@ __global__ void __launch_bounds__(256) k_grad(MeshView m, Phys ph, StateView s) 
         }
 #pragma unroll
         for (int k = 0; k < NG; ++k) { gx[k] = gx[k] * lim[k]; gy[k] = gy[k] * lim[k]; gz[k] = gz[k] * lim[k]; }
+        if (limo) {
+#pragma unroll
+            for (int k = 0; k < NG; ++k) limo[(size_t)c * NFLD + k] = lim[k];
+        }
+    } else if (limo) {
+#pragma unroll
+        for (int k = 0; k < NG; ++k) limo[(size_t)c * NFLD + k] = 1.0;
     }
     double* go = s.grd + (size_t)c * NFLD * 3;
 #pragma unroll
     for (int k = 0; k < NG; ++k) { go[3 * k] = gx[k]; go[3 * k + 1] = gy[k]; go[3 * k + 2] = gz[k]; }
-}
-
-// L (owner-side) and R (neighbour-side) reconstructed value of field k on
-// internal face f (linearUpwind with pos_/neg_ weights).
-__device__ __forceinline__ void recon(const MeshView& m, const StateView& s, int recon, int f, int o, int n, int k,
-                                      double& L, double& R) {
-    const double vo = s.fld[(size_t)o * NFLD + k], vn = s.fld[(size_t)n * NFLD + k];
-    double l = 1.0 * (vo - vn) + vn;
-    double r = 0.0 * (vo - vn) + vn;
-    if (recon == 1) {
-        const V3 fc = ld3(m.Cf + 3 * (size_t)f);
-        const double* go = s.grd + ((size_t)o * NFLD + k) * 3;
-        const double* gn = s.grd + ((size_t)n * NFLD + k) * 3;
-        l += dot(fc - ld3(m.C + 3 * (size_t)o), V3{go[0], go[1], go[2]});
-        r += dot(fc - ld3(m.C + 3 * (size_t)n), V3{gn[0], gn[1], gn[2]});
-    }
-    L = l;
-    R = r;
 }
 
 __device__ __forceinline__ double soundSpeed(const Phys& ph, double T) {
@@ -636,10 +620,19 @@ void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const d
 
 void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st) {
     if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
-    if (ph.flux == FLUX_ROE) k_grad<7><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
-    else k_grad<8><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
+    if (ph.flux == FLUX_ROE) k_grad<7><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, nullptr, nullptr);
+    else k_grad<8><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, nullptr, nullptr);
     k_face_terms<<<nblk(m.nF, 256), 256, 0, st>>>(m, ph, s);
     k_cell_residual<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
+}
+
+void launchAdjointForward(const MeshView& m, const Phys& ph, const StateView& s, double* gun, double* limo,
+                          hipStream_t st) {
+    if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
+    if (ph.recon == 1) {
+        if (ph.flux == FLUX_ROE) k_grad<7><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, gun, limo);
+        else k_grad<8><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, gun, limo);
+    }
 }
 
 void launchFaceFlux(const MeshView& m, const Phys& ph, const StateView& s, double* phi, double* phiU, double* phiE,

@@ -109,6 +109,10 @@ struct icsf_case {
     bool bcDirty = true;
     MeshView mv{};
     StateView sv{};
+    // residual adjoint (allocated on first use)
+    AdjView adj{};
+    double *adjGS = nullptr, *adjGC = nullptr;
+    bool adjReady = false;
     // Jacobian
     void *blk = nullptr, *dg = nullptr, *dinv = nullptr;
     double* rD = nullptr;
@@ -535,6 +539,59 @@ int fgmresSolve(icsf_case* c, const double* W, const double* b, int& nIter) {
 }  // namespace
 
 // ================================================================ C ABI ===
+namespace {
+
+int adjointPrepare(icsf_case* c) {
+    if (c->ph.viscous) return fail(ICSF_ERR_ARG, "residual VJP: viscous terms are not differentiated (inviscid only)");
+    if (c->ph.flux != FLUX_AUSM) return fail(ICSF_ERR_ARG, "residual VJP: implemented for the AUSMPlusUp flux scheme");
+    if (c->adjReady) return ICSF_OK;
+    const size_t nC = c->mv.nC, nF = c->mv.nF, nB = std::max(c->mv.nB, 1);
+    AdjView& a = c->adj;
+    if (!a.W0) RC(c->buf.alloc(a.W0, 5 * nC));
+    RC(c->buf.alloc(a.gun, nC * NFLD * 3));
+    RC(c->buf.alloc(a.lim, nC * NFLD));
+    RC(c->buf.alloc(a.fLR, nF * 2 * NFLD));
+    RC(c->buf.alloc(a.G, nC * NFLD * 3));
+    RC(c->buf.alloc(a.qc, nC * NFLD));
+    RC(c->buf.alloc(a.tgt, nC * NFLD));
+    RC(c->buf.alloc(a.tval, nC * NFLD));
+    RC(c->buf.alloc(a.fst, nC * NFLD));
+    RC(c->buf.alloc(a.fext, nC * NFLD));
+    RC(c->buf.alloc(a.qf, nF * NFLD));
+    RC(c->buf.alloc(a.bprim, nB * 5));
+    RC(c->buf.alloc(c->adjGS, 3 * nF));
+    RC(c->buf.alloc(c->adjGC, 3 * nF));
+    c->adjReady = true;
+    return ICSF_OK;
+}
+
+// State from W (device, or the current W when NULL) through updateFields
+// with a zero increment; W is kept in adj.W0.
+int stateFromW(icsf_case* c, const double* W) {
+    const size_t n5 = 5 * (size_t)c->mv.nC;
+    if (!c->adj.W0) RC(c->buf.alloc(c->adj.W0, n5));
+    if (W) HIPCHK(hipMemcpyAsync(c->sv.W, W, n5 * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->adj.W0, c->sv.W, n5 * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    launchUpdate(c->mv, c->ph, c->sv, nullptr, 1, nullptr, nullptr, nullptr, c->st);
+    c->haveFields = true;
+    c->haveResidual = c->assembled = false;
+    return ICSF_OK;
+}
+
+int vjpDevice(icsf_case* c, const double* W, const double* v, double* gW, double* gSf, double* gCf) {
+    if (!W && !c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
+    RC(adjointPrepare(c));
+    RC(uploadBC(c));
+    RC(stateFromW(c, W));
+    launchAdjointForward(c->mv, c->ph, c->sv, c->adj.gun, c->adj.lim, c->st);
+    launchResidualVjp(c->mv, c->ph, c->sv, c->adj, v, gW, gSf ? gSf : c->adjGS, gCf ? gCf : c->adjGC, c->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    return ICSF_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* icsf_last_error(void) { return g_err.c_str(); }
@@ -895,6 +952,38 @@ int icsf_residual(icsf_case* c, double* b, double* rDT) {
     const int nC = c->mv.nC;
     if (b) HIPCHK(hipMemcpy(b, c->sv.b, (size_t)5 * nC * sizeof(double), hipMemcpyDeviceToHost));
     if (rDT) HIPCHK(hipMemcpy(rDT, c->sv.rDT, (size_t)nC * sizeof(double), hipMemcpyDeviceToHost));
+    return ICSF_OK;
+}
+
+int icsf_residual_vjp_device(icsf_case* c, const double* W, const double* v, double* gW, double* gSf,
+                             double* gCf) {
+    if (!c || !v || !gW) return fail(ICSF_ERR_ARG, "icsf_residual_vjp_device: NULL argument");
+    return vjpDevice(c, W, v, gW, gSf, gCf);
+}
+
+int icsf_residual_vjp(icsf_case* c, const double* W, const double* v, double* gW, double* gSf, double* gCf) {
+    if (!c || !v || !gW) return fail(ICSF_ERR_ARG, "icsf_residual_vjp: NULL argument");
+    const size_t n5 = 5 * (size_t)c->mv.nC, n3 = 3 * (size_t)c->mv.nF;
+    DevBuf tmp;
+    double *dW = nullptr, *dv, *dg;
+    if (W) RC(tmp.upload(dW, W, n5));
+    RC(tmp.upload(dv, v, n5));
+    RC(tmp.alloc(dg, n5));
+    RC(vjpDevice(c, dW, dv, dg, nullptr, nullptr));
+    HIPCHK(hipMemcpy(gW, dg, n5 * sizeof(double), hipMemcpyDeviceToHost));
+    if (gSf) HIPCHK(hipMemcpy(gSf, c->adjGS, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (gCf) HIPCHK(hipMemcpy(gCf, c->adjGC, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    return ICSF_OK;
+}
+
+int icsf_residual_device(icsf_case* c, const double* W, double* b) {
+    if (!c || !b) return fail(ICSF_ERR_ARG, "icsf_residual_device: NULL argument");
+    if (!W && !c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
+    RC(uploadBC(c));
+    RC(stateFromW(c, W));
+    RC(residualStage(c));
+    HIPCHK(hipMemcpyAsync(b, c->sv.b, 5 * (size_t)c->mv.nC * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
     return ICSF_OK;
 }
 

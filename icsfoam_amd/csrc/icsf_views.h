@@ -72,6 +72,23 @@ struct StateView {
     double* ft;       // [nF][NFT] face terms
 };
 
+// Buffers of the residual adjoint (vector-Jacobian product).
+struct AdjView {
+    double* W0;     // [5][nC] conserved state the residual is differentiated at
+    double* gun;    // [nC][NFLD][3] unlimited Gauss gradients
+    double* lim;    // [nC][NFLD] limiter values
+    double* fLR;    // [nF][2*NFLD] adjoint of owner-/neighbour-side face values
+                    //   (boundary faces: slot k holds the boundary value's adjoint)
+    double* G;      // [nC][NFLD][3] adjoint of the unlimited gradient / V
+    double* qc;     // [nC][NFLD] partial cell-value adjoint
+    int* tgt;       // [nC][NFLD] limiter max/min element: cell, nC + boundary face, -1 none
+    double* tval;   // [nC][NFLD] adjoint sent to that element
+    int* fst;       // [nC][NFLD] face that set the limiter, -1 none
+    double* fext;   // [nC][NFLD] adjoint of that face's extrapolation
+    double* qf;     // [nF][NFLD] adjoint of the Gauss face values (internal faces)
+    double* bprim;  // [nB][5] boundary adjoint forwarded to the owner's p, T, U
+};
+
 // Block storage offset in the sliced-ELL layout: slot index (as stored in
 // col[]) and block entry e.
 __host__ __device__ __forceinline__ size_t sellOff(size_t slot, int e) {

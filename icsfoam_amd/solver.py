@@ -102,6 +102,17 @@ class dbnsCase:
         check(lib().icsf_residual(self._h, ptr(b), ptr(rdt)))
         return b, rdt
 
+    def residual_vjp(self, v, W=None):
+        """(gW, gSf, gCf) = ((db/dW)^T v, (db/dSf)^T v, (db/dCf)^T v) of the
+        inviscid AUSM+up residual at W ([5][nCells]; None = current state).
+        W becomes the case state."""
+        v = np.ascontiguousarray(v, np.float64)
+        Wc = None if W is None else np.ascontiguousarray(W, np.float64)
+        gW = np.empty((5, self.nCells))
+        gS, gC = np.empty((self.mesh.nFaces, 3)), np.empty((self.mesh.nFaces, 3))
+        check(lib().icsf_residual_vjp(self._h, ptr(Wc), ptr(v), ptr(gW), ptr(gS), ptr(gC)))
+        return gW, gS, gC
+
     def createJacobian(self):
         check(lib().icsf_assemble_jacobian(self._h))
 

@@ -160,6 +160,25 @@ int icsf_calc_flux(icsf_case* c, double* phi, double* phiUp, double* phiEp);
  * local rPseudoDeltaT of setCoAndDeltaT.H:57-141.  Either may be NULL. */
 int icsf_residual(icsf_case* c, double* b, double* rPseudoDeltaT);
 
+/* ---- residual adjoint (no counterpart in the reference) --------------
+ * Vector-Jacobian product of the inviscid AUSM+up residual b(W, Sf, Cf)
+ * (the forward chain of updateFields.H:7-104, ausmPlusUpFluxScheme.C:73-299
+ * and residualsUpdate.H:1-14), cell centres, volumes and weights fixed:
+ *   gW  = (db/dW)^T  v   [5][nCells]
+ *   gSf = (db/dSf)^T v   [nFaces][3]   (NULL: not returned)
+ *   gCf = (db/dCf)^T v   [nFaces][3]   (NULL: not returned)
+ * W ([5][nCells], conserved) becomes the case state through updateFields
+ * with a zero increment; W NULL keeps the current state.  Viscous cases and
+ * the Roe/HLLC schemes return ICSF_ERR_ARG.  Host pointers; returns when
+ * the results are written. */
+int icsf_residual_vjp(icsf_case* c, const double* W, const double* v, double* gW, double* gSf, double* gCf);
+/* Same with device pointers (for framework bindings); synchronous. */
+int icsf_residual_vjp_device(icsf_case* c, const double* W, const double* v, double* gW, double* gSf,
+                             double* gCf);
+/* b(W) of icsf_residual with device pointers: W ([5][nCells] or NULL for
+ * the current state) becomes the case state, b receives R*V.  Synchronous. */
+int icsf_residual_device(icsf_case* c, const double* W, double* b);
+
 /* Assemble the coupled Jacobian for the current state (after
  * icsf_residual): convectiveFluxScheme::createConvectiveJacobian
  * (convectiveFluxScheme.C:537-546) and viscousFluxScheme::

@@ -1604,6 +1604,8 @@ int pseudoStep(Case& k, Res& out) {
     return 0;
 }
 
+#include "icsf_tangent.h"
+
 }  // namespace
 
 // ======================================================== C interface ====
@@ -1988,6 +1990,80 @@ void orc_reconstruct(void* h, int field, double* L, double* R) {
     reconstruct(k, vc.data(), vb.data(), 1, l, r);
     std::copy(l.begin(), l.end(), L);
     std::copy(r.begin(), r.end(), R);
+}
+
+// ------------------------------------------ tangent-linear residual -----
+// State from conserved W ([5][nC], [rho, rhoE, rhoU]) through updateFields
+// with a zero increment.
+void orc_set_state_w(void* h, const double* W) {
+    Case& k = *(Case*)h;
+    const int nC = k.m->nC;
+    k.rho.assign(W, W + nC);
+    k.rhoE.assign(W + nC, W + 2 * (size_t)nC);
+    k.rhoU.assign(3 * (size_t)nC, 0.0);
+    k.U.assign(3 * (size_t)nC, 0.0);
+    k.e.assign(nC, 0.0); k.T.assign(nC, 0.0); k.p.assign(nC, 0.0);
+    for (int c = 0; c < nC; ++c)
+        for (int d = 0; d < 3; ++d) k.rhoU[3 * c + d] = W[(2 + d) * (size_t)nC + c];
+    CVec z;
+    z.init(nC);
+    updateFields(k, z);
+}
+
+static const double* sfOr(const Case& k, const double* Sf) { return Sf ? Sf : k.m->Sf.data(); }
+static const double* cfOr(const Case& k, const double* Cf) { return Cf ? Cf : k.m->Cf.data(); }
+
+// Primal b(W, Sf, Cf) of the tangent restatement (Sf/Cf NULL: the mesh's).
+void orc_residual_w(void* h, const double* W, const double* Sf, const double* Cf, double* b) {
+    Case& k = *(Case*)h;
+    residualT<double>(k, W, sfOr(k, Sf), cfOr(k, Cf), b);
+}
+
+// b and the Jacobian-vector product db = J_W dW + J_S dSf + J_C dCf
+// (NULL tangents are zero).
+void orc_residual_jvp(void* h, const double* W, const double* dW, const double* dSf, const double* dCf, double* b,
+                      double* db) {
+    Case& k = *(Case*)h;
+    const Mesh& m = *k.m;
+    const size_t n5 = 5 * (size_t)m.nC, n3 = 3 * (size_t)m.nF;
+    std::vector<Dual> w(n5), s(n3), c(n3), r(n5);
+    for (size_t i = 0; i < n5; ++i) w[i] = Dual(W[i], dW ? dW[i] : 0.0);
+    for (size_t i = 0; i < n3; ++i) {
+        s[i] = Dual(m.Sf[i], dSf ? dSf[i] : 0.0);
+        c[i] = Dual(m.Cf[i], dCf ? dCf[i] : 0.0);
+    }
+    residualT<Dual>(k, w.data(), s.data(), c.data(), r.data());
+    for (size_t i = 0; i < n5; ++i) {
+        if (b) b[i] = r[i].v;
+        db[i] = r[i].d;
+    }
+}
+
+// Dense Jacobians by unit tangents (small meshes): JW[5nC][5nC],
+// JS[5nC][3nF], JC[5nC][3nF], row = residual entry, column = input entry.
+void orc_residual_jac_dense(void* h, const double* W, double* JW, double* JS, double* JC) {
+    Case& k = *(Case*)h;
+    const Mesh& m = *k.m;
+    const size_t n5 = 5 * (size_t)m.nC, n3 = 3 * (size_t)m.nF;
+    std::vector<Dual> w(n5), s(n3), c(n3), r(n5);
+    for (size_t i = 0; i < n5; ++i) w[i] = Dual(W[i], 0.0);
+    for (size_t i = 0; i < n3; ++i) { s[i] = Dual(m.Sf[i], 0.0); c[i] = Dual(m.Cf[i], 0.0); }
+    for (size_t j = 0; j < n5; ++j) {
+        w[j].d = 1.0;
+        residualT<Dual>(k, w.data(), s.data(), c.data(), r.data());
+        for (size_t i = 0; i < n5; ++i) JW[i * n5 + j] = r[i].d;
+        w[j].d = 0.0;
+    }
+    for (size_t j = 0; j < n3; ++j) {
+        s[j].d = 1.0;
+        residualT<Dual>(k, w.data(), s.data(), c.data(), r.data());
+        for (size_t i = 0; i < n5; ++i) JS[i * n3 + j] = r[i].d;
+        s[j].d = 0.0;
+        c[j].d = 1.0;
+        residualT<Dual>(k, w.data(), s.data(), c.data(), r.data());
+        for (size_t i = 0; i < n5; ++i) JC[i * n3 + j] = r[i].d;
+        c[j].d = 0.0;
+    }
 }
 
 }  // extern "C"

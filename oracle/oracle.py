@@ -41,6 +41,10 @@ _SIG = {
     "orc_get_pseudo_co": (None, [_vp, _vp]),
     "orc_face_flux": (None, [_i, _i, C.c_double, C.c_double, C.c_double, _vp, _vp]),
     "orc_reconstruct": (None, [_vp, _i, _vp, _vp]),
+    "orc_set_state_w": (None, [_vp, _vp]),
+    "orc_residual_w": (None, [_vp, _vp, _vp, _vp, _vp]),
+    "orc_residual_jvp": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "orc_residual_jac_dense": (None, [_vp, _vp, _vp, _vp, _vp]),
 }
 _lib = None
 
@@ -248,3 +252,30 @@ class OracleCase:
         co = np.empty(self.nCells)
         lib().orc_get_pseudo_co(self.h, _p(co))
         return co
+
+    # ---- tangent-linear restatement of the inviscid AUSM+up residual ----
+    def set_state_w(self, W):
+        """State from conserved W [5][nC] via updateFields (zero increment)."""
+        lib().orc_set_state_w(self.h, _p(_f64(W)))
+
+    def residual_w(self, W, Sf=None, Cf=None):
+        """b(W, Sf, Cf) of the dual-number restatement evaluated in FP64."""
+        b = np.empty((5, self.nCells))
+        Sf = None if Sf is None else _f64(Sf)
+        Cf = None if Cf is None else _f64(Cf)
+        lib().orc_residual_w(self.h, _p(_f64(W)), _p(Sf), _p(Cf), _p(b))
+        return b
+
+    def residual_jvp(self, W, dW=None, dSf=None, dCf=None):
+        """(b, J_W dW + J_S dSf + J_C dCf) by forward-mode dual numbers."""
+        b, db = np.empty((5, self.nCells)), np.empty((5, self.nCells))
+        a = [None if x is None else _f64(x) for x in (dW, dSf, dCf)]
+        lib().orc_residual_jvp(self.h, _p(_f64(W)), _p(a[0]), _p(a[1]), _p(a[2]), _p(b), _p(db))
+        return b, db
+
+    def residual_jac_dense(self, W):
+        """Dense J_W [5nC,5nC], J_S [5nC,3nF], J_C [5nC,3nF] (small meshes)."""
+        n5, n3 = 5 * self.nCells, 3 * self.nFaces
+        JW, JS, JC = np.empty((n5, n5)), np.empty((n5, n3)), np.empty((n5, n3))
+        lib().orc_residual_jac_dense(self.h, _p(_f64(W)), _p(JW), _p(JS), _p(JC))
+        return JW, JS, JC

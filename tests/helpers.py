@@ -19,3 +19,38 @@ def box_small(n=(10, 6, 5), mu=1.8e-5, flux="ROE", **kw):
 def rel_err(a, b):
     a = np.asarray(a); b = np.asarray(b)
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def ausm_inviscid(c, limiter=None, recon=None):
+    """The same case with the inviscid AUSM+up scheme (optionally another
+    limiter / reconstruction)."""
+    import dataclasses
+    kw = {}
+    if limiter is not None:
+        kw["limiter"] = limiter
+    if recon is not None:
+        kw["recon"] = recon
+    return dataclasses.replace(c, thermo=cases.thermo_dict(0.0), fvSchemes=cases.schemes_dict("AUSMPlusUp", **kw))
+
+
+def generic_state(W, seed, amp=1e-3):
+    """W with small uncorrelated per-cell noise on all five components, so
+    that no two neighbouring values tie and no velocity component is exactly
+    zero (ties and exact zeros are points where the residual has no
+    derivative)."""
+    rng = np.random.default_rng(seed)
+    W = np.array(W, dtype=np.float64, copy=True)
+    scale = np.abs(W[2:]).max()
+    W[2:] += amp * scale * rng.standard_normal(W[2:].shape)
+    W[:2] *= 1 + amp * rng.standard_normal(W[:2].shape)
+    return W
+
+
+def smooth_field(X, rng, ncomp, modes=4, k=3.0):
+    """ncomp smooth random fields (sums of plane waves) at points X [n,3]."""
+    out = np.zeros((ncomp, X.shape[0]))
+    for q in range(ncomp):
+        kv = rng.standard_normal((modes, 3)) * k
+        ph = rng.uniform(0, 2 * np.pi, modes)
+        out[q] = sum(np.sin(X @ kv[j] + ph[j]) for j in range(modes)) / modes
+    return out
