@@ -144,6 +144,70 @@ def cpu_baseline(n_sample: int, n_full: int, steps: int = 2):
                        f"ratio {n_sample ** 3}/{n_full ** 3}")
 
 
+def bench_hb(steps, warmup):
+    """Harmonic-balance pseudo-steps/s on a 64x64x32 annular sector with 7
+    instants (3 harmonics of 100 Hz), Roe inviscid, GMRES(30)+LU-SGS, FP64."""
+    from icsfoam_amd import cases
+    from icsfoam_amd.mesh import prepare_case
+    from icsfoam_amd.solver import hbCase
+
+    t0 = time.time()
+    insts, D, t = cases.case_hb(n=(64, 64, 32), n_harmonics=3, flux="ROE")
+    insts = [prepare_case(c, permute_seed=2024) for c in insts]
+    log(f"[bench] HB case (7 instants, {insts[0].mesh.n_cells} cells) prepared in {time.time() - t0:.1f}s")
+    g = hbCase.from_cases(insts, D)
+    g.pseudo_steps(warmup, residuals=False)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.pseudo_steps(steps, residuals=False)
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    res = g.pseudo_steps(1)[0]
+    log(f"[bench] HB: {steps} steps in {dt:.3f}s; instant-0 residuals {res[0, :5]}")
+    nC = insts[0].mesh.n_cells
+    return {"metric": "harmonic-balance pseudo-steps/s (7 instants)", "value": round(steps / dt, 4),
+            "unit": "pseudo-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": round(1e3 * dt / steps, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded annular sector, periodic inflow)",
+            "config": {"workload": f"64x64x32 annular sector ({nC} cells) x 7 instants "
+                                   f"({7 * nC} block rows), Roe inviscid, GMRES(30)+LU-SGS"}}
+
+
+def bench_vjp(calls):
+    """Residual VJP (inviscid AUSM+up, 2nd order) at 100^3: ms per call with
+    device-resident W and v (torch tensors)."""
+    import torch
+
+    from icsfoam_amd import cases, dbnsCase
+    from icsfoam_amd.adjoint import residual_device, residual_vjp_device
+    from icsfoam_amd.mesh import prepare_case
+
+    c = cases.case_box(n=(100, 100, 100), lengths=(1.0, 1.0, 1.0), jitter=0.1, seed=7, flux="AUSMPlusUp", mu=0.0)
+    c = prepare_case(c, permute_seed=2024)
+    g = dbnsCase.from_case(c)
+    W = torch.tensor(g.fields()["W"], device="cuda")
+    v = torch.randn(W.shape, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+    for _ in range(3):
+        residual_vjp_device(g, W, v)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        residual_vjp_device(g, W, v)
+    dt = (time.perf_counter() - t0) / calls
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        residual_device(g, W)
+    dr = (time.perf_counter() - t0) / calls
+    log(f"[bench] VJP {1e3 * dt:.3f} ms/call, residual {1e3 * dr:.3f} ms/call")
+    return {"metric": "residual VJP calls/s at 1M cells (AUSM+up 2nd order)", "value": round(1.0 / dt, 3),
+            "unit": "calls/s", "n_gpus": 1, "steps": calls, "warmup": 3, "ms_per_step": round(1e3 * dt, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded jittered 100^3 box)",
+            "config": {"workload": "100^3 jittered box (1000000 cells), inviscid AUSM+up, linearUpwind + "
+                                   "cellLimited<Venkatakrishnan>; gW, gSf, gCf per call",
+                       "residual_ms": round(1e3 * dr, 3)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,11 +217,20 @@ def main():
     ap.add_argument("--orth", default="CGS", choices=["CGS", "MGS"])
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="c5", choices=["c5", "hb", "vjp"],
+                    help="c5: the headline 160^3 pseudo-step (default); hb: 7-instant harmonic balance; "
+                         "vjp: residual adjoint at 100^3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the pseudo-step on exactly one GPU; --gpus {args.gpus} is not "
               "supported (multi-GPU sharding is designed, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "hb":
+        print(json.dumps(bench_hb(args.steps, args.warmup)), flush=True)
+        return
+    if args.workload == "vjp":
+        print(json.dumps(bench_vjp(max(args.steps, 10))), flush=True)
+        return
     n = args.n
     base = prepared(n)
     nC, nI = base.mesh.n_cells, base.mesh.n_internal

@@ -62,6 +62,19 @@ SIGNATURES = {
     "icsf_get_fields": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_calc_flux": (_i, [_vp, _vp, _vp, _vp]),
     "icsf_residual": (_i, [_vp, _vp, _vp]),
+    "icsf_hb_operator": (_i, [_i, _vp, _vp, _i, C.c_double, _vp, _vp, _vp]),
+    "icsf_hb_create": (_i, [_vp, C.c_char_p, C.c_char_p, C.c_char_p, _i, _vp, _vp]),
+    "icsf_hb_destroy": (_i, [_vp]),
+    "icsf_hb_set_boundary": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp]),
+    "icsf_hb_set_fields": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "icsf_hb_get_fields": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "icsf_hb_residual": (_i, [_vp, _vp]),
+    "icsf_hb_assemble_jacobian": (_i, [_vp]),
+    "icsf_hb_matrix_mul": (_i, [_vp, _vp, _vp]),
+    "icsf_hb_precondition": (_i, [_vp, _vp]),
+    "icsf_hb_solve_for_incr": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "icsf_hb_pseudo_steps": (_i, [_vp, _i, _vp]),
+    "icsf_hb_synchronize": (_i, [_vp]),
     "icsf_residual_vjp": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_residual_vjp_device": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_residual_device": (_i, [_vp, _vp, _vp]),
@@ -88,6 +101,16 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m icsfoam_amd.build` "
                               "(the solver has no CPU fallback)")
+        # torch bundles its own HIP runtime under the same soname
+        # (libamdhip64.so.7).  Importing torch first makes the dynamic loader
+        # bind libicsf.so to that already-loaded runtime, so that one HIP
+        # runtime serves both and torch device tensors can be handed to the
+        # library; loading libicsf.so first would bring in a second runtime
+        # and torch's device initialisation would then fail.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         h = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(h, name)

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libicsf.so")
 
-SOURCES = ["icsf_mesh.cpp", "icsf_dict.cpp", "icsf_residual.hip", "icsf_krylov.hip", "icsf_adjoint.hip",
+SOURCES = ["icsf_mesh.cpp", "icsf_dict.cpp", "icsf_hb.cpp", "icsf_residual.hip", "icsf_krylov.hip", "icsf_adjoint.hip",
            "icsf_solver.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -433,3 +433,75 @@ def permute_case(case: Case, new_to_old: np.ndarray, renumber) -> Case:
     return dataclasses.replace(case, mesh=mesh, rho=case.rho[new_to_old],
                                U=case.U[new_to_old], T=case.T[new_to_old],
                                cell_order=order)
+
+
+# --------------------------------------------------------------------------
+# harmonic balance: annular sector with a periodic inflow
+# --------------------------------------------------------------------------
+
+def annulus_sector(nt: int, nr: int, nz: int, theta: float = np.pi / 8, r0: float = 0.5, r1: float = 1.0,
+                   length: float = 1.0, jitter: float = 0.0, seed: int = 0) -> PolyMesh:
+    """Annular sector theta in [0, theta], r in [r0, r1], z in [0, length]
+    (cells ordered theta fastest); interior nodes jittered in the index
+    space.  Patches: inlet (z = 0), outlet (z = length), sides (the two
+    theta planes) and hub/shroud walls."""
+    kk, jj, ii = np.meshgrid(np.arange(nz + 1), np.arange(nr + 1), np.arange(nt + 1), indexing="ij")
+    a, b, c = ii.astype(np.float64), jj.astype(np.float64), kk.astype(np.float64)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        d = rng.uniform(-jitter, jitter, size=(3,) + a.shape)
+        a = a + d[0] * ((ii > 0) & (ii < nt))
+        b = b + d[1] * ((jj > 0) & (jj < nr))
+        c = c + d[2] * ((kk > 0) & (kk < nz))
+    th = theta * a / nt
+    r = r0 + (r1 - r0) * b / nr
+    z = length * c / nz
+    pts = np.stack([(r * np.cos(th)).ravel(), (r * np.sin(th)).ravel(), z.ravel()], 1).astype(np.float64)
+    fv, own, nei, patches = _hex_topology(
+        nt, nr, nz,
+        [("inlet", PATCH_PATCH, ["-z"]), ("outlet", PATCH_PATCH, ["+z"]),
+         ("sides", PATCH_WALL, ["-x", "+x"]), ("hubShroud", PATCH_WALL, ["-y", "+y"])])
+    return _mesh_from_quads(pts, fv, own, nei, patches, nt * nr * nz, pts.reshape(nz + 1, nr + 1, nt + 1, 3))
+
+
+def case_hb(n=(64, 64, 32), n_harmonics: int = 3, frequency_hz: float = 100.0, amplitude: float = 0.05,
+            mach: float = 0.5, noise: float = 0.02, seed: int = 3, co: float = 20.0, nDirections: int = 30,
+            jitter: float = 0.1, flux: str = "ROE", mu: float = 0.0):
+    """Harmonic-balance case on an annular sector: 2 n_harmonics + 1
+    instants over one period of frequency_hz (selectedPeriod = 1/f), the
+    inflow velocity of instant n scaled by 1 + amplitude sin(omega t_n),
+    steady axial flow at the given Mach number with smooth noise as the
+    initial state of every instant (each instant with its own phase of the
+    perturbation).  Returns (instances, D, instants)."""
+    from .solver import hb_operator
+
+    omega = 2.0 * np.pi * frequency_hz
+    nO = 2 * n_harmonics + 1
+    D, t, _ = hb_operator([omega], [n_harmonics], nO, 1.0 / frequency_hz)
+    mesh = annulus_sector(*n, jitter=jitter, seed=seed)
+    C = approx_cell_centres(mesh)
+    T0, p0 = 300.0, 1e5
+    rho0, U0 = _state_from_mach(mach, p0, T0)
+    s = noise * fourier_noise(C, seed + 1)
+    instances = []
+    for J in range(nO):
+        ph = np.sin(omega * t[J])
+        rho = rho0 * (1 + s * (1 + 0.5 * ph))
+        T = T0 * (1 + 0.5 * s)
+        U = np.zeros((mesh.n_cells, 3))
+        U[:, 2] = U0 * (1 + s + amplitude * ph)
+        U[:, 0] = 0.02 * U0 * s
+        bcs = []
+        for pt in mesh.patches:
+            if pt.name == "inlet":
+                Uv = np.zeros((pt.size, 3)); Uv[:, 2] = U0 * (1 + amplitude * ph)
+                bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_FIXED_VALUE, BC_FIXED_VALUE, None, Uv,
+                                        np.full(pt.size, T0)))
+            elif pt.name == "outlet":
+                bcs.append(BoundarySpec(BC_FIXED_VALUE, BC_ZERO_GRADIENT, BC_ZERO_GRADIENT,
+                                        np.full(pt.size, p0), None, None))
+            else:
+                bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_SLIP, BC_ZERO_GRADIENT))
+        instances.append(Case(f"hb{J}", mesh, rho, U, T, bcs, thermo_dict(mu), schemes_dict(flux),
+                              solution_dict(nDirections=nDirections, co=co)))
+    return instances, D, t

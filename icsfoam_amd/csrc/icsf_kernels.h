@@ -44,6 +44,23 @@ struct GmresArgs {
     void* V;          // Krylov basis, nDirs vectors of [5][nC]
     // optional timing marks: mark(kernelClass, begin?1:0)
     std::function<void(int, int)> mark;
+    // Batched (harmonic-balance) systems: vectors of 5*nVec entries and
+    // caller-supplied operator y = A (x/scal) (vout = x/scal when given)
+    // and preconditioner v <- M^-1 v.  nVec 0 / empty hooks: the single
+    // mesh SpMV and LU-SGS/Jacobi above.
+    int nVec = 0;
+    std::function<void(const void*, const double*, void*, void*)> op;
+    std::function<void(void*)> pre;
+};
+
+// Harmonic-balance coupling: per-instance block storage (FP64) and the
+// spectral operator D [nO][nO] (device).
+constexpr int HB_MAXO = 16;
+struct HBBlocks {
+    int nO;
+    const double* blk[HB_MAXO];
+    const double* dg[HB_MAXO];
+    const double* D;
 };
 
 // kernel classes of the timing marks
@@ -103,6 +120,22 @@ void fgmresUpdateAndResidual(const MeshView& m, Work& wk, GmresScalars* g, int b
                              const void* dg, int nDirs, const void* Z, void* dW, const void* b, void* r,
                              hipStream_t st);
 void launchCombine(int nC, int vecFp32, const void* V, int nv, const GmresScalars* g, void* z, hipStream_t st);
+
+// Harmonic balance (icsf_krylov.hip): y = A_HB (x/scal) over [nO][5][nC]
+// vectors in coupledMatrix order, with vout = x/scal when given.
+void launchSpmvHB(const MeshView& m, const HBBlocks& hb, const double* x, const double* scal, double* vout,
+                  double* y, hipStream_t st);
+// Per-variable bookkeeping of one instance slice ([5][nC]) into g: W minus
+// its per-component average; norm factors and initial residuals from
+// t = A (W - avg) and b; final residuals from r = b - A dW (r holds A dW on
+// entry).
+void sliceSubAverage(const MeshView& m, Work& wk, GmresScalars* g, const double* W, double* out, hipStream_t st);
+void sliceNormFactors(const MeshView& m, Work& wk, GmresScalars* g, const double* t, const double* b,
+                      hipStream_t st);
+void sliceFinalResidual(const MeshView& m, Work& wk, GmresScalars* g, const double* b, double* r, hipStream_t st);
+// dW += sum_i y_i V_i over 5*nVec entries
+void launchUpdateSolution(size_t nVec, double* dW, const double* V, size_t vstride, int nv, const GmresScalars* g,
+                          hipStream_t st);
 void gmresReadResiduals(const GmresScalars* g, double* res10, hipStream_t st);
 const double* gmresResidualsDevice(const GmresScalars* g);
 

@@ -84,6 +84,94 @@ __global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__
     }
 }
 
+// Harmonic-balance SpMV: row cell c of instance J = blockIdx.y.  The own
+// instance's partial sums are those of k_spmv; the cross-instance terms
+// (V_c D_JK) x_K are added in coupledMatrix::matrixMul order
+// (coupledMatrix.C:66-123): scalar rows add the scalar columns of
+// instances 0..nO-1 (own rho, rhoE blocks at K = J) and then the own
+// vector column; vector rows add the own scalar columns and then the vector
+// columns of instances 0..nO-1 (own block at K = J).
+__global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const double* __restrict__ x,
+                                                 const double* __restrict__ scal, double* __restrict__ vout,
+                                                 double* __restrict__ y) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y;
+    if (c >= m.nC) return;
+    const size_t nC = m.nC;
+    const size_t vs = 5 * nC;
+    const double* blk = hb.blk[J];
+    const double* dg = hb.dg[J];
+    const double beta = scal ? *scal : 1.0;
+    auto X = [&](int K, int k, int j) -> double {
+        const double v = x[(size_t)K * vs + (size_t)k * nC + j];
+        return scal ? v / beta : v;
+    };
+    double xc[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) xc[k] = X(J, k, c);
+    if (vout)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) vout[(size_t)J * vs + (size_t)k * nC + c] = xc[k];
+    double acc[5][3];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const double b0 = dg[diagOff(c, 5 * i)], b1 = dg[diagOff(c, 5 * i + 1)];
+        const double b2 = dg[diagOff(c, 5 * i + 2)], b3 = dg[diagOff(c, 5 * i + 3)], b4 = dg[diagOff(c, 5 * i + 4)];
+        acc[i][0] = b0 * xc[0];
+        acc[i][1] = b1 * xc[1];
+        acc[i][2] = b2 * xc[2] + b3 * xc[3] + b4 * xc[4];
+    }
+    const int sl = c >> 6, lane = c & 63;
+    const int len = m.rowLen[c];
+    const size_t base = (size_t)m.slicePtr[sl];
+    for (int k = 0; k < len; ++k) {
+        const size_t slot = (base + k) * 64 + lane;
+        const int j = m.col[slot];
+        double xj[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) xj[q] = X(J, q, j);
+        const size_t bb = (base + k) * NB * 64 + lane;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const double b0 = blk[bb + (5 * i) * 64], b1 = blk[bb + (5 * i + 1) * 64];
+            const double b2 = blk[bb + (5 * i + 2) * 64], b3 = blk[bb + (5 * i + 3) * 64],
+                         b4 = blk[bb + (5 * i + 4) * 64];
+            acc[i][0] += b0 * xj[0];
+            acc[i][1] += b1 * xj[1];
+            const double d = b2 * xj[2] + b3 * xj[3] + b4 * xj[4];
+            acc[i][2] += d;
+        }
+    }
+    const double Vc = m.V[c];
+    const int nO = hb.nO;
+    // scalar rows
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        double r = 0.0;
+        for (int K = 0; K < nO; ++K) {
+            if (K == J) {
+                r = r + acc[i][0];
+                r = r + acc[i][1];
+            } else {
+                r = r + (Vc * hb.D[J * nO + K]) * X(K, i, c);
+            }
+        }
+        r = r + acc[i][2];
+        y[(size_t)J * vs + (size_t)i * nC + c] = r;
+    }
+    // vector rows
+#pragma unroll
+    for (int i = 2; i < 5; ++i) {
+        double r = 0.0 + acc[i][0];
+        r = r + acc[i][1];
+        for (int K = 0; K < nO; ++K) {
+            if (K == J) r = r + acc[i][2];
+            else r = r + (Vc * hb.D[J * nO + K]) * X(K, i, c);
+        }
+        y[(size_t)J * vs + (size_t)i * nC + c] = r;
+    }
+}
+
 // ------------------------------------------------------------- LU-SGS -----
 // Forward sweep of one level: r[c] -= sum over lower neighbours p (in
 // increasing p) of L_cp * (rD[p] r[p]).
@@ -600,7 +688,7 @@ void launchJacobiApply(int nC, int blkFp32, int vecFp32, const void* dinv, void*
 template <typename TV>
 void gmresCycleT(const GmresArgs& a, hipStream_t st) {
     const MeshView& m = *a.m;
-    const int nC = m.nC;
+    const int nC = a.nVec > 0 ? a.nVec : m.nC;
     const int nbk = reduceBlocks(nC);
     const size_t vs = (size_t)5 * nC;
     TV* r = (TV*)a.r;    // current w
@@ -613,7 +701,8 @@ void gmresCycleT(const GmresArgs& a, hipStream_t st) {
     };
     auto precond = [&](TV* v) {
         mark(TK_PRECOND, 1);
-        if (a.precond == 0) launchLusgs(m, *a.lev, a.blkFp32, sizeof(TV) == 4, a.blk, a.rD, v, st);
+        if (a.pre) a.pre(v);
+        else if (a.precond == 0) launchLusgs(m, *a.lev, a.blkFp32, sizeof(TV) == 4, a.blk, a.rD, v, st);
         else if (a.precond == 1) launchJacobiApply(nC, a.blkFp32, sizeof(TV) == 4, a.dinv, v, st);
         mark(TK_PRECOND, 0);
     };
@@ -627,7 +716,8 @@ void gmresCycleT(const GmresArgs& a, hipStream_t st) {
         // v_i = r / beta ; w = A v_i (into the other buffer: the SpMV gathers
         // neighbour entries of its input, so it never runs in place)
         mark(TK_SPMV, 1);
-        launchSpmv(m, a.blkFp32, sizeof(TV) == 4, a.blk, a.dg, r, &g->beta, vi, r2, st);
+        if (a.op) a.op(r, &g->beta, vi, r2);
+        else launchSpmv(m, a.blkFp32, sizeof(TV) == 4, a.blk, a.dg, r, &g->beta, vi, r2, st);
         mark(TK_SPMV, 0);
         {
             TV* t = r;
@@ -757,6 +847,47 @@ void launchZero(size_t n, int fp32, void* x, hipStream_t st) {
 void launchZeroComponent(int nC, int comp, int fp32, void* x, hipStream_t st) {
     if (fp32) k_zero_comp<float><<<nb(nC, 256), 256, 0, st>>>(nC, comp, (float*)x);
     else k_zero_comp<double><<<nb(nC, 256), 256, 0, st>>>(nC, comp, (double*)x);
+}
+
+}  // namespace icsf
+
+// ------------------------------------------------- harmonic balance -----
+namespace icsf {
+
+void launchSpmvHB(const MeshView& m, const HBBlocks& hb, const double* x, const double* scal, double* vout,
+                  double* y, hipStream_t st) {
+    dim3 grid(nb(m.nC, 256), hb.nO);
+    k_spmv_hb<<<grid, 256, 0, st>>>(m, hb, x, scal, vout, y);
+}
+
+void sliceSubAverage(const MeshView& m, Work& wk, GmresScalars* g, const double* W, double* out, hipStream_t st) {
+    const int nC = m.nC;
+    const int nbk = reduceBlocks(nC);
+    k_sum5<double><<<nbk, 256, 0, st>>>(nC, W, wk.part);
+    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 5, g->red);
+    k_sub_avg<double><<<nb(nC, 256), 256, 0, st>>>(nC, W, g->red, out);
+}
+
+void sliceNormFactors(const MeshView& m, Work& wk, GmresScalars* g, const double* t, const double* b,
+                      hipStream_t st) {
+    const int nC = m.nC;
+    const int nbk = reduceBlocks(nC);
+    k_normfactor<double><<<nbk, 256, 0, st>>>(nC, t, b, wk.part);
+    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 8, g->red);
+    k_gm_init<<<1, 64, 0, st>>>(g);
+}
+
+void sliceFinalResidual(const MeshView& m, Work& wk, GmresScalars* g, const double* b, double* r, hipStream_t st) {
+    const int nC = m.nC;
+    const int nbk = reduceBlocks(nC);
+    k_resid_l1<double><<<nbk, 256, 0, st>>>(nC, b, r, wk.part);
+    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 5, g->red);
+    k_gm_final<<<1, 64, 0, st>>>(g, m.emptyDir);
+}
+
+void launchUpdateSolution(size_t nVec, double* dW, const double* V, size_t vstride, int nv, const GmresScalars* g,
+                          hipStream_t st) {
+    k_update_sol<double, double><<<nb(5 * nVec, 256), 256, 0, st>>>((int)nVec, dW, V, vstride, nv, g->yh);
 }
 
 }  // namespace icsf

@@ -15,6 +15,7 @@
 
 #include "icsf.h"
 #include "icsf_dict.h"
+#include "icsf_hb.h"
 #include "icsf_kernels.h"
 #include "icsf_math.h"
 #include "icsf_mesh.h"
@@ -140,11 +141,12 @@ struct icsf_case {
     double tms[NTIMER] = {0};
     long long tcnt[NTIMER] = {0};
     DevBuf buf;
+    bool ownStream = true;
     ~icsf_case() {
         for (auto& e : events) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
         for (auto& e : pool) (void)hipEventDestroy(e);
         for (auto& e : open) (void)hipEventDestroy(e);
-        if (st) (void)hipStreamDestroy(st);
+        if (st && ownStream) (void)hipStreamDestroy(st);
     }
 };
 
@@ -592,6 +594,228 @@ int vjpDevice(icsf_case* c, const double* W, const double* v, double* gW, double
 
 }  // namespace
 
+
+// ====================================================== harmonic balance ==
+// dbnsFullyImplicitHBFoam (outerLoop.H:1-230): nO instances of one case on
+// one mesh, coupled by the spectral source -V sum_K D_JK W_K
+// (HBZoneTemplates.C:39-93) and the diagonal blocks V D_JK I
+// (HBZone.C:435-518).  Coupled vectors are instance-major [nO][5][nC].
+struct icsf_hb {
+    icsf_mesh* mesh = nullptr;
+    int nO = 0;
+    std::vector<icsf_case*> inst;  // owned
+    hipStream_t st = nullptr;
+    std::vector<double> D;
+    double* dD = nullptr;
+    HBBlocks hbb{};
+    double *Wall = nullptr, *ball = nullptr, *r = nullptr, *r2 = nullptr, *V = nullptr, *dW = nullptr;
+    double *t1 = nullptr, *t2 = nullptr, *rD = nullptr, *dres = nullptr;
+    int* derr = nullptr;
+    GmresScalars* g = nullptr;
+    std::vector<GmresScalars*> gJ;
+    Work wk;
+    int nDirs = 30, maxIter = 1, minIter = 0, orth = 0;
+    double tol = 1e-12, relTol = 1e-2;
+    int corr = 0;
+    bool haveInit = false, havePrev = false, assembled = false;
+    DevBuf buf;
+    ~icsf_hb() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (icsf_case* c : inst) delete c;
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace {
+
+struct HBDiag {
+    double* dg[HB_MAXO];
+};
+
+// b_J + source_J, source_J = ((0 - V D_J0 W_0) - V D_J1 W_1) - ...
+__global__ void k_hb_source(int nC, int nO, int J, const double* __restrict__ D, const double* __restrict__ Vol,
+                            const double* __restrict__ Wall, const double* __restrict__ bJ, double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    const size_t vs = (size_t)5 * nC;
+    for (int q = 0; q < 5; ++q) {
+        double src = 0.0;
+        for (int K = 0; K < nO; ++K) src -= Vol[c] * D[J * nO + K] * Wall[K * vs + (size_t)q * nC + c];
+        out[(size_t)q * nC + c] = bJ[(size_t)q * nC + c] + src;
+    }
+}
+
+// V D_JJ on the instance diagonals, then lusgs::lusgs over all 2 nO scalar
+// and nO vector diagonals in coupledMatrix order (lusgs.C:50-125).
+__global__ void k_hb_diag(int nC, int nO, const double* __restrict__ D, const double* __restrict__ Vol, HBDiag h,
+                          double* __restrict__ rD, int* err) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    for (int J = 0; J < nO; ++J) {
+        const double VD = Vol[c] * D[J * nO + J];
+        double* dg = h.dg[J];
+        for (int e = 0; e < 25; e += 6) dg[diagOff(c, e)] += VD;
+    }
+    double r = kGREAT;
+    for (int J = 0; J < nO; ++J) {
+        r = 1.0 / fmax(1.0 / r, fabs(h.dg[J][diagOff(c, 0)]));
+        r = 1.0 / fmax(1.0 / r, fabs(h.dg[J][diagOff(c, 6)]));
+    }
+    for (int J = 0; J < nO; ++J) {
+        r = 1.0 / fmax(1.0 / r, fabs(h.dg[J][diagOff(c, 12)]));
+        r = 1.0 / fmax(1.0 / r, fabs(h.dg[J][diagOff(c, 18)]));
+        r = 1.0 / fmax(1.0 / r, fabs(h.dg[J][diagOff(c, 24)]));
+    }
+    rD[c] = r;
+    if (!(r >= kVSMALL)) atomicOr(err, 1);
+}
+
+// SER over all instances (outerLoop.H:30-66): dres = [init nO*5][prev nO*5]
+__global__ void k_hb_ser(int nC, int nO, double* co, const double* dres, double coIncr, double coDecr, double coMin,
+                         double coMax) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    double nI = 0, nP = 0;
+    for (int J = 0; J < nO; ++J) {
+        const double* a = dres + 5 * J;
+        const double* b = dres + 5 * (nO + J);
+        nI += a[0] * a[0] + a[1] * a[1] + (a[2] * a[2] + a[3] * a[3] + a[4] * a[4]);
+        nP += b[0] * b[0] + b[1] * b[1] + (b[2] * b[2] + b[3] * b[3] + b[4] * b[4]);
+    }
+    double ratio = sqrt(nP) / sqrt(nI);
+    ratio = fmax(fmin(ratio, coIncr), coDecr);
+    double v = co[c] * ratio;
+    v = fmax(fmin(v, coMax), coMin);
+    co[c] = v;
+}
+
+__global__ void k_copyn(int n, const double* a, double* b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+void hbLusgs(icsf_hb* h, double* v) {
+    const MeshView& m = h->inst[0]->mv;
+    const size_t vs = (size_t)5 * m.nC;
+    for (int J = 0; J < h->nO; ++J)
+        launchLusgs(m, h->mesh->lev, 0, 0, h->inst[J]->blk, h->rD, v + J * vs, h->st);
+}
+
+int hbUploadBC(icsf_hb* h) {
+    for (icsf_case* c : h->inst) RC(uploadBC(c));
+    return ICSF_OK;
+}
+
+int hbResidualStage(icsf_hb* h) {
+    const MeshView& m = h->inst[0]->mv;
+    const size_t vs = (size_t)5 * m.nC;
+    RC(hbUploadBC(h));
+    for (int J = 0; J < h->nO; ++J) launchCopy(vs, 0, h->inst[J]->sv.W, h->Wall + J * vs, h->st);
+    for (int J = 0; J < h->nO; ++J) RC(residualStage(h->inst[J]));
+    for (int J = 0; J < h->nO; ++J)
+        k_hb_source<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, h->nO, J, h->dD, m.V, h->Wall, h->inst[J]->sv.b,
+                                                         h->ball + J * vs);
+    return ICSF_OK;
+}
+
+int hbJacobianStage(icsf_hb* h) {
+    const MeshView& m = h->inst[0]->mv;
+    for (int J = 0; J < h->nO; ++J) RC(jacobianStage(h->inst[J]));
+    HBDiag hd{};
+    for (int J = 0; J < h->nO; ++J) hd.dg[J] = (double*)h->inst[J]->dg;
+    k_hb_diag<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, h->nO, h->dD, m.V, hd, h->rD, h->derr);
+    h->assembled = true;
+    return ICSF_OK;
+}
+
+void hbReadResiduals(icsf_hb* h, std::vector<double>& res) {
+    res.assign(10 * (size_t)h->nO, 0.0);
+    (void)hipStreamSynchronize(h->st);
+    for (int J = 0; J < h->nO; ++J) gmresReadResiduals(h->gJ[J], &res[10 * (size_t)J], h->st);
+}
+
+// gmres::solveDelta on the HB system (left LU-SGS, reference algorithm)
+int hbSolve(icsf_hb* h, const double* b, int& nIter) {
+    const MeshView& m = h->inst[0]->mv;
+    const size_t vs = (size_t)5 * m.nC, vA = vs * h->nO;
+    for (int J = 0; J < h->nO; ++J) sliceSubAverage(m, h->wk, h->gJ[J], h->Wall + J * vs, h->t1 + J * vs, h->st);
+    launchSpmvHB(m, h->hbb, h->t1, nullptr, nullptr, h->t2, h->st);
+    for (int J = 0; J < h->nO; ++J) sliceNormFactors(m, h->wk, h->gJ[J], h->t2 + J * vs, b + J * vs, h->st);
+    launchZero(vA, 0, h->dW, h->st);
+    launchCopy(vA, 0, b, h->r, h->st);
+    nIter = 0;
+    for (;;) {
+        GmresArgs a{&m, &h->mesh->lev, &h->wk, h->g, h->nDirs, 0, h->orth, 0, nullptr, nullptr, h->rD, nullptr,
+                    h->r, h->r2, h->V};
+        a.nVec = m.nC * h->nO;
+        a.op = [h, &m](const void* x, const double* scal, void* vout, void* y) {
+            launchSpmvHB(m, h->hbb, (const double*)x, scal, (double*)vout, (double*)y, h->st);
+        };
+        a.pre = [h](void* v) { hbLusgs(h, (double*)v); };
+        gmresCycle(a, 0, h->st);
+        launchUpdateSolution((size_t)m.nC * h->nO, h->dW, h->V, vA, h->nDirs, h->g, h->st);
+        launchSpmvHB(m, h->hbb, h->dW, nullptr, nullptr, h->r, h->st);
+        for (int J = 0; J < h->nO; ++J) sliceFinalResidual(m, h->wk, h->gJ[J], b + J * vs, h->r + J * vs, h->st);
+        ++nIter;
+        if (nIter < h->minIter) continue;
+        if (nIter >= h->maxIter) break;
+        std::vector<double> res;
+        hbReadResiduals(h, res);
+        double mx = -1e300, rel = -1e300;
+        for (int J = 0; J < h->nO; ++J) {
+            const double* q = &res[10 * (size_t)J];
+            for (int k = 5; k < 10; ++k) mx = std::max(mx, q[k]);
+            for (int k = 0; k < 5; ++k) {
+                if (k >= 2 && (k - 2) == m.emptyDir) continue;
+                rel = std::max(rel, q[5 + k] / (q[k] + kROOTVSMALL));
+            }
+        }
+        if (mx < h->tol || rel < h->relTol) break;
+    }
+    if (m.emptyDir >= 0)
+        for (int J = 0; J < h->nO; ++J) launchZeroComponent(m.nC, 2 + m.emptyDir, 0, h->dW + J * vs, h->st);
+    return ICSF_OK;
+}
+
+int hbPseudoStep(icsf_hb* h) {
+    const MeshView& m = h->inst[0]->mv;
+    const size_t vs = (size_t)5 * m.nC;
+    const icsf_case* c0 = h->inst[0];
+    h->corr++;
+    if (h->haveInit) {
+        if (h->corr != 1 && h->havePrev)
+            for (int J = 0; J < h->nO; ++J)
+                k_hb_ser<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, h->nO, h->inst[J]->sv.pseudoCo, h->dres,
+                                                             c0->coIncr, c0->coDecr, c0->coMin, c0->coMax);
+        k_copyn<<<1, 5 * HB_MAXO, 0, h->st>>>(5 * h->nO, h->dres, h->dres + 5 * h->nO);
+        h->havePrev = true;
+    }
+    RC(hbResidualStage(h));
+    RC(hbJacobianStage(h));
+    int nIter = 0;
+    RC(hbSolve(h, h->ball, nIter));
+    for (int J = 0; J < h->nO; ++J) k_copy5<<<1, 64, 0, h->st>>>(gmresResidualsDevice(h->gJ[J]), h->dres + 5 * J);
+    h->haveInit = true;
+    for (int J = 0; J < h->nO; ++J) {
+        icsf_case* c = h->inst[J];
+        if (c->localTS && c->bounding) {
+            k_bound_flag<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, c->lowerBound, c->sv.W, c->fac);
+            k_bound_apply<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, m.nI, m.cfOff, m.cf, m.own, m.nei, c->fac,
+                                                               c->sv.pseudoCo);
+        }
+        launchUpdate(m, c->ph, c->sv, h->dW + J * vs, 1, nullptr, nullptr, nullptr, h->st);
+    }
+    return ICSF_OK;
+}
+
+int hbCheckInstance(const icsf_hb* h, int J) {
+    if (!h) return fail(ICSF_ERR_ARG, "NULL HB case");
+    if (J < 0 || J >= h->nO) return fail(ICSF_ERR_ARG, "instant index out of range");
+    return ICSF_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* icsf_last_error(void) { return g_err.c_str(); }
@@ -773,15 +997,22 @@ int icsf_mesh_info(const icsf_mesh* m, int* info) {
     return ICSF_OK;
 }
 
-int icsf_case_create(icsf_mesh* mesh, const char* thermo, const char* schemes, const char* solution,
-                     icsf_case** out) {
-    if (!mesh || !out) return fail(ICSF_ERR_ARG, "icsf_case_create: NULL argument");
+namespace {
+// A case; harmonic-balance instances (shared != nullptr) run on the shared
+// stream and keep no Krylov storage of their own.
+int createCase(icsf_mesh* mesh, const char* thermo, const char* schemes, const char* solution, hipStream_t shared,
+               icsf_case** out) {
     *out = nullptr;
     RC(checkDevice());
     std::unique_ptr<icsf_case> C(new icsf_case);
     C->mesh = mesh;
     RC(readSettings(C.get(), thermo, schemes, solution));
-    HIPCHK(hipStreamCreate(&C->st));
+    if (shared) {
+        C->st = shared;
+        C->ownStream = false;
+    } else {
+        HIPCHK(hipStreamCreate(&C->st));
+    }
     const HostMesh& h = mesh->h;
     const int nC = h.nCells, nB = h.nFaces - h.nInternal;
     const size_t nBs = (size_t)std::max(nB, 1);
@@ -830,6 +1061,12 @@ int icsf_case_create(icsf_mesh* mesh, const char* thermo, const char* schemes, c
     RC(B.alloc(C->rD, (size_t)nC));
     RC(B.alloc(C->derr, 1));
     HIPCHK(hipMemset(C->derr, 0, sizeof(int)));
+    RC(B.alloc(C->fac, (size_t)nC));
+    if (shared) {
+        HIPCHK(hipStreamSynchronize(C->st));
+        *out = C.release();
+        return ICSF_OK;
+    }
     // Krylov storage
     const size_t vs = (size_t)5 * nC;
     const int nOuter = C->nDirs;
@@ -839,7 +1076,6 @@ int icsf_case_create(icsf_mesh* mesh, const char* thermo, const char* schemes, c
     RC(B.alloc(C->t2, vs));
     RC(B.alloc(C->dW, vs));
     RC(B.alloc(C->dres, 10));
-    RC(B.alloc(C->fac, (size_t)nC));
     if (C->solver == 1) {
         RC(B.alloc(C->Z, vs * (size_t)nOuter));
         if (C->mixed) {
@@ -859,6 +1095,13 @@ int icsf_case_create(icsf_mesh* mesh, const char* thermo, const char* schemes, c
     HIPCHK(hipStreamSynchronize(C->st));
     *out = C.release();
     return ICSF_OK;
+}
+}  // namespace
+
+int icsf_case_create(icsf_mesh* mesh, const char* thermo, const char* schemes, const char* solution,
+                     icsf_case** out) {
+    if (!mesh || !out) return fail(ICSF_ERR_ARG, "icsf_case_create: NULL argument");
+    return createCase(mesh, thermo, schemes, solution, nullptr, out);
 }
 
 int icsf_case_destroy(icsf_case* c) {
@@ -1161,6 +1404,195 @@ int icsf_timing_read(icsf_case* c, int n, double* ms, long long* count) {
         if (ms) ms[k] = c->tms[k];
         if (count) count[k] = c->tcnt[k];
     }
+    return ICSF_OK;
+}
+
+
+// ------------------------------------------------------ harmonic balance --
+int icsf_hb_operator(int nFreq, const double* frequencies, const int* harmonics, int nInstants,
+                     double selectedPeriod, double* instants, double* D, double* condition) {
+    if (nFreq < 0 || (nFreq > 0 && (!frequencies || !harmonics)) || nInstants < 1 || !D)
+        return fail(ICSF_ERR_ARG, "icsf_hb_operator: bad argument");
+    std::vector<double> f(frequencies, frequencies + nFreq);
+    std::vector<int> hm(harmonics, harmonics + nFreq);
+    for (int k : hm)
+        if (k < 0) return fail(ICSF_ERR_ARG, "icsf_hb_operator: negative harmonic count");
+    const std::vector<double> om = hbOmegaList(f, hm);
+    if ((int)om.size() > nInstants)
+        return fail(ICSF_ERR_ARG, "specified number of instants is lower than frequency vector dimension");
+    double cond = 0;
+    const std::vector<double> t = hbInstants(nInstants, selectedPeriod, om, &cond);
+    std::vector<double> Dv;
+    std::string err;
+    if (!hbOperator(om, t, Dv, err)) return fail(ICSF_ERR_ARG, err);
+    if (instants) std::copy(t.begin(), t.end(), instants);
+    std::copy(Dv.begin(), Dv.end(), D);
+    if (condition) *condition = cond;
+    return ICSF_OK;
+}
+
+int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, const char* solution, int nInstants,
+                   const double* D, icsf_hb** out) {
+    if (!mesh || !D || !out) return fail(ICSF_ERR_ARG, "icsf_hb_create: NULL argument");
+    if (nInstants < 1 || nInstants > HB_MAXO) return fail(ICSF_ERR_ARG, "icsf_hb_create: 1 <= nInstants <= 16");
+    *out = nullptr;
+    RC(checkDevice());
+    std::unique_ptr<icsf_hb> H(new icsf_hb);
+    H->mesh = mesh;
+    H->nO = nInstants;
+    H->D.assign(D, D + (size_t)nInstants * nInstants);
+    HIPCHK(hipStreamCreate(&H->st));
+    for (int J = 0; J < nInstants; ++J) {
+        icsf_case* c = nullptr;
+        RC(createCase(mesh, thermo, schemes, solution, H->st, &c));
+        H->inst.push_back(c);
+    }
+    const icsf_case* c0 = H->inst[0];
+    if (c0->solver != 0 || c0->precond != 0 || c0->mixed || c0->blkFp32)
+        return fail(ICSF_ERR_KEYWORD, "harmonic balance: flowSolver must be GMRES with preconditioner LUSGS in "
+                                      "double precision");
+    H->nDirs = c0->nDirs; H->maxIter = c0->maxIter; H->minIter = c0->minIter; H->orth = c0->orth;
+    H->tol = c0->tol; H->relTol = c0->relTol;
+    const size_t nC = mesh->h.nCells, vs = 5 * nC, vA = vs * nInstants;
+    DevBuf& B = H->buf;
+    RC(B.upload(H->dD, H->D.data(), H->D.size()));
+    RC(B.alloc(H->Wall, vA)); RC(B.alloc(H->ball, vA)); RC(B.alloc(H->r, vA)); RC(B.alloc(H->r2, vA));
+    RC(B.alloc(H->dW, vA)); RC(B.alloc(H->t1, vA)); RC(B.alloc(H->t2, vA));
+    RC(B.alloc(H->V, vA * (size_t)(H->nDirs + 1)));
+    RC(B.alloc(H->rD, nC));
+    RC(B.alloc(H->dres, 10 * (size_t)HB_MAXO));
+    RC(B.alloc(H->derr, 1));
+    HIPCHK(hipMemset(H->derr, 0, sizeof(int)));
+    RC(B.alloc(H->wk.part, (size_t)1024 * 64));
+    {
+        char* p;
+        RC(B.alloc(p, gmresScalarsBytes())); H->g = (GmresScalars*)p;
+        HIPCHK(hipMemset(H->g, 0, gmresScalarsBytes()));
+        for (int J = 0; J < nInstants; ++J) {
+            RC(B.alloc(p, gmresScalarsBytes()));
+            HIPCHK(hipMemset(p, 0, gmresScalarsBytes()));
+            H->gJ.push_back((GmresScalars*)p);
+        }
+    }
+    H->hbb.nO = nInstants;
+    H->hbb.D = H->dD;
+    for (int J = 0; J < nInstants; ++J) {
+        H->hbb.blk[J] = (const double*)H->inst[J]->blk;
+        H->hbb.dg[J] = (const double*)H->inst[J]->dg;
+    }
+    *out = H.release();
+    return ICSF_OK;
+}
+
+int icsf_hb_destroy(icsf_hb* h) {
+    delete h;
+    return ICSF_OK;
+}
+
+int icsf_hb_set_boundary(icsf_hb* h, int instant, int patch, int pType, int UType, int TType, const double* pValue,
+                         const double* UValue, const double* TValue) {
+    RC(hbCheckInstance(h, instant));
+    return icsf_set_boundary(h->inst[instant], patch, pType, UType, TType, pValue, UValue, TValue);
+}
+
+int icsf_hb_set_fields(icsf_hb* h, int instant, const double* rho, const double* U, const double* T) {
+    RC(hbCheckInstance(h, instant));
+    h->assembled = false;
+    return icsf_set_fields(h->inst[instant], rho, U, T);
+}
+
+int icsf_hb_get_fields(icsf_hb* h, int instant, double* rho, double* U, double* T, double* p, double* W) {
+    RC(hbCheckInstance(h, instant));
+    return icsf_get_fields(h->inst[instant], rho, U, T, p, W);
+}
+
+static int hbReady(icsf_hb* h) {
+    if (!h) return fail(ICSF_ERR_ARG, "NULL HB case");
+    for (icsf_case* c : h->inst)
+        if (!c->haveFields) return fail(ICSF_ERR_STATE, "fields not set for every instant");
+    return ICSF_OK;
+}
+
+int icsf_hb_residual(icsf_hb* h, double* b) {
+    RC(hbReady(h));
+    RC(hbResidualStage(h));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (b)
+        HIPCHK(hipMemcpy(b, h->ball, 5 * (size_t)h->inst[0]->mv.nC * h->nO * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return ICSF_OK;
+}
+
+int icsf_hb_assemble_jacobian(icsf_hb* h) {
+    RC(hbReady(h));
+    for (icsf_case* c : h->inst)
+        if (!c->haveResidual) return fail(ICSF_ERR_STATE, "icsf_hb_residual must run before the Jacobian assembly");
+    RC(hbJacobianStage(h));
+    int e = 0;
+    HIPCHK(hipMemcpy(&e, h->derr, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) return fail(ICSF_ERR_ZERO_DIAG, "All diagonals of coupledMatrix are zero.");
+    return ICSF_OK;
+}
+
+int icsf_hb_matrix_mul(icsf_hb* h, const double* x, double* y) {
+    if (!h || !x || !y) return fail(ICSF_ERR_ARG, "icsf_hb_matrix_mul: NULL argument");
+    if (!h->assembled) return fail(ICSF_ERR_STATE, "HB Jacobian not assembled");
+    const size_t vA = 5 * (size_t)h->inst[0]->mv.nC * h->nO;
+    HIPCHK(hipMemcpy(h->t1, x, vA * sizeof(double), hipMemcpyHostToDevice));
+    launchSpmvHB(h->inst[0]->mv, h->hbb, h->t1, nullptr, nullptr, h->t2, h->st);
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemcpy(y, h->t2, vA * sizeof(double), hipMemcpyDeviceToHost));
+    return ICSF_OK;
+}
+
+int icsf_hb_precondition(icsf_hb* h, double* r) {
+    if (!h || !r) return fail(ICSF_ERR_ARG, "icsf_hb_precondition: NULL argument");
+    if (!h->assembled) return fail(ICSF_ERR_STATE, "HB Jacobian not assembled");
+    const size_t vA = 5 * (size_t)h->inst[0]->mv.nC * h->nO;
+    HIPCHK(hipMemcpy(h->t1, r, vA * sizeof(double), hipMemcpyHostToDevice));
+    hbLusgs(h, h->t1);
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemcpy(r, h->t1, vA * sizeof(double), hipMemcpyDeviceToHost));
+    return ICSF_OK;
+}
+
+int icsf_hb_solve_for_incr(icsf_hb* h, const double* b, double* dW, double* residuals, int* nIter) {
+    if (!h || !b || !dW) return fail(ICSF_ERR_ARG, "icsf_hb_solve_for_incr: NULL argument");
+    if (!h->assembled) return fail(ICSF_ERR_STATE, "HB Jacobian not assembled");
+    const size_t vs = 5 * (size_t)h->inst[0]->mv.nC, vA = vs * h->nO;
+    for (int J = 0; J < h->nO; ++J) launchCopy(vs, 0, h->inst[J]->sv.W, h->Wall + J * vs, h->st);
+    HIPCHK(hipMemcpy(h->ball, b, vA * sizeof(double), hipMemcpyHostToDevice));
+    int it = 0;
+    RC(hbSolve(h, h->ball, it));
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemcpy(dW, h->dW, vA * sizeof(double), hipMemcpyDeviceToHost));
+    if (residuals) {
+        std::vector<double> res;
+        hbReadResiduals(h, res);
+        std::copy(res.begin(), res.end(), residuals);
+    }
+    if (nIter) *nIter = it;
+    return ICSF_OK;
+}
+
+int icsf_hb_pseudo_steps(icsf_hb* h, int nSteps, double* residuals) {
+    RC(hbReady(h));
+    if (nSteps < 0) return fail(ICSF_ERR_ARG, "icsf_hb_pseudo_steps: nSteps < 0");
+    for (int i = 0; i < nSteps; ++i) {
+        RC(hbPseudoStep(h));
+        if (residuals) {
+            std::vector<double> res;
+            hbReadResiduals(h, res);
+            std::copy(res.begin(), res.end(), residuals + (size_t)i * 10 * h->nO);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return ICSF_OK;
+}
+
+int icsf_hb_synchronize(icsf_hb* h) {
+    if (!h) return fail(ICSF_ERR_ARG, "NULL HB case");
+    HIPCHK(hipStreamSynchronize(h->st));
     return ICSF_OK;
 }
 

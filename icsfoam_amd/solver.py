@@ -166,3 +166,98 @@ class dbnsCase:
         cnt = np.zeros(6, np.int64)
         check(lib().icsf_timing_read(self._h, 6, ptr(ms), ptr(cnt)))
         return ms, cnt
+
+
+def hb_operator(frequencies, harmonics, n_instants: int, selected_period: float = 0.0):
+    """(D [nO, nO], instants [nO], condition number) of the harmonic-balance
+    spectral operator (HBZone::updateHBOperators / HBZoneList::setInstants).
+    Frequencies are angular (rad/s); selected_period <= 0 selects the
+    instants by the condition-number scan."""
+    f = as_f64(np.atleast_1d(frequencies))
+    h = as_i32(np.atleast_1d(harmonics))
+    D = np.empty((n_instants, n_instants))
+    t = np.empty(n_instants)
+    cond = np.zeros(1)
+    check(lib().icsf_hb_operator(len(f), ptr(f), ptr(h), n_instants, float(selected_period), ptr(t), ptr(D),
+                                 ptr(cond)))
+    return D, t, float(cond[0])
+
+
+class hbCase:
+    """dbnsFullyImplicitHBFoam on the GPU: nO instants of one case on one
+    mesh coupled by the spectral operator D."""
+
+    def __init__(self, mesh: fvMesh, thermo: str, fvSchemes: str, fvSolution: str, D):
+        self.mesh = mesh
+        self.D = as_f64(D)
+        self.nO = int(self.D.shape[0])
+        h = C.c_void_p()
+        check(lib().icsf_hb_create(mesh._h, thermo.encode(), fvSchemes.encode(), fvSolution.encode(), self.nO,
+                                   ptr(self.D), C.byref(h)))
+        self._h = h
+        self.nCells = mesh.nCells
+
+    @classmethod
+    def from_cases(cls, instances, D, mesh: Optional[fvMesh] = None) -> "hbCase":
+        c0 = instances[0]
+        mesh = mesh or fvMesh(c0.mesh)
+        self = cls(mesh, c0.thermo, c0.fvSchemes, c0.fvSolution, D)
+        for J, c in enumerate(instances):
+            for i, bc in enumerate(c.bcs):
+                self.set_boundary(J, i, bc.p_type, bc.U_type, bc.T_type, bc.p_val, bc.U_val, bc.T_val)
+            self.set_fields(J, c.rho, c.U, c.T)
+        return self
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().icsf_hb_destroy(h)
+            self._h = None
+
+    def set_boundary(self, J, patch, p_type, U_type, T_type, p_val=None, U_val=None, T_val=None):
+        pv = as_f64(p_val) if p_type == BC_FIXED_VALUE else None
+        uv = as_f64(U_val) if U_type == BC_FIXED_VALUE else None
+        tv = as_f64(T_val) if T_type == BC_FIXED_VALUE else None
+        check(lib().icsf_hb_set_boundary(self._h, J, patch, p_type, U_type, T_type, ptr(pv), ptr(uv), ptr(tv)))
+
+    def set_fields(self, J, rho, U, T):
+        check(lib().icsf_hb_set_fields(self._h, J, ptr(as_f64(rho)), ptr(as_f64(U)), ptr(as_f64(T))))
+
+    def fields(self, J):
+        n = self.nCells
+        rho, U, T, p, W = np.empty(n), np.empty((n, 3)), np.empty(n), np.empty(n), np.empty((5, n))
+        check(lib().icsf_hb_get_fields(self._h, J, ptr(rho), ptr(U), ptr(T), ptr(p), ptr(W)))
+        return dict(rho=rho, U=U, T=T, p=p, W=W)
+
+    def residual(self):
+        b = np.empty((self.nO, 5, self.nCells))
+        check(lib().icsf_hb_residual(self._h, ptr(b)))
+        return b
+
+    def createJacobian(self):
+        check(lib().icsf_hb_assemble_jacobian(self._h))
+
+    def matrixMul(self, x):
+        y = np.empty((self.nO, 5, self.nCells))
+        check(lib().icsf_hb_matrix_mul(self._h, ptr(as_f64(x)), ptr(y)))
+        return y
+
+    def precondition(self, r):
+        r = np.array(r, dtype=np.float64, order="C", copy=True)
+        check(lib().icsf_hb_precondition(self._h, ptr(r)))
+        return r
+
+    def solveForIncr(self, b):
+        dW = np.empty((self.nO, 5, self.nCells))
+        res = np.empty((self.nO, 10))
+        it = np.zeros(1, np.int32)
+        check(lib().icsf_hb_solve_for_incr(self._h, ptr(as_f64(b)), ptr(dW), ptr(res), ptr(it)))
+        return dW, res, int(it[0])
+
+    def pseudo_steps(self, n: int, residuals: bool = True):
+        res = np.empty((n, self.nO, 10)) if residuals else None
+        check(lib().icsf_hb_pseudo_steps(self._h, n, ptr(res)))
+        return res
+
+    def synchronize(self):
+        check(lib().icsf_hb_synchronize(self._h))

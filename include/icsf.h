@@ -44,6 +44,7 @@ extern "C" {
 
 typedef struct icsf_mesh icsf_mesh;
 typedef struct icsf_case icsf_case;
+typedef struct icsf_hb icsf_hb;
 
 /* Text of the last error on the calling thread. */
 const char* icsf_last_error(void);
@@ -229,6 +230,50 @@ int icsf_synchronize(icsf_case* c);
  * orthogonalisation + Givens of one Arnoldi step, 5 other. */
 int icsf_timing_enable(icsf_case* c, int mask);
 int icsf_timing_read(icsf_case* c, int n, double* ms, long long* count);
+
+/* ---- harmonic balance (dbnsFullyImplicitHBFoam) ----------------------
+ * Spectral operator D [nInstants][nInstants] (row-major) of
+ * HBZone::updateHBOperators (HBZone.C:270-356) for the omega list of
+ * HBZone::setOmegaList (HBZone.C:46-91): 0, +k w_i (k = 1..harmonics_i),
+ * then the negative frequencies in reverse order; frequencies are angular
+ * (rad/s).  Instants (HBZoneList::setInstants, HBZoneList.C:253-346):
+ * t_n = n selectedPeriod / nInstants, or, for selectedPeriod <= 0, the
+ * period in [T0, 5 T0] with the smallest condition number.  instants
+ * [nInstants] and condition may be NULL. */
+int icsf_hb_operator(int nFreq, const double* frequencies, const int* harmonics, int nInstants,
+                     double selectedPeriod, double* instants, double* D, double* condition);
+
+/* HB case: nInstants instances of the case (same dictionaries) on one
+ * mesh, coupled by D (the coupledMatrix of 2 nInstants scalars and
+ * nInstants vectors built in dbnsFullyImplicitHBFoam/outerLoop.H:91-205).
+ * fvSolution must select GMRES with LUSGS in double precision. */
+int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, const char* solution, int nInstants,
+                   const double* D, icsf_hb** out);
+int icsf_hb_destroy(icsf_hb* h);
+/* Per-instant boundary conditions and fields (as icsf_set_boundary /
+ * icsf_set_fields / icsf_get_fields). */
+int icsf_hb_set_boundary(icsf_hb* h, int instant, int patch, int pType, int UType, int TType, const double* pValue,
+                         const double* UValue, const double* TValue);
+int icsf_hb_set_fields(icsf_hb* h, int instant, const double* rho, const double* U, const double* T);
+int icsf_hb_get_fields(icsf_hb* h, int instant, double* rho, double* U, double* T, double* p, double* W);
+/* b [nInstants][5][nCells]: R V of each instance plus the HB source
+ * -V sum_K D_JK W_K (residualsUpdate.H:72-74, HBZoneTemplates.C:39-93). */
+int icsf_hb_residual(icsf_hb* h, double* b);
+/* Instance Jacobians plus V D_JK I blocks (outerLoop.H:155-205,
+ * HBZone.C:435-518); LU-SGS diagonal over all instances (lusgs.C:50-125). */
+int icsf_hb_assemble_jacobian(icsf_hb* h);
+/* y = A x over [nInstants][5][nCells] vectors (coupledMatrix::matrixMul,
+ * coupledMatrix.C:66-123). */
+int icsf_hb_matrix_mul(icsf_hb* h, const double* x, double* y);
+/* r <- LU-SGS(r) (lusgs::precondition, lusgs.C:220-382). */
+int icsf_hb_precondition(icsf_hb* h, double* r);
+/* GMRES increment for b at the current states (coupledMatrix::solveForIncr,
+ * outerLoop.H:213-219); residuals [nInstants][10] may be NULL. */
+int icsf_hb_solve_for_incr(icsf_hb* h, const double* b, double* dW, double* residuals, int* nIter);
+/* nSteps HB pseudo-steps (dbnsFullyImplicitHBFoam/outerLoop.H:1-230 +
+ * updateFields.H per instance); residuals [nSteps][nInstants][10] or NULL. */
+int icsf_hb_pseudo_steps(icsf_hb* h, int nSteps, double* residuals);
+int icsf_hb_synchronize(icsf_hb* h);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,7 @@
 // (coupledMatrix with nScal=2, nVect=1; outerLoop.H:53).
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <deque>
@@ -1606,6 +1607,387 @@ int pseudoStep(Case& k, Res& out) {
 
 #include "icsf_tangent.h"
 
+// ----------------------------------------------------- harmonic balance ---
+// dbnsFullyImplicitHBFoam (outerLoop.H:1-230, residualsUpdate.H:72-74) with
+// the spectral operator of HBZone (HBZone.C:270-518, HBZoneTemplates.C:
+// 39-93).  nO instances of the single-instance case on one mesh, coupled
+// by the source -V sum_K D_JK W_K and by diagonal-only blocks V D_JK I.
+// Coupled-vector variable order as coupledMatrix(2 nO scalars, nO vectors):
+// rho_0, rhoE_0, rho_1, rhoE_1, ..., then rhoU_0, rhoU_1, ...
+struct HB {
+    int nO = 0;
+    std::vector<Case*> k;
+    vec D;   // [nO][nO]
+    vec rD;  // LU-SGS diagonal shared by all instances
+    int corr = 0;
+    bool haveInit = false, havePrev = false;
+    vec initS, initV, prevS, prevV;  // [2 nO], [3 nO]
+};
+
+// HBZone::addSource (HBZoneTemplates.C:39-93): source_J = -sum_K V D_JK W_K
+void hbSource(const HB& h, int J, vec& src) {
+    const Mesh& m = *h.k[0]->m;
+    const int nC = m.nC;
+    src.assign(5 * (size_t)nC, 0.0);
+    for (int c = 0; c < nC; ++c) {
+        double s0 = 0, s1 = 0, v[3] = {0, 0, 0};
+        for (int K = 0; K < h.nO; ++K) {
+            const Case& q = *h.k[K];
+            const double VD = m.V[c] * h.D[(size_t)J * h.nO + K];
+            s0 -= VD * q.rho[c];
+            s1 -= VD * q.rhoE[c];
+            for (int d = 0; d < 3; ++d) v[d] -= VD * q.rhoU[3 * c + d];
+        }
+        src[c] = s0;
+        src[nC + c] = s1;
+        for (int d = 0; d < 3; ++d) src[(2 + d) * (size_t)nC + c] = v[d];
+    }
+}
+
+// b_J = R_J V + source_J  (residualsUpdate.H:72-74)
+void hbResidual(HB& h, std::vector<vec>& b) {
+    b.resize(h.nO);
+    std::vector<vec> src(h.nO);
+    for (int J = 0; J < h.nO; ++J) hbSource(h, J, src[J]);
+    for (int J = 0; J < h.nO; ++J) {
+        residual(*h.k[J], b[J]);
+        for (size_t i = 0; i < b[J].size(); ++i) b[J][i] = b[J][i] + src[J][i];
+    }
+}
+
+// Per-instance Jacobians (after localDeltaT) with V D_JJ added to the
+// diagonal (HB.addBlock, outerLoop.H:159-177), then the shared LU-SGS
+// diagonal of lusgs::lusgs over all 2 nO scalar and nO vector diagonals.
+int hbAssemble(HB& h) {
+    const Mesh& m = *h.k[0]->m;
+    for (int J = 0; J < h.nO; ++J) {
+        Case& q = *h.k[J];
+        localDeltaT(q);
+        assemble(q);
+        const double dJJ = h.D[(size_t)J * h.nO + J];
+        for (int c = 0; c < m.nC; ++c) {
+            const double VD = m.V[c] * dJJ;
+            q.SS[0][0].diag[c] += VD;
+            q.SS[1][1].diag[c] += VD;
+            q.VV.diag[9 * c] += VD;
+            q.VV.diag[9 * c + 4] += VD;
+            q.VV.diag[9 * c + 8] += VD;
+        }
+    }
+    h.rD.assign(m.nC, GREAT);
+    for (int c = 0; c < m.nC; ++c) {
+        double r = h.rD[c];
+        for (int J = 0; J < h.nO; ++J) {
+            r = 1.0 / std::max(1.0 / r, std::fabs(h.k[J]->SS[0][0].diag[c]));
+            r = 1.0 / std::max(1.0 / r, std::fabs(h.k[J]->SS[1][1].diag[c]));
+        }
+        for (int J = 0; J < h.nO; ++J) {
+            r = 1.0 / std::max(1.0 / r, std::fabs(h.k[J]->VV.diag[9 * c]));
+            r = 1.0 / std::max(1.0 / r, std::fabs(h.k[J]->VV.diag[9 * c + 4]));
+            r = 1.0 / std::max(1.0 / r, std::fabs(h.k[J]->VV.diag[9 * c + 8]));
+        }
+        h.rD[c] = r;
+        if (r < VSMALL) return -3;
+    }
+    for (int J = 0; J < h.nO; ++J) h.k[J]->rD = h.rD;
+    return 0;
+}
+
+// coupledMatrix::matrixMul (coupledMatrix.C:66-123) over the HB blocks, in
+// its order: scalar rows sum the scalar columns (rho_K, rhoE_K for K =
+// 0..nO-1) then the vector columns; vector rows the scalar columns then the
+// vector columns.  Cross-instance blocks are diagonal V D_JK (I); the
+// instance-diagonal blocks are the instance's Jacobian; all-zero
+// cross-instance blocks add nothing.
+void hbAmul(const HB& h, const std::vector<CVec>& x, std::vector<CVec>& y) {
+    const Mesh& m = *h.k[0]->m;
+    const int nC = m.nC;
+    y.resize(h.nO);
+    vec t;
+    for (int J = 0; J < h.nO; ++J) {
+        const Case& q = *h.k[J];
+        CVec& yJ = y[J];
+        yJ.init(nC);
+        vec* S[2] = {&yJ.s0, &yJ.s1};
+        const vec* XJ[2] = {&x[J].s0, &x[J].s1};
+        for (int i = 0; i < 2; ++i) {
+            for (int K = 0; K < h.nO; ++K) {
+                if (K == J) {
+                    for (int j = 0; j < 2; ++j) {
+                        amulSS(m, q.SS[i][j], *XJ[j], t);
+                        for (int c = 0; c < nC; ++c) (*S[i])[c] += t[c];
+                    }
+                } else {
+                    const vec& xk = i == 0 ? x[K].s0 : x[K].s1;
+                    const double dJK = h.D[(size_t)J * h.nO + K];
+                    for (int c = 0; c < nC; ++c) (*S[i])[c] += (m.V[c] * dJK) * xk[c];
+                }
+            }
+            amulSV(m, q.SV[i], x[J].v, t);
+            for (int c = 0; c < nC; ++c) (*S[i])[c] += t[c];
+        }
+        for (int j = 0; j < 2; ++j) {
+            amulVS(m, q.VS[j], *XJ[j], t);
+            for (size_t c = 0; c < t.size(); ++c) yJ.v[c] += t[c];
+        }
+        for (int K = 0; K < h.nO; ++K) {
+            if (K == J) {
+                amulVV(m, q.VV, x[J].v, t);
+                for (size_t c = 0; c < t.size(); ++c) yJ.v[c] += t[c];
+            } else {
+                const double dJK = h.D[(size_t)J * h.nO + K];
+                for (int c = 0; c < nC; ++c)
+                    for (int d = 0; d < 3; ++d) yJ.v[3 * c + d] += (m.V[c] * dJK) * x[K].v[3 * c + d];
+            }
+        }
+    }
+}
+
+// lusgs::precondition: the cross-instance blocks have no faces, so the
+// sweeps act per instance, with the shared diagonal.
+void hbPrecondition(const HB& h, std::vector<CVec>& r) {
+    for (int J = 0; J < h.nO; ++J) precondition(*h.k[J], r[J]);
+}
+
+struct HBRes {
+    vec sInit, sFinal, vInit, vFinal;  // [2 nO], [3 nO]
+    int nIter = 0;
+};
+
+// gmres::solveDelta over the HB coupled vectors (same algorithm as
+// gmresSolve(): left preconditioning, MGS, per-variable norm factors).
+HBRes hbGmres(HB& h, const std::vector<CVec>& W, const std::vector<CVec>& b, std::vector<CVec>& dW) {
+    const Mesh& m = *h.k[0]->m;
+    const int nC = m.nC, nO = h.nO, N = h.k[0]->s.nDirs;
+    const Settings& st = h.k[0]->s;
+    HBRes res;
+    res.sInit.assign(2 * nO, 0); res.sFinal.assign(2 * nO, 0); res.vInit.assign(3 * nO, 0); res.vFinal.assign(3 * nO, 0);
+    dW = W;
+    for (int J = 0; J < nO; ++J) {
+        double a0 = 0, a1 = 0; V3 av{0, 0, 0};
+        for (int c = 0; c < nC; ++c) { a0 += W[J].s0[c]; a1 += W[J].s1[c]; av = av + v3(&W[J].v[3 * c]); }
+        a0 /= nC; a1 /= nC; av = av / (double)nC;
+        for (int c = 0; c < nC; ++c) {
+            dW[J].s0[c] -= a0; dW[J].s1[c] -= a1;
+            dW[J].v[3 * c] -= av.x; dW[J].v[3 * c + 1] -= av.y; dW[J].v[3 * c + 2] -= av.z;
+        }
+    }
+    std::vector<CVec> t;
+    hbAmul(h, dW, t);
+    vec sNF(2 * nO), vNF(nO);
+    for (int J = 0; J < nO; ++J) {
+        double a = 0, bb = 0, v = 0;
+        for (int c = 0; c < nC; ++c) a += std::fabs(t[J].s0[c]) + std::fabs(b[J].s0[c]);
+        for (int c = 0; c < nC; ++c) bb += std::fabs(t[J].s1[c]) + std::fabs(b[J].s1[c]);
+        for (int c = 0; c < nC; ++c) v += mag(v3(&t[J].v[3 * c])) + mag(v3(&b[J].v[3 * c]));
+        sNF[2 * J] = a + VSMALL; sNF[2 * J + 1] = bb + VSMALL; vNF[J] = v + VSMALL;
+        res.sInit[2 * J] = vsumMag(b[J].s0) / sNF[2 * J];
+        res.sInit[2 * J + 1] = vsumMag(b[J].s1) / sNF[2 * J + 1];
+        double cm[3] = {0, 0, 0};
+        for (int c = 0; c < nC; ++c) for (int d = 0; d < 3; ++d) cm[d] += std::fabs(b[J].v[3 * c + d]);
+        for (int d = 0; d < 3; ++d) res.vInit[3 * J + d] = cm[d] / vNF[J];
+    }
+    res.sFinal = res.sInit; res.vFinal = res.vInit;
+    for (int J = 0; J < nO; ++J) dW[J].init(nC);
+    std::vector<CVec> r = b;
+    auto dotv = [&](const std::vector<CVec>& x, const std::vector<CVec>& y) {
+        double s = 0, a;
+        for (int J = 0; J < nO; ++J) {
+            a = 0; for (int c = 0; c < nC; ++c) a += x[J].s0[c] * y[J].s0[c];
+            s += a;
+            a = 0; for (int c = 0; c < nC; ++c) a += x[J].s1[c] * y[J].s1[c];
+            s += a;
+        }
+        for (int J = 0; J < nO; ++J) {
+            a = 0; for (int c = 0; c < nC; ++c) a += dot(v3(&x[J].v[3 * c]), v3(&y[J].v[3 * c]));
+            s += a;
+        }
+        return s;
+    };
+    std::vector<std::vector<double>> H(N, std::vector<double>(N, 0.0));
+    vec yh(N, 0), bh(N + 1, 0), cs(N, 0), sn(N, 0);
+    std::vector<std::vector<CVec>> Vk(N);
+    while (true) {
+        hbPrecondition(h, r);
+        double beta = std::sqrt(dotv(r, r));
+        std::fill(bh.begin(), bh.end(), 0.0);
+        bh[0] = beta;
+        for (int i = 0; i < N; ++i) {
+            Vk[i].resize(nO);
+            for (int J = 0; J < nO; ++J) {
+                CVec& v = Vk[i][J];
+                v.init(nC);
+                for (int c = 0; c < nC; ++c) { v.s0[c] = r[J].s0[c] / beta; v.s1[c] = r[J].s1[c] / beta; }
+                for (size_t q = 0; q < v.v.size(); ++q) v.v[q] = r[J].v[q] / beta;
+            }
+            hbAmul(h, Vk[i], r);
+            hbPrecondition(h, r);
+            for (int j = 0; j <= i; ++j) {
+                H[j][i] = dotv(r, Vk[j]);
+                for (int J = 0; J < nO; ++J) {
+                    for (int c = 0; c < nC; ++c) { r[J].s0[c] -= H[j][i] * Vk[j][J].s0[c]; r[J].s1[c] -= H[j][i] * Vk[j][J].s1[c]; }
+                    for (size_t q = 0; q < r[J].v.size(); ++q) r[J].v[q] -= H[j][i] * Vk[j][J].v[q];
+                }
+            }
+            beta = std::sqrt(dotv(r, r));
+            for (int j = 0; j < i; ++j) {
+                const double Hji = H[j][i];
+                H[j][i] = cs[j] * Hji - sn[j] * H[j + 1][i];
+                H[j + 1][i] = sn[j] * Hji + cs[j] * H[j + 1][i];
+            }
+            givens(H[i][i], beta, cs[i], sn[i]);
+            const double bhi = bh[i];
+            bh[i] = cs[i] * bhi - sn[i] * bh[i + 1];
+            bh[i + 1] = sn[i] * bhi + cs[i] * bh[i + 1];
+            H[i][i] = cs[i] * H[i][i] - sn[i] * beta;
+        }
+        for (int i = N - 1; i >= 0; --i) {
+            double s = bh[i];
+            for (int j = i + 1; j < N; ++j) s -= H[i][j] * yh[j];
+            double hd = H[i][i];
+            if (hd >= 0) hd = hd + VSMALL; else hd = hd - VSMALL;
+            yh[i] = s / hd;
+        }
+        for (int J = 0; J < nO; ++J)
+            for (int i = 0; i < N; ++i) {
+                for (int c = 0; c < nC; ++c) { dW[J].s0[c] += yh[i] * Vk[i][J].s0[c]; dW[J].s1[c] += yh[i] * Vk[i][J].s1[c]; }
+                for (size_t q = 0; q < dW[J].v.size(); ++q) dW[J].v[q] += yh[i] * Vk[i][J].v[q];
+            }
+        hbAmul(h, dW, r);
+        for (int J = 0; J < nO; ++J) {
+            for (int c = 0; c < nC; ++c) { r[J].s0[c] = b[J].s0[c] - r[J].s0[c]; r[J].s1[c] = b[J].s1[c] - r[J].s1[c]; }
+            for (size_t q = 0; q < r[J].v.size(); ++q) r[J].v[q] = b[J].v[q] - r[J].v[q];
+            res.sFinal[2 * J] = vsumMag(r[J].s0) / sNF[2 * J];
+            res.sFinal[2 * J + 1] = vsumMag(r[J].s1) / sNF[2 * J + 1];
+            double cm[3] = {0, 0, 0};
+            for (int c = 0; c < nC; ++c) for (int d = 0; d < 3; ++d) cm[d] += std::fabs(r[J].v[3 * c + d]);
+            for (int d = 0; d < 3; ++d) res.vFinal[3 * J + d] = (d == m.emptyDir) ? 0.0 : cm[d] / vNF[J];
+        }
+        res.nIter++;
+        if (res.nIter < st.minIter) continue;
+        double mx = -1e300, rel = -1e300;
+        const double RVS = 1.0e-150;
+        for (int i = 0; i < 2 * nO; ++i) { mx = std::max(mx, res.sFinal[i]); rel = std::max(rel, res.sFinal[i] / (res.sInit[i] + RVS)); }
+        for (int i = 0; i < 3 * nO; ++i) {
+            mx = std::max(mx, res.vFinal[i]);
+            if (i % 3 != m.emptyDir) rel = std::max(rel, res.vFinal[i] / (res.vInit[i] + RVS));
+        }
+        if (res.nIter >= st.maxIter || mx < st.tol || rel < st.relTol) break;
+    }
+    if (m.emptyDir >= 0)
+        for (int J = 0; J < nO; ++J)
+            for (int c = 0; c < nC; ++c) dW[J].v[3 * c + m.emptyDir] = 0.0;
+    return res;
+}
+
+void hbConserved(const Case& q, CVec& W) {
+    const int nC = q.m->nC;
+    W.init(nC);
+    W.s0 = q.rho;
+    W.s1 = q.rhoE;
+    W.v = q.rhoU;
+}
+
+// One HB pseudo-step (outerLoop.H:1-230 + updateFields per instance).
+int hbPseudoStep(HB& h, HBRes& out) {
+    const Mesh& m = *h.k[0]->m;
+    const int nC = m.nC, nO = h.nO;
+    const Settings& st = h.k[0]->s;
+    h.corr++;
+    if (h.haveInit) {
+        if (h.corr != 1 && h.havePrev) {
+            double nI = 0, nP = 0;
+            for (int J = 0; J < nO; ++J) {
+                nI += sqr(h.initS[2 * J]) + sqr(h.initS[2 * J + 1]) +
+                      (sqr(h.initV[3 * J]) + sqr(h.initV[3 * J + 1]) + sqr(h.initV[3 * J + 2]));
+                nP += sqr(h.prevS[2 * J]) + sqr(h.prevS[2 * J + 1]) +
+                      (sqr(h.prevV[3 * J]) + sqr(h.prevV[3 * J + 1]) + sqr(h.prevV[3 * J + 2]));
+            }
+            double ratio = std::sqrt(nP) / std::sqrt(nI);
+            ratio = std::max(std::min(ratio, st.coIncr), st.coDecr);
+            for (int J = 0; J < nO; ++J)
+                for (int c = 0; c < nC; ++c) {
+                    h.k[J]->pseudoCo[c] *= ratio;
+                    h.k[J]->pseudoCo[c] = std::max(std::min(h.k[J]->pseudoCo[c], st.coMax), st.coMin);
+                }
+        }
+        h.prevS = h.initS;
+        h.prevV = h.initV;
+        h.havePrev = true;
+    }
+    std::vector<vec> b;
+    hbResidual(h, b);
+    int rc = hbAssemble(h);
+    if (rc) return rc;
+    std::vector<CVec> W(nO), B(nO), dW;
+    for (int J = 0; J < nO; ++J) {
+        hbConserved(*h.k[J], W[J]);
+        toC(b[J].data(), nC, B[J]);
+    }
+    out = hbGmres(h, W, B, dW);
+    h.haveInit = true;
+    h.initS = out.sInit;
+    h.initV = out.vInit;
+    for (int J = 0; J < nO; ++J) {
+        Case& k = *h.k[J];
+        if (k.s.localTS && k.s.bounding) {
+            vec fac(nC, 1.0);
+            auto bad = [&](int c) {
+                const double rmin = k.s.lowerBound * W[J].s0[c];
+                const double Uo[3] = {W[J].v[3 * c] / W[J].s0[c], W[J].v[3 * c + 1] / W[J].s0[c], W[J].v[3 * c + 2] / W[J].s0[c]};
+                const double emin = k.s.lowerBound * (W[J].s1[c] / W[J].s0[c] - 0.5 * (Uo[0] * Uo[0] + Uo[1] * Uo[1] + Uo[2] * Uo[2]));
+                const double Un[3] = {k.rhoU[3 * c] / k.rho[c], k.rhoU[3 * c + 1] / k.rho[c], k.rhoU[3 * c + 2] / k.rho[c]};
+                const double et = k.rhoE[c] / k.rho[c] - 0.5 * (Un[0] * Un[0] + Un[1] * Un[1] + Un[2] * Un[2]);
+                return k.rho[c] < rmin || et < emin || et < SMALL;
+            };
+            for (int f = 0; f < m.nI; ++f) {
+                const int o = m.own[f], n = m.nei[f];
+                if (bad(o)) { fac[o] = std::min(0.5, fac[o]); fac[n] = std::min(0.75, fac[n]); }
+                if (bad(n)) { fac[n] = std::min(0.5, fac[n]); fac[o] = std::min(0.75, fac[o]); }
+            }
+            for (int c = 0; c < nC; ++c) k.pseudoCo[c] *= fac[c];
+        }
+        updateFields(k, dW[J]);
+    }
+    return 0;
+}
+
+// HBZone::setOmegaList / updateHBOperators restated: E^-1 = [exp(i w_k t_n)],
+// E = its inverse (square case), D = -(Im(E^-1) A Re(E) + Re(E^-1) A Im(E)).
+int hbOperator(int nT, const double* omega, int nO, const double* t, double* D) {
+    if (nO != nT) return -1;
+    typedef std::complex<double> cx;
+    std::vector<cx> A((size_t)nT * nT), I((size_t)nT * nT, cx(0, 0));
+    for (int n = 0; n < nO; ++n)
+        for (int k = 0; k < nT; ++k) A[(size_t)n * nT + k] = cx(std::cos(omega[k] * t[n]), std::sin(omega[k] * t[n]));
+    const std::vector<cx> Einv = A;
+    for (int i = 0; i < nT; ++i) I[(size_t)i * nT + i] = 1.0;
+    for (int col = 0; col < nT; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < nT; ++r)
+            if (std::abs(A[(size_t)r * nT + col]) > std::abs(A[(size_t)piv * nT + col])) piv = r;
+        if (std::abs(A[(size_t)piv * nT + col]) < 1e-13) return -2;
+        for (int q = 0; q < nT; ++q) { std::swap(A[(size_t)piv * nT + q], A[(size_t)col * nT + q]); std::swap(I[(size_t)piv * nT + q], I[(size_t)col * nT + q]); }
+        const cx d = A[(size_t)col * nT + col];
+        for (int q = 0; q < nT; ++q) { A[(size_t)col * nT + q] /= d; I[(size_t)col * nT + q] /= d; }
+        for (int r = 0; r < nT; ++r) {
+            if (r == col) continue;
+            const cx f = A[(size_t)r * nT + col];
+            for (int q = 0; q < nT; ++q) { A[(size_t)r * nT + q] -= f * A[(size_t)col * nT + q]; I[(size_t)r * nT + q] -= f * I[(size_t)col * nT + q]; }
+        }
+    }
+    for (int l = 0; l < nO; ++l)
+        for (int m2 = 0; m2 < nO; ++m2) {
+            double d0 = 0, d1 = 0;
+            for (int n = 0; n < nT; ++n) {
+                d0 += (Einv[(size_t)l * nT + n].imag() * omega[n]) * I[(size_t)n * nO + m2].real();
+                d1 += (Einv[(size_t)l * nT + n].real() * omega[n]) * I[(size_t)n * nO + m2].imag();
+            }
+            D[(size_t)l * nO + m2] = -(d0 + d1);
+        }
+    return 0;
+}
+
 }  // namespace
 
 // ======================================================== C interface ====
@@ -2064,6 +2446,91 @@ void orc_residual_jac_dense(void* h, const double* W, double* JW, double* JS, do
         for (size_t i = 0; i < n5; ++i) JC[i * n3 + j] = r[i].d;
         c[j].d = 0.0;
     }
+}
+
+
+// ------------------------------------------------------ harmonic balance --
+void* orc_hb_new(int nO, void** cases, const double* D) {
+    HB* h = new HB;
+    h->nO = nO;
+    for (int J = 0; J < nO; ++J) h->k.push_back((Case*)cases[J]);
+    h->D.assign(D, D + (size_t)nO * nO);
+    return h;
+}
+void orc_hb_free(void* h) { delete (HB*)h; }
+
+void orc_hb_residual(void* hp, double* b) {
+    HB& h = *(HB*)hp;
+    std::vector<vec> bb;
+    hbResidual(h, bb);
+    const size_t n5 = 5 * (size_t)h.k[0]->m->nC;
+    for (int J = 0; J < h.nO; ++J) std::copy(bb[J].begin(), bb[J].end(), b + J * n5);
+}
+
+int orc_hb_assemble(void* hp) { return hbAssemble(*(HB*)hp); }
+
+static void hbSplit(const HB& h, const double* x, std::vector<CVec>& v) {
+    const int nC = h.k[0]->m->nC;
+    v.resize(h.nO);
+    for (int J = 0; J < h.nO; ++J) toC(x + (size_t)J * 5 * nC, nC, v[J]);
+}
+static void hbJoin(const HB& h, const std::vector<CVec>& v, double* x) {
+    const int nC = h.k[0]->m->nC;
+    for (int J = 0; J < h.nO; ++J) fromC(v[J], nC, x + (size_t)J * 5 * nC);
+}
+
+void orc_hb_amul(void* hp, const double* x, double* y) {
+    HB& h = *(HB*)hp;
+    std::vector<CVec> X, Y;
+    hbSplit(h, x, X);
+    hbAmul(h, X, Y);
+    hbJoin(h, Y, y);
+}
+
+void orc_hb_precondition(void* hp, double* r) {
+    HB& h = *(HB*)hp;
+    std::vector<CVec> R;
+    hbSplit(h, r, R);
+    hbPrecondition(h, R);
+    hbJoin(h, R, r);
+}
+
+static void hbResOut(const HB& h, const HBRes& r, double* res) {
+    for (int J = 0; J < h.nO; ++J) {
+        double* o = res + 10 * J;
+        o[0] = r.sInit[2 * J]; o[1] = r.sInit[2 * J + 1];
+        for (int d = 0; d < 3; ++d) o[2 + d] = r.vInit[3 * J + d];
+        o[5] = r.sFinal[2 * J]; o[6] = r.sFinal[2 * J + 1];
+        for (int d = 0; d < 3; ++d) o[7 + d] = r.vFinal[3 * J + d];
+    }
+}
+
+// GMRES on the assembled HB system for the current state and the given b.
+int orc_hb_solve(void* hp, const double* b, double* dW, double* res, int* nIter) {
+    HB& h = *(HB*)hp;
+    std::vector<CVec> W(h.nO), B, D;
+    for (int J = 0; J < h.nO; ++J) hbConserved(*h.k[J], W[J]);
+    hbSplit(h, b, B);
+    HBRes r = hbGmres(h, W, B, D);
+    hbJoin(h, D, dW);
+    hbResOut(h, r, res);
+    if (nIter) *nIter = r.nIter;
+    return 0;
+}
+
+int orc_hb_pseudo_steps(void* hp, int n, double* res) {
+    HB& h = *(HB*)hp;
+    for (int i = 0; i < n; ++i) {
+        HBRes r;
+        int rc = hbPseudoStep(h, r);
+        if (rc) return rc;
+        hbResOut(h, r, res + (size_t)i * 10 * h.nO);
+    }
+    return 0;
+}
+
+int orc_hb_operator(int nT, const double* omega, int nO, const double* t, double* D) {
+    return hbOperator(nT, omega, nO, t, D);
 }
 
 }  // extern "C"

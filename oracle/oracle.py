@@ -45,6 +45,15 @@ _SIG = {
     "orc_residual_w": (None, [_vp, _vp, _vp, _vp, _vp]),
     "orc_residual_jvp": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "orc_residual_jac_dense": (None, [_vp, _vp, _vp, _vp, _vp]),
+    "orc_hb_new": (_vp, [_i, _vp, _vp]),
+    "orc_hb_free": (None, [_vp]),
+    "orc_hb_residual": (None, [_vp, _vp]),
+    "orc_hb_assemble": (_i, [_vp]),
+    "orc_hb_amul": (None, [_vp, _vp, _vp]),
+    "orc_hb_precondition": (None, [_vp, _vp]),
+    "orc_hb_solve": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "orc_hb_pseudo_steps": (_i, [_vp, _i, _vp]),
+    "orc_hb_operator": (_i, [_i, _vp, _i, _vp, _vp]),
 }
 _lib = None
 
@@ -279,3 +288,70 @@ class OracleCase:
         JW, JS, JC = np.empty((n5, n5)), np.empty((n5, n3)), np.empty((n5, n3))
         lib().orc_residual_jac_dense(self.h, _p(_f64(W)), _p(JW), _p(JS), _p(JC))
         return JW, JS, JC
+
+
+def hb_operator(omegas, instants):
+    """D [nO][nO] of the oracle's restatement of HBZone::updateHBOperators."""
+    om, t = _f64(omegas), _f64(instants)
+    D = np.empty((len(t), len(t)))
+    rc = lib().orc_hb_operator(len(om), _p(om), len(t), _p(t), _p(D))
+    if rc:
+        raise RuntimeError(f"oracle HB operator failed ({rc})")
+    return D
+
+
+class OracleHB:
+    """nO instances of one case coupled by the HB operator D."""
+
+    def __init__(self, instances, D):
+        self.inst = [OracleCase(c) for c in instances]
+        self.nO = len(self.inst)
+        self.nCells = self.inst[0].nCells
+        self.D = _f64(D)
+        arr = (C.c_void_p * self.nO)(*[o.h for o in self.inst])
+        self._arr = arr
+        self.h = lib().orc_hb_new(self.nO, C.cast(arr, C.c_void_p), _p(self.D))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_hb_free(self.h)
+            self.h = None
+
+    def residual(self):
+        b = np.empty((self.nO, 5, self.nCells))
+        lib().orc_hb_residual(self.h, _p(b))
+        return b
+
+    def createJacobian(self):
+        rc = lib().orc_hb_assemble(self.h)
+        if rc:
+            raise RuntimeError(f"oracle HB assemble failed ({rc})")
+
+    def matrixMul(self, x):
+        y = np.empty((self.nO, 5, self.nCells))
+        lib().orc_hb_amul(self.h, _p(_f64(x)), _p(y))
+        return y
+
+    def precondition(self, r):
+        r = np.array(r, dtype=np.float64, order="C", copy=True)
+        lib().orc_hb_precondition(self.h, _p(r))
+        return r
+
+    def solveForIncr(self, b):
+        dW = np.empty((self.nO, 5, self.nCells))
+        res = np.empty((self.nO, 10))
+        it = np.zeros(1, np.int32)
+        rc = lib().orc_hb_solve(self.h, _p(_f64(b)), _p(dW), _p(res), _p(it))
+        if rc:
+            raise RuntimeError(f"oracle HB solve failed ({rc})")
+        return dW, res, int(it[0])
+
+    def pseudo_steps(self, n):
+        res = np.empty((n, self.nO, 10))
+        rc = lib().orc_hb_pseudo_steps(self.h, n, _p(res))
+        if rc:
+            raise RuntimeError(f"oracle HB pseudo-step failed ({rc})")
+        return res
+
+    def fields(self, J):
+        return self.inst[J].fields()
