@@ -17,6 +17,9 @@ struct Levels {
     int* fwdCells = nullptr;                  // device, [nC]
     int* bwdCells = nullptr;                  // device, [nC]
     bool fwdSkip = true;                      // level 0 has no lower neighbours
+    // two levels each way, forward level 1 == backward level 0 (a two-colour
+    // numbering: every neighbour of a level-1 cell is a level-0 cell)
+    bool twoColour = false;
 };
 
 // Scratch for two-stage reductions.
@@ -51,6 +54,10 @@ struct GmresArgs {
     int nVec = 0;
     std::function<void(const void*, const double*, void*, void*)> op;
     std::function<void(void*)> pre;
+    // fuse the Arnoldi SpMV with the LU-SGS sweeps when the level structure
+    // allows it (same algorithm; the forward-sweep terms of a row are summed
+    // in three partial sums instead of one running difference)
+    bool fuse = false;
 };
 
 // Harmonic-balance coupling: per-instance block storage (FP64) and the
@@ -104,6 +111,10 @@ void launchSpmv(const MeshView& m, int blkFp32, int vecFp32, const void* blk, co
                 const double* scal, void* vout, void* y, hipStream_t st);
 void launchLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp32, const void* blk, const double* rD,
                  void* r, hipStream_t st);
+// y = LU-SGS(A (x/scal)) in three passes on a two-colour level structure
+// (false: not applicable, nothing launched).
+bool launchSpmvLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp32, const void* blk, const void* dg,
+                     const double* rD, const void* x, const double* scal, void* vout, void* y, hipStream_t st);
 void launchJacobiSetup(int nC, int blkFp32, const void* dg, void* dinv, hipStream_t st);
 void launchJacobiApply(int nC, int blkFp32, int vecFp32, const void* dinv, void* r, hipStream_t st);
 

@@ -25,12 +25,20 @@ __device__ __forceinline__ double ld(const T* p, size_t i) { return (double)p[i]
 // Per output component three partial sums are kept (scalar-scalar,
 // scalar-scalar', scalar/vector-vector parts) and combined in the order
 // coupledMatrix::matrixMul adds its sub-block products.
-template <typename TB, typename TV>
-__global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__ blk, const TB* __restrict__ dg,
-                                              const TV* __restrict__ x, const double* __restrict__ scal,
-                                              TV* __restrict__ vout, TV* __restrict__ y) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m.nC) return;
+// One block row of y = A (x/scal): the diagonal block, then the
+// off-diagonal slots in column order, three partial sums per row (scalar
+// rho column, scalar rhoE column, vector rhoU columns) combined as the
+// blockFvMatrix sub-block sums (coupledMatrix.C:66-123).  With FWD, the
+// slots are all lower neighbours p of a colour-1 row and the forward
+// LU-SGS terms L_cp (rD_p y_p) are accumulated from the same block loads
+// (three partial sums per row as well); the row result is then
+// rD_c (y_c - fwd), i.e. the forward and the first backward level of the
+// LU-SGS sweep of a two-colour level structure.
+template <typename TB, typename TV, bool FWD>
+__device__ __forceinline__ void spmvRow(const MeshView& m, const TB* __restrict__ blk, const TB* __restrict__ dg,
+                                        const TV* __restrict__ x, const double* __restrict__ scal,
+                                        TV* __restrict__ vout, TV* __restrict__ y, const double* __restrict__ rD,
+                                        int c) {
     const size_t nC = m.nC;
     typedef TV A;
     const A beta = scal ? (A)(*scal) : (A)1;
@@ -45,6 +53,7 @@ __global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__
 #pragma unroll
         for (int k = 0; k < 5; ++k) vout[(size_t)k * nC + c] = xc[k];
     A acc[5][3];
+    A fw[5][3];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
         const A b0 = (A)dg[diagOff(c, 5 * i)], b1 = (A)dg[diagOff(c, 5 * i + 1)];
@@ -53,6 +62,7 @@ __global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__
         acc[i][0] = b0 * xc[0];
         acc[i][1] = b1 * xc[1];
         acc[i][2] = b2 * xc[2] + b3 * xc[3] + b4 * xc[4];
+        if (FWD) fw[i][0] = fw[i][1] = fw[i][2] = (A)0;
     }
     const int sl = c >> 6, lane = c & 63;
     const int len = m.rowLen[c];
@@ -60,9 +70,14 @@ __global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__
     for (int k = 0; k < len; ++k) {
         const size_t slot = (base + k) * 64 + lane;
         const int j = m.col[slot];
-        A xj[5];
+        A xj[5], d[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) xj[q] = X(q, j);
+        if (FWD) {
+            const A rd = (A)rD[j];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) d[q] = rd * y[(size_t)q * nC + j];
+        }
         const size_t bb = (base + k) * NB * 64 + lane;
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
@@ -71,17 +86,52 @@ __global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__
                     b4 = (A)blk[bb + (5 * i + 4) * 64];
             acc[i][0] += b0 * xj[0];
             acc[i][1] += b1 * xj[1];
-            const A d = b2 * xj[2] + b3 * xj[3] + b4 * xj[4];
-            acc[i][2] += d;
+            const A dd = b2 * xj[2] + b3 * xj[3] + b4 * xj[4];
+            acc[i][2] += dd;
+            if (FWD) {
+                fw[i][0] += b0 * d[0];
+                fw[i][1] += b1 * d[1];
+                const A de = b2 * d[2] + b3 * d[3] + b4 * d[4];
+                fw[i][2] += de;
+            }
         }
     }
+    A rdc = (A)1;
+    if (FWD) rdc = (A)rD[c];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
         A r = (A)0 + acc[i][0];
         r = r + acc[i][1];
         r = r + acc[i][2];
+        if (FWD) {
+            r = r - fw[i][0];
+            r = r - fw[i][1];
+            r = r - fw[i][2];
+            r = rdc * r;
+        }
         y[(size_t)i * nC + c] = r;
     }
+}
+
+template <typename TB, typename TV>
+__global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__ blk, const TB* __restrict__ dg,
+                                              const TV* __restrict__ x, const double* __restrict__ scal,
+                                              TV* __restrict__ vout, TV* __restrict__ y) {
+    const int c = xcdBlock() * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    spmvRow<TB, TV, false>(m, blk, dg, x, scal, vout, y, nullptr, c);
+}
+
+// SpMV of the rows in a cell list; FWD: fused forward sweep (see spmvRow).
+template <typename TB, typename TV, bool FWD>
+__global__ void __launch_bounds__(256) k_spmv_cells(MeshView m, const TB* __restrict__ blk,
+                                                    const TB* __restrict__ dg, const TV* __restrict__ x,
+                                                    const double* __restrict__ scal, TV* __restrict__ vout,
+                                                    TV* __restrict__ y, const double* __restrict__ rD,
+                                                    const int* __restrict__ cells, int n) {
+    const int t = xcdBlock() * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    spmvRow<TB, TV, FWD>(m, blk, dg, x, scal, vout, y, rD, cells[t]);
 }
 
 // Harmonic-balance SpMV: row cell c of instance J = blockIdx.y.  The own
@@ -94,7 +144,7 @@ __global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__
 __global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const double* __restrict__ x,
                                                  const double* __restrict__ scal, double* __restrict__ vout,
                                                  double* __restrict__ y) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcdBlock() * blockDim.x + threadIdx.x;
     const int J = blockIdx.y;
     if (c >= m.nC) return;
     const size_t nC = m.nC;
@@ -179,7 +229,7 @@ template <typename TB, typename TV>
 __global__ void __launch_bounds__(256) k_lusgs_fwd(MeshView m, const TB* __restrict__ blk,
                                                    const double* __restrict__ rD, TV* __restrict__ r,
                                                    const int* __restrict__ cells, int n) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcdBlock() * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int c = cells[t];
     const int nl = m.nLower[c];
@@ -220,7 +270,7 @@ template <typename TB, typename TV>
 __global__ void __launch_bounds__(256) k_lusgs_bwd(MeshView m, const TB* __restrict__ blk,
                                                    const double* __restrict__ rD, TV* __restrict__ r,
                                                    const int* __restrict__ cells, int n) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcdBlock() * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int c = cells[t];
     const size_t nC = m.nC;
@@ -442,12 +492,34 @@ __global__ void __launch_bounds__(256) k_cgs_update(int nC, TV* __restrict__ w, 
                                                     double* __restrict__ part) {
     double v[1] = {0};
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nC; c += gridDim.x * blockDim.x) {
+        // five independent chains (one per component), basis vectors in
+        // order j = 0..nv-1 on each; four basis vectors' loads in flight
+        TV a[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) a[q] = w[(size_t)q * nC + c];
+        int j = 0;
+        for (; j + 4 <= nv; j += 4) {
+            TV x[4][5];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int q = 0; q < 5; ++q) x[u][q] = Vb[(size_t)(j + u) * vstride + (size_t)q * nC + c];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const TV hj = (TV)h[j + u];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) a[q] -= hj * x[u][q];
+            }
+        }
+        for (; j < nv; ++j) {
+            const TV hj = (TV)h[j];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) a[q] -= hj * Vb[(size_t)j * vstride + (size_t)q * nC + c];
+        }
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-            TV a = w[(size_t)q * nC + c];
-            for (int j = 0; j < nv; ++j) a -= (TV)h[j] * Vb[(size_t)j * vstride + (size_t)q * nC + c];
-            w[(size_t)q * nC + c] = a;
-            v[0] += (double)a * (double)a;
+            w[(size_t)q * nC + c] = a[q];
+            v[0] += (double)a[q] * (double)a[q];
         }
     }
     blockReduceStore<1>(v, part);
@@ -633,7 +705,7 @@ int reduceBlocks(int nC) {
 template <typename TB, typename TV>
 void spmv(const MeshView& m, const void* blk, const void* dg, const void* x, const double* scal, void* vout, void* y,
           hipStream_t st) {
-    k_spmv<TB, TV><<<nb(m.nC, 256), 256, 0, st>>>(m, (const TB*)blk, (const TB*)dg, (const TV*)x, scal, (TV*)vout,
+    k_spmv<TB, TV><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, (const TB*)blk, (const TB*)dg, (const TV*)x, scal, (TV*)vout,
                                                   (TV*)y);
 }
 
@@ -645,30 +717,54 @@ void launchSpmv(const MeshView& m, int blkFp32, int vecFp32, const void* blk, co
     else spmv<double, double>(m, blk, dg, x, scal, vout, y, st);
 }
 
+template <typename TB, typename TV>
+static void spmvLusgsT(const MeshView& m, const Levels& L, const void* blk, const void* dg, const double* rD,
+                       const void* x, const double* scal, void* vout, void* y, hipStream_t st) {
+    const int n0 = L.fwdPtrHost[1], n1 = L.fwdPtrHost[2] - n0;
+    const TB* B = (const TB*)blk;
+    const TB* G = (const TB*)dg;
+    k_spmv_cells<TB, TV, false><<<xcdGrid(n0, 256), 256, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout, (TV*)y, rD,
+                                                             L.fwdCells, n0);
+    k_spmv_cells<TB, TV, true><<<xcdGrid(n1, 256), 256, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout, (TV*)y, rD,
+                                                            L.fwdCells + n0, n1);
+    const int s0 = L.bwdPtrHost[1], nbw = L.bwdPtrHost[2] - s0;
+    k_lusgs_bwd<TB, TV><<<xcdGrid(nbw, 256), 256, 0, st>>>(m, B, rD, (TV*)y, L.bwdCells + s0, nbw);
+}
+
+bool launchSpmvLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp32, const void* blk, const void* dg,
+                     const double* rD, const void* x, const double* scal, void* vout, void* y, hipStream_t st) {
+    if (!L.twoColour) return false;
+    if (blkFp32 && vecFp32) spmvLusgsT<float, float>(m, L, blk, dg, rD, x, scal, vout, y, st);
+    else if (blkFp32) spmvLusgsT<float, double>(m, L, blk, dg, rD, x, scal, vout, y, st);
+    else if (vecFp32) spmvLusgsT<double, float>(m, L, blk, dg, rD, x, scal, vout, y, st);
+    else spmvLusgsT<double, double>(m, L, blk, dg, rD, x, scal, vout, y, st);
+    return true;
+}
+
 void launchLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp32, const void* blk, const double* rD,
                  void* r, hipStream_t st) {
     for (int l = 0; l < L.nFwd; ++l) {
         const int s0 = L.fwdPtrHost[l], n = L.fwdPtrHost[l + 1] - s0;
         if (L.fwdSkip && l == 0) continue;  // first level has no lower neighbours
         if (blkFp32 && vecFp32)
-            k_lusgs_fwd<float, float><<<nb(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (float*)r, L.fwdCells + s0, n);
+            k_lusgs_fwd<float, float><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (float*)r, L.fwdCells + s0, n);
         else if (blkFp32)
-            k_lusgs_fwd<float, double><<<nb(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (double*)r, L.fwdCells + s0, n);
+            k_lusgs_fwd<float, double><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (double*)r, L.fwdCells + s0, n);
         else if (vecFp32)
-            k_lusgs_fwd<double, float><<<nb(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (float*)r, L.fwdCells + s0, n);
+            k_lusgs_fwd<double, float><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (float*)r, L.fwdCells + s0, n);
         else
-            k_lusgs_fwd<double, double><<<nb(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (double*)r, L.fwdCells + s0, n);
+            k_lusgs_fwd<double, double><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (double*)r, L.fwdCells + s0, n);
     }
     for (int l = 0; l < L.nBwd; ++l) {
         const int s0 = L.bwdPtrHost[l], n = L.bwdPtrHost[l + 1] - s0;
         if (blkFp32 && vecFp32)
-            k_lusgs_bwd<float, float><<<nb(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (float*)r, L.bwdCells + s0, n);
+            k_lusgs_bwd<float, float><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (float*)r, L.bwdCells + s0, n);
         else if (blkFp32)
-            k_lusgs_bwd<float, double><<<nb(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (double*)r, L.bwdCells + s0, n);
+            k_lusgs_bwd<float, double><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (double*)r, L.bwdCells + s0, n);
         else if (vecFp32)
-            k_lusgs_bwd<double, float><<<nb(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (float*)r, L.bwdCells + s0, n);
+            k_lusgs_bwd<double, float><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (float*)r, L.bwdCells + s0, n);
         else
-            k_lusgs_bwd<double, double><<<nb(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (double*)r, L.bwdCells + s0, n);
+            k_lusgs_bwd<double, double><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (double*)r, L.bwdCells + s0, n);
     }
 }
 
@@ -715,16 +811,25 @@ void gmresCycleT(const GmresArgs& a, hipStream_t st) {
         TV* vi = V + (size_t)i * vs;
         // v_i = r / beta ; w = A v_i (into the other buffer: the SpMV gathers
         // neighbour entries of its input, so it never runs in place)
+        // fused SpMV + LU-SGS on two-colour level structures (the colour-1
+        // blocks are read once for both; the SPMV timing class then covers
+        // the sweeps too); otherwise SpMV, then the preconditioner
         mark(TK_SPMV, 1);
-        if (a.op) a.op(r, &g->beta, vi, r2);
-        else launchSpmv(m, a.blkFp32, sizeof(TV) == 4, a.blk, a.dg, r, &g->beta, vi, r2, st);
+        bool fused = false;
+        if (!a.op && !a.pre && a.precond == 0 && a.fuse)
+            fused = launchSpmvLusgs(m, *a.lev, a.blkFp32, sizeof(TV) == 4, a.blk, a.dg, a.rD, r, &g->beta, vi, r2,
+                                    st);
+        if (!fused) {
+            if (a.op) a.op(r, &g->beta, vi, r2);
+            else launchSpmv(m, a.blkFp32, sizeof(TV) == 4, a.blk, a.dg, r, &g->beta, vi, r2, st);
+        }
         mark(TK_SPMV, 0);
         {
             TV* t = r;
             r = r2;
             r2 = t;
         }
-        precond(r);
+        if (!fused) precond(r);
         mark(TK_ORTH, 1);
         if (a.orth == 0) {
             // modified Gram-Schmidt, reference order
@@ -856,7 +961,7 @@ namespace icsf {
 
 void launchSpmvHB(const MeshView& m, const HBBlocks& hb, const double* x, const double* scal, double* vout,
                   double* y, hipStream_t st) {
-    dim3 grid(nb(m.nC, 256), hb.nO);
+    dim3 grid(xcdGrid(m.nC, 256), hb.nO);
     k_spmv_hb<<<grid, 256, 0, st>>>(m, hb, x, scal, vout, y);
 }
 

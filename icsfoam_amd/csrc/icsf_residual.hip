@@ -119,7 +119,7 @@ __global__ void k_boundary(MeshView m, Phys ph, StateView s) {
 template <int NG>
 __global__ void __launch_bounds__(256) k_grad(MeshView m, Phys ph, StateView s, double* __restrict__ gun,
                                               double* __restrict__ limo) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     double gx[NG], gy[NG], gz[NG], mx[NG], mn[NG], vc[NG];
 #pragma unroll
@@ -278,7 +278,7 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
 //   FT_SV        0.5*lambdaVisc*|S|*deltaCoeffs (internal, viscous)
 // Cell kernels then sum these in face order (the reference's face loops).
 __global__ void __launch_bounds__(256) k_face_terms(MeshView m, Phys ph, StateView s) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcdBlock() * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
     if (f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY) return;
     double* o = s.ft + (size_t)f * NFT;
@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(256) k_face_terms(MeshView m, Phys ph, StateVi
 // Residual b = R*V (residualsUpdate.H) and rPseudoDeltaT
 // (setCoAndDeltaT.H:57-141) of each cell from the face terms.
 __global__ void __launch_bounds__(256) k_cell_residual(MeshView m, Phys ph, StateView s) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     double ar = 0, ae = 0;
     V3 au{0, 0, 0}, aLap{0, 0, 0}, aTau{0, 0, 0};
@@ -447,7 +447,7 @@ template <typename TB>
 __global__ void __launch_bounds__(256) k_cell_jacobian(MeshView m, Phys ph, StateView s, TB* __restrict__ blk,
                                                        TB* __restrict__ dg, double* __restrict__ rD,
                                                        int* __restrict__ err) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const double g = ph.gamma;
     double D[NB];
@@ -620,18 +620,18 @@ void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const d
 
 void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st) {
     if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
-    if (ph.flux == FLUX_ROE) k_grad<7><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, nullptr, nullptr);
-    else k_grad<8><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, nullptr, nullptr);
-    k_face_terms<<<nblk(m.nF, 256), 256, 0, st>>>(m, ph, s);
-    k_cell_residual<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s);
+    if (ph.flux == FLUX_ROE) k_grad<7><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, nullptr, nullptr);
+    else k_grad<8><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, nullptr, nullptr);
+    k_face_terms<<<xcdGrid(m.nF, 256), 256, 0, st>>>(m, ph, s);
+    k_cell_residual<<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s);
 }
 
 void launchAdjointForward(const MeshView& m, const Phys& ph, const StateView& s, double* gun, double* limo,
                           hipStream_t st) {
     if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
     if (ph.recon == 1) {
-        if (ph.flux == FLUX_ROE) k_grad<7><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, gun, limo);
-        else k_grad<8><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, gun, limo);
+        if (ph.flux == FLUX_ROE) k_grad<7><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, gun, limo);
+        else k_grad<8><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, gun, limo);
     }
 }
 
@@ -645,9 +645,9 @@ void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void*
                     double* rD, double* dg64, int* err, hipStream_t st) {
     (void)dg64;
     if (fp32)
-        k_cell_jacobian<float><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
+        k_cell_jacobian<float><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
     else
-        k_cell_jacobian<double><<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
+        k_cell_jacobian<double><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
 }
 
 }  // namespace icsf

@@ -99,6 +99,7 @@ struct icsf_case {
     int solver = 0;      // 0 GMRES (left, reference), 1 FGMRES (right, outer)
     int precond = 0;     // 0 LUSGS, 1 Jacobi, 2 none
     int nDirs = 30, maxIter = 1, minIter = 0, innerDirs = 30, orth = 0, mixed = 0;
+    bool fuse = true;  // SpMV fused with the LU-SGS sweeps on two-colour numberings
     double tol = 1e-12, relTol = 1e-2;
     double co = 1, coMin = 0.1, coMax = 25, coIncr = 1.25, coDecr = 0.1, lowerBound = 0.95;
     int localTS = 1, bounding = 1;
@@ -338,6 +339,10 @@ int readSettings(icsf_case* c, const char* thermo, const char* schemes, const ch
     if (orth == "MGS") c->orth = 0;
     else if (orth == "CGS") c->orth = 1;
     else return fail(ICSF_ERR_KEYWORD, "orthogonalisation must be MGS or CGS");
+    const std::string fsw = fd.word(base + "fusedSweeps", "true");
+    if (fsw == "true" || fsw == "on" || fsw == "yes") c->fuse = true;
+    else if (fsw == "false" || fsw == "off" || fsw == "no") c->fuse = false;
+    else return fail(ICSF_ERR_KEYWORD, "fusedSweeps must be true or false");
     if (c->nDirs < 1 || c->nDirs > KMAX || c->innerDirs < 1 || c->innerDirs > KMAX)
         return fail(ICSF_ERR_ARG, "nDirections must be in [1, 64]");
     if (c->mixed && c->solver != 1)
@@ -417,6 +422,7 @@ int gmresSolve(icsf_case* c, const double* W, const double* b, int& nIter) {
         GmresArgs a{&c->mv, &c->mesh->lev, &c->wk, c->g, c->nDirs, c->precond, c->orth, c->blkFp32,
                     c->blk, c->dg, c->rD, c->dinv, c->r, c->t2, c->V};
         a.mark = [c](int k, int b) { markClass(c, k, b); };
+        a.fuse = c->fuse;
         gmresCycle(a, 0, c->st);
         {
             Timed t(c, 5);
@@ -500,6 +506,7 @@ int fgmresSolve(icsf_case* c, const double* W, const double* b, int& nIter) {
                 GmresArgs a{&c->mv, &c->mesh->lev, &c->wk, c->g2, c->innerDirs, c->precond, c->orth, c->blkFp32,
                             c->blk, c->dg, c->rD, c->dinv, c->r32, c->r32b, c->V32};
                 a.mark = [c](int k, int b) { markClass(c, k, b); };
+                a.fuse = c->fuse;
                 gmresCycle(a, 1, c->st);
                 Timed t(c, 5);
                 launchCombine(c->mv.nC, 1, c->V32, c->innerDirs, c->g2, c->r32, c->st);
@@ -940,6 +947,8 @@ int icsf_mesh_create(int nPoints, const double* points, int nFaces, const int* f
     std::vector<int> fc, bc;
     bucket(fl, nf, L.fwdPtrHost, fc);
     bucket(bl, nbw, L.bwdPtrHost, bc);
+    L.twoColour = nf == 2 && nbw == 2 && L.fwdPtrHost[1] == L.bwdPtrHost[2] - L.bwdPtrHost[1] &&
+                  std::equal(fc.begin() + L.fwdPtrHost[1], fc.end(), bc.begin());
     DevBuf& B = M->buf;
     MeshView& v = M->view;
     v.nC = h.nCells;

@@ -89,6 +89,21 @@ struct AdjView {
     double* bprim;  // [nB][5] boundary adjoint forwarded to the owner's p, T, U
 };
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs, each
+// with its own L2.  Kernels that gather neighbour data launch a multiple of
+// 8 workgroups (xcdGrid) and remap with xcdBlock(), so that XCD x runs the
+// contiguous workgroup range [x g/8, (x+1) g/8): consecutive rows, whose
+// neighbour gathers overlap, then share one L2.
+constexpr int kXCD = 8;
+inline int xcdGrid(size_t n, int b) {
+    const int g = (int)((n + b - 1) / b);
+    return (g + kXCD - 1) / kXCD * kXCD;
+}
+__device__ __forceinline__ int xcdBlock() {
+    const int per = gridDim.x / kXCD;
+    return (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+}
+
 // Block storage offset in the sliced-ELL layout: slot index (as stored in
 // col[]) and block entry e.
 __host__ __device__ __forceinline__ size_t sellOff(size_t slot, int e) {
