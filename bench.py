@@ -78,14 +78,27 @@ def algorithmic_bytes(nC, nI, p, q, m=30):
     return (2 * nI + nC) * 25 * p + 2 * nI * 4 + nC * 5 * q * 2 + m / (m + 2) * nC * 5 * q
 
 
+def algorithmic_bytes_fused(nC, nI, p, q):
+    """Per-group algorithmic bytes of the fused Arnoldi operator (one group
+    per Arnoldi step: SpMV of the colour-0 rows; SpMV + forward sweep +
+    first backward level of the colour-1 rows; backward sweep of the
+    colour-0 rows).  Blocks: every off-diagonal block of a colour-0 row
+    twice, of a colour-1 row once, diagonals once; column indices per block
+    read; vectors: x twice, y0 once more by the colour-1 rows, y once by
+    the last pass, writes of y, the basis vector and the final colour-0
+    rows; rD twice for colour 0 (n0 = n1 = nC/2 on these meshes)."""
+    return (3 * nI + nC) * 25 * p + 3 * nI * 4 + 30 * q * nC + 12 * nC
+
+
 def measured_traffic(n):
-    """HBM bytes per k_spmv<double,double> launch from the committed
-    FETCH_SIZE/WRITE_SIZE profile of the 160^3 workload (None otherwise)."""
-    path = os.path.join(ROOT, "profiles", "k_spmv_160cube_pmc.txt")
+    """HBM bytes per fused Arnoldi-operator group (FP64 form) from the
+    committed FETCH_SIZE/WRITE_SIZE profile of the 160^3 workload (None
+    otherwise)."""
+    path = os.path.join(ROOT, "profiles", "arnoldi_operator_160cube_pmc.txt")
     if n != 160 or not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith("traffic_bytes_per_group:"):
             return int(line.split(":")[1])
     return None
 
@@ -115,8 +128,9 @@ def time_form(case, steps, warmup, label):
     g.pseudo_steps(1, residuals=False)
     bms, bcnt = g.timing_read()
     g.timing(0)
+    fused = g.mesh.nFwdLevels == 2 and g.mesh.nBwdLevels == 2
     del g
-    return dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, spmv_ms=ms[2] / max(cnt[2], 1),
+    return dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, spmv_ms=ms[2] / max(cnt[2], 1), fused=fused,
                 spmv_count=int(cnt[2]), breakdown_ms=[round(float(x), 3) for x in bms],
                 breakdown_count=[int(x) for x in bcnt])
 
@@ -208,6 +222,43 @@ def bench_vjp(calls):
                        "residual_ms": round(1e3 * dr, 3)}}
 
 
+def bench_converge(n, max_steps, chunk=25, drop=1e-8, time_limit=400.0):
+    """Time to a `drop` reduction of ||R V||_2 (all five components, the
+    norm dbnsFoam.C:136-137 prints for rho) for the FP64 and the mixed form
+    of the n^3 workload."""
+    base = prepared(n)
+    out = {}
+    for form in ("fp64", "mixed"):
+        from icsfoam_amd import dbnsCase
+
+        g = dbnsCase.from_case(with_form(base, n, form, "CGS"))
+        b, _ = g.residual()
+        r0 = float(np.sqrt(np.sum(b * b)))
+        steps, hist = 0, [(0, 0.0, 1.0)]
+        g.synchronize()
+        t0 = time.perf_counter()
+        while steps < max_steps:
+            g.pseudo_steps(chunk, residuals=False)
+            steps += chunk
+            b, _ = g.residual()
+            rel = float(np.sqrt(np.sum(b * b))) / r0
+            el = time.perf_counter() - t0
+            hist.append((steps, round(el, 3), rel))
+            log(f"[bench] {form}: {steps} steps {el:.1f}s  ||R||/||R0|| = {rel:.3e}")
+            if rel < drop or not np.isfinite(rel) or el > time_limit:
+                break
+        out[form] = dict(steps=steps, seconds=round(time.perf_counter() - t0, 3), drop=hist[-1][2],
+                         history=hist[::max(1, len(hist) // 20)])
+        del g
+    return {"metric": f"time to a {drop:g} drop of ||R V||_2 at {n}^3 (FP64 vs mixed)",
+            "value": out["mixed"]["seconds"], "unit": "s", "n_gpus": 1, "steps": out["mixed"]["steps"],
+            "warmup": 0, "ms_per_step": round(1e3 * out["mixed"]["seconds"] / max(out["mixed"]["steps"], 1), 3),
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64 residual/outer + f32 Jacobian/inner", "data": "synthetic",
+            "config": {"workload": f"{n}^3 jittered hex box, viscous Roe, local CFL 50", "fp64": out["fp64"],
+                       "mixed": out["mixed"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,7 +268,7 @@ def main():
     ap.add_argument("--orth", default="CGS", choices=["CGS", "MGS"])
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="c5", choices=["c5", "hb", "vjp"],
+    ap.add_argument("--workload", default="c5", choices=["c5", "hb", "vjp", "converge"],
                     help="c5: the headline 160^3 pseudo-step (default); hb: 7-instant harmonic balance; "
                          "vjp: residual adjoint at 100^3")
     args = ap.parse_args()
@@ -231,15 +282,19 @@ def main():
     if args.workload == "vjp":
         print(json.dumps(bench_vjp(max(args.steps, 10))), flush=True)
         return
+    if args.workload == "converge":
+        print(json.dumps(bench_converge(args.n, max(args.steps, 25))), flush=True)
+        return
     n = args.n
     base = prepared(n)
     nC, nI = base.mesh.n_cells, base.mesh.n_internal
     r64 = time_form(with_form(base, n, "fp64", args.orth), args.steps, args.warmup, "fp64")
     rmx = time_form(with_form(base, n, "mixed", args.orth), args.steps, args.warmup, "mixed")
-    # dominant kernel: block SpMV of the mixed form's FP64 outer/FP32 inner
-    # launches; report the FP64-form SpMV (FP64 blocks and vectors) whose
-    # algorithmic bytes are unambiguous.
-    bytes64 = algorithmic_bytes(nC, nI, 8, 8)
+    # dominant kernel group: the Arnoldi operator of the FP64 form (FP64
+    # blocks and vectors, algorithmic bytes unambiguous): the fused SpMV +
+    # LU-SGS group on two-colour numberings, else the block SpMV
+    fused = r64["fused"]
+    bytes64 = algorithmic_bytes_fused(nC, nI, 8, 8) if fused else algorithmic_bytes(nC, nI, 8, 8)
     ach = bytes64 / (r64["spmv_ms"] * 1e-3)
     cpu = None
     if not args.no_cpu:
@@ -265,9 +320,13 @@ def main():
                    "mixed_over_fp64": round(rmx["steps_per_s"] / r64["steps_per_s"], 3),
                    "orthogonalisation": args.orth,
                    "breakdown_ms_fp64": r64["breakdown_ms"], "breakdown_ms_mixed": rmx["breakdown_ms"]},
-        "roofline": {"bound": "hbm", "kernel": "k_spmv<double,double> (block SpMV, FP64 form)",
+        "roofline": {"bound": "hbm",
+                     "kernel": ("fused Arnoldi operator, FP64 form: k_spmv_cells<double,double,false> + "
+                                "k_spmv_cells<double,double,true> + k_lusgs_bwd<double,double> (one group per "
+                                "Arnoldi step, timed as a group)") if fused else
+                               "k_spmv<double,double> (block SpMV, FP64 form)",
                      "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK, 4), "traffic": measured_traffic(n),
+                     "frac": round(ach / HBM_PEAK, 4), "traffic": measured_traffic(n) if fused else None,
                      "bytes_per_launch": int(bytes64), "avg_launch_ms": round(r64["spmv_ms"], 4)},
         "cpu_baseline": cpu,
     }

@@ -443,26 +443,27 @@ def permute_case(case: Case, new_to_old: np.ndarray, renumber) -> Case:
 def annulus_sector(nt: int, nr: int, nz: int, theta: float = np.pi / 8, r0: float = 0.5, r1: float = 1.0,
                    length: float = 1.0, jitter: float = 0.0, seed: int = 0) -> PolyMesh:
     """Annular sector theta in [0, theta], r in [r0, r1], z in [0, length]
-    (cells ordered theta fastest); interior nodes jittered in the index
-    space.  Patches: inlet (z = 0), outlet (z = length), sides (the two
-    theta planes) and hub/shroud walls."""
-    kk, jj, ii = np.meshgrid(np.arange(nz + 1), np.arange(nr + 1), np.arange(nt + 1), indexing="ij")
+    (index directions r, theta, z: a right-handed map, cells ordered r
+    fastest); interior nodes jittered in the index space.  Patches: inlet
+    (z = 0), outlet (z = length), sides (the two theta planes) and
+    hub/shroud walls."""
+    kk, jj, ii = np.meshgrid(np.arange(nz + 1), np.arange(nt + 1), np.arange(nr + 1), indexing="ij")
     a, b, c = ii.astype(np.float64), jj.astype(np.float64), kk.astype(np.float64)
     if jitter > 0:
         rng = np.random.default_rng(seed)
         d = rng.uniform(-jitter, jitter, size=(3,) + a.shape)
-        a = a + d[0] * ((ii > 0) & (ii < nt))
-        b = b + d[1] * ((jj > 0) & (jj < nr))
+        a = a + d[0] * ((ii > 0) & (ii < nr))
+        b = b + d[1] * ((jj > 0) & (jj < nt))
         c = c + d[2] * ((kk > 0) & (kk < nz))
-    th = theta * a / nt
-    r = r0 + (r1 - r0) * b / nr
+    r = r0 + (r1 - r0) * a / nr
+    th = theta * b / nt
     z = length * c / nz
     pts = np.stack([(r * np.cos(th)).ravel(), (r * np.sin(th)).ravel(), z.ravel()], 1).astype(np.float64)
     fv, own, nei, patches = _hex_topology(
-        nt, nr, nz,
+        nr, nt, nz,
         [("inlet", PATCH_PATCH, ["-z"]), ("outlet", PATCH_PATCH, ["+z"]),
-         ("sides", PATCH_WALL, ["-x", "+x"]), ("hubShroud", PATCH_WALL, ["-y", "+y"])])
-    return _mesh_from_quads(pts, fv, own, nei, patches, nt * nr * nz, pts.reshape(nz + 1, nr + 1, nt + 1, 3))
+         ("hubShroud", PATCH_WALL, ["-x", "+x"]), ("sides", PATCH_WALL, ["-y", "+y"])])
+    return _mesh_from_quads(pts, fv, own, nei, patches, nt * nr * nz, pts.reshape(nz + 1, nt + 1, nr + 1, 3))
 
 
 def case_hb(n=(64, 64, 32), n_harmonics: int = 3, frequency_hz: float = 100.0, amplitude: float = 0.05,

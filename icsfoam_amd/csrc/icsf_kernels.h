@@ -140,6 +140,9 @@ void launchSpmvHB(const MeshView& m, const HBBlocks& hb, const double* x, const 
 // its per-component average; norm factors and initial residuals from
 // t = A (W - avg) and b; final residuals from r = b - A dW (r holds A dW on
 // entry).
+// LU-SGS of all HB instances (per-instance sweeps, shared diagonal rD).
+void launchLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, double* r,
+                   hipStream_t st);
 void sliceSubAverage(const MeshView& m, Work& wk, GmresScalars* g, const double* W, double* out, hipStream_t st);
 void sliceNormFactors(const MeshView& m, Work& wk, GmresScalars* g, const double* t, const double* b,
                       hipStream_t st);

@@ -226,12 +226,8 @@ __global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const 
 // Forward sweep of one level: r[c] -= sum over lower neighbours p (in
 // increasing p) of L_cp * (rD[p] r[p]).
 template <typename TB, typename TV>
-__global__ void __launch_bounds__(256) k_lusgs_fwd(MeshView m, const TB* __restrict__ blk,
-                                                   const double* __restrict__ rD, TV* __restrict__ r,
-                                                   const int* __restrict__ cells, int n) {
-    const int t = xcdBlock() * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int c = cells[t];
+__device__ __forceinline__ void lusgsFwdRow(const MeshView& m, const TB* __restrict__ blk,
+                                            const double* __restrict__ rD, TV* __restrict__ r, int c) {
     const int nl = m.nLower[c];
     if (nl == 0) return;
     const size_t nC = m.nC;
@@ -267,12 +263,8 @@ __global__ void __launch_bounds__(256) k_lusgs_fwd(MeshView m, const TB* __restr
 // Backward sweep of one level: r[c] = rD[c] (r[c] - sum over upper
 // neighbours q (decreasing q) of U_cq r[q]), r[q] already final.
 template <typename TB, typename TV>
-__global__ void __launch_bounds__(256) k_lusgs_bwd(MeshView m, const TB* __restrict__ blk,
-                                                   const double* __restrict__ rD, TV* __restrict__ r,
-                                                   const int* __restrict__ cells, int n) {
-    const int t = xcdBlock() * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int c = cells[t];
+__device__ __forceinline__ void lusgsBwdRow(const MeshView& m, const TB* __restrict__ blk,
+                                            const double* __restrict__ rD, TV* __restrict__ r, int c) {
     const size_t nC = m.nC;
     typedef TV A;
     A acc[5];
@@ -302,6 +294,38 @@ __global__ void __launch_bounds__(256) k_lusgs_bwd(MeshView m, const TB* __restr
     const A rd = (A)rD[c];
 #pragma unroll
     for (int i = 0; i < 5; ++i) r[(size_t)i * nC + c] = rd * acc[i];
+}
+
+template <typename TB, typename TV>
+__global__ void __launch_bounds__(256) k_lusgs_fwd(MeshView m, const TB* __restrict__ blk,
+                                                   const double* __restrict__ rD, TV* __restrict__ r,
+                                                   const int* __restrict__ cells, int n) {
+    const int t = xcdBlock() * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    lusgsFwdRow<TB, TV>(m, blk, rD, r, cells[t]);
+}
+
+template <typename TB, typename TV>
+__global__ void __launch_bounds__(256) k_lusgs_bwd(MeshView m, const TB* __restrict__ blk,
+                                                   const double* __restrict__ rD, TV* __restrict__ r,
+                                                   const int* __restrict__ cells, int n) {
+    const int t = xcdBlock() * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    lusgsBwdRow<TB, TV>(m, blk, rD, r, cells[t]);
+}
+
+// Harmonic balance: one launch per level for all instances (blockIdx.y =
+// instance; the cross-instance blocks have no faces, so the sweeps are
+// independent per instance and share the diagonal rD).
+template <bool FWD>
+__global__ void __launch_bounds__(256) k_lusgs_hb(MeshView m, HBBlocks hb, const double* __restrict__ rD,
+                                                  double* __restrict__ r, const int* __restrict__ cells, int n) {
+    const int t = xcdBlock() * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int J = blockIdx.y;
+    double* rJ = r + (size_t)J * 5 * m.nC;
+    if (FWD) lusgsFwdRow<double, double>(m, hb.blk[J], rD, rJ, cells[t]);
+    else lusgsBwdRow<double, double>(m, hb.blk[J], rD, rJ, cells[t]);
 }
 
 // ------------------------------------------------------- block-Jacobi -----
@@ -963,6 +987,21 @@ void launchSpmvHB(const MeshView& m, const HBBlocks& hb, const double* x, const 
                   double* y, hipStream_t st) {
     dim3 grid(xcdGrid(m.nC, 256), hb.nO);
     k_spmv_hb<<<grid, 256, 0, st>>>(m, hb, x, scal, vout, y);
+}
+
+void launchLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, double* r,
+                   hipStream_t st) {
+    for (int l = 0; l < L.nFwd; ++l) {
+        const int s0 = L.fwdPtrHost[l], n = L.fwdPtrHost[l + 1] - s0;
+        if (L.fwdSkip && l == 0) continue;
+        dim3 grid(xcdGrid(n, 256), hb.nO);
+        k_lusgs_hb<true><<<grid, 256, 0, st>>>(m, hb, rD, r, L.fwdCells + s0, n);
+    }
+    for (int l = 0; l < L.nBwd; ++l) {
+        const int s0 = L.bwdPtrHost[l], n = L.bwdPtrHost[l + 1] - s0;
+        dim3 grid(xcdGrid(n, 256), hb.nO);
+        k_lusgs_hb<false><<<grid, 256, 0, st>>>(m, hb, rD, r, L.bwdCells + s0, n);
+    }
 }
 
 void sliceSubAverage(const MeshView& m, Work& wk, GmresScalars* g, const double* W, double* out, hipStream_t st) {

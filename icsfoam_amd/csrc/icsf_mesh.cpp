@@ -70,6 +70,10 @@ std::string buildHostMesh(HostMesh& m, int nPoints, const double* points, int nF
     }
     if (expect != nFaces) return "patches must cover all boundary faces";
     computeGeometry(m);
+    // checkMesh's "negative volume": an inverted cell (faces ordered the
+    // wrong way round) would make every face flux change sign
+    for (int c = 0; c < m.nCells; ++c)
+        if (!(m.V[c] > 0)) return "cell " + std::to_string(c) + " has non-positive volume (inverted face order?)";
     buildCellFaces(m);
     return "";
 }

@@ -702,10 +702,7 @@ __global__ void k_copyn(int n, const double* a, double* b) {
 }
 
 void hbLusgs(icsf_hb* h, double* v) {
-    const MeshView& m = h->inst[0]->mv;
-    const size_t vs = (size_t)5 * m.nC;
-    for (int J = 0; J < h->nO; ++J)
-        launchLusgs(m, h->mesh->lev, 0, 0, h->inst[J]->blk, h->rD, v + J * vs, h->st);
+    launchLusgsHB(h->inst[0]->mv, h->mesh->lev, h->hbb, h->rD, v, h->st);
 }
 
 int hbUploadBC(icsf_hb* h) {

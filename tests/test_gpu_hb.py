@@ -7,8 +7,11 @@
   order for the cross-instance terms);
 * GMRES increment: relative 1e-10; pseudo-steps: first-step residuals 1e-12,
   histories and fields 1e-8 after 4 steps;
-* one instant with D = 0 equals the single-instance GPU path bit for bit.
+* one instant with D = 0 equals the single-instance GPU path (unfused
+  Arnoldi step) bit for bit.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -77,8 +80,11 @@ def test_hb_pseudo_steps(pair):
 
 
 def test_single_instant_equals_single_case():
+    """Same kernels, same order: bit-for-bit (the single-instance solver
+    with the unfused Arnoldi step, which the HB solver uses)."""
     print("[hb single instant]", flush=True)
     c = box_small()
+    c = dataclasses.replace(c, fvSolution=cases.solution_dict(fusedSweeps=False))
     s = dbnsCase.from_case(c)
     h = hbCase.from_cases([c], np.zeros((1, 1)), mesh=s.mesh)
     rs = s.pseudo_steps(3)

@@ -451,3 +451,23 @@ def test_native_block_mesh_matches_numpy_statement():
     for x, y in zip(a[:3], b[:3]):
         np.testing.assert_array_equal(x, y)
     assert [(p.name, p.start, p.size) for p in a[3]] == [(p.name, p.start, p.size) for p in b[3]]
+
+
+def test_inverted_cells_rejected():
+    """checkMesh's negative-volume test: faces listed the wrong way round
+    give negative volumes, which the library refuses (the sector generator
+    maps (r, theta, z) right-handed; (theta, r, z) would invert every
+    cell)."""
+    import dataclasses
+
+    from icsfoam_amd import IcsfError
+
+    m = cases.hex_box(3, 2, 2)
+    fo, fv = m.face_off, m.face_verts.copy()
+    for f in range(m.n_faces):
+        fv[fo[f]:fo[f + 1]] = fv[fo[f]:fo[f + 1]][::-1]
+    bad = dataclasses.replace(m, face_verts=fv)
+    with pytest.raises(IcsfError, match="non-positive volume"):
+        M.mesh_geometry(bad)
+    g = M.mesh_geometry(cases.annulus_sector(4, 3, 2))
+    assert g["V"].min() > 0
