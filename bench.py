@@ -160,13 +160,14 @@ def cpu_baseline(n_sample: int, n_full: int, steps: int = 2):
 
 def bench_hb(steps, warmup):
     """Harmonic-balance pseudo-steps/s on a 64x64x32 annular sector with 7
-    instants (3 harmonics of 100 Hz), Roe inviscid, GMRES(30)+LU-SGS, FP64."""
+    instants (3 harmonics of 100 Hz), Roe inviscid, GMRES(30) with CGS +
+    LU-SGS, FP64."""
     from icsfoam_amd import cases
     from icsfoam_amd.mesh import prepare_case
     from icsfoam_amd.solver import hbCase
 
     t0 = time.time()
-    insts, D, t = cases.case_hb(n=(64, 64, 32), n_harmonics=3, flux="ROE")
+    insts, D, t = cases.case_hb(n=(64, 64, 32), n_harmonics=3, flux="ROE", orthogonalisation="CGS")
     insts = [prepare_case(c, permute_seed=2024) for c in insts]
     log(f"[bench] HB case (7 instants, {insts[0].mesh.n_cells} cells) prepared in {time.time() - t0:.1f}s")
     g = hbCase.from_cases(insts, D)
@@ -184,7 +185,7 @@ def bench_hb(steps, warmup):
             "ms_per_step": round(1e3 * dt / steps, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded annular sector, periodic inflow)",
             "config": {"workload": f"64x64x32 annular sector ({nC} cells) x 7 instants "
-                                   f"({7 * nC} block rows), Roe inviscid, GMRES(30)+LU-SGS"}}
+                                   f"({7 * nC} block rows), Roe inviscid, GMRES(30, CGS)+LU-SGS"}}
 
 
 def bench_vjp(calls):

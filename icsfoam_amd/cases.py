@@ -468,7 +468,7 @@ def annulus_sector(nt: int, nr: int, nz: int, theta: float = np.pi / 8, r0: floa
 
 def case_hb(n=(64, 64, 32), n_harmonics: int = 3, frequency_hz: float = 100.0, amplitude: float = 0.05,
             mach: float = 0.5, noise: float = 0.02, seed: int = 3, co: float = 20.0, nDirections: int = 30,
-            jitter: float = 0.1, flux: str = "ROE", mu: float = 0.0):
+            jitter: float = 0.1, flux: str = "ROE", mu: float = 0.0, orthogonalisation: str = "MGS"):
     """Harmonic-balance case on an annular sector: 2 n_harmonics + 1
     instants over one period of frequency_hz (selectedPeriod = 1/f), the
     inflow velocity of instant n scaled by 1 + amplitude sin(omega t_n),
@@ -505,5 +505,6 @@ def case_hb(n=(64, 64, 32), n_harmonics: int = 3, frequency_hz: float = 100.0, a
             else:
                 bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_SLIP, BC_ZERO_GRADIENT))
         instances.append(Case(f"hb{J}", mesh, rho, U, T, bcs, thermo_dict(mu), schemes_dict(flux),
-                              solution_dict(nDirections=nDirections, co=co)))
+                              solution_dict(nDirections=nDirections, co=co,
+                                            orthogonalisation=orthogonalisation)))
     return instances, D, t
