@@ -33,8 +33,6 @@ namespace {
 
 #define DF __device__ __forceinline__
 
-inline int nblk(size_t n, int b) { return (int)((n + b - 1) / b); }
-
 // Fields differentiated for AUSM+up (E is not reconstructed by the scheme).
 DF bool used(int k) { return k != F_E; }
 
@@ -181,7 +179,7 @@ DF void ausmFaceAdj(bool lowMach, V3 S, double magS, const double* L, const doub
 __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView s, AdjView a,
                                                   const double* __restrict__ v, double* __restrict__ gS,
                                                   double* __restrict__ gC) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcdBlock() * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
     double* lr = a.fLR + (size_t)f * 2 * NFLD;
     if (f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY) {
@@ -233,100 +231,137 @@ __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView
     gC[3 * (size_t)f] = bC.x; gC[3 * (size_t)f + 1] = bC.y; gC[3 * (size_t)f + 2] = bC.z;
 }
 
-// Cell pass: per field, the adjoint of the limited gradient and of the
-// cell value (linearUpwind), then the limiter reverse (recomputing the
-// cell's max/min neighbour and the face that set the limiter), then the
-// Gauss-gradient adjoint divided by V.
+// Cell pass: the adjoint of the limited gradients and of the cell values
+// (linearUpwind), then the limiter reverse (recomputing each field's
+// max/min neighbour and the face that set its limiter), then the
+// Gauss-gradient adjoint divided by V.  All fields are carried together,
+// so the cell's faces are walked twice (once for the face adjoints and the
+// max/min, once for the limiter faces) whatever the number of fields; per
+// field the operations and their order are those of the forward kernels.
 __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView s, AdjView a) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
     const V3 Cc = ld3(m.C + 3 * (size_t)c);
     const double Vc = m.V[c];
+    const bool rec = ph.recon == 1;
+    const bool limit = rec && ph.limiter != 0;
+    double qd[NFLD], mx[NFLD], mn[NFLD];
+    V3 gl[NFLD];
+    int imx[NFLD], imn[NFLD];
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k) {
+        qd[k] = 0.0;
+        gl[k] = V3{0, 0, 0};
+        mx[k] = mn[k] = s.fld[(size_t)c * NFLD + k];
+        imx[k] = imn[k] = -1;
+    }
+    for (int q = e0; q < e1; ++q) {
+        const int f = m.cf[q];
+        const V3 d = ld3(m.Cf + 3 * (size_t)f) - Cc;
+        const double* vo;
+        int id;
+        if (f < m.nI) {
+            const bool isOwn = m.own[f] == c;
+            const double* lr = a.fLR + (size_t)f * 2 * NFLD + (isOwn ? 0 : NFLD);
+#pragma unroll
+            for (int k = 0; k < NFLD; ++k) {
+                if (!used(k)) continue;
+                const double av = lr[k];
+                qd[k] += av;
+                if (rec) gl[k] = gl[k] + av * d;
+            }
+            id = isOwn ? m.nei[f] : m.own[f];
+            vo = s.fld + (size_t)id * NFLD;
+        } else {
+            const int b = f - m.nI;
+            id = m.nC + b;
+            vo = s.bfld + (size_t)b * NFLD;
+        }
+        if (limit) {
+#pragma unroll
+            for (int k = 0; k < NFLD; ++k) {
+                if (!used(k)) continue;
+                const double v = vo[k];
+                if (mx[k] < v) { mx[k] = v; imx[k] = id; }
+                if (v < mn[k]) { mn[k] = v; imn[k] = id; }
+            }
+        }
+    }
+    V3 g[NFLD], gb[NFLD];
+    double limb[NFLD];
+    bool any = false;
+#pragma unroll
     for (int k = 0; k < NFLD; ++k) {
         const size_t ck = (size_t)c * NFLD + k;
-        if (!used(k)) {
-            a.qc[ck] = 0.0; a.tgt[ck] = -1; a.tval[ck] = 0.0; a.fst[ck] = -1; a.fext[ck] = 0.0;
-            a.G[3 * ck] = a.G[3 * ck + 1] = a.G[3 * ck + 2] = 0.0;
-            continue;
-        }
-        double qd = 0.0;
-        V3 gl{0, 0, 0};
+        g[k] = gb[k] = V3{0, 0, 0};
+        limb[k] = 0.0;
+        if (!used(k) || !rec) continue;
+        g[k] = ld3(a.gun + 3 * ck);
+        gb[k] = a.lim[ck] * gl[k];
+        limb[k] = dot(gl[k], g[k]);
+        if (limit && limb[k] != 0.0) any = true;
+    }
+    int fs[NFLD], br[NFLD];
+    double lm[NFLD], rs[NFLD], es[NFLD], mxD[NFLD], mnD[NFLD];
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k) {
+        fs[k] = -1; br[k] = 0; lm[k] = 1.0; rs[k] = 1.0; es[k] = 0.0;
+        const double vc = s.fld[(size_t)c * NFLD + k];
+        mxD[k] = mx[k] - vc;
+        mnD[k] = mn[k] - vc;
+    }
+    if (any) {
         for (int q = e0; q < e1; ++q) {
             const int f = m.cf[q];
-            if (f >= m.nI) continue;
-            const bool isOwn = m.own[f] == c;
-            const double av = a.fLR[(size_t)f * 2 * NFLD + (isOwn ? k : NFLD + k)];
-            qd += av;
-            if (ph.recon == 1) gl = gl + av * (ld3(m.Cf + 3 * (size_t)f) - Cc);
+            const V3 d = ld3(m.Cf + 3 * (size_t)f) - Cc;
+#pragma unroll
+            for (int k = 0; k < NFLD; ++k) {
+                if (!used(k) || limb[k] == 0.0) continue;
+                const double ext = dot(d, g[k]);
+                double r = 1;
+                int bq = 0;
+                if (ext > kSMALL) { r = mxD[k] / ext; bq = 1; }
+                else if (ext < -kSMALL) { r = mnD[k] / ext; bq = -1; }
+                double l;
+                if (ph.limiter == 1) l = (sqr(r) + 2 * r) / (sqr(r) + r + 2);
+                else l = fmin(r, 1.0);
+                if (l < lm[k]) { lm[k] = l; fs[k] = f; rs[k] = r; es[k] = ext; br[k] = bq; }
+            }
         }
-        int tgt = -1, fs = -1;
-        double tv = 0.0, fx = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k) {
+        const size_t ck = (size_t)c * NFLD + k;
+        int tgt = -1, fsk = -1;
+        double tv = 0.0, fx = 0.0, qk = qd[k];
         V3 G{0, 0, 0};
-        if (ph.recon == 1) {
-            const V3 g = ld3(a.gun + 3 * ck);
-            const double lim = a.lim[ck];
-            V3 gb = lim * gl;
-            const double limb = dot(gl, g);
-            if (ph.limiter != 0 && limb != 0.0) {
-                const double vc = s.fld[ck];
-                double mx = vc, mn = vc;
-                int imx = -1, imn = -1;
-                for (int q = e0; q < e1; ++q) {
-                    const int f = m.cf[q];
-                    double vo;
-                    int id;
-                    if (f < m.nI) {
-                        id = m.own[f] == c ? m.nei[f] : m.own[f];
-                        vo = s.fld[(size_t)id * NFLD + k];
-                    } else {
-                        const int b = f - m.nI;
-                        id = m.nC + b;
-                        vo = s.bfld[(size_t)b * NFLD + k];
-                    }
-                    if (mx < vo) { mx = vo; imx = id; }
-                    if (vo < mn) { mn = vo; imn = id; }
+        if (used(k) && rec) {
+            V3 gbk = gb[k];
+            if (limit && limb[k] != 0.0 && fs[k] >= 0 && br[k] != 0) {
+                double dl;
+                if (ph.limiter == 1) {
+                    const double den = sqr(rs[k]) + rs[k] + 2;
+                    dl = (-sqr(rs[k]) + 4 * rs[k] + 4) / sqr(den);
+                } else dl = rs[k] <= 1.0 ? 1.0 : 0.0;
+                const double rb = limb[k] * dl;
+                const double numb = rb / es[k];
+                fx = -rb * rs[k] / es[k];
+                fsk = fs[k];
+                gbk = gbk + fx * (ld3(m.Cf + 3 * (size_t)fsk) - Cc);
+                const int id = br[k] > 0 ? imx[k] : imn[k];
+                if (id >= 0) {
+                    tgt = id;
+                    tv = numb;
+                    qk -= numb;
                 }
-                const double mxD = mx - vc, mnD = mn - vc;
-                double lm = 1.0, rs = 1.0, es = 0.0;
-                int br = 0;
-                for (int q = e0; q < e1; ++q) {
-                    const int f = m.cf[q];
-                    const V3 d = ld3(m.Cf + 3 * (size_t)f) - Cc;
-                    const double ext = dot(d, g);
-                    double r = 1;
-                    int bq = 0;
-                    if (ext > kSMALL) { r = mxD / ext; bq = 1; }
-                    else if (ext < -kSMALL) { r = mnD / ext; bq = -1; }
-                    double l;
-                    if (ph.limiter == 1) l = (sqr(r) + 2 * r) / (sqr(r) + r + 2);
-                    else l = fmin(r, 1.0);
-                    if (l < lm) { lm = l; fs = f; rs = r; es = ext; br = bq; }
-                }
-                if (fs >= 0 && br != 0) {
-                    double dl;
-                    if (ph.limiter == 1) {
-                        const double den = sqr(rs) + rs + 2;
-                        dl = (-sqr(rs) + 4 * rs + 4) / sqr(den);
-                    } else dl = rs <= 1.0 ? 1.0 : 0.0;
-                    const double rb = limb * dl;
-                    const double numb = rb / es;
-                    fx = -rb * rs / es;
-                    gb = gb + fx * (ld3(m.Cf + 3 * (size_t)fs) - Cc);
-                    const int id = br > 0 ? imx : imn;
-                    if (id >= 0) {
-                        tgt = id;
-                        tv = numb;
-                        qd -= numb;
-                    }
-                } else fs = -1;
-            } else fs = -1;
-            G = gb / Vc;
+            }
+            G = gbk / Vc;
         }
-        a.qc[ck] = qd;
+        a.qc[ck] = used(k) ? qk : 0.0;
         a.tgt[ck] = tgt;
         a.tval[ck] = tv;
-        a.fst[ck] = fs;
+        a.fst[ck] = fsk;
         a.fext[ck] = fx;
         a.G[3 * ck] = G.x; a.G[3 * ck + 1] = G.y; a.G[3 * ck + 2] = G.z;
     }
@@ -339,7 +374,7 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
 // limiter's Cf part.
 __global__ void __launch_bounds__(256) k_adj_gauss(MeshView m, Phys ph, StateView s, AdjView a,
                                                    double* __restrict__ gS, double* __restrict__ gC) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcdBlock() * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
     double* qf = a.qf + (size_t)f * NFLD;
     if (f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY) {
@@ -434,7 +469,7 @@ __global__ void __launch_bounds__(256) k_adj_gauss(MeshView m, Phys ph, StateVie
 // c*, H, E -> rho, p, U, T and the updateFields chain back to W.
 __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateView s, AdjView a,
                                                    double* __restrict__ gW) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
     double q[NFLD];
@@ -512,10 +547,10 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
 
 void launchResidualVjp(const MeshView& m, const Phys& ph, const StateView& s, const AdjView& a, const double* v,
                        double* gW, double* gS, double* gC, hipStream_t st) {
-    k_adj_face<<<nblk(m.nF, 256), 256, 0, st>>>(m, ph, s, a, v, gS, gC);
-    k_adj_cell<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, a);
-    k_adj_gauss<<<nblk(m.nF, 256), 256, 0, st>>>(m, ph, s, a, gS, gC);
-    k_adj_state<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, a, gW);
+    k_adj_face<<<xcdGrid(m.nF, 256), 256, 0, st>>>(m, ph, s, a, v, gS, gC);
+    k_adj_cell<<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, a);
+    k_adj_gauss<<<xcdGrid(m.nF, 256), 256, 0, st>>>(m, ph, s, a, gS, gC);
+    k_adj_state<<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, a, gW);
 }
 
 }  // namespace icsf
