@@ -58,6 +58,9 @@ struct GmresArgs {
     // allows it (same algorithm; the forward-sweep terms of a row are summed
     // in three partial sums instead of one running difference)
     bool fuse = false;
+    // batched systems: fused operator + preconditioner (returns false when
+    // not applicable; op and pre are then used)
+    std::function<bool(const void*, const double*, void*, void*)> opPre;
 };
 
 // Harmonic-balance coupling: per-instance block storage (FP64) and the
@@ -140,6 +143,10 @@ void launchSpmvHB(const MeshView& m, const HBBlocks& hb, const double* x, const 
 // its per-component average; norm factors and initial residuals from
 // t = A (W - avg) and b; final residuals from r = b - A dW (r holds A dW on
 // entry).
+// y = LU-SGS(A_HB (x/scal)) in three passes on a two-colour level
+// structure (false: not applicable, nothing launched).
+bool launchSpmvLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, const double* x,
+                       const double* scal, double* vout, double* y, hipStream_t st);
 // LU-SGS of all HB instances (per-instance sweeps, shared diagonal rD).
 void launchLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, double* r,
                    hipStream_t st);

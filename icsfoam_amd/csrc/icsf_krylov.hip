@@ -141,12 +141,13 @@ __global__ void __launch_bounds__(256) k_spmv_cells(MeshView m, const TB* __rest
 // instances 0..nO-1 (own rho, rhoE blocks at K = J) and then the own
 // vector column; vector rows add the own scalar columns and then the vector
 // columns of instances 0..nO-1 (own block at K = J).
-__global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const double* __restrict__ x,
-                                                 const double* __restrict__ scal, double* __restrict__ vout,
-                                                 double* __restrict__ y) {
-    const int c = xcdBlock() * blockDim.x + threadIdx.x;
-    const int J = blockIdx.y;
-    if (c >= m.nC) return;
+// With FWD (colour-1 rows of a two-colour numbering): the forward LU-SGS
+// terms of the own instance are accumulated from the same block loads and
+// the row result is rD_c (y_c - fwd), as in spmvRow.
+template <bool FWD>
+__device__ __forceinline__ void spmvRowHB(const MeshView& m, const HBBlocks& hb, int J, const double* __restrict__ x,
+                                          const double* __restrict__ scal, double* __restrict__ vout,
+                                          double* __restrict__ y, const double* __restrict__ rD, int c) {
     const size_t nC = m.nC;
     const size_t vs = 5 * nC;
     const double* blk = hb.blk[J];
@@ -156,13 +157,14 @@ __global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const 
         const double v = x[(size_t)K * vs + (size_t)k * nC + j];
         return scal ? v / beta : v;
     };
+    const double* yJ = y + (size_t)J * vs;
     double xc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) xc[k] = X(J, k, c);
     if (vout)
 #pragma unroll
         for (int k = 0; k < 5; ++k) vout[(size_t)J * vs + (size_t)k * nC + c] = xc[k];
-    double acc[5][3];
+    double acc[5][3], fw[5][3];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
         const double b0 = dg[diagOff(c, 5 * i)], b1 = dg[diagOff(c, 5 * i + 1)];
@@ -170,6 +172,7 @@ __global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const 
         acc[i][0] = b0 * xc[0];
         acc[i][1] = b1 * xc[1];
         acc[i][2] = b2 * xc[2] + b3 * xc[3] + b4 * xc[4];
+        if (FWD) fw[i][0] = fw[i][1] = fw[i][2] = 0.0;
     }
     const int sl = c >> 6, lane = c & 63;
     const int len = m.rowLen[c];
@@ -177,9 +180,14 @@ __global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const 
     for (int k = 0; k < len; ++k) {
         const size_t slot = (base + k) * 64 + lane;
         const int j = m.col[slot];
-        double xj[5];
+        double xj[5], d[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) xj[q] = X(J, q, j);
+        if (FWD) {
+            const double rd = rD[j];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) d[q] = rd * yJ[(size_t)q * nC + j];
+        }
         const size_t bb = (base + k) * NB * 64 + lane;
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
@@ -188,12 +196,19 @@ __global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const 
                          b4 = blk[bb + (5 * i + 4) * 64];
             acc[i][0] += b0 * xj[0];
             acc[i][1] += b1 * xj[1];
-            const double d = b2 * xj[2] + b3 * xj[3] + b4 * xj[4];
-            acc[i][2] += d;
+            const double dd = b2 * xj[2] + b3 * xj[3] + b4 * xj[4];
+            acc[i][2] += dd;
+            if (FWD) {
+                fw[i][0] += b0 * d[0];
+                fw[i][1] += b1 * d[1];
+                const double de = b2 * d[2] + b3 * d[3] + b4 * d[4];
+                fw[i][2] += de;
+            }
         }
     }
     const double Vc = m.V[c];
     const int nO = hb.nO;
+    double out[5];
     // scalar rows
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -206,8 +221,7 @@ __global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const 
                 r = r + (Vc * hb.D[J * nO + K]) * X(K, i, c);
             }
         }
-        r = r + acc[i][2];
-        y[(size_t)J * vs + (size_t)i * nC + c] = r;
+        out[i] = r + acc[i][2];
     }
     // vector rows
 #pragma unroll
@@ -218,8 +232,40 @@ __global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const 
             if (K == J) r = r + acc[i][2];
             else r = r + (Vc * hb.D[J * nO + K]) * X(K, i, c);
         }
+        out[i] = r;
+    }
+    double rdc = 1.0;
+    if (FWD) rdc = rD[c];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double r = out[i];
+        if (FWD) {
+            r = r - fw[i][0];
+            r = r - fw[i][1];
+            r = r - fw[i][2];
+            r = rdc * r;
+        }
         y[(size_t)J * vs + (size_t)i * nC + c] = r;
     }
+}
+
+__global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const double* __restrict__ x,
+                                                 const double* __restrict__ scal, double* __restrict__ vout,
+                                                 double* __restrict__ y) {
+    const int c = xcdBlock() * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    spmvRowHB<false>(m, hb, blockIdx.y, x, scal, vout, y, nullptr, c);
+}
+
+// HB rows of a cell list (blockIdx.y = instance); FWD: fused forward sweep.
+template <bool FWD>
+__global__ void __launch_bounds__(256) k_spmv_hb_cells(MeshView m, HBBlocks hb, const double* __restrict__ x,
+                                                       const double* __restrict__ scal, double* __restrict__ vout,
+                                                       double* __restrict__ y, const double* __restrict__ rD,
+                                                       const int* __restrict__ cells, int n) {
+    const int t = xcdBlock() * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    spmvRowHB<FWD>(m, hb, blockIdx.y, x, scal, vout, y, rD, cells[t]);
 }
 
 // ------------------------------------------------------------- LU-SGS -----
@@ -840,7 +886,8 @@ void gmresCycleT(const GmresArgs& a, hipStream_t st) {
         // the sweeps too); otherwise SpMV, then the preconditioner
         mark(TK_SPMV, 1);
         bool fused = false;
-        if (!a.op && !a.pre && a.precond == 0 && a.fuse)
+        if (a.opPre) fused = a.opPre(r, &g->beta, vi, r2);
+        else if (!a.op && !a.pre && a.precond == 0 && a.fuse)
             fused = launchSpmvLusgs(m, *a.lev, a.blkFp32, sizeof(TV) == 4, a.blk, a.dg, a.rD, r, &g->beta, vi, r2,
                                     st);
         if (!fused) {
@@ -1002,6 +1049,19 @@ void launchLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, const
         dim3 grid(xcdGrid(n, 256), hb.nO);
         k_lusgs_hb<false><<<grid, 256, 0, st>>>(m, hb, rD, r, L.bwdCells + s0, n);
     }
+}
+
+bool launchSpmvLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, const double* x,
+                       const double* scal, double* vout, double* y, hipStream_t st) {
+    if (!L.twoColour) return false;
+    const int n0 = L.fwdPtrHost[1], n1 = L.fwdPtrHost[2] - n0;
+    k_spmv_hb_cells<false><<<dim3(xcdGrid(n0, 256), hb.nO), 256, 0, st>>>(m, hb, x, scal, vout, y, rD, L.fwdCells,
+                                                                         n0);
+    k_spmv_hb_cells<true><<<dim3(xcdGrid(n1, 256), hb.nO), 256, 0, st>>>(m, hb, x, scal, vout, y, rD,
+                                                                        L.fwdCells + n0, n1);
+    const int s0 = L.bwdPtrHost[1], nbw = L.bwdPtrHost[2] - s0;
+    k_lusgs_hb<false><<<dim3(xcdGrid(nbw, 256), hb.nO), 256, 0, st>>>(m, hb, rD, y, L.bwdCells + s0, nbw);
+    return true;
 }
 
 void sliceSubAverage(const MeshView& m, Work& wk, GmresScalars* g, const double* W, double* out, hipStream_t st) {

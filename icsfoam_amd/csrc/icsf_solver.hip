@@ -143,7 +143,13 @@ struct icsf_case {
     long long tcnt[NTIMER] = {0};
     DevBuf buf;
     bool ownStream = true;
+    // one steady-state pseudo-step captured as a HIP graph and replayed
+    bool useGraph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     ~icsf_case() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
         for (auto& e : events) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
         for (auto& e : pool) (void)hipEventDestroy(e);
         for (auto& e : open) (void)hipEventDestroy(e);
@@ -339,6 +345,10 @@ int readSettings(icsf_case* c, const char* thermo, const char* schemes, const ch
     if (orth == "MGS") c->orth = 0;
     else if (orth == "CGS") c->orth = 1;
     else return fail(ICSF_ERR_KEYWORD, "orthogonalisation must be MGS or CGS");
+    const std::string gcw = fd.word(base + "graphCapture", "true");
+    if (gcw == "true" || gcw == "on" || gcw == "yes") c->useGraph = true;
+    else if (gcw == "false" || gcw == "off" || gcw == "no") c->useGraph = false;
+    else return fail(ICSF_ERR_KEYWORD, "graphCapture must be true or false");
     const std::string fsw = fd.word(base + "fusedSweeps", "true");
     if (fsw == "true" || fsw == "on" || fsw == "yes") c->fuse = true;
     else if (fsw == "false" || fsw == "off" || fsw == "no") c->fuse = false;
@@ -457,7 +467,7 @@ int solveStage(icsf_case* c, const double* W, const double* b, int& nIter) {
     return gmresSolve(c, W, b, nIter);
 }
 
-int pseudoStep(icsf_case* c) {
+int pseudoStep(icsf_case* c, int& nIterOut) {
     const MeshView& m = c->mv;
     c->corr++;
     if (c->haveInit) {
@@ -471,6 +481,7 @@ int pseudoStep(icsf_case* c) {
     RC(jacobianStage(c));
     int nIter = 0;
     RC(solveStage(c, c->sv.W, c->sv.b, nIter));
+    nIterOut = nIter;
     k_copy5<<<1, 64, 0, c->st>>>(gmresResidualsDevice(c->g), c->dres);
     c->haveInit = true;
     if (c->localTS && c->bounding) {
@@ -482,6 +493,43 @@ int pseudoStep(icsf_case* c) {
         Timed t(c, 5);
         launchUpdate(m, c->ph, c->sv, c->dW, 1, nullptr, nullptr, nullptr, c->st);
     }
+    return ICSF_OK;
+}
+
+// From the third step on (SER active, initial/previous residuals held on
+// the device) a pseudo-step launches the same kernels with the same
+// arguments every time and, with one GMRES cycle (maxIter 1), never waits
+// on the host: it is captured once as a HIP graph and replayed.  Not with
+// event timing (its events are created per launch) or pending boundary
+// uploads.
+bool graphable(const icsf_case* c) {
+    return c->useGraph && !c->timing && c->maxIter == 1 && c->minIter <= 1 && c->haveInit && c->havePrev &&
+           c->corr >= 2 && !c->bcDirty;
+}
+
+int graphStep(icsf_case* c) {
+    if (!c->gexec) {
+        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        int it = 0;
+        const int rc = pseudoStep(c, it);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->st, &g);
+        c->corr--;  // the capture enqueued nothing; the replay below is the step
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc) return rc;
+            HIPCHK(e);
+        }
+        const hipError_t ie = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            c->gexec = nullptr;
+            HIPCHK(ie);
+        }
+        c->graph = g;
+    }
+    c->corr++;
+    HIPCHK(hipGraphLaunch(c->gexec, c->st));
     return ICSF_OK;
 }
 
@@ -622,6 +670,7 @@ struct icsf_hb {
     std::vector<GmresScalars*> gJ;
     Work wk;
     int nDirs = 30, maxIter = 1, minIter = 0, orth = 0;
+    bool fuse = true;
     double tol = 1e-12, relTol = 1e-2;
     int corr = 0;
     bool haveInit = false, havePrev = false, assembled = false;
@@ -756,6 +805,11 @@ int hbSolve(icsf_hb* h, const double* b, int& nIter) {
             launchSpmvHB(m, h->hbb, (const double*)x, scal, (double*)vout, (double*)y, h->st);
         };
         a.pre = [h](void* v) { hbLusgs(h, (double*)v); };
+        if (h->fuse)
+            a.opPre = [h, &m](const void* x, const double* scal, void* vout, void* y) {
+                return launchSpmvLusgsHB(m, h->mesh->lev, h->hbb, h->rD, (const double*)x, scal, (double*)vout,
+                                         (double*)y, h->st);
+            };
         gmresCycle(a, 0, h->st);
         launchUpdateSolution((size_t)m.nC * h->nO, h->dW, h->V, vA, h->nDirs, h->g, h->st);
         launchSpmvHB(m, h->hbb, h->dW, nullptr, nullptr, h->r, h->st);
@@ -1366,9 +1420,11 @@ int icsf_pseudo_steps(icsf_case* c, int nSteps, double* residuals, int* nIter) {
     if (!c || nSteps < 0) return fail(ICSF_ERR_ARG, "icsf_pseudo_steps: bad arguments");
     if (!c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
     for (int i = 0; i < nSteps; ++i) {
-        RC(pseudoStep(c));
+        int it = 1;
+        if (graphable(c)) RC(graphStep(c));
+        else RC(pseudoStep(c, it));
         if (residuals) gmresReadResiduals(c->g, residuals + 10 * (size_t)i, c->st);
-        if (nIter) nIter[i] = 1;
+        if (nIter) nIter[i] = it;
     }
     int e = 0;
     HIPCHK(hipMemcpyAsync(&e, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->st));
@@ -1458,6 +1514,7 @@ int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, con
         return fail(ICSF_ERR_KEYWORD, "harmonic balance: flowSolver must be GMRES with preconditioner LUSGS in "
                                       "double precision");
     H->nDirs = c0->nDirs; H->maxIter = c0->maxIter; H->minIter = c0->minIter; H->orth = c0->orth;
+    H->fuse = c0->fuse;
     H->tol = c0->tol; H->relTol = c0->relTol;
     const size_t nC = mesh->h.nCells, vs = 5 * nC, vA = vs * nInstants;
     DevBuf& B = H->buf;

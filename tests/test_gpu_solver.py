@@ -78,6 +78,25 @@ def test_fused_spmv_lusgs_matches_unfused(which):
         assert rel_err(rf, ru) < tol, kw
 
 
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_graph_replay_equals_eager(form):
+    """From the third step on a pseudo-step is replayed from a captured HIP
+    graph: 7 steps (5 replays, each followed by a host read of the
+    residuals) give the eager path's residual history and state bit for
+    bit."""
+    print(f"[graph {form}]", flush=True)
+    kw = dict() if form == "fp64" else dict(solver="FGMRES", nDirections=1, precision="mixed", innerDirections=20)
+    c = box_small()
+    ge = dbnsCase.from_case(_with(c, graphCapture=False, **kw))
+    gg = dbnsCase.from_case(_with(c, graphCapture=True, **kw))
+    he = ge.pseudo_steps(7)
+    hg = gg.pseudo_steps(7)
+    np.testing.assert_array_equal(hg, he)
+    for k in ("rho", "U", "T", "p", "W"):
+        np.testing.assert_array_equal(gg.fields()[k], ge.fields()[k])
+    np.testing.assert_array_equal(gg.pseudoCo(), ge.pseudoCo())
+
+
 def test_block_jacobi_bitwise():
     c = _with(box_small(), precond="Jacobi")
     print("[jacobi]", flush=True)
