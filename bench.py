@@ -148,12 +148,15 @@ def cpu_baseline(n_sample: int, n_full: int, steps: int = 2):
     """Oracle (sequential C++ restatement of the reference path, FP64) on a
     bounded sample of the same workload; steps/s scaled to the full cell
     count by the cell ratio (the work per step is linear in cells)."""
+    from oracle import oracle as O
+    from oracle.build import FAST_FLAGS, FAST_LIB
     from oracle.build import build as build_oracle
-    from oracle.oracle import OracleCase
     from icsfoam_amd.mesh import prepare_case
 
-    if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
-        build_oracle()
+    if not os.path.exists(FAST_LIB):
+        build_oracle(fast=True)
+    O.use_library(FAST_LIB)
+    OracleCase = O.OracleCase
     c = prepare_case(make_case(n_sample, "fp64", "MGS"), permute_seed=2024)
     o = OracleCase(c)
     o.pseudo_steps(1)
@@ -161,10 +164,19 @@ def cpu_baseline(n_sample: int, n_full: int, steps: int = 2):
     o.pseudo_steps(steps)
     dt = (time.perf_counter() - t0) / steps
     scale = (n_sample ** 3) / float(n_full ** 3)
+    cpu = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return dict(value=scale / dt, unit="pseudo-steps/s", cores=1, kind="port",
-                sample=f"oracle (sequential C++ FP64 restatement) on the {n_sample}^3 box of the same "
-                       f"workload, {steps} timed steps after 1 warm-up: {dt:.2f} s/step, scaled by the cell "
-                       f"ratio {n_sample ** 3}/{n_full ** 3}")
+                sample=f"oracle (sequential C++ FP64 restatement of the reference path, g++ "
+                       f"{' '.join(FAST_FLAGS)}, one thread as one OpenFOAM rank) on the {n_sample}^3 box of "
+                       f"the same workload, {steps} timed steps after 1 warm-up: {dt:.2f} s/step, scaled by the "
+                       f"cell ratio {n_sample ** 3}/{n_full ** 3}; host CPU: {cpu}; nproc {os.cpu_count()}")
 
 
 def bench_hb(steps, warmup):

@@ -15,10 +15,21 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 
-def build(openmp: bool = False) -> str:
+FAST_LIB = os.path.join(HERE, "liboracle_fast.so")
+FAST_FLAGS = ["-O3", "-march=x86-64-v3"]
+
+
+def build(openmp: bool = False, fast: bool = False) -> str:
+    """fast: the timing build of the CPU baseline (-O3, AVX2/FMA, contraction
+    allowed; x86-64-v3 rather than -march=native so that a library built on
+    one host runs on another); otherwise the bit-exact checker build."""
     src = os.path.join(HERE, "icsf_oracle.cpp")
-    out = LIB if not openmp else os.path.join(HERE, "liboracle_omp.so")
-    cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", src, "-o", out + ".tmp"]
+    if fast:
+        out = FAST_LIB
+        cmd = ["g++", *FAST_FLAGS, "-std=c++17", "-fPIC", "-shared", src, "-o", out + ".tmp"]
+    else:
+        out = LIB if not openmp else os.path.join(HERE, "liboracle_omp.so")
+        cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", src, "-o", out + ".tmp"]
     if openmp:
         cmd.insert(1, "-fopenmp")
     r = subprocess.run(cmd, capture_output=True, text=True)

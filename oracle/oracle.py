@@ -58,6 +58,15 @@ _SIG = {
 _lib = None
 
 
+def use_library(path):
+    """Load the given build of the oracle instead of the checker build (the
+    CPU-baseline timing uses the -O3 build); only before the first use."""
+    global LIB
+    if _lib is not None and LIB != path:
+        raise RuntimeError("the oracle library is already loaded")
+    LIB = path
+
+
 def lib():
     global _lib
     if _lib is None:
