@@ -63,6 +63,20 @@ struct GmresArgs {
     std::function<bool(const void*, const double*, void*, void*)> opPre;
 };
 
+// Harmonic balance, one launch per stage for all instances
+// (blockIdx.y = instance): each instance's mesh view (its own boundary
+// conditions), state and FP64 Jacobian storage, in device memory.
+struct HBInst {
+    MeshView m;
+    StateView s;
+    double* blk;
+    double* dg;
+    double* rD;
+    int* err;
+};
+void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st);
+void launchJacobianHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st);
+
 // Harmonic-balance coupling: per-instance block storage (FP64) and the
 // spectral operator D [nO][nO] (device).
 constexpr int HB_MAXO = 16;

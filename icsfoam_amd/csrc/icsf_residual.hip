@@ -78,7 +78,7 @@ __global__ void k_update(MeshView m, Phys ph, StateView s, const double* __restr
 
 // Boundary values of p, U, T (fixedValue / zeroGradient / slip) and the
 // derived reconstruction fields on every non-empty boundary face.
-__global__ void k_boundary(MeshView m, Phys ph, StateView s) {
+__device__ __forceinline__ void boundaryBody(const MeshView& m, const Phys& ph, const StateView& s) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nB) return;
     if (m.bKind[b] == PK_EMPTY) return;
@@ -112,13 +112,19 @@ __global__ void k_boundary(MeshView m, Phys ph, StateView s) {
     s.bT[b] = TB;
 }
 
+__global__ void k_boundary(MeshView m, Phys ph, StateView s) { boundaryBody(m, ph, s); }
+__global__ void k_boundary_hb(const HBInst* __restrict__ I, Phys ph) {
+    const HBInst q = I[blockIdx.y];
+    boundaryBody(q.m, ph, q.s);
+}
+
 // Gauss linear gradients with cellLimited<limiter> (k = 1) of the NG
 // reconstruction fields; unlimited grad(U) and grad(ecalc) when viscous.
 // gun/limo (optional, [nC][NFLD][3] and [nC][NFLD]) receive the unlimited
 // gradients and the limiter values for the adjoint pass.
 template <int NG>
-__global__ void __launch_bounds__(256) k_grad(MeshView m, Phys ph, StateView s, double* __restrict__ gun,
-                                              double* __restrict__ limo) {
+__device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, const StateView& s,
+                                         double* __restrict__ gun, double* __restrict__ limo) {
     const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     double gx[NG], gy[NG], gz[NG], mx[NG], mn[NG], vc[NG];
@@ -222,6 +228,17 @@ __global__ void __launch_bounds__(256) k_grad(MeshView m, Phys ph, StateView s, 
     for (int k = 0; k < NG; ++k) { go[3 * k] = gx[k]; go[3 * k + 1] = gy[k]; go[3 * k + 2] = gz[k]; }
 }
 
+template <int NG>
+__global__ void __launch_bounds__(256) k_grad(MeshView m, Phys ph, StateView s, double* __restrict__ gun,
+                                              double* __restrict__ limo) {
+    gradBody<NG>(m, ph, s, gun, limo);
+}
+template <int NG>
+__global__ void __launch_bounds__(256) k_grad_hb(const HBInst* __restrict__ I, Phys ph) {
+    const HBInst q = I[blockIdx.y];
+    gradBody<NG>(q.m, ph, q.s, nullptr, nullptr);
+}
+
 __device__ __forceinline__ double soundSpeed(const Phys& ph, double T) {
     const double psi = 1.0 / (ph.R * T);
     return sqrt(ph.gamma / psi);
@@ -277,7 +294,7 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
 //                neighbour side (internal faces)
 //   FT_SV        0.5*lambdaVisc*|S|*deltaCoeffs (internal, viscous)
 // Cell kernels then sum these in face order (the reference's face loops).
-__global__ void __launch_bounds__(256) k_face_terms(MeshView m, Phys ph, StateView s) {
+__device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph, const StateView& s) {
     const int f = xcdBlock() * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
     if (f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY) return;
@@ -376,9 +393,15 @@ __global__ void __launch_bounds__(256) k_face_terms(MeshView m, Phys ph, StateVi
     }
 }
 
+__global__ void __launch_bounds__(256) k_face_terms(MeshView m, Phys ph, StateView s) { faceTermsBody(m, ph, s); }
+__global__ void __launch_bounds__(256) k_face_terms_hb(const HBInst* __restrict__ I, Phys ph) {
+    const HBInst q = I[blockIdx.y];
+    faceTermsBody(q.m, ph, q.s);
+}
+
 // Residual b = R*V (residualsUpdate.H) and rPseudoDeltaT
 // (setCoAndDeltaT.H:57-141) of each cell from the face terms.
-__global__ void __launch_bounds__(256) k_cell_residual(MeshView m, Phys ph, StateView s) {
+__device__ __forceinline__ void cellResidualBody(const MeshView& m, const Phys& ph, const StateView& s) {
     const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     double ar = 0, ae = 0;
@@ -421,6 +444,14 @@ __global__ void __launch_bounds__(256) k_cell_residual(MeshView m, Phys ph, Stat
     s.rDT[c] = rdt / s.pseudoCo[c];
 }
 
+__global__ void __launch_bounds__(256) k_cell_residual(MeshView m, Phys ph, StateView s) {
+    cellResidualBody(m, ph, s);
+}
+__global__ void __launch_bounds__(256) k_cell_residual_hb(const HBInst* __restrict__ I, Phys ph) {
+    const HBInst q = I[blockIdx.y];
+    cellResidualBody(q.m, ph, q.s);
+}
+
 // Face fluxes of every face (convectiveFluxScheme::calcFlux output) from
 // the face terms; fluxes of empty faces are zero.
 __global__ void k_face_flux(MeshView m, StateView s, double* phi, double* phiU, double* phiE) {
@@ -444,9 +475,9 @@ __global__ void k_face_flux(MeshView m, StateView s, double* phi, double* phiU, 
 // dissipation and the viscous parts separately and combines them in the
 // order the reference adds its temporary matrices.
 template <typename TB>
-__global__ void __launch_bounds__(256) k_cell_jacobian(MeshView m, Phys ph, StateView s, TB* __restrict__ blk,
-                                                       TB* __restrict__ dg, double* __restrict__ rD,
-                                                       int* __restrict__ err) {
+__device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& ph, const StateView& s,
+                                                 TB* __restrict__ blk, TB* __restrict__ dg,
+                                                 double* __restrict__ rD, int* __restrict__ err) {
     const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const double g = ph.gamma;
@@ -608,6 +639,17 @@ __global__ void __launch_bounds__(256) k_cell_jacobian(MeshView m, Phys ph, Stat
     if (!(r >= kVSMALL)) atomicOr(err, 1);
 }
 
+template <typename TB>
+__global__ void __launch_bounds__(256) k_cell_jacobian(MeshView m, Phys ph, StateView s, TB* __restrict__ blk,
+                                                       TB* __restrict__ dg, double* __restrict__ rD,
+                                                       int* __restrict__ err) {
+    cellJacobianBody<TB>(m, ph, s, blk, dg, rD, err);
+}
+__global__ void __launch_bounds__(256) k_cell_jacobian_hb(const HBInst* __restrict__ I, Phys ph) {
+    const HBInst q = I[blockIdx.y];
+    cellJacobianBody<double>(q.m, ph, q.s, q.blk, q.dg, q.rD, q.err);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ launchers --
@@ -624,6 +666,18 @@ void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipSt
     else k_grad<8><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, nullptr, nullptr);
     k_face_terms<<<xcdGrid(m.nF, 256), 256, 0, st>>>(m, ph, s);
     k_cell_residual<<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s);
+}
+
+void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st) {
+    if (m.nB > 0) k_boundary_hb<<<dim3(nblk(m.nB, 256), nO), 256, 0, st>>>(inst, ph);
+    if (ph.flux == FLUX_ROE) k_grad_hb<7><<<dim3(xcdGrid(m.nC, 256), nO), 256, 0, st>>>(inst, ph);
+    else k_grad_hb<8><<<dim3(xcdGrid(m.nC, 256), nO), 256, 0, st>>>(inst, ph);
+    k_face_terms_hb<<<dim3(xcdGrid(m.nF, 256), nO), 256, 0, st>>>(inst, ph);
+    k_cell_residual_hb<<<dim3(xcdGrid(m.nC, 256), nO), 256, 0, st>>>(inst, ph);
+}
+
+void launchJacobianHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st) {
+    k_cell_jacobian_hb<<<dim3(xcdGrid(m.nC, 256), nO), 256, 0, st>>>(inst, ph);
 }
 
 void launchAdjointForward(const MeshView& m, const Phys& ph, const StateView& s, double* gun, double* limo,

@@ -666,6 +666,7 @@ struct icsf_hb {
     double *Wall = nullptr, *ball = nullptr, *r = nullptr, *r2 = nullptr, *V = nullptr, *dW = nullptr;
     double *t1 = nullptr, *t2 = nullptr, *rD = nullptr, *dres = nullptr;
     int* derr = nullptr;
+    HBInst* dInst = nullptr;  // per-instance views, blockIdx.y of the batched launches
     GmresScalars* g = nullptr;
     std::vector<GmresScalars*> gJ;
     Work wk;
@@ -688,16 +689,22 @@ struct HBDiag {
     double* dg[HB_MAXO];
 };
 
-// b_J + source_J, source_J = ((0 - V D_J0 W_0) - V D_J1 W_1) - ...
-__global__ void k_hb_source(int nC, int nO, int J, const double* __restrict__ D, const double* __restrict__ Vol,
-                            const double* __restrict__ Wall, const double* __restrict__ bJ, double* __restrict__ out) {
+// Instance J = blockIdx.y: out_J = b_J + source_J with
+// source_J = ((0 - V D_J0 W_0) - V D_J1 W_1) - ..., and W_J copied into the
+// coupled state vector Wall (written only, never read here).
+__global__ void k_hb_source(int nC, int nO, const HBInst* __restrict__ I, const double* __restrict__ D,
+                            const double* __restrict__ Vol, double* __restrict__ Wall, double* __restrict__ out) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y;
     if (c >= nC) return;
     const size_t vs = (size_t)5 * nC;
+    const double* WJ = I[J].s.W;
+    const double* bJ = I[J].s.b;
     for (int q = 0; q < 5; ++q) {
         double src = 0.0;
-        for (int K = 0; K < nO; ++K) src -= Vol[c] * D[J * nO + K] * Wall[K * vs + (size_t)q * nC + c];
-        out[(size_t)q * nC + c] = bJ[(size_t)q * nC + c] + src;
+        for (int K = 0; K < nO; ++K) src -= Vol[c] * D[J * nO + K] * I[K].s.W[(size_t)q * nC + c];
+        out[J * vs + (size_t)q * nC + c] = bJ[(size_t)q * nC + c] + src;
+        Wall[J * vs + (size_t)q * nC + c] = WJ[(size_t)q * nC + c];
     }
 }
 
@@ -759,21 +766,22 @@ int hbUploadBC(icsf_hb* h) {
     return ICSF_OK;
 }
 
+// Residuals of all instances, each kernel launched once over the
+// instances (blockIdx.y), then the spectral source.
 int hbResidualStage(icsf_hb* h) {
     const MeshView& m = h->inst[0]->mv;
-    const size_t vs = (size_t)5 * m.nC;
     RC(hbUploadBC(h));
-    for (int J = 0; J < h->nO; ++J) launchCopy(vs, 0, h->inst[J]->sv.W, h->Wall + J * vs, h->st);
-    for (int J = 0; J < h->nO; ++J) RC(residualStage(h->inst[J]));
-    for (int J = 0; J < h->nO; ++J)
-        k_hb_source<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, h->nO, J, h->dD, m.V, h->Wall, h->inst[J]->sv.b,
-                                                         h->ball + J * vs);
+    launchResidualHB(h->dInst, h->nO, m, h->inst[0]->ph, h->st);
+    for (icsf_case* c : h->inst) c->haveResidual = true;
+    k_hb_source<<<dim3(nblk(m.nC, 256), h->nO), 256, 0, h->st>>>(m.nC, h->nO, h->dInst, h->dD, m.V, h->Wall,
+                                                                  h->ball);
     return ICSF_OK;
 }
 
 int hbJacobianStage(icsf_hb* h) {
     const MeshView& m = h->inst[0]->mv;
-    for (int J = 0; J < h->nO; ++J) RC(jacobianStage(h->inst[J]));
+    launchJacobianHB(h->dInst, h->nO, m, h->inst[0]->ph, h->st);
+    for (icsf_case* c : h->inst) c->assembled = true;
     HBDiag hd{};
     for (int J = 0; J < h->nO; ++J) hd.dg[J] = (double*)h->inst[J]->dg;
     k_hb_diag<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, h->nO, h->dD, m.V, hd, h->rD, h->derr);
@@ -1539,10 +1547,15 @@ int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, con
     }
     H->hbb.nO = nInstants;
     H->hbb.D = H->dD;
+    std::vector<HBInst> hi(nInstants);
     for (int J = 0; J < nInstants; ++J) {
-        H->hbb.blk[J] = (const double*)H->inst[J]->blk;
-        H->hbb.dg[J] = (const double*)H->inst[J]->dg;
+        icsf_case* c = H->inst[J];
+        H->hbb.blk[J] = (const double*)c->blk;
+        H->hbb.dg[J] = (const double*)c->dg;
+        hi[J] = HBInst{c->mv, c->sv, (double*)c->blk, (double*)c->dg, c->rD, c->derr};
     }
+    RC(B.alloc(H->dInst, (size_t)nInstants));
+    HIPCHK(hipMemcpy(H->dInst, hi.data(), sizeof(HBInst) * nInstants, hipMemcpyHostToDevice));
     *out = H.release();
     return ICSF_OK;
 }
