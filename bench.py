@@ -180,9 +180,9 @@ def cpu_baseline(n_sample: int, n_full: int, steps: int = 2):
 
 
 def bench_hb(steps, warmup):
-    """Harmonic-balance pseudo-steps/s on a 64x64x32 annular sector with 7
-    instants (3 harmonics of 100 Hz), Roe inviscid, GMRES(30) with CGS +
-    LU-SGS, FP64."""
+    """Harmonic-balance pseudo-steps/s on a 64x64x32 annular sector with
+    rotational cyclic theta patches and 7 instants (3 harmonics of 100 Hz),
+    Roe inviscid, GMRES(30) with CGS + LU-SGS, FP64."""
     from icsfoam_amd import cases
     from icsfoam_amd.mesh import prepare_case
     from icsfoam_amd.solver import hbCase
@@ -205,8 +205,8 @@ def bench_hb(steps, warmup):
             "unit": "pseudo-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
             "ms_per_step": round(1e3 * dt / steps, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded annular sector, periodic inflow)",
-            "config": {"workload": f"64x64x32 annular sector ({nC} cells) x 7 instants "
-                                   f"({7 * nC} block rows), Roe inviscid, GMRES(30, CGS)+LU-SGS"}}
+            "config": {"workload": f"64x64x32 annular sector with rotational cyclic patches ({nC} cells) x 7 "
+                                   f"instants ({7 * nC} block rows), Roe inviscid, GMRES(30, CGS)+LU-SGS"}}
 
 
 def bench_vjp(calls):

@@ -46,13 +46,17 @@ SIGNATURES = {
     "icsf_last_error": (C.c_char_p, []),
     "icsf_version": (_i, []),
     "icsf_rcm_order": (_i, [_i, _i, _vp, _vp, _vp]),
+    "icsf_rcm_order_coupled": (_i, [_i, _i, _vp, _vp, _i, _vp, _vp, _vp]),
+    "icsf_mesh_geometry_patches": (_i, [_i, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp,
+                                        _vp, _vp, _vp, _vp, _vp]),
+    "icsf_get_interface_blocks": (_i, [_vp, _vp]),
     "icsf_colour_cells": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
     "icsf_colour_major_order": (_i, [_i, _vp, _i, _vp]),
     "icsf_renumber_mesh": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_sweep_levels": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_block_mesh_hex": (_i, [_i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp]),
     "icsf_mesh_geometry": (_i, [_i, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "icsf_mesh_create": (_i, [_i, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "icsf_mesh_create": (_i, [_i, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_mesh_destroy": (_i, [_vp]),
     "icsf_mesh_info": (_i, [_vp, _vp]),
     "icsf_case_create": (_i, [_vp, C.c_char_p, C.c_char_p, C.c_char_p, _vp]),

@@ -19,7 +19,7 @@ from typing import Dict, List, Optional, Tuple
 import numpy as np
 
 # patch kinds understood by the library (mirrors OpenFOAM polyPatch types)
-PATCH_PATCH, PATCH_WALL, PATCH_EMPTY = 0, 1, 2
+PATCH_PATCH, PATCH_WALL, PATCH_EMPTY, PATCH_CYCLIC = 0, 1, 2, 3
 # boundary-condition kinds (OpenFOAM fvPatchField type names)
 BC_ZERO_GRADIENT, BC_FIXED_VALUE, BC_SLIP = 0, 1, 2
 
@@ -33,6 +33,11 @@ class Patch:
     kind: int
     start: int
     size: int
+    # cyclic patches: index of the paired patch and forwardT (3x3, maps the
+    # paired side's vectors into this side's frame); face i of the patch is
+    # coupled to face i of the paired patch
+    neighbour: int = -1
+    transform: Optional[np.ndarray] = None
 
 
 @dataclasses.dataclass
@@ -61,6 +66,35 @@ class PolyMesh:
         kind = np.array([p.kind for p in self.patches], np.int32)
         return start, size, kind
 
+    def coupling_arrays(self):
+        """(neighbour patch [nPatches], forwardT [nPatches, 9]) of the
+        cyclic patches (-1 and zeros elsewhere)."""
+        nbr = np.array([p.neighbour for p in self.patches], np.int32)
+        T = np.zeros((len(self.patches), 9))
+        for i, p in enumerate(self.patches):
+            if p.transform is not None:
+                T[i] = np.asarray(p.transform, np.float64).reshape(9)
+        return nbr, T
+
+    @property
+    def has_cyclic(self) -> bool:
+        return any(p.kind == PATCH_CYCLIC for p in self.patches)
+
+    def coupled_cells(self):
+        """Boundary faces of cyclic patches in face order: (owner cell,
+        cell across the face), as decompositionMethod::calcCellCells adds
+        them."""
+        own, nbr = [], []
+        for p in self.patches:
+            if p.kind != PATCH_CYCLIC:
+                continue
+            q = self.patches[p.neighbour]
+            own.append(self.owner[p.start:p.start + p.size])
+            nbr.append(self.owner[q.start:q.start + q.size])
+        if not own:
+            return np.zeros(0, np.int32), np.zeros(0, np.int32)
+        return np.concatenate(own).astype(np.int32), np.concatenate(nbr).astype(np.int32)
+
 
 @dataclasses.dataclass
 class BoundarySpec:
@@ -80,7 +114,7 @@ class Case:
     rho: np.ndarray     # [nC]
     U: np.ndarray       # [nC, 3]
     T: np.ndarray       # [nC]
-    bcs: List[BoundarySpec]
+    bcs: List[Optional[BoundarySpec]]   # None on cyclic (constraint) patches
     thermo: str
     fvSchemes: str
     fvSolution: str
@@ -440,36 +474,115 @@ def permute_case(case: Case, new_to_old: np.ndarray, renumber) -> Case:
 # harmonic balance: annular sector with a periodic inflow
 # --------------------------------------------------------------------------
 
+def rotation_z(angle: float) -> np.ndarray:
+    c, s = math.cos(angle), math.sin(angle)
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
 def annulus_sector(nt: int, nr: int, nz: int, theta: float = np.pi / 8, r0: float = 0.5, r1: float = 1.0,
-                   length: float = 1.0, jitter: float = 0.0, seed: int = 0) -> PolyMesh:
+                   length: float = 1.0, jitter: float = 0.0, seed: int = 0, cyclic: bool = False,
+                   jitter_period: Optional[int] = None, warp_planes: bool = False) -> PolyMesh:
     """Annular sector theta in [0, theta], r in [r0, r1], z in [0, length]
     (index directions r, theta, z: a right-handed map, cells ordered r
     fastest); interior nodes jittered in the index space.  Patches: inlet
-    (z = 0), outlet (z = length), sides (the two theta planes) and
-    hub/shroud walls."""
+    (z = 0), outlet (z = length), hub/shroud walls and the two theta planes:
+    one slip-wall patch "sides", or (cyclic) the rotational cyclic pair
+    cyclicLow (theta = 0, forwardT = rotation by -theta about z) and
+    cyclicHigh (theta = theta, rotation by +theta).  With cyclic, the jitter
+    repeats every jitter_period theta cells (default nt), so the theta = 0
+    and theta = theta planes carry rotated copies of the same nodes; with
+    warp_planes those planes are jittered in theta too (non-planar cyclic
+    faces)."""
     kk, jj, ii = np.meshgrid(np.arange(nz + 1), np.arange(nt + 1), np.arange(nr + 1), indexing="ij")
     a, b, c = ii.astype(np.float64), jj.astype(np.float64), kk.astype(np.float64)
     if jitter > 0:
         rng = np.random.default_rng(seed)
-        d = rng.uniform(-jitter, jitter, size=(3,) + a.shape)
+        tmask = (jj > 0) & (jj < nt)
+        if cyclic:
+            per = jitter_period or nt
+            d = rng.uniform(-jitter, jitter, size=(3, nz + 1, per, nr + 1))[:, :, np.arange(nt + 1) % per, :]
+            if warp_planes:
+                tmask = np.ones_like(tmask)
+            else:
+                tmask &= (jj % per) != 0  # planes that are images of theta = 0 stay planar
+        else:
+            d = rng.uniform(-jitter, jitter, size=(3,) + a.shape)
         a = a + d[0] * ((ii > 0) & (ii < nr))
-        b = b + d[1] * ((jj > 0) & (jj < nt))
+        b = b + d[1] * tmask
         c = c + d[2] * ((kk > 0) & (kk < nz))
     r = r0 + (r1 - r0) * a / nr
     th = theta * b / nt
     z = length * c / nz
     pts = np.stack([(r * np.cos(th)).ravel(), (r * np.sin(th)).ravel(), z.ravel()], 1).astype(np.float64)
+    if cyclic:
+        sides = [("cyclicLow", PATCH_CYCLIC, ["-y"]), ("cyclicHigh", PATCH_CYCLIC, ["+y"])]
+    else:
+        sides = [("sides", PATCH_WALL, ["-y", "+y"])]
     fv, own, nei, patches = _hex_topology(
         nr, nt, nz,
         [("inlet", PATCH_PATCH, ["-z"]), ("outlet", PATCH_PATCH, ["+z"]),
-         ("hubShroud", PATCH_WALL, ["-x", "+x"]), ("sides", PATCH_WALL, ["-y", "+y"])])
+         ("hubShroud", PATCH_WALL, ["-x", "+x"])] + sides)
+    if cyclic:
+        lo, hi = len(patches) - 2, len(patches) - 1
+        patches[lo] = dataclasses.replace(patches[lo], neighbour=hi, transform=rotation_z(-theta))
+        patches[hi] = dataclasses.replace(patches[hi], neighbour=lo, transform=rotation_z(theta))
     return _mesh_from_quads(pts, fv, own, nei, patches, nt * nr * nz, pts.reshape(nz + 1, nt + 1, nr + 1, 3))
+
+
+def case_sector(nt: int = 8, nr: int = 6, nz: int = 5, theta: float = np.pi / 8, copies: int = 1,
+                jitter: float = 0.1, seed: int = 3, flux: str = "ROE", swirl: bool = True,
+                recon: str = "linearUpwind limitedGrad", noise: float = 1.0, warp_planes: bool = False,
+                **sol) -> Case:
+    """Rotation-periodic state on an annular sector with rotational cyclic
+    patches: `copies` rotated copies of a base sector of nt theta cells and
+    angle theta (the mesh spans copies*theta).  Cell values of the base
+    sector are seeded (density, temperature, radial/swirl/axial velocity in
+    the local cylindrical frame); copy m carries the same values with the
+    velocity rotated by m*theta, so every copy sees identical neighbours.
+    Boundary conditions: fixed axial inflow velocity and temperature at z = 0,
+    fixed pressure at z = length, slip hub/shroud walls."""
+    mesh = annulus_sector(copies * nt, nr, nz, theta=copies * theta, jitter=jitter, seed=seed, cyclic=True,
+                          jitter_period=nt, warp_planes=warp_planes)
+    rng = np.random.default_rng(seed + 2)
+    base = rng.standard_normal((nz, nt, nr, 5)) * noise
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(copies * nt), np.arange(nr), indexing="ij")
+    v = base[k, j % nt, i]
+    copy = j // nt
+    rho = 1.2 * (1 + 0.05 * v[..., 0])
+    T = 300.0 * (1 + 0.03 * v[..., 1])
+    ur = 20.0 * v[..., 2] if swirl else 0.0 * v[..., 2]
+    ut = 30.0 + 10.0 * v[..., 3] if swirl else 0.0 * v[..., 3]
+    uz = 150.0 + 20.0 * v[..., 4]
+    C = approx_cell_centres(mesh).reshape(nz, copies * nt, nr, 3)
+    th0 = np.arctan2(C[..., 1], C[..., 0]) - theta * copy   # angle of the base-sector cell
+    U = np.stack([ur * np.cos(th0) - ut * np.sin(th0), ur * np.sin(th0) + ut * np.cos(th0), uz], -1)
+    for m_ in range(1, copies):
+        sel = copy == m_
+        U[sel] = U[sel] @ rotation_z(m_ * theta).T
+    bcs: List[Optional[BoundarySpec]] = []
+    for p in mesh.patches:
+        if p.kind == PATCH_CYCLIC:
+            bcs.append(None)
+        elif p.name == "inlet":
+            Uv = np.zeros((p.size, 3))
+            Uv[:, 2] = 150.0
+            bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_FIXED_VALUE, BC_FIXED_VALUE, None, Uv,
+                                    np.full(p.size, 300.0)))
+        elif p.name == "outlet":
+            bcs.append(BoundarySpec(BC_FIXED_VALUE, BC_ZERO_GRADIENT, BC_ZERO_GRADIENT,
+                                    np.full(p.size, 1.0e5), None, None))
+        else:
+            bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_SLIP, BC_ZERO_GRADIENT))
+    return Case(f"sector{copies}", mesh, rho.ravel(), U.reshape(-1, 3), T.ravel(), bcs, thermo_dict(0.0),
+                schemes_dict(flux, recon=recon), solution_dict(**sol))
 
 
 def case_hb(n=(64, 64, 32), n_harmonics: int = 3, frequency_hz: float = 100.0, amplitude: float = 0.05,
             mach: float = 0.5, noise: float = 0.02, seed: int = 3, co: float = 20.0, nDirections: int = 30,
-            jitter: float = 0.1, flux: str = "ROE", mu: float = 0.0, orthogonalisation: str = "MGS"):
-    """Harmonic-balance case on an annular sector: 2 n_harmonics + 1
+            jitter: float = 0.1, flux: str = "ROE", mu: float = 0.0, orthogonalisation: str = "MGS",
+            cyclic: bool = True):
+    """Harmonic-balance case on an annular sector with rotational cyclic
+    theta patches (slip side walls with cyclic=False): 2 n_harmonics + 1
     instants over one period of frequency_hz (selectedPeriod = 1/f), the
     inflow velocity of instant n scaled by 1 + amplitude sin(omega t_n),
     steady axial flow at the given Mach number with smooth noise as the
@@ -480,7 +593,7 @@ def case_hb(n=(64, 64, 32), n_harmonics: int = 3, frequency_hz: float = 100.0, a
     omega = 2.0 * np.pi * frequency_hz
     nO = 2 * n_harmonics + 1
     D, t, _ = hb_operator([omega], [n_harmonics], nO, 1.0 / frequency_hz)
-    mesh = annulus_sector(*n, jitter=jitter, seed=seed)
+    mesh = annulus_sector(*n, jitter=jitter, seed=seed, cyclic=cyclic)
     C = approx_cell_centres(mesh)
     T0, p0 = 300.0, 1e5
     rho0, U0 = _state_from_mach(mach, p0, T0)
@@ -493,9 +606,11 @@ def case_hb(n=(64, 64, 32), n_harmonics: int = 3, frequency_hz: float = 100.0, a
         U = np.zeros((mesh.n_cells, 3))
         U[:, 2] = U0 * (1 + s + amplitude * ph)
         U[:, 0] = 0.02 * U0 * s
-        bcs = []
+        bcs: List[Optional[BoundarySpec]] = []
         for pt in mesh.patches:
-            if pt.name == "inlet":
+            if pt.kind == PATCH_CYCLIC:
+                bcs.append(None)
+            elif pt.name == "inlet":
                 Uv = np.zeros((pt.size, 3)); Uv[:, 2] = U0 * (1 + amplitude * ph)
                 bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_FIXED_VALUE, BC_FIXED_VALUE, None, Uv,
                                         np.full(pt.size, T0)))

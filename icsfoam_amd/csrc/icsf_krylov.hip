@@ -34,6 +34,35 @@ __device__ __forceinline__ double ld(const T* p, size_t i) { return (double)p[i]
 // (three partial sums per row as well); the row result is then
 // rD_c (y_c - fwd), i.e. the forward and the first backward level of the
 // LU-SGS sweep of a two-colour level structure.
+// Cyclic faces of row c: interfacesUpper & (x_far with its momentum part
+// transformed by forwardT), added to the three partial sums after the
+// internal faces (blockFvMatrix::Amul, blockFvMatrix.C:398-420).  Not part
+// of the LU-SGS sweeps (lusgs.C:143-186 visits internal faces only).
+template <typename TB, typename A, typename XF>
+__device__ __forceinline__ void spmvCyclic(const MeshView& m, const TB* __restrict__ blk, int c, XF X,
+                                           A acc[5][3]) {
+    const int q1 = m.cycRowOff[c + 1];
+    for (int q = m.cycRowOff[c]; q < q1; ++q) {
+        const int kc = m.cycRowK[q];
+        const int j = m.cycCell[kc];
+        const double* R = m.cycR + 9 * (size_t)kc;
+        const A x0 = X(0, j), x1 = X(1, j), x2 = X(2, j), x3 = X(3, j), x4 = X(4, j);
+        const A v2 = (A)R[0] * x2 + (A)R[1] * x3 + (A)R[2] * x4;
+        const A v3 = (A)R[3] * x2 + (A)R[4] * x3 + (A)R[5] * x4;
+        const A v4 = (A)R[6] * x2 + (A)R[7] * x3 + (A)R[8] * x4;
+        const TB* ib = blk + m.cycBase + (size_t)kc * NB;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const A b0 = (A)ib[5 * i], b1 = (A)ib[5 * i + 1], b2 = (A)ib[5 * i + 2], b3 = (A)ib[5 * i + 3],
+                    b4 = (A)ib[5 * i + 4];
+            acc[i][0] += b0 * x0;
+            acc[i][1] += b1 * x1;
+            const A dd = b2 * v2 + b3 * v3 + b4 * v4;
+            acc[i][2] += dd;
+        }
+    }
+}
+
 template <typename TB, typename TV, bool FWD>
 __device__ __forceinline__ void spmvRow(const MeshView& m, const TB* __restrict__ blk, const TB* __restrict__ dg,
                                         const TV* __restrict__ x, const double* __restrict__ scal,
@@ -96,6 +125,7 @@ __device__ __forceinline__ void spmvRow(const MeshView& m, const TB* __restrict_
             }
         }
     }
+    if (m.nCyc > 0) spmvCyclic<TB, A>(m, blk, c, X, acc);
     A rdc = (A)1;
     if (FWD) rdc = (A)rD[c];
 #pragma unroll
@@ -206,6 +236,7 @@ __device__ __forceinline__ void spmvRowHB(const MeshView& m, const HBBlocks& hb,
             }
         }
     }
+    if (m.nCyc > 0) spmvCyclic<double, double>(m, blk, c, [&](int k, int j) { return X(J, k, j); }, acc);
     const double Vc = m.V[c];
     const int nO = hb.nO;
     double out[5];

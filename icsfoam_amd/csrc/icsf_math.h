@@ -31,6 +31,11 @@ HD double magSqr(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
 HD double mag(V3 a) { return sqrt(magSqr(a)); }
 HD V3 cross(V3 a, V3 b) { return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 HD double sqr(double x) { return x * x; }
+// transform(R, v) = R & v for a row-major rotation tensor R
+HD V3 rot3(const double* R, V3 v) {
+    return V3{R[0] * v.x + R[1] * v.y + R[2] * v.z, R[3] * v.x + R[4] * v.y + R[5] * v.z,
+              R[6] * v.x + R[7] * v.y + R[8] * v.z};
+}
 
 struct T9 {
     double t[9];

@@ -34,7 +34,8 @@ int HostMesh::patchOf(int face) const {
 std::string buildHostMesh(HostMesh& m, int nPoints, const double* points, int nFaces,
                           const int* faceOff, const int* faceVerts, int nInternal,
                           const int* owner, const int* neighbour, int nPatches,
-                          const int* patchStart, const int* patchSize, const int* patchKind) {
+                          const int* patchStart, const int* patchSize, const int* patchKind,
+                          const int* patchNbr, const double* patchT) {
     if (nPoints <= 0 || nFaces <= 0 || nInternal < 0 || nInternal > nFaces)
         return "invalid mesh sizes";
     m.nPoints = nPoints;
@@ -48,6 +49,14 @@ std::string buildHostMesh(HostMesh& m, int nPoints, const double* points, int nF
     m.patchStart.assign(patchStart, patchStart + nPatches);
     m.patchSize.assign(patchSize, patchSize + nPatches);
     m.patchKind.assign(patchKind, patchKind + nPatches);
+    m.patchNbr.assign(nPatches, -1);
+    m.patchT.assign(9 * (size_t)nPatches, 0.0);
+    for (int p = 0; p < nPatches; ++p) {
+        if (patchKind[p] != PATCH_CYCLIC) continue;
+        if (!patchNbr || !patchT) return "cyclic patches need a neighbour patch and a transform";
+        m.patchNbr[p] = patchNbr[p];
+        for (int e = 0; e < 9; ++e) m.patchT[9 * (size_t)p + e] = patchT[9 * (size_t)p + e];
+    }
     int nc = 0;
     for (int f = 0; f < nFaces; ++f) nc = std::max(nc, owner[f] + 1);
     for (int f = 0; f < nInternal; ++f) {
@@ -69,12 +78,99 @@ std::string buildHostMesh(HostMesh& m, int nPoints, const double* points, int nF
         expect += patchSize[p];
     }
     if (expect != nFaces) return "patches must cover all boundary faces";
+    for (int p = 0; p < nPatches; ++p) {
+        if (patchKind[p] != PATCH_CYCLIC) continue;
+        const int q = m.patchNbr[p];
+        if (q < 0 || q >= nPatches || q == p || patchKind[q] != PATCH_CYCLIC || m.patchNbr[q] != p)
+            return "cyclic patch " + std::to_string(p) + ": neighbour patch must be a distinct cyclic patch "
+                   "naming it back";
+        if (patchSize[q] != patchSize[p]) return "cyclic patch " + std::to_string(p) + ": size differs from its "
+                                                 "neighbour patch";
+        const double* A = &m.patchT[9 * (size_t)p];
+        const double* Bq = &m.patchT[9 * (size_t)q];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double aat = 0, ab = 0;
+                for (int k = 0; k < 3; ++k) {
+                    aat += A[3 * i + k] * A[3 * j + k];
+                    ab += A[3 * i + k] * Bq[3 * k + j];
+                }
+                const double id = i == j ? 1.0 : 0.0;
+                if (std::fabs(aat - id) > 1e-12)
+                    return "cyclic patch " + std::to_string(p) + ": transform is not a rotation";
+                if (std::fabs(ab - id) > 1e-12)
+                    return "cyclic patch " + std::to_string(p) + ": transform is not the inverse of the "
+                           "neighbour patch's";
+            }
+    }
     computeGeometry(m);
     // checkMesh's "negative volume": an inverted cell (faces ordered the
     // wrong way round) would make every face flux change sign
     for (int c = 0; c < m.nCells; ++c)
         if (!(m.V[c] > 0)) return "cell " + std::to_string(c) + " has non-positive volume (inverted face order?)";
+    std::string err = computeCyclic(m);
+    if (!err.empty()) return err;
     buildCellFaces(m);
+    return "";
+}
+
+// Cyclic pairs: face i of a cyclic patch is coupled to face i of its
+// neighbour patch; forwardT of the patch maps the neighbour side's vectors
+// into this side's frame.  Weights, deltas and deltaCoeffs follow OpenFOAM
+// cyclicFvPatch::makeWeights (w = dn/(d + dn), d = nf & (Cf - C)) and
+// cyclicFvPatch::delta ((Cf - C) - forwardT & (Cf' - C')), which
+// surfaceInterpolation turns into deltaCoeffs, nonOrthDeltaCoeffs and
+// nonOrthCorrectionVectors as on internal faces.
+std::string computeCyclic(HostMesh& m) {
+    const int nB = m.nFaces - m.nInternal;
+    m.cycOf.assign(std::max(nB, 0), -1);
+    m.cycFace.clear(); m.cycCell.clear(); m.cycPair.clear(); m.cycR.clear(); m.cycDelta.clear();
+    for (size_t p = 0; p < m.patchStart.size(); ++p) {
+        if (m.patchKind[p] != PATCH_CYCLIC) continue;
+        const int q = m.patchNbr[p];
+        const double* R = &m.patchT[9 * p];
+        for (int i = 0; i < m.patchSize[p]; ++i) {
+            const int f = m.patchStart[p] + i, g = m.patchStart[q] + i;
+            const int c = m.owner[f], n = m.owner[g];
+            const double* Sf = &m.Sf[3 * (size_t)f];
+            const double* Sg = &m.Sf[3 * (size_t)g];
+            double nf[3], ng[3], d[3], dn[3], Rdn[3], Rng[3];
+            for (int k = 0; k < 3; ++k) {
+                nf[k] = Sf[k] / m.magSf[f];
+                ng[k] = Sg[k] / m.magSf[g];
+                d[k] = m.Cf[3 * (size_t)f + k] - m.C[3 * (size_t)c + k];
+                dn[k] = m.Cf[3 * (size_t)g + k] - m.C[3 * (size_t)n + k];
+            }
+            for (int k = 0; k < 3; ++k) {
+                Rdn[k] = R[3 * k] * dn[0] + R[3 * k + 1] * dn[1] + R[3 * k + 2] * dn[2];
+                Rng[k] = R[3 * k] * ng[0] + R[3 * k + 1] * ng[1] + R[3 * k + 2] * ng[2];
+            }
+            // paired faces must face each other after the transform
+            if (std::fabs(Rng[0] + nf[0]) + std::fabs(Rng[1] + nf[1]) + std::fabs(Rng[2] + nf[2]) > 1e-6 ||
+                std::fabs(m.magSf[f] - m.magSf[g]) > 1e-6 * m.magSf[f])
+                return "cyclic patch " + std::to_string(p) + " face " + std::to_string(i) +
+                       ": does not match face " + std::to_string(i) + " of its neighbour patch";
+            const double di = dot3(nf, d), dni = dot3(ng, dn);
+            m.weights[f] = dni / (di + dni);
+            double delta[3] = {d[0] - Rdn[0], d[1] - Rdn[1], d[2] - Rdn[2]};
+            m.deltaCoeffs[f] = 1.0 / mag3(delta);
+            m.nonOrthDeltaCoeffs[f] = 1.0 / std::max(dot3(nf, delta), 0.05 * mag3(delta));
+            for (int k = 0; k < 3; ++k) m.corrVecs[3 * (size_t)f + k] = nf[k] - delta[k] * m.nonOrthDeltaCoeffs[f];
+            m.cycOf[f - m.nInternal] = (int)m.cycFace.size();
+            m.cycFace.push_back(f);
+            m.cycCell.push_back(n);
+            m.cycPair.push_back(g);
+            for (int e = 0; e < 9; ++e) m.cycR.push_back(R[e]);
+            for (int k = 0; k < 3; ++k) m.cycDelta.push_back(delta[k]);
+        }
+    }
+    const int nC = m.nCells;
+    m.cycRowOff.assign(nC + 1, 0);
+    for (int f : m.cycFace) m.cycRowOff[m.owner[f] + 1]++;
+    for (int c = 0; c < nC; ++c) m.cycRowOff[c + 1] += m.cycRowOff[c];
+    m.cycRowK.assign(m.cycFace.size(), 0);
+    std::vector<int> pos(m.cycRowOff.begin(), m.cycRowOff.end() - 1);
+    for (size_t k = 0; k < m.cycFace.size(); ++k) m.cycRowK[pos[m.owner[m.cycFace[k]]]++] = (int)k;
     return "";
 }
 
@@ -261,6 +357,21 @@ void buildCellFaces(HostMesh& m) {
     for (int c = 0; c < nC; ++c) std::sort(m.cellFaces.begin() + cnt[c], m.cellFaces.begin() + cnt[c + 1]);
 }
 
+void cellCellsCoupled(int nCells, int nInternal, const int* owner, const int* neighbour, int nCoupled,
+                      const int* cOwner, const int* cNbr, std::vector<int>& off, std::vector<int>& adj) {
+    off.assign(nCells + 1, 0);
+    for (int f = 0; f < nInternal; ++f) { off[owner[f] + 1]++; off[neighbour[f] + 1]++; }
+    for (int b = 0; b < nCoupled; ++b) off[cOwner[b] + 1]++;
+    for (int c = 0; c < nCells; ++c) off[c + 1] += off[c];
+    adj.assign(off[nCells], 0);
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (int f = 0; f < nInternal; ++f) {
+        adj[pos[owner[f]]++] = neighbour[f];
+        adj[pos[neighbour[f]]++] = owner[f];
+    }
+    for (int b = 0; b < nCoupled; ++b) adj[pos[cOwner[b]]++] = cNbr[b];
+}
+
 void cellCells(int nCells, int nInternal, const int* owner, const int* neighbour,
                std::vector<int>& off, std::vector<int>& adj) {
     off.assign(nCells + 1, 0);
@@ -312,9 +423,9 @@ void cuthillMcKee(int nCells, const std::vector<int>& off, const std::vector<int
 }  // namespace
 
 void rcmOrder(int nCells, int nInternal, const int* owner, const int* neighbour,
-              std::vector<int>& newToOld) {
+              std::vector<int>& newToOld, int nCoupled, const int* cOwner, const int* cNbr) {
     std::vector<int> off, adj, order;
-    cellCells(nCells, nInternal, owner, neighbour, off, adj);
+    cellCellsCoupled(nCells, nInternal, owner, neighbour, nCoupled, cOwner, cNbr, off, adj);
     cuthillMcKee(nCells, off, adj, order);
     newToOld.assign(order.rbegin(), order.rend());
 }

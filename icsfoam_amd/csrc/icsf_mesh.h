@@ -8,7 +8,7 @@
 
 namespace icsf {
 
-enum PatchKind { PATCH_PATCH = 0, PATCH_WALL = 1, PATCH_EMPTY = 2 };
+enum PatchKind { PATCH_PATCH = 0, PATCH_WALL = 1, PATCH_EMPTY = 2, PATCH_CYCLIC = 3 };
 
 // Slice height of the sliced-ELL block storage: one wavefront of rows.
 constexpr int SLICE = 64;
@@ -19,6 +19,8 @@ struct HostMesh {
     std::vector<int> faceOff, faceVerts;     // CSR polygon list
     std::vector<int> owner, neighbour;       // [nFaces], [nInternal]
     std::vector<int> patchStart, patchSize, patchKind;
+    std::vector<int> patchNbr;               // [nPatches] paired cyclic patch, -1
+    std::vector<double> patchT;              // [nPatches][9] cyclic forwardT (row-major)
 
     // geometry (OpenFOAM primitiveMesh / surfaceInterpolation definitions)
     std::vector<double> Sf, magSf, Cf;       // [nFaces*3], [nFaces], [nFaces*3]
@@ -32,6 +34,15 @@ struct HostMesh {
     std::vector<int> cellFaceOff, cellFaces; // faces of each cell sorted by face
                                              // index, empty-patch faces omitted
     std::vector<int> faceIsEmpty;            // [nFaces - nInternal]
+
+    // cyclic faces, numbered k = 0.. in face order
+    std::vector<int> cycOf;                  // [nFaces - nInternal] k, or -1
+    std::vector<int> cycFace;                // [nCyc] face
+    std::vector<int> cycCell;                // [nCyc] cell on the far side (owner of the paired face)
+    std::vector<int> cycPair;                // [nCyc] paired face
+    std::vector<double> cycR;                // [nCyc][9] forwardT of the face's patch
+    std::vector<double> cycDelta;            // [nCyc][3] cyclicFvPatch::delta()
+    std::vector<int> cycRowOff, cycRowK;     // cell -> its cyclic faces (face order)
     bool twoD = false;                       // has an empty patch
     int emptyDir = -1;                       // direction removed by the empty patch
 
@@ -42,19 +53,30 @@ struct HostMesh {
 std::string buildHostMesh(HostMesh& m, int nPoints, const double* points, int nFaces,
                           const int* faceOff, const int* faceVerts, int nInternal,
                           const int* owner, const int* neighbour, int nPatches,
-                          const int* patchStart, const int* patchSize, const int* patchKind);
+                          const int* patchStart, const int* patchSize, const int* patchKind,
+                          const int* patchNbr = nullptr, const double* patchT = nullptr);
 
 void computeGeometry(HostMesh& m);
+// Pairing and geometry of cyclic faces (after computeGeometry); error text
+// or empty.
+std::string computeCyclic(HostMesh& m);
 void buildCellFaces(HostMesh& m);
 
 // --- renumbering (integer work; must be reproducible bit-for-bit) -------
 // cellCells in internal-face order (OpenFOAM primitiveMesh::calcCellCells).
 void cellCells(int nCells, int nInternal, const int* owner, const int* neighbour,
                std::vector<int>& off, std::vector<int>& adj);
+// The same with the connections across coupled (cyclic) faces appended to
+// each cell's list in boundary-face order (decompositionMethod::calcCellCells,
+// which OpenFOAM's renumberMesh uses): nCoupled faces with the cell on each
+// side.
+void cellCellsCoupled(int nCells, int nInternal, const int* owner, const int* neighbour, int nCoupled,
+                      const int* cOwner, const int* cNbr, std::vector<int>& off, std::vector<int>& adj);
 // Cuthill-McKee band compression (OpenFOAM meshTools::bandCompression),
 // reversed: newToOld.
 void rcmOrder(int nCells, int nInternal, const int* owner, const int* neighbour,
-              std::vector<int>& newToOld);
+              std::vector<int>& newToOld, int nCoupled = 0, const int* cOwner = nullptr,
+              const int* cNbr = nullptr);
 // Greedy colouring in cell-index order.
 int greedyColour(int nCells, int nInternal, const int* owner, const int* neighbour,
                  std::vector<int>& colour);

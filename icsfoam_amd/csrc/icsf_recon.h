@@ -41,4 +41,38 @@ __device__ __forceinline__ void recon(const MeshView& m, const StateView& s, int
     R = r;
 }
 
+// Far-side value of field k on cyclic face k-th (cyclicFvPatchField::
+// patchNeighbourField as patched by the reference,
+// originalOFFiles/constraintFvPatchFields/cyclic/cyclicFvPatchField.C:127-188):
+// the far cell's value, and for the velocity components ("U.component(i)")
+// component i of forwardT & U of the far cell.
+__device__ __forceinline__ double cycNbrValue(const StateView& s, const double* R, int n, int k) {
+    if (k >= F_UX && k <= F_UZ) {
+        const V3 u = rot3(R, cellU(s.fld, n));
+        return k == F_UX ? u.x : (k == F_UY ? u.y : u.z);
+    }
+    return s.fld[(size_t)n * NFLD + k];
+}
+
+// L (owner side) and R (far side) reconstructed value of field k on cyclic
+// face f (linearUpwind::correction, coupled branch): L = v_P + (Cf - C_P) &
+// grad_P, R = vn + ((Cf - delta) - C_P) & (forwardT & grad_N).
+__device__ __forceinline__ void reconCyclic(const MeshView& m, const StateView& s, int recon, int f, int o, int kc,
+                                            int k, double& L, double& R) {
+    const double* Rt = m.cycR + 9 * (size_t)kc;
+    const int n = m.cycCell[kc];
+    double l = s.fld[(size_t)o * NFLD + k];
+    double r = cycNbrValue(s, Rt, n, k);
+    if (recon == 1) {
+        const V3 fc = ld3(m.Cf + 3 * (size_t)f);
+        const double* go = s.grd + ((size_t)o * NFLD + k) * 3;
+        const double* gn = s.grd + ((size_t)n * NFLD + k) * 3;
+        l += dot(fc - ld3(m.C + 3 * (size_t)o), V3{go[0], go[1], go[2]});
+        const V3 gR = rot3(Rt, V3{gn[0], gn[1], gn[2]});
+        r += dot((fc - ld3(m.cycDelta + 3 * (size_t)kc)) - ld3(m.C + 3 * (size_t)o), gR);
+    }
+    L = l;
+    R = r;
+}
+
 }  // namespace icsf

@@ -81,7 +81,7 @@ __global__ void k_update(MeshView m, Phys ph, StateView s, const double* __restr
 __device__ __forceinline__ void boundaryBody(const MeshView& m, const Phys& ph, const StateView& s) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nB) return;
-    if (m.bKind[b] == PK_EMPTY) return;
+    if (m.bKind[b] == PK_EMPTY || m.bKind[b] == PK_CYCLIC) return;
     const int f = m.nI + b;
     const int c = m.own[f];
     const double* fc = s.fld + (size_t)c * NFLD;
@@ -162,6 +162,22 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
                 const double vf = w * (eo - en) + en;
                 const V3 t = S * vf;
                 gex = isOwn ? gex + t : gex - t;
+            }
+        } else if (m.bKind[f - m.nI] == PK_CYCLIC) {
+            // coupled linear interpolation w v_P + (1 - w) v_far; max/min
+            // over the far-side value (cellLimitedGrad, coupled patches)
+            const int kc = m.cycOf[f - m.nI];
+            const double* R = m.cycR + 9 * (size_t)kc;
+            const int n = m.cycCell[kc];
+            const double w = m.w[f];
+#pragma unroll
+            for (int k = 0; k < NG; ++k) {
+                const double vn = cycNbrValue(s, R, n, k);
+                const double vf = w * vc[k] + (1.0 - w) * vn;
+                const V3 t = S * vf;
+                gx[k] = gx[k] + t.x; gy[k] = gy[k] + t.y; gz[k] = gz[k] + t.z;
+                mx[k] = fmax(mx[k], vn);
+                mn[k] = fmin(mn[k], vn);
             }
         } else {
             const int b = f - m.nI;
@@ -250,7 +266,19 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
     const V3 S = ld3(m.Sf + 3 * (size_t)f);
     const double ms = m.magSf[f];
     double rL, rR, pL, pR, uxL, uxR, uyL, uyR, uzL, uzR, HL, HR, xL, xR, eL = 0, eR = 0;
-    if (f < m.nI) {
+    double gI = ph.gamma;  // fvc::interpolate(gamma): exact on internal faces
+    if (f >= m.nI && m.bKind[f - m.nI] == PK_CYCLIC) {
+        const int o = m.own[f], kc = m.cycOf[f - m.nI];
+        reconCyclic(m, s, ph.recon, f, o, kc, F_UX, uxL, uxR);
+        reconCyclic(m, s, ph.recon, f, o, kc, F_UY, uyL, uyR);
+        reconCyclic(m, s, ph.recon, f, o, kc, F_UZ, uzL, uzR);
+        reconCyclic(m, s, ph.recon, f, o, kc, F_P, pL, pR);
+        reconCyclic(m, s, ph.recon, f, o, kc, F_RHO, rL, rR);
+        reconCyclic(m, s, ph.recon, f, o, kc, F_H, HL, HR);
+        reconCyclic(m, s, ph.recon, f, o, kc, ph.flux == FLUX_ROE ? F_E : F_CS, xL, xR);
+        if (ph.flux == FLUX_HLLC) reconCyclic(m, s, ph.recon, f, o, kc, F_E, eL, eR);
+        gI = m.w[f] * ph.gamma + (1.0 - m.w[f]) * ph.gamma;
+    } else if (f < m.nI) {
         const int o = m.own[f], n = m.nei[f];
         recon(m, s, ph.recon, f, o, n, F_UX, uxL, uxR);
         recon(m, s, ph.recon, f, o, n, F_UY, uyL, uyR);
@@ -275,9 +303,9 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
     if (ph.flux == FLUX_AUSM)
         ausmFace(ph.lowMach, S, ms, UL, UR, xL, xR, pL, pR, rL, rR, HL, HR, fr, fu, fe);
     else if (ph.flux == FLUX_ROE)
-        roeFace(ph.gamma, ph.entropyFix, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, HL, HR, fr, fu, fe);
+        roeFace(gI, ph.entropyFix, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, HL, HR, fr, fu, fe);
     else
-        hllcFace(ph.gamma, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, eL, eR, HL, HR, fr, fu, fe);
+        hllcFace(gI, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, eL, eR, HL, HR, fr, fu, fe);
 }
 
 // Face terms, one thread per face (each face evaluated once):
@@ -357,6 +385,26 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
             const double lv = (muf + alf) / rf;
             o[FT_SV] = 0.5 * lv * ms * m.dc[f];
         }
+    } else if (m.bKind[f - m.nI] == PK_CYCLIC) {
+        // coupled interpolation of c and U (setCoAndDeltaT.H:44-56,78-89,
+        // convectiveFluxScheme.C:509-524); reconstructed U, E for the
+        // interface block (owner side, far side)
+        const int ow = m.own[f], kc = m.cycOf[f - m.nI];
+        const int ne = m.cycCell[kc];
+        const double w = m.w[f];
+        const double cf = w * soundSpeed(ph, s.T[ow]) + (1.0 - w) * soundSpeed(ph, s.T[ne]);
+        const V3 Un = rot3(m.cycR + 9 * (size_t)kc, cellU(s.fld, ne));
+        const V3 Uf = w * cellU(s.fld, ow) + (1.0 - w) * Un;
+        const double lam = cf + fabs(dot(Uf, S / ms));
+        o[FT_LAM] = lam;
+        o[FT_FRDT] = m.nodc[f] * lam;
+        double uxL, uxR, uyL, uyR, uzL, uzR, EL, ER;
+        reconCyclic(m, s, ph.recon, f, ow, kc, F_UX, uxL, uxR);
+        reconCyclic(m, s, ph.recon, f, ow, kc, F_UY, uyL, uyR);
+        reconCyclic(m, s, ph.recon, f, ow, kc, F_UZ, uzL, uzR);
+        reconCyclic(m, s, ph.recon, f, ow, kc, F_E, EL, ER);
+        o[FT_REC] = uxL; o[FT_REC + 1] = uyL; o[FT_REC + 2] = uzL; o[FT_REC + 3] = EL;
+        o[FT_REC + 4] = uxR; o[FT_REC + 5] = uyR; o[FT_REC + 6] = uzR; o[FT_REC + 7] = ER;
     } else {
         const int b = f - m.nI;
         const int c = m.own[f];
@@ -492,7 +540,34 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
     int slot = 0;
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
-        if (f >= m.nI) continue;
+        if (f >= m.nI) {
+            // cyclic face: interface block (interfacesUpper: 1/2|S| A_R -
+            // 1/2|S| lambda I) and diag -= interfacesLower (+1/2|S| A_L),
+            // after all internal faces as in insertBlock
+            // (blockFvMatrix.C:241-265, :297-323); gamma interpolated
+            if (m.nCyc == 0 || m.bKind[f - m.nI] != PK_CYCLIC) continue;
+            const int kc = m.cycOf[f - m.nI];
+            const double* t = s.ft + (size_t)f * NFT;
+            const double ms = m.magSf[f];
+            const V3 nrm = ld3(m.Sf + 3 * (size_t)f) / ms;
+            const double gf = m.w[f] * g + (1.0 - m.w[f]) * g;
+            double A[NB];
+            fluxJacobian(gf, V3{t[FT_REC + 4], t[FT_REC + 5], t[FT_REC + 6]}, t[FT_REC + 7], nrm, A);
+            const double hb = 0.5 * ms;
+            const double v = 0.5 * ms * t[FT_LAM];
+            TB* ib = blk + m.cycBase + (size_t)kc * NB;
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                double x = hb * A[e];
+                if (e == 0 || e == 6 || e == 12 || e == 18 || e == 24) x = x - v;
+                ib[e] = (TB)x;
+            }
+            fluxJacobian(gf, V3{t[FT_REC], t[FT_REC + 1], t[FT_REC + 2]}, t[FT_REC + 3], nrm, A);
+            const double hd = -0.5 * ms;
+#pragma unroll
+            for (int e = 0; e < NB; ++e) D[e] -= hd * A[e];
+            continue;
+        }
         const bool isOwn = (m.own[f] == c);
         const double* t = s.ft + (size_t)f * NFT;
         const V3 S = ld3(m.Sf + 3 * (size_t)f);
@@ -540,6 +615,7 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
         const int f = m.cf[q];
         if (f < m.nI) continue;
         const int b = f - m.nI;
+        if (m.bKind[b] == PK_CYCLIC) continue;  // coupled patches carry no boundary terms
         const V3 Sf = ld3(m.Sf + 3 * (size_t)f);
         const V3 n = Sf / m.magSf[f];
         const double pVIC = m.bPT[b] == BC_FIXED ? 0.0 : 1.0;

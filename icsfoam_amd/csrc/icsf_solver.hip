@@ -240,15 +240,18 @@ __global__ void k_bound_flag(int nC, double lb, const double* W, double* bad) {
     bad[c] = (rho < rhoMin || eT < eMin || eT < 1.0e-15) ? 1.0 : 0.0;
 }
 
-__global__ void k_bound_apply(int nC, int nI, const int* cfOff, const int* cf, const int* own, const int* nei,
-                              const double* bad, double* co) {
+// boundLocalTimeStep.H:28-97: internal faces, then cyclic faces (the far
+// cell through patchNeighbourField).
+__global__ void k_bound_apply(MeshView m, const double* bad, double* co) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nC) return;
+    if (c >= m.nC) return;
     double f = 1.0;
-    for (int q = cfOff[c]; q < cfOff[c + 1]; ++q) {
-        const int fc = cf[q];
-        if (fc >= nI) continue;
-        const int other = own[fc] == c ? nei[fc] : own[fc];
+    for (int q = m.cfOff[c]; q < m.cfOff[c + 1]; ++q) {
+        const int fc = m.cf[q];
+        int other;
+        if (fc < m.nI) other = m.own[fc] == c ? m.nei[fc] : m.own[fc];
+        else if (m.nCyc > 0 && m.cycOf[fc - m.nI] >= 0) other = m.cycCell[m.cycOf[fc - m.nI]];
+        else continue;
         if (bad[c] != 0.0) f = fmin(0.5, f);
         if (bad[other] != 0.0) f = fmin(0.75, f);
     }
@@ -486,8 +489,7 @@ int pseudoStep(icsf_case* c, int& nIterOut) {
     c->haveInit = true;
     if (c->localTS && c->bounding) {
         k_bound_flag<<<nblk(m.nC, 256), 256, 0, c->st>>>(m.nC, c->lowerBound, c->sv.W, c->fac);
-        k_bound_apply<<<nblk(m.nC, 256), 256, 0, c->st>>>(m.nC, m.nI, m.cfOff, m.cf, m.own, m.nei, c->fac,
-                                                          c->sv.pseudoCo);
+        k_bound_apply<<<nblk(m.nC, 256), 256, 0, c->st>>>(c->mv, c->fac, c->sv.pseudoCo);
     }
     {
         Timed t(c, 5);
@@ -601,6 +603,7 @@ namespace {
 int adjointPrepare(icsf_case* c) {
     if (c->ph.viscous) return fail(ICSF_ERR_ARG, "residual VJP: viscous terms are not differentiated (inviscid only)");
     if (c->ph.flux != FLUX_AUSM) return fail(ICSF_ERR_ARG, "residual VJP: implemented for the AUSMPlusUp flux scheme");
+    if (c->mv.nCyc > 0) return fail(ICSF_ERR_ARG, "residual VJP: meshes with cyclic patches are not supported");
     if (c->adjReady) return ICSF_OK;
     const size_t nC = c->mv.nC, nF = c->mv.nF, nB = std::max(c->mv.nB, 1);
     AdjView& a = c->adj;
@@ -675,9 +678,14 @@ struct icsf_hb {
     double tol = 1e-12, relTol = 1e-2;
     int corr = 0;
     bool haveInit = false, havePrev = false, assembled = false;
+    bool useGraph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     DevBuf buf;
     ~icsf_hb() {
         if (st) (void)hipStreamSynchronize(st);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
         for (icsf_case* c : inst) delete c;
         if (st) (void)hipStreamDestroy(st);
     }
@@ -866,11 +874,46 @@ int hbPseudoStep(icsf_hb* h) {
         icsf_case* c = h->inst[J];
         if (c->localTS && c->bounding) {
             k_bound_flag<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, c->lowerBound, c->sv.W, c->fac);
-            k_bound_apply<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, m.nI, m.cfOff, m.cf, m.own, m.nei, c->fac,
-                                                               c->sv.pseudoCo);
+            k_bound_apply<<<nblk(m.nC, 256), 256, 0, h->st>>>(c->mv, c->fac, c->sv.pseudoCo);
         }
         launchUpdate(m, c->ph, c->sv, h->dW + J * vs, 1, nullptr, nullptr, nullptr, h->st);
     }
+    return ICSF_OK;
+}
+
+// As graphStep: from the third step on an HB pseudo-step with one GMRES
+// cycle is a fixed launch sequence without host waits; it is captured once
+// and replayed.
+bool hbGraphable(const icsf_hb* h) {
+    if (!h->useGraph || h->maxIter != 1 || h->minIter > 1 || !h->haveInit || !h->havePrev || h->corr < 2)
+        return false;
+    for (const icsf_case* c : h->inst)
+        if (c->bcDirty) return false;
+    return true;
+}
+
+int hbGraphStep(icsf_hb* h) {
+    if (!h->gexec) {
+        HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+        const int rc = hbPseudoStep(h);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(h->st, &g);
+        h->corr--;  // the capture enqueued nothing; the replay below is the step
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc) return rc;
+            HIPCHK(e);
+        }
+        const hipError_t ie = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            h->gexec = nullptr;
+            HIPCHK(ie);
+        }
+        h->graph = g;
+    }
+    h->corr++;
+    HIPCHK(hipGraphLaunch(h->gexec, h->st));
     return ICSF_OK;
 }
 
@@ -891,6 +934,21 @@ int icsf_rcm_order(int nCells, int nInternalFaces, const int* owner, const int* 
     if (nCells <= 0 || nInternalFaces < 0 || !newToOld) return fail(ICSF_ERR_ARG, "icsf_rcm_order: bad arguments");
     std::vector<int> o;
     rcmOrder(nCells, nInternalFaces, owner, neighbour, o);
+    std::copy(o.begin(), o.end(), newToOld);
+    return ICSF_OK;
+}
+
+int icsf_rcm_order_coupled(int nCells, int nInternalFaces, const int* owner, const int* neighbour, int nCoupled,
+                           const int* coupledOwner, const int* coupledNeighbour, int* newToOld) {
+    if (nCells <= 0 || nInternalFaces < 0 || nCoupled < 0 || !newToOld || (nCoupled > 0 && (!coupledOwner ||
+                                                                                          !coupledNeighbour)))
+        return fail(ICSF_ERR_ARG, "icsf_rcm_order_coupled: bad arguments");
+    for (int b = 0; b < nCoupled; ++b)
+        if (coupledOwner[b] < 0 || coupledOwner[b] >= nCells || coupledNeighbour[b] < 0 ||
+            coupledNeighbour[b] >= nCells)
+            return fail(ICSF_ERR_ARG, "icsf_rcm_order_coupled: coupled cell out of range");
+    std::vector<int> o;
+    rcmOrder(nCells, nInternalFaces, owner, neighbour, o, nCoupled, coupledOwner, coupledNeighbour);
     std::copy(o.begin(), o.end(), newToOld);
     return ICSF_OK;
 }
@@ -976,15 +1034,38 @@ int icsf_mesh_geometry(int nPoints, const double* points, int nFaces, const int*
     return ICSF_OK;
 }
 
+int icsf_mesh_geometry_patches(int nPoints, const double* points, int nFaces, const int* faceOffsets,
+                               const int* faceVerts, int nInternalFaces, const int* owner, const int* neighbour,
+                               int nPatches, const int* patchStart, const int* patchSize, const int* patchKind,
+                               const int* patchNeighbour, const double* patchTransform, double* weights,
+                               double* deltaCoeffs, double* nonOrthDeltaCoeffs, double* corrVecs,
+                               int* cyclicCell) {
+    HostMesh h;
+    std::string err = buildHostMesh(h, nPoints, points, nFaces, faceOffsets, faceVerts, nInternalFaces, owner,
+                                    neighbour, nPatches, patchStart, patchSize, patchKind, patchNeighbour,
+                                    patchTransform);
+    if (!err.empty()) return fail(ICSF_ERR_ARG, err);
+    if (weights) std::copy(h.weights.begin(), h.weights.end(), weights);
+    if (deltaCoeffs) std::copy(h.deltaCoeffs.begin(), h.deltaCoeffs.end(), deltaCoeffs);
+    if (nonOrthDeltaCoeffs) std::copy(h.nonOrthDeltaCoeffs.begin(), h.nonOrthDeltaCoeffs.end(), nonOrthDeltaCoeffs);
+    if (corrVecs) std::copy(h.corrVecs.begin(), h.corrVecs.end(), corrVecs);
+    if (cyclicCell)
+        for (int b = 0; b < nFaces - nInternalFaces; ++b)
+            cyclicCell[b] = h.cycOf[b] < 0 ? -1 : h.cycCell[h.cycOf[b]];
+    return ICSF_OK;
+}
+
 int icsf_mesh_create(int nPoints, const double* points, int nFaces, const int* faceOffsets, const int* faceVerts,
                      int nInternalFaces, const int* owner, const int* neighbour, int nPatches,
-                     const int* patchStart, const int* patchSize, const int* patchKind, icsf_mesh** out) {
+                     const int* patchStart, const int* patchSize, const int* patchKind, const int* patchNeighbour,
+                     const double* patchTransform, icsf_mesh** out) {
     if (!out) return fail(ICSF_ERR_ARG, "icsf_mesh_create: out is NULL");
     *out = nullptr;
     RC(checkDevice());
     std::unique_ptr<icsf_mesh> M(new icsf_mesh);
     std::string err = buildHostMesh(M->h, nPoints, points, nFaces, faceOffsets, faceVerts, nInternalFaces, owner,
-                                    neighbour, nPatches, patchStart, patchSize, patchKind);
+                                    neighbour, nPatches, patchStart, patchSize, patchKind, patchNeighbour,
+                                    patchTransform);
     if (!err.empty()) return fail(ICSF_ERR_ARG, err);
     HostMesh& h = M->h;
     buildSell(h.nCells, h.nInternal, h.owner.data(), h.neighbour.data(), M->sell);
@@ -1043,6 +1124,19 @@ int icsf_mesh_create(int nPoints, const double* points, int nFaces, const int* f
     v.corr = corr; v.C = C; v.V = V; v.cfOff = cfOff; v.cf = cf; v.bKind = bKind;
     v.nSlices = M->sell.nSlices; v.slicePtr = slicePtr; v.col = col; v.rowLen = rowLen; v.nLower = nLower;
     v.emptyDir = h.emptyDir;
+    v.nCyc = (int)h.cycFace.size();
+    v.cycBase = (size_t)M->sell.slicePtr.back() * SLICE * NB;
+    {
+        int *cycOf, *cycCell, *cycRowOff, *cycRowK;
+        double *cycR, *cycDelta;
+        auto up = [&](auto*& d, const auto& hv) {
+            return hv.empty() ? B.alloc(d, 1) : B.upload(d, hv.data(), hv.size());
+        };
+        RC(up(cycOf, h.cycOf)); RC(up(cycCell, h.cycCell)); RC(up(cycR, h.cycR));
+        RC(up(cycDelta, h.cycDelta)); RC(up(cycRowOff, h.cycRowOff)); RC(up(cycRowK, h.cycRowK));
+        v.cycOf = cycOf; v.cycCell = cycCell; v.cycR = cycR; v.cycDelta = cycDelta;
+        v.cycRowOff = cycRowOff; v.cycRowK = cycRowK;
+    }
     *out = M.release();
     return ICSF_OK;
 }
@@ -1075,6 +1169,8 @@ int createCase(icsf_mesh* mesh, const char* thermo, const char* schemes, const c
     std::unique_ptr<icsf_case> C(new icsf_case);
     C->mesh = mesh;
     RC(readSettings(C.get(), thermo, schemes, solution));
+    if (mesh->view.nCyc > 0 && C->ph.viscous)
+        return fail(ICSF_ERR_ARG, "viscous terms on meshes with cyclic patches are not supported");
     if (shared) {
         C->st = shared;
         C->ownStream = false;
@@ -1120,8 +1216,10 @@ int createCase(icsf_mesh* mesh, const char* thermo, const char* schemes, const c
     const size_t eb = C->blkFp32 ? 4 : 8;
     {
         char* p;
-        RC(B.alloc(p, slots * NB * eb)); C->blk = p;
-        HIPCHK(hipMemset(C->blk, 0, slots * NB * eb));
+        // sliced-ELL blocks, then the interface blocks of the cyclic faces
+        const size_t nBlk = slots * NB + (size_t)mesh->view.nCyc * NB;
+        RC(B.alloc(p, nBlk * eb)); C->blk = p;
+        HIPCHK(hipMemset(C->blk, 0, nBlk * eb));
         RC(B.alloc(p, dgN * eb)); C->dg = p;
         HIPCHK(hipMemset(C->dg, 0, dgN * eb));
         if (C->precond == 1) { RC(B.alloc(p, dgN * eb)); C->dinv = p; }
@@ -1183,6 +1281,9 @@ int icsf_set_boundary(icsf_case* c, int patch, int pType, int UType, int TType, 
     if (!c) return fail(ICSF_ERR_ARG, "NULL case");
     const HostMesh& h = c->mesh->h;
     if (patch < 0 || patch >= (int)h.patchStart.size()) return fail(ICSF_ERR_ARG, "patch index out of range");
+    if (h.patchKind[patch] == PATCH_CYCLIC)
+        return fail(ICSF_ERR_ARG, "patch " + std::to_string(patch) + " is cyclic: its values come from the "
+                                  "coupled cells");
     auto ok = [](int t, bool vec) { return t == BC_ZG || t == BC_FIXED || (vec && t == BC_SLIP); };
     if (!ok(pType, false) || !ok(UType, true) || !ok(TType, false))
         return fail(ICSF_ERR_KEYWORD, "unsupported boundary-condition type");
@@ -1335,6 +1436,28 @@ int icsf_get_jacobian(icsf_case* c, double* diag, double* upper, double* lower) 
             if (upper) upper[(size_t)f * NB + e] = get(hb, sellOff(M->sell.slotUpper[f], e));
             if (lower) lower[(size_t)f * NB + e] = get(hb, sellOff(M->sell.slotLower[f], e));
         }
+    return ICSF_OK;
+}
+
+int icsf_get_interface_blocks(icsf_case* c, double* blocks) {
+    if (!c || !blocks) return fail(ICSF_ERR_ARG, "icsf_get_interface_blocks: NULL argument");
+    if (!c->assembled) return fail(ICSF_ERR_STATE, "Jacobian not assembled");
+    const HostMesh& h = c->mesh->h;
+    const int nB = h.nFaces - h.nInternal, nCyc = c->mv.nCyc;
+    std::fill(blocks, blocks + (size_t)nB * NB, 0.0);
+    if (nCyc == 0) return ICSF_OK;
+    const size_t eb = c->blkFp32 ? 4 : 8;
+    std::vector<char> hv((size_t)nCyc * NB * eb);
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(hv.data(), (const char*)c->blk + c->mv.cycBase * eb, hv.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nCyc; ++k) {
+        const int b = h.cycFace[k] - h.nInternal;
+        for (int e = 0; e < NB; ++e) {
+            const size_t i = (size_t)k * NB + e;
+            blocks[(size_t)b * NB + e] = c->blkFp32 ? (double)((const float*)hv.data())[i]
+                                                    : ((const double*)hv.data())[i];
+        }
+    }
     return ICSF_OK;
 }
 
@@ -1523,6 +1646,7 @@ int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, con
                                       "double precision");
     H->nDirs = c0->nDirs; H->maxIter = c0->maxIter; H->minIter = c0->minIter; H->orth = c0->orth;
     H->fuse = c0->fuse;
+    H->useGraph = c0->useGraph;
     H->tol = c0->tol; H->relTol = c0->relTol;
     const size_t nC = mesh->h.nCells, vs = 5 * nC, vA = vs * nInstants;
     DevBuf& B = H->buf;
@@ -1655,7 +1779,8 @@ int icsf_hb_pseudo_steps(icsf_hb* h, int nSteps, double* residuals) {
     RC(hbReady(h));
     if (nSteps < 0) return fail(ICSF_ERR_ARG, "icsf_hb_pseudo_steps: nSteps < 0");
     for (int i = 0; i < nSteps; ++i) {
-        RC(hbPseudoStep(h));
+        if (hbGraphable(h)) RC(hbGraphStep(h));
+        else RC(hbPseudoStep(h));
         if (residuals) {
             std::vector<double> res;
             hbReadResiduals(h, res);

@@ -12,7 +12,7 @@ namespace icsf {
 enum { F_RHO = 0, F_P = 1, F_UX = 2, F_UY = 3, F_UZ = 4, F_E = 5, F_H = 6, F_CS = 7, NFLD = 8 };
 enum { FLUX_AUSM = 0, FLUX_ROE = 1, FLUX_HLLC = 2 };
 enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2 };
-enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2 };
+enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2, PK_CYCLIC = 3 };
 constexpr int NB = 25;  // entries of a 5x5 block
 
 // per-face terms written by the face kernel, AoS [nFaces][NFT]
@@ -48,6 +48,17 @@ struct MeshView {
     const int* rowLen;    // [nC]
     const int* nLower;    // [nC]
     int emptyDir;
+    // cyclic faces k = 0..nCyc-1 (face order); their 5x5 interface blocks
+    // follow the sliced-ELL blocks in the same allocation, at
+    // cycBase + k*25 + e
+    int nCyc;
+    size_t cycBase;
+    const int* cycOf;        // [nB] k, or -1
+    const int* cycCell;      // [nCyc] cell on the far side
+    const double* cycR;      // [nCyc][9] forwardT (far side -> this side)
+    const double* cycDelta;  // [nCyc][3] cyclicFvPatch::delta()
+    const int* cycRowOff;    // [nC+1] cell -> its cyclic faces
+    const int* cycRowK;      // [nCyc]
 };
 
 struct Phys {

@@ -14,10 +14,18 @@ from .cases import Case, PolyMesh, permute_case
 
 
 def rcm_order(mesh: PolyMesh) -> np.ndarray:
-    """Reverse Cuthill-McKee order (new -> old) of the mesh cells."""
+    """Reverse Cuthill-McKee order (new -> old) of the mesh cells; on meshes
+    with cyclic patches the cells across cyclic faces count as neighbours
+    (renumberMesh's cellCells)."""
     out = np.empty(mesh.n_cells, np.int32)
-    check(lib().icsf_rcm_order(mesh.n_cells, mesh.n_internal, ptr(as_i32(mesh.owner)),
-                               ptr(as_i32(mesh.neighbour)), ptr(out)))
+    if mesh.has_cyclic:
+        co, cn = mesh.coupled_cells()
+        check(lib().icsf_rcm_order_coupled(mesh.n_cells, mesh.n_internal, ptr(as_i32(mesh.owner)),
+                                           ptr(as_i32(mesh.neighbour)), int(co.shape[0]), ptr(co), ptr(cn),
+                                           ptr(out)))
+    else:
+        check(lib().icsf_rcm_order(mesh.n_cells, mesh.n_internal, ptr(as_i32(mesh.owner)),
+                                   ptr(as_i32(mesh.neighbour)), ptr(out)))
     return out
 
 
@@ -74,6 +82,24 @@ def mesh_geometry(mesh: PolyMesh) -> dict:
                                    ptr(as_i32(mesh.neighbour)), ptr(out["C"]), ptr(out["V"]), ptr(out["Sf"]),
                                    ptr(out["Cf"]), ptr(out["w"]), ptr(out["dc"]), ptr(out["nodc"]),
                                    ptr(out["corr"])))
+    return out
+
+
+def cyclic_geometry(mesh: PolyMesh) -> dict:
+    """Interpolation weights, deltaCoeffs, nonOrthDeltaCoeffs, correction
+    vectors of every face (cyclic faces with their coupled definitions) and
+    the cell across each boundary face (-1 off cyclic patches)."""
+    nF, nB = mesh.n_faces, mesh.n_faces - mesh.n_internal
+    start, size, kind = mesh.patch_arrays()
+    nbr, T = mesh.coupling_arrays()
+    out = {k: np.empty(s) for k, s in [("w", nF), ("dc", nF), ("nodc", nF), ("corr", (nF, 3))]}
+    out["cell"] = np.empty(nB, np.int32)
+    check(lib().icsf_mesh_geometry_patches(mesh.points.shape[0], ptr(as_f64(mesh.points)), nF,
+                                           ptr(as_i32(mesh.face_off)), ptr(as_i32(mesh.face_verts)),
+                                           mesh.n_internal, ptr(as_i32(mesh.owner)), ptr(as_i32(mesh.neighbour)),
+                                           len(mesh.patches), ptr(start), ptr(size), ptr(kind), ptr(nbr),
+                                           ptr(as_f64(T)), ptr(out["w"]), ptr(out["dc"]), ptr(out["nodc"]),
+                                           ptr(out["corr"]), ptr(out["cell"])))
     return out
 
 

@@ -29,11 +29,12 @@ class fvMesh:
     def __init__(self, mesh: PolyMesh):
         self.poly = mesh
         start, size, kind = mesh.patch_arrays()
+        nbr, T = mesh.coupling_arrays()
         h = C.c_void_p()
         check(lib().icsf_mesh_create(mesh.points.shape[0], ptr(as_f64(mesh.points)), mesh.n_faces,
                                      ptr(as_i32(mesh.face_off)), ptr(as_i32(mesh.face_verts)), mesh.n_internal,
                                      ptr(as_i32(mesh.owner)), ptr(as_i32(mesh.neighbour)), len(mesh.patches),
-                                     ptr(start), ptr(size), ptr(kind), C.byref(h)))
+                                     ptr(start), ptr(size), ptr(kind), ptr(nbr), ptr(as_f64(T)), C.byref(h)))
         self._h = h
         info = np.zeros(8, np.int32)
         check(lib().icsf_mesh_info(h, ptr(info)))
@@ -64,6 +65,8 @@ class dbnsCase:
         mesh = mesh or fvMesh(case.mesh)
         self = cls(mesh, case.thermo, case.fvSchemes, case.fvSolution)
         for i, bc in enumerate(case.bcs):
+            if bc is None:  # constraint patch (cyclic): no conditions to set
+                continue
             self.set_boundary(i, bc.p_type, bc.U_type, bc.T_type, bc.p_val, bc.U_val, bc.T_val)
         self.set_fields(case.rho, case.U, case.T)
         return self
@@ -121,6 +124,14 @@ class dbnsCase:
         d, u, l = np.empty((nC, 5, 5)), np.empty((nI, 5, 5)), np.empty((nI, 5, 5))
         check(lib().icsf_get_jacobian(self._h, ptr(d), ptr(u), ptr(l)))
         return d, u, l
+
+    def interfaces(self):
+        """interfacesUpper blocks of the boundary faces [nB, 5, 5] (non-zero
+        on cyclic faces only)."""
+        nB = self.mesh.nFaces - self.mesh.nInternalFaces
+        iu = np.empty((nB, 5, 5))
+        check(lib().icsf_get_interface_blocks(self._h, ptr(iu)))
+        return iu
 
     def set_jacobian(self, diag, upper, lower):
         check(lib().icsf_set_jacobian(self._h, ptr(as_f64(diag)), ptr(as_f64(upper)), ptr(as_f64(lower))))
@@ -204,6 +215,8 @@ class hbCase:
         self = cls(mesh, c0.thermo, c0.fvSchemes, c0.fvSolution, D)
         for J, c in enumerate(instances):
             for i, bc in enumerate(c.bcs):
+                if bc is None:  # constraint patch (cyclic)
+                    continue
                 self.set_boundary(J, i, bc.p_type, bc.U_type, bc.T_type, bc.p_val, bc.U_val, bc.T_val)
             self.set_fields(J, c.rho, c.U, c.T)
         return self

@@ -37,6 +37,7 @@ extern "C" {
 #define ICSF_PATCH 0
 #define ICSF_WALL 1
 #define ICSF_EMPTY 2
+#define ICSF_CYCLIC 3 /* cyclic (rotational or translational) pair member */
 /* boundary-condition kinds (OpenFOAM fvPatchField types) */
 #define ICSF_BC_ZERO_GRADIENT 0
 #define ICSF_BC_FIXED_VALUE 1
@@ -62,6 +63,14 @@ int icsf_version(void);
  * ICSFoam itself runs on file order (src/blockFvMatrix/blockFvMatrix.C:220-234
  * consumes the resulting lduAddressing). */
 int icsf_rcm_order(int nCells, int nInternalFaces, const int* owner, const int* neighbour, int* newToOld);
+
+/* The same on a mesh with coupled (cyclic) patches: each cell's neighbour
+ * list is its internal-face neighbours followed by the cells across its
+ * coupled faces, given as nCoupled (coupledOwner, coupledNeighbour) pairs
+ * in boundary-face order; OpenFOAM renumberMesh's CuthillMcKeeRenumber
+ * builds this list with decompositionMethod::calcCellCells. */
+int icsf_rcm_order_coupled(int nCells, int nInternalFaces, const int* owner, const int* neighbour, int nCoupled,
+                           const int* coupledOwner, const int* coupledNeighbour, int* newToOld);
 
 /* Greedy cell colouring in index order; returns the colour of each cell.
  * No counterpart in the reference (sweeps there are sequential,
@@ -110,13 +119,31 @@ int icsf_mesh_geometry(int nPoints, const double* points, int nFaces, const int*
  * Device mesh (fvMesh replacement for the path).
  * ------------------------------------------------------------------- */
 
+/* Geometry of a mesh with patches, cyclic ones included: interpolation
+ * weights, deltaCoeffs, nonOrthDeltaCoeffs and correction vectors of every
+ * face (cyclic faces: cyclicFvPatch::makeWeights and cyclicFvPatch::delta),
+ * and for every boundary face the cell across it (-1 off cyclic patches).
+ * Patch arguments as icsf_mesh_create.  Any output may be NULL. */
+int icsf_mesh_geometry_patches(int nPoints, const double* points, int nFaces, const int* faceOffsets,
+                               const int* faceVerts, int nInternalFaces, const int* owner, const int* neighbour,
+                               int nPatches, const int* patchStart, const int* patchSize, const int* patchKind,
+                               const int* patchNeighbour, const double* patchTransform, double* weights,
+                               double* deltaCoeffs, double* nonOrthDeltaCoeffs, double* corrVecs,
+                               int* cyclicCell);
+
 /* Upload a polyMesh (owner < neighbour, upper-triangular internal faces,
  * patches tiling the boundary faces in order).  Computes the geometry,
  * the face-ordered cell->face pull lists, the sliced-ELL block layout and
- * the LU-SGS sweep levels. */
+ * the LU-SGS sweep levels.  Cyclic patches (patchKind ICSF_CYCLIC) name
+ * their paired patch in patchNeighbour[p] and carry forwardT, the 3x3
+ * row-major tensor mapping the paired side's vectors into this side's
+ * frame, in patchTransform[9p..9p+8] (OpenFOAM cyclicPolyPatch /
+ * cyclicTransform); face i of a cyclic patch is coupled to face i of its
+ * pair.  Both arrays may be NULL when there is no cyclic patch. */
 int icsf_mesh_create(int nPoints, const double* points, int nFaces, const int* faceOffsets, const int* faceVerts,
                      int nInternalFaces, const int* owner, const int* neighbour, int nPatches,
-                     const int* patchStart, const int* patchSize, const int* patchKind, icsf_mesh** out);
+                     const int* patchStart, const int* patchSize, const int* patchKind, const int* patchNeighbour,
+                     const double* patchTransform, icsf_mesh** out);
 int icsf_mesh_destroy(icsf_mesh* mesh);
 /* info[0..7] = nCells, nFaces, nInternalFaces, nFwdLevels, nBwdLevels,
  * nSlices, sellSlots, emptyDirection (-1 if 3-D) */
@@ -190,6 +217,12 @@ int icsf_assemble_jacobian(icsf_case* c);
 /* LDU view of the assembled matrix: diag [nCells][25], upper/lower
  * [nInternalFaces][25] (blockFvMatrix diag/upper/lower).  Any may be NULL. */
 int icsf_get_jacobian(icsf_case* c, double* diag, double* upper, double* lower);
+/* Interface blocks of the boundary faces [nFaces - nInternalFaces][25]
+ * (blockFvMatrix interfacesUpper, blockFvMatrix.C:241-265 and :297-323):
+ * the coupling block of each cyclic face, applied to the far cell's
+ * vector with its momentum part transformed by forwardT; zero on the
+ * other boundary faces. */
+int icsf_get_interface_blocks(icsf_case* c, double* blocks);
 /* Replace the assembled matrix by given LDU blocks (same layout). */
 int icsf_set_jacobian(icsf_case* c, const double* diag, const double* upper, const double* lower);
 

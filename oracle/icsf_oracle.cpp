@@ -117,7 +117,7 @@ inline T9 dev2(const T9& A) {
     return r;
 }
 
-enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2 };
+enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2, PK_CYCLIC = 3 };
 enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2 };
 enum { FLUX_AUSM = 0, FLUX_ROE = 1, FLUX_HLLC = 2 };
 
@@ -136,7 +136,22 @@ struct Mesh {
         return false;
     }
     std::vector<int> facePatch;  // boundary face -> patch
+    // cyclic patches: neighbour patch and forwardT (maps vectors of the
+    // neighbour side into this side's frame), per patch
+    std::vector<int> pNbr;
+    vec pT;
+    // cyclic faces: per boundary face b, cycCell[b] = cell on the far side
+    // (owner of the paired face), -1 if not cyclic; cycR[9b], cycDelta[3b]
+    std::vector<int> cycCell;
+    vec cycR, cycDelta;
+    bool cyclicFace(int f) const { return f >= nI && cycCell[f - nI] >= 0; }
+    bool anyCyclic = false;
 };
+
+inline V3 transformV(const double* R, V3 v) {
+    return V3{R[0] * v.x + R[1] * v.y + R[2] * v.z, R[3] * v.x + R[4] * v.y + R[5] * v.z,
+              R[6] * v.x + R[7] * v.y + R[8] * v.z};
+}
 
 // primitiveMesh::makeFaceCentresAndAreas / makeCellCentresAndVols and
 // surfaceInterpolation::makeWeights / makeDeltaCoeffs /
@@ -251,6 +266,40 @@ void makeGeometry(Mesh& m) {
             for (int d = 0; d < 3; ++d) acc[d] += std::fabs(m.Sf[3 * f + d]);
         }
     if (any) m.emptyDir = (int)(std::max_element(acc, acc + 3) - acc);
+    // cyclic faces: face i of a cyclic patch couples to face i of its
+    // neighbour patch.  cyclicFvPatch::makeWeights: w = dn/(d + dn) with
+    // d = nf & (Cf - C) on each side; cyclicFvPatch::delta:
+    // (Cf - C) - forwardT & (Cf' - C'); deltaCoeffs, nonOrthDeltaCoeffs and
+    // the correction vectors from that delta as for internal faces.
+    const int nB = m.nF - m.nI;
+    m.cycCell.assign(nB, -1);
+    m.cycR.assign(9 * (size_t)nB, 0.0);
+    m.cycDelta.assign(3 * (size_t)nB, 0.0);
+    m.anyCyclic = false;
+    for (size_t p = 0; p < m.pStart.size(); ++p) {
+        if (m.pKind[p] != PK_CYCLIC) continue;
+        m.anyCyclic = true;
+        const int q = m.pNbr[p];
+        const double* R = &m.pT[9 * p];
+        for (int i = 0; i < m.pSize[p]; ++i) {
+            const int f = m.pStart[p] + i, g = m.pStart[q] + i;
+            const int o = m.own[f], n = m.own[g];
+            V3 nf = v3(&m.Sf[3 * f]) / m.magSf[f], ng = v3(&m.Sf[3 * g]) / m.magSf[g];
+            V3 d = v3(&m.Cf[3 * f]) - v3(&m.C[3 * o]);
+            V3 dn = v3(&m.Cf[3 * g]) - v3(&m.C[3 * n]);
+            const double di = dot(nf, d), dni = dot(ng, dn);
+            m.w[f] = dni / (di + dni);
+            V3 delta = d - transformV(R, dn);
+            m.dc[f] = 1.0 / mag(delta);
+            m.nodc[f] = 1.0 / std::max(dot(nf, delta), 0.05 * mag(delta));
+            V3 k = nf - delta * m.nodc[f];
+            m.corr[3 * f] = k.x; m.corr[3 * f + 1] = k.y; m.corr[3 * f + 2] = k.z;
+            const int b = f - m.nI;
+            m.cycCell[b] = n;
+            for (int e = 0; e < 9; ++e) m.cycR[9 * b + e] = R[e];
+            m.cycDelta[3 * b] = delta.x; m.cycDelta[3 * b + 1] = delta.y; m.cycDelta[3 * b + 2] = delta.z;
+        }
+    }
 }
 
 // ---------------------------------------------------------------- case ----
@@ -267,15 +316,18 @@ struct BC {
     vec pv, uv, tv;
 };
 
-// Sub-block of a blockFvMatrix: diag[nC], upper/lower[nI], each ncomp wide.
+// Sub-block of a blockFvMatrix: diag[nC], upper/lower[nI], each ncomp
+// wide; iu[nB] the interfacesUpper blocks of the boundary faces (used on
+// cyclic faces only; empty when no cyclic face contributed).
 struct LduBlock {
     int ncomp = 1;
-    vec diag, upper, lower;
+    vec diag, upper, lower, iu;
     void init(int nC, int nI, int nc) {
         ncomp = nc;
         diag.assign((size_t)nC * nc, 0);
         upper.assign((size_t)nI * nc, 0);
         lower.assign((size_t)nI * nc, 0);
+        iu.clear();
     }
 };
 
@@ -310,7 +362,7 @@ void evalBoundary(Case& k) {
     k.pB.assign(nB, 0); k.UB.assign(3 * nB, 0); k.TB.assign(nB, 0);
     k.rhoB.assign(nB, 0); k.eB.assign(nB, 0); k.rhoUB.assign(3 * nB, 0); k.rhoEB.assign(nB, 0);
     for (size_t pi = 0; pi < m.pStart.size(); ++pi) {
-        if (m.pKind[pi] == PK_EMPTY) continue;
+        if (m.pKind[pi] == PK_EMPTY || m.pKind[pi] == PK_CYCLIC) continue;
         const BC& bc = k.bcs[pi];
         for (int i = 0; i < m.pSize[pi]; ++i) {
             const int f = m.pStart[pi] + i, b = f - m.nI, c = m.own[f];
@@ -340,7 +392,12 @@ void evalBoundary(Case& k) {
 // ------------------------------------------------------ reconstruction ----
 // Gauss linear gradient (gaussGrad::gradf) of a cell field with boundary
 // face values vb (indexed by boundary face).
-void gaussGrad(const Mesh& m, const double* vc, const double* vb, int stride, std::vector<V3>& g) {
+// On cyclic faces the face value is the coupled linear interpolation
+// w*v_P + (1 - w)*vn (surfaceInterpolationScheme::interpolate), vn being
+// the far-side value as cyclicFvPatchField::patchNeighbourField gives it
+// (vn[b], per boundary face).
+void gaussGrad(const Mesh& m, const double* vc, const double* vb, int stride, std::vector<V3>& g,
+               const double* vn = nullptr) {
     g.assign(m.nC, V3{0, 0, 0});
     for (int f = 0; f < m.nI; ++f) {
         const int o = m.own[f], n = m.nei[f];
@@ -352,7 +409,11 @@ void gaussGrad(const Mesh& m, const double* vc, const double* vb, int stride, st
     for (int f = m.nI; f < m.nF; ++f) {
         const int pi = m.facePatch[f - m.nI];
         if (m.pKind[pi] == PK_EMPTY) continue;
-        g[m.own[f]] = g[m.own[f]] + v3(&m.Sf[3 * f]) * vb[(f - m.nI) * stride];
+        const int o = m.own[f], b = f - m.nI;
+        double vf;
+        if (m.cyclicFace(f)) vf = m.w[f] * vc[o * stride] + (1.0 - m.w[f]) * vn[b];
+        else vf = vb[b * stride];
+        g[o] = g[o] + v3(&m.Sf[3 * f]) * vf;
     }
     for (int c = 0; c < m.nC; ++c) g[c] = g[c] / m.V[c];
 }
@@ -368,9 +429,10 @@ inline void limitFace(double& lim, double maxD, double minD, double ext, int lim
 }
 
 // cellLimitedGrad<limiter> Gauss linear 1 (k = 1).
-void limitedGrad(const Case& k, const double* vc, const double* vb, int stride, std::vector<V3>& g) {
+void limitedGrad(const Case& k, const double* vc, const double* vb, int stride, std::vector<V3>& g,
+                 const double* vn = nullptr) {
     const Mesh& m = *k.m;
-    gaussGrad(m, vc, vb, stride, g);
+    gaussGrad(m, vc, vb, stride, g, vn);
     if (k.s.limiter == 0) return;
     vec mx(m.nC), mn(m.nC);
     for (int c = 0; c < m.nC; ++c) mx[c] = mn[c] = vc[c * stride];
@@ -382,7 +444,7 @@ void limitedGrad(const Case& k, const double* vc, const double* vb, int stride, 
     for (int f = m.nI; f < m.nF; ++f) {
         if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
         const int o = m.own[f];
-        const double b = vb[(f - m.nI) * stride];
+        const double b = m.cyclicFace(f) ? vn[f - m.nI] : vb[(f - m.nI) * stride];
         mx[o] = std::max(mx[o], b); mn[o] = std::min(mn[o], b);
     }
     for (int c = 0; c < m.nC; ++c) { mx[c] -= vc[c * stride]; mn[c] -= vc[c * stride]; }
@@ -403,11 +465,16 @@ void limitedGrad(const Case& k, const double* vc, const double* vb, int stride, 
 
 // fvc::interpolate(vf, pos_/neg_, "reconstruct(X)") with linearUpwind (or
 // upwind) weights: returns L (owner side) and R (neighbour side) values on
-// every face; boundary faces carry the patch value on both sides.
-void reconstruct(const Case& k, const double* vc, const double* vb, int stride, vec& L, vec& R) {
+// every face; boundary faces carry the patch value on both sides.  Cyclic
+// faces (linearUpwind::correction, coupled branch): L = v_P + (Cf - C_P) &
+// grad_P; R = vn + ((Cf - delta) - C_P) & (forwardT & grad_N), the
+// far-side gradient transformed as a vector (gradient fields are not
+// U-named, so cyclicFvPatchField transforms them).
+void reconstruct(const Case& k, const double* vc, const double* vb, int stride, vec& L, vec& R,
+                 const double* vn = nullptr) {
     const Mesh& m = *k.m;
     std::vector<V3> g;
-    if (k.s.recon == 1) limitedGrad(k, vc, vb, stride, g);
+    if (k.s.recon == 1) limitedGrad(k, vc, vb, stride, g, vn);
     L.assign(m.nF, 0);
     R.assign(m.nF, 0);
     for (int f = 0; f < m.nI; ++f) {
@@ -424,8 +491,56 @@ void reconstruct(const Case& k, const double* vc, const double* vb, int stride, 
     }
     for (int f = m.nI; f < m.nF; ++f) {
         if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
-        L[f] = R[f] = vb[(f - m.nI) * stride];
+        const int b = f - m.nI;
+        if (m.cyclicFace(f)) {
+            const int o = m.own[f], n = m.cycCell[b];
+            double l = vc[o * stride], r = vn[b];
+            if (k.s.recon == 1) {
+                V3 fc = v3(&m.Cf[3 * f]);
+                l += dot(fc - v3(&m.C[3 * o]), g[o]);
+                V3 gR = transformV(&m.cycR[9 * b], g[n]);
+                r += dot((fc - v3(&m.cycDelta[3 * b])) - v3(&m.C[3 * o]), gR);
+            }
+            L[f] = l;
+            R[f] = r;
+        } else {
+            L[f] = R[f] = vb[b * stride];
+        }
     }
+}
+
+// Far-side values on cyclic faces (cyclicFvPatchField::patchNeighbourField
+// as patched in originalOFFiles/constraintFvPatchFields/cyclic/
+// cyclicFvPatchField.C:127-188): a scalar field's value at the far cell;
+// for the components of U (fields "U.component(i)") component i of
+// forwardT & U at the far cell.  comp = -1: scalar field vc.
+vec cyclicNbr(const Case& k, const double* vc, int stride, int comp) {
+    const Mesh& m = *k.m;
+    const int nB = m.nF - m.nI;
+    vec vn(nB, 0.0);
+    if (!m.anyCyclic) return vn;
+    for (int b = 0; b < nB; ++b) {
+        const int n = m.cycCell[b];
+        if (n < 0) continue;
+        if (comp < 0) vn[b] = vc[n * stride];
+        else {
+            V3 u = transformV(&m.cycR[9 * b], v3(&k.U[3 * n]));
+            vn[b] = (&u.x)[comp];
+        }
+    }
+    return vn;
+}
+
+// Coupled linear interpolation of a cell scalar on cyclic face f.
+inline double interpCyclic(const Mesh& m, int f, double vP, double vN) {
+    return m.w[f] * vP + (1.0 - m.w[f]) * vN;
+}
+// ... and of U: w*U_P + (1 - w)*(forwardT & U_N)
+inline V3 interpCyclicU(const Case& k, int f) {
+    const Mesh& m = *k.m;
+    const int b = f - m.nI;
+    V3 UN = transformV(&m.cycR[9 * b], v3(&k.U[3 * m.cycCell[b]]));
+    return m.w[f] * v3(&k.U[3 * m.own[f]]) + (1.0 - m.w[f]) * UN;
 }
 
 inline double interpLinear(const Mesh& m, int f, const double* vc, int stride) {
@@ -568,14 +683,13 @@ RoeDiss roeDissipation(double rho, V3 Vel, double c, double gI, double u, V3 n, 
 }
 
 // roeFluxScheme::calcFlux (roeFluxScheme.C:276-409), one face.
-void roeFace(const Case& k, V3 S, double magS, double rl, double rr, double pl, double pr, V3 Ul, V3 Ur,
+void roeFace(const Case& k, double gI, V3 S, double magS, double rl, double rr, double pl, double pr, V3 Ul, V3 Ur,
              double El, double Er, double Hl, double Hr, double& phi, V3& phiU, double& phiE) {
     const V3 n = S / magS;
     const double coefR = std::sqrt(std::max(VSMALL, rr) / std::max(VSMALL, rl));
     const double rhoRoe = coefR * rl;
     const V3 URoe = (coefR * Ur + Ul) / (coefR + 1.0);
     const double HRoe = (coefR * Hr + Hl) / (coefR + 1.0);
-    const double gI = k.s.gamma;
     const double cRoe = std::sqrt(std::fabs((gI - 1.0) * (HRoe - 0.5 * magSqr(URoe))));
     const double uMl = dot(Ul, n), uMr = dot(Ur, n);
     const double uProj = dot(URoe, n);
@@ -596,14 +710,15 @@ void roeFace(const Case& k, V3 S, double magS, double rl, double rr, double pl, 
 
 // hllcFluxScheme::calcFlux (hllcFluxScheme.C:70-240), one face; wave
 // speeds from Roe averages, switches pos(x) = x > 0, neg(x) = x < 0.
-void hllcFace(const Case& k, V3 S, double magS, double rl, double rr, double pl, double pr, V3 Ul, V3 Ur,
-              double cl, double cr, double El, double Er, double Hl, double Hr, double& phi, V3& phiU,
+void hllcFace(const Case& k, double gI, V3 S, double magS, double rl, double rr, double pl, double pr, V3 Ul,
+              V3 Ur, double cl, double cr, double El, double Er, double Hl, double Hr, double& phi, V3& phiU,
               double& phiE) {
     const V3 n = S / magS;
     const double coefR = std::sqrt(std::max(VSMALL, rr) / std::max(VSMALL, rl));
     const V3 uAvg = (coefR * Ur + Ul) / (coefR + 1.0);
     const double HAvg = (coefR * Hr + Hl) / (coefR + 1.0);
-    const double cAvg = std::sqrt(std::fabs((k.s.gamma - 1.0) * (HAvg - 0.5 * magSqr(uAvg))));
+    (void)k;
+    const double cAvg = std::sqrt(std::fabs((gI - 1.0) * (HAvg - 0.5 * magSqr(uAvg))));
     const double ul = dot(Ul, n), ur = dot(Ur, n), ua = dot(uAvg, n);
     const double Sl = std::min(ul - cl, ua - cAvg);
     const double Sr = std::max(ur + cr, ua + cAvg);
@@ -637,15 +752,19 @@ void calcFlux(Case& k, vec& phi, vec& phiU, vec& phiE) {
     for (int c = 0; c < m.nC; ++c) { Ux[c] = k.U[3 * c]; Uy[c] = k.U[3 * c + 1]; Uz[c] = k.U[3 * c + 2]; }
     for (int b = 0; b < nB; ++b) { Ubx[b] = k.UB[3 * b]; Uby[b] = k.UB[3 * b + 1]; Ubz[b] = k.UB[3 * b + 2]; }
     vec UxL, UxR, UyL, UyR, UzL, UzR, pL, pR, rL, rR, HL, HR, cL, cR, EL, ER;
-    reconstruct(k, Ux.data(), Ubx.data(), 1, UxL, UxR);
-    reconstruct(k, Uy.data(), Uby.data(), 1, UyL, UyR);
-    reconstruct(k, Uz.data(), Ubz.data(), 1, UzL, UzR);
-    reconstruct(k, k.p.data(), k.pB.data(), 1, pL, pR);
-    reconstruct(k, k.rho.data(), k.rhoB.data(), 1, rL, rR);
-    reconstruct(k, d.H.data(), d.Hb.data(), 1, HL, HR);
-    if (k.s.flux == FLUX_AUSM) reconstruct(k, d.cstar.data(), d.cstarb.data(), 1, cL, cR);
-    else reconstruct(k, d.E.data(), d.Eb.data(), 1, EL, ER);
-    if (k.s.flux == FLUX_HLLC) reconstruct(k, d.chl.data(), d.chlb.data(), 1, cL, cR);
+    auto rec = [&](const double* vc, const double* vb, int comp, vec& L, vec& R) {
+        const vec vn = cyclicNbr(k, vc, 1, comp);
+        reconstruct(k, vc, vb, 1, L, R, vn.data());
+    };
+    rec(Ux.data(), Ubx.data(), 0, UxL, UxR);
+    rec(Uy.data(), Uby.data(), 1, UyL, UyR);
+    rec(Uz.data(), Ubz.data(), 2, UzL, UzR);
+    rec(k.p.data(), k.pB.data(), -1, pL, pR);
+    rec(k.rho.data(), k.rhoB.data(), -1, rL, rR);
+    rec(d.H.data(), d.Hb.data(), -1, HL, HR);
+    if (k.s.flux == FLUX_AUSM) rec(d.cstar.data(), d.cstarb.data(), -1, cL, cR);
+    else rec(d.E.data(), d.Eb.data(), -1, EL, ER);
+    if (k.s.flux == FLUX_HLLC) rec(d.chl.data(), d.chlb.data(), -1, cL, cR);
     phi.assign(m.nF, 0);
     phiU.assign(3 * m.nF, 0);
     phiE.assign(m.nF, 0);
@@ -655,13 +774,17 @@ void calcFlux(Case& k, vec& phi, vec& phiU, vec& phiE) {
         V3 UL{UxL[f], UyL[f], UzL[f]}, UR{UxR[f], UyR[f], UzR[f]};
         double ph, pe;
         V3 pu;
+        // fvc::interpolate(gamma): exact on internal faces (w (g - g) + g),
+        // w g + (1 - w) g on cyclic faces
+        const double gI = m.cyclicFace(f) ? interpCyclic(m, f, k.s.gamma, k.s.gamma) : k.s.gamma;
         if (k.s.flux == FLUX_AUSM)
             ausmFace(k, S, m.magSf[f], UL, UR, cL[f], cR[f], pL[f], pR[f], rL[f], rR[f], HL[f], HR[f], ph, pu, pe);
         else if (k.s.flux == FLUX_ROE)
-            roeFace(k, S, m.magSf[f], rL[f], rR[f], pL[f], pR[f], UL, UR, EL[f], ER[f], HL[f], HR[f], ph, pu, pe);
+            roeFace(k, gI, S, m.magSf[f], rL[f], rR[f], pL[f], pR[f], UL, UR, EL[f], ER[f], HL[f], HR[f], ph, pu,
+                    pe);
         else
-            hllcFace(k, S, m.magSf[f], rL[f], rR[f], pL[f], pR[f], UL, UR, cL[f], cR[f], EL[f], ER[f], HL[f], HR[f],
-                     ph, pu, pe);
+            hllcFace(k, gI, S, m.magSf[f], rL[f], rR[f], pL[f], pR[f], UL, UR, cL[f], cR[f], EL[f], ER[f], HL[f],
+                     HR[f], ph, pu, pe);
         phi[f] = ph;
         phiU[3 * f] = pu.x; phiU[3 * f + 1] = pu.y; phiU[3 * f + 2] = pu.z;
         phiE[f] = pe;
@@ -825,6 +948,18 @@ void localDeltaT(Case& k) {
         k.rDT[m.nei[f]] = std::max(k.rDT[m.nei[f]], frdt);
     }
     for (size_t pi = 0; pi < m.pStart.size(); ++pi) {
+        if (m.pKind[pi] == PK_CYCLIC) {
+            // coupled patches: nonOrthDeltaCoeffs * lambda of the coupled
+            // interpolation (setCoAndDeltaT.H:78-89)
+            for (int f = m.pStart[pi]; f < m.pStart[pi] + m.pSize[pi]; ++f) {
+                const int b = f - m.nI, c = m.own[f];
+                const double cf = interpCyclic(m, f, d.c[c], d.c[m.cycCell[b]]);
+                const V3 Uf = interpCyclicU(k, f);
+                const double lam = cf + std::fabs(dot(Uf, v3(&m.Sf[3 * f]) / m.magSf[f]));
+                k.rDT[c] = std::max(k.rDT[c], m.nodc[f] * lam);
+            }
+            continue;
+        }
         if (m.pKind[pi] != PK_WALL) continue;
         for (int f = m.pStart[pi]; f < m.pStart[pi] + m.pSize[pi]; ++f) {
             const int c = m.own[f];
@@ -846,6 +981,10 @@ void addBlockTo(LduBlock& A, const LduBlock& B, double sign) {
     for (size_t i = 0; i < A.diag.size(); ++i) A.diag[i] += sign * B.diag[i];
     for (size_t i = 0; i < A.upper.size(); ++i) A.upper[i] += sign * B.upper[i];
     for (size_t i = 0; i < A.lower.size(); ++i) A.lower[i] += sign * B.lower[i];
+    if (!B.iu.empty()) {
+        if (A.iu.empty()) A.iu.assign(B.iu.size(), 0.0);
+        for (size_t i = 0; i < A.iu.size(); ++i) A.iu[i] += sign * B.iu[i];
+    }
 }
 void negSumDiag(const Mesh& m, LduBlock& B) {
     const int nc = B.ncomp;
@@ -871,10 +1010,14 @@ void assemble(Case& k) {
     for (int c = 0; c < nC; ++c) { Ux[c] = k.U[3 * c]; Uy[c] = k.U[3 * c + 1]; Uz[c] = k.U[3 * c + 2]; }
     for (int b = 0; b < nB; ++b) { Ubx[b] = k.UB[3 * b]; Uby[b] = k.UB[3 * b + 1]; Ubz[b] = k.UB[3 * b + 2]; }
     vec UxL, UxR, UyL, UyR, UzL, UzR, EL, ER;
-    reconstruct(k, Ux.data(), Ubx.data(), 1, UxL, UxR);
-    reconstruct(k, Uy.data(), Uby.data(), 1, UyL, UyR);
-    reconstruct(k, Uz.data(), Ubz.data(), 1, UzL, UzR);
-    reconstruct(k, d.E.data(), d.Eb.data(), 1, EL, ER);
+    {
+        vec n0 = cyclicNbr(k, Ux.data(), 1, 0), n1 = cyclicNbr(k, Uy.data(), 1, 1), n2 = cyclicNbr(k, Uz.data(), 1, 2);
+        vec nE = cyclicNbr(k, d.E.data(), 1, -1);
+        reconstruct(k, Ux.data(), Ubx.data(), 1, UxL, UxR, n0.data());
+        reconstruct(k, Uy.data(), Uby.data(), 1, UyL, UyR, n1.data());
+        reconstruct(k, Uz.data(), Ubz.data(), 1, UzL, UzR, n2.data());
+        reconstruct(k, d.E.data(), d.Eb.data(), 1, EL, ER, nE.data());
+    }
     // insertBlock for each sub-block: temporaries with negSumDiag then +=
     LduBlock tS, tV, tT;
     auto insert = [&](LduBlock& target, int nc, auto leftFn, auto rightFn) {
@@ -891,6 +1034,22 @@ void assemble(Case& k) {
             }
         }
         negSumDiag(m, mx);
+        // coupled patches: interfacesUpper = 1/2|S| right, interfacesLower =
+        // -1/2|S| left, diag[own] -= interfacesLower (blockFvMatrix.C:241-265)
+        if (m.anyCyclic) {
+            mx.iu.assign((size_t)nB * nc, 0.0);
+            for (int f = nI; f < m.nF; ++f) {
+                if (!m.cyclicFace(f)) continue;
+                leftFn(f, lv);
+                rightFn(f, rv);
+                const double ms = m.magSf[f];
+                const int b = f - nI, o = m.own[f];
+                for (int q = 0; q < nc; ++q) {
+                    mx.iu[b * nc + q] = 0.5 * ms * rv[q];
+                    mx.diag[o * nc + q] -= -0.5 * ms * lv[q];
+                }
+            }
+        }
         addBlockTo(target, mx, 1.0);
     };
     auto state = [&](int f, bool left, V3& U, double& E) {
@@ -898,8 +1057,10 @@ void assemble(Case& k) {
         else { U = V3{UxR[f], UyR[f], UzR[f]}; E = ER[f]; }
     };
     auto nrm = [&](int f) { return v3(&m.Sf[3 * f]) / m.magSf[f]; };
-    const double gI = g;  // linear interpolation of a uniform gamma
-    auto theta = [&](V3 U) { return 0.5 * (gI - 1) * magSqr(U); };
+    // linear interpolation of a uniform gamma: exact on internal faces,
+    // w g + (1 - w) g on cyclic faces
+    auto gam = [&](int f) { return m.cyclicFace(f) ? interpCyclic(m, f, g, g) : g; };
+    auto theta = [&](int f, V3 U) { return 0.5 * (gam(f) - 1) * magSqr(U); };
     // dContByRhoU: n, n
     insert(k.SV[0], 3, [&](int f, double* o) { V3 n = nrm(f); o[0] = n.x; o[1] = n.y; o[2] = n.z; },
            [&](int f, double* o) { V3 n = nrm(f); o[0] = n.x; o[1] = n.y; o[2] = n.z; });
@@ -907,7 +1068,7 @@ void assemble(Case& k) {
         return [&, left](int f, double* o) {
             V3 U; double E; state(f, left, U, E);
             V3 n = nrm(f);
-            V3 r = (n * theta(U)) - (U * dot(U, n));
+            V3 r = (n * theta(f, U)) - (U * dot(U, n));
             o[0] = r.x; o[1] = r.y; o[2] = r.z;
         };
     };
@@ -916,7 +1077,7 @@ void assemble(Case& k) {
         return [&, left](int f, double* o) {
             V3 U; double E; state(f, left, U, E);
             V3 n = nrm(f);
-            const double a2 = gI - 1;
+            const double a2 = gam(f) - 1;
             T9 r = outer(U, n) - outer(a2 * n, U);
             const double pu = dot(U, n);
             r.t[0] += pu; r.t[4] += pu; r.t[8] += pu;
@@ -927,7 +1088,7 @@ void assemble(Case& k) {
     auto momByRhoE = [&](bool left) {
         return [&, left](int f, double* o) {
             V3 n = nrm(f);
-            V3 r = n * (gI - 1);
+            V3 r = n * (gam(f) - 1);
             (void)left;
             o[0] = r.x; o[1] = r.y; o[2] = r.z;
         };
@@ -937,8 +1098,8 @@ void assemble(Case& k) {
         return [&, left](int f, double* o) {
             V3 U; double E; state(f, left, U, E);
             V3 n = nrm(f);
-            const double th = theta(U);
-            const double a1 = gI * E - th;
+            const double th = theta(f, U);
+            const double a1 = gam(f) * E - th;
             o[0] = dot(U, n) * (th - a1);
         };
     };
@@ -947,9 +1108,9 @@ void assemble(Case& k) {
         return [&, left](int f, double* o) {
             V3 U; double E; state(f, left, U, E);
             V3 n = nrm(f);
-            const double th = theta(U);
-            const double a1 = gI * E - th;
-            const double a2 = gI - 1;
+            const double th = theta(f, U);
+            const double a1 = gam(f) * E - th;
+            const double a2 = gam(f) - 1;
             V3 r = (n * a1) - ((a2 * U) * dot(U, n));
             o[0] = r.x; o[1] = r.y; o[2] = r.z;
         };
@@ -958,7 +1119,7 @@ void assemble(Case& k) {
     auto enByRhoE = [&](bool left) {
         return [&, left](int f, double* o) {
             V3 U; double E; state(f, left, U, E);
-            o[0] = gI * dot(U, nrm(f));
+            o[0] = gam(f) * dot(U, nrm(f));
         };
     };
     insert(k.SS[1][1], 1, enByRhoE(true), enByRhoE(false));
@@ -971,7 +1132,12 @@ void assemble(Case& k) {
     }
     for (int f = nI; f < m.nF; ++f) {
         if (m.pKind[m.facePatch[f - nI]] == PK_EMPTY) continue;
-        lamB[f - nI] = d.cb[f - nI] + std::fabs(dot(v3(&k.UB[3 * (f - nI)]), nrm(f)));
+        if (m.cyclicFace(f)) {
+            const double cf = interpCyclic(m, f, d.c[m.own[f]], d.c[m.cycCell[f - nI]]);
+            lamB[f - nI] = cf + std::fabs(dot(interpCyclicU(k, f), nrm(f)));
+        } else {
+            lamB[f - nI] = d.cb[f - nI] + std::fabs(dot(v3(&k.UB[3 * (f - nI)]), nrm(f)));
+        }
     }
     auto dissip = [&](LduBlock& target, int nc) {
         LduBlock mx;
@@ -986,21 +1152,27 @@ void assemble(Case& k) {
             }
         }
         negSumDiag(m, mx);
+        if (m.anyCyclic) mx.iu.assign((size_t)nB * nc, 0.0);
         for (int f = nI; f < m.nF; ++f) {
             if (m.pKind[m.facePatch[f - nI]] == PK_EMPTY) continue;
             const double v = 0.5 * m.magSf[f] * lamB[f - nI];
             const int o = m.own[f];
             if (nc == 1) mx.diag[o] -= v;
             else { mx.diag[9 * o] -= v; mx.diag[9 * o + 4] -= v; mx.diag[9 * o + 8] -= v; }
+            if (m.cyclicFace(f)) {
+                const int b = f - nI;
+                if (nc == 1) mx.iu[b] = v;
+                else { mx.iu[9 * b] = v; mx.iu[9 * b + 4] = v; mx.iu[9 * b + 8] = v; }
+            }
         }
         addBlockTo(target, mx, -1.0);
     };
     dissip(k.SS[0][0], 1);
     dissip(k.VV, 9);
     dissip(k.SS[1][1], 1);
-    // addBoundaryTerms (convectiveFluxScheme.C:219-355)
+    // addBoundaryTerms (convectiveFluxScheme.C:219-355), non-coupled patches
     for (size_t pi = 0; pi < m.pStart.size(); ++pi) {
-        if (m.pKind[pi] == PK_EMPTY) continue;
+        if (m.pKind[pi] == PK_EMPTY || m.pKind[pi] == PK_CYCLIC) continue;
         const BC& bc = k.bcs[pi];
         for (int i = 0; i < m.pSize[pi]; ++i) {
             const int f = m.pStart[pi] + i, b = f - nI, c = m.own[f];
@@ -1138,6 +1310,9 @@ void fromC(const CVec& c, int nC, double* x) {
 }
 
 // blockFvMatrix::Amul for (sourceType, blockType) pairs.
+// Cyclic faces add interfacesUpper & patchNeighbourField(x) to the owner
+// row after the internal faces (blockFvMatrix.C:398-420); vector columns
+// are transformed by forwardT, scalar columns are not.
 void amulSS(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {
     y.assign(m.nC, 0);
     for (int c = 0; c < m.nC; ++c) y[c] = A.diag[c] * x[c];
@@ -1145,6 +1320,9 @@ void amulSS(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {
         y[m.nei[f]] += A.lower[f] * x[m.own[f]];
         y[m.own[f]] += A.upper[f] * x[m.nei[f]];
     }
+    if (!A.iu.empty())
+        for (int f = m.nI; f < m.nF; ++f)
+            if (m.cyclicFace(f)) y[m.own[f]] += A.iu[f - m.nI] * x[m.cycCell[f - m.nI]];
 }
 void amulSV(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {  // scalar row, vector col
     y.assign(m.nC, 0);
@@ -1153,6 +1331,12 @@ void amulSV(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {  // scalar
         y[m.nei[f]] += dot(v3(&A.lower[3 * f]), v3(&x[3 * m.own[f]]));
         y[m.own[f]] += dot(v3(&A.upper[3 * f]), v3(&x[3 * m.nei[f]]));
     }
+    if (!A.iu.empty())
+        for (int f = m.nI; f < m.nF; ++f) {
+            if (!m.cyclicFace(f)) continue;
+            const int b = f - m.nI;
+            y[m.own[f]] += dot(v3(&A.iu[3 * b]), transformV(&m.cycR[9 * b], v3(&x[3 * m.cycCell[b]])));
+        }
 }
 void amulVS(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {  // vector row, scalar col
     y.assign(3 * (size_t)m.nC, 0);
@@ -1162,6 +1346,12 @@ void amulVS(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {  // vector
         for (int d = 0; d < 3; ++d) {
             y[3 * m.nei[f] + d] += A.lower[3 * f + d] * x[m.own[f]];
             y[3 * m.own[f] + d] += A.upper[3 * f + d] * x[m.nei[f]];
+        }
+    if (!A.iu.empty())
+        for (int f = m.nI; f < m.nF; ++f) {
+            if (!m.cyclicFace(f)) continue;
+            const int b = f - m.nI;
+            for (int d = 0; d < 3; ++d) y[3 * m.own[f] + d] += A.iu[3 * b + d] * x[m.cycCell[b]];
         }
 }
 void amulVV(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {
@@ -1177,6 +1367,13 @@ void amulVV(const Mesh& m, const LduBlock& A, const vec& x, vec& y) {
         for (int d = 0; d < 3; ++d) y[3 * m.nei[f] + d] += (&a.x)[d];
         for (int d = 0; d < 3; ++d) y[3 * m.own[f] + d] += (&b.x)[d];
     }
+    if (!A.iu.empty())
+        for (int f = m.nI; f < m.nF; ++f) {
+            if (!m.cyclicFace(f)) continue;
+            const int bf = f - m.nI;
+            V3 r = tdotv(T(A.iu, bf), transformV(&m.cycR[9 * bf], v3(&x[3 * m.cycCell[bf]])));
+            for (int d = 0; d < 3; ++d) y[3 * m.own[f] + d] += (&r.x)[d];
+        }
 }
 
 // coupledMatrix::matrixMul (coupledMatrix.C:66-123)
@@ -1599,6 +1796,13 @@ int pseudoStep(Case& k, Res& out) {
             if (bad(o)) { fac[o] = std::min(0.5, fac[o]); fac[n] = std::min(0.75, fac[n]); }
             if (bad(n)) { fac[n] = std::min(0.5, fac[n]); fac[o] = std::min(0.75, fac[o]); }
         }
+        // coupled patches: the far cell through patchNeighbourField
+        for (int f = m.nI; f < m.nF; ++f) {
+            if (!m.cyclicFace(f)) continue;
+            const int o = m.own[f], n = m.cycCell[f - m.nI];
+            if (bad(o)) fac[o] = std::min(0.5, fac[o]);
+            if (bad(n)) fac[o] = std::min(0.75, fac[o]);
+        }
         for (int c = 0; c < nC; ++c) k.pseudoCo[c] *= fac[c];
     }
     updateFields(k, dW);
@@ -1945,6 +2149,12 @@ int hbPseudoStep(HB& h, HBRes& out) {
                 if (bad(o)) { fac[o] = std::min(0.5, fac[o]); fac[n] = std::min(0.75, fac[n]); }
                 if (bad(n)) { fac[n] = std::min(0.5, fac[n]); fac[o] = std::min(0.75, fac[o]); }
             }
+            for (int f = m.nI; f < m.nF; ++f) {
+                if (!m.cyclicFace(f)) continue;
+                const int o = m.own[f], n = m.cycCell[f - m.nI];
+                if (bad(o)) fac[o] = std::min(0.5, fac[o]);
+                if (bad(n)) fac[o] = std::min(0.75, fac[o]);
+            }
             for (int c = 0; c < nC; ++c) k.pseudoCo[c] *= fac[c];
         }
         updateFields(k, dW[J]);
@@ -1995,7 +2205,7 @@ extern "C" {
 
 void* orc_mesh_new(int nPoints, const double* pts, int nFaces, const int* fOff, const int* fV, int nInternal,
                    const int* own, const int* nei, int nPatches, const int* pStart, const int* pSize,
-                   const int* pKind) {
+                   const int* pKind, const int* pNbr, const double* pT) {
     Mesh* m = new Mesh;
     m->nP = nPoints; m->nF = nFaces; m->nI = nInternal;
     m->pts.assign(pts, pts + 3 * (size_t)nPoints);
@@ -2006,6 +2216,13 @@ void* orc_mesh_new(int nPoints, const double* pts, int nFaces, const int* fOff, 
     m->pStart.assign(pStart, pStart + nPatches);
     m->pSize.assign(pSize, pSize + nPatches);
     m->pKind.assign(pKind, pKind + nPatches);
+    m->pNbr.assign(nPatches, -1);
+    m->pT.assign(9 * (size_t)nPatches, 0.0);
+    for (int p = 0; p < nPatches; ++p)
+        if (pKind[p] == PK_CYCLIC) {
+            m->pNbr[p] = pNbr[p];
+            for (int e = 0; e < 9; ++e) m->pT[9 * (size_t)p + e] = pT[9 * (size_t)p + e];
+        }
     int nc = 0;
     for (int f = 0; f < nFaces; ++f) nc = std::max(nc, own[f] + 1);
     for (int f = 0; f < nInternal; ++f) nc = std::max(nc, nei[f] + 1);
@@ -2031,10 +2248,14 @@ void orc_mesh_geometry(void* h, double* C, double* V, double* Sf, double* Cf, do
 
 // meshTools::bandCompression (Cuthill-McKee from the lowest unvisited cell,
 // unvisited neighbours appended by increasing connectivity, stable); the
-// neighbour lists follow primitiveMesh::calcCellCells (internal-face order).
-static std::vector<int> cmOrder(int nC, int nI, const int* own, const int* nei) {
+// neighbour lists follow primitiveMesh::calcCellCells (internal-face order),
+// then (decompositionMethod::calcCellCells, used by renumberMesh) the
+// cells across coupled faces in boundary-face order.
+static std::vector<int> cmOrder(int nC, int nI, const int* own, const int* nei, int nCp = 0,
+                                const int* cOwn = nullptr, const int* cNbr = nullptr) {
     std::vector<std::vector<int>> cc(nC);
     for (int f = 0; f < nI; ++f) { cc[own[f]].push_back(nei[f]); cc[nei[f]].push_back(own[f]); }
+    for (int b = 0; b < nCp; ++b) cc[cOwn[b]].push_back(cNbr[b]);
     std::vector<char> vis(nC, 0);
     std::vector<int> order;
     std::deque<int> q;
@@ -2061,6 +2282,20 @@ static std::vector<int> cmOrder(int nC, int nI, const int* own, const int* nei) 
 void orc_rcm(int nC, int nI, const int* own, const int* nei, int* newToOld) {
     std::vector<int> order = cmOrder(nC, nI, own, nei);
     for (int i = 0; i < nC; ++i) newToOld[i] = order[nC - 1 - i];
+}
+void orc_rcm_coupled(int nC, int nI, const int* own, const int* nei, int nCp, const int* cOwn, const int* cNbr,
+                     int* newToOld) {
+    std::vector<int> order = cmOrder(nC, nI, own, nei, nCp, cOwn, cNbr);
+    for (int i = 0; i < nC; ++i) newToOld[i] = order[nC - 1 - i];
+}
+// Cyclic-face geometry: w, dc, nodc per face, cell across each boundary
+// face (-1 if not cyclic).
+void orc_mesh_cyclic(void* h, double* w, double* dc, double* nodc, int* cycCell) {
+    Mesh& m = *(Mesh*)h;
+    std::copy(m.w.begin(), m.w.end(), w);
+    std::copy(m.dc.begin(), m.dc.end(), dc);
+    std::copy(m.nodc.begin(), m.nodc.end(), nodc);
+    std::copy(m.cycCell.begin(), m.cycCell.end(), cycCell);
 }
 
 // Greedy colouring, cells visited in Cuthill-McKee order.
@@ -2211,6 +2446,28 @@ void orc_assemble(void* h, double* diag, double* upper, double* lower) {
                  k.SV[1].lower, k.VS[0].lower, k.VS[1].lower, k.VV.lower);
 }
 
+// Interface blocks of the boundary faces ([nB][25], zero off cyclic faces),
+// same layout as orc_assemble; call after orc_assemble.
+void orc_get_interfaces(void* h, double* iu) {
+    Case& k = *(Case*)h;
+    const Mesh& m = *k.m;
+    const int nB = m.nF - m.nI;
+    std::fill(iu, iu + 25 * (size_t)nB, 0.0);
+    auto get = [&](const LduBlock& A, int b, int q) { return A.iu.empty() ? 0.0 : A.iu[(size_t)b * A.ncomp + q]; };
+    for (int b = 0; b < nB; ++b) {
+        double* B = iu + 25 * (size_t)b;
+        B[0] = get(k.SS[0][0], b, 0); B[1] = get(k.SS[0][1], b, 0);
+        B[5] = get(k.SS[1][0], b, 0); B[6] = get(k.SS[1][1], b, 0);
+        for (int d = 0; d < 3; ++d) {
+            B[2 + d] = get(k.SV[0], b, d);
+            B[7 + d] = get(k.SV[1], b, d);
+            B[5 * (2 + d)] = get(k.VS[0], b, d);
+            B[5 * (2 + d) + 1] = get(k.VS[1], b, d);
+            for (int e = 0; e < 3; ++e) B[5 * (2 + d) + 2 + e] = get(k.VV, b, 3 * d + e);
+        }
+    }
+}
+
 // Replace the coupled matrix by given 5x5 blocks (same layout as above).
 void orc_set_matrix(void* h, const double* diag, const double* upper, const double* lower) {
     Case& k = *(Case*)h;
@@ -2329,12 +2586,12 @@ void orc_face_flux(int scheme, int lowMach, double eFix, double R, double gamma,
     double ph, pe;
     V3 pu;
     if (scheme == FLUX_AUSM) ausmFace(k, S, magS, U[0], U[1], cs[0], cs[1], p[0], p[1], rho[0], rho[1], H[0], H[1], ph, pu, pe);
-    else if (scheme == FLUX_ROE) roeFace(k, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], E[0], E[1], H[0], H[1], ph, pu, pe);
+    else if (scheme == FLUX_ROE) roeFace(k, gamma, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], E[0], E[1], H[0], H[1], ph, pu, pe);
     else {
         double ch[2];
         // sqrt(gamma/psi) with psi = 1/(R T)
         for (int q = 0; q < 2; ++q) ch[q] = std::max(std::sqrt(gamma / (1.0 / (R * T[q]))), VSMALL);
-        hllcFace(k, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], ch[0], ch[1], E[0], E[1], H[0], H[1], ph, pu, pe);
+        hllcFace(k, gamma, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], ch[0], ch[1], E[0], E[1], H[0], H[1], ph, pu, pe);
     }
     out[0] = ph; out[1] = pu.x; out[2] = pu.y; out[3] = pu.z; out[4] = pe;
 }

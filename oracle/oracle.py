@@ -17,7 +17,10 @@ LIB = os.path.join(HERE, "liboracle.so")
 
 _vp, _i = C.c_void_p, C.c_int
 _SIG = {
-    "orc_mesh_new": (_vp, [_i, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp]),
+    "orc_mesh_new": (_vp, [_i, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "orc_mesh_cyclic": (None, [_vp, _vp, _vp, _vp, _vp]),
+    "orc_rcm_coupled": (None, [_i, _i, _vp, _vp, _i, _vp, _vp, _vp]),
+    "orc_get_interfaces": (None, [_vp, _vp]),
     "orc_mesh_free": (None, [_vp]),
     "orc_mesh_ncells": (_i, [_vp]),
     "orc_mesh_geometry": (None, [_vp] + [_vp] * 8),
@@ -126,12 +129,13 @@ def parse_settings(thermo: str, schemes: str, solution: str):
 class OracleMesh:
     def __init__(self, mesh):
         start, size, kind = mesh.patch_arrays()
+        nbr, T = mesh.coupling_arrays()
         self._keep = [_f64(mesh.points), _i32(mesh.face_off), _i32(mesh.face_verts), _i32(mesh.owner),
-                      _i32(mesh.neighbour), start, size, kind]
+                      _i32(mesh.neighbour), start, size, kind, _i32(nbr), _f64(T)]
         k = self._keep
         self.h = lib().orc_mesh_new(mesh.points.shape[0], _p(k[0]), mesh.n_faces, _p(k[1]), _p(k[2]),
                                     mesh.n_internal, _p(k[3]), _p(k[4]), len(mesh.patches), _p(start), _p(size),
-                                    _p(kind))
+                                    _p(kind), _p(k[8]), _p(k[9]))
         self.poly = mesh
         self.nCells = lib().orc_mesh_ncells(self.h)
 
@@ -141,6 +145,12 @@ class OracleMesh:
                                         ("w", nF), ("dc", nF), ("nodc", nF), ("corr", (nF, 3))]}
         lib().orc_mesh_geometry(self.h, *[_p(o[k]) for k in ["C", "V", "Sf", "Cf", "w", "dc", "nodc", "corr"]])
         return o
+
+    def cyclic(self):
+        nF, nB = self.poly.n_faces, self.poly.n_faces - self.poly.n_internal
+        w, dc, nodc, cell = np.empty(nF), np.empty(nF), np.empty(nF), np.empty(nB, np.int32)
+        lib().orc_mesh_cyclic(self.h, _p(w), _p(dc), _p(nodc), _p(cell))
+        return dict(w=w, dc=dc, nodc=nodc, cell=cell)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -158,7 +168,12 @@ def face_flux(scheme, S, left, right, lowMach=True, eFix=0.05, R=287.0, gamma=1.
 
 def rcm(mesh):
     out = np.empty(mesh.n_cells, np.int32)
-    lib().orc_rcm(mesh.n_cells, mesh.n_internal, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)), _p(out))
+    if mesh.has_cyclic:
+        co, cn = mesh.coupled_cells()
+        lib().orc_rcm_coupled(mesh.n_cells, mesh.n_internal, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)),
+                              int(co.shape[0]), _p(_i32(co)), _p(_i32(cn)), _p(out))
+    else:
+        lib().orc_rcm(mesh.n_cells, mesh.n_internal, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)), _p(out))
     return out
 
 
@@ -189,6 +204,8 @@ class OracleCase:
         self.nInternal = case.mesh.n_internal
         self.nFaces = case.mesh.n_faces
         for i, bc in enumerate(case.bcs):
+            if bc is None:  # constraint patch (cyclic)
+                continue
             pv = _f64(bc.p_val) if bc.p_val is not None else None
             uv = _f64(bc.U_val) if bc.U_val is not None else None
             tv = _f64(bc.T_val) if bc.T_val is not None else None
@@ -228,6 +245,13 @@ class OracleCase:
         d, u, l = np.empty((nC, 5, 5)), np.empty((nI, 5, 5)), np.empty((nI, 5, 5))
         lib().orc_assemble(self.h, _p(d), _p(u), _p(l))
         return d, u, l
+
+    def interfaces(self):
+        """Interface (cyclic-face) blocks [nB, 5, 5] after createJacobian."""
+        nB = self.nFaces - self.nInternal
+        iu = np.empty((nB, 5, 5))
+        lib().orc_get_interfaces(self.h, _p(iu))
+        return iu
 
     def set_matrix(self, d, u, l):
         lib().orc_set_matrix(self.h, _p(_f64(d)), _p(_f64(u)), _p(_f64(l)))

@@ -56,7 +56,8 @@ def test_no_device_is_reported_cleanly():
     lib = _lib.lib()
     rc = lib.icsf_mesh_create(m.points.shape[0], _lib.ptr(m.points), m.n_faces, _lib.ptr(m.face_off),
                               _lib.ptr(m.face_verts), m.n_internal, _lib.ptr(m.owner), _lib.ptr(m.neighbour),
-                              len(m.patches), _lib.ptr(start), _lib.ptr(size), _lib.ptr(kind), C.byref(h))
+                              len(m.patches), _lib.ptr(start), _lib.ptr(size), _lib.ptr(kind), None, None,
+                              C.byref(h))
     assert rc == _lib.ERR_NODEVICE
     assert b"no HIP device" in lib.icsf_last_error()
     assert np.all(np.asarray([0]) == 0)

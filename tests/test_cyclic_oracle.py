@@ -1,0 +1,157 @@
+"""Rotational cyclic patches: pins of the oracle's coupled-patch semantics.
+
+The reference couples the theta planes of its annular sectors with OpenFOAM
+cyclic patches (its patched cyclicFvPatchField,
+originalOFFiles/constraintFvPatchFields/cyclic/cyclicFvPatchField.C).  No
+reference output exists here, so the oracle's restatement is pinned by
+what the geometry fixes:
+
+* a sector of angle theta with cyclic patches and the double sector of
+  angle 2 theta (whose two halves are joined by ordinary internal faces)
+  carry the same rotation-periodic state; the cells of the first half must
+  see exactly what the cells of the single sector see.  With first-order
+  (upwind) reconstruction, or with a purely axial velocity (rotation about
+  z leaves it unchanged), the residual, rPseudoDeltaT, the assembled
+  diagonal blocks and the matrix product A x of a rotation-periodic x agree
+  to round-off (1e-13 relative).  A wrong transform direction, a missed
+  far-side value, wrong cyclic weights/deltas or a coupled block missing
+  from the matrix product break this by O(1e-2).
+* second-order reconstruction of a swirling state does not agree: the
+  reference reconstructs the velocity component-wise, rotating the far
+  cell's value but only rotating (not mixing) the component gradients, so
+  its cyclic faces are not equivalent to internal faces.  The test checks
+  the oracle keeps that behaviour (a difference well above round-off).
+* the library's host geometry (cyclic weights, deltaCoeffs,
+  nonOrthDeltaCoeffs, far cells) equals the oracle's bit for bit, and
+  inconsistent cyclic definitions are rejected.
+* Cuthill-McKee with the coupled cells appended (renumberMesh) agrees
+  between library and oracle, and its bandwidth is no worse than scipy's
+  reverse Cuthill-McKee on the same graph.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from icsfoam_amd import cases
+from icsfoam_amd._lib import IcsfError
+from icsfoam_amd.mesh import cyclic_geometry, rcm_order
+from oracle import oracle as orc
+from oracle.oracle import OracleCase, OracleMesh
+
+NT, NR, NZ, TH = 6, 5, 4, np.pi / 8
+FIRST_ORDER = "upwind"
+
+
+def _pair(recon="linearUpwind limitedGrad", swirl=True, flux="ROE"):
+    # warped (theta-jittered) cyclic planes keep the normal velocity off
+    # zero there: AUSM+up's M_p damping switch (ausmPlusUpFluxScheme.C:
+    # 236-256) is discontinuous at M_1/2 = 0, where round-off in the rotated
+    # copy would pick the other branch
+    kw = dict(recon=recon, swirl=swirl, flux=flux, warp_planes=True)
+    c1 = cases.case_sector(NT, NR, NZ, TH, copies=1, **kw)
+    c2 = cases.case_sector(NT, NR, NZ, TH, copies=2, **kw)
+    n1 = c1.mesh.n_cells
+    grid = np.arange(2 * n1).reshape(NZ, 2 * NT, NR)
+    return c1, c2, grid[:, :NT, :].ravel(), grid[:, NT:, :].ravel()
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("flux", ["ROE", "AUSMPlusUp", "HLLC"])
+@pytest.mark.parametrize("recon,swirl", [(FIRST_ORDER, True), ("linearUpwind limitedGrad", False)])
+def test_sector_equals_double_sector(flux, recon, swirl):
+    c1, c2, first, second = _pair(recon, swirl, flux)
+    o1, o2 = OracleCase(c1), OracleCase(c2)
+    b1, r1 = o1.residual()
+    b2, r2 = o2.residual()
+    for q in range(5):
+        assert _rel(b2[q, first], b1[q]) < 1e-12, (q, _rel(b2[q, first], b1[q]))
+    assert _rel(r2[first], r1) < 1e-13
+    # second half: scalars equal, momentum rotated by theta
+    R = cases.rotation_z(TH)
+    assert _rel(b2[:2, second], b1[:2]) < 1e-12
+    assert _rel(b2[2:, second], R @ b1[2:]) < 1e-12
+    d1, _, _ = o1.createJacobian()
+    d2, _, _ = o2.createJacobian()
+    assert _rel(d2[first], d1) < 1e-13
+    x1 = np.random.default_rng(1).standard_normal((5, c1.mesh.n_cells))
+    x2 = np.zeros((5, c2.mesh.n_cells))
+    x2[:, first] = x1
+    x2[:2, second] = x1[:2]
+    x2[2:, second] = R @ x1[2:]
+    y1, y2 = o1.matrixMul(x1), o2.matrixMul(x2)
+    for q in range(5):
+        assert _rel(y2[q, first], y1[q]) < 1e-13, q
+    iu = o1.interfaces()
+    nCyc = sum(p.size for p in c1.mesh.patches if p.kind == cases.PATCH_CYCLIC)
+    assert np.count_nonzero(np.abs(iu).sum(axis=(1, 2))) == nCyc
+
+
+def test_second_order_swirl_follows_componentwise_reconstruction():
+    c1, c2, first, _ = _pair("linearUpwind limitedGrad", True)
+    b1, _ = OracleCase(c1).residual()
+    b2, _ = OracleCase(c2).residual()
+    assert max(_rel(b2[q, first], b1[q]) for q in range(5)) > 1e-4
+
+
+def test_cyclic_geometry_library_equals_oracle():
+    m = cases.annulus_sector(NT, NR, NZ, theta=TH, jitter=0.1, seed=3, cyclic=True)
+    g = cyclic_geometry(m)
+    o = OracleMesh(m).cyclic()
+    for k in ("w", "dc", "nodc"):
+        np.testing.assert_array_equal(g[k], o[k])
+    np.testing.assert_array_equal(g["cell"], o["cell"])
+    lo = next(p for p in m.patches if p.name == "cyclicLow")
+    hi = next(p for p in m.patches if p.name == "cyclicHigh")
+    nI = m.n_internal
+    w_lo = g["w"][lo.start:lo.start + lo.size]
+    w_hi = g["w"][hi.start:hi.start + hi.size]
+    assert np.all((w_lo > 0.3) & (w_lo < 0.7))
+    np.testing.assert_allclose(w_lo + w_hi, 1.0, rtol=0, atol=1e-14)
+    # far cells are the owners of the paired faces
+    np.testing.assert_array_equal(g["cell"][lo.start - nI:lo.start - nI + lo.size],
+                                  m.owner[hi.start:hi.start + hi.size])
+    # paired face centres coincide after the transform
+    Cf = cases.face_centres_of(m, hi.start, hi.size)
+    Cl = cases.face_centres_of(m, lo.start, lo.size)
+    np.testing.assert_allclose(Cf @ np.asarray(lo.transform).T, Cl, atol=1e-12)
+
+
+def test_inconsistent_cyclic_definitions_rejected():
+    m = cases.annulus_sector(NT, NR, NZ, theta=TH, cyclic=True)
+    lo = next(i for i, p in enumerate(m.patches) if p.name == "cyclicLow")
+    bad_T = dataclasses.replace(m, patches=list(m.patches))
+    bad_T.patches[lo] = dataclasses.replace(m.patches[lo], transform=cases.rotation_z(TH))
+    with pytest.raises(IcsfError, match="inverse|match"):
+        cyclic_geometry(bad_T)
+    bad_n = dataclasses.replace(m, patches=list(m.patches))
+    bad_n.patches[lo] = dataclasses.replace(m.patches[lo], neighbour=0)
+    with pytest.raises(IcsfError, match="neighbour patch"):
+        cyclic_geometry(bad_n)
+
+
+def test_coupled_rcm_library_equals_oracle_and_bandwidth():
+    sp = pytest.importorskip("scipy.sparse")
+    csg = pytest.importorskip("scipy.sparse.csgraph")
+    m = cases.annulus_sector(NT, NR, NZ, theta=TH, jitter=0.1, seed=3, cyclic=True)
+    perm = np.random.default_rng(4).permutation(m.n_cells).astype(np.int32)
+    from icsfoam_amd.mesh import renumber_mesh
+    m = renumber_mesh(m, perm)
+    a = rcm_order(m)
+    b = orc.rcm(m)
+    np.testing.assert_array_equal(a, b)
+    co, cn = m.coupled_cells()
+    rows = np.concatenate([m.owner[:m.n_internal], m.neighbour, co])
+    cols = np.concatenate([m.neighbour, m.owner[:m.n_internal], cn])
+    A = sp.csr_matrix((np.ones(rows.shape[0]), (rows, cols)), shape=(m.n_cells,) * 2)
+
+    def bandwidth(order):
+        inv = np.empty_like(order)
+        inv[order] = np.arange(order.shape[0])
+        return int(np.abs(inv[rows] - inv[cols]).max())
+
+    ref = csg.reverse_cuthill_mckee(A, symmetric_mode=True)
+    assert bandwidth(a) <= bandwidth(np.asarray(ref))

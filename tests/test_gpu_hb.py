@@ -125,3 +125,18 @@ def test_single_instant_equals_single_case():
     rh = h.pseudo_steps(3)[:, 0, :]
     np.testing.assert_array_equal(rh, rs)
     np.testing.assert_array_equal(h.fields(0)["W"], s.fields()["W"])
+
+
+def test_hb_graph_replay_equals_eager():
+    """From the third step on the HB pseudo-step is replayed from a captured
+    HIP graph: 6 steps (4 replays, each followed by a host read of the
+    residuals) equal the eager path bit for bit."""
+    print("[hb graph]", flush=True)
+    insts, D, t = cases.case_hb(n=(8, 6, 5), n_harmonics=2, flux="ROE")
+    insts = [prepare_case(c, permute_seed=17) for c in insts]
+    eager = [dataclasses.replace(c, fvSolution=cases.solution_dict(co=20.0, graphCapture=False)) for c in insts]
+    graph = [dataclasses.replace(c, fvSolution=cases.solution_dict(co=20.0, graphCapture=True)) for c in insts]
+    ge, gg = hbCase.from_cases(eager, D), hbCase.from_cases(graph, D)
+    np.testing.assert_array_equal(gg.pseudo_steps(6), ge.pseudo_steps(6))
+    for J in range(len(insts)):
+        np.testing.assert_array_equal(gg.fields(J)["W"], ge.fields(J)["W"])
