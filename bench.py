@@ -16,6 +16,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 160]
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import platform
@@ -78,16 +79,51 @@ def algorithmic_bytes(nC, nI, p, q, m=30):
     return (2 * nI + nC) * 25 * p + 2 * nI * 4 + nC * 5 * q * 2 + m / (m + 2) * nC * 5 * q
 
 
-def algorithmic_bytes_fused(nC, nI, p, q):
+NGEN = 10  # generator values per off-diagonal block (jacobianStorage generators)
+
+
+def algorithmic_bytes_fused(nC, nI, p, q, gen=True):
     """Per-group algorithmic bytes of the fused Arnoldi operator (one group
     per Arnoldi step: SpMV of the colour-0 rows; SpMV + forward sweep +
     first backward level of the colour-1 rows; backward sweep of the
-    colour-0 rows).  Blocks: every off-diagonal block of a colour-0 row
-    twice, of a colour-1 row once, diagonals once; column indices per block
-    read; vectors: x twice, y0 once more by the colour-1 rows, y once by
-    the last pass, writes of y, the basis vector and the final colour-0
-    rows; rD twice for colour 0 (n0 = n1 = nC/2 on these meshes)."""
-    return (3 * nI + nC) * 25 * p + 3 * nI * 4 + 30 * q * nC + 12 * nC
+    colour-0 rows).  Off-diagonal blocks (10 generators each with generator
+    storage, else 25 entries): every block of a colour-0 row twice, of a
+    colour-1 row once; diagonal blocks once; column indices per block read;
+    vectors: x twice, y0 once more by the colour-1 rows, y once by the last
+    pass, writes of y, the basis vector and the final colour-0 rows; rD
+    twice for colour 0 (n0 = n1 = nC/2 on these meshes)."""
+    per = NGEN if gen else 25
+    return (3 * nI * per + nC * 25) * p + 3 * nI * 4 + 30 * q * nC + 12 * nC
+
+
+def step_bytes(nC, nI, nB, form, m=30):
+    """Algorithmic HBM bytes of one pseudo-step (one GMRES/FGMRES cycle of
+    m Arnoldi steps), summed over the kernels' per-launch figures:
+    gradients nC (8 + 24) 8 + 112 nI; face terms + cell residual
+    nF (56 + 128 + 384 + 384) + nC 104 (viscous Roe); Jacobian rows
+    2 nI (10 p + 192) + 25 p nC (generator storage); per Arnoldi step the fused operator group
+    and the CGS projection (w and j + 1 basis vectors read, then w, j + 1
+    basis vectors read and w written: 5 q nC (2 j + 5)); per cycle the
+    normalisation pass (average, one SpMV, norm factors), the solution
+    update (m + 1 vectors) and the final residual SpMV; the state update
+    5 * 8 * 4 nC.  FP64 form: p = q = 8.  Mixed form: the inner GMRES(m) with
+    p = q = 4 plus the FP64 outer FGMRES(1) (two SpMVs with FP32 blocks, a
+    two-vector update)."""
+    nF = nI + nB
+    res = nC * (8 + 24) * 8 + 112 * nI + nF * (56 + 128 + 384 + 384) + nC * 104
+    fused = lambda p, q: algorithmic_bytes_fused(nC, nI, p, q)
+    spmv = lambda p, q: (2 * nI * NGEN + nC * 25) * p + 2 * nI * 4 + nC * 5 * q * 2
+    orth = lambda q: sum(5 * q * nC * (2 * j + 5) for j in range(m))
+    if form == "fp64":
+        p = q = 8
+        jac = 2 * nI * (NGEN * p + 192) + 25 * p * nC
+        cycle = m * fused(p, q) + orth(q) + 3 * spmv(p, q) + 5 * q * nC * (m + 2)
+    else:
+        jac = 2 * nI * (NGEN * 4 + 192) + 25 * 4 * nC
+        inner = m * fused(4, 4) + orth(4) + 5 * 4 * nC * (m + 2)
+        outer = 3 * spmv(4, 8) + 5 * 8 * nC * 6
+        cycle = inner + outer
+    return res + jac + cycle + 5 * 8 * 4 * nC
 
 
 def measured_traffic(n):
@@ -137,8 +173,9 @@ def time_form(case, steps, warmup, label):
     bms, bcnt = g.timing_read()
     g.timing(0)
     fused = g.mesh.nFwdLevels == 2 and g.mesh.nBwdLevels == 2
+    tuning = g.tuning()
     del g
-    return dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, eager_steps_per_s=steps / dte,
+    return dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, eager_steps_per_s=steps / dte, tuning=tuning,
                 spmv_ms=ms[2] / max(cnt[2], 1), fused=fused,
                 spmv_count=int(cnt[2]), breakdown_ms=[round(float(x), 3) for x in bms],
                 breakdown_count=[int(x) for x in bcnt])
@@ -182,7 +219,9 @@ def cpu_baseline(n_sample: int, n_full: int, steps: int = 2):
 def bench_hb(steps, warmup):
     """Harmonic-balance pseudo-steps/s on a 64x64x32 annular sector with
     rotational cyclic theta patches and 7 instants (3 harmonics of 100 Hz),
-    Roe inviscid, GMRES(30) with CGS + LU-SGS, FP64."""
+    Roe inviscid, LU-SGS.  value: the mixed form (outer FGMRES(1) on the
+    FP64 coupled vectors, FP32 HB Jacobian and FP32 inner GMRES(30) with CGS,
+    hipGraph-captured steps); the FP64 form (GMRES(30), CGS) beside it."""
     from icsfoam_amd import cases
     from icsfoam_amd.mesh import prepare_case
     from icsfoam_amd.solver import hbCase
@@ -191,22 +230,33 @@ def bench_hb(steps, warmup):
     insts, D, t = cases.case_hb(n=(64, 64, 32), n_harmonics=3, flux="ROE", orthogonalisation="CGS")
     insts = [prepare_case(c, permute_seed=2024) for c in insts]
     log(f"[bench] HB case (7 instants, {insts[0].mesh.n_cells} cells) prepared in {time.time() - t0:.1f}s")
-    g = hbCase.from_cases(insts, D)
-    g.pseudo_steps(warmup, residuals=False)
-    g.synchronize()
-    t0 = time.perf_counter()
-    g.pseudo_steps(steps, residuals=False)
-    g.synchronize()
-    dt = time.perf_counter() - t0
-    res = g.pseudo_steps(1)[0]
-    log(f"[bench] HB: {steps} steps in {dt:.3f}s; instant-0 residuals {res[0, :5]}")
+    forms = {"fp64": cases.solution_dict(co=20.0, orthogonalisation="CGS"),
+             "mixed": cases.solution_dict(co=20.0, solver="FGMRES", nDirections=1, innerDirections=30,
+                                          precision="mixed", orthogonalisation="CGS")}
+    rate = {}
+    for form, sol in forms.items():
+        g = hbCase.from_cases([dataclasses.replace(c, fvSolution=sol) for c in insts], D)
+        g.pseudo_steps(max(warmup, 3), residuals=False)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.pseudo_steps(steps, residuals=False)
+        g.synchronize()
+        dt = time.perf_counter() - t0
+        res = g.pseudo_steps(1)[0]
+        rate[form] = steps / dt
+        log(f"[bench] HB {form}: {steps} steps in {dt:.3f}s; instant-0 residuals {res[0, :5]}")
+        del g
     nC = insts[0].mesh.n_cells
-    return {"metric": "harmonic-balance pseudo-steps/s (7 instants)", "value": round(steps / dt, 4),
-            "unit": "pseudo-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
-            "ms_per_step": round(1e3 * dt / steps, 3), "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded annular sector, periodic inflow)",
+    return {"metric": "harmonic-balance pseudo-steps/s (7 instants)", "value": round(rate["mixed"], 4),
+            "unit": "pseudo-steps/s", "n_gpus": 1, "steps": steps, "warmup": max(warmup, 3),
+            "ms_per_step": round(1e3 / rate["mixed"], 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64 residual/outer + f32 Jacobian/inner",
+            "data": "synthetic (seeded annular sector, periodic inflow)",
             "config": {"workload": f"64x64x32 annular sector with rotational cyclic patches ({nC} cells) x 7 "
-                                   f"instants ({7 * nC} block rows), Roe inviscid, GMRES(30, CGS)+LU-SGS"}}
+                                   f"instants ({7 * nC} block rows), Roe inviscid, LU-SGS, hipGraph steps",
+                       "mixed": "FGMRES(1) FP64 outer + FP32 GMRES(30, CGS) inner",
+                       "fp64": "GMRES(30, CGS)", "fp64_steps_per_s": round(rate["fp64"], 4),
+                       "mixed_over_fp64": round(rate["mixed"] / rate["fp64"], 3)}}
 
 
 def bench_vjp(calls):
@@ -244,13 +294,13 @@ def bench_vjp(calls):
                        "residual_ms": round(1e3 * dr, 3)}}
 
 
-def bench_converge(n, max_steps, chunk=25, drop=1e-8, time_limit=400.0):
+def bench_converge(n, max_steps, chunk=25, drop=1e-8, time_limit=400.0, forms=("fp64", "mixed")):
     """Time to a `drop` reduction of ||R V||_2 (all five components, the
     norm dbnsFoam.C:136-137 prints for rho) for the FP64 and the mixed form
     of the n^3 workload."""
     base = prepared(n)
     out = {}
-    for form in ("fp64", "mixed"):
+    for form in forms:
         from icsfoam_amd import dbnsCase
 
         g = dbnsCase.from_case(with_form(base, n, form, "CGS"))
@@ -272,13 +322,14 @@ def bench_converge(n, max_steps, chunk=25, drop=1e-8, time_limit=400.0):
         out[form] = dict(steps=steps, seconds=round(time.perf_counter() - t0, 3), drop=hist[-1][2],
                          history=hist[::max(1, len(hist) // 20)])
         del g
-    return {"metric": f"time to a {drop:g} drop of ||R V||_2 at {n}^3 (FP64 vs mixed)",
-            "value": out["mixed"]["seconds"], "unit": "s", "n_gpus": 1, "steps": out["mixed"]["steps"],
-            "warmup": 0, "ms_per_step": round(1e3 * out["mixed"]["seconds"] / max(out["mixed"]["steps"], 1), 3),
+    head = out[forms[-1]]
+    return {"metric": f"time to a {drop:g} drop of ||R V||_2 at {n}^3 ({' vs '.join(forms)})",
+            "value": head["seconds"], "unit": "s", "n_gpus": 1, "steps": head["steps"],
+            "warmup": 0, "ms_per_step": round(1e3 * head["seconds"] / max(head["steps"], 1), 3),
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64 residual/outer + f32 Jacobian/inner", "data": "synthetic",
-            "config": {"workload": f"{n}^3 jittered hex box, viscous Roe, local CFL 50", "fp64": out["fp64"],
-                       "mixed": out["mixed"]}}
+            "dtype": "f64 residual/outer + f32 Jacobian/inner" if forms[-1] == "mixed" else "f64",
+            "data": "synthetic",
+            "config": dict({"workload": f"{n}^3 jittered hex box, viscous Roe, local CFL 50"}, **out)}
 
 
 def main():
@@ -290,6 +341,8 @@ def main():
     ap.add_argument("--orth", default="CGS", choices=["CGS", "MGS"])
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp64,mixed", help="converge: forms to run, in order")
+    ap.add_argument("--time-limit", type=float, default=400.0, help="converge: seconds per form")
     ap.add_argument("--workload", default="c5", choices=["c5", "hb", "vjp", "converge"],
                     help="c5: the headline 160^3 pseudo-step (default); hb: 7-instant harmonic balance; "
                          "vjp: residual adjoint at 100^3")
@@ -305,11 +358,13 @@ def main():
         print(json.dumps(bench_vjp(max(args.steps, 10))), flush=True)
         return
     if args.workload == "converge":
-        print(json.dumps(bench_converge(args.n, max(args.steps, 25))), flush=True)
+        print(json.dumps(bench_converge(args.n, max(args.steps, 25), time_limit=args.time_limit,
+                                        forms=tuple(args.forms.split(",")))), flush=True)
         return
     n = args.n
     base = prepared(n)
     nC, nI = base.mesh.n_cells, base.mesh.n_internal
+    nB = base.mesh.n_faces - nI
     # at least 3 untimed steps so that the graph capture (third step) is
     # not inside the timed region
     warmup = max(args.warmup, 3)
@@ -346,7 +401,12 @@ def main():
                    "eager_steps_per_s": {"fp64": round(r64["eager_steps_per_s"], 4),
                                          "mixed": round(rmx["eager_steps_per_s"], 4)},
                    "orthogonalisation": args.orth,
-                   "breakdown_ms_fp64": r64["breakdown_ms"], "breakdown_ms_mixed": rmx["breakdown_ms"]},
+                   "breakdown_ms_fp64": r64["breakdown_ms"], "breakdown_ms_mixed": rmx["breakdown_ms"],
+                   "autotuned": {"fp64": r64["tuning"], "mixed": rmx["tuning"]},
+                   "step_hbm_frac": {f: round(step_bytes(nC, nI, nB, f) * r["steps_per_s"] / HBM_PEAK, 4)
+                                     for f, r in (("fp64", r64), ("mixed", rmx))},
+                   "step_algorithmic_gb": {f: round(step_bytes(nC, nI, nB, f) / 1e9, 2)
+                                           for f in ("fp64", "mixed")}},
         "roofline": {"bound": "hbm",
                      "kernel": ("fused Arnoldi operator, FP64 form: k_spmv_cells<double,double,false> + "
                                 "k_spmv_cells<double,double,true> + k_lusgs_bwd<double,double> (one group per "

@@ -50,6 +50,7 @@ SIGNATURES = {
     "icsf_mesh_geometry_patches": (_i, [_i, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp,
                                         _vp, _vp, _vp, _vp, _vp]),
     "icsf_get_interface_blocks": (_i, [_vp, _vp]),
+    "icsf_get_tuning": (_i, [_vp, _vp]),
     "icsf_colour_cells": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
     "icsf_colour_major_order": (_i, [_i, _vp, _i, _vp]),
     "icsf_renumber_mesh": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

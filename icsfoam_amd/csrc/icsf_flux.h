@@ -182,5 +182,46 @@ DF void fluxJacobian(double gI, V3 U, double E, V3 n, double A[25]) {
     A[20] = mr.z; A[21] = me.z; A[22] = mu.t[6]; A[23] = mu.t[7]; A[24] = mu.t[8];
 }
 
+// The same in precision C from scalar arguments, operation for operation
+// (C = double gives fluxJacobian's bits).
+template <typename C>
+DF void fluxJacobianT(C gI, C ux, C uy, C uz, C E, C nx, C ny, C nz, C A[25]) {
+    const C th = (C)0.5 * (gI - (C)1) * (ux * ux + uy * uy + uz * uz);
+    const C a1 = gI * E - th;
+    const C a2 = gI - (C)1;
+    const C pu = ux * nx + uy * ny + uz * nz;
+    A[0] = (C)0; A[1] = (C)0; A[2] = nx; A[3] = ny; A[4] = nz;
+    A[5] = pu * (th - a1);
+    A[6] = gI * pu;
+    A[7] = nx * a1 - (a2 * ux) * pu;
+    A[8] = ny * a1 - (a2 * uy) * pu;
+    A[9] = nz * a1 - (a2 * uz) * pu;
+    A[10] = nx * th - ux * pu; A[11] = nx * a2;
+    A[15] = ny * th - uy * pu; A[16] = ny * a2;
+    A[20] = nz * th - uz * pu; A[21] = nz * a2;
+    const C an0 = a2 * nx, an1 = a2 * ny, an2 = a2 * nz;
+    A[12] = ux * nx - an0 * ux; A[13] = ux * ny - an0 * uy; A[14] = ux * nz - an0 * uz;
+    A[17] = uy * nx - an1 * ux; A[18] = uy * ny - an1 * uy; A[19] = uy * nz - an1 * uz;
+    A[22] = uz * nx - an2 * ux; A[23] = uz * ny - an2 * uy; A[24] = uz * nz - an2 * uz;
+    A[12] += pu; A[18] += pu; A[24] += pu;
+}
+
+// Entries of one off-diagonal block from its generators (genOff layout):
+// h A(U, E, n), diagonal entries then minus v and minus sv, as the
+// Jacobian kernel forms the stored block.
+template <typename C>
+DF void blockFromGen(const C g[10], C gamma, C b[25]) {
+    fluxJacobianT<C>(gamma, g[0], g[1], g[2], g[3], g[4], g[5], g[6], b);
+#pragma unroll
+    for (int e = 0; e < 25; ++e) {
+        C x = g[7] * b[e];
+        if (e == 0 || e == 6 || e == 12 || e == 18 || e == 24) {
+            x = x - g[8];
+            x = x - g[9];
+        }
+        b[e] = x;
+    }
+}
+
 #undef DF
 }  // namespace icsf

@@ -69,21 +69,22 @@ struct GmresArgs {
 struct HBInst {
     MeshView m;
     StateView s;
-    double* blk;
-    double* dg;
+    void* blk;    // FP64, or FP32 with mixed precision
+    void* dg;
     double* rD;
     int* err;
 };
 void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st);
-void launchJacobianHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st);
+void launchJacobianHB(const HBInst* inst, int nO, int fp32, const MeshView& m, const Phys& ph, hipStream_t st);
 
 // Harmonic-balance coupling: per-instance block storage (FP64) and the
 // spectral operator D [nO][nO] (device).
 constexpr int HB_MAXO = 16;
 struct HBBlocks {
     int nO;
-    const double* blk[HB_MAXO];
-    const double* dg[HB_MAXO];
+    int fp32;                     // blocks stored in FP32 (mixed precision)
+    const void* blk[HB_MAXO];
+    const void* dg[HB_MAXO];
     const double* D;
 };
 
@@ -105,9 +106,17 @@ struct FgmresArgs {
     void* Z;  // nDirs FP64 vectors
     std::function<void(const double*, double*)> precond;
     std::function<void(int, int)> mark;
+    // batched (harmonic-balance) systems: 5*nVec entries per vector and a
+    // caller-supplied y = A z (FP64 vectors); nVec 0 / empty: the mesh SpMV
+    int nVec = 0;
+    std::function<void(const double*, double*)> op;
 };
 
 int reduceBlocks(int nC);
+// Full 5x5 off-diagonal blocks (sliced layout) from generator storage.
+void launchExpandBlocks(const MeshView& m, int fp32, const void* gen, void* out, hipStream_t st);
+void launchProjectionPasses(const MeshView& m, int vecFp32, int ch, const void* w, const void* V, int nv,
+                            double* part, hipStream_t st);
 
 void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const double* dW, int mode,
                   const double* rho0, const double* U0, const double* T0, hipStream_t st);
@@ -151,18 +160,18 @@ void launchCombine(int nC, int vecFp32, const void* V, int nv, const GmresScalar
 
 // Harmonic balance (icsf_krylov.hip): y = A_HB (x/scal) over [nO][5][nC]
 // vectors in coupledMatrix order, with vout = x/scal when given.
-void launchSpmvHB(const MeshView& m, const HBBlocks& hb, const double* x, const double* scal, double* vout,
-                  double* y, hipStream_t st);
+void launchSpmvHB(const MeshView& m, const HBBlocks& hb, int vecFp32, const void* x, const double* scal, void* vout,
+                  void* y, hipStream_t st);
 // Per-variable bookkeeping of one instance slice ([5][nC]) into g: W minus
 // its per-component average; norm factors and initial residuals from
 // t = A (W - avg) and b; final residuals from r = b - A dW (r holds A dW on
 // entry).
 // y = LU-SGS(A_HB (x/scal)) in three passes on a two-colour level
 // structure (false: not applicable, nothing launched).
-bool launchSpmvLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, const double* x,
-                       const double* scal, double* vout, double* y, hipStream_t st);
+bool launchSpmvLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, int vecFp32, const double* rD,
+                       const void* x, const double* scal, void* vout, void* y, hipStream_t st);
 // LU-SGS of all HB instances (per-instance sweeps, shared diagonal rD).
-void launchLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, double* r,
+void launchLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, int vecFp32, const double* rD, void* r,
                    hipStream_t st);
 void sliceSubAverage(const MeshView& m, Work& wk, GmresScalars* g, const double* W, double* out, hipStream_t st);
 void sliceNormFactors(const MeshView& m, Work& wk, GmresScalars* g, const double* t, const double* b,

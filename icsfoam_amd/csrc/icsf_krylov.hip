@@ -10,6 +10,7 @@
 // Off-diagonal 5x5 blocks live in a sliced-ELL layout (64-row slices, one
 // wavefront per slice, entry-major inside a slot) so every block load of a
 // wavefront is one contiguous 64-element run.
+#include "icsf_flux.h"
 #include "icsf_kernels.h"
 #include "icsf_math.h"
 #include "icsf_views.h"
@@ -34,6 +35,28 @@ __device__ __forceinline__ double ld(const T* p, size_t i) { return (double)p[i]
 // (three partial sums per row as well); the row result is then
 // rD_c (y_c - fwd), i.e. the forward and the first backward level of the
 // LU-SGS sweep of a two-colour level structure.
+// Off-diagonal block of sliced-ELL slot sk for the row in lane `lane`, in
+// arithmetic type A: read from full storage, or generated from the slot's
+// NGEN generators in the storage precision (the same bits as the block the
+// Jacobian kernel would have stored, for FP64).
+template <typename TB, typename A, bool GEN>
+__device__ __forceinline__ void slotBlock(const MeshView& m, const TB* __restrict__ blk, size_t sk, int lane,
+                                          A b[NB]) {
+    if (GEN) {
+        const size_t gb = sk * NGEN * 64 + lane;
+        TB g[NGEN], t[NB];
+#pragma unroll
+        for (int e = 0; e < NGEN; ++e) g[e] = blk[gb + (size_t)e * 64];
+        blockFromGen<TB>(g, (TB)m.gamma, t);
+#pragma unroll
+        for (int e = 0; e < NB; ++e) b[e] = (A)t[e];
+    } else {
+        const size_t bb = sk * NB * 64 + lane;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) b[e] = (A)blk[bb + (size_t)e * 64];
+    }
+}
+
 // Cyclic faces of row c: interfacesUpper & (x_far with its momentum part
 // transformed by forwardT), added to the three partial sums after the
 // internal faces (blockFvMatrix::Amul, blockFvMatrix.C:398-420).  Not part
@@ -63,7 +86,7 @@ __device__ __forceinline__ void spmvCyclic(const MeshView& m, const TB* __restri
     }
 }
 
-template <typename TB, typename TV, bool FWD>
+template <typename TB, typename TV, bool FWD, bool GEN>
 __device__ __forceinline__ void spmvRow(const MeshView& m, const TB* __restrict__ blk, const TB* __restrict__ dg,
                                         const TV* __restrict__ x, const double* __restrict__ scal,
                                         TV* __restrict__ vout, TV* __restrict__ y, const double* __restrict__ rD,
@@ -107,12 +130,12 @@ __device__ __forceinline__ void spmvRow(const MeshView& m, const TB* __restrict_
 #pragma unroll
             for (int q = 0; q < 5; ++q) d[q] = rd * y[(size_t)q * nC + j];
         }
-        const size_t bb = (base + k) * NB * 64 + lane;
+        A bk[NB];
+        slotBlock<TB, A, GEN>(m, blk, base + k, lane, bk);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            const A b0 = (A)blk[bb + (5 * i) * 64], b1 = (A)blk[bb + (5 * i + 1) * 64];
-            const A b2 = (A)blk[bb + (5 * i + 2) * 64], b3 = (A)blk[bb + (5 * i + 3) * 64],
-                    b4 = (A)blk[bb + (5 * i + 4) * 64];
+            const A b0 = bk[5 * i], b1 = bk[5 * i + 1];
+            const A b2 = bk[5 * i + 2], b3 = bk[5 * i + 3], b4 = bk[5 * i + 4];
             acc[i][0] += b0 * xj[0];
             acc[i][1] += b1 * xj[1];
             const A dd = b2 * xj[2] + b3 * xj[3] + b4 * xj[4];
@@ -143,17 +166,17 @@ __device__ __forceinline__ void spmvRow(const MeshView& m, const TB* __restrict_
     }
 }
 
-template <typename TB, typename TV>
+template <typename TB, typename TV, bool GEN>
 __global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__ blk, const TB* __restrict__ dg,
                                               const TV* __restrict__ x, const double* __restrict__ scal,
                                               TV* __restrict__ vout, TV* __restrict__ y) {
     const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
-    spmvRow<TB, TV, false>(m, blk, dg, x, scal, vout, y, nullptr, c);
+    spmvRow<TB, TV, false, GEN>(m, blk, dg, x, scal, vout, y, nullptr, c);
 }
 
 // SpMV of the rows in a cell list; FWD: fused forward sweep (see spmvRow).
-template <typename TB, typename TV, bool FWD>
+template <typename TB, typename TV, bool FWD, bool GEN>
 __global__ void __launch_bounds__(256) k_spmv_cells(MeshView m, const TB* __restrict__ blk,
                                                     const TB* __restrict__ dg, const TV* __restrict__ x,
                                                     const double* __restrict__ scal, TV* __restrict__ vout,
@@ -161,7 +184,7 @@ __global__ void __launch_bounds__(256) k_spmv_cells(MeshView m, const TB* __rest
                                                     const int* __restrict__ cells, int n) {
     const int t = xcdBlock() * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    spmvRow<TB, TV, FWD>(m, blk, dg, x, scal, vout, y, rD, cells[t]);
+    spmvRow<TB, TV, FWD, GEN>(m, blk, dg, x, scal, vout, y, rD, cells[t]);
 }
 
 // Harmonic-balance SpMV: row cell c of instance J = blockIdx.y.  The own
@@ -174,35 +197,37 @@ __global__ void __launch_bounds__(256) k_spmv_cells(MeshView m, const TB* __rest
 // With FWD (colour-1 rows of a two-colour numbering): the forward LU-SGS
 // terms of the own instance are accumulated from the same block loads and
 // the row result is rD_c (y_c - fwd), as in spmvRow.
-template <bool FWD>
-__device__ __forceinline__ void spmvRowHB(const MeshView& m, const HBBlocks& hb, int J, const double* __restrict__ x,
-                                          const double* __restrict__ scal, double* __restrict__ vout,
-                                          double* __restrict__ y, const double* __restrict__ rD, int c) {
+template <typename TB, typename TV, bool FWD, bool GEN>
+__device__ __forceinline__ void spmvRowHB(const MeshView& m, const HBBlocks& hb, int J, const TV* __restrict__ x,
+                                          const double* __restrict__ scal, TV* __restrict__ vout,
+                                          TV* __restrict__ y, const double* __restrict__ rD, int c) {
+    typedef TV A;
     const size_t nC = m.nC;
     const size_t vs = 5 * nC;
-    const double* blk = hb.blk[J];
-    const double* dg = hb.dg[J];
-    const double beta = scal ? *scal : 1.0;
-    auto X = [&](int K, int k, int j) -> double {
-        const double v = x[(size_t)K * vs + (size_t)k * nC + j];
+    const TB* blk = (const TB*)hb.blk[J];
+    const TB* dg = (const TB*)hb.dg[J];
+    const A beta = scal ? (A)(*scal) : (A)1;
+    auto X = [&](int K, int k, int j) -> A {
+        const A v = x[(size_t)K * vs + (size_t)k * nC + j];
         return scal ? v / beta : v;
     };
-    const double* yJ = y + (size_t)J * vs;
-    double xc[5];
+    const TV* yJ = y + (size_t)J * vs;
+    A xc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) xc[k] = X(J, k, c);
     if (vout)
 #pragma unroll
         for (int k = 0; k < 5; ++k) vout[(size_t)J * vs + (size_t)k * nC + c] = xc[k];
-    double acc[5][3], fw[5][3];
+    A acc[5][3], fw[5][3];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-        const double b0 = dg[diagOff(c, 5 * i)], b1 = dg[diagOff(c, 5 * i + 1)];
-        const double b2 = dg[diagOff(c, 5 * i + 2)], b3 = dg[diagOff(c, 5 * i + 3)], b4 = dg[diagOff(c, 5 * i + 4)];
+        const A b0 = (A)dg[diagOff(c, 5 * i)], b1 = (A)dg[diagOff(c, 5 * i + 1)];
+        const A b2 = (A)dg[diagOff(c, 5 * i + 2)], b3 = (A)dg[diagOff(c, 5 * i + 3)],
+                b4 = (A)dg[diagOff(c, 5 * i + 4)];
         acc[i][0] = b0 * xc[0];
         acc[i][1] = b1 * xc[1];
         acc[i][2] = b2 * xc[2] + b3 * xc[3] + b4 * xc[4];
-        if (FWD) fw[i][0] = fw[i][1] = fw[i][2] = 0.0;
+        if (FWD) fw[i][0] = fw[i][1] = fw[i][2] = (A)0;
     }
     const int sl = c >> 6, lane = c & 63;
     const int len = m.rowLen[c];
@@ -210,46 +235,46 @@ __device__ __forceinline__ void spmvRowHB(const MeshView& m, const HBBlocks& hb,
     for (int k = 0; k < len; ++k) {
         const size_t slot = (base + k) * 64 + lane;
         const int j = m.col[slot];
-        double xj[5], d[5];
+        A xj[5], d[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) xj[q] = X(J, q, j);
         if (FWD) {
-            const double rd = rD[j];
+            const A rd = (A)rD[j];
 #pragma unroll
             for (int q = 0; q < 5; ++q) d[q] = rd * yJ[(size_t)q * nC + j];
         }
-        const size_t bb = (base + k) * NB * 64 + lane;
+        A bk[NB];
+        slotBlock<TB, A, GEN>(m, blk, base + k, lane, bk);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            const double b0 = blk[bb + (5 * i) * 64], b1 = blk[bb + (5 * i + 1) * 64];
-            const double b2 = blk[bb + (5 * i + 2) * 64], b3 = blk[bb + (5 * i + 3) * 64],
-                         b4 = blk[bb + (5 * i + 4) * 64];
+            const A b0 = bk[5 * i], b1 = bk[5 * i + 1];
+            const A b2 = bk[5 * i + 2], b3 = bk[5 * i + 3], b4 = bk[5 * i + 4];
             acc[i][0] += b0 * xj[0];
             acc[i][1] += b1 * xj[1];
-            const double dd = b2 * xj[2] + b3 * xj[3] + b4 * xj[4];
+            const A dd = b2 * xj[2] + b3 * xj[3] + b4 * xj[4];
             acc[i][2] += dd;
             if (FWD) {
                 fw[i][0] += b0 * d[0];
                 fw[i][1] += b1 * d[1];
-                const double de = b2 * d[2] + b3 * d[3] + b4 * d[4];
+                const A de = b2 * d[2] + b3 * d[3] + b4 * d[4];
                 fw[i][2] += de;
             }
         }
     }
-    if (m.nCyc > 0) spmvCyclic<double, double>(m, blk, c, [&](int k, int j) { return X(J, k, j); }, acc);
+    if (m.nCyc > 0) spmvCyclic<TB, A>(m, blk, c, [&](int k, int j) { return X(J, k, j); }, acc);
     const double Vc = m.V[c];
     const int nO = hb.nO;
-    double out[5];
+    A out[5];
     // scalar rows
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        double r = 0.0;
+        A r = (A)0;
         for (int K = 0; K < nO; ++K) {
             if (K == J) {
                 r = r + acc[i][0];
                 r = r + acc[i][1];
             } else {
-                r = r + (Vc * hb.D[J * nO + K]) * X(K, i, c);
+                r = r + (A)(Vc * hb.D[J * nO + K]) * X(K, i, c);
             }
         }
         out[i] = r + acc[i][2];
@@ -257,19 +282,19 @@ __device__ __forceinline__ void spmvRowHB(const MeshView& m, const HBBlocks& hb,
     // vector rows
 #pragma unroll
     for (int i = 2; i < 5; ++i) {
-        double r = 0.0 + acc[i][0];
+        A r = (A)0 + acc[i][0];
         r = r + acc[i][1];
         for (int K = 0; K < nO; ++K) {
             if (K == J) r = r + acc[i][2];
-            else r = r + (Vc * hb.D[J * nO + K]) * X(K, i, c);
+            else r = r + (A)(Vc * hb.D[J * nO + K]) * X(K, i, c);
         }
         out[i] = r;
     }
-    double rdc = 1.0;
-    if (FWD) rdc = rD[c];
+    A rdc = (A)1;
+    if (FWD) rdc = (A)rD[c];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-        double r = out[i];
+        A r = out[i];
         if (FWD) {
             r = r - fw[i][0];
             r = r - fw[i][1];
@@ -280,29 +305,30 @@ __device__ __forceinline__ void spmvRowHB(const MeshView& m, const HBBlocks& hb,
     }
 }
 
-__global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const double* __restrict__ x,
-                                                 const double* __restrict__ scal, double* __restrict__ vout,
-                                                 double* __restrict__ y) {
+template <typename TB, typename TV, bool GEN>
+__global__ void __launch_bounds__(256) k_spmv_hb(MeshView m, HBBlocks hb, const TV* __restrict__ x,
+                                                 const double* __restrict__ scal, TV* __restrict__ vout,
+                                                 TV* __restrict__ y) {
     const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
-    spmvRowHB<false>(m, hb, blockIdx.y, x, scal, vout, y, nullptr, c);
+    spmvRowHB<TB, TV, false, GEN>(m, hb, blockIdx.y, x, scal, vout, y, nullptr, c);
 }
 
 // HB rows of a cell list (blockIdx.y = instance); FWD: fused forward sweep.
-template <bool FWD>
-__global__ void __launch_bounds__(256) k_spmv_hb_cells(MeshView m, HBBlocks hb, const double* __restrict__ x,
-                                                       const double* __restrict__ scal, double* __restrict__ vout,
-                                                       double* __restrict__ y, const double* __restrict__ rD,
+template <typename TB, typename TV, bool FWD, bool GEN>
+__global__ void __launch_bounds__(256) k_spmv_hb_cells(MeshView m, HBBlocks hb, const TV* __restrict__ x,
+                                                       const double* __restrict__ scal, TV* __restrict__ vout,
+                                                       TV* __restrict__ y, const double* __restrict__ rD,
                                                        const int* __restrict__ cells, int n) {
     const int t = xcdBlock() * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    spmvRowHB<FWD>(m, hb, blockIdx.y, x, scal, vout, y, rD, cells[t]);
+    spmvRowHB<TB, TV, FWD, GEN>(m, hb, blockIdx.y, x, scal, vout, y, rD, cells[t]);
 }
 
 // ------------------------------------------------------------- LU-SGS -----
 // Forward sweep of one level: r[c] -= sum over lower neighbours p (in
 // increasing p) of L_cp * (rD[p] r[p]).
-template <typename TB, typename TV>
+template <typename TB, typename TV, bool GEN>
 __device__ __forceinline__ void lusgsFwdRow(const MeshView& m, const TB* __restrict__ blk,
                                             const double* __restrict__ rD, TV* __restrict__ r, int c) {
     const int nl = m.nLower[c];
@@ -321,12 +347,12 @@ __device__ __forceinline__ void lusgsFwdRow(const MeshView& m, const TB* __restr
         A d[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) d[q] = rd * r[(size_t)q * nC + p];
-        const size_t bb = (base + k) * NB * 64 + lane;
+        A bk[NB];
+        slotBlock<TB, A, GEN>(m, blk, base + k, lane, bk);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            const A b0 = (A)blk[bb + (5 * i) * 64], b1 = (A)blk[bb + (5 * i + 1) * 64];
-            const A b2 = (A)blk[bb + (5 * i + 2) * 64], b3 = (A)blk[bb + (5 * i + 3) * 64],
-                    b4 = (A)blk[bb + (5 * i + 4) * 64];
+            const A b0 = bk[5 * i], b1 = bk[5 * i + 1];
+            const A b2 = bk[5 * i + 2], b3 = bk[5 * i + 3], b4 = bk[5 * i + 4];
             acc[i] -= b0 * d[0];
             acc[i] -= b1 * d[1];
             const A dd = b2 * d[2] + b3 * d[3] + b4 * d[4];
@@ -339,7 +365,7 @@ __device__ __forceinline__ void lusgsFwdRow(const MeshView& m, const TB* __restr
 
 // Backward sweep of one level: r[c] = rD[c] (r[c] - sum over upper
 // neighbours q (decreasing q) of U_cq r[q]), r[q] already final.
-template <typename TB, typename TV>
+template <typename TB, typename TV, bool GEN>
 __device__ __forceinline__ void lusgsBwdRow(const MeshView& m, const TB* __restrict__ blk,
                                             const double* __restrict__ rD, TV* __restrict__ r, int c) {
     const size_t nC = m.nC;
@@ -356,12 +382,12 @@ __device__ __forceinline__ void lusgsBwdRow(const MeshView& m, const TB* __restr
         A d[5];
 #pragma unroll
         for (int z = 0; z < 5; ++z) d[z] = r[(size_t)z * nC + q];
-        const size_t bb = (base + k) * NB * 64 + lane;
+        A bk[NB];
+        slotBlock<TB, A, GEN>(m, blk, base + k, lane, bk);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            const A b0 = (A)blk[bb + (5 * i) * 64], b1 = (A)blk[bb + (5 * i + 1) * 64];
-            const A b2 = (A)blk[bb + (5 * i + 2) * 64], b3 = (A)blk[bb + (5 * i + 3) * 64],
-                    b4 = (A)blk[bb + (5 * i + 4) * 64];
+            const A b0 = bk[5 * i], b1 = bk[5 * i + 1];
+            const A b2 = bk[5 * i + 2], b3 = bk[5 * i + 3], b4 = bk[5 * i + 4];
             acc[i] -= b0 * d[0];
             acc[i] -= b1 * d[1];
             const A dd = b2 * d[2] + b3 * d[3] + b4 * d[4];
@@ -373,36 +399,36 @@ __device__ __forceinline__ void lusgsBwdRow(const MeshView& m, const TB* __restr
     for (int i = 0; i < 5; ++i) r[(size_t)i * nC + c] = rd * acc[i];
 }
 
-template <typename TB, typename TV>
+template <typename TB, typename TV, bool GEN>
 __global__ void __launch_bounds__(256) k_lusgs_fwd(MeshView m, const TB* __restrict__ blk,
                                                    const double* __restrict__ rD, TV* __restrict__ r,
                                                    const int* __restrict__ cells, int n) {
     const int t = xcdBlock() * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    lusgsFwdRow<TB, TV>(m, blk, rD, r, cells[t]);
+    lusgsFwdRow<TB, TV, GEN>(m, blk, rD, r, cells[t]);
 }
 
-template <typename TB, typename TV>
+template <typename TB, typename TV, bool GEN>
 __global__ void __launch_bounds__(256) k_lusgs_bwd(MeshView m, const TB* __restrict__ blk,
                                                    const double* __restrict__ rD, TV* __restrict__ r,
                                                    const int* __restrict__ cells, int n) {
     const int t = xcdBlock() * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    lusgsBwdRow<TB, TV>(m, blk, rD, r, cells[t]);
+    lusgsBwdRow<TB, TV, GEN>(m, blk, rD, r, cells[t]);
 }
 
 // Harmonic balance: one launch per level for all instances (blockIdx.y =
 // instance; the cross-instance blocks have no faces, so the sweeps are
 // independent per instance and share the diagonal rD).
-template <bool FWD>
+template <typename TB, typename TV, bool FWD, bool GEN>
 __global__ void __launch_bounds__(256) k_lusgs_hb(MeshView m, HBBlocks hb, const double* __restrict__ rD,
-                                                  double* __restrict__ r, const int* __restrict__ cells, int n) {
+                                                  TV* __restrict__ r, const int* __restrict__ cells, int n) {
     const int t = xcdBlock() * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int J = blockIdx.y;
-    double* rJ = r + (size_t)J * 5 * m.nC;
-    if (FWD) lusgsFwdRow<double, double>(m, hb.blk[J], rD, rJ, cells[t]);
-    else lusgsBwdRow<double, double>(m, hb.blk[J], rD, rJ, cells[t]);
+    TV* rJ = r + (size_t)J * 5 * m.nC;
+    if (FWD) lusgsFwdRow<TB, TV, GEN>(m, (const TB*)hb.blk[J], rD, rJ, cells[t]);
+    else lusgsBwdRow<TB, TV, GEN>(m, (const TB*)hb.blk[J], rD, rJ, cells[t]);
 }
 
 // ------------------------------------------------------- block-Jacobi -----
@@ -584,6 +610,15 @@ __global__ void __launch_bounds__(256) k_multidot(int nC, const TV* __restrict__
         }
     }
     blockReduceStore<CH>(v, part);
+}
+
+// CGS projections of one pass over w: ch (8, 16 or 32) basis vectors.
+template <typename TV>
+void launchMultidot(int ch, int nbk, int nC, const TV* w, const TV* V, size_t vs, int j0, int nv, double* part,
+                    hipStream_t st) {
+    if (ch == 8) k_multidot<TV, 8><<<nbk, 256, 0, st>>>(nC, w, V, vs, j0, nv, part);
+    else if (ch == 32) k_multidot<TV, 32><<<nbk, 256, 0, st>>>(nC, w, V, vs, j0, nv, part);
+    else k_multidot<TV, 16><<<nbk, 256, 0, st>>>(nC, w, V, vs, j0, nv, part);
 }
 
 // w -= sum_j h_j V_j (j < nv), then ||w||^2 partials
@@ -798,16 +833,55 @@ __global__ void k_gm_final(GmresScalars* g, int emptyDir) {
 // --------------------------------------------------------------- host side --
 static inline int nb(size_t n, int b) { return (int)((n + b - 1) / b); }
 
+// The CGS projection passes of one Arnoldi step against nv basis vectors,
+// ch vectors per pass (autotuner timing).
+void launchProjectionPasses(const MeshView& m, int vecFp32, int ch, const void* w, const void* V, int nv,
+                            double* part, hipStream_t st) {
+    const int nC = m.nC, nbk = reduceBlocks(nC);
+    const size_t vs = (size_t)5 * nC;
+    for (int j0 = 0; j0 < nv; j0 += ch) {
+        const int n = (nv - j0) < ch ? (nv - j0) : ch;
+        if (vecFp32) launchMultidot<float>(ch, nbk, nC, (const float*)w, (const float*)V, vs, j0, n, part, st);
+        else launchMultidot<double>(ch, nbk, nC, (const double*)w, (const double*)V, vs, j0, n, part, st);
+    }
+}
+
 int reduceBlocks(int nC) {
     int n = nb(nC, 256);
     return n < 1024 ? (n < 1 ? 1 : n) : 1024;
 }
 
+// Full off-diagonal blocks from generator storage (read-back of the
+// assembled matrix; padding slots are left untouched).
+template <typename TB>
+__global__ void k_expand_blocks(MeshView m, const TB* __restrict__ gen, TB* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const int sl = c >> 6, lane = c & 63;
+    const size_t base = (size_t)m.slicePtr[sl];
+    for (int k = 0; k < m.rowLen[c]; ++k) {
+        TB b[NB];
+        slotBlock<TB, TB, true>(m, gen, base + k, lane, b);
+#pragma unroll
+        for (int e = 0; e < NB; ++e) out[((base + k) * NB + e) * 64 + lane] = b[e];
+    }
+}
+
+void launchExpandBlocks(const MeshView& m, int fp32, const void* gen, void* out, hipStream_t st) {
+    if (fp32) k_expand_blocks<float><<<nb(m.nC, 256), 256, 0, st>>>(m, (const float*)gen, (float*)out);
+    else k_expand_blocks<double><<<nb(m.nC, 256), 256, 0, st>>>(m, (const double*)gen, (double*)out);
+}
+
 template <typename TB, typename TV>
 void spmv(const MeshView& m, const void* blk, const void* dg, const void* x, const double* scal, void* vout, void* y,
           hipStream_t st) {
-    k_spmv<TB, TV><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, (const TB*)blk, (const TB*)dg, (const TV*)x, scal, (TV*)vout,
-                                                  (TV*)y);
+    const dim3 g(xcdGrid(m.nC, m.bsOp));
+    if (m.blkGen)
+        k_spmv<TB, TV, true><<<g, m.bsOp, 0, st>>>(m, (const TB*)blk, (const TB*)dg, (const TV*)x, scal, (TV*)vout,
+                                                   (TV*)y);
+    else
+        k_spmv<TB, TV, false><<<g, m.bsOp, 0, st>>>(m, (const TB*)blk, (const TB*)dg, (const TV*)x, scal, (TV*)vout,
+                                                    (TV*)y);
 }
 
 void launchSpmv(const MeshView& m, int blkFp32, int vecFp32, const void* blk, const void* dg, const void* x,
@@ -818,18 +892,26 @@ void launchSpmv(const MeshView& m, int blkFp32, int vecFp32, const void* blk, co
     else spmv<double, double>(m, blk, dg, x, scal, vout, y, st);
 }
 
-template <typename TB, typename TV>
-static void spmvLusgsT(const MeshView& m, const Levels& L, const void* blk, const void* dg, const double* rD,
+template <typename TB, typename TV, bool GEN>
+static void spmvLusgsG(const MeshView& m, const Levels& L, const void* blk, const void* dg, const double* rD,
                        const void* x, const double* scal, void* vout, void* y, hipStream_t st) {
     const int n0 = L.fwdPtrHost[1], n1 = L.fwdPtrHost[2] - n0;
     const TB* B = (const TB*)blk;
     const TB* G = (const TB*)dg;
-    k_spmv_cells<TB, TV, false><<<xcdGrid(n0, 256), 256, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout, (TV*)y, rD,
-                                                             L.fwdCells, n0);
-    k_spmv_cells<TB, TV, true><<<xcdGrid(n1, 256), 256, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout, (TV*)y, rD,
-                                                            L.fwdCells + n0, n1);
+    const int bs = m.bsOp;
+    k_spmv_cells<TB, TV, false, GEN><<<xcdGrid(n0, bs), bs, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout,
+                                                                     (TV*)y, rD, L.fwdCells, n0);
+    k_spmv_cells<TB, TV, true, GEN><<<xcdGrid(n1, bs), bs, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout, (TV*)y,
+                                                                    rD, L.fwdCells + n0, n1);
     const int s0 = L.bwdPtrHost[1], nbw = L.bwdPtrHost[2] - s0;
-    k_lusgs_bwd<TB, TV><<<xcdGrid(nbw, 256), 256, 0, st>>>(m, B, rD, (TV*)y, L.bwdCells + s0, nbw);
+    k_lusgs_bwd<TB, TV, GEN><<<xcdGrid(nbw, bs), bs, 0, st>>>(m, B, rD, (TV*)y, L.bwdCells + s0, nbw);
+}
+
+template <typename TB, typename TV>
+static void spmvLusgsT(const MeshView& m, const Levels& L, const void* blk, const void* dg, const double* rD,
+                       const void* x, const double* scal, void* vout, void* y, hipStream_t st) {
+    if (m.blkGen) spmvLusgsG<TB, TV, true>(m, L, blk, dg, rD, x, scal, vout, y, st);
+    else spmvLusgsG<TB, TV, false>(m, L, blk, dg, rD, x, scal, vout, y, st);
 }
 
 bool launchSpmvLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp32, const void* blk, const void* dg,
@@ -842,31 +924,32 @@ bool launchSpmvLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp3
     return true;
 }
 
-void launchLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp32, const void* blk, const double* rD,
-                 void* r, hipStream_t st) {
+template <typename TB, typename TV, bool GEN>
+static void lusgsG(const MeshView& m, const Levels& L, const void* blk, const double* rD, void* r, hipStream_t st) {
+    const int bs = m.bsOp;
     for (int l = 0; l < L.nFwd; ++l) {
         const int s0 = L.fwdPtrHost[l], n = L.fwdPtrHost[l + 1] - s0;
         if (L.fwdSkip && l == 0) continue;  // first level has no lower neighbours
-        if (blkFp32 && vecFp32)
-            k_lusgs_fwd<float, float><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (float*)r, L.fwdCells + s0, n);
-        else if (blkFp32)
-            k_lusgs_fwd<float, double><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (double*)r, L.fwdCells + s0, n);
-        else if (vecFp32)
-            k_lusgs_fwd<double, float><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (float*)r, L.fwdCells + s0, n);
-        else
-            k_lusgs_fwd<double, double><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (double*)r, L.fwdCells + s0, n);
+        k_lusgs_fwd<TB, TV, GEN><<<xcdGrid(n, bs), bs, 0, st>>>(m, (const TB*)blk, rD, (TV*)r, L.fwdCells + s0, n);
     }
     for (int l = 0; l < L.nBwd; ++l) {
         const int s0 = L.bwdPtrHost[l], n = L.bwdPtrHost[l + 1] - s0;
-        if (blkFp32 && vecFp32)
-            k_lusgs_bwd<float, float><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (float*)r, L.bwdCells + s0, n);
-        else if (blkFp32)
-            k_lusgs_bwd<float, double><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const float*)blk, rD, (double*)r, L.bwdCells + s0, n);
-        else if (vecFp32)
-            k_lusgs_bwd<double, float><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (float*)r, L.bwdCells + s0, n);
-        else
-            k_lusgs_bwd<double, double><<<xcdGrid(n, 256), 256, 0, st>>>(m, (const double*)blk, rD, (double*)r, L.bwdCells + s0, n);
+        k_lusgs_bwd<TB, TV, GEN><<<xcdGrid(n, bs), bs, 0, st>>>(m, (const TB*)blk, rD, (TV*)r, L.bwdCells + s0, n);
     }
+}
+
+template <typename TB, typename TV>
+static void lusgsT(const MeshView& m, const Levels& L, const void* blk, const double* rD, void* r, hipStream_t st) {
+    if (m.blkGen) lusgsG<TB, TV, true>(m, L, blk, rD, r, st);
+    else lusgsG<TB, TV, false>(m, L, blk, rD, r, st);
+}
+
+void launchLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp32, const void* blk, const double* rD,
+                 void* r, hipStream_t st) {
+    if (blkFp32 && vecFp32) lusgsT<float, float>(m, L, blk, rD, r, st);
+    else if (blkFp32) lusgsT<float, double>(m, L, blk, rD, r, st);
+    else if (vecFp32) lusgsT<double, float>(m, L, blk, rD, r, st);
+    else lusgsT<double, double>(m, L, blk, rD, r, st);
 }
 
 void launchJacobiSetup(int nC, int blkFp32, const void* dg, void* dinv, hipStream_t st) {
@@ -946,10 +1029,11 @@ void gmresCycleT(const GmresArgs& a, hipStream_t st) {
             k_gm_givens<<<1, 64, 0, st>>>(g, i, 1);
         } else {
             // classical Gram-Schmidt: one projection pass + one update pass
-            for (int j0 = 0; j0 <= i; j0 += 16) {
-                const int nv = (i + 1 - j0) < 16 ? (i + 1 - j0) : 16;
-                k_multidot<TV, 16><<<nbk, 256, 0, st>>>(nC, r, V, vs, j0, nv, a.wk->part);
-                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 16, red);
+            const int ch = m.mdCh;
+            for (int j0 = 0; j0 <= i; j0 += ch) {
+                const int nv = (i + 1 - j0) < ch ? (i + 1 - j0) : ch;
+                launchMultidot<TV>(ch, nbk, nC, r, V, vs, j0, nv, a.wk->part, st);
+                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, ch, red);
                 k_gm_store_col<<<1, 64, 0, st>>>(g, i, j0, nv);
             }
             k_cgs_update<TV><<<nbk, 256, 0, st>>>(nC, r, V, vs, i + 1, g->hcol, a.wk->part);
@@ -1061,37 +1145,78 @@ void launchZeroComponent(int nC, int comp, int fp32, void* x, hipStream_t st) {
 // ------------------------------------------------- harmonic balance -----
 namespace icsf {
 
-void launchSpmvHB(const MeshView& m, const HBBlocks& hb, const double* x, const double* scal, double* vout,
-                  double* y, hipStream_t st) {
+template <typename TB, typename TV>
+void spmvHBT(const MeshView& m, const HBBlocks& hb, const void* x, const double* scal, void* vout, void* y,
+             hipStream_t st) {
     dim3 grid(xcdGrid(m.nC, 256), hb.nO);
-    k_spmv_hb<<<grid, 256, 0, st>>>(m, hb, x, scal, vout, y);
+    if (m.blkGen) k_spmv_hb<TB, TV, true><<<grid, 256, 0, st>>>(m, hb, (const TV*)x, scal, (TV*)vout, (TV*)y);
+    else k_spmv_hb<TB, TV, false><<<grid, 256, 0, st>>>(m, hb, (const TV*)x, scal, (TV*)vout, (TV*)y);
 }
 
-void launchLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, double* r,
-                   hipStream_t st) {
+void launchSpmvHB(const MeshView& m, const HBBlocks& hb, int vecFp32, const void* x, const double* scal, void* vout,
+                  void* y, hipStream_t st) {
+    if (hb.fp32 && vecFp32) spmvHBT<float, float>(m, hb, x, scal, vout, y, st);
+    else if (hb.fp32) spmvHBT<float, double>(m, hb, x, scal, vout, y, st);
+    else if (vecFp32) spmvHBT<double, float>(m, hb, x, scal, vout, y, st);
+    else spmvHBT<double, double>(m, hb, x, scal, vout, y, st);
+}
+
+template <typename TB, typename TV, bool GEN>
+void lusgsHBG(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, void* r, hipStream_t st) {
     for (int l = 0; l < L.nFwd; ++l) {
         const int s0 = L.fwdPtrHost[l], n = L.fwdPtrHost[l + 1] - s0;
         if (L.fwdSkip && l == 0) continue;
         dim3 grid(xcdGrid(n, 256), hb.nO);
-        k_lusgs_hb<true><<<grid, 256, 0, st>>>(m, hb, rD, r, L.fwdCells + s0, n);
+        k_lusgs_hb<TB, TV, true, GEN><<<grid, 256, 0, st>>>(m, hb, rD, (TV*)r, L.fwdCells + s0, n);
     }
     for (int l = 0; l < L.nBwd; ++l) {
         const int s0 = L.bwdPtrHost[l], n = L.bwdPtrHost[l + 1] - s0;
         dim3 grid(xcdGrid(n, 256), hb.nO);
-        k_lusgs_hb<false><<<grid, 256, 0, st>>>(m, hb, rD, r, L.bwdCells + s0, n);
+        k_lusgs_hb<TB, TV, false, GEN><<<grid, 256, 0, st>>>(m, hb, rD, (TV*)r, L.bwdCells + s0, n);
     }
 }
 
-bool launchSpmvLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, const double* x,
-                       const double* scal, double* vout, double* y, hipStream_t st) {
-    if (!L.twoColour) return false;
+template <typename TB, typename TV>
+void lusgsHBT(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, void* r, hipStream_t st) {
+    if (m.blkGen) lusgsHBG<TB, TV, true>(m, L, hb, rD, r, st);
+    else lusgsHBG<TB, TV, false>(m, L, hb, rD, r, st);
+}
+
+void launchLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, int vecFp32, const double* rD, void* r,
+                   hipStream_t st) {
+    if (hb.fp32 && vecFp32) lusgsHBT<float, float>(m, L, hb, rD, r, st);
+    else if (hb.fp32) lusgsHBT<float, double>(m, L, hb, rD, r, st);
+    else if (vecFp32) lusgsHBT<double, float>(m, L, hb, rD, r, st);
+    else lusgsHBT<double, double>(m, L, hb, rD, r, st);
+}
+
+template <typename TB, typename TV, bool GEN>
+void spmvLusgsHBG(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, const void* x,
+                  const double* scal, void* vout, void* y, hipStream_t st) {
     const int n0 = L.fwdPtrHost[1], n1 = L.fwdPtrHost[2] - n0;
-    k_spmv_hb_cells<false><<<dim3(xcdGrid(n0, 256), hb.nO), 256, 0, st>>>(m, hb, x, scal, vout, y, rD, L.fwdCells,
-                                                                         n0);
-    k_spmv_hb_cells<true><<<dim3(xcdGrid(n1, 256), hb.nO), 256, 0, st>>>(m, hb, x, scal, vout, y, rD,
-                                                                        L.fwdCells + n0, n1);
+    k_spmv_hb_cells<TB, TV, false, GEN><<<dim3(xcdGrid(n0, 256), hb.nO), 256, 0, st>>>(
+        m, hb, (const TV*)x, scal, (TV*)vout, (TV*)y, rD, L.fwdCells, n0);
+    k_spmv_hb_cells<TB, TV, true, GEN><<<dim3(xcdGrid(n1, 256), hb.nO), 256, 0, st>>>(
+        m, hb, (const TV*)x, scal, (TV*)vout, (TV*)y, rD, L.fwdCells + n0, n1);
     const int s0 = L.bwdPtrHost[1], nbw = L.bwdPtrHost[2] - s0;
-    k_lusgs_hb<false><<<dim3(xcdGrid(nbw, 256), hb.nO), 256, 0, st>>>(m, hb, rD, y, L.bwdCells + s0, nbw);
+    k_lusgs_hb<TB, TV, false, GEN><<<dim3(xcdGrid(nbw, 256), hb.nO), 256, 0, st>>>(m, hb, rD, (TV*)y,
+                                                                                  L.bwdCells + s0, nbw);
+}
+
+template <typename TB, typename TV>
+void spmvLusgsHBT(const MeshView& m, const Levels& L, const HBBlocks& hb, const double* rD, const void* x,
+                  const double* scal, void* vout, void* y, hipStream_t st) {
+    if (m.blkGen) spmvLusgsHBG<TB, TV, true>(m, L, hb, rD, x, scal, vout, y, st);
+    else spmvLusgsHBG<TB, TV, false>(m, L, hb, rD, x, scal, vout, y, st);
+}
+
+bool launchSpmvLusgsHB(const MeshView& m, const Levels& L, const HBBlocks& hb, int vecFp32, const double* rD,
+                       const void* x, const double* scal, void* vout, void* y, hipStream_t st) {
+    if (!L.twoColour) return false;
+    if (hb.fp32 && vecFp32) spmvLusgsHBT<float, float>(m, L, hb, rD, x, scal, vout, y, st);
+    else if (hb.fp32) spmvLusgsHBT<float, double>(m, L, hb, rD, x, scal, vout, y, st);
+    else if (vecFp32) spmvLusgsHBT<double, float>(m, L, hb, rD, x, scal, vout, y, st);
+    else spmvLusgsHBT<double, double>(m, L, hb, rD, x, scal, vout, y, st);
     return true;
 }
 
@@ -1143,7 +1268,7 @@ __global__ void k_scale_by(size_t n, const TV* __restrict__ a, const double* __r
 // against v_0..v_j; Givens; back-substitution into g->yh.  Z holds z_j.
 void fgmresCycle(const FgmresArgs& a, hipStream_t st) {
     const MeshView& m = *a.m;
-    const int nC = m.nC;
+    const int nC = a.nVec ? a.nVec : m.nC;
     const int nbk = reduceBlocks(nC);
     const size_t vs = (size_t)5 * nC;
     double* r = (double*)a.r;
@@ -1160,7 +1285,8 @@ void fgmresCycle(const FgmresArgs& a, hipStream_t st) {
         k_scale_by<double><<<nb(vs, 256), 256, 0, st>>>(vs, r, &g->beta, vi);
         a.precond(vi, zi);
         if (a.mark) a.mark(TK_SPMV, 1);
-        launchSpmv(m, a.blkFp32, 0, a.blk, a.dg, zi, nullptr, nullptr, r, st);
+        if (a.op) a.op(zi, r);
+        else launchSpmv(m, a.blkFp32, 0, a.blk, a.dg, zi, nullptr, nullptr, r, st);
         if (a.mark) a.mark(TK_SPMV, 0);
         if (a.mark) a.mark(TK_ORTH, 1);
         if (a.orth == 0) {
@@ -1174,10 +1300,11 @@ void fgmresCycle(const FgmresArgs& a, hipStream_t st) {
             }
             k_gm_givens<<<1, 64, 0, st>>>(g, i, 1);
         } else {
-            for (int j0 = 0; j0 <= i; j0 += 16) {
-                const int nv = (i + 1 - j0) < 16 ? (i + 1 - j0) : 16;
-                k_multidot<double, 16><<<nbk, 256, 0, st>>>(nC, r, V, vs, j0, nv, a.wk->part);
-                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 16, red);
+            const int ch = m.mdCh;
+            for (int j0 = 0; j0 <= i; j0 += ch) {
+                const int nv = (i + 1 - j0) < ch ? (i + 1 - j0) : ch;
+                launchMultidot<double>(ch, nbk, nC, r, V, vs, j0, nv, a.wk->part, st);
+                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, ch, red);
                 k_gm_store_col<<<1, 64, 0, st>>>(g, i, j0, nv);
             }
             k_cgs_update<double><<<nbk, 256, 0, st>>>(nC, r, V, vs, i + 1, g->hcol, a.wk->part);

@@ -578,19 +578,27 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
         const int bs = isOwn ? FT_REC + 4 : FT_REC;
         const int ds = isOwn ? FT_REC : FT_REC + 4;
         double A[NB];
-        fluxJacobian(g, V3{t[bs], t[bs + 1], t[bs + 2]}, t[bs + 3], nrm, A);
         const double hb = isOwn ? 0.5 * ms : -0.5 * ms;
-        const size_t base = ((rowSlot0 + slot) * NB) * 64 + lane;
         const double v = 0.5 * ms * t[FT_LAM];
         const double sv = visc ? t[FT_SV] : 0.0;
+        if (m.blkGen) {
+            // generators: the solver kernels rebuild h A(U, E, n) - v I - sv I
+            const size_t gb = ((rowSlot0 + slot) * NGEN) * 64 + lane;
+            const double gv[NGEN] = {t[bs], t[bs + 1], t[bs + 2], t[bs + 3], nrm.x, nrm.y, nrm.z, hb, v, sv};
 #pragma unroll
-        for (int e = 0; e < NB; ++e) {
-            double x = hb * A[e];
-            if (e == 0 || e == 6 || e == 12 || e == 18 || e == 24) {
-                x = x - v;
-                if (visc) x = x - sv;
+            for (int e = 0; e < NGEN; ++e) blk[gb + (size_t)e * 64] = (TB)gv[e];
+        } else {
+            fluxJacobian(g, V3{t[bs], t[bs + 1], t[bs + 2]}, t[bs + 3], nrm, A);
+            const size_t base = ((rowSlot0 + slot) * NB) * 64 + lane;
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                double x = hb * A[e];
+                if (e == 0 || e == 6 || e == 12 || e == 18 || e == 24) {
+                    x = x - v;
+                    if (visc) x = x - sv;
+                }
+                blk[base + (size_t)e * 64] = (TB)x;
             }
-            blk[base + (size_t)e * 64] = (TB)x;
         }
         fluxJacobian(g, V3{t[ds], t[ds + 1], t[ds + 2]}, t[ds + 3], nrm, A);
         const double hd = isOwn ? -0.5 * ms : 0.5 * ms;
@@ -721,9 +729,10 @@ __global__ void __launch_bounds__(256) k_cell_jacobian(MeshView m, Phys ph, Stat
                                                        int* __restrict__ err) {
     cellJacobianBody<TB>(m, ph, s, blk, dg, rD, err);
 }
+template <typename TB>
 __global__ void __launch_bounds__(256) k_cell_jacobian_hb(const HBInst* __restrict__ I, Phys ph) {
     const HBInst q = I[blockIdx.y];
-    cellJacobianBody<double>(q.m, ph, q.s, q.blk, q.dg, q.rD, q.err);
+    cellJacobianBody<TB>(q.m, ph, q.s, (TB*)q.blk, (TB*)q.dg, q.rD, q.err);
 }
 
 }  // namespace
@@ -738,22 +747,25 @@ void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const d
 
 void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st) {
     if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
-    if (ph.flux == FLUX_ROE) k_grad<7><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, nullptr, nullptr);
-    else k_grad<8><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, nullptr, nullptr);
-    k_face_terms<<<xcdGrid(m.nF, 256), 256, 0, st>>>(m, ph, s);
-    k_cell_residual<<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s);
+    const int bc = m.bsCell, bf = m.bsFace;
+    if (ph.flux == FLUX_ROE) k_grad<7><<<xcdGrid(m.nC, bc), bc, 0, st>>>(m, ph, s, nullptr, nullptr);
+    else k_grad<8><<<xcdGrid(m.nC, bc), bc, 0, st>>>(m, ph, s, nullptr, nullptr);
+    k_face_terms<<<xcdGrid(m.nF, bf), bf, 0, st>>>(m, ph, s);
+    k_cell_residual<<<xcdGrid(m.nC, bc), bc, 0, st>>>(m, ph, s);
 }
 
 void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st) {
     if (m.nB > 0) k_boundary_hb<<<dim3(nblk(m.nB, 256), nO), 256, 0, st>>>(inst, ph);
-    if (ph.flux == FLUX_ROE) k_grad_hb<7><<<dim3(xcdGrid(m.nC, 256), nO), 256, 0, st>>>(inst, ph);
-    else k_grad_hb<8><<<dim3(xcdGrid(m.nC, 256), nO), 256, 0, st>>>(inst, ph);
-    k_face_terms_hb<<<dim3(xcdGrid(m.nF, 256), nO), 256, 0, st>>>(inst, ph);
-    k_cell_residual_hb<<<dim3(xcdGrid(m.nC, 256), nO), 256, 0, st>>>(inst, ph);
+    const int bc = m.bsCell, bf = m.bsFace;
+    if (ph.flux == FLUX_ROE) k_grad_hb<7><<<dim3(xcdGrid(m.nC, bc), nO), bc, 0, st>>>(inst, ph);
+    else k_grad_hb<8><<<dim3(xcdGrid(m.nC, bc), nO), bc, 0, st>>>(inst, ph);
+    k_face_terms_hb<<<dim3(xcdGrid(m.nF, bf), nO), bf, 0, st>>>(inst, ph);
+    k_cell_residual_hb<<<dim3(xcdGrid(m.nC, bc), nO), bc, 0, st>>>(inst, ph);
 }
 
-void launchJacobianHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st) {
-    k_cell_jacobian_hb<<<dim3(xcdGrid(m.nC, 256), nO), 256, 0, st>>>(inst, ph);
+void launchJacobianHB(const HBInst* inst, int nO, int fp32, const MeshView& m, const Phys& ph, hipStream_t st) {
+    if (fp32) k_cell_jacobian_hb<float><<<dim3(xcdGrid(m.nC, m.bsJac), nO), m.bsJac, 0, st>>>(inst, ph);
+    else k_cell_jacobian_hb<double><<<dim3(xcdGrid(m.nC, m.bsJac), nO), m.bsJac, 0, st>>>(inst, ph);
 }
 
 void launchAdjointForward(const MeshView& m, const Phys& ph, const StateView& s, double* gun, double* limo,
@@ -774,10 +786,11 @@ void launchFaceFlux(const MeshView& m, const Phys& ph, const StateView& s, doubl
 void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
                     double* rD, double* dg64, int* err, hipStream_t st) {
     (void)dg64;
+    const int bj = m.bsJac;
     if (fp32)
-        k_cell_jacobian<float><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
+        k_cell_jacobian<float><<<xcdGrid(m.nC, bj), bj, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
     else
-        k_cell_jacobian<double><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
+        k_cell_jacobian<double><<<xcdGrid(m.nC, bj), bj, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
 }
 
 }  // namespace icsf

@@ -100,6 +100,8 @@ struct icsf_case {
     int precond = 0;     // 0 LUSGS, 1 Jacobi, 2 none
     int nDirs = 30, maxIter = 1, minIter = 0, innerDirs = 30, orth = 0, mixed = 0;
     bool fuse = true;  // SpMV fused with the LU-SGS sweeps on two-colour numberings
+    bool autotune = true, tuned = false;
+    bool blkGen = true;  // off-diagonal blocks as generators (jacobianStorage)
     double tol = 1e-12, relTol = 1e-2;
     double co = 1, coMin = 0.1, coMax = 25, coIncr = 1.25, coDecr = 0.1, lowerBound = 0.95;
     int localTS = 1, bounding = 1;
@@ -152,7 +154,8 @@ struct icsf_case {
         if (graph) (void)hipGraphDestroy(graph);
         for (auto& e : events) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
         for (auto& e : pool) (void)hipEventDestroy(e);
-        for (auto& e : open) (void)hipEventDestroy(e);
+        for (auto& e : open)
+            if (e) (void)hipEventDestroy(e);
         if (st && ownStream) (void)hipStreamDestroy(st);
     }
 };
@@ -352,6 +355,14 @@ int readSettings(icsf_case* c, const char* thermo, const char* schemes, const ch
     if (gcw == "true" || gcw == "on" || gcw == "yes") c->useGraph = true;
     else if (gcw == "false" || gcw == "off" || gcw == "no") c->useGraph = false;
     else return fail(ICSF_ERR_KEYWORD, "graphCapture must be true or false");
+    const std::string jsw = fd.word(base + "jacobianStorage", "generators");
+    if (jsw == "generators") c->blkGen = true;
+    else if (jsw == "blocks") c->blkGen = false;
+    else return fail(ICSF_ERR_KEYWORD, "jacobianStorage must be generators or blocks");
+    const std::string atw = fd.word(base + "autotune", "true");
+    if (atw == "true" || atw == "on" || atw == "yes") c->autotune = true;
+    else if (atw == "false" || atw == "off" || atw == "no") c->autotune = false;
+    else return fail(ICSF_ERR_KEYWORD, "autotune must be true or false");
     const std::string fsw = fd.word(base + "fusedSweeps", "true");
     if (fsw == "true" || fsw == "on" || fsw == "yes") c->fuse = true;
     else if (fsw == "false" || fsw == "off" || fsw == "no") c->fuse = false;
@@ -470,6 +481,83 @@ int solveStage(icsf_case* c, const double* W, const double* b, int& nIter) {
     return gmresSolve(c, W, b, nIter);
 }
 
+// Autotuner: on the first pseudo-step, after the residual and Jacobian
+// (every buffer then holds the case's real data), each launch parameter is
+// timed with HIP events on the case stream over its candidates and the
+// fastest kept: workgroup sizes 64/128/256 of the face-terms, gradient +
+// cell-residual, Jacobian-row and Arnoldi-operator kernels, and 8/16/32
+// basis vectors per CGS projection pass.  Only scratch vectors are written
+// (or state is recomputed identically); no candidate changes a result bit.
+int autotune(icsf_case* c) {
+    {
+        const hipError_t pend = hipGetLastError();
+        if (pend != hipSuccess)
+            return fail(ICSF_ERR_HIP, std::string("autotune: HIP error pending on entry: ") + hipGetErrorString(pend));
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipError_t terr = hipSuccess;
+    const char* twhere = "";
+    auto note = [&](hipError_t e, const char* where) {
+        if (e != hipSuccess && terr == hipSuccess) { terr = e; twhere = where; }
+    };
+    auto timeIt = [&](const std::function<void()>& fn) -> float {
+        fn();
+        note(hipGetLastError(), "launch");
+        note(hipEventRecord(e0, c->st), "hipEventRecord");
+        for (int r = 0; r < 3; ++r) fn();
+        note(hipGetLastError(), "launch");
+        note(hipEventRecord(e1, c->st), "hipEventRecord");
+        note(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0;
+        note(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        return ms;
+    };
+    auto pick = [&](int& param, const std::vector<int>& cand, const std::function<void()>& fn) {
+        int best = param;
+        float bt = 1e30f;
+        for (int v : cand) {
+            param = v;
+            const float t = timeIt(fn);
+            if (t < bt) { bt = t; best = v; }
+        }
+        param = best;
+    };
+    MeshView& m = c->mv;
+    const std::vector<int> bs{64, 128, 256};
+    pick(m.bsFace, bs, [&] { launchResidual(m, c->ph, c->sv, c->st); });
+    pick(m.bsCell, bs, [&] { launchResidual(m, c->ph, c->sv, c->st); });
+    pick(m.bsJac, bs, [&] {
+        launchJacobian(m, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, nullptr, c->derr, c->st);
+    });
+    // the Arnoldi operator in the precision of the inner iteration
+    const int v32 = c->mixed ? 1 : 0;
+    void* x = v32 ? (void*)c->r32 : (void*)c->t1;
+    void* y = v32 ? (void*)c->r32b : (void*)c->t2;
+    if (x && y && c->precond == 0) {
+        launchZero((size_t)5 * m.nC, v32, x, c->st);
+        pick(m.bsOp, bs, [&] {
+            if (!(c->fuse && launchSpmvLusgs(m, c->mesh->lev, c->blkFp32, v32, c->blk, c->dg, c->rD, x, nullptr,
+                                             nullptr, y, c->st))) {
+                launchSpmv(m, c->blkFp32, v32, c->blk, c->dg, x, nullptr, nullptr, y, c->st);
+                launchLusgs(m, c->mesh->lev, c->blkFp32, v32, c->blk, c->rD, y, c->st);
+            }
+        });
+        const void* V = v32 ? (const void*)c->V32 : (const void*)c->V;
+        const int nv = v32 ? c->innerDirs : c->nDirs;
+        if (c->orth == 1 && V && nv > 0)
+            pick(m.mdCh, std::vector<int>{8, 16, 32},
+                 [&] { launchProjectionPasses(m, v32, m.mdCh, x, V, nv, c->wk.part, c->st); });
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (terr != hipSuccess)
+        return fail(ICSF_ERR_HIP, std::string("autotune: ") + twhere + ": " + hipGetErrorString(terr));
+    c->tuned = true;
+    return ICSF_OK;
+}
+
 int pseudoStep(icsf_case* c, int& nIterOut) {
     const MeshView& m = c->mv;
     c->corr++;
@@ -482,6 +570,7 @@ int pseudoStep(icsf_case* c, int& nIterOut) {
     }
     RC(residualStage(c));
     RC(jacobianStage(c));
+    if (c->autotune && !c->tuned) RC(autotune(c));
     int nIter = 0;
     RC(solveStage(c, c->sv.W, c->sv.b, nIter));
     nIterOut = nIter;
@@ -668,6 +757,12 @@ struct icsf_hb {
     HBBlocks hbb{};
     double *Wall = nullptr, *ball = nullptr, *r = nullptr, *r2 = nullptr, *V = nullptr, *dW = nullptr;
     double *t1 = nullptr, *t2 = nullptr, *rD = nullptr, *dres = nullptr;
+    // mixed precision: FP64 outer FGMRES (Z), FP32 inner GMRES over all
+    // instances (r32, r32b, V32, g2)
+    int solver = 0, mixed = 0, innerDirs = 30;
+    double *Z = nullptr;
+    float *r32 = nullptr, *r32b = nullptr, *V32 = nullptr;
+    GmresScalars* g2 = nullptr;
     int* derr = nullptr;
     HBInst* dInst = nullptr;  // per-instance views, blockIdx.y of the batched launches
     GmresScalars* g = nullptr;
@@ -694,7 +789,7 @@ struct icsf_hb {
 namespace {
 
 struct HBDiag {
-    double* dg[HB_MAXO];
+    void* dg[HB_MAXO];
 };
 
 // Instance J = blockIdx.y: out_J = b_J + source_J with
@@ -718,24 +813,27 @@ __global__ void k_hb_source(int nC, int nO, const HBInst* __restrict__ I, const 
 
 // V D_JJ on the instance diagonals, then lusgs::lusgs over all 2 nO scalar
 // and nO vector diagonals in coupledMatrix order (lusgs.C:50-125).
+// FP32 storage (mixed precision): the sum is formed in FP64 and rounded.
+template <typename TB>
 __global__ void k_hb_diag(int nC, int nO, const double* __restrict__ D, const double* __restrict__ Vol, HBDiag h,
                           double* __restrict__ rD, int* err) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nC) return;
     for (int J = 0; J < nO; ++J) {
         const double VD = Vol[c] * D[J * nO + J];
-        double* dg = h.dg[J];
-        for (int e = 0; e < 25; e += 6) dg[diagOff(c, e)] += VD;
+        TB* dg = (TB*)h.dg[J];
+        for (int e = 0; e < 25; e += 6) dg[diagOff(c, e)] = (TB)((double)dg[diagOff(c, e)] + VD);
     }
+    auto d = [&](int J, int e) { return fabs((double)((const TB*)h.dg[J])[diagOff(c, e)]); };
     double r = kGREAT;
     for (int J = 0; J < nO; ++J) {
-        r = 1.0 / fmax(1.0 / r, fabs(h.dg[J][diagOff(c, 0)]));
-        r = 1.0 / fmax(1.0 / r, fabs(h.dg[J][diagOff(c, 6)]));
+        r = 1.0 / fmax(1.0 / r, d(J, 0));
+        r = 1.0 / fmax(1.0 / r, d(J, 6));
     }
     for (int J = 0; J < nO; ++J) {
-        r = 1.0 / fmax(1.0 / r, fabs(h.dg[J][diagOff(c, 12)]));
-        r = 1.0 / fmax(1.0 / r, fabs(h.dg[J][diagOff(c, 18)]));
-        r = 1.0 / fmax(1.0 / r, fabs(h.dg[J][diagOff(c, 24)]));
+        r = 1.0 / fmax(1.0 / r, d(J, 12));
+        r = 1.0 / fmax(1.0 / r, d(J, 18));
+        r = 1.0 / fmax(1.0 / r, d(J, 24));
     }
     rD[c] = r;
     if (!(r >= kVSMALL)) atomicOr(err, 1);
@@ -765,8 +863,8 @@ __global__ void k_copyn(int n, const double* a, double* b) {
     if (i < n) b[i] = a[i];
 }
 
-void hbLusgs(icsf_hb* h, double* v) {
-    launchLusgsHB(h->inst[0]->mv, h->mesh->lev, h->hbb, h->rD, v, h->st);
+void hbLusgs(icsf_hb* h, void* v, int vecFp32 = 0) {
+    launchLusgsHB(h->inst[0]->mv, h->mesh->lev, h->hbb, vecFp32, h->rD, v, h->st);
 }
 
 int hbUploadBC(icsf_hb* h) {
@@ -788,11 +886,12 @@ int hbResidualStage(icsf_hb* h) {
 
 int hbJacobianStage(icsf_hb* h) {
     const MeshView& m = h->inst[0]->mv;
-    launchJacobianHB(h->dInst, h->nO, m, h->inst[0]->ph, h->st);
+    launchJacobianHB(h->dInst, h->nO, h->hbb.fp32, m, h->inst[0]->ph, h->st);
     for (icsf_case* c : h->inst) c->assembled = true;
     HBDiag hd{};
-    for (int J = 0; J < h->nO; ++J) hd.dg[J] = (double*)h->inst[J]->dg;
-    k_hb_diag<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, h->nO, h->dD, m.V, hd, h->rD, h->derr);
+    for (int J = 0; J < h->nO; ++J) hd.dg[J] = h->inst[J]->dg;
+    if (h->hbb.fp32) k_hb_diag<float><<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, h->nO, h->dD, m.V, hd, h->rD, h->derr);
+    else k_hb_diag<double><<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, h->nO, h->dD, m.V, hd, h->rD, h->derr);
     h->assembled = true;
     return ICSF_OK;
 }
@@ -808,7 +907,7 @@ int hbSolve(icsf_hb* h, const double* b, int& nIter) {
     const MeshView& m = h->inst[0]->mv;
     const size_t vs = (size_t)5 * m.nC, vA = vs * h->nO;
     for (int J = 0; J < h->nO; ++J) sliceSubAverage(m, h->wk, h->gJ[J], h->Wall + J * vs, h->t1 + J * vs, h->st);
-    launchSpmvHB(m, h->hbb, h->t1, nullptr, nullptr, h->t2, h->st);
+    launchSpmvHB(m, h->hbb, 0, h->t1, nullptr, nullptr, h->t2, h->st);
     for (int J = 0; J < h->nO; ++J) sliceNormFactors(m, h->wk, h->gJ[J], h->t2 + J * vs, b + J * vs, h->st);
     launchZero(vA, 0, h->dW, h->st);
     launchCopy(vA, 0, b, h->r, h->st);
@@ -818,17 +917,77 @@ int hbSolve(icsf_hb* h, const double* b, int& nIter) {
                     h->r, h->r2, h->V};
         a.nVec = m.nC * h->nO;
         a.op = [h, &m](const void* x, const double* scal, void* vout, void* y) {
-            launchSpmvHB(m, h->hbb, (const double*)x, scal, (double*)vout, (double*)y, h->st);
+            launchSpmvHB(m, h->hbb, 0, x, scal, vout, y, h->st);
         };
-        a.pre = [h](void* v) { hbLusgs(h, (double*)v); };
+        a.pre = [h](void* v) { hbLusgs(h, v); };
         if (h->fuse)
             a.opPre = [h, &m](const void* x, const double* scal, void* vout, void* y) {
-                return launchSpmvLusgsHB(m, h->mesh->lev, h->hbb, h->rD, (const double*)x, scal, (double*)vout,
-                                         (double*)y, h->st);
+                return launchSpmvLusgsHB(m, h->mesh->lev, h->hbb, 0, h->rD, x, scal, vout, y, h->st);
             };
         gmresCycle(a, 0, h->st);
         launchUpdateSolution((size_t)m.nC * h->nO, h->dW, h->V, vA, h->nDirs, h->g, h->st);
-        launchSpmvHB(m, h->hbb, h->dW, nullptr, nullptr, h->r, h->st);
+        launchSpmvHB(m, h->hbb, 0, h->dW, nullptr, nullptr, h->r, h->st);
+        for (int J = 0; J < h->nO; ++J) sliceFinalResidual(m, h->wk, h->gJ[J], b + J * vs, h->r + J * vs, h->st);
+        ++nIter;
+        if (nIter < h->minIter) continue;
+        if (nIter >= h->maxIter) break;
+        std::vector<double> res;
+        hbReadResiduals(h, res);
+        double mx = -1e300, rel = -1e300;
+        for (int J = 0; J < h->nO; ++J) {
+            const double* q = &res[10 * (size_t)J];
+            for (int k = 5; k < 10; ++k) mx = std::max(mx, q[k]);
+            for (int k = 0; k < 5; ++k) {
+                if (k >= 2 && (k - 2) == m.emptyDir) continue;
+                rel = std::max(rel, q[5 + k] / (q[k] + kROOTVSMALL));
+            }
+        }
+        if (mx < h->tol || rel < h->relTol) break;
+    }
+    if (m.emptyDir >= 0)
+        for (int J = 0; J < h->nO; ++J) launchZeroComponent(m.nC, 2 + m.emptyDir, 0, h->dW + J * vs, h->st);
+    return ICSF_OK;
+}
+
+// Mixed precision (precision mixed, solver FGMRES): outer FGMRES on the FP64
+// coupled vectors with the FP32-stored HB operator, each preconditioner
+// application an inner left-preconditioned GMRES(innerDirections) in FP32
+// over all instances (FP32 HB SpMV, shared-diagonal LU-SGS, fused when the
+// numbering allows), as the single-instance mixed form.
+int hbSolveMixed(icsf_hb* h, const double* b, int& nIter) {
+    const MeshView& m = h->inst[0]->mv;
+    const size_t vs = (size_t)5 * m.nC, vA = vs * h->nO;
+    const int nVec = m.nC * h->nO;
+    for (int J = 0; J < h->nO; ++J) sliceSubAverage(m, h->wk, h->gJ[J], h->Wall + J * vs, h->t1 + J * vs, h->st);
+    launchSpmvHB(m, h->hbb, 0, h->t1, nullptr, nullptr, h->t2, h->st);
+    for (int J = 0; J < h->nO; ++J) sliceNormFactors(m, h->wk, h->gJ[J], h->t2 + J * vs, b + J * vs, h->st);
+    launchZero(vA, 0, h->dW, h->st);
+    launchCopy(vA, 0, b, h->r, h->st);
+    auto precond = [h, &m, vA, nVec](const double* v, double* z) {
+        launchConvert(vA, 0, v, h->r32, h->st);
+        GmresArgs a{&m, &h->mesh->lev, &h->wk, h->g2, h->innerDirs, 0, h->orth, 1, nullptr, nullptr, h->rD,
+                    nullptr, h->r32, h->r32b, h->V32};
+        a.nVec = nVec;
+        a.op = [h, &m](const void* x, const double* scal, void* vout, void* y) {
+            launchSpmvHB(m, h->hbb, 1, x, scal, vout, y, h->st);
+        };
+        a.pre = [h](void* v) { hbLusgs(h, v, 1); };
+        if (h->fuse)
+            a.opPre = [h, &m](const void* x, const double* scal, void* vout, void* y) {
+                return launchSpmvLusgsHB(m, h->mesh->lev, h->hbb, 1, h->rD, x, scal, vout, y, h->st);
+            };
+        gmresCycle(a, 1, h->st);
+        launchCombine(nVec, 1, h->V32, h->innerDirs, h->g2, h->r32, h->st);
+        launchConvert(vA, 1, h->r32, z, h->st);
+    };
+    nIter = 0;
+    for (;;) {
+        FgmresArgs a{&m, &h->wk, h->g, h->nDirs, h->orth, 1, nullptr, nullptr, h->r, h->V, h->Z, precond, nullptr};
+        a.nVec = nVec;
+        a.op = [h, &m](const double* z, double* y) { launchSpmvHB(m, h->hbb, 0, z, nullptr, nullptr, y, h->st); };
+        fgmresCycle(a, h->st);
+        launchUpdateSolution((size_t)nVec, h->dW, h->Z, vA, h->nDirs, h->g, h->st);
+        launchSpmvHB(m, h->hbb, 0, h->dW, nullptr, nullptr, h->r, h->st);
         for (int J = 0; J < h->nO; ++J) sliceFinalResidual(m, h->wk, h->gJ[J], b + J * vs, h->r + J * vs, h->st);
         ++nIter;
         if (nIter < h->minIter) continue;
@@ -867,7 +1026,8 @@ int hbPseudoStep(icsf_hb* h) {
     RC(hbResidualStage(h));
     RC(hbJacobianStage(h));
     int nIter = 0;
-    RC(hbSolve(h, h->ball, nIter));
+    if (h->solver == 1) RC(hbSolveMixed(h, h->ball, nIter));
+    else RC(hbSolve(h, h->ball, nIter));
     for (int J = 0; J < h->nO; ++J) k_copy5<<<1, 64, 0, h->st>>>(gmresResidualsDevice(h->gJ[J]), h->dres + 5 * J);
     h->haveInit = true;
     for (int J = 0; J < h->nO; ++J) {
@@ -1216,8 +1376,12 @@ int createCase(icsf_mesh* mesh, const char* thermo, const char* schemes, const c
     const size_t eb = C->blkFp32 ? 4 : 8;
     {
         char* p;
-        // sliced-ELL blocks, then the interface blocks of the cyclic faces
-        const size_t nBlk = slots * NB + (size_t)mesh->view.nCyc * NB;
+        // sliced-ELL blocks (or their generators), then the interface blocks
+        // of the cyclic faces
+        C->mv.blkGen = C->blkGen ? 1 : 0;
+        C->mv.gamma = C->ph.gamma;
+        C->mv.cycBase = slots * (C->blkGen ? NGEN : NB);
+        const size_t nBlk = C->mv.cycBase + (size_t)mesh->view.nCyc * NB;
         RC(B.alloc(p, nBlk * eb)); C->blk = p;
         HIPCHK(hipMemset(C->blk, 0, nBlk * eb));
         RC(B.alloc(p, dgN * eb)); C->dg = p;
@@ -1422,7 +1586,18 @@ int icsf_get_jacobian(icsf_case* c, double* diag, double* upper, double* lower) 
     const size_t eb = c->blkFp32 ? 4 : 8;
     std::vector<char> hb(slots * NB * eb), hd(dgN * eb);
     HIPCHK(hipStreamSynchronize(c->st));
-    HIPCHK(hipMemcpy(hb.data(), c->blk, hb.size(), hipMemcpyDeviceToHost));
+    if (c->mv.blkGen) {
+        // generator storage: expand the blocks on the device
+        void* tmp = nullptr;
+        HIPCHK(hipMalloc(&tmp, hb.size()));
+        launchExpandBlocks(c->mv, c->blkFp32, c->blk, tmp, c->st);
+        const hipError_t e1 = hipStreamSynchronize(c->st);
+        const hipError_t e2 = e1 == hipSuccess ? hipMemcpy(hb.data(), tmp, hb.size(), hipMemcpyDeviceToHost) : e1;
+        (void)hipFree(tmp);
+        HIPCHK(e2);
+    } else {
+        HIPCHK(hipMemcpy(hb.data(), c->blk, hb.size(), hipMemcpyDeviceToHost));
+    }
     HIPCHK(hipMemcpy(hd.data(), c->dg, hd.size(), hipMemcpyDeviceToHost));
     auto get = [&](const std::vector<char>& v, size_t i) -> double {
         return c->blkFp32 ? (double)((const float*)v.data())[i] : ((const double*)v.data())[i];
@@ -1436,6 +1611,14 @@ int icsf_get_jacobian(icsf_case* c, double* diag, double* upper, double* lower) 
             if (upper) upper[(size_t)f * NB + e] = get(hb, sellOff(M->sell.slotUpper[f], e));
             if (lower) lower[(size_t)f * NB + e] = get(hb, sellOff(M->sell.slotLower[f], e));
         }
+    return ICSF_OK;
+}
+
+int icsf_get_tuning(icsf_case* c, int* params) {
+    if (!c || !params) return fail(ICSF_ERR_ARG, "icsf_get_tuning: NULL argument");
+    const MeshView& m = c->mv;
+    params[0] = m.bsOp; params[1] = m.bsFace; params[2] = m.bsCell; params[3] = m.bsJac; params[4] = m.mdCh;
+    params[5] = c->tuned ? 1 : 0;
     return ICSF_OK;
 }
 
@@ -1463,6 +1646,9 @@ int icsf_get_interface_blocks(icsf_case* c, double* blocks) {
 
 int icsf_set_jacobian(icsf_case* c, const double* diag, const double* upper, const double* lower) {
     if (!c || !diag || !upper || !lower) return fail(ICSF_ERR_ARG, "icsf_set_jacobian: NULL argument");
+    if (c->mv.blkGen)
+        return fail(ICSF_ERR_STATE, "icsf_set_jacobian: the case stores its off-diagonal blocks as generators; "
+                                    "create it with jacobianStorage blocks to supply blocks");
     const icsf_mesh* M = c->mesh;
     const size_t slots = (size_t)M->sell.slicePtr.back() * SLICE;
     const size_t dgN = (size_t)M->sell.nSlices * SLICE * NB;
@@ -1641,9 +1827,12 @@ int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, con
         H->inst.push_back(c);
     }
     const icsf_case* c0 = H->inst[0];
-    if (c0->solver != 0 || c0->precond != 0 || c0->mixed || c0->blkFp32)
-        return fail(ICSF_ERR_KEYWORD, "harmonic balance: flowSolver must be GMRES with preconditioner LUSGS in "
-                                      "double precision");
+    const bool fp64Gmres = c0->solver == 0 && !c0->mixed && !c0->blkFp32;
+    const bool mixedFgmres = c0->solver == 1 && c0->mixed && c0->blkFp32;
+    if (c0->precond != 0 || !(fp64Gmres || mixedFgmres))
+        return fail(ICSF_ERR_KEYWORD, "harmonic balance: flowSolver must be GMRES in double precision or FGMRES "
+                                      "with precision mixed, preconditioner LUSGS");
+    H->solver = c0->solver; H->mixed = c0->mixed; H->innerDirs = c0->innerDirs;
     H->nDirs = c0->nDirs; H->maxIter = c0->maxIter; H->minIter = c0->minIter; H->orth = c0->orth;
     H->fuse = c0->fuse;
     H->useGraph = c0->useGraph;
@@ -1654,6 +1843,12 @@ int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, con
     RC(B.alloc(H->Wall, vA)); RC(B.alloc(H->ball, vA)); RC(B.alloc(H->r, vA)); RC(B.alloc(H->r2, vA));
     RC(B.alloc(H->dW, vA)); RC(B.alloc(H->t1, vA)); RC(B.alloc(H->t2, vA));
     RC(B.alloc(H->V, vA * (size_t)(H->nDirs + 1)));
+    if (H->mixed) {
+        RC(B.alloc(H->Z, vA * (size_t)H->nDirs));
+        RC(B.alloc(H->r32, vA));
+        RC(B.alloc(H->r32b, vA));
+        RC(B.alloc(H->V32, vA * (size_t)(H->innerDirs + 1)));
+    }
     RC(B.alloc(H->rD, nC));
     RC(B.alloc(H->dres, 10 * (size_t)HB_MAXO));
     RC(B.alloc(H->derr, 1));
@@ -1663,6 +1858,8 @@ int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, con
         char* p;
         RC(B.alloc(p, gmresScalarsBytes())); H->g = (GmresScalars*)p;
         HIPCHK(hipMemset(H->g, 0, gmresScalarsBytes()));
+        RC(B.alloc(p, gmresScalarsBytes())); H->g2 = (GmresScalars*)p;
+        HIPCHK(hipMemset(H->g2, 0, gmresScalarsBytes()));
         for (int J = 0; J < nInstants; ++J) {
             RC(B.alloc(p, gmresScalarsBytes()));
             HIPCHK(hipMemset(p, 0, gmresScalarsBytes()));
@@ -1670,13 +1867,14 @@ int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, con
         }
     }
     H->hbb.nO = nInstants;
+    H->hbb.fp32 = c0->blkFp32;
     H->hbb.D = H->dD;
     std::vector<HBInst> hi(nInstants);
     for (int J = 0; J < nInstants; ++J) {
         icsf_case* c = H->inst[J];
-        H->hbb.blk[J] = (const double*)c->blk;
-        H->hbb.dg[J] = (const double*)c->dg;
-        hi[J] = HBInst{c->mv, c->sv, (double*)c->blk, (double*)c->dg, c->rD, c->derr};
+        H->hbb.blk[J] = c->blk;
+        H->hbb.dg[J] = c->dg;
+        hi[J] = HBInst{c->mv, c->sv, c->blk, c->dg, c->rD, c->derr};
     }
     RC(B.alloc(H->dInst, (size_t)nInstants));
     HIPCHK(hipMemcpy(H->dInst, hi.data(), sizeof(HBInst) * nInstants, hipMemcpyHostToDevice));
@@ -1739,7 +1937,7 @@ int icsf_hb_matrix_mul(icsf_hb* h, const double* x, double* y) {
     if (!h->assembled) return fail(ICSF_ERR_STATE, "HB Jacobian not assembled");
     const size_t vA = 5 * (size_t)h->inst[0]->mv.nC * h->nO;
     HIPCHK(hipMemcpy(h->t1, x, vA * sizeof(double), hipMemcpyHostToDevice));
-    launchSpmvHB(h->inst[0]->mv, h->hbb, h->t1, nullptr, nullptr, h->t2, h->st);
+    launchSpmvHB(h->inst[0]->mv, h->hbb, 0, h->t1, nullptr, nullptr, h->t2, h->st);
     HIPCHK(hipStreamSynchronize(h->st));
     HIPCHK(hipMemcpy(y, h->t2, vA * sizeof(double), hipMemcpyDeviceToHost));
     return ICSF_OK;
@@ -1763,7 +1961,8 @@ int icsf_hb_solve_for_incr(icsf_hb* h, const double* b, double* dW, double* resi
     for (int J = 0; J < h->nO; ++J) launchCopy(vs, 0, h->inst[J]->sv.W, h->Wall + J * vs, h->st);
     HIPCHK(hipMemcpy(h->ball, b, vA * sizeof(double), hipMemcpyHostToDevice));
     int it = 0;
-    RC(hbSolve(h, h->ball, it));
+    if (h->solver == 1) RC(hbSolveMixed(h, h->ball, it));
+    else RC(hbSolve(h, h->ball, it));
     HIPCHK(hipStreamSynchronize(h->st));
     HIPCHK(hipMemcpy(dW, h->dW, vA * sizeof(double), hipMemcpyDeviceToHost));
     if (residuals) {

@@ -59,6 +59,17 @@ struct MeshView {
     const double* cycDelta;  // [nCyc][3] cyclicFvPatch::delta()
     const int* cycRowOff;    // [nC+1] cell -> its cyclic faces
     const int* cycRowK;      // [nCyc]
+    // launch parameters (host side; chosen by the autotuner, else these
+    // defaults): workgroup sizes of the Arnoldi-operator row kernels (SpMV,
+    // LU-SGS), the face-terms kernel, the gradient and cell-residual
+    // kernels and the Jacobian rows; basis vectors per CGS projection pass.
+    // None of them changes a result bit: row kernels compute each row in
+    // one thread, and the projection sums are per vector.
+    int bsOp = 256, bsFace = 256, bsCell = 256, bsJac = 256, mdCh = 16;
+    // off-diagonal blocks of the internal faces stored as generators (1) or
+    // as full 5x5 blocks (0); gamma for generating them
+    int blkGen = 1;
+    double gamma = 1.4;
 };
 
 struct Phys {
@@ -119,6 +130,14 @@ __device__ __forceinline__ int xcdBlock() {
 // col[]) and block entry e.
 __host__ __device__ __forceinline__ size_t sellOff(size_t slot, int e) {
     return ((slot >> 6) * NB + (size_t)e) * 64 + (slot & 63);
+}
+// Generator storage of an off-diagonal LF block (NGEN values per slot,
+// same sliced layout): Ux, Uy, Uz, E of the face side the block belongs
+// to, the unit normal, the signed scale h (+|S|/2 upper, -|S|/2 lower) and
+// the diagonal shifts v = |S| lambda / 2 and sv (viscous).
+constexpr int NGEN = 10;
+__host__ __device__ __forceinline__ size_t genOff(size_t slot, int e) {
+    return ((slot >> 6) * NGEN + (size_t)e) * 64 + (slot & 63);
 }
 // Diagonal block of row r, entry e.
 __host__ __device__ __forceinline__ size_t diagOff(int r, int e) {

@@ -125,6 +125,14 @@ class dbnsCase:
         check(lib().icsf_get_jacobian(self._h, ptr(d), ptr(u), ptr(l)))
         return d, u, l
 
+    def tuning(self) -> dict:
+        """Launch parameters in use (chosen by the autotuner on the first
+        pseudo-step unless `autotune false`)."""
+        p = np.zeros(6, np.int32)
+        check(lib().icsf_get_tuning(self._h, ptr(p)))
+        return dict(op_workgroup=int(p[0]), face_workgroup=int(p[1]), cell_workgroup=int(p[2]),
+                    jacobian_workgroup=int(p[3]), projection_chunk=int(p[4]), tuned=bool(p[5]))
+
     def interfaces(self):
         """interfacesUpper blocks of the boundary faces [nB, 5, 5] (non-zero
         on cyclic faces only)."""

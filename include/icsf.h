@@ -223,6 +223,15 @@ int icsf_get_jacobian(icsf_case* c, double* diag, double* upper, double* lower);
  * vector with its momentum part transformed by forwardT; zero on the
  * other boundary faces. */
 int icsf_get_interface_blocks(icsf_case* c, double* blocks);
+
+/* Launch parameters in use (no counterpart in the reference): params[0..5]
+ * = workgroup size of the Arnoldi-operator row kernels (SpMV, LU-SGS),
+ * of the face-terms kernel, of the gradient and cell-residual kernels, of
+ * the Jacobian rows; basis vectors per CGS projection pass; 1 if the
+ * autotuner has run (fvSolution keyword autotune, default true: the first
+ * pseudo-step times each candidate on the case's own data and keeps the
+ * fastest; results are the same bits for every choice). */
+int icsf_get_tuning(icsf_case* c, int* params);
 /* Replace the assembled matrix by given LDU blocks (same layout). */
 int icsf_set_jacobian(icsf_case* c, const double* diag, const double* upper, const double* lower);
 

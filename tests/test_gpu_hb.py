@@ -140,3 +140,26 @@ def test_hb_graph_replay_equals_eager():
     np.testing.assert_array_equal(gg.pseudo_steps(6), ge.pseudo_steps(6))
     for J in range(len(insts)):
         np.testing.assert_array_equal(gg.fields(J)["W"], ge.fields(J)["W"])
+
+
+def test_hb_mixed_precision_vs_fp64_oracle():
+    """C3's solver form: outer FGMRES on the FP64 coupled vectors, FP32 HB
+    Jacobian and an FP32 inner GMRES(30) + LU-SGS over all instances.  After
+    4 pseudo-steps every instance stays within 1e-3 of the FP64 oracle and
+    the nonlinear residuals fall at the FP64 pace."""
+    from oracle.oracle import OracleHB
+    print("[hb mixed]", flush=True)
+    insts, D = _hb_cases(1, "ROE")
+    mixed = [dataclasses.replace(c, fvSolution=cases.solution_dict(
+        co=20.0, solver="FGMRES", nDirections=1, innerDirections=30, precision="mixed",
+        orthogonalisation="CGS")) for c in insts]
+    gm = hbCase.from_cases(mixed, D)
+    o = OracleHB(insts, D)
+    hm = gm.pseudo_steps(4)
+    ho = o.pseudo_steps(4)
+    for J in range(len(insts)):
+        fm, fo = gm.fields(J), o.fields(J)
+        errs = {k: rel_err(fm[k], fo[k]) for k in ("rho", "U", "T", "p")}
+        print(f"instant {J}: mixed vs FP64 oracle after 4 steps {errs}")
+        assert max(errs.values()) < 1e-3
+    assert np.all(hm[-1][:, :2] < 1.5 * ho[-1][:, :2])
