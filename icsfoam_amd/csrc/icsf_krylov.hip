@@ -166,8 +166,8 @@ __device__ __forceinline__ void spmvRow(const MeshView& m, const TB* __restrict_
     }
 }
 
-template <typename TB, typename TV, bool GEN>
-__global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__ blk, const TB* __restrict__ dg,
+template <typename TB, typename TV, bool GEN, int W = 1>
+__global__ void __launch_bounds__(256, W) k_spmv(MeshView m, const TB* __restrict__ blk, const TB* __restrict__ dg,
                                               const TV* __restrict__ x, const double* __restrict__ scal,
                                               TV* __restrict__ vout, TV* __restrict__ y) {
     const int c = xcdBlock() * blockDim.x + threadIdx.x;
@@ -176,8 +176,8 @@ __global__ void __launch_bounds__(256) k_spmv(MeshView m, const TB* __restrict__
 }
 
 // SpMV of the rows in a cell list; FWD: fused forward sweep (see spmvRow).
-template <typename TB, typename TV, bool FWD, bool GEN>
-__global__ void __launch_bounds__(256) k_spmv_cells(MeshView m, const TB* __restrict__ blk,
+template <typename TB, typename TV, bool FWD, bool GEN, int W = 1>
+__global__ void __launch_bounds__(256, W) k_spmv_cells(MeshView m, const TB* __restrict__ blk,
                                                     const TB* __restrict__ dg, const TV* __restrict__ x,
                                                     const double* __restrict__ scal, TV* __restrict__ vout,
                                                     TV* __restrict__ y, const double* __restrict__ rD,
@@ -399,8 +399,8 @@ __device__ __forceinline__ void lusgsBwdRow(const MeshView& m, const TB* __restr
     for (int i = 0; i < 5; ++i) r[(size_t)i * nC + c] = rd * acc[i];
 }
 
-template <typename TB, typename TV, bool GEN>
-__global__ void __launch_bounds__(256) k_lusgs_fwd(MeshView m, const TB* __restrict__ blk,
+template <typename TB, typename TV, bool GEN, int W = 1>
+__global__ void __launch_bounds__(256, W) k_lusgs_fwd(MeshView m, const TB* __restrict__ blk,
                                                    const double* __restrict__ rD, TV* __restrict__ r,
                                                    const int* __restrict__ cells, int n) {
     const int t = xcdBlock() * blockDim.x + threadIdx.x;
@@ -408,8 +408,8 @@ __global__ void __launch_bounds__(256) k_lusgs_fwd(MeshView m, const TB* __restr
     lusgsFwdRow<TB, TV, GEN>(m, blk, rD, r, cells[t]);
 }
 
-template <typename TB, typename TV, bool GEN>
-__global__ void __launch_bounds__(256) k_lusgs_bwd(MeshView m, const TB* __restrict__ blk,
+template <typename TB, typename TV, bool GEN, int W = 1>
+__global__ void __launch_bounds__(256, W) k_lusgs_bwd(MeshView m, const TB* __restrict__ blk,
                                                    const double* __restrict__ rD, TV* __restrict__ r,
                                                    const int* __restrict__ cells, int n) {
     const int t = xcdBlock() * blockDim.x + threadIdx.x;
@@ -876,12 +876,13 @@ template <typename TB, typename TV>
 void spmv(const MeshView& m, const void* blk, const void* dg, const void* x, const double* scal, void* vout, void* y,
           hipStream_t st) {
     const dim3 g(xcdGrid(m.nC, m.bsOp));
-    if (m.blkGen)
-        k_spmv<TB, TV, true><<<g, m.bsOp, 0, st>>>(m, (const TB*)blk, (const TB*)dg, (const TV*)x, scal, (TV*)vout,
-                                                   (TV*)y);
+    const TB *B = (const TB*)blk, *G = (const TB*)dg;
+    if (m.blkGen && m.occOp)
+        k_spmv<TB, TV, true, 4><<<g, m.bsOp, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout, (TV*)y);
+    else if (m.blkGen)
+        k_spmv<TB, TV, true><<<g, m.bsOp, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout, (TV*)y);
     else
-        k_spmv<TB, TV, false><<<g, m.bsOp, 0, st>>>(m, (const TB*)blk, (const TB*)dg, (const TV*)x, scal, (TV*)vout,
-                                                    (TV*)y);
+        k_spmv<TB, TV, false><<<g, m.bsOp, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout, (TV*)y);
 }
 
 void launchSpmv(const MeshView& m, int blkFp32, int vecFp32, const void* blk, const void* dg, const void* x,
@@ -892,26 +893,27 @@ void launchSpmv(const MeshView& m, int blkFp32, int vecFp32, const void* blk, co
     else spmv<double, double>(m, blk, dg, x, scal, vout, y, st);
 }
 
-template <typename TB, typename TV, bool GEN>
+template <typename TB, typename TV, bool GEN, int W>
 static void spmvLusgsG(const MeshView& m, const Levels& L, const void* blk, const void* dg, const double* rD,
                        const void* x, const double* scal, void* vout, void* y, hipStream_t st) {
     const int n0 = L.fwdPtrHost[1], n1 = L.fwdPtrHost[2] - n0;
     const TB* B = (const TB*)blk;
     const TB* G = (const TB*)dg;
     const int bs = m.bsOp;
-    k_spmv_cells<TB, TV, false, GEN><<<xcdGrid(n0, bs), bs, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout,
-                                                                     (TV*)y, rD, L.fwdCells, n0);
-    k_spmv_cells<TB, TV, true, GEN><<<xcdGrid(n1, bs), bs, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout, (TV*)y,
-                                                                    rD, L.fwdCells + n0, n1);
+    k_spmv_cells<TB, TV, false, GEN, W><<<xcdGrid(n0, bs), bs, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout,
+                                                                        (TV*)y, rD, L.fwdCells, n0);
+    k_spmv_cells<TB, TV, true, GEN, W><<<xcdGrid(n1, bs), bs, 0, st>>>(m, B, G, (const TV*)x, scal, (TV*)vout,
+                                                                       (TV*)y, rD, L.fwdCells + n0, n1);
     const int s0 = L.bwdPtrHost[1], nbw = L.bwdPtrHost[2] - s0;
-    k_lusgs_bwd<TB, TV, GEN><<<xcdGrid(nbw, bs), bs, 0, st>>>(m, B, rD, (TV*)y, L.bwdCells + s0, nbw);
+    k_lusgs_bwd<TB, TV, GEN, W><<<xcdGrid(nbw, bs), bs, 0, st>>>(m, B, rD, (TV*)y, L.bwdCells + s0, nbw);
 }
 
 template <typename TB, typename TV>
 static void spmvLusgsT(const MeshView& m, const Levels& L, const void* blk, const void* dg, const double* rD,
                        const void* x, const double* scal, void* vout, void* y, hipStream_t st) {
-    if (m.blkGen) spmvLusgsG<TB, TV, true>(m, L, blk, dg, rD, x, scal, vout, y, st);
-    else spmvLusgsG<TB, TV, false>(m, L, blk, dg, rD, x, scal, vout, y, st);
+    if (m.blkGen && m.occOp) spmvLusgsG<TB, TV, true, 4>(m, L, blk, dg, rD, x, scal, vout, y, st);
+    else if (m.blkGen) spmvLusgsG<TB, TV, true, 1>(m, L, blk, dg, rD, x, scal, vout, y, st);
+    else spmvLusgsG<TB, TV, false, 1>(m, L, blk, dg, rD, x, scal, vout, y, st);
 }
 
 bool launchSpmvLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp32, const void* blk, const void* dg,
@@ -924,24 +926,27 @@ bool launchSpmvLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp3
     return true;
 }
 
-template <typename TB, typename TV, bool GEN>
+template <typename TB, typename TV, bool GEN, int W>
 static void lusgsG(const MeshView& m, const Levels& L, const void* blk, const double* rD, void* r, hipStream_t st) {
     const int bs = m.bsOp;
     for (int l = 0; l < L.nFwd; ++l) {
         const int s0 = L.fwdPtrHost[l], n = L.fwdPtrHost[l + 1] - s0;
         if (L.fwdSkip && l == 0) continue;  // first level has no lower neighbours
-        k_lusgs_fwd<TB, TV, GEN><<<xcdGrid(n, bs), bs, 0, st>>>(m, (const TB*)blk, rD, (TV*)r, L.fwdCells + s0, n);
+        k_lusgs_fwd<TB, TV, GEN, W><<<xcdGrid(n, bs), bs, 0, st>>>(m, (const TB*)blk, rD, (TV*)r, L.fwdCells + s0,
+                                                                   n);
     }
     for (int l = 0; l < L.nBwd; ++l) {
         const int s0 = L.bwdPtrHost[l], n = L.bwdPtrHost[l + 1] - s0;
-        k_lusgs_bwd<TB, TV, GEN><<<xcdGrid(n, bs), bs, 0, st>>>(m, (const TB*)blk, rD, (TV*)r, L.bwdCells + s0, n);
+        k_lusgs_bwd<TB, TV, GEN, W><<<xcdGrid(n, bs), bs, 0, st>>>(m, (const TB*)blk, rD, (TV*)r, L.bwdCells + s0,
+                                                                   n);
     }
 }
 
 template <typename TB, typename TV>
 static void lusgsT(const MeshView& m, const Levels& L, const void* blk, const double* rD, void* r, hipStream_t st) {
-    if (m.blkGen) lusgsG<TB, TV, true>(m, L, blk, rD, r, st);
-    else lusgsG<TB, TV, false>(m, L, blk, rD, r, st);
+    if (m.blkGen && m.occOp) lusgsG<TB, TV, true, 4>(m, L, blk, rD, r, st);
+    else if (m.blkGen) lusgsG<TB, TV, true, 1>(m, L, blk, rD, r, st);
+    else lusgsG<TB, TV, false, 1>(m, L, blk, rD, r, st);
 }
 
 void launchLusgs(const MeshView& m, const Levels& L, int blkFp32, int vecFp32, const void* blk, const double* rD,

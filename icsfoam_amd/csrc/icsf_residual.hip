@@ -244,9 +244,11 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
     for (int k = 0; k < NG; ++k) { go[3 * k] = gx[k]; go[3 * k + 1] = gy[k]; go[3 * k + 2] = gz[k]; }
 }
 
-template <int NG>
-__global__ void __launch_bounds__(256) k_grad(MeshView m, Phys ph, StateView s, double* __restrict__ gun,
-                                              double* __restrict__ limo) {
+// W: minimum waves per SIMD the register allocation must allow (1: the
+// compiler's choice; 4: at most 128 VGPRs); the autotuner picks.
+template <int NG, int W = 1>
+__global__ void __launch_bounds__(256, W) k_grad(MeshView m, Phys ph, StateView s, double* __restrict__ gun,
+                                                 double* __restrict__ limo) {
     gradBody<NG>(m, ph, s, gun, limo);
 }
 template <int NG>
@@ -441,7 +443,8 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
     }
 }
 
-__global__ void __launch_bounds__(256) k_face_terms(MeshView m, Phys ph, StateView s) { faceTermsBody(m, ph, s); }
+template <int W>
+__global__ void __launch_bounds__(256, W) k_face_terms(MeshView m, Phys ph, StateView s) { faceTermsBody(m, ph, s); }
 __global__ void __launch_bounds__(256) k_face_terms_hb(const HBInst* __restrict__ I, Phys ph) {
     const HBInst q = I[blockIdx.y];
     faceTermsBody(q.m, ph, q.s);
@@ -723,10 +726,10 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
     if (!(r >= kVSMALL)) atomicOr(err, 1);
 }
 
-template <typename TB>
-__global__ void __launch_bounds__(256) k_cell_jacobian(MeshView m, Phys ph, StateView s, TB* __restrict__ blk,
-                                                       TB* __restrict__ dg, double* __restrict__ rD,
-                                                       int* __restrict__ err) {
+template <typename TB, int W = 1>
+__global__ void __launch_bounds__(256, W) k_cell_jacobian(MeshView m, Phys ph, StateView s, TB* __restrict__ blk,
+                                                          TB* __restrict__ dg, double* __restrict__ rD,
+                                                          int* __restrict__ err) {
     cellJacobianBody<TB>(m, ph, s, blk, dg, rD, err);
 }
 template <typename TB>
@@ -748,10 +751,17 @@ void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const d
 void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st) {
     if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
     const int bc = m.bsCell, bf = m.bsFace;
-    if (ph.flux == FLUX_ROE) k_grad<7><<<xcdGrid(m.nC, bc), bc, 0, st>>>(m, ph, s, nullptr, nullptr);
-    else k_grad<8><<<xcdGrid(m.nC, bc), bc, 0, st>>>(m, ph, s, nullptr, nullptr);
-    k_face_terms<<<xcdGrid(m.nF, bf), bf, 0, st>>>(m, ph, s);
-    k_cell_residual<<<xcdGrid(m.nC, bc), bc, 0, st>>>(m, ph, s);
+    const dim3 gc(xcdGrid(m.nC, bc)), gf(xcdGrid(m.nF, bf));
+    if (ph.flux == FLUX_ROE) {
+        if (m.occCell) k_grad<7, 4><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
+        else k_grad<7><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
+    } else {
+        if (m.occCell) k_grad<8, 4><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
+        else k_grad<8><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
+    }
+    if (m.occFace) k_face_terms<4><<<gf, bf, 0, st>>>(m, ph, s);
+    else k_face_terms<1><<<gf, bf, 0, st>>>(m, ph, s);
+    k_cell_residual<<<gc, bc, 0, st>>>(m, ph, s);
 }
 
 void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st) {
@@ -787,10 +797,14 @@ void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void*
                     double* rD, double* dg64, int* err, hipStream_t st) {
     (void)dg64;
     const int bj = m.bsJac;
-    if (fp32)
-        k_cell_jacobian<float><<<xcdGrid(m.nC, bj), bj, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
-    else
-        k_cell_jacobian<double><<<xcdGrid(m.nC, bj), bj, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
+    const dim3 g(xcdGrid(m.nC, bj));
+    if (fp32) {
+        if (m.occJac) k_cell_jacobian<float, 4><<<g, bj, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
+        else k_cell_jacobian<float><<<g, bj, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
+    } else {
+        if (m.occJac) k_cell_jacobian<double, 4><<<g, bj, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
+        else k_cell_jacobian<double><<<g, bj, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
+    }
 }
 
 }  // namespace icsf

@@ -524,11 +524,25 @@ int autotune(icsf_case* c) {
         }
         param = best;
     };
+    // workgroup size and register budget of one kernel class, jointly
+    auto pick2 = [&](int& bsP, int& occP, const std::function<void()>& fn) {
+        int bb = bsP, bo = occP;
+        float bt = 1e30f;
+        for (int o : {0, 1})
+            for (int b : {64, 128, 256}) {
+                bsP = b;
+                occP = o;
+                const float t = timeIt(fn);
+                if (t < bt) { bt = t; bb = b; bo = o; }
+            }
+        bsP = bb;
+        occP = bo;
+    };
     MeshView& m = c->mv;
     const std::vector<int> bs{64, 128, 256};
-    pick(m.bsFace, bs, [&] { launchResidual(m, c->ph, c->sv, c->st); });
-    pick(m.bsCell, bs, [&] { launchResidual(m, c->ph, c->sv, c->st); });
-    pick(m.bsJac, bs, [&] {
+    pick2(m.bsFace, m.occFace, [&] { launchResidual(m, c->ph, c->sv, c->st); });
+    pick2(m.bsCell, m.occCell, [&] { launchResidual(m, c->ph, c->sv, c->st); });
+    pick2(m.bsJac, m.occJac, [&] {
         launchJacobian(m, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, nullptr, c->derr, c->st);
     });
     // the Arnoldi operator in the precision of the inner iteration
@@ -537,7 +551,8 @@ int autotune(icsf_case* c) {
     void* y = v32 ? (void*)c->r32b : (void*)c->t2;
     if (x && y && c->precond == 0) {
         launchZero((size_t)5 * m.nC, v32, x, c->st);
-        pick(m.bsOp, bs, [&] {
+        int occDummy = 0;
+        pick2(m.bsOp, m.blkGen ? m.occOp : occDummy, [&] {
             if (!(c->fuse && launchSpmvLusgs(m, c->mesh->lev, c->blkFp32, v32, c->blk, c->dg, c->rD, x, nullptr,
                                              nullptr, y, c->st))) {
                 launchSpmv(m, c->blkFp32, v32, c->blk, c->dg, x, nullptr, nullptr, y, c->st);
@@ -1619,6 +1634,7 @@ int icsf_get_tuning(icsf_case* c, int* params) {
     const MeshView& m = c->mv;
     params[0] = m.bsOp; params[1] = m.bsFace; params[2] = m.bsCell; params[3] = m.bsJac; params[4] = m.mdCh;
     params[5] = c->tuned ? 1 : 0;
+    params[6] = m.occOp; params[7] = m.occFace; params[8] = m.occCell; params[9] = m.occJac;
     return ICSF_OK;
 }
 

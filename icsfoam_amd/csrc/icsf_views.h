@@ -66,6 +66,9 @@ struct MeshView {
     // None of them changes a result bit: row kernels compute each row in
     // one thread, and the projection sums are per vector.
     int bsOp = 256, bsFace = 256, bsCell = 256, bsJac = 256, mdCh = 16;
+    // register budget of the same kernel classes: 0 the compiler's choice,
+    // 1 at most 128 VGPRs (4 waves per SIMD, spilling what does not fit)
+    int occOp = 0, occFace = 0, occCell = 0, occJac = 0;
     // off-diagonal blocks of the internal faces stored as generators (1) or
     // as full 5x5 blocks (0); gamma for generating them
     int blkGen = 1;

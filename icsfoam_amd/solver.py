@@ -128,10 +128,12 @@ class dbnsCase:
     def tuning(self) -> dict:
         """Launch parameters in use (chosen by the autotuner on the first
         pseudo-step unless `autotune false`)."""
-        p = np.zeros(6, np.int32)
+        p = np.zeros(10, np.int32)
         check(lib().icsf_get_tuning(self._h, ptr(p)))
         return dict(op_workgroup=int(p[0]), face_workgroup=int(p[1]), cell_workgroup=int(p[2]),
-                    jacobian_workgroup=int(p[3]), projection_chunk=int(p[4]), tuned=bool(p[5]))
+                    jacobian_workgroup=int(p[3]), projection_chunk=int(p[4]), tuned=bool(p[5]),
+                    op_vgpr_cap=bool(p[6]), face_vgpr_cap=bool(p[7]), cell_vgpr_cap=bool(p[8]),
+                    jacobian_vgpr_cap=bool(p[9]))
 
     def interfaces(self):
         """interfacesUpper blocks of the boundary faces [nB, 5, 5] (non-zero

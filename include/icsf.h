@@ -224,13 +224,15 @@ int icsf_get_jacobian(icsf_case* c, double* diag, double* upper, double* lower);
  * other boundary faces. */
 int icsf_get_interface_blocks(icsf_case* c, double* blocks);
 
-/* Launch parameters in use (no counterpart in the reference): params[0..5]
+/* Launch parameters in use (no counterpart in the reference): params[0..9]
  * = workgroup size of the Arnoldi-operator row kernels (SpMV, LU-SGS),
  * of the face-terms kernel, of the gradient and cell-residual kernels, of
  * the Jacobian rows; basis vectors per CGS projection pass; 1 if the
- * autotuner has run (fvSolution keyword autotune, default true: the first
- * pseudo-step times each candidate on the case's own data and keeps the
- * fastest; results are the same bits for every choice). */
+ * autotuner has run; then the register budget (0 the compiler's choice,
+ * 1 at most 128 VGPRs for 4 waves per SIMD) of the operator, face-terms,
+ * gradient and Jacobian kernels.  fvSolution keyword autotune (default
+ * true): the first pseudo-step times each candidate on the case's own data
+ * and keeps the fastest; results are the same bits for every choice. */
 int icsf_get_tuning(icsf_case* c, int* params);
 /* Replace the assembled matrix by given LDU blocks (same layout). */
 int icsf_set_jacobian(icsf_case* c, const double* diag, const double* upper, const double* lower);
