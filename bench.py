@@ -408,9 +408,10 @@ def main():
                    "step_algorithmic_gb": {f: round(step_bytes(nC, nI, nB, f) / 1e9, 2)
                                            for f in ("fp64", "mixed")}},
         "roofline": {"bound": "hbm",
-                     "kernel": ("fused Arnoldi operator, FP64 form: k_spmv_cells<double,double,false> + "
-                                "k_spmv_cells<double,double,true> + k_lusgs_bwd<double,double> (one group per "
-                                "Arnoldi step, timed as a group)") if fused else
+                     "kernel": ("fused Arnoldi operator, FP64 form, off-diagonal blocks as generators: "
+                                "k_spmv_cells<double,double,false,true> + k_spmv_cells<double,double,true,true> "
+                                "+ k_lusgs_bwd<double,double,true> (one group per Arnoldi step, timed as a "
+                                "group)") if fused else
                                "k_spmv<double,double> (block SpMV, FP64 form)",
                      "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK, 4), "traffic": measured_traffic(n) if fused else None,

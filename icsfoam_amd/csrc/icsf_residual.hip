@@ -263,8 +263,12 @@ __device__ __forceinline__ double soundSpeed(const Phys& ph, double T) {
 }
 
 // Convective flux of face f (internal or boundary) -> (rho, rhoE, rhoU).
+// rec (internal and cyclic faces): the reconstructed Ux, Uy, Uz, E of the
+// owner side, then of the neighbour side, for the Jacobian (the same
+// values the flux used; E reconstructed once more for AUSM+up, whose flux
+// uses c* instead).
 __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, const StateView& s, int f, double& fr,
-                                         double& fe, V3& fu) {
+                                         double& fe, V3& fu, double* rec) {
     const V3 S = ld3(m.Sf + 3 * (size_t)f);
     const double ms = m.magSf[f];
     double rL, rR, pL, pR, uxL, uxR, uyL, uyR, uzL, uzR, HL, HR, xL, xR, eL = 0, eR = 0;
@@ -279,6 +283,8 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
         reconCyclic(m, s, ph.recon, f, o, kc, F_H, HL, HR);
         reconCyclic(m, s, ph.recon, f, o, kc, ph.flux == FLUX_ROE ? F_E : F_CS, xL, xR);
         if (ph.flux == FLUX_HLLC) reconCyclic(m, s, ph.recon, f, o, kc, F_E, eL, eR);
+        else if (ph.flux == FLUX_ROE) { eL = xL; eR = xR; }
+        else reconCyclic(m, s, ph.recon, f, o, kc, F_E, eL, eR);
         gI = m.w[f] * ph.gamma + (1.0 - m.w[f]) * ph.gamma;
     } else if (f < m.nI) {
         const int o = m.own[f], n = m.nei[f];
@@ -290,6 +296,8 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
         recon(m, s, ph.recon, f, o, n, F_H, HL, HR);
         recon(m, s, ph.recon, f, o, n, ph.flux == FLUX_ROE ? F_E : F_CS, xL, xR);
         if (ph.flux == FLUX_HLLC) recon(m, s, ph.recon, f, o, n, F_E, eL, eR);
+        else if (ph.flux == FLUX_ROE) { eL = xL; eR = xR; }
+        else recon(m, s, ph.recon, f, o, n, F_E, eL, eR);
     } else {
         const double* bf = s.bfld + (size_t)(f - m.nI) * NFLD;
         rL = rR = bf[F_RHO];
@@ -302,6 +310,8 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
         eL = eR = bf[F_E];
     }
     const V3 UL{uxL, uyL, uzL}, UR{uxR, uyR, uzR};
+    rec[0] = uxL; rec[1] = uyL; rec[2] = uzL; rec[3] = eL;
+    rec[4] = uxR; rec[5] = uyR; rec[6] = uzR; rec[7] = eR;
     if (ph.flux == FLUX_AUSM)
         ausmFace(ph.lowMach, S, ms, UL, UR, xL, xR, pL, pR, rL, rR, HL, HR, fr, fu, fe);
     else if (ph.flux == FLUX_ROE)
@@ -331,7 +341,8 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
     double* o = s.ft + (size_t)f * NFT;
     double fr, fe;
     V3 fu;
-    faceFlux(m, ph, s, f, fr, fe, fu);
+    double rec[8];
+    faceFlux(m, ph, s, f, fr, fe, fu, rec);
     o[FT_CONV] = fr;
     o[FT_CONV + 1] = fe;
     o[FT_CONV + 2] = fu.x;
@@ -351,13 +362,8 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
         const double lam = cf + fabs(dot(Uf, S / ms));
         o[FT_LAM] = lam;
         o[FT_FRDT] = m.nodc[f] * lam;
-        double uxL, uxR, uyL, uyR, uzL, uzR, EL, ER;
-        recon(m, s, ph.recon, f, ow, ne, F_UX, uxL, uxR);
-        recon(m, s, ph.recon, f, ow, ne, F_UY, uyL, uyR);
-        recon(m, s, ph.recon, f, ow, ne, F_UZ, uzL, uzR);
-        recon(m, s, ph.recon, f, ow, ne, F_E, EL, ER);
-        o[FT_REC] = uxL; o[FT_REC + 1] = uyL; o[FT_REC + 2] = uzL; o[FT_REC + 3] = EL;
-        o[FT_REC + 4] = uxR; o[FT_REC + 5] = uyR; o[FT_REC + 6] = uzR; o[FT_REC + 7] = ER;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) o[FT_REC + k] = rec[k];
         if (visc) {
             const double* g9o = s.gU + (size_t)ow * 9;
             const double* g9n = s.gU + (size_t)ne * 9;
@@ -400,13 +406,8 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
         const double lam = cf + fabs(dot(Uf, S / ms));
         o[FT_LAM] = lam;
         o[FT_FRDT] = m.nodc[f] * lam;
-        double uxL, uxR, uyL, uyR, uzL, uzR, EL, ER;
-        reconCyclic(m, s, ph.recon, f, ow, kc, F_UX, uxL, uxR);
-        reconCyclic(m, s, ph.recon, f, ow, kc, F_UY, uyL, uyR);
-        reconCyclic(m, s, ph.recon, f, ow, kc, F_UZ, uzL, uzR);
-        reconCyclic(m, s, ph.recon, f, ow, kc, F_E, EL, ER);
-        o[FT_REC] = uxL; o[FT_REC + 1] = uyL; o[FT_REC + 2] = uzL; o[FT_REC + 3] = EL;
-        o[FT_REC + 4] = uxR; o[FT_REC + 5] = uyR; o[FT_REC + 6] = uzR; o[FT_REC + 7] = ER;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) o[FT_REC + k] = rec[k];
     } else {
         const int b = f - m.nI;
         const int c = m.own[f];
