@@ -514,18 +514,29 @@ __device__ __forceinline__ void blockReduceStore(double (&v)[NV], double* out) {
     }
 }
 
-// Final reduction of [nb][nv] partials with one 256-thread block; out[k].
-__global__ void k_reduce_final(const double* __restrict__ part, int nb, int nv, double* __restrict__ out) {
+// Sum of value k over nb partial rows of `stride` values, in a fixed order
+// (thread t adds rows t, t+256, ...; then the wave sums; then the four
+// waves in order).  Called by all threads of a 256-thread block; every
+// thread gets the sum.
+__device__ __forceinline__ double sumPartials(const double* __restrict__ part, int nb, int stride, int k) {
     __shared__ double sh[4];
-    for (int k = 0; k < nv; ++k) {
-        double v = 0;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) v += part[(size_t)b * nv + k];
-        v = waveSum(v);
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) out[k] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
-        __syncthreads();
-    }
+    double v = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += part[(size_t)b * stride + k];
+    v = waveSum(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double s = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    __syncthreads();
+    return s;
+}
+
+// Final reduction of [nb][nv] partials: value k = blockIdx.x (grid nv,
+// 256 threads); out[k].
+__global__ void __launch_bounds__(256) k_reduce_final(const double* __restrict__ part, int nb, int nv,
+                                                      double* __restrict__ out) {
+    const int k = blockIdx.x;
+    const double s = sumPartials(part, nb, nv, k);
+    if (threadIdx.x == 0) out[k] = s;
 }
 
 // sums of the 5 components of x (for the average) -> partials
@@ -776,25 +787,9 @@ __global__ void k_gm_beta0(GmresScalars* g, int nDirs) {
     g->bh[0] = g->beta;
 }
 
-// store the CGS projections as column i of H
-__global__ void k_gm_store_col(GmresScalars* g, int i, int j0, int nv) {
-    const int k = threadIdx.x;
-    if (k < nv) {
-        g->H[j0 + k][i] = g->red[k];
-        g->hcol[j0 + k] = g->red[k];
-    }
-}
-
-// MGS: H[j][i] from the 3 per-variable partial sums
-__global__ void k_gm_store_h(GmresScalars* g, int i, int j) {
-    if (threadIdx.x != 0) return;
-    g->H[j][i] = (0.0 + g->red[0] + g->red[1]) + g->red[2];
-}
-
-// apply previous rotations, new rotation, update bh (gmres.C:999-1027)
-__global__ void k_gm_givens(GmresScalars* g, int i, int mgsNorm) {
-    if (threadIdx.x != 0) return;
-    double beta2 = mgsNorm ? ((0.0 + g->red[0] + g->red[1]) + g->red[2]) : g->red[0];
+// apply previous rotations, new rotation, update bh (gmres.C:999-1027);
+// beta2 = ||w||^2 after the orthogonalisation
+__device__ void givensStep(GmresScalars* g, int i, double beta2) {
     const double beta = sqrt(beta2);
     for (int j = 0; j < i; ++j) {
         const double Hji = g->H[j][i];
@@ -807,6 +802,45 @@ __global__ void k_gm_givens(GmresScalars* g, int i, int mgsNorm) {
     g->bh[i + 1] = g->sn[i] * bhi + g->cs[i] * g->bh[i + 1];
     g->H[i][i] = g->cs[i] * g->H[i][i] - g->sn[i] * beta;
     g->beta = beta;
+}
+
+// CGS projections of one multidot pass (partial rows of ch values): value
+// k = blockIdx.x (grid nv) is stored as H[j0+k][i] and hcol[j0+k].
+__global__ void __launch_bounds__(256) k_reduce_col(const double* __restrict__ part, int nb, int ch,
+                                                    GmresScalars* g, int i, int j0) {
+    const int k = blockIdx.x;
+    const double s = sumPartials(part, nb, ch, k);
+    if (threadIdx.x == 0) {
+        g->red[k] = s;
+        g->H[j0 + k][i] = s;
+        g->hcol[j0 + k] = s;
+    }
+}
+
+// ||w||^2 of the CGS update pass, then the Givens step (one block).
+__global__ void __launch_bounds__(256) k_reduce_givens(const double* __restrict__ part, int nb, GmresScalars* g,
+                                                       int i) {
+    const double s = sumPartials(part, nb, 1, 0);
+    if (threadIdx.x == 0) {
+        g->red[0] = s;
+        givensStep(g, i, s);
+    }
+}
+
+// MGS step j of column i: the three per-variable partial sums of <w, V_j>
+// (j <= i, stored as H[j][i]) or of ||w||^2 (j = i+1, then the Givens
+// step), combined in the reference's order (one block).
+__global__ void __launch_bounds__(256) k_reduce_mgs(const double* __restrict__ part, int nb, GmresScalars* g, int i,
+                                                    int j) {
+    double s[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[k] = sumPartials(part, nb, 3, k);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 3; ++k) g->red[k] = s[k];
+        const double hs = (0.0 + s[0] + s[1]) + s[2];
+        if (j <= i) g->H[j][i] = hs;
+        else givensStep(g, i, hs);
+    }
 }
 
 __global__ void k_gm_backsub(GmresScalars* g, int n) {
@@ -1028,22 +1062,18 @@ void gmresCycleT(const GmresArgs& a, hipStream_t st) {
                 const TV* vnext = j <= i ? V + (size_t)j * vs : nullptr;
                 const double* hprev = j > 0 ? &g->H[j - 1][i] : nullptr;
                 k_mgs_step<TV><<<nbk, 256, 0, st>>>(nC, r, vprev, hprev, vnext, j == i + 1, a.wk->part);
-                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 3, red);
-                if (j <= i) k_gm_store_h<<<1, 64, 0, st>>>(g, i, j);
+                k_reduce_mgs<<<1, 256, 0, st>>>(a.wk->part, nbk, g, i, j);
             }
-            k_gm_givens<<<1, 64, 0, st>>>(g, i, 1);
         } else {
             // classical Gram-Schmidt: one projection pass + one update pass
             const int ch = m.mdCh;
             for (int j0 = 0; j0 <= i; j0 += ch) {
                 const int nv = (i + 1 - j0) < ch ? (i + 1 - j0) : ch;
                 launchMultidot<TV>(ch, nbk, nC, r, V, vs, j0, nv, a.wk->part, st);
-                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, ch, red);
-                k_gm_store_col<<<1, 64, 0, st>>>(g, i, j0, nv);
+                k_reduce_col<<<nv, 256, 0, st>>>(a.wk->part, nbk, ch, g, i, j0);
             }
             k_cgs_update<TV><<<nbk, 256, 0, st>>>(nC, r, V, vs, i + 1, g->hcol, a.wk->part);
-            k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 1, red);
-            k_gm_givens<<<1, 64, 0, st>>>(g, i, 0);
+            k_reduce_givens<<<1, 256, 0, st>>>(a.wk->part, nbk, g, i);
         }
         mark(TK_ORTH, 0);
     }
@@ -1075,11 +1105,11 @@ static void prepareT(const MeshView& m, Work& wk, GmresScalars* g, const void* W
     const int nC = m.nC;
     const int nbk = reduceBlocks(nC);
     k_sum5<TV><<<nbk, 256, 0, st>>>(nC, (const TV*)W, wk.part);
-    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 5, g->red);
+    k_reduce_final<<<5, 256, 0, st>>>(wk.part, nbk, 5, g->red);
     k_sub_avg<TV><<<nb(nC, 256), 256, 0, st>>>(nC, (const TV*)W, g->red, (TV*)tmp1);
     launchSpmv(m, blkFp32, sizeof(TV) == 4, blk, dg, tmp1, nullptr, nullptr, tmp2, st);
     k_normfactor<TV><<<nbk, 256, 0, st>>>(nC, (const TV*)tmp2, (const TV*)b, wk.part);
-    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 8, g->red);
+    k_reduce_final<<<8, 256, 0, st>>>(wk.part, nbk, 8, g->red);
     k_gm_init<<<1, 64, 0, st>>>(g);
 }
 
@@ -1098,7 +1128,7 @@ static void updResT(const MeshView& m, Work& wk, GmresScalars* g, int blkFp32, c
     k_update_sol<TV, TV><<<nb(vs, 256), 256, 0, st>>>(nC, (TV*)dW, (const TV*)V, vs, nDirs, g->yh);
     launchSpmv(m, blkFp32, sizeof(TV) == 4, blk, dg, dW, nullptr, nullptr, r, st);
     k_resid_l1<TV><<<nbk, 256, 0, st>>>(nC, (const TV*)b, (TV*)r, wk.part);
-    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 5, g->red);
+    k_reduce_final<<<5, 256, 0, st>>>(wk.part, nbk, 5, g->red);
     k_gm_final<<<1, 64, 0, st>>>(g, m.emptyDir);
 }
 
@@ -1229,7 +1259,7 @@ void sliceSubAverage(const MeshView& m, Work& wk, GmresScalars* g, const double*
     const int nC = m.nC;
     const int nbk = reduceBlocks(nC);
     k_sum5<double><<<nbk, 256, 0, st>>>(nC, W, wk.part);
-    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 5, g->red);
+    k_reduce_final<<<5, 256, 0, st>>>(wk.part, nbk, 5, g->red);
     k_sub_avg<double><<<nb(nC, 256), 256, 0, st>>>(nC, W, g->red, out);
 }
 
@@ -1238,7 +1268,7 @@ void sliceNormFactors(const MeshView& m, Work& wk, GmresScalars* g, const double
     const int nC = m.nC;
     const int nbk = reduceBlocks(nC);
     k_normfactor<double><<<nbk, 256, 0, st>>>(nC, t, b, wk.part);
-    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 8, g->red);
+    k_reduce_final<<<8, 256, 0, st>>>(wk.part, nbk, 8, g->red);
     k_gm_init<<<1, 64, 0, st>>>(g);
 }
 
@@ -1246,7 +1276,7 @@ void sliceFinalResidual(const MeshView& m, Work& wk, GmresScalars* g, const doub
     const int nC = m.nC;
     const int nbk = reduceBlocks(nC);
     k_resid_l1<double><<<nbk, 256, 0, st>>>(nC, b, r, wk.part);
-    k_reduce_final<<<1, 256, 0, st>>>(wk.part, nbk, 5, g->red);
+    k_reduce_final<<<5, 256, 0, st>>>(wk.part, nbk, 5, g->red);
     k_gm_final<<<1, 64, 0, st>>>(g, m.emptyDir);
 }
 
@@ -1300,21 +1330,17 @@ void fgmresCycle(const FgmresArgs& a, hipStream_t st) {
                 const double* vnext = j <= i ? V + (size_t)j * vs : nullptr;
                 const double* hprev = j > 0 ? &g->H[j - 1][i] : nullptr;
                 k_mgs_step<double><<<nbk, 256, 0, st>>>(nC, r, vprev, hprev, vnext, j == i + 1, a.wk->part);
-                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 3, red);
-                if (j <= i) k_gm_store_h<<<1, 64, 0, st>>>(g, i, j);
+                k_reduce_mgs<<<1, 256, 0, st>>>(a.wk->part, nbk, g, i, j);
             }
-            k_gm_givens<<<1, 64, 0, st>>>(g, i, 1);
         } else {
             const int ch = m.mdCh;
             for (int j0 = 0; j0 <= i; j0 += ch) {
                 const int nv = (i + 1 - j0) < ch ? (i + 1 - j0) : ch;
                 launchMultidot<double>(ch, nbk, nC, r, V, vs, j0, nv, a.wk->part, st);
-                k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, ch, red);
-                k_gm_store_col<<<1, 64, 0, st>>>(g, i, j0, nv);
+                k_reduce_col<<<nv, 256, 0, st>>>(a.wk->part, nbk, ch, g, i, j0);
             }
             k_cgs_update<double><<<nbk, 256, 0, st>>>(nC, r, V, vs, i + 1, g->hcol, a.wk->part);
-            k_reduce_final<<<1, 256, 0, st>>>(a.wk->part, nbk, 1, red);
-            k_gm_givens<<<1, 64, 0, st>>>(g, i, 0);
+            k_reduce_givens<<<1, 256, 0, st>>>(a.wk->part, nbk, g, i);
         }
         if (a.mark) a.mark(TK_ORTH, 0);
     }
