@@ -334,11 +334,10 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
 //                neighbour side (internal faces)
 //   FT_SV        0.5*lambdaVisc*|S|*deltaCoeffs (internal, viscous)
 // Cell kernels then sum these in face order (the reference's face loops).
-__device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph, const StateView& s) {
-    const int f = xcdBlock() * blockDim.x + threadIdx.x;
-    if (f >= m.nF) return;
-    if (f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY) return;
-    double* o = s.ft + (size_t)f * NFT;
+// o: the face's NFT-entry record (entries a face kind does not use are
+// left as they are).
+__device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph, const StateView& s, int f,
+                                              double* o) {
     double fr, fe;
     V3 fu;
     double rec[8];
@@ -444,11 +443,37 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
     }
 }
 
+// Face records of workgroup blk: assembled in LDS (row stride NFT + 1
+// doubles, zero where a face kind writes nothing, as the buffer holds from
+// its allocation), then stored as one contiguous range of whole records.
+// Each thread storing its own record directly issues NFT 8-byte stores at a
+// 192-byte stride across the wave, which reach memory as partial cache
+// lines, about twice the record bytes.
+__device__ __forceinline__ void faceTermsBlock(const MeshView& m, const Phys& ph, const StateView& s, int blk) {
+    extern __shared__ double shFT[];
+    const int bs = blockDim.x;
+    const int f0 = blk * bs;
+    double* o = shFT + (size_t)threadIdx.x * (NFT + 1);
+#pragma unroll
+    for (int k = 0; k < NFT; ++k) o[k] = 0.0;
+    const int f = f0 + threadIdx.x;
+    if (f < m.nF && !(f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY)) faceTermsBody(m, ph, s, f, o);
+    __syncthreads();
+    const int nRec = min(bs, m.nF - f0);
+    double* g = s.ft + (size_t)f0 * NFT;
+    for (int i = threadIdx.x; i < nRec * NFT; i += bs) {
+        const int r = i / NFT, k = i - r * NFT;
+        g[i] = shFT[r * (NFT + 1) + k];
+    }
+}
+
 template <int W>
-__global__ void __launch_bounds__(256, W) k_face_terms(MeshView m, Phys ph, StateView s) { faceTermsBody(m, ph, s); }
+__global__ void __launch_bounds__(256, W) k_face_terms(MeshView m, Phys ph, StateView s) {
+    faceTermsBlock(m, ph, s, xcdBlock());
+}
 __global__ void __launch_bounds__(256) k_face_terms_hb(const HBInst* __restrict__ I, Phys ph) {
     const HBInst q = I[blockIdx.y];
-    faceTermsBody(q.m, ph, q.s);
+    faceTermsBlock(q.m, ph, q.s, xcdBlock());
 }
 
 // Residual b = R*V (residualsUpdate.H) and rPseudoDeltaT
@@ -760,8 +785,9 @@ void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipSt
         if (m.occCell) k_grad<8, 4><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
         else k_grad<8><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
     }
-    if (m.occFace) k_face_terms<4><<<gf, bf, 0, st>>>(m, ph, s);
-    else k_face_terms<1><<<gf, bf, 0, st>>>(m, ph, s);
+    const size_t lds = (size_t)bf * (NFT + 1) * sizeof(double);
+    if (m.occFace) k_face_terms<4><<<gf, bf, lds, st>>>(m, ph, s);
+    else k_face_terms<1><<<gf, bf, lds, st>>>(m, ph, s);
     k_cell_residual<<<gc, bc, 0, st>>>(m, ph, s);
 }
 
@@ -770,7 +796,7 @@ void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys&
     const int bc = m.bsCell, bf = m.bsFace;
     if (ph.flux == FLUX_ROE) k_grad_hb<7><<<dim3(xcdGrid(m.nC, bc), nO), bc, 0, st>>>(inst, ph);
     else k_grad_hb<8><<<dim3(xcdGrid(m.nC, bc), nO), bc, 0, st>>>(inst, ph);
-    k_face_terms_hb<<<dim3(xcdGrid(m.nF, bf), nO), bf, 0, st>>>(inst, ph);
+    k_face_terms_hb<<<dim3(xcdGrid(m.nF, bf), nO), bf, (size_t)bf * (NFT + 1) * sizeof(double), st>>>(inst, ph);
     k_cell_residual_hb<<<dim3(xcdGrid(m.nC, bc), nO), bc, 0, st>>>(inst, ph);
 }
 
