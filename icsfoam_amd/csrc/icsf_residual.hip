@@ -137,33 +137,40 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
     const bool visc = ph.viscous;
     V3 gex{0, 0, 0};
     const double ecc = visc ? s.ecalc[c] : 0.0;
+    // internal faces in face order from the cell's slots (column, slot
+    // geometry), then its boundary faces from the cell-face list
     const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
-    for (int q = e0; q < e1; ++q) {
+    const int len = m.rowLen[c], nLo = m.nLower[c];
+    const size_t rs = (size_t)m.slicePtr[c >> 6] * 64 + (c & 63);
+    for (int j = 0; j < len; ++j) {
+        const size_t sk = rs + 64 * (size_t)j;
+        const int other = m.col[sk];
+        const bool isOwn = j >= nLo;
+        const int o = isOwn ? c : other, n = isOwn ? other : c;
+        const V3 S{m.sgeo[sgOff(sk, SG_SX)], m.sgeo[sgOff(sk, SG_SX + 1)], m.sgeo[sgOff(sk, SG_SX + 2)]};
+        const double w = m.sgeo[sgOff(sk, SG_W)];
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+            const double vo = s.fld[(size_t)o * NFLD + k], vn = s.fld[(size_t)n * NFLD + k];
+            const double vf = w * (vo - vn) + vn;
+            const V3 t = S * vf;
+            if (isOwn) { gx[k] = gx[k] + t.x; gy[k] = gy[k] + t.y; gz[k] = gz[k] + t.z; }
+            else { gx[k] = gx[k] - t.x; gy[k] = gy[k] - t.y; gz[k] = gz[k] - t.z; }
+            const double vOther = s.fld[(size_t)other * NFLD + k];
+            mx[k] = fmax(mx[k], vOther);
+            mn[k] = fmin(mn[k], vOther);
+        }
+        if (visc) {
+            const double eo = s.ecalc[o], en = s.ecalc[n];
+            const double vf = w * (eo - en) + en;
+            const V3 t = S * vf;
+            gex = isOwn ? gex + t : gex - t;
+        }
+    }
+    for (int q = e0 + len; q < e1; ++q) {
         const int f = m.cf[q];
         const V3 S = ld3(m.Sf + 3 * (size_t)f);
-        if (f < m.nI) {
-            const int o = m.own[f], n = m.nei[f];
-            const double w = m.w[f];
-            const bool isOwn = (o == c);
-            const int other = isOwn ? n : o;
-#pragma unroll
-            for (int k = 0; k < NG; ++k) {
-                const double vo = s.fld[(size_t)o * NFLD + k], vn = s.fld[(size_t)n * NFLD + k];
-                const double vf = w * (vo - vn) + vn;
-                const V3 t = S * vf;
-                if (isOwn) { gx[k] = gx[k] + t.x; gy[k] = gy[k] + t.y; gz[k] = gz[k] + t.z; }
-                else { gx[k] = gx[k] - t.x; gy[k] = gy[k] - t.y; gz[k] = gz[k] - t.z; }
-                const double vOther = s.fld[(size_t)other * NFLD + k];
-                mx[k] = fmax(mx[k], vOther);
-                mn[k] = fmin(mn[k], vOther);
-            }
-            if (visc) {
-                const double eo = s.ecalc[o], en = s.ecalc[n];
-                const double vf = w * (eo - en) + en;
-                const V3 t = S * vf;
-                gex = isOwn ? gex + t : gex - t;
-            }
-        } else if (m.bKind[f - m.nI] == PK_CYCLIC) {
+        if (m.bKind[f - m.nI] == PK_CYCLIC) {
             // coupled linear interpolation w v_P + (1 - w) v_far; max/min
             // over the far-side value (cellLimitedGrad, coupled patches)
             const int kc = m.cycOf[f - m.nI];
@@ -222,7 +229,14 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
 #pragma unroll
         for (int k = 0; k < NG; ++k) { mx[k] -= vc[k]; mn[k] -= vc[k]; lim[k] = 1.0; }
         const V3 Cc = ld3(m.C + 3 * (size_t)c);
-        for (int q = e0; q < e1; ++q) {
+        for (int j = 0; j < len; ++j) {
+            const size_t sk = rs + 64 * (size_t)j;
+            const V3 d = V3{m.sgeo[sgOff(sk, SG_CX)], m.sgeo[sgOff(sk, SG_CX + 1)], m.sgeo[sgOff(sk, SG_CX + 2)]} - Cc;
+#pragma unroll
+            for (int k = 0; k < NG; ++k)
+                limitFace(lim[k], mx[k], mn[k], dot(d, V3{gx[k], gy[k], gz[k]}), ph.limiter);
+        }
+        for (int q = e0 + len; q < e1; ++q) {
             const int f = m.cf[q];
             const V3 d = ld3(m.Cf + 3 * (size_t)f) - Cc;
 #pragma unroll
@@ -486,12 +500,13 @@ __device__ __forceinline__ void cellResidualBody(const MeshView& m, const Phys& 
     double aSig = 0, aLapE = 0, rdt = 0.0;
     const bool visc = ph.viscous;
     const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
+    const int nLo = m.nLower[c];
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
         const double* t = s.ft + (size_t)f * NFT;
         const double fr = t[FT_CONV], fe = t[FT_CONV + 1];
         const V3 fu{t[FT_CONV + 2], t[FT_CONV + 3], t[FT_CONV + 4]};
-        const bool plus = f >= m.nI || m.own[f] == c;
+        const bool plus = q - e0 >= nLo;  // boundary and upper faces follow the lower faces
         if (plus) { ar += fr; ae += fe; au = au + fu; }
         else { ar -= fr; ae -= fe; au = au - fu; }
         if (t[FT_FRDT] >= 0) rdt = fmax(rdt, t[FT_FRDT]);
@@ -566,6 +581,8 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
     const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
     const int sl = c >> 6, lane = c & 63;
     const size_t rowSlot0 = (size_t)m.slicePtr[sl];
+    const size_t rs = rowSlot0 * 64 + lane;
+    const int nLo = m.nLower[c];
     int slot = 0;
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
@@ -597,10 +614,11 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
             for (int e = 0; e < NB; ++e) D[e] -= hd * A[e];
             continue;
         }
-        const bool isOwn = (m.own[f] == c);
+        const size_t sk = rs + 64 * (size_t)slot;
+        const bool isOwn = slot >= nLo;
         const double* t = s.ft + (size_t)f * NFT;
-        const V3 S = ld3(m.Sf + 3 * (size_t)f);
-        const double ms = m.magSf[f];
+        const V3 S{m.sgeo[sgOff(sk, SG_SX)], m.sgeo[sgOff(sk, SG_SX + 1)], m.sgeo[sgOff(sk, SG_SX + 2)]};
+        const double ms = m.sgeo[sgOff(sk, SG_MAG)];
         const V3 nrm = S / ms;
         // own side: block = upper (A_R), diagonal -= lower (A_L);
         // neighbour side: block = lower (A_L), diagonal -= upper (A_R)

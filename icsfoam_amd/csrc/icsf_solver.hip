@@ -1293,11 +1293,27 @@ int icsf_mesh_create(int nPoints, const double* points, int nFaces, const int* f
     RC(B.upload(col, M->sell.col.data(), M->sell.col.size()));
     RC(B.upload(rowLen, M->sell.rowLen.data(), M->sell.rowLen.size()));
     RC(B.upload(nLower, M->sell.nLower.data(), M->sell.nLower.size()));
+    double* sgeo;
+    {
+        const size_t nSlot = (size_t)M->sell.slicePtr.back() * SLICE;
+        std::vector<double> sg(std::max<size_t>(nSlot, 1) * NSG, 0.0);
+        for (int f = 0; f < h.nInternal; ++f)
+            for (const int slot : {M->sell.slotUpper[f], M->sell.slotLower[f]}) {
+                for (int d = 0; d < 3; ++d) {
+                    sg[sgOff(slot, SG_SX + d)] = h.Sf[3 * (size_t)f + d];
+                    sg[sgOff(slot, SG_CX + d)] = h.Cf[3 * (size_t)f + d];
+                }
+                sg[sgOff(slot, SG_W)] = h.weights[f];
+                sg[sgOff(slot, SG_MAG)] = h.magSf[f];
+            }
+        RC(B.upload(sgeo, sg.data(), sg.size()));
+    }
     RC(B.upload(L.fwdCells, fc.data(), fc.size()));
     RC(B.upload(L.bwdCells, bc.data(), bc.size()));
     v.own = own; v.nei = nei; v.Sf = Sf; v.magSf = magSf; v.Cf = Cf; v.w = w; v.dc = dc; v.nodc = nodc;
     v.corr = corr; v.C = C; v.V = V; v.cfOff = cfOff; v.cf = cf; v.bKind = bKind;
     v.nSlices = M->sell.nSlices; v.slicePtr = slicePtr; v.col = col; v.rowLen = rowLen; v.nLower = nLower;
+    v.sgeo = sgeo;
     v.emptyDir = h.emptyDir;
     v.nCyc = (int)h.cycFace.size();
     v.cycBase = (size_t)M->sell.slicePtr.back() * SLICE * NB;

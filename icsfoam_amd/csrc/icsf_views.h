@@ -47,6 +47,11 @@ struct MeshView {
     const int* col;       // [slicePtr[nSlices]*64]
     const int* rowLen;    // [nC]
     const int* nLower;    // [nC]
+    // geometry of each slot's internal face, same sliced layout (sgOff):
+    // Sf (3), weight, Cf (3), magSf; a cell's slots follow its internal
+    // faces in face order (upper-triangular numbering: lower faces, then
+    // upper faces, both by column), so cell kernels read it contiguously
+    const double* sgeo;
     int emptyDir;
     // cyclic faces k = 0..nCyc-1 (face order); their 5x5 interface blocks
     // follow the sliced-ELL blocks in the same allocation, at
@@ -141,6 +146,12 @@ __host__ __device__ __forceinline__ size_t sellOff(size_t slot, int e) {
 constexpr int NGEN = 10;
 __host__ __device__ __forceinline__ size_t genOff(size_t slot, int e) {
     return ((slot >> 6) * NGEN + (size_t)e) * 64 + (slot & 63);
+}
+// Slot geometry (MeshView::sgeo) of flat slot index `slot`, entry e.
+constexpr int NSG = 8;
+enum { SG_SX = 0, SG_W = 3, SG_CX = 4, SG_MAG = 7 };
+__host__ __device__ __forceinline__ size_t sgOff(size_t slot, int e) {
+    return ((slot >> 6) * NSG + (size_t)e) * 64 + (slot & 63);
 }
 // Diagonal block of row r, entry e.
 __host__ __device__ __forceinline__ size_t diagOff(int r, int e) {
