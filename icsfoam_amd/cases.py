@@ -349,14 +349,15 @@ def solution_dict(solver: str = "GMRES", precond: str = "LUSGS", nDirections: in
                   co: float = 50.0, coMin: Optional[float] = None, coMax: Optional[float] = None,
                   precision: str = "double", nPseudoCorr: int = 1, innerDirections: int = 30,
                   orthogonalisation: str = "MGS", fusedSweeps: bool = True, graphCapture: bool = True,
-                  autotune: bool = True, jacobianStorage: str = "generators") -> str:
+                  autotune: bool = True, jacobianStorage: str = "generators", fusedCellSums: bool = True) -> str:
     coMin = co if coMin is None else coMin
     coMax = co if coMax is None else coMax
     return f"""flowSolver {{ solver {solver}; {solver} {{ preconditioner {precond}; nDirections {nDirections};
  maxIter {maxIter}; minIter 0; tolerance {tol}; relTol {relTol}; precision {precision};
  innerDirections {innerDirections}; orthogonalisation {orthogonalisation};
  fusedSweeps {'true' if fusedSweeps else 'false'}; graphCapture {'true' if graphCapture else 'false'};
- autotune {'true' if autotune else 'false'}; jacobianStorage {jacobianStorage}; }} }}
+ autotune {'true' if autotune else 'false'}; jacobianStorage {jacobianStorage};
+ fusedCellSums {'true' if fusedCellSums else 'false'}; }} }}
 pseudoTime {{ nPseudoCorr {nPseudoCorr}; pseudoTol 1e-12; pseudoTolRel 1e-12; pseudoCoNum {co};
  pseudoCoNumMin {coMin}; pseudoCoNumMax {coMax}; localTimestepping true; localTimesteppingBounding true; }}
 """

@@ -127,11 +127,18 @@ void launchAdjointForward(const MeshView& m, const Phys& ph, const StateView& s,
 // Reverse pass: gW = (db/dW)^T v, gS = (db/dSf)^T v, gC = (db/dCf)^T v.
 void launchResidualVjp(const MeshView& m, const Phys& ph, const StateView& s, const AdjView& a, const double* v,
                        double* gW, double* gS, double* gC, hipStream_t st);
-void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st);
+// cellSums false: everything up to the face terms; the cell sums (b,
+// rPseudoDeltaT) are then formed by launchCellSumsJacobian.
+void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st, bool cellSums = true);
 void launchFaceFlux(const MeshView& m, const Phys& ph, const StateView& s, double* phi, double* phiU, double* phiE,
                     hipStream_t st);
 void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
                     double* rD, double* dg64, int* err, hipStream_t st);
+// Cell sums of the residual (b, rPseudoDeltaT) and the Jacobian rows in one
+// kernel, after launchResidual(..., cellSums = false); same results as
+// k_cell_residual followed by launchJacobian.
+void launchCellSumsJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
+                            double* rD, int* err, hipStream_t st);
 
 void launchSpmv(const MeshView& m, int blkFp32, int vecFp32, const void* blk, const void* dg, const void* x,
                 const double* scal, void* vout, void* y, hipStream_t st);

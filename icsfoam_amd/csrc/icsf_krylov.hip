@@ -787,21 +787,39 @@ __global__ void k_gm_beta0(GmresScalars* g, int nDirs) {
     g->bh[0] = g->beta;
 }
 
-// apply previous rotations, new rotation, update bh (gmres.C:999-1027);
-// beta2 = ||w||^2 after the orthogonalisation
-__device__ void givensStep(GmresScalars* g, int i, double beta2) {
-    const double beta = sqrt(beta2);
-    for (int j = 0; j < i; ++j) {
-        const double Hji = g->H[j][i];
-        g->H[j][i] = g->cs[j] * Hji - g->sn[j] * g->H[j + 1][i];
-        g->H[j + 1][i] = g->sn[j] * Hji + g->cs[j] * g->H[j + 1][i];
+// Apply the previous rotations to column i of H, form the new rotation and
+// update bh (gmres.C:999-1027); beta2 = ||w||^2 after the orthogonalisation.
+// The block's threads stage column i and the previous rotations in LDS, so
+// that the chain of dependent updates run by one thread waits on LDS, not
+// on device memory.  Called by all threads of the block.
+__device__ void givensBlock(GmresScalars* g, int i, double beta2) {
+    __shared__ double hc[KMAX], cc[KMAX], ss[KMAX];
+    for (int t = threadIdx.x; t <= i; t += blockDim.x) hc[t] = g->H[t][i];
+    for (int t = threadIdx.x; t < i; t += blockDim.x) {
+        cc[t] = g->cs[t];
+        ss[t] = g->sn[t];
     }
-    givensRot(g->H[i][i], beta, g->cs[i], g->sn[i]);
-    const double bhi = g->bh[i];
-    g->bh[i] = g->cs[i] * bhi - g->sn[i] * g->bh[i + 1];
-    g->bh[i + 1] = g->sn[i] * bhi + g->cs[i] * g->bh[i + 1];
-    g->H[i][i] = g->cs[i] * g->H[i][i] - g->sn[i] * beta;
-    g->beta = beta;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double beta = sqrt(beta2);
+        for (int j = 0; j < i; ++j) {
+            const double Hji = hc[j];
+            hc[j] = cc[j] * Hji - ss[j] * hc[j + 1];
+            hc[j + 1] = ss[j] * Hji + cc[j] * hc[j + 1];
+        }
+        double ci, si;
+        givensRot(hc[i], beta, ci, si);
+        g->cs[i] = ci;
+        g->sn[i] = si;
+        const double bhi = g->bh[i];
+        const double bhn = g->bh[i + 1];
+        g->bh[i] = ci * bhi - si * bhn;
+        g->bh[i + 1] = si * bhi + ci * bhn;
+        hc[i] = ci * hc[i] - si * beta;
+        g->beta = beta;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t <= i; t += blockDim.x) g->H[t][i] = hc[t];
 }
 
 // CGS projections of one multidot pass (partial rows of ch values): value
@@ -821,10 +839,8 @@ __global__ void __launch_bounds__(256) k_reduce_col(const double* __restrict__ p
 __global__ void __launch_bounds__(256) k_reduce_givens(const double* __restrict__ part, int nb, GmresScalars* g,
                                                        int i) {
     const double s = sumPartials(part, nb, 1, 0);
-    if (threadIdx.x == 0) {
-        g->red[0] = s;
-        givensStep(g, i, s);
-    }
+    if (threadIdx.x == 0) g->red[0] = s;
+    givensBlock(g, i, s);
 }
 
 // MGS step j of column i: the three per-variable partial sums of <w, V_j>
@@ -835,12 +851,12 @@ __global__ void __launch_bounds__(256) k_reduce_mgs(const double* __restrict__ p
     double s[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) s[k] = sumPartials(part, nb, 3, k);
+    const double hs = (0.0 + s[0] + s[1]) + s[2];
     if (threadIdx.x == 0) {
         for (int k = 0; k < 3; ++k) g->red[k] = s[k];
-        const double hs = (0.0 + s[0] + s[1]) + s[2];
         if (j <= i) g->H[j][i] = hs;
-        else givensStep(g, i, hs);
     }
+    if (j > i) givensBlock(g, i, hs);
 }
 
 __global__ void k_gm_backsub(GmresScalars* g, int n) {

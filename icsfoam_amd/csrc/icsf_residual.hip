@@ -491,22 +491,16 @@ __global__ void __launch_bounds__(256) k_face_terms_hb(const HBInst* __restrict_
 }
 
 // Residual b = R*V (residualsUpdate.H) and rPseudoDeltaT
-// (setCoAndDeltaT.H:57-141) of each cell from the face terms.
-__device__ __forceinline__ void cellResidualBody(const MeshView& m, const Phys& ph, const StateView& s) {
-    const int c = xcdBlock() * blockDim.x + threadIdx.x;
-    if (c >= m.nC) return;
+// (setCoAndDeltaT.H:57-141) of a cell from the face terms: add() each face
+// record in face order (plus: the cell owns the face, or it is a boundary
+// face), then store().
+struct CellSums {
     double ar = 0, ae = 0;
     V3 au{0, 0, 0}, aLap{0, 0, 0}, aTau{0, 0, 0};
     double aSig = 0, aLapE = 0, rdt = 0.0;
-    const bool visc = ph.viscous;
-    const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
-    const int nLo = m.nLower[c];
-    for (int q = e0; q < e1; ++q) {
-        const int f = m.cf[q];
-        const double* t = s.ft + (size_t)f * NFT;
+    __device__ __forceinline__ void add(const double* t, bool plus, bool visc) {
         const double fr = t[FT_CONV], fe = t[FT_CONV + 1];
         const V3 fu{t[FT_CONV + 2], t[FT_CONV + 3], t[FT_CONV + 4]};
-        const bool plus = q - e0 >= nLo;  // boundary and upper faces follow the lower faces
         if (plus) { ar += fr; ae += fe; au = au + fu; }
         else { ar -= fr; ae -= fe; au = au - fu; }
         if (t[FT_FRDT] >= 0) rdt = fmax(rdt, t[FT_FRDT]);
@@ -517,23 +511,42 @@ __device__ __forceinline__ void cellResidualBody(const MeshView& m, const Phys& 
             else { aLap = aLap - lap; aTau = aTau - dt; aSig -= t[FT_SIG]; aLapE -= t[FT_LAPE]; }
         }
     }
-    const double Vc = m.V[c];
-    const int nC = m.nC;
-    double Rr = -(ar / Vc);
-    double Re = -(ae / Vc);
-    V3 Ru = V3{-(au.x / Vc), -(au.y / Vc), -(au.z / Vc)};
-    if (visc) {
-        Ru = Ru + aLap / Vc;
-        Ru = Ru + aTau / Vc;
-        Re += aSig / Vc;
-        Re += aLapE / Vc;
+    // writes b and rDT of cell c; returns rDT
+    __device__ __forceinline__ double store(const MeshView& m, const StateView& s, int c, bool visc) const {
+        const double Vc = m.V[c];
+        const int nC = m.nC;
+        double Rr = -(ar / Vc);
+        double Re = -(ae / Vc);
+        V3 Ru = V3{-(au.x / Vc), -(au.y / Vc), -(au.z / Vc)};
+        if (visc) {
+            Ru = Ru + aLap / Vc;
+            Ru = Ru + aTau / Vc;
+            Re += aSig / Vc;
+            Re += aLapE / Vc;
+        }
+        s.b[c] = Rr * Vc;
+        s.b[(size_t)nC + c] = Re * Vc;
+        s.b[2 * (size_t)nC + c] = Ru.x * Vc;
+        s.b[3 * (size_t)nC + c] = Ru.y * Vc;
+        s.b[4 * (size_t)nC + c] = Ru.z * Vc;
+        const double r = rdt / s.pseudoCo[c];
+        s.rDT[c] = r;
+        return r;
     }
-    s.b[c] = Rr * Vc;
-    s.b[(size_t)nC + c] = Re * Vc;
-    s.b[2 * (size_t)nC + c] = Ru.x * Vc;
-    s.b[3 * (size_t)nC + c] = Ru.y * Vc;
-    s.b[4 * (size_t)nC + c] = Ru.z * Vc;
-    s.rDT[c] = rdt / s.pseudoCo[c];
+};
+
+__device__ __forceinline__ void cellResidualBody(const MeshView& m, const Phys& ph, const StateView& s) {
+    const int c = xcdBlock() * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const bool visc = ph.viscous;
+    const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
+    const int nLo = m.nLower[c];
+    CellSums cs;
+    for (int q = e0; q < e1; ++q) {
+        const int f = m.cf[q];
+        cs.add(s.ft + (size_t)f * NFT, q - e0 >= nLo, visc);  // boundary and upper faces follow the lower faces
+    }
+    cs.store(m, s, c, visc);
 }
 
 __global__ void __launch_bounds__(256) k_cell_residual(MeshView m, Phys ph, StateView s) {
@@ -566,7 +579,10 @@ __global__ void k_face_flux(MeshView m, StateView s, double* phi, double* phiU, 
 // (lusgs.C:50-125).  The diagonal accumulates the convective part, the
 // dissipation and the viscous parts separately and combines them in the
 // order the reference adds its temporary matrices.
-template <typename TB>
+// RES: the cell's residual sums (cellResidualBody) are formed in the same
+// face walk, from the same face-record loads, and rPseudoDeltaT is taken
+// from them (one pass over the records instead of two; same bits).
+template <typename TB, bool RES>
 __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& ph, const StateView& s,
                                                  TB* __restrict__ blk, TB* __restrict__ dg,
                                                  double* __restrict__ rD, int* __restrict__ err) {
@@ -583,9 +599,11 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
     const size_t rowSlot0 = (size_t)m.slicePtr[sl];
     const size_t rs = rowSlot0 * 64 + lane;
     const int nLo = m.nLower[c];
+    CellSums cs;
     int slot = 0;
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
+        if (RES) cs.add(s.ft + (size_t)f * NFT, q - e0 >= nLo, visc);
         if (f >= m.nI) {
             // cyclic face: interface block (interfacesUpper: 1/2|S| A_R -
             // 1/2|S| lambda I) and diag -= interfacesLower (+1/2|S| A_L),
@@ -655,6 +673,8 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
         if (visc) vis -= sv;
         ++slot;
     }
+    double rDTc = 0.0;
+    if (RES) rDTc = cs.store(m, s, c, visc);
     // boundary dissipation (all non-empty patches) follows the internal faces
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
@@ -752,7 +772,7 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
     // temporal term: ddtCoeff = (rDT*V)/V, diag += ddtCoeff*V
     {
         const double Vc = m.V[c];
-        const double ddtCoeff = (s.rDT[c] * Vc) / Vc;
+        const double ddtCoeff = ((RES ? rDTc : s.rDT[c]) * Vc) / Vc;
         const double dcv = ddtCoeff * Vc;
         D[0] += dcv; D[12] += dcv; D[18] += dcv; D[24] += dcv; D[6] += dcv;
     }
@@ -770,16 +790,16 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
     if (!(r >= kVSMALL)) atomicOr(err, 1);
 }
 
-template <typename TB, int W = 1>
+template <typename TB, int W = 1, bool RES = false>
 __global__ void __launch_bounds__(256, W) k_cell_jacobian(MeshView m, Phys ph, StateView s, TB* __restrict__ blk,
                                                           TB* __restrict__ dg, double* __restrict__ rD,
                                                           int* __restrict__ err) {
-    cellJacobianBody<TB>(m, ph, s, blk, dg, rD, err);
+    cellJacobianBody<TB, RES>(m, ph, s, blk, dg, rD, err);
 }
 template <typename TB>
 __global__ void __launch_bounds__(256) k_cell_jacobian_hb(const HBInst* __restrict__ I, Phys ph) {
     const HBInst q = I[blockIdx.y];
-    cellJacobianBody<TB>(q.m, ph, q.s, (TB*)q.blk, (TB*)q.dg, q.rD, q.err);
+    cellJacobianBody<TB, false>(q.m, ph, q.s, (TB*)q.blk, (TB*)q.dg, q.rD, q.err);
 }
 
 }  // namespace
@@ -792,7 +812,7 @@ void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const d
     k_update<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, dW, mode, rho0, U0, T0);
 }
 
-void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st) {
+void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st, bool cellSums) {
     if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
     const int bc = m.bsCell, bf = m.bsFace;
     const dim3 gc(xcdGrid(m.nC, bc)), gf(xcdGrid(m.nF, bf));
@@ -806,7 +826,7 @@ void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipSt
     const size_t lds = (size_t)bf * (NFT + 1) * sizeof(double);
     if (m.occFace) k_face_terms<4><<<gf, bf, lds, st>>>(m, ph, s);
     else k_face_terms<1><<<gf, bf, lds, st>>>(m, ph, s);
-    k_cell_residual<<<gc, bc, 0, st>>>(m, ph, s);
+    if (cellSums) k_cell_residual<<<gc, bc, 0, st>>>(m, ph, s);
 }
 
 void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st) {
@@ -838,18 +858,28 @@ void launchFaceFlux(const MeshView& m, const Phys& ph, const StateView& s, doubl
     k_face_flux<<<nblk(m.nF, 256), 256, 0, st>>>(m, s, phi, phiU, phiE);
 }
 
-void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
-                    double* rD, double* dg64, int* err, hipStream_t st) {
-    (void)dg64;
+template <bool RES>
+static void launchCellJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
+                               double* rD, int* err, hipStream_t st) {
     const int bj = m.bsJac;
     const dim3 g(xcdGrid(m.nC, bj));
     if (fp32) {
-        if (m.occJac) k_cell_jacobian<float, 4><<<g, bj, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
-        else k_cell_jacobian<float><<<g, bj, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
+        if (m.occJac) k_cell_jacobian<float, 4, RES><<<g, bj, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
+        else k_cell_jacobian<float, 1, RES><<<g, bj, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
     } else {
-        if (m.occJac) k_cell_jacobian<double, 4><<<g, bj, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
-        else k_cell_jacobian<double><<<g, bj, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
+        if (m.occJac)
+            k_cell_jacobian<double, 4, RES><<<g, bj, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
+        else k_cell_jacobian<double, 1, RES><<<g, bj, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
     }
+}
+void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
+                    double* rD, double* dg64, int* err, hipStream_t st) {
+    (void)dg64;
+    launchCellJacobian<false>(m, ph, s, blk, dg, fp32, rD, err, st);
+}
+void launchCellSumsJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
+                            double* rD, int* err, hipStream_t st) {
+    launchCellJacobian<true>(m, ph, s, blk, dg, fp32, rD, err, st);
 }
 
 }  // namespace icsf

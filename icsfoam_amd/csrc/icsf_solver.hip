@@ -100,6 +100,7 @@ struct icsf_case {
     int precond = 0;     // 0 LUSGS, 1 Jacobi, 2 none
     int nDirs = 30, maxIter = 1, minIter = 0, innerDirs = 30, orth = 0, mixed = 0;
     bool fuse = true;  // SpMV fused with the LU-SGS sweeps on two-colour numberings
+    bool fuseCells = true;  // pseudo-step: residual cell sums formed in the Jacobian's face walk
     bool autotune = true, tuned = false;
     bool blkGen = true;  // off-diagonal blocks as generators (jacobianStorage)
     double tol = 1e-12, relTol = 1e-2;
@@ -367,6 +368,10 @@ int readSettings(icsf_case* c, const char* thermo, const char* schemes, const ch
     if (fsw == "true" || fsw == "on" || fsw == "yes") c->fuse = true;
     else if (fsw == "false" || fsw == "off" || fsw == "no") c->fuse = false;
     else return fail(ICSF_ERR_KEYWORD, "fusedSweeps must be true or false");
+    const std::string fcw = fd.word(base + "fusedCellSums", "true");
+    if (fcw == "true" || fcw == "on" || fcw == "yes") c->fuseCells = true;
+    else if (fcw == "false" || fcw == "off" || fcw == "no") c->fuseCells = false;
+    else return fail(ICSF_ERR_KEYWORD, "fusedCellSums must be true or false");
     if (c->nDirs < 1 || c->nDirs > KMAX || c->innerDirs < 1 || c->innerDirs > KMAX)
         return fail(ICSF_ERR_ARG, "nDirections must be in [1, 64]");
     if (c->mixed && c->solver != 1)
@@ -404,20 +409,23 @@ int uploadBC(icsf_case* c) {
     return ICSF_OK;
 }
 
-int residualStage(icsf_case* c) {
+// cellSums false (pseudo-step): the residual's cell sums are left to
+// jacobianStage(c, true), which forms them in the Jacobian's face walk.
+int residualStage(icsf_case* c, bool cellSums = true) {
     RC(uploadBC(c));
     {
         Timed t(c, TK_RESIDUAL);
-        launchResidual(c->mv, c->ph, c->sv, c->st);
+        launchResidual(c->mv, c->ph, c->sv, c->st, cellSums);
     }
     c->haveResidual = true;
     return ICSF_OK;
 }
 
-int jacobianStage(icsf_case* c) {
+int jacobianStage(icsf_case* c, bool cellSums = false) {
     {
         Timed t(c, TK_JACOBIAN);
-        launchJacobian(c->mv, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, nullptr, c->derr, c->st);
+        if (cellSums) launchCellSumsJacobian(c->mv, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, c->derr, c->st);
+        else launchJacobian(c->mv, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, nullptr, c->derr, c->st);
     }
     if (c->precond == 1) launchJacobiSetup(c->mv.nC, c->blkFp32, c->dg, c->dinv, c->st);
     c->assembled = true;
@@ -484,8 +492,8 @@ int solveStage(icsf_case* c, const double* W, const double* b, int& nIter) {
 // Autotuner: on the first pseudo-step, after the residual and Jacobian
 // (every buffer then holds the case's real data), each launch parameter is
 // timed with HIP events on the case stream over its candidates and the
-// fastest kept: workgroup sizes 64/128/256 of the face-terms, gradient +
-// cell-residual, Jacobian-row and Arnoldi-operator kernels, and 8/16/32
+// fastest kept: workgroup sizes 64/128/256 of the face-terms, gradient,
+// cell-sums + Jacobian-row and Arnoldi-operator kernels, and 8/16/32
 // basis vectors per CGS projection pass.  Only scratch vectors are written
 // (or state is recomputed identically); no candidate changes a result bit.
 int autotune(icsf_case* c) {
@@ -540,10 +548,15 @@ int autotune(icsf_case* c) {
     };
     MeshView& m = c->mv;
     const std::vector<int> bs{64, 128, 256};
-    pick2(m.bsFace, m.occFace, [&] { launchResidual(m, c->ph, c->sv, c->st); });
-    pick2(m.bsCell, m.occCell, [&] { launchResidual(m, c->ph, c->sv, c->st); });
+    // the pseudo-step's launches: residual up to the face terms, then the
+    // cell sums with the Jacobian rows (the last candidate leaves b, rDT and
+    // the matrix as the step computed them)
+    const bool fc = c->fuseCells;
+    pick2(m.bsFace, m.occFace, [&] { launchResidual(m, c->ph, c->sv, c->st, !fc); });
+    pick2(m.bsCell, m.occCell, [&] { launchResidual(m, c->ph, c->sv, c->st, !fc); });
     pick2(m.bsJac, m.occJac, [&] {
-        launchJacobian(m, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, nullptr, c->derr, c->st);
+        if (fc) launchCellSumsJacobian(m, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, c->derr, c->st);
+        else launchJacobian(m, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, nullptr, c->derr, c->st);
     });
     // the Arnoldi operator in the precision of the inner iteration
     const int v32 = c->mixed ? 1 : 0;
@@ -583,8 +596,8 @@ int pseudoStep(icsf_case* c, int& nIterOut) {
         k_copy5<<<1, 64, 0, c->st>>>(c->dres, c->dres + 5);
         c->havePrev = true;
     }
-    RC(residualStage(c));
-    RC(jacobianStage(c));
+    RC(residualStage(c, !c->fuseCells));
+    RC(jacobianStage(c, c->fuseCells));
     if (c->autotune && !c->tuned) RC(autotune(c));
     int nIter = 0;
     RC(solveStage(c, c->sv.W, c->sv.b, nIter));

@@ -107,6 +107,23 @@ def test_generator_storage_equals_block_storage(which):
 
 
 @pytest.mark.parametrize("form", ["fp64", "mixed"])
+@pytest.mark.parametrize("which", ["bump", "box"])
+def test_fused_cell_sums_give_the_same_bits(which, form):
+    """The pseudo-step forms the residual's cell sums (b, rPseudoDeltaT) in
+    the Jacobian kernel's face walk: same residual history and state bit
+    for bit as the separate cell-residual and Jacobian kernels."""
+    print(f"[fused cell sums {which} {form}]", flush=True)
+    kw = dict()
+    if form == "mixed":
+        kw.update(solver="FGMRES", nDirections=1, precision="mixed", innerDirections=20)
+    c = bump_small() if which == "bump" else box_small()
+    gf = dbnsCase.from_case(_with(c, fusedCellSums=True, **kw))
+    gu = dbnsCase.from_case(_with(c, fusedCellSums=False, **kw))
+    np.testing.assert_array_equal(gf.pseudo_steps(4), gu.pseudo_steps(4))
+    np.testing.assert_array_equal(gf.fields()["W"], gu.fields()["W"])
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
 def test_autotuned_launches_give_the_same_bits(form):
     """The autotuner only picks workgroup sizes and projection chunks: a
     tuned and an untuned case give the same residual history and state bit
