@@ -160,7 +160,9 @@ int icsf_mesh_info(const icsf_mesh* mesh, int* info);
  * reconstruct(rho|U|T)) and fvSolution (flowSolver{solver GMRES|FGMRES;
  * <solver>{preconditioner LUSGS|Jacobi|none; nDirections; maxIter; minIter;
  * tolerance; relTol; precision double|mixed; innerDirections;
- * orthogonalisation MGS|CGS}}, pseudoTime{...}).
+ * orthogonalisation MGS|CGS; fusedSweeps, graphCapture, autotune,
+ * fusedCellSums true|false; jacobianStorage generators|blocks}},
+ * pseudoTime{...}).
  * Replaces convectiveFluxScheme::New (newConvectiveFluxScheme.C:39-64),
  * viscousFluxScheme construction (viscousFluxScheme.H:105-113),
  * coupledMatrix::solver::New (coupledMatrixSolver.C:41-67) and
