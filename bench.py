@@ -99,9 +99,10 @@ def algorithmic_bytes_fused(nC, nI, p, q, gen=True):
 def step_bytes(nC, nI, nB, form, m=30):
     """Algorithmic HBM bytes of one pseudo-step (one GMRES/FGMRES cycle of
     m Arnoldi steps), summed over the kernels' per-launch figures:
-    gradients nC (8 + 24) 8 + 112 nI; face terms + cell residual
-    nF (56 + 128 + 384 + 384) + nC 104 (viscous Roe); Jacobian rows
-    2 nI (10 p + 192) + 25 p nC (generator storage); per Arnoldi step the fused operator group
+    gradients nC (8 + 24) 8 + 112 nI; face terms nF (56 + 128 + 384 + 192)
+    + nC 104 (viscous Roe); residual cell sums and Jacobian rows in one
+    pass over the face records 2 nI (10 p + 192) + 25 p nC (generator
+    storage); per Arnoldi step the fused operator group
     and the CGS projection (w and j + 1 basis vectors read, then w, j + 1
     basis vectors read and w written: 5 q nC (2 j + 5)); per cycle the
     normalisation pass (average, one SpMV, norm factors), the solution
@@ -110,7 +111,7 @@ def step_bytes(nC, nI, nB, form, m=30):
     p = q = 4 plus the FP64 outer FGMRES(1) (two SpMVs with FP32 blocks, a
     two-vector update)."""
     nF = nI + nB
-    res = nC * (8 + 24) * 8 + 112 * nI + nF * (56 + 128 + 384 + 384) + nC * 104
+    res = nC * (8 + 24) * 8 + 112 * nI + nF * (56 + 128 + 384 + 192) + nC * 104
     fused = lambda p, q: algorithmic_bytes_fused(nC, nI, p, q)
     spmv = lambda p, q: (2 * nI * NGEN + nC * 25) * p + 2 * nI * 4 + nC * 5 * q * 2
     orth = lambda q: sum(5 * q * nC * (2 * j + 5) for j in range(m))
@@ -401,6 +402,9 @@ def main():
                    "eager_steps_per_s": {"fp64": round(r64["eager_steps_per_s"], 4),
                                          "mixed": round(rmx["eager_steps_per_s"], 4)},
                    "orthogonalisation": args.orth,
+                   "breakdown_classes": ["residual up to the face terms", "residual cell sums + Jacobian rows",
+                                         "Arnoldi operator (SpMV + LU-SGS)", "preconditioner outside the operator",
+                                         "orthogonalisation", "state update, solution combine, norms"],
                    "breakdown_ms_fp64": r64["breakdown_ms"], "breakdown_ms_mixed": rmx["breakdown_ms"],
                    "autotuned": {"fp64": r64["tuning"], "mixed": rmx["tuning"]},
                    "step_hbm_frac": {f: round(step_bytes(nC, nI, nB, f) * r["steps_per_s"] / HBM_PEAK, 4)
