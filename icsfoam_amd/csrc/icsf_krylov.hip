@@ -733,7 +733,16 @@ __global__ void __launch_bounds__(256) k_update_sol(int nC, TX* __restrict__ dW,
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     TX a = dW[i];
-    for (int j = 0; j < nv; ++j) a += (TX)yh[j] * (TX)Vb[(size_t)j * vstride + i];
+    // four basis vectors' loads in flight, added in order j = 0..nv-1
+    int j = 0;
+    for (; j + 4 <= nv; j += 4) {
+        TV x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = Vb[(size_t)(j + u) * vstride + i];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a += (TX)yh[j + u] * (TX)x[u];
+    }
+    for (; j < nv; ++j) a += (TX)yh[j] * (TX)Vb[(size_t)j * vstride + i];
     dW[i] = a;
 }
 
