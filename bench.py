@@ -127,15 +127,16 @@ def step_bytes(nC, nI, nB, form, m=30):
     return res + jac + cycle + 5 * 8 * 4 * nC
 
 
-def measured_traffic(n):
-    """HBM bytes per fused Arnoldi-operator group (FP64 form) from the
-    committed FETCH_SIZE/WRITE_SIZE profile of the 160^3 workload (None
+def measured_traffic(n, form="fp64"):
+    """HBM bytes per fused Arnoldi-operator group (FP64 or mixed form) from
+    the committed FETCH_SIZE/WRITE_SIZE profile of the 160^3 workload (None
     otherwise)."""
     path = os.path.join(ROOT, "profiles", "arnoldi_operator_160cube_pmc.txt")
     if n != 160 or not os.path.exists(path):
         return None
+    key = "traffic_bytes_per_group:" if form == "fp64" else "traffic_bytes_per_group_fp32:"
     for line in open(path):
-        if line.startswith("traffic_bytes_per_group:"):
+        if line.startswith(key):
             return int(line.split(":")[1])
     return None
 
@@ -377,6 +378,8 @@ def main():
     fused = r64["fused"]
     bytes64 = algorithmic_bytes_fused(nC, nI, 8, 8) if fused else algorithmic_bytes(nC, nI, 8, 8)
     ach = bytes64 / (r64["spmv_ms"] * 1e-3)
+    bytes32 = algorithmic_bytes_fused(nC, nI, 4, 4) if fused else algorithmic_bytes(nC, nI, 4, 4)
+    ach32 = bytes32 / (rmx["spmv_ms"] * 1e-3)
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline on {args.cpu_sample}^3 ...")
@@ -420,6 +423,13 @@ def main():
                      "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK, 4), "traffic": measured_traffic(n) if fused else None,
                      "bytes_per_launch": int(bytes64), "avg_launch_ms": round(r64["spmv_ms"], 4)},
+        # the same operator group in the form `value` is quoted on
+        "roofline_mixed": {"bound": "hbm",
+                           "kernel": "fused Arnoldi operator of the FP32 inner GMRES (FP32 generators and vectors)",
+                           "achieved": round(ach32 / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                           "frac": round(ach32 / HBM_PEAK, 4),
+                           "traffic": measured_traffic(n, "mixed") if fused else None,
+                           "bytes_per_launch": int(bytes32), "avg_launch_ms": round(rmx["spmv_ms"], 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -271,9 +271,12 @@ int icsf_synchronize(icsf_case* c);
  * icsf_timing_enable(c, mask) resets and starts accumulating for classes
  * whose bit is set (mask 0 stops); icsf_timing_read returns, per class
  * k < n, total milliseconds and event-pair count.  Classes: 0 residual
- * (boundary + gradient + residual kernels), 1 Jacobian, 2 block SpMV,
- * 3 preconditioner (all sweep levels of one application), 4
- * orthogonalisation + Givens of one Arnoldi step, 5 other. */
+ * (boundary + gradient + residual kernels; in a pseudo-step up to the face
+ * terms), 1 Jacobian (in a pseudo-step with the residual's cell sums),
+ * 2 Arnoldi operator (the block SpMV of each Arnoldi step, with the fused
+ * LU-SGS sweeps where they apply), 3 preconditioner (all sweep levels of
+ * one application), 4 orthogonalisation + Givens of one Arnoldi step,
+ * 5 other (including the outer product of FGMRES). */
 int icsf_timing_enable(icsf_case* c, int mask);
 int icsf_timing_read(icsf_case* c, int n, double* ms, long long* count);
 
