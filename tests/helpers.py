@@ -3,6 +3,7 @@ colour-major) through the library's host-side renumbering."""
 import numpy as np
 
 from icsfoam_amd import cases
+from oracle.oracle import OracleMesh
 from icsfoam_amd.mesh import prepare_case
 
 
@@ -54,3 +55,25 @@ def smooth_field(X, rng, ncomp, modes=4, k=3.0):
         ph = rng.uniform(0, 2 * np.pi, modes)
         out[q] = sum(np.sin(X @ kv[j] + ph[j]) for j in range(modes)) / modes
     return out
+
+
+def linear_velocity_case(mu, R=287.0):
+    """Uniform orthogonal box, constant rho and T, U = A x + u0 with a
+    non-symmetric A, fixedValue U (the linear field at the face centres) and
+    T on every patch; returns (case, oracle geometry, A)."""
+    c = cases.case_box(n=(6, 5, 4), lengths=(1.0, 0.8, 0.6), jitter=0.0, flux="ROE", mu=mu, p_jump=1.0,
+                       noise=0.0)
+    geo = OracleMesh(c.mesh).geometry()
+    A = np.array([[3.0, 7.0, -2.0], [-5.0, 1.0, 4.0], [6.0, -8.0, 2.0]])
+    u0 = np.array([150.0, -40.0, 25.0])
+    T0, p0 = 300.0, 1e5
+    c.rho[:] = p0 / (R * T0)
+    c.T[:] = T0
+    c.U[:] = geo["C"] @ A.T + u0
+    bcs = []
+    for pt in c.mesh.patches:
+        Cf = geo["Cf"][pt.start:pt.start + pt.size]
+        bcs.append(cases.BoundarySpec(cases.BC_ZERO_GRADIENT, cases.BC_FIXED_VALUE, cases.BC_FIXED_VALUE,
+                                      None, Cf @ A.T + u0, np.full(pt.size, T0)))
+    c.bcs = bcs
+    return c, geo, A

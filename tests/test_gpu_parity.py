@@ -14,7 +14,7 @@ import pytest
 from icsfoam_amd import dbnsCase
 from oracle.oracle import OracleCase
 
-from helpers import box_small, bump_small, rel_err
+from helpers import box_small, bump_small, linear_velocity_case, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -25,6 +25,7 @@ CASES = {
     "box_ausm_viscous": lambda: box_small(flux="AUSMPlusUp"),
     "bump_hllc": lambda: bump_small(flux="HLLC"),
     "box_hllc_viscous": lambda: box_small(flux="HLLC"),
+    "box_linear_fixed_walls": lambda: linear_velocity_case(2.0e-3)[0],
 }
 
 

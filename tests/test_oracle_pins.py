@@ -14,7 +14,7 @@ from icsfoam_amd import cases
 from icsfoam_amd import mesh as M
 from oracle import oracle as O
 
-from helpers import rel_err
+from helpers import linear_velocity_case, rel_err
 
 GAMMA, R = 1.4, 287.0
 CV = R / (GAMMA - 1)
@@ -262,6 +262,34 @@ def test_freestream_preservation_jittered(flux):
     b, _ = O.OracleCase(c).residual()
     scale = rho0 * U0 * U0 * 0.1 * 0.1
     assert np.abs(b).max() < 1e-9 * scale
+
+
+def test_viscous_terms_exact_for_linear_velocity():
+    """Viscous residual terms (residualsUpdate.H: laplacian(muEff, U),
+    div(muEff dev2(T(grad U))), div(sigmaDotU & Sf), laplacian(alphaEff, e))
+    for a linear velocity field on a uniform orthogonal box, where the Gauss
+    gradient, the linear face interpolation and snGrad are exact: with
+    G = grad U (G_ij = dU_j/dx_i) constant, the momentum terms are surface
+    integrals of constant tensors and vanish, and the energy term is
+    V * M:G with M = mu G + mu (G^T - 2/3 tr(G) I) (divergence theorem on
+    the face-midpoint rule, exact for the linear integrand); e is constant.
+    A transposed stress, a wrong dev2 factor or a sigmaDotU built from the
+    wrong tensor changes M:G for the non-symmetric G used here.  The
+    difference of the residuals with and without viscosity isolates these
+    terms (the convective flux does not depend on mu)."""
+    mu = 2.0e-3
+    cv, geo, A = linear_velocity_case(mu)
+    c0, _, _ = linear_velocity_case(0.0)
+    bv, _ = O.OracleCase(cv).residual()
+    b0, _ = O.OracleCase(c0).residual()
+    d = bv - b0
+    G = A.T
+    M = mu * G + mu * (G.T - 2.0 / 3.0 * np.trace(G) * np.eye(3))
+    expect_e = geo["V"] * np.sum(M * G)
+    assert np.array_equal(d[0], np.zeros_like(d[0]))
+    scale = np.abs(expect_e).max()
+    assert np.abs(d[2:]).max() < 1e-9 * scale
+    np.testing.assert_allclose(d[1], expect_e, rtol=1e-8, atol=1e-9 * scale)
 
 
 def test_discrete_conservation():
