@@ -1,0 +1,93 @@
+"""Size-independent properties at the benchmark's full size (GPU).
+
+The 160^3 jittered box (4 096 000 cells, viscous Roe, permuted + RCM +
+colour-major numbering) is far beyond what the oracle runs in seconds, so
+at this size the HIP path is checked through properties that do not depend
+on an oracle run:
+
+* discrete conservation: the cell residuals sum to minus the net boundary
+  flux (inviscid form of the case, whose face fluxes calcFlux returns),
+  to 1e-9 of the summed magnitudes;
+* the pseudo-step with the residual's cell sums formed in the Jacobian
+  kernel equals the one with separate kernels bit for bit, eager and from
+  the captured graph (4 steps), in both the FP64 and the mixed form;
+* steps replayed from the captured graph equal eager steps bit for bit
+  (mixed form, 5 steps: 2 eager, the captured one, 2 replays).
+
+The FP64 and the mixed form are not compared step by step here: each
+pseudo-step runs one GMRES(30) cycle to relTol 1e-2, and at this size the
+left-preconditioned GMRES and the flexible outer iteration stop at
+different iterates (4 steps differ by a few percent); both reach the same
+1e-8 drop of the residual (the convergence record of the benchmark).
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from icsfoam_amd import cases, dbnsCase
+from icsfoam_amd.mesh import prepare_case
+
+pytestmark = pytest.mark.gpu
+
+N = 160
+
+
+@pytest.fixture(scope="module")
+def box160():
+    print(f"[fullsize] preparing the {N}^3 box", flush=True)
+    c = cases.case_box(n=(N, N, N), lengths=(1.0, 1.0, 1.0), jitter=0.1, seed=7, flux="ROE", mu=1.8e-5,
+                       mach=(0.5, 0.9), p_jump=2.0, noise=0.02, co=50.0, orthogonalisation="CGS")
+    return prepare_case(c, permute_seed=2024)
+
+
+def _form(case, form, **kw):
+    if form == "fp64":
+        base = dict(solver="GMRES", nDirections=30, precision="double")
+    else:
+        base = dict(solver="FGMRES", nDirections=1, precision="mixed", innerDirections=30)
+    base.update(co=50.0, orthogonalisation="CGS", **kw)
+    return dataclasses.replace(case, fvSolution=cases.solution_dict(**base))
+
+
+def test_conservation_160cube(box160):
+    print("[fullsize] conservation", flush=True)
+    c = dataclasses.replace(box160, thermo=cases.thermo_dict(0.0))
+    g = dbnsCase.from_case(_form(c, "fp64"))
+    b, _ = g.residual()
+    phi, phiU, phiE = g.calcFlux()
+    nI = c.mesh.n_internal
+    net = np.array([phi[nI:].sum(), phiE[nI:].sum(), *phiU[nI:].sum(axis=0)])
+    tot = b.sum(axis=1)
+    scale = np.abs(b).sum(axis=1)
+    assert np.all(np.abs(tot + net) <= 1e-9 * scale), (tot, net, scale)
+    del g
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_fused_cell_sums_bitwise_160cube(box160, form):
+    print(f"[fullsize] fused cell sums {form}", flush=True)
+    gf = dbnsCase.from_case(_form(box160, form, fusedCellSums=True))
+    hf = gf.pseudo_steps(4)
+    Wf = gf.fields()["W"]
+    del gf
+    gu = dbnsCase.from_case(_form(box160, form, fusedCellSums=False))
+    hu = gu.pseudo_steps(4)
+    Wu = gu.fields()["W"]
+    del gu
+    np.testing.assert_array_equal(hf, hu)
+    np.testing.assert_array_equal(Wf, Wu)
+
+
+def test_graph_replay_equals_eager_160cube(box160):
+    print("[fullsize] graph replay vs eager (mixed)", flush=True)
+    ge = dbnsCase.from_case(_form(box160, "mixed", graphCapture=False))
+    he = ge.pseudo_steps(5)
+    We = ge.fields()["W"]
+    del ge
+    gg = dbnsCase.from_case(_form(box160, "mixed", graphCapture=True))
+    hg = gg.pseudo_steps(5)
+    Wg = gg.fields()["W"]
+    del gg
+    np.testing.assert_array_equal(hg, he)
+    np.testing.assert_array_equal(Wg, We)
