@@ -12,7 +12,9 @@ on an oracle run:
   kernel equals the one with separate kernels bit for bit, eager and from
   the captured graph (4 steps), in both the FP64 and the mixed form;
 * steps replayed from the captured graph equal eager steps bit for bit
-  (mixed form, 5 steps: 2 eager, the captured one, 2 replays).
+  (mixed form, 5 steps: 2 eager, the captured one, 2 replays), here and on
+  the harmonic-balance workload at its benchmark size (64x64x32 cyclic
+  sector x 7 instants).
 
 The FP64 and the mixed form are not compared step by step here: each
 pseudo-step runs one GMRES(30) cycle to relTol 1e-2, and at this size the
@@ -91,3 +93,24 @@ def test_graph_replay_equals_eager_160cube(box160):
     del gg
     np.testing.assert_array_equal(hg, he)
     np.testing.assert_array_equal(Wg, We)
+
+
+def test_hb_graph_replay_equals_eager_full_size():
+    """The harmonic-balance workload at its benchmark size (64x64x32 annular
+    sector with rotational cyclic patches, 7 instants, mixed form): steps
+    replayed from the captured graph equal eager steps bit for bit."""
+    from icsfoam_amd.solver import hbCase
+
+    print("[fullsize] HB 64x64x32 x 7 graph replay vs eager", flush=True)
+    insts, D, _ = cases.case_hb(n=(64, 64, 32), n_harmonics=3, flux="ROE", orthogonalisation="CGS")
+    insts = [prepare_case(c, permute_seed=2024) for c in insts]
+    out = {}
+    for graph in (False, True):
+        sol = cases.solution_dict(co=20.0, solver="FGMRES", nDirections=1, innerDirections=30, precision="mixed",
+                                  orthogonalisation="CGS", graphCapture=graph)
+        g = hbCase.from_cases([dataclasses.replace(c, fvSolution=sol) for c in insts], D)
+        out[graph] = (g.pseudo_steps(5), [g.fields(J)["W"] for J in range(len(insts))])
+        del g
+    np.testing.assert_array_equal(out[True][0], out[False][0])
+    for a, b in zip(out[True][1], out[False][1]):
+        np.testing.assert_array_equal(a, b)
