@@ -246,19 +246,27 @@ __global__ void k_bound_flag(int nC, double lb, const double* W, double* bad) {
 
 // boundLocalTimeStep.H:28-97: internal faces, then cyclic faces (the far
 // cell through patchNeighbourField).
+// Over the cell's internal and cyclic faces (the factor is a minimum of
+// 0.5 and 0.75, independent of the face order): the neighbours come from
+// the cell's matrix slots and cyclic-face list, read contiguously.
 __global__ void k_bound_apply(MeshView m, const double* bad, double* co) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     double f = 1.0;
-    for (int q = m.cfOff[c]; q < m.cfOff[c + 1]; ++q) {
-        const int fc = m.cf[q];
-        int other;
-        if (fc < m.nI) other = m.own[fc] == c ? m.nei[fc] : m.own[fc];
-        else if (m.nCyc > 0 && m.cycOf[fc - m.nI] >= 0) other = m.cycCell[m.cycOf[fc - m.nI]];
-        else continue;
-        if (bad[c] != 0.0) f = fmin(0.5, f);
+    const bool badC = bad[c] != 0.0;
+    const int len = m.rowLen[c];
+    const size_t rs = (size_t)m.slicePtr[c >> 6] * 64 + (c & 63);
+    for (int k = 0; k < len; ++k) {
+        const int other = m.col[rs + 64 * (size_t)k];
+        if (badC) f = fmin(0.5, f);
         if (bad[other] != 0.0) f = fmin(0.75, f);
     }
+    if (m.nCyc > 0)
+        for (int q = m.cycRowOff[c]; q < m.cycRowOff[c + 1]; ++q) {
+            const int other = m.cycCell[m.cycRowK[q]];
+            if (badC) f = fmin(0.5, f);
+            if (bad[other] != 0.0) f = fmin(0.75, f);
+        }
     co[c] *= f;
 }
 
