@@ -99,3 +99,25 @@ def test_box_128x64x64_mixed_solve_converges_to_fp64_solution():
     err = rel_err(out["mixed"], out["fp64"])
     print(f"[config] mixed vs fp64 increment (tight): {err:.2e}", flush=True)
     assert err < 1e-5
+
+
+def test_hb_sector_64x64x32_seven_instants():
+    """The harmonic-balance configuration at its own size: 64x64x32 annular
+    sector with rotational cyclic patches, 3 harmonics = 7 instants,
+    inviscid Roe, cells permuted then RCM-renumbered.  The coupled residual
+    (7 x 131 072 rows, spectral source included) equals the oracle bit for
+    bit; one FP64 pseudo-step (GMRES(30), MGS, LU-SGS) follows the oracle."""
+    from icsfoam_amd.solver import hbCase
+    from oracle.oracle import OracleHB
+
+    print("[config] HB sector 64x64x32 x 7", flush=True)
+    insts, D, _ = cases.case_hb(n=(64, 64, 32), n_harmonics=3, flux="ROE", orthogonalisation="MGS")
+    insts = [prepare_case(c, permute_seed=2024, colour_major=False) for c in insts]
+    g, o = hbCase.from_cases(insts, D), OracleHB(insts, D)
+    np.testing.assert_array_equal(g.residual(), o.residual())
+    print("[config] HB residual bitwise", flush=True)
+    g.pseudo_steps(1)
+    o.pseudo_steps(1)
+    err = max(rel_err(g.fields(J)["W"], o.fields(J)["W"]) for J in range(len(insts)))
+    print(f"[config] HB one FP64 step vs oracle: {err:.2e}", flush=True)
+    assert err < 1e-9
