@@ -14,6 +14,9 @@
   stop at different iterates (increments about 2.5e-2 apart relative to
   the increment, states 2.7e-3); solved to relTol 1e-8, the mixed
   increment equals the FP64 one to 1e-5 (2.2e-7 measured).
+* 64x64x32 annular sector with rotational cyclic patches x 7 instants
+  (harmonic balance): coupled residual bit for bit, one FP64 pseudo-step
+  within 1e-9 (4.3e-16 measured).
 """
 import dataclasses
 
