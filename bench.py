@@ -218,6 +218,52 @@ def cpu_baseline(n_sample: int, n_full: int, steps: int = 2):
                        f"cell ratio {n_sample ** 3}/{n_full ** 3}; host CPU: {cpu}; nproc {os.cpu_count()}")
 
 
+def bench_bump(steps=50):
+    """The reference's CPU-runnable case: 2-D transonic bump, 200x50x1
+    cells, inviscid AUSM+up with linearUpwind + cellLimited<Venkatakrishnan>,
+    GMRES(30) (MGS, the reference's order) with LU-SGS, local CFL 50, FP64,
+    cells renumbered RCM + colour-major (renumberMesh's role; in blockMesh
+    order the sequential LU-SGS sweep has 250 dependency levels, i.e. 500
+    launches per application); `steps` pseudo-steps timed on the GPU (one
+    residual read per step as the reference's loop does, steps from the
+    third on replayed from the captured graph) and, beside them, the same
+    steps of the oracle on one host core (the whole run, no sampling).  At
+    10 000 cells the GPU step is launch-bound."""
+    from icsfoam_amd import cases, dbnsCase
+    from icsfoam_amd.mesh import prepare_case
+    from oracle import oracle as O
+    from oracle.build import FAST_LIB
+    from oracle.build import build as build_oracle
+
+    c = prepare_case(cases.case_bump(200, 50, seed=1))
+    g = dbnsCase.from_case(c)
+    g.pseudo_steps(3)
+    g.synchronize()
+    t0 = time.perf_counter()
+    hg = g.pseudo_steps(steps)
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"[bench] bump GPU: {steps} steps in {dt:.3f}s; final residuals {hg[-1, :5]}")
+    del g
+    if not os.path.exists(FAST_LIB):
+        build_oracle(fast=True)
+    O.use_library(FAST_LIB)
+    o = O.OracleCase(c)
+    o.pseudo_steps(3)
+    t0 = time.perf_counter()
+    o.pseudo_steps(steps)
+    dtc = time.perf_counter() - t0
+    log(f"[bench] bump oracle: {steps} steps in {dtc:.3f}s")
+    return {"metric": "implicit pseudo-steps/s, 2-D bump 200x50 (FP64)", "value": round(steps / dt, 3),
+            "unit": "pseudo-steps/s", "n_gpus": 1, "steps": steps, "warmup": 3, "ms_per_step": round(1e3 * dt / steps, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (bump channel, M 0.675, 1% seeded 8-mode perturbation)",
+            "config": {"workload": "200x50x1 bump (10000 cells), AUSM+up, GMRES(30) MGS + LU-SGS, CFL 50, "
+                                   "RCM + colour-major"},
+            "cpu_baseline": {"value": round(steps / dtc, 3), "unit": "pseudo-steps/s", "cores": 1, "kind": "port",
+                             "sample": f"oracle (fast build) on the same case, the same {steps} steps"}}
+
+
 def bench_hb(steps, warmup):
     """Harmonic-balance pseudo-steps/s on a 64x64x32 annular sector with
     rotational cyclic theta patches and 7 instants (3 harmonics of 100 Hz),
@@ -345,9 +391,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed", help="converge: forms to run, in order")
     ap.add_argument("--time-limit", type=float, default=400.0, help="converge: seconds per form")
-    ap.add_argument("--workload", default="c5", choices=["c5", "hb", "vjp", "converge"],
+    ap.add_argument("--workload", default="c5", choices=["c5", "hb", "vjp", "converge", "bump"],
                     help="c5: the headline 160^3 pseudo-step (default); hb: 7-instant harmonic balance; "
-                         "vjp: residual adjoint at 100^3")
+                         "vjp: residual adjoint at 100^3; bump: the 200x50 bump, 50 steps, GPU and oracle")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the pseudo-step on exactly one GPU; --gpus {args.gpus} is not "
@@ -355,6 +401,9 @@ def main():
         sys.exit(2)
     if args.workload == "hb":
         print(json.dumps(bench_hb(args.steps, args.warmup)), flush=True)
+        return
+    if args.workload == "bump":
+        print(json.dumps(bench_bump(max(args.steps, 50))), flush=True)
         return
     if args.workload == "vjp":
         print(json.dumps(bench_vjp(max(args.steps, 10))), flush=True)
