@@ -425,7 +425,9 @@ int residualStage(icsf_case* c, bool cellSums = true) {
         Timed t(c, TK_RESIDUAL);
         launchResidual(c->mv, c->ph, c->sv, c->st, cellSums);
     }
-    c->haveResidual = true;
+    // without the cell sums b and rDT are complete only after
+    // jacobianStage(c, true)
+    c->haveResidual = cellSums;
     return ICSF_OK;
 }
 
@@ -435,6 +437,7 @@ int jacobianStage(icsf_case* c, bool cellSums = false) {
         if (cellSums) launchCellSumsJacobian(c->mv, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, c->derr, c->st);
         else launchJacobian(c->mv, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, nullptr, c->derr, c->st);
     }
+    if (cellSums) c->haveResidual = true;
     if (c->precond == 1) launchJacobiSetup(c->mv.nC, c->blkFp32, c->dg, c->dinv, c->st);
     c->assembled = true;
     return ICSF_OK;
@@ -620,6 +623,8 @@ int pseudoStep(icsf_case* c, int& nIterOut) {
         Timed t(c, 5);
         launchUpdate(m, c->ph, c->sv, c->dW, 1, nullptr, nullptr, nullptr, c->st);
     }
+    // the residual and the matrix belong to the state before the update
+    c->haveResidual = c->assembled = false;
     return ICSF_OK;
 }
 
