@@ -256,13 +256,21 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
         mx[k] = mn[k] = s.fld[(size_t)c * NFLD + k];
         imx[k] = imn[k] = -1;
     }
+    // internal faces first, in face order = the cell's slot order (column,
+    // orientation and Cf from the slots), then the boundary faces
+    const int len = m.rowLen[c], nLo = m.nLower[c];
+    const size_t rslot = (size_t)m.slicePtr[c >> 6] * 64 + (c & 63);
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
-        const V3 d = ld3(m.Cf + 3 * (size_t)f) - Cc;
+        const int j = q - e0;
+        const size_t sk = rslot + 64 * (size_t)j;
+        const V3 d = (j < len ? V3{m.sgeo[sgOff(sk, SG_CX)], m.sgeo[sgOff(sk, SG_CX + 1)],
+                                   m.sgeo[sgOff(sk, SG_CX + 2)]}
+                              : ld3(m.Cf + 3 * (size_t)f)) - Cc;
         const double* vo;
         int id;
         if (f < m.nI) {
-            const bool isOwn = m.own[f] == c;
+            const bool isOwn = j >= nLo;
             const double* lr = a.fLR + (size_t)f * 2 * NFLD + (isOwn ? 0 : NFLD);
 #pragma unroll
             for (int k = 0; k < NFLD; ++k) {
@@ -271,7 +279,7 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
                 qd[k] += av;
                 if (rec) gl[k] = gl[k] + av * d;
             }
-            id = isOwn ? m.nei[f] : m.own[f];
+            id = m.col[sk];
             vo = s.fld + (size_t)id * NFLD;
         } else {
             const int b = f - m.nI;
@@ -477,12 +485,17 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
     for (int k = 0; k < NFLD; ++k) q[k] = a.qc[(size_t)c * NFLD + k];
     double pbnd = 0.0, Tbnd = 0.0;
     V3 Ubnd{0, 0, 0};
+    const int nLo = m.nLower[c];
+    const size_t rslot = (size_t)m.slicePtr[c >> 6] * 64 + (c & 63);
     for (int e = e0; e < e1; ++e) {
         const int f = m.cf[e];
         if (f < m.nI) {
-            const bool isOwn = m.own[f] == c;
-            const int j = isOwn ? m.nei[f] : m.own[f];
-            const double wt = isOwn ? m.w[f] : 1.0 - m.w[f];
+            // internal faces: the cell's slots in face order
+            const size_t sk = rslot + 64 * (size_t)(e - e0);
+            const bool isOwn = e - e0 >= nLo;
+            const int j = m.col[sk];
+            const double wf = m.sgeo[sgOff(sk, SG_W)];
+            const double wt = isOwn ? wf : 1.0 - wf;
             const double* qf = a.qf + (size_t)f * NFLD;
 #pragma unroll
             for (int k = 0; k < NFLD; ++k) {
