@@ -74,8 +74,11 @@ struct HBInst {
     double* rD;
     int* err;
 };
-void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st);
-void launchJacobianHB(const HBInst* inst, int nO, int fp32, const MeshView& m, const Phys& ph, hipStream_t st);
+// cellSums: as launchResidual / launchCellSumsJacobian, for all instances
+void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st,
+                      bool cellSums = true);
+void launchJacobianHB(const HBInst* inst, int nO, int fp32, const MeshView& m, const Phys& ph, hipStream_t st,
+                      bool cellSums = false);
 
 // Harmonic-balance coupling: per-instance block storage (FP64) and the
 // spectral operator D [nO][nO] (device).

@@ -796,10 +796,10 @@ __global__ void __launch_bounds__(256, W) k_cell_jacobian(MeshView m, Phys ph, S
                                                           int* __restrict__ err) {
     cellJacobianBody<TB, RES>(m, ph, s, blk, dg, rD, err);
 }
-template <typename TB>
+template <typename TB, bool RES = false>
 __global__ void __launch_bounds__(256) k_cell_jacobian_hb(const HBInst* __restrict__ I, Phys ph) {
     const HBInst q = I[blockIdx.y];
-    cellJacobianBody<TB, false>(q.m, ph, q.s, (TB*)q.blk, (TB*)q.dg, q.rD, q.err);
+    cellJacobianBody<TB, RES>(q.m, ph, q.s, (TB*)q.blk, (TB*)q.dg, q.rD, q.err);
 }
 
 }  // namespace
@@ -829,18 +829,26 @@ void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipSt
     if (cellSums) k_cell_residual<<<gc, bc, 0, st>>>(m, ph, s);
 }
 
-void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st) {
+void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st,
+                      bool cellSums) {
     if (m.nB > 0) k_boundary_hb<<<dim3(nblk(m.nB, 256), nO), 256, 0, st>>>(inst, ph);
     const int bc = m.bsCell, bf = m.bsFace;
     if (ph.flux == FLUX_ROE) k_grad_hb<7><<<dim3(xcdGrid(m.nC, bc), nO), bc, 0, st>>>(inst, ph);
     else k_grad_hb<8><<<dim3(xcdGrid(m.nC, bc), nO), bc, 0, st>>>(inst, ph);
     k_face_terms_hb<<<dim3(xcdGrid(m.nF, bf), nO), bf, (size_t)bf * (NFT + 1) * sizeof(double), st>>>(inst, ph);
-    k_cell_residual_hb<<<dim3(xcdGrid(m.nC, bc), nO), bc, 0, st>>>(inst, ph);
+    if (cellSums) k_cell_residual_hb<<<dim3(xcdGrid(m.nC, bc), nO), bc, 0, st>>>(inst, ph);
 }
 
-void launchJacobianHB(const HBInst* inst, int nO, int fp32, const MeshView& m, const Phys& ph, hipStream_t st) {
-    if (fp32) k_cell_jacobian_hb<float><<<dim3(xcdGrid(m.nC, m.bsJac), nO), m.bsJac, 0, st>>>(inst, ph);
-    else k_cell_jacobian_hb<double><<<dim3(xcdGrid(m.nC, m.bsJac), nO), m.bsJac, 0, st>>>(inst, ph);
+void launchJacobianHB(const HBInst* inst, int nO, int fp32, const MeshView& m, const Phys& ph, hipStream_t st,
+                      bool cellSums) {
+    const dim3 g(xcdGrid(m.nC, m.bsJac), nO);
+    if (cellSums) {
+        if (fp32) k_cell_jacobian_hb<float, true><<<g, m.bsJac, 0, st>>>(inst, ph);
+        else k_cell_jacobian_hb<double, true><<<g, m.bsJac, 0, st>>>(inst, ph);
+    } else {
+        if (fp32) k_cell_jacobian_hb<float><<<g, m.bsJac, 0, st>>>(inst, ph);
+        else k_cell_jacobian_hb<double><<<g, m.bsJac, 0, st>>>(inst, ph);
+    }
 }
 
 void launchAdjointForward(const MeshView& m, const Phys& ph, const StateView& s, double* gun, double* limo,

@@ -915,19 +915,28 @@ int hbUploadBC(icsf_hb* h) {
 
 // Residuals of all instances, each kernel launched once over the
 // instances (blockIdx.y), then the spectral source.
-int hbResidualStage(icsf_hb* h) {
+// cellSums false (pseudo-step): the instances' cell sums and the spectral
+// source are left to hbJacobianStage(h, true), which forms the cell sums in
+// the Jacobian kernel's face walk and then adds the source.
+int hbResidualStage(icsf_hb* h, bool cellSums = true) {
     const MeshView& m = h->inst[0]->mv;
     RC(hbUploadBC(h));
-    launchResidualHB(h->dInst, h->nO, m, h->inst[0]->ph, h->st);
-    for (icsf_case* c : h->inst) c->haveResidual = true;
-    k_hb_source<<<dim3(nblk(m.nC, 256), h->nO), 256, 0, h->st>>>(m.nC, h->nO, h->dInst, h->dD, m.V, h->Wall,
-                                                                  h->ball);
+    launchResidualHB(h->dInst, h->nO, m, h->inst[0]->ph, h->st, cellSums);
+    for (icsf_case* c : h->inst) c->haveResidual = cellSums;
+    if (cellSums)
+        k_hb_source<<<dim3(nblk(m.nC, 256), h->nO), 256, 0, h->st>>>(m.nC, h->nO, h->dInst, h->dD, m.V, h->Wall,
+                                                                      h->ball);
     return ICSF_OK;
 }
 
-int hbJacobianStage(icsf_hb* h) {
+int hbJacobianStage(icsf_hb* h, bool cellSums = false) {
     const MeshView& m = h->inst[0]->mv;
-    launchJacobianHB(h->dInst, h->nO, h->hbb.fp32, m, h->inst[0]->ph, h->st);
+    launchJacobianHB(h->dInst, h->nO, h->hbb.fp32, m, h->inst[0]->ph, h->st, cellSums);
+    if (cellSums) {
+        for (icsf_case* c : h->inst) c->haveResidual = true;
+        k_hb_source<<<dim3(nblk(m.nC, 256), h->nO), 256, 0, h->st>>>(m.nC, h->nO, h->dInst, h->dD, m.V, h->Wall,
+                                                                      h->ball);
+    }
     for (icsf_case* c : h->inst) c->assembled = true;
     HBDiag hd{};
     for (int J = 0; J < h->nO; ++J) hd.dg[J] = h->inst[J]->dg;
@@ -1064,8 +1073,9 @@ int hbPseudoStep(icsf_hb* h) {
         k_copyn<<<1, 5 * HB_MAXO, 0, h->st>>>(5 * h->nO, h->dres, h->dres + 5 * h->nO);
         h->havePrev = true;
     }
-    RC(hbResidualStage(h));
-    RC(hbJacobianStage(h));
+    const bool fc = c0->fuseCells;
+    RC(hbResidualStage(h, !fc));
+    RC(hbJacobianStage(h, fc));
     int nIter = 0;
     if (h->solver == 1) RC(hbSolveMixed(h, h->ball, nIter));
     else RC(hbSolve(h, h->ball, nIter));

@@ -113,6 +113,28 @@ def test_hb_fused_operator_matches_unfused():
     assert rel_err(rf, ru) < 1e-12
 
 
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_hb_fused_cell_sums_give_the_same_bits(form):
+    """The HB pseudo-step forms the instances' residual cell sums in the
+    Jacobian kernel's face walk and adds the spectral source afterwards:
+    same residual history and states bit for bit as the separate kernels."""
+    print(f"[hb fused cell sums {form}]", flush=True)
+    insts, D, t = cases.case_hb(n=(8, 6, 5), n_harmonics=2, flux="ROE")
+    insts = [prepare_case(c, permute_seed=17) for c in insts]
+    kw = dict(co=20.0)
+    if form == "mixed":
+        kw.update(solver="FGMRES", nDirections=1, precision="mixed", innerDirections=20)
+    out = []
+    for fused in (True, False):
+        sol = cases.solution_dict(fusedCellSums=fused, **kw)
+        g = hbCase.from_cases([dataclasses.replace(c, fvSolution=sol) for c in insts], D)
+        out.append((g.pseudo_steps(4), [g.fields(J)["W"] for J in range(len(insts))]))
+        del g
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_single_instant_equals_single_case():
     """Same kernels, same order: bit-for-bit (the single-instance solver
     with the unfused Arnoldi step, which the HB solver uses)."""
