@@ -1088,7 +1088,10 @@ int hbPseudoStep(icsf_hb* h) {
             k_bound_apply<<<nblk(m.nC, 256), 256, 0, h->st>>>(c->mv, c->fac, c->sv.pseudoCo);
         }
         launchUpdate(m, c->ph, c->sv, h->dW + J * vs, 1, nullptr, nullptr, nullptr, h->st);
+        // the residual and the matrix belong to the state before the update
+        c->haveResidual = c->assembled = false;
     }
+    h->assembled = false;
     return ICSF_OK;
 }
 
