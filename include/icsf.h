@@ -257,7 +257,11 @@ int icsf_solve_for_incr(icsf_case* c, const double* W, const double* b, double* 
 /* nSteps steady pseudo-time steps of dbnsFoam (dbnsFoam.C:110-124:
  * outerLoop.H + updateFields.H, SER of setCoAndDeltaT.H:3-37, bounding of
  * boundLocalTimeStep.H).  residuals: [nSteps][10] (may be NULL), nIter:
- * [nSteps] (may be NULL).  Residuals are read back only when requested. */
+ * [nSteps] (may be NULL).  Residuals are read back only when requested.
+ * Afterwards the state is the updated one: the residual and the matrix of
+ * the last step refer to the state before its update, so the matrix calls
+ * (icsf_solve_for_incr, icsf_matrix_mul, ...) need icsf_residual and
+ * icsf_assemble_jacobian again (ICSF_ERR_STATE otherwise). */
 int icsf_pseudo_steps(icsf_case* c, int nSteps, double* residuals, int* nIter);
 
 /* Pseudo-Courant field [nCells] (pseudoCoField). */
@@ -320,7 +324,10 @@ int icsf_hb_precondition(icsf_hb* h, double* r);
  * outerLoop.H:213-219); residuals [nInstants][10] may be NULL. */
 int icsf_hb_solve_for_incr(icsf_hb* h, const double* b, double* dW, double* residuals, int* nIter);
 /* nSteps HB pseudo-steps (dbnsFullyImplicitHBFoam/outerLoop.H:1-230 +
- * updateFields.H per instance); residuals [nSteps][nInstants][10] or NULL. */
+ * updateFields.H per instance); residuals [nSteps][nInstants][10] or NULL.
+ * As icsf_pseudo_steps, the residuals and the matrix refer to the states
+ * before the last update: the matrix calls need icsf_hb_residual and
+ * icsf_hb_assemble_jacobian again. */
 int icsf_hb_pseudo_steps(icsf_hb* h, int nSteps, double* residuals);
 int icsf_hb_synchronize(icsf_hb* h);
 
