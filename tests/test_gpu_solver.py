@@ -230,6 +230,24 @@ def test_mixed_pseudo_steps_vs_fp64_oracle(which):
     assert hm[-1][0] < 1.5 * ho[-1][0] and hm[-1][1] < 1.5 * ho[-1][1]
 
 
+def test_matrix_calls_after_pseudo_steps_need_a_new_assembly():
+    """After pseudo-steps the state is the updated one; the residual and the
+    matrix of the last step belong to the previous state, so the matrix
+    calls refuse until the residual and the Jacobian are formed again."""
+    from icsfoam_amd._lib import IcsfError
+
+    g = dbnsCase.from_case(box_small())
+    g.pseudo_steps(2)
+    with pytest.raises(IcsfError, match="not assembled"):
+        g.solveForIncr()
+    with pytest.raises(IcsfError, match="must run before"):
+        g.createJacobian()
+    g.residual()
+    g.createJacobian()
+    dW, _, _ = g.solveForIncr()
+    assert np.all(np.isfinite(dW))
+
+
 def test_case_after_timed_case_is_clean():
     """A case destroyed after event timing leaves no pending HIP error for
     the next case (its first step runs the autotuner, which checks)."""
