@@ -39,14 +39,18 @@ __device__ __forceinline__ double ld(const T* p, size_t i) { return (double)p[i]
 // arithmetic type A: read from full storage, or generated from the slot's
 // NGEN generators in the storage precision (the same bits as the block the
 // Jacobian kernel would have stored, for FP64).
-template <typename TB, typename A, bool GEN>
+// NT: non-temporal loads (the SpMV rows: each slot is read once per pass
+// and a full L2 pass lies between reads; keeping these lines out of L2
+// leaves it to the gathered vectors).
+template <typename TB, typename A, bool GEN, bool NT = false>
 __device__ __forceinline__ void slotBlock(const MeshView& m, const TB* __restrict__ blk, size_t sk, int lane,
                                           A b[NB]) {
     if (GEN) {
         const size_t gb = sk * NGEN * 64 + lane;
         TB g[NGEN], t[NB];
 #pragma unroll
-        for (int e = 0; e < NGEN; ++e) g[e] = blk[gb + (size_t)e * 64];
+        for (int e = 0; e < NGEN; ++e)
+            g[e] = NT ? __builtin_nontemporal_load(blk + gb + (size_t)e * 64) : blk[gb + (size_t)e * 64];
         blockFromGen<TB>(g, (TB)m.gamma, t);
 #pragma unroll
         for (int e = 0; e < NB; ++e) b[e] = (A)t[e];
@@ -108,9 +112,11 @@ __device__ __forceinline__ void spmvRow(const MeshView& m, const TB* __restrict_
     A fw[5][3];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-        const A b0 = (A)dg[diagOff(c, 5 * i)], b1 = (A)dg[diagOff(c, 5 * i + 1)];
-        const A b2 = (A)dg[diagOff(c, 5 * i + 2)], b3 = (A)dg[diagOff(c, 5 * i + 3)],
-                b4 = (A)dg[diagOff(c, 5 * i + 4)];
+        const A b0 = (A)__builtin_nontemporal_load(dg + diagOff(c, 5 * i)),
+                b1 = (A)__builtin_nontemporal_load(dg + diagOff(c, 5 * i + 1));
+        const A b2 = (A)__builtin_nontemporal_load(dg + diagOff(c, 5 * i + 2)),
+                b3 = (A)__builtin_nontemporal_load(dg + diagOff(c, 5 * i + 3)),
+                b4 = (A)__builtin_nontemporal_load(dg + diagOff(c, 5 * i + 4));
         acc[i][0] = b0 * xc[0];
         acc[i][1] = b1 * xc[1];
         acc[i][2] = b2 * xc[2] + b3 * xc[3] + b4 * xc[4];
@@ -131,7 +137,7 @@ __device__ __forceinline__ void spmvRow(const MeshView& m, const TB* __restrict_
             for (int q = 0; q < 5; ++q) d[q] = rd * y[(size_t)q * nC + j];
         }
         A bk[NB];
-        slotBlock<TB, A, GEN>(m, blk, base + k, lane, bk);
+        slotBlock<TB, A, GEN, true>(m, blk, base + k, lane, bk);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             const A b0 = bk[5 * i], b1 = bk[5 * i + 1];
@@ -221,9 +227,11 @@ __device__ __forceinline__ void spmvRowHB(const MeshView& m, const HBBlocks& hb,
     A acc[5][3], fw[5][3];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-        const A b0 = (A)dg[diagOff(c, 5 * i)], b1 = (A)dg[diagOff(c, 5 * i + 1)];
-        const A b2 = (A)dg[diagOff(c, 5 * i + 2)], b3 = (A)dg[diagOff(c, 5 * i + 3)],
-                b4 = (A)dg[diagOff(c, 5 * i + 4)];
+        const A b0 = (A)__builtin_nontemporal_load(dg + diagOff(c, 5 * i)),
+                b1 = (A)__builtin_nontemporal_load(dg + diagOff(c, 5 * i + 1));
+        const A b2 = (A)__builtin_nontemporal_load(dg + diagOff(c, 5 * i + 2)),
+                b3 = (A)__builtin_nontemporal_load(dg + diagOff(c, 5 * i + 3)),
+                b4 = (A)__builtin_nontemporal_load(dg + diagOff(c, 5 * i + 4));
         acc[i][0] = b0 * xc[0];
         acc[i][1] = b1 * xc[1];
         acc[i][2] = b2 * xc[2] + b3 * xc[3] + b4 * xc[4];
@@ -244,7 +252,7 @@ __device__ __forceinline__ void spmvRowHB(const MeshView& m, const HBBlocks& hb,
             for (int q = 0; q < 5; ++q) d[q] = rd * yJ[(size_t)q * nC + j];
         }
         A bk[NB];
-        slotBlock<TB, A, GEN>(m, blk, base + k, lane, bk);
+        slotBlock<TB, A, GEN, true>(m, blk, base + k, lane, bk);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             const A b0 = bk[5 * i], b1 = bk[5 * i + 1];
