@@ -606,7 +606,10 @@ __global__ void __launch_bounds__(256) k_norm2(int nC, const TV* __restrict__ x,
     blockReduceStore<1>(v, part);
 }
 
-// CGS projection partials: <w, V_j> for j < nv (nv <= 64, chunked by 16)
+// CGS projection partials: <w, V_j> for j < nv (nv <= 64, chunked by 16).
+// The basis vectors are streamed with non-temporal loads here and in the
+// update and combine kernels: each is read once per pass and the passes
+// between two reads of one vector move far more than L2 holds.
 template <typename TV, int CH>
 __global__ void __launch_bounds__(256) k_multidot(int nC, const TV* __restrict__ w, const TV* __restrict__ Vb,
                                                   size_t vstride, int j0, int nv, double* __restrict__ part) {
@@ -623,7 +626,7 @@ __global__ void __launch_bounds__(256) k_multidot(int nC, const TV* __restrict__
                 const TV* vj = Vb + (size_t)(j0 + k) * vstride;
                 double s = 0;
 #pragma unroll
-                for (int q = 0; q < 5; ++q) s += (double)(wv[q] * vj[(size_t)q * nC + c]);
+                for (int q = 0; q < 5; ++q) s += (double)(wv[q] * __builtin_nontemporal_load(vj + (size_t)q * nC + c));
                 v[k] += s;
             }
         }
@@ -658,7 +661,8 @@ __global__ void __launch_bounds__(256) k_cgs_update(int nC, TV* __restrict__ w, 
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll
-                for (int q = 0; q < 5; ++q) x[u][q] = Vb[(size_t)(j + u) * vstride + (size_t)q * nC + c];
+                for (int q = 0; q < 5; ++q)
+                    x[u][q] = __builtin_nontemporal_load(Vb + (size_t)(j + u) * vstride + (size_t)q * nC + c);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const TV hj = (TV)h[j + u];
@@ -746,7 +750,7 @@ __global__ void __launch_bounds__(256) k_update_sol(int nC, TX* __restrict__ dW,
     for (; j + 4 <= nv; j += 4) {
         TV x[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) x[u] = Vb[(size_t)(j + u) * vstride + i];
+        for (int u = 0; u < 4; ++u) x[u] = __builtin_nontemporal_load(Vb + (size_t)(j + u) * vstride + i);
 #pragma unroll
         for (int u = 0; u < 4; ++u) a += (TX)yh[j + u] * (TX)x[u];
     }
