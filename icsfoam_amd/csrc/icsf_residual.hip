@@ -144,11 +144,12 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
     const size_t rs = (size_t)m.slicePtr[c >> 6] * 64 + (c & 63);
     for (int j = 0; j < len; ++j) {
         const size_t sk = rs + 64 * (size_t)j;
-        const int other = m.col[sk];
+        const int other = ldnt(m.col + sk);
         const bool isOwn = j >= nLo;
         const int o = isOwn ? c : other, n = isOwn ? other : c;
-        const V3 S{m.sgeo[sgOff(sk, SG_SX)], m.sgeo[sgOff(sk, SG_SX + 1)], m.sgeo[sgOff(sk, SG_SX + 2)]};
-        const double w = m.sgeo[sgOff(sk, SG_W)];
+        const V3 S{ldnt(m.sgeo + sgOff(sk, SG_SX)), ldnt(m.sgeo + sgOff(sk, SG_SX + 1)),
+                   ldnt(m.sgeo + sgOff(sk, SG_SX + 2))};
+        const double w = ldnt(m.sgeo + sgOff(sk, SG_W));
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
             const double vo = s.fld[(size_t)o * NFLD + k], vn = s.fld[(size_t)n * NFLD + k];
@@ -231,7 +232,8 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
         const V3 Cc = ld3(m.C + 3 * (size_t)c);
         for (int j = 0; j < len; ++j) {
             const size_t sk = rs + 64 * (size_t)j;
-            const V3 d = V3{m.sgeo[sgOff(sk, SG_CX)], m.sgeo[sgOff(sk, SG_CX + 1)], m.sgeo[sgOff(sk, SG_CX + 2)]} - Cc;
+            const V3 d = V3{ldnt(m.sgeo + sgOff(sk, SG_CX)), ldnt(m.sgeo + sgOff(sk, SG_CX + 1)),
+                            ldnt(m.sgeo + sgOff(sk, SG_CX + 2))} - Cc;
 #pragma unroll
             for (int k = 0; k < NG; ++k)
                 limitFace(lim[k], mx[k], mn[k], dot(d, V3{gx[k], gy[k], gz[k]}), ph.limiter);
@@ -283,8 +285,8 @@ __device__ __forceinline__ double soundSpeed(const Phys& ph, double T) {
 // uses c* instead).
 __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, const StateView& s, int f, double& fr,
                                          double& fe, V3& fu, double* rec) {
-    const V3 S = ld3(m.Sf + 3 * (size_t)f);
-    const double ms = m.magSf[f];
+    const V3 S{ldnt(m.Sf + 3 * (size_t)f), ldnt(m.Sf + 3 * (size_t)f + 1), ldnt(m.Sf + 3 * (size_t)f + 2)};
+    const double ms = ldnt(m.magSf + f);
     double rL, rR, pL, pR, uxL, uxR, uyL, uyR, uzL, uzR, HL, HR, xL, xR, eL = 0, eR = 0;
     double gI = ph.gamma;  // fvc::interpolate(gamma): exact on internal faces
     if (f >= m.nI && m.bKind[f - m.nI] == PK_CYCLIC) {
@@ -301,7 +303,7 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
         else reconCyclic(m, s, ph.recon, f, o, kc, F_E, eL, eR);
         gI = m.w[f] * ph.gamma + (1.0 - m.w[f]) * ph.gamma;
     } else if (f < m.nI) {
-        const int o = m.own[f], n = m.nei[f];
+        const int o = ldnt(m.own + f), n = ldnt(m.nei + f);
         recon(m, s, ph.recon, f, o, n, F_UX, uxL, uxR);
         recon(m, s, ph.recon, f, o, n, F_UY, uyL, uyR);
         recon(m, s, ph.recon, f, o, n, F_UZ, uzL, uzR);
@@ -361,13 +363,13 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
     o[FT_CONV + 2] = fu.x;
     o[FT_CONV + 3] = fu.y;
     o[FT_CONV + 4] = fu.z;
-    const V3 S = ld3(m.Sf + 3 * (size_t)f);
-    const double ms = m.magSf[f];
+    const V3 S{ldnt(m.Sf + 3 * (size_t)f), ldnt(m.Sf + 3 * (size_t)f + 1), ldnt(m.Sf + 3 * (size_t)f + 2)};
+    const double ms = ldnt(m.magSf + f);
     const bool visc = ph.viscous;
     const double mu = ph.mu, alpha = ph.alpha;
     if (f < m.nI) {
-        const int ow = m.own[f], ne = m.nei[f];
-        const double w = m.w[f];
+        const int ow = ldnt(m.own + f), ne = ldnt(m.nei + f);
+        const double w = ldnt(m.w + f);
         const double co = soundSpeed(ph, s.T[ow]), cn = soundSpeed(ph, s.T[ne]);
         const double cf = w * (co - cn) + cn;
         const V3 Uo = cellU(s.fld, ow), Un = cellU(s.fld, ne);
@@ -389,7 +391,7 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
             const T9 tauf = w * (tauO - tauN) + tauN;
             const double muf = w * (mu - mu) + mu;
             const double alf = w * (alpha - alpha) + alpha;
-            const V3 k3 = ld3(m.corr + 3 * (size_t)f);
+            const V3 k3{ldnt(m.corr + 3 * (size_t)f), ldnt(m.corr + 3 * (size_t)f + 1), ldnt(m.corr + 3 * (size_t)f + 2)};
             const V3 sn = m.nodc[f] * (Un - Uo) + vdott(k3, gUf);
             const V3 lap = (muf * sn) * ms;
             const V3 dt = vdott(S, tauf);
@@ -477,7 +479,7 @@ __device__ __forceinline__ void faceTermsBlock(const MeshView& m, const Phys& ph
     double* g = s.ft + (size_t)f0 * NFT;
     for (int i = threadIdx.x; i < nRec * NFT; i += bs) {
         const int r = i / NFT, k = i - r * NFT;
-        g[i] = shFT[r * (NFT + 1) + k];
+        stnt(g + i, shFT[r * (NFT + 1) + k]);
     }
 }
 
@@ -635,8 +637,9 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
         const size_t sk = rs + 64 * (size_t)slot;
         const bool isOwn = slot >= nLo;
         const double* t = s.ft + (size_t)f * NFT;
-        const V3 S{m.sgeo[sgOff(sk, SG_SX)], m.sgeo[sgOff(sk, SG_SX + 1)], m.sgeo[sgOff(sk, SG_SX + 2)]};
-        const double ms = m.sgeo[sgOff(sk, SG_MAG)];
+        const V3 S{ldnt(m.sgeo + sgOff(sk, SG_SX)), ldnt(m.sgeo + sgOff(sk, SG_SX + 1)),
+                   ldnt(m.sgeo + sgOff(sk, SG_SX + 2))};
+        const double ms = ldnt(m.sgeo + sgOff(sk, SG_MAG));
         const V3 nrm = S / ms;
         // own side: block = upper (A_R), diagonal -= lower (A_L);
         // neighbour side: block = lower (A_L), diagonal -= upper (A_R)
@@ -651,7 +654,7 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
             const size_t gb = ((rowSlot0 + slot) * NGEN) * 64 + lane;
             const double gv[NGEN] = {t[bs], t[bs + 1], t[bs + 2], t[bs + 3], nrm.x, nrm.y, nrm.z, hb, v, sv};
 #pragma unroll
-            for (int e = 0; e < NGEN; ++e) blk[gb + (size_t)e * 64] = (TB)gv[e];
+            for (int e = 0; e < NGEN; ++e) stnt(blk + gb + (size_t)e * 64, (TB)gv[e]);
         } else {
             fluxJacobian(g, V3{t[bs], t[bs + 1], t[bs + 2]}, t[bs + 3], nrm, A);
             const size_t base = ((rowSlot0 + slot) * NB) * 64 + lane;
@@ -778,7 +781,7 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
     }
     if (visc) { D[0] = D[0] - vis; D[12] = D[12] - vis; D[18] = D[18] - vis; D[24] = D[24] - vis; D[6] = D[6] - vis; }
 #pragma unroll
-    for (int e = 0; e < NB; ++e) dg[diagOff(c, e)] = (TB)D[e];
+    for (int e = 0; e < NB; ++e) stnt(dg + diagOff(c, e), (TB)D[e]);
     // LU-SGS scalar diagonal
     double r = kGREAT;
     r = 1.0 / fmax(1.0 / r, fabs(D[0]));

@@ -147,6 +147,14 @@ constexpr int NGEN = 10;
 __host__ __device__ __forceinline__ size_t genOff(size_t slot, int e) {
     return ((slot >> 6) * NGEN + (size_t)e) * 64 + (slot & 63);
 }
+// Non-temporal loads and stores for data streamed once per pass (mesh
+// geometry in face or slot order, face records, matrix output): they keep
+// L2 for the neighbour gathers.  Values are unchanged.
+template <typename T>
+__device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void stnt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 // Slot geometry (MeshView::sgeo) of flat slot index `slot`, entry e.
 constexpr int NSG = 8;
 enum { SG_SX = 0, SG_W = 3, SG_CX = 4, SG_MAG = 7 };
