@@ -107,7 +107,7 @@ __device__ __forceinline__ void spmvRow(const MeshView& m, const TB* __restrict_
     for (int k = 0; k < 5; ++k) xc[k] = X(k, c);
     if (vout)
 #pragma unroll
-        for (int k = 0; k < 5; ++k) vout[(size_t)k * nC + c] = xc[k];
+        for (int k = 0; k < 5; ++k) stnt(vout + (size_t)k * nC + c, xc[k]);
     A acc[5][3];
     A fw[5][3];
 #pragma unroll
