@@ -127,7 +127,7 @@ __device__ __forceinline__ void spmvRow(const MeshView& m, const TB* __restrict_
     const size_t base = (size_t)m.slicePtr[sl];
     for (int k = 0; k < len; ++k) {
         const size_t slot = (base + k) * 64 + lane;
-        const int j = m.col[slot];
+        const int j = ldnt(m.col + slot);
         A xj[5], d[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) xj[q] = X(q, j);
@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(256, W) k_spmv_cells(MeshView m, const TB* __r
                                                     const int* __restrict__ cells, int n) {
     const int t = xcdBlock() * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    spmvRow<TB, TV, FWD, GEN>(m, blk, dg, x, scal, vout, y, rD, cells[t]);
+    spmvRow<TB, TV, FWD, GEN>(m, blk, dg, x, scal, vout, y, rD, ldnt(cells + t));
 }
 
 // Harmonic-balance SpMV: row cell c of instance J = blockIdx.y.  The own
@@ -242,7 +242,7 @@ __device__ __forceinline__ void spmvRowHB(const MeshView& m, const HBBlocks& hb,
     const size_t base = (size_t)m.slicePtr[sl];
     for (int k = 0; k < len; ++k) {
         const size_t slot = (base + k) * 64 + lane;
-        const int j = m.col[slot];
+        const int j = ldnt(m.col + slot);
         A xj[5], d[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) xj[q] = X(J, q, j);
