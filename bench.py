@@ -380,6 +380,51 @@ def bench_converge(n, max_steps, chunk=25, drop=1e-8, time_limit=400.0, forms=("
             "config": dict({"workload": f"{n}^3 jittered hex box, viscous Roe, local CFL 50"}, **out)}
 
 
+def bench_hb_converge(max_steps, chunk=25, drop=1e-8, time_limit=300.0):
+    """Harmonic balance (64x64x32 cyclic sector, 7 instants): time to a
+    `drop` reduction of the coupled residual ||R V||_2 (all instances, all
+    five components, spectral source included) for the FP64 and the mixed
+    form of the benchmark's HB case."""
+    from icsfoam_amd import cases
+    from icsfoam_amd.mesh import prepare_case
+    from icsfoam_amd.solver import hbCase
+
+    insts, D, _ = cases.case_hb(n=(64, 64, 32), n_harmonics=3, flux="ROE", orthogonalisation="CGS")
+    insts = [prepare_case(c, permute_seed=2024) for c in insts]
+    forms = {"fp64": cases.solution_dict(co=20.0, orthogonalisation="CGS"),
+             "mixed": cases.solution_dict(co=20.0, solver="FGMRES", nDirections=1, innerDirections=30,
+                                          precision="mixed", orthogonalisation="CGS")}
+    out = {}
+    for form, sol in forms.items():
+        g = hbCase.from_cases([dataclasses.replace(c, fvSolution=sol) for c in insts], D)
+        b = g.residual()
+        r0 = float(np.sqrt(np.sum(b * b)))
+        steps, hist = 0, [(0, 0.0, 1.0)]
+        g.synchronize()
+        t0 = time.perf_counter()
+        while steps < max_steps:
+            g.pseudo_steps(chunk, residuals=False)
+            steps += chunk
+            b = g.residual()
+            rel = float(np.sqrt(np.sum(b * b))) / r0
+            el = time.perf_counter() - t0
+            hist.append((steps, round(el, 3), rel))
+            log(f"[bench] HB {form}: {steps} steps {el:.1f}s  ||R||/||R0|| = {rel:.3e}")
+            if rel < drop or not np.isfinite(rel) or el > time_limit:
+                break
+        out[form] = dict(steps=steps, seconds=round(time.perf_counter() - t0, 3), drop=hist[-1][2],
+                         history=hist[::max(1, len(hist) // 20)])
+        del g
+    head = out["mixed"]
+    return {"metric": f"time to a {drop:g} drop of the coupled HB residual (fp64 vs mixed)",
+            "value": head["seconds"], "unit": "s", "n_gpus": 1, "steps": head["steps"], "warmup": 0,
+            "ms_per_step": round(1e3 * head["seconds"] / max(head["steps"], 1), 3), "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64 residual/outer + f32 Jacobian/inner",
+            "data": "synthetic (seeded annular sector, periodic inflow)",
+            "config": dict({"workload": "64x64x32 annular sector with rotational cyclic patches x 7 instants, "
+                                        "Roe inviscid, CFL 20"}, **out)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -391,9 +436,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed", help="converge: forms to run, in order")
     ap.add_argument("--time-limit", type=float, default=400.0, help="converge: seconds per form")
-    ap.add_argument("--workload", default="c5", choices=["c5", "hb", "vjp", "converge", "bump"],
+    ap.add_argument("--workload", default="c5", choices=["c5", "hb", "vjp", "converge", "bump", "hbconverge"],
                     help="c5: the headline 160^3 pseudo-step (default); hb: 7-instant harmonic balance; "
-                         "vjp: residual adjoint at 100^3; bump: the 200x50 bump, 50 steps, GPU and oracle")
+                         "vjp: residual adjoint at 100^3; bump: the 200x50 bump, 50 steps, GPU and oracle; "
+                         "hbconverge: HB time to a 1e-8 drop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the pseudo-step on exactly one GPU; --gpus {args.gpus} is not "
@@ -401,6 +447,9 @@ def main():
         sys.exit(2)
     if args.workload == "hb":
         print(json.dumps(bench_hb(args.steps, args.warmup)), flush=True)
+        return
+    if args.workload == "hbconverge":
+        print(json.dumps(bench_hb_converge(max(args.steps, 25), time_limit=args.time_limit)), flush=True)
         return
     if args.workload == "bump":
         print(json.dumps(bench_bump(max(args.steps, 50))), flush=True)
