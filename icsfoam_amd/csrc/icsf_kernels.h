@@ -136,7 +136,7 @@ void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipSt
 void launchFaceFlux(const MeshView& m, const Phys& ph, const StateView& s, double* phi, double* phiU, double* phiE,
                     hipStream_t st);
 void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
-                    double* rD, double* dg64, int* err, hipStream_t st);
+                    double* rD, int* err, hipStream_t st);
 // Cell sums of the residual (b, rPseudoDeltaT) and the Jacobian rows in one
 // kernel, after launchResidual(..., cellSums = false); same results as
 // k_cell_residual followed by launchJacobian.

@@ -884,8 +884,7 @@ static void launchCellJacobian(const MeshView& m, const Phys& ph, const StateVie
     }
 }
 void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,
-                    double* rD, double* dg64, int* err, hipStream_t st) {
-    (void)dg64;
+                    double* rD, int* err, hipStream_t st) {
     launchCellJacobian<false>(m, ph, s, blk, dg, fp32, rD, err, st);
 }
 void launchCellSumsJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,

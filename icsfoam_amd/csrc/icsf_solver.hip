@@ -435,7 +435,7 @@ int jacobianStage(icsf_case* c, bool cellSums = false) {
     {
         Timed t(c, TK_JACOBIAN);
         if (cellSums) launchCellSumsJacobian(c->mv, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, c->derr, c->st);
-        else launchJacobian(c->mv, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, nullptr, c->derr, c->st);
+        else launchJacobian(c->mv, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, c->derr, c->st);
     }
     if (cellSums) c->haveResidual = true;
     if (c->precond == 1) launchJacobiSetup(c->mv.nC, c->blkFp32, c->dg, c->dinv, c->st);
@@ -567,7 +567,7 @@ int autotune(icsf_case* c) {
     pick2(m.bsCell, m.occCell, [&] { launchResidual(m, c->ph, c->sv, c->st, !fc); });
     pick2(m.bsJac, m.occJac, [&] {
         if (fc) launchCellSumsJacobian(m, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, c->derr, c->st);
-        else launchJacobian(m, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, nullptr, c->derr, c->st);
+        else launchJacobian(m, c->ph, c->sv, c->blk, c->dg, c->blkFp32, c->rD, c->derr, c->st);
     });
     // the Arnoldi operator in the precision of the inner iteration
     const int v32 = c->mixed ? 1 : 0;
