@@ -132,12 +132,17 @@ def check(rc: int) -> None:
 
 
 def ptr(a):
-    """Data pointer of a contiguous numpy array (None passes NULL)."""
+    """Data pointer of a contiguous numpy array (None passes NULL).
+
+    The returned ctypes pointer holds a reference to the array, so a
+    temporary made by a dtype or contiguity conversion in the argument list
+    (``ptr(as_i32(x))``) stays alive until the C call that receives the
+    pointer has returned."""
     if a is None:
         return None
     if not a.flags["C_CONTIGUOUS"]:
         raise ValueError("array must be C-contiguous")
-    return a.ctypes.data
+    return a.ctypes.data_as(C.c_void_p)
 
 
 def as_i32(a) -> np.ndarray:

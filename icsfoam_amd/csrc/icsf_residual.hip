@@ -40,6 +40,10 @@ __global__ void k_update(MeshView m, Phys ph, StateView s, const double* __restr
             rU = V3{rU.x + dW[2 * (size_t)nC + c], rU.y + dW[3 * (size_t)nC + c], rU.z + dW[4 * (size_t)nC + c]};
             rE += dW[(size_t)nC + c];
         }
+        // a non-finite conserved state (before the bounds, which would mask
+        // a NaN) is reported through the error flags
+        if (s.err && !(isfinite(rho) && isfinite(rE) && isfinite(rU.x) && isfinite(rU.y) && isfinite(rU.z)))
+            atomicOr(s.err, ERR_FLAG_NONFINITE);
         rho = fmax(rho, ph.rhoMin);
         U = rU / rho;
         e = rE / rho - 0.5 * magSqr(U);
@@ -790,7 +794,7 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
     r = 1.0 / fmax(1.0 / r, fabs(D[18]));
     r = 1.0 / fmax(1.0 / r, fabs(D[24]));
     rD[c] = r;
-    if (!(r >= kVSMALL)) atomicOr(err, 1);
+    if (!(r >= kVSMALL)) atomicOr(err, ERR_FLAG_DIAG);
 }
 
 template <typename TB, int W = 1, bool RES = false>

@@ -146,6 +146,7 @@ struct icsf_case {
     long long tcnt[NTIMER] = {0};
     DevBuf buf;
     bool ownStream = true;
+    int device = 0;  // HIP device the case's buffers and stream live on
     // one steady-state pseudo-step captured as a HIP graph and replayed
     bool useGraph = true;
     hipGraph_t graph = nullptr;
@@ -277,6 +278,25 @@ __global__ void k_fill(int n, double v, double* x) {
 
 int nblk(size_t n, int b) { return (int)((n + b - 1) / b); }
 
+// Read the device error flags of the given flag words (after the work on
+// `st`), clear them, and turn them into a return code: a non-finite state
+// first, then a diagonal whose LU-SGS coefficient fell below VSMALL
+// (lusgs.C:119-123).
+int takeDeviceErrors(hipStream_t st, const std::vector<int*>& flags) {
+    int any = 0;
+    for (int* f : flags) {
+        if (!f) continue;
+        int e = 0;
+        HIPCHK(hipMemcpyAsync(&e, f, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (e) HIPCHK(hipMemset(f, 0, sizeof(int)));
+        any |= e;
+    }
+    if (any & ERR_FLAG_NONFINITE) return fail(ICSF_ERR_NONFINITE, "non-finite conserved state after the update");
+    if (any & ERR_FLAG_DIAG) return fail(ICSF_ERR_ZERO_DIAG, "All diagonals of coupledMatrix are zero.");
+    return ICSF_OK;
+}
+
 // ---- settings -----------------------------------------------------------
 int readSettings(icsf_case* c, const char* thermo, const char* schemes, const char* solution) {
     Dict td, sd, fd;
@@ -380,8 +400,10 @@ int readSettings(icsf_case* c, const char* thermo, const char* schemes, const ch
     if (fcw == "true" || fcw == "on" || fcw == "yes") c->fuseCells = true;
     else if (fcw == "false" || fcw == "off" || fcw == "no") c->fuseCells = false;
     else return fail(ICSF_ERR_KEYWORD, "fusedCellSums must be true or false");
-    if (c->nDirs < 1 || c->nDirs > KMAX || c->innerDirs < 1 || c->innerDirs > KMAX)
-        return fail(ICSF_ERR_ARG, "nDirections must be in [1, 64]");
+    if (c->nDirs < 1 || c->nDirs > KMAX) return fail(ICSF_ERR_ARG, "nDirections must be in [1, 64]");
+    // innerDirections 0: the mixed FGMRES is right-preconditioned by the
+    // FP32 preconditioner alone (no inner Krylov iteration)
+    if (c->innerDirs < 0 || c->innerDirs > KMAX) return fail(ICSF_ERR_ARG, "innerDirections must be in [0, 64]");
     if (c->mixed && c->solver != 1)
         return fail(ICSF_ERR_KEYWORD, "precision mixed requires solver FGMRES (FP64 outer, FP32 inner)");
     c->co = fd.scalar("pseudoTime/pseudoCoNum", 1.0);
@@ -730,10 +752,15 @@ int fgmresSolve(icsf_case* c, const double* W, const double* b, int& nIter) {
 // ================================================================ C ABI ===
 namespace {
 
-int adjointPrepare(icsf_case* c) {
+int adjointSupported(const icsf_case* c) {
     if (c->ph.viscous) return fail(ICSF_ERR_ARG, "residual VJP: viscous terms are not differentiated (inviscid only)");
     if (c->ph.flux != FLUX_AUSM) return fail(ICSF_ERR_ARG, "residual VJP: implemented for the AUSMPlusUp flux scheme");
     if (c->mv.nCyc > 0) return fail(ICSF_ERR_ARG, "residual VJP: meshes with cyclic patches are not supported");
+    return ICSF_OK;
+}
+
+int adjointPrepare(icsf_case* c) {
+    RC(adjointSupported(c));
     if (c->adjReady) return ICSF_OK;
     const size_t nC = c->mv.nC, nF = c->mv.nF, nB = std::max(c->mv.nB, 1);
     AdjView& a = c->adj;
@@ -777,7 +804,7 @@ int vjpDevice(icsf_case* c, const double* W, const double* v, double* gW, double
     launchResidualVjp(c->mv, c->ph, c->sv, c->adj, v, gW, gSf ? gSf : c->adjGS, gCf ? gCf : c->adjGC, c->st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
-    return ICSF_OK;
+    return takeDeviceErrors(c->st, {c->derr});
 }
 
 }  // namespace
@@ -877,7 +904,7 @@ __global__ void k_hb_diag(int nC, int nO, const double* __restrict__ D, const do
         r = 1.0 / fmax(1.0 / r, d(J, 24));
     }
     rD[c] = r;
-    if (!(r >= kVSMALL)) atomicOr(err, 1);
+    if (!(r >= kVSMALL)) atomicOr(err, ERR_FLAG_DIAG);
 }
 
 // SER over all instances (outerLoop.H:30-66): dres = [init nO*5][prev nO*5]
@@ -1015,6 +1042,11 @@ int hbSolveMixed(icsf_hb* h, const double* b, int& nIter) {
     launchCopy(vA, 0, b, h->r, h->st);
     auto precond = [h, &m, vA, nVec](const double* v, double* z) {
         launchConvert(vA, 0, v, h->r32, h->st);
+        if (h->innerDirs == 0) {
+            hbLusgs(h, h->r32, 1);
+            launchConvert(vA, 1, h->r32, z, h->st);
+            return;
+        }
         GmresArgs a{&m, &h->mesh->lev, &h->wk, h->g2, h->innerDirs, 0, h->orth, 1, nullptr, nullptr, h->rD,
                     nullptr, h->r32, h->r32b, h->V32};
         a.nVec = nVec;
@@ -1398,6 +1430,7 @@ int createCase(icsf_mesh* mesh, const char* thermo, const char* schemes, const c
     RC(checkDevice());
     std::unique_ptr<icsf_case> C(new icsf_case);
     C->mesh = mesh;
+    HIPCHK(hipGetDevice(&C->device));
     RC(readSettings(C.get(), thermo, schemes, solution));
     if (mesh->view.nCyc > 0 && C->ph.viscous)
         return fail(ICSF_ERR_ARG, "viscous terms on meshes with cyclic patches are not supported");
@@ -1461,6 +1494,7 @@ int createCase(icsf_mesh* mesh, const char* thermo, const char* schemes, const c
     RC(B.alloc(C->rD, (size_t)nC));
     RC(B.alloc(C->derr, 1));
     HIPCHK(hipMemset(C->derr, 0, sizeof(int)));
+    C->sv.err = C->derr;
     RC(B.alloc(C->fac, (size_t)nC));
     if (shared) {
         HIPCHK(hipStreamSynchronize(C->st));
@@ -1524,6 +1558,13 @@ int icsf_set_boundary(icsf_case* c, int patch, int pType, int UType, int TType, 
     if ((pType == BC_FIXED && !pValue) || (UType == BC_FIXED && !UValue) || (TType == BC_FIXED && !TValue))
         return fail(ICSF_ERR_ARG, "fixedValue condition without values");
     const int s = h.patchStart[patch] - h.nInternal, n = h.patchSize[patch];
+    for (int i = 0; i < n; ++i) {
+        if ((pType == BC_FIXED && !std::isfinite(pValue[i])) ||
+            (UType == BC_FIXED && !(std::isfinite(UValue[3 * i]) && std::isfinite(UValue[3 * i + 1]) &&
+                                    std::isfinite(UValue[3 * i + 2]))) ||
+            (TType == BC_FIXED && !std::isfinite(TValue[i])))
+            return fail(ICSF_ERR_NONFINITE, "fixedValue conditions must be finite");
+    }
     for (int i = 0; i < n; ++i) {
         c->bPT[s + i] = pType;
         c->bUT[s + i] = UType;
@@ -1598,6 +1639,9 @@ int icsf_residual(icsf_case* c, double* b, double* rDT) {
     const int nC = c->mv.nC;
     if (b) HIPCHK(hipMemcpy(b, c->sv.b, (size_t)5 * nC * sizeof(double), hipMemcpyDeviceToHost));
     if (rDT) HIPCHK(hipMemcpy(rDT, c->sv.rDT, (size_t)nC * sizeof(double), hipMemcpyDeviceToHost));
+    if (b)
+        for (size_t i = 0; i < (size_t)5 * nC; ++i)
+            if (!std::isfinite(b[i])) return fail(ICSF_ERR_NONFINITE, "non-finite residual");
     return ICSF_OK;
 }
 
@@ -1630,21 +1674,14 @@ int icsf_residual_device(icsf_case* c, const double* W, double* b) {
     RC(residualStage(c));
     HIPCHK(hipMemcpyAsync(b, c->sv.b, 5 * (size_t)c->mv.nC * sizeof(double), hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    return ICSF_OK;
+    return takeDeviceErrors(c->st, {c->derr});
 }
 
 int icsf_assemble_jacobian(icsf_case* c) {
     if (!c) return fail(ICSF_ERR_ARG, "NULL case");
     if (!c->haveResidual) return fail(ICSF_ERR_STATE, "icsf_residual must run before the Jacobian assembly");
     RC(jacobianStage(c));
-    int e = 0;
-    HIPCHK(hipMemcpyAsync(&e, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    if (e) {
-        HIPCHK(hipMemset(c->derr, 0, sizeof(int)));
-        return fail(ICSF_ERR_ZERO_DIAG, "All diagonals of coupledMatrix are zero.");
-    }
-    return ICSF_OK;
+    return takeDeviceErrors(c->st, {c->derr});
 }
 
 int icsf_get_jacobian(icsf_case* c, double* diag, double* upper, double* lower) {
@@ -1807,6 +1844,10 @@ int icsf_solve_for_incr(icsf_case* c, const double* W, const double* b, double* 
 int icsf_pseudo_steps(icsf_case* c, int nSteps, double* residuals, int* nIter) {
     if (!c || nSteps < 0) return fail(ICSF_ERR_ARG, "icsf_pseudo_steps: bad arguments");
     if (!c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
+    // flags left by earlier calls (e.g. a state handed to the VJP) do not
+    // belong to these steps
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemset(c->derr, 0, sizeof(int)));
     for (int i = 0; i < nSteps; ++i) {
         int it = 1;
         if (graphable(c)) RC(graphStep(c));
@@ -1814,15 +1855,10 @@ int icsf_pseudo_steps(icsf_case* c, int nSteps, double* residuals, int* nIter) {
         if (residuals) gmresReadResiduals(c->g, residuals + 10 * (size_t)i, c->st);
         if (nIter) nIter[i] = it;
     }
-    int e = 0;
-    HIPCHK(hipMemcpyAsync(&e, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    if (e) {
-        HIPCHK(hipMemset(c->derr, 0, sizeof(int)));
-        return fail(ICSF_ERR_ZERO_DIAG, "All diagonals of coupledMatrix are zero.");
-    }
+    HIPCHK(hipGetLastError());
+    const int rc = takeDeviceErrors(c->st, {c->derr});
     resolveEvents(c);
-    return ICSF_OK;
+    return rc;
 }
 
 int icsf_get_pseudo_co(icsf_case* c, double* co) {
@@ -1975,6 +2011,13 @@ int icsf_hb_get_fields(icsf_hb* h, int instant, double* rho, double* U, double* 
     return icsf_get_fields(h->inst[instant], rho, U, T, p, W);
 }
 
+// error flags of the coupled diagonal and of every instance
+static std::vector<int*> hbErrorFlags(const icsf_hb* h) {
+    std::vector<int*> f{h->derr};
+    for (icsf_case* c : h->inst) f.push_back(c->derr);
+    return f;
+}
+
 static int hbReady(icsf_hb* h) {
     if (!h) return fail(ICSF_ERR_ARG, "NULL HB case");
     for (icsf_case* c : h->inst)
@@ -1997,10 +2040,7 @@ int icsf_hb_assemble_jacobian(icsf_hb* h) {
     for (icsf_case* c : h->inst)
         if (!c->haveResidual) return fail(ICSF_ERR_STATE, "icsf_hb_residual must run before the Jacobian assembly");
     RC(hbJacobianStage(h));
-    int e = 0;
-    HIPCHK(hipMemcpy(&e, h->derr, sizeof(int), hipMemcpyDeviceToHost));
-    if (e) return fail(ICSF_ERR_ZERO_DIAG, "All diagonals of coupledMatrix are zero.");
-    return ICSF_OK;
+    return takeDeviceErrors(h->st, hbErrorFlags(h));
 }
 
 int icsf_hb_matrix_mul(icsf_hb* h, const double* x, double* y) {
@@ -2048,6 +2088,9 @@ int icsf_hb_solve_for_incr(icsf_hb* h, const double* b, double* dW, double* resi
 int icsf_hb_pseudo_steps(icsf_hb* h, int nSteps, double* residuals) {
     RC(hbReady(h));
     if (nSteps < 0) return fail(ICSF_ERR_ARG, "icsf_hb_pseudo_steps: nSteps < 0");
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemset(h->derr, 0, sizeof(int)));
+    for (icsf_case* c : h->inst) HIPCHK(hipMemset(c->derr, 0, sizeof(int)));
     for (int i = 0; i < nSteps; ++i) {
         if (hbGraphable(h)) RC(hbGraphStep(h));
         else RC(hbPseudoStep(h));
@@ -2058,7 +2101,7 @@ int icsf_hb_pseudo_steps(icsf_hb* h, int nSteps, double* residuals) {
         }
     }
     HIPCHK(hipGetLastError());
-    return ICSF_OK;
+    return takeDeviceErrors(h->st, hbErrorFlags(h));
 }
 
 int icsf_hb_synchronize(icsf_hb* h) {

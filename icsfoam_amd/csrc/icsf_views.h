@@ -100,7 +100,11 @@ struct StateView {
     double* rDT;      // [nC]
     double* b;        // [5][nC]
     double* ft;       // [nF][NFT] face terms
+    int* err;         // device error flags: ERR_FLAG_DIAG | ERR_FLAG_NONFINITE
 };
+
+// bits of the device error flags
+enum { ERR_FLAG_DIAG = 1, ERR_FLAG_NONFINITE = 2 };
 
 // Buffers of the residual adjoint (vector-Jacobian product).
 struct AdjView {

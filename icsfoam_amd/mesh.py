@@ -48,7 +48,7 @@ def renumber_mesh(mesh: PolyMesh, new_to_old: np.ndarray) -> PolyMesh:
     """Apply a cell permutation (OpenFOAM renumberMesh semantics)."""
     new_to_old = as_i32(new_to_old)
     nF, nI = mesh.n_faces, mesh.n_internal
-    fv = np.empty_like(mesh.face_verts)
+    fv = np.empty(mesh.face_verts.shape[0], np.int32)
     own = np.empty(nF, np.int32)
     nei = np.empty(nI, np.int32)
     fmap = np.empty(nF, np.int32)

@@ -27,8 +27,11 @@ extern "C" {
 #define ICSF_ERR_ARG (-1)        /* invalid argument / size */
 #define ICSF_ERR_KEYWORD (-2)    /* unknown scheme/solver keyword: newConvectiveFluxScheme.C:51-60,
                                     coupledMatrixSolver.C:53-61, coupledMatrixPreconditioner.C:49 */
-#define ICSF_ERR_ZERO_DIAG (-3)  /* all diagonals zero: lusgs.C:119-123 */
-#define ICSF_ERR_NONFINITE (-4)  /* non-finite state or residual */
+#define ICSF_ERR_ZERO_DIAG (-3)  /* lusgs.C:119-123 ("All diagonals of coupledMatrix are zero."): a
+                                    cell's LU-SGS coefficient 1/max|diag| below VSMALL, i.e. a diagonal
+                                    entry above 1/VSMALL (as the reference tests it; an all-zero
+                                    diagonal gives 1/SMALL there and passes) or not a number */
+#define ICSF_ERR_NONFINITE (-4)  /* non-finite input value, residual or conserved state */
 #define ICSF_ERR_HIP (-5)        /* HIP runtime failure */
 #define ICSF_ERR_STATE (-6)      /* call-order error (e.g. solve before assemble) */
 #define ICSF_ERR_NODEVICE (-7)   /* no HIP device available */
@@ -172,7 +175,8 @@ int icsf_case_create(icsf_mesh* mesh, const char* thermophysicalProperties, cons
 int icsf_case_destroy(icsf_case* c);
 
 /* Boundary conditions of p, U, T on one patch; fixed values per patch face
- * (U as [size][3]); pointers may be NULL for non-fixed conditions. */
+ * (U as [size][3]); pointers may be NULL for non-fixed conditions.
+ * Non-finite values are rejected (ICSF_ERR_NONFINITE). */
 int icsf_set_boundary(icsf_case* c, int patch, int pType, int UType, int TType, const double* pValue,
                       const double* UValue, const double* TValue);
 
@@ -261,7 +265,10 @@ int icsf_solve_for_incr(icsf_case* c, const double* W, const double* b, double* 
  * Afterwards the state is the updated one: the residual and the matrix of
  * the last step refer to the state before its update, so the matrix calls
  * (icsf_solve_for_incr, icsf_matrix_mul, ...) need icsf_residual and
- * icsf_assemble_jacobian again (ICSF_ERR_STATE otherwise). */
+ * icsf_assemble_jacobian again (ICSF_ERR_STATE otherwise).  A conserved
+ * state that turns non-finite during the steps is reported as
+ * ICSF_ERR_NONFINITE after the last step (the device flags it in the state
+ * update; graph replays do not stop early). */
 int icsf_pseudo_steps(icsf_case* c, int nSteps, double* residuals, int* nIter);
 
 /* Pseudo-Courant field [nCells] (pseudoCoField). */
@@ -299,7 +306,10 @@ int icsf_hb_operator(int nFreq, const double* frequencies, const int* harmonics,
 /* HB case: nInstants instances of the case (same dictionaries) on one
  * mesh, coupled by D (the coupledMatrix of 2 nInstants scalars and
  * nInstants vectors built in dbnsFullyImplicitHBFoam/outerLoop.H:91-205).
- * fvSolution must select GMRES with LUSGS in double precision. */
+ * fvSolution selects GMRES (double precision) or FGMRES (precision double,
+ * or mixed: FP64 outer FGMRES around an FP32 inner GMRES(innerDirections),
+ * or around the FP32 LU-SGS alone with innerDirections 0), both with the
+ * LUSGS preconditioner. */
 int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, const char* solution, int nInstants,
                    const double* D, icsf_hb** out);
 int icsf_hb_destroy(icsf_hb* h);

@@ -86,7 +86,9 @@ def lib():
 
 
 def _p(a):
-    return None if a is None else a.ctypes.data
+    # the ctypes pointer keeps the array (possibly a conversion temporary)
+    # alive for the duration of the call it is passed to
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def _i32(a):
@@ -185,7 +187,7 @@ def colour(mesh):
 
 def renumber(mesh, new_to_old):
     nF, nI = mesh.n_faces, mesh.n_internal
-    fv = np.empty_like(mesh.face_verts)
+    fv = np.empty(mesh.face_verts.shape[0], np.int32)
     own = np.empty(nF, np.int32)
     nei = np.empty(nI, np.int32)
     fm = np.empty(nF, np.int32)
