@@ -25,6 +25,18 @@
 //
 // Unknown ordering everywhere at this interface: [rho, rhoE, rhoUx, rhoUy, rhoUz]
 // (coupledMatrix with nScal=2, nVect=1; outerLoop.H:53).
+//
+// Pins: every function below is checked against closed forms, invariants
+// or dense brute force by tests/test_oracle_pins.py, test_hb_oracle.py,
+// test_cyclic_oracle.py and test_adjoint_oracle.py (geometry, renumbering,
+// limiter function, each flux scheme's coefficients and branches, viscous
+// and heat-flux terms, local time step, SER, bounding, state update, LF
+// Jacobian, SpMV, LU-SGS, Jacobi, GMRES, HB operator and source).
+// Parity unpinned (only an OpenFOAM-v2112 build could fix them): the
+// gaussGrad boundary correction on non-orthogonal boundary faces, the
+// limited-scheme values on coupled faces beyond the sector/double-sector
+// identity, and slip's valueInternalCoeffs on walls that are not
+// axis-aligned.
 #include <algorithm>
 #include <cmath>
 #include <complex>

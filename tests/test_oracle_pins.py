@@ -499,3 +499,518 @@ def test_inverted_cells_rejected():
         M.mesh_geometry(bad)
     g = M.mesh_geometry(cases.annulus_sector(4, 3, 2))
     assert g["V"].min() > 0
+
+
+# ------------------------------------------- closed-form pins, flux level ---
+# Each pin below isolates one coefficient or branch of a flux scheme on a
+# state where the published method has a closed-form answer, so that a wrong
+# coefficient, sign or branch in the oracle fails it.
+KSTAR = 2.0 * (GAMMA - 1.0) / (GAMMA + 1.0)
+
+
+def _face(scheme, S, L, Rr, lowMach=True):
+    """Oracle face flux: L/R = (rho, p, U, T); returns (phi, phiU, phiE)."""
+    o = O.face_flux(scheme, S, [L[0], L[1], *L[2], L[3]], [Rr[0], Rr[1], *Rr[2], Rr[3]], lowMach=lowMach)
+    return o[0], o[1:4], o[4]
+
+
+def _cstar(rho, p, U, T):
+    """Critical sound speed of Liou (2006): c*^2 = 2(gamma-1)/(gamma+1) H,
+    H = e + |U|^2/2 + p/rho with e = Cv T."""
+    return np.sqrt(KSTAR * (CV * T + 0.5 * (U @ U) + p / rho))
+
+
+def _frame(seed):
+    rng = np.random.default_rng(seed)
+    S = rng.standard_normal(3) * rng.uniform(0.5, 2.0)
+    n = S / np.linalg.norm(S)
+    t = rng.standard_normal(3); t -= (t @ n) * n; t /= np.linalg.norm(t)
+    return S, n, t
+
+
+@pytest.mark.parametrize("scheme", ["AUSMPlusUp", "ROE", "HLLC"])
+def test_flux_stationary_contact(scheme):
+    """u = 0 on both sides, equal p, rho_L != rho_R: every scheme here keeps
+    a stationary contact exactly (mass and energy flux 0, momentum p S).
+    AUSM+up: M+(0) + M-(0) = 0.375 - 0.375 and P+(0) + P-(0) = 1 (Liou
+    2006, beta = 1/8, alpha = 3/16); Roe: the jump is the entropy wave, whose
+    eigenvalue u = 0 is replaced by the entropy fix, so the mass flux is
+    not zero for Roe (pinned in test_roe_entropy_fix_on_contacts) but the
+    momentum and energy fluxes are; HLLC: S_M = 0."""
+    for seed in range(8):
+        S, n, _ = _frame(seed)
+        p = 1e5 * (1 + 0.5 * seed)
+        rL, rR = 1.2, 0.4 + 0.2 * seed
+        L = (rL, p, np.zeros(3), p / (rL * R)); Rr = (rR, p, np.zeros(3), p / (rR * R))
+        phi, phiU, phiE = _face(scheme, S, L, Rr)
+        np.testing.assert_allclose(phiU, p * S, rtol=1e-14, atol=1e-14 * p * np.abs(S).max())
+        assert abs(phiE) <= 1e-9 * p * np.linalg.norm(S)
+        if scheme != "ROE":
+            assert phi == 0.0
+
+
+@pytest.mark.parametrize("scheme", ["AUSMPlusUp", "HLLC"])
+def test_flux_moving_contact_is_upwind(scheme):
+    """Equal velocity U (normal part u, any tangential part) and equal p on
+    both sides, rho_L != rho_R: AUSM+up (M+(M) + M-(M) = M, P+ + P- = 1, no
+    pressure diffusion at dp = 0, no velocity diffusion at du = 0) and HLLC
+    (S_M = u, star state = upwind state) return the exact upwind Euler flux
+    of the side the contact comes from."""
+    for seed in range(8):
+        S, n, t = _frame(10 + seed)
+        p = 8e4 + 1e4 * seed
+        rL, rR = 1.1, 0.5 + 0.15 * seed
+        c0 = np.sqrt(GAMMA * p / rL)
+        for u in (0.35 * c0, -0.2 * c0):
+            U = u * n + 0.3 * c0 * t
+            L = (rL, p, U, p / (rL * R)); Rr = (rR, p, U, p / (rR * R))
+            up = L if u > 0 else Rr
+            rho_up, T_up = up[0], up[3]
+            H_up = CV * T_up + 0.5 * U @ U + p / rho_up
+            mdot = rho_up * (U @ S)
+            phi, phiU, phiE = _face(scheme, S, L, Rr)
+            assert abs(phi - mdot) <= 1e-13 * abs(mdot)
+            np.testing.assert_allclose(phiU, mdot * U + p * S, rtol=0, atol=1e-13 * np.abs(mdot * U + p * S).max())
+            assert abs(phiE - mdot * H_up) <= 1e-13 * abs(mdot * H_up)
+
+
+def test_ausm_split_functions_known_values():
+    """Left state at M_L = +-0.5 against a supersonic right state (M_R = 1.5,
+    so M-(M_R) = P-(M_R) = 0) and mean Mach above 1 (no pressure diffusion):
+    the mass flux is M+(M_L) c_1/2 rho_L |S| and the pressure flux P+(M_L) p_L.
+    Liou (2006) polynomials: M+(4)(M) = M+(2)(1 - 16 beta M-(2)),
+    P+(5)(M) = M+(2)((2 - M) - 16 alpha M M-(2)), M+-(2) = +-(M +- 1)^2/4,
+    beta = 1/8, alpha = 3/16, evaluated by hand:
+      M+(0.5)  = 0.5625 * 1.125   = 0.6328125
+      P+(0.5)  = 0.5625 * 1.59375 = 0.896484375
+      M+(-0.5) = 0.0625 * 2.125   = 0.1328125
+      P+(-0.5) = 0.0625 * 1.65625 = 0.103515625
+    c_1/2 = min(c*_L^2/max(c*_L, u_L), c*_R^2/max(c*_R, -u_R)) = c*_L here."""
+    known = {0.5: (0.6328125, 0.896484375), -0.5: (0.1328125, 0.103515625)}
+    for seed, (ML, (Mp, Pp)) in enumerate(sorted(known.items())):
+        S, n, t = _frame(20 + seed)
+        rL, pL, TL = 1.3, 1.1e5, 290.0
+        # c*_L with |U_L| = |M_L| c*_L: c*^2 (1 - KSTAR M^2/2) = KSTAR (Cv T + p/rho)
+        cs = np.sqrt(KSTAR * (CV * TL + pL / rL) / (1.0 - 0.5 * KSTAR * ML * ML))
+        UL = ML * cs * n
+        assert abs(_cstar(rL, pL, UL, TL) - cs) < 1e-12 * cs
+        TR = 4.0 * TL
+        rR, pR = 0.7, 0.9e5
+        UR = 1.5 * cs * n
+        assert _cstar(rR, pR, UR, TR) > cs
+        phi, phiU, phiE = _face("AUSMPlusUp", S, (rL, pL, UL, TL), (rR, pR, UR, TR))
+        ms = np.linalg.norm(S)
+        np.testing.assert_allclose(phi, Mp * cs * rL * ms, rtol=1e-13)
+        p12 = (phiU @ S - phi * (UL @ S)) / (S @ S)
+        np.testing.assert_allclose(p12, Pp * pL, rtol=1e-13)
+
+
+def test_ausm_pressure_diffusion_coefficients():
+    """Equal velocity u n on both sides (M_L = M_R = M, so M+(M) + M-(M) = M),
+    equal H (c*_L = c*_R = c), p_L != p_R: the mass flux is
+    (M + M_p) c rho_up |S| with M_p = -K_p max(1 - sigma M^2, 0) (p_R - p_L) /
+    (rho_1/2 c^2), rho_1/2 = (rho_L + rho_R)/2.  K_p and sigma are solved
+    from two Mach numbers and must be the reference's 0.25 and 1
+    (ausmPlusUpFluxScheme.C:231-232)."""
+    S, n, t = _frame(30)
+    ms = np.linalg.norm(S)
+    rL, pL, T = 1.2, 1.0e5, 300.0
+    pR = 1.002e5
+    rR = rL * pR / pL          # p/rho equal: equal H, equal c*
+    rows, rhs = [], []
+    for M in (0.2, 0.6):
+        H0 = CV * T + pL / rL
+        c = np.sqrt(KSTAR * H0 / (1.0 - 0.5 * KSTAR * M * M))
+        U = M * c * n
+        assert abs(_cstar(rL, pL, U, T) - c) < 1e-12 * c and abs(_cstar(rR, pR, U, T) - c) < 1e-12 * c
+        phi, _, _ = _face("AUSMPlusUp", S, (rL, pL, U, T), (rR, pR, U, T), lowMach=False)
+        Mp = phi / (c * rL * ms) - M      # M12 > 0: upwind left
+        # Mp = -(K_p - K_p sigma M^2) dp / (rho_1/2 c^2)
+        k = -Mp * 0.5 * (rL + rR) * c * c / (pR - pL)
+        rows.append([1.0, -M * M]); rhs.append(k)
+    Kp, Kps = np.linalg.solve(np.array(rows), np.array(rhs))
+    assert abs(Kp - 0.25) < 1e-9
+    assert abs(Kps / Kp - 1.0) < 1e-8
+
+
+def test_ausm_pressure_diffusion_damping_branch():
+    """When the pressure diffusion would flip the sign of M_1/2 (M > 0,
+    M + M_p <= 0) the reference adds 0.2 M_p instead of M_p
+    (ausmPlusUpFluxScheme.C:236-256); otherwise the full M_p.  Same setup as
+    the coefficient test, M = 0.005, a pressure rise across the face large
+    enough to reverse the sign, and a small one that does not."""
+    S, n, t = _frame(31)
+    ms = np.linalg.norm(S)
+    rL, pL, T, M = 1.2, 1.0e5, 300.0, 0.005
+    c = np.sqrt(KSTAR * (CV * T + pL / rL) / (1.0 - 0.5 * KSTAR * M * M))
+    U = M * c * n
+    for dp_rel, damped in ((0.05, True), (0.002, False)):
+        pR = pL * (1 + dp_rel)
+        rR = rL * pR / pL
+        Mp = -0.25 * (1 - M * M) * (pR - pL) / (0.5 * (rL + rR) * c * c)
+        assert (M + Mp <= 0) == damped
+        M12 = M + (0.2 * Mp if damped else Mp)
+        rho_up = rL if M12 >= 0 else rR
+        phi, _, _ = _face("AUSMPlusUp", S, (rL, pL, U, T), (rR, pR, U, T), lowMach=False)
+        np.testing.assert_allclose(phi, M12 * c * rho_up * ms, rtol=1e-12)
+
+
+def test_ausm_velocity_diffusion_coefficient():
+    """The low-Mach pressure term p_u = -K_u P+(M_L) P-(M_R) (rho_L + rho_R)
+    c_1/2 (u_R - u_L) is the difference of the pressure fluxes with
+    lowMachAusm on and off (the mass flux does not depend on it).  At
+    M_L = 0.5, M_R = -0.5 with equal H: P+(0.5) = P-(-0.5) = 0.896484375
+    (Liou 2006, alpha = 3/16) and c_1/2 = c*, so K_u follows from the two
+    fluxes and must be the reference's 0.25 (ausmPlusUpFluxScheme.C:261)."""
+    S, n, t = _frame(32)
+    rL, rR, p, T = 1.2, 0.8, 1.0e5, 300.0
+    # equal H on both sides: equal p/rho and T
+    pL, pR = p, p * rR / rL
+    # |U|^2 = (0.25 + 0.01) c*^2 on both sides
+    c = np.sqrt(KSTAR * (CV * T + pL / rL) / (1.0 - 0.5 * KSTAR * 0.26))
+    UL, UR = 0.5 * c * n + 0.1 * c * t, -0.5 * c * n + 0.1 * c * t
+    assert abs(_cstar(rL, pL, UL, T) - c) < 1e-12 * c and abs(_cstar(rR, pR, UR, T) - c) < 1e-12 * c
+    on = _face("AUSMPlusUp", S, (rL, pL, UL, T), (rR, pR, UR, T), lowMach=True)
+    off = _face("AUSMPlusUp", S, (rL, pL, UL, T), (rR, pR, UR, T), lowMach=False)
+    assert on[0] == off[0]
+    dp12 = (on[1] - off[1]) @ S / (S @ S)
+    P = 0.896484375
+    Ku = dp12 / (-P * P * (rL + rR) * c * (-c))
+    assert abs(Ku - 0.25) < 1e-10
+
+
+def test_roe_entropy_fix_on_contacts():
+    """Roe flux of a contact (equal U and p, rho_L != rho_R): the jump is
+    the entropy wave (1, U, |U|^2/2) d rho with eigenvalue u = U.n, so
+    P|L|P^-1 dW = |lambda_1| dW and
+      phi = |S| (u (rho_L + rho_R) - |lambda_1| d rho)/2.
+    Harten's fix (roeFluxScheme.C:89-139, coefficient 0.05 of :255): with
+    eps = 0.05 max(|u + c|, |u - c|) (c the Roe-averaged sound speed,
+    c^2 = gamma p / sqrt(rho_L rho_R) for equal p), |lambda_1| = |u| when
+    |u| >= eps, else (u^2 + eps^2)/(2 eps) -- eps/2 for a stationary contact."""
+    for seed, (frac, fixed) in enumerate(((0.0, True), (0.02, True), (0.2, False), (-0.03, True))):
+        S, n, t = _frame(40 + seed)
+        ms = np.linalg.norm(S)
+        p, rL, rR = 1.0e5, 1.2, 0.5
+        cbar = np.sqrt(GAMMA * p / np.sqrt(rL * rR))
+        u = frac * cbar
+        U = u * n + 0.2 * cbar * t
+        eps = 0.05 * (cbar + abs(u))
+        assert (abs(u) < eps) == fixed
+        lam = (u * u + eps * eps) / (2 * eps) if fixed else abs(u)
+        dr = rR - rL
+        phi, phiU, phiE = _face("ROE", S, (rL, p, U, p / (rL * R)), (rR, p, U, p / (rR * R)))
+        expect = 0.5 * ms * (u * (rL + rR) - lam * dr)
+        assert abs(phi - expect) <= 1e-11 * ms * (rL + rR) * cbar
+        expU = expect * U + p * S
+        np.testing.assert_allclose(phiU, expU, rtol=0, atol=1e-11 * np.abs(expU).max())
+        HL, HR = GAMMA / (GAMMA - 1) * p / rL + 0.5 * U @ U, GAMMA / (GAMMA - 1) * p / rR + 0.5 * U @ U
+        expE = 0.5 * ms * (u * (rL * HL + rR * HR) - lam * dr * 0.5 * (U @ U))
+        assert abs(phiE - expE) <= 1e-11 * ms * (rL + rR) * cbar * abs(HL)
+
+
+def test_hllc_star_states_textbook_form():
+    """HLLC in the textbook star-state form (Toro, Riemann Solvers, 3rd ed.,
+    eqs. 10.37-10.39): for S_L < 0 < S_M the flux is F*_L = F_L + S_L (U*_L -
+    U_L) with U*_L = rho_L (S_L - u_L)/(S_L - S_M) [1, S_M n + U_L - u_L n,
+    E_L + (S_M - u_L)(S_M + p_L/(rho_L (S_L - u_L)))]; for S_M < 0 < S_R the
+    mirror F*_R.  The reference writes the same flux through the star
+    pressure (hllcFluxScheme.C:170-240); its wave speeds are the Einfeldt
+    estimates S_L = min(u_L - c_L, u~ - c~), S_R = max(u_R + c_R, u~ + c~)
+    with Roe averages (:128-167), and S_M the contact speed."""
+    rng = np.random.default_rng(50)
+    for it in range(40):
+        S, n, t = _frame(60 + it)
+        ms = np.linalg.norm(S)
+
+        def state():
+            T = rng.uniform(250, 350); p = rng.uniform(8e4, 1.2e5)
+            c = np.sqrt(GAMMA * R * T)
+            return p / (R * T), p, rng.uniform(-0.6, 0.6) * c * n + rng.uniform(-0.3, 0.3) * c * t, T
+        L, Rr = state(), state()
+        rl, pl, Ul, Tl = L; rr, pr, Ur, Tr = Rr
+        cl, cr = np.sqrt(GAMMA * R * Tl), np.sqrt(GAMMA * R * Tr)
+        El, Er = CV * Tl + 0.5 * Ul @ Ul, CV * Tr + 0.5 * Ur @ Ur
+        Hl, Hr = El + pl / rl, Er + pr / rr
+        ul, ur = Ul @ n, Ur @ n
+        k = np.sqrt(rr / rl)
+        ua = ((k * Ur + Ul) / (k + 1)) @ n
+        Ha = (k * Hr + Hl) / (k + 1)
+        Uav = (k * Ur + Ul) / (k + 1)
+        ca = np.sqrt((GAMMA - 1) * (Ha - 0.5 * Uav @ Uav))
+        SL, SR = min(ul - cl, ua - ca), max(ur + cr, ua + ca)
+        SM = (pr - pl + rl * ul * (SL - ul) - rr * ur * (SR - ur)) / (rl * (SL - ul) - rr * (SR - ur))
+
+        def cons(r, U, E):
+            return np.concatenate([[r, r * E], r * U])
+
+        def flux(r, p, U, H):
+            un = U @ n
+            return np.concatenate([[r * un, r * H * un], r * un * U + p * n])
+
+        def star(r, p, U, E, u, Sk):
+            f = r * (Sk - u) / (Sk - SM)
+            Us = SM * n + (U - u * n)
+            Es = E + (SM - u) * (SM + p / (r * (Sk - u)))
+            return f * np.concatenate([[1.0, Es], Us])
+        if SL >= 0 or SR <= 0:
+            continue
+        if SM >= 0:
+            F = flux(rl, pl, Ul, Hl) + SL * (star(rl, pl, Ul, El, ul, SL) - cons(rl, Ul, El))
+        else:
+            F = flux(rr, pr, Ur, Hr) + SR * (star(rr, pr, Ur, Er, ur, SR) - cons(rr, Ur, Er))
+        F *= ms
+        phi, phiU, phiE = _face("HLLC", S, L, Rr)
+        got = np.concatenate([[phi, phiE], phiU])
+        assert rel_err(got, F) < 1e-12
+
+
+# ------------------------------------------------- limiter function pins ---
+def _line_case(values, limiter):
+    """Five cells in a row (h = 1), empty patches on +-y and +-z, so each
+    inner cell's gradient and limiter see exactly its two x-neighbours
+    (empty faces are skipped by the Gauss gradient and the limiter)."""
+    from icsfoam_amd.cases import PATCH_EMPTY, PATCH_PATCH
+    nx = len(values)
+    pts_k, pts_j, pts_i = np.meshgrid(np.arange(2), np.arange(2), np.arange(nx + 1), indexing="ij")
+    pts = np.stack([pts_i.ravel() * 1.0, pts_j.ravel() * 1.0, pts_k.ravel() * 1.0], 1)
+    fv, own, nei, patches = cases._hex_topology(
+        nx, 1, 1, [("inlet", PATCH_PATCH, ["-x"]), ("outlet", PATCH_PATCH, ["+x"]),
+                   ("sides", PATCH_EMPTY, ["-y", "+y", "-z", "+z"])])
+    m = cases._mesh_from_quads(pts, fv, own, nei, patches, nx, pts.reshape(2, 2, nx + 1, 3))
+    p0, T0 = 1e5, 300.0
+    rho = np.asarray(values, np.float64)
+    U = np.zeros((nx, 3)); U[:, 0] = 50.0
+    T = p0 / (R * rho)
+    bcs = [cases.BoundarySpec(cases.BC_ZERO_GRADIENT, cases.BC_ZERO_GRADIENT, cases.BC_ZERO_GRADIENT)
+           for _ in m.patches]
+    return cases.Case("line", m, rho, U, T, bcs, cases.thermo_dict(0.0),
+                      cases.schemes_dict("AUSMPlusUp", limiter=limiter), cases.solution_dict())
+
+
+@pytest.mark.parametrize("limiter", ["Venkatakrishnan", "minmod"])
+def test_limiter_function_on_three_cell_stencil(limiter):
+    """Middle cell of values (v_W, v_P, v_E) = (1, 1.1, 1.1 + 0.1 a): Gauss
+    gradient g = (v_E - v_W)/2, extrapolations +-g/2 to the two faces,
+    max/min neighbour differences a 0.1 and -0.1, so r_E = 0.4 a/(g/2) < 2
+    and r_W = -0.1/(-g/2) > 2.  The limited east face value is
+    v_P + psi g/2 with psi = min(1, l(r_E), l(r_W)), l the limiter function:
+    Venkatakrishnan (r^2 + 2r)/(r^2 + r + 2), minmod min(r, 1)."""
+    fn = {"Venkatakrishnan": lambda r: (r * r + 2 * r) / (r * r + r + 2), "minmod": lambda r: min(r, 1.0)}[limiter]
+    for a in (0.1, 0.25, 0.5, 0.9):
+        vals = [1.0, 1.0, 1.1, 1.1 + 0.1 * a, 1.1 + 0.1 * a]
+        c = _line_case(vals, f"cellLimited<{limiter}> Gauss linear 1")
+        o = O.OracleCase(c)
+        L, Rr = o.reconstruct(0)
+        m = c.mesh
+        f = int(np.nonzero((m.owner[:m.n_internal] == 2) & (m.neighbour == 3))[0][0])
+        g = (vals[3] - vals[1]) / 2.0
+        rE = (vals[3] - vals[2]) / (0.5 * g)
+        rW = (vals[1] - vals[2]) / (-0.5 * g)
+        assert rE < 2.0 < rW
+        psi = min(1.0, fn(rE), fn(rW))
+        assert psi < 1.0 or (limiter == "minmod" and rE >= 1.0)
+        np.testing.assert_allclose(L[f], vals[2] + psi * 0.5 * g, rtol=1e-14)
+        # the west face of the middle cell (its neighbour side) uses the same psi
+        fw = int(np.nonzero((m.owner[:m.n_internal] == 1) & (m.neighbour == 2))[0][0])
+        np.testing.assert_allclose(Rr[fw], vals[2] - psi * 0.5 * g, rtol=1e-14)
+
+
+# ------------------------------------------------------- heat-flux pins ---
+def _still_gas_case(Tfun, mu):
+    """Uniform orthogonal box, U = 0, uniform p, T(x) given; fixedValue T on
+    inlet/outlet (exact face values), zeroGradient T on the walls (dT/dn = 0
+    there for T = T(x)); returns (case, geometry)."""
+    c = cases.case_box(n=(8, 4, 3), lengths=(1.0, 0.5, 0.375), jitter=0.0, flux="AUSMPlusUp", mu=mu,
+                       p_jump=1.0, noise=0.0)
+    geo = O.OracleMesh(c.mesh).geometry()
+    p0 = 1e5
+    c.T[:] = Tfun(geo["C"][:, 0])
+    c.rho[:] = p0 / (R * c.T)
+    c.U[:] = 0.0
+    bcs = []
+    for pt in c.mesh.patches:
+        xf = geo["Cf"][pt.start:pt.start + pt.size, 0]
+        if pt.name in ("inlet", "outlet"):
+            bcs.append(cases.BoundarySpec(cases.BC_ZERO_GRADIENT, cases.BC_FIXED_VALUE, cases.BC_FIXED_VALUE,
+                                          None, np.zeros((pt.size, 3)), Tfun(xf)))
+        else:
+            bcs.append(cases.BoundarySpec(cases.BC_ZERO_GRADIENT, cases.BC_SLIP, cases.BC_ZERO_GRADIENT))
+    c.bcs = bcs
+    return c, geo
+
+
+@pytest.mark.parametrize("shape", ["linear", "quadratic"])
+def test_heat_flux_term_closed_form(shape):
+    """laplacian(alphaEff, e) of residualsUpdate.H:16-43 on a still gas
+    (U = 0: the stress and sigma.U terms vanish) with e = Cv T(x) on a
+    uniform orthogonal box, where corrected snGrad is exact: the viscous part
+    of the energy residual (b with mu minus b without) is V alphaEff Cv T''
+    with alphaEff = kappa/Cv = gamma mu / Pr (OpenFOAM's alphahe of
+    sensible internal energy) -- zero for linear T in every cell, and
+    V gamma mu Cv 2a / Pr for T = T0 + a x^2 in every cell away from the
+    inlet/outlet (where the half-cell boundary snGrad is not exact for a
+    quadratic).  A missing gamma, Cp instead of Cv, or a sign error fails."""
+    mu, Pr = 2.0e-3, 0.72
+    a = 60.0
+    Tfun = {"linear": lambda x: 300.0 + 40.0 * x, "quadratic": lambda x: 300.0 + a * (x - 0.3) ** 2}[shape]
+    cv, geo = _still_gas_case(Tfun, mu)
+    c0, _ = _still_gas_case(Tfun, 0.0)
+    d = O.OracleCase(cv).residual()[0] - O.OracleCase(c0).residual()[0]
+    V = geo["V"]
+    expect = np.zeros_like(V) if shape == "linear" else V * GAMMA * mu / Pr * CV * 2 * a
+    scale = V.max() * GAMMA * mu / Pr * CV * 2 * a
+    assert np.array_equal(d[0], np.zeros_like(d[0]))
+    assert np.abs(d[2:]).max() < 1e-9 * scale
+    x = geo["C"][:, 0]
+    inner = (x > 1.0 / 8) & (x < 1.0 - 1.0 / 8)
+    sel = np.ones_like(inner) if shape == "linear" else inner
+    assert sel.sum() > 0
+    np.testing.assert_allclose(d[1][sel], expect[sel], rtol=1e-8, atol=1e-9 * scale)
+
+
+# ------------------------------------------------ pseudo-time step pins ---
+def test_local_time_step_uniform_state_closed_form():
+    """rPseudoDeltaT of setCoAndDeltaT.H:57-141 for a uniform state on an
+    orthogonal box: each internal face gives nonOrthDeltaCoeffs (c + |U.n|)
+    = (c + |U_d|)/h_d, each wall face 0.5 nonOrthDeltaCoeffs (c + |U_P.n|) =
+    (c + |U_d|)/h_d (boundary delta h_d/2), inlet/outlet faces nothing; the
+    cell value is the MAXIMUM over its faces divided by the CFL number.
+    One cell thick in z (walls on +-z, h_z smallest), so the wall term sets
+    every cell's value: (c + |U_z|)/h_z / CFL.  A factor-1 wall term, a sum
+    over faces or deltaCoeffs of the wrong side fail."""
+    co = 7.0
+    c = cases.case_box(n=(4, 3, 1), lengths=(4.0, 3.0, 0.25), jitter=0.0, flux="ROE", mu=0.0, p_jump=1.0,
+                       noise=0.0, co=co)
+    T0, p0 = 300.0, 1e5
+    cs = np.sqrt(GAMMA * R * T0)
+    U0 = np.array([0.4 * cs, -0.25 * cs, 0.3 * cs])
+    c.rho[:] = p0 / (R * T0); c.T[:] = T0; c.U[:] = U0
+    _, rdt = O.OracleCase(c).residual()
+    faces = [(cs + abs(U0[0])) / 1.0, (cs + abs(U0[1])) / 1.0, (cs + abs(U0[2])) / 0.25]
+    np.testing.assert_allclose(rdt, max(faces) / co, rtol=1e-14)
+    # three cells thick in z: the internal z faces give the same value
+    c3 = cases.case_box(n=(4, 3, 3), lengths=(4.0, 3.0, 0.75), jitter=0.0, flux="ROE", mu=0.0, p_jump=1.0,
+                        noise=0.0, co=co)
+    c3.rho[:] = p0 / (R * T0); c3.T[:] = T0; c3.U[:] = U0
+    _, rdt3 = O.OracleCase(c3).residual()
+    np.testing.assert_allclose(rdt3, max(faces) / co, rtol=1e-14)
+
+
+def test_switched_evolution_relaxation_recurrence():
+    """SER of setCoAndDeltaT.H:3-37 read off the oracle's own history: the
+    CFL field used in pseudo-step j >= 3 is the one of step j-1 times
+    clamp(|r_{j-2}|/|r_{j-1}|, 0.1, 1.25), then clamped to
+    [pseudoCoNumMin, pseudoCoNumMax]; |r| = sqrt(sInit_rho^2 + sInit_rhoE^2
+    + |vInit_rhoU|^2) of the step's initial GMRES residuals; steps 1 and 2
+    keep the initial CFL.  (No cell is bounded in this mild case, so the
+    field stays uniform.)"""
+    c = cases.case_box(n=(6, 5, 4), jitter=0.1, flux="ROE", mu=1.8e-5, noise=0.01, p_jump=1.2, co=5.0)
+    c.fvSolution = cases.solution_dict(co=5.0, coMin=0.5, coMax=9.0)
+    o = O.OracleCase(c)
+    cos, norms = [], []
+    hit_max = hit_incr = False
+    for j in range(10):
+        r = o.pseudo_steps(1)[0]
+        co = o.pseudoCo()
+        assert np.ptp(co) == 0.0
+        cos.append(co[0])
+        norms.append(np.sqrt(r[0] ** 2 + r[1] ** 2 + (r[2] ** 2 + r[3] ** 2 + r[4] ** 2)))
+    assert cos[0] == cos[1] == 5.0
+    for j in range(2, 10):
+        ratio = min(max(norms[j - 2] / norms[j - 1], 0.1), 1.25)
+        expect = min(max(cos[j - 1] * ratio, 0.5), 9.0)
+        hit_incr |= norms[j - 2] / norms[j - 1] > 1.25
+        hit_max |= cos[j - 1] * ratio > 9.0
+        assert abs(cos[j] - expect) <= 1e-14 * expect
+    assert hit_max or hit_incr
+
+
+# ----------------------------------------------------- HB source sign pin --
+def test_hb_source_is_minus_volume_times_time_derivative():
+    """For instances that are each uniform in space (no spatial residual:
+    zeroGradient conditions on every patch), the HB residual
+    is the spectral source alone, b_J = -V (dW/dt)(t_J) (HBZone::addSource,
+    HBZoneTemplates.C:39-93: the harmonic-balance time derivative moved to
+    the right-hand side, where the dbnsFoam residual is b = R V with
+    dW/dtau = R).  With rho = rho0 (1 + 0.1 sin wt), U = U0 (1 + 0.05 cos wt)
+    and constant T the conserved variables hold harmonics up to 3, which
+    D of 7 instants differentiates exactly; the test compares with the
+    analytic time derivative."""
+    from icsfoam_amd.solver import hb_operator as lib_hb_operator
+    f = 100.0
+    w = 2 * np.pi * f
+    D, t, _ = lib_hb_operator([w], [3], 7, 1.0 / f)
+    base = cases.case_box(n=(4, 3, 3), jitter=0.1, flux="ROE", mu=0.0, noise=0.0, p_jump=1.0)
+    m = base.mesh
+    bcs = [cases.BoundarySpec(cases.BC_ZERO_GRADIENT, cases.BC_ZERO_GRADIENT, cases.BC_ZERO_GRADIENT)
+           for _ in m.patches]
+    rho0, T0 = 1.1, 310.0
+    U0 = np.array([120.0, -30.0, 15.0])
+    insts = []
+    import dataclasses
+    for J in range(7):
+        s, co = np.sin(w * t[J]), np.cos(w * t[J])
+        rho = np.full(m.n_cells, rho0 * (1 + 0.1 * s))
+        U = np.tile(U0 * (1 + 0.05 * co), (m.n_cells, 1))
+        insts.append(dataclasses.replace(base, rho=rho, U=U, T=np.full(m.n_cells, T0), bcs=bcs))
+    b = O.OracleHB(insts, D).residual()
+    V = O.OracleMesh(m).geometry()["V"]
+    for J in range(7):
+        s, co = np.sin(w * t[J]), np.cos(w * t[J])
+        r, dr = rho0 * (1 + 0.1 * s), rho0 * 0.1 * w * co
+        u, du = U0 * (1 + 0.05 * co), -U0 * 0.05 * w * s
+        E = CV * T0 + 0.5 * u @ u
+        dW = np.concatenate([[dr, dr * E + r * (u @ du)], dr * u + r * du])
+        expect = -V[None, :] * dW[:, None]
+        scale = np.abs(expect).max()
+        np.testing.assert_allclose(b[J], expect, rtol=0, atol=1e-9 * scale)
+
+
+# ----------------------------------------------------- state update pins --
+def test_state_update_primitive_recovery():
+    """updateFields.H:7-105 on a conserved state W = [rho, rhoE, rhoU]:
+    U = rhoU/rho, e = rhoE/rho - |U|^2/2 bounded below by Cv T_min, T = e/Cv,
+    p = rho R T (perfectGas, psi = 1/(R T)); a cell whose internal energy is
+    negative comes back at T_min (1e-15 K here, the bound the library's
+    thermo uses)."""
+    c = cases.case_box(n=(4, 3, 3), jitter=0.1, flux="ROE", mu=0.0, noise=0.05)
+    o = O.OracleCase(c)
+    rng = np.random.default_rng(3)
+    n = c.mesh.n_cells
+    rho = rng.uniform(0.5, 2.0, n)
+    U = rng.uniform(-200, 200, (n, 3))
+    e = rng.uniform(1e5, 3e5, n)
+    e[5] = -1e3
+    W = np.vstack([rho, rho * (e + 0.5 * np.sum(U * U, 1)), (rho[:, None] * U).T])
+    o.set_state_w(W)
+    f = o.fields()
+    np.testing.assert_allclose(f["rho"], rho, rtol=1e-15)
+    np.testing.assert_allclose(f["U"], U, rtol=1e-12)
+    T = np.maximum(e, CV * 1e-15) / CV
+    np.testing.assert_allclose(f["T"], T, rtol=1e-10)
+    np.testing.assert_allclose(f["p"], rho * R * T, rtol=1e-10)
+    assert f["T"][5] == pytest.approx(1e-15, rel=1e-12)
+
+
+def test_time_step_bounding_factors():
+    """boundLocalTimeStep.H: a cell whose internal energy is below SMALL (or
+    whose rho / e fell under 0.95 of the previous iterate) has its CFL number
+    multiplied by 0.5, and every cell sharing a face with it by 0.75; the
+    others keep theirs.  The factor is evaluated before updateFields (the
+    reference's order), so one step from a state with one cell at
+    T = 1e-20 K (e < SMALL) flags exactly that cell."""
+    co = 20.0
+    c = cases.case_box(n=(5, 4, 3), jitter=0.1, flux="ROE", mu=0.0, noise=0.01, p_jump=1.0, co=co)
+    bad = 27          # (i, j, k) = (2, 1, 1): an interior cell
+    c.T[bad] = 1e-20
+    o = O.OracleCase(c)
+    o.pseudo_steps(1)
+    got = o.pseudoCo()
+    m = c.mesh
+    nb = set(m.neighbour[m.owner[:m.n_internal] == bad]) | set(m.owner[:m.n_internal][m.neighbour == bad])
+    expect = np.full(m.n_cells, co)
+    expect[list(nb)] = 0.75 * co
+    expect[bad] = 0.5 * co
+    assert len(nb) == 6
+    np.testing.assert_array_equal(got, expect)
