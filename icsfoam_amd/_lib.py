@@ -63,6 +63,9 @@ SIGNATURES = {
     "icsf_case_create": (_i, [_vp, C.c_char_p, C.c_char_p, C.c_char_p, _vp]),
     "icsf_case_destroy": (_i, [_vp]),
     "icsf_set_boundary": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp]),
+    "icsf_set_boundary_values": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "icsf_set_state": (_i, [_vp, _vp]),
+    "icsf_get_boundary_values": (_i, [_vp, _vp, _vp, _vp]),
     "icsf_set_fields": (_i, [_vp, _vp, _vp, _vp]),
     "icsf_get_fields": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_calc_flux": (_i, [_vp, _vp, _vp, _vp]),
@@ -71,6 +74,7 @@ SIGNATURES = {
     "icsf_hb_create": (_i, [_vp, C.c_char_p, C.c_char_p, C.c_char_p, _i, _vp, _vp]),
     "icsf_hb_destroy": (_i, [_vp]),
     "icsf_hb_set_boundary": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp]),
+    "icsf_hb_set_boundary_values": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
     "icsf_hb_set_fields": (_i, [_vp, _i, _vp, _vp, _vp]),
     "icsf_hb_get_fields": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
     "icsf_hb_residual": (_i, [_vp, _vp]),

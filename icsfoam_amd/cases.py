@@ -22,6 +22,9 @@ import numpy as np
 PATCH_PATCH, PATCH_WALL, PATCH_EMPTY, PATCH_CYCLIC = 0, 1, 2, 3
 # boundary-condition kinds (OpenFOAM fvPatchField type names)
 BC_ZERO_GRADIENT, BC_FIXED_VALUE, BC_SLIP = 0, 1, 2
+# derived inflow conditions: totalPressure (p), pressureInletOutletVelocity
+# (U), totalTemperature (T)
+BC_TOTAL_PRESSURE, BC_PRESSURE_INLET_OUTLET_VELOCITY, BC_TOTAL_TEMPERATURE = 3, 4, 5
 
 R_GAS = 287.0
 GAMMA = 1.4
@@ -98,13 +101,20 @@ class PolyMesh:
 
 @dataclasses.dataclass
 class BoundarySpec:
-    """Per-patch p, U, T conditions. Fixed values are per patch face."""
+    """Per-patch p, U, T conditions; values per patch face: the fixed value,
+    or p0 (totalPressure), tangentialVelocity (pressureInletOutletVelocity,
+    None: zero), T0 (totalTemperature).  *_init: initial boundary values of
+    the derived conditions (the field files' `value` entries; None: p0, the
+    owner cell's U, T0)."""
     p_type: int
     U_type: int
     T_type: int
     p_val: Optional[np.ndarray] = None   # [size]
     U_val: Optional[np.ndarray] = None   # [size, 3]
     T_val: Optional[np.ndarray] = None   # [size]
+    p_init: Optional[np.ndarray] = None  # [size]
+    U_init: Optional[np.ndarray] = None  # [size, 3]
+    T_init: Optional[np.ndarray] = None  # [size]
 
 
 @dataclasses.dataclass
@@ -383,9 +393,13 @@ def _state_from_mach(M, p, T):
 
 def case_bump(nx: int = 200, ny: int = 50, seed: int = 1, flux: str = "AUSMPlusUp",
               co: float = 50.0, nDirections: int = 30, precision: str = "double",
-              solver: str = "GMRES", precond: str = "LUSGS") -> Case:
+              solver: str = "GMRES", precond: str = "LUSGS", bcs: str = "fixed") -> Case:
     """Inviscid transonic bump channel: M=0.675, p=1e5 Pa, T=300 K, plus a
-    1% seeded 8-mode Fourier perturbation on rho, U and T."""
+    1% seeded 8-mode Fourier perturbation on rho, U and T.  bcs "fixed":
+    fixed inflow velocity and temperature, fixed outlet pressure; "tutorial":
+    the circularArcBump tutorial's set (tutorials/circularArcBump/transonic/
+    0/{p,U,T}): totalPressure / pressureInletOutletVelocity / totalTemperature
+    inlet at the total state of the freestream, fixed outlet pressure."""
     mesh = bump_channel(nx, ny)
     C = approx_cell_centres(mesh)
     s = 0.01 * fourier_noise(C[:, :2], seed)
@@ -394,7 +408,16 @@ def case_bump(nx: int = 200, ny: int = 50, seed: int = 1, flux: str = "AUSMPlusU
     rho = rho0 * (1 + s)
     T = T0 * (1 + s)
     U = np.zeros((mesh.n_cells, 3)); U[:, 0] = U0 * (1 + s)
-    bcs = _bump_bcs(mesh, p0, U0, T0)
+    bcl = _bump_bcs(mesh, p0, U0, T0)
+    if bcs == "tutorial":
+        fac = 1.0 + 0.5 * (GAMMA - 1.0) * M0 * M0
+        for i, pt in enumerate(mesh.patches):
+            if pt.name == "inlet":
+                Uv = np.zeros((pt.size, 3)); Uv[:, 0] = U0
+                bcl[i] = total_inlet(pt.size, p0 * fac ** (GAMMA / (GAMMA - 1.0)), T0 * fac, U_init=Uv, T_init=T0)
+    elif bcs != "fixed":
+        raise ValueError(f"unknown boundary set {bcs!r}")
+    bcs = bcl
     return Case("bump", mesh, rho, U, T, bcs, thermo_dict(0.0),
                 schemes_dict(flux), solution_dict(solver=solver, precond=precond,
                                                   nDirections=nDirections, co=co,
@@ -461,6 +484,18 @@ def case_box(n=(32, 16, 16), lengths=(2.0, 1.0, 1.0), jitter: float = 0.1, seed:
                                                   precision=precision,
                                                   innerDirections=innerDirections,
                                                   orthogonalisation=orthogonalisation))
+
+
+def total_inlet(size: int, p0, T0, U_init=None, T_init=None, p_init=None) -> BoundarySpec:
+    """Inflow patch of the ICSFoam tutorials (tutorials/circularArcBump/
+    transonic/0/{p,U,T}, INLE1): totalPressure p0, pressureInletOutletVelocity
+    (zero tangential velocity), totalTemperature T0; p0/T0 scalars or per
+    face, optional initial `value` entries."""
+    def arr(v, shape):
+        return None if v is None else np.broadcast_to(np.asarray(v, np.float64), shape).copy()
+    return BoundarySpec(BC_TOTAL_PRESSURE, BC_PRESSURE_INLET_OUTLET_VELOCITY, BC_TOTAL_TEMPERATURE,
+                        arr(p0, (size,)), None, arr(T0, (size,)),
+                        p_init=arr(p_init, (size,)), U_init=arr(U_init, (size, 3)), T_init=arr(T_init, (size,)))
 
 
 def permute_case(case: Case, new_to_old: np.ndarray, renumber) -> Case:
@@ -586,11 +621,15 @@ def case_hb(n=(64, 64, 32), n_harmonics: int = 3, frequency_hz: float = 100.0, a
             cyclic: bool = True):
     """Harmonic-balance case on an annular sector with rotational cyclic
     theta patches (slip side walls with cyclic=False): 2 n_harmonics + 1
-    instants over one period of frequency_hz (selectedPeriod = 1/f), the
-    inflow velocity of instant n scaled by 1 + amplitude sin(omega t_n),
-    steady axial flow at the given Mach number with smooth noise as the
-    initial state of every instant (each instant with its own phase of the
-    perturbation).  Returns (instances, D, instants)."""
+    instants over one period of frequency_hz (selectedPeriod = 1/f).  The
+    inlet is the tutorials' inflow patch (totalPressure,
+    pressureInletOutletVelocity, totalTemperature) with the total pressure of
+    instant n perturbed by amplitude * sin(omega t_n + phi), phi seeded; the
+    mean total state is that of the given axial Mach number at p = 1e5 Pa,
+    T = 300 K, and the outlet holds the static pressure.  Initial state of
+    every instant: axial flow at that Mach number with smooth noise (the
+    density with its own phase of the perturbation).  Returns (instances,
+    D, instants)."""
     from .solver import hb_operator
 
     omega = 2.0 * np.pi * frequency_hz
@@ -600,6 +639,9 @@ def case_hb(n=(64, 64, 32), n_harmonics: int = 3, frequency_hz: float = 100.0, a
     C = approx_cell_centres(mesh)
     T0, p0 = 300.0, 1e5
     rho0, U0 = _state_from_mach(mach, p0, T0)
+    fac = 1.0 + 0.5 * (GAMMA - 1.0) * mach * mach
+    pt0, Tt0 = p0 * fac ** (GAMMA / (GAMMA - 1.0)), T0 * fac
+    phase = np.random.default_rng(seed + 2).uniform(0.0, 2.0 * np.pi)
     s = noise * fourier_noise(C, seed + 1)
     instances = []
     for J in range(nO):
@@ -607,16 +649,16 @@ def case_hb(n=(64, 64, 32), n_harmonics: int = 3, frequency_hz: float = 100.0, a
         rho = rho0 * (1 + s * (1 + 0.5 * ph))
         T = T0 * (1 + 0.5 * s)
         U = np.zeros((mesh.n_cells, 3))
-        U[:, 2] = U0 * (1 + s + amplitude * ph)
+        U[:, 2] = U0 * (1 + s)
         U[:, 0] = 0.02 * U0 * s
         bcs: List[Optional[BoundarySpec]] = []
         for pt in mesh.patches:
             if pt.kind == PATCH_CYCLIC:
                 bcs.append(None)
             elif pt.name == "inlet":
-                Uv = np.zeros((pt.size, 3)); Uv[:, 2] = U0 * (1 + amplitude * ph)
-                bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_FIXED_VALUE, BC_FIXED_VALUE, None, Uv,
-                                        np.full(pt.size, T0)))
+                Uv = np.zeros((pt.size, 3)); Uv[:, 2] = U0
+                bcs.append(total_inlet(pt.size, pt0 * (1 + amplitude * np.sin(omega * t[J] + phase)), Tt0,
+                                       U_init=Uv, T_init=T0))
             elif pt.name == "outlet":
                 bcs.append(BoundarySpec(BC_FIXED_VALUE, BC_ZERO_GRADIENT, BC_ZERO_GRADIENT,
                                         np.full(pt.size, p0), None, None))

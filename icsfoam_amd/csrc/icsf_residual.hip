@@ -80,8 +80,93 @@ __global__ void k_update(MeshView m, Phys ph, StateView s, const double* __restr
     s.ecalc[c] = rE / rho - 0.5 * magSqr(U);
 }
 
-// Boundary values of p, U, T (fixedValue / zeroGradient / slip) and the
-// derived reconstruction fields on every non-empty boundary face.
+// Boundary values of p, U and T after a state update, on every non-empty,
+// non-coupled boundary face, in the order updateFields.H:90-105 corrects
+// them (p, then U, then T), into StateView::bcV.  fixedValue, zeroGradient
+// and slip from the updated cells; the derived inflow conditions
+// (OpenFOAM-v2112 finiteVolume, as ICSFoam's tutorials use them):
+//   totalPressure (psi, gamma > 1): p0 / (1 + 0.5 psi (gamma-1)/gamma
+//     (1 - pos0(phi)) |U_b|^2)^(gamma/(gamma-1)), psi = 1/(R T_b) and U_b
+//     the values before this update (thermo.correct() precedes the p
+//     correction, U is corrected after p);
+//   pressureInletOutletVelocity: directionMixed, valueFraction
+//     neg(phi) (I - n n), refValue the tangential part of
+//     tangentialVelocity: U_b = vf & ref + (I - vf) & U_P;
+//   totalTemperature: T0 / (1 + 0.5 psi (gamma-1)/gamma (1 - pos0(phi))
+//     |U_b|^2) with the corrected U_b;
+// phi = bcPhi, the boundary mass flux of the last face-terms pass.
+// init: the createFields state (createFields.H:51-145): the derived
+// conditions take their initial values from bcInit, valueFraction starts
+// at zero, and phi = Sf & (rho_b U_b).
+__global__ void k_bc_update(MeshView m, Phys ph, StateView s, int init) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nB) return;
+    if (m.bKind[b] == PK_EMPTY || m.bKind[b] == PK_CYCLIC) return;
+    const int f = m.nI + b;
+    const int c = m.own[f];
+    double* v = s.bcV + 5 * (size_t)b;
+    const V3 UOld{v[1], v[2], v[3]};
+    const double TOld = v[4];
+    const double gM1ByG = (ph.gamma - 1.0) / ph.gamma;
+    const double phi = init ? 0.0 : s.bcPhi[b];
+    const double npos = 1.0 - (phi >= 0 ? 1.0 : 0.0);
+    const double* q = s.bcInit + 5 * (size_t)b;
+    const int pt = m.bPT[b], ut = m.bUT[b], tt = m.bTT[b];
+    double pB;
+    if (pt == BC_FIXED) pB = m.bPv[b];
+    else if (pt == BC_TOTAL_P) {
+        if (init) pB = q[0];
+        else {
+            const double psip = 1.0 / (ph.R * TOld);
+            pB = m.bPv[b] / pow(1.0 + 0.5 * psip * gM1ByG * npos * magSqr(UOld), 1.0 / gM1ByG);
+        }
+    } else pB = s.fld[(size_t)c * NFLD + F_P];
+    const V3 Uc = cellU(s.fld, c);
+    V3 UB;
+    if (ut == BC_FIXED) UB = ld3(m.bUv + 3 * (size_t)b);
+    else if (ut == BC_SLIP) {
+        const V3 n = ld3(m.Sf + 3 * (size_t)f) / m.magSf[f];
+        UB = (Uc + reflect(n, Uc)) / 2.0;
+    } else if (ut == BC_PIOV) {
+        if (init) {
+            UB = V3{q[1], q[2], q[3]};
+            s.bcIn[b] = 0;
+        } else if (phi < 0) {
+            s.bcIn[b] = 1;
+            const V3 n = ld3(m.Sf + 3 * (size_t)f) / m.magSf[f];
+            const V3 tv = ld3(m.bUv + 3 * (size_t)b);
+            const V3 ref = tv - n * dot(n, tv);
+            const T9 vf{{1 - n.x * n.x, -n.x * n.y, -n.x * n.z,
+                         -n.y * n.x, 1 - n.y * n.y, -n.y * n.z,
+                         -n.z * n.x, -n.z * n.y, 1 - n.z * n.z}};
+            const T9 I{{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+            UB = tdotv(vf, ref) + tdotv(I - vf, Uc);
+        } else {
+            s.bcIn[b] = 0;
+            UB = Uc;
+        }
+    } else UB = Uc;
+    double TB;
+    if (tt == BC_FIXED) TB = m.bTv[b];
+    else if (tt == BC_TOTAL_T) {
+        if (init) TB = q[4];
+        else {
+            const double psip = 1.0 / (ph.R * TOld);
+            TB = m.bTv[b] / (1.0 + 0.5 * psip * gM1ByG * npos * magSqr(UB));
+        }
+    } else TB = s.T[c];
+    v[0] = pB; v[1] = UB.x; v[2] = UB.y; v[3] = UB.z; v[4] = TB;
+    if (init) {
+        const double psi = 1.0 / (ph.R * TB);
+        const double rhoB = psi * pB;
+        const V3 rUB{rhoB * UB.x, rhoB * UB.y, rhoB * UB.z};
+        s.bcPhi[b] = dot(ld3(m.Sf + 3 * (size_t)f), rUB);
+    }
+}
+
+// Boundary values of p, U, T and the derived reconstruction fields on
+// every non-empty boundary face: fixedValue / zeroGradient / slip from the
+// current cells, the derived inflow conditions from bcV.
 __device__ __forceinline__ void boundaryBody(const MeshView& m, const Phys& ph, const StateView& s) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nB) return;
@@ -89,11 +174,14 @@ __device__ __forceinline__ void boundaryBody(const MeshView& m, const Phys& ph, 
     const int f = m.nI + b;
     const int c = m.own[f];
     const double* fc = s.fld + (size_t)c * NFLD;
-    const double pB = m.bPT[b] == BC_FIXED ? m.bPv[b] : fc[F_P];
-    const double TB = m.bTT[b] == BC_FIXED ? m.bTv[b] : s.T[c];
+    const double* bv = s.bcV + 5 * (size_t)b;
+    const int pt = m.bPT[b], tt = m.bTT[b];
+    const double pB = pt == BC_FIXED ? m.bPv[b] : pt == BC_TOTAL_P ? bv[0] : fc[F_P];
+    const double TB = tt == BC_FIXED ? m.bTv[b] : tt == BC_TOTAL_T ? bv[4] : s.T[c];
     const V3 Uc = cellU(s.fld, c);
     V3 UB;
     if (m.bUT[b] == BC_FIXED) UB = ld3(m.bUv + 3 * (size_t)b);
+    else if (m.bUT[b] == BC_PIOV) UB = V3{bv[1], bv[2], bv[3]};
     else if (m.bUT[b] == BC_SLIP) {
         const V3 n = ld3(m.Sf + 3 * (size_t)f) / m.magSf[f];
         UB = (Uc + reflect(n, Uc)) / 2.0;
@@ -430,6 +518,7 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
     } else {
         const int b = f - m.nI;
         const int c = m.own[f];
+        s.bcPhi[b] = fr;  // the registered phi the derived conditions read
         const double* bf = s.bfld + (size_t)b * NFLD;
         const V3 UB{bf[F_UX], bf[F_UY], bf[F_UZ]};
         o[FT_LAM] = soundSpeed(ph, s.bT[b]) + fabs(dot(UB, S / ms));
@@ -700,11 +789,18 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
         if (m.bKind[b] == PK_CYCLIC) continue;  // coupled patches carry no boundary terms
         const V3 Sf = ld3(m.Sf + 3 * (size_t)f);
         const V3 n = Sf / m.magSf[f];
-        const double pVIC = m.bPT[b] == BC_FIXED ? 0.0 : 1.0;
-        const double tVIC = m.bTT[b] == BC_FIXED ? 0.0 : 1.0;
+        // valueInternalCoeffs: 0 for fixedValue, totalPressure and
+        // totalTemperature, 1 for zeroGradient, 1 - |n| for slip, and
+        // 1 - sqrt(|diag(valueFraction)|) for pressureInletOutletVelocity
+        // (valueFraction I - n n after an inflow update, else 0)
+        const int pt = m.bPT[b], tt = m.bTT[b], ut = m.bUT[b];
+        const double pVIC = (pt == BC_FIXED || pt == BC_TOTAL_P) ? 0.0 : 1.0;
+        const double tVIC = (tt == BC_FIXED || tt == BC_TOTAL_T) ? 0.0 : 1.0;
         V3 uVIC;
-        if (m.bUT[b] == BC_FIXED) uVIC = V3{0, 0, 0};
-        else if (m.bUT[b] == BC_SLIP) uVIC = V3{1 - fabs(n.x), 1 - fabs(n.y), 1 - fabs(n.z)};
+        if (ut == BC_FIXED) uVIC = V3{0, 0, 0};
+        else if (ut == BC_SLIP) uVIC = V3{1 - fabs(n.x), 1 - fabs(n.y), 1 - fabs(n.z)};
+        else if (ut == BC_PIOV && s.bcIn[b])
+            uVIC = V3{1 - sqrt(fabs(1 - n.x * n.x)), 1 - sqrt(fabs(1 - n.y * n.y)), 1 - sqrt(fabs(1 - n.z * n.z))};
         else uVIC = V3{1, 1, 1};
         const double* bf = s.bfld + (size_t)b * NFLD;
         const double rhoB = bf[F_RHO], pB = bf[F_P], TB_ = s.bT[b];
@@ -817,6 +913,8 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const double* dW, int mode,
                   const double* rho0, const double* U0, const double* T0, hipStream_t st) {
     k_update<<<nblk(m.nC, 256), 256, 0, st>>>(m, ph, s, dW, mode, rho0, U0, T0);
+    // boundary conditions follow every state update (updateFields.H:90-105)
+    if (m.nB > 0) k_bc_update<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s, mode == 0 ? 1 : 0);
 }
 
 void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st, bool cellSums) {

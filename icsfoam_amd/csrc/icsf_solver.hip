@@ -111,6 +111,10 @@ struct icsf_case {
     std::vector<double> bPv, bUv, bTv;
     int *d_bPT = nullptr, *d_bUT = nullptr, *d_bTT = nullptr;
     double *d_bPv = nullptr, *d_bUv = nullptr, *d_bTv = nullptr;
+    // initial values of the derived conditions [nB][5] (p, U, T; NaN: the
+    // default p0, the owner cell's U, T0), uploaded by icsf_set_fields
+    std::vector<double> bInit;
+    double* d_bInit = nullptr;
     bool bcDirty = true;
     MeshView mv{};
     StateView sv{};
@@ -754,6 +758,10 @@ namespace {
 
 int adjointSupported(const icsf_case* c) {
     if (c->ph.viscous) return fail(ICSF_ERR_ARG, "residual VJP: viscous terms are not differentiated (inviscid only)");
+    for (size_t b = 0; b < c->bPT.size(); ++b)
+        if (c->bPT[b] == BC_TOTAL_P || c->bUT[b] == BC_PIOV || c->bTT[b] == BC_TOTAL_T)
+            return fail(ICSF_ERR_ARG, "residual VJP: totalPressure, pressureInletOutletVelocity and totalTemperature "
+                                      "conditions are not supported");
     if (c->ph.flux != FLUX_AUSM) return fail(ICSF_ERR_ARG, "residual VJP: implemented for the AUSMPlusUp flux scheme");
     if (c->mv.nCyc > 0) return fail(ICSF_ERR_ARG, "residual VJP: meshes with cyclic patches are not supported");
     return ICSF_OK;
@@ -1449,9 +1457,11 @@ int createCase(icsf_mesh* mesh, const char* thermo, const char* schemes, const c
     C->bPv.assign(nBs, 0.0);
     C->bUv.assign(3 * nBs, 0.0);
     C->bTv.assign(nBs, 0.0);
+    C->bInit.assign(5 * nBs, std::nan(""));
     DevBuf& B = C->buf;
     RC(B.alloc(C->d_bPT, nBs)); RC(B.alloc(C->d_bUT, nBs)); RC(B.alloc(C->d_bTT, nBs));
     RC(B.alloc(C->d_bPv, nBs)); RC(B.alloc(C->d_bUv, 3 * nBs)); RC(B.alloc(C->d_bTv, nBs));
+    RC(B.alloc(C->d_bInit, 5 * nBs));
     C->mv = mesh->view;
     C->mv.bPT = C->d_bPT; C->mv.bUT = C->d_bUT; C->mv.bTT = C->d_bTT;
     C->mv.bPv = C->d_bPv; C->mv.bUv = C->d_bUv; C->mv.bTv = C->d_bTv;
@@ -1469,6 +1479,13 @@ int createCase(icsf_mesh* mesh, const char* thermo, const char* schemes, const c
     RC(B.alloc(s.rDT, (size_t)nC));
     RC(B.alloc(s.b, (size_t)5 * nC));
     RC(B.alloc(s.ft, (size_t)NFT * h.nFaces));
+    RC(B.alloc(s.bcV, 5 * nBs));
+    RC(B.alloc(s.bcPhi, nBs));
+    RC(B.alloc(s.bcIn, nBs));
+    HIPCHK(hipMemset(s.bcV, 0, 5 * nBs * sizeof(double)));
+    HIPCHK(hipMemset(s.bcPhi, 0, nBs * sizeof(double)));
+    HIPCHK(hipMemset(s.bcIn, 0, nBs * sizeof(int)));
+    s.bcInit = C->d_bInit;
     HIPCHK(hipMemset(s.ft, 0, (size_t)NFT * h.nFaces * sizeof(double)));
     HIPCHK(hipMemset(s.bfld, 0, NFLD * nBs * sizeof(double)));
     HIPCHK(hipMemset(s.bT, 0, nBs * sizeof(double)));
@@ -1552,31 +1569,58 @@ int icsf_set_boundary(icsf_case* c, int patch, int pType, int UType, int TType, 
     if (h.patchKind[patch] == PATCH_CYCLIC)
         return fail(ICSF_ERR_ARG, "patch " + std::to_string(patch) + " is cyclic: its values come from the "
                                   "coupled cells");
-    auto ok = [](int t, bool vec) { return t == BC_ZG || t == BC_FIXED || (vec && t == BC_SLIP); };
-    if (!ok(pType, false) || !ok(UType, true) || !ok(TType, false))
-        return fail(ICSF_ERR_KEYWORD, "unsupported boundary-condition type");
-    if ((pType == BC_FIXED && !pValue) || (UType == BC_FIXED && !UValue) || (TType == BC_FIXED && !TValue))
-        return fail(ICSF_ERR_ARG, "fixedValue condition without values");
+    const bool pOk = pType == BC_ZG || pType == BC_FIXED || pType == BC_TOTAL_P;
+    const bool uOk = UType == BC_ZG || UType == BC_FIXED || UType == BC_SLIP || UType == BC_PIOV;
+    const bool tOk = TType == BC_ZG || TType == BC_FIXED || TType == BC_TOTAL_T;
+    if (!pOk || !uOk || !tOk) return fail(ICSF_ERR_KEYWORD, "unsupported boundary-condition type");
+    const bool pv = pType == BC_FIXED || pType == BC_TOTAL_P, uv = UType == BC_FIXED,
+               tv = TType == BC_FIXED || TType == BC_TOTAL_T;
+    if ((pv && !pValue) || (uv && !UValue) || (tv && !TValue))
+        return fail(ICSF_ERR_ARG, "fixedValue / totalPressure / totalTemperature condition without values");
+    const bool uIn = UValue && (uv || UType == BC_PIOV);  // fixed U or tangentialVelocity
     const int s = h.patchStart[patch] - h.nInternal, n = h.patchSize[patch];
     for (int i = 0; i < n; ++i) {
-        if ((pType == BC_FIXED && !std::isfinite(pValue[i])) ||
-            (UType == BC_FIXED && !(std::isfinite(UValue[3 * i]) && std::isfinite(UValue[3 * i + 1]) &&
-                                    std::isfinite(UValue[3 * i + 2]))) ||
-            (TType == BC_FIXED && !std::isfinite(TValue[i])))
-            return fail(ICSF_ERR_NONFINITE, "fixedValue conditions must be finite");
+        if ((pv && !std::isfinite(pValue[i])) ||
+            (uIn && !(std::isfinite(UValue[3 * i]) && std::isfinite(UValue[3 * i + 1]) &&
+                      std::isfinite(UValue[3 * i + 2]))) ||
+            (tv && !std::isfinite(TValue[i])))
+            return fail(ICSF_ERR_NONFINITE, "boundary values must be finite");
+        if ((pType == BC_TOTAL_P && !(pValue[i] > 0)) || (tv && !(TValue[i] > 0)))
+            return fail(ICSF_ERR_NONFINITE, "fixed / total temperature and total pressure must be positive");
     }
+    if ((pType == BC_TOTAL_P || TType == BC_TOTAL_T) && !(c->ph.gamma > 1))
+        return fail(ICSF_ERR_ARG, "totalPressure / totalTemperature need gamma > 1");
     for (int i = 0; i < n; ++i) {
         c->bPT[s + i] = pType;
         c->bUT[s + i] = UType;
         c->bTT[s + i] = TType;
-        if (pType == BC_FIXED) c->bPv[s + i] = pValue[i];
-        if (UType == BC_FIXED) for (int d = 0; d < 3; ++d) c->bUv[3 * (s + i) + d] = UValue[3 * i + d];
-        if (TType == BC_FIXED) {
-            if (!(TValue[i] > 0)) return fail(ICSF_ERR_NONFINITE, "fixed temperature must be positive");
-            c->bTv[s + i] = TValue[i];
-        }
+        if (pv) c->bPv[s + i] = pValue[i];
+        for (int d = 0; d < 3; ++d) c->bUv[3 * (s + i) + d] = uIn ? UValue[3 * i + d] : 0.0;
+        if (tv) c->bTv[s + i] = TValue[i];
     }
     c->bcDirty = true;
+    return ICSF_OK;
+}
+
+int icsf_set_boundary_values(icsf_case* c, int patch, const double* pValue, const double* UValue,
+                             const double* TValue) {
+    if (!c) return fail(ICSF_ERR_ARG, "NULL case");
+    const HostMesh& h = c->mesh->h;
+    if (patch < 0 || patch >= (int)h.patchStart.size()) return fail(ICSF_ERR_ARG, "patch index out of range");
+    const int s = h.patchStart[patch] - h.nInternal, n = h.patchSize[patch];
+    for (int i = 0; i < n; ++i) {
+        if ((pValue && !(pValue[i] > 0 && std::isfinite(pValue[i]))) ||
+            (TValue && !(TValue[i] > 0 && std::isfinite(TValue[i]))) ||
+            (UValue && !(std::isfinite(UValue[3 * i]) && std::isfinite(UValue[3 * i + 1]) &&
+                         std::isfinite(UValue[3 * i + 2]))))
+            return fail(ICSF_ERR_NONFINITE, "initial boundary values must be finite, p and T positive");
+    }
+    for (int i = 0; i < n; ++i) {
+        double* q = &c->bInit[5 * (size_t)(s + i)];
+        if (pValue) q[0] = pValue[i];
+        if (UValue) for (int d = 0; d < 3; ++d) q[1 + d] = UValue[3 * i + d];
+        if (TValue) q[4] = TValue[i];
+    }
     return ICSF_OK;
 }
 
@@ -1592,6 +1636,22 @@ int icsf_set_fields(icsf_case* c, const double* rho, const double* U, const doub
     RC(tmp.upload(dr, rho, nC));
     RC(tmp.upload(du, U, (size_t)3 * nC));
     RC(tmp.upload(dt, T, nC));
+    // initial values of the derived conditions (createFields.H: the
+    // fields' `value` entries; defaults p0, the owner cell's U, T0)
+    {
+        const HostMesh& h = c->mesh->h;
+        const int nB = h.nFaces - h.nInternal;
+        std::vector<double> q((size_t)5 * std::max(nB, 1), 0.0);
+        for (int b = 0; b < nB; ++b) {
+            const double* g = &c->bInit[5 * (size_t)b];
+            const int o = h.owner[h.nInternal + b];
+            q[5 * (size_t)b] = std::isnan(g[0]) ? c->bPv[b] : g[0];
+            for (int d = 0; d < 3; ++d) q[5 * (size_t)b + 1 + d] = std::isnan(g[1 + d]) ? U[3 * (size_t)o + d] : g[1 + d];
+            q[5 * (size_t)b + 4] = std::isnan(g[4]) ? c->bTv[b] : g[4];
+        }
+        HIPCHK(hipMemcpy(c->d_bInit, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    RC(uploadBC(c));
     launchUpdate(c->mv, c->ph, c->sv, nullptr, 0, dr, du, dt, c->st);
     HIPCHK(hipStreamSynchronize(c->st));
     c->haveFields = true;
@@ -1663,6 +1723,37 @@ int icsf_residual_vjp(icsf_case* c, const double* W, const double* v, double* gW
     HIPCHK(hipMemcpy(gW, dg, n5 * sizeof(double), hipMemcpyDeviceToHost));
     if (gSf) HIPCHK(hipMemcpy(gSf, c->adjGS, n3 * sizeof(double), hipMemcpyDeviceToHost));
     if (gCf) HIPCHK(hipMemcpy(gCf, c->adjGC, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    return ICSF_OK;
+}
+
+int icsf_set_state(icsf_case* c, const double* W) {
+    if (!c || !W) return fail(ICSF_ERR_ARG, "icsf_set_state: NULL argument");
+    if (!c->haveFields) return fail(ICSF_ERR_STATE, "fields not set (icsf_set_fields initialises the boundary state)");
+    const size_t n5 = 5 * (size_t)c->mv.nC;
+    for (size_t i = 0; i < n5; ++i)
+        if (!std::isfinite(W[i])) return fail(ICSF_ERR_NONFINITE, "icsf_set_state: non-finite conserved state");
+    DevBuf tmp;
+    double* dW;
+    RC(tmp.upload(dW, W, n5));
+    RC(uploadBC(c));
+    RC(stateFromW(c, dW));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return takeDeviceErrors(c->st, {c->derr});
+}
+
+int icsf_get_boundary_values(icsf_case* c, double* p, double* U, double* T) {
+    if (!c) return fail(ICSF_ERR_ARG, "NULL case");
+    if (!c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
+    const int nB = c->mv.nB;
+    if (nB == 0) return ICSF_OK;
+    std::vector<double> v((size_t)5 * nB);
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(v.data(), c->sv.bcV, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int b = 0; b < nB; ++b) {
+        if (p) p[b] = v[5 * (size_t)b];
+        if (U) for (int d = 0; d < 3; ++d) U[3 * (size_t)b + d] = v[5 * (size_t)b + 1 + d];
+        if (T) T[b] = v[5 * (size_t)b + 4];
+    }
     return ICSF_OK;
 }
 
@@ -1998,6 +2089,12 @@ int icsf_hb_set_boundary(icsf_hb* h, int instant, int patch, int pType, int UTyp
                          const double* UValue, const double* TValue) {
     RC(hbCheckInstance(h, instant));
     return icsf_set_boundary(h->inst[instant], patch, pType, UType, TType, pValue, UValue, TValue);
+}
+
+int icsf_hb_set_boundary_values(icsf_hb* h, int instant, int patch, const double* pValue, const double* UValue,
+                                const double* TValue) {
+    RC(hbCheckInstance(h, instant));
+    return icsf_set_boundary_values(h->inst[instant], patch, pValue, UValue, TValue);
 }
 
 int icsf_hb_set_fields(icsf_hb* h, int instant, const double* rho, const double* U, const double* T) {

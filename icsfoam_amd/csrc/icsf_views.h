@@ -11,7 +11,10 @@ namespace icsf {
 // speed max(sqrt(gamma/psi), VSMALL) (HLLC); unused by Roe.
 enum { F_RHO = 0, F_P = 1, F_UX = 2, F_UY = 3, F_UZ = 4, F_E = 5, F_H = 6, F_CS = 7, NFLD = 8 };
 enum { FLUX_AUSM = 0, FLUX_ROE = 1, FLUX_HLLC = 2 };
-enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2 };
+// boundary-condition kinds: p zeroGradient / fixedValue / totalPressure;
+// U zeroGradient / fixedValue / slip / pressureInletOutletVelocity;
+// T zeroGradient / fixedValue / totalTemperature
+enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2, BC_TOTAL_P = 3, BC_PIOV = 4, BC_TOTAL_T = 5 };
 enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2, PK_CYCLIC = 3 };
 constexpr int NB = 25;  // entries of a 5x5 block
 
@@ -38,9 +41,10 @@ struct MeshView {
     const int* bPT;       // [nB] p boundary-condition type
     const int* bUT;       // [nB] U boundary-condition type
     const int* bTT;       // [nB] T boundary-condition type
-    const double* bPv;    // [nB] fixed p
-    const double* bUv;    // [nB][3] fixed U
-    const double* bTv;    // [nB] fixed T
+    const double* bPv;    // [nB] fixed p, or totalPressure p0
+    const double* bUv;    // [nB][3] fixed U, or the tangentialVelocity of
+                          //   pressureInletOutletVelocity
+    const double* bTv;    // [nB] fixed T, or totalTemperature T0
     // sliced-ELL off-diagonal layout
     int nSlices;
     const int* slicePtr;  // [nSlices+1]
@@ -101,6 +105,18 @@ struct StateView {
     double* b;        // [5][nC]
     double* ft;       // [nF][NFT] face terms
     int* err;         // device error flags: ERR_FLAG_DIAG | ERR_FLAG_NONFINITE
+    // boundary state of the derived conditions (totalPressure,
+    // pressureInletOutletVelocity, totalTemperature), which depend on
+    // lagged values: bcV [nB][5] = p, Ux, Uy, Uz, T of every boundary face
+    // as of the last state update (updateFields.H:90-105); bcPhi [nB] the
+    // boundary mass flux of the last face-terms pass (the registered phi);
+    // bcIn [nB] the valueFraction switch of pressureInletOutletVelocity (1:
+    // inflow at its last update); bcInit [nB][5] the initial (`value`)
+    // boundary values used by the initialisation
+    double* bcV;
+    double* bcPhi;
+    int* bcIn;
+    const double* bcInit;
 };
 
 // bits of the device error flags

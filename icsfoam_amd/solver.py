@@ -22,7 +22,19 @@ from typing import Optional
 import numpy as np
 
 from ._lib import as_f64, as_i32, check, lib, ptr
-from .cases import BC_FIXED_VALUE, Case, PolyMesh
+from .cases import (BC_FIXED_VALUE, BC_PRESSURE_INLET_OUTLET_VELOCITY, BC_TOTAL_PRESSURE, BC_TOTAL_TEMPERATURE,
+                    Case, PolyMesh)
+
+
+def _bc_values(p_type, U_type, T_type, p_val, U_val, T_val):
+    """The value arrays a condition kind takes: fixed values, p0
+    (totalPressure), tangentialVelocity (pressureInletOutletVelocity, may be
+    None), T0 (totalTemperature)."""
+    pv = as_f64(p_val) if p_type in (BC_FIXED_VALUE, BC_TOTAL_PRESSURE) else None
+    uv = as_f64(U_val) if (U_type == BC_FIXED_VALUE or
+                           (U_type == BC_PRESSURE_INLET_OUTLET_VELOCITY and U_val is not None)) else None
+    tv = as_f64(T_val) if T_type in (BC_FIXED_VALUE, BC_TOTAL_TEMPERATURE) else None
+    return pv, uv, tv
 
 
 class fvMesh:
@@ -68,6 +80,8 @@ class dbnsCase:
             if bc is None:  # constraint patch (cyclic): no conditions to set
                 continue
             self.set_boundary(i, bc.p_type, bc.U_type, bc.T_type, bc.p_val, bc.U_val, bc.T_val)
+            if bc.p_init is not None or bc.U_init is not None or bc.T_init is not None:
+                self.set_boundary_values(i, bc.p_init, bc.U_init, bc.T_init)
         self.set_fields(case.rho, case.U, case.T)
         return self
 
@@ -79,10 +93,13 @@ class dbnsCase:
 
     # ---- state -------------------------------------------------------
     def set_boundary(self, patch, p_type, U_type, T_type, p_val=None, U_val=None, T_val=None):
-        pv = as_f64(p_val) if p_type == BC_FIXED_VALUE else None
-        uv = as_f64(U_val) if U_type == BC_FIXED_VALUE else None
-        tv = as_f64(T_val) if T_type == BC_FIXED_VALUE else None
+        pv, uv, tv = _bc_values(p_type, U_type, T_type, p_val, U_val, T_val)
         check(lib().icsf_set_boundary(self._h, patch, p_type, U_type, T_type, ptr(pv), ptr(uv), ptr(tv)))
+
+    def set_boundary_values(self, patch, p=None, U=None, T=None):
+        """Initial boundary values of the derived conditions (`value` entries)."""
+        pv, uv, tv = (None if a is None else as_f64(a) for a in (p, U, T))
+        check(lib().icsf_set_boundary_values(self._h, patch, ptr(pv), ptr(uv), ptr(tv)))
 
     def set_fields(self, rho, U, T):
         check(lib().icsf_set_fields(self._h, ptr(as_f64(rho)), ptr(as_f64(U)), ptr(as_f64(T))))
@@ -92,6 +109,18 @@ class dbnsCase:
         rho, U, T, p, W = np.empty(n), np.empty((n, 3)), np.empty(n), np.empty(n), np.empty((5, n))
         check(lib().icsf_get_fields(self._h, ptr(rho), ptr(U), ptr(T), ptr(p), ptr(W)))
         return dict(rho=rho, U=U, T=T, p=p, W=W)
+
+    def set_state(self, W):
+        """Conserved state W [5][nC] through updateFields (zero increment)."""
+        check(lib().icsf_set_state(self._h, ptr(as_f64(W))))
+
+    def boundary(self):
+        """Boundary values of every boundary face (p, U [nB,3], T) as of the
+        last state update."""
+        nB = self.mesh.nFaces - self.mesh.nInternalFaces
+        p, U, T = np.empty(nB), np.empty((nB, 3)), np.empty(nB)
+        check(lib().icsf_get_boundary_values(self._h, ptr(p), ptr(U), ptr(T)))
+        return dict(p=p, U=U, T=T)
 
     # ---- flux / residual / Jacobian ----------------------------------
     def calcFlux(self):
@@ -228,6 +257,8 @@ class hbCase:
                 if bc is None:  # constraint patch (cyclic)
                     continue
                 self.set_boundary(J, i, bc.p_type, bc.U_type, bc.T_type, bc.p_val, bc.U_val, bc.T_val)
+                if bc.p_init is not None or bc.U_init is not None or bc.T_init is not None:
+                    self.set_boundary_values(J, i, bc.p_init, bc.U_init, bc.T_init)
             self.set_fields(J, c.rho, c.U, c.T)
         return self
 
@@ -238,10 +269,12 @@ class hbCase:
             self._h = None
 
     def set_boundary(self, J, patch, p_type, U_type, T_type, p_val=None, U_val=None, T_val=None):
-        pv = as_f64(p_val) if p_type == BC_FIXED_VALUE else None
-        uv = as_f64(U_val) if U_type == BC_FIXED_VALUE else None
-        tv = as_f64(T_val) if T_type == BC_FIXED_VALUE else None
+        pv, uv, tv = _bc_values(p_type, U_type, T_type, p_val, U_val, T_val)
         check(lib().icsf_hb_set_boundary(self._h, J, patch, p_type, U_type, T_type, ptr(pv), ptr(uv), ptr(tv)))
+
+    def set_boundary_values(self, J, patch, p=None, U=None, T=None):
+        pv, uv, tv = (None if a is None else as_f64(a) for a in (p, U, T))
+        check(lib().icsf_hb_set_boundary_values(self._h, J, patch, ptr(pv), ptr(uv), ptr(tv)))
 
     def set_fields(self, J, rho, U, T):
         check(lib().icsf_hb_set_fields(self._h, J, ptr(as_f64(rho)), ptr(as_f64(U)), ptr(as_f64(T))))

@@ -44,7 +44,10 @@ extern "C" {
 /* boundary-condition kinds (OpenFOAM fvPatchField types) */
 #define ICSF_BC_ZERO_GRADIENT 0
 #define ICSF_BC_FIXED_VALUE 1
-#define ICSF_BC_SLIP 2
+#define ICSF_BC_SLIP 2                          /* U only */
+#define ICSF_BC_TOTAL_PRESSURE 3                /* p only: value p0 (psi thermo:psi, gamma of the thermo) */
+#define ICSF_BC_PRESSURE_INLET_OUTLET_VELOCITY 4 /* U only: value tangentialVelocity (NULL: zero) */
+#define ICSF_BC_TOTAL_TEMPERATURE 5             /* T only: value T0 (gamma of the thermo) */
 
 typedef struct icsf_mesh icsf_mesh;
 typedef struct icsf_case icsf_case;
@@ -174,16 +177,43 @@ int icsf_case_create(icsf_mesh* mesh, const char* thermophysicalProperties, cons
                      const char* fvSolution, icsf_case** out);
 int icsf_case_destroy(icsf_case* c);
 
-/* Boundary conditions of p, U, T on one patch; fixed values per patch face
- * (U as [size][3]); pointers may be NULL for non-fixed conditions.
- * Non-finite values are rejected (ICSF_ERR_NONFINITE). */
+/* Boundary conditions of p, U, T on one patch; values per patch face (U as
+ * [size][3]): the fixed value, or p0 of totalPressure, T0 of
+ * totalTemperature, tangentialVelocity of pressureInletOutletVelocity;
+ * pointers may be NULL where a condition takes no value.  Replaces the
+ * boundaryField entries of 0/p, 0/U, 0/T as the ICSF tutorials set them
+ * (tutorials/circularArcBump/transonic/0/{p,U,T}).  The derived inflow
+ * conditions follow OpenFOAM-v2112 (rho none, psi thermo:psi, gamma of the
+ * thermophysical model, phi the convective mass flux of the last residual
+ * evaluation): they are corrected after every state update from lagged
+ * values (updateFields.H:90-105), and enter the Jacobian through their
+ * valueInternalCoeffs (convectiveFluxScheme.C:47-120).  Non-finite values
+ * are rejected (ICSF_ERR_NONFINITE), unknown kinds with ICSF_ERR_KEYWORD. */
 int icsf_set_boundary(icsf_case* c, int patch, int pType, int UType, int TType, const double* pValue,
                       const double* UValue, const double* TValue);
+
+/* Initial boundary values of the derived conditions (the `value` entries
+ * of the field files, read by createFields.H:51-145); any pointer may be
+ * NULL.  Defaults: p0 for totalPressure (OpenFOAM's default without a
+ * value entry), the owner cell's U for pressureInletOutletVelocity, T0 for
+ * totalTemperature.  Takes effect at the next icsf_set_fields. */
+int icsf_set_boundary_values(icsf_case* c, int patch, const double* pValue, const double* UValue,
+                             const double* TValue);
 
 /* Initial state from rho, U, T (createFields.H: p = rho/psi, rhoU, rhoE). */
 int icsf_set_fields(icsf_case* c, const double* rho, const double* U, const double* T);
 /* Current state; W is the conserved vector [5][nCells]. Any may be NULL. */
 int icsf_get_fields(icsf_case* c, double* rho, double* U, double* T, double* p, double* W);
+/* Conserved state W [5][nCells] as the new case state through updateFields
+ * with a zero increment (updateFields.H:7-105: primitive recovery, bounds,
+ * boundary conditions corrected); needs icsf_set_fields first, which
+ * initialises the boundary state. */
+int icsf_set_state(icsf_case* c, const double* W);
+/* Boundary values of every boundary face as of the last state update
+ * (p.boundaryField(), U.boundaryField(), T.boundaryField(); [nFaces -
+ * nInternalFaces], U as [..][3]; zero on empty and cyclic faces).  Any may
+ * be NULL. */
+int icsf_get_boundary_values(icsf_case* c, double* p, double* U, double* T);
 
 /* Face fluxes of every face (phi [nFaces], phiUp [nFaces][3], phiEp
  * [nFaces]); replaces convectiveFluxScheme::calcFlux
@@ -313,10 +343,13 @@ int icsf_hb_operator(int nFreq, const double* frequencies, const int* harmonics,
 int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, const char* solution, int nInstants,
                    const double* D, icsf_hb** out);
 int icsf_hb_destroy(icsf_hb* h);
-/* Per-instant boundary conditions and fields (as icsf_set_boundary /
- * icsf_set_fields / icsf_get_fields). */
+/* Per-instant boundary conditions, initial boundary values and fields (as
+ * icsf_set_boundary / icsf_set_boundary_values / icsf_set_fields /
+ * icsf_get_fields). */
 int icsf_hb_set_boundary(icsf_hb* h, int instant, int patch, int pType, int UType, int TType, const double* pValue,
                          const double* UValue, const double* TValue);
+int icsf_hb_set_boundary_values(icsf_hb* h, int instant, int patch, const double* pValue, const double* UValue,
+                                const double* TValue);
 int icsf_hb_set_fields(icsf_hb* h, int instant, const double* rho, const double* U, const double* T);
 int icsf_hb_get_fields(icsf_hb* h, int instant, double* rho, double* U, double* T, double* p, double* W);
 /* b [nInstants][5][nCells]: R V of each instance plus the HB source

@@ -130,7 +130,10 @@ inline T9 dev2(const T9& A) {
 }
 
 enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2, PK_CYCLIC = 3 };
-enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2 };
+// p: zeroGradient, fixedValue, totalPressure; U: zeroGradient, fixedValue,
+// slip, pressureInletOutletVelocity; T: zeroGradient, fixedValue,
+// totalTemperature
+enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2, BC_TOTAL_P = 3, BC_PIOV = 4, BC_TOTAL_T = 5 };
 enum { FLUX_AUSM = 0, FLUX_ROE = 1, FLUX_HLLC = 2 };
 
 // ---------------------------------------------------------------- mesh ----
@@ -323,9 +326,15 @@ struct Settings {
     double lowerBound = 0.95, rhoMin = -GREAT, TMin = SMALL, TMax = GREAT;
 };
 
+// pv: fixed p or totalPressure p0; uv: fixed U or the tangentialVelocity of
+// pressureInletOutletVelocity; tv: fixed T or totalTemperature T0.
+// pInit/uInit/tInit: initial boundary values of the derived conditions
+// (the `value` entries of the field files; empty: p0, the owner cell's U,
+// T0).
 struct BC {
     int pT = BC_ZG, uT = BC_ZG, tT = BC_ZG;
     vec pv, uv, tv;
+    vec pInit, uInit, tInit;
 };
 
 // Sub-block of a blockFvMatrix: diag[nC], upper/lower[nI], each ncomp
@@ -352,6 +361,12 @@ struct Case {
     vec rho, U, T, p, e, rhoU, rhoE;
     // boundary values (per boundary face index f - nI)
     vec pB, UB, TB, rhoB, eB, rhoUB, rhoEB;
+    // mass flux of the last flux evaluation on the boundary faces (the
+    // registered surfaceScalarField phi the derived conditions look up) and
+    // the valueFraction switch of pressureInletOutletVelocity (1: inflow at
+    // its last update)
+    vec phiB;
+    std::vector<int> inflowB;
     // pseudo-time state
     vec pseudoCo, rDT;
     bool haveInit = false, havePrev = false;
@@ -366,20 +381,48 @@ struct Case {
 
 inline double gammaOf(const Case& c) { return c.s.gamma; }
 
-// fixedValue / zeroGradient / slip evaluation of p, U, T on the boundary and
-// the derived rho = psi*p, e = Cv*T fields (updateFields.H:80-104).
-void evalBoundary(Case& k) {
+// Boundary values of p, U, T and the derived rho = psi*p, e = Cv*T fields
+// (updateFields.H:90-105: p, U and T corrected in that order).
+// fixedValue / zeroGradient / slip from the current cells; the derived
+// inflow conditions (OpenFOAM-v2112 finiteVolume, restated):
+//   totalPressure (psi, gamma > 1): p0 / (1 + 0.5 psi (gamma-1)/gamma
+//       (1 - pos0(phi)) |U_b|^2)^(gamma/(gamma-1)), with psi = 1/(R T_b)
+//       and U_b the boundary values before this correction (thermo.correct()
+//       runs first and U is corrected after p);
+//   pressureInletOutletVelocity: directionMixed with valueFraction
+//       neg(phi) (I - n n) and refValue the tangential part of
+//       tangentialVelocity: U_b = vf & ref + (I - vf) & U_P;
+//   totalTemperature: T0 / (1 + 0.5 psi (gamma-1)/gamma (1 - pos0(phi))
+//       |U_b|^2) with the corrected U_b;
+// phi the boundary mass flux of the last flux evaluation.  init: the
+// createFields state (createFields.H:51-145): the derived conditions take
+// their `value` entries (defaults p0, the owner cell's U, T0),
+// valueFraction starts at zero and phi = Sf & (rho_b U_b).
+void evalBoundary(Case& k, bool init) {
     Mesh& m = *k.m;
     const int nB = m.nF - m.nI;
+    const vec UOld = k.UB, TOld = k.TB;
     k.pB.assign(nB, 0); k.UB.assign(3 * nB, 0); k.TB.assign(nB, 0);
     k.rhoB.assign(nB, 0); k.eB.assign(nB, 0); k.rhoUB.assign(3 * nB, 0); k.rhoEB.assign(nB, 0);
+    if (init) { k.phiB.assign(nB, 0.0); k.inflowB.assign(nB, 0); }
+    const double gM1ByG = (k.s.gamma - 1.0) / k.s.gamma;
     for (size_t pi = 0; pi < m.pStart.size(); ++pi) {
         if (m.pKind[pi] == PK_EMPTY || m.pKind[pi] == PK_CYCLIC) continue;
         const BC& bc = k.bcs[pi];
         for (int i = 0; i < m.pSize[pi]; ++i) {
             const int f = m.pStart[pi] + i, b = f - m.nI, c = m.own[f];
-            k.pB[b] = bc.pT == BC_FIXED ? bc.pv[i] : k.p[c];
-            k.TB[b] = bc.tT == BC_FIXED ? bc.tv[i] : k.T[c];
+            const double npos = 1.0 - (k.phiB[b] >= 0 ? 1.0 : 0.0);
+            // p
+            if (bc.pT == BC_FIXED) k.pB[b] = bc.pv[i];
+            else if (bc.pT == BC_TOTAL_P) {
+                if (init) k.pB[b] = bc.pInit.empty() ? bc.pv[i] : bc.pInit[i];
+                else {
+                    const double psip = 1.0 / (k.s.R * TOld[b]);
+                    const V3 Up = v3(&UOld[3 * b]);
+                    k.pB[b] = bc.pv[i] / std::pow(1.0 + 0.5 * psip * gM1ByG * npos * magSqr(Up), 1.0 / gM1ByG);
+                }
+            } else k.pB[b] = k.p[c];
+            // U
             V3 Uc = v3(&k.U[3 * c]);
             V3 ub;
             if (bc.uT == BC_FIXED) ub = v3(&bc.uv[3 * i]);
@@ -390,13 +433,39 @@ void evalBoundary(Case& k) {
                        -2.0 * n.y * n.x, 1 - 2.0 * n.y * n.y, -2.0 * n.y * n.z,
                        -2.0 * n.z * n.x, -2.0 * n.z * n.y, 1 - 2.0 * n.z * n.z}};
                 ub = (Uc + tdotv(Tr, Uc)) / 2.0;
+            } else if (bc.uT == BC_PIOV) {
+                if (init) ub = bc.uInit.empty() ? Uc : v3(&bc.uInit[3 * i]);
+                else if (k.phiB[b] < 0) {
+                    k.inflowB[b] = 1;
+                    const V3 n = v3(&m.Sf[3 * f]) / m.magSf[f];
+                    const V3 tv = v3(&bc.uv[3 * i]);
+                    const V3 ref = tv - n * dot(n, tv);
+                    const T9 vf{{1 - n.x * n.x, -n.x * n.y, -n.x * n.z,
+                                 -n.y * n.x, 1 - n.y * n.y, -n.y * n.z,
+                                 -n.z * n.x, -n.z * n.y, 1 - n.z * n.z}};
+                    const T9 I{{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+                    ub = tdotv(vf, ref) + tdotv(I - vf, Uc);
+                } else {
+                    k.inflowB[b] = 0;
+                    ub = Uc;
+                }
             } else ub = Uc;
             k.UB[3 * b] = ub.x; k.UB[3 * b + 1] = ub.y; k.UB[3 * b + 2] = ub.z;
+            // T
+            if (bc.tT == BC_FIXED) k.TB[b] = bc.tv[i];
+            else if (bc.tT == BC_TOTAL_T) {
+                if (init) k.TB[b] = bc.tInit.empty() ? bc.tv[i] : bc.tInit[i];
+                else {
+                    const double psip = 1.0 / (k.s.R * TOld[b]);
+                    k.TB[b] = bc.tv[i] / (1.0 + 0.5 * psip * gM1ByG * npos * magSqr(ub));
+                }
+            } else k.TB[b] = k.T[c];
             const double psi = 1.0 / (k.s.R * k.TB[b]);
             k.eB[b] = k.Cv * k.TB[b];
             k.rhoB[b] = psi * k.pB[b];
             for (int d = 0; d < 3; ++d) k.rhoUB[3 * b + d] = k.rhoB[b] * k.UB[3 * b + d];
             k.rhoEB[b] = k.rhoB[b] * (k.eB[b] + 0.5 * magSqr(ub));
+            if (init) k.phiB[b] = dot(v3(&m.Sf[3 * f]), v3(&k.rhoUB[3 * b]));
         }
     }
 }
@@ -798,6 +867,7 @@ void calcFlux(Case& k, vec& phi, vec& phiU, vec& phiE) {
             hllcFace(k, gI, S, m.magSf[f], rL[f], rR[f], pL[f], pR[f], UL, UR, cL[f], cR[f], EL[f], ER[f], HL[f],
                      HR[f], ph, pu, pe);
         phi[f] = ph;
+        if (f >= m.nI && !m.cyclicFace(f)) k.phiB[f - m.nI] = ph;
         phiU[3 * f] = pu.x; phiU[3 * f + 1] = pu.y; phiU[3 * f + 2] = pu.z;
         phiE[f] = pe;
     }
@@ -1190,11 +1260,19 @@ void assemble(Case& k) {
             const int f = m.pStart[pi] + i, b = f - nI, c = m.own[f];
             V3 Sf = v3(&m.Sf[3 * f]);
             V3 n = Sf / m.magSf[f];
-            const double pVIC = bc.pT == BC_FIXED ? 0.0 : 1.0;
-            const double tVIC = bc.tT == BC_FIXED ? 0.0 : 1.0;
+            // valueInternalCoeffs: 0 for fixedValue and the fixedValue-derived
+            // totalPressure / totalTemperature, 1 for zeroGradient; for the
+            // transform conditions 1 - snGradTransformDiag: slip 1 - |n|,
+            // pressureInletOutletVelocity 1 - sqrt(|diag(valueFraction)|)
+            // with valueFraction = (I - n n) at inflow, 0 otherwise
+            const double pVIC = (bc.pT == BC_FIXED || bc.pT == BC_TOTAL_P) ? 0.0 : 1.0;
+            const double tVIC = (bc.tT == BC_FIXED || bc.tT == BC_TOTAL_T) ? 0.0 : 1.0;
             V3 uVIC;
             if (bc.uT == BC_FIXED) uVIC = V3{0, 0, 0};
             else if (bc.uT == BC_SLIP) uVIC = V3{1 - std::fabs(n.x), 1 - std::fabs(n.y), 1 - std::fabs(n.z)};
+            else if (bc.uT == BC_PIOV && k.inflowB[b])
+                uVIC = V3{1 - std::sqrt(std::fabs(1 - n.x * n.x)), 1 - std::sqrt(std::fabs(1 - n.y * n.y)),
+                          1 - std::sqrt(std::fabs(1 - n.z * n.z))};
             else uVIC = V3{1, 1, 1};
             const double rhoB = k.rhoB[b], pB = k.pB[b], TB = k.TB[b];
             V3 UB = v3(&k.UB[3 * b]);
@@ -1729,7 +1807,7 @@ void updateFields(Case& k, const CVec& dW) {
         for (int d = 0; d < 3; ++d) k.rhoU[3 * c + d] = k.rho[c] * k.U[3 * c + d];
         k.rhoE[c] = k.rho[c] * (k.e[c] + 0.5 * magSqr(v3(&k.U[3 * c])));
     }
-    evalBoundary(k);
+    evalBoundary(k, false);
 }
 
 void setFields(Case& k, const double* rho, const double* U, const double* T) {
@@ -1745,7 +1823,7 @@ void setFields(Case& k, const double* rho, const double* U, const double* T) {
         k.p[c] = k.rho[c] / psi;
     }
     makeConserved(k);
-    evalBoundary(k);
+    evalBoundary(k, true);
 }
 
 // One dbnsFoam steady pseudo-step: outerLoop.H + updateFields.H, with the
@@ -2381,9 +2459,34 @@ void orc_set_bc(void* h, int patch, int pT, int uT, int tT, const double* pv, co
     BC& b = k.bcs[patch];
     const int n = k.m->pSize[patch];
     b.pT = pT; b.uT = uT; b.tT = tT;
-    if (pT == BC_FIXED) b.pv.assign(pv, pv + n);
+    if (pT == BC_FIXED || pT == BC_TOTAL_P) b.pv.assign(pv, pv + n);
     if (uT == BC_FIXED) b.uv.assign(uv, uv + 3 * n);
-    if (tT == BC_FIXED) b.tv.assign(tv, tv + n);
+    if (uT == BC_PIOV) {
+        if (uv) b.uv.assign(uv, uv + 3 * n);
+        else b.uv.assign(3 * (size_t)n, 0.0);
+    }
+    if (tT == BC_FIXED || tT == BC_TOTAL_T) b.tv.assign(tv, tv + n);
+}
+
+// Boundary values, boundary mass flux and inflow switch of every boundary
+// face (zero on empty and cyclic faces).
+void orc_get_boundary(void* h, double* p, double* U, double* T, double* phi, int* inflow) {
+    Case& k = *(Case*)h;
+    const int nB = k.m->nF - k.m->nI;
+    for (int b = 0; b < nB; ++b) {
+        p[b] = k.pB[b]; T[b] = k.TB[b]; phi[b] = k.phiB[b]; inflow[b] = k.inflowB[b];
+        for (int d = 0; d < 3; ++d) U[3 * b + d] = k.UB[3 * b + d];
+    }
+}
+
+// Initial boundary values of the derived conditions (any may be NULL).
+void orc_set_bc_values(void* h, int patch, const double* p, const double* U, const double* T) {
+    Case& k = *(Case*)h;
+    BC& b = k.bcs[patch];
+    const int n = k.m->pSize[patch];
+    if (p) b.pInit.assign(p, p + n);
+    if (U) b.uInit.assign(U, U + 3 * n);
+    if (T) b.tInit.assign(T, T + n);
 }
 
 void orc_set_fields(void* h, const double* rho, const double* U, const double* T) {

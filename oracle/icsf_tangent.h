@@ -105,11 +105,14 @@ void primsFromW(const Case& k, const T* W, const T* Sf, StateT<T>& s) {
         const BC& bc = k.bcs[pi];
         for (int i = 0; i < m.pSize[pi]; ++i) {
             const int f = m.pStart[pi] + i, b = f - m.nI, c = m.own[f];
-            const T pB = bc.pT == BC_FIXED ? T(bc.pv[i]) : s.p[c];
-            const T TB = bc.tT == BC_FIXED ? T(bc.tv[i]) : s.Tt[c];
+            // the derived inflow conditions hold values set at the last state
+            // update from lagged quantities: constants of this evaluation
+            const T pB = bc.pT == BC_FIXED ? T(bc.pv[i]) : bc.pT == BC_TOTAL_P ? T(k.pB[b]) : s.p[c];
+            const T TB = bc.tT == BC_FIXED ? T(bc.tv[i]) : bc.tT == BC_TOTAL_T ? T(k.TB[b]) : s.Tt[c];
             V3T<T> Uc{s.U[3 * c], s.U[3 * c + 1], s.U[3 * c + 2]};
             V3T<T> ub;
             if (bc.uT == BC_FIXED) ub = lift<T>(&bc.uv[3 * i]);
+            else if (bc.uT == BC_PIOV) ub = lift<T>(&k.UB[3 * (size_t)b]);
             else if (bc.uT == BC_SLIP) {
                 const V3T<T> S = ld(&Sf[3 * (size_t)f]);
                 const V3T<T> n = S / sqrtT(magSqrT(S));

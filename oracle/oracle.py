@@ -30,6 +30,8 @@ _SIG = {
     "orc_case_new": (_vp, [_vp, _vp, _vp]),
     "orc_case_free": (None, [_vp]),
     "orc_set_bc": (None, [_vp, _i, _i, _i, _i, _vp, _vp, _vp]),
+    "orc_set_bc_values": (None, [_vp, _i, _vp, _vp, _vp]),
+    "orc_get_boundary": (None, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "orc_set_fields": (None, [_vp, _vp, _vp, _vp]),
     "orc_get_fields": (None, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "orc_calc_flux": (None, [_vp, _vp, _vp, _vp]),
@@ -211,8 +213,10 @@ class OracleCase:
             pv = _f64(bc.p_val) if bc.p_val is not None else None
             uv = _f64(bc.U_val) if bc.U_val is not None else None
             tv = _f64(bc.T_val) if bc.T_val is not None else None
-            self._bc = (pv, uv, tv)
             lib().orc_set_bc(self.h, i, bc.p_type, bc.U_type, bc.T_type, _p(pv), _p(uv), _p(tv))
+            if bc.p_init is not None or bc.U_init is not None or bc.T_init is not None:
+                init = [None if a is None else _f64(a) for a in (bc.p_init, bc.U_init, bc.T_init)]
+                lib().orc_set_bc_values(self.h, i, *[_p(a) for a in init])
         self.set_fields(case.rho, case.U, case.T)
 
     def __del__(self):
@@ -228,6 +232,15 @@ class OracleCase:
         rho, U, T, p, W = np.empty(n), np.empty((n, 3)), np.empty(n), np.empty(n), np.empty((5, n))
         lib().orc_get_fields(self.h, _p(rho), _p(U), _p(T), _p(p), _p(W))
         return dict(rho=rho, U=U, T=T, p=p, W=W)
+
+    def boundary(self):
+        """Boundary values [nB] (p, U [nB,3], T), the boundary mass flux phi
+        of the last flux evaluation and the pressureInletOutletVelocity
+        inflow switch of every boundary face."""
+        nB = self.nFaces - self.nInternal
+        p, U, T, phi, inflow = np.empty(nB), np.empty((nB, 3)), np.empty(nB), np.empty(nB), np.empty(nB, np.int32)
+        lib().orc_get_boundary(self.h, _p(p), _p(U), _p(T), _p(phi), _p(inflow))
+        return dict(p=p, U=U, T=T, phi=phi, inflow=inflow)
 
     def calcFlux(self):
         nF = self.nFaces

@@ -77,3 +77,17 @@ def linear_velocity_case(mu, R=287.0):
                                       None, Cf @ A.T + u0, np.full(pt.size, T0)))
     c.bcs = bcs
     return c, geo, A
+
+
+def total_inlet_box(n=(10, 6, 5), mu=1.8e-5, flux="ROE", mach=0.5, **kw):
+    """Jittered box (permuted + RCM) with the tutorials' inflow patch on -x:
+    totalPressure, pressureInletOutletVelocity, totalTemperature at the
+    total state of the given Mach number; fixed static p at +x, slip walls."""
+    c = cases.case_box(n=n, mu=mu, flux=flux, mach=(mach, mach), p_jump=1.0, **kw)
+    fac = 1.0 + 0.5 * (cases.GAMMA - 1.0) * mach * mach
+    for i, pt in enumerate(c.mesh.patches):
+        if pt.name == "inlet":
+            Uv = np.zeros((pt.size, 3)); Uv[:, 0] = mach * np.sqrt(cases.GAMMA * cases.R_GAS * 300.0)
+            c.bcs[i] = cases.total_inlet(pt.size, 1e5 * fac ** (cases.GAMMA / (cases.GAMMA - 1.0)), 300.0 * fac,
+                                         U_init=Uv, T_init=300.0)
+    return prepare_case(c, permute_seed=17)

@@ -338,15 +338,28 @@ def _dense(d, u, l, m):
     return A
 
 
-@pytest.mark.parametrize("mu", [0.0, 1.8e-5])
-def test_jacobian_is_derivative_of_first_order_lf_model(mu):
+@pytest.mark.parametrize("mu,bcset", [(0.0, "box"), (1.8e-5, "box"), (0.0, "total"), (1.8e-5, "total")])
+def test_jacobian_is_derivative_of_first_order_lf_model(mu, bcset):
     """The assembled matrix equals the complex-step derivative of the
     first-order Lax-Friedrichs residual model with frozen wave speeds:
     internal faces  |S|/2 (F(W_P)+F(W_N)) - |S|/2 lam (W_N-W_P) - s (W_N-W_P) [rho,rhoE,rhoU rows]
     boundary faces  |S| F(W_b(W_P)) + |S|/2 lam_b W_P
-    cells           V rDT W_c."""
-    c = _upwind_case(mu=mu)
+    cells           V rDT W_c.
+    bcset "total": the tutorials' inflow patch (totalPressure,
+    pressureInletOutletVelocity, totalTemperature) after one pseudo-step, so
+    that the inflow switch is set: p_b and T_b are constants of the state
+    (corrected from lagged values), U_b = (n n) U_P on the axis-aligned
+    inlet, where the reference's diagonal valueInternalCoeffs
+    1 - sqrt(|1 - n_i^2|) equal n n exactly."""
+    if bcset == "box":
+        c = _upwind_case(mu=mu)
+    else:
+        c, _, _ = _total_inlet_case(n=(4, 3, 3), mu=mu)
+        c.fvSchemes = c.fvSchemes.replace("linearUpwind limitedGrad", "upwind")
     o = O.OracleCase(c)
+    if bcset == "total":
+        o.pseudo_steps(1)
+    bnd = o.boundary()
     b, rdt = o.residual()
     d, u, l = o.createJacobian()
     m = c.mesh
@@ -400,6 +413,14 @@ def test_jacobian_is_derivative_of_first_order_lf_model(mu):
                 Ub = U - nrm * (nrm @ U)
             else:
                 Ub = U
+            bb = fi - m.n_internal
+            if bc.p_type == cases.BC_TOTAL_PRESSURE:
+                pb = bnd["p"][bb]
+            if bc.T_type == cases.BC_TOTAL_TEMPERATURE:
+                Tb = bnd["T"][bb]
+            if bc.U_type == cases.BC_PRESSURE_INLET_OUTLET_VELOCITY:
+                assert bnd["inflow"][bb] == 1
+                Ub = nrm * (nrm @ U)
             rb = pb / (R * Tb)
             Wb = np.concatenate([[rb, rb * (CV * Tb + 0.5 * (Ub @ Ub))], rb * Ub])
             Ubr = np.real(Ub)
@@ -1014,3 +1035,75 @@ def test_time_step_bounding_factors():
     expect[bad] = 0.5 * co
     assert len(nb) == 6
     np.testing.assert_array_equal(got, expect)
+
+
+# ------------------------------------------ derived inflow condition pins --
+def _total_inlet_case(mach=0.5, flux="ROE", mu=0.0, n=(5, 4, 3), outflow=False):
+    """Box with the tutorials' inflow patch on -x (totalPressure,
+    pressureInletOutletVelocity, totalTemperature), fixed static p on +x,
+    slip walls; uniform axial state of the given Mach number (reversed with
+    outflow=True, so the inlet patch sees outflow)."""
+    c = cases.case_box(n=n, jitter=0.1, flux=flux, mu=mu, noise=0.01, p_jump=1.0, mach=(mach, mach))
+    T, p = 300.0, 1e5
+    p0 = p * (1 + 0.5 * (GAMMA - 1) * mach ** 2) ** (GAMMA / (GAMMA - 1))
+    T0 = T * (1 + 0.5 * (GAMMA - 1) * mach ** 2)
+    if outflow:
+        c.U[:, 0] = -np.abs(c.U[:, 0])
+    for i, pt in enumerate(c.mesh.patches):
+        if pt.name == "inlet":
+            Uin = np.zeros((pt.size, 3)); Uin[:, 0] = (-1 if outflow else 1) * mach * np.sqrt(GAMMA * R * T)
+            c.bcs[i] = cases.total_inlet(pt.size, p0, T0, U_init=Uin, T_init=T)
+    return c, p0, T0
+
+
+def test_total_conditions_isentropic_fixed_point():
+    """totalPressure / totalTemperature / pressureInletOutletVelocity at an
+    inflow patch (phi < 0), re-corrected with the cells frozen until the
+    lagged values settle: the fixed point obeys the textbook relations of a
+    steady isentropic inflow, T_b = T0 - |U_b|^2/(2 Cp) (total enthalpy)
+    and p_b = p0 (T_b/T0)^(gamma/(gamma-1)) (isentropic), with U_b the
+    normal part (n.U_P) n of the adjacent cell velocity (zero tangential
+    velocity).  Initial state: p_b = p0 (no `value` entry), U_b and T_b from
+    the `value` entries, phi = Sf.(rho_b U_b)."""
+    c, p0, T0 = _total_inlet_case()
+    o = O.OracleCase(c)
+    m = c.mesh
+    ip = [i for i, pt in enumerate(m.patches) if pt.name == "inlet"][0]
+    pt = m.patches[ip]
+    sl = slice(pt.start - m.n_internal, pt.start - m.n_internal + pt.size)
+    b0 = o.boundary()
+    np.testing.assert_array_equal(b0["p"][sl], p0)
+    np.testing.assert_array_equal(b0["T"][sl], 300.0)
+    assert (b0["phi"][sl] < 0).all() and (b0["inflow"][sl] == 0).all()
+    W = o.fields()["W"]
+    for _ in range(60):
+        o.set_state_w(W)
+    b = o.boundary()
+    assert (b["inflow"][sl] == 1).all()
+    geo = O.OracleMesh(m).geometry()
+    n = geo["Sf"][pt.start:pt.start + pt.size] / np.linalg.norm(geo["Sf"][pt.start:pt.start + pt.size], axis=1)[:, None]
+    Uc = o.fields()["U"][m.owner[pt.start:pt.start + pt.size]]
+    Un = np.sum(n * Uc, axis=1)[:, None] * n
+    np.testing.assert_allclose(b["U"][sl], Un, rtol=0, atol=1e-12 * np.abs(Uc).max())
+    Cp = GAMMA * R / (GAMMA - 1)
+    Tb = T0 - np.sum(b["U"][sl] ** 2, axis=1) / (2 * Cp)
+    np.testing.assert_allclose(b["T"][sl], Tb, rtol=1e-13)
+    np.testing.assert_allclose(b["p"][sl], p0 * (b["T"][sl] / T0) ** (GAMMA / (GAMMA - 1)), rtol=1e-13)
+
+
+def test_total_conditions_at_outflow():
+    """The same patch seeing outflow (phi >= 0): pos0(phi) = 1 switches the
+    dynamic terms off (p_b = p0, T_b = T0) and pressureInletOutletVelocity is
+    zeroGradient (U_b = U_P, valueFraction 0)."""
+    c, p0, T0 = _total_inlet_case(outflow=True)
+    o = O.OracleCase(c)
+    m = c.mesh
+    pt = [p for p in m.patches if p.name == "inlet"][0]
+    sl = slice(pt.start - m.n_internal, pt.start - m.n_internal + pt.size)
+    assert (o.boundary()["phi"][sl] > 0).all()
+    o.set_state_w(o.fields()["W"])
+    b = o.boundary()
+    np.testing.assert_array_equal(b["p"][sl], p0)
+    np.testing.assert_array_equal(b["T"][sl], T0)
+    np.testing.assert_array_equal(b["U"][sl], o.fields()["U"][m.owner[pt.start:pt.start + pt.size]])
+    assert (b["inflow"][sl] == 0).all()
