@@ -65,6 +65,7 @@ SIGNATURES = {
     "icsf_set_boundary": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp]),
     "icsf_set_boundary_values": (_i, [_vp, _i, _vp, _vp, _vp]),
     "icsf_set_state": (_i, [_vp, _vp]),
+    "icsf_pseudo_time_loop": (_i, [_vp, _i, _i, _vp, _vp]),
     "icsf_get_boundary_values": (_i, [_vp, _vp, _vp, _vp]),
     "icsf_set_fields": (_i, [_vp, _vp, _vp, _vp]),
     "icsf_get_fields": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),

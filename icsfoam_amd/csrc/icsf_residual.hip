@@ -20,6 +20,7 @@ __global__ void k_update(MeshView m, Phys ph, StateView s, const double* __restr
                          const double* __restrict__ T0) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
+    if (s.halt && *s.halt) return;
     const int nC = m.nC;
     double rho, e, T, p;
     V3 U;
@@ -101,6 +102,7 @@ __global__ void k_update(MeshView m, Phys ph, StateView s, const double* __restr
 __global__ void k_bc_update(MeshView m, Phys ph, StateView s, int init) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.nB) return;
+    if (s.halt && *s.halt) return;
     if (m.bKind[b] == PK_EMPTY || m.bKind[b] == PK_CYCLIC) return;
     const int f = m.nI + b;
     const int c = m.own[f];
@@ -518,7 +520,7 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
     } else {
         const int b = f - m.nI;
         const int c = m.own[f];
-        s.bcPhi[b] = fr;  // the registered phi the derived conditions read
+        if (!(s.halt && *s.halt)) s.bcPhi[b] = fr;  // the registered phi the derived conditions read
         const double* bf = s.bfld + (size_t)b * NFLD;
         const V3 UB{bf[F_UX], bf[F_UY], bf[F_UZ]};
         o[FT_LAM] = soundSpeed(ph, s.bT[b]) + fabs(dot(UB, S / ms));

@@ -126,6 +126,13 @@ struct icsf_case {
     void *blk = nullptr, *dg = nullptr, *dinv = nullptr;
     double* rD = nullptr;
     int* derr = nullptr;
+    // pseudo-time loop control (pseudotimeControl): device block [NCTL],
+    // stored initial residuals [5]; criteria from fvSolution pseudoTime
+    // (pseudoTol / pseudoTolRel NaN when absent)
+    int* dctl = nullptr;
+    double* dctlInit = nullptr;
+    double pseudoTol = 0, pseudoTolRel = 0;
+    int nPseudoCorr = 20, nPseudoCorrMin = 1;
     int blkFp32 = 0;
     bool haveResidual = false, assembled = false, haveFields = false;
     // Krylov work (FP64)
@@ -218,9 +225,9 @@ void resolveEvents(icsf_case* c) {
 
 // ---- device kernels local to the driver --------------------------------
 __global__ void k_ser(int nC, double* co, const double* dres, double coIncr, double coDecr, double coMin,
-                      double coMax) {
+                      double coMax, const int* halt) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nC) return;
+    if (c >= nC || *halt) return;
     const double* ini = dres;      // latest initial residuals
     const double* prv = dres + 5;  // previous
     const double nI = sqrt(ini[0] * ini[0] + ini[1] * ini[1] + (ini[2] * ini[2] + ini[3] * ini[3] + ini[4] * ini[4]));
@@ -234,6 +241,51 @@ __global__ void k_ser(int nC, double* co, const double* dres, double coIncr, dou
 
 __global__ void k_copy5(const double* a, double* b) {
     if (threadIdx.x < 5) b[threadIdx.x] = a[threadIdx.x];
+}
+// the same inside a pseudo-step (nothing once the loop has ended)
+__global__ void k_copy5_step(const double* a, double* b, const int* halt) {
+    if (threadIdx.x < 5 && !*halt) b[threadIdx.x] = a[threadIdx.x];
+}
+
+// pseudotimeControl::loop and criteriaSatisfied (pseudotimeControl.C:72-101,
+// 159-241; steady state) at the start of a pseudo-step, on the device so
+// that replayed steps evaluate it without the host: ctl[CTL_CORR] is the
+// outer-iteration counter corr_, dres the initial residuals of the
+// previous step's solve (residuals_, outerLoop.H:97), init the residuals
+// stored at corr_ == 2.  A step started after convergence ends the loop
+// (CTL_FIN, the halt flag of the state kernels); the step in which the
+// criteria first hold is the final iteration and runs.  maxInit takes the
+// largest of the five initial residuals; the relative check divides by
+// init + ROOTVSMALL (residualsIO operator/).  Disabled (CTL_EN 0) outside
+// icsf_pseudo_time_loop.
+__global__ void k_pseudo_ctl(int* ctl, double* init, const double* dres, double tol, double tolRel, int nCorr,
+                             int nCorrMin) {
+    if (threadIdx.x != 0 || !ctl[CTL_EN] || ctl[CTL_FIN]) return;
+    const int corr = ++ctl[CTL_CORR];
+    if (ctl[CTL_CONV]) {
+        ctl[CTL_FIN] = 1;
+        return;
+    }
+    bool crit = false;
+    if (ctl[CTL_FIRST] || corr == 1 || corr == nCorr) {
+        ctl[CTL_FIRST] = 0;
+    } else {
+        const bool store = corr == 2;
+        if (store)
+            for (int i = 0; i < 5; ++i) init[i] = dres[i];
+        double mx = -1.0e300;
+        for (int i = 0; i < 5; ++i) mx = fmax(mx, dres[i]);
+        const bool absC = mx < tol;
+        bool relC = false;
+        if (!store) {
+            double mr = -1.0e300;
+            for (int i = 0; i < 5; ++i) mr = fmax(mr, dres[i] / (init[i] + 1.0e-150));
+            relC = mr < tolRel;
+        }
+        crit = corr >= nCorrMin && (absC || relC);
+    }
+    if (crit) ctl[CTL_CONV] = 1;
+    ctl[CTL_RUN]++;
 }
 
 // boundLocalTimeStep.H:174-271 evaluated before the state update (as in
@@ -254,9 +306,9 @@ __global__ void k_bound_flag(int nC, double lb, const double* W, double* bad) {
 // Over the cell's internal and cyclic faces (the factor is a minimum of
 // 0.5 and 0.75, independent of the face order): the neighbours come from
 // the cell's matrix slots and cyclic-face list, read contiguously.
-__global__ void k_bound_apply(MeshView m, const double* bad, double* co) {
+__global__ void k_bound_apply(MeshView m, const double* bad, double* co, const int* halt) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m.nC) return;
+    if (c >= m.nC || (halt && *halt)) return;
     double f = 1.0;
     const bool badC = bad[c] != 0.0;
     const int len = m.rowLen[c];
@@ -422,6 +474,10 @@ int readSettings(icsf_case* c, const char* thermo, const char* schemes, const ch
     lb = lb > 0 ? lb : 0.0;
     lb = lb < 0.99 ? lb : 0.99;
     c->lowerBound = lb;
+    c->pseudoTol = fd.scalar("pseudoTime/pseudoTol", std::nan(""));
+    c->pseudoTolRel = fd.scalar("pseudoTime/pseudoTolRel", std::nan(""));
+    c->nPseudoCorr = (int)fd.scalar("pseudoTime/nPseudoCorr", 20);
+    c->nPseudoCorrMin = (int)fd.scalar("pseudoTime/nPseudoCorrMin", 1);
     ph.rhoMin = fd.scalar("pseudoTime/rhoMin", -kGREAT);
     ph.TMin = fd.scalar("pseudoTime/TMin", kSMALL);
     ph.TMax = fd.scalar("pseudoTime/TMax", kGREAT);
@@ -625,12 +681,15 @@ int autotune(icsf_case* c) {
 
 int pseudoStep(icsf_case* c, int& nIterOut) {
     const MeshView& m = c->mv;
+    const int* halt = c->sv.halt;
     c->corr++;
+    k_pseudo_ctl<<<1, 64, 0, c->st>>>(c->dctl, c->dctlInit, c->dres, c->pseudoTol, c->pseudoTolRel,
+                                       c->nPseudoCorr, c->nPseudoCorrMin);
     if (c->haveInit) {
         if (c->corr != 1 && c->havePrev)
             k_ser<<<nblk(m.nC, 256), 256, 0, c->st>>>(m.nC, c->sv.pseudoCo, c->dres, c->coIncr, c->coDecr, c->coMin,
-                                                      c->coMax);
-        k_copy5<<<1, 64, 0, c->st>>>(c->dres, c->dres + 5);
+                                                      c->coMax, halt);
+        k_copy5_step<<<1, 64, 0, c->st>>>(c->dres, c->dres + 5, halt);
         c->havePrev = true;
     }
     RC(residualStage(c, !c->fuseCells));
@@ -639,11 +698,11 @@ int pseudoStep(icsf_case* c, int& nIterOut) {
     int nIter = 0;
     RC(solveStage(c, c->sv.W, c->sv.b, nIter));
     nIterOut = nIter;
-    k_copy5<<<1, 64, 0, c->st>>>(gmresResidualsDevice(c->g), c->dres);
+    k_copy5_step<<<1, 64, 0, c->st>>>(gmresResidualsDevice(c->g), c->dres, halt);
     c->haveInit = true;
     if (c->localTS && c->bounding) {
         k_bound_flag<<<nblk(m.nC, 256), 256, 0, c->st>>>(m.nC, c->lowerBound, c->sv.W, c->fac);
-        k_bound_apply<<<nblk(m.nC, 256), 256, 0, c->st>>>(c->mv, c->fac, c->sv.pseudoCo);
+        k_bound_apply<<<nblk(m.nC, 256), 256, 0, c->st>>>(c->mv, c->fac, c->sv.pseudoCo, halt);
     }
     {
         Timed t(c, 5);
@@ -1125,7 +1184,7 @@ int hbPseudoStep(icsf_hb* h) {
         icsf_case* c = h->inst[J];
         if (c->localTS && c->bounding) {
             k_bound_flag<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, c->lowerBound, c->sv.W, c->fac);
-            k_bound_apply<<<nblk(m.nC, 256), 256, 0, h->st>>>(c->mv, c->fac, c->sv.pseudoCo);
+            k_bound_apply<<<nblk(m.nC, 256), 256, 0, h->st>>>(c->mv, c->fac, c->sv.pseudoCo, nullptr);
         }
         launchUpdate(m, c->ph, c->sv, h->dW + J * vs, 1, nullptr, nullptr, nullptr, h->st);
         // the residual and the matrix belong to the state before the update
@@ -1512,6 +1571,15 @@ int createCase(icsf_mesh* mesh, const char* thermo, const char* schemes, const c
     RC(B.alloc(C->derr, 1));
     HIPCHK(hipMemset(C->derr, 0, sizeof(int)));
     C->sv.err = C->derr;
+    RC(B.alloc(C->dctl, NCTL));
+    RC(B.alloc(C->dctlInit, 5));
+    {
+        int z[NCTL] = {0};
+        z[CTL_FIRST] = 1;
+        HIPCHK(hipMemcpy(C->dctl, z, sizeof(z), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(C->dctlInit, 0, 5 * sizeof(double)));
+    }
+    C->sv.halt = C->dctl + CTL_FIN;
     RC(B.alloc(C->fac, (size_t)nC));
     if (shared) {
         HIPCHK(hipStreamSynchronize(C->st));
@@ -1948,6 +2016,57 @@ int icsf_pseudo_steps(icsf_case* c, int nSteps, double* residuals, int* nIter) {
     }
     HIPCHK(hipGetLastError());
     const int rc = takeDeviceErrors(c->st, {c->derr});
+    resolveEvents(c);
+    return rc;
+}
+
+int icsf_pseudo_time_loop(icsf_case* c, int maxSteps, int batch, int* stepsRun, int* converged) {
+    if (!c || maxSteps < 0 || batch < 1) return fail(ICSF_ERR_ARG, "icsf_pseudo_time_loop: bad arguments");
+    if (!c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
+    if (std::isnan(c->pseudoTol) || std::isnan(c->pseudoTolRel))
+        return fail(ICSF_ERR_KEYWORD, "pseudoTime: keywords pseudoTol and pseudoTolRel are required "
+                                      "(pseudotimeControl.C:56-58)");
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemset(c->derr, 0, sizeof(int)));
+    // corr_ continues from the steps run so far (solnControl.setCorr of a
+    // restart, initialise.H); the loop starts enabled and not converged
+    {
+        int h[NCTL];
+        HIPCHK(hipMemcpy(h, c->dctl, sizeof(h), hipMemcpyDeviceToHost));
+        h[CTL_CORR] = c->corr;
+        h[CTL_CONV] = h[CTL_FIN] = h[CTL_RUN] = 0;
+        h[CTL_EN] = 1;
+        HIPCHK(hipMemcpy(c->dctl, h, sizeof(h), hipMemcpyHostToDevice));
+    }
+    const int corr0 = c->corr;
+    int issued = 0, h[NCTL] = {0};
+    int rc = ICSF_OK;
+    while (issued < maxSteps) {
+        const int n = std::min(batch, maxSteps - issued);
+        for (int i = 0; i < n && rc == ICSF_OK; ++i) {
+            int it = 1;
+            rc = graphable(c) ? graphStep(c) : pseudoStep(c, it);
+        }
+        if (rc) break;
+        issued += n;
+        HIPCHK(hipMemcpyAsync(h, c->dctl, sizeof(h), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (h[CTL_FIN]) break;
+    }
+    // pseudotimeControl resets converged_ when the loop ends; the halt flag
+    // is cleared so that later calls change the state again
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(h, c->dctl, sizeof(h), hipMemcpyDeviceToHost));
+    const int run = h[CTL_RUN];
+    const bool conv = h[CTL_CONV] || h[CTL_FIN];
+    h[CTL_CONV] = h[CTL_FIN] = h[CTL_EN] = 0;
+    HIPCHK(hipMemcpy(c->dctl, h, sizeof(h), hipMemcpyHostToDevice));
+    c->corr = corr0 + run;
+    if (stepsRun) *stepsRun = run;
+    if (converged) *converged = conv ? 1 : 0;
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    rc = takeDeviceErrors(c->st, {c->derr});
     resolveEvents(c);
     return rc;
 }

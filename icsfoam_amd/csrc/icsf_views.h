@@ -117,7 +117,16 @@ struct StateView {
     double* bcPhi;
     int* bcIn;
     const double* bcInit;
+    // end flag of the device-controlled pseudo-time loop (nonzero: the loop
+    // has ended, and steps still queued leave the state as it is: the
+    // state update, the boundary correction and the boundary mass flux
+    // return at once)
+    const int* halt;
 };
+
+// Device control block of the pseudo-time loop (pseudotimeControl):
+// ints [NCTL] and the stored initial residuals [5].
+enum { CTL_CORR = 0, CTL_CONV = 1, CTL_FIN = 2, CTL_RUN = 3, CTL_EN = 4, CTL_FIRST = 5, NCTL = 8 };
 
 // bits of the device error flags
 enum { ERR_FLAG_DIAG = 1, ERR_FLAG_NONFINITE = 2 };

@@ -199,6 +199,14 @@ class dbnsCase:
         check(lib().icsf_pseudo_steps(self._h, n, ptr(res), None))
         return res
 
+    def pseudo_time_loop(self, max_steps: int, batch: int = 8):
+        """dbnsFoam's steady loop under pseudotimeControl: steps until the
+        pseudoTol / pseudoTolRel criteria end it (one final step after they
+        hold) or max_steps; returns (steps run, converged)."""
+        run, conv = C.c_int(0), C.c_int(0)
+        check(lib().icsf_pseudo_time_loop(self._h, int(max_steps), int(batch), C.byref(run), C.byref(conv)))
+        return run.value, bool(conv.value)
+
     def pseudoCo(self):
         co = np.empty(self.nCells)
         check(lib().icsf_get_pseudo_co(self._h, ptr(co)))

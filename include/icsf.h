@@ -301,6 +301,20 @@ int icsf_solve_for_incr(icsf_case* c, const double* W, const double* b, double* 
  * update; graph replays do not stop early). */
 int icsf_pseudo_steps(icsf_case* c, int nSteps, double* residuals, int* nIter);
 
+/* The steady pseudo-time loop as dbnsFoam runs it (dbnsFoam.C:105-124 with
+ * pseudotimeControl::loop, pseudotimeControl.C:72-101,159-241): before each
+ * pseudo-step the criteria of fvSolution pseudoTime (pseudoTol,
+ * pseudoTolRel -- both required, ICSF_ERR_KEYWORD otherwise --, nPseudoCorr,
+ * nPseudoCorrMin) are evaluated on the initial residuals of the previous
+ * step: absolute max(initial) < pseudoTol, or relative max(initial /
+ * initial of step 1) < pseudoTolRel (from step 3 on); no check at step 1 or
+ * at step nPseudoCorr.  When they hold, that step is the final one.  The
+ * check runs on the device inside every (graph-replayed) step; the host
+ * issues `batch` steps between reads of the end flag, and steps issued after
+ * the end leave the state unchanged.  At most maxSteps steps; stepsRun and
+ * converged (1 if the criteria ended the loop) may be NULL. */
+int icsf_pseudo_time_loop(icsf_case* c, int maxSteps, int batch, int* stepsRun, int* converged);
+
 /* Pseudo-Courant field [nCells] (pseudoCoField). */
 int icsf_get_pseudo_co(icsf_case* c, double* co);
 

@@ -372,6 +372,9 @@ struct Case {
     bool haveInit = false, havePrev = false;
     double initS[2], initV[3], prevS[2], prevV[3];
     int corr = 0;
+    // pseudotimeControl state: firstIteration_, initResiduals_
+    bool ptFirst = true;
+    double ptInit[5] = {0, 0, 0, 0, 0};
     // coupled matrix: sub-blocks SS[i][j], SV[i] (i scalar row, vector col),
     // VS[j] (vector row, scalar col j), VV
     LduBlock SS[2][2], SV[2], VS[2], VV;
@@ -2466,6 +2469,51 @@ void orc_set_bc(void* h, int patch, int pT, int uT, int tT, const double* pv, co
         else b.uv.assign(3 * (size_t)n, 0.0);
     }
     if (tT == BC_FIXED || tT == BC_TOTAL_T) b.tv.assign(tv, tv + n);
+}
+
+// dbnsFoam's steady loop (dbnsFoam.C:105-124) under pseudotimeControl::loop
+// (pseudotimeControl.C:159-241) with criteriaSatisfied (:72-101): before
+// each step, corr_ advances; a converged control ends the loop; otherwise
+// the criteria are tested on residuals_ (the initial residuals of the last
+// solve, outerLoop.H:97), except on the first call, at corr_ == 1 and at
+// corr_ == nPseudoCorr (finalIter); corr_ == 2 stores initResiduals_ and
+// skips the relative test; maxInit is the largest initial residual
+// (residualsIO.H:243-256) and the relative ratio divides by
+// initial + ROOTVSMALL (:183-200).
+int orc_pseudo_time_loop(void* h, int maxSteps, double tol, double tolRel, int nCorr, int nCorrMin, int* run,
+                         int* conv) {
+    Case& k = *(Case*)h;
+    int corr = k.corr, n = 0;
+    bool converged = false;
+    while (n < maxSteps) {
+        ++corr;
+        if (converged) break;
+        bool crit = false;
+        if (k.ptFirst || corr == 1 || corr == nCorr) {
+            k.ptFirst = false;
+        } else {
+            const double res[5] = {k.initS[0], k.initS[1], k.initV[0], k.initV[1], k.initV[2]};
+            const bool store = corr == 2;
+            if (store) std::copy(res, res + 5, k.ptInit);
+            double mx = -1e300;
+            for (double v : res) mx = std::max(mx, v);
+            bool rel = false;
+            if (!store) {
+                double mr = -1e300;
+                for (int i = 0; i < 5; ++i) mr = std::max(mr, res[i] / (k.ptInit[i] + ROOTVSMALL));
+                rel = mr < tolRel;
+            }
+            crit = corr >= nCorrMin && (mx < tol || rel);
+        }
+        if (crit) converged = true;
+        Res r;
+        const int rc = pseudoStep(k, r);
+        if (rc) return rc;
+        ++n;
+    }
+    *run = n;
+    *conv = converged ? 1 : 0;
+    return 0;
 }
 
 // Boundary values, boundary mass flux and inflow switch of every boundary

@@ -32,6 +32,7 @@ _SIG = {
     "orc_set_bc": (None, [_vp, _i, _i, _i, _i, _vp, _vp, _vp]),
     "orc_set_bc_values": (None, [_vp, _i, _vp, _vp, _vp]),
     "orc_get_boundary": (None, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "orc_pseudo_time_loop": (_i, [_vp, _i, C.c_double, C.c_double, _i, _i, _vp, _vp]),
     "orc_set_fields": (None, [_vp, _vp, _vp, _vp]),
     "orc_get_fields": (None, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "orc_calc_flux": (None, [_vp, _vp, _vp, _vp]),
@@ -130,6 +131,17 @@ def parse_settings(thermo: str, schemes: str, solution: str):
     return iset, rset
 
 
+def pseudo_controls(solution: str):
+    """(pseudoTol, pseudoTolRel, nPseudoCorr, nPseudoCorrMin) of pseudoTime."""
+    import re
+
+    def num(key, default):
+        m = re.search(r"\b" + key + r"\s+([-+0-9.eE]+)\s*;", solution)
+        return float(m.group(1)) if m else default
+    return num("pseudoTol", float("nan")), num("pseudoTolRel", float("nan")), int(num("nPseudoCorr", 20)), \
+        int(num("nPseudoCorrMin", 1))
+
+
 class OracleMesh:
     def __init__(self, mesh):
         start, size, kind = mesh.patch_arrays()
@@ -203,6 +215,7 @@ class OracleCase:
     def __init__(self, case):
         self.mesh = OracleMesh(case.mesh)
         self.iset, self.rset = parse_settings(case.thermo, case.fvSchemes, case.fvSolution)
+        self.ptol = pseudo_controls(case.fvSolution)
         self.h = lib().orc_case_new(self.mesh.h, _p(self.iset), _p(self.rset))
         self.nCells = self.mesh.nCells
         self.nInternal = case.mesh.n_internal
@@ -304,6 +317,16 @@ class OracleCase:
         L, R = np.empty(self.nFaces), np.empty(self.nFaces)
         lib().orc_reconstruct(self.h, field, _p(L), _p(R))
         return L, R
+
+    def pseudo_time_loop(self, max_steps):
+        """dbnsFoam's steady loop under pseudotimeControl (criteria from the
+        case's pseudoTime dictionary); returns (steps run, converged)."""
+        run, conv = C.c_int(0), C.c_int(0)
+        rc = lib().orc_pseudo_time_loop(self.h, int(max_steps), self.ptol[0], self.ptol[1], self.ptol[2],
+                                        self.ptol[3], C.byref(run), C.byref(conv))
+        if rc:
+            raise RuntimeError(f"oracle pseudo-step failed ({rc})")
+        return run.value, bool(conv.value)
 
     def pseudoCo(self):
         co = np.empty(self.nCells)

@@ -1107,3 +1107,47 @@ def test_total_conditions_at_outflow():
     np.testing.assert_array_equal(b["T"][sl], T0)
     np.testing.assert_array_equal(b["U"][sl], o.fields()["U"][m.owner[pt.start:pt.start + pt.size]])
     assert (b["inflow"][sl] == 0).all()
+
+
+# ------------------------------------------------ pseudotimeControl pins --
+def _expected_stop(hist, tol, tolRel, nCorr=20, nCorrMin=1):
+    """Step at which pseudotimeControl ends a steady run, restated from the
+    reference's criteria on a recorded history of initial residuals:
+    before step j (j >= 2, j != nCorr) the previous step's residuals r are
+    tested, absolutely (max r < tol) and, from j = 3, relative to the
+    residuals of step 1 (max r / r_1 < tolRel); the first step j at which a
+    test holds (and j >= nCorrMin) is the last one run."""
+    r1 = hist[0]
+    for j in range(2, len(hist) + 1):
+        if j == nCorr:
+            continue
+        r = hist[j - 2]
+        ok = r.max() < tol or (j >= 3 and (r / (r1 + 1e-150)).max() < tolRel)
+        if ok and j >= nCorrMin:
+            return j
+    return None
+
+
+@pytest.mark.parametrize("tol,tolRel,nCorr,nCorrMin", [(1e-12, 1e-3, 20, 1), (3e-4, 1e-12, 20, 1),
+                                                      (1e-12, 1e-2, 20, 40), (1e-12, 1e-2, 3, 1)])
+def test_pseudotime_control_stops_where_criteria_hold(tol, tolRel, nCorr, nCorrMin):
+    """The oracle's loop (pseudotimeControl::loop/criteriaSatisfied) runs
+    exactly the steps the criteria restated on an independently recorded
+    residual history allow, and ends in the same state; with neither
+    criterion reachable it stops at maxSteps unconverged."""
+    import dataclasses
+    c = cases.case_bump(40, 10)
+    sol = c.fvSolution.replace("pseudoTol 1e-12;", f"pseudoTol {tol!r};").replace(
+        "pseudoTolRel 1e-12;", f"pseudoTolRel {tolRel!r}; nPseudoCorrMin {nCorrMin};").replace(
+        "nPseudoCorr 1;", f"nPseudoCorr {nCorr};")
+    c = dataclasses.replace(c, fvSolution=sol)
+    a, b = O.OracleCase(c), O.OracleCase(c)
+    run, conv = a.pseudo_time_loop(400)
+    hist = np.array([b.pseudo_steps(1)[0][:5] for _ in range(max(run, 1))])
+    stop = _expected_stop(hist, tol, tolRel, nCorr, nCorrMin)
+    assert conv and stop == run
+    for k in ("W", "p", "T"):
+        np.testing.assert_array_equal(a.fields()[k], b.fields()[k])
+    c2 = dataclasses.replace(c, fvSolution=sol.replace(f"pseudoTol {tol!r};", "pseudoTol 1e-300;").replace(
+        f"pseudoTolRel {tolRel!r};", "pseudoTolRel 1e-300;"))
+    assert O.OracleCase(c2).pseudo_time_loop(12) == (12, False)
