@@ -235,7 +235,10 @@ def bench_bump(steps=50):
     from oracle.build import FAST_LIB
     from oracle.build import build as build_oracle
 
-    c = prepare_case(cases.case_bump(200, 50, seed=1))
+    # the circularArcBump tutorial's inflow patch (totalPressure /
+    # pressureInletOutletVelocity / totalTemperature at the freestream's
+    # total state); with a fixed inflow velocity the channel chokes
+    c = prepare_case(cases.case_bump(200, 50, seed=1, bcs="tutorial"))
     g = dbnsCase.from_case(c)
     g.pseudo_steps(3)
     g.synchronize()
@@ -259,7 +262,7 @@ def bench_bump(steps=50):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (bump channel, M 0.675, 1% seeded 8-mode perturbation)",
             "config": {"workload": "200x50x1 bump (10000 cells), AUSM+up, GMRES(30) MGS + LU-SGS, CFL 50, "
-                                   "RCM + colour-major"},
+                                   "total-pressure inflow patch, RCM + colour-major"},
             "cpu_baseline": {"value": round(steps / dtc, 3), "unit": "pseudo-steps/s", "cores": 1, "kind": "port",
                              "sample": f"oracle (fast build) on the same case, the same {steps} steps"}}
 

@@ -97,6 +97,19 @@ SIGNATURES = {
     "icsf_pseudo_steps": (_i, [_vp, _i, _vp, _vp]),
     "icsf_get_pseudo_co": (_i, [_vp, _vp]),
     "icsf_synchronize": (_i, [_vp]),
+    "icsf_foam_mesh_read": (_i, [C.c_char_p, _vp]),
+    "icsf_foam_mesh_sizes": (_i, [_vp, _vp]),
+    "icsf_foam_mesh_get": (_i, [_vp] + [_vp] * 10),
+    "icsf_foam_mesh_patch_name": (_i, [_vp, _i, C.c_char_p, _i]),
+    "icsf_foam_mesh_free": (_i, [_vp]),
+    "icsf_foam_mesh_write": (_i, [C.c_char_p, _i, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp,
+                                  _vp]),
+    "icsf_foam_field_read": (_i, [C.c_char_p, _i, _vp]),
+    "icsf_foam_field_internal": (_i, [_vp, _vp, _vp]),
+    "icsf_foam_field_patch_type": (_i, [_vp, C.c_char_p, C.c_char_p, _i]),
+    "icsf_foam_field_patch_entry": (_i, [_vp, C.c_char_p, C.c_char_p, _i, _vp]),
+    "icsf_foam_field_free": (_i, [_vp]),
+    "icsf_foam_field_write": (_i, [C.c_char_p, C.c_char_p, C.c_char_p, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp]),
     "icsf_timing_enable": (_i, [_vp, _i]),
     "icsf_timing_read": (_i, [_vp, _i, _vp, _vp]),
 }

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libicsf.so")
 
-SOURCES = ["icsf_mesh.cpp", "icsf_dict.cpp", "icsf_hb.cpp", "icsf_residual.hip", "icsf_krylov.hip", "icsf_adjoint.hip",
+SOURCES = ["icsf_mesh.cpp", "icsf_dict.cpp", "icsf_hb.cpp", "icsf_foamio.cpp", "icsf_residual.hip", "icsf_krylov.hip", "icsf_adjoint.hip",
            "icsf_solver.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -51,7 +51,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), 8)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *DEVICE, "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [HIPCC, *DEVICE, "-shared", "-fPIC", *objs, "-lz", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

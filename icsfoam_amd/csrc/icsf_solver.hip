@@ -1238,6 +1238,11 @@ int hbCheckInstance(const icsf_hb* h, int J) {
 
 }  // namespace
 
+namespace icsf {
+// error text for the other host translation units (case file I/O)
+int reportError(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace icsf
+
 extern "C" {
 
 const char* icsf_last_error(void) { return g_err.c_str(); }

@@ -1,0 +1,266 @@
+"""OpenFOAM case files: polyMesh and volFields (ASCII, optionally .gz).
+
+Host mirror over the C ABI of libicsf.so (icsf_foam_*): what dbnsFoam reads
+at start-up (createDynamicFvMesh.H: constant/polyMesh; createFields.H:51-145:
+the p, U, T fields with their boundary conditions) and what it writes
+(runTime.write(): the fields of a time directory).
+
+  read_polymesh(dir)                  -> (PolyMesh, patch names)
+  read_case(case_dir, ...)            -> Case (fields, boundary conditions)
+  write_polymesh(dir, mesh, names)
+  write_fields(time_dir, mesh, names, fields, boundary)
+
+Boundary-condition types mapped onto the library's kinds: zeroGradient,
+fixedValue, slip, noSlip (U fixed at zero), totalPressure,
+pressureInletOutletVelocity, totalTemperature, symmetryPlane/symmetry
+(p, T zeroGradient; U slip); empty and cyclic patches take their constraint
+types.  Other types raise unless the caller maps them (`overrides`).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from ._lib import IcsfError, as_f64, as_i32, check, lib, ptr
+from .cases import (BC_FIXED_VALUE, BC_PRESSURE_INLET_OUTLET_VELOCITY, BC_SLIP, BC_TOTAL_PRESSURE,
+                    BC_TOTAL_TEMPERATURE, BC_ZERO_GRADIENT, PATCH_CYCLIC, PATCH_EMPTY, BoundarySpec, Case,
+                    Patch, PolyMesh, R_GAS)
+
+
+def read_polymesh(poly_mesh_dir: str) -> Tuple[PolyMesh, List[str]]:
+    """constant/polyMesh -> (PolyMesh, patch names)."""
+    h = C.c_void_p()
+    check(lib().icsf_foam_mesh_read(os.fsencode(poly_mesh_dir), C.byref(h)))
+    try:
+        sz = np.zeros(6, np.int32)
+        check(lib().icsf_foam_mesh_sizes(h, ptr(sz)))
+        nP, nF, nI, nC, nPa, nV = (int(x) for x in sz)
+        pts = np.empty((nP, 3)); off = np.empty(nF + 1, np.int32); fv = np.empty(nV, np.int32)
+        own = np.empty(nF, np.int32); nei = np.empty(nI, np.int32)
+        ps, pz, pk, pn = (np.empty(nPa, np.int32) for _ in range(4))
+        pT = np.empty((nPa, 9))
+        check(lib().icsf_foam_mesh_get(h, ptr(pts), ptr(off), ptr(fv), ptr(own), ptr(nei), ptr(ps), ptr(pz),
+                                       ptr(pk), ptr(pn), ptr(pT)))
+        names = []
+        buf = C.create_string_buffer(256)
+        for p in range(nPa):
+            check(lib().icsf_foam_mesh_patch_name(h, p, buf, 256))
+            names.append(buf.value.decode())
+    finally:
+        lib().icsf_foam_mesh_free(h)
+    patches = [Patch(names[p], int(pk[p]), int(ps[p]), int(pz[p]), int(pn[p]),
+                     pT[p].reshape(3, 3).copy() if pk[p] == PATCH_CYCLIC else None) for p in range(nPa)]
+    return PolyMesh(points=pts, face_off=off, face_verts=fv, owner=own, neighbour=nei, patches=patches,
+                    n_cells=nC), names
+
+
+def write_polymesh(poly_mesh_dir: str, mesh: PolyMesh, names: Optional[List[str]] = None) -> None:
+    os.makedirs(poly_mesh_dir, exist_ok=True)
+    names = names or [p.name for p in mesh.patches]
+    start, size, kind = mesh.patch_arrays()
+    nbr, T = mesh.coupling_arrays()
+    cn = (C.c_char_p * len(names))(*[n.encode() for n in names])
+    check(lib().icsf_foam_mesh_write(os.fsencode(poly_mesh_dir), mesh.points.shape[0], ptr(as_f64(mesh.points)),
+                                     mesh.n_faces, ptr(as_i32(mesh.face_off)), ptr(as_i32(mesh.face_verts)),
+                                     mesh.n_internal, ptr(as_i32(mesh.owner)), ptr(as_i32(mesh.neighbour)),
+                                     len(names), C.cast(cn, C.c_void_p), ptr(start), ptr(size), ptr(kind),
+                                     ptr(nbr), ptr(as_f64(T))))
+
+
+class FoamField:
+    """One volScalarField / volVectorField file."""
+
+    def __init__(self, path: str, n_cells: int):
+        h = C.c_void_p()
+        check(lib().icsf_foam_field_read(os.fsencode(path), int(n_cells), C.byref(h)))
+        self._h = h
+        nc = C.c_int(0)
+        check(lib().icsf_foam_field_internal(h, C.byref(nc), None))
+        self.n_comp = nc.value
+        v = np.empty(n_cells * self.n_comp)
+        check(lib().icsf_foam_field_internal(h, None, ptr(v)))
+        self.internal = v if self.n_comp == 1 else v.reshape(n_cells, 3)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().icsf_foam_field_free(h)
+            self._h = None
+
+    def patch_type(self, patch: str) -> str:
+        buf = C.create_string_buffer(128)
+        check(lib().icsf_foam_field_patch_type(self._h, patch.encode(), buf, 128))
+        return buf.value.decode()
+
+    def patch_entry(self, patch: str, key: str, n_faces: int) -> Optional[np.ndarray]:
+        v = np.empty(n_faces * self.n_comp)
+        try:
+            check(lib().icsf_foam_field_patch_entry(self._h, patch.encode(), key.encode(), n_faces, ptr(v)))
+        except IcsfError:
+            return None
+        return v if self.n_comp == 1 else v.reshape(n_faces, 3)
+
+
+def _bc(pt: Patch, fp: FoamField, fU: FoamField, fT: FoamField, overrides: Dict[str, Dict[str, str]]):
+    """BoundarySpec of one patch from its p, U, T entries."""
+    n, name = pt.size, pt.name
+
+    def typ(field, f):
+        return overrides.get(field, {}).get(name) or f.patch_type(name)
+
+    tp, tu, tt = typ("p", fp), typ("U", fU), typ("T", fT)
+    if pt.kind == PATCH_CYCLIC:
+        return None
+    sym = ("symmetryPlane", "symmetry")
+    spec = dict(p_type=None, U_type=None, T_type=None)
+    vals = {}
+    if tp in ("zeroGradient",) + sym:
+        spec["p_type"] = BC_ZERO_GRADIENT
+    elif tp == "fixedValue":
+        spec["p_type"] = BC_FIXED_VALUE; vals["p_val"] = fp.patch_entry(name, "value", n)
+    elif tp == "totalPressure":
+        spec["p_type"] = BC_TOTAL_PRESSURE; vals["p_val"] = fp.patch_entry(name, "p0", n)
+        vals["p_init"] = fp.patch_entry(name, "value", n)
+    if tu in ("zeroGradient",):
+        spec["U_type"] = BC_ZERO_GRADIENT
+    elif tu in ("slip",) + sym:
+        spec["U_type"] = BC_SLIP
+    elif tu == "noSlip":
+        spec["U_type"] = BC_FIXED_VALUE; vals["U_val"] = np.zeros((n, 3))
+    elif tu == "fixedValue":
+        spec["U_type"] = BC_FIXED_VALUE; vals["U_val"] = fU.patch_entry(name, "value", n)
+    elif tu == "pressureInletOutletVelocity":
+        spec["U_type"] = BC_PRESSURE_INLET_OUTLET_VELOCITY
+        vals["U_val"] = fU.patch_entry(name, "tangentialVelocity", n)
+        vals["U_init"] = fU.patch_entry(name, "value", n)
+    if tt in ("zeroGradient",) + sym:
+        spec["T_type"] = BC_ZERO_GRADIENT
+    elif tt == "fixedValue":
+        spec["T_type"] = BC_FIXED_VALUE; vals["T_val"] = fT.patch_entry(name, "value", n)
+    elif tt == "totalTemperature":
+        spec["T_type"] = BC_TOTAL_TEMPERATURE; vals["T_val"] = fT.patch_entry(name, "T0", n)
+        vals["T_init"] = fT.patch_entry(name, "value", n)
+    for field, t, k in (("p", tp, "p_type"), ("U", tu, "U_type"), ("T", tt, "T_type")):
+        if spec[k] is None:
+            raise IcsfError(-2, f"patch {name}: boundary condition {t} of {field} is not supported "
+                                f"(map it with overrides)")
+    for k, v in list(vals.items()):
+        if v is None and k in ("p_val", "T_val") or (k == "U_val" and spec["U_type"] == BC_FIXED_VALUE and v is None):
+            raise IcsfError(-1, f"patch {name}: missing value for {k}")
+    return BoundarySpec(spec["p_type"], spec["U_type"], spec["T_type"], **vals)
+
+
+def read_case(case_dir: str, time: str = "0", thermo: Optional[str] = None, fvSchemes: Optional[str] = None,
+              fvSolution: Optional[str] = None, overrides: Optional[Dict[str, Dict[str, str]]] = None,
+              R: float = R_GAS) -> Tuple[Case, List[str]]:
+    """A case directory -> (Case, patch names): constant/polyMesh, the p, U,
+    T files of `time` (rho = p/(R T), createFields.H), and the dictionaries
+    (given text, else constant/thermophysicalProperties, system/fvSchemes,
+    system/fvSolution).  overrides: {"T": {"outlet": "zeroGradient"}, ...}
+    replaces boundary-condition types the library does not provide."""
+    mesh, names = read_polymesh(os.path.join(case_dir, "constant", "polyMesh"))
+    n = mesh.n_cells
+    fp = FoamField(os.path.join(case_dir, time, "p"), n)
+    fU = FoamField(os.path.join(case_dir, time, "U"), n)
+    fT = FoamField(os.path.join(case_dir, time, "T"), n)
+    overrides = overrides or {}
+    bcs = [None if pt.kind == PATCH_EMPTY else _bc(pt, fp, fU, fT, overrides) for pt in mesh.patches]
+    for i, pt in enumerate(mesh.patches):
+        if pt.kind == PATCH_EMPTY:
+            bcs[i] = BoundarySpec(BC_ZERO_GRADIENT, BC_ZERO_GRADIENT, BC_ZERO_GRADIENT)
+
+    def text(given, *path):
+        if given is not None:
+            return given
+        with open(os.path.join(case_dir, *path)) as f:
+            return f.read()
+    p = np.asarray(fp.internal, np.float64)
+    T = np.asarray(fT.internal, np.float64)
+    U = np.asarray(fU.internal, np.float64)
+    case = Case(os.path.basename(os.path.abspath(case_dir)), mesh, p / (R * T), U.copy(), T.copy(), bcs,
+                text(thermo, "constant", "thermophysicalProperties"), text(fvSchemes, "system", "fvSchemes"),
+                text(fvSolution, "system", "fvSolution"))
+    return case, names
+
+
+_TYPES_P = {BC_ZERO_GRADIENT: "zeroGradient", BC_FIXED_VALUE: "fixedValue", BC_TOTAL_PRESSURE: "totalPressure"}
+_TYPES_U = {BC_ZERO_GRADIENT: "zeroGradient", BC_FIXED_VALUE: "fixedValue", BC_SLIP: "slip",
+            BC_PRESSURE_INLET_OUTLET_VELOCITY: "pressureInletOutletVelocity"}
+_TYPES_T = {BC_ZERO_GRADIENT: "zeroGradient", BC_FIXED_VALUE: "fixedValue", BC_TOTAL_TEMPERATURE: "totalTemperature"}
+_DIMS = {"p": "[1 -1 -2 0 0 0 0]", "U": "[0 1 -1 0 0 0 0]", "T": "[0 0 0 1 0 0 0]", "rho": "[1 -3 0 0 0 0 0]"}
+
+
+def _list(v, n, vec):
+    """A nonuniform list entry body with 17 significant digits."""
+    a = np.broadcast_to(np.asarray(v, np.float64), (n, 3) if vec else (n,))
+    items = " ".join("(%s)" % " ".join(repr(float(x)) for x in r) for r in a) if vec else \
+        " ".join(repr(float(x)) for x in a)
+    return f"nonuniform List<{'vector' if vec else 'scalar'}> {n}({items})"
+
+
+def _entries(key, typ, bc, n, gamma=1.4):
+    """Keyword lines of the derived conditions (OpenFOAM names)."""
+    if bc is None:
+        return None
+    if key == "p" and typ == "totalPressure":
+        return (f"        p0              {_list(bc.p_val, n, False)};\n        rho             none;\n"
+                f"        psi             thermo:psi;\n        gamma           {gamma!r};\n")
+    if key == "U" and typ == "pressureInletOutletVelocity":
+        tv = bc.U_val if bc.U_val is not None else np.zeros((n, 3))
+        return f"        tangentialVelocity {_list(tv, n, True)};\n"
+    if key == "T" and typ == "totalTemperature":
+        return f"        T0              {_list(bc.T_val, n, False)};\n        gamma           {gamma!r};\n"
+    return None
+
+
+def write_fields(time_dir: str, mesh: PolyMesh, names: List[str], fields: Dict[str, np.ndarray],
+                 bcs: Optional[List[Optional[BoundarySpec]]] = None,
+                 boundary: Optional[Dict[str, np.ndarray]] = None) -> None:
+    """Write p, U, T (and rho if given) of a state as volFields of a time
+    directory (runTime.write()).  Patch types follow the case's boundary
+    conditions (`calculated` where bcs is None); boundary values [nB] (as
+    dbnsCase.boundary() returns) become the patches' value entries."""
+    os.makedirs(time_dir, exist_ok=True)
+    nI = mesh.n_internal
+    for key, arr in fields.items():
+        arr = as_f64(arr)
+        ncomp = 3 if arr.ndim == 2 else 1
+        types, values, sizes, entries = [], [], [], []
+        for i, pt in enumerate(mesh.patches):
+            sizes.append(pt.size)
+            entries.append(None)
+            bc = bcs[i] if bcs is not None else None
+            if pt.kind == PATCH_EMPTY:
+                types.append("empty"); values.append(None); continue
+            if pt.kind == PATCH_CYCLIC:
+                types.append("cyclic"); values.append(None); continue
+            if bc is None:
+                t = "calculated"
+            elif key == "p":
+                t = _TYPES_P[bc.p_type]
+            elif key == "U":
+                t = _TYPES_U[bc.U_type]
+            elif key == "T":
+                t = _TYPES_T[bc.T_type]
+            else:
+                t = "calculated"
+            types.append(t)
+            entries[-1] = _entries(key, t, bc, pt.size)
+            bv = None
+            if boundary is not None and key in boundary:
+                b = np.asarray(boundary[key])
+                bv = as_f64(b[pt.start - nI:pt.start - nI + pt.size])
+            values.append(bv)
+        cn = (C.c_char_p * len(names))(*[x.encode() for x in names])
+        ct = (C.c_char_p * len(types))(*[x.encode() for x in types])
+        keep = [v for v in values if v is not None]
+        cv = (C.c_void_p * len(values))(*[None if v is None else v.ctypes.data for v in values])
+        ce = (C.c_char_p * len(entries))(*[None if e is None else e.encode() for e in entries])
+        check(lib().icsf_foam_field_write(os.fsencode(os.path.join(time_dir, key)), key.encode(),
+                                          _DIMS.get(key, "[0 0 0 0 0 0 0]").encode(), ncomp, mesh.n_cells,
+                                          ptr(arr), len(names), C.cast(cn, C.c_void_p), C.cast(ct, C.c_void_p),
+                                          ptr(as_i32(sizes)), C.cast(cv, C.c_void_p), C.cast(ce, C.c_void_p)))
+        del keep

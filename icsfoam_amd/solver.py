@@ -30,10 +30,11 @@ def _bc_values(p_type, U_type, T_type, p_val, U_val, T_val):
     """The value arrays a condition kind takes: fixed values, p0
     (totalPressure), tangentialVelocity (pressureInletOutletVelocity, may be
     None), T0 (totalTemperature)."""
-    pv = as_f64(p_val) if p_type in (BC_FIXED_VALUE, BC_TOTAL_PRESSURE) else None
-    uv = as_f64(U_val) if (U_type == BC_FIXED_VALUE or
-                           (U_type == BC_PRESSURE_INLET_OUTLET_VELOCITY and U_val is not None)) else None
-    tv = as_f64(T_val) if T_type in (BC_FIXED_VALUE, BC_TOTAL_TEMPERATURE) else None
+    def arr(v, wanted):
+        return as_f64(v) if (wanted and v is not None) else None
+    pv = arr(p_val, p_type in (BC_FIXED_VALUE, BC_TOTAL_PRESSURE))
+    uv = arr(U_val, U_type in (BC_FIXED_VALUE, BC_PRESSURE_INLET_OUTLET_VELOCITY))
+    tv = arr(T_val, T_type in (BC_FIXED_VALUE, BC_TOTAL_TEMPERATURE))
     return pv, uv, tv
 
 

@@ -335,6 +335,64 @@ int icsf_synchronize(icsf_case* c);
 int icsf_timing_enable(icsf_case* c, int mask);
 int icsf_timing_read(icsf_case* c, int n, double* ms, long long* count);
 
+/* ---- OpenFOAM case files (host; no GPU use) --------------------------
+ * ASCII files as OpenFOAM writes them; each may instead be present as
+ * "<name>.gz" (gzip), which OpenFOAM reads transparently. */
+typedef struct icsf_foam_mesh icsf_foam_mesh;
+typedef struct icsf_foam_field icsf_foam_field;
+
+/* Read constant/polyMesh (points, faces as faceList or faceCompactList,
+ * owner, neighbour, boundary), as fvMesh construction reads it for
+ * dbnsFoam (createDynamicFvMesh.H).  Patch types patch, wall, empty, cyclic,
+ * symmetryPlane/symmetry (kind ICSF_PATCH); cyclic pairs get their
+ * neighbourPatch index and forwardT: identity for translational pairs,
+ * for rotational ones the rotation about rotationAxis taking the paired
+ * patch's mean normal to minus this patch's.  Sizes are checked (owner /
+ * faces, patches tiling the boundary faces in order, vertex labels). */
+int icsf_foam_mesh_read(const char* polyMeshDir, icsf_foam_mesh** out);
+/* sizes[0..5] = nPoints, nFaces, nInternalFaces, nCells, nPatches, length
+ * of the face-vertex list */
+int icsf_foam_mesh_sizes(const icsf_foam_mesh* m, int* sizes);
+/* The arrays icsf_mesh_create takes (any may be NULL); patchTransform
+ * [nPatches][9], zero off cyclic patches. */
+int icsf_foam_mesh_get(const icsf_foam_mesh* m, double* points, int* faceOffsets, int* faceVerts, int* owner,
+                       int* neighbour, int* patchStart, int* patchSize, int* patchKind, int* patchNeighbour,
+                       double* patchTransform);
+int icsf_foam_mesh_patch_name(const icsf_foam_mesh* m, int patch, char* name, int len);
+int icsf_foam_mesh_free(icsf_foam_mesh* m);
+/* Write a polyMesh directory (the five files, ASCII) from the arrays of
+ * icsf_mesh_create; cyclic patches get neighbourPatch and a translational
+ * or rotational transform. */
+int icsf_foam_mesh_write(const char* polyMeshDir, int nPoints, const double* points, int nFaces,
+                         const int* faceOffsets, const int* faceVerts, int nInternalFaces, const int* owner,
+                         const int* neighbour, int nPatches, const char* const* patchNames, const int* patchStart,
+                         const int* patchSize, const int* patchKind, const int* patchNeighbour,
+                         const double* patchTransform);
+
+/* Read a volScalarField or volVectorField file of nCells cells (the fields
+ * createFields.H:51-145 reads: p, U, T): internalField given uniform or
+ * nonuniform; boundaryField entries queried by patch name. */
+int icsf_foam_field_read(const char* path, int nCells, icsf_foam_field** out);
+/* nComp (1 or 3) and the internal values [nCells][nComp] (either may be NULL) */
+int icsf_foam_field_internal(const icsf_foam_field* f, int* nComp, double* values);
+/* boundaryField/<patch>/type */
+int icsf_foam_field_patch_type(const icsf_foam_field* f, const char* patch, char* type, int len);
+/* boundaryField/<patch>/<key> (value, p0, T0, tangentialVelocity,
+ * inletValue, ...) expanded to nFaces values [nFaces][nComp]; uniform,
+ * nonuniform and $internalField; ICSF_ERR_ARG when absent. */
+int icsf_foam_field_patch_entry(const icsf_foam_field* f, const char* patch, const char* key, int nFaces,
+                                double* values);
+int icsf_foam_field_free(icsf_foam_field* f);
+/* Write a volScalarField (nComp 1) or volVectorField (nComp 3): internal
+ * values nonuniform, per patch its type, the entry text patchEntries[p]
+ * (complete "keyword value;" lines, e.g. p0 and gamma of totalPressure;
+ * array or element may be NULL) and, where patchValues[p] is not NULL, a
+ * nonuniform value entry. */
+int icsf_foam_field_write(const char* path, const char* name, const char* dimensions, int nComp, int nCells,
+                          const double* internal, int nPatches, const char* const* patchNames,
+                          const char* const* patchTypes, const int* patchSizes, const double* const* patchValues,
+                          const char* const* patchEntries);
+
 /* ---- harmonic balance (dbnsFullyImplicitHBFoam) ----------------------
  * Spectral operator D [nInstants][nInstants] (row-major) of
  * HBZone::updateHBOperators (HBZone.C:270-356) for the omega list of

@@ -91,3 +91,10 @@ def total_inlet_box(n=(10, 6, 5), mu=1.8e-5, flux="ROE", mach=0.5, **kw):
             c.bcs[i] = cases.total_inlet(pt.size, 1e5 * fac ** (cases.GAMMA / (cases.GAMMA - 1.0)), 300.0 * fac,
                                          U_init=Uv, T_init=300.0)
     return prepare_case(c, permute_seed=17)
+
+
+def bump_tutorial_100x25():
+    """The converged-state parity case: 100x25 bump with the tutorial's
+    inflow patch and the 200x50 bump's schemes and solver (the case of
+    tests/golden/make_bump_converged.py)."""
+    return prepare_case(cases.case_bump(100, 25, bcs="tutorial"), permute_seed=11)

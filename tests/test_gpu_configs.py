@@ -32,9 +32,10 @@ from helpers import rel_err
 pytestmark = pytest.mark.gpu
 
 
-def test_bump_200x50_fifty_steps():
-    print("[config] bump 200x50, 50 pseudo-steps", flush=True)
-    c = cases.case_bump(200, 50, seed=1)
+@pytest.mark.parametrize("bcs", ["tutorial", "fixed"])
+def test_bump_200x50_fifty_steps(bcs):
+    print(f"[config] bump 200x50 ({bcs} inflow), 50 pseudo-steps", flush=True)
+    c = cases.case_bump(200, 50, seed=1, bcs=bcs)
     g, o = dbnsCase.from_case(c), OracleCase(c)
     hg = g.pseudo_steps(50)
     ho = o.pseudo_steps(50)
