@@ -6,7 +6,9 @@ colour-major numbering; viscous Roe flux (mu = 1.8e-5, Pr = 0.72), Mach
 0.5 -> 0.9 ramp, planar tanh pressure jump (ratio 2), 2% smooth noise;
 local pseudo-time CFL 50.  Two forms of the same pseudo-step:
   fp64  : GMRES(30) (left LU-SGS, one cycle) with FP64 Jacobian, factors,
-          vectors and basis - the reference algorithm;
+          vectors and basis - the reference's precision throughout, with
+          classical Gram-Schmidt (the reference orthogonalises with
+          modified Gram-Schmidt; --orth MGS selects it);
   mixed : FP64 residual/update and FP64 outer FGMRES(1) around an FP32
           inner GMRES(30) with FP32 LU-SGS on the FP32 Jacobian.
 `value` is the mixed form's pseudo-steps/s; both forms are reported.
@@ -183,26 +185,63 @@ def time_form(case, steps, warmup, label):
                 breakdown_count=[int(x) for x in bcnt])
 
 
-def cpu_baseline(n_sample: int, n_full: int, steps: int = 2):
-    """Oracle (sequential C++ restatement of the reference path, FP64) on a
-    bounded sample of the same workload; steps/s scaled to the full cell
-    count by the cell ratio (the work per step is linear in cells)."""
+def _oracle_rank(job):
+    """One CPU rank of the baseline (runs in a child process that loads only
+    the oracle library, never the HIP one): `warmup` untimed pseudo-steps,
+    then `steps` timed ones of the prepared case; returns (cells, per-step
+    seconds)."""
+    case, steps, warmup, lib_path = job
     from oracle import oracle as O
+
+    O.use_library(lib_path)
+    o = O.OracleCase(case)
+    o.pseudo_steps(warmup)
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        o.pseudo_steps(1)
+        ts.append(time.perf_counter() - t0)
+    return case.mesh.n_cells, ts
+
+
+def cpu_baseline(n_sample: int, n_full: int, steps: int = 5, warmup: int = 2, ranks: int = 0):
+    """The oracle (sequential C++ FP64 restatement of the reference path,
+    timing build) on the host cores, in the reference's own parallel model:
+    one single-threaded rank per core, each owning its share of the cells
+    (dbnsFoam under mpirun, per-rank LU-SGS).  One rank alone gives the
+    1-core figure; `ranks` ranks run concurrently (default: the host's
+    cores, at most 16, the GPU box's CPU share) for the all-core figure.
+    Each rank runs the n_sample^3 box of the same workload (a bounded sample
+    of the 160^3 case's per-rank share), `warmup` untimed steps, then the
+    median of `steps` timed steps; steps/s of the full case = aggregate
+    cells per second / full cell count (the work per step is linear in
+    cells).  The oracle runs the reference's orthogonalisation (modified
+    Gram-Schmidt)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+
     from oracle.build import FAST_FLAGS, FAST_LIB
     from oracle.build import build as build_oracle
     from icsfoam_amd.mesh import prepare_case
 
     if not os.path.exists(FAST_LIB):
         build_oracle(fast=True)
-    O.use_library(FAST_LIB)
-    OracleCase = O.OracleCase
     c = prepare_case(make_case(n_sample, "fp64", "MGS"), permute_seed=2024)
-    o = OracleCase(c)
-    o.pseudo_steps(1)
-    t0 = time.perf_counter()
-    o.pseudo_steps(steps)
-    dt = (time.perf_counter() - t0) / steps
-    scale = (n_sample ** 3) / float(n_full ** 3)
+    ranks = ranks or min(16, os.cpu_count() or 1)
+    ctx = mp.get_context("spawn")
+    t_wall = time.perf_counter()
+    with cf.ProcessPoolExecutor(max_workers=1, mp_context=ctx) as ex:
+        cells1, ts1 = ex.submit(_oracle_rank, (c, steps, warmup, FAST_LIB)).result()
+    one = float(np.median(ts1))
+    log(f"[bench] cpu baseline: 1 rank, {n_sample}^3, median {one:.3f} s/step "
+        f"({time.perf_counter() - t_wall:.0f} s)")
+    t_wall = time.perf_counter()
+    with cf.ProcessPoolExecutor(max_workers=ranks, mp_context=ctx) as ex:
+        res = list(ex.map(_oracle_rank, [(c, steps, warmup, FAST_LIB)] * ranks))
+    rate_all = sum(cells / float(np.median(ts)) for cells, ts in res)  # cells * steps / s
+    med_all = float(np.median([np.median(ts) for _, ts in res]))
+    log(f"[bench] cpu baseline: {ranks} concurrent ranks, median {med_all:.3f} s/step per rank "
+        f"({time.perf_counter() - t_wall:.0f} s)")
     cpu = platform.processor() or ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -211,11 +250,17 @@ def cpu_baseline(n_sample: int, n_full: int, steps: int = 2):
                 break
     except OSError:
         pass
-    return dict(value=scale / dt, unit="pseudo-steps/s", cores=1, kind="port",
-                sample=f"oracle (sequential C++ FP64 restatement of the reference path, g++ "
-                       f"{' '.join(FAST_FLAGS)}, one thread as one OpenFOAM rank) on the {n_sample}^3 box of "
-                       f"the same workload, {steps} timed steps after 1 warm-up: {dt:.2f} s/step, scaled by the "
-                       f"cell ratio {n_sample ** 3}/{n_full ** 3}; host CPU: {cpu}; nproc {os.cpu_count()}")
+    full = float(n_full) ** 3
+    return dict(value=rate_all / full, unit="pseudo-steps/s", cores=ranks, kind="port",
+                one_core_value=cells1 / one / full,
+                sample=f"oracle (sequential C++ FP64 restatement of the reference path: MGS GMRES(30), "
+                       f"sequential LU-SGS; g++ {' '.join(FAST_FLAGS)}) as {ranks} concurrent single-threaded "
+                       f"ranks, one per host core, each on the {n_sample}^3 box of the same workload "
+                       f"({cells1} cells; the reference's one-rank-per-core model); median of {steps} timed "
+                       f"steps after {warmup} warm-up steps per rank: {med_all:.2f} s/step per rank "
+                       f"concurrently, {one:.2f} s/step for one rank alone (one_core_value); steps/s of the "
+                       f"{n_full}^3 case = aggregate cells/s / {int(full)}; host CPU: {cpu}; "
+                       f"nproc {os.cpu_count()}")
 
 
 def bench_bump(steps=50):

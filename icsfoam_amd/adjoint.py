@@ -60,6 +60,23 @@ def residual_vjp_device(case, W: Optional[torch.Tensor], v: torch.Tensor,
     return gW, gS, gC
 
 
+def residual_records_device(case, W: Optional[torch.Tensor]) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Records of the residual's non-differentiable points at state W
+    (icsf_residual_records_device): (AUSM+up branch bits [nF], limiter
+    max/min element [nC, 8], limiter face [nC, 8]), int32 device tensors.
+    Equal records at two states: one smooth piece of the residual."""
+    n, nF = case.nCells, case.mesh.nFaces
+    Wc = None if W is None else _dev_f64(W, (5, n), "W")
+    dev = Wc.device if Wc is not None else torch.device("cuda", torch.cuda.current_device())
+    fb = torch.empty((nF,), dtype=torch.int32, device=dev)
+    te = torch.empty((n, 8), dtype=torch.int32, device=dev)
+    tf = torch.empty((n, 8), dtype=torch.int32, device=dev)
+    torch.cuda.current_stream(dev).synchronize()
+    check(lib().icsf_residual_records_device(case._h, None if Wc is None else Wc.data_ptr(), fb.data_ptr(),
+                                             te.data_ptr(), tf.data_ptr()))
+    return fb, te, tf
+
+
 class ResidualFunction(torch.autograd.Function):
     """b = residual(W) with d/dW by the library's reverse pass."""
 

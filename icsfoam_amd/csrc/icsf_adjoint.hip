@@ -40,7 +40,7 @@ DF bool used(int k) { return k != F_E; }
 // (R) face values, indexed by field, and of Sf.  Recomputes the forward
 // quantities with the same expressions.
 DF void ausmFaceAdj(bool lowMach, V3 S, double magS, const double* L, const double* R, double bphi, V3 bphiU,
-                    double bphiE, double* bL, double* bR, V3& bS) {
+                    double bphiE, double* bL, double* bR, V3& bS, int* branches) {
     const V3 UL{L[F_UX], L[F_UY], L[F_UZ]}, UR{R[F_UX], R[F_UY], R[F_UZ]};
     const double cL = L[F_CS], cR = R[F_CS], pL = L[F_P], pR = R[F_P], rL = L[F_RHO], rR = R[F_RHO];
     const double HL = L[F_H], HR = R[F_H];
@@ -98,6 +98,11 @@ DF void ausmFaceAdj(bool lowMach, V3 S, double magS, const double* L, const doub
     const double sr = rL + rR, X = unR - unL;
     if (lowMach) p12 += -0.25 * PpL * PmR * sr * cf * X;
     const bool up = M12f >= 0;
+    if (branches)  // every switch of the forward flux: where the derivative jumps
+        *branches = (int)bl | (int)br << 1 | (int)bc << 2 | (int)(fabs(ML) < 1.0) << 3 |
+                    (int)(fabs(ML) >= 1.0 && ML > 0) << 4 | (int)(fabs(MR) < 1.0) << 5 |
+                    (int)(fabs(MR) >= 1.0 && MR < 0) << 6 | (int)(omM < 0.0) << 7 | (int)(kap == 0.2) << 8 |
+                    (int)up << 9;
     const double rup = up ? rL : rR, Hup = up ? HL : HR;
     const V3 Uup = up ? UL : UR;
     const double rhoa = M12f * cf * rup;
@@ -186,6 +191,7 @@ __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView
 #pragma unroll
         for (int k = 0; k < 2 * NFLD; ++k) lr[k] = 0.0;
         for (int d = 0; d < 3; ++d) { gS[3 * (size_t)f + d] = 0.0; gC[3 * (size_t)f + d] = 0.0; }
+        if (a.fbr) a.fbr[f] = 0;
         return;
     }
     const size_t nC = m.nC;
@@ -209,7 +215,8 @@ __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView
     const V3 S = ld3(m.Sf + 3 * (size_t)f);
     double bL[NFLD], bR[NFLD];
     V3 bS;
-    ausmFaceAdj(ph.lowMach, S, m.magSf[f], L, R, bf[0], V3{bf[2], bf[3], bf[4]}, bf[1], bL, bR, bS);
+    ausmFaceAdj(ph.lowMach, S, m.magSf[f], L, R, bf[0], V3{bf[2], bf[3], bf[4]}, bf[1], bL, bR, bS,
+                a.fbr ? a.fbr + f : nullptr);
     V3 bC{0, 0, 0};
     if (internal) {
 #pragma unroll

@@ -1830,6 +1830,32 @@ int icsf_get_boundary_values(icsf_case* c, double* p, double* U, double* T) {
     return ICSF_OK;
 }
 
+int icsf_residual_records_device(icsf_case* c, const double* W, int* faceBranches, int* limiterElement,
+                                 int* limiterFace) {
+    if (!c || !faceBranches || !limiterElement || !limiterFace)
+        return fail(ICSF_ERR_ARG, "icsf_residual_records_device: NULL argument");
+    if (!W && !c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
+    RC(adjointPrepare(c));
+    RC(uploadBC(c));
+    RC(stateFromW(c, W));
+    const size_t nC = c->mv.nC, nF = c->mv.nF;
+    DevBuf tmp;
+    double *v, *gW;
+    RC(tmp.alloc(v, 5 * nC));
+    RC(tmp.alloc(gW, 5 * nC));
+    HIPCHK(hipMemsetAsync(v, 0, 5 * nC * sizeof(double), c->st));
+    AdjView a = c->adj;
+    a.fbr = faceBranches;
+    launchAdjointForward(c->mv, c->ph, c->sv, a.gun, a.lim, c->st);
+    launchResidualVjp(c->mv, c->ph, c->sv, a, v, gW, c->adjGS, c->adjGC, c->st);
+    HIPCHK(hipMemcpyAsync(limiterElement, a.tgt, nC * NFLD * sizeof(int), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(limiterFace, a.fst, nC * NFLD * sizeof(int), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    (void)nF;
+    return takeDeviceErrors(c->st, {c->derr});
+}
+
 int icsf_residual_device(icsf_case* c, const double* W, double* b) {
     if (!c || !b) return fail(ICSF_ERR_ARG, "icsf_residual_device: NULL argument");
     if (!W && !c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");

@@ -146,6 +146,8 @@ struct AdjView {
     double* fext;   // [nC][NFLD] adjoint of that face's extrapolation
     double* qf;     // [nF][NFLD] adjoint of the Gauss face values (internal faces)
     double* bprim;  // [nB][5] boundary adjoint forwarded to the owner's p, T, U
+    int* fbr;       // [nF] AUSM+up branch bits of each face (records for derivative
+                    //   checks; null: not written)
 };
 
 // Workgroups of a launch are dispatched round-robin over the 8 XCDs, each

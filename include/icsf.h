@@ -239,6 +239,19 @@ int icsf_residual_vjp(icsf_case* c, const double* W, const double* v, double* gW
 /* Same with device pointers (for framework bindings); synchronous. */
 int icsf_residual_vjp_device(icsf_case* c, const double* W, const double* v, double* gW, double* gSf,
                              double* gCf);
+/* Records of the points where the residual of icsf_residual_vjp is not
+ * differentiable (no counterpart in the reference; for derivative checks):
+ * W (device, [5][nCells], or NULL) becomes the state as in the VJP;
+ * faceBranches [nFaces] gets each face's AUSM+up switches as bits (c*
+ * against the normal velocity on either side, the smaller c, subsonic or
+ * supersonic split Mach and its sign on either side, the pressure-diffusion
+ * clamp, the 0.2 damping, the upwind side), limiterElement [nCells][8] the
+ * cell or boundary face (nCells + b) that is the max/min of each
+ * reconstructed field's limiter (-1: none), limiterFace [nCells][8] the face
+ * that set the limiter (-1: limiter 1).  Two states with equal records lie
+ * in one smooth piece of the residual.  Device pointers; synchronous. */
+int icsf_residual_records_device(icsf_case* c, const double* W, int* faceBranches, int* limiterElement,
+                                 int* limiterFace);
 /* b(W) of icsf_residual with device pointers: W ([5][nCells] or NULL for
  * the current state) becomes the case state, b receives R*V.  Synchronous. */
 int icsf_residual_device(icsf_case* c, const double* W, double* b);
