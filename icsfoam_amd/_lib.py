@@ -76,6 +76,8 @@ SIGNATURES = {
     "icsf_hb_destroy": (_i, [_vp]),
     "icsf_hb_set_boundary": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp]),
     "icsf_hb_set_boundary_values": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
+    "icsf_hb_reconstruct_weights": (_i, [_i, _vp, _vp, _i, _vp, C.c_double, _vp]),
+    "icsf_hb_reconstruct": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_hb_set_fields": (_i, [_vp, _i, _vp, _vp, _vp]),
     "icsf_hb_get_fields": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
     "icsf_hb_residual": (_i, [_vp, _vp]),

@@ -153,12 +153,14 @@ std::vector<double> hbInstants(int nInstants, double selectedPeriod, const std::
     return sel;
 }
 
-bool hbOperator(const std::vector<double>& omegas, const std::vector<double>& t, std::vector<double>& D,
-                std::string& err) {
+namespace {
+// E = pinv(E^-1) [nT][nO] (HBZone.C:292): the inverse when square, else
+// ((E^-1)^H E^-1)^-1 (E^-1)^H
+bool hbPinv(const std::vector<double>& omegas, const std::vector<double>& t, const std::vector<cplx>& Einv,
+            std::vector<cplx>& E, std::string& err) {
     const size_t nO = t.size(), nT = omegas.size();
     if (nO < nT) { err = "HB: fewer instants than frequencies"; return false; }
-    const std::vector<cplx> Einv = synthesis(omegas, t);  // [nO][nT]
-    std::vector<cplx> E(nT * nO);                          // pinv, [nT][nO]
+    E.assign(nT * nO, cplx(0, 0));
     if (nO == nT) {
         std::vector<cplx> A = Einv;
         if (!invert(A, nT)) { err = "HB: E^-1 is singular for these instants"; return false; }
@@ -176,6 +178,16 @@ bool hbOperator(const std::vector<double>& omegas, const std::vector<double>& t,
                 E[l * nO + n] = s;
             }
     }
+    return true;
+}
+}  // namespace
+
+bool hbOperator(const std::vector<double>& omegas, const std::vector<double>& t, std::vector<double>& D,
+                std::string& err) {
+    const size_t nO = t.size(), nT = omegas.size();
+    const std::vector<cplx> Einv = synthesis(omegas, t);  // [nO][nT]
+    std::vector<cplx> E;                                   // pinv, [nT][nO]
+    if (!hbPinv(omegas, t, Einv, E, err)) return false;
     // temp_0 = Im(E^-1) A, temp_1 = Re(E^-1) A ; d_0 = temp_0 Re(E), d_1 = temp_1 Im(E)
     std::vector<double> t0(nO * nT, 0.0), t1(nO * nT, 0.0), d0(nO * nO, 0.0), d1(nO * nO, 0.0);
     for (size_t l = 0; l < nO; ++l)
@@ -193,6 +205,22 @@ bool hbOperator(const std::vector<double>& omegas, const std::vector<double>& t,
             }
     D.assign(nO * nO, 0.0);
     for (size_t i = 0; i < nO * nO; ++i) D[i] = -(d0[i] + d1[i]);
+    return true;
+}
+
+bool hbReconstructWeights(const std::vector<double>& omegas, const std::vector<double>& t, double time,
+                          std::vector<double>& w, std::string& err) {
+    const size_t nO = t.size(), nT = omegas.size();
+    std::vector<cplx> E;
+    if (!hbPinv(omegas, t, synthesis(omegas, t), E, err)) return false;
+    std::vector<cplx> e1(nT);
+    for (size_t k = 0; k < nT; ++k) e1[k] = cplx(std::cos(omegas[k] * time), std::sin(omegas[k] * time));
+    w.assign(nO, 0.0);
+    for (size_t m = 0; m < nO; ++m) {
+        cplx s(0, 0);
+        for (size_t n = 0; n < nT; ++n) s += e1[n] * E[n * nO + m];
+        w[m] = s.real();
+    }
     return true;
 }
 

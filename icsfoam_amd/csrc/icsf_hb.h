@@ -35,4 +35,10 @@ std::vector<double> hbInstants(int nInstants, double selectedPeriod, const std::
 bool hbOperator(const std::vector<double>& omegas, const std::vector<double>& t, std::vector<double>& D,
                 std::string& err);
 
+// Weights of the Fourier reconstruction at `time` (HBZone::reconstruct,
+// HBZoneTemplates.C:328-365): w[m] = Re(sum_n exp(i omega_n time) E[n][m]),
+// E = pinv(E^-1); a field at that time is sum_m w[m] field_m.
+bool hbReconstructWeights(const std::vector<double>& omegas, const std::vector<double>& t, double time,
+                          std::vector<double>& w, std::string& err);
+
 }  // namespace icsf

@@ -77,6 +77,8 @@ struct HBInst {
 // cellSums: as launchResidual / launchCellSumsJacobian, for all instances
 void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st,
                       bool cellSums = true);
+// Fourier reconstruction of the instances' fields at one time, out [nC][6]
+void launchHBReconstruct(int nC, int nO, const HBInst* inst, const double* w, double* out, hipStream_t st);
 void launchJacobianHB(const HBInst* inst, int nO, int fp32, const MeshView& m, const Phys& ph, hipStream_t st,
                       bool cellSums = false);
 

@@ -909,7 +909,39 @@ __global__ void __launch_bounds__(256) k_cell_jacobian_hb(const HBInst* __restri
 
 }  // namespace
 
+// Fourier reconstruction of the instances at one time (HBZone::reconstruct,
+// HBZoneTemplates.C:328-386): each primitive field of a cell is
+// sum_l w_l field_l, summed from zero in instance order.  out [nC][6]:
+// rho, Ux, Uy, Uz, T, p.  (For a static mesh the reference's cylindrical
+// variant for U gives the same field: the rotation to cylindrical
+// components is the same for every instance and commutes with the sum.)
+namespace {
+__global__ void k_hb_reconstruct(int nC, int nO, const HBInst* __restrict__ I, const double* __restrict__ w,
+                                 double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    double r = 0, ux = 0, uy = 0, uz = 0, T = 0, p = 0;
+    for (int l = 0; l < nO; ++l) {
+        const double* f = I[l].s.fld + (size_t)c * NFLD;
+        const double wl = w[l];
+        r += wl * f[F_RHO];
+        ux += wl * f[F_UX];
+        uy += wl * f[F_UY];
+        uz += wl * f[F_UZ];
+        T += wl * I[l].s.T[c];
+        p += wl * f[F_P];
+    }
+    double* o = out + 6 * (size_t)c;
+    o[0] = r; o[1] = ux; o[2] = uy; o[3] = uz; o[4] = T; o[5] = p;
+}
+}  // namespace
+
 // ------------------------------------------------------------ launchers --
+void launchHBReconstruct(int nC, int nO, const HBInst* inst, const double* w, double* out, hipStream_t st) {
+    k_hb_reconstruct<<<(nC + 255) / 256, 256, 0, st>>>(nC, nO, inst, w, out);
+}
+
+
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
 void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const double* dW, int mode,

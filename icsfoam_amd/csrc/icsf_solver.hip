@@ -2158,6 +2158,47 @@ int icsf_hb_operator(int nFreq, const double* frequencies, const int* harmonics,
     return ICSF_OK;
 }
 
+int icsf_hb_reconstruct_weights(int nFreq, const double* frequencies, const int* harmonics, int nInstants,
+                                const double* instants, double time, double* weights) {
+    if (nFreq < 0 || (nFreq > 0 && (!frequencies || !harmonics)) || nInstants < 1 || !instants || !weights)
+        return fail(ICSF_ERR_ARG, "icsf_hb_reconstruct_weights: bad argument");
+    std::vector<double> f(frequencies, frequencies + nFreq);
+    std::vector<int> hm(harmonics, harmonics + nFreq);
+    for (int k : hm)
+        if (k < 0) return fail(ICSF_ERR_ARG, "icsf_hb_reconstruct_weights: negative harmonic count");
+    const std::vector<double> om = hbOmegaList(f, hm);
+    std::vector<double> w;
+    std::string err;
+    if (!hbReconstructWeights(om, std::vector<double>(instants, instants + nInstants), time, w, err))
+        return fail(ICSF_ERR_ARG, err);
+    std::copy(w.begin(), w.end(), weights);
+    return ICSF_OK;
+}
+
+int icsf_hb_reconstruct(icsf_hb* h, const double* weights, double* rho, double* U, double* T, double* p) {
+    if (!h || !weights) return fail(ICSF_ERR_ARG, "icsf_hb_reconstruct: NULL argument");
+    for (int J = 0; J < h->nO; ++J)
+        if (!std::isfinite(weights[J])) return fail(ICSF_ERR_ARG, "icsf_hb_reconstruct: non-finite weight");
+    const int nC = h->inst[0]->mv.nC;
+    DevBuf tmp;
+    double *dw, *out;
+    RC(tmp.upload(dw, weights, (size_t)h->nO));
+    RC(tmp.alloc(out, (size_t)6 * nC));
+    launchHBReconstruct(nC, h->nO, h->dInst, dw, out, h->st);
+    HIPCHK(hipGetLastError());
+    std::vector<double> o((size_t)6 * nC);
+    HIPCHK(hipMemcpyAsync(o.data(), out, o.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (int c = 0; c < nC; ++c) {
+        const double* q = &o[6 * (size_t)c];
+        if (rho) rho[c] = q[0];
+        if (U) for (int d = 0; d < 3; ++d) U[3 * (size_t)c + d] = q[1 + d];
+        if (T) T[c] = q[4];
+        if (p) p[c] = q[5];
+    }
+    return ICSF_OK;
+}
+
 int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, const char* solution, int nInstants,
                    const double* D, icsf_hb** out) {
     if (!mesh || !D || !out) return fail(ICSF_ERR_ARG, "icsf_hb_create: NULL argument");

@@ -227,6 +227,17 @@ class dbnsCase:
         return ms, cnt
 
 
+def hb_reconstruct_weights(frequencies, harmonics, instants, time: float):
+    """Weights of the Fourier reconstruction of the instants at `time`
+    (HBZone::reconstruct); frequencies angular, as hb_operator."""
+    f = as_f64(frequencies)
+    h = as_i32(harmonics)
+    t = as_f64(instants)
+    w = np.empty(t.size)
+    check(lib().icsf_hb_reconstruct_weights(int(f.size), ptr(f), ptr(h), int(t.size), ptr(t), float(time), ptr(w)))
+    return w
+
+
 def hb_operator(frequencies, harmonics, n_instants: int, selected_period: float = 0.0):
     """(D [nO, nO], instants [nO], condition number) of the harmonic-balance
     spectral operator (HBZone::updateHBOperators / HBZoneList::setInstants).
@@ -287,6 +298,14 @@ class hbCase:
 
     def set_fields(self, J, rho, U, T):
         check(lib().icsf_hb_set_fields(self._h, J, ptr(as_f64(rho)), ptr(as_f64(U)), ptr(as_f64(T))))
+
+    def reconstruct(self, weights):
+        """Fields at one time from the instances (reconstructHBFoam): rho,
+        U, T, p = sum_m w[m] field_m, weights from hb_reconstruct_weights."""
+        n = self.nCells
+        rho, U, T, p = np.empty(n), np.empty((n, 3)), np.empty(n), np.empty(n)
+        check(lib().icsf_hb_reconstruct(self._h, ptr(as_f64(weights)), ptr(rho), ptr(U), ptr(T), ptr(p)))
+        return dict(rho=rho, U=U, T=T, p=p)
 
     def fields(self, J):
         n = self.nCells

@@ -427,6 +427,16 @@ int icsf_hb_operator(int nFreq, const double* frequencies, const int* harmonics,
  * LUSGS preconditioner. */
 int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, const char* solution, int nInstants,
                    const double* D, icsf_hb** out);
+/* Post-processing of reconstructHBFoam (applications/solvers/
+ * reconstructHBFoam/reconstructHBFoam.C:79-85, HBZone::reconstruct,
+ * HBZoneTemplates.C:328-386): the weights w[m] = Re(sum_n exp(i omega_n t)
+ * E[n][m]), E = pinv(E^-1) of the instants (HBZone.C:292), of the Fourier
+ * reconstruction at time t (frequencies and harmonics as
+ * icsf_hb_operator); then the fields at t, sum_m w[m] field_m for rho, U,
+ * T, p of every cell ([nCells], U [nCells][3]; any may be NULL). */
+int icsf_hb_reconstruct_weights(int nFreq, const double* frequencies, const int* harmonics, int nInstants,
+                                const double* instants, double time, double* weights);
+int icsf_hb_reconstruct(icsf_hb* h, const double* weights, double* rho, double* U, double* T, double* p);
 int icsf_hb_destroy(icsf_hb* h);
 /* Per-instant boundary conditions, initial boundary values and fields (as
  * icsf_set_boundary / icsf_set_boundary_values / icsf_set_fields /

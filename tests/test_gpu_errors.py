@@ -103,10 +103,21 @@ def test_nonfinite_state_during_pseudo_steps(graph):
     assert np.all(np.isfinite(h))
 
 
+def _fixed_inlet(c):
+    """The harmonic-balance case's inflow patch as a fixed velocity and
+    temperature (so that _overflowing_inlet acts in the first step)."""
+    i = [p.name for p in c.mesh.patches].index("inlet")
+    bcs = list(c.bcs)
+    n = c.mesh.patches[i].size
+    bcs[i] = cases.BoundarySpec(cases.BC_ZERO_GRADIENT, cases.BC_FIXED_VALUE, cases.BC_FIXED_VALUE, None,
+                                np.tile([0.0, 0.0, 170.0], (n, 1)), np.full(n, 300.0))
+    return dataclasses.replace(c, bcs=bcs)
+
+
 def test_nonfinite_state_during_hb_pseudo_steps():
     print("[nan hb]", flush=True)
     inst, D, _ = cases.case_hb(n=(8, 6, 5), n_harmonics=1, noise=0.02, jitter=0.1)
-    inst[1] = _overflowing_inlet(inst[1])
+    inst[1] = _overflowing_inlet(_fixed_inlet(inst[1]))
     h = hbCase.from_cases(inst, D, fvMesh(inst[0].mesh))
     with pytest.raises(IcsfError) as e:
         h.pseudo_steps(2, residuals=False)

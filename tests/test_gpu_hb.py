@@ -185,3 +185,26 @@ def test_hb_mixed_precision_vs_fp64_oracle():
         print(f"instant {J}: mixed vs FP64 oracle after 4 steps {errs}")
         assert max(errs.values()) < 1e-3
     assert np.all(hm[-1][:, :2] < 1.5 * ho[-1][:, :2])
+
+
+def test_fourier_reconstruction_of_the_instances():
+    """reconstructHBFoam on the device: at an instant the fields of that
+    instance, at any time sum_m w_m field_m in instance order (the same
+    sums on the host, bit for bit)."""
+    from icsfoam_amd.solver import hb_reconstruct_weights
+    insts, D, t = cases.case_hb(n=(8, 6, 5), n_harmonics=3, flux="ROE")
+    h = hbCase.from_cases(insts, D)
+    h.pseudo_steps(2)
+    om = [2 * np.pi * 100.0]
+    F = [h.fields(J) for J in range(len(t))]
+    for J in (0, 3):
+        r = h.reconstruct(hb_reconstruct_weights(om, [3], t, t[J]))
+        for k in ("rho", "U", "T", "p"):
+            assert rel_err(r[k], F[J][k]) < 1e-13, (J, k)
+    w = hb_reconstruct_weights(om, [3], t, 0.00377)
+    r = h.reconstruct(w)
+    for k in ("rho", "U", "T", "p"):
+        ref = np.zeros_like(F[0][k])
+        for J in range(len(t)):
+            ref = ref + w[J] * F[J][k]
+        np.testing.assert_array_equal(r[k], ref)

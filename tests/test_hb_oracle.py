@@ -96,3 +96,31 @@ def test_coupled_product_cross_terms():
     V = OracleCase(insts[0]).mesh.geometry()["V"]
     cross = np.einsum("jk,c,kqc->jqc", D - np.diag(np.diag(D)), V, x)
     np.testing.assert_allclose(y - y0, cross, rtol=0, atol=1e-12 * np.abs(y).max())
+
+
+# ----------------------------------------------- Fourier reconstruction --
+@pytest.mark.parametrize("freqs,harm,nO,T", [([2 * np.pi * 100.0], [3], 7, 0.01), ([2 * np.pi * 50.0], [2], 5, 0.02),
+                                             ([2 * np.pi * 80.0, 2.7 * 2 * np.pi * 80.0], [1, 1], 5, 0.0)])
+def test_reconstruction_weights_interpolate_exactly(freqs, harm, nO, T):
+    """HBZone::reconstruct's weights w_m(t) (E = pinv(E^-1) of the
+    instants): at an instant t_J they select that instant, they sum to 1,
+    and for any field made of the case's frequencies sum_m w_m(t) f(t_m)
+    reproduces f(t) (trigonometric interpolation is exact on its own
+    basis), also for the scanned, non-equispaced instants of two
+    incommensurate frequencies."""
+    from icsfoam_amd.solver import hb_reconstruct_weights
+    _, t, _ = lib_hb_operator(freqs, harm, nO, T)
+    om = [0.0] + [k * f for f, h in zip(freqs, harm) for k in range(1, h + 1)]
+    for J in range(nO):
+        w = hb_reconstruct_weights(freqs, harm, t, t[J])
+        np.testing.assert_allclose(w, np.eye(nO)[J], rtol=0, atol=1e-12)
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal(len(om))
+    b = rng.standard_normal(len(om))
+
+    def f(x):
+        return sum(a[i] * np.cos(om[i] * x) + b[i] * np.sin(om[i] * x) for i in range(len(om)))
+    for tt in rng.uniform(-0.05, 0.05, 6):
+        w = hb_reconstruct_weights(freqs, harm, t, tt)
+        assert abs(w.sum() - 1.0) < 1e-12
+        assert abs(w @ f(t) - f(tt)) < 1e-11 * np.abs(a).sum()
