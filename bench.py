@@ -98,13 +98,44 @@ def algorithmic_bytes_fused(nC, nI, p, q, gen=True):
     return (3 * nI * per + nC * 25) * p + 3 * nI * 4 + 30 * q * nC + 12 * nC
 
 
+def residual_kernel_bytes(nC, nI, nB, p=8, ng=7):
+    """Algorithmic bytes per launch of the residual and Jacobian kernels of
+    the viscous second-order workload (each array element the kernel needs
+    counted once: a neighbour's row gathered by several threads is one read
+    of that row; face records are read by both cells of their face):
+
+    k_grad<ng>: per cell the field row 64, e, V 8 each, C 24, row length,
+    lower count and offset 12 read, the limited gradients 24 ng and the
+    viscous gradient row 96 written; per slot (2 nI) column 4 and slot
+    geometry Sf, w, Cf 56; per boundary face 128 (face index, Sf, Cf, kind,
+    boundary row, T).
+    k_face_terms: per internal face owner, neighbour 8, Sf, magSf, Cf, w,
+    nonOrthDeltaCoeffs, correction vector, deltaCoeffs 112 read and the
+    192 B record written; per cell its rows gathered once: fields 64,
+    gradients 192, viscous gradients 96, C 24, T 8, e 8; per boundary face
+    356 (geometry, kind and condition types, boundary row, T, record,
+    boundary mass flux).
+    k_cell_jacobian<p, cell sums>: per slot the record 192, the slot's Sf
+    and magSf 32, the face index 4 read and 10 generators of p bytes
+    written; per cell offsets, lower count, V, pseudoCo 24 read and the
+    diagonal block 25 p, rD, b (5), rDT 56 written; per boundary face 390
+    (record, geometry, kinds, boundary row and T, the owner's row and T).
+    k_cell_residual: per slot the record and the face index 196; per cell
+    24 read, 48 written; per boundary face 196."""
+    return {
+        "k_grad": nC * (64 + 8 + 8 + 24 + 12 + 24 * ng + 96) + 2 * nI * (4 + 56) + nB * 128,
+        "k_face_terms": nI * (112 + 192) + nC * (64 + 192 + 96 + 24 + 8 + 8) + nB * 356,
+        "k_cell_jacobian": 2 * nI * (192 + 32 + 4 + NGEN * p) + nC * (24 + 25 * p + 56) + nB * 390,
+        "k_cell_residual": 2 * nI * 196 + nC * (24 + 48) + nB * 196,
+    }
+
+
 def step_bytes(nC, nI, nB, form, m=30):
     """Algorithmic HBM bytes of one pseudo-step (one GMRES/FGMRES cycle of
-    m Arnoldi steps), summed over the kernels' per-launch figures:
-    gradients nC (8 + 24) 8 + 112 nI; face terms nF (56 + 128 + 384 + 192)
-    + nC 104 (viscous Roe); residual cell sums and Jacobian rows in one
-    pass over the face records 2 nI (10 p + 192) + 25 p nC (generator
-    storage); per Arnoldi step the fused operator group
+    m Arnoldi steps), summed over the kernels' per-launch figures: the
+    residual and Jacobian kernels as in residual_kernel_bytes (gradients,
+    face terms, cell sums and Jacobian rows in one pass over the face
+    records, generator storage); per Arnoldi step the fused operator group
     and the CGS projection (w and j + 1 basis vectors read, then w, j + 1
     basis vectors read and w written: 5 q nC (2 j + 5)); per cycle the
     normalisation pass (average, one SpMV, norm factors), the solution
@@ -112,21 +143,29 @@ def step_bytes(nC, nI, nB, form, m=30):
     5 * 8 * 4 nC.  FP64 form: p = q = 8.  Mixed form: the inner GMRES(m) with
     p = q = 4 plus the FP64 outer FGMRES(1) (two SpMVs with FP32 blocks, a
     two-vector update)."""
-    nF = nI + nB
-    res = nC * (8 + 24) * 8 + 112 * nI + nF * (56 + 128 + 384 + 192) + nC * 104
+    kb = residual_kernel_bytes(nC, nI, nB, 8 if form == "fp64" else 4)
+    res = kb["k_grad"] + kb["k_face_terms"]
+    jac = kb["k_cell_jacobian"]
     fused = lambda p, q: algorithmic_bytes_fused(nC, nI, p, q)
     spmv = lambda p, q: (2 * nI * NGEN + nC * 25) * p + 2 * nI * 4 + nC * 5 * q * 2
     orth = lambda q: sum(5 * q * nC * (2 * j + 5) for j in range(m))
     if form == "fp64":
         p = q = 8
-        jac = 2 * nI * (NGEN * p + 192) + 25 * p * nC
         cycle = m * fused(p, q) + orth(q) + 3 * spmv(p, q) + 5 * q * nC * (m + 2)
     else:
-        jac = 2 * nI * (NGEN * 4 + 192) + 25 * 4 * nC
         inner = m * fused(4, 4) + orth(4) + 5 * 4 * nC * (m + 2)
         outer = 3 * spmv(4, 8) + 5 * 8 * nC * 6
         cycle = inner + outer
     return res + jac + cycle + 5 * 8 * 4 * nC
+
+
+def _class_frac(nC, nI, nB, form, r, cls):
+    """HBM fraction of breakdown class 0 (gradients + face terms) or 1
+    (cell sums + Jacobian rows) of one profiled pseudo-step."""
+    kb = residual_kernel_bytes(nC, nI, nB, 8 if form == "fp64" else 4)
+    b = kb["k_grad"] + kb["k_face_terms"] if cls == 0 else kb["k_cell_jacobian"]
+    ms = r["breakdown_ms"][cls] / max(r["breakdown_count"][cls], 1)
+    return b / (ms * 1e-3) / HBM_PEAK if ms > 0 else 0.0
 
 
 def measured_traffic(n, form="fp64"):
@@ -390,12 +429,73 @@ def bench_vjp(calls):
                        "residual_ms": round(1e3 * dr, 3)}}
 
 
+def bench_c5_vjp(n, max_steps, calls=10, n_dirs=32, chunk=25, drop=1e-8, time_limit=300.0):
+    """FP64 residual VJP of the n^3 headline workload (viscous Roe) on its
+    converged state: the mixed form run to a `drop` reduction of ||R V||_2
+    (or the time limit), then ms per VJP call with device-resident W and v,
+    and the finite-difference gate of icsfoam_amd.adjoint.fd_gate over
+    n_dirs directions on that state with 1e-3 per-cell noise."""
+    import torch
+
+    from icsfoam_amd import cases, dbnsCase
+    from icsfoam_amd.adjoint import fd_gate, residual_device, residual_vjp_device
+
+    base = prepared(n)
+    g = dbnsCase.from_case(with_form(base, n, "mixed", "CGS"))
+    b, _ = g.residual()
+    r0 = float(np.sqrt(np.sum(b * b)))
+    steps, rel = 0, 1.0
+    t0 = time.perf_counter()
+    while steps < max_steps and rel >= drop and time.perf_counter() - t0 < time_limit:
+        g.pseudo_steps(chunk, residuals=False)
+        steps += chunk
+        b, _ = g.residual()
+        rel = float(np.sqrt(np.sum(b * b))) / r0
+    t_conv = time.perf_counter() - t0
+    log(f"[bench] {n}^3 mixed: {steps} steps in {t_conv:.1f}s, ||R||/||R0|| = {rel:.3e}")
+    W = torch.tensor(g.fields()["W"], device="cuda")
+    v = torch.randn(W.shape, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+    for _ in range(2):
+        residual_vjp_device(g, W, v, geometry=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        residual_vjp_device(g, W, v, geometry=False)
+    dt = (time.perf_counter() - t0) / calls
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        residual_device(g, W)
+    dr = (time.perf_counter() - t0) / calls
+    log(f"[bench] VJP {1e3 * dt:.3f} ms/call, residual {1e3 * dr:.3f} ms/call; FD gate ...")
+    # the converged state is the uniform stream (limiter ties in every cell,
+    # cross-stream velocity at rounding level): the residual has no
+    # derivative there, so the finite-difference gate runs on it with 1e-3
+    # per-cell noise, as the adjoint tests do
+    rng = np.random.default_rng(11)
+    Wn = W.cpu().numpy().copy()
+    Wn[2:] += 1e-3 * np.abs(Wn[2:]).max() * rng.standard_normal(Wn[2:].shape)
+    Wn[:2] *= 1 + 1e-3 * rng.standard_normal(Wn[:2].shape)
+    X = torch.tensor(cases.approx_cell_centres(base.mesh), device="cuda")
+    fdr = fd_gate(g, torch.tensor(Wn, device="cuda"), X, n_dirs=n_dirs)
+    log(f"[bench] FD gate: {fdr}")
+    return {"metric": f"residual VJP calls/s at {n}^3 on the converged state (viscous Roe, FP64)",
+            "value": round(1.0 / dt, 3), "unit": "calls/s", "n_gpus": 1, "steps": calls, "warmup": 2,
+            "ms_per_step": round(1e3 * dt, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic (seeded jittered box, converged by the solver)",
+            "config": {"workload": f"{n}^3 jittered hex box, viscous Roe, linearUpwind + "
+                                   f"cellLimited<Venkatakrishnan>; gW per call",
+                       "converge_steps": steps, "converge_seconds": round(t_conv, 1), "converged_drop": rel,
+                       "residual_ms": round(1e3 * dr, 3),
+                       "fd_gate_on_noised_state": {k: (v if k != "unexplained" else len(v))
+                                                   for k, v in fdr.items()}}}
+
+
 def bench_converge(n, max_steps, chunk=25, drop=1e-8, time_limit=400.0, forms=("fp64", "mixed")):
     """Time to a `drop` reduction of ||R V||_2 (all five components, the
     norm dbnsFoam.C:136-137 prints for rho) for the FP64 and the mixed form
     of the n^3 workload."""
     base = prepared(n)
-    out = {}
+    out, Ws = {}, {}
     for form in forms:
         from icsfoam_amd import dbnsCase
 
@@ -417,7 +517,18 @@ def bench_converge(n, max_steps, chunk=25, drop=1e-8, time_limit=400.0, forms=("
                 break
         out[form] = dict(steps=steps, seconds=round(time.perf_counter() - t0, 3), drop=hist[-1][2],
                          history=hist[::max(1, len(hist) // 20)])
+        Ws[form] = g.fields()["W"]
         del g
+    if len(Ws) == 2:
+        # converged states of the two forms: relative difference per conserved
+        # component, max-norm and 2-norm, each against the component's own norm
+        (fa, Wa), (fb, Wb) = Ws.items()
+        d = Wa - Wb
+        out[f"W_rel_diff_{fa}_vs_{fb}"] = {
+            name: {"max": float(np.abs(d[k]).max() / np.abs(Wb[k]).max()),
+                   "l2": float(np.linalg.norm(d[k]) / np.linalg.norm(Wb[k]))}
+            for k, name in enumerate(("rho", "rhoUx", "rhoUy", "rhoUz", "rhoE"))}
+        log(f"[bench] converged W, {fa} vs {fb}: {out[f'W_rel_diff_{fa}_vs_{fb}']}")
     head = out[forms[-1]]
     return {"metric": f"time to a {drop:g} drop of ||R V||_2 at {n}^3 ({' vs '.join(forms)})",
             "value": head["seconds"], "unit": "s", "n_gpus": 1, "steps": head["steps"],
@@ -484,10 +595,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed", help="converge: forms to run, in order")
     ap.add_argument("--time-limit", type=float, default=400.0, help="converge: seconds per form")
-    ap.add_argument("--workload", default="c5", choices=["c5", "hb", "vjp", "converge", "bump", "hbconverge"],
+    ap.add_argument("--workload", default="c5",
+                    choices=["c5", "hb", "vjp", "c5vjp", "converge", "bump", "hbconverge"],
                     help="c5: the headline 160^3 pseudo-step (default); hb: 7-instant harmonic balance; "
                          "vjp: residual adjoint at 100^3; bump: the 200x50 bump, 50 steps, GPU and oracle; "
-                         "hbconverge: HB time to a 1e-8 drop")
+                         "hbconverge: HB time to a 1e-8 drop; c5vjp: FP64 VJP of the n^3 workload on its "
+                         "converged state, with the finite-difference gate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the pseudo-step on exactly one GPU; --gpus {args.gpus} is not "
@@ -504,6 +617,9 @@ def main():
         return
     if args.workload == "vjp":
         print(json.dumps(bench_vjp(max(args.steps, 10))), flush=True)
+        return
+    if args.workload == "c5vjp":
+        print(json.dumps(bench_c5_vjp(args.n, 8000, time_limit=args.time_limit)), flush=True)
         return
     if args.workload == "converge":
         print(json.dumps(bench_converge(args.n, max(args.steps, 25), time_limit=args.time_limit,
@@ -559,7 +675,14 @@ def main():
                    "step_hbm_frac": {f: round(step_bytes(nC, nI, nB, f) * r["steps_per_s"] / HBM_PEAK, 4)
                                      for f, r in (("fp64", r64), ("mixed", rmx))},
                    "step_algorithmic_gb": {f: round(step_bytes(nC, nI, nB, f) / 1e9, 2)
-                                           for f in ("fp64", "mixed")}},
+                                           for f in ("fp64", "mixed")},
+                   # residual (gradients + face terms) and Jacobian (cell sums +
+                   # rows) classes of the profiled step: algorithmic bytes of
+                   # their kernels / class time, as a fraction of the HBM peak
+                   "residual_jacobian_hbm_frac": {
+                       f: {"residual": round(_class_frac(nC, nI, nB, f, r, 0), 4),
+                           "jacobian": round(_class_frac(nC, nI, nB, f, r, 1), 4)}
+                       for f, r in (("fp64", r64), ("mixed", rmx))}},
         "roofline": {"bound": "hbm",
                      "kernel": ("fused Arnoldi operator, FP64 form, off-diagonal blocks as generators: "
                                 "k_spmv_cells<double,double,false,true> + k_spmv_cells<double,double,true,true> "

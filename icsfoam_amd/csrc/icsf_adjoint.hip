@@ -23,6 +23,7 @@
 // Branches follow the primal value: max(a, b) takes b only if b > a,
 // min(a, b) takes b only if b < a, and the limiter's min over faces keeps
 // the first face that attains it.
+#include "icsf_dual.h"
 #include "icsf_kernels.h"
 #include "icsf_recon.h"
 #include "icsf_views.h"
@@ -33,8 +34,9 @@ namespace {
 
 #define DF __device__ __forceinline__
 
-// Fields differentiated for AUSM+up (E is not reconstructed by the scheme).
-DF bool used(int k) { return k != F_E; }
+// Reconstructed fields the scheme uses: AUSM+up takes c* and not E, Roe
+// takes E and not c*.
+DF bool used(int flux, int k) { return flux == FLUX_ROE ? k != F_CS : k != F_E; }
 
 // Reverse of ausmFace(): adjoints of the owner-side (L) and neighbour-side
 // (R) face values, indexed by field, and of Sf.  Recomputes the forward
@@ -179,8 +181,41 @@ DF void ausmFaceAdj(bool lowMach, V3 S, double magS, const double* L, const doub
     bL[F_H] = HL_b; bR[F_H] = HR_b;
 }
 
+// Reverse of roeFace(): adjoints of the owner-side (L) and neighbour-side
+// (R) face values rho, p, U, E, H and of Sf, by one forward dual pass per
+// input (inputs 0-6: L fields 0-6, 7-13: R fields 0-6, 14-16: Sf, with
+// |Sf| following Sf), each contracted with the flux adjoint.
+DF void roeFaceAdj(double gI, double eFix, V3 S, double magS, const double* L, const double* R, double bphi,
+                   V3 bphiU, double bphiE, double* bL, double* bR, V3& bS, int* branches) {
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k) bL[k] = bR[k] = 0.0;
+    bS = V3{0, 0, 0};
+#pragma unroll 1
+    for (int j = 0; j < 17; ++j) {
+        const auto sl = [&](int k) { return Dd{L[k], j == k ? 1.0 : 0.0}; };
+        const auto sr = [&](int k) { return Dd{R[k], j == 7 + k ? 1.0 : 0.0}; };
+        const V3D Sd{Dd{S.x, j == 14 ? 1.0 : 0.0}, Dd{S.y, j == 15 ? 1.0 : 0.0}, Dd{S.z, j == 16 ? 1.0 : 0.0}};
+        const Dd ms{magS, j == 14 ? S.x / magS : (j == 15 ? S.y / magS : (j == 16 ? S.z / magS : 0.0))};
+        Dd ph, pe;
+        V3D pu;
+        roeFaceD(gI, eFix, Sd, ms, sl(F_RHO), sr(F_RHO), sl(F_P), sr(F_P), V3D{sl(F_UX), sl(F_UY), sl(F_UZ)},
+                 V3D{sr(F_UX), sr(F_UY), sr(F_UZ)}, sl(F_E), sr(F_E), sl(F_H), sr(F_H), ph, pu, pe,
+                 j == 0 ? branches : nullptr);
+        const double g = bphi * ph.d + (bphiU.x * pu.x.d + bphiU.y * pu.y.d + bphiU.z * pu.z.d) + bphiE * pe.d;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            if (j == k) bL[k] = g;
+            if (j == 7 + k) bR[k] = g;
+        }
+        if (j == 14) bS.x = g;
+        if (j == 15) bS.y = g;
+        if (j == 16) bS.z = g;
+    }
+}
+
 // Face pass: adjoint of the face flux from v (b_owner gets -F, b_neighbour
 // +F), AUSM+up reverse, reconstruction reverse into Cf.
+template <bool ROE>
 __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView s, AdjView a,
                                                   const double* __restrict__ v, double* __restrict__ gS,
                                                   double* __restrict__ gC) {
@@ -205,7 +240,7 @@ __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView
     if (internal) {
 #pragma unroll
         for (int k = 0; k < NFLD; ++k)
-            if (used(k)) recon(m, s, ph.recon, f, o, n, k, L[k], R[k]);
+            if (used(ph.flux, k)) recon(m, s, ph.recon, f, o, n, k, L[k], R[k]);
             else L[k] = R[k] = 0.0;
     } else {
         const double* bv = s.bfld + (size_t)(f - m.nI) * NFLD;
@@ -215,8 +250,23 @@ __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView
     const V3 S = ld3(m.Sf + 3 * (size_t)f);
     double bL[NFLD], bR[NFLD];
     V3 bS;
-    ausmFaceAdj(ph.lowMach, S, m.magSf[f], L, R, bf[0], V3{bf[2], bf[3], bf[4]}, bf[1], bL, bR, bS,
-                a.fbr ? a.fbr + f : nullptr);
+    if constexpr (ROE) {
+        roeFaceAdj(ph.gamma, ph.entropyFix, S, m.magSf[f], L, R, bf[0], V3{bf[2], bf[3], bf[4]}, bf[1], bL, bR, bS,
+                   a.fbr ? a.fbr + f : nullptr);
+        // on a boundary face both sides are the boundary state: the
+        // dissipation multiplies a zero jump, whose derivative is zero on
+        // either side of every switch
+        if (a.fbr && !internal) a.fbr[f] = 0;
+    } else {
+        ausmFaceAdj(ph.lowMach, S, m.magSf[f], L, R, bf[0], V3{bf[2], bf[3], bf[4]}, bf[1], bL, bR, bS,
+                    a.fbr ? a.fbr + f : nullptr);
+        // on a boundary face both sides are the boundary state: the choice
+        // of the smaller c* (bit 2) and of the upwind side (bit 9) selects
+        // between equal functions of the state, so it is no switch of the
+        // derivative (on slip walls both follow the sign of a rounding-level
+        // U.n)
+        if (a.fbr && !internal) a.fbr[f] &= ~((1 << 2) | (1 << 9));
+    }
     V3 bC{0, 0, 0};
     if (internal) {
 #pragma unroll
@@ -224,7 +274,7 @@ __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView
         if (ph.recon == 1) {
 #pragma unroll
             for (int k = 0; k < NFLD; ++k) {
-                if (!used(k)) continue;
+                if (!used(ph.flux, k)) continue;
                 const V3 go = ld3(s.grd + ((size_t)o * NFLD + k) * 3);
                 const V3 gn = ld3(s.grd + ((size_t)n * NFLD + k) * 3);
                 bC = bC + bL[k] * go + bR[k] * gn;
@@ -238,6 +288,121 @@ __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView
     gC[3 * (size_t)f] = bC.x; gC[3 * (size_t)f + 1] = bC.y; gC[3 * (size_t)f + 2] = bC.z;
 }
 
+// Viscous face pass (viscous cases): adjoint of the face terms of
+// residualsUpdate.H:16-43 - laplacian(muEff, U), div(tauMC),
+// div(sigmaDotU), laplacian(alphaEff, e) - for the seed v, into a.vis.
+// The terms add to b of the owner and subtract from b of the neighbour
+// (the opposite sign of the convective flux).  They are linear in grad(U),
+// U, e and grad(e) except div(sigmaDotU), bilinear in (grad(U), tau) and U;
+// tau = mu dev2(grad(U)^T), dev2 being its own adjoint.
+__global__ void __launch_bounds__(256) k_adj_visc(MeshView m, Phys ph, StateView s, AdjView a,
+                                                  const double* __restrict__ v) {
+    const int f = xcdBlock() * blockDim.x + threadIdx.x;
+    if (f >= m.nF) return;
+    double out[NVIS];
+#pragma unroll
+    for (int i = 0; i < NVIS; ++i) out[i] = 0.0;
+    double* vs = a.vis + (size_t)f * NVIS;
+    if (f >= m.nI && m.bKind[f - m.nI] == PK_EMPTY) {
+        stRow(vs, out);
+        return;
+    }
+    const size_t nC = m.nC;
+    const double mu = ph.mu, alpha = ph.alpha;
+    const V3 S = ld3(m.Sf + 3 * (size_t)f);
+    const double ms = m.magSf[f];
+    const int o = m.own[f];
+    if (f < m.nI) {
+        const int n = m.nei[f];
+        const double w = m.w[f], nodc = m.nodc[f];
+        const V3 k3 = ld3(m.corr + 3 * (size_t)f);
+        const V3 lamU{v[2 * nC + o] - v[2 * nC + n], v[3 * nC + o] - v[3 * nC + n], v[4 * nC + o] - v[4 * nC + n]};
+        const double lamE = v[nC + o] - v[nC + n];
+        double go[NGV], gn[NGV];
+        ldRow(s.gv + (size_t)o * NGV, go);
+        ldRow(s.gv + (size_t)n * NGV, gn);
+        T9 gUo, gUn;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) { gUo.t[i] = go[GV_U + i]; gUn.t[i] = gn[GV_U + i]; }
+        const T9 gUf = w * (gUo - gUn) + gUn;
+        const T9 tauO = mu * dev2(transposeT(gUo)), tauN = mu * dev2(transposeT(gUn));
+        const T9 tauf = w * (tauO - tauN) + tauN;
+        const V3 Uo = cellU(s.fld, o), Un = cellU(s.fld, n);
+        const V3 Uf = w * (Uo - Un) + Un;
+        const double muf = w * (mu - mu) + mu, alf = w * (alpha - alpha) + alpha;
+        // (muf sn) |S|, sn = nodc (Un - Uo) + k3 & gUf
+        const V3 snb = (muf * ms) * lamU;
+        V3 UoB = (-nodc) * snb, UnB = nodc * snb;
+        T9 gUfB = outer(k3, snb);
+        // Sf & tauf
+        T9 taufB = outer(S, lamU);
+        // ((muf gUf + tauf) & Uf) & Sf
+        const T9 M = muf * gUf + tauf;
+        const T9 MB = lamE * outer(S, Uf);
+        const V3 UfB = lamE * vdott(S, M);
+        gUfB = gUfB + muf * MB;
+        taufB = taufB + MB;
+        UoB = UoB + w * UfB;
+        UnB = UnB + (1.0 - w) * UfB;
+        // tauf interpolates mu dev2(grad(U)^T) with gUf's weights: one
+        // adjoint Z of the interpolated gradient, split w / (1 - w)
+        const T9 Z = gUfB + transposeT(dev2(mu * taufB));
+        // (alf sne) |S|, sne = nodc (en - eo) + k3 . gef
+        const double sneB = lamE * alf * ms;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[VS_Z + i] = Z.t[i];
+        out[VS_GE] = sneB * k3.x; out[VS_GE + 1] = sneB * k3.y; out[VS_GE + 2] = sneB * k3.z;
+        out[VS_UO] = UoB.x; out[VS_UO + 1] = UoB.y; out[VS_UO + 2] = UoB.z;
+        out[VS_UN] = UnB.x; out[VS_UN + 1] = UnB.y; out[VS_UN + 2] = UnB.z;
+        out[VS_EO] = -nodc * sneB;
+        out[VS_EN] = nodc * sneB;
+    } else {
+        const int b = f - m.nI;
+        const V3 lamU{v[2 * nC + o], v[3 * nC + o], v[4 * nC + o]};
+        const double lamE = v[nC + o];
+        const V3 nn = S / ms;
+        const double dc = m.dc[f];
+        const double* bf = s.bfld + (size_t)b * NFLD;
+        const V3 UB{bf[F_UX], bf[F_UY], bf[F_UZ]};
+        const V3 Uc = cellU(s.fld, o);
+        double g[NGV];
+        ldRow(s.gv + (size_t)o * NGV, g);
+        T9 gUc;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) gUc.t[i] = g[GV_U + i];
+        const int ut = m.bUT[b];
+        V3 snU;
+        if (ut == BC_ZG) snU = V3{0, 0, 0};
+        else if (ut == BC_SLIP) snU = (reflect(nn, Uc) - Uc) * (dc / 2.0);
+        else snU = dc * (UB - Uc);
+        const T9 gUb = gUc + outer(nn, snU - vdott(nn, gUc));
+        const T9 taub = mu * dev2(transposeT(gUb));
+        // (mu snU) |S|
+        V3 snUB = (mu * ms) * lamU;
+        // Sf & taub and ((mu gUb + taub) & UB) & Sf
+        const T9 M = mu * gUb + taub;
+        const T9 MB = lamE * outer(S, UB);
+        const V3 UBB = lamE * vdott(S, M);
+        const T9 gUbB = mu * MB + transposeT(dev2(mu * (outer(S, lamU) + MB)));
+        // gUb = gUc + n (snU - n & gUc)
+        const V3 XB = vdott(nn, gUbB);
+        snUB = snUB + XB;
+        const T9 gUcB = gUbB - outer(nn, XB);
+        V3 UcB{0, 0, 0};
+        if (ut == BC_SLIP) UcB = (reflect(nn, snUB) - snUB) * (dc / 2.0);
+        else if (ut != BC_ZG) UcB = (-dc) * snUB;  // fixedValue U: U_b is constant
+        // (alpha sne) |S|, sne = dc (e_b - e_P)
+        const double sneB = lamE * alpha * ms;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[VS_Z + i] = gUcB.t[i];
+        out[VS_UO] = UcB.x; out[VS_UO + 1] = UcB.y; out[VS_UO + 2] = UcB.z;
+        out[VS_UN] = UBB.x; out[VS_UN + 1] = UBB.y; out[VS_UN + 2] = UBB.z;
+        out[VS_EO] = -dc * sneB;
+        out[VS_EN] = dc * sneB;
+    }
+    stRow(vs, out);
+}
+
 // Cell pass: the adjoint of the limited gradients and of the cell values
 // (linearUpwind), then the limiter reverse (recomputing each field's
 // max/min neighbour and the face that set its limiter), then the
@@ -245,6 +410,7 @@ __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView
 // so the cell's faces are walked twice (once for the face adjoints and the
 // max/min, once for the limiter faces) whatever the number of fields; per
 // field the operations and their order are those of the forward kernels.
+template <bool VISC>
 __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView s, AdjView a) {
     const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
@@ -253,6 +419,9 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
     const double Vc = m.V[c];
     const bool rec = ph.recon == 1;
     const bool limit = rec && ph.limiter != 0;
+    // records pass (a.fbr set): the limiter's active face and max/min
+    // element are found for every field, whatever the adjoint seed
+    const bool records = a.fbr != nullptr;
     double qd[NFLD], mx[NFLD], mn[NFLD];
     V3 gl[NFLD];
     int imx[NFLD], imn[NFLD];
@@ -263,6 +432,13 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
         mx[k] = mn[k] = s.fld[(size_t)c * NFLD + k];
         imx[k] = imn[k] = -1;
     }
+    // viscous: adjoint of the cell's Gauss grad(U) (zu, row-major) and
+    // grad(e) sums from the viscous face pass
+    constexpr bool visc = VISC;
+    double zu[9];
+    V3 ges{0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 9; ++i) zu[i] = 0.0;
     // internal faces first, in face order = the cell's slot order (column,
     // orientation and Cf from the slots), then the boundary faces
     const int len = m.rowLen[c], nLo = m.nLower[c];
@@ -276,12 +452,19 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
                               : ld3(m.Cf + 3 * (size_t)f)) - Cc;
         const double* vo;
         int id;
+        if (visc) {
+            const double* vs = a.vis + (size_t)f * NVIS;
+            const double wt = f < m.nI ? (j >= nLo ? m.w[f] : 1.0 - m.w[f]) : 1.0;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) zu[i] += wt * vs[VS_Z + i];
+            ges = ges + wt * V3{vs[VS_GE], vs[VS_GE + 1], vs[VS_GE + 2]};
+        }
         if (f < m.nI) {
             const bool isOwn = j >= nLo;
             const double* lr = a.fLR + (size_t)f * 2 * NFLD + (isOwn ? 0 : NFLD);
 #pragma unroll
             for (int k = 0; k < NFLD; ++k) {
-                if (!used(k)) continue;
+                if (!used(ph.flux, k)) continue;
                 const double av = lr[k];
                 qd[k] += av;
                 if (rec) gl[k] = gl[k] + av * d;
@@ -296,10 +479,18 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
         if (limit) {
 #pragma unroll
             for (int k = 0; k < NFLD; ++k) {
-                if (!used(k)) continue;
+                if (!used(ph.flux, k)) continue;
                 const double v = vo[k];
-                if (mx[k] < v) { mx[k] = v; imx[k] = id; }
-                if (v < mn[k]) { mn[k] = v; imn[k] = id; }
+                // records pass: a candidate within rounding of the current
+                // max/min does not displace it.  Such ties are two equal
+                // functions of the state computed two ways (H of a cell and
+                // of its zeroGradient outlet face: E + p/rho with p/rho = R T
+                // on both sides), whose order follows the rounding, so that
+                // any step flips them while the derivative does not change.
+                const double tx = records ? 1e-12 * fabs(mx[k]) : 0.0;
+                const double tn = records ? 1e-12 * fabs(mn[k]) : 0.0;
+                if (mx[k] + tx < v) { mx[k] = v; imx[k] = id; }
+                if (v < mn[k] - tn) { mn[k] = v; imn[k] = id; }
             }
         }
     }
@@ -311,11 +502,11 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
         const size_t ck = (size_t)c * NFLD + k;
         g[k] = gb[k] = V3{0, 0, 0};
         limb[k] = 0.0;
-        if (!used(k) || !rec) continue;
+        if (!used(ph.flux, k) || !rec) continue;
         g[k] = ld3(a.gun + 3 * ck);
         gb[k] = a.lim[ck] * gl[k];
         limb[k] = dot(gl[k], g[k]);
-        if (limit && limb[k] != 0.0) any = true;
+        if (limit && (limb[k] != 0.0 || records)) any = true;
     }
     int fs[NFLD], br[NFLD];
     double lm[NFLD], rs[NFLD], es[NFLD], mxD[NFLD], mnD[NFLD];
@@ -332,7 +523,7 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
             const V3 d = ld3(m.Cf + 3 * (size_t)f) - Cc;
 #pragma unroll
             for (int k = 0; k < NFLD; ++k) {
-                if (!used(k) || limb[k] == 0.0) continue;
+                if (!used(ph.flux, k) || (limb[k] == 0.0 && !records)) continue;
                 const double ext = dot(d, g[k]);
                 double r = 1;
                 int bq = 0;
@@ -351,9 +542,13 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
         int tgt = -1, fsk = -1;
         double tv = 0.0, fx = 0.0, qk = qd[k];
         V3 G{0, 0, 0};
-        if (used(k) && rec) {
+        // viscous: grad(U_j) is the unlimited Gauss gradient of field U_j
+        const bool vk = visc && k >= F_UX && k <= F_UZ;
+        if (vk && !rec) G = V3{zu[k - F_UX], zu[3 + k - F_UX], zu[6 + k - F_UX]} / Vc;
+        if (used(ph.flux, k) && rec) {
             V3 gbk = gb[k];
-            if (limit && limb[k] != 0.0 && fs[k] >= 0 && br[k] != 0) {
+            if (vk) gbk = gbk + V3{zu[k - F_UX], zu[3 + k - F_UX], zu[6 + k - F_UX]};
+            if (limit && (limb[k] != 0.0 || records) && fs[k] >= 0 && br[k] != 0) {
                 double dl;
                 if (ph.limiter == 1) {
                     const double den = sqr(rs[k]) + rs[k] + 2;
@@ -373,12 +568,16 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
             }
             G = gbk / Vc;
         }
-        a.qc[ck] = used(k) ? qk : 0.0;
+        a.qc[ck] = used(ph.flux, k) ? qk : 0.0;
         a.tgt[ck] = tgt;
         a.tval[ck] = tv;
         a.fst[ck] = fsk;
         a.fext[ck] = fx;
         a.G[3 * ck] = G.x; a.G[3 * ck + 1] = G.y; a.G[3 * ck + 2] = G.z;
+    }
+    if (visc) {
+        const V3 Ge = ges / Vc;
+        a.Ge[3 * (size_t)c] = Ge.x; a.Ge[3 * (size_t)c + 1] = Ge.y; a.Ge[3 * (size_t)c + 2] = Ge.z;
     }
 }
 
@@ -387,6 +586,7 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
 // (flux + Gauss + limiter max/min) chained through rho_b = p_b/(R T_b),
 // E_b, H_b, c*_b to the owner's p, T, U and to Sf (slip normal), and the
 // limiter's Cf part.
+template <bool VISC>
 __global__ void __launch_bounds__(256) k_adj_gauss(MeshView m, Phys ph, StateView s, AdjView a,
                                                    double* __restrict__ gS, double* __restrict__ gC) {
     const int f = xcdBlock() * blockDim.x + threadIdx.x;
@@ -403,12 +603,17 @@ __global__ void __launch_bounds__(256) k_adj_gauss(MeshView m, Phys ph, StateVie
     const int o = m.own[f];
     V3 bS{0, 0, 0}, bC{0, 0, 0};
     const bool rec = ph.recon == 1;
+    constexpr bool visc = VISC;
+    // fields whose Gauss gradient carries an adjoint: the reconstructed ones
+    // (second order) and, viscous, the velocity components
+    const auto grads = [&](int k) { return (used(ph.flux, k) && rec) || (visc && k >= F_UX && k <= F_UZ); };
     if (f < m.nI) {
         const int n = m.nei[f];
         const double w = m.w[f];
+        if (visc) a.qe[f] = dot(S, ld3(a.Ge + 3 * (size_t)o) - ld3(a.Ge + 3 * (size_t)n));
 #pragma unroll
         for (int k = 0; k < NFLD; ++k) {
-            if (!used(k) || !rec) { qf[k] = 0.0; continue; }
+            if (!grads(k)) { qf[k] = 0.0; continue; }
             const size_t ok = (size_t)o * NFLD + k, nk = (size_t)n * NFLD + k;
             const V3 d = ld3(a.G + 3 * ok) - ld3(a.G + 3 * nk);
             qf[k] = dot(S, d);
@@ -426,9 +631,8 @@ __global__ void __launch_bounds__(256) k_adj_gauss(MeshView m, Phys ph, StateVie
         for (int k = 0; k < NFLD; ++k) {
             qf[k] = 0.0;
             qb[k] = 0.0;
-            if (!used(k)) continue;
-            qb[k] = lr[k];
-            if (rec) {
+            if (used(ph.flux, k)) qb[k] = lr[k];
+            if (grads(k)) {
                 const size_t ok = (size_t)o * NFLD + k;
                 const V3 Go = ld3(a.G + 3 * ok);
                 qb[k] += dot(S, Go);
@@ -453,6 +657,14 @@ __global__ void __launch_bounds__(256) k_adj_gauss(MeshView m, Phys ph, StateVie
         if (!(pr < kSMALL)) { pb += Hb / rhoB; rb -= Hb * pr / rhoB; }
         double Tb = ph.Cv * Eb;
         Ub = Ub + Eb * UB;
+        if (visc) {
+            // the viscous terms' boundary U, and the boundary e of grad(e)
+            // and of laplacian(alphaEff, e): e_b = rhoE_b / rho_b - |U_b|^2 / 2
+            // with rhoE_b = rho_b (Cv T_b + |U_b|^2 / 2), i.e. Cv T_b
+            const double* vs = a.vis + (size_t)f * NVIS;
+            Ub = Ub + V3{vs[VS_UN], vs[VS_UN + 1], vs[VS_UN + 2]};
+            Tb += ph.Cv * (vs[VS_EN] + dot(S, ld3(a.Ge + 3 * (size_t)o)));
+        }
         // rho_b = psi p_b, psi = 1/(R T_b)
         const double psi = 1.0 / (ph.R * TB);
         pb += rb * psi;
@@ -482,6 +694,7 @@ __global__ void __launch_bounds__(256) k_adj_gauss(MeshView m, Phys ph, StateVie
 // Cell pass 2: total cell-value adjoints (direct + Gauss face values +
 // limiter targets of the neighbours + boundary primitives), then
 // c*, H, E -> rho, p, U, T and the updateFields chain back to W.
+template <bool VISC>
 __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateView s, AdjView a,
                                                    double* __restrict__ gW) {
     const int c = xcdBlock() * blockDim.x + threadIdx.x;
@@ -492,6 +705,11 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
     for (int k = 0; k < NFLD; ++k) q[k] = a.qc[(size_t)c * NFLD + k];
     double pbnd = 0.0, Tbnd = 0.0;
     V3 Ubnd{0, 0, 0};
+    // viscous: direct U and e adjoints of the viscous face terms, and the
+    // Gauss face values of e
+    constexpr bool visc = VISC;
+    V3 Uv{0, 0, 0};
+    double ecB = 0.0;
     const int nLo = m.nLower[c];
     const size_t rslot = (size_t)m.slicePtr[c >> 6] * 64 + (c & 63);
     for (int e = e0; e < e1; ++e) {
@@ -506,16 +724,27 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
             const double* qf = a.qf + (size_t)f * NFLD;
 #pragma unroll
             for (int k = 0; k < NFLD; ++k) {
-                if (!used(k)) continue;
+                if (!used(ph.flux, k) && !(visc && k >= F_UX && k <= F_UZ)) continue;
                 q[k] += wt * qf[k];
                 const size_t jk = (size_t)j * NFLD + k;
                 if (a.tgt[jk] == c) q[k] += a.tval[jk];
+            }
+            if (visc) {
+                const double* vs = a.vis + (size_t)f * NVIS;
+                const int u0 = isOwn ? VS_UO : VS_UN;
+                Uv = Uv + V3{vs[u0], vs[u0 + 1], vs[u0 + 2]};
+                ecB += (isOwn ? vs[VS_EO] : vs[VS_EN]) + wt * a.qe[f];
             }
         } else {
             const double* bp = a.bprim + (size_t)(f - m.nI) * 5;
             pbnd += bp[0];
             Tbnd += bp[1];
             Ubnd = Ubnd + V3{bp[2], bp[3], bp[4]};
+            if (visc) {
+                const double* vs = a.vis + (size_t)f * NVIS;
+                Uv = Uv + V3{vs[VS_UO], vs[VS_UO + 1], vs[VS_UO + 2]};
+                ecB += vs[VS_EO];
+            }
         }
     }
     const size_t nC = m.nC;
@@ -528,7 +757,7 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
     double Hb = q[F_H];
     if (cs > 0.0) Hb += q[F_CS] * kap / (2.0 * cs);
     double rb = q[F_RHO], pb = q[F_P] + pbnd, Eb = q[F_E];
-    V3 Ub = V3{q[F_UX], q[F_UY], q[F_UZ]} + Ubnd;
+    V3 Ub = V3{q[F_UX], q[F_UY], q[F_UZ]} + Ubnd + Uv;
     if (!(E < kSMALL)) Eb += Hb;
     const double pr = p / rho;
     if (!(pr < kSMALL)) { pb += Hb / rho; rb -= Hb * pr / rho; }
@@ -548,7 +777,8 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
     const double eU = W1 / r0 - 0.5 * magSqr(U0);
     bool eFree = !(eU < ph.Cv * ph.TMin);
     if (ph.TMax < kGREAT && ph.Cv * ph.TMax < eU) eFree = false;
-    const double eb = eFree ? Tb / ph.Cv : 0.0;
+    // e_calc = rhoE/rho - |U|^2/2 with rhoE rebuilt from the clipped e: e
+    const double eb = eFree ? Tb / ph.Cv + ecB : 0.0;
     double rEb = eb / r0;
     rb -= eb * (W1 / r0) / r0;
     Ub = Ub - eb * U0;
@@ -567,10 +797,19 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
 
 void launchResidualVjp(const MeshView& m, const Phys& ph, const StateView& s, const AdjView& a, const double* v,
                        double* gW, double* gS, double* gC, hipStream_t st) {
-    k_adj_face<<<xcdGrid(m.nF, 256), 256, 0, st>>>(m, ph, s, a, v, gS, gC);
-    k_adj_cell<<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, a);
-    k_adj_gauss<<<xcdGrid(m.nF, 256), 256, 0, st>>>(m, ph, s, a, gS, gC);
-    k_adj_state<<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, a, gW);
+    const dim3 gf(xcdGrid(m.nF, 256)), gc(xcdGrid(m.nC, 256));
+    if (ph.flux == FLUX_ROE) k_adj_face<true><<<gf, 256, 0, st>>>(m, ph, s, a, v, gS, gC);
+    else k_adj_face<false><<<gf, 256, 0, st>>>(m, ph, s, a, v, gS, gC);
+    if (ph.viscous) {
+        k_adj_visc<<<gf, 256, 0, st>>>(m, ph, s, a, v);
+        k_adj_cell<true><<<gc, 256, 0, st>>>(m, ph, s, a);
+        k_adj_gauss<true><<<gf, 256, 0, st>>>(m, ph, s, a, gS, gC);
+        k_adj_state<true><<<gc, 256, 0, st>>>(m, ph, s, a, gW);
+    } else {
+        k_adj_cell<false><<<gc, 256, 0, st>>>(m, ph, s, a);
+        k_adj_gauss<false><<<gf, 256, 0, st>>>(m, ph, s, a, gS, gC);
+        k_adj_state<false><<<gc, 256, 0, st>>>(m, ph, s, a, gW);
+    }
 }
 
 }  // namespace icsf

@@ -41,6 +41,32 @@ __device__ __forceinline__ void recon(const MeshView& m, const StateView& s, int
     R = r;
 }
 
+// recon() of the first nk fields at once from the owner's and the
+// neighbour's field rows (fo, fn), each cell's gradient row gathered whole.
+__device__ __forceinline__ void reconRows(const MeshView& m, const StateView& s, int recon, int nk, int f, int o,
+                                          int n, const double (&fo)[NFLD], const double (&fn)[NFLD],
+                                          double (&L)[NFLD], double (&R)[NFLD]) {
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k) {
+        L[k] = 1.0 * (fo[k] - fn[k]) + fn[k];
+        R[k] = 0.0 * (fo[k] - fn[k]) + fn[k];
+    }
+    if (recon == 1) {
+        const V3 fc = ld3(m.Cf + 3 * (size_t)f);
+        double g[3 * NFLD];
+        ldRow(s.grd + (size_t)o * NFLD * 3, g);
+        const V3 dO = fc - ld3(m.C + 3 * (size_t)o);
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k)
+            if (k < nk) L[k] += dot(dO, V3{g[3 * k], g[3 * k + 1], g[3 * k + 2]});
+        ldRow(s.grd + (size_t)n * NFLD * 3, g);
+        const V3 dN = fc - ld3(m.C + 3 * (size_t)n);
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k)
+            if (k < nk) R[k] += dot(dN, V3{g[3 * k], g[3 * k + 1], g[3 * k + 2]});
+    }
+}
+
 // Far-side value of field k on cyclic face k-th (cyclicFvPatchField::
 // patchNeighbourField as patched by the reference,
 // originalOFFiles/constraintFvPatchFields/cyclic/cyclicFvPatchField.C:127-188):

@@ -221,10 +221,10 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
                                          double* __restrict__ gun, double* __restrict__ limo) {
     const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
-    double gx[NG], gy[NG], gz[NG], mx[NG], mn[NG], vc[NG];
+    double gx[NG], gy[NG], gz[NG], mx[NG], mn[NG], vc[NFLD];
+    ldRow(s.fld + (size_t)c * NFLD, vc);
 #pragma unroll
     for (int k = 0; k < NG; ++k) {
-        vc[k] = s.fld[(size_t)c * NFLD + k];
         gx[k] = gy[k] = gz[k] = 0.0;
         mx[k] = mn[k] = vc[k];
     }
@@ -240,23 +240,26 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
         const size_t sk = rs + 64 * (size_t)j;
         const int other = ldnt(m.col + sk);
         const bool isOwn = j >= nLo;
-        const int o = isOwn ? c : other, n = isOwn ? other : c;
         const V3 S{ldnt(m.sgeo + sgOff(sk, SG_SX)), ldnt(m.sgeo + sgOff(sk, SG_SX + 1)),
                    ldnt(m.sgeo + sgOff(sk, SG_SX + 2))};
         const double w = ldnt(m.sgeo + sgOff(sk, SG_W));
+        // the face's owner and neighbour values are this cell's row and the
+        // other cell's, gathered once
+        double vO[NFLD];
+        ldRow(s.fld + (size_t)other * NFLD, vO);
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
-            const double vo = s.fld[(size_t)o * NFLD + k], vn = s.fld[(size_t)n * NFLD + k];
+            const double vo = isOwn ? vc[k] : vO[k], vn = isOwn ? vO[k] : vc[k];
             const double vf = w * (vo - vn) + vn;
             const V3 t = S * vf;
             if (isOwn) { gx[k] = gx[k] + t.x; gy[k] = gy[k] + t.y; gz[k] = gz[k] + t.z; }
             else { gx[k] = gx[k] - t.x; gy[k] = gy[k] - t.y; gz[k] = gz[k] - t.z; }
-            const double vOther = s.fld[(size_t)other * NFLD + k];
-            mx[k] = fmax(mx[k], vOther);
-            mn[k] = fmin(mn[k], vOther);
+            mx[k] = fmax(mx[k], vO[k]);
+            mn[k] = fmin(mn[k], vO[k]);
         }
         if (visc) {
-            const double eo = s.ecalc[o], en = s.ecalc[n];
+            const double eO = s.ecalc[other];
+            const double eo = isOwn ? ecc : eO, en = isOwn ? eO : ecc;
             const double vf = w * (eo - en) + en;
             const V3 t = S * vf;
             gex = isOwn ? gex + t : gex - t;
@@ -283,16 +286,17 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
             }
         } else {
             const int b = f - m.nI;
+            double bf[NFLD];
+            ldRow(s.bfld + (size_t)b * NFLD, bf);
 #pragma unroll
             for (int k = 0; k < NG; ++k) {
-                const double vb = s.bfld[(size_t)b * NFLD + k];
+                const double vb = bf[k];
                 const V3 t = S * vb;
                 gx[k] = gx[k] + t.x; gy[k] = gy[k] + t.y; gz[k] = gz[k] + t.z;
                 mx[k] = fmax(mx[k], vb);
                 mn[k] = fmin(mn[k], vb);
             }
             if (visc) {
-                const double* bf = s.bfld + (size_t)b * NFLD;
                 const V3 UB = V3{bf[F_UX], bf[F_UY], bf[F_UZ]};
                 const double rhoB = bf[F_RHO];
                 const double eB = ph.Cv * s.bT[b];
@@ -306,18 +310,17 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
 #pragma unroll
     for (int k = 0; k < NG; ++k) { gx[k] = gx[k] / Vc; gy[k] = gy[k] / Vc; gz[k] = gz[k] / Vc; }
     if (gun) {
-        double* gq = gun + (size_t)c * NFLD * 3;
+        double o[3 * NG];
 #pragma unroll
-        for (int k = 0; k < NG; ++k) { gq[3 * k] = gx[k]; gq[3 * k + 1] = gy[k]; gq[3 * k + 2] = gz[k]; }
+        for (int k = 0; k < NG; ++k) { o[3 * k] = gx[k]; o[3 * k + 1] = gy[k]; o[3 * k + 2] = gz[k]; }
+        stRow(gun + (size_t)c * NFLD * 3, o);
     }
     if (visc) {
         // grad(U)_ij = d U_j / d x_i; column j is the gradient of U_j
-        double* g9 = s.gU + (size_t)c * 9;
-        g9[0] = gx[F_UX]; g9[1] = gx[F_UY]; g9[2] = gx[F_UZ];
-        g9[3] = gy[F_UX]; g9[4] = gy[F_UY]; g9[5] = gy[F_UZ];
-        g9[6] = gz[F_UX]; g9[7] = gz[F_UY]; g9[8] = gz[F_UZ];
         gex = gex / Vc;
-        s.ge[3 * (size_t)c] = gex.x; s.ge[3 * (size_t)c + 1] = gex.y; s.ge[3 * (size_t)c + 2] = gex.z;
+        const double o[NGV] = {gx[F_UX], gx[F_UY], gx[F_UZ], gy[F_UX], gy[F_UY], gy[F_UZ],
+                               gz[F_UX], gz[F_UY], gz[F_UZ], gex.x,    gex.y,    gex.z};
+        stRow(s.gv + (size_t)c * NGV, o);
     }
     if (ph.recon == 1 && ph.limiter != 0) {
         double lim[NG];
@@ -341,17 +344,17 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
         }
 #pragma unroll
         for (int k = 0; k < NG; ++k) { gx[k] = gx[k] * lim[k]; gy[k] = gy[k] * lim[k]; gz[k] = gz[k] * lim[k]; }
-        if (limo) {
-#pragma unroll
-            for (int k = 0; k < NG; ++k) limo[(size_t)c * NFLD + k] = lim[k];
-        }
+        if (limo) stRow(limo + (size_t)c * NFLD, lim);
     } else if (limo) {
+        double one[NG];
 #pragma unroll
-        for (int k = 0; k < NG; ++k) limo[(size_t)c * NFLD + k] = 1.0;
+        for (int k = 0; k < NG; ++k) one[k] = 1.0;
+        stRow(limo + (size_t)c * NFLD, one);
     }
-    double* go = s.grd + (size_t)c * NFLD * 3;
+    double o[3 * NG];
 #pragma unroll
-    for (int k = 0; k < NG; ++k) { go[3 * k] = gx[k]; go[3 * k + 1] = gy[k]; go[3 * k + 2] = gz[k]; }
+    for (int k = 0; k < NG; ++k) { o[3 * k] = gx[k]; o[3 * k + 1] = gy[k]; o[3 * k + 2] = gz[k]; }
+    stRow(s.grd + (size_t)c * NFLD * 3, o);
 }
 
 // W: minimum waves per SIMD the register allocation must allow (1: the
@@ -377,14 +380,17 @@ __device__ __forceinline__ double soundSpeed(const Phys& ph, double T) {
 // owner side, then of the neighbour side, for the Jacobian (the same
 // values the flux used; E reconstructed once more for AUSM+up, whose flux
 // uses c* instead).
-__device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, const StateView& s, int f, double& fr,
+// Internal faces take the field rows of their owner o and neighbour n
+// (fo, fn), which the caller gathers once for the flux and its own terms.
+__device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, const StateView& s, int f, int o, int n,
+                                         const double (&fo)[NFLD], const double (&fn)[NFLD], double& fr,
                                          double& fe, V3& fu, double* rec) {
     const V3 S{ldnt(m.Sf + 3 * (size_t)f), ldnt(m.Sf + 3 * (size_t)f + 1), ldnt(m.Sf + 3 * (size_t)f + 2)};
     const double ms = ldnt(m.magSf + f);
     double rL, rR, pL, pR, uxL, uxR, uyL, uyR, uzL, uzR, HL, HR, xL, xR, eL = 0, eR = 0;
     double gI = ph.gamma;  // fvc::interpolate(gamma): exact on internal faces
     if (f >= m.nI && m.bKind[f - m.nI] == PK_CYCLIC) {
-        const int o = m.own[f], kc = m.cycOf[f - m.nI];
+        const int kc = m.cycOf[f - m.nI];
         reconCyclic(m, s, ph.recon, f, o, kc, F_UX, uxL, uxR);
         reconCyclic(m, s, ph.recon, f, o, kc, F_UY, uyL, uyR);
         reconCyclic(m, s, ph.recon, f, o, kc, F_UZ, uzL, uzR);
@@ -397,17 +403,17 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
         else reconCyclic(m, s, ph.recon, f, o, kc, F_E, eL, eR);
         gI = m.w[f] * ph.gamma + (1.0 - m.w[f]) * ph.gamma;
     } else if (f < m.nI) {
-        const int o = ldnt(m.own + f), n = ldnt(m.nei + f);
-        recon(m, s, ph.recon, f, o, n, F_UX, uxL, uxR);
-        recon(m, s, ph.recon, f, o, n, F_UY, uyL, uyR);
-        recon(m, s, ph.recon, f, o, n, F_UZ, uzL, uzR);
-        recon(m, s, ph.recon, f, o, n, F_P, pL, pR);
-        recon(m, s, ph.recon, f, o, n, F_RHO, rL, rR);
-        recon(m, s, ph.recon, f, o, n, F_H, HL, HR);
-        recon(m, s, ph.recon, f, o, n, ph.flux == FLUX_ROE ? F_E : F_CS, xL, xR);
-        if (ph.flux == FLUX_HLLC) recon(m, s, ph.recon, f, o, n, F_E, eL, eR);
-        else if (ph.flux == FLUX_ROE) { eL = xL; eR = xR; }
-        else recon(m, s, ph.recon, f, o, n, F_E, eL, eR);
+        double L[NFLD], R[NFLD];
+        reconRows(m, s, ph.recon, ph.flux == FLUX_ROE ? NFLD - 1 : NFLD, f, o, n, fo, fn, L, R);
+        uxL = L[F_UX]; uxR = R[F_UX];
+        uyL = L[F_UY]; uyR = R[F_UY];
+        uzL = L[F_UZ]; uzR = R[F_UZ];
+        pL = L[F_P]; pR = R[F_P];
+        rL = L[F_RHO]; rR = R[F_RHO];
+        HL = L[F_H]; HR = R[F_H];
+        const int kx = ph.flux == FLUX_ROE ? F_E : F_CS;
+        xL = L[kx]; xR = R[kx];
+        eL = L[F_E]; eR = R[F_E];
     } else {
         const double* bf = s.bfld + (size_t)(f - m.nI) * NFLD;
         rL = rR = bf[F_RHO];
@@ -451,7 +457,14 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
     double fr, fe;
     V3 fu;
     double rec[8];
-    faceFlux(m, ph, s, f, fr, fe, fu, rec);
+    const bool internal = f < m.nI;
+    const int ow = internal ? ldnt(m.own + f) : m.own[f], ne = internal ? ldnt(m.nei + f) : -1;
+    double fo[NFLD], fn[NFLD];
+    if (internal) {
+        ldRow(s.fld + (size_t)ow * NFLD, fo);
+        ldRow(s.fld + (size_t)ne * NFLD, fn);
+    }
+    faceFlux(m, ph, s, f, ow, ne, fo, fn, fr, fe, fu, rec);
     o[FT_CONV] = fr;
     o[FT_CONV + 1] = fe;
     o[FT_CONV + 2] = fu.x;
@@ -461,12 +474,11 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
     const double ms = ldnt(m.magSf + f);
     const bool visc = ph.viscous;
     const double mu = ph.mu, alpha = ph.alpha;
-    if (f < m.nI) {
-        const int ow = ldnt(m.own + f), ne = ldnt(m.nei + f);
+    if (internal) {
         const double w = ldnt(m.w + f);
         const double co = soundSpeed(ph, s.T[ow]), cn = soundSpeed(ph, s.T[ne]);
         const double cf = w * (co - cn) + cn;
-        const V3 Uo = cellU(s.fld, ow), Un = cellU(s.fld, ne);
+        const V3 Uo{fo[F_UX], fo[F_UY], fo[F_UZ]}, Un{fn[F_UX], fn[F_UY], fn[F_UZ]};
         const V3 Uf = w * (Uo - Un) + Un;
         const double lam = cf + fabs(dot(Uf, S / ms));
         o[FT_LAM] = lam;
@@ -474,11 +486,12 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
 #pragma unroll
         for (int k = 0; k < 8; ++k) o[FT_REC + k] = rec[k];
         if (visc) {
-            const double* g9o = s.gU + (size_t)ow * 9;
-            const double* g9n = s.gU + (size_t)ne * 9;
+            double vo[NGV], vn[NGV];
+            ldRow(s.gv + (size_t)ow * NGV, vo);
+            ldRow(s.gv + (size_t)ne * NGV, vn);
             T9 gUo, gUn;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) { gUo.t[i] = g9o[i]; gUn.t[i] = g9n[i]; }
+            for (int i = 0; i < 9; ++i) { gUo.t[i] = vo[GV_U + i]; gUn.t[i] = vn[GV_U + i]; }
             const T9 gUf = w * (gUo - gUn) + gUn;
             const T9 tauO = mu * dev2(transposeT(gUo));
             const T9 tauN = mu * dev2(transposeT(gUn));
@@ -490,15 +503,14 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
             const V3 lap = (muf * sn) * ms;
             const V3 dt = vdott(S, tauf);
             const V3 sig = tdotv(muf * gUf + tauf, Uf);
-            const V3 geO = ld3(s.ge + 3 * (size_t)ow), geN = ld3(s.ge + 3 * (size_t)ne);
+            const V3 geO{vo[GV_E], vo[GV_E + 1], vo[GV_E + 2]}, geN{vn[GV_E], vn[GV_E + 1], vn[GV_E + 2]};
             const V3 gef = w * (geO - geN) + geN;
             const double sne = m.nodc[f] * (s.ecalc[ne] - s.ecalc[ow]) + dot(k3, gef);
             o[FT_LAP] = lap.x; o[FT_LAP + 1] = lap.y; o[FT_LAP + 2] = lap.z;
             o[FT_TAU] = dt.x; o[FT_TAU + 1] = dt.y; o[FT_TAU + 2] = dt.z;
             o[FT_SIG] = dot(sig, S);
             o[FT_LAPE] = (alf * sne) * ms;
-            const double rf = w * (s.fld[(size_t)ow * NFLD + F_RHO] - s.fld[(size_t)ne * NFLD + F_RHO]) +
-                              s.fld[(size_t)ne * NFLD + F_RHO];
+            const double rf = w * (fo[F_RHO] - fn[F_RHO]) + fn[F_RHO];
             const double lv = (muf + alf) / rf;
             o[FT_SV] = 0.5 * lv * ms * m.dc[f];
         }
@@ -506,11 +518,11 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
         // coupled interpolation of c and U (setCoAndDeltaT.H:44-56,78-89,
         // convectiveFluxScheme.C:509-524); reconstructed U, E for the
         // interface block (owner side, far side)
-        const int ow = m.own[f], kc = m.cycOf[f - m.nI];
-        const int ne = m.cycCell[kc];
+        const int kc = m.cycOf[f - m.nI];
+        const int nc = m.cycCell[kc];
         const double w = m.w[f];
-        const double cf = w * soundSpeed(ph, s.T[ow]) + (1.0 - w) * soundSpeed(ph, s.T[ne]);
-        const V3 Un = rot3(m.cycR + 9 * (size_t)kc, cellU(s.fld, ne));
+        const double cf = w * soundSpeed(ph, s.T[ow]) + (1.0 - w) * soundSpeed(ph, s.T[nc]);
+        const V3 Un = rot3(m.cycR + 9 * (size_t)kc, cellU(s.fld, nc));
         const V3 Uf = w * cellU(s.fld, ow) + (1.0 - w) * Un;
         const double lam = cf + fabs(dot(Uf, S / ms));
         o[FT_LAM] = lam;
@@ -537,7 +549,7 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
             else snU = m.dc[f] * (UB - Uc);
             T9 gUc;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) gUc.t[i] = s.gU[(size_t)c * 9 + i];
+            for (int i = 0; i < 9; ++i) gUc.t[i] = s.gv[(size_t)c * NGV + GV_U + i];
             const T9 gUb = gUc + outer(n, snU - vdott(n, gUc));
             const T9 taub = mu * dev2(transposeT(gUb));
             const V3 lap = (mu * snU) * ms;
@@ -700,7 +712,10 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
     int slot = 0;
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
-        if (RES) cs.add(s.ft + (size_t)f * NFT, q - e0 >= nLo, visc);
+        // the face record, gathered whole (12 128-bit loads)
+        double t[NFT];
+        if (RES || f < m.nI || m.nCyc) ldRow(s.ft + (size_t)f * NFT, t);
+        if (RES) cs.add(t, q - e0 >= nLo, visc);
         if (f >= m.nI) {
             // cyclic face: interface block (interfacesUpper: 1/2|S| A_R -
             // 1/2|S| lambda I) and diag -= interfacesLower (+1/2|S| A_L),
@@ -708,7 +723,6 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
             // (blockFvMatrix.C:241-265, :297-323); gamma interpolated
             if (m.nCyc == 0 || m.bKind[f - m.nI] != PK_CYCLIC) continue;
             const int kc = m.cycOf[f - m.nI];
-            const double* t = s.ft + (size_t)f * NFT;
             const double ms = m.magSf[f];
             const V3 nrm = ld3(m.Sf + 3 * (size_t)f) / ms;
             const double gf = m.w[f] * g + (1.0 - m.w[f]) * g;
@@ -731,7 +745,6 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
         }
         const size_t sk = rs + 64 * (size_t)slot;
         const bool isOwn = slot >= nLo;
-        const double* t = s.ft + (size_t)f * NFT;
         const V3 S{ldnt(m.sgeo + sgOff(sk, SG_SX)), ldnt(m.sgeo + sgOff(sk, SG_SX + 1)),
                    ldnt(m.sgeo + sgOff(sk, SG_SX + 2))};
         const double ms = ldnt(m.sgeo + sgOff(sk, SG_MAG));
@@ -955,15 +968,20 @@ void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipSt
     if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
     const int bc = m.bsCell, bf = m.bsFace;
     const dim3 gc(xcdGrid(m.nC, bc)), gf(xcdGrid(m.nF, bf));
+    // register budget (occ*): 0 the compiler's choice, 1 at most 128 VGPRs
+    // (4 waves per SIMD), 2 at most 168 (3 waves)
     if (ph.flux == FLUX_ROE) {
-        if (m.occCell) k_grad<7, 4><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
+        if (m.occCell == 1) k_grad<7, 4><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
+        else if (m.occCell == 2) k_grad<7, 3><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
         else k_grad<7><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
     } else {
-        if (m.occCell) k_grad<8, 4><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
+        if (m.occCell == 1) k_grad<8, 4><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
+        else if (m.occCell == 2) k_grad<8, 3><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
         else k_grad<8><<<gc, bc, 0, st>>>(m, ph, s, nullptr, nullptr);
     }
     const size_t lds = (size_t)bf * (NFT + 1) * sizeof(double);
-    if (m.occFace) k_face_terms<4><<<gf, bf, lds, st>>>(m, ph, s);
+    if (m.occFace == 1) k_face_terms<4><<<gf, bf, lds, st>>>(m, ph, s);
+    else if (m.occFace == 2) k_face_terms<3><<<gf, bf, lds, st>>>(m, ph, s);
     else k_face_terms<1><<<gf, bf, lds, st>>>(m, ph, s);
     if (cellSums) k_cell_residual<<<gc, bc, 0, st>>>(m, ph, s);
 }
@@ -993,7 +1011,8 @@ void launchJacobianHB(const HBInst* inst, int nO, int fp32, const MeshView& m, c
 void launchAdjointForward(const MeshView& m, const Phys& ph, const StateView& s, double* gun, double* limo,
                           hipStream_t st) {
     if (m.nB > 0) k_boundary<<<nblk(m.nB, 256), 256, 0, st>>>(m, ph, s);
-    if (ph.recon == 1) {
+    // limited gradients (second order) and the viscous Gauss gradients
+    if (ph.recon == 1 || ph.viscous) {
         if (ph.flux == FLUX_ROE) k_grad<7><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, gun, limo);
         else k_grad<8><<<xcdGrid(m.nC, 256), 256, 0, st>>>(m, ph, s, gun, limo);
     }
@@ -1011,12 +1030,15 @@ static void launchCellJacobian(const MeshView& m, const Phys& ph, const StateVie
     const int bj = m.bsJac;
     const dim3 g(xcdGrid(m.nC, bj));
     if (fp32) {
-        if (m.occJac) k_cell_jacobian<float, 4, RES><<<g, bj, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
-        else k_cell_jacobian<float, 1, RES><<<g, bj, 0, st>>>(m, ph, s, (float*)blk, (float*)dg, rD, err);
+        float *b = (float*)blk, *d = (float*)dg;
+        if (m.occJac == 1) k_cell_jacobian<float, 4, RES><<<g, bj, 0, st>>>(m, ph, s, b, d, rD, err);
+        else if (m.occJac == 2) k_cell_jacobian<float, 3, RES><<<g, bj, 0, st>>>(m, ph, s, b, d, rD, err);
+        else k_cell_jacobian<float, 1, RES><<<g, bj, 0, st>>>(m, ph, s, b, d, rD, err);
     } else {
-        if (m.occJac)
-            k_cell_jacobian<double, 4, RES><<<g, bj, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
-        else k_cell_jacobian<double, 1, RES><<<g, bj, 0, st>>>(m, ph, s, (double*)blk, (double*)dg, rD, err);
+        double *b = (double*)blk, *d = (double*)dg;
+        if (m.occJac == 1) k_cell_jacobian<double, 4, RES><<<g, bj, 0, st>>>(m, ph, s, b, d, rD, err);
+        else if (m.occJac == 2) k_cell_jacobian<double, 3, RES><<<g, bj, 0, st>>>(m, ph, s, b, d, rD, err);
+        else k_cell_jacobian<double, 1, RES><<<g, bj, 0, st>>>(m, ph, s, b, d, rD, err);
     }
 }
 void launchJacobian(const MeshView& m, const Phys& ph, const StateView& s, void* blk, void* dg, int fp32,

@@ -626,10 +626,11 @@ int autotune(icsf_case* c) {
         param = best;
     };
     // workgroup size and register budget of one kernel class, jointly
-    auto pick2 = [&](int& bsP, int& occP, const std::function<void()>& fn) {
+    // (budgets: 0 the compiler's choice, 1 at most 128 VGPRs, 2 at most 168)
+    auto pick2 = [&](int& bsP, int& occP, const std::function<void()>& fn, int nOcc = 3) {
         int bb = bsP, bo = occP;
         float bt = 1e30f;
-        for (int o : {0, 1})
+        for (int o = 0; o < nOcc; ++o)
             for (int b : {64, 128, 256}) {
                 bsP = b;
                 occP = o;
@@ -664,7 +665,7 @@ int autotune(icsf_case* c) {
                 launchSpmv(m, c->blkFp32, v32, c->blk, c->dg, x, nullptr, nullptr, y, c->st);
                 launchLusgs(m, c->mesh->lev, c->blkFp32, v32, c->blk, c->rD, y, c->st);
             }
-        });
+        }, 2);
         const void* V = v32 ? (const void*)c->V32 : (const void*)c->V;
         const int nv = v32 ? c->innerDirs : c->nDirs;
         if (c->orth == 1 && V && nv > 0)
@@ -816,12 +817,12 @@ int fgmresSolve(icsf_case* c, const double* W, const double* b, int& nIter) {
 namespace {
 
 int adjointSupported(const icsf_case* c) {
-    if (c->ph.viscous) return fail(ICSF_ERR_ARG, "residual VJP: viscous terms are not differentiated (inviscid only)");
     for (size_t b = 0; b < c->bPT.size(); ++b)
         if (c->bPT[b] == BC_TOTAL_P || c->bUT[b] == BC_PIOV || c->bTT[b] == BC_TOTAL_T)
             return fail(ICSF_ERR_ARG, "residual VJP: totalPressure, pressureInletOutletVelocity and totalTemperature "
                                       "conditions are not supported");
-    if (c->ph.flux != FLUX_AUSM) return fail(ICSF_ERR_ARG, "residual VJP: implemented for the AUSMPlusUp flux scheme");
+    if (c->ph.flux == FLUX_HLLC)
+        return fail(ICSF_ERR_ARG, "residual VJP: implemented for the AUSMPlusUp and Roe flux schemes");
     if (c->mv.nCyc > 0) return fail(ICSF_ERR_ARG, "residual VJP: meshes with cyclic patches are not supported");
     return ICSF_OK;
 }
@@ -845,6 +846,11 @@ int adjointPrepare(icsf_case* c) {
     RC(c->buf.alloc(a.bprim, nB * 5));
     RC(c->buf.alloc(c->adjGS, 3 * nF));
     RC(c->buf.alloc(c->adjGC, 3 * nF));
+    if (c->ph.viscous) {
+        RC(c->buf.alloc(a.vis, nF * NVIS));
+        RC(c->buf.alloc(a.qe, nF));
+        RC(c->buf.alloc(a.Ge, nC * 3));
+    }
     c->adjReady = true;
     return ICSF_OK;
 }
@@ -1186,7 +1192,9 @@ int hbPseudoStep(icsf_hb* h) {
             k_bound_flag<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, c->lowerBound, c->sv.W, c->fac);
             k_bound_apply<<<nblk(m.nC, 256), 256, 0, h->st>>>(c->mv, c->fac, c->sv.pseudoCo, nullptr);
         }
-        launchUpdate(m, c->ph, c->sv, h->dW + J * vs, 1, nullptr, nullptr, nullptr, h->st);
+        // the instance's own view: its boundary conditions and values (the
+        // instants' inflow total pressures differ)
+        launchUpdate(c->mv, c->ph, c->sv, h->dW + J * vs, 1, nullptr, nullptr, nullptr, h->st);
         // the residual and the matrix belong to the state before the update
         c->haveResidual = c->assembled = false;
     }
@@ -1537,8 +1545,7 @@ int createCase(icsf_mesh* mesh, const char* thermo, const char* schemes, const c
     RC(B.alloc(s.bfld, (size_t)NFLD * nBs));
     RC(B.alloc(s.bT, nBs));
     RC(B.alloc(s.grd, (size_t)NFLD * 3 * nC));
-    RC(B.alloc(s.gU, (size_t)9 * nC));
-    RC(B.alloc(s.ge, (size_t)3 * nC));
+    RC(B.alloc(s.gv, (size_t)NGV * nC));
     RC(B.alloc(s.pseudoCo, (size_t)nC));
     RC(B.alloc(s.rDT, (size_t)nC));
     RC(B.alloc(s.b, (size_t)5 * nC));
@@ -1778,14 +1785,23 @@ int icsf_residual(icsf_case* c, double* b, double* rDT) {
     return ICSF_OK;
 }
 
+// The geometry adjoint covers the inviscid residual only.
+static int geometryAdjointSupported(const icsf_case* c, const double* gSf, const double* gCf) {
+    if (c->ph.viscous && (gSf || gCf))
+        return fail(ICSF_ERR_ARG, "residual VJP: the Sf/Cf adjoint is built for inviscid residuals (pass NULL)");
+    return ICSF_OK;
+}
+
 int icsf_residual_vjp_device(icsf_case* c, const double* W, const double* v, double* gW, double* gSf,
                              double* gCf) {
     if (!c || !v || !gW) return fail(ICSF_ERR_ARG, "icsf_residual_vjp_device: NULL argument");
+    RC(geometryAdjointSupported(c, gSf, gCf));
     return vjpDevice(c, W, v, gW, gSf, gCf);
 }
 
 int icsf_residual_vjp(icsf_case* c, const double* W, const double* v, double* gW, double* gSf, double* gCf) {
     if (!c || !v || !gW) return fail(ICSF_ERR_ARG, "icsf_residual_vjp: NULL argument");
+    RC(geometryAdjointSupported(c, gSf, gCf));
     const size_t n5 = 5 * (size_t)c->mv.nC, n3 = 3 * (size_t)c->mv.nF;
     DevBuf tmp;
     double *dW = nullptr, *dv, *dg;

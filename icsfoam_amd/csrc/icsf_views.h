@@ -18,6 +18,10 @@ enum { BC_ZG = 0, BC_FIXED = 1, BC_SLIP = 2, BC_TOTAL_P = 3, BC_PIOV = 4, BC_TOT
 enum { PK_PATCH = 0, PK_WALL = 1, PK_EMPTY = 2, PK_CYCLIC = 3 };
 constexpr int NB = 25;  // entries of a 5x5 block
 
+// viscous gradient row of a cell (StateView::gv): 12 doubles (96 B), so a
+// row is six 16-byte accesses
+enum { GV_U = 0, GV_E = 9, NGV = 12 };
+
 // per-face terms written by the face kernel, AoS [nFaces][NFT]
 enum { FT_CONV = 0, FT_LAP = 5, FT_TAU = 8, FT_SIG = 11, FT_LAPE = 12, FT_LAM = 13, FT_FRDT = 14, FT_REC = 15,
        FT_SV = 23, NFT = 24 };
@@ -76,7 +80,8 @@ struct MeshView {
     // one thread, and the projection sums are per vector.
     int bsOp = 256, bsFace = 256, bsCell = 256, bsJac = 256, mdCh = 16;
     // register budget of the same kernel classes: 0 the compiler's choice,
-    // 1 at most 128 VGPRs (4 waves per SIMD, spilling what does not fit)
+    // 1 at most 128 VGPRs (4 waves per SIMD, spilling what does not fit),
+    // 2 at most 168 (3 waves; residual and Jacobian kernels)
     int occOp = 0, occFace = 0, occCell = 0, occJac = 0;
     // off-diagonal blocks of the internal faces stored as generators (1) or
     // as full 5x5 blocks (0); gamma for generating them
@@ -98,8 +103,9 @@ struct StateView {
     double* bfld;     // [nB][NFLD]
     double* bT;       // [nB]
     double* grd;      // [nC][NFLD][3] limited gradients
-    double* gU;       // [nC][9] Gauss grad(U) (viscous)
-    double* ge;       // [nC][3] Gauss grad(ecalc) (viscous)
+    double* gv;       // [nC][NGV] viscous gradients: Gauss grad(U) (GV_U..+8,
+                      //   row-major, grad(U)_ij = d U_j / d x_i), then Gauss
+                      //   grad(ecalc) (GV_E..+2)
     double* pseudoCo; // [nC]
     double* rDT;      // [nC]
     double* b;        // [5][nC]
@@ -146,9 +152,19 @@ struct AdjView {
     double* fext;   // [nC][NFLD] adjoint of that face's extrapolation
     double* qf;     // [nF][NFLD] adjoint of the Gauss face values (internal faces)
     double* bprim;  // [nB][5] boundary adjoint forwarded to the owner's p, T, U
-    int* fbr;       // [nF] AUSM+up branch bits of each face (records for derivative
+    int* fbr;       // [nF] branch bits of each face's flux (records for derivative
                     //   checks; null: not written)
+    // viscous cases (null otherwise):
+    double* vis;    // [nF][NVIS] face adjoints of the viscous terms: internal faces
+                    //   Z (9, the interpolated grad(U), tau path included), grad(e)
+                    //   (3), U of the owner (3) and of the neighbour (3), e of the
+                    //   owner and of the neighbour; boundary faces grad(U) of the
+                    //   owner (9), 0 (3), U of the owner (3), the boundary U (3), e
+                    //   of the owner, the boundary e
+    double* qe;     // [nF] adjoint of the Gauss face value of e (internal faces)
+    double* Ge;     // [nC][3] adjoint of the Gauss sum of grad(e), divided by V
 };
+enum { VS_Z = 0, VS_GE = 9, VS_UO = 12, VS_UN = 15, VS_EO = 18, VS_EN = 19, NVIS = 20 };
 
 // Workgroups of a launch are dispatched round-robin over the 8 XCDs, each
 // with its own L2.  Kernels that gather neighbour data launch a multiple of
@@ -185,6 +201,31 @@ template <typename T>
 __device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
 template <typename T>
 __device__ __forceinline__ void stnt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
+// Rows read or written whole (cell fields, gradients, face records): N
+// doubles at a 16-byte aligned address as 128-bit accesses, one
+// global_load_dwordx4 / global_store_dwordx4 per pair.  A gather of a
+// neighbour's row then takes N/2 memory instructions instead of N; in the
+// cell and face kernels, where every lane of a gather touches its own cache
+// line, the number of gather instructions is what bounds the kernel.
+template <int N>
+__device__ __forceinline__ void ldRow(const double* p, double (&v)[N]) {
+    static_assert(N % 2 == 0, "rows of an even number of doubles");
+    const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+        const double2 t = q[i];
+        v[2 * i] = t.x;
+        v[2 * i + 1] = t.y;
+    }
+}
+template <int N>
+__device__ __forceinline__ void stRow(double* p, const double (&v)[N]) {
+    double2* q = reinterpret_cast<double2*>(p);
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) q[i] = double2{v[2 * i], v[2 * i + 1]};
+    if (N % 2) p[N - 1] = v[N - 1];
+}
 
 // Slot geometry (MeshView::sgeo) of flat slot index `slot`, entry e.
 constexpr int NSG = 8;

@@ -38,6 +38,12 @@ def _bc_values(p_type, U_type, T_type, p_val, U_val, T_val):
     return pv, uv, tv
 
 
+def _cap(occ: int):
+    """Register budget of an autotuned kernel class: None (the compiler's
+    choice), 128 or 168 VGPRs."""
+    return {0: None, 1: 128, 2: 168}.get(int(occ), None)
+
+
 class fvMesh:
     def __init__(self, mesh: PolyMesh):
         self.poly = mesh
@@ -135,14 +141,16 @@ class dbnsCase:
         check(lib().icsf_residual(self._h, ptr(b), ptr(rdt)))
         return b, rdt
 
-    def residual_vjp(self, v, W=None):
+    def residual_vjp(self, v, W=None, geometry=True):
         """(gW, gSf, gCf) = ((db/dW)^T v, (db/dSf)^T v, (db/dCf)^T v) of the
-        inviscid AUSM+up residual at W ([5][nCells]; None = current state).
-        W becomes the case state."""
+        residual (AUSM+up or Roe, inviscid or viscous) at W ([5][nCells];
+        None = current state).  W becomes the case state.  gSf and gCf
+        (geometry=True) cover inviscid residuals; with geometry=False they
+        are None."""
         v = np.ascontiguousarray(v, np.float64)
         Wc = None if W is None else np.ascontiguousarray(W, np.float64)
         gW = np.empty((5, self.nCells))
-        gS, gC = np.empty((self.mesh.nFaces, 3)), np.empty((self.mesh.nFaces, 3))
+        gS, gC = (np.empty((self.mesh.nFaces, 3)), np.empty((self.mesh.nFaces, 3))) if geometry else (None, None)
         check(lib().icsf_residual_vjp(self._h, ptr(Wc), ptr(v), ptr(gW), ptr(gS), ptr(gC)))
         return gW, gS, gC
 
@@ -162,8 +170,8 @@ class dbnsCase:
         check(lib().icsf_get_tuning(self._h, ptr(p)))
         return dict(op_workgroup=int(p[0]), face_workgroup=int(p[1]), cell_workgroup=int(p[2]),
                     jacobian_workgroup=int(p[3]), projection_chunk=int(p[4]), tuned=bool(p[5]),
-                    op_vgpr_cap=bool(p[6]), face_vgpr_cap=bool(p[7]), cell_vgpr_cap=bool(p[8]),
-                    jacobian_vgpr_cap=bool(p[9]))
+                    op_vgpr_cap=_cap(p[6]), face_vgpr_cap=_cap(p[7]), cell_vgpr_cap=_cap(p[8]),
+                    jacobian_vgpr_cap=_cap(p[9]))
 
     def interfaces(self):
         """interfacesUpper blocks of the boundary faces [nB, 5, 5] (non-zero

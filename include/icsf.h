@@ -225,16 +225,19 @@ int icsf_calc_flux(icsf_case* c, double* phi, double* phiUp, double* phiEp);
 int icsf_residual(icsf_case* c, double* b, double* rPseudoDeltaT);
 
 /* ---- residual adjoint (no counterpart in the reference) --------------
- * Vector-Jacobian product of the inviscid AUSM+up residual b(W, Sf, Cf)
- * (the forward chain of updateFields.H:7-104, ausmPlusUpFluxScheme.C:73-299
- * and residualsUpdate.H:1-14), cell centres, volumes and weights fixed:
+ * Vector-Jacobian product of the residual b(W, Sf, Cf) with the AUSM+up or
+ * Roe scheme, inviscid or viscous (the forward chain of
+ * updateFields.H:7-104, ausmPlusUpFluxScheme.C:73-299 or
+ * roeFluxScheme.C:41-241,276-409, and residualsUpdate.H:1-43), cell
+ * centres, volumes and weights fixed:
  *   gW  = (db/dW)^T  v   [5][nCells]
  *   gSf = (db/dSf)^T v   [nFaces][3]   (NULL: not returned)
  *   gCf = (db/dCf)^T v   [nFaces][3]   (NULL: not returned)
  * W ([5][nCells], conserved) becomes the case state through updateFields
- * with a zero increment; W NULL keeps the current state.  Viscous cases and
- * the Roe/HLLC schemes return ICSF_ERR_ARG.  Host pointers; returns when
- * the results are written. */
+ * with a zero increment; W NULL keeps the current state.  gSf and gCf are
+ * built for inviscid residuals (viscous cases must pass NULL); HLLC, cyclic
+ * meshes and the derived inflow conditions return ICSF_ERR_ARG.  Host
+ * pointers; returns when the results are written. */
 int icsf_residual_vjp(icsf_case* c, const double* W, const double* v, double* gW, double* gSf, double* gCf);
 /* Same with device pointers (for framework bindings); synchronous. */
 int icsf_residual_vjp_device(icsf_case* c, const double* W, const double* v, double* gW, double* gSf,
@@ -242,14 +245,21 @@ int icsf_residual_vjp_device(icsf_case* c, const double* W, const double* v, dou
 /* Records of the points where the residual of icsf_residual_vjp is not
  * differentiable (no counterpart in the reference; for derivative checks):
  * W (device, [5][nCells], or NULL) becomes the state as in the VJP;
- * faceBranches [nFaces] gets each face's AUSM+up switches as bits (c*
- * against the normal velocity on either side, the smaller c, subsonic or
- * supersonic split Mach and its sign on either side, the pressure-diffusion
- * clamp, the 0.2 damping, the upwind side), limiterElement [nCells][8] the
+ * faceBranches [nFaces] gets each face's flux switches as bits - AUSM+up:
+ * c* against the normal velocity on either side, the smaller c, subsonic
+ * or supersonic split Mach and its sign on either side, the
+ * pressure-diffusion clamp, the 0.2 damping, the upwind side (boundary
+ * faces, whose two sides are one state: the smaller-c and upwind-side bits
+ * 0); Roe: the signs of u, u + c, u - c, the entropy fix of each wave, the
+ * larger of |u + c| and |u - c|, the sign of c^2's argument, the two
+ * density clamps (boundary faces: 0, the dissipation there acting on a
+ * zero jump) -, limiterElement [nCells][8] the
  * cell or boundary face (nCells + b) that is the max/min of each
  * reconstructed field's limiter (-1: none), limiterFace [nCells][8] the face
- * that set the limiter (-1: limiter 1).  Two states with equal records lie
- * in one smooth piece of the residual.  Device pointers; synchronous. */
+ * that set the limiter (-1: limiter 1); max/min candidates within 1e-12
+ * relative of each other count as equal (the first is kept).  Two states
+ * with equal records lie in one smooth piece of the residual.  Device
+ * pointers; synchronous. */
 int icsf_residual_records_device(icsf_case* c, const double* W, int* faceBranches, int* limiterElement,
                                  int* limiterFace);
 /* b(W) of icsf_residual with device pointers: W ([5][nCells] or NULL for

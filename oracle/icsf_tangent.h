@@ -23,7 +23,15 @@
 //   Gauss linear + cellLimited<Venkatakrishnan|minmod>, linearUpwind
 //                                   ausmPlusUpFluxScheme.C:82-87 (reconstruct)
 //   AUSM+up face flux               ausmPlusUpFluxScheme.C:112-299
+//   Roe face flux (E, H, rho, p, U reconstructed; P|Lambda|P^-1 with the
+//   entropy fix)                    roeFluxScheme.C:41-241,276-409
 //   surfaceIntegrate                residualsUpdate.H:1-14
+//   viscous terms (laplacian(muEff, U), div(tauMC), div(sigmaDotU),
+//   laplacian(alphaEff, e) with Gauss linear grad(U), grad(e) and
+//   corrected snGrad)               residualsUpdate.H:16-43
+// With the viscous terms the derivatives in Sf and Cf are not defined
+// (nonOrthDeltaCoeffs, deltaCoeffs and the non-orthogonal correction
+// vectors, functions of the geometry, are held at the mesh's values).
 
 struct Dual {
     double v, d;
@@ -54,6 +62,9 @@ template <class T>
 inline T tmin(T a, T b) { return val(b) < val(a) ? b : a; }
 template <class T>
 inline T sqrT(T a) { return a * a; }
+// |a|: the sign of the primal value picks the branch (a == 0: +)
+inline double fabsT(double a) { return std::fabs(a); }
+inline Dual fabsT(Dual a) { return Dual(std::fabs(a.v), a.v < 0 ? -a.d : a.d); }
 
 template <class T>
 struct V3T {
@@ -68,12 +79,76 @@ template <class T> inline T dotT(V3T<T> a, V3T<T> b) { return a.x * b.x + a.y * 
 template <class T> inline T magSqrT(V3T<T> a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
 template <class T> inline V3T<T> ld(const T* p) { return {p[0], p[1], p[2]}; }
 template <class T> inline V3T<T> lift(const double* p) { return {T(p[0]), T(p[1]), T(p[2])}; }
+template <class T> inline V3T<T> crossT(V3T<T> a, V3T<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// 3x3 tensor, row-major, the oracle's T9 operations in the same order
+template <class T>
+struct T9T {
+    T t[9];
+};
+template <class T> inline T9T<T> zero9() {
+    T9T<T> r;
+    for (int i = 0; i < 9; ++i) r.t[i] = T(0);
+    return r;
+}
+template <class T> inline T9T<T> outerT(V3T<T> a, V3T<T> b) {
+    return T9T<T>{{a.x * b.x, a.x * b.y, a.x * b.z, a.y * b.x, a.y * b.y, a.y * b.z, a.z * b.x, a.z * b.y, a.z * b.z}};
+}
+template <class T> inline T9T<T> operator+(const T9T<T>& a, const T9T<T>& b) {
+    T9T<T> r;
+    for (int i = 0; i < 9; ++i) r.t[i] = a.t[i] + b.t[i];
+    return r;
+}
+template <class T> inline T9T<T> operator-(const T9T<T>& a, const T9T<T>& b) {
+    T9T<T> r;
+    for (int i = 0; i < 9; ++i) r.t[i] = a.t[i] - b.t[i];
+    return r;
+}
+template <class T> inline T9T<T> operator*(T s, const T9T<T>& a) {
+    T9T<T> r;
+    for (int i = 0; i < 9; ++i) r.t[i] = s * a.t[i];
+    return r;
+}
+template <class T> inline T9T<T> operator*(const T9T<T>& a, T s) {
+    T9T<T> r;
+    for (int i = 0; i < 9; ++i) r.t[i] = a.t[i] * s;
+    return r;
+}
+template <class T> inline V3T<T> tdotvT(const T9T<T>& A, V3T<T> v) {
+    return {A.t[0] * v.x + A.t[1] * v.y + A.t[2] * v.z, A.t[3] * v.x + A.t[4] * v.y + A.t[5] * v.z,
+            A.t[6] * v.x + A.t[7] * v.y + A.t[8] * v.z};
+}
+template <class T> inline V3T<T> vdottT(V3T<T> v, const T9T<T>& A) {
+    return {v.x * A.t[0] + v.y * A.t[3] + v.z * A.t[6], v.x * A.t[1] + v.y * A.t[4] + v.z * A.t[7],
+            v.x * A.t[2] + v.y * A.t[5] + v.z * A.t[8]};
+}
+template <class T> inline T9T<T> tdottT(const T9T<T>& A, const T9T<T>& B) {
+    T9T<T> r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            r.t[3 * i + j] = A.t[3 * i] * B.t[j] + A.t[3 * i + 1] * B.t[3 + j] + A.t[3 * i + 2] * B.t[6 + j];
+    return r;
+}
+template <class T> inline T9T<T> transposeTT(const T9T<T>& A) {
+    return T9T<T>{{A.t[0], A.t[3], A.t[6], A.t[1], A.t[4], A.t[7], A.t[2], A.t[5], A.t[8]}};
+}
+template <class T> inline T9T<T> dev2T(const T9T<T>& A) {
+    T9T<T> r = A;
+    const T tr = A.t[0] + A.t[4] + A.t[8];
+    const T s = T(2.0 / 3.0) * tr;
+    r.t[0] = A.t[0] - s;
+    r.t[4] = A.t[4] - s;
+    r.t[8] = A.t[8] - s;
+    return r;
+}
 
 // Inviscid state of one evaluation: cell and boundary-face fields.
 template <class T>
 struct StateT {
-    std::vector<T> rho, p, Tt, U;        // cells (U: 3 per cell)
-    std::vector<T> pB, TB, UB, rhoB;     // boundary faces (UB: 3 per face)
+    std::vector<T> rho, p, Tt, U, rhoE;  // cells (U: 3 per cell; rhoE rebuilt as updateFields does)
+    std::vector<T> pB, TB, UB, rhoB, rhoEB;  // boundary faces (UB: 3 per face)
 };
 
 // updateFields.H:7-78 with a zero increment, then the boundary values of
@@ -83,6 +158,7 @@ void primsFromW(const Case& k, const T* W, const T* Sf, StateT<T>& s) {
     const Mesh& m = *k.m;
     const int nC = m.nC, nB = m.nF - m.nI;
     s.rho.assign(nC, T(0)); s.p.assign(nC, T(0)); s.Tt.assign(nC, T(0)); s.U.assign(3 * (size_t)nC, T(0));
+    s.rhoE.assign(nC, T(0));
     for (int c = 0; c < nC; ++c) {
         T rho = W[c] + T(0.0);
         V3T<T> rU{W[2 * (size_t)nC + c] + T(0.0), W[3 * (size_t)nC + c] + T(0.0), W[4 * (size_t)nC + c] + T(0.0)};
@@ -98,8 +174,10 @@ void primsFromW(const Case& k, const T* W, const T* Sf, StateT<T>& s) {
         s.U[3 * c] = U.x; s.U[3 * c + 1] = U.y; s.U[3 * c + 2] = U.z;
         s.Tt[c] = Tc;
         s.p[c] = rho / psi;
+        s.rhoE[c] = rho * (e + T(0.5) * magSqrT(U));
     }
     s.pB.assign(nB, T(0)); s.TB.assign(nB, T(0)); s.UB.assign(3 * (size_t)nB, T(0)); s.rhoB.assign(nB, T(0));
+    s.rhoEB.assign(nB, T(0));
     for (size_t pi = 0; pi < m.pStart.size(); ++pi) {
         if (m.pKind[pi] == PK_EMPTY) continue;
         const BC& bc = k.bcs[pi];
@@ -129,6 +207,7 @@ void primsFromW(const Case& k, const T* W, const T* Sf, StateT<T>& s) {
             s.TB[b] = TB;
             s.UB[3 * b] = ub.x; s.UB[3 * b + 1] = ub.y; s.UB[3 * b + 2] = ub.z;
             s.rhoB[b] = psi * pB;
+            s.rhoEB[b] = s.rhoB[b] * (T(k.Cv) * TB + T(0.5) * magSqrT(ub));
         }
     }
 }
@@ -269,7 +348,216 @@ void ausmFaceT(int lowMach, V3T<T> S, T magS, V3T<T> UL, V3T<T> UR, T cL, T cR, 
     phiE = rhoah * magS;
 }
 
-// b(W, Sf, Cf) of the inviscid AUSM+up residual in [rho, rhoE, rhoU] order.
+// roeFluxScheme::getRoeDissipation (roeFluxScheme.C:41-241), same
+// expression order as roeDissipation() in icsf_oracle.cpp.
+template <class T>
+struct RoeDissT {
+    T CR, CE, ER, EE;
+    V3T<T> CU, MR, ME, EU;
+    T9T<T> MU;
+};
+template <class T>
+RoeDissT<T> roeDissipationT(T rho, V3T<T> Vel, T c, double gI, T u, V3T<T> n, double eFix) {
+    const double a1 = gI - 1;
+    const T theta = T(0.5 * a1) * magSqrT(Vel);
+    const T c2 = sqrT(c);
+    const T a2 = T(1) / (rho * c * T(std::sqrt(2.0)));
+    const T a3 = rho / (c * T(std::sqrt(2.0)));
+    const T a4 = (theta + c2) / T(a1);
+    const T a5 = T(1) - theta / c2;
+    const T a6 = theta / T(a1);
+    const V3T<T> iP11 = (n * a5) - crossT(Vel, n) / rho;
+    T9T<T> iP12 = outerT(T(a1) / c2 * n, Vel);
+    iP12.t[1] += n.z / rho; iP12.t[2] -= n.y / rho; iP12.t[3] -= n.z / rho;
+    iP12.t[5] += n.x / rho; iP12.t[6] += n.y / rho; iP12.t[7] -= n.x / rho;
+    const V3T<T> iP13 = T(-a1) / c2 * n;
+    const T iP21 = a2 * (theta - c * u);
+    const V3T<T> iP22 = (-a2) * (T(a1) * Vel - c * n);
+    const T iP23 = T(a1) * a2;
+    const T iP31 = a2 * (theta + c * u);
+    const V3T<T> iP32 = (-a2) * (T(a1) * Vel + c * n);
+    const T iP33 = T(a1) * a2;
+    T L1 = fabsT(u), L2 = fabsT(u + c), L3 = fabsT(u - c);
+    const T eps = T(eFix) * tmax(L2, L3);
+    if (val(L1) < val(eps)) L1 = (sqrT(L1) + sqrT(eps)) / (T(2.0) * eps);
+    if (val(L2) < val(eps)) L2 = (sqrT(L2) + sqrT(eps)) / (T(2.0) * eps);
+    if (val(L3) < val(eps)) L3 = (sqrT(L3) + sqrT(eps)) / (T(2.0) * eps);
+    const V3T<T> P11 = n;
+    const T P12 = a3, P13 = a3;
+    T9T<T> P21 = outerT(Vel, n);
+    P21.t[1] -= n.z * rho; P21.t[2] += n.y * rho; P21.t[3] += n.z * rho;
+    P21.t[5] -= n.x * rho; P21.t[6] -= n.y * rho; P21.t[7] += n.x * rho;
+    const V3T<T> P22 = a3 * (Vel + c * n);
+    const V3T<T> P23 = a3 * (Vel - c * n);
+    const V3T<T> P31 = n * a6 + rho * crossT(Vel, n);
+    const T P32 = a3 * (a4 + c * u);
+    const T P33 = a3 * (a4 - c * u);
+    RoeDissT<T> D;
+    const V3T<T> P11L = P11 * L1;
+    const T9T<T> P21L = P21 * L1;
+    const V3T<T> P31L = P31 * L1;
+    D.CR = dotT(P11L, iP11) + L2 * P12 * iP21 + L3 * P13 * iP31;
+    D.CU = vdottT(P11L, iP12) + (L2 * P12) * iP22 + (L3 * P13) * iP32;
+    D.CE = dotT(P11L, iP13) + L2 * P12 * iP23 + L3 * P13 * iP33;
+    D.MR = tdotvT(P21L, iP11) + (L2 * P22) * iP21 + (L3 * P23) * iP31;
+    D.MU = tdottT(P21L, iP12) + outerT(L2 * P22, iP22) + outerT(L3 * P23, iP32);
+    D.ME = tdotvT(P21L, iP13) + (L2 * P22) * iP23 + (L3 * P23) * iP33;
+    D.ER = dotT(P31L, iP11) + L2 * P32 * iP21 + L3 * P33 * iP31;
+    D.EU = vdottT(P31L, iP12) + (L2 * P32) * iP22 + (L3 * P33) * iP32;
+    D.EE = dotT(P31L, iP13) + L2 * P32 * iP23 + L3 * P33 * iP33;
+    return D;
+}
+
+// roeFluxScheme::calcFlux (roeFluxScheme.C:276-409), one face, same order
+// as roeFace() in icsf_oracle.cpp.
+template <class T>
+void roeFaceT(double gI, double eFix, V3T<T> S, T magS, T rl, T rr, T pl, T pr, V3T<T> Ul, V3T<T> Ur, T El, T Er,
+              T Hl, T Hr, T& phi, V3T<T>& phiU, T& phiE) {
+    const V3T<T> n = S / magS;
+    const T coefR = sqrtT(tmax(T(VSMALL), rr) / tmax(T(VSMALL), rl));
+    const T rhoRoe = coefR * rl;
+    const V3T<T> URoe = (coefR * Ur + Ul) / (coefR + T(1.0));
+    const T HRoe = (coefR * Hr + Hl) / (coefR + T(1.0));
+    const T cRoe = sqrtT(fabsT(T(gI - 1.0) * (HRoe - T(0.5) * magSqrT(URoe))));
+    const T uMl = dotT(Ul, n), uMr = dotT(Ur, n);
+    const T uProj = dotT(URoe, n);
+    const V3T<T> rUl = rl * Ul, rUr = rr * Ur;
+    const T rEl = rl * El, rEr = rr * Er;
+    const RoeDissT<T> D = roeDissipationT(rhoRoe, URoe, cRoe, gI, uProj, n, eFix);
+    const T dR = rr - rl;
+    const V3T<T> dRU = rUr - rUl;
+    const T dRE = rEr - rEl;
+    phi = T(-0.5) * magS * (D.CR * dR + dotT(D.CU, dRU) + D.CE * dRE);
+    phiU = T(-0.5) * magS * (D.MR * dR + tdotvT(D.MU, dRU) + D.ME * dRE);
+    phiE = T(-0.5) * magS * (D.ER * dR + dotT(D.EU, dRU) + D.EE * dRE);
+    const T rnl = rl * uMl, rnr = rr * uMr;
+    phi = phi + T(0.5) * magS * (rnl + rnr);
+    phiU = phiU + T(0.5) * magS * (rnl * Ul + rnr * Ur + n * (pl + pr));
+    phiE = phiE + T(0.5) * magS * (rnl * Hl + rnr * Hr);
+}
+
+// Viscous terms of residualsUpdate.H:16-43 as the oracle's
+// viscousResidual() forms them: per face |S| mu snGrad(U), Sf & tauMC,
+// (mu grad(U) + tauMC) & U & Sf and |S| alpha snGrad(e), each summed per
+// cell (surfaceIntegrate, divided by V) into RU [3 nC] and RE [nC].
+template <class T>
+void viscousT(const Case& k, const StateT<T>& s, const T* Sf, std::vector<T>& RU, std::vector<T>& RE) {
+    const Mesh& m = *k.m;
+    const int nC = m.nC, nB = m.nF - m.nI;
+    const double mu = k.s.mu;
+    const double alpha = k.s.gamma * mu / k.s.Pr;
+    auto Uc = [&](int c) { return V3T<T>{s.U[3 * c], s.U[3 * c + 1], s.U[3 * c + 2]}; };
+    auto Ub = [&](int b) { return V3T<T>{s.UB[3 * b], s.UB[3 * b + 1], s.UB[3 * b + 2]}; };
+    auto magSf = [&](int f) { return sqrtT(magSqrT(ld(&Sf[3 * (size_t)f]))); };
+    std::vector<T9T<T>> gU(nC, zero9<T>());
+    for (int f = 0; f < m.nI; ++f) {
+        const int o = m.own[f], n = m.nei[f];
+        const V3T<T> Uf = T(m.w[f]) * (Uc(o) - Uc(n)) + Uc(n);
+        const T9T<T> t = outerT(ld(&Sf[3 * (size_t)f]), Uf);
+        gU[o] = gU[o] + t;
+        gU[n] = gU[n] - t;
+    }
+    for (int f = m.nI; f < m.nF; ++f) {
+        if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
+        const T9T<T> t = outerT(ld(&Sf[3 * (size_t)f]), Ub(f - m.nI));
+        gU[m.own[f]] = gU[m.own[f]] + t;
+    }
+    for (int c = 0; c < nC; ++c)
+        for (int i = 0; i < 9; ++i) gU[c].t[i] = gU[c].t[i] / T(m.V[c]);
+    std::vector<V3T<T>> snU(nB, V3T<T>{T(0), T(0), T(0)});
+    std::vector<T9T<T>> gUb(nB, zero9<T>());
+    for (size_t pi = 0; pi < m.pStart.size(); ++pi) {
+        if (m.pKind[pi] == PK_EMPTY) continue;
+        for (int i = 0; i < m.pSize[pi]; ++i) {
+            const int f = m.pStart[pi] + i, b = f - m.nI, c = m.own[f];
+            const V3T<T> n = ld(&Sf[3 * (size_t)f]) / magSf(f);
+            const V3T<T> U = Uc(c);
+            const int ut = k.bcs[pi].uT;
+            if (ut == BC_ZG) snU[b] = V3T<T>{T(0), T(0), T(0)};
+            else if (ut == BC_SLIP) {
+                const T9T<T> Tr{{T(1) - T(2.0) * n.x * n.x, T(-2.0) * n.x * n.y, T(-2.0) * n.x * n.z,
+                                 T(-2.0) * n.y * n.x, T(1) - T(2.0) * n.y * n.y, T(-2.0) * n.y * n.z,
+                                 T(-2.0) * n.z * n.x, T(-2.0) * n.z * n.y, T(1) - T(2.0) * n.z * n.z}};
+                snU[b] = (tdotvT(Tr, U) - U) * (T(m.dc[f]) / T(2.0));
+            } else snU[b] = T(m.dc[f]) * (Ub(b) - U);
+            gUb[b] = gU[c] + outerT(n, snU[b] - vdottT(n, gU[c]));
+        }
+    }
+    std::vector<T9T<T>> tau(nC), taub(nB, zero9<T>());
+    for (int c = 0; c < nC; ++c) tau[c] = T(mu) * dev2T(transposeTT(gU[c]));
+    for (int b = 0; b < nB; ++b) taub[b] = T(mu) * dev2T(transposeTT(gUb[b]));
+    std::vector<T> ec(nC), eb(nB, T(0));
+    for (int c = 0; c < nC; ++c) ec[c] = s.rhoE[c] / s.rho[c] - T(0.5) * magSqrT(Uc(c));
+    for (int b = 0; b < nB; ++b)
+        if (val(s.rhoB[b]) != 0) eb[b] = s.rhoEB[b] / s.rhoB[b] - T(0.5) * magSqrT(Ub(b));
+    std::vector<V3T<T>> ge;
+    gaussGradT(m, ec.data(), eb.data(), Sf, ge);
+    std::vector<T> lapU(3 * (size_t)m.nF, T(0)), divTau(3 * (size_t)m.nF, T(0)), sDotU(m.nF, T(0)),
+        lapE(m.nF, T(0));
+    for (int f = 0; f < m.nI; ++f) {
+        const int o = m.own[f], n = m.nei[f];
+        const double w = m.w[f];
+        const V3T<T> S = ld(&Sf[3 * (size_t)f]);
+        const V3T<T> k3 = lift<T>(&m.corr[3 * (size_t)f]);
+        const T9T<T> gUf = T(w) * (gU[o] - gU[n]) + gU[n];
+        const T9T<T> tauf = T(w) * (tau[o] - tau[n]) + tau[n];
+        const V3T<T> Uf = T(w) * (Uc(o) - Uc(n)) + Uc(n);
+        const T muf = T(w * (mu - mu) + mu);
+        const T alf = T(w * (alpha - alpha) + alpha);
+        const V3T<T> sn = T(m.nodc[f]) * (Uc(n) - Uc(o)) + vdottT(k3, gUf);
+        const V3T<T> l = (muf * sn) * magSf(f);
+        lapU[3 * f] = l.x; lapU[3 * f + 1] = l.y; lapU[3 * f + 2] = l.z;
+        const V3T<T> dt = vdottT(S, tauf);
+        divTau[3 * f] = dt.x; divTau[3 * f + 1] = dt.y; divTau[3 * f + 2] = dt.z;
+        const V3T<T> sig = tdotvT(muf * gUf + tauf, Uf);
+        sDotU[f] = dotT(sig, S);
+        const V3T<T> gef = T(w) * (ge[o] - ge[n]) + ge[n];
+        const T sne = T(m.nodc[f]) * (ec[n] - ec[o]) + dotT(k3, gef);
+        lapE[f] = (alf * sne) * magSf(f);
+    }
+    for (int f = m.nI; f < m.nF; ++f) {
+        if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
+        const int b = f - m.nI, c = m.own[f];
+        const V3T<T> S = ld(&Sf[3 * (size_t)f]);
+        const V3T<T> l = (T(mu) * snU[b]) * magSf(f);
+        lapU[3 * f] = l.x; lapU[3 * f + 1] = l.y; lapU[3 * f + 2] = l.z;
+        const V3T<T> dt = vdottT(S, taub[b]);
+        divTau[3 * f] = dt.x; divTau[3 * f + 1] = dt.y; divTau[3 * f + 2] = dt.z;
+        const V3T<T> sig = tdotvT(T(mu) * gUb[b] + taub[b], Ub(b));
+        sDotU[f] = dotT(sig, S);
+        const T sne = T(m.dc[f]) * (eb[b] - ec[c]);
+        lapE[f] = (T(alpha) * sne) * magSf(f);
+    }
+    // surfaceIntegrate of one component (stride, comp), divided by V
+    auto integ = [&](const std::vector<T>& sf, int stride, int comp, std::vector<T>& out) {
+        out.assign(nC, T(0));
+        for (int f = 0; f < m.nI; ++f) {
+            out[m.own[f]] = out[m.own[f]] + sf[(size_t)f * stride + comp];
+            out[m.nei[f]] = out[m.nei[f]] - sf[(size_t)f * stride + comp];
+        }
+        for (int f = m.nI; f < m.nF; ++f) {
+            if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
+            out[m.own[f]] = out[m.own[f]] + sf[(size_t)f * stride + comp];
+        }
+        for (int c = 0; c < nC; ++c) out[c] = out[c] / T(m.V[c]);
+    };
+    std::vector<T> t;
+    for (int d = 0; d < 3; ++d) {
+        integ(lapU, 3, d, t);
+        for (int c = 0; c < nC; ++c) RU[3 * (size_t)c + d] = RU[3 * (size_t)c + d] + t[c];
+    }
+    for (int d = 0; d < 3; ++d) {
+        integ(divTau, 3, d, t);
+        for (int c = 0; c < nC; ++c) RU[3 * (size_t)c + d] = RU[3 * (size_t)c + d] + t[c];
+    }
+    integ(sDotU, 1, 0, t);
+    for (int c = 0; c < nC; ++c) RE[c] = RE[c] + t[c];
+    integ(lapE, 1, 0, t);
+    for (int c = 0; c < nC; ++c) RE[c] = RE[c] + t[c];
+}
+
+// b(W, Sf, Cf) of the residual in [rho, rhoE, rhoU] order: AUSM+up or Roe,
+// inviscid or viscous.
 template <class T>
 void residualT(const Case& k, const T* W, const T* Sf, const T* Cf, T* b) {
     const Mesh& m = *k.m;
@@ -302,7 +590,16 @@ void residualT(const Case& k, const T* W, const T* Sf, const T* Cf, T* b) {
     reconstructT(k, s.p.data(), s.pB.data(), Sf, Cf, pL, pR);
     reconstructT(k, s.rho.data(), s.rhoB.data(), Sf, Cf, rL, rR);
     reconstructT(k, H.data(), Hb.data(), Sf, Cf, HL, HR);
-    reconstructT(k, cs.data(), csb.data(), Sf, Cf, cL, cR);
+    const bool roe = k.s.flux == FLUX_ROE;
+    std::vector<T> E(nC), Eb(nB, T(0)), EL, ER;
+    if (roe) {
+        for (int c = 0; c < nC; ++c)
+            E[c] = T(k.Cv) * s.Tt[c] + T(0.5) * magSqrT(V3T<T>{s.U[3 * c], s.U[3 * c + 1], s.U[3 * c + 2]});
+        for (int bb = 0; bb < nB; ++bb)
+            if (val(s.TB[bb]) != 0)
+                Eb[bb] = T(k.Cv) * s.TB[bb] + T(0.5) * magSqrT(V3T<T>{s.UB[3 * bb], s.UB[3 * bb + 1], s.UB[3 * bb + 2]});
+        reconstructT(k, E.data(), Eb.data(), Sf, Cf, EL, ER);
+    } else reconstructT(k, cs.data(), csb.data(), Sf, Cf, cL, cR);
     std::vector<T> acc(5 * (size_t)nC, T(0));
     auto add = [&](int c, T sgn, T ph, V3T<T> pu, T pe) {
         acc[c] = acc[c] + sgn * ph;
@@ -317,8 +614,13 @@ void residualT(const Case& k, const T* W, const T* Sf, const T* Cf, T* b) {
         if (f >= m.nI && m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
         const V3T<T> S = ld(&Sf[3 * (size_t)f]);
         const T magS = sqrtT(magSqrT(S));
-        ausmFaceT(k.s.lowMach, S, magS, V3T<T>{UxL[f], UyL[f], UzL[f]}, V3T<T>{UxR[f], UyR[f], UzR[f]}, cL[f], cR[f],
-                  pL[f], pR[f], rL[f], rR[f], HL[f], HR[f], PH[f], PU[f], PE[f]);
+        const V3T<T> UL{UxL[f], UyL[f], UzL[f]}, UR{UxR[f], UyR[f], UzR[f]};
+        if (roe)
+            roeFaceT(k.s.gamma, k.s.entropyFix, S, magS, rL[f], rR[f], pL[f], pR[f], UL, UR, EL[f], ER[f], HL[f],
+                     HR[f], PH[f], PU[f], PE[f]);
+        else
+            ausmFaceT(k.s.lowMach, S, magS, UL, UR, cL[f], cR[f], pL[f], pR[f], rL[f], rR[f], HL[f], HR[f], PH[f],
+                      PU[f], PE[f]);
     }
     // surfaceIntegrate: the internal-face loop, then the boundary faces
     for (int f = 0; f < m.nI; ++f) {
@@ -329,9 +631,22 @@ void residualT(const Case& k, const T* W, const T* Sf, const T* Cf, T* b) {
         if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
         add(m.own[f], T(1.0), PH[f], PU[f], PE[f]);
     }
+    std::vector<T> RU, RE;
+    if (k.s.viscous) {
+        // the convective part of R, then the viscous sums in the oracle's order
+        RU.assign(3 * (size_t)nC, T(0));
+        RE.assign(nC, T(0));
+        for (int c = 0; c < nC; ++c) {
+            RE[c] = -(acc[(size_t)nC + c] / T(m.V[c]));
+            for (int d = 0; d < 3; ++d) RU[3 * (size_t)c + d] = -(acc[(size_t)(2 + d) * nC + c] / T(m.V[c]));
+        }
+        viscousT(k, s, Sf, RU, RE);
+    }
     for (int c = 0; c < nC; ++c)
         for (int q = 0; q < 5; ++q) {
-            const T R = -(acc[(size_t)q * nC + c] / T(m.V[c]));
+            T R = -(acc[(size_t)q * nC + c] / T(m.V[c]));
+            if (k.s.viscous && q == 1) R = RE[c];
+            if (k.s.viscous && q >= 2) R = RU[3 * (size_t)c + q - 2];
             b[(size_t)q * nC + c] = R * T(m.V[c]);
         }
 }

@@ -34,6 +34,27 @@ def ausm_inviscid(c, limiter=None, recon=None):
     return dataclasses.replace(c, thermo=cases.thermo_dict(0.0), fvSchemes=cases.schemes_dict("AUSMPlusUp", **kw))
 
 
+def roe(c, mu=None, limiter=None, recon=None):
+    """The same case with the Roe scheme: viscous with viscosity mu (None:
+    the case's own), optionally another limiter / reconstruction."""
+    import dataclasses
+    kw = {}
+    if limiter is not None:
+        kw["limiter"] = limiter
+    if recon is not None:
+        kw["recon"] = recon
+    c = dataclasses.replace(c, fvSchemes=cases.schemes_dict("ROE", **kw))
+    if mu is not None:
+        c = dataclasses.replace(c, thermo=cases.thermo_dict(mu))
+    return c
+
+
+def with_mu(c, mu):
+    """The same case with viscosity mu (0: inviscid)."""
+    import dataclasses
+    return dataclasses.replace(c, thermo=cases.thermo_dict(mu))
+
+
 def generic_state(W, seed, amp=1e-3):
     """W with small uncorrelated per-cell noise on all five components, so
     that no two neighbouring values tie and no velocity component is exactly

@@ -10,7 +10,7 @@ pressureInletOutletVelocity), fixed outlet pressure with
 pressureInletOutletVelocity, the outlet's inletOutlet temperature taken as
 zeroGradient, a slip blade.  Bitwise: fluxes, residual, local time step,
 Jacobian incl. interface blocks, SpMV, LU-SGS; GMRES increment 1e-10;
-three pseudo-steps 1e-8."""
+first-step history 1e-10, three pseudo-steps 1e-8."""
 import os
 
 import numpy as np
@@ -68,7 +68,10 @@ def test_pseudo_steps(pair):
     c = pair
     g, o = dbnsCase.from_case(c), OracleCase(c)
     hg, ho = g.pseudo_steps(3), o.pseudo_steps(3)
-    assert rel_err(hg[0], ho[0]) < 1e-12
+    # the first step's history holds GMRES residual norms: parallel
+    # reductions on the device, sequential sums in the oracle, the class of
+    # the increments' 1e-10 above
+    assert rel_err(hg[0], ho[0]) < 1e-10
     assert rel_err(hg, ho) < 1e-8
     fg, fo = g.fields(), o.fields()
     for k in ("rho", "U", "T", "p", "W"):
