@@ -89,6 +89,7 @@ SIGNATURES = {
     "icsf_hb_synchronize": (_i, [_vp]),
     "icsf_residual_vjp": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_residual_vjp_device": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "icsf_residual_vjp_device_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "icsf_residual_device": (_i, [_vp, _vp, _vp]),
     "icsf_residual_records_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "icsf_assemble_jacobian": (_i, [_vp]),

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -68,6 +69,45 @@ struct DevBuf {
     }
     ~DevBuf() {
         for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+// Fixed work space of icsf_residual_vjp_device_noalloc, for callers that
+// cannot allocate device memory (code under graph capture, serving loops):
+// one array per GPU, held until the process ends.  In host code the name
+// `scratch` is only a stand-in; its device address on each GPU comes from
+// hipGetSymbolAddress (scratchAddress), taken once per device.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+std::mutex g_scratchMu;
+char* g_scratchDev[64] = {};
+
+// device address of `scratch` on device dev (the current device)
+int scratchAddress(int dev, char** out) {
+    if (dev < 0 || dev >= 64) return fail(ICSF_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchDev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ICSF_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratchDev[dev] = (char*)p;
+    }
+    *out = g_scratchDev[dev];
+    return ICSF_OK;
+}
+
+// Consecutive 256-byte aligned pieces of the scratch array (base nullptr:
+// sizes only).
+struct ScratchCarve {
+    char* base;
+    size_t off;
+    template <class T>
+    T* take(size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = base ? (T*)(base + off) : nullptr;
+        off += (n ? n : 1) * sizeof(T);
+        return p;
     }
 };
 
@@ -1869,6 +1909,64 @@ int icsf_residual_records_device(icsf_case* c, const double* W, int* faceBranche
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     (void)nF;
+    return takeDeviceErrors(c->st, {c->derr});
+}
+
+int icsf_residual_vjp_device_noalloc(icsf_case* c, const double* W, const double* v, double* gW, double* gSf,
+                                     double* gCf) {
+    if (!c || !v || !gW) return fail(ICSF_ERR_ARG, "icsf_residual_vjp_device_noalloc: NULL argument");
+    RC(geometryAdjointSupported(c, gSf, gCf));
+    if (!W && !c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
+    RC(adjointSupported(c));
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != c->device) return fail(ICSF_ERR_ARG, "icsf_residual_vjp_device_noalloc: the case's device is not current");
+    const size_t nC = c->mv.nC, nF = c->mv.nF, nB = std::max(c->mv.nB, 1);
+    // the adjoint's work buffers carved from the fixed scratch array
+    ScratchCarve sc{nullptr, 0};
+    auto carve = [&](AdjView& a, double*& gS, double*& gC) {
+        a.W0 = sc.take<double>(5 * nC);
+        a.gun = sc.take<double>(nC * NFLD * 3);
+        a.lim = sc.take<double>(nC * NFLD);
+        a.fLR = sc.take<double>(nF * 2 * NFLD);
+        a.G = sc.take<double>(nC * NFLD * 3);
+        a.qc = sc.take<double>(nC * NFLD);
+        a.tgt = sc.take<int>(nC * NFLD);
+        a.tval = sc.take<double>(nC * NFLD);
+        a.fst = sc.take<int>(nC * NFLD);
+        a.fext = sc.take<double>(nC * NFLD);
+        a.qf = sc.take<double>(nF * NFLD);
+        a.bprim = sc.take<double>(nB * 5);
+        gS = sc.take<double>(3 * nF);
+        gC = sc.take<double>(3 * nF);
+        if (c->ph.viscous) {
+            a.vis = sc.take<double>(nF * NVIS);
+            a.qe = sc.take<double>(nF);
+            a.Ge = sc.take<double>(nC * 3);
+        }
+    };
+    AdjView a{};
+    double *gS = nullptr, *gC = nullptr;
+    carve(a, gS, gC);  // sizes only
+    if (sc.off > kScratchBytes)
+        return fail(ICSF_ERR_ARG, "icsf_residual_vjp_device_noalloc: the case needs " + std::to_string(sc.off) +
+                                      " bytes of work space, more than the fixed " + std::to_string(kScratchBytes) +
+                                      "-byte scratch array (use icsf_residual_vjp_device)");
+    char* base = nullptr;
+    RC(scratchAddress(dev, &base));
+    sc = ScratchCarve{base, 0};
+    carve(a, gS, gC);
+    RC(uploadBC(c));
+    const size_t n5 = 5 * nC;
+    if (W) HIPCHK(hipMemcpyAsync(c->sv.W, W, n5 * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(a.W0, c->sv.W, n5 * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    launchUpdate(c->mv, c->ph, c->sv, nullptr, 1, nullptr, nullptr, nullptr, c->st);
+    c->haveFields = true;
+    c->haveResidual = c->assembled = false;
+    launchAdjointForward(c->mv, c->ph, c->sv, a.gun, a.lim, c->st);
+    launchResidualVjp(c->mv, c->ph, c->sv, a, v, gW, gSf ? gSf : gS, gCf ? gCf : gC, c->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
     return takeDeviceErrors(c->st, {c->derr});
 }
 

@@ -242,6 +242,17 @@ int icsf_residual_vjp(icsf_case* c, const double* W, const double* v, double* gW
 /* Same with device pointers (for framework bindings); synchronous. */
 int icsf_residual_vjp_device(icsf_case* c, const double* W, const double* v, double* gW, double* gSf,
                              double* gCf);
+/* icsf_residual_vjp_device without any device allocation (no counterpart
+ * in the reference;
+ * for callers that cannot allocate: code under HIP graph capture, serving
+ * loops): the adjoint's work buffers (about 750 bytes per cell and 240 per
+ * face, 24 and 170 more with the viscous terms) live in one fixed 128 MiB
+ * array per GPU, held until the process ends; cases needing more are
+ * rejected with ICSF_ERR_ARG.  The case's device must be current.  Calls on
+ * one GPU share the array and must not overlap; results are those of
+ * icsf_residual_vjp_device. */
+int icsf_residual_vjp_device_noalloc(icsf_case* c, const double* W, const double* v, double* gW, double* gSf,
+                                     double* gCf);
 /* Records of the points where the residual of icsf_residual_vjp is not
  * differentiable (no counterpart in the reference; for derivative checks):
  * W (device, [5][nCells], or NULL) becomes the state as in the VJP;
