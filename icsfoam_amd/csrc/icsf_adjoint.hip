@@ -1,18 +1,23 @@
-// Adjoint (vector-Jacobian product) of the inviscid AUSM+up residual
-//     b(W, Sf, Cf) = -(sum_f +-F_f / V) * V,
+// Adjoint (vector-Jacobian product) of the coupled residual
+//     b(W, Sf, Cf) = -(sum_f +-F_f / V) * V   (+ the viscous terms),
 // v -> (db/dW)^T v, (db/dSf)^T v, (db/dCf)^T v, with the cell centres,
 // volumes and interpolation weights held fixed.  The reference has no
 // adjoint; the forward chain differentiated here is the one of
 // updateFields.H:7-104 (W -> rho, U, e, T, p and the boundary values),
-// ausmPlusUpFluxScheme.C:73-299 (E, H, c*, reconstruction with
-// cellLimited Gauss gradients, AUSM+up face flux) and residualsUpdate.H:1-14
-// (surfaceIntegrate).
+// ausmPlusUpFluxScheme.C:73-299 / roeFluxScheme.C:41-241 (E, H, c*,
+// reconstruction with cellLimited Gauss gradients, the face flux) and
+// residualsUpdate.H:1-43 (surfaceIntegrate, laplacian(muEff, U),
+// div(tauMC), div(sigmaDotU), laplacian(alphaEff, e)).
 //
 // The reconstruction is recomputed, not taped: the forward pass re-runs the
 // boundary and gradient kernels (keeping the unlimited gradients and the
-// limiter values), then four reverse kernels pull adjoints without atomics:
-//   k_adj_face   one thread per face: AUSM+up reverse, adjoint of the face
-//                values (owner/neighbour side) and of Sf, Cf (flux part)
+// limiter values), then the reverse kernels pull adjoints without atomics:
+//   k_adj_face   one thread per face: AUSM+up or Roe reverse (each one
+//                forward evaluation and one backward sweep), adjoint of
+//                the face values (owner/neighbour side) and of Sf, Cf
+//                (flux part)
+//   k_adj_visc   one thread per face (viscous cases): adjoint of the
+//                viscous face terms
 //   k_adj_cell   one thread per cell: adjoint of the limited gradients, of
 //                the limiter (its active face and max/min neighbour) and of
 //                the Gauss gradient sum divided by V
@@ -23,9 +28,9 @@
 // Branches follow the primal value: max(a, b) takes b only if b > a,
 // min(a, b) takes b only if b < a, and the limiter's min over faces keeps
 // the first face that attains it.
-#include "icsf_dual.h"
 #include "icsf_kernels.h"
 #include "icsf_recon.h"
+#include "icsf_roe_rev.h"
 #include "icsf_views.h"
 
 namespace icsf {
@@ -181,44 +186,12 @@ DF void ausmFaceAdj(bool lowMach, V3 S, double magS, const double* L, const doub
     bL[F_H] = HL_b; bR[F_H] = HR_b;
 }
 
-// Reverse of roeFace(): adjoints of the owner-side (L) and neighbour-side
-// (R) face values rho, p, U, E, H and of Sf, by one forward dual pass per
-// input (inputs 0-6: L fields 0-6, 7-13: R fields 0-6, 14-16: Sf, with
-// |Sf| following Sf), each contracted with the flux adjoint.
-DF void roeFaceAdj(double gI, double eFix, V3 S, double magS, const double* L, const double* R, double bphi,
-                   V3 bphiU, double bphiE, double* bL, double* bR, V3& bS, int* branches) {
-#pragma unroll
-    for (int k = 0; k < NFLD; ++k) bL[k] = bR[k] = 0.0;
-    bS = V3{0, 0, 0};
-#pragma unroll 1
-    for (int j = 0; j < 17; ++j) {
-        const auto sl = [&](int k) { return Dd{L[k], j == k ? 1.0 : 0.0}; };
-        const auto sr = [&](int k) { return Dd{R[k], j == 7 + k ? 1.0 : 0.0}; };
-        const V3D Sd{Dd{S.x, j == 14 ? 1.0 : 0.0}, Dd{S.y, j == 15 ? 1.0 : 0.0}, Dd{S.z, j == 16 ? 1.0 : 0.0}};
-        const Dd ms{magS, j == 14 ? S.x / magS : (j == 15 ? S.y / magS : (j == 16 ? S.z / magS : 0.0))};
-        Dd ph, pe;
-        V3D pu;
-        roeFaceD(gI, eFix, Sd, ms, sl(F_RHO), sr(F_RHO), sl(F_P), sr(F_P), V3D{sl(F_UX), sl(F_UY), sl(F_UZ)},
-                 V3D{sr(F_UX), sr(F_UY), sr(F_UZ)}, sl(F_E), sr(F_E), sl(F_H), sr(F_H), ph, pu, pe,
-                 j == 0 ? branches : nullptr);
-        const double g = bphi * ph.d + (bphiU.x * pu.x.d + bphiU.y * pu.y.d + bphiU.z * pu.z.d) + bphiE * pe.d;
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            if (j == k) bL[k] = g;
-            if (j == 7 + k) bR[k] = g;
-        }
-        if (j == 14) bS.x = g;
-        if (j == 15) bS.y = g;
-        if (j == 16) bS.z = g;
-    }
-}
-
 // Face pass: adjoint of the face flux from v (b_owner gets -F, b_neighbour
 // +F), AUSM+up reverse, reconstruction reverse into Cf.
 template <bool ROE>
 __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView s, AdjView a,
                                                   const double* __restrict__ v, double* __restrict__ gS,
-                                                  double* __restrict__ gC) {
+                                                  double* __restrict__ gC, bool geom) {
     const int f = xcdBlock() * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
     double* lr = a.fLR + (size_t)f * 2 * NFLD;
@@ -251,8 +224,8 @@ __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView
     double bL[NFLD], bR[NFLD];
     V3 bS;
     if constexpr (ROE) {
-        roeFaceAdj(ph.gamma, ph.entropyFix, S, m.magSf[f], L, R, bf[0], V3{bf[2], bf[3], bf[4]}, bf[1], bL, bR, bS,
-                   a.fbr ? a.fbr + f : nullptr);
+        roeFaceRev(ph.gamma, ph.entropyFix, S, m.magSf[f], L, R, bf[0], V3{bf[2], bf[3], bf[4]}, bf[1], bL, bR, bS,
+                   a.fbr ? a.fbr + f : nullptr, geom);
         // on a boundary face both sides are the boundary state: the
         // dissipation multiplies a zero jump, whose derivative is zero on
         // either side of every switch
@@ -422,14 +395,15 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
     // records pass (a.fbr set): the limiter's active face and max/min
     // element are found for every field, whatever the adjoint seed
     const bool records = a.fbr != nullptr;
-    double qd[NFLD], mx[NFLD], mn[NFLD];
+    double qd[NFLD], mx[NFLD], mn[NFLD], vcr[NFLD];
     V3 gl[NFLD];
     int imx[NFLD], imn[NFLD];
+    ldRow(s.fld + (size_t)c * NFLD, vcr);
 #pragma unroll
     for (int k = 0; k < NFLD; ++k) {
         qd[k] = 0.0;
         gl[k] = V3{0, 0, 0};
-        mx[k] = mn[k] = s.fld[(size_t)c * NFLD + k];
+        mx[k] = mn[k] = vcr[k];
         imx[k] = imn[k] = -1;
     }
     // viscous: adjoint of the cell's Gauss grad(U) (zu, row-major) and
@@ -453,15 +427,18 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
         const double* vo;
         int id;
         if (visc) {
-            const double* vs = a.vis + (size_t)f * NVIS;
+            // grad(U) and grad(e) sum adjoints: the first 12 entries of the row
+            double vz[VS_GE + 3];
+            ldRow(a.vis + (size_t)f * NVIS, vz);
             const double wt = f < m.nI ? (j >= nLo ? m.w[f] : 1.0 - m.w[f]) : 1.0;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) zu[i] += wt * vs[VS_Z + i];
-            ges = ges + wt * V3{vs[VS_GE], vs[VS_GE + 1], vs[VS_GE + 2]};
+            for (int i = 0; i < 9; ++i) zu[i] += wt * vz[VS_Z + i];
+            ges = ges + wt * V3{vz[VS_GE], vz[VS_GE + 1], vz[VS_GE + 2]};
         }
         if (f < m.nI) {
             const bool isOwn = j >= nLo;
-            const double* lr = a.fLR + (size_t)f * 2 * NFLD + (isOwn ? 0 : NFLD);
+            double lr[NFLD];
+            ldRow(a.fLR + (size_t)f * 2 * NFLD + (isOwn ? 0 : NFLD), lr);
 #pragma unroll
             for (int k = 0; k < NFLD; ++k) {
                 if (!used(ph.flux, k)) continue;
@@ -477,10 +454,12 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
             vo = s.bfld + (size_t)b * NFLD;
         }
         if (limit) {
+            double vr[NFLD];
+            ldRow(vo, vr);
 #pragma unroll
             for (int k = 0; k < NFLD; ++k) {
                 if (!used(ph.flux, k)) continue;
-                const double v = vo[k];
+                const double v = vr[k];
                 // records pass: a candidate within rounding of the current
                 // max/min does not displace it.  Such ties are two equal
                 // functions of the state computed two ways (H of a cell and
@@ -497,14 +476,18 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
     V3 g[NFLD], gb[NFLD];
     double limb[NFLD];
     bool any = false;
+    double gunr[3 * NFLD], limr[NFLD];
+    if (rec) {
+        ldRow(a.gun + (size_t)c * NFLD * 3, gunr);
+        ldRow(a.lim + (size_t)c * NFLD, limr);
+    }
 #pragma unroll
     for (int k = 0; k < NFLD; ++k) {
-        const size_t ck = (size_t)c * NFLD + k;
         g[k] = gb[k] = V3{0, 0, 0};
         limb[k] = 0.0;
         if (!used(ph.flux, k) || !rec) continue;
-        g[k] = ld3(a.gun + 3 * ck);
-        gb[k] = a.lim[ck] * gl[k];
+        g[k] = V3{gunr[3 * k], gunr[3 * k + 1], gunr[3 * k + 2]};
+        gb[k] = limr[k] * gl[k];
         limb[k] = dot(gl[k], g[k]);
         if (limit && (limb[k] != 0.0 || records)) any = true;
     }
@@ -513,9 +496,8 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
 #pragma unroll
     for (int k = 0; k < NFLD; ++k) {
         fs[k] = -1; br[k] = 0; lm[k] = 1.0; rs[k] = 1.0; es[k] = 0.0;
-        const double vc = s.fld[(size_t)c * NFLD + k];
-        mxD[k] = mx[k] - vc;
-        mnD[k] = mn[k] - vc;
+        mxD[k] = mx[k] - vcr[k];
+        mnD[k] = mn[k] - vcr[k];
     }
     if (any) {
         for (int q = e0; q < e1; ++q) {
@@ -536,6 +518,10 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
             }
         }
     }
+    static_assert(NFLD % 2 == 0, "fields stored in pairs");
+    double pq = 0.0, pv = 0.0, px = 0.0;
+    int pt = -1, pf = -1;
+    V3 pG{0, 0, 0};
 #pragma unroll
     for (int k = 0; k < NFLD; ++k) {
         const size_t ck = (size_t)c * NFLD + k;
@@ -568,12 +554,23 @@ __global__ void __launch_bounds__(256) k_adj_cell(MeshView m, Phys ph, StateView
             }
             G = gbk / Vc;
         }
-        a.qc[ck] = used(ph.flux, k) ? qk : 0.0;
-        a.tgt[ck] = tgt;
-        a.tval[ck] = tv;
-        a.fst[ck] = fsk;
-        a.fext[ck] = fx;
-        a.G[3 * ck] = G.x; a.G[3 * ck + 1] = G.y; a.G[3 * ck + 2] = G.z;
+        // stored a pair of fields at a time (128-bit stores; 64-bit for
+        // the int rows): the even field is held until the odd one is done
+        const double qv = used(ph.flux, k) ? qk : 0.0;
+        if (k & 1) {
+            const size_t c0 = ck - 1;
+            *reinterpret_cast<double2*>(a.qc + c0) = double2{pq, qv};
+            *reinterpret_cast<int2*>(a.tgt + c0) = int2{pt, tgt};
+            *reinterpret_cast<double2*>(a.tval + c0) = double2{pv, tv};
+            *reinterpret_cast<int2*>(a.fst + c0) = int2{pf, fsk};
+            *reinterpret_cast<double2*>(a.fext + c0) = double2{px, fx};
+            double2* gq = reinterpret_cast<double2*>(a.G + 3 * c0);
+            gq[0] = double2{pG.x, pG.y};
+            gq[1] = double2{pG.z, G.x};
+            gq[2] = double2{G.y, G.z};
+        } else {
+            pq = qv; pt = tgt; pv = tv; pf = fsk; px = fx; pG = G;
+        }
     }
     if (visc) {
         const V3 Ge = ges / Vc;
@@ -611,17 +608,35 @@ __global__ void __launch_bounds__(256) k_adj_gauss(MeshView m, Phys ph, StateVie
         const int n = m.nei[f];
         const double w = m.w[f];
         if (visc) a.qe[f] = dot(S, ld3(a.Ge + 3 * (size_t)o) - ld3(a.Ge + 3 * (size_t)n));
+        // both cells' rows: field values, the limiter faces, and the
+        // gradient adjoints two fields (six doubles) at a time
+        double fo[NFLD], fn[NFLD], qv[NFLD];
+        int so[NFLD], sn[NFLD];
+        ldRow(s.fld + (size_t)o * NFLD, fo);
+        ldRow(s.fld + (size_t)n * NFLD, fn);
+        ldRowI(a.fst + (size_t)o * NFLD, so);
+        ldRowI(a.fst + (size_t)n * NFLD, sn);
 #pragma unroll
-        for (int k = 0; k < NFLD; ++k) {
-            if (!grads(k)) { qf[k] = 0.0; continue; }
-            const size_t ok = (size_t)o * NFLD + k, nk = (size_t)n * NFLD + k;
-            const V3 d = ld3(a.G + 3 * ok) - ld3(a.G + 3 * nk);
-            qf[k] = dot(S, d);
-            const double vo = s.fld[ok], vn = s.fld[nk];
-            bS = bS + (w * (vo - vn) + vn) * d;
-            if (a.fst[ok] == f) bC = bC + a.fext[ok] * ld3(a.gun + 3 * ok);
-            if (a.fst[nk] == f) bC = bC + a.fext[nk] * ld3(a.gun + 3 * nk);
+        for (int kp = 0; kp < NFLD; kp += 2) {
+            double Go[6], Gn[6];
+            if (grads(kp) || grads(kp + 1)) {
+                ldRow(a.G + 3 * ((size_t)o * NFLD + kp), Go);
+                ldRow(a.G + 3 * ((size_t)n * NFLD + kp), Gn);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k = kp + h;
+                if (!grads(k)) { qv[k] = 0.0; continue; }
+                const size_t ok = (size_t)o * NFLD + k, nk = (size_t)n * NFLD + k;
+                const V3 d = V3{Go[3 * h], Go[3 * h + 1], Go[3 * h + 2]} - V3{Gn[3 * h], Gn[3 * h + 1], Gn[3 * h + 2]};
+                qv[k] = dot(S, d);
+                const double vo = fo[k], vn = fn[k];
+                bS = bS + (w * (vo - vn) + vn) * d;
+                if (so[k] == f) bC = bC + a.fext[ok] * ld3(a.gun + 3 * ok);
+                if (sn[k] == f) bC = bC + a.fext[nk] * ld3(a.gun + 3 * nk);
+            }
         }
+        stRow(qf, qv);
     } else {
         const int b = f - m.nI;
         const double* bv = s.bfld + (size_t)b * NFLD;
@@ -701,8 +716,7 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
     if (c >= m.nC) return;
     const int e0 = m.cfOff[c], e1 = m.cfOff[c + 1];
     double q[NFLD];
-#pragma unroll
-    for (int k = 0; k < NFLD; ++k) q[k] = a.qc[(size_t)c * NFLD + k];
+    ldRow(a.qc + (size_t)c * NFLD, q);
     double pbnd = 0.0, Tbnd = 0.0;
     V3 Ubnd{0, 0, 0};
     // viscous: direct U and e adjoints of the viscous face terms, and the
@@ -721,19 +735,24 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
             const int j = m.col[sk];
             const double wf = m.sgeo[sgOff(sk, SG_W)];
             const double wt = isOwn ? wf : 1.0 - wf;
-            const double* qf = a.qf + (size_t)f * NFLD;
+            double qf[NFLD];
+            int tg[NFLD];
+            ldRow(a.qf + (size_t)f * NFLD, qf);
+            ldRowI(a.tgt + (size_t)j * NFLD, tg);
 #pragma unroll
             for (int k = 0; k < NFLD; ++k) {
                 if (!used(ph.flux, k) && !(visc && k >= F_UX && k <= F_UZ)) continue;
                 q[k] += wt * qf[k];
-                const size_t jk = (size_t)j * NFLD + k;
-                if (a.tgt[jk] == c) q[k] += a.tval[jk];
+                if (tg[k] == c) q[k] += a.tval[(size_t)j * NFLD + k];
             }
             if (visc) {
-                const double* vs = a.vis + (size_t)f * NVIS;
-                const int u0 = isOwn ? VS_UO : VS_UN;
-                Uv = Uv + V3{vs[u0], vs[u0 + 1], vs[u0 + 2]};
-                ecB += (isOwn ? vs[VS_EO] : vs[VS_EN]) + wt * a.qe[f];
+                // the state adjoints of the viscous face terms: entries
+                // VS_UO .. VS_EN of the row
+                double vu[NVIS - VS_UO];
+                ldRow(a.vis + (size_t)f * NVIS + VS_UO, vu);
+                const int u0 = (isOwn ? VS_UO : VS_UN) - VS_UO;
+                Uv = Uv + V3{vu[u0], vu[u0 + 1], vu[u0 + 2]};
+                ecB += (isOwn ? vu[VS_EO - VS_UO] : vu[VS_EN - VS_UO]) + wt * a.qe[f];
             }
         } else {
             const double* bp = a.bprim + (size_t)(f - m.nI) * 5;
@@ -796,10 +815,10 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
 }  // namespace
 
 void launchResidualVjp(const MeshView& m, const Phys& ph, const StateView& s, const AdjView& a, const double* v,
-                       double* gW, double* gS, double* gC, hipStream_t st) {
+                       double* gW, double* gS, double* gC, bool geom, hipStream_t st) {
     const dim3 gf(xcdGrid(m.nF, 256)), gc(xcdGrid(m.nC, 256));
-    if (ph.flux == FLUX_ROE) k_adj_face<true><<<gf, 256, 0, st>>>(m, ph, s, a, v, gS, gC);
-    else k_adj_face<false><<<gf, 256, 0, st>>>(m, ph, s, a, v, gS, gC);
+    if (ph.flux == FLUX_ROE) k_adj_face<true><<<gf, 256, 0, st>>>(m, ph, s, a, v, gS, gC, geom);
+    else k_adj_face<false><<<gf, 256, 0, st>>>(m, ph, s, a, v, gS, gC, geom);
     if (ph.viscous) {
         k_adj_visc<<<gf, 256, 0, st>>>(m, ph, s, a, v);
         k_adj_cell<true><<<gc, 256, 0, st>>>(m, ph, s, a);

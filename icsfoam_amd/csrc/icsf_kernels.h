@@ -74,13 +74,25 @@ struct HBInst {
     double* rD;
     int* err;
 };
+// The harmonic-balance terms formed in the instances' Jacobian launch
+// (HBZone.C:359-518): with the residual cell sums, the spectral source into
+// the coupled right-hand side; V D_JJ on each instance's diagonal blocks;
+// then (one small launch) the shared LU-SGS diagonal over all instances.
+struct HBCouple {
+    int nO;
+    const double* D;  // [nO][nO] spectral operator
+    double* Wall;     // [nO][5][nC] coupled state (written with the source)
+    double* ball;     // [nO][5][nC] coupled right-hand side b + source
+    double* rD;       // [nC] shared LU-SGS reciprocal diagonal
+    int* err;         // zero-diagonal flag of the coupled system
+};
 // cellSums: as launchResidual / launchCellSumsJacobian, for all instances
 void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys& ph, hipStream_t st,
                       bool cellSums = true);
 // Fourier reconstruction of the instances' fields at one time, out [nC][6]
 void launchHBReconstruct(int nC, int nO, const HBInst* inst, const double* w, double* out, hipStream_t st);
 void launchJacobianHB(const HBInst* inst, int nO, int fp32, const MeshView& m, const Phys& ph, hipStream_t st,
-                      bool cellSums = false);
+                      bool cellSums, const HBCouple& hc);
 
 // Harmonic-balance coupling: per-instance block storage (FP64) and the
 // spectral operator D [nO][nO] (device).
@@ -129,9 +141,11 @@ void launchUpdate(const MeshView& m, const Phys& ph, const StateView& s, const d
 // keeping the unlimited gradients and limiter values.
 void launchAdjointForward(const MeshView& m, const Phys& ph, const StateView& s, double* gun, double* limo,
                           hipStream_t st);
-// Reverse pass: gW = (db/dW)^T v, gS = (db/dSf)^T v, gC = (db/dCf)^T v.
+// Reverse pass: gW = (db/dW)^T v, gS = (db/dSf)^T v, gC = (db/dCf)^T v
+// (geom false: gS and gC are work space only, and the Roe reverse skips
+// its Sf passes).
 void launchResidualVjp(const MeshView& m, const Phys& ph, const StateView& s, const AdjView& a, const double* v,
-                       double* gW, double* gS, double* gC, hipStream_t st);
+                       double* gW, double* gS, double* gC, bool geom, hipStream_t st);
 // cellSums false: everything up to the face terms; the cell sums (b,
 // rPseudoDeltaT) are then formed by launchCellSumsJacobian.
 void launchResidual(const MeshView& m, const Phys& ph, const StateView& s, hipStream_t st, bool cellSums = true);

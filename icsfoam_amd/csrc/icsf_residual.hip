@@ -691,10 +691,18 @@ __global__ void k_face_flux(MeshView m, StateView s, double* phi, double* phiU, 
 // RES: the cell's residual sums (cellResidualBody) are formed in the same
 // face walk, from the same face-record loads, and rPseudoDeltaT is taken
 // from them (one pass over the records instead of two; same bits).
-template <typename TB, bool RES>
+// HBC (harmonic balance, instance J of the instances I): with RES the
+// spectral source -V sum_K D_JK W_K is added to b into the coupled
+// right-hand side (HBZone::addSource, HBZoneTemplates.C:39-93), and V D_JJ
+// to the diagonal block as stored (HBZone::addBlock, HBZone.C:435-518);
+// the instance's own scalar diagonal is not formed (the LU-SGS sweeps of
+// the coupled system take the shared one, k_hb_shared_diag).
+template <typename TB, bool RES, bool HBC = false>
 __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& ph, const StateView& s,
                                                  TB* __restrict__ blk, TB* __restrict__ dg,
-                                                 double* __restrict__ rD, int* __restrict__ err) {
+                                                 double* __restrict__ rD, int* __restrict__ err,
+                                                 const HBInst* __restrict__ I = nullptr, int J = 0,
+                                                 const HBCouple* hc = nullptr) {
     const int c = xcdBlock() * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const double g = ph.gamma;
@@ -786,6 +794,16 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
     }
     double rDTc = 0.0;
     if (RES) rDTc = cs.store(m, s, c, visc);
+    if constexpr (HBC && RES) {
+        const size_t nC = m.nC, vs = 5 * nC;
+        const int nO = hc->nO;
+        for (int q = 0; q < 5; ++q) {
+            double src = 0.0;
+            for (int K = 0; K < nO; ++K) src -= m.V[c] * hc->D[J * nO + K] * I[K].s.W[(size_t)q * nC + c];
+            hc->ball[J * vs + (size_t)q * nC + c] = s.b[(size_t)q * nC + c] + src;
+            hc->Wall[J * vs + (size_t)q * nC + c] = s.W[(size_t)q * nC + c];
+        }
+    }
     // boundary dissipation (all non-empty patches) follows the internal faces
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
@@ -895,8 +913,14 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
         D[0] += dcv; D[12] += dcv; D[18] += dcv; D[24] += dcv; D[6] += dcv;
     }
     if (visc) { D[0] = D[0] - vis; D[12] = D[12] - vis; D[18] = D[18] - vis; D[24] = D[24] - vis; D[6] = D[6] - vis; }
+    if constexpr (HBC) {
+        // added to the diagonal as stored (FP32 with mixed precision)
+        const double VD = m.V[c] * hc->D[J * hc->nO + J];
+        for (int e = 0; e < NB; e += 6) D[e] = (double)(TB)D[e] + VD;
+    }
 #pragma unroll
     for (int e = 0; e < NB; ++e) stnt(dg + diagOff(c, e), (TB)D[e]);
+    if constexpr (HBC) return;
     // LU-SGS scalar diagonal
     double r = kGREAT;
     r = 1.0 / fmax(1.0 / r, fabs(D[0]));
@@ -914,10 +938,34 @@ __global__ void __launch_bounds__(256, W) k_cell_jacobian(MeshView m, Phys ph, S
                                                           int* __restrict__ err) {
     cellJacobianBody<TB, RES>(m, ph, s, blk, dg, rD, err);
 }
+// The shared LU-SGS diagonal of the coupled harmonic-balance system
+// (lusgs::lusgs, lusgs.C:50-125, over the 2 nO scalar and nO vector
+// diagonals in coupledMatrix order), one thread per cell, after the
+// instances' Jacobian launch has stored their diagonal blocks.
+template <typename TB>
+__global__ void __launch_bounds__(256) k_hb_shared_diag(const HBInst* __restrict__ I, HBCouple hc, int nC) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    const auto d = [&](int K, int e) { return fabs((double)((const TB*)I[K].dg)[diagOff(c, e)]); };
+    double r = kGREAT;
+    for (int K = 0; K < hc.nO; ++K) {
+        r = 1.0 / fmax(1.0 / r, d(K, 0));
+        r = 1.0 / fmax(1.0 / r, d(K, 6));
+    }
+    for (int K = 0; K < hc.nO; ++K) {
+        r = 1.0 / fmax(1.0 / r, d(K, 12));
+        r = 1.0 / fmax(1.0 / r, d(K, 18));
+        r = 1.0 / fmax(1.0 / r, d(K, 24));
+    }
+    hc.rD[c] = r;
+    if (!(r >= kVSMALL)) atomicOr(hc.err, ERR_FLAG_DIAG);
+}
+
 template <typename TB, bool RES = false>
-__global__ void __launch_bounds__(256) k_cell_jacobian_hb(const HBInst* __restrict__ I, Phys ph) {
-    const HBInst q = I[blockIdx.y];
-    cellJacobianBody<TB, RES>(q.m, ph, q.s, (TB*)q.blk, (TB*)q.dg, q.rD, q.err);
+__global__ void __launch_bounds__(256) k_cell_jacobian_hb(const HBInst* __restrict__ I, Phys ph, HBCouple hc) {
+    const int J = blockIdx.y;
+    const HBInst q = I[J];
+    cellJacobianBody<TB, RES, true>(q.m, ph, q.s, (TB*)q.blk, (TB*)q.dg, q.rD, q.err, I, J, &hc);
 }
 
 }  // namespace
@@ -997,15 +1045,18 @@ void launchResidualHB(const HBInst* inst, int nO, const MeshView& m, const Phys&
 }
 
 void launchJacobianHB(const HBInst* inst, int nO, int fp32, const MeshView& m, const Phys& ph, hipStream_t st,
-                      bool cellSums) {
+                      bool cellSums, const HBCouple& hc) {
     const dim3 g(xcdGrid(m.nC, m.bsJac), nO);
     if (cellSums) {
-        if (fp32) k_cell_jacobian_hb<float, true><<<g, m.bsJac, 0, st>>>(inst, ph);
-        else k_cell_jacobian_hb<double, true><<<g, m.bsJac, 0, st>>>(inst, ph);
+        if (fp32) k_cell_jacobian_hb<float, true><<<g, m.bsJac, 0, st>>>(inst, ph, hc);
+        else k_cell_jacobian_hb<double, true><<<g, m.bsJac, 0, st>>>(inst, ph, hc);
     } else {
-        if (fp32) k_cell_jacobian_hb<float><<<g, m.bsJac, 0, st>>>(inst, ph);
-        else k_cell_jacobian_hb<double><<<g, m.bsJac, 0, st>>>(inst, ph);
+        if (fp32) k_cell_jacobian_hb<float><<<g, m.bsJac, 0, st>>>(inst, ph, hc);
+        else k_cell_jacobian_hb<double><<<g, m.bsJac, 0, st>>>(inst, ph, hc);
     }
+    const int nb = (m.nC + 255) / 256;
+    if (fp32) k_hb_shared_diag<float><<<nb, 256, 0, st>>>(inst, hc, m.nC);
+    else k_hb_shared_diag<double><<<nb, 256, 0, st>>>(inst, hc, m.nC);
 }
 
 void launchAdjointForward(const MeshView& m, const Phys& ph, const StateView& s, double* gun, double* limo,

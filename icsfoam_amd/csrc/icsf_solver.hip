@@ -908,13 +908,13 @@ int stateFromW(icsf_case* c, const double* W) {
     return ICSF_OK;
 }
 
-int vjpDevice(icsf_case* c, const double* W, const double* v, double* gW, double* gSf, double* gCf) {
+int vjpDevice(icsf_case* c, const double* W, const double* v, double* gW, double* gSf, double* gCf, bool geom) {
     if (!W && !c->haveFields) return fail(ICSF_ERR_STATE, "fields not set");
     RC(adjointPrepare(c));
     RC(uploadBC(c));
     RC(stateFromW(c, W));
     launchAdjointForward(c->mv, c->ph, c->sv, c->adj.gun, c->adj.lim, c->st);
-    launchResidualVjp(c->mv, c->ph, c->sv, c->adj, v, gW, gSf ? gSf : c->adjGS, gCf ? gCf : c->adjGC, c->st);
+    launchResidualVjp(c->mv, c->ph, c->sv, c->adj, v, gW, gSf ? gSf : c->adjGS, gCf ? gCf : c->adjGC, geom, c->st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     return takeDeviceErrors(c->st, {c->derr});
@@ -969,10 +969,6 @@ struct icsf_hb {
 
 namespace {
 
-struct HBDiag {
-    void* dg[HB_MAXO];
-};
-
 // Instance J = blockIdx.y: out_J = b_J + source_J with
 // source_J = ((0 - V D_J0 W_0) - V D_J1 W_1) - ..., and W_J copied into the
 // coupled state vector Wall (written only, never read here).
@@ -990,34 +986,6 @@ __global__ void k_hb_source(int nC, int nO, const HBInst* __restrict__ I, const 
         out[J * vs + (size_t)q * nC + c] = bJ[(size_t)q * nC + c] + src;
         Wall[J * vs + (size_t)q * nC + c] = WJ[(size_t)q * nC + c];
     }
-}
-
-// V D_JJ on the instance diagonals, then lusgs::lusgs over all 2 nO scalar
-// and nO vector diagonals in coupledMatrix order (lusgs.C:50-125).
-// FP32 storage (mixed precision): the sum is formed in FP64 and rounded.
-template <typename TB>
-__global__ void k_hb_diag(int nC, int nO, const double* __restrict__ D, const double* __restrict__ Vol, HBDiag h,
-                          double* __restrict__ rD, int* err) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nC) return;
-    for (int J = 0; J < nO; ++J) {
-        const double VD = Vol[c] * D[J * nO + J];
-        TB* dg = (TB*)h.dg[J];
-        for (int e = 0; e < 25; e += 6) dg[diagOff(c, e)] = (TB)((double)dg[diagOff(c, e)] + VD);
-    }
-    auto d = [&](int J, int e) { return fabs((double)((const TB*)h.dg[J])[diagOff(c, e)]); };
-    double r = kGREAT;
-    for (int J = 0; J < nO; ++J) {
-        r = 1.0 / fmax(1.0 / r, d(J, 0));
-        r = 1.0 / fmax(1.0 / r, d(J, 6));
-    }
-    for (int J = 0; J < nO; ++J) {
-        r = 1.0 / fmax(1.0 / r, d(J, 12));
-        r = 1.0 / fmax(1.0 / r, d(J, 18));
-        r = 1.0 / fmax(1.0 / r, d(J, 24));
-    }
-    rD[c] = r;
-    if (!(r >= kVSMALL)) atomicOr(err, ERR_FLAG_DIAG);
 }
 
 // SER over all instances (outerLoop.H:30-66): dres = [init nO*5][prev nO*5]
@@ -1069,19 +1037,17 @@ int hbResidualStage(icsf_hb* h, bool cellSums = true) {
     return ICSF_OK;
 }
 
+// All instances' Jacobian rows in one launch, with V D_JJ on their
+// diagonals (then the shared LU-SGS diagonal); cellSums: also the
+// instances' residual cell sums and the spectral source into the coupled
+// right-hand side (the pseudo-step: gradients, face terms, then this).
 int hbJacobianStage(icsf_hb* h, bool cellSums = false) {
     const MeshView& m = h->inst[0]->mv;
-    launchJacobianHB(h->dInst, h->nO, h->hbb.fp32, m, h->inst[0]->ph, h->st, cellSums);
-    if (cellSums) {
+    const HBCouple hc{h->nO, h->dD, h->Wall, h->ball, h->rD, h->derr};
+    launchJacobianHB(h->dInst, h->nO, h->hbb.fp32, m, h->inst[0]->ph, h->st, cellSums, hc);
+    if (cellSums)
         for (icsf_case* c : h->inst) c->haveResidual = true;
-        k_hb_source<<<dim3(nblk(m.nC, 256), h->nO), 256, 0, h->st>>>(m.nC, h->nO, h->dInst, h->dD, m.V, h->Wall,
-                                                                      h->ball);
-    }
     for (icsf_case* c : h->inst) c->assembled = true;
-    HBDiag hd{};
-    for (int J = 0; J < h->nO; ++J) hd.dg[J] = h->inst[J]->dg;
-    if (h->hbb.fp32) k_hb_diag<float><<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, h->nO, h->dD, m.V, hd, h->rD, h->derr);
-    else k_hb_diag<double><<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, h->nO, h->dD, m.V, hd, h->rD, h->derr);
     h->assembled = true;
     return ICSF_OK;
 }
@@ -1836,7 +1802,7 @@ int icsf_residual_vjp_device(icsf_case* c, const double* W, const double* v, dou
                              double* gCf) {
     if (!c || !v || !gW) return fail(ICSF_ERR_ARG, "icsf_residual_vjp_device: NULL argument");
     RC(geometryAdjointSupported(c, gSf, gCf));
-    return vjpDevice(c, W, v, gW, gSf, gCf);
+    return vjpDevice(c, W, v, gW, gSf, gCf, gSf || gCf);
 }
 
 int icsf_residual_vjp(icsf_case* c, const double* W, const double* v, double* gW, double* gSf, double* gCf) {
@@ -1848,7 +1814,7 @@ int icsf_residual_vjp(icsf_case* c, const double* W, const double* v, double* gW
     if (W) RC(tmp.upload(dW, W, n5));
     RC(tmp.upload(dv, v, n5));
     RC(tmp.alloc(dg, n5));
-    RC(vjpDevice(c, dW, dv, dg, nullptr, nullptr));
+    RC(vjpDevice(c, dW, dv, dg, nullptr, nullptr, gSf || gCf));
     HIPCHK(hipMemcpy(gW, dg, n5 * sizeof(double), hipMemcpyDeviceToHost));
     if (gSf) HIPCHK(hipMemcpy(gSf, c->adjGS, n3 * sizeof(double), hipMemcpyDeviceToHost));
     if (gCf) HIPCHK(hipMemcpy(gCf, c->adjGC, n3 * sizeof(double), hipMemcpyDeviceToHost));
@@ -1903,7 +1869,7 @@ int icsf_residual_records_device(icsf_case* c, const double* W, int* faceBranche
     AdjView a = c->adj;
     a.fbr = faceBranches;
     launchAdjointForward(c->mv, c->ph, c->sv, a.gun, a.lim, c->st);
-    launchResidualVjp(c->mv, c->ph, c->sv, a, v, gW, c->adjGS, c->adjGC, c->st);
+    launchResidualVjp(c->mv, c->ph, c->sv, a, v, gW, c->adjGS, c->adjGC, false, c->st);
     HIPCHK(hipMemcpyAsync(limiterElement, a.tgt, nC * NFLD * sizeof(int), hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipMemcpyAsync(limiterFace, a.fst, nC * NFLD * sizeof(int), hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());
@@ -1964,7 +1930,7 @@ int icsf_residual_vjp_device_noalloc(icsf_case* c, const double* W, const double
     c->haveFields = true;
     c->haveResidual = c->assembled = false;
     launchAdjointForward(c->mv, c->ph, c->sv, a.gun, a.lim, c->st);
-    launchResidualVjp(c->mv, c->ph, c->sv, a, v, gW, gSf ? gSf : gS, gCf ? gCf : gC, c->st);
+    launchResidualVjp(c->mv, c->ph, c->sv, a, v, gW, gSf ? gSf : gS, gCf ? gCf : gC, gSf || gCf, c->st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     return takeDeviceErrors(c->st, {c->derr});
@@ -2031,6 +1997,29 @@ int icsf_get_tuning(icsf_case* c, int* params) {
     params[0] = m.bsOp; params[1] = m.bsFace; params[2] = m.bsCell; params[3] = m.bsJac; params[4] = m.mdCh;
     params[5] = c->tuned ? 1 : 0;
     params[6] = m.occOp; params[7] = m.occFace; params[8] = m.occCell; params[9] = m.occJac;
+    return ICSF_OK;
+}
+
+int icsf_set_tuning(icsf_case* c, const int* params) {
+    if (!c || !params) return fail(ICSF_ERR_ARG, "icsf_set_tuning: NULL argument");
+    const auto wg = [](int b) { return b == 64 || b == 128 || b == 256; };
+    for (int i : {0, 1, 2, 3})
+        if (!wg(params[i])) return fail(ICSF_ERR_ARG, "icsf_set_tuning: workgroup sizes are 64, 128 or 256");
+    if (params[4] != 8 && params[4] != 16 && params[4] != 32)
+        return fail(ICSF_ERR_ARG, "icsf_set_tuning: projection passes take 8, 16 or 32 vectors");
+    if (params[6] < 0 || params[6] > 1 || params[7] < 0 || params[7] > 2 || params[8] < 0 || params[8] > 2 ||
+        params[9] < 0 || params[9] > 2)
+        return fail(ICSF_ERR_ARG, "icsf_set_tuning: register budget out of range");
+    MeshView& m = c->mv;
+    m.bsOp = params[0]; m.bsFace = params[1]; m.bsCell = params[2]; m.bsJac = params[3]; m.mdCh = params[4];
+    m.occOp = params[6]; m.occFace = params[7]; m.occCell = params[8]; m.occJac = params[9];
+    c->tuned = true;
+    // a captured step holds the old launch parameters
+    if (c->gexec) {
+        HIPCHK(hipStreamSynchronize(c->st));
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
     return ICSF_OK;
 }
 

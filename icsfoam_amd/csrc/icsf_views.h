@@ -227,6 +227,18 @@ __device__ __forceinline__ void stRow(double* p, const double (&v)[N]) {
     if (N % 2) p[N - 1] = v[N - 1];
 }
 
+// Integer rows (N a multiple of 4, 16-byte aligned): int4 accesses.
+template <int N>
+__device__ __forceinline__ void ldRowI(const int* p, int (&v)[N]) {
+    static_assert(N % 4 == 0, "rows of a multiple of four ints");
+    const int4* q = reinterpret_cast<const int4*>(p);
+#pragma unroll
+    for (int i = 0; i < N / 4; ++i) {
+        const int4 t = q[i];
+        v[4 * i] = t.x; v[4 * i + 1] = t.y; v[4 * i + 2] = t.z; v[4 * i + 3] = t.w;
+    }
+}
+
 // Slot geometry (MeshView::sgeo) of flat slot index `slot`, entry e.
 constexpr int NSG = 8;
 enum { SG_SX = 0, SG_W = 3, SG_CX = 4, SG_MAG = 7 };

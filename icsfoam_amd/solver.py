@@ -173,6 +173,17 @@ class dbnsCase:
                     op_vgpr_cap=_cap(p[6]), face_vgpr_cap=_cap(p[7]), cell_vgpr_cap=_cap(p[8]),
                     jacobian_vgpr_cap=_cap(p[9]))
 
+    def set_tuning(self, params):
+        """Pins the launch parameters (icsf_get_tuning's 10-int layout) and
+        skips the autotuner."""
+        check(lib().icsf_set_tuning(self._h, ptr(as_i32(params))))
+
+    def tuning_params(self) -> np.ndarray:
+        """The raw 10-int launch parameters (icsf_get_tuning)."""
+        p = np.zeros(10, np.int32)
+        check(lib().icsf_get_tuning(self._h, ptr(p)))
+        return p
+
     def interfaces(self):
         """interfacesUpper blocks of the boundary faces [nB, 5, 5] (non-zero
         on cyclic faces only)."""

@@ -299,11 +299,18 @@ int icsf_get_interface_blocks(icsf_case* c, double* blocks);
  * of the face-terms kernel, of the gradient and cell-residual kernels, of
  * the Jacobian rows; basis vectors per CGS projection pass; 1 if the
  * autotuner has run; then the register budget (0 the compiler's choice,
- * 1 at most 128 VGPRs for 4 waves per SIMD) of the operator, face-terms,
+ * 1 at most 128 VGPRs for 4 waves per SIMD, 2 at most 168 for 3) of the
+ * operator, face-terms,
  * gradient and Jacobian kernels.  fvSolution keyword autotune (default
  * true): the first pseudo-step times each candidate on the case's own data
  * and keeps the fastest; results are the same bits for every choice. */
 int icsf_get_tuning(icsf_case* c, int* params);
+/* Sets the launch parameters (the layout of icsf_get_tuning; params[5]
+ * ignored) and marks the case tuned, so the autotuner does not run (no
+ * counterpart in the reference; for pinning a measured choice).
+ * Workgroup sizes 64, 128 or 256; projection passes 8, 16 or 32; budgets
+ * as above (operator 0 or 1); anything else returns ICSF_ERR_ARG. */
+int icsf_set_tuning(icsf_case* c, const int* params);
 /* Replace the assembled matrix by given LDU blocks (same layout). */
 int icsf_set_jacobian(icsf_case* c, const double* diag, const double* upper, const double* lower);
 

@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 from icsfoam_amd import cases, dbnsCase
+from icsfoam_amd._lib import IcsfError
 from icsfoam_amd.mesh import prepare_case
 from oracle.oracle import OracleCase
 
@@ -145,6 +146,37 @@ def test_autotuned_launches_give_the_same_bits(form):
     for k in ("op_workgroup", "face_workgroup", "cell_workgroup", "jacobian_workgroup"):
         assert t[k] in (64, 128, 256) and u[k] == 256
     assert t["projection_chunk"] in (8, 16, 32) and u["projection_chunk"] == 16
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+@pytest.mark.parametrize("which", ["bump", "box"])
+def test_every_launch_variant_gives_the_same_bits(which, form):
+    """Every launch-parameter candidate the autotuner may pick, pinned with
+    icsf_set_tuning - workgroup sizes, the 128- and 168-VGPR budgets,
+    8/16/32 vectors per projection pass - gives the untuned case's residual
+    history and state bit for bit (AUSM+up bump: 8 reconstructed fields;
+    viscous Roe box: 7); parameters outside the candidates are refused."""
+    print(f"[launch variants {which} {form}]", flush=True)
+    kw = dict(orthogonalisation="CGS")
+    if form == "mixed":
+        kw.update(solver="FGMRES", nDirections=1, precision="mixed", innerDirections=20)
+    c = bump_small() if which == "bump" else box_small()
+    g0 = dbnsCase.from_case(_with(c, autotune=False, **kw))
+    h0 = g0.pseudo_steps(3)
+    W0 = g0.fields()["W"]
+    # [op, face, cell, jacobian workgroups, chunk, -, op, face, cell, jacobian budgets]
+    for p in ([128, 64, 128, 64, 8, 0, 1, 1, 1, 1], [64, 128, 64, 128, 32, 0, 0, 2, 2, 2],
+              [256, 256, 256, 256, 16, 0, 1, 0, 0, 0], [256, 64, 64, 256, 32, 0, 0, 1, 2, 2]):
+        g = dbnsCase.from_case(_with(c, autotune=False, **kw))
+        g.set_tuning(p)
+        np.testing.assert_array_equal(g.tuning_params()[[0, 1, 2, 3, 4, 6, 7, 8, 9]],
+                                      np.array(p)[[0, 1, 2, 3, 4, 6, 7, 8, 9]])
+        np.testing.assert_array_equal(g.pseudo_steps(3), h0)
+        np.testing.assert_array_equal(g.fields()["W"], W0)
+    for bad in ([96, 64, 64, 64, 16, 0, 0, 0, 0, 0], [64, 64, 64, 64, 12, 0, 0, 0, 0, 0],
+                [64, 64, 64, 64, 16, 0, 2, 0, 0, 0], [64, 64, 64, 64, 16, 0, 0, 0, 3, 0]):
+        with pytest.raises(IcsfError):
+            g0.set_tuning(bad)
 
 
 @pytest.mark.parametrize("form", ["fp64", "mixed"])
