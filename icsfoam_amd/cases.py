@@ -118,6 +118,21 @@ class BoundarySpec:
 
 
 @dataclasses.dataclass
+class MRFZone:
+    """One rotating zone (the reference's MRFProperties entry, read by
+    MRFCoupledZone): cells of the zone (bool [nC]), origin, axis (normalised
+    on use), omega [rad/s], and the patches that do not rotate with it
+    (nonRotatingPatches, patch indices).  The zone's other non-empty,
+    non-coupled patch faces rotate with the frame: their U must be
+    fixedValue, and the frame sets it to Omega ^ (Cf - origin)."""
+    cells: np.ndarray
+    origin: np.ndarray
+    axis: np.ndarray
+    omega: float
+    non_rotating: List[int] = dataclasses.field(default_factory=list)
+
+
+@dataclasses.dataclass
 class Case:
     name: str
     mesh: PolyMesh
@@ -129,6 +144,7 @@ class Case:
     fvSchemes: str
     fvSolution: str
     cell_order: Optional[np.ndarray] = None   # new -> original cell index
+    mrf: Optional[MRFZone] = None
 
 
 # --------------------------------------------------------------------------
@@ -503,9 +519,10 @@ def permute_case(case: Case, new_to_old: np.ndarray, renumber) -> Case:
     renumbering routine and carry the cell fields along."""
     mesh = renumber(case.mesh, new_to_old)
     order = new_to_old if case.cell_order is None else case.cell_order[new_to_old]
+    mrf = None if case.mrf is None else dataclasses.replace(case.mrf, cells=case.mrf.cells[new_to_old])
     return dataclasses.replace(case, mesh=mesh, rho=case.rho[new_to_old],
                                U=case.U[new_to_old], T=case.T[new_to_old],
-                               cell_order=order)
+                               cell_order=order, mrf=mrf)
 
 
 # --------------------------------------------------------------------------

@@ -380,6 +380,13 @@ struct Case {
     LduBlock SS[2][2], SV[2], VS[2], VV;
     vec source;  // [5][nC]
     vec rD;      // LU-SGS reciprocal scalar diagonal
+    // MRF, one rotating zone (MRFCoupledZone, MRFCoupledZone.C:42-675):
+    // umrf[f] = faceU & Sf/|Sf| (zero outside the zone), omegaC[3c] = Omega
+    // in zone cells (zero elsewhere), mrfIncl[b] = 1 on the boundary faces
+    // whose U the frame sets (correctBoundaryVelocity), to mrfUb[3b]
+    bool mrf = false;
+    vec umrf, omegaC, mrfUb;
+    std::vector<char> mrfIncl;
 };
 
 inline double gammaOf(const Case& c) { return c.s.gamma; }
@@ -453,6 +460,10 @@ void evalBoundary(Case& k, bool init) {
                     ub = Uc;
                 }
             } else ub = Uc;
+            // MRFCoupledZone::correctBoundaryVelocity: the frame's velocity
+            // Omega ^ (Cf - origin) on the rotating patches (all fixedValue),
+            // assigned at the start of every outer iteration
+            if (k.mrf && k.mrfIncl[b]) ub = v3(&k.mrfUb[3 * b]);
             k.UB[3 * b] = ub.x; k.UB[3 * b + 1] = ub.y; k.UB[3 * b + 2] = ub.z;
             // T
             if (bc.tT == BC_FIXED) k.TB[b] = bc.tv[i];
@@ -664,10 +675,14 @@ void derive(const Case& k, Derived& d) {
 
 // ---------------------------------------------------------------- fluxes --
 // AUSM+up (ausmPlusUpFluxScheme.C:73-299) on one face.
+// um: the face's MRF normal velocity (ausmPlusUpFluxScheme.C:101-102, 293):
+// the normal velocities relative to the frame, and p_1/2 um |S| in the
+// energy flux.
 void ausmFace(const Case& k, V3 S, double magS, V3 UL, V3 UR, double cL, double cR, double pL,
-              double pR, double rL, double rR, double HL, double HR, double& phi, V3& phiU, double& phiE) {
-    const double unL = dot(UL, S) / magS;
-    const double unR = dot(UR, S) / magS;
+              double pR, double rL, double rR, double HL, double HR, double um, double& phi, V3& phiU,
+              double& phiE) {
+    const double unL = dot(UL, S) / magS - um;
+    const double unR = dot(UR, S) / magS - um;
     const double cl = sqr(cL) / std::max(cL, unL);
     const double cr = sqr(cR) / std::max(cR, -unR);
     const double cf = std::min(cl, cr);
@@ -705,7 +720,7 @@ void ausmFace(const Case& k, V3 S, double magS, V3 UL, V3 UR, double cL, double 
     const double rhoah = rhoa * (up ? HL : HR);
     phi = rhoa * magS;
     phiU = rhoaU * magS + p12 * S;
-    phiE = rhoah * magS;
+    phiE = rhoah * magS + p12 * um * magS;
 }
 
 struct RoeDiss {
@@ -767,8 +782,11 @@ RoeDiss roeDissipation(double rho, V3 Vel, double c, double gI, double u, V3 n, 
 }
 
 // roeFluxScheme::calcFlux (roeFluxScheme.C:276-409), one face.
+// um: the face's MRF normal velocity (roeFluxScheme.C:361-363, 404-408): the
+// Roe-averaged normal velocity of the dissipation relative to the frame,
+// and -1/2 |S| um (Q_l + Q_r) after the central flux.
 void roeFace(const Case& k, double gI, V3 S, double magS, double rl, double rr, double pl, double pr, V3 Ul, V3 Ur,
-             double El, double Er, double Hl, double Hr, double& phi, V3& phiU, double& phiE) {
+             double El, double Er, double Hl, double Hr, double um, double& phi, V3& phiU, double& phiE) {
     const V3 n = S / magS;
     const double coefR = std::sqrt(std::max(VSMALL, rr) / std::max(VSMALL, rl));
     const double rhoRoe = coefR * rl;
@@ -776,7 +794,7 @@ void roeFace(const Case& k, double gI, V3 S, double magS, double rl, double rr, 
     const double HRoe = (coefR * Hr + Hl) / (coefR + 1.0);
     const double cRoe = std::sqrt(std::fabs((gI - 1.0) * (HRoe - 0.5 * magSqr(URoe))));
     const double uMl = dot(Ul, n), uMr = dot(Ur, n);
-    const double uProj = dot(URoe, n);
+    const double uProj = dot(URoe, n) - um;
     const V3 rUl = rl * Ul, rUr = rr * Ur;
     const double rEl = rl * El, rEr = rr * Er;
     RoeDiss D = roeDissipation(rhoRoe, URoe, cRoe, gI, uProj, n, k.s.entropyFix);
@@ -790,20 +808,26 @@ void roeFace(const Case& k, double gI, V3 S, double magS, double rl, double rr, 
     phi += 0.5 * magS * (rnl + rnr);
     phiU = phiU + 0.5 * magS * (rnl * Ul + rnr * Ur + n * (pl + pr));
     phiE += 0.5 * magS * (rnl * Hl + rnr * Hr);
+    const double hm = 0.5 * magS * um;
+    phi -= hm * (rl + rr);
+    phiU = phiU - hm * (rUl + rUr);
+    phiE -= hm * (rEl + rEr);
 }
 
 // hllcFluxScheme::calcFlux (hllcFluxScheme.C:70-240), one face; wave
 // speeds from Roe averages, switches pos(x) = x > 0, neg(x) = x < 0.
+// um: the face's MRF normal velocity (hllcFluxScheme.C:157-161, 217-218):
+// normal velocities relative to the frame, p* um in the star energy fluxes.
 void hllcFace(const Case& k, double gI, V3 S, double magS, double rl, double rr, double pl, double pr, V3 Ul,
-              V3 Ur, double cl, double cr, double El, double Er, double Hl, double Hr, double& phi, V3& phiU,
-              double& phiE) {
+              V3 Ur, double cl, double cr, double El, double Er, double Hl, double Hr, double um, double& phi,
+              V3& phiU, double& phiE) {
     const V3 n = S / magS;
     const double coefR = std::sqrt(std::max(VSMALL, rr) / std::max(VSMALL, rl));
     const V3 uAvg = (coefR * Ur + Ul) / (coefR + 1.0);
     const double HAvg = (coefR * Hr + Hl) / (coefR + 1.0);
     (void)k;
     const double cAvg = std::sqrt(std::fabs((gI - 1.0) * (HAvg - 0.5 * magSqr(uAvg))));
-    const double ul = dot(Ul, n), ur = dot(Ur, n), ua = dot(uAvg, n);
+    const double ul = dot(Ul, n) - um, ur = dot(Ur, n) - um, ua = dot(uAvg, n) - um;
     const double Sl = std::min(ul - cl, ua - cAvg);
     const double Sr = std::max(ur + cr, ua + cAvg);
     const double Sm = (rr * ur * (Sr - ur) - rl * ul * (Sl - ul) + pl - pr) / (rr * (Sr - ur) - rl * (Sl - ul));
@@ -821,8 +845,8 @@ void hllcFace(const Case& k, double gI, V3 S, double magS, double rl, double rr,
     phiU = (cSl * (rl * ul * Ul + pl * n) + cSlm * fUl + cSmr * fUr + cSr * (rr * ur * Ur + pr * n)) * magS;
     const double rEsl = 1.0 / (Sl - Sm) * ((Sl - ul) * (rl * El) - pl * ul + psl * Sm);
     const double rEsr = 1.0 / (Sr - Sm) * ((Sr - ur) * (rr * Er) - pr * ur + psr * Sm);
-    const double fEl = Sm * (rEsl + psl);
-    const double fEr = Sm * (rEsr + psr);
+    const double fEl = Sm * (rEsl + psl) + psl * um;
+    const double fEr = Sm * (rEsr + psr) + psr * um;
     phiE = (cSl * rl * Hl * ul + cSlm * fEl + cSmr * fEr + cSr * rr * Hr * ur) * magS;
 }
 
@@ -861,14 +885,16 @@ void calcFlux(Case& k, vec& phi, vec& phiU, vec& phiE) {
         // fvc::interpolate(gamma): exact on internal faces (w (g - g) + g),
         // w g + (1 - w) g on cyclic faces
         const double gI = m.cyclicFace(f) ? interpCyclic(m, f, k.s.gamma, k.s.gamma) : k.s.gamma;
+        const double um = k.mrf ? k.umrf[f] : 0.0;
         if (k.s.flux == FLUX_AUSM)
-            ausmFace(k, S, m.magSf[f], UL, UR, cL[f], cR[f], pL[f], pR[f], rL[f], rR[f], HL[f], HR[f], ph, pu, pe);
+            ausmFace(k, S, m.magSf[f], UL, UR, cL[f], cR[f], pL[f], pR[f], rL[f], rR[f], HL[f], HR[f], um, ph, pu,
+                     pe);
         else if (k.s.flux == FLUX_ROE)
-            roeFace(k, gI, S, m.magSf[f], rL[f], rR[f], pL[f], pR[f], UL, UR, EL[f], ER[f], HL[f], HR[f], ph, pu,
-                    pe);
+            roeFace(k, gI, S, m.magSf[f], rL[f], rR[f], pL[f], pR[f], UL, UR, EL[f], ER[f], HL[f], HR[f], um, ph,
+                    pu, pe);
         else
             hllcFace(k, gI, S, m.magSf[f], rL[f], rR[f], pL[f], pR[f], UL, UR, cL[f], cR[f], EL[f], ER[f], HL[f],
-                     HR[f], ph, pu, pe);
+                     HR[f], um, ph, pu, pe);
         phi[f] = ph;
         if (f >= m.nI && !m.cyclicFace(f)) k.phiB[f - m.nI] = ph;
         phiU[3 * f] = pu.x; phiU[3 * f + 1] = pu.y; phiU[3 * f + 2] = pu.z;
@@ -1016,6 +1042,17 @@ void residual(Case& k, vec& b) {
         b[nC + c] = RE[c] * m.V[c];
         for (int d = 0; d < 3; ++d) b[(2 + d) * (size_t)nC + c] = RU[3 * c + d] * m.V[c];
     }
+    // MRF: the momentum source -(Omega ^ rho U) V that addMRFSource adds to
+    // the coupled system's source before every solve
+    // (convectiveFluxScheme.C:123-129)
+    if (k.mrf)
+        for (int c = 0; c < nC; ++c) {
+            const V3 rU{k.rho[c] * k.U[3 * c], k.rho[c] * k.U[3 * c + 1], k.rho[c] * k.U[3 * c + 2]};
+            const V3 s = cross(v3(&k.omegaC[3 * c]), rU) * m.V[c];
+            b[2 * (size_t)nC + c] -= s.x;
+            b[3 * (size_t)nC + c] -= s.y;
+            b[4 * (size_t)nC + c] -= s.z;
+        }
 }
 
 // setCoAndDeltaT.H:39-142 (local time stepping): rPseudoDeltaT.
@@ -1027,7 +1064,7 @@ void localDeltaT(Case& k) {
     for (int f = 0; f < m.nI; ++f) {
         const double cf = interpLinear(m, f, d.c.data(), 1);
         V3 Uf = m.w[f] * (v3(&k.U[3 * m.own[f]]) - v3(&k.U[3 * m.nei[f]])) + v3(&k.U[3 * m.nei[f]]);
-        const double lam = cf + std::fabs(dot(Uf, v3(&m.Sf[3 * f]) / m.magSf[f]));
+        const double lam = cf + std::fabs(dot(Uf, v3(&m.Sf[3 * f]) / m.magSf[f]) - (k.mrf ? k.umrf[f] : 0.0));
         const double frdt = m.nodc[f] * lam;
         k.rDT[m.own[f]] = std::max(k.rDT[m.own[f]], frdt);
         k.rDT[m.nei[f]] = std::max(k.rDT[m.nei[f]], frdt);
@@ -1040,7 +1077,7 @@ void localDeltaT(Case& k) {
                 const int b = f - m.nI, c = m.own[f];
                 const double cf = interpCyclic(m, f, d.c[c], d.c[m.cycCell[b]]);
                 const V3 Uf = interpCyclicU(k, f);
-                const double lam = cf + std::fabs(dot(Uf, v3(&m.Sf[3 * f]) / m.magSf[f]));
+                const double lam = cf + std::fabs(dot(Uf, v3(&m.Sf[3 * f]) / m.magSf[f]) - (k.mrf ? k.umrf[f] : 0.0));
                 k.rDT[c] = std::max(k.rDT[c], m.nodc[f] * lam);
             }
             continue;
@@ -1048,7 +1085,9 @@ void localDeltaT(Case& k) {
         if (m.pKind[pi] != PK_WALL) continue;
         for (int f = m.pStart[pi]; f < m.pStart[pi] + m.pSize[pi]; ++f) {
             const int c = m.own[f];
-            const double pl = 0.5 * m.nodc[f] * (d.c[c] + std::fabs(dot(v3(&k.U[3 * c]), v3(&m.Sf[3 * f]) / m.magSf[f])));
+            const double pl = 0.5 * m.nodc[f] *
+                              (d.c[c] + std::fabs(dot(v3(&k.U[3 * c]), v3(&m.Sf[3 * f]) / m.magSf[f]) -
+                                                  (k.mrf ? k.umrf[f] : 0.0)));
             k.rDT[c] = std::max(k.rDT[c], pl);
         }
     }
@@ -1208,20 +1247,53 @@ void assemble(Case& k) {
         };
     };
     insert(k.SS[1][1], 1, enByRhoE(true), enByRhoE(false));
-    // Dissipation: lambdaConv = interp(c) + |interp(U).n|
+    // MRF: fvj::div(w, um |S|) (blockFvOperatorsTemplates.C:148-200: upper
+    // sf (1 - w), lower -sf w, negSumDiag, coupled patches diag -=
+    // interfacesLower) subtracted from the rho, rhoU (times I) and rhoE
+    // diagonal sub-blocks (convectiveFluxScheme.C:477-481)
+    if (k.mrf) {
+        LduBlock mx;
+        mx.init(nC, nI, 1);
+        for (int f = 0; f < nI; ++f) {
+            const double sf = k.umrf[f] * m.magSf[f];
+            mx.upper[f] = sf * (1 - m.w[f]);
+            mx.lower[f] = -sf * m.w[f];
+        }
+        negSumDiag(m, mx);
+        if (m.anyCyclic) {
+            mx.iu.assign((size_t)nB, 0.0);
+            for (int f = nI; f < m.nF; ++f) {
+                if (!m.cyclicFace(f)) continue;
+                const double sf = k.umrf[f] * m.magSf[f];
+                mx.iu[f - nI] = sf * (1 - m.w[f]);
+                mx.diag[m.own[f]] -= -sf * m.w[f];
+            }
+        }
+        LduBlock mt;
+        mt.init(nC, nI, 9);
+        auto toI = [](const vec& s, vec& t) {
+            for (size_t i = 0; i < s.size(); ++i) t[9 * i] = t[9 * i + 4] = t[9 * i + 8] = s[i];
+        };
+        toI(mx.diag, mt.diag); toI(mx.upper, mt.upper); toI(mx.lower, mt.lower);
+        if (!mx.iu.empty()) { mt.iu.assign(9 * mx.iu.size(), 0.0); toI(mx.iu, mt.iu); }
+        addBlockTo(k.SS[0][0], mx, -1.0);
+        addBlockTo(k.VV, mt, -1.0);
+        addBlockTo(k.SS[1][1], mx, -1.0);
+    }
+    // Dissipation: lambdaConv = interp(c) + |interp(U).n - um|
     vec lamI(nI), lamB(nB, 0);
     for (int f = 0; f < nI; ++f) {
         const double cf = interpLinear(m, f, d.c.data(), 1);
         V3 Uf = m.w[f] * (v3(&k.U[3 * m.own[f]]) - v3(&k.U[3 * m.nei[f]])) + v3(&k.U[3 * m.nei[f]]);
-        lamI[f] = cf + std::fabs(dot(Uf, nrm(f)));
+        lamI[f] = cf + std::fabs(dot(Uf, nrm(f)) - (k.mrf ? k.umrf[f] : 0.0));
     }
     for (int f = nI; f < m.nF; ++f) {
         if (m.pKind[m.facePatch[f - nI]] == PK_EMPTY) continue;
         if (m.cyclicFace(f)) {
             const double cf = interpCyclic(m, f, d.c[m.own[f]], d.c[m.cycCell[f - nI]]);
-            lamB[f - nI] = cf + std::fabs(dot(interpCyclicU(k, f), nrm(f)));
+            lamB[f - nI] = cf + std::fabs(dot(interpCyclicU(k, f), nrm(f)) - (k.mrf ? k.umrf[f] : 0.0));
         } else {
-            lamB[f - nI] = d.cb[f - nI] + std::fabs(dot(v3(&k.UB[3 * (f - nI)]), nrm(f)));
+            lamB[f - nI] = d.cb[f - nI] + std::fabs(dot(v3(&k.UB[3 * (f - nI)]), nrm(f)) - (k.mrf ? k.umrf[f] : 0.0));
         }
     }
     auto dissip = [&](LduBlock& target, int nc) {
@@ -1279,7 +1351,8 @@ void assemble(Case& k) {
             else uVIC = V3{1, 1, 1};
             const double rhoB = k.rhoB[b], pB = k.pB[b], TB = k.TB[b];
             V3 UB = v3(&k.UB[3 * b]);
-            const double UrelSf = dot(UB, Sf);
+            double UrelSf = dot(UB, Sf);
+            if (k.mrf) UrelSf -= k.umrf[f] * m.magSf[f];
             const double rhoEB = rhoB * (k.Cv * TB + 0.5 * magSqr(UB));
             const double cvB = k.Cv;
             V3 SfV{Sf.x * uVIC.x, Sf.y * uVIC.y, Sf.z * uVIC.z};
@@ -1353,6 +1426,19 @@ void assemble(Case& k) {
         k.VV.diag[9 * c] += dc; k.VV.diag[9 * c + 4] += dc; k.VV.diag[9 * c + 8] += dc;
         k.SS[1][1].diag[c] += dc;
     }
+    // addMRFSource (convectiveFluxScheme.C:123-138): the derivative of
+    // -(Omega ^ rhoU) V on the rhoU diagonal block
+    if (k.mrf)
+        for (int c = 0; c < nC; ++c) {
+            const double* om = &k.omegaC[3 * c];
+            double* t = &k.VV.diag[9 * c];
+            t[1] -= om[2] * m.V[c];
+            t[2] += om[1] * m.V[c];
+            t[3] += om[2] * m.V[c];
+            t[5] -= om[0] * m.V[c];
+            t[6] -= om[1] * m.V[c];
+            t[7] += om[0] * m.V[c];
+        }
     // viscous LF Jacobian: laplacian(0.5*lambdaVisc, 1) subtracted
     if (k.s.viscous) {
         const double mu = k.s.mu, alpha = k.s.gamma * mu / k.s.Pr;
@@ -2457,6 +2543,73 @@ void* orc_case_new(void* mesh, const int* is, const double* rs) {
 }
 void orc_case_free(void* h) { delete (Case*)h; }
 
+// One MRF zone (MRFCoupledZone.C:42-189 setMRFFaces, :240-296 constructor,
+// :549-577 faceU, :580-602 correctBoundaryVelocity): zone cells, origin,
+// axis (normalised), omega [rad/s], non-rotating patches.  Face types: 1
+// internal faces with a zone cell on either side and the zone cells' faces
+// on other non-empty, non-coupled patches; 2 the zone cells' faces on
+// coupled or non-rotating patches; 0 the rest.  faceU = Omega ^ (Cf -
+// origin) on types 1 and 2; the rotating patches (type 1 boundary faces)
+// take U = faceU.  Returns 1 if a rotating patch's U is not fixedValue (the
+// frame assigns its value, which only a fixedValue condition keeps), 0 on
+// success.
+int orc_set_mrf(void* h, int nZone, const int* zone, const double* origin, const double* axis, double omega,
+                int nNonRot, const int* nonRot) {
+    Case& k = *(Case*)h;
+    const Mesh& m = *k.m;
+    const int nB = m.nF - m.nI;
+    V3 ax = v3(axis);
+    const double am = mag(ax);
+    ax = am < ROOTVSMALL ? V3{0, 0, 0} : ax / am;
+    const V3 Om = omega * ax, o = v3(origin);
+    std::vector<char> zc(m.nC, 0), excl(m.pStart.size(), 0);
+    for (int i = 0; i < nZone; ++i) zc[zone[i]] = 1;
+    for (int i = 0; i < nNonRot; ++i) excl[nonRot[i]] = 1;
+    std::vector<int> type(m.nF, 0);
+    for (int f = 0; f < m.nI; ++f)
+        if (zc[m.own[f]] || zc[m.nei[f]]) type[f] = 1;
+    for (size_t pi = 0; pi < m.pStart.size(); ++pi)
+        for (int f = m.pStart[pi]; f < m.pStart[pi] + m.pSize[pi]; ++f) {
+            if (!zc[m.own[f]]) continue;
+            if (m.pKind[pi] == PK_CYCLIC || excl[pi]) type[f] = 2;
+            else if (m.pKind[pi] != PK_EMPTY) type[f] = 1;
+        }
+    for (size_t pi = 0; pi < m.pStart.size(); ++pi)
+        for (int f = m.pStart[pi]; f < m.pStart[pi] + m.pSize[pi]; ++f)
+            if (type[f] == 1 && k.bcs[pi].uT != BC_FIXED) return 1;
+    k.mrf = true;
+    k.umrf.assign(m.nF, 0.0);
+    k.omegaC.assign(3 * (size_t)m.nC, 0.0);
+    k.mrfUb.assign(3 * (size_t)nB, 0.0);
+    k.mrfIncl.assign(nB, 0);
+    for (int f = 0; f < m.nF; ++f) {
+        V3 fu{0, 0, 0};
+        if (type[f]) fu = cross(Om, v3(&m.Cf[3 * f]) - o);
+        k.umrf[f] = dot(fu, v3(&m.Sf[3 * f]) / m.magSf[f]);
+        if (f >= m.nI && type[f] == 1) {
+            const int b = f - m.nI;
+            k.mrfIncl[b] = 1;
+            k.mrfUb[3 * b] = fu.x; k.mrfUb[3 * b + 1] = fu.y; k.mrfUb[3 * b + 2] = fu.z;
+        }
+    }
+    for (int c = 0; c < m.nC; ++c)
+        if (zc[c]) { k.omegaC[3 * c] = Om.x; k.omegaC[3 * c + 1] = Om.y; k.omegaC[3 * c + 2] = Om.z; }
+    return 0;
+}
+// The MRF set-up: umrf [nF], omegaC [3 nC], rotating boundary faces [nB]
+// and their velocity [3 nB].
+void orc_get_mrf(void* h, double* umrf, double* omegaC, int* incl, double* Ub) {
+    const Case& k = *(const Case*)h;
+    const Mesh& m = *k.m;
+    const int nB = m.nF - m.nI;
+    for (int f = 0; f < m.nF; ++f) umrf[f] = k.mrf ? k.umrf[f] : 0.0;
+    for (int i = 0; i < 3 * m.nC; ++i) omegaC[i] = k.mrf ? k.omegaC[i] : 0.0;
+    for (int b = 0; b < nB; ++b) {
+        incl[b] = k.mrf ? k.mrfIncl[b] : 0;
+        for (int d = 0; d < 3; ++d) Ub[3 * b + d] = k.mrf ? k.mrfUb[3 * b + d] : 0.0;
+    }
+}
+
 void orc_set_bc(void* h, int patch, int pT, int uT, int tT, const double* pv, const double* uv, const double* tv) {
     Case& k = *(Case*)h;
     BC& b = k.bcs[patch];
@@ -2730,8 +2883,10 @@ void orc_get_pseudo_co(void* h, double* co) {
 // Face-level flux for pinning tests: scheme 0 AUSM+up, 1 Roe.
 // in = [Sx,Sy,Sz, rhoL,pL,ULx,ULy,ULz,TL, rhoR,pR,URx,URy,URz,TR]; the
 // reconstructed E, H, c* are formed from each side's state as the cell
-// fields are.  out = [phi, phiUx, phiUy, phiUz, phiE].
-void orc_face_flux(int scheme, int lowMach, double eFix, double R, double gamma, const double* in, double* out) {
+// fields are.  out = [phi, phiUx, phiUy, phiUz, phiE].  um: the face's
+// MRF normal velocity (orc_face_flux: 0).
+void orc_face_flux_mrf(int scheme, int lowMach, double eFix, double R, double gamma, const double* in, double um,
+                       double* out) {
     Case k;
     k.s.flux = scheme; k.s.lowMach = lowMach; k.s.entropyFix = eFix; k.s.R = R; k.s.gamma = gamma;
     k.Cv = R / (gamma - 1.0);
@@ -2748,15 +2903,21 @@ void orc_face_flux(int scheme, int lowMach, double eFix, double R, double gamma,
     }
     double ph, pe;
     V3 pu;
-    if (scheme == FLUX_AUSM) ausmFace(k, S, magS, U[0], U[1], cs[0], cs[1], p[0], p[1], rho[0], rho[1], H[0], H[1], ph, pu, pe);
-    else if (scheme == FLUX_ROE) roeFace(k, gamma, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], E[0], E[1], H[0], H[1], ph, pu, pe);
+    if (scheme == FLUX_AUSM)
+        ausmFace(k, S, magS, U[0], U[1], cs[0], cs[1], p[0], p[1], rho[0], rho[1], H[0], H[1], um, ph, pu, pe);
+    else if (scheme == FLUX_ROE)
+        roeFace(k, gamma, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], E[0], E[1], H[0], H[1], um, ph, pu, pe);
     else {
         double ch[2];
         // sqrt(gamma/psi) with psi = 1/(R T)
         for (int q = 0; q < 2; ++q) ch[q] = std::max(std::sqrt(gamma / (1.0 / (R * T[q]))), VSMALL);
-        hllcFace(k, gamma, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], ch[0], ch[1], E[0], E[1], H[0], H[1], ph, pu, pe);
+        hllcFace(k, gamma, S, magS, rho[0], rho[1], p[0], p[1], U[0], U[1], ch[0], ch[1], E[0], E[1], H[0], H[1], um,
+                 ph, pu, pe);
     }
     out[0] = ph; out[1] = pu.x; out[2] = pu.y; out[3] = pu.z; out[4] = pe;
+}
+void orc_face_flux(int scheme, int lowMach, double eFix, double R, double gamma, const double* in, double* out) {
+    orc_face_flux_mrf(scheme, lowMach, eFix, R, gamma, in, 0.0, out);
 }
 
 // Reconstructed face values (L, R) of one cell field of the current state:

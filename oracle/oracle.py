@@ -51,6 +51,9 @@ _SIG = {
     "orc_residual_w": (None, [_vp, _vp, _vp, _vp, _vp]),
     "orc_residual_jvp": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "orc_residual_jac_dense": (None, [_vp, _vp, _vp, _vp, _vp]),
+    "orc_set_mrf": (_i, [_vp, _i, _vp, _vp, _vp, C.c_double, _i, _vp]),
+    "orc_face_flux_mrf": (None, [_i, _i, C.c_double, C.c_double, C.c_double, _vp, C.c_double, _vp]),
+    "orc_get_mrf": (None, [_vp, _vp, _vp, _vp, _vp]),
     "orc_hb_new": (_vp, [_i, _vp, _vp]),
     "orc_hb_free": (None, [_vp]),
     "orc_hb_residual": (None, [_vp, _vp]),
@@ -174,11 +177,13 @@ class OracleMesh:
             self.h = None
 
 
-def face_flux(scheme, S, left, right, lowMach=True, eFix=0.05, R=287.0, gamma=1.4):
-    """Flux through one face; left/right = (rho, p, Ux, Uy, Uz, T)."""
+def face_flux(scheme, S, left, right, lowMach=True, eFix=0.05, R=287.0, gamma=1.4, um=0.0):
+    """Flux through one face; left/right = (rho, p, Ux, Uy, Uz, T); um: the
+    face's MRF normal velocity (frame velocity . Sf/|Sf|)."""
     inp = np.ascontiguousarray(np.concatenate([S, left, right]), np.float64)
     out = np.empty(5)
-    lib().orc_face_flux({"AUSMPlusUp": 0, "ROE": 1, "HLLC": 2}[scheme], int(lowMach), eFix, R, gamma, _p(inp), _p(out))
+    lib().orc_face_flux_mrf({"AUSMPlusUp": 0, "ROE": 1, "HLLC": 2}[scheme], int(lowMach), eFix, R, gamma, _p(inp),
+                            float(um), _p(out))
     return out
 
 
@@ -230,7 +235,20 @@ class OracleCase:
             if bc.p_init is not None or bc.U_init is not None or bc.T_init is not None:
                 init = [None if a is None else _f64(a) for a in (bc.p_init, bc.U_init, bc.T_init)]
                 lib().orc_set_bc_values(self.h, i, *[_p(a) for a in init])
+        if case.mrf is not None:
+            self.set_mrf(case.mrf)
         self.set_fields(case.rho, case.U, case.T)
+
+    def set_mrf(self, z):
+        """One rotating zone (cases.MRFZone); before set_fields, as the
+        frame sets the rotating patches' U when the fields are created."""
+        cells = _i32(np.flatnonzero(np.asarray(z.cells, bool)))
+        nr = _i32(z.non_rotating)
+        rc = lib().orc_set_mrf(self.h, int(cells.shape[0]), _p(cells), _p(_f64(z.origin)), _p(_f64(z.axis)),
+                               float(z.omega), int(nr.shape[0]), _p(nr))
+        if rc:
+            raise ValueError("MRF: a patch that rotates with the zone needs a fixedValue U condition "
+                             "(list it in non_rotating otherwise)")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -239,6 +257,14 @@ class OracleCase:
 
     def set_fields(self, rho, U, T):
         lib().orc_set_fields(self.h, _p(_f64(rho)), _p(_f64(U)), _p(_f64(T)))
+
+    def mrf(self):
+        """The MRF set-up: face normal frame velocity [nF], Omega per cell
+        [nC, 3], rotating boundary faces [nB] and their U [nB, 3]."""
+        nF, nB = self.nFaces, self.nFaces - self.nInternal
+        um, om, inc, ub = np.empty(nF), np.empty((self.nCells, 3)), np.empty(nB, np.int32), np.empty((nB, 3))
+        lib().orc_get_mrf(self.h, _p(um), _p(om), _p(inc), _p(ub))
+        return dict(umrf=um, omega=om, rotating=inc.astype(bool), U=ub)
 
     def fields(self):
         n = self.nCells
