@@ -9,10 +9,13 @@ namespace icsf {
 
 #define DF __device__ __forceinline__
 
+// um: the face's normal frame velocity of a rotating zone (0 outside one):
+// AUSM+up transports with the normal velocities relative to the frame and
+// adds p_1/2 um |S| to the energy flux (ausmPlusUpFluxScheme.C:101-102,293).
 DF void ausmFace(bool lowMach, V3 S, double magS, V3 UL, V3 UR, double cL, double cR, double pL, double pR,
-                 double rL, double rR, double HL, double HR, double& phi, V3& phiU, double& phiE) {
-    const double unL = dot(UL, S) / magS;
-    const double unR = dot(UR, S) / magS;
+                 double rL, double rR, double HL, double HR, double um, double& phi, V3& phiU, double& phiE) {
+    const double unL = dot(UL, S) / magS - um;
+    const double unR = dot(UR, S) / magS - um;
     const double cl = sqr(cL) / fmax(cL, unL);
     const double cr = sqr(cR) / fmax(cR, -unR);
     const double cf = fmin(cl, cr);
@@ -50,14 +53,17 @@ DF void ausmFace(bool lowMach, V3 S, double magS, V3 UL, V3 UR, double cL, doubl
     const double rhoah = rhoa * (up ? HL : HR);
     phi = rhoa * magS;
     phiU = rhoaU * magS + p12 * S;
-    phiE = rhoah * magS;
+    phiE = rhoah * magS + p12 * um * magS;
 }
 
 // Roe flux on one face; the P|Lambda|P^-1 dissipation matrix is formed
 // explicitly exactly as the reference does and applied to the jump of the
 // reconstructed conserved state.
+// um: the face's normal frame velocity: the dissipation takes the
+// Roe-averaged normal velocity relative to the frame, and -1/2 |S| um
+// (Q_l + Q_r) follows the central flux (roeFluxScheme.C:361-363,404-408).
 DF void roeFace(double gI, double eFix, V3 S, double magS, double rl, double rr, double pl, double pr, V3 Ul,
-                V3 Ur, double El, double Er, double Hl, double Hr, double& phi, V3& phiU, double& phiE) {
+                V3 Ur, double El, double Er, double Hl, double Hr, double um, double& phi, V3& phiU, double& phiE) {
     const V3 n = S / magS;
     const double coefR = sqrt(fmax(kVSMALL, rr) / fmax(kVSMALL, rl));
     const double rho = coefR * rl;
@@ -65,7 +71,7 @@ DF void roeFace(double gI, double eFix, V3 S, double magS, double rl, double rr,
     const double HRoe = (coefR * Hr + Hl) / (coefR + 1.0);
     const double c = sqrt(fabs((gI - 1.0) * (HRoe - 0.5 * magSqr(Vel))));
     const double uMl = dot(Ul, n), uMr = dot(Ur, n);
-    const double u = dot(Vel, n);
+    const double u = dot(Vel, n) - um;
     const V3 rUl = rl * Ul, rUr = rr * Ur;
     const double rEl = rl * El, rEr = rr * Er;
 
@@ -125,19 +131,26 @@ DF void roeFace(double gI, double eFix, V3 S, double magS, double rl, double rr,
     phi += 0.5 * magS * (rnl + rnr);
     phiU = phiU + 0.5 * magS * (rnl * Ul + rnr * Ur + n * (pl + pr));
     phiE += 0.5 * magS * (rnl * Hl + rnr * Hr);
+    const double hm = 0.5 * magS * um;
+    phi -= hm * (rl + rr);
+    phiU = phiU - hm * (rUl + rUr);
+    phiE -= hm * (rEl + rEr);
 }
 
 // HLLC flux on one face (hllcFluxScheme.C:70-240): Roe-averaged wave
 // speeds, pos(x) = x > 0 and neg(x) = x < 0 switches, all four branch terms
 // evaluated and weighted as in the reference.
+// um: the face's normal frame velocity: relative normal velocities, p* um
+// in the star energy fluxes (hllcFluxScheme.C:157-161,217-218).
 DF void hllcFace(double gI, V3 S, double magS, double rl, double rr, double pl, double pr, V3 Ul, V3 Ur, double cl,
-                 double cr, double El, double Er, double Hl, double Hr, double& phi, V3& phiU, double& phiE) {
+                 double cr, double El, double Er, double Hl, double Hr, double um, double& phi, V3& phiU,
+                 double& phiE) {
     const V3 n = S / magS;
     const double coefR = sqrt(fmax(kVSMALL, rr) / fmax(kVSMALL, rl));
     const V3 uAvg = (coefR * Ur + Ul) / (coefR + 1.0);
     const double HAvg = (coefR * Hr + Hl) / (coefR + 1.0);
     const double cAvg = sqrt(fabs((gI - 1.0) * (HAvg - 0.5 * magSqr(uAvg))));
-    const double ul = dot(Ul, n), ur = dot(Ur, n), ua = dot(uAvg, n);
+    const double ul = dot(Ul, n) - um, ur = dot(Ur, n) - um, ua = dot(uAvg, n) - um;
     const double Sl = fmin(ul - cl, ua - cAvg);
     const double Sr = fmax(ur + cr, ua + cAvg);
     const double Sm = (rr * ur * (Sr - ur) - rl * ul * (Sl - ul) + pl - pr) / (rr * (Sr - ur) - rl * (Sl - ul));
@@ -155,8 +168,8 @@ DF void hllcFace(double gI, V3 S, double magS, double rl, double rr, double pl, 
     phiU = (cSl * (rl * ul * Ul + pl * n) + cSlm * fUl + cSmr * fUr + cSr * (rr * ur * Ur + pr * n)) * magS;
     const double rEsl = 1.0 / (Sl - Sm) * ((Sl - ul) * (rl * El) - pl * ul + psl * Sm);
     const double rEsr = 1.0 / (Sr - Sm) * ((Sr - ur) * (rr * Er) - pr * ur + psr * Sm);
-    const double fEl = Sm * (rEsl + psl);
-    const double fEr = Sm * (rEsr + psr);
+    const double fEl = Sm * (rEsl + psl) + psl * um;
+    const double fEr = Sm * (rEsr + psr) + psr * um;
     phiE = (cSl * rl * Hl * ul + cSlm * fEl + cSmr * fEr + cSr * rr * Hr * ur) * magS;
 }
 

@@ -382,8 +382,9 @@ __device__ __forceinline__ double soundSpeed(const Phys& ph, double T) {
 // uses c* instead).
 // Internal faces take the field rows of their owner o and neighbour n
 // (fo, fn), which the caller gathers once for the flux and its own terms.
+// um: the face's normal frame velocity (rotating zone; 0 otherwise).
 __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, const StateView& s, int f, int o, int n,
-                                         const double (&fo)[NFLD], const double (&fn)[NFLD], double& fr,
+                                         const double (&fo)[NFLD], const double (&fn)[NFLD], double um, double& fr,
                                          double& fe, V3& fu, double* rec) {
     const V3 S{ldnt(m.Sf + 3 * (size_t)f), ldnt(m.Sf + 3 * (size_t)f + 1), ldnt(m.Sf + 3 * (size_t)f + 2)};
     const double ms = ldnt(m.magSf + f);
@@ -429,11 +430,11 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
     rec[0] = uxL; rec[1] = uyL; rec[2] = uzL; rec[3] = eL;
     rec[4] = uxR; rec[5] = uyR; rec[6] = uzR; rec[7] = eR;
     if (ph.flux == FLUX_AUSM)
-        ausmFace(ph.lowMach, S, ms, UL, UR, xL, xR, pL, pR, rL, rR, HL, HR, fr, fu, fe);
+        ausmFace(ph.lowMach, S, ms, UL, UR, xL, xR, pL, pR, rL, rR, HL, HR, um, fr, fu, fe);
     else if (ph.flux == FLUX_ROE)
-        roeFace(gI, ph.entropyFix, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, HL, HR, fr, fu, fe);
+        roeFace(gI, ph.entropyFix, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, HL, HR, um, fr, fu, fe);
     else
-        hllcFace(gI, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, eL, eR, HL, HR, fr, fu, fe);
+        hllcFace(gI, S, ms, rL, rR, pL, pR, UL, UR, xL, xR, eL, eR, HL, HR, um, fr, fu, fe);
 }
 
 // Face terms, one thread per face (each face evaluated once):
@@ -442,10 +443,11 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
 //   FT_TAU..+2   Sf & tauMC_f                  (div(tauMC))
 //   FT_SIG       (mu_f grad(U)_f + tauMC_f) & U_f & Sf (div(sigmaDotU & Sf))
 //   FT_LAPE      |S| alpha_f snGrad(e)         (laplacian(alphaEff,e))
-//   FT_LAM       lambda_f = interp(c) + |interp(U).n| (internal) or
-//                c_b + |U_b.n| (boundary)
+//   FT_LAM       lambda_f = interp(c) + |interp(U).n - um| (internal) or
+//                c_b + |U_b.n - um| (boundary); um the normal frame
+//                velocity of a rotating zone, else 0
 //   FT_FRDT      nonOrthDeltaCoeffs*lambda (internal), wall-face
-//                0.5*nonOrthDeltaCoeffs*(c_P + |U_P.n|), else -1
+//                0.5*nonOrthDeltaCoeffs*(c_P + |U_P.n - um|), else -1
 //   FT_REC..+7   reconstructed Ux,Uy,Uz,E on the owner side, then the
 //                neighbour side (internal faces)
 //   FT_SV        0.5*lambdaVisc*|S|*deltaCoeffs (internal, viscous)
@@ -464,7 +466,8 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
         ldRow(s.fld + (size_t)ow * NFLD, fo);
         ldRow(s.fld + (size_t)ne * NFLD, fn);
     }
-    faceFlux(m, ph, s, f, ow, ne, fo, fn, fr, fe, fu, rec);
+    const double um = m.umrf ? m.umrf[f] : 0.0;
+    faceFlux(m, ph, s, f, ow, ne, fo, fn, um, fr, fe, fu, rec);
     o[FT_CONV] = fr;
     o[FT_CONV + 1] = fe;
     o[FT_CONV + 2] = fu.x;
@@ -480,7 +483,7 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
         const double cf = w * (co - cn) + cn;
         const V3 Uo{fo[F_UX], fo[F_UY], fo[F_UZ]}, Un{fn[F_UX], fn[F_UY], fn[F_UZ]};
         const V3 Uf = w * (Uo - Un) + Un;
-        const double lam = cf + fabs(dot(Uf, S / ms));
+        const double lam = cf + fabs(dot(Uf, S / ms) - um);
         o[FT_LAM] = lam;
         o[FT_FRDT] = m.nodc[f] * lam;
 #pragma unroll
@@ -524,7 +527,7 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
         const double cf = w * soundSpeed(ph, s.T[ow]) + (1.0 - w) * soundSpeed(ph, s.T[nc]);
         const V3 Un = rot3(m.cycR + 9 * (size_t)kc, cellU(s.fld, nc));
         const V3 Uf = w * cellU(s.fld, ow) + (1.0 - w) * Un;
-        const double lam = cf + fabs(dot(Uf, S / ms));
+        const double lam = cf + fabs(dot(Uf, S / ms) - um);
         o[FT_LAM] = lam;
         o[FT_FRDT] = m.nodc[f] * lam;
 #pragma unroll
@@ -535,9 +538,9 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
         if (!(s.halt && *s.halt)) s.bcPhi[b] = fr;  // the registered phi the derived conditions read
         const double* bf = s.bfld + (size_t)b * NFLD;
         const V3 UB{bf[F_UX], bf[F_UY], bf[F_UZ]};
-        o[FT_LAM] = soundSpeed(ph, s.bT[b]) + fabs(dot(UB, S / ms));
+        o[FT_LAM] = soundSpeed(ph, s.bT[b]) + fabs(dot(UB, S / ms) - um);
         o[FT_FRDT] = m.bKind[b] == PK_WALL
-                         ? 0.5 * m.nodc[f] * (soundSpeed(ph, s.T[c]) + fabs(dot(cellU(s.fld, c), S / ms)))
+                         ? 0.5 * m.nodc[f] * (soundSpeed(ph, s.T[c]) + fabs(dot(cellU(s.fld, c), S / ms) - um))
                          : -1.0;
         if (visc) {
             const V3 n = S / ms;
@@ -638,6 +641,16 @@ struct CellSums {
         s.b[2 * (size_t)nC + c] = Ru.x * Vc;
         s.b[3 * (size_t)nC + c] = Ru.y * Vc;
         s.b[4 * (size_t)nC + c] = Ru.z * Vc;
+        if (m.omC) {
+            // rotating zone: -(Omega ^ rho U) V, which addMRFSource adds to
+            // the coupled system's source (convectiveFluxScheme.C:123-129)
+            const double rho = s.fld[(size_t)c * NFLD + F_RHO];
+            const V3 U = cellU(s.fld, c);
+            const V3 sc = cross(ld3(m.omC + 3 * (size_t)c), V3{rho * U.x, rho * U.y, rho * U.z}) * Vc;
+            s.b[2 * (size_t)nC + c] -= sc.x;
+            s.b[3 * (size_t)nC + c] -= sc.y;
+            s.b[4 * (size_t)nC + c] -= sc.z;
+        }
         const double r = rdt / s.pseudoCo[c];
         s.rDT[c] = r;
         return r;
@@ -718,6 +731,12 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
     const int nLo = m.nLower[c];
     CellSums cs;
     int slot = 0;
+    // rotating zone: fvj::div(w, um |S|) (upper sf (1 - w), lower -sf w,
+    // its own negSumDiag mrfD, coupled faces diag -= -sf w), subtracted
+    // from the rho, rhoU and rhoE diagonal sub-blocks after the central part
+    // (convectiveFluxScheme.C:477-481, blockFvOperatorsTemplates.C:148-200)
+    const bool mrf = m.umrf != nullptr;
+    double mrfD = 0.0;
     for (int q = e0; q < e1; ++q) {
         const int f = m.cf[q];
         // the face record, gathered whole (12 128-bit loads)
@@ -738,17 +757,22 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
             fluxJacobian(gf, V3{t[FT_REC + 4], t[FT_REC + 5], t[FT_REC + 6]}, t[FT_REC + 7], nrm, A);
             const double hb = 0.5 * ms;
             const double v = 0.5 * ms * t[FT_LAM];
+            const double sf = mrf ? m.umrf[f] * ms : 0.0;
             TB* ib = blk + m.cycBase + (size_t)kc * NB;
 #pragma unroll
             for (int e = 0; e < NB; ++e) {
                 double x = hb * A[e];
-                if (e == 0 || e == 6 || e == 12 || e == 18 || e == 24) x = x - v;
+                if (e == 0 || e == 6 || e == 12 || e == 18 || e == 24) {
+                    if (mrf) x = x - sf * (1 - m.w[f]);
+                    x = x - v;
+                }
                 ib[e] = (TB)x;
             }
             fluxJacobian(gf, V3{t[FT_REC], t[FT_REC + 1], t[FT_REC + 2]}, t[FT_REC + 3], nrm, A);
             const double hd = -0.5 * ms;
 #pragma unroll
             for (int e = 0; e < NB; ++e) D[e] -= hd * A[e];
+            if (mrf) mrfD -= -sf * m.w[f];
             continue;
         }
         const size_t sk = rs + 64 * (size_t)slot;
@@ -765,6 +789,16 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
         const double hb = isOwn ? 0.5 * ms : -0.5 * ms;
         const double v = 0.5 * ms * t[FT_LAM];
         const double sv = visc ? t[FT_SV] : 0.0;
+        // the frame divergence: upper sf (1 - w) in the owner's row, lower
+        // -sf w in the neighbour's; negSumDiag gives the owner -lower and
+        // the neighbour -upper
+        double mo = 0.0, md = 0.0;
+        if (mrf) {
+            const double sf = m.umrf[f] * ms, w = ldnt(m.sgeo + sgOff(sk, SG_W));
+            const double up = sf * (1 - w), lo = -sf * w;
+            mo = isOwn ? up : lo;
+            md = isOwn ? lo : up;
+        }
         if (m.blkGen) {
             // generators: the solver kernels rebuild h A(U, E, n) - v I - sv I
             const size_t gb = ((rowSlot0 + slot) * NGEN) * 64 + lane;
@@ -778,6 +812,7 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
             for (int e = 0; e < NB; ++e) {
                 double x = hb * A[e];
                 if (e == 0 || e == 6 || e == 12 || e == 18 || e == 24) {
+                    if (mrf) x = x - mo;
                     x = x - v;
                     if (visc) x = x - sv;
                 }
@@ -788,10 +823,13 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
         const double hd = isOwn ? -0.5 * ms : 0.5 * ms;
 #pragma unroll
         for (int e = 0; e < NB; ++e) D[e] -= hd * A[e];
+        // negSumDiag of the frame divergence, in face order
+        if (mrf) mrfD -= md;
         diss -= v;
         if (visc) vis -= sv;
         ++slot;
     }
+    if (mrf) { D[0] = D[0] - mrfD; D[6] = D[6] - mrfD; D[12] = D[12] - mrfD; D[18] = D[18] - mrfD; D[24] = D[24] - mrfD; }
     double rDTc = 0.0;
     if (RES) rDTc = cs.store(m, s, c, visc);
     if constexpr (HBC && RES) {
@@ -838,7 +876,8 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
         const double* bf = s.bfld + (size_t)b * NFLD;
         const double rhoB = bf[F_RHO], pB = bf[F_P], TB_ = s.bT[b];
         const V3 UB{bf[F_UX], bf[F_UY], bf[F_UZ]};
-        const double UrelSf = dot(UB, Sf);
+        double UrelSf = dot(UB, Sf);
+        if (mrf) UrelSf -= m.umrf[f] * m.magSf[f];  // convectiveFluxScheme.C:94
         const double rhoEB = rhoB * (Cv * TB_ + 0.5 * magSqr(UB));
         const V3 SfV{Sf.x * uVIC.x, Sf.y * uVIC.y, Sf.z * uVIC.z};
         const double dCdp = rhoB / pB * UrelSf * pVIC;
@@ -911,6 +950,14 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
         const double ddtCoeff = ((RES ? rDTc : s.rDT[c]) * Vc) / Vc;
         const double dcv = ddtCoeff * Vc;
         D[0] += dcv; D[12] += dcv; D[18] += dcv; D[24] += dcv; D[6] += dcv;
+    }
+    if (m.omC) {
+        // addMRFSource (convectiveFluxScheme.C:130-137): the Coriolis term's
+        // derivative on the rhoU block, entries (row, col) xy xz yx yz zx zy
+        const double Vc = m.V[c];
+        const V3 om = ld3(m.omC + 3 * (size_t)c);
+        D[13] -= om.z * Vc; D[14] += om.y * Vc; D[17] += om.z * Vc;
+        D[19] -= om.x * Vc; D[22] -= om.y * Vc; D[23] += om.x * Vc;
     }
     if (visc) { D[0] = D[0] - vis; D[12] = D[12] - vis; D[18] = D[18] - vis; D[24] = D[24] - vis; D[6] = D[6] - vis; }
     if constexpr (HBC) {

@@ -156,6 +156,12 @@ struct icsf_case {
     std::vector<double> bInit;
     double* d_bInit = nullptr;
     bool bcDirty = true;
+    // rotating zone (icsf_set_mrf): the boundary faces that rotate with it
+    // and their frame velocity (their fixedValue U is set to it)
+    bool mrf = false;
+    std::vector<char> mrfIncl;
+    std::vector<double> mrfUb;
+    double *d_umrf = nullptr, *d_omC = nullptr;
     MeshView mv{};
     StateView sv{};
     // residual adjoint (allocated on first use)
@@ -857,6 +863,7 @@ int fgmresSolve(icsf_case* c, const double* W, const double* b, int& nIter) {
 namespace {
 
 int adjointSupported(const icsf_case* c) {
+    if (c->mrf) return fail(ICSF_ERR_ARG, "residual VJP: cases with an MRF zone are not supported");
     for (size_t b = 0; b < c->bPT.size(); ++b)
         if (c->bPT[b] == BC_TOTAL_P || c->bUT[b] == BC_PIOV || c->bTT[b] == BC_TOTAL_T)
             return fail(ICSF_ERR_ARG, "residual VJP: totalPressure, pressureInletOutletVelocity and totalTemperature "
@@ -1676,6 +1683,11 @@ int icsf_set_boundary(icsf_case* c, int patch, int pType, int UType, int TType, 
     }
     if ((pType == BC_TOTAL_P || TType == BC_TOTAL_T) && !(c->ph.gamma > 1))
         return fail(ICSF_ERR_ARG, "totalPressure / totalTemperature need gamma > 1");
+    if (c->mrf)
+        for (int i = 0; i < n; ++i)
+            if (c->mrfIncl[s + i] && UType != BC_FIXED)
+                return fail(ICSF_ERR_ARG, "patch " + std::to_string(patch) + " rotates with the MRF zone: its U "
+                                          "condition must be fixedValue");
     for (int i = 0; i < n; ++i) {
         c->bPT[s + i] = pType;
         c->bUT[s + i] = UType;
@@ -1683,8 +1695,89 @@ int icsf_set_boundary(icsf_case* c, int patch, int pType, int UType, int TType, 
         if (pv) c->bPv[s + i] = pValue[i];
         for (int d = 0; d < 3; ++d) c->bUv[3 * (s + i) + d] = uIn ? UValue[3 * i + d] : 0.0;
         if (tv) c->bTv[s + i] = TValue[i];
+        // the zone's rotating patches take the frame velocity
+        // (MRFCoupledZone::correctBoundaryVelocity, every outer iteration)
+        if (c->mrf && c->mrfIncl[s + i])
+            for (int d = 0; d < 3; ++d) c->bUv[3 * (s + i) + d] = c->mrfUb[3 * (s + i) + d];
     }
     c->bcDirty = true;
+    return ICSF_OK;
+}
+
+int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const double* origin, const double* axis,
+                 double omega, int nNonRotating, const int* nonRotatingPatches) {
+    if (!c || (nZoneCells > 0 && !zoneCells) || !origin || !axis || (nNonRotating > 0 && !nonRotatingPatches))
+        return fail(ICSF_ERR_ARG, "icsf_set_mrf: NULL argument");
+    if (c->mrf) return fail(ICSF_ERR_STATE, "icsf_set_mrf: the case already has an MRF zone");
+    if (c->blkGen)
+        return fail(ICSF_ERR_KEYWORD, "MRF needs jacobianStorage blocks (the frame terms are not generators)");
+    if (!std::isfinite(omega) || !std::isfinite(origin[0]) || !std::isfinite(origin[1]) || !std::isfinite(origin[2]) ||
+        !std::isfinite(axis[0]) || !std::isfinite(axis[1]) || !std::isfinite(axis[2]))
+        return fail(ICSF_ERR_NONFINITE, "icsf_set_mrf: origin, axis and omega must be finite");
+    const HostMesh& h = c->mesh->h;
+    const int nC = h.nCells, nF = h.nFaces, nI = h.nInternal, nB = nF - nI, nP = (int)h.patchStart.size();
+    std::vector<char> zc(nC, 0), excl(nP, 0);
+    for (int i = 0; i < nZoneCells; ++i) {
+        if (zoneCells[i] < 0 || zoneCells[i] >= nC) return fail(ICSF_ERR_ARG, "icsf_set_mrf: zone cell out of range");
+        zc[zoneCells[i]] = 1;
+    }
+    for (int i = 0; i < nNonRotating; ++i) {
+        if (nonRotatingPatches[i] < 0 || nonRotatingPatches[i] >= nP)
+            return fail(ICSF_ERR_ARG, "icsf_set_mrf: patch index out of range");
+        excl[nonRotatingPatches[i]] = 1;
+    }
+    // MRFCoupledZone: axis normalised, Omega = omega axis; face types of
+    // setMRFFaces (MRFCoupledZone.C:42-189): 1 internal faces with a zone
+    // cell on either side and the zone cells' faces on the other non-empty,
+    // non-coupled patches, 2 the zone cells' faces on coupled or
+    // non-rotating patches; faceU = Omega ^ (Cf - origin) on both (faceU,
+    // :549-577), the normal component um = faceU & Sf/|Sf| (outerLoop.H:18)
+    V3 ax{axis[0], axis[1], axis[2]};
+    const double am = mag(ax);
+    ax = am < kROOTVSMALL ? V3{0, 0, 0} : ax / am;
+    const V3 Om = omega * ax, o{origin[0], origin[1], origin[2]};
+    std::vector<int> type(nF, 0);
+    for (int f = 0; f < nI; ++f)
+        if (zc[h.owner[f]] || zc[h.neighbour[f]]) type[f] = 1;
+    for (int p = 0; p < nP; ++p)
+        for (int f = h.patchStart[p]; f < h.patchStart[p] + h.patchSize[p]; ++f) {
+            if (!zc[h.owner[f]]) continue;
+            if (h.patchKind[p] == PATCH_CYCLIC || excl[p]) type[f] = 2;
+            else if (h.patchKind[p] != PATCH_EMPTY) type[f] = 1;
+        }
+    for (int f = nI; f < nF; ++f)
+        if (type[f] == 1 && c->bUT[f - nI] != BC_FIXED)
+            return fail(ICSF_ERR_ARG, "a patch that rotates with the MRF zone needs a fixedValue U condition "
+                                      "(list it among the non-rotating patches otherwise)");
+    std::vector<double> um(nF, 0.0), omc((size_t)3 * nC, 0.0);
+    c->mrfIncl.assign(std::max(nB, 1), 0);
+    c->mrfUb.assign((size_t)3 * std::max(nB, 1), 0.0);
+    for (int f = 0; f < nF; ++f) {
+        V3 fu{0, 0, 0};
+        if (type[f]) fu = cross(Om, V3{h.Cf[3 * f], h.Cf[3 * f + 1], h.Cf[3 * f + 2]} - o);
+        um[f] = dot(fu, V3{h.Sf[3 * f], h.Sf[3 * f + 1], h.Sf[3 * f + 2]} / h.magSf[f]);
+        if (f >= nI && type[f] == 1) {
+            const int b = f - nI;
+            c->mrfIncl[b] = 1;
+            c->mrfUb[3 * b] = fu.x; c->mrfUb[3 * b + 1] = fu.y; c->mrfUb[3 * b + 2] = fu.z;
+            for (int d = 0; d < 3; ++d) c->bUv[3 * b + d] = c->mrfUb[3 * b + d];
+        }
+    }
+    for (int i = 0; i < nC; ++i)
+        if (zc[i]) { omc[3 * i] = Om.x; omc[3 * i + 1] = Om.y; omc[3 * i + 2] = Om.z; }
+    RC(c->buf.upload(c->d_umrf, um.data(), um.size()));
+    RC(c->buf.upload(c->d_omC, omc.data(), omc.size()));
+    c->mv.umrf = c->d_umrf;
+    c->mv.omC = c->d_omC;
+    c->mrf = true;
+    c->bcDirty = true;
+    c->haveResidual = c->assembled = false;
+    // a captured step holds the mesh view without the zone
+    if (c->gexec) {
+        HIPCHK(hipStreamSynchronize(c->st));
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
     return ICSF_OK;
 }
 
@@ -2383,6 +2476,27 @@ int icsf_hb_set_boundary(icsf_hb* h, int instant, int patch, int pType, int UTyp
                          const double* UValue, const double* TValue) {
     RC(hbCheckInstance(h, instant));
     return icsf_set_boundary(h->inst[instant], patch, pType, UType, TType, pValue, UValue, TValue);
+}
+
+int icsf_hb_set_mrf(icsf_hb* h, int nZoneCells, const int* zoneCells, const double* origin, const double* axis,
+                    double omega, int nNonRotating, const int* nonRotatingPatches) {
+    if (!h) return fail(ICSF_ERR_ARG, "NULL harmonic-balance case");
+    for (icsf_case* c : h->inst)
+        RC(icsf_set_mrf(c, nZoneCells, zoneCells, origin, axis, omega, nNonRotating, nonRotatingPatches));
+    // the batched launches read the instances' views from device memory
+    std::vector<HBInst> hi(h->nO);
+    for (int J = 0; J < h->nO; ++J) {
+        icsf_case* c = h->inst[J];
+        hi[J] = HBInst{c->mv, c->sv, c->blk, c->dg, c->rD, c->derr};
+    }
+    HIPCHK(hipMemcpy(h->dInst, hi.data(), sizeof(HBInst) * h->nO, hipMemcpyHostToDevice));
+    h->assembled = false;
+    if (h->gexec) {
+        HIPCHK(hipStreamSynchronize(h->st));
+        (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+    }
+    return ICSF_OK;
 }
 
 int icsf_hb_set_boundary_values(icsf_hb* h, int instant, int patch, const double* pValue, const double* UValue,

@@ -87,6 +87,11 @@ struct MeshView {
     // as full 5x5 blocks (0); gamma for generating them
     int blkGen = 1;
     double gamma = 1.4;
+    // rotating zone (MRF): normal frame velocity of every face
+    // (faceU & Sf/|Sf|, zero outside the zone) and Omega of every cell (zero
+    // outside); null without one
+    const double* umrf = nullptr;  // [nF]
+    const double* omC = nullptr;   // [nC][3]
 };
 
 struct Phys {

@@ -89,6 +89,8 @@ class dbnsCase:
             self.set_boundary(i, bc.p_type, bc.U_type, bc.T_type, bc.p_val, bc.U_val, bc.T_val)
             if bc.p_init is not None or bc.U_init is not None or bc.T_init is not None:
                 self.set_boundary_values(i, bc.p_init, bc.U_init, bc.T_init)
+        if getattr(case, "mrf", None) is not None:
+            self.set_mrf(case.mrf)
         self.set_fields(case.rho, case.U, case.T)
         return self
 
@@ -110,6 +112,15 @@ class dbnsCase:
 
     def set_fields(self, rho, U, T):
         check(lib().icsf_set_fields(self._h, ptr(as_f64(rho)), ptr(as_f64(U)), ptr(as_f64(T))))
+
+    def set_mrf(self, zone):
+        """One rotating zone (cases.MRFZone: bool cell mask, origin, axis,
+        omega, non-rotating patch indices); after the boundary conditions
+        and before the fields (icsf_set_mrf)."""
+        cells = as_i32(np.flatnonzero(np.asarray(zone.cells, bool)))
+        nr = as_i32(np.asarray(zone.non_rotating, np.int32).reshape(-1))
+        check(lib().icsf_set_mrf(self._h, int(cells.shape[0]), ptr(cells), ptr(as_f64(zone.origin)),
+                                 ptr(as_f64(zone.axis)), float(zone.omega), int(nr.shape[0]), ptr(nr)))
 
     def fields(self):
         n = self.nCells
@@ -298,8 +309,18 @@ class hbCase:
                 self.set_boundary(J, i, bc.p_type, bc.U_type, bc.T_type, bc.p_val, bc.U_val, bc.T_val)
                 if bc.p_init is not None or bc.U_init is not None or bc.T_init is not None:
                     self.set_boundary_values(J, i, bc.p_init, bc.U_init, bc.T_init)
+        if getattr(c0, "mrf", None) is not None:
+            self.set_mrf(c0.mrf)
+        for J, c in enumerate(instances):
             self.set_fields(J, c.rho, c.U, c.T)
         return self
+
+    def set_mrf(self, zone):
+        """One rotating zone for all instances (cases.MRFZone)."""
+        cells = as_i32(np.flatnonzero(np.asarray(zone.cells, bool)))
+        nr = as_i32(np.asarray(zone.non_rotating, np.int32).reshape(-1))
+        check(lib().icsf_hb_set_mrf(self._h, int(cells.shape[0]), ptr(cells), ptr(as_f64(zone.origin)),
+                                    ptr(as_f64(zone.axis)), float(zone.omega), int(nr.shape[0]), ptr(nr)))
 
     def __del__(self):
         h = getattr(self, "_h", None)

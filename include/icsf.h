@@ -305,6 +305,24 @@ int icsf_get_interface_blocks(icsf_case* c, double* blocks);
  * true): the first pseudo-step times each candidate on the case's own data
  * and keeps the fastest; results are the same bits for every choice. */
 int icsf_get_tuning(icsf_case* c, int* params);
+/* One rotating zone (MRF; replaces MRFCoupledZoneList / MRFProperties,
+ * MRFCoupledZone.C:42-675, and the frame terms of the flux schemes and of
+ * the Jacobian): the zone's cells, origin, axis (normalised here), omega
+ * [rad/s] and the patches that do not rotate (nonRotatingPatches).  Faces of
+ * the zone transport with U.n - um, um = (Omega ^ (Cf - origin)) . Sf/|Sf|;
+ * the coupled system gets -(Omega ^ rho U) V and its derivative; the zone
+ * cells' faces on the other non-empty, non-coupled patches rotate with it:
+ * their U condition must be fixedValue and its value becomes
+ * Omega ^ (Cf - origin).  Call after the boundary conditions and before the
+ * fields; needs fvSolution jacobianStorage blocks.  One zone per case;
+ * the residual VJP refuses such cases. */
+int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const double* origin, const double* axis,
+                 double omega, int nNonRotating, const int* nonRotatingPatches);
+/* The same zone for every instance of a harmonic-balance case (the
+ * instances share the mesh and the frame); after the instances' boundary
+ * conditions, before their fields. */
+int icsf_hb_set_mrf(icsf_hb* h, int nZoneCells, const int* zoneCells, const double* origin, const double* axis,
+                    double omega, int nNonRotating, const int* nonRotatingPatches);
 /* Sets the launch parameters (the layout of icsf_get_tuning; params[5]
  * ignored) and marks the case tuned, so the autotuner does not run (no
  * counterpart in the reference; for pinning a measured choice).
