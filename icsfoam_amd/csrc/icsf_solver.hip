@@ -67,6 +67,15 @@ struct DevBuf {
         (void)st;
         return ICSF_OK;
     }
+    // frees one allocation of this buffer set (after the stream is idle)
+    void release(void* p) {
+        for (size_t i = 0; i < ptrs.size(); ++i)
+            if (ptrs[i] == p) {
+                (void)hipFree(p);
+                ptrs.erase(ptrs.begin() + (long)i);
+                return;
+            }
+    }
     ~DevBuf() {
         for (void* p : ptrs) (void)hipFree(p);
     }
@@ -1709,8 +1718,6 @@ int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const doubl
     if (!c || (nZoneCells > 0 && !zoneCells) || !origin || !axis || (nNonRotating > 0 && !nonRotatingPatches))
         return fail(ICSF_ERR_ARG, "icsf_set_mrf: NULL argument");
     if (c->mrf) return fail(ICSF_ERR_STATE, "icsf_set_mrf: the case already has an MRF zone");
-    if (c->blkGen)
-        return fail(ICSF_ERR_KEYWORD, "MRF needs jacobianStorage blocks (the frame terms are not generators)");
     if (!std::isfinite(omega) || !std::isfinite(origin[0]) || !std::isfinite(origin[1]) || !std::isfinite(origin[2]) ||
         !std::isfinite(axis[0]) || !std::isfinite(axis[1]) || !std::isfinite(axis[2]))
         return fail(ICSF_ERR_NONFINITE, "icsf_set_mrf: origin, axis and omega must be finite");
@@ -1767,6 +1774,22 @@ int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const doubl
         if (zc[i]) { omc[3 * i] = Om.x; omc[3 * i + 1] = Om.y; omc[3 * i + 2] = Om.z; }
     RC(c->buf.upload(c->d_umrf, um.data(), um.size()));
     RC(c->buf.upload(c->d_omC, omc.data(), omc.size()));
+    if (c->blkGen) {
+        // the frame divergence adds a diagonal term to every off-diagonal
+        // block that the generator form does not carry: store full blocks
+        HIPCHK(hipStreamSynchronize(c->st));
+        const size_t slots = (size_t)c->mesh->sell.slicePtr.back() * SLICE;
+        const size_t nBlk = slots * NB + (size_t)c->mesh->view.nCyc * NB;
+        const size_t eb = c->blkFp32 ? 4 : 8;
+        char* p;
+        RC(c->buf.alloc(p, nBlk * eb));
+        HIPCHK(hipMemset(p, 0, nBlk * eb));
+        c->buf.release(c->blk);
+        c->blk = p;
+        c->blkGen = false;
+        c->mv.blkGen = 0;
+        c->mv.cycBase = slots * NB;
+    }
     c->mv.umrf = c->d_umrf;
     c->mv.omC = c->d_omC;
     c->mrf = true;
@@ -2484,10 +2507,12 @@ int icsf_hb_set_mrf(icsf_hb* h, int nZoneCells, const int* zoneCells, const doub
     for (icsf_case* c : h->inst)
         RC(icsf_set_mrf(c, nZoneCells, zoneCells, origin, axis, omega, nNonRotating, nonRotatingPatches));
     // the batched launches read the instances' views from device memory
+    // (with the frame arrays, and full-block storage)
     std::vector<HBInst> hi(h->nO);
     for (int J = 0; J < h->nO; ++J) {
         icsf_case* c = h->inst[J];
         hi[J] = HBInst{c->mv, c->sv, c->blk, c->dg, c->rD, c->derr};
+        h->hbb.blk[J] = c->blk;
     }
     HIPCHK(hipMemcpy(h->dInst, hi.data(), sizeof(HBInst) * h->nO, hipMemcpyHostToDevice));
     h->assembled = false;

@@ -6,7 +6,11 @@ the p, U, T fields with their boundary conditions) and what it writes
 (runTime.write(): the fields of a time directory).
 
   read_polymesh(dir)                  -> (PolyMesh, patch names)
-  read_case(case_dir, ...)            -> Case (fields, boundary conditions)
+  read_case(case_dir, ...)            -> Case (fields, boundary conditions,
+                                         rotating zone)
+  read_mrf(case_dir, n_cells, names)  -> MRFZone or None
+                                         (constant/MRFProperties and the
+                                         zone's polyMesh/cellZones entry)
   write_polymesh(dir, mesh, names)
   write_fields(time_dir, mesh, names, fields, boundary)
 
@@ -20,6 +24,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import re
 from typing import Dict, List, Optional, Tuple
 
 import numpy as np
@@ -27,7 +32,7 @@ import numpy as np
 from ._lib import IcsfError, as_f64, as_i32, check, lib, ptr
 from .cases import (BC_FIXED_VALUE, BC_PRESSURE_INLET_OUTLET_VELOCITY, BC_SLIP, BC_TOTAL_PRESSURE,
                     BC_TOTAL_TEMPERATURE, BC_ZERO_GRADIENT, PATCH_CYCLIC, PATCH_EMPTY, BoundarySpec, Case,
-                    Patch, PolyMesh, R_GAS)
+                    MRFZone, Patch, PolyMesh, R_GAS)
 
 
 def read_polymesh(poly_mesh_dir: str) -> Tuple[PolyMesh, List[str]]:
@@ -153,6 +158,103 @@ def _bc(pt: Patch, fp: FoamField, fU: FoamField, fT: FoamField, overrides: Dict[
     return BoundarySpec(spec["p_type"], spec["U_type"], spec["T_type"], **vals)
 
 
+def _strip_comments(text: str) -> str:
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    return re.sub(r"//[^\n]*", " ", text)
+
+
+def _blocks(text: str) -> Dict[str, str]:
+    """Top-level `name { ... }` entries of a dictionary file (FoamFile
+    header excluded), brace-matched."""
+    out, i = {}, 0
+    for mt in re.finditer(r"([A-Za-z_][\w.:-]*)\s*\{", text):
+        if mt.start() < i:
+            continue
+        depth, j = 1, mt.end()
+        while j < len(text) and depth:
+            depth += {"{": 1, "}": -1}.get(text[j], 0)
+            j += 1
+        if mt.group(1) != "FoamFile":
+            out[mt.group(1)] = text[mt.end():j - 1]
+        i = j
+    return out
+
+
+def _vector(body: str, key: str) -> np.ndarray:
+    mt = re.search(r"\b" + key + r"\s*\(\s*([^)]*)\)\s*;", body)
+    if not mt:
+        raise IcsfError(-2, f"MRFProperties: missing {key}")
+    return np.array([float(x) for x in mt.group(1).split()], np.float64)
+
+
+def _cell_zone(poly_mesh_dir: str, name: str, n_cells: int) -> np.ndarray:
+    """The cells of zone `name` of polyMesh/cellZones (cellLabels list)."""
+    path = os.path.join(poly_mesh_dir, "cellZones")
+    if not os.path.exists(path) and os.path.exists(path + ".gz"):
+        import gzip
+        with gzip.open(path + ".gz", "rt") as f:
+            text = f.read()
+    else:
+        with open(path) as f:
+            text = f.read()
+    body = _blocks(_strip_comments(text)).get(name)
+    if body is None:
+        raise IcsfError(-2, f"cellZones: no zone {name}")
+    mt = re.search(r"cellLabels\s+(?:List<label>\s*)?(\d+)\s*\(([^)]*)\)", body)
+    if mt:
+        labels = np.array(mt.group(2).split(), np.int64)
+    else:  # uniform short form N{label}
+        mt = re.search(r"cellLabels\s+(?:List<label>\s*)?(\d+)\s*\{\s*(\d+)\s*\}", body)
+        if not mt:
+            raise IcsfError(-2, f"cellZones: zone {name} has no readable cellLabels")
+        labels = np.full(int(mt.group(1)), int(mt.group(2)), np.int64)
+    if labels.shape[0] != int(mt.group(1)) or (labels.size and (labels.min() < 0 or labels.max() >= n_cells)):
+        raise IcsfError(-1, f"cellZones: zone {name} has bad cell labels")
+    cells = np.zeros(n_cells, bool)
+    cells[labels] = True
+    return cells
+
+
+def read_mrf(case_dir: str, n_cells: int, names: List[str]) -> Optional[MRFZone]:
+    """constant/MRFProperties (IOMRFCoupledZoneList / MRFCoupledZone.C:
+    240-296): the active zone's cellZone, origin, axis, omega (a constant:
+    `omega 100;` or `omega constant 100;`) and nonRotatingPatches (names or
+    regular expressions, matched against the patch names); None without the
+    file or without an active zone.  One active zone is supported."""
+    path = os.path.join(case_dir, "constant", "MRFProperties")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        text = _strip_comments(f.read())
+    zones = []
+    for zname, body in _blocks(text).items():
+        mt = re.search(r"\bactive\s+(\w+)\s*;", body)
+        if mt and mt.group(1).lower() in ("no", "false", "off", "n", "none"):
+            continue
+        zones.append((zname, body))
+    if not zones:
+        return None
+    if len(zones) > 1:
+        raise IcsfError(-1, "MRFProperties: one active MRF zone is supported")
+    zname, body = zones[0]
+    mt = re.search(r"\bcellZone\s+([\w.:-]+)\s*;", body)
+    if not mt:
+        raise IcsfError(-2, f"MRFProperties {zname}: missing cellZone")
+    cells = _cell_zone(os.path.join(case_dir, "constant", "polyMesh"), mt.group(1), n_cells)
+    mo = re.search(r"\bomega\s+(?:constant\s+)?([-+0-9.eE]+)\s*;", body)
+    if not mo:
+        raise IcsfError(-2, f"MRFProperties {zname}: omega must be a constant")
+    non_rot = []
+    mn = re.search(r"\bnonRotatingPatches\s*\(([^)]*)\)\s*;", body)
+    if mn:
+        for pat in mn.group(1).split():
+            pat = pat.strip('"')
+            hits = [i for i, n in enumerate(names) if re.fullmatch(pat, n)]
+            non_rot += [i for i in hits if i not in non_rot]
+    return MRFZone(cells=cells, origin=_vector(body, "origin"), axis=_vector(body, "axis"),
+                   omega=float(mo.group(1)), non_rotating=sorted(non_rot))
+
+
 def read_case(case_dir: str, time: str = "0", thermo: Optional[str] = None, fvSchemes: Optional[str] = None,
               fvSolution: Optional[str] = None, overrides: Optional[Dict[str, Dict[str, str]]] = None,
               R: float = R_GAS) -> Tuple[Case, List[str]]:
@@ -182,7 +284,7 @@ def read_case(case_dir: str, time: str = "0", thermo: Optional[str] = None, fvSc
     U = np.asarray(fU.internal, np.float64)
     case = Case(os.path.basename(os.path.abspath(case_dir)), mesh, p / (R * T), U.copy(), T.copy(), bcs,
                 text(thermo, "constant", "thermophysicalProperties"), text(fvSchemes, "system", "fvSchemes"),
-                text(fvSolution, "system", "fvSolution"))
+                text(fvSolution, "system", "fvSolution"), mrf=read_mrf(case_dir, n, names))
     return case, names
 
 

@@ -314,8 +314,10 @@ int icsf_get_tuning(icsf_case* c, int* params);
  * cells' faces on the other non-empty, non-coupled patches rotate with it:
  * their U condition must be fixedValue and its value becomes
  * Omega ^ (Cf - origin).  Call after the boundary conditions and before the
- * fields; needs fvSolution jacobianStorage blocks.  One zone per case;
- * the residual VJP refuses such cases. */
+ * fields.  The frame divergence puts a diagonal term on every off-diagonal
+ * block, which the generator form does not hold: a case with
+ * jacobianStorage generators switches to full blocks here.  One zone per
+ * case; the residual VJP refuses such cases. */
 int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const double* origin, const double* axis,
                  double omega, int nNonRotating, const int* nonRotatingPatches);
 /* The same zone for every instance of a harmonic-balance case (the

@@ -123,3 +123,65 @@ def test_ls89_fields_map_to_boundary_conditions():
     np.testing.assert_allclose(c.rho, 1e5 / (cases.R_GAS * 400.0), rtol=1e-15)
     with pytest.raises(Exception, match="inletOutlet"):
         foam.read_case(LS89, thermo="", fvSchemes="", fvSolution="")
+
+
+MRF_PROPS = """FoamFile { version 2.0; format ascii; class dictionary; object MRFProperties; }
+// one rotating zone, one switched off
+MRF1
+{
+    cellZone    rotor;
+    active      yes;
+    nonRotatingPatches (inlet "out.*" walls);
+    origin    (0.1 0.2 -0.3);
+    axis      (0 0 2);
+    omega     constant 314.159;
+}
+MRF2
+{
+    cellZone    rotor;
+    active      no;
+    origin (0 0 0); axis (1 0 0); omega 1;
+}
+"""
+
+
+def test_mrf_properties_and_cell_zones_read(tmp_path):
+    """constant/MRFProperties and polyMesh/cellZones (the rotor zone's
+    cellLabels over several lines) read into the rotating zone the library
+    takes: the zone's cells, origin, axis, constant omega and the
+    non-rotating patches (names and regular expressions); the case with
+    the zone read from files gives the oracle the same residual as the
+    zone given directly."""
+    from oracle.oracle import OracleCase
+
+    c = cases.case_box(n=(4, 3, 3), jitter=0.1, noise=0.05)
+    names = [p.name for p in c.mesh.patches]
+    case_dir = tmp_path / "case"
+    pm = case_dir / "constant" / "polyMesh"
+    foam.write_polymesh(str(pm), c.mesh)
+    p = c.rho * cases.R_GAS * c.T
+    nB = c.mesh.n_faces - c.mesh.n_internal
+    bnd = {"p": np.full(nB, 1e5), "U": np.zeros((nB, 3)), "T": np.full(nB, 300.0)}
+    foam.write_fields(str(case_dir / "0"), c.mesh, names, {"p": p, "U": c.U, "T": c.T}, c.bcs, bnd)
+    zone = np.arange(c.mesh.n_cells) % 3 == 1
+    labels = "\n".join(str(i) for i in np.flatnonzero(zone))
+    (pm / "cellZones").write_text("FoamFile { version 2.0; format ascii; class regIOobject; object cellZones; }\n"
+                                  f"1\n(\nrotor\n{{\n    type cellZone;\ncellLabels List<label>\n{zone.sum()}\n(\n"
+                                  f"{labels}\n)\n;\n}}\n)\n")
+    (case_dir / "constant" / "MRFProperties").write_text(MRF_PROPS)
+    r, _ = foam.read_case(str(case_dir), thermo=c.thermo, fvSchemes=c.fvSchemes, fvSolution=c.fvSolution)
+    z = r.mrf
+    np.testing.assert_array_equal(z.cells, zone)
+    np.testing.assert_array_equal(z.origin, [0.1, 0.2, -0.3])
+    np.testing.assert_array_equal(z.axis, [0, 0, 2])
+    assert z.omega == 314.159
+    assert z.non_rotating == sorted(names.index(n) for n in ("inlet", "outlet", "walls"))
+    # the same zone given directly: the same oracle residual
+    direct = cases.MRFZone(cells=zone, origin=np.array([0.1, 0.2, -0.3]), axis=np.array([0, 0, 2.0]),
+                           omega=314.159, non_rotating=z.non_rotating)
+    b1, _ = OracleCase(r).residual()
+    import dataclasses
+    b2, _ = OracleCase(dataclasses.replace(r, mrf=direct)).residual()
+    np.testing.assert_array_equal(b1, b2)
+    b0, _ = OracleCase(dataclasses.replace(r, mrf=None)).residual()
+    assert np.abs(b1 - b0).max() > 0

@@ -11,8 +11,10 @@ cyclic sector rotating about its own axis (Roe second order):
 * 5 pseudo-steps: first-step residuals 1e-12, histories and fields 1e-8
   (FP64 form); the mixed form within 1e-3 of the FP64 oracle; a replayed
   hipGraph step equals the eager step bit for bit;
-* set-up errors: generator storage, a rotating patch without a fixedValue
-  U condition, the residual VJP.
+* a case asking for generator storage switches to full blocks and gives
+  the same bits;
+* set-up errors: a rotating patch without a fixedValue U condition, the
+  residual VJP.
 """
 import dataclasses
 
@@ -20,7 +22,7 @@ import numpy as np
 import pytest
 
 from icsfoam_amd import cases, dbnsCase
-from icsfoam_amd._lib import IcsfError, KeywordError
+from icsfoam_amd._lib import IcsfError
 from icsfoam_amd.mesh import prepare_case
 from oracle.oracle import OracleCase, OracleMesh
 
@@ -132,9 +134,12 @@ def test_mrf_graph_replay_and_mixed_form():
 def test_mrf_set_up_errors():
     c = _box("ROE", rotating_walls=False)
     print("[mrf errors]", flush=True)
-    # generator storage cannot carry the frame terms
-    with pytest.raises(KeywordError):
-        dbnsCase.from_case(dataclasses.replace(c, fvSolution=_sol(jacobianStorage="generators")))
+    # generator storage cannot carry the frame terms: the case switches to
+    # full blocks, with the same results
+    gg = dbnsCase.from_case(dataclasses.replace(c, fvSolution=_sol(jacobianStorage="generators")))
+    gb = dbnsCase.from_case(c)
+    np.testing.assert_array_equal(gg.pseudo_steps(3), gb.pseudo_steps(3))
+    np.testing.assert_array_equal(gg.fields()["W"], gb.fields()["W"])
     # slip walls that would rotate
     with pytest.raises(IcsfError):
         dbnsCase.from_case(dataclasses.replace(c, mrf=dataclasses.replace(c.mrf, non_rotating=[])))
