@@ -133,6 +133,18 @@ class MRFZone:
 
 
 @dataclasses.dataclass
+class TranslatingZone:
+    """One translating zone (the reference's MRFTranslatingProperties entry,
+    MRFTranslatingZone): cells (bool [nC]), frameVelocity, and the patches
+    that do not move with it (nonMovingPatches, patch indices).  The zone's
+    other non-empty, non-coupled patch faces move with the frame: their U
+    must be fixedValue, and the frame sets it to the frame velocity."""
+    cells: np.ndarray
+    velocity: np.ndarray
+    non_moving: List[int] = dataclasses.field(default_factory=list)
+
+
+@dataclasses.dataclass
 class Case:
     name: str
     mesh: PolyMesh
@@ -145,6 +157,7 @@ class Case:
     fvSolution: str
     cell_order: Optional[np.ndarray] = None   # new -> original cell index
     mrf: Optional[MRFZone] = None
+    translating: Optional[TranslatingZone] = None
 
 
 # --------------------------------------------------------------------------
@@ -520,9 +533,11 @@ def permute_case(case: Case, new_to_old: np.ndarray, renumber) -> Case:
     mesh = renumber(case.mesh, new_to_old)
     order = new_to_old if case.cell_order is None else case.cell_order[new_to_old]
     mrf = None if case.mrf is None else dataclasses.replace(case.mrf, cells=case.mrf.cells[new_to_old])
+    tr = None if case.translating is None else \
+        dataclasses.replace(case.translating, cells=case.translating.cells[new_to_old])
     return dataclasses.replace(case, mesh=mesh, rho=case.rho[new_to_old],
                                U=case.U[new_to_old], T=case.T[new_to_old],
-                               cell_order=order, mrf=mrf)
+                               cell_order=order, mrf=mrf, translating=tr)
 
 
 # --------------------------------------------------------------------------

@@ -12,6 +12,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "icsf.h"
@@ -140,6 +141,15 @@ struct icsf_mesh {
 // ------------------------------------------------------------------ case --
 enum { NTIMER = 6 };
 
+// One frame zone's contribution (rotating or translating): faceU & Sf/|Sf|
+// per face, the moving boundary faces and their U.
+struct FramePart {
+    bool on = false;
+    std::vector<double> un;
+    std::vector<char> incl;
+    std::vector<double> Ub;
+};
+
 struct icsf_case {
     icsf_mesh* mesh = nullptr;
     hipStream_t st = nullptr;
@@ -165,9 +175,11 @@ struct icsf_case {
     std::vector<double> bInit;
     double* d_bInit = nullptr;
     bool bcDirty = true;
-    // rotating zone (icsf_set_mrf): the boundary faces that rotate with it
-    // and their frame velocity (their fixedValue U is set to it)
+    // frame zones (icsf_set_mrf, icsf_set_translating_frame): their parts,
+    // the boundary faces that move with them and the frame velocity there
+    // (their fixedValue U is set to it)
     bool mrf = false;
+    FramePart frot, ftr;
     std::vector<char> mrfIncl;
     std::vector<double> mrfUb;
     double *d_umrf = nullptr, *d_omC = nullptr;
@@ -1713,36 +1725,32 @@ int icsf_set_boundary(icsf_case* c, int patch, int pType, int UType, int TType, 
     return ICSF_OK;
 }
 
-int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const double* origin, const double* axis,
-                 double omega, int nNonRotating, const int* nonRotatingPatches) {
-    if (!c || (nZoneCells > 0 && !zoneCells) || !origin || !axis || (nNonRotating > 0 && !nonRotatingPatches))
-        return fail(ICSF_ERR_ARG, "icsf_set_mrf: NULL argument");
-    if (c->mrf) return fail(ICSF_ERR_STATE, "icsf_set_mrf: the case already has an MRF zone");
-    if (!std::isfinite(omega) || !std::isfinite(origin[0]) || !std::isfinite(origin[1]) || !std::isfinite(origin[2]) ||
-        !std::isfinite(axis[0]) || !std::isfinite(axis[1]) || !std::isfinite(axis[2]))
-        return fail(ICSF_ERR_NONFINITE, "icsf_set_mrf: origin, axis and omega must be finite");
+}  // extern "C"
+
+namespace {
+
+// One frame zone's contribution (rotating or translating): faceU & Sf/|Sf|
+// per face, the moving boundary faces and their U.  Face types of
+// setMRFFaces (MRFCoupledZone.C:42-189, MRFTranslatingZone.C the same):
+// 1 internal faces with a zone cell on either side and the zone cells'
+// faces on the other non-empty, non-coupled patches, 2 the zone cells'
+// faces on coupled or non-moving patches; faceU on both (faceU,
+// MRFCoupledZone.C:549-577), its normal component um = faceU & Sf/|Sf|
+// (outerLoop.H:18-21); the type-1 boundary faces move with the frame
+// (correctBoundaryVelocity).
+int framePart(icsf_case* c, const char* who, int nZone, const int* zone, int nNonMov, const int* nonMov,
+              const std::function<V3(int)>& faceU, FramePart& out) {
     const HostMesh& h = c->mesh->h;
     const int nC = h.nCells, nF = h.nFaces, nI = h.nInternal, nB = nF - nI, nP = (int)h.patchStart.size();
     std::vector<char> zc(nC, 0), excl(nP, 0);
-    for (int i = 0; i < nZoneCells; ++i) {
-        if (zoneCells[i] < 0 || zoneCells[i] >= nC) return fail(ICSF_ERR_ARG, "icsf_set_mrf: zone cell out of range");
-        zc[zoneCells[i]] = 1;
+    for (int i = 0; i < nZone; ++i) {
+        if (zone[i] < 0 || zone[i] >= nC) return fail(ICSF_ERR_ARG, std::string(who) + ": zone cell out of range");
+        zc[zone[i]] = 1;
     }
-    for (int i = 0; i < nNonRotating; ++i) {
-        if (nonRotatingPatches[i] < 0 || nonRotatingPatches[i] >= nP)
-            return fail(ICSF_ERR_ARG, "icsf_set_mrf: patch index out of range");
-        excl[nonRotatingPatches[i]] = 1;
+    for (int i = 0; i < nNonMov; ++i) {
+        if (nonMov[i] < 0 || nonMov[i] >= nP) return fail(ICSF_ERR_ARG, std::string(who) + ": patch index out of range");
+        excl[nonMov[i]] = 1;
     }
-    // MRFCoupledZone: axis normalised, Omega = omega axis; face types of
-    // setMRFFaces (MRFCoupledZone.C:42-189): 1 internal faces with a zone
-    // cell on either side and the zone cells' faces on the other non-empty,
-    // non-coupled patches, 2 the zone cells' faces on coupled or
-    // non-rotating patches; faceU = Omega ^ (Cf - origin) on both (faceU,
-    // :549-577), the normal component um = faceU & Sf/|Sf| (outerLoop.H:18)
-    V3 ax{axis[0], axis[1], axis[2]};
-    const double am = mag(ax);
-    ax = am < kROOTVSMALL ? V3{0, 0, 0} : ax / am;
-    const V3 Om = omega * ax, o{origin[0], origin[1], origin[2]};
     std::vector<int> type(nF, 0);
     for (int f = 0; f < nI; ++f)
         if (zc[h.owner[f]] || zc[h.neighbour[f]]) type[f] = 1;
@@ -1754,30 +1762,49 @@ int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const doubl
         }
     for (int f = nI; f < nF; ++f)
         if (type[f] == 1 && c->bUT[f - nI] != BC_FIXED)
-            return fail(ICSF_ERR_ARG, "a patch that rotates with the MRF zone needs a fixedValue U condition "
-                                      "(list it among the non-rotating patches otherwise)");
-    std::vector<double> um(nF, 0.0), omc((size_t)3 * nC, 0.0);
-    c->mrfIncl.assign(std::max(nB, 1), 0);
-    c->mrfUb.assign((size_t)3 * std::max(nB, 1), 0.0);
+            return fail(ICSF_ERR_ARG, std::string(who) + ": a patch that moves with the zone needs a fixedValue U "
+                                      "condition (list it among the non-moving patches otherwise)");
+    out.on = true;
+    out.un.assign(nF, 0.0);
+    out.incl.assign(std::max(nB, 1), 0);
+    out.Ub.assign((size_t)3 * std::max(nB, 1), 0.0);
     for (int f = 0; f < nF; ++f) {
         V3 fu{0, 0, 0};
-        if (type[f]) fu = cross(Om, V3{h.Cf[3 * f], h.Cf[3 * f + 1], h.Cf[3 * f + 2]} - o);
-        um[f] = dot(fu, V3{h.Sf[3 * f], h.Sf[3 * f + 1], h.Sf[3 * f + 2]} / h.magSf[f]);
+        if (type[f]) fu = faceU(f);
+        out.un[f] = dot(fu, V3{h.Sf[3 * f], h.Sf[3 * f + 1], h.Sf[3 * f + 2]} / h.magSf[f]);
         if (f >= nI && type[f] == 1) {
             const int b = f - nI;
-            c->mrfIncl[b] = 1;
-            c->mrfUb[3 * b] = fu.x; c->mrfUb[3 * b + 1] = fu.y; c->mrfUb[3 * b + 2] = fu.z;
-            for (int d = 0; d < 3; ++d) c->bUv[3 * b + d] = c->mrfUb[3 * b + d];
+            out.incl[b] = 1;
+            out.Ub[3 * b] = fu.x; out.Ub[3 * b + 1] = fu.y; out.Ub[3 * b + 2] = fu.z;
         }
     }
-    for (int i = 0; i < nC; ++i)
-        if (zc[i]) { omc[3 * i] = Om.x; omc[3 * i + 1] = Om.y; omc[3 * i + 2] = Om.z; }
-    RC(c->buf.upload(c->d_umrf, um.data(), um.size()));
-    RC(c->buf.upload(c->d_omC, omc.data(), omc.size()));
+    return ICSF_OK;
+}
+
+// The case's frame arrays from its zones: um of the rotating zone plus um
+// of the translating one, the moving faces' U (the translating zone's
+// after the rotating one's, outerLoop.H:6-7) written into their fixed
+// values; Omega from the rotating zone.  The frame divergence adds a
+// diagonal term to every off-diagonal block that the generator form does
+// not carry, so a case with generators switches to full blocks.
+int frameCompose(icsf_case* c) {
+    const HostMesh& h = c->mesh->h;
+    const int nF = h.nFaces, nI = h.nInternal, nB = nF - nI;
+    const FramePart &R = c->frot, &T = c->ftr;
+    std::vector<double> um(nF, 0.0);
+    for (int f = 0; f < nF; ++f) um[f] = R.on && T.on ? R.un[f] + T.un[f] : (R.on ? R.un[f] : T.un[f]);
+    c->mrfIncl.assign(std::max(nB, 1), 0);
+    c->mrfUb.assign((size_t)3 * std::max(nB, 1), 0.0);
+    for (int b = 0; b < nB; ++b)
+        for (const FramePart* P : {&R, &T})
+            if (P->on && P->incl[b]) {
+                c->mrfIncl[b] = 1;
+                for (int d = 0; d < 3; ++d) c->mrfUb[3 * b + d] = c->bUv[3 * b + d] = P->Ub[3 * b + d];
+            }
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (!c->d_umrf) RC(c->buf.alloc(c->d_umrf, (size_t)nF));
+    HIPCHK(hipMemcpy(c->d_umrf, um.data(), (size_t)nF * sizeof(double), hipMemcpyHostToDevice));
     if (c->blkGen) {
-        // the frame divergence adds a diagonal term to every off-diagonal
-        // block that the generator form does not carry: store full blocks
-        HIPCHK(hipStreamSynchronize(c->st));
         const size_t slots = (size_t)c->mesh->sell.slicePtr.back() * SLICE;
         const size_t nBlk = slots * NB + (size_t)c->mesh->view.nCyc * NB;
         const size_t eb = c->blkFp32 ? 4 : 8;
@@ -1791,17 +1818,64 @@ int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const doubl
         c->mv.cycBase = slots * NB;
     }
     c->mv.umrf = c->d_umrf;
-    c->mv.omC = c->d_omC;
+    c->mv.omC = R.on ? c->d_omC : nullptr;
     c->mrf = true;
     c->bcDirty = true;
     c->haveResidual = c->assembled = false;
     // a captured step holds the mesh view without the zone
     if (c->gexec) {
-        HIPCHK(hipStreamSynchronize(c->st));
         (void)hipGraphExecDestroy(c->gexec);
         c->gexec = nullptr;
     }
     return ICSF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const double* origin, const double* axis,
+                 double omega, int nNonRotating, const int* nonRotatingPatches) {
+    if (!c || (nZoneCells > 0 && !zoneCells) || !origin || !axis || (nNonRotating > 0 && !nonRotatingPatches))
+        return fail(ICSF_ERR_ARG, "icsf_set_mrf: NULL argument");
+    if (c->frot.on) return fail(ICSF_ERR_STATE, "icsf_set_mrf: the case already has a rotating zone");
+    if (!std::isfinite(omega) || !std::isfinite(origin[0]) || !std::isfinite(origin[1]) || !std::isfinite(origin[2]) ||
+        !std::isfinite(axis[0]) || !std::isfinite(axis[1]) || !std::isfinite(axis[2]))
+        return fail(ICSF_ERR_NONFINITE, "icsf_set_mrf: origin, axis and omega must be finite");
+    const HostMesh& h = c->mesh->h;
+    const int nC = h.nCells;
+    // MRFCoupledZone: axis normalised, Omega = omega axis, faceU = Omega ^
+    // (Cf - origin)
+    V3 ax{axis[0], axis[1], axis[2]};
+    const double am = mag(ax);
+    ax = am < kROOTVSMALL ? V3{0, 0, 0} : ax / am;
+    const V3 Om = omega * ax, o{origin[0], origin[1], origin[2]};
+    FramePart P;
+    RC(framePart(c, "icsf_set_mrf", nZoneCells, zoneCells, nNonRotating, nonRotatingPatches,
+                 [&](int f) { return cross(Om, V3{h.Cf[3 * f], h.Cf[3 * f + 1], h.Cf[3 * f + 2]} - o); }, P));
+    std::vector<double> omc((size_t)3 * nC, 0.0);
+    for (int i = 0; i < nZoneCells; ++i) {
+        const int z = zoneCells[i];
+        omc[3 * (size_t)z] = Om.x; omc[3 * (size_t)z + 1] = Om.y; omc[3 * (size_t)z + 2] = Om.z;
+    }
+    RC(c->buf.upload(c->d_omC, omc.data(), omc.size()));
+    c->frot = std::move(P);
+    return frameCompose(c);
+}
+
+int icsf_set_translating_frame(icsf_case* c, int nZoneCells, const int* zoneCells, const double* velocity,
+                               int nNonMoving, const int* nonMovingPatches) {
+    if (!c || (nZoneCells > 0 && !zoneCells) || !velocity || (nNonMoving > 0 && !nonMovingPatches))
+        return fail(ICSF_ERR_ARG, "icsf_set_translating_frame: NULL argument");
+    if (c->ftr.on) return fail(ICSF_ERR_STATE, "icsf_set_translating_frame: the case already has a translating zone");
+    if (!std::isfinite(velocity[0]) || !std::isfinite(velocity[1]) || !std::isfinite(velocity[2]))
+        return fail(ICSF_ERR_NONFINITE, "icsf_set_translating_frame: the frame velocity must be finite");
+    const V3 V{velocity[0], velocity[1], velocity[2]};
+    FramePart P;
+    RC(framePart(c, "icsf_set_translating_frame", nZoneCells, zoneCells, nNonMoving, nonMovingPatches,
+                 [&](int) { return V; }, P));
+    c->ftr = std::move(P);
+    return frameCompose(c);
 }
 
 int icsf_set_boundary_values(icsf_case* c, int patch, const double* pValue, const double* UValue,
@@ -2501,11 +2575,25 @@ int icsf_hb_set_boundary(icsf_hb* h, int instant, int patch, int pType, int UTyp
     return icsf_set_boundary(h->inst[instant], patch, pType, UType, TType, pValue, UValue, TValue);
 }
 
+static int hbRefreshInstances(icsf_hb* h);
+
 int icsf_hb_set_mrf(icsf_hb* h, int nZoneCells, const int* zoneCells, const double* origin, const double* axis,
                     double omega, int nNonRotating, const int* nonRotatingPatches) {
     if (!h) return fail(ICSF_ERR_ARG, "NULL harmonic-balance case");
     for (icsf_case* c : h->inst)
         RC(icsf_set_mrf(c, nZoneCells, zoneCells, origin, axis, omega, nNonRotating, nonRotatingPatches));
+    return hbRefreshInstances(h);
+}
+
+int icsf_hb_set_translating_frame(icsf_hb* h, int nZoneCells, const int* zoneCells, const double* velocity,
+                                  int nNonMoving, const int* nonMovingPatches) {
+    if (!h) return fail(ICSF_ERR_ARG, "NULL harmonic-balance case");
+    for (icsf_case* c : h->inst)
+        RC(icsf_set_translating_frame(c, nZoneCells, zoneCells, velocity, nNonMoving, nonMovingPatches));
+    return hbRefreshInstances(h);
+}
+
+static int hbRefreshInstances(icsf_hb* h) {
     // the batched launches read the instances' views from device memory
     // (with the frame arrays, and full-block storage)
     std::vector<HBInst> hi(h->nO);

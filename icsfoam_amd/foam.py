@@ -11,6 +11,8 @@ the p, U, T fields with their boundary conditions) and what it writes
   read_mrf(case_dir, n_cells, names)  -> MRFZone or None
                                          (constant/MRFProperties and the
                                          zone's polyMesh/cellZones entry)
+  read_translating(case_dir, ...)     -> TranslatingZone or None
+                                         (constant/MRFTranslatingProperties)
   write_polymesh(dir, mesh, names)
   write_fields(time_dir, mesh, names, fields, boundary)
 
@@ -32,7 +34,7 @@ import numpy as np
 from ._lib import IcsfError, as_f64, as_i32, check, lib, ptr
 from .cases import (BC_FIXED_VALUE, BC_PRESSURE_INLET_OUTLET_VELOCITY, BC_SLIP, BC_TOTAL_PRESSURE,
                     BC_TOTAL_TEMPERATURE, BC_ZERO_GRADIENT, PATCH_CYCLIC, PATCH_EMPTY, BoundarySpec, Case,
-                    MRFZone, Patch, PolyMesh, R_GAS)
+                    MRFZone, Patch, PolyMesh, R_GAS, TranslatingZone)
 
 
 def read_polymesh(poly_mesh_dir: str) -> Tuple[PolyMesh, List[str]]:
@@ -215,13 +217,9 @@ def _cell_zone(poly_mesh_dir: str, name: str, n_cells: int) -> np.ndarray:
     return cells
 
 
-def read_mrf(case_dir: str, n_cells: int, names: List[str]) -> Optional[MRFZone]:
-    """constant/MRFProperties (IOMRFCoupledZoneList / MRFCoupledZone.C:
-    240-296): the active zone's cellZone, origin, axis, omega (a constant:
-    `omega 100;` or `omega constant 100;`) and nonRotatingPatches (names or
-    regular expressions, matched against the patch names); None without the
-    file or without an active zone.  One active zone is supported."""
-    path = os.path.join(case_dir, "constant", "MRFProperties")
+def _active_zone(path: str, what: str):
+    """The one active top-level zone entry of a zone-list file: (name,
+    body), or None without the file or an active zone."""
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -235,8 +233,51 @@ def read_mrf(case_dir: str, n_cells: int, names: List[str]) -> Optional[MRFZone]
     if not zones:
         return None
     if len(zones) > 1:
-        raise IcsfError(-1, "MRFProperties: one active MRF zone is supported")
-    zname, body = zones[0]
+        raise IcsfError(-1, f"{what}: one active zone is supported")
+    return zones[0]
+
+
+def _patches(body: str, key: str, names: List[str]) -> List[int]:
+    """A wordRes entry (names or regular expressions) -> patch indices."""
+    out = []
+    mn = re.search(r"\b" + key + r"\s*\(([^)]*)\)\s*;", body)
+    if mn:
+        for pat in mn.group(1).split():
+            pat = pat.strip('"')
+            out += [i for i, n in enumerate(names) if re.fullmatch(pat, n) and i not in out]
+    return sorted(out)
+
+
+def read_translating(case_dir: str, n_cells: int, names: List[str]) -> Optional[TranslatingZone]:
+    """constant/MRFTranslatingProperties (MRFTranslatingZone.C:240-296): the
+    active zone's cellZone, frameVelocity (a constant: `frameVelocity (u v
+    w);` or `frameVelocity constant (u v w);`) and nonMovingPatches; None
+    without the file or an active zone."""
+    z = _active_zone(os.path.join(case_dir, "constant", "MRFTranslatingProperties"), "MRFTranslatingProperties")
+    if z is None:
+        return None
+    zname, body = z
+    mt = re.search(r"\bcellZone\s+([\w.:-]+)\s*;", body)
+    if not mt:
+        raise IcsfError(-2, f"MRFTranslatingProperties {zname}: missing cellZone")
+    cells = _cell_zone(os.path.join(case_dir, "constant", "polyMesh"), mt.group(1), n_cells)
+    mv = re.search(r"\bframeVelocity\s+(?:constant\s+)?\(\s*([^)]*)\)\s*;", body)
+    if not mv:
+        raise IcsfError(-2, f"MRFTranslatingProperties {zname}: frameVelocity must be a constant vector")
+    return TranslatingZone(cells=cells, velocity=np.array([float(x) for x in mv.group(1).split()], np.float64),
+                           non_moving=_patches(body, "nonMovingPatches", names))
+
+
+def read_mrf(case_dir: str, n_cells: int, names: List[str]) -> Optional[MRFZone]:
+    """constant/MRFProperties (IOMRFCoupledZoneList / MRFCoupledZone.C:
+    240-296): the active zone's cellZone, origin, axis, omega (a constant:
+    `omega 100;` or `omega constant 100;`) and nonRotatingPatches (names or
+    regular expressions, matched against the patch names); None without the
+    file or without an active zone.  One active zone is supported."""
+    z = _active_zone(os.path.join(case_dir, "constant", "MRFProperties"), "MRFProperties")
+    if z is None:
+        return None
+    zname, body = z
     mt = re.search(r"\bcellZone\s+([\w.:-]+)\s*;", body)
     if not mt:
         raise IcsfError(-2, f"MRFProperties {zname}: missing cellZone")
@@ -244,15 +285,8 @@ def read_mrf(case_dir: str, n_cells: int, names: List[str]) -> Optional[MRFZone]
     mo = re.search(r"\bomega\s+(?:constant\s+)?([-+0-9.eE]+)\s*;", body)
     if not mo:
         raise IcsfError(-2, f"MRFProperties {zname}: omega must be a constant")
-    non_rot = []
-    mn = re.search(r"\bnonRotatingPatches\s*\(([^)]*)\)\s*;", body)
-    if mn:
-        for pat in mn.group(1).split():
-            pat = pat.strip('"')
-            hits = [i for i, n in enumerate(names) if re.fullmatch(pat, n)]
-            non_rot += [i for i in hits if i not in non_rot]
     return MRFZone(cells=cells, origin=_vector(body, "origin"), axis=_vector(body, "axis"),
-                   omega=float(mo.group(1)), non_rotating=sorted(non_rot))
+                   omega=float(mo.group(1)), non_rotating=_patches(body, "nonRotatingPatches", names))
 
 
 def read_case(case_dir: str, time: str = "0", thermo: Optional[str] = None, fvSchemes: Optional[str] = None,
@@ -284,7 +318,8 @@ def read_case(case_dir: str, time: str = "0", thermo: Optional[str] = None, fvSc
     U = np.asarray(fU.internal, np.float64)
     case = Case(os.path.basename(os.path.abspath(case_dir)), mesh, p / (R * T), U.copy(), T.copy(), bcs,
                 text(thermo, "constant", "thermophysicalProperties"), text(fvSchemes, "system", "fvSchemes"),
-                text(fvSolution, "system", "fvSolution"), mrf=read_mrf(case_dir, n, names))
+                text(fvSolution, "system", "fvSolution"), mrf=read_mrf(case_dir, n, names),
+                translating=read_translating(case_dir, n, names))
     return case, names
 
 

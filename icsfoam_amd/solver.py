@@ -91,6 +91,8 @@ class dbnsCase:
                 self.set_boundary_values(i, bc.p_init, bc.U_init, bc.T_init)
         if getattr(case, "mrf", None) is not None:
             self.set_mrf(case.mrf)
+        if getattr(case, "translating", None) is not None:
+            self.set_translating(case.translating)
         self.set_fields(case.rho, case.U, case.T)
         return self
 
@@ -121,6 +123,15 @@ class dbnsCase:
         nr = as_i32(np.asarray(zone.non_rotating, np.int32).reshape(-1))
         check(lib().icsf_set_mrf(self._h, int(cells.shape[0]), ptr(cells), ptr(as_f64(zone.origin)),
                                  ptr(as_f64(zone.axis)), float(zone.omega), int(nr.shape[0]), ptr(nr)))
+
+    def set_translating(self, zone):
+        """One translating zone (cases.TranslatingZone: bool cell mask,
+        frame velocity, non-moving patch indices); after the boundary
+        conditions and before the fields (icsf_set_translating_frame)."""
+        cells = as_i32(np.flatnonzero(np.asarray(zone.cells, bool)))
+        nm = as_i32(np.asarray(zone.non_moving, np.int32).reshape(-1))
+        check(lib().icsf_set_translating_frame(self._h, int(cells.shape[0]), ptr(cells), ptr(as_f64(zone.velocity)),
+                                               int(nm.shape[0]), ptr(nm)))
 
     def fields(self):
         n = self.nCells
@@ -311,6 +322,12 @@ class hbCase:
                     self.set_boundary_values(J, i, bc.p_init, bc.U_init, bc.T_init)
         if getattr(c0, "mrf", None) is not None:
             self.set_mrf(c0.mrf)
+        if getattr(c0, "translating", None) is not None:
+            z = c0.translating
+            cells = as_i32(np.flatnonzero(np.asarray(z.cells, bool)))
+            nm = as_i32(np.asarray(z.non_moving, np.int32).reshape(-1))
+            check(lib().icsf_hb_set_translating_frame(self._h, int(cells.shape[0]), ptr(cells),
+                                                      ptr(as_f64(z.velocity)), int(nm.shape[0]), ptr(nm)))
         for J, c in enumerate(instances):
             self.set_fields(J, c.rho, c.U, c.T)
         return self

@@ -320,11 +320,23 @@ int icsf_get_tuning(icsf_case* c, int* params);
  * case; the residual VJP refuses such cases. */
 int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const double* origin, const double* axis,
                  double omega, int nNonRotating, const int* nonRotatingPatches);
-/* The same zone for every instance of a harmonic-balance case (the
+/* One translating zone (replaces MRFTranslatingZoneList /
+ * MRFTranslatingProperties, MRFTranslatingZone.C; outerLoop.H:7,21): the
+ * zone's cells, frameVelocity and nonMovingPatches.  The same face sets as
+ * the rotating zone; its faces add frameVelocity . Sf/|Sf| to the face
+ * velocity the schemes see, its moving patches (fixedValue U) take the
+ * frame velocity (after a rotating zone's), no Coriolis term.  One per case,
+ * beside at most one rotating zone; the same storage switch and VJP
+ * refusal as icsf_set_mrf. */
+int icsf_set_translating_frame(icsf_case* c, int nZoneCells, const int* zoneCells, const double* velocity,
+                               int nNonMoving, const int* nonMovingPatches);
+/* The same zones for every instance of a harmonic-balance case (the
  * instances share the mesh and the frame); after the instances' boundary
  * conditions, before their fields. */
 int icsf_hb_set_mrf(icsf_hb* h, int nZoneCells, const int* zoneCells, const double* origin, const double* axis,
                     double omega, int nNonRotating, const int* nonRotatingPatches);
+int icsf_hb_set_translating_frame(icsf_hb* h, int nZoneCells, const int* zoneCells, const double* velocity,
+                                  int nNonMoving, const int* nonMovingPatches);
 /* Sets the launch parameters (the layout of icsf_get_tuning; params[5]
  * ignored) and marks the case tuned, so the autotuner does not run (no
  * counterpart in the reference; for pinning a measured choice).

@@ -43,6 +43,7 @@
 #include <cstdint>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <numeric>
 #include <vector>
 
@@ -352,6 +353,15 @@ struct LduBlock {
     }
 };
 
+// One frame zone's contribution (set-up of the rotating or the translating
+// zone): faceU & Sf/|Sf| per face, the moving boundary faces and their U.
+struct FramePart {
+    bool on = false;
+    vec un;                // faceU & Sf/|Sf| per face
+    std::vector<char> incl;  // moving boundary faces
+    vec Ub;                // their U
+};
+
 struct Case {
     Mesh* m;
     Settings s;
@@ -387,6 +397,8 @@ struct Case {
     bool mrf = false;
     vec umrf, omegaC, mrfUb;
     std::vector<char> mrfIncl;
+    // the rotating and the translating zone's parts (set-up)
+    FramePart frot, ftr;
 };
 
 inline double gammaOf(const Case& c) { return c.s.gamma; }
@@ -1044,8 +1056,8 @@ void residual(Case& k, vec& b) {
     }
     // MRF: the momentum source -(Omega ^ rho U) V that addMRFSource adds to
     // the coupled system's source before every solve
-    // (convectiveFluxScheme.C:123-129)
-    if (k.mrf)
+    // (convectiveFluxScheme.C:123-129); a rotating zone only
+    if (k.mrf && k.frot.on)
         for (int c = 0; c < nC; ++c) {
             const V3 rU{k.rho[c] * k.U[3 * c], k.rho[c] * k.U[3 * c + 1], k.rho[c] * k.U[3 * c + 2]};
             const V3 s = cross(v3(&k.omegaC[3 * c]), rU) * m.V[c];
@@ -1428,7 +1440,7 @@ void assemble(Case& k) {
     }
     // addMRFSource (convectiveFluxScheme.C:123-138): the derivative of
     // -(Omega ^ rhoU) V on the rhoU diagonal block
-    if (k.mrf)
+    if (k.mrf && k.frot.on)
         for (int c = 0; c < nC; ++c) {
             const double* om = &k.omegaC[3 * c];
             double* t = &k.VV.diag[9 * c];
@@ -2543,28 +2555,25 @@ void* orc_case_new(void* mesh, const int* is, const double* rs) {
 }
 void orc_case_free(void* h) { delete (Case*)h; }
 
-// One MRF zone (MRFCoupledZone.C:42-189 setMRFFaces, :240-296 constructor,
-// :549-577 faceU, :580-602 correctBoundaryVelocity): zone cells, origin,
-// axis (normalised), omega [rad/s], non-rotating patches.  Face types: 1
-// internal faces with a zone cell on either side and the zone cells' faces
-// on other non-empty, non-coupled patches; 2 the zone cells' faces on
-// coupled or non-rotating patches; 0 the rest.  faceU = Omega ^ (Cf -
-// origin) on types 1 and 2; the rotating patches (type 1 boundary faces)
-// take U = faceU.  Returns 1 if a rotating patch's U is not fixedValue (the
-// frame assigns its value, which only a fixedValue condition keeps), 0 on
-// success.
-int orc_set_mrf(void* h, int nZone, const int* zone, const double* origin, const double* axis, double omega,
-                int nNonRot, const int* nonRot) {
-    Case& k = *(Case*)h;
+// Frame zones: one rotating zone (MRFCoupledZone.C:42-189 setMRFFaces,
+// :240-296 constructor, :549-577 faceU, :580-602 correctBoundaryVelocity)
+// and one translating zone (MRFTranslatingZone.C, the same face sets, faceU =
+// the frame velocity).  Face types: 1 internal faces with a zone cell on
+// either side and the zone cells' faces on other non-empty, non-coupled
+// patches; 2 the zone cells' faces on coupled or non-rotating (non-moving)
+// patches; 0 the rest.  faceU on types 1 and 2; the moving patches (type 1
+// boundary faces) take U = faceU, the translating zone's after the
+// rotating one's (outerLoop.H:6-7).  The face velocity the schemes see is
+// MRF.faceU() & n + MRFTrans.faceU() & n (outerLoop.H:18-21); Omega (the
+// Coriolis terms) comes from the rotating zone only.
+// returns 1 if a moving patch's U condition is not fixedValue
+int framePart(const Case& k, const int* zone, int nZone, const int* nonMov, int nNonMov,
+              const std::function<V3(int)>& faceU, FramePart& out) {
     const Mesh& m = *k.m;
     const int nB = m.nF - m.nI;
-    V3 ax = v3(axis);
-    const double am = mag(ax);
-    ax = am < ROOTVSMALL ? V3{0, 0, 0} : ax / am;
-    const V3 Om = omega * ax, o = v3(origin);
     std::vector<char> zc(m.nC, 0), excl(m.pStart.size(), 0);
     for (int i = 0; i < nZone; ++i) zc[zone[i]] = 1;
-    for (int i = 0; i < nNonRot; ++i) excl[nonRot[i]] = 1;
+    for (int i = 0; i < nNonMov; ++i) excl[nonMov[i]] = 1;
     std::vector<int> type(m.nF, 0);
     for (int f = 0; f < m.nI; ++f)
         if (zc[m.own[f]] || zc[m.nei[f]]) type[f] = 1;
@@ -2577,23 +2586,77 @@ int orc_set_mrf(void* h, int nZone, const int* zone, const double* origin, const
     for (size_t pi = 0; pi < m.pStart.size(); ++pi)
         for (int f = m.pStart[pi]; f < m.pStart[pi] + m.pSize[pi]; ++f)
             if (type[f] == 1 && k.bcs[pi].uT != BC_FIXED) return 1;
-    k.mrf = true;
-    k.umrf.assign(m.nF, 0.0);
-    k.omegaC.assign(3 * (size_t)m.nC, 0.0);
-    k.mrfUb.assign(3 * (size_t)nB, 0.0);
-    k.mrfIncl.assign(nB, 0);
+    out.on = true;
+    out.un.assign(m.nF, 0.0);
+    out.incl.assign(nB, 0);
+    out.Ub.assign(3 * (size_t)nB, 0.0);
     for (int f = 0; f < m.nF; ++f) {
         V3 fu{0, 0, 0};
-        if (type[f]) fu = cross(Om, v3(&m.Cf[3 * f]) - o);
-        k.umrf[f] = dot(fu, v3(&m.Sf[3 * f]) / m.magSf[f]);
+        if (type[f]) fu = faceU(f);
+        out.un[f] = dot(fu, v3(&m.Sf[3 * f]) / m.magSf[f]);
         if (f >= m.nI && type[f] == 1) {
             const int b = f - m.nI;
-            k.mrfIncl[b] = 1;
-            k.mrfUb[3 * b] = fu.x; k.mrfUb[3 * b + 1] = fu.y; k.mrfUb[3 * b + 2] = fu.z;
+            out.incl[b] = 1;
+            out.Ub[3 * b] = fu.x; out.Ub[3 * b + 1] = fu.y; out.Ub[3 * b + 2] = fu.z;
         }
     }
-    for (int c = 0; c < m.nC; ++c)
-        if (zc[c]) { k.omegaC[3 * c] = Om.x; k.omegaC[3 * c + 1] = Om.y; k.omegaC[3 * c + 2] = Om.z; }
+    return 0;
+}
+
+// the case's frame arrays from its parts
+void frameCompose(Case& k) {
+    const Mesh& m = *k.m;
+    const int nB = m.nF - m.nI;
+    const FramePart &R = k.frot, &T = k.ftr;
+    k.mrf = R.on || T.on;
+    k.umrf.assign(m.nF, 0.0);
+    for (int f = 0; f < m.nF; ++f)
+        k.umrf[f] = R.on && T.on ? R.un[f] + T.un[f] : (R.on ? R.un[f] : T.un[f]);
+    k.mrfIncl.assign(nB, 0);
+    k.mrfUb.assign(3 * (size_t)nB, 0.0);
+    for (int b = 0; b < nB; ++b)
+        for (const FramePart* P : {&R, &T})
+            if (P->on && P->incl[b]) {
+                k.mrfIncl[b] = 1;
+                for (int d = 0; d < 3; ++d) k.mrfUb[3 * b + d] = P->Ub[3 * b + d];
+            }
+    if (k.omegaC.empty()) k.omegaC.assign(3 * (size_t)m.nC, 0.0);
+}
+
+// The rotating zone: cells, origin, axis (normalised), omega [rad/s],
+// non-rotating patches.  Returns 1 if a rotating patch's U is not
+// fixedValue (the frame assigns its value, which only a fixedValue
+// condition keeps), 0 on success.
+int orc_set_mrf(void* h, int nZone, const int* zone, const double* origin, const double* axis, double omega,
+                int nNonRot, const int* nonRot) {
+    Case& k = *(Case*)h;
+    const Mesh& m = *k.m;
+    V3 ax = v3(axis);
+    const double am = mag(ax);
+    ax = am < ROOTVSMALL ? V3{0, 0, 0} : ax / am;
+    const V3 Om = omega * ax, o = v3(origin);
+    FramePart P;
+    if (framePart(k, zone, nZone, nonRot, nNonRot, [&](int f) { return cross(Om, v3(&m.Cf[3 * f]) - o); }, P))
+        return 1;
+    k.frot = P;
+    k.omegaC.assign(3 * (size_t)m.nC, 0.0);
+    for (int i = 0; i < nZone; ++i) {
+        const int c = zone[i];
+        k.omegaC[3 * c] = Om.x; k.omegaC[3 * c + 1] = Om.y; k.omegaC[3 * c + 2] = Om.z;
+    }
+    frameCompose(k);
+    return 0;
+}
+// The translating zone (MRFTranslatingZone: cellZone, frameVelocity,
+// nonMovingPatches); same return as orc_set_mrf.
+int orc_set_translating(void* h, int nZone, const int* zone, const double* velocity, int nNonMov,
+                        const int* nonMov) {
+    Case& k = *(Case*)h;
+    const V3 V = v3(velocity);
+    FramePart P;
+    if (framePart(k, zone, nZone, nonMov, nNonMov, [&](int) { return V; }, P)) return 1;
+    k.ftr = P;
+    frameCompose(k);
     return 0;
 }
 // The MRF set-up: umrf [nF], omegaC [3 nC], rotating boundary faces [nB]

@@ -52,6 +52,7 @@ _SIG = {
     "orc_residual_jvp": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "orc_residual_jac_dense": (None, [_vp, _vp, _vp, _vp, _vp]),
     "orc_set_mrf": (_i, [_vp, _i, _vp, _vp, _vp, C.c_double, _i, _vp]),
+    "orc_set_translating": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
     "orc_face_flux_mrf": (None, [_i, _i, C.c_double, C.c_double, C.c_double, _vp, C.c_double, _vp]),
     "orc_get_mrf": (None, [_vp, _vp, _vp, _vp, _vp]),
     "orc_hb_new": (_vp, [_i, _vp, _vp]),
@@ -237,6 +238,8 @@ class OracleCase:
                 lib().orc_set_bc_values(self.h, i, *[_p(a) for a in init])
         if case.mrf is not None:
             self.set_mrf(case.mrf)
+        if getattr(case, "translating", None) is not None:
+            self.set_translating(case.translating)
         self.set_fields(case.rho, case.U, case.T)
 
     def set_mrf(self, z):
@@ -249,6 +252,16 @@ class OracleCase:
         if rc:
             raise ValueError("MRF: a patch that rotates with the zone needs a fixedValue U condition "
                              "(list it in non_rotating otherwise)")
+
+    def set_translating(self, z):
+        """One translating zone (cases.TranslatingZone); before set_fields."""
+        cells = _i32(np.flatnonzero(np.asarray(z.cells, bool)))
+        nm = _i32(z.non_moving)
+        rc = lib().orc_set_translating(self.h, int(cells.shape[0]), _p(cells), _p(_f64(z.velocity)),
+                                       int(nm.shape[0]), _p(nm))
+        if rc:
+            raise ValueError("translating frame: a moving patch needs a fixedValue U condition "
+                             "(list it in non_moving otherwise)")
 
     def __del__(self):
         if getattr(self, "h", None):

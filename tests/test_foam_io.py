@@ -185,3 +185,16 @@ def test_mrf_properties_and_cell_zones_read(tmp_path):
     np.testing.assert_array_equal(b1, b2)
     b0, _ = OracleCase(dataclasses.replace(r, mrf=None)).residual()
     assert np.abs(b1 - b0).max() > 0
+    assert r.translating is None
+    # a translating zone beside it (MRFTranslatingProperties)
+    (case_dir / "constant" / "MRFTranslatingProperties").write_text(
+        "FoamFile { version 2.0; format ascii; class dictionary; object MRFTranslatingProperties; }\n"
+        "slide { cellZone rotor; active true; nonMovingPatches (\"in.*\" outlet walls); "
+        "frameVelocity constant (12.5 -3 0.25); }\n")
+    r2, _ = foam.read_case(str(case_dir), thermo=c.thermo, fvSchemes=c.fvSchemes, fvSolution=c.fvSolution)
+    t = r2.translating
+    np.testing.assert_array_equal(t.cells, zone)
+    np.testing.assert_array_equal(t.velocity, [12.5, -3.0, 0.25])
+    assert t.non_moving == z.non_rotating
+    bt, _ = OracleCase(r2).residual()
+    assert np.abs(bt - b1).max() > 0

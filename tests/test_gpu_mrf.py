@@ -2,7 +2,8 @@
 
 A jittered box with half of its cells rotating about an oblique axis
 (viscous Roe and AUSM+up second order, HLLC first order; slip walls that
-do not rotate, or no-slip walls that rotate with the zone) and a rotational
+do not rotate, or no-slip walls that rotate with the zone), the same half
+translating (alone, and beside the other half rotating), and a rotational
 cyclic sector rotating about its own axis (Roe second order):
 
 * face fluxes, residual (with the Coriolis source), rPseudoDeltaT,
@@ -67,7 +68,20 @@ def _sector():
     return prepare_case(c, permute_seed=19)
 
 
+def _translating(both):
+    """The box of _box with moving no-slip walls, its zone translating (and,
+    with `both`, the other half rotating as well: the face velocities add)."""
+    c = _box("ROE", rotating_walls=True)
+    z = c.mrf
+    t = cases.TranslatingZone(cells=z.cells.copy(), velocity=np.array([40.0, -25.0, 15.0]),
+                              non_moving=list(z.non_rotating))
+    rot = dataclasses.replace(z, cells=~z.cells, non_rotating=list(range(len(c.mesh.patches)))) if both else None
+    return dataclasses.replace(c, mrf=rot, translating=t)
+
+
 CASES = {
+    "box_translating": lambda: _translating(False),
+    "box_translating_and_rotating": lambda: _translating(True),
     "box_roe_viscous": lambda: _box("ROE", rotating_walls=False),
     "box_ausm_rotating_walls": lambda: _box("AUSMPlusUp", rotating_walls=True),
     "box_hllc_upwind": lambda: _box("HLLC", rotating_walls=True, recon="upwind", mu=0.0),
@@ -96,7 +110,7 @@ def test_mrf_fluxes_residual_and_jacobian_bitwise(pair):
     if c.mesh.has_cyclic:
         np.testing.assert_array_equal(g.interfaces(), o.interfaces())
     # the frame terms are present: the same case without the zone differs
-    o0 = OracleCase(dataclasses.replace(c, mrf=None))
+    o0 = OracleCase(dataclasses.replace(c, mrf=None, translating=None))
     b0, _ = o0.residual()
     assert rel_err(bo, b0) > 1e-3
 

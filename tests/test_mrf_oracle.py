@@ -16,7 +16,8 @@ and the boundary Jacobian takes U_b.Sf - um |Sf| (:93-94).
   u_r = U.n - um, except HLLC on its supersonic branches, which drops
   p um (the reference's form);
 * a fluid at rest in a rotating zone of a closed box is steady: the frame
-  fluxes sum to zero over every closed cell;
+  fluxes sum to zero over every closed cell; a uniform flow in a box that
+  translates with it (MRFTranslatingZone) is steady with no mass flux;
 * set-up: zone faces, frame velocity, rotating patches (fixedValue only);
 * the assembled matrix equals the complex-step derivative of a first-order
   Lax-Friedrichs model with the frame terms;
@@ -99,6 +100,29 @@ def test_fluid_at_rest_in_a_rotating_zone_is_steady(flux):
     assert np.abs(ph).max() > 10.0  # the frame fluxes are not small
     scale = np.abs(phE).max()
     assert np.abs(b).max() < 1e-13 * scale
+
+
+def test_uniform_flow_at_rest_in_a_translating_frame_is_steady():
+    """A uniform flow U in a box whose cells all translate with the frame
+    velocity U (every patch moves with it: fixedValue U, which the frame
+    sets to U): the transport velocity U.n - V.n vanishes on every face, the
+    fluxes reduce to p n and p U.n, and the residual is zero; a wrong sign
+    of the frame velocity leaves 2 rho U.n |S| on the faces."""
+    c = cases.case_box(n=(6, 5, 4), jitter=0.0, flux="ROE", mu=0.0, noise=0.0)
+    nC = c.mesh.n_cells
+    U0 = np.array([120.0, -40.0, 25.0])
+    bcs = [cases.BoundarySpec(cases.BC_ZERO_GRADIENT, cases.BC_FIXED_VALUE, cases.BC_ZERO_GRADIENT,
+                              U_val=np.zeros((p.size, 3))) for p in c.mesh.patches]
+    c = dataclasses.replace(c, U=np.tile(U0, (nC, 1)), T=np.full(nC, 300.0), rho=np.full(nC, 1.1), bcs=bcs)
+    for flux in ("AUSMPlusUp", "ROE", "HLLC"):
+        cc = dataclasses.replace(c, fvSchemes=c.fvSchemes.replace("fluxScheme ROE", "fluxScheme " + flux))
+        z = cases.TranslatingZone(cells=np.ones(nC, bool), velocity=U0.copy())
+        o = O.OracleCase(dataclasses.replace(cc, translating=z))
+        b, _ = o.residual()
+        ph, _, phE = o.calcFlux()
+        np.testing.assert_array_equal(o.boundary()["U"], np.tile(U0, (c.mesh.n_faces - c.mesh.n_internal, 1)))
+        assert np.abs(b).max() < 1e-13 * np.abs(phE).max(), flux
+        assert np.abs(ph).max() < 1e-9  # no mass crosses a face moving with the flow
 
 
 def _mrf_case(flux="ROE", mu=0.0, rotating_walls=False, upwind=True, jitter=0.15, n=(4, 3, 3)):
@@ -188,9 +212,9 @@ def _dense(d, u, l, m):
     return A
 
 
-@pytest.mark.parametrize("rotating_walls", [False, True])
+@pytest.mark.parametrize("frame", ["rotating", "rotating_walls", "translating_walls"])
 @pytest.mark.parametrize("mu", [0.0, 1.8e-5])
-def test_jacobian_is_derivative_of_lf_model_in_the_frame(rotating_walls, mu):
+def test_jacobian_is_derivative_of_lf_model_in_the_frame(frame, mu):
     """The assembled matrix equals the complex-step derivative of the
     first-order Lax-Friedrichs model with the frame terms:
     internal faces  |S|/2 (F(W_P)+F(W_N)) - um|S| (w W_P + (1-w) W_N)
@@ -199,9 +223,14 @@ def test_jacobian_is_derivative_of_lf_model_in_the_frame(rotating_walls, mu):
                     rhoE_b u_rel + p_b U_b.Sf] (u_rel = U_b.Sf - um|Sf|)
                     + |S|/2 lam_b W_P
     cells           V rDT W_c + (Omega ^ rhoU_c) V.
-    Half of the box rotates about an oblique axis; walls either
-    non-rotating slip walls or rotating no-slip walls."""
-    c, g = _mrf_case(mu=mu, rotating_walls=rotating_walls)
+    Half of the box rotates about an oblique axis (walls either
+    non-rotating slip walls or rotating no-slip walls) or translates (no
+    Coriolis term; no-slip walls moving with it)."""
+    c, g = _mrf_case(mu=mu, rotating_walls=frame != "rotating")
+    if frame == "translating_walls":
+        # the same zone translating (no Coriolis terms), walls moving with it
+        c = dataclasses.replace(c, mrf=None, translating=cases.TranslatingZone(
+            cells=c.mrf.cells, velocity=np.array([40.0, -25.0, 15.0]), non_moving=c.mrf.non_rotating))
     o = O.OracleCase(c)
     mrf = o.mrf()
     bnd = o.boundary()
@@ -273,7 +302,7 @@ def test_jacobian_is_derivative_of_lf_model_in_the_frame(rotating_walls, mu):
     A = _dense(d, u, l, m)
     assert rel_err(A, J) < 1e-9
     # the frame terms matter at this tolerance
-    o0 = O.OracleCase(dataclasses.replace(c, mrf=None))
+    o0 = O.OracleCase(dataclasses.replace(c, mrf=None, translating=None))
     o0.residual()
     A0 = _dense(*o0.createJacobian(), m)
     assert rel_err(A0, J) > 1e-4
