@@ -56,6 +56,8 @@ SIGNATURES = {
     "icsf_hb_set_mrf": (_i, [_vp, _i, _vp, _vp, _vp, C.c_double, _i, _vp]),
     "icsf_set_translating_frame": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
     "icsf_hb_set_translating_frame": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
+    "icsf_hb_phase_lag_operator": (_i, [_i, _vp, _vp, _i, _vp, C.c_double, _vp]),
+    "icsf_hb_set_phase_lag": (_i, [_vp, _i, _vp, _i, _vp, _vp]),
     "icsf_colour_cells": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
     "icsf_colour_major_order": (_i, [_i, _vp, _i, _vp]),
     "icsf_renumber_mesh": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -145,6 +145,24 @@ class TranslatingZone:
 
 
 @dataclasses.dataclass
+class PhaseLagPair:
+    """A phase-lagged cyclic pair of a harmonic-balance case (the
+    reference's phaseLagCyclic patch type: IBPA, cylCoords,
+    phaseLagCyclicPolyPatch.C:79-90).  owner: the pair's lower patch index,
+    whose IBPA holds (the neighbour side takes minus it); the HB zone's
+    angular frequencies, harmonics and instants give the time shift; axis
+    and centre: the pair's rotation axis and centre (cylCoords)."""
+    owner: int
+    ibpa: float
+    frequencies: np.ndarray
+    harmonics: np.ndarray
+    instants: np.ndarray
+    cyl_coords: bool = False
+    axis: np.ndarray = dataclasses.field(default_factory=lambda: np.array([0.0, 0.0, 1.0]))
+    centre: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
+
+
+@dataclasses.dataclass
 class Case:
     name: str
     mesh: PolyMesh
@@ -158,6 +176,7 @@ class Case:
     cell_order: Optional[np.ndarray] = None   # new -> original cell index
     mrf: Optional[MRFZone] = None
     translating: Optional[TranslatingZone] = None
+    phase_lag: Optional[PhaseLagPair] = None
 
 
 # --------------------------------------------------------------------------

@@ -208,6 +208,33 @@ bool hbOperator(const std::vector<double>& omegas, const std::vector<double>& t,
     return true;
 }
 
+bool hbLagOperator(const std::vector<double>& omegas, const std::vector<double>& t, double ibpa,
+                   std::vector<double>& Dl, std::string& err) {
+    const size_t nO = t.size(), nT = omegas.size();
+    const std::vector<cplx> Einv = synthesis(omegas, t);  // [nO][nT]
+    std::vector<cplx> E;                                   // [nT][nO]
+    if (!hbPinv(omegas, t, Einv, E, err)) return false;
+    // M = diag(1, e^{i n IBPA} (n = 1..nH), e^{-i n IBPA} at nT - n): the
+    // product M E scales row l of E by M[l][l] (the off-diagonal zeros add
+    // nothing to the sums)
+    const size_t nH = (nT - 1) / 2;
+    std::vector<cplx> mdiag(nT, cplx(0, 0));
+    mdiag[0] = cplx(1, 0);
+    for (size_t n = 1; n <= nH; ++n) {
+        const cplx e(std::cos((double)n * ibpa), std::sin((double)n * ibpa));
+        mdiag[n] = e;
+        mdiag[nT - n] = std::conj(e);
+    }
+    Dl.assign(nO * nO, 0.0);
+    for (size_t l = 0; l < nO; ++l)
+        for (size_t m = 0; m < nO; ++m) {
+            cplx d(0, 0);
+            for (size_t n = 0; n < nT; ++n) d += Einv[l * nT + n] * (mdiag[n] * E[n * nO + m]);
+            Dl[l * nO + m] = d.real();
+        }
+    return true;
+}
+
 bool hbReconstructWeights(const std::vector<double>& omegas, const std::vector<double>& t, double time,
                           std::vector<double>& w, std::string& err) {
     const size_t nO = t.size(), nT = omegas.size();

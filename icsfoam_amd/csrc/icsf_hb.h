@@ -35,6 +35,15 @@ std::vector<double> hbInstants(int nInstants, double selectedPeriod, const std::
 bool hbOperator(const std::vector<double>& omegas, const std::vector<double>& t, std::vector<double>& D,
                 std::string& err);
 
+// Time shift of a phase-lagged cyclic patch side
+// (phaseLagCyclicFvPatchField.C:289-326): Dl[nO][nO] = Re(E^-1 M E), E =
+// pinv(E^-1), M = diag(1, e^{i n IBPA} (n = 1..nH), e^{-i n IBPA} at
+// nT - n) over the omega list's indices.  A field sampled at the instants
+// is moved by IBPA / omega_1 in time: sum_m Dl[l][m] x(t_m) = x(t_l +
+// IBPA/omega_1) for x in the span of the omega list.
+bool hbLagOperator(const std::vector<double>& omegas, const std::vector<double>& t, double ibpa,
+                   std::vector<double>& Dl, std::string& err);
+
 // Weights of the Fourier reconstruction at `time` (HBZone::reconstruct,
 // HBZoneTemplates.C:328-365): w[m] = Re(sum_n exp(i omega_n time) E[n][m]),
 // E = pinv(E^-1); a field at that time is sum_m w[m] field_m.

@@ -67,12 +67,21 @@ __device__ __forceinline__ void reconRows(const MeshView& m, const StateView& s,
     }
 }
 
-// Far-side value of field k on cyclic face k-th (cyclicFvPatchField::
+// Whether cyclic face kc is phase-lagged (its far values in s.plFar).
+__device__ __forceinline__ bool plFace(const MeshView& m, const StateView& s, int kc) {
+    return s.plFar != nullptr && m.plSide[kc] >= 0;
+}
+
+// Far-side value of field k on the kc-th cyclic face (cyclicFvPatchField::
 // patchNeighbourField as patched by the reference,
 // originalOFFiles/constraintFvPatchFields/cyclic/cyclicFvPatchField.C:127-188):
 // the far cell's value, and for the velocity components ("U.component(i)")
-// component i of forwardT & U of the far cell.
-__device__ __forceinline__ double cycNbrValue(const StateView& s, const double* R, int n, int k) {
+// component i of forwardT & U of the far cell; on a phase-lagged face the
+// lagged record (phaseLagCyclicFvPatchField.C:343-396).
+__device__ __forceinline__ double cycNbrValue(const MeshView& m, const StateView& s, int kc, int k) {
+    if (plFace(m, s, kc)) return s.plFar[(size_t)kc * PLW + k];
+    const double* R = m.cycR + 9 * (size_t)kc;
+    const int n = m.cycCell[kc];
     if (k >= F_UX && k <= F_UZ) {
         const V3 u = rot3(R, cellU(s.fld, n));
         return k == F_UX ? u.x : (k == F_UY ? u.y : u.z);
@@ -88,7 +97,7 @@ __device__ __forceinline__ void reconCyclic(const MeshView& m, const StateView& 
     const double* Rt = m.cycR + 9 * (size_t)kc;
     const int n = m.cycCell[kc];
     double l = s.fld[(size_t)o * NFLD + k];
-    double r = cycNbrValue(s, Rt, n, k);
+    double r = cycNbrValue(m, s, kc, k);
     if (recon == 1) {
         const V3 fc = ld3(m.Cf + 3 * (size_t)f);
         const double* go = s.grd + ((size_t)o * NFLD + k) * 3;

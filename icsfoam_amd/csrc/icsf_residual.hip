@@ -79,6 +79,7 @@ __global__ void k_update(MeshView m, Phys ph, StateView s, const double* __restr
     }
     s.T[c] = T;
     s.ecalc[c] = rE / rho - 0.5 * magSqr(U);
+    if (s.e) s.e[c] = e;
 }
 
 // Boundary values of p, U and T after a state update, on every non-empty,
@@ -272,12 +273,10 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
             // coupled linear interpolation w v_P + (1 - w) v_far; max/min
             // over the far-side value (cellLimitedGrad, coupled patches)
             const int kc = m.cycOf[f - m.nI];
-            const double* R = m.cycR + 9 * (size_t)kc;
-            const int n = m.cycCell[kc];
             const double w = m.w[f];
 #pragma unroll
             for (int k = 0; k < NG; ++k) {
-                const double vn = cycNbrValue(s, R, n, k);
+                const double vn = cycNbrValue(m, s, kc, k);
                 const double vf = w * vc[k] + (1.0 - w) * vn;
                 const V3 t = S * vf;
                 gx[k] = gx[k] + t.x; gy[k] = gy[k] + t.y; gz[k] = gz[k] + t.z;
@@ -402,7 +401,7 @@ __device__ __forceinline__ void faceFlux(const MeshView& m, const Phys& ph, cons
         if (ph.flux == FLUX_HLLC) reconCyclic(m, s, ph.recon, f, o, kc, F_E, eL, eR);
         else if (ph.flux == FLUX_ROE) { eL = xL; eR = xR; }
         else reconCyclic(m, s, ph.recon, f, o, kc, F_E, eL, eR);
-        gI = m.w[f] * ph.gamma + (1.0 - m.w[f]) * ph.gamma;
+        gI = m.w[f] * ph.gamma + (1.0 - m.w[f]) * (plFace(m, s, kc) ? s.plFar[(size_t)kc * PLW + PL_G] : ph.gamma);
     } else if (f < m.nI) {
         double L[NFLD], R[NFLD];
         reconRows(m, s, ph.recon, ph.flux == FLUX_ROE ? NFLD - 1 : NFLD, f, o, n, fo, fn, L, R);
@@ -521,11 +520,14 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
         // coupled interpolation of c and U (setCoAndDeltaT.H:44-56,78-89,
         // convectiveFluxScheme.C:509-524); reconstructed U, E for the
         // interface block (owner side, far side)
+        // (the far side lagged on a phase-lagged face)
         const int kc = m.cycOf[f - m.nI];
         const int nc = m.cycCell[kc];
         const double w = m.w[f];
-        const double cf = w * soundSpeed(ph, s.T[ow]) + (1.0 - w) * soundSpeed(ph, s.T[nc]);
-        const V3 Un = rot3(m.cycR + 9 * (size_t)kc, cellU(s.fld, nc));
+        const bool pl = plFace(m, s, kc);
+        const double* pf = pl ? s.plFar + (size_t)kc * PLW : nullptr;
+        const double cf = w * soundSpeed(ph, s.T[ow]) + (1.0 - w) * (pl ? pf[PL_C] : soundSpeed(ph, s.T[nc]));
+        const V3 Un = pl ? V3{pf[PL_UV], pf[PL_UV + 1], pf[PL_UV + 2]} : rot3(m.cycR + 9 * (size_t)kc, cellU(s.fld, nc));
         const V3 Uf = w * cellU(s.fld, ow) + (1.0 - w) * Un;
         const double lam = cf + fabs(dot(Uf, S / ms) - um);
         o[FT_LAM] = lam;
@@ -752,7 +754,7 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
             const int kc = m.cycOf[f - m.nI];
             const double ms = m.magSf[f];
             const V3 nrm = ld3(m.Sf + 3 * (size_t)f) / ms;
-            const double gf = m.w[f] * g + (1.0 - m.w[f]) * g;
+            const double gf = m.w[f] * g + (1.0 - m.w[f]) * (plFace(m, s, kc) ? s.plFar[(size_t)kc * PLW + PL_G] : g);
             double A[NB];
             fluxJacobian(gf, V3{t[FT_REC + 4], t[FT_REC + 5], t[FT_REC + 6]}, t[FT_REC + 7], nrm, A);
             const double hb = 0.5 * ms;

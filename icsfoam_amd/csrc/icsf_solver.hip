@@ -365,15 +365,28 @@ __global__ void k_bound_flag(int nC, double lb, const double* W, double* bad) {
     const double rhoMin = lb * rho;
     const double eMin = lb * (rE / rho - 0.5 * (ux * ux + uy * uy + uz * uz));
     const double eT = rE / rho - 0.5 * (ux * ux + uy * uy + uz * uz);
-    bad[c] = (rho < rhoMin || eT < eMin || eT < 1.0e-15) ? 1.0 : 0.0;
+    // bit 1: the density test, bit 2: the energy tests
+    bad[c] = (rho < rhoMin ? 1.0 : 0.0) + ((eT < eMin || eT < 1.0e-15) ? 2.0 : 0.0);
 }
+
+// The harmonic-balance instance J's phase-lagged far test: rho_ is lagged
+// over the instances as they stand (those before J already updated,
+// outerLoop.H:221-229), rhoMin, eTemp and eMin are not (their far cell's
+// bits).
+struct BoundLag {
+    const HBInst* I = nullptr;
+    int nO = 0, J = 0;
+    const int* side = nullptr;
+    const double* Dl = nullptr;
+    double lb = 0;
+};
 
 // boundLocalTimeStep.H:28-97: internal faces, then cyclic faces (the far
 // cell through patchNeighbourField).
 // Over the cell's internal and cyclic faces (the factor is a minimum of
 // 0.5 and 0.75, independent of the face order): the neighbours come from
 // the cell's matrix slots and cyclic-face list, read contiguously.
-__global__ void k_bound_apply(MeshView m, const double* bad, double* co, const int* halt) {
+__global__ void k_bound_apply(MeshView m, const double* bad, double* co, const int* halt, BoundLag bl = {}) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC || (halt && *halt)) return;
     double f = 1.0;
@@ -387,9 +400,17 @@ __global__ void k_bound_apply(MeshView m, const double* bad, double* co, const i
     }
     if (m.nCyc > 0)
         for (int q = m.cycRowOff[c]; q < m.cycRowOff[c + 1]; ++q) {
-            const int other = m.cycCell[m.cycRowK[q]];
+            const int kc = m.cycRowK[q];
+            const int other = m.cycCell[kc];
             if (badC) f = fmin(0.5, f);
-            if (bad[other] != 0.0) f = fmin(0.75, f);
+            bool farBad = bad[other] != 0.0;
+            if (bl.side && bl.side[kc] >= 0) {
+                const double* D = bl.Dl + ((size_t)bl.side[kc] * bl.nO + bl.J) * bl.nO;
+                double r = 0.0;
+                for (int K = 0; K < bl.nO; ++K) r = r + D[K] * bl.I[K].s.W[other];
+                farBad = r < bl.lb * bl.I[bl.J].s.W[other] || bad[other] >= 2.0;
+            }
+            if (farBad) f = fmin(0.75, f);
         }
     co[c] *= f;
 }
@@ -986,6 +1007,15 @@ struct icsf_hb {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     DevBuf buf;
+    // phase-lagged cyclic patch sides: the side of every cyclic face (-1
+    // none), the sides' time shifts [nSides][nO][nO] and geometry
+    // [nSides][8] (unit rotation axis, rotation centre, cylCoords), the
+    // paired faces' centres [nCyc][3] and the lagged far-side records
+    // [nO][nCyc][PLW]
+    std::vector<int> plSide;
+    std::vector<double> plD, plGeo;
+    int* dplSide = nullptr;
+    double *dplD = nullptr, *dplGeo = nullptr, *dplCf = nullptr, *dplFar = nullptr;
     ~icsf_hb() {
         if (st) (void)hipStreamSynchronize(st);
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1014,6 +1044,73 @@ __global__ void k_hb_source(int nC, int nO, const HBInst* __restrict__ I, const 
         out[J * vs + (size_t)q * nC + c] = bJ[(size_t)q * nC + c] + src;
         Wall[J * vs + (size_t)q * nC + c] = WJ[(size_t)q * nC + c];
     }
+}
+
+// Phase-lagged cyclic patches (phaseLagCyclicFvPatchField.C:179-396,
+// phaseLagCyclicFvPatchFields.C:41-702): instance J = blockIdx.y, one
+// thread per cyclic face of a lagged side: its far-side record, each slot
+// sum_K Dl[J][K] x_K(far cell) accumulated from zero in K order, x_K formed
+// from instance K's registered rho, p, U, e and psi = 1/(R T):
+// E = e + 0.5|U|^2, H = E + p/rho, c = sqrt(gamma/psi); the velocity
+// components unrotated, or with cylCoords combined as (r, theta, axis)
+// components at the paired face's centre and forwardT applied.  HLLC's
+// reconstructed speed keeps the plain far value.
+__global__ void k_phase_lag_hb(const HBInst* __restrict__ I, int nO, int nCyc, const int* __restrict__ side,
+                               const double* __restrict__ Dl, const double* __restrict__ geo,
+                               const double* __restrict__ cfPair, Phys ph, double* __restrict__ far) {
+    const int kc = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y;
+    if (kc >= nCyc) return;
+    const int sd = side[kc];
+    if (sd < 0) return;
+    const int n = I[J].m.cycCell[kc];
+    const double* D = Dl + ((size_t)sd * nO + J) * nO;
+    const double* g = geo + 8 * (size_t)sd;
+    const bool cyl = g[6] != 0.0;
+    const V3 ah{g[0], g[1], g[2]};
+    V3 rh{0, 0, 0}, th{0, 0, 0};
+    if (cyl) {
+        V3 r = ld3(cfPair + 3 * (size_t)kc) - V3{g[3], g[4], g[5]};
+        const double ar = dot(ah, r);
+        r = r - ar * ah;
+        const double mr = sqrt(magSqr(r));
+        rh = V3{r.x / mr, r.y / mr, r.z / mr};
+        th = cross(ah, rh);
+    }
+    double aR = 0, aP = 0, aE = 0, aH = 0, aC = 0, aG = 0;
+    V3 aU{0, 0, 0}, sc{0, 0, 0};
+    for (int K = 0; K < nO; ++K) {
+        const StateView& q = I[K].s;
+        const double* f = q.fld + (size_t)n * NFLD;
+        const V3 u{f[F_UX], f[F_UY], f[F_UZ]};
+        const double e = q.e[n];
+        const double dk = D[K];
+        aR = aR + dk * f[F_RHO];
+        aP = aP + dk * f[F_P];
+        aU = aU + dk * u;
+        if (cyl) sc = sc + dk * V3{dot(u, rh), dot(u, th), dot(u, ah)};
+        aE = aE + dk * (e + 0.5 * magSqr(u));
+        aH = aH + dk * ((e + 0.5 * magSqr(u)) + f[F_P] / f[F_RHO]);
+        aC = aC + dk * sqrt(ph.gamma / (1.0 / (ph.R * q.T[n])));
+        aG = aG + dk * ph.gamma;
+    }
+    const double* Rt = I[J].m.cycR + 9 * (size_t)kc;
+    const V3 uv = cyl ? rot3(Rt, (sc.x * rh + sc.y * th) + sc.z * ah) : rot3(Rt, aU);
+    const V3 uc = cyl ? uv : aU;
+    double* o = far + ((size_t)J * nCyc + kc) * PLW;
+    o[F_RHO] = aR;
+    o[F_P] = aP;
+    o[F_UX] = uc.x;
+    o[F_UY] = uc.y;
+    o[F_UZ] = uc.z;
+    o[F_E] = aE;
+    o[F_H] = aH;
+    o[F_CS] = ph.flux == FLUX_HLLC ? I[J].s.fld[(size_t)n * NFLD + F_CS] : aC;
+    o[PL_UV] = uv.x;
+    o[PL_UV + 1] = uv.y;
+    o[PL_UV + 2] = uv.z;
+    o[PL_C] = aC;
+    o[PL_G] = aG;
 }
 
 // SER over all instances (outerLoop.H:30-66): dres = [init nO*5][prev nO*5]
@@ -1057,6 +1154,9 @@ int hbUploadBC(icsf_hb* h) {
 int hbResidualStage(icsf_hb* h, bool cellSums = true) {
     const MeshView& m = h->inst[0]->mv;
     RC(hbUploadBC(h));
+    if (h->dplFar)
+        k_phase_lag_hb<<<dim3(nblk(m.nCyc, 64), h->nO), 64, 0, h->st>>>(
+            h->dInst, h->nO, m.nCyc, h->dplSide, h->dplD, h->dplGeo, h->dplCf, h->inst[0]->ph, h->dplFar);
     launchResidualHB(h->dInst, h->nO, m, h->inst[0]->ph, h->st, cellSums);
     for (icsf_case* c : h->inst) c->haveResidual = cellSums;
     if (cellSums)
@@ -1223,8 +1323,10 @@ int hbPseudoStep(icsf_hb* h) {
     for (int J = 0; J < h->nO; ++J) {
         icsf_case* c = h->inst[J];
         if (c->localTS && c->bounding) {
+            BoundLag bl;
+            if (h->dplFar) bl = BoundLag{h->dInst, h->nO, J, h->dplSide, h->dplD, c->lowerBound};
             k_bound_flag<<<nblk(m.nC, 256), 256, 0, h->st>>>(m.nC, c->lowerBound, c->sv.W, c->fac);
-            k_bound_apply<<<nblk(m.nC, 256), 256, 0, h->st>>>(c->mv, c->fac, c->sv.pseudoCo, nullptr);
+            k_bound_apply<<<nblk(m.nC, 256), 256, 0, h->st>>>(c->mv, c->fac, c->sv.pseudoCo, nullptr, bl);
         }
         // the instance's own view: its boundary conditions and values (the
         // instants' inflow total pressures differ)
@@ -2548,6 +2650,13 @@ int icsf_hb_create(icsf_mesh* mesh, const char* thermo, const char* schemes, con
             H->gJ.push_back((GmresScalars*)p);
         }
     }
+    // the registered e of every instance, which phase-lagged patches read
+    for (icsf_case* c : H->inst) {
+        double* e;
+        RC(B.alloc(e, nC));
+        HIPCHK(hipMemset(e, 0, nC * sizeof(double)));
+        c->sv.e = e;
+    }
     H->hbb.nO = nInstants;
     H->hbb.fp32 = c0->blkFp32;
     H->hbb.D = H->dD;
@@ -2590,6 +2699,73 @@ int icsf_hb_set_translating_frame(icsf_hb* h, int nZoneCells, const int* zoneCel
     if (!h) return fail(ICSF_ERR_ARG, "NULL harmonic-balance case");
     for (icsf_case* c : h->inst)
         RC(icsf_set_translating_frame(c, nZoneCells, zoneCells, velocity, nNonMoving, nonMovingPatches));
+    return hbRefreshInstances(h);
+}
+
+int icsf_hb_phase_lag_operator(int nFreq, const double* frequencies, const int* harmonics, int nInstants,
+                               const double* instants, double ibpa, double* Dl) {
+    if (nFreq < 0 || (nFreq > 0 && (!frequencies || !harmonics)) || nInstants < 1 || !instants || !Dl ||
+        !std::isfinite(ibpa))
+        return fail(ICSF_ERR_ARG, "icsf_hb_phase_lag_operator: bad argument");
+    std::vector<double> f(frequencies, frequencies + nFreq);
+    std::vector<int> hm(harmonics, harmonics + nFreq);
+    for (int k : hm)
+        if (k < 0) return fail(ICSF_ERR_ARG, "icsf_hb_phase_lag_operator: negative harmonic count");
+    const std::vector<double> om = hbOmegaList(f, hm);
+    std::vector<double> d;
+    std::string err;
+    if (!hbLagOperator(om, std::vector<double>(instants, instants + nInstants), ibpa, d, err))
+        return fail(ICSF_ERR_ARG, err);
+    std::copy(d.begin(), d.end(), Dl);
+    return ICSF_OK;
+}
+
+int icsf_hb_set_phase_lag(icsf_hb* h, int patch, const double* Dl, int cylCoords, const double* rotationAxis,
+                          const double* rotationCentre) {
+    if (!h || !Dl) return fail(ICSF_ERR_ARG, "icsf_hb_set_phase_lag: NULL argument");
+    const HostMesh& hm = h->mesh->h;
+    if (patch < 0 || patch >= (int)hm.patchStart.size() || hm.patchKind[patch] != ICSF_CYCLIC)
+        return fail(ICSF_ERR_ARG, "icsf_hb_set_phase_lag: not a cyclic patch");
+    const int nO = h->nO;
+    for (int i = 0; i < nO * nO; ++i)
+        if (!std::isfinite(Dl[i])) return fail(ICSF_ERR_ARG, "icsf_hb_set_phase_lag: non-finite time shift");
+    double geo[8] = {0, 0, 1, 0, 0, 0, 0, 0};
+    if (cylCoords) {
+        if (!rotationAxis || !rotationCentre)
+            return fail(ICSF_ERR_ARG, "icsf_hb_set_phase_lag: cylCoords needs the rotation axis and centre");
+        const double* a = rotationAxis;
+        const double la = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        if (!(la > 0) || !std::isfinite(la)) return fail(ICSF_ERR_ARG, "icsf_hb_set_phase_lag: zero rotation axis");
+        for (int d = 0; d < 3; ++d) { geo[d] = a[d] / la; geo[3 + d] = rotationCentre[d]; }
+        geo[6] = 1.0;
+    }
+    const int nCyc = (int)hm.cycFace.size();
+    const int sideIx = (int)(h->plGeo.size() / 8);
+    if (h->plSide.empty()) h->plSide.assign(nCyc, -1);
+    for (int f = hm.patchStart[patch]; f < hm.patchStart[patch] + hm.patchSize[patch]; ++f)
+        h->plSide[hm.cycOf[f - hm.nInternal]] = sideIx;
+    h->plD.insert(h->plD.end(), Dl, Dl + (size_t)nO * nO);
+    h->plGeo.insert(h->plGeo.end(), geo, geo + 8);
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (void* p : {(void*)h->dplSide, (void*)h->dplD, (void*)h->dplGeo})
+        if (p) h->buf.release(p);
+    RC(h->buf.upload(h->dplSide, h->plSide.data(), h->plSide.size()));
+    RC(h->buf.upload(h->dplD, h->plD.data(), h->plD.size()));
+    RC(h->buf.upload(h->dplGeo, h->plGeo.data(), h->plGeo.size()));
+    if (!h->dplFar) {
+        std::vector<double> cf(3 * (size_t)nCyc);
+        for (int kc = 0; kc < nCyc; ++kc)
+            for (int d = 0; d < 3; ++d) cf[3 * (size_t)kc + d] = hm.Cf[3 * (size_t)hm.cycPair[kc] + d];
+        RC(h->buf.upload(h->dplCf, cf.data(), cf.size()));
+        RC(h->buf.alloc(h->dplFar, (size_t)nO * nCyc * PLW));
+        HIPCHK(hipMemset(h->dplFar, 0, (size_t)nO * nCyc * PLW * sizeof(double)));
+    }
+    for (int J = 0; J < nO; ++J) {
+        icsf_case* c = h->inst[J];
+        c->mv.plSide = h->dplSide;
+        c->sv.plFar = h->dplFar + (size_t)J * nCyc * PLW;
+        c->haveResidual = c->assembled = false;
+    }
     return hbRefreshInstances(h);
 }
 

@@ -92,7 +92,17 @@ struct MeshView {
     // outside); null without one
     const double* umrf = nullptr;  // [nF]
     const double* omC = nullptr;   // [nC][3]
+    // harmonic balance with phase-lagged cyclic patches: the lagged side of
+    // every cyclic face (-1: a plain cyclic face); null without any
+    const int* plSide = nullptr;  // [nCyc]
 };
+
+// Lagged far-side record of a phase-lagged cyclic face (PLW doubles per
+// face and instance): slots 0..NFLD-1 the values cycNbrValue gives field k
+// (the velocity components unrotated unless cylCoords), PL_UV..+2 the
+// lagged velocity vector with forwardT applied, PL_C the lagged
+// sqrt(gamma/psi), PL_G the lagged gamma.
+enum { PL_UV = 8, PL_C = 11, PL_G = 12, PLW = 16 };
 
 struct Phys {
     double R, gamma, Cv, mu, alpha, entropyFix;
@@ -133,6 +143,12 @@ struct StateView {
     // state update, the boundary correction and the boundary mass flux
     // return at once)
     const int* halt;
+    // phase-lagged cyclic patches (harmonic balance): e [nC] the cells'
+    // internal energy as the state update leaves it (the registered e),
+    // written when non-null; plFar [nCyc][PLW] this instance's lagged
+    // far-side records (MeshView::plSide)
+    double* e = nullptr;
+    const double* plFar = nullptr;
 };
 
 // Device control block of the pseudo-time loop (pseudotimeControl):

@@ -294,6 +294,18 @@ def hb_operator(frequencies, harmonics, n_instants: int, selected_period: float 
     return D, t, float(cond[0])
 
 
+def hb_phase_lag_operator(frequencies, harmonics, instants, ibpa: float):
+    """Time shift Dl [nO, nO] of a phase-lagged cyclic patch side
+    (phaseLagCyclicFvPatchField::phaseLaggedField): sum_m Dl[l, m] x(t_m)
+    is x at t_l + ibpa / omega_1 for x in the span of the omega list."""
+    f = as_f64(np.atleast_1d(frequencies))
+    h = as_i32(np.atleast_1d(harmonics))
+    t = as_f64(instants)
+    Dl = np.empty((t.size, t.size))
+    check(lib().icsf_hb_phase_lag_operator(int(f.size), ptr(f), ptr(h), int(t.size), ptr(t), float(ibpa), ptr(Dl)))
+    return Dl
+
+
 class hbCase:
     """dbnsFullyImplicitHBFoam on the GPU: nO instants of one case on one
     mesh coupled by the spectral operator D."""
@@ -328,9 +340,26 @@ class hbCase:
             nm = as_i32(np.asarray(z.non_moving, np.int32).reshape(-1))
             check(lib().icsf_hb_set_translating_frame(self._h, int(cells.shape[0]), ptr(cells),
                                                       ptr(as_f64(z.velocity)), int(nm.shape[0]), ptr(nm)))
+        if getattr(c0, "phase_lag", None) is not None:
+            self.set_phase_lag(c0.phase_lag, c0.mesh.patches[c0.phase_lag.owner].neighbour)
         for J, c in enumerate(instances):
             self.set_fields(J, c.rho, c.U, c.T)
         return self
+
+    def set_phase_lag(self, pair, neighbour: int):
+        """Both sides of a phase-lagged cyclic pair (cases.PhaseLagPair; the
+        neighbour patch index of pair.owner): the owner side shifted by
+        +IBPA, the neighbour side by -IBPA.  Returns their time shifts."""
+        if neighbour <= pair.owner:
+            raise ValueError("phase lag: the owner side is the pair's lower patch index")
+        Do = hb_phase_lag_operator(pair.frequencies, pair.harmonics, pair.instants, pair.ibpa)
+        Dn = hb_phase_lag_operator(pair.frequencies, pair.harmonics, pair.instants, -pair.ibpa)
+        ax, ce = as_f64(pair.axis), as_f64(pair.centre)
+        for patch, Dl in ((pair.owner, Do), (neighbour, Dn)):
+            check(lib().icsf_hb_set_phase_lag(self._h, int(patch), ptr(as_f64(Dl)), int(bool(pair.cyl_coords)),
+                                              ptr(ax), ptr(ce)))
+        self.phase_lag_sides = [(pair.owner, Do), (neighbour, Dn)]
+        return Do, Dn
 
     def set_mrf(self, zone):
         """One rotating zone for all instances (cases.MRFZone)."""

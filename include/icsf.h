@@ -337,6 +337,31 @@ int icsf_hb_set_mrf(icsf_hb* h, int nZoneCells, const int* zoneCells, const doub
                     double omega, int nNonRotating, const int* nonRotatingPatches);
 int icsf_hb_set_translating_frame(icsf_hb* h, int nZoneCells, const int* zoneCells, const double* velocity,
                                   int nNonMoving, const int* nonMovingPatches);
+/* Time shift of one side of a phase-lagged cyclic pair
+ * (phaseLagCyclicFvPatchField.C:289-326): Dl [nInstants][nInstants]
+ * (row-major) = Re(E^-1 M E), E = pinv(E^-1) of the instants, M =
+ * diag(1, e^{i n IBPA} for n = 1..nH, e^{-i n IBPA} at nT - n) over the
+ * omega list of icsf_hb_operator.  The owner side of a pair takes the
+ * patch's IBPA, the neighbour side minus the owner's IBPA
+ * (phaseLagCyclicFvPatchFields.C:51-75). */
+int icsf_hb_phase_lag_operator(int nFreq, const double* frequencies, const int* harmonics, int nInstants,
+                               const double* instants, double ibpa, double* Dl);
+/* Makes one side of a cyclic pair phase-lagged (replaces the
+ * phaseLagCyclic patch type: phaseLagCyclicPolyPatch.C:79-90 IBPA,
+ * HBZoneName, cylCoords; phaseLagCyclicFvPatchField.C:179-396): on its
+ * faces the far-side values of the fields the reference lags (U and its
+ * components, p, rho, gamma, c, sqrt(gamma/psi), E and H, as
+ * phaseLagCyclicFvPatchField.C:351-358 names them) are sum_J Dl[I][J]
+ * x_J(far cell) over the instances, in the residual, the local time step,
+ * the Jacobian's face states and the bounding of the local time step;
+ * gradients and the linear solver's interfaces stay the plain cyclic ones.
+ * Call once per side, with that side's Dl; cylCoords (nonzero) combines
+ * the velocities in cylindrical components about rotationAxis through
+ * rotationCentre (the patch's), else the velocity components are
+ * combined unrotated as the reference does.  Set before the instances'
+ * fields or after them; the next residual uses it. */
+int icsf_hb_set_phase_lag(icsf_hb* h, int patch, const double* Dl, int cylCoords, const double* rotationAxis,
+                          const double* rotationCentre);
 /* Sets the launch parameters (the layout of icsf_get_tuning; params[5]
  * ignored) and marks the case tuned, so the autotuner does not run (no
  * counterpart in the reference; for pinning a measured choice).

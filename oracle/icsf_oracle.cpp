@@ -399,9 +399,34 @@ struct Case {
     std::vector<char> mrfIncl;
     // the rotating and the translating zone's parts (set-up)
     FramePart frot, ftr;
+    // harmonic balance with phase-lagged cyclic patches: the instances'
+    // shared set-up and this case's instance index
+    const struct PhaseLag* pl = nullptr;
+    int inst = 0;
 };
 
 inline double gammaOf(const Case& c) { return c.s.gamma; }
+
+// Phase-lagged cyclic patches of harmonic balance (phaseLagCyclicPolyPatch.C,
+// phaseLagCyclicFvPatchField.C:303-396, phaseLagCyclicFvPatchFields.C:
+// 41-702).  On such a face the far-side value of a lagged field for
+// instance I is sum_J Dl[I][J] x_J(far cell), accumulated from zero in J
+// order, with Dl = Re(E^-1 M E) the patch side's time shift (M = diag(1,
+// e^{i n IBPA}, ..., e^{-i n IBPA}); the owner side takes the patch's IBPA,
+// the neighbour side minus the owner's).  Per side: Dl, cylCoords, the
+// patch's unit rotation axis and rotation centre.
+enum { PL_NONE = 0, PL_RHO, PL_P, PL_U, PL_E, PL_H, PL_C, PL_GAMMA };
+struct PhaseLag {
+    struct Side {
+        vec D;  // [nO][nO]
+        bool cyl = false;
+        V3 axis{0, 0, 1}, origin{0, 0, 0};
+    };
+    int nO = 0;
+    std::vector<Case*> inst;
+    std::vector<int> side;  // per boundary face: its side, -1 off the lagged patches
+    std::vector<Side> sides;
+};
 
 // Boundary values of p, U, T and the derived rho = psi*p, e = Cv*T fields
 // (updateFields.H:90-105: p, U and T corrected in that order).
@@ -620,8 +645,10 @@ void reconstruct(const Case& k, const double* vc, const double* vb, int stride, 
 // as patched in originalOFFiles/constraintFvPatchFields/cyclic/
 // cyclicFvPatchField.C:127-188): a scalar field's value at the far cell;
 // for the components of U (fields "U.component(i)") component i of
-// forwardT & U at the far cell.  comp = -1: scalar field vc.
-vec cyclicNbr(const Case& k, const double* vc, int stride, int comp) {
+// forwardT & U at the far cell.  comp = -1: scalar field vc.  lag: the
+// field's phase-lag rule (PL_*) on phase-lagged faces.
+double lagScalar(const Case& k, int b, int field, int comp);
+vec cyclicNbr(const Case& k, const double* vc, int stride, int comp, int lag = PL_NONE) {
     const Mesh& m = *k.m;
     const int nB = m.nF - m.nI;
     vec vn(nB, 0.0);
@@ -629,7 +656,8 @@ vec cyclicNbr(const Case& k, const double* vc, int stride, int comp) {
     for (int b = 0; b < nB; ++b) {
         const int n = m.cycCell[b];
         if (n < 0) continue;
-        if (comp < 0) vn[b] = vc[n * stride];
+        if (lag != PL_NONE && k.pl && k.pl->side[b] >= 0) vn[b] = lagScalar(k, b, lag, comp < 0 ? 0 : comp);
+        else if (comp < 0) vn[b] = vc[n * stride];
         else {
             V3 u = transformV(&m.cycR[9 * b], v3(&k.U[3 * n]));
             vn[b] = (&u.x)[comp];
@@ -643,11 +671,101 @@ inline double interpCyclic(const Mesh& m, int f, double vP, double vN) {
     return m.w[f] * vP + (1.0 - m.w[f]) * vN;
 }
 // ... and of U: w*U_P + (1 - w)*(forwardT & U_N)
+inline V3 lagU(const Case& k, int b);
+inline bool lagFace(const Case& k, int b);
 inline V3 interpCyclicU(const Case& k, int f) {
     const Mesh& m = *k.m;
     const int b = f - m.nI;
-    V3 UN = transformV(&m.cycR[9 * b], v3(&k.U[3 * m.cycCell[b]]));
+    V3 UN = lagFace(k, b) ? lagU(k, b) : transformV(&m.cycR[9 * b], v3(&k.U[3 * m.cycCell[b]]));
     return m.w[f] * v3(&k.U[3 * m.own[f]]) + (1.0 - m.w[f]) * UN;
+}
+
+// ---- phase-lagged far-side values ----
+// The fields patchNeighbourField lags (phaseLagCyclicFvPatchField.C:
+// 351-358): names starting with U, p, rho, gamma, c, sqrt((gamma|thermo:psi)),
+// (he+(0.5*magSqr(U))) and H.  Unregistered ones are formed per instance
+// from its registered U, p, rho, e and psi (phaseLagCyclicFvPatchFields.C:
+// 128-243): E = e + 0.5|U|^2, H = E + p/rho (without the flux schemes'
+// SMALL bounds), c = sqrt(gamma/psi) (also for AUSM+up's renamed c*), the
+// velocity components unrotated.  Every other field (gradients, HLLC's
+// max(sqrt(gamma/psi), c0), e) keeps the plain cyclic rule.
+inline bool lagFace(const Case& k, int b) { return k.pl && k.pl->side[b] >= 0; }
+
+// The lagged velocity vector of boundary face b (the vector "U" field,
+// forwardT applied).  cylCoords (phaseLagCyclicFvPatchFields.C:582-640):
+// each instance's far velocity in the (r, theta, axis) components at the
+// paired face's centre, combined, back to Cartesian there, then forwardT.
+V3 lagU(const Case& k, int b) {
+    const Mesh& m = *k.m;
+    const PhaseLag& L = *k.pl;
+    const PhaseLag::Side& sd = L.sides[L.side[b]];
+    const double* D = &sd.D[(size_t)k.inst * L.nO];
+    const int n = m.cycCell[b];
+    const double* R = &m.cycR[9 * (size_t)b];
+    if (!sd.cyl) {
+        V3 a{0, 0, 0};
+        for (int J = 0; J < L.nO; ++J) a = a + D[J] * v3(&L.inst[J]->U[3 * n]);
+        return transformV(R, a);
+    }
+    const int p = m.facePatch[b];
+    const int g = m.pStart[m.pNbr[p]] + (m.nI + b - m.pStart[p]);
+    const V3 ah = sd.axis;
+    V3 r = v3(&m.Cf[3 * (size_t)g]) - sd.origin;
+    const double ar = dot(ah, r);
+    r = r - ar * ah;
+    const double mr = std::sqrt(magSqr(r));
+    const V3 rh{r.x / mr, r.y / mr, r.z / mr};
+    const V3 th = cross(ah, rh);
+    V3 sc{0, 0, 0};
+    for (int J = 0; J < L.nO; ++J) {
+        const V3 u = v3(&L.inst[J]->U[3 * n]);
+        sc = sc + D[J] * V3{dot(u, rh), dot(u, th), dot(u, ah)};
+    }
+    const V3 c = (sc.x * rh + sc.y * th) + sc.z * ah;
+    return transformV(R, c);
+}
+
+// The lagged value of a scalar field (comp: the velocity component for
+// PL_U) on boundary face b.
+double lagScalar(const Case& k, int b, int field, int comp) {
+    const Mesh& m = *k.m;
+    const PhaseLag& L = *k.pl;
+    const PhaseLag::Side& sd = L.sides[L.side[b]];
+    if (field == PL_U && sd.cyl) {
+        const V3 u = lagU(k, b);
+        return (&u.x)[comp];
+    }
+    const double* D = &sd.D[(size_t)k.inst * L.nO];
+    const int n = m.cycCell[b];
+    double a = 0.0;
+    for (int J = 0; J < L.nO; ++J) {
+        const Case& q = *L.inst[J];
+        const V3 u = v3(&q.U[3 * n]);
+        double x;
+        switch (field) {
+            case PL_RHO: x = q.rho[n]; break;
+            case PL_P: x = q.p[n]; break;
+            case PL_U: x = (&u.x)[comp]; break;
+            case PL_E: x = q.e[n] + 0.5 * magSqr(u); break;
+            case PL_H: x = (q.e[n] + 0.5 * magSqr(u)) + q.p[n] / q.rho[n]; break;
+            case PL_C: x = std::sqrt(q.s.gamma / (1.0 / (q.s.R * q.T[n]))); break;
+            default: x = q.s.gamma; break;
+        }
+        a = a + D[J] * x;
+    }
+    return a;
+}
+
+// fvc::interpolate of c = sqrt(gamma/psi) and of gamma on cyclic face f
+// (the far side lagged on phase-lagged faces)
+inline double interpCyclicC(const Case& k, int f, const double* c) {
+    const Mesh& m = *k.m;
+    const int b = f - m.nI;
+    return interpCyclic(m, f, c[m.own[f]], lagFace(k, b) ? lagScalar(k, b, PL_C, 0) : c[m.cycCell[b]]);
+}
+inline double interpCyclicGamma(const Case& k, int f) {
+    const int b = f - k.m->nI;
+    return interpCyclic(*k.m, f, k.s.gamma, lagFace(k, b) ? lagScalar(k, b, PL_GAMMA, 0) : k.s.gamma);
 }
 
 inline double interpLinear(const Mesh& m, int f, const double* vc, int stride) {
@@ -872,19 +990,21 @@ void calcFlux(Case& k, vec& phi, vec& phiU, vec& phiE) {
     for (int c = 0; c < m.nC; ++c) { Ux[c] = k.U[3 * c]; Uy[c] = k.U[3 * c + 1]; Uz[c] = k.U[3 * c + 2]; }
     for (int b = 0; b < nB; ++b) { Ubx[b] = k.UB[3 * b]; Uby[b] = k.UB[3 * b + 1]; Ubz[b] = k.UB[3 * b + 2]; }
     vec UxL, UxR, UyL, UyR, UzL, UzR, pL, pR, rL, rR, HL, HR, cL, cR, EL, ER;
-    auto rec = [&](const double* vc, const double* vb, int comp, vec& L, vec& R) {
-        const vec vn = cyclicNbr(k, vc, 1, comp);
+    auto rec = [&](const double* vc, const double* vb, int comp, vec& L, vec& R, int lag) {
+        const vec vn = cyclicNbr(k, vc, 1, comp, lag);
         reconstruct(k, vc, vb, 1, L, R, vn.data());
     };
-    rec(Ux.data(), Ubx.data(), 0, UxL, UxR);
-    rec(Uy.data(), Uby.data(), 1, UyL, UyR);
-    rec(Uz.data(), Ubz.data(), 2, UzL, UzR);
-    rec(k.p.data(), k.pB.data(), -1, pL, pR);
-    rec(k.rho.data(), k.rhoB.data(), -1, rL, rR);
-    rec(d.H.data(), d.Hb.data(), -1, HL, HR);
-    if (k.s.flux == FLUX_AUSM) rec(d.cstar.data(), d.cstarb.data(), -1, cL, cR);
-    else rec(d.E.data(), d.Eb.data(), -1, EL, ER);
-    if (k.s.flux == FLUX_HLLC) rec(d.chl.data(), d.chlb.data(), -1, cL, cR);
+    rec(Ux.data(), Ubx.data(), 0, UxL, UxR, PL_U);
+    rec(Uy.data(), Uby.data(), 1, UyL, UyR, PL_U);
+    rec(Uz.data(), Ubz.data(), 2, UzL, UzR, PL_U);
+    rec(k.p.data(), k.pB.data(), -1, pL, pR, PL_P);
+    rec(k.rho.data(), k.rhoB.data(), -1, rL, rR, PL_RHO);
+    rec(d.H.data(), d.Hb.data(), -1, HL, HR, PL_H);
+    // AUSM+up's c* is named "c": lagged as sqrt(gamma/psi); HLLC's
+    // max(sqrt(gamma/psi), c0) is not lagged
+    if (k.s.flux == FLUX_AUSM) rec(d.cstar.data(), d.cstarb.data(), -1, cL, cR, PL_C);
+    else rec(d.E.data(), d.Eb.data(), -1, EL, ER, PL_E);
+    if (k.s.flux == FLUX_HLLC) rec(d.chl.data(), d.chlb.data(), -1, cL, cR, PL_NONE);
     phi.assign(m.nF, 0);
     phiU.assign(3 * m.nF, 0);
     phiE.assign(m.nF, 0);
@@ -895,8 +1015,9 @@ void calcFlux(Case& k, vec& phi, vec& phiU, vec& phiE) {
         double ph, pe;
         V3 pu;
         // fvc::interpolate(gamma): exact on internal faces (w (g - g) + g),
-        // w g + (1 - w) g on cyclic faces
-        const double gI = m.cyclicFace(f) ? interpCyclic(m, f, k.s.gamma, k.s.gamma) : k.s.gamma;
+        // w g + (1 - w) g on cyclic faces (the far gamma lagged on
+        // phase-lagged faces)
+        const double gI = m.cyclicFace(f) ? interpCyclicGamma(k, f) : k.s.gamma;
         const double um = k.mrf ? k.umrf[f] : 0.0;
         if (k.s.flux == FLUX_AUSM)
             ausmFace(k, S, m.magSf[f], UL, UR, cL[f], cR[f], pL[f], pR[f], rL[f], rR[f], HL[f], HR[f], um, ph, pu,
@@ -1087,7 +1208,7 @@ void localDeltaT(Case& k) {
             // interpolation (setCoAndDeltaT.H:78-89)
             for (int f = m.pStart[pi]; f < m.pStart[pi] + m.pSize[pi]; ++f) {
                 const int b = f - m.nI, c = m.own[f];
-                const double cf = interpCyclic(m, f, d.c[c], d.c[m.cycCell[b]]);
+                const double cf = interpCyclicC(k, f, d.c.data());
                 const V3 Uf = interpCyclicU(k, f);
                 const double lam = cf + std::fabs(dot(Uf, v3(&m.Sf[3 * f]) / m.magSf[f]) - (k.mrf ? k.umrf[f] : 0.0));
                 k.rDT[c] = std::max(k.rDT[c], m.nodc[f] * lam);
@@ -1147,8 +1268,9 @@ void assemble(Case& k) {
     for (int b = 0; b < nB; ++b) { Ubx[b] = k.UB[3 * b]; Uby[b] = k.UB[3 * b + 1]; Ubz[b] = k.UB[3 * b + 2]; }
     vec UxL, UxR, UyL, UyR, UzL, UzR, EL, ER;
     {
-        vec n0 = cyclicNbr(k, Ux.data(), 1, 0), n1 = cyclicNbr(k, Uy.data(), 1, 1), n2 = cyclicNbr(k, Uz.data(), 1, 2);
-        vec nE = cyclicNbr(k, d.E.data(), 1, -1);
+        vec n0 = cyclicNbr(k, Ux.data(), 1, 0, PL_U), n1 = cyclicNbr(k, Uy.data(), 1, 1, PL_U),
+            n2 = cyclicNbr(k, Uz.data(), 1, 2, PL_U);
+        vec nE = cyclicNbr(k, d.E.data(), 1, -1, PL_E);
         reconstruct(k, Ux.data(), Ubx.data(), 1, UxL, UxR, n0.data());
         reconstruct(k, Uy.data(), Uby.data(), 1, UyL, UyR, n1.data());
         reconstruct(k, Uz.data(), Ubz.data(), 1, UzL, UzR, n2.data());
@@ -1194,8 +1316,9 @@ void assemble(Case& k) {
     };
     auto nrm = [&](int f) { return v3(&m.Sf[3 * f]) / m.magSf[f]; };
     // linear interpolation of a uniform gamma: exact on internal faces,
-    // w g + (1 - w) g on cyclic faces
-    auto gam = [&](int f) { return m.cyclicFace(f) ? interpCyclic(m, f, g, g) : g; };
+    // w g + (1 - w) g on cyclic faces (the far gamma lagged on
+    // phase-lagged faces)
+    auto gam = [&](int f) { return m.cyclicFace(f) ? interpCyclicGamma(k, f) : g; };
     auto theta = [&](int f, V3 U) { return 0.5 * (gam(f) - 1) * magSqr(U); };
     // dContByRhoU: n, n
     insert(k.SV[0], 3, [&](int f, double* o) { V3 n = nrm(f); o[0] = n.x; o[1] = n.y; o[2] = n.z; },
@@ -1302,7 +1425,7 @@ void assemble(Case& k) {
     for (int f = nI; f < m.nF; ++f) {
         if (m.pKind[m.facePatch[f - nI]] == PK_EMPTY) continue;
         if (m.cyclicFace(f)) {
-            const double cf = interpCyclic(m, f, d.c[m.own[f]], d.c[m.cycCell[f - nI]]);
+            const double cf = interpCyclicC(k, f, d.c.data());
             lamB[f - nI] = cf + std::fabs(dot(interpCyclicU(k, f), nrm(f)) - (k.mrf ? k.umrf[f] : 0.0));
         } else {
             lamB[f - nI] = d.cb[f - nI] + std::fabs(dot(v3(&k.UB[3 * (f - nI)]), nrm(f)) - (k.mrf ? k.umrf[f] : 0.0));
@@ -2017,6 +2140,7 @@ struct HB {
     int corr = 0;
     bool haveInit = false, havePrev = false;
     vec initS, initV, prevS, prevV;  // [2 nO], [3 nO]
+    PhaseLag pl;  // phase-lagged cyclic patch sides (none: pl.sides empty)
 };
 
 // HBZone::addSource (HBZoneTemplates.C:39-93): source_J = -sum_K V D_JK W_K
@@ -2340,11 +2464,24 @@ int hbPseudoStep(HB& h, HBRes& out) {
                 if (bad(o)) { fac[o] = std::min(0.5, fac[o]); fac[n] = std::min(0.75, fac[n]); }
                 if (bad(n)) { fac[n] = std::min(0.5, fac[n]); fac[o] = std::min(0.75, fac[o]); }
             }
+            // phase-lagged faces: rho_.patchNeighbourField() is lagged over
+            // the instances as they stand (those before J already updated,
+            // outerLoop.H:221-229); rhoMin, eTemp and eMin are not
+            auto badFar = [&](int f) {
+                const int b = f - m.nI, n = m.cycCell[b];
+                if (!lagFace(k, b)) return bad(n);
+                const double rmin = k.s.lowerBound * W[J].s0[n];
+                const double Uo[3] = {W[J].v[3 * n] / W[J].s0[n], W[J].v[3 * n + 1] / W[J].s0[n], W[J].v[3 * n + 2] / W[J].s0[n]};
+                const double emin = k.s.lowerBound * (W[J].s1[n] / W[J].s0[n] - 0.5 * (Uo[0] * Uo[0] + Uo[1] * Uo[1] + Uo[2] * Uo[2]));
+                const double Un[3] = {k.rhoU[3 * n] / k.rho[n], k.rhoU[3 * n + 1] / k.rho[n], k.rhoU[3 * n + 2] / k.rho[n]};
+                const double et = k.rhoE[n] / k.rho[n] - 0.5 * (Un[0] * Un[0] + Un[1] * Un[1] + Un[2] * Un[2]);
+                return lagScalar(k, b, PL_RHO, 0) < rmin || et < emin || et < SMALL;
+            };
             for (int f = m.nI; f < m.nF; ++f) {
                 if (!m.cyclicFace(f)) continue;
-                const int o = m.own[f], n = m.cycCell[f - m.nI];
+                const int o = m.own[f];
                 if (bad(o)) fac[o] = std::min(0.5, fac[o]);
-                if (bad(n)) fac[o] = std::min(0.75, fac[o]);
+                if (badFar(f)) fac[o] = std::min(0.75, fac[o]);
             }
             for (int c = 0; c < nC; ++c) k.pseudoCo[c] *= fac[c];
         }
@@ -2355,9 +2492,62 @@ int hbPseudoStep(HB& h, HBRes& out) {
 
 // HBZone::setOmegaList / updateHBOperators restated: E^-1 = [exp(i w_k t_n)],
 // E = its inverse (square case), D = -(Im(E^-1) A Re(E) + Re(E^-1) A Im(E)).
+typedef std::complex<double> cx;
+// E^-1 = [exp(i w_k t_n)] and E = its inverse (Gauss-Jordan, partial
+// pivoting); false if singular.
+bool hbFourier(int nT, const double* omega, const double* t, std::vector<cx>& Einv, std::vector<cx>& I) {
+    std::vector<cx> A((size_t)nT * nT);
+    I.assign((size_t)nT * nT, cx(0, 0));
+    for (int n = 0; n < nT; ++n)
+        for (int k = 0; k < nT; ++k) A[(size_t)n * nT + k] = cx(std::cos(omega[k] * t[n]), std::sin(omega[k] * t[n]));
+    Einv = A;
+    for (int i = 0; i < nT; ++i) I[(size_t)i * nT + i] = 1.0;
+    for (int col = 0; col < nT; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < nT; ++r)
+            if (std::abs(A[(size_t)r * nT + col]) > std::abs(A[(size_t)piv * nT + col])) piv = r;
+        if (std::abs(A[(size_t)piv * nT + col]) < 1e-13) return false;
+        for (int q = 0; q < nT; ++q) { std::swap(A[(size_t)piv * nT + q], A[(size_t)col * nT + q]); std::swap(I[(size_t)piv * nT + q], I[(size_t)col * nT + q]); }
+        const cx d = A[(size_t)col * nT + col];
+        for (int q = 0; q < nT; ++q) { A[(size_t)col * nT + q] /= d; I[(size_t)col * nT + q] /= d; }
+        for (int r = 0; r < nT; ++r) {
+            if (r == col) continue;
+            const cx f = A[(size_t)r * nT + col];
+            for (int q = 0; q < nT; ++q) { A[(size_t)r * nT + q] -= f * A[(size_t)col * nT + q]; I[(size_t)r * nT + q] -= f * I[(size_t)col * nT + q]; }
+        }
+    }
+    return true;
+}
+
+// The time shift of a phase-lagged patch side (phaseLagCyclicFvPatchField.C:
+// 289-326): Dl = Re(E^-1 (M E)), M = diag(1, e^{i n IBPA} (n = 1..nH),
+// e^{-i n IBPA} at nT - n), the products summed over every n as written.
+int hbLagOperator(int nT, const double* omega, const double* t, double ibpa, double* Dl) {
+    std::vector<cx> Einv, E;
+    if (!hbFourier(nT, omega, t, Einv, E)) return -2;
+    const int nH = (nT - 1) / 2;
+    std::vector<cx> M((size_t)nT * nT, cx(0, 0)), T0((size_t)nT * nT, cx(0, 0));
+    M[0] = cx(1, 0);
+    for (int n = 1; n <= nH; ++n) {
+        const cx e(std::cos(n * ibpa), std::sin(n * ibpa));
+        M[(size_t)n * nT + n] = e;
+        M[(size_t)(nT - n) * nT + (nT - n)] = std::conj(e);
+    }
+    auto mul = [](cx a, cx b) { return cx(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real()); };
+    for (int l = 0; l < nT; ++l)
+        for (int m2 = 0; m2 < nT; ++m2)
+            for (int n = 0; n < nT; ++n) T0[(size_t)l * nT + m2] += mul(M[(size_t)l * nT + n], E[(size_t)n * nT + m2]);
+    for (int l = 0; l < nT; ++l)
+        for (int m2 = 0; m2 < nT; ++m2) {
+            cx d(0, 0);
+            for (int n = 0; n < nT; ++n) d += mul(Einv[(size_t)l * nT + n], T0[(size_t)n * nT + m2]);
+            Dl[(size_t)l * nT + m2] = d.real();
+        }
+    return 0;
+}
+
 int hbOperator(int nT, const double* omega, int nO, const double* t, double* D) {
     if (nO != nT) return -1;
-    typedef std::complex<double> cx;
     std::vector<cx> A((size_t)nT * nT), I((size_t)nT * nT, cx(0, 0));
     for (int n = 0; n < nO; ++n)
         for (int k = 0; k < nT; ++k) A[(size_t)n * nT + k] = cx(std::cos(omega[k] * t[n]), std::sin(omega[k] * t[n]));
@@ -3101,7 +3291,60 @@ void* orc_hb_new(int nO, void** cases, const double* D) {
     h->D.assign(D, D + (size_t)nO * nO);
     return h;
 }
-void orc_hb_free(void* h) { delete (HB*)h; }
+void orc_hb_free(void* hp) {
+    HB* h = (HB*)hp;
+    for (Case* k : h->k)
+        if (k->pl == &h->pl) k->pl = nullptr;
+    delete h;
+}
+
+// One side of a phase-lagged cyclic pair: its patch, time shift Dl
+// [nO][nO] (hbLagOperator of the side's IBPA), cylCoords with the patch's
+// rotation axis (normalised here, rotAxis/mag(rotAxis)) and centre.
+int orc_hb_set_phase_lag(void* hp, int patch, const double* Dl, int cyl, const double* axis, const double* origin) {
+    HB& h = *(HB*)hp;
+    const Mesh& m = *h.k[0]->m;
+    if (patch < 0 || patch >= (int)m.pStart.size() || m.pKind[patch] != PK_CYCLIC) return -1;
+    PhaseLag& L = h.pl;
+    if (L.sides.empty()) {
+        L.nO = h.nO;
+        L.inst = h.k;
+        L.side.assign(m.nF - m.nI, -1);
+    }
+    PhaseLag::Side sd;
+    sd.D.assign(Dl, Dl + (size_t)h.nO * h.nO);
+    sd.cyl = cyl != 0;
+    const V3 a = v3(axis);
+    const double la = std::sqrt(magSqr(a));
+    sd.axis = V3{a.x / la, a.y / la, a.z / la};
+    sd.origin = v3(origin);
+    L.sides.push_back(sd);
+    for (int f = m.pStart[patch]; f < m.pStart[patch] + m.pSize[patch]; ++f) L.side[f - m.nI] = (int)L.sides.size() - 1;
+    for (int J = 0; J < h.nO; ++J) { h.k[J]->pl = &L; h.k[J]->inst = J; }
+    return 0;
+}
+
+// Far-side values of instance J's boundary faces ([nB]; zero off the
+// phase-lagged faces): field PL_RHO .. PL_GAMMA (comp: velocity component;
+// PL_U with comp 3..5: the velocity vector's component comp - 3, forwardT
+// applied).
+void orc_hb_lag_far(void* hp, int J, int field, int comp, double* out) {
+    HB& h = *(HB*)hp;
+    const Case& k = *h.k[J];
+    const Mesh& m = *k.m;
+    for (int b = 0; b < m.nF - m.nI; ++b) {
+        out[b] = 0.0;
+        if (!lagFace(k, b)) continue;
+        if (field == PL_U && comp >= 3) {
+            const V3 u = lagU(k, b);
+            out[b] = (&u.x)[comp - 3];
+        } else out[b] = lagScalar(k, b, field, comp);
+    }
+}
+
+int orc_hb_lag_operator(int nT, const double* omega, const double* t, double ibpa, double* Dl) {
+    return hbLagOperator(nT, omega, t, ibpa, Dl);
+}
 
 void orc_hb_residual(void* hp, double* b) {
     HB& h = *(HB*)hp;

@@ -64,6 +64,9 @@ _SIG = {
     "orc_hb_solve": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "orc_hb_pseudo_steps": (_i, [_vp, _i, _vp]),
     "orc_hb_operator": (_i, [_i, _vp, _i, _vp, _vp]),
+    "orc_hb_set_phase_lag": (_i, [_vp, _i, _vp, _i, _vp, _vp]),
+    "orc_hb_lag_operator": (_i, [_i, _vp, _vp, C.c_double, _vp]),
+    "orc_hb_lag_far": (None, [_vp, _i, _i, _i, _vp]),
 }
 _lib = None
 
@@ -410,6 +413,17 @@ def hb_operator(omegas, instants):
     return D
 
 
+def hb_lag_operator(omegas, instants, ibpa):
+    """Dl [nO][nO]: the oracle's restatement of a phase-lagged patch side's
+    time shift Re(E^-1 M E) (omega list as hb_operator's)."""
+    om, t = _f64(omegas), _f64(instants)
+    Dl = np.empty((len(t), len(t)))
+    rc = lib().orc_hb_lag_operator(len(t), _p(om), _p(t), float(ibpa), _p(Dl))
+    if rc:
+        raise RuntimeError(f"oracle lag operator failed ({rc})")
+    return Dl
+
+
 class OracleHB:
     """nO instances of one case coupled by the HB operator D."""
 
@@ -426,6 +440,27 @@ class OracleHB:
         if getattr(self, "h", None):
             lib().orc_hb_free(self.h)
             self.h = None
+
+    def set_phase_lag(self, patch, Dl, cyl=False, axis=(0.0, 0.0, 1.0), origin=(0.0, 0.0, 0.0)):
+        """Make cyclic patch `patch` (one side of its pair) phase-lagged with
+        the time shift Dl [nO][nO]; cyl: cylCoords about the patch's
+        rotation axis and centre."""
+        rc = lib().orc_hb_set_phase_lag(self.h, int(patch), _p(_f64(Dl)), int(bool(cyl)), _p(_f64(axis)),
+                                        _p(_f64(origin)))
+        if rc:
+            raise RuntimeError(f"oracle phase lag set-up failed ({rc})")
+
+    LAG_FIELDS = {"rho": 1, "p": 2, "U": 3, "E": 4, "H": 5, "c": 6, "gamma": 7}
+
+    def lag_far(self, J, field, comp=0):
+        """Far-side values of instance J's boundary faces on the
+        phase-lagged faces (zero elsewhere): field in LAG_FIELDS; for "U"
+        comp 0..2 the unrotated components, 3..5 the vector's (forwardT
+        applied)."""
+        nB = self.inst[0].nFaces - self.inst[0].nInternal
+        out = np.empty(nB)
+        lib().orc_hb_lag_far(self.h, int(J), self.LAG_FIELDS[field], int(comp), _p(out))
+        return out
 
     def residual(self):
         b = np.empty((self.nO, 5, self.nCells))
