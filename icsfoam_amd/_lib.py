@@ -112,6 +112,7 @@ SIGNATURES = {
     "icsf_foam_mesh_sizes": (_i, [_vp, _vp]),
     "icsf_foam_mesh_get": (_i, [_vp] + [_vp] * 10),
     "icsf_foam_mesh_patch_name": (_i, [_vp, _i, C.c_char_p, _i]),
+    "icsf_foam_mesh_patch_phase_lag": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, C.c_char_p, _i]),
     "icsf_foam_mesh_free": (_i, [_vp]),
     "icsf_foam_mesh_write": (_i, [C.c_char_p, _i, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp,
                                   _vp]),

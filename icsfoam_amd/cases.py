@@ -41,6 +41,9 @@ class Patch:
     # coupled to face i of the paired patch
     neighbour: int = -1
     transform: Optional[np.ndarray] = None
+    # phaseLagCyclic patches as read from a boundary file: IBPA,
+    # cyl_coords, axis, centre, zone (HBZoneName); None otherwise
+    phase_lag: Optional[dict] = None
 
 
 @dataclasses.dataclass

@@ -187,6 +187,11 @@ struct PatchInfo {
     double T[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     std::string transform;
     double axis[3] = {0, 0, 1};
+    // phaseLagCyclic (phaseLagCyclicPolyPatch.C:79-90): IBPA, HBZoneName,
+    // cylCoords (a required entry), the pair's rotationCentre
+    bool phaseLag = false, cylCoords = false;
+    double ibpa = 0, centre[3] = {0, 0, 0};
+    std::string hbZone;
 };
 
 bool vec3(const std::vector<std::string>* v, double* out) {
@@ -227,11 +232,21 @@ bool readBoundary(const std::string& text, std::vector<PatchInfo>& out, std::str
         if (p.size < 0 || p.start < 0) { err = "patch " + p.name + ": nFaces/startFace missing"; return false; }
         if (p.type == "wall") p.kind = ICSF_WALL;
         else if (p.type == "empty") p.kind = ICSF_EMPTY;
-        else if (p.type == "cyclic") {
+        else if (p.type == "cyclic" || p.type == "phaseLagCyclic") {
             p.kind = ICSF_CYCLIC;
             p.nbrName = d.word("neighbourPatch", "");
             p.transform = d.word("transform", d.word("transformType", "unknown"));
             vec3(d.find("rotationAxis"), p.axis);
+            vec3(d.find("rotationCentre"), p.centre);
+            if (p.type == "phaseLagCyclic") {
+                p.phaseLag = true;
+                p.ibpa = d.scalar("IBPA", 0.0);
+                p.hbZone = d.word("HBZoneName", "");
+                const std::string cc = d.word("cylCoords", "");
+                if (cc == "true" || cc == "yes" || cc == "on" || cc == "1") p.cylCoords = true;
+                else if (cc == "false" || cc == "no" || cc == "off" || cc == "0") p.cylCoords = false;
+                else { err = "patch " + p.name + ": phaseLagCyclic needs cylCoords"; return false; }
+            }
         } else if (p.type == "patch" || p.type == "symmetryPlane" || p.type == "symmetry") p.kind = ICSF_PATCH;
         else { err = "patch " + p.name + ": unsupported patch type " + p.type; return false; }
         out.push_back(p);
@@ -466,6 +481,22 @@ int icsf_foam_mesh_patch_name(const icsf_foam_mesh* m, int patch, char* name, in
     if (!m || !name || len < 1 || patch < 0 || patch >= (int)m->patches.size())
         return reportError(ICSF_ERR_ARG, "icsf_foam_mesh_patch_name: bad argument");
     std::snprintf(name, (size_t)len, "%s", m->patches[patch].name.c_str());
+    return ICSF_OK;
+}
+
+int icsf_foam_mesh_patch_phase_lag(const icsf_foam_mesh* m, int patch, int* phaseLag, double* ibpa, int* cylCoords,
+                                   double* rotationAxis, double* rotationCentre, char* hbZoneName, int len) {
+    if (!m || !phaseLag || patch < 0 || patch >= (int)m->patches.size())
+        return reportError(ICSF_ERR_ARG, "icsf_foam_mesh_patch_phase_lag: bad argument");
+    const PatchInfo& p = m->patches[patch];
+    *phaseLag = p.phaseLag ? 1 : 0;
+    if (ibpa) *ibpa = p.ibpa;
+    if (cylCoords) *cylCoords = p.cylCoords ? 1 : 0;
+    for (int d = 0; d < 3; ++d) {
+        if (rotationAxis) rotationAxis[d] = p.axis[d];
+        if (rotationCentre) rotationCentre[d] = p.centre[d];
+    }
+    if (hbZoneName && len > 0) std::snprintf(hbZoneName, (size_t)len, "%s", p.hbZone.c_str());
     return ICSF_OK;
 }
 

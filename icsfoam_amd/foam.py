@@ -34,7 +34,7 @@ import numpy as np
 from ._lib import IcsfError, as_f64, as_i32, check, lib, ptr
 from .cases import (BC_FIXED_VALUE, BC_PRESSURE_INLET_OUTLET_VELOCITY, BC_SLIP, BC_TOTAL_PRESSURE,
                     BC_TOTAL_TEMPERATURE, BC_ZERO_GRADIENT, PATCH_CYCLIC, PATCH_EMPTY, BoundarySpec, Case,
-                    MRFZone, Patch, PolyMesh, R_GAS, TranslatingZone)
+                    MRFZone, Patch, PhaseLagPair, PolyMesh, R_GAS, TranslatingZone)
 
 
 def read_polymesh(poly_mesh_dir: str) -> Tuple[PolyMesh, List[str]]:
@@ -52,14 +52,23 @@ def read_polymesh(poly_mesh_dir: str) -> Tuple[PolyMesh, List[str]]:
         check(lib().icsf_foam_mesh_get(h, ptr(pts), ptr(off), ptr(fv), ptr(own), ptr(nei), ptr(ps), ptr(pz),
                                        ptr(pk), ptr(pn), ptr(pT)))
         names = []
+        lags = {}
         buf = C.create_string_buffer(256)
         for p in range(nPa):
             check(lib().icsf_foam_mesh_patch_name(h, p, buf, 256))
             names.append(buf.value.decode())
+            on, cyl = np.zeros(1, np.int32), np.zeros(1, np.int32)
+            ibpa, ax, ce = np.zeros(1), np.zeros(3), np.zeros(3)
+            check(lib().icsf_foam_mesh_patch_phase_lag(h, p, ptr(on), ptr(ibpa), ptr(cyl), ptr(ax), ptr(ce), buf,
+                                                       256))
+            if on[0]:
+                lags[p] = dict(ibpa=float(ibpa[0]), cyl_coords=bool(cyl[0]), axis=ax, centre=ce,
+                               zone=buf.value.decode())
     finally:
         lib().icsf_foam_mesh_free(h)
     patches = [Patch(names[p], int(pk[p]), int(ps[p]), int(pz[p]), int(pn[p]),
-                     pT[p].reshape(3, 3).copy() if pk[p] == PATCH_CYCLIC else None) for p in range(nPa)]
+                     pT[p].reshape(3, 3).copy() if pk[p] == PATCH_CYCLIC else None, lags.get(p))
+               for p in range(nPa)]
     return PolyMesh(points=pts, face_off=off, face_verts=fv, owner=own, neighbour=nei, patches=patches,
                     n_cells=nC), names
 
@@ -289,6 +298,69 @@ def read_mrf(case_dir: str, n_cells: int, names: List[str]) -> Optional[MRFZone]
                    omega=float(mo.group(1)), non_rotating=_patches(body, "nonRotatingPatches", names))
 
 
+def read_hb_properties(case_dir: str) -> Dict[str, Dict]:
+    """constant/HBProperties (IOHBZoneList; HBZone.C:46-91, 95-110): the
+    active zones' frequenciesList (angular), harmonicsList and cylCoords,
+    and system/fvSolution's harmonicBalance instantsNumber (default 3) and
+    selectedPeriod (0 when absent: the condition-number scan).  {} without
+    the file."""
+    path = os.path.join(case_dir, "constant", "HBProperties")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        text = _strip_comments(f.read())
+    sol_path = os.path.join(case_dir, "system", "fvSolution")
+    hb = ""
+    if os.path.exists(sol_path):
+        with open(sol_path) as f:
+            hb = _blocks(_strip_comments(f.read())).get("harmonicBalance", "")
+    mi = re.search(r"\binstantsNumber\s+(\d+)\s*;", hb)
+    mp = re.search(r"\bselectedPeriod\s+([-+0-9.eE]+)\s*;", hb)
+    out = {}
+    for zname, body in _blocks(text).items():
+        mt = re.search(r"\bactive\s+(\w+)\s*;", body)
+        if mt and mt.group(1).lower() in ("no", "false", "off", "n", "none"):
+            continue
+        mf = re.search(r"\bfrequenciesList\s*(?:\d+\s*)?\(([^)]*)\)\s*;", body)
+        mh = re.search(r"\bharmonicsList\s*(?:\d+\s*)?\(([^)]*)\)\s*;", body)
+        if not mf or not mh:
+            raise IcsfError(-2, f"HBProperties {zname}: frequenciesList and harmonicsList are required")
+        mc = re.search(r"\bcylCoords\s+(\w+)\s*;", body)
+        out[zname] = dict(frequencies=np.array([float(x) for x in mf.group(1).split()], np.float64),
+                          harmonics=np.array([int(x) for x in mh.group(1).split()], np.int32),
+                          cyl_coords=bool(mc and mc.group(1).lower() in ("yes", "true", "on", "y")),
+                          instants=int(mi.group(1)) if mi else 3,
+                          selected_period=float(mp.group(1)) if mp else 0.0)
+    return out
+
+
+def read_phase_lag(case_dir: str, mesh: PolyMesh) -> Optional[PhaseLagPair]:
+    """The phaseLagCyclic pair of the mesh (one pair supported) with its HB
+    zone (HBZoneName of the pair's owner, the lower patch index): IBPA,
+    cylCoords, rotation axis and centre of the owner; the zone's
+    frequencies and harmonics and the instants of hb_operator.  None
+    without phaseLagCyclic patches."""
+    from .solver import hb_operator
+    lag = [i for i, p in enumerate(mesh.patches) if p.phase_lag is not None]
+    if not lag:
+        return None
+    owners = sorted({min(i, mesh.patches[i].neighbour) for i in lag})
+    if len(owners) > 1:
+        raise IcsfError(-1, "phaseLagCyclic: one phase-lagged pair is supported")
+    own = mesh.patches[owners[0]]
+    if own.phase_lag is None or mesh.patches[own.neighbour].phase_lag is None:
+        raise IcsfError(-1, f"phaseLagCyclic: both sides of {own.name} must be phaseLagCyclic")
+    zones = read_hb_properties(case_dir)
+    z = zones.get(own.phase_lag["zone"])
+    if z is None:
+        raise IcsfError(-2, f"phaseLagCyclic {own.name}: HB zone {own.phase_lag['zone']!r} not in HBProperties")
+    _, t, _ = hb_operator(z["frequencies"], z["harmonics"], z["instants"], z["selected_period"])
+    pl = own.phase_lag
+    return PhaseLagPair(owner=owners[0], ibpa=pl["ibpa"], frequencies=z["frequencies"], harmonics=z["harmonics"],
+                        instants=t, cyl_coords=pl["cyl_coords"], axis=np.asarray(pl["axis"], np.float64),
+                        centre=np.asarray(pl["centre"], np.float64))
+
+
 def read_case(case_dir: str, time: str = "0", thermo: Optional[str] = None, fvSchemes: Optional[str] = None,
               fvSolution: Optional[str] = None, overrides: Optional[Dict[str, Dict[str, str]]] = None,
               R: float = R_GAS) -> Tuple[Case, List[str]]:
@@ -319,7 +391,7 @@ def read_case(case_dir: str, time: str = "0", thermo: Optional[str] = None, fvSc
     case = Case(os.path.basename(os.path.abspath(case_dir)), mesh, p / (R * T), U.copy(), T.copy(), bcs,
                 text(thermo, "constant", "thermophysicalProperties"), text(fvSchemes, "system", "fvSchemes"),
                 text(fvSolution, "system", "fvSolution"), mrf=read_mrf(case_dir, n, names),
-                translating=read_translating(case_dir, n, names))
+                translating=read_translating(case_dir, n, names), phase_lag=read_phase_lag(case_dir, mesh))
     return case, names
 
 

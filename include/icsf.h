@@ -442,7 +442,8 @@ typedef struct icsf_foam_field icsf_foam_field;
 /* Read constant/polyMesh (points, faces as faceList or faceCompactList,
  * owner, neighbour, boundary), as fvMesh construction reads it for
  * dbnsFoam (createDynamicFvMesh.H).  Patch types patch, wall, empty, cyclic,
- * symmetryPlane/symmetry (kind ICSF_PATCH); cyclic pairs get their
+ * phaseLagCyclic (kind ICSF_CYCLIC), symmetryPlane/symmetry (kind
+ * ICSF_PATCH); cyclic pairs get their
  * neighbourPatch index and forwardT: identity for translational pairs,
  * for rotational ones the rotation about rotationAxis taking the paired
  * patch's mean normal to minus this patch's.  Sizes are checked (owner /
@@ -457,6 +458,11 @@ int icsf_foam_mesh_get(const icsf_foam_mesh* m, double* points, int* faceOffsets
                        int* neighbour, int* patchStart, int* patchSize, int* patchKind, int* patchNeighbour,
                        double* patchTransform);
 int icsf_foam_mesh_patch_name(const icsf_foam_mesh* m, int patch, char* name, int len);
+/* A phaseLagCyclic patch (read as a cyclic one; phaseLagCyclicPolyPatch.C:
+ * 79-90): *phaseLag 1 and its IBPA, cylCoords, HBZoneName, rotationAxis
+ * and rotationCentre; 0 for any other patch (outputs may be NULL). */
+int icsf_foam_mesh_patch_phase_lag(const icsf_foam_mesh* m, int patch, int* phaseLag, double* ibpa, int* cylCoords,
+                                   double* rotationAxis, double* rotationCentre, char* hbZoneName, int len);
 int icsf_foam_mesh_free(icsf_foam_mesh* m);
 /* Write a polyMesh directory (the five files, ASCII) from the arrays of
  * icsf_mesh_create; cyclic patches get neighbourPatch and a translational

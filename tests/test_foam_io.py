@@ -198,3 +198,56 @@ def test_mrf_properties_and_cell_zones_read(tmp_path):
     assert t.non_moving == z.non_rotating
     bt, _ = OracleCase(r2).residual()
     assert np.abs(bt - b1).max() > 0
+
+
+def test_phase_lag_cyclic_patches_and_hb_properties_read(tmp_path):
+    """A boundary file whose rotational cyclic pair is phaseLagCyclic
+    (IBPA, HBZoneName, cylCoords, rotationCentre), constant/HBProperties
+    and fvSolution's harmonicBalance controls read into the phase-lagged
+    pair the library takes: the owner side (the lower patch index) with its
+    IBPA, cylCoords and rotation axis and centre, the zone's frequencies
+    and harmonics, and the equispaced instants of selectedPeriod."""
+    import re
+
+    from icsfoam_amd.solver import hb_operator
+
+    insts, D, t = cases.case_hb(n=(6, 5, 4), n_harmonics=2, frequency_hz=100.0)
+    m = insts[0].mesh
+    case_dir = tmp_path / "hb"
+    pm = case_dir / "constant" / "polyMesh"
+    foam.write_polymesh(str(pm), m)
+    text = (pm / "boundary").read_text()
+    text, n = re.subn(r"type\s+cyclic;", "type phaseLagCyclic; IBPA 0.6981; HBZoneName blade; cylCoords true;",
+                      text)
+    assert n == 2
+    text, n = re.subn(r"rotationCentre\s*\(0 0 0\);", "rotationCentre (0 0 0.25);", text)
+    assert n == 2
+    (pm / "boundary").write_text(text)
+    (case_dir / "system").mkdir()
+    (case_dir / "system" / "fvSolution").write_text(
+        "FoamFile { version 2.0; format ascii; class dictionary; object fvSolution; }\n"
+        "harmonicBalance { instantsNumber 5; selectedPeriod 0.01; }\n")
+    w = 2 * np.pi * 100.0
+    (case_dir / "constant" / "HBProperties").write_text(
+        "FoamFile { version 2.0; format ascii; class dictionary; object HBProperties; }\n"
+        f"blade {{ active true; allMesh true; frequenciesList ({w!r}); harmonicsList (2); cylCoords true; }}\n"
+        "other { active false; frequenciesList (1); harmonicsList (1); cylCoords false; }\n")
+    r, names = foam.read_polymesh(str(pm))
+    lo = next(i for i, p in enumerate(r.patches) if p.kind == cases.PATCH_CYCLIC)
+    hi = r.patches[lo].neighbour
+    assert hi > lo and r.patches[lo].phase_lag["zone"] == "blade"
+    z = foam.read_hb_properties(str(case_dir))
+    assert list(z) == ["blade"] and z["blade"]["instants"] == 5 and z["blade"]["selected_period"] == 0.01
+    pl = foam.read_phase_lag(str(case_dir), r)
+    assert pl.owner == lo and pl.ibpa == 0.6981 and pl.cyl_coords
+    np.testing.assert_array_equal(pl.frequencies, [w])
+    np.testing.assert_array_equal(pl.harmonics, [2])
+    np.testing.assert_array_equal(pl.centre, [0, 0, 0.25])
+    # the writer's rotationAxis (the pair's rotation about z)
+    assert pl.axis[0] == 0 and pl.axis[1] == 0 and pl.axis[2] != 0
+    np.testing.assert_allclose(pl.instants, hb_operator([w], [2], 5, 0.01)[1], rtol=0, atol=0)
+    np.testing.assert_allclose(pl.instants, t, rtol=0, atol=1e-15)
+    # a phaseLagCyclic patch needs cylCoords (phaseLagCyclicPolyPatch.C:89)
+    (pm / "boundary").write_text(text.replace("cylCoords true;", "", 1))
+    with pytest.raises(Exception, match="cylCoords"):
+        foam.read_polymesh(str(pm))
