@@ -20,9 +20,13 @@ and the oracle's lagged far-side values.
   (Roe, HLLC); without cylCoords a swirling velocity shows the unrotated
   components; AUSM+up differs, its c* taking the lagged sound speed;
 * a residual with a phase-lagged pair differs from the plain cyclic one
-  when the instances differ.
+  when the instances differ;
+* the bounded local time step reads the lagged far density: uniform
+  instances half a period apart flag exactly the lagged-patch cells of the
+  instance whose lagged density falls under the bound.
 """
 import dataclasses
+import re
 
 import numpy as np
 import pytest
@@ -181,3 +185,40 @@ def test_lagged_residual_differs_from_plain_when_instances_differ():
     o.set_phase_lag(hi, hb_phase_lag_operator([w], [K], t, -0.8), cyl=True)
     b1 = o.residual()
     assert np.abs(b1 - b0).max() > 1e-4 * np.abs(b0).max()
+
+
+def test_bounding_reads_the_lagged_far_density():
+    """boundLocalTimeStep.H:62-100 on a phase-lagged pair: rho_'s far value
+    is lagged (the other tests read their far cell).  Spatially uniform
+    instances with rho_J = rho0 (1 + 0.2 sin(w t_J)) and IBPA = pi: the
+    lagged far density of instance 1 is rho0 (1 - 0.2 sin(w t_1)) < 0.95
+    rho_1, so after one pseudo-step its cells on either lagged patch have
+    the CFL number times 0.75 and all other cells (and instances 0 and 2,
+    whose lagged density is not below the bound) keep theirs; without the
+    phase lag no cell is touched."""
+    K, f = 1, 100.0
+    w = 2 * np.pi * f
+    insts, D, t, m, lo = _sector_hb(K, f)
+    hi = m.patches[lo].neighbour
+    c0 = insts[0]
+    rho0, T0 = float(c0.rho.mean()), float(c0.T.mean())
+    U0 = c0.U.mean(axis=0)
+    same = [dataclasses.replace(c, rho=np.full(m.n_cells, rho0 * (1 + 0.2 * np.sin(w * t[J]))),
+                                U=np.tile(U0, (m.n_cells, 1)), T=np.full(m.n_cells, T0))
+            for J, c in enumerate(insts)]
+    co = float(re.search(r"pseudoCoNum\s+([0-9.eE+-]+)", c0.fvSolution).group(1))
+    o0 = OracleHB(same, D)
+    o0.pseudo_steps(1)
+    for J in range(3):
+        np.testing.assert_array_equal(o0.inst[J].pseudoCo(), co)
+    o = OracleHB(same, D)
+    o.set_phase_lag(lo, hb_phase_lag_operator([w], [K], t, np.pi))
+    o.set_phase_lag(hi, hb_phase_lag_operator([w], [K], t, -np.pi))
+    o.pseudo_steps(1)
+    edge = np.zeros(m.n_cells, bool)
+    for p in (lo, hi):
+        pt = m.patches[p]
+        edge[m.owner[pt.start:pt.start + pt.size]] = True
+    np.testing.assert_array_equal(o.inst[0].pseudoCo(), co)
+    np.testing.assert_array_equal(o.inst[2].pseudoCo(), co)
+    np.testing.assert_array_equal(o.inst[1].pseudoCo(), np.where(edge, 0.75 * co, co))
