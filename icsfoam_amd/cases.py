@@ -624,7 +624,7 @@ def annulus_sector(nt: int, nr: int, nz: int, theta: float = np.pi / 8, r0: floa
 def case_sector(nt: int = 8, nr: int = 6, nz: int = 5, theta: float = np.pi / 8, copies: int = 1,
                 jitter: float = 0.1, seed: int = 3, flux: str = "ROE", swirl: bool = True,
                 recon: str = "linearUpwind limitedGrad", noise: float = 1.0, warp_planes: bool = False,
-                **sol) -> Case:
+                mu: float = 0.0, **sol) -> Case:
     """Rotation-periodic state on an annular sector with rotational cyclic
     patches: `copies` rotated copies of a base sector of nt theta cells and
     angle theta (the mesh spans copies*theta).  Cell values of the base
@@ -632,7 +632,7 @@ def case_sector(nt: int = 8, nr: int = 6, nz: int = 5, theta: float = np.pi / 8,
     the local cylindrical frame); copy m carries the same values with the
     velocity rotated by m*theta, so every copy sees identical neighbours.
     Boundary conditions: fixed axial inflow velocity and temperature at z = 0,
-    fixed pressure at z = length, slip hub/shroud walls."""
+    fixed pressure at z = length, slip hub/shroud walls; mu > 0: viscous."""
     mesh = annulus_sector(copies * nt, nr, nz, theta=copies * theta, jitter=jitter, seed=seed, cyclic=True,
                           jitter_period=nt, warp_planes=warp_planes)
     rng = np.random.default_rng(seed + 2)
@@ -665,7 +665,7 @@ def case_sector(nt: int = 8, nr: int = 6, nz: int = 5, theta: float = np.pi / 8,
                                     np.full(p.size, 1.0e5), None, None))
         else:
             bcs.append(BoundarySpec(BC_ZERO_GRADIENT, BC_SLIP, BC_ZERO_GRADIENT))
-    return Case(f"sector{copies}", mesh, rho.ravel(), U.reshape(-1, 3), T.ravel(), bcs, thermo_dict(0.0),
+    return Case(f"sector{copies}", mesh, rho.ravel(), U.reshape(-1, 3), T.ravel(), bcs, thermo_dict(mu),
                 schemes_dict(flux, recon=recon), solution_dict(**sol))
 
 

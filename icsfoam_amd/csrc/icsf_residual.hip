@@ -283,6 +283,11 @@ __device__ __forceinline__ void gradBody(const MeshView& m, const Phys& ph, cons
                 mx[k] = fmax(mx[k], vn);
                 mn[k] = fmin(mn[k], vn);
             }
+            if (visc) {
+                // grad(ecalc): the far cell's value (eCalc is not lagged)
+                const double ef = w * ecc + (1.0 - w) * s.ecalc[m.cycCell[kc]];
+                gex = gex + S * ef;
+            }
         } else {
             const int b = f - m.nI;
             double bf[NFLD];
@@ -534,6 +539,48 @@ __device__ __forceinline__ void faceTermsBody(const MeshView& m, const Phys& ph,
         o[FT_FRDT] = m.nodc[f] * lam;
 #pragma unroll
         for (int k = 0; k < 8; ++k) o[FT_REC + k] = rec[k];
+        if (visc) {
+            // coupled viscous terms: every interpolation w v_P + (1 - w)
+            // v_far with the far value through patchNeighbourField: U
+            // (lagged on a phase-lagged face), grad(U) and tauMC as tensors
+            // (forwardT & T & forwardT^T), grad(eCalc) as a vector; the
+            // snGrad correction interpolates the component gradients
+            // grad(U.component(i)), each rotated as a vector (forwardT &
+            // grad(U)); the coupled face's nonOrthDeltaCoeffs, correction
+            // vector and deltaCoeffs
+            const double* R = m.cycR + 9 * (size_t)kc;
+            const T9 Rt{{R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7], R[8]}};
+            double vo[NGV], vn[NGV];
+            ldRow(s.gv + (size_t)ow * NGV, vo);
+            ldRow(s.gv + (size_t)nc * NGV, vn);
+            T9 gUo, gUn;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) { gUo.t[i] = vo[GV_U + i]; gUn.t[i] = vn[GV_U + i]; }
+            const T9 tauO = mu * dev2(transposeT(gUo));
+            const T9 tauN = mu * dev2(transposeT(gUn));
+            const T9 gUc = w * gUo + (1.0 - w) * tdott(Rt, gUn);
+            const T9 gUf = w * gUo + (1.0 - w) * tdott(tdott(Rt, gUn), transposeT(Rt));
+            const T9 tauf = w * tauO + (1.0 - w) * tdott(tdott(Rt, tauN), transposeT(Rt));
+            const V3 UP = cellU(s.fld, ow);
+            const double muf = w * mu + (1.0 - w) * mu;
+            const double alf = w * alpha + (1.0 - w) * alpha;
+            const V3 k3 = ld3(m.corr + 3 * (size_t)f);
+            const V3 sn = m.nodc[f] * (Un - UP) + vdott(k3, gUc);
+            const V3 lap = (muf * sn) * ms;
+            const V3 dt = vdott(S, tauf);
+            const V3 sig = tdotv(muf * gUf + tauf, Uf);
+            const V3 geO{vo[GV_E], vo[GV_E + 1], vo[GV_E + 2]}, geN{vn[GV_E], vn[GV_E + 1], vn[GV_E + 2]};
+            const V3 gef = w * geO + (1.0 - w) * rot3(R, geN);
+            const double sne = m.nodc[f] * (s.ecalc[nc] - s.ecalc[ow]) + dot(k3, gef);
+            o[FT_LAP] = lap.x; o[FT_LAP + 1] = lap.y; o[FT_LAP + 2] = lap.z;
+            o[FT_TAU] = dt.x; o[FT_TAU + 1] = dt.y; o[FT_TAU + 2] = dt.z;
+            o[FT_SIG] = dot(sig, S);
+            o[FT_LAPE] = (alf * sne) * ms;
+            const double rN = pl ? pf[F_RHO] : s.fld[(size_t)nc * NFLD + F_RHO];
+            const double rf = w * s.fld[(size_t)ow * NFLD + F_RHO] + (1.0 - w) * rN;
+            const double lv = (muf + alf) / rf;
+            o[FT_SV] = 0.5 * lv * ms * m.dc[f];
+        }
     } else {
         const int b = f - m.nI;
         const int c = m.own[f];
@@ -760,6 +807,9 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
             const double hb = 0.5 * ms;
             const double v = 0.5 * ms * t[FT_LAM];
             const double sf = mrf ? m.umrf[f] * ms : 0.0;
+            // the viscous LF coefficient on the coupled face: interface
+            // diagonal - sv, owner diagonal + sv (after the internal faces)
+            const double sv = visc ? t[FT_SV] : 0.0;
             TB* ib = blk + m.cycBase + (size_t)kc * NB;
 #pragma unroll
             for (int e = 0; e < NB; ++e) {
@@ -767,9 +817,11 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
                 if (e == 0 || e == 6 || e == 12 || e == 18 || e == 24) {
                     if (mrf) x = x - sf * (1 - m.w[f]);
                     x = x - v;
+                    if (visc) x = x - sv;
                 }
                 ib[e] = (TB)x;
             }
+            if (visc) vis -= sv;
             fluxJacobian(gf, V3{t[FT_REC], t[FT_REC + 1], t[FT_REC + 2]}, t[FT_REC + 3], nrm, A);
             const double hd = -0.5 * ms;
 #pragma unroll

@@ -1648,8 +1648,6 @@ int createCase(icsf_mesh* mesh, const char* thermo, const char* schemes, const c
     C->mesh = mesh;
     HIPCHK(hipGetDevice(&C->device));
     RC(readSettings(C.get(), thermo, schemes, solution));
-    if (mesh->view.nCyc > 0 && C->ph.viscous)
-        return fail(ICSF_ERR_ARG, "viscous terms on meshes with cyclic patches are not supported");
     if (shared) {
         C->st = shared;
         C->ownStream = false;
@@ -2729,6 +2727,10 @@ int icsf_hb_set_phase_lag(icsf_hb* h, int patch, const double* Dl, int cylCoords
     const int nO = h->nO;
     for (int i = 0; i < nO * nO; ++i)
         if (!std::isfinite(Dl[i])) return fail(ICSF_ERR_ARG, "icsf_hb_set_phase_lag: non-finite time shift");
+    // the viscous terms take grad(U) whole: the reference's unrotated lagged
+    // velocity components would need component gradients of their own
+    if (h->inst[0]->ph.viscous && !cylCoords)
+        return fail(ICSF_ERR_ARG, "icsf_hb_set_phase_lag: viscous cases need cylCoords");
     double geo[8] = {0, 0, 1, 0, 0, 0, 0, 0};
     if (cylCoords) {
         if (!rotationAxis || !rotationCentre)

@@ -358,8 +358,9 @@ int icsf_hb_phase_lag_operator(int nFreq, const double* frequencies, const int* 
  * Call once per side, with that side's Dl; cylCoords (nonzero) combines
  * the velocities in cylindrical components about rotationAxis through
  * rotationCentre (the patch's), else the velocity components are
- * combined unrotated as the reference does.  Set before the instances'
- * fields or after them; the next residual uses it. */
+ * combined unrotated as the reference does (refused for viscous cases,
+ * whose terms take grad(U) whole).  Set before the instances' fields or
+ * after them; the next residual uses it. */
 int icsf_hb_set_phase_lag(icsf_hb* h, int patch, const double* Dl, int cylCoords, const double* rotationAxis,
                           const double* rotationCentre);
 /* Sets the launch parameters (the layout of icsf_get_tuning; params[5]

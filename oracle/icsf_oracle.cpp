@@ -1065,18 +1065,20 @@ void viscousResidual(Case& k, vec& RU /*3nC*/, vec& RE) {
         gU[o] = gU[o] + t;
         gU[n] = gU[n] - t;
     }
+    // (cyclic faces: the coupled interpolation w U_P + (1 - w) U_far)
     for (int f = m.nI; f < m.nF; ++f) {
         if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
-        T9 t = outer(v3(&m.Sf[3 * f]), v3(&k.UB[3 * (f - m.nI)]));
+        const V3 Ub = m.cyclicFace(f) ? interpCyclicU(k, f) : v3(&k.UB[3 * (f - m.nI)]);
+        T9 t = outer(v3(&m.Sf[3 * f]), Ub);
         gU[m.own[f]] = gU[m.own[f]] + t;
     }
     for (int c = 0; c < m.nC; ++c)
         for (int i = 0; i < 9; ++i) gU[c].t[i] /= m.V[c];
-    // boundary snGrad of U and gradient boundary values
+    // boundary snGrad of U and gradient boundary values (physical patches)
     std::vector<V3> snU(nB, V3{0, 0, 0});
     std::vector<T9> gUb(nB, T9{{0}});
     for (size_t pi = 0; pi < m.pStart.size(); ++pi) {
-        if (m.pKind[pi] == PK_EMPTY) continue;
+        if (m.pKind[pi] == PK_EMPTY || m.pKind[pi] == PK_CYCLIC) continue;
         for (int i = 0; i < m.pSize[pi]; ++i) {
             const int f = m.pStart[pi] + i, b = f - m.nI, c = m.own[f];
             V3 n = v3(&m.Sf[3 * f]) / m.magSf[f];
@@ -1101,7 +1103,8 @@ void viscousResidual(Case& k, vec& RU /*3nC*/, vec& RE) {
     for (int b = 0; b < nB; ++b)
         if (k.rhoB[b] != 0) eb[b] = k.rhoEB[b] / k.rhoB[b] - 0.5 * magSqr(v3(&k.UB[3 * b]));
     std::vector<V3> ge;
-    gaussGrad(m, ec.data(), eb.data(), 1, ge);
+    const vec ecFar = cyclicNbr(k, ec.data(), 1, -1, PL_NONE);
+    gaussGrad(m, ec.data(), eb.data(), 1, ge, ecFar.data());
     // surface fields
     vec lapU(3 * m.nF, 0), divTau(3 * m.nF, 0), sDotU(m.nF, 0), lapE(m.nF, 0);
     for (int f = 0; f < m.nI; ++f) {
@@ -1129,6 +1132,40 @@ void viscousResidual(Case& k, vec& RU /*3nC*/, vec& RE) {
         if (m.pKind[m.facePatch[f - m.nI]] == PK_EMPTY) continue;
         const int b = f - m.nI, c = m.own[f];
         V3 S = v3(&m.Sf[3 * f]);
+        if (m.cyclicFace(f)) {
+            // coupled faces: every interpolation w v_P + (1 - w) v_far with
+            // the far value through patchNeighbourField: U (lagged on a
+            // phase-lagged face), grad(U) and tauMC as tensors (forwardT &
+            // T & forwardT^T), grad(eCalc) as a vector; the snGrad
+            // correction interpolates the component gradients
+            // grad(U.component(i)) (correctedSnGrad<vector>, each rotated as
+            // a vector: forwardT & grad(U)); nonOrthDeltaCoeffs and the
+            // correction vectors of the coupled face
+            const int n = m.cycCell[b];
+            const double w = m.w[f];
+            const double* R9 = &m.cycR[9 * (size_t)b];
+            const T9 Rt{{R9[0], R9[1], R9[2], R9[3], R9[4], R9[5], R9[6], R9[7], R9[8]}};
+            const V3 k3 = v3(&m.corr[3 * f]);
+            const V3 UP = v3(&k.U[3 * c]);
+            const V3 UN = lagFace(k, b) ? lagU(k, b) : transformV(R9, v3(&k.U[3 * n]));
+            const T9 gUc = w * gU[c] + (1.0 - w) * tdott(Rt, gU[n]);
+            const T9 gUf = w * gU[c] + (1.0 - w) * tdott(tdott(Rt, gU[n]), transposeT(Rt));
+            const T9 tauf = w * tau[c] + (1.0 - w) * tdott(tdott(Rt, tau[n]), transposeT(Rt));
+            const V3 Uf = w * UP + (1.0 - w) * UN;
+            const double muf = w * mu + (1.0 - w) * mu;
+            const double alf = w * alpha + (1.0 - w) * alpha;
+            const V3 sn = m.nodc[f] * (UN - UP) + vdott(k3, gUc);
+            V3 l = (muf * sn) * m.magSf[f];
+            lapU[3 * f] = l.x; lapU[3 * f + 1] = l.y; lapU[3 * f + 2] = l.z;
+            V3 dt = vdott(S, tauf);
+            divTau[3 * f] = dt.x; divTau[3 * f + 1] = dt.y; divTau[3 * f + 2] = dt.z;
+            V3 sig = tdotv(muf * gUf + tauf, Uf);
+            sDotU[f] = dot(sig, S);
+            const V3 gef = w * ge[c] + (1.0 - w) * transformV(R9, ge[n]);
+            const double sne = m.nodc[f] * (ec[n] - ec[c]) + dot(k3, gef);
+            lapE[f] = (alf * sne) * m.magSf[f];
+            continue;
+        }
         V3 l = (mu * snU[b]) * m.magSf[f];
         lapU[3 * f] = l.x; lapU[3 * f + 1] = l.y; lapU[3 * f + 2] = l.z;
         V3 dt = vdott(S, taub[b]);
@@ -1593,6 +1630,26 @@ void assemble(Case& k) {
                 }
             }
             negSumDiag(m, mx);
+            // coupled faces (blockFvOperatorsTemplates.C:501-556):
+            // interfacesUpper = interfacesLower = the face's coefficient,
+            // diag[own] -= interfacesLower; rho_f = w rho_P + (1 - w)
+            // rho_far (lagged on a phase-lagged face)
+            if (m.anyCyclic) {
+                mx.iu.assign((size_t)(m.nF - nI) * nc, 0.0);
+                for (int f = nI; f < m.nF; ++f) {
+                    if (!m.cyclicFace(f)) continue;
+                    const int b = f - nI, o = m.own[f];
+                    const double rN = lagFace(k, b) ? lagScalar(k, b, PL_RHO, 0) : k.rho[m.cycCell[b]];
+                    const double rf = interpCyclic(m, f, k.rho[o], rN);
+                    const double muf = m.w[f] * mu + (1.0 - m.w[f]) * mu;
+                    const double alf = m.w[f] * alpha + (1.0 - m.w[f]) * alpha;
+                    const double lv = (muf + alf) / rf;
+                    const double s = 0.5 * lv * m.magSf[f] * m.dc[f];
+                    if (nc == 1) { mx.iu[b] = s; mx.diag[o] -= s; }
+                    else
+                        for (int d = 0; d < 3; ++d) { mx.iu[9 * b + 4 * d] = s; mx.diag[9 * o + 4 * d] -= s; }
+                }
+            }
             addBlockTo(target, mx, -1.0);
         };
         stab(k.SS[0][0], 1);
@@ -3305,6 +3362,7 @@ int orc_hb_set_phase_lag(void* hp, int patch, const double* Dl, int cyl, const d
     HB& h = *(HB*)hp;
     const Mesh& m = *h.k[0]->m;
     if (patch < 0 || patch >= (int)m.pStart.size() || m.pKind[patch] != PK_CYCLIC) return -1;
+    if (h.k[0]->s.viscous && !cyl) return -2;  // the viscous terms take grad(U) whole
     PhaseLag& L = h.pl;
     if (L.sides.empty()) {
         L.nO = h.nO;

@@ -90,6 +90,61 @@ def test_sector_equals_double_sector(flux, recon, swirl):
     assert np.count_nonzero(np.abs(iu).sum(axis=(1, 2))) == nCyc
 
 
+@pytest.mark.parametrize("flux", ["ROE", "AUSMPlusUp"])
+def test_viscous_sector_equals_double_sector(flux):
+    """The viscous terms across cyclic faces (laplacian(muEff, U) with the
+    corrected snGrad, div(tauMC), div(sigmaDotU), laplacian(alphaEff, e) and
+    the LF viscous Jacobian's interface coefficients) on an axial,
+    rotation-periodic flow (second order): the sector's cells see what the
+    first half of the double sector sees, residual, diagonal blocks and
+    A x to round-off; the far grad(U), tauMC and grad(e) enter rotated (an
+    unrotated far tensor or vector breaks this), and the viscous terms are
+    present (the inviscid residual differs).  A swirling velocity does not
+    agree: the snGrad correction rotates the far component gradients
+    grad(U.component(i)) without mixing them, as for the reconstruction."""
+    kw = dict(recon="linearUpwind limitedGrad", swirl=False, flux=flux, warp_planes=True, mu=2.0e-3)
+
+    def axial_walls(c):
+        # hub and shroud hold an axial velocity (fixedValue), so no face
+        # value has a radial or swirl part: grad(U) has only its axial
+        # column, whose rotation equals the tensor transform
+        bcs = [b if b is None or b.U_type != cases.BC_SLIP else
+               cases.BoundarySpec(cases.BC_ZERO_GRADIENT, cases.BC_FIXED_VALUE, cases.BC_ZERO_GRADIENT,
+                                  U_val=np.tile([0.0, 0.0, 150.0], (p.size, 1)))
+               for b, p in zip(c.bcs, c.mesh.patches)]
+        return dataclasses.replace(c, bcs=bcs)
+    c1 = axial_walls(cases.case_sector(NT, NR, NZ, TH, copies=1, **kw))
+    c2 = axial_walls(cases.case_sector(NT, NR, NZ, TH, copies=2, **kw))
+    grid = np.arange(2 * c1.mesh.n_cells).reshape(NZ, 2 * NT, NR)
+    first = grid[:, :NT, :].ravel()
+    o1, o2 = OracleCase(c1), OracleCase(c2)
+    b1, _ = o1.residual()
+    b2, _ = o2.residual()
+    for q in range(5):
+        assert _rel(b2[q, first], b1[q]) < 1e-12, (q, _rel(b2[q, first], b1[q]))
+    d1, _, _ = o1.createJacobian()
+    d2, _, _ = o2.createJacobian()
+    assert _rel(d2[first], d1) < 1e-13
+    x1 = np.random.default_rng(1).standard_normal((5, c1.mesh.n_cells))
+    R = cases.rotation_z(TH)
+    x2 = np.zeros((5, c2.mesh.n_cells))
+    x2[:, first] = x1
+    second = grid[:, NT:, :].ravel()
+    x2[:2, second] = x1[:2]
+    x2[2:, second] = R @ x1[2:]
+    y1, y2 = o1.matrixMul(x1), o2.matrixMul(x2)
+    for q in range(5):
+        assert _rel(y2[q, first], y1[q]) < 1e-13, q
+    b0, _ = OracleCase(dataclasses.replace(c1, thermo=cases.thermo_dict(0.0))).residual()
+    assert _rel(b1, b0) > 1e-6
+    cs = dict(kw, swirl=True)
+    s1 = axial_walls(cases.case_sector(NT, NR, NZ, TH, copies=1, **cs))
+    s2 = axial_walls(cases.case_sector(NT, NR, NZ, TH, copies=2, **cs))
+    e1, _ = OracleCase(s1).residual()
+    e2, _ = OracleCase(s2).residual()
+    assert max(_rel(e2[q, first], e1[q]) for q in range(5)) > 1e-8
+
+
 def test_second_order_swirl_follows_componentwise_reconstruction():
     c1, c2, first, _ = _pair("linearUpwind limitedGrad", True)
     b1, _ = OracleCase(c1).residual()

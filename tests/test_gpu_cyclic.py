@@ -2,7 +2,9 @@
 
 Annular sectors whose theta planes are a rotational cyclic pair (seeded
 rotation-periodic swirling state, permuted + RCM with the coupled cells +
-colour-major numbering), AUSM+up / Roe second order and HLLC first order:
+colour-major numbering), AUSM+up / Roe second order (inviscid and
+viscous: the viscous terms and their LF Jacobian across the cyclic faces)
+and HLLC first order:
 
 * face fluxes, residual, rPseudoDeltaT, Jacobian blocks (diagonal,
   upper/lower and the cyclic interface blocks), block SpMV and LU-SGS:
@@ -28,8 +30,8 @@ from helpers import rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _sector(flux, recon="linearUpwind limitedGrad", **sol):
-    c = cases.case_sector(8, 6, 5, np.pi / 8, flux=flux, recon=recon, swirl=True, co=20.0, **sol)
+def _sector(flux, recon="linearUpwind limitedGrad", mu=0.0, **sol):
+    c = cases.case_sector(8, 6, 5, np.pi / 8, flux=flux, recon=recon, swirl=True, co=20.0, mu=mu, **sol)
     return prepare_case(c, permute_seed=19)
 
 
@@ -37,6 +39,8 @@ CASES = {
     "sector_roe": lambda: _sector("ROE"),
     "sector_ausm": lambda: _sector("AUSMPlusUp"),
     "sector_hllc_upwind": lambda: _sector("HLLC", recon="upwind"),
+    "sector_roe_viscous": lambda: _sector("ROE", mu=2.0e-3),
+    "sector_ausm_viscous": lambda: _sector("AUSMPlusUp", mu=2.0e-3),
 }
 
 

@@ -1,7 +1,9 @@
 """Phase-lagged cyclic patches of harmonic balance on the GPU against the
 oracle: an annular sector whose rotational cyclic pair is phase-lagged
 (inter-blade phase angle 2 pi/9), 3 and 5 instants, Roe, AUSM+up and HLLC,
-velocity combined unrotated or in cylindrical components (cylCoords).
+velocity combined unrotated or in cylindrical components (cylCoords), and a
+viscous Roe sector (cylCoords; the unrotated form is refused for viscous
+cases).
 
 * residual with the lagged far-side values and the HB source, coupled SpMV
   and LU-SGS (the Jacobian's face states lagged, the interfaces plain):
@@ -28,8 +30,8 @@ pytestmark = pytest.mark.gpu
 FREQ = 100.0
 
 
-def _hb(K, flux, cyl, edit=None):
-    insts, D, t = cases.case_hb(n=(8, 6, 5), n_harmonics=K, frequency_hz=FREQ, flux=flux, jitter=0.1)
+def _hb(K, flux, cyl, edit=None, mu=0.0):
+    insts, D, t = cases.case_hb(n=(8, 6, 5), n_harmonics=K, frequency_hz=FREQ, flux=flux, jitter=0.1, mu=mu)
     m = insts[0].mesh
     lo = next(i for i, p in enumerate(m.patches) if p.kind == cases.PATCH_CYCLIC)
     pair = cases.PhaseLagPair(owner=lo, ibpa=2 * np.pi / 9, frequencies=np.array([2 * np.pi * FREQ]),
@@ -54,6 +56,7 @@ CASES = {
     "hb3_roe_cyl": lambda: _hb(1, "ROE", True),
     "hb5_ausm": lambda: _hb(2, "AUSMPlusUp", False),
     "hb3_hllc_cyl": lambda: _hb(1, "HLLC", True),
+    "hb3_roe_viscous_cyl": lambda: _hb(1, "ROE", True, mu=2.0e-3),
 }
 
 
@@ -84,6 +87,15 @@ def test_phase_lag_pseudo_steps(pair):
         fg, fo = g.fields(J), o.fields(J)
         for k in ("rho", "U", "T", "p", "W"):
             assert rel_err(fg[k], fo[k]) < 1e-8, (J, k)
+
+
+def test_phase_lag_viscous_needs_cyl_coords():
+    """The viscous terms take grad(U) whole: the unrotated lagged velocity
+    components are refused for viscous cases."""
+    from icsfoam_amd._lib import IcsfError
+    insts, D = _hb(1, "ROE", False, mu=2.0e-3)
+    with pytest.raises(IcsfError, match="cylCoords"):
+        hbCase.from_cases(insts, D)
 
 
 def test_phase_lag_graph_replay_and_mixed_form():
