@@ -177,7 +177,7 @@ class Case:
     fvSchemes: str
     fvSolution: str
     cell_order: Optional[np.ndarray] = None   # new -> original cell index
-    mrf: Optional[MRFZone] = None
+    mrf: Optional[MRFZone] = None           # or a list of zones (MRFCoupledZoneList order)
     translating: Optional[TranslatingZone] = None
     phase_lag: Optional[PhaseLagPair] = None
 
@@ -554,7 +554,12 @@ def permute_case(case: Case, new_to_old: np.ndarray, renumber) -> Case:
     renumbering routine and carry the cell fields along."""
     mesh = renumber(case.mesh, new_to_old)
     order = new_to_old if case.cell_order is None else case.cell_order[new_to_old]
-    mrf = None if case.mrf is None else dataclasses.replace(case.mrf, cells=case.mrf.cells[new_to_old])
+    if case.mrf is None:
+        mrf = None
+    elif isinstance(case.mrf, (list, tuple)):
+        mrf = [dataclasses.replace(z, cells=z.cells[new_to_old]) for z in case.mrf]
+    else:
+        mrf = dataclasses.replace(case.mrf, cells=case.mrf.cells[new_to_old])
     tr = None if case.translating is None else \
         dataclasses.replace(case.translating, cells=case.translating.cells[new_to_old])
     return dataclasses.replace(case, mesh=mesh, rho=case.rho[new_to_old],

@@ -148,6 +148,7 @@ struct FramePart {
     std::vector<double> un;
     std::vector<char> incl;
     std::vector<double> Ub;
+    std::vector<char> set;  // faces the zone assigns (face types 1 and 2)
 };
 
 struct icsf_case {
@@ -179,7 +180,11 @@ struct icsf_case {
     // the boundary faces that move with them and the frame velocity there
     // (their fixedValue U is set to it)
     bool mrf = false;
+    // rotating zones in list order (MRFCoupledZoneList), their combination
+    // frot and the translating zone ftr; Omega of every cell
+    std::vector<FramePart> rotZones;
     FramePart frot, ftr;
+    std::vector<double> omcHost;
     std::vector<char> mrfIncl;
     std::vector<double> mrfUb;
     double *d_umrf = nullptr, *d_omC = nullptr;
@@ -1868,9 +1873,11 @@ int framePart(icsf_case* c, const char* who, int nZone, const int* zone, int nNo
     out.un.assign(nF, 0.0);
     out.incl.assign(std::max(nB, 1), 0);
     out.Ub.assign((size_t)3 * std::max(nB, 1), 0.0);
+    out.set.assign(nF, 0);
     for (int f = 0; f < nF; ++f) {
         V3 fu{0, 0, 0};
         if (type[f]) fu = faceU(f);
+        out.set[f] = type[f] != 0;
         out.un[f] = dot(fu, V3{h.Sf[3 * f], h.Sf[3 * f + 1], h.Sf[3 * f + 2]} / h.magSf[f]);
         if (f >= nI && type[f] == 1) {
             const int b = f - nI;
@@ -1881,10 +1888,30 @@ int framePart(icsf_case* c, const char* who, int nZone, const int* zone, int nNo
     return ICSF_OK;
 }
 
-// The case's frame arrays from its zones: um of the rotating zone plus um
+// The rotating zones combined as MRFCoupledZoneList combines them
+// (MRFCoupledZoneList.C:55-142, MRFCoupledZone.C:307-318,563-627): each
+// zone in list order assigns faceU on its faces, Omega in its cells and U
+// on its moving patch faces, so a later zone overrides an earlier one
+// where they meet.
+FramePart rotCombined(const std::vector<FramePart>& zones) {
+    FramePart R = zones.front();
+    for (size_t z = 1; z < zones.size(); ++z) {
+        const FramePart& Z = zones[z];
+        for (size_t f = 0; f < R.un.size(); ++f)
+            if (Z.set[f]) { R.un[f] = Z.un[f]; R.set[f] = 1; }
+        for (size_t b = 0; b < R.incl.size(); ++b)
+            if (Z.incl[b]) {
+                R.incl[b] = 1;
+                for (int d = 0; d < 3; ++d) R.Ub[3 * b + d] = Z.Ub[3 * b + d];
+            }
+    }
+    return R;
+}
+
+// The case's frame arrays from its zones: um of the rotating zones plus um
 // of the translating one, the moving faces' U (the translating zone's
-// after the rotating one's, outerLoop.H:6-7) written into their fixed
-// values; Omega from the rotating zone.  The frame divergence adds a
+// after the rotating ones', outerLoop.H:6-7) written into their fixed
+// values; Omega from the rotating zones.  The frame divergence adds a
 // diagonal term to every off-diagonal block that the generator form does
 // not carry, so a case with generators switches to full blocks.
 int frameCompose(icsf_case* c) {
@@ -1938,7 +1965,6 @@ int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const doubl
                  double omega, int nNonRotating, const int* nonRotatingPatches) {
     if (!c || (nZoneCells > 0 && !zoneCells) || !origin || !axis || (nNonRotating > 0 && !nonRotatingPatches))
         return fail(ICSF_ERR_ARG, "icsf_set_mrf: NULL argument");
-    if (c->frot.on) return fail(ICSF_ERR_STATE, "icsf_set_mrf: the case already has a rotating zone");
     if (!std::isfinite(omega) || !std::isfinite(origin[0]) || !std::isfinite(origin[1]) || !std::isfinite(origin[2]) ||
         !std::isfinite(axis[0]) || !std::isfinite(axis[1]) || !std::isfinite(axis[2]))
         return fail(ICSF_ERR_NONFINITE, "icsf_set_mrf: origin, axis and omega must be finite");
@@ -1953,13 +1979,17 @@ int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const doubl
     FramePart P;
     RC(framePart(c, "icsf_set_mrf", nZoneCells, zoneCells, nNonRotating, nonRotatingPatches,
                  [&](int f) { return cross(Om, V3{h.Cf[3 * f], h.Cf[3 * f + 1], h.Cf[3 * f + 2]} - o); }, P));
-    std::vector<double> omc((size_t)3 * nC, 0.0);
+    std::vector<double>& omc = c->omcHost;
+    if (omc.empty()) omc.assign((size_t)3 * nC, 0.0);
     for (int i = 0; i < nZoneCells; ++i) {
         const int z = zoneCells[i];
         omc[3 * (size_t)z] = Om.x; omc[3 * (size_t)z + 1] = Om.y; omc[3 * (size_t)z + 2] = Om.z;
     }
-    RC(c->buf.upload(c->d_omC, omc.data(), omc.size()));
-    c->frot = std::move(P);
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (!c->d_omC) RC(c->buf.alloc(c->d_omC, omc.size()));
+    HIPCHK(hipMemcpy(c->d_omC, omc.data(), omc.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->rotZones.push_back(std::move(P));
+    c->frot = rotCombined(c->rotZones);
     return frameCompose(c);
 }
 

@@ -226,11 +226,11 @@ def _cell_zone(poly_mesh_dir: str, name: str, n_cells: int) -> np.ndarray:
     return cells
 
 
-def _active_zone(path: str, what: str):
-    """The one active top-level zone entry of a zone-list file: (name,
-    body), or None without the file or an active zone."""
+def _active_zones(path: str) -> List[Tuple[str, str]]:
+    """The active top-level zone entries of a zone-list file, in file order:
+    [(name, body)], empty without the file."""
     if not os.path.exists(path):
-        return None
+        return []
     with open(path) as f:
         text = _strip_comments(f.read())
     zones = []
@@ -239,6 +239,13 @@ def _active_zone(path: str, what: str):
         if mt and mt.group(1).lower() in ("no", "false", "off", "n", "none"):
             continue
         zones.append((zname, body))
+    return zones
+
+
+def _active_zone(path: str, what: str):
+    """The one active top-level zone entry of a zone-list file: (name,
+    body), or None without the file or an active zone."""
+    zones = _active_zones(path)
     if not zones:
         return None
     if len(zones) > 1:
@@ -277,16 +284,21 @@ def read_translating(case_dir: str, n_cells: int, names: List[str]) -> Optional[
                            non_moving=_patches(body, "nonMovingPatches", names))
 
 
-def read_mrf(case_dir: str, n_cells: int, names: List[str]) -> Optional[MRFZone]:
+def read_mrf(case_dir: str, n_cells: int, names: List[str]):
     """constant/MRFProperties (IOMRFCoupledZoneList / MRFCoupledZone.C:
-    240-296): the active zone's cellZone, origin, axis, omega (a constant:
+    240-296): each active zone's cellZone, origin, axis, omega (a constant:
     `omega 100;` or `omega constant 100;`) and nonRotatingPatches (names or
     regular expressions, matched against the patch names); None without the
-    file or without an active zone.  One active zone is supported."""
-    z = _active_zone(os.path.join(case_dir, "constant", "MRFProperties"), "MRFProperties")
-    if z is None:
+    file or without an active zone, one MRFZone for one active zone, else
+    the list of zones in file order (the zone list's order)."""
+    zones = [_mrf_zone(case_dir, n_cells, names, zname, body)
+             for zname, body in _active_zones(os.path.join(case_dir, "constant", "MRFProperties"))]
+    if not zones:
         return None
-    zname, body = z
+    return zones[0] if len(zones) == 1 else zones
+
+
+def _mrf_zone(case_dir: str, n_cells: int, names: List[str], zname: str, body: str) -> MRFZone:
     mt = re.search(r"\bcellZone\s+([\w.:-]+)\s*;", body)
     if not mt:
         raise IcsfError(-2, f"MRFProperties {zname}: missing cellZone")

@@ -68,6 +68,14 @@ class fvMesh:
             self._h = None
 
 
+
+def _zones(mrf):
+    """A case's rotating zones as a list (cases.MRFZone, a list of them, or
+    None)."""
+    if mrf is None:
+        return []
+    return list(mrf) if isinstance(mrf, (list, tuple)) else [mrf]
+
 class dbnsCase:
     """Fields + schemes + solution controls of one dbnsFoam case on the GPU."""
 
@@ -89,8 +97,8 @@ class dbnsCase:
             self.set_boundary(i, bc.p_type, bc.U_type, bc.T_type, bc.p_val, bc.U_val, bc.T_val)
             if bc.p_init is not None or bc.U_init is not None or bc.T_init is not None:
                 self.set_boundary_values(i, bc.p_init, bc.U_init, bc.T_init)
-        if getattr(case, "mrf", None) is not None:
-            self.set_mrf(case.mrf)
+        for z in _zones(getattr(case, "mrf", None)):
+            self.set_mrf(z)
         if getattr(case, "translating", None) is not None:
             self.set_translating(case.translating)
         self.set_fields(case.rho, case.U, case.T)
@@ -116,9 +124,9 @@ class dbnsCase:
         check(lib().icsf_set_fields(self._h, ptr(as_f64(rho)), ptr(as_f64(U)), ptr(as_f64(T))))
 
     def set_mrf(self, zone):
-        """One rotating zone (cases.MRFZone: bool cell mask, origin, axis,
-        omega, non-rotating patch indices); after the boundary conditions
-        and before the fields (icsf_set_mrf)."""
+        """Adds a rotating zone (cases.MRFZone: bool cell mask, origin,
+        axis, omega, non-rotating patch indices; the zone list's next entry);
+        after the boundary conditions and before the fields (icsf_set_mrf)."""
         cells = as_i32(np.flatnonzero(np.asarray(zone.cells, bool)))
         nr = as_i32(np.asarray(zone.non_rotating, np.int32).reshape(-1))
         check(lib().icsf_set_mrf(self._h, int(cells.shape[0]), ptr(cells), ptr(as_f64(zone.origin)),
@@ -332,8 +340,8 @@ class hbCase:
                 self.set_boundary(J, i, bc.p_type, bc.U_type, bc.T_type, bc.p_val, bc.U_val, bc.T_val)
                 if bc.p_init is not None or bc.U_init is not None or bc.T_init is not None:
                     self.set_boundary_values(J, i, bc.p_init, bc.U_init, bc.T_init)
-        if getattr(c0, "mrf", None) is not None:
-            self.set_mrf(c0.mrf)
+        for z in _zones(getattr(c0, "mrf", None)):
+            self.set_mrf(z)
         if getattr(c0, "translating", None) is not None:
             z = c0.translating
             cells = as_i32(np.flatnonzero(np.asarray(z.cells, bool)))
@@ -362,7 +370,7 @@ class hbCase:
         return Do, Dn
 
     def set_mrf(self, zone):
-        """One rotating zone for all instances (cases.MRFZone)."""
+        """Adds a rotating zone for all instances (cases.MRFZone)."""
         cells = as_i32(np.flatnonzero(np.asarray(zone.cells, bool)))
         nr = as_i32(np.asarray(zone.non_rotating, np.int32).reshape(-1))
         check(lib().icsf_hb_set_mrf(self._h, int(cells.shape[0]), ptr(cells), ptr(as_f64(zone.origin)),

@@ -305,7 +305,7 @@ int icsf_get_interface_blocks(icsf_case* c, double* blocks);
  * true): the first pseudo-step times each candidate on the case's own data
  * and keeps the fastest; results are the same bits for every choice. */
 int icsf_get_tuning(icsf_case* c, int* params);
-/* One rotating zone (MRF; replaces MRFCoupledZoneList / MRFProperties,
+/* Adds a rotating zone (MRF; replaces MRFCoupledZoneList / MRFProperties,
  * MRFCoupledZone.C:42-675, and the frame terms of the flux schemes and of
  * the Jacobian): the zone's cells, origin, axis (normalised here), omega
  * [rad/s] and the patches that do not rotate (nonRotatingPatches).  Faces of
@@ -316,8 +316,11 @@ int icsf_get_tuning(icsf_case* c, int* params);
  * Omega ^ (Cf - origin).  Call after the boundary conditions and before the
  * fields.  The frame divergence puts a diagonal term on every off-diagonal
  * block, which the generator form does not hold: a case with
- * jacobianStorage generators switches to full blocks here.  One zone per
- * case; the residual VJP refuses such cases. */
+ * jacobianStorage generators switches to full blocks here.  Each call adds
+ * the zone list's next entry: as MRFCoupledZoneList (MRFCoupledZoneList.C:
+ * 55-142), every zone in call order assigns its faces' um, its cells' Omega
+ * and its rotating faces' U, so a later zone overrides an earlier one where
+ * they meet.  The residual VJP refuses such cases. */
 int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const double* origin, const double* axis,
                  double omega, int nNonRotating, const int* nonRotatingPatches);
 /* One translating zone (replaces MRFTranslatingZoneList /
@@ -325,9 +328,9 @@ int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const doubl
  * zone's cells, frameVelocity and nonMovingPatches.  The same face sets as
  * the rotating zone; its faces add frameVelocity . Sf/|Sf| to the face
  * velocity the schemes see, its moving patches (fixedValue U) take the
- * frame velocity (after a rotating zone's), no Coriolis term.  One per case,
- * beside at most one rotating zone; the same storage switch and VJP
- * refusal as icsf_set_mrf. */
+ * frame velocity (after the rotating zones'), no Coriolis term.  One per
+ * case, beside any rotating zones; the same storage switch and VJP refusal
+ * as icsf_set_mrf. */
 int icsf_set_translating_frame(icsf_case* c, int nZoneCells, const int* zoneCells, const double* velocity,
                                int nNonMoving, const int* nonMovingPatches);
 /* The same zones for every instance of a harmonic-balance case (the

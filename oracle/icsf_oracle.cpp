@@ -360,6 +360,7 @@ struct FramePart {
     vec un;                // faceU & Sf/|Sf| per face
     std::vector<char> incl;  // moving boundary faces
     vec Ub;                // their U
+    std::vector<char> set;   // faces the zone assigns (types 1 and 2)
 };
 
 struct Case {
@@ -397,7 +398,9 @@ struct Case {
     bool mrf = false;
     vec umrf, omegaC, mrfUb;
     std::vector<char> mrfIncl;
-    // the rotating and the translating zone's parts (set-up)
+    // the rotating zones in list order, their combination and the
+    // translating zone's part (set-up)
+    std::vector<FramePart> rotZones;
     FramePart frot, ftr;
     // harmonic balance with phase-lagged cyclic patches: the instances'
     // shared set-up and this case's instance index
@@ -2802,7 +2805,7 @@ void* orc_case_new(void* mesh, const int* is, const double* rs) {
 }
 void orc_case_free(void* h) { delete (Case*)h; }
 
-// Frame zones: one rotating zone (MRFCoupledZone.C:42-189 setMRFFaces,
+// Frame zones: rotating zones (MRFCoupledZone.C:42-189 setMRFFaces,
 // :240-296 constructor, :549-577 faceU, :580-602 correctBoundaryVelocity)
 // and one translating zone (MRFTranslatingZone.C, the same face sets, faceU =
 // the frame velocity).  Face types: 1 internal faces with a zone cell on
@@ -2812,7 +2815,7 @@ void orc_case_free(void* h) { delete (Case*)h; }
 // boundary faces) take U = faceU, the translating zone's after the
 // rotating one's (outerLoop.H:6-7).  The face velocity the schemes see is
 // MRF.faceU() & n + MRFTrans.faceU() & n (outerLoop.H:18-21); Omega (the
-// Coriolis terms) comes from the rotating zone only.
+// Coriolis terms) comes from the rotating zones only.
 // returns 1 if a moving patch's U condition is not fixedValue
 int framePart(const Case& k, const int* zone, int nZone, const int* nonMov, int nNonMov,
               const std::function<V3(int)>& faceU, FramePart& out) {
@@ -2837,9 +2840,11 @@ int framePart(const Case& k, const int* zone, int nZone, const int* nonMov, int 
     out.un.assign(m.nF, 0.0);
     out.incl.assign(nB, 0);
     out.Ub.assign(3 * (size_t)nB, 0.0);
+    out.set.assign(m.nF, 0);
     for (int f = 0; f < m.nF; ++f) {
         V3 fu{0, 0, 0};
         if (type[f]) fu = faceU(f);
+        out.set[f] = type[f] != 0;
         out.un[f] = dot(fu, v3(&m.Sf[3 * f]) / m.magSf[f]);
         if (f >= m.nI && type[f] == 1) {
             const int b = f - m.nI;
@@ -2848,6 +2853,25 @@ int framePart(const Case& k, const int* zone, int nZone, const int* nonMov, int 
         }
     }
     return 0;
+}
+
+// Several rotating zones (MRFCoupledZoneList.C:55-142): each zone in list
+// order assigns faceU on its faces, Omega in its cells and U on its moving
+// patch faces (MRFCoupledZone.C:307-318, 563-627), so a later zone
+// overrides an earlier one where they meet.
+FramePart rotCombined(const std::vector<FramePart>& zones) {
+    FramePart R = zones.front();
+    for (size_t z = 1; z < zones.size(); ++z) {
+        const FramePart& Z = zones[z];
+        for (size_t f = 0; f < R.un.size(); ++f)
+            if (Z.set[f]) { R.un[f] = Z.un[f]; R.set[f] = 1; }
+        for (size_t b = 0; b < R.incl.size(); ++b)
+            if (Z.incl[b]) {
+                R.incl[b] = 1;
+                for (int d = 0; d < 3; ++d) R.Ub[3 * b + d] = Z.Ub[3 * b + d];
+            }
+    }
+    return R;
 }
 
 // the case's frame arrays from its parts
@@ -2870,8 +2894,8 @@ void frameCompose(Case& k) {
     if (k.omegaC.empty()) k.omegaC.assign(3 * (size_t)m.nC, 0.0);
 }
 
-// The rotating zone: cells, origin, axis (normalised), omega [rad/s],
-// non-rotating patches.  Returns 1 if a rotating patch's U is not
+// One more rotating zone (the zone list's next entry): cells, origin,
+// axis (normalised), omega [rad/s], non-rotating patches.  Returns 1 if a rotating patch's U is not
 // fixedValue (the frame assigns its value, which only a fixedValue
 // condition keeps), 0 on success.
 int orc_set_mrf(void* h, int nZone, const int* zone, const double* origin, const double* axis, double omega,
@@ -2885,8 +2909,9 @@ int orc_set_mrf(void* h, int nZone, const int* zone, const double* origin, const
     FramePart P;
     if (framePart(k, zone, nZone, nonRot, nNonRot, [&](int f) { return cross(Om, v3(&m.Cf[3 * f]) - o); }, P))
         return 1;
-    k.frot = P;
-    k.omegaC.assign(3 * (size_t)m.nC, 0.0);
+    k.rotZones.push_back(P);
+    k.frot = rotCombined(k.rotZones);
+    if (k.omegaC.empty()) k.omegaC.assign(3 * (size_t)m.nC, 0.0);
     for (int i = 0; i < nZone; ++i) {
         const int c = zone[i];
         k.omegaC[3 * c] = Om.x; k.omegaC[3 * c + 1] = Om.y; k.omegaC[3 * c + 2] = Om.z;

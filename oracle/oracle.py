@@ -239,8 +239,9 @@ class OracleCase:
             if bc.p_init is not None or bc.U_init is not None or bc.T_init is not None:
                 init = [None if a is None else _f64(a) for a in (bc.p_init, bc.U_init, bc.T_init)]
                 lib().orc_set_bc_values(self.h, i, *[_p(a) for a in init])
-        if case.mrf is not None:
-            self.set_mrf(case.mrf)
+        for z in ([] if case.mrf is None else list(case.mrf) if isinstance(case.mrf, (list, tuple))
+                  else [case.mrf]):
+            self.set_mrf(z)
         if getattr(case, "translating", None) is not None:
             self.set_translating(case.translating)
         self.set_fields(case.rho, case.U, case.T)

@@ -198,6 +198,12 @@ def test_mrf_properties_and_cell_zones_read(tmp_path):
     assert t.non_moving == z.non_rotating
     bt, _ = OracleCase(r2).residual()
     assert np.abs(bt - b1).max() > 0
+    # two active rotating zones: the zone list in file order
+    (case_dir / "constant" / "MRFProperties").write_text(MRF_PROPS.replace("active      no;", "active yes;"))
+    r3, _ = foam.read_case(str(case_dir), thermo=c.thermo, fvSchemes=c.fvSchemes, fvSolution=c.fvSolution)
+    assert isinstance(r3.mrf, list) and len(r3.mrf) == 2
+    assert r3.mrf[0].omega == 314.159 and r3.mrf[1].omega == 1.0
+    np.testing.assert_array_equal(r3.mrf[1].axis, [1, 0, 0])
 
 
 def test_phase_lag_cyclic_patches_and_hb_properties_read(tmp_path):

@@ -3,8 +3,10 @@
 A jittered box with half of its cells rotating about an oblique axis
 (viscous Roe and AUSM+up second order, HLLC first order; slip walls that
 do not rotate, or no-slip walls that rotate with the zone), the same half
-translating (alone, and beside the other half rotating), and a rotational
-cyclic sector rotating about its own axis (Roe second order):
+translating (alone, and beside the other half rotating), the rotating half
+split into two zones of a zone list (other axis, origin and speed in the
+second), and a rotational cyclic sector rotating about its own axis (Roe
+second order):
 
 * face fluxes, residual (with the Coriolis source), rPseudoDeltaT,
   Jacobian blocks (diagonal with the frame divergence and the Coriolis
@@ -79,7 +81,22 @@ def _translating(both):
     return dataclasses.replace(c, mrf=rot, translating=t)
 
 
+def _two_zones():
+    """The box of _box with rotating walls, its zone split into two zones
+    (the zone list) that meet inside it: the second with another axis,
+    origin and speed."""
+    c = _box("ROE", rotating_walls=True)
+    z = c.mrf
+    C = OracleMesh(c.mesh).geometry()["C"]
+    half = C[:, 1] < np.median(C[:, 1])
+    z1 = dataclasses.replace(z, cells=z.cells & half)
+    z2 = dataclasses.replace(z, cells=z.cells & ~half, axis=np.array([-0.4, 0.1, 0.8]), omega=-250.0,
+                             origin=np.array([0.0, 0.2, 0.3]))
+    return dataclasses.replace(c, mrf=[z1, z2])
+
+
 CASES = {
+    "box_two_rotating_zones": _two_zones,
     "box_translating": lambda: _translating(False),
     "box_translating_and_rotating": lambda: _translating(True),
     "box_roe_viscous": lambda: _box("ROE", rotating_walls=False),

@@ -184,6 +184,47 @@ def test_zone_faces_and_frame_velocity():
         O.OracleCase(c2)
 
 
+def test_zone_list_assigns_in_order():
+    """MRFCoupledZoneList: each zone in list order assigns faceU on its
+    faces, Omega in its cells and U on its rotating patch faces.  Two zones
+    splitting the box's cells (they meet on the faces between the halves):
+    um is the second zone's value on its faces (the shared ones included)
+    and the first zone's elsewhere, Omega zone by zone; two zones with the
+    same Omega and origin give bit for bit the residual and the Jacobian of
+    the one zone of their union."""
+    c, g = _mrf_case(rotating_walls=True, upwind=False)
+    z = c.mrf
+    half = g["C"][:, 1] < np.median(g["C"][:, 1])
+    z1 = dataclasses.replace(z, cells=z.cells & half)
+    z2 = dataclasses.replace(z, cells=z.cells & ~half, axis=np.array([-0.4, 0.1, 0.8]), omega=-250.0,
+                             origin=np.array([0.0, 0.2, 0.3]))
+    s1 = O.OracleCase(dataclasses.replace(c, mrf=z1)).mrf()
+    s2 = O.OracleCase(dataclasses.replace(c, mrf=z2)).mrf()
+    s12 = O.OracleCase(dataclasses.replace(c, mrf=[z1, z2])).mrf()
+    m = c.mesh
+    nI = m.n_internal
+    in2 = np.zeros(m.n_faces, bool)
+    in2[:nI] = z2.cells[m.owner[:nI]] | z2.cells[m.neighbour]
+    in2[nI:] = z2.cells[m.owner[nI:]]
+    shared = np.zeros(m.n_faces, bool)
+    shared[:nI] = (z1.cells[m.owner[:nI]] & z2.cells[m.neighbour]) | (z2.cells[m.owner[:nI]] & z1.cells[m.neighbour])
+    assert shared.any() and (s1["umrf"][shared] != s2["umrf"][shared]).all()
+    np.testing.assert_array_equal(s12["umrf"], np.where(in2, s2["umrf"], s1["umrf"]))
+    np.testing.assert_array_equal(s12["omega"], np.where(z2.cells[:, None], s2["omega"], s1["omega"]))
+    np.testing.assert_array_equal(s12["rotating"], s1["rotating"] | s2["rotating"])
+    np.testing.assert_array_equal(s12["U"], np.where(s2["rotating"][:, None], s2["U"], s1["U"]))
+    # the same frame in two zones is the frame of their union
+    za, zb = dataclasses.replace(z, cells=z.cells & half), dataclasses.replace(z, cells=z.cells & ~half)
+    ou = O.OracleCase(c)
+    o2 = O.OracleCase(dataclasses.replace(c, mrf=[za, zb]))
+    bu, ru = ou.residual()
+    b2, r2 = o2.residual()
+    np.testing.assert_array_equal(b2, bu)
+    np.testing.assert_array_equal(r2, ru)
+    for a, b in zip(o2.createJacobian(), ou.createJacobian()):
+        np.testing.assert_array_equal(a, b)
+
+
 def _flux_cplx(S, W):
     rho, rE, rU = W[0], W[1], W[2:]
     U = rU / rho
