@@ -1,7 +1,7 @@
 """torch binding of the residual adjoint.
 
-`ResidualFunction` makes the residual b(W) of a `dbnsCase` (AUSM+up or
-Roe, inviscid or viscous) an autograd node: the forward pass calls
+`ResidualFunction` makes the residual b(W) of a `dbnsCase` (AUSM+up, Roe
+or HLLC, inviscid or viscous) an autograd node: the forward pass calls
 `icsf_residual_device`, the backward pass `icsf_residual_vjp_device`
 (hand-written reverse kernels, reconstruction recomputed), both on device
 tensors owned by torch.  The reference has no adjoint; the differentiated

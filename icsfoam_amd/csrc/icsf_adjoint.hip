@@ -4,7 +4,8 @@
 // volumes and interpolation weights held fixed.  The reference has no
 // adjoint; the forward chain differentiated here is the one of
 // updateFields.H:7-104 (W -> rho, U, e, T, p and the boundary values),
-// ausmPlusUpFluxScheme.C:73-299 / roeFluxScheme.C:41-241 (E, H, c*,
+// ausmPlusUpFluxScheme.C:73-299 / roeFluxScheme.C:41-241 /
+// hllcFluxScheme.C:76-221 (E, H, c* or the sound speed,
 // reconstruction with cellLimited Gauss gradients, the face flux) and
 // residualsUpdate.H:1-43 (surfaceIntegrate, laplacian(muEff, U),
 // div(tauMC), div(sigmaDotU), laplacian(alphaEff, e)).
@@ -13,7 +14,8 @@
 // boundary and gradient kernels (keeping the unlimited gradients and the
 // limiter values), then the reverse kernels pull adjoints without atomics:
 //   k_adj_face   one thread per face: AUSM+up or Roe reverse (each one
-//                forward evaluation and one backward sweep), adjoint of
+//                forward evaluation and one backward sweep) or HLLC (one
+//                dual-number pass per face value), adjoint of
 //                the face values (owner/neighbour side) and of Sf, Cf
 //                (flux part)
 //   k_adj_visc   one thread per face (viscous cases): adjoint of the
@@ -40,8 +42,109 @@ namespace {
 #define DF __device__ __forceinline__
 
 // Reconstructed fields the scheme uses: AUSM+up takes c* and not E, Roe
-// takes E and not c*.
-DF bool used(int flux, int k) { return flux == FLUX_ROE ? k != F_CS : k != F_E; }
+// takes E and not c*, HLLC all of them (its F_CS is max(sqrt(gamma/psi),
+// VSMALL)).
+DF bool used(int flux, int k) {
+    return flux == FLUX_ROE ? k != F_CS : (flux == FLUX_HLLC ? true : k != F_E);
+}
+
+// ---- HLLC: forward-mode passes of the face flux on dual numbers (value
+// and one tangent), one per input, contracted with the flux adjoint
+struct Dn {
+    double v, d;
+};
+DF Dn operator+(Dn a, Dn b) { return Dn{a.v + b.v, a.d + b.d}; }
+DF Dn operator-(Dn a, Dn b) { return Dn{a.v - b.v, a.d - b.d}; }
+DF Dn operator*(Dn a, Dn b) { return Dn{a.v * b.v, a.d * b.v + a.v * b.d}; }
+DF Dn operator/(Dn a, Dn b) {
+    const double q = a.v / b.v;
+    return Dn{q, (a.d - q * b.d) / b.v};
+}
+DF Dn dsqrt(Dn a) {
+    const double r = sqrt(a.v);
+    return Dn{r, r > 0.0 ? a.d / (2.0 * r) : 0.0};
+}
+DF Dn dfabs(Dn a) { return a.v < 0.0 ? Dn{-a.v, -a.d} : a; }
+DF Dn dmin(Dn a, Dn b) { return b.v < a.v ? b : a; }  // std::min(a, b)
+DF Dn dmax(Dn a, Dn b) { return a.v < b.v ? b : a; }  // std::max(a, b)
+DF Dn cst(double x) { return Dn{x, 0.0}; }
+struct V3n {
+    Dn x, y, z;
+};
+DF V3n operator+(V3n a, V3n b) { return V3n{a.x + b.x, a.y + b.y, a.z + b.z}; }
+DF V3n operator*(Dn s, V3n a) { return V3n{s * a.x, s * a.y, s * a.z}; }
+DF V3n operator*(V3n a, Dn s) { return V3n{a.x * s, a.y * s, a.z * s}; }
+DF V3n operator/(V3n a, Dn s) { return V3n{a.x / s, a.y / s, a.z / s}; }
+DF Dn dotn(V3n a, V3n b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// hllcFace() (icsf_flux.h, hllcFluxScheme.C:76-221) on dual numbers, no
+// frame velocity; br: the switches (Sl, Sr choices, the signs of Sl, Sr,
+// Sm, the fabs argument).
+DF void hllcDual(double gIv, V3n S, Dn magS, Dn rl, Dn rr, Dn pl, Dn pr, V3n Ul, V3n Ur, Dn cl, Dn cr, Dn El,
+                 Dn Er, Dn Hl, Dn Hr, Dn& phi, V3n& phiU, Dn& phiE, int* br) {
+    const Dn gI = cst(gIv), one = cst(1.0);
+    const V3n n = S / magS;
+    const Dn coefR = dsqrt(dmax(cst(kVSMALL), rr) / dmax(cst(kVSMALL), rl));
+    const V3n uAvg = (coefR * Ur + Ul) / (coefR + one);
+    const Dn HAvg = (coefR * Hr + Hl) / (coefR + one);
+    const Dn arg = (gI - one) * (HAvg - cst(0.5) * dotn(uAvg, uAvg));
+    const Dn cAvg = dsqrt(dfabs(arg));
+    const Dn ul = dotn(Ul, n), ur = dotn(Ur, n), ua = dotn(uAvg, n);
+    const Dn Sl = dmin(ul - cl, ua - cAvg);
+    const Dn Sr = dmax(ur + cr, ua + cAvg);
+    const Dn Sm = (rr * ur * (Sr - ur) - rl * ul * (Sl - ul) + pl - pr) / (rr * (Sr - ur) - rl * (Sl - ul));
+    const Dn cSl = cst(Sl.v > 0 ? 1.0 : 0.0), cSr = cst(Sr.v < 0 ? 1.0 : 0.0), cSm = cst(Sm.v > 0 ? 1.0 : 0.0);
+    if (br)
+        *br = (int)((ua - cAvg).v < (ul - cl).v) | (int)((ur + cr).v < (ua + cAvg).v) << 1 | (int)(Sl.v > 0) << 2 |
+              (int)(Sr.v < 0) << 3 | (int)(Sm.v > 0) << 4 | (int)(arg.v < 0) << 5;
+    const Dn cSlm = (one - cSl) * cSm, cSmr = (one - cSm) * (one - cSr);
+    const Dn frl = Sm / (Sl - Sm) * ((Sl - ul) * rl);
+    const Dn frr = Sm / (Sr - Sm) * ((Sr - ur) * rr);
+    phi = (cSl * rl * ul + cSlm * frl + cSmr * frr + cSr * rr * ur) * magS;
+    const Dn psl = rl * (ul - Sl) * (ul - Sm) + pl;
+    const Dn psr = rr * (ur - Sr) * (ur - Sm) + pr;
+    const V3n rUsl = one / (Sl - Sm) * ((Sl - ul) * rl * Ul + (psl - pl) * n);
+    const V3n rUsr = one / (Sr - Sm) * ((Sr - ur) * rr * Ur + (psr - pr) * n);
+    const V3n fUl = Sm * rUsl + psl * n;
+    const V3n fUr = Sm * rUsr + psr * n;
+    phiU = (cSl * (rl * ul * Ul + pl * n) + cSlm * fUl + cSmr * fUr + cSr * (rr * ur * Ur + pr * n)) * magS;
+    const Dn rEsl = one / (Sl - Sm) * ((Sl - ul) * (rl * El) - pl * ul + psl * Sm);
+    const Dn rEsr = one / (Sr - Sm) * ((Sr - ur) * (rr * Er) - pr * ur + psr * Sm);
+    const Dn fEl = Sm * (rEsl + psl);
+    const Dn fEr = Sm * (rEsr + psr);
+    phiE = (cSl * rl * Hl * ul + cSlm * fEl + cSmr * fEr + cSr * rr * Hr * ur) * magS;
+}
+
+// Reverse of the HLLC face flux: one dual pass per face value (owner side
+// L, neighbour side R, fields F_RHO..F_CS) and, with geom, per Sf
+// component; each input's adjoint is the flux adjoint contracted with
+// that pass's tangent.
+DF void hllcFaceRev(double gI, V3 S, const double* L, const double* R, double bphi, V3 bphiU, double bphiE,
+                    double* bL, double* bR, V3& bS, int* branches, bool geom) {
+    const int nIn = geom ? 2 * NFLD + 3 : 2 * NFLD;
+    double bs[3] = {0, 0, 0};
+    for (int i = 0; i < nIn; ++i) {
+        Dn l[NFLD], r[NFLD];
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k) {
+            l[k] = Dn{L[k], i == k ? 1.0 : 0.0};
+            r[k] = Dn{R[k], i == NFLD + k ? 1.0 : 0.0};
+        }
+        const V3n Sd{Dn{S.x, i == 2 * NFLD ? 1.0 : 0.0}, Dn{S.y, i == 2 * NFLD + 1 ? 1.0 : 0.0},
+                     Dn{S.z, i == 2 * NFLD + 2 ? 1.0 : 0.0}};
+        const Dn magS = dsqrt(dotn(Sd, Sd));
+        Dn ph, pe;
+        V3n pu;
+        hllcDual(gI, Sd, magS, l[F_RHO], r[F_RHO], l[F_P], r[F_P], V3n{l[F_UX], l[F_UY], l[F_UZ]},
+                 V3n{r[F_UX], r[F_UY], r[F_UZ]}, l[F_CS], r[F_CS], l[F_E], r[F_E], l[F_H], r[F_H], ph, pu, pe,
+                 i == 0 ? branches : nullptr);
+        const double t = bphi * ph.d + (bphiU.x * pu.x.d + bphiU.y * pu.y.d + bphiU.z * pu.z.d) + bphiE * pe.d;
+        if (i < NFLD) bL[i] = t;
+        else if (i < 2 * NFLD) bR[i - NFLD] = t;
+        else bs[i - 2 * NFLD] = t;
+    }
+    bS = V3{bs[0], bs[1], bs[2]};
+}
 
 // Reverse of ausmFace(): adjoints of the owner-side (L) and neighbour-side
 // (R) face values, indexed by field, and of Sf.  Recomputes the forward
@@ -188,7 +291,7 @@ DF void ausmFaceAdj(bool lowMach, V3 S, double magS, const double* L, const doub
 
 // Face pass: adjoint of the face flux from v (b_owner gets -F, b_neighbour
 // +F), AUSM+up reverse, reconstruction reverse into Cf.
-template <bool ROE>
+template <int SCH>
 __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView s, AdjView a,
                                                   const double* __restrict__ v, double* __restrict__ gS,
                                                   double* __restrict__ gC, bool geom) {
@@ -223,7 +326,13 @@ __global__ void __launch_bounds__(256) k_adj_face(MeshView m, Phys ph, StateView
     const V3 S = ld3(m.Sf + 3 * (size_t)f);
     double bL[NFLD], bR[NFLD];
     V3 bS;
-    if constexpr (ROE) {
+    if constexpr (SCH == FLUX_HLLC) {
+        hllcFaceRev(ph.gamma, S, L, R, bf[0], V3{bf[2], bf[3], bf[4]}, bf[1], bL, bR, bS,
+                    a.fbr ? a.fbr + f : nullptr, geom);
+        // on a boundary face both sides are the boundary state, where every
+        // branch gives the physical flux: no switch of the derivative
+        if (a.fbr && !internal) a.fbr[f] = 0;
+    } else if constexpr (SCH == FLUX_ROE) {
         roeFaceRev(ph.gamma, ph.entropyFix, S, m.magSf[f], L, R, bf[0], V3{bf[2], bf[3], bf[4]}, bf[1], bL, bR, bS,
                    a.fbr ? a.fbr + f : nullptr, geom);
         // on a boundary face both sides are the boundary state: the
@@ -660,7 +769,8 @@ __global__ void __launch_bounds__(256) k_adj_gauss(MeshView m, Phys ph, StateVie
         const double g = ph.gamma;
         const double kap = 2.0 * (g - 1.0) / (g + 1.0);
         double Hb = qb[F_H];
-        if (bv[F_CS] > 0.0) Hb += qb[F_CS] * kap / (2.0 * bv[F_CS]);
+        const bool hllc = ph.flux == FLUX_HLLC;
+        if (!hllc && bv[F_CS] > 0.0) Hb += qb[F_CS] * kap / (2.0 * bv[F_CS]);
         const double TB = s.bT[b];
         const double pB = bv[F_P], rhoB = bv[F_RHO];
         const V3 UB{bv[F_UX], bv[F_UY], bv[F_UZ]};
@@ -671,6 +781,8 @@ __global__ void __launch_bounds__(256) k_adj_gauss(MeshView m, Phys ph, StateVie
         const double pr = pB / rhoB;
         if (!(pr < kSMALL)) { pb += Hb / rhoB; rb -= Hb * pr / rhoB; }
         double Tb = ph.Cv * Eb;
+        // HLLC: c_b = max(sqrt(gamma/psi_b), VSMALL), sqrt(gamma R T_b)
+        if (hllc && !(bv[F_CS] < kVSMALL)) Tb += qb[F_CS] * g * ph.R / (2.0 * bv[F_CS]);
         Ub = Ub + Eb * UB;
         if (visc) {
             // the viscous terms' boundary U, and the boundary e of grad(e)
@@ -774,13 +886,16 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
     const double g = ph.gamma;
     const double kap = 2.0 * (g - 1.0) / (g + 1.0);
     double Hb = q[F_H];
-    if (cs > 0.0) Hb += q[F_CS] * kap / (2.0 * cs);
+    const bool hllc = ph.flux == FLUX_HLLC;
+    if (!hllc && cs > 0.0) Hb += q[F_CS] * kap / (2.0 * cs);
     double rb = q[F_RHO], pb = q[F_P] + pbnd, Eb = q[F_E];
     V3 Ub = V3{q[F_UX], q[F_UY], q[F_UZ]} + Ubnd + Uv;
     if (!(E < kSMALL)) Eb += Hb;
     const double pr = p / rho;
     if (!(pr < kSMALL)) { pb += Hb / rho; rb -= Hb * pr / rho; }
     double Tb = ph.Cv * Eb + Tbnd;
+    // HLLC: c = max(sqrt(gamma/psi), VSMALL), sqrt(gamma R T)
+    if (hllc && !(cs < kVSMALL)) Tb += q[F_CS] * g * ph.R / (2.0 * cs);
     Ub = Ub + Eb * U;
     // p = rho/psi = rho R T
     const double psi = 1.0 / (ph.R * T);
@@ -817,8 +932,9 @@ __global__ void __launch_bounds__(256) k_adj_state(MeshView m, Phys ph, StateVie
 void launchResidualVjp(const MeshView& m, const Phys& ph, const StateView& s, const AdjView& a, const double* v,
                        double* gW, double* gS, double* gC, bool geom, hipStream_t st) {
     const dim3 gf(xcdGrid(m.nF, 256)), gc(xcdGrid(m.nC, 256));
-    if (ph.flux == FLUX_ROE) k_adj_face<true><<<gf, 256, 0, st>>>(m, ph, s, a, v, gS, gC, geom);
-    else k_adj_face<false><<<gf, 256, 0, st>>>(m, ph, s, a, v, gS, gC, geom);
+    if (ph.flux == FLUX_ROE) k_adj_face<FLUX_ROE><<<gf, 256, 0, st>>>(m, ph, s, a, v, gS, gC, geom);
+    else if (ph.flux == FLUX_HLLC) k_adj_face<FLUX_HLLC><<<gf, 256, 0, st>>>(m, ph, s, a, v, gS, gC, geom);
+    else k_adj_face<FLUX_AUSM><<<gf, 256, 0, st>>>(m, ph, s, a, v, gS, gC, geom);
     if (ph.viscous) {
         k_adj_visc<<<gf, 256, 0, st>>>(m, ph, s, a, v);
         k_adj_cell<true><<<gc, 256, 0, st>>>(m, ph, s, a);

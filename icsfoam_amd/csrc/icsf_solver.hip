@@ -915,8 +915,6 @@ int adjointSupported(const icsf_case* c) {
         if (c->bPT[b] == BC_TOTAL_P || c->bUT[b] == BC_PIOV || c->bTT[b] == BC_TOTAL_T)
             return fail(ICSF_ERR_ARG, "residual VJP: totalPressure, pressureInletOutletVelocity and totalTemperature "
                                       "conditions are not supported");
-    if (c->ph.flux == FLUX_HLLC)
-        return fail(ICSF_ERR_ARG, "residual VJP: implemented for the AUSMPlusUp and Roe flux schemes");
     if (c->mv.nCyc > 0) return fail(ICSF_ERR_ARG, "residual VJP: meshes with cyclic patches are not supported");
     return ICSF_OK;
 }

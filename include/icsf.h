@@ -225,18 +225,20 @@ int icsf_calc_flux(icsf_case* c, double* phi, double* phiUp, double* phiEp);
 int icsf_residual(icsf_case* c, double* b, double* rPseudoDeltaT);
 
 /* ---- residual adjoint (no counterpart in the reference) --------------
- * Vector-Jacobian product of the residual b(W, Sf, Cf) with the AUSM+up or
- * Roe scheme, inviscid or viscous (the forward chain of
- * updateFields.H:7-104, ausmPlusUpFluxScheme.C:73-299 or
- * roeFluxScheme.C:41-241,276-409, and residualsUpdate.H:1-43), cell
+ * Vector-Jacobian product of the residual b(W, Sf, Cf) with the AUSM+up,
+ * Roe or HLLC scheme, inviscid or viscous (the forward chain of
+ * updateFields.H:7-104, ausmPlusUpFluxScheme.C:73-299,
+ * roeFluxScheme.C:41-241,276-409 or hllcFluxScheme.C:76-221, and
+ * residualsUpdate.H:1-43), cell
  * centres, volumes and weights fixed:
  *   gW  = (db/dW)^T  v   [5][nCells]
  *   gSf = (db/dSf)^T v   [nFaces][3]   (NULL: not returned)
  *   gCf = (db/dCf)^T v   [nFaces][3]   (NULL: not returned)
  * W ([5][nCells], conserved) becomes the case state through updateFields
  * with a zero increment; W NULL keeps the current state.  gSf and gCf are
- * built for inviscid residuals (viscous cases must pass NULL); HLLC, cyclic
- * meshes and the derived inflow conditions return ICSF_ERR_ARG.  Host
+ * built for inviscid residuals (viscous cases must pass NULL); cyclic
+ * meshes, frame zones and the derived inflow conditions return
+ * ICSF_ERR_ARG.  Host
  * pointers; returns when the results are written. */
 int icsf_residual_vjp(icsf_case* c, const double* W, const double* v, double* gW, double* gSf, double* gCf);
 /* Same with device pointers (for framework bindings); synchronous. */

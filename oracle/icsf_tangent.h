@@ -25,6 +25,8 @@
 //   AUSM+up face flux               ausmPlusUpFluxScheme.C:112-299
 //   Roe face flux (E, H, rho, p, U reconstructed; P|Lambda|P^-1 with the
 //   entropy fix)                    roeFluxScheme.C:41-241,276-409
+//   HLLC face flux (E, H, rho, p, U and max(sqrt(gamma/psi), VSMALL)
+//   reconstructed)                  hllcFluxScheme.C:76-221
 //   surfaceIntegrate                residualsUpdate.H:1-14
 //   viscous terms (laplacian(muEff, U), div(tauMC), div(sigmaDotU),
 //   laplacian(alphaEff, e) with Gauss linear grad(U), grad(e) and
@@ -436,6 +438,40 @@ void roeFaceT(double gI, double eFix, V3T<T> S, T magS, T rl, T rr, T pl, T pr, 
     phiE = phiE + T(0.5) * magS * (rnl * Hl + rnr * Hr);
 }
 
+// HLLC face flux (hllcFluxScheme.C:76-221), same expression order as
+// hllcFace() in icsf_oracle.cpp (no frame velocity).
+template <class T>
+void hllcFaceT(double gIv, V3T<T> S, T magS, T rl, T rr, T pl, T pr, V3T<T> Ul, V3T<T> Ur, T cl, T cr, T El,
+               T Er, T Hl, T Hr, T& phi, V3T<T>& phiU, T& phiE) {
+    const T gI(gIv), one(1.0);
+    const V3T<T> n = S / magS;
+    const T coefR = sqrtT(tmax(T(VSMALL), rr) / tmax(T(VSMALL), rl));
+    const V3T<T> uAvg = (coefR * Ur + Ul) / (coefR + one);
+    const T HAvg = (coefR * Hr + Hl) / (coefR + one);
+    const T cAvg = sqrtT(fabsT((gI - one) * (HAvg - T(0.5) * magSqrT(uAvg))));
+    const T ul = dotT(Ul, n), ur = dotT(Ur, n), ua = dotT(uAvg, n);
+    const T Sl = tmin(ul - cl, ua - cAvg);
+    const T Sr = tmax(ur + cr, ua + cAvg);
+    const T Sm = (rr * ur * (Sr - ur) - rl * ul * (Sl - ul) + pl - pr) / (rr * (Sr - ur) - rl * (Sl - ul));
+    const T cSl(val(Sl) > 0 ? 1.0 : 0.0), cSr(val(Sr) < 0 ? 1.0 : 0.0), cSm(val(Sm) > 0 ? 1.0 : 0.0);
+    const T cSlm = (one - cSl) * cSm, cSmr = (one - cSm) * (one - cSr);
+    const T frl = Sm / (Sl - Sm) * ((Sl - ul) * rl);
+    const T frr = Sm / (Sr - Sm) * ((Sr - ur) * rr);
+    phi = (cSl * rl * ul + cSlm * frl + cSmr * frr + cSr * rr * ur) * magS;
+    const T psl = rl * (ul - Sl) * (ul - Sm) + pl;
+    const T psr = rr * (ur - Sr) * (ur - Sm) + pr;
+    const V3T<T> rUsl = one / (Sl - Sm) * ((Sl - ul) * rl * Ul + (psl - pl) * n);
+    const V3T<T> rUsr = one / (Sr - Sm) * ((Sr - ur) * rr * Ur + (psr - pr) * n);
+    const V3T<T> fUl = Sm * rUsl + psl * n;
+    const V3T<T> fUr = Sm * rUsr + psr * n;
+    phiU = (cSl * (rl * ul * Ul + pl * n) + cSlm * fUl + cSmr * fUr + cSr * (rr * ur * Ur + pr * n)) * magS;
+    const T rEsl = one / (Sl - Sm) * ((Sl - ul) * (rl * El) - pl * ul + psl * Sm);
+    const T rEsr = one / (Sr - Sm) * ((Sr - ur) * (rr * Er) - pr * ur + psr * Sm);
+    const T fEl = Sm * (rEsl + psl);
+    const T fEr = Sm * (rEsr + psr);
+    phiE = (cSl * rl * Hl * ul + cSlm * fEl + cSmr * fEr + cSr * rr * Hr * ur) * magS;
+}
+
 // Viscous terms of residualsUpdate.H:16-43 as the oracle's
 // viscousResidual() forms them: per face |S| mu snGrad(U), Sf & tauMC,
 // (mu grad(U) + tauMC) & U & Sf and |S| alpha snGrad(e), each summed per
@@ -590,16 +626,31 @@ void residualT(const Case& k, const T* W, const T* Sf, const T* Cf, T* b) {
     reconstructT(k, s.p.data(), s.pB.data(), Sf, Cf, pL, pR);
     reconstructT(k, s.rho.data(), s.rhoB.data(), Sf, Cf, rL, rR);
     reconstructT(k, H.data(), Hb.data(), Sf, Cf, HL, HR);
-    const bool roe = k.s.flux == FLUX_ROE;
+    const bool roe = k.s.flux == FLUX_ROE, hllc = k.s.flux == FLUX_HLLC;
     std::vector<T> E(nC), Eb(nB, T(0)), EL, ER;
-    if (roe) {
+    if (roe || hllc) {
         for (int c = 0; c < nC; ++c)
             E[c] = T(k.Cv) * s.Tt[c] + T(0.5) * magSqrT(V3T<T>{s.U[3 * c], s.U[3 * c + 1], s.U[3 * c + 2]});
         for (int bb = 0; bb < nB; ++bb)
             if (val(s.TB[bb]) != 0)
                 Eb[bb] = T(k.Cv) * s.TB[bb] + T(0.5) * magSqrT(V3T<T>{s.UB[3 * bb], s.UB[3 * bb + 1], s.UB[3 * bb + 2]});
         reconstructT(k, E.data(), Eb.data(), Sf, Cf, EL, ER);
-    } else reconstructT(k, cs.data(), csb.data(), Sf, Cf, cL, cR);
+    }
+    if (hllc) {
+        // HLLC's speed: max(sqrt(gamma/psi), VSMALL), psi = 1/(R T)
+        std::vector<T> ch(nC), chb(nB, T(0));
+        for (int c = 0; c < nC; ++c) {
+            const T psi = T(1.0) / (T(k.s.R) * s.Tt[c]);
+            ch[c] = tmax(sqrtT(T(g) / psi), T(VSMALL));
+        }
+        for (int bb = 0; bb < nB; ++bb)
+            if (val(s.TB[bb]) != 0) {
+                const T psi = T(1.0) / (T(k.s.R) * s.TB[bb]);
+                chb[bb] = tmax(sqrtT(T(g) / psi), T(VSMALL));
+            }
+        reconstructT(k, ch.data(), chb.data(), Sf, Cf, cL, cR);
+    } else if (!roe)
+        reconstructT(k, cs.data(), csb.data(), Sf, Cf, cL, cR);
     std::vector<T> acc(5 * (size_t)nC, T(0));
     auto add = [&](int c, T sgn, T ph, V3T<T> pu, T pe) {
         acc[c] = acc[c] + sgn * ph;
@@ -618,6 +669,9 @@ void residualT(const Case& k, const T* W, const T* Sf, const T* Cf, T* b) {
         if (roe)
             roeFaceT(k.s.gamma, k.s.entropyFix, S, magS, rL[f], rR[f], pL[f], pR[f], UL, UR, EL[f], ER[f], HL[f],
                      HR[f], PH[f], PU[f], PE[f]);
+        else if (hllc)
+            hllcFaceT(k.s.gamma, S, magS, rL[f], rR[f], pL[f], pR[f], UL, UR, cL[f], cR[f], EL[f], ER[f], HL[f],
+                      HR[f], PH[f], PU[f], PE[f]);
         else
             ausmFaceT(k.s.lowMach, S, magS, UL, UR, cL[f], cR[f], pL[f], pR[f], rL[f], rR[f], HL[f], HR[f], PH[f],
                       PU[f], PE[f]);

@@ -49,6 +49,21 @@ def roe(c, mu=None, limiter=None, recon=None):
     return c
 
 
+def hllc(c, mu=None, limiter=None, recon=None):
+    """The same case with the HLLC scheme (viscosity mu; None: the case's
+    own), optionally another limiter / reconstruction."""
+    import dataclasses
+    kw = {}
+    if limiter is not None:
+        kw["limiter"] = limiter
+    if recon is not None:
+        kw["recon"] = recon
+    c = dataclasses.replace(c, fvSchemes=cases.schemes_dict("HLLC", **kw))
+    if mu is not None:
+        c = dataclasses.replace(c, thermo=cases.thermo_dict(mu))
+    return c
+
+
 def with_mu(c, mu):
     """The same case with viscosity mu (0: inviscid)."""
     import dataclasses

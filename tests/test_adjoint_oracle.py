@@ -1,5 +1,6 @@
 """Pins of the oracle's tangent-linear (dual-number) residual, the checker of
-the residual adjoint (AUSM+up and Roe, inviscid and viscous).  CPU only.
+the residual adjoint (AUSM+up, Roe and HLLC, inviscid and viscous).  CPU
+only.
 
 * its FP64 primal is bit-identical to the oracle residual of the reference
   path (same state obtained from W through updateFields);
@@ -15,7 +16,7 @@ import pytest
 
 from oracle.oracle import OracleCase
 
-from helpers import ausm_inviscid, box_small, bump_small, generic_state, roe, smooth_field, with_mu
+from helpers import ausm_inviscid, box_small, bump_small, generic_state, hllc, roe, smooth_field, with_mu
 
 VENK = "cellLimited<Venkatakrishnan> Gauss linear 1"
 MINMOD = "cellLimited<minmod> Gauss linear 1"
@@ -33,6 +34,9 @@ CASES = {
     "box_roe_viscous": lambda: roe(box_small(), mu=20.0),
     "bump_roe_viscous": lambda: roe(bump_small(), mu=0.5),
     "box_ausm_viscous": lambda: with_mu(ausm_inviscid(box_small()), 20.0),
+    "box_hllc_venk": lambda: hllc(box_small(), mu=0.0),
+    "box_hllc_minmod": lambda: hllc(box_small(), mu=0.0, limiter=MINMOD),
+    "box_hllc_viscous": lambda: hllc(box_small(), mu=20.0),
 }
 
 

@@ -25,7 +25,7 @@ from icsfoam_amd import cases, dbnsCase
 from icsfoam_amd.mesh import prepare_case
 from oracle.oracle import OracleCase
 
-from helpers import ausm_inviscid, box_small, bump_small, generic_state, roe, smooth_field, with_mu
+from helpers import ausm_inviscid, box_small, bump_small, generic_state, hllc, roe, smooth_field, with_mu
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +47,9 @@ TINY = {
     "box_roe_viscous_upwind": lambda: roe(box_small(n=(5, 4, 3)), mu=20.0, recon="upwind"),
     "bump_roe_viscous": lambda: roe(bump_small(nx=12, ny=4), mu=0.5),
     "box_ausm_viscous": lambda: with_mu(ausm_inviscid(box_small(n=(5, 4, 3))), 20.0),
+    "box_hllc_venk": lambda: hllc(box_small(n=(5, 4, 3)), mu=0.0),
+    "bump_hllc_minmod": lambda: hllc(bump_small(nx=12, ny=4), mu=0.0, limiter=MINMOD),
+    "box_hllc_viscous": lambda: hllc(box_small(n=(5, 4, 3)), mu=20.0),
 }
 SMALL = {
     "bump_venk": lambda: ausm_inviscid(bump_small()),
@@ -55,6 +58,8 @@ SMALL = {
     "box_roe_venk": lambda: roe(box_small(), mu=0.0),
     "box_roe_viscous": lambda: roe(box_small(), mu=20.0),
     "bump_roe_viscous": lambda: roe(bump_small(), mu=0.5),
+    "box_hllc_venk": lambda: hllc(box_small(), mu=0.0),
+    "box_hllc_viscous": lambda: hllc(box_small(), mu=20.0),
 }
 
 
