@@ -249,7 +249,8 @@ def cpu_baseline(n_sample: int, n_full: int, steps: int = 5, warmup: int = 2, ra
     one single-threaded rank per core, each owning its share of the cells
     (dbnsFoam under mpirun, per-rank LU-SGS).  One rank alone gives the
     1-core figure; `ranks` ranks run concurrently (default: the host's
-    cores, at most 16, the GPU box's CPU share) for the all-core figure.
+    cores, at most 16, the CPU share of a one-GPU host) for the all-core
+    figure.
     Each rank runs the n_sample^3 box of the same workload (a bounded sample
     of the 160^3 case's per-rank share), `warmup` untimed steps, then the
     median of `steps` timed steps; steps/s of the full case = aggregate
