@@ -352,18 +352,32 @@ def bench_bump(steps=50):
                              "sample": f"oracle (fast build) on the same case, the same {steps} steps"}}
 
 
-def bench_hb(steps, warmup):
+def bench_hb(steps, warmup, frame="none"):
     """Harmonic-balance pseudo-steps/s on a 64x64x32 annular sector with
     rotational cyclic theta patches and 7 instants (3 harmonics of 100 Hz),
     Roe inviscid, LU-SGS.  value: the mixed form (outer FGMRES(1) on the
     FP64 coupled vectors, FP32 HB Jacobian and FP32 inner GMRES(30) with CGS,
-    hipGraph-captured steps); the FP64 form (GMRES(30), CGS) beside it."""
+    hipGraph-captured steps); the FP64 form (GMRES(30), CGS) beside it.
+    frame: "phase-lag" makes the cyclic pair phase-lagged (inter-blade
+    phase angle 2 pi/9, cylCoords); "mrf" lets the whole sector rotate about
+    its axis at 300 rad/s (every physical patch non-rotating; full-block
+    Jacobian storage)."""
     from icsfoam_amd import cases
     from icsfoam_amd.mesh import prepare_case
     from icsfoam_amd.solver import hbCase
 
     t0 = time.time()
     insts, D, t = cases.case_hb(n=(64, 64, 32), n_harmonics=3, flux="ROE", orthogonalisation="CGS")
+    m = insts[0].mesh
+    if frame == "phase-lag":
+        lo = next(i for i, p in enumerate(m.patches) if p.kind == cases.PATCH_CYCLIC)
+        pair = cases.PhaseLagPair(owner=lo, ibpa=2 * np.pi / 9, frequencies=np.array([2 * np.pi * 100.0]),
+                                  harmonics=np.array([3]), instants=t, cyl_coords=True)
+        insts = [dataclasses.replace(c, phase_lag=pair) for c in insts]
+    elif frame == "mrf":
+        z = cases.MRFZone(cells=np.ones(m.n_cells, bool), origin=np.zeros(3), axis=np.array([0.0, 0.0, 1.0]),
+                          omega=300.0, non_rotating=[i for i, p in enumerate(m.patches) if p.kind != cases.PATCH_CYCLIC])
+        insts = [dataclasses.replace(c, mrf=z) for c in insts]
     insts = [prepare_case(c, permute_seed=2024) for c in insts]
     log(f"[bench] HB case (7 instants, {insts[0].mesh.n_cells} cells) prepared in {time.time() - t0:.1f}s")
     forms = {"fp64": cases.solution_dict(co=20.0, orthogonalisation="CGS"),
@@ -383,13 +397,15 @@ def bench_hb(steps, warmup):
         log(f"[bench] HB {form}: {steps} steps in {dt:.3f}s; instant-0 residuals {res[0, :5]}")
         del g
     nC = insts[0].mesh.n_cells
+    fdesc = {"none": "", "phase-lag": ", phase-lagged cyclic pair (IBPA 2 pi/9, cylCoords)",
+             "mrf": ", whole sector rotating at 300 rad/s (MRF, full-block Jacobian)"}[frame]
     return {"metric": "harmonic-balance pseudo-steps/s (7 instants)", "value": round(rate["mixed"], 4),
             "unit": "pseudo-steps/s", "n_gpus": 1, "steps": steps, "warmup": max(warmup, 3),
             "ms_per_step": round(1e3 / rate["mixed"], 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64 residual/outer + f32 Jacobian/inner",
             "data": "synthetic (seeded annular sector, periodic inflow)",
             "config": {"workload": f"64x64x32 annular sector with rotational cyclic patches ({nC} cells) x 7 "
-                                   f"instants ({7 * nC} block rows), Roe inviscid, LU-SGS, hipGraph steps",
+                                   f"instants ({7 * nC} block rows), Roe inviscid, LU-SGS, hipGraph steps{fdesc}",
                        "mixed": "FGMRES(1) FP64 outer + FP32 GMRES(30, CGS) inner",
                        "fp64": "GMRES(30, CGS)", "fp64_steps_per_s": round(rate["fp64"], 4),
                        "mixed_over_fp64": round(rate["mixed"] / rate["fp64"], 3)}}
@@ -599,6 +615,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed", help="converge: forms to run, in order")
+    ap.add_argument("--frame", default="none", choices=["none", "phase-lag", "mrf"],
+                    help="hb: a phase-lagged cyclic pair or a rotating sector")
     ap.add_argument("--time-limit", type=float, default=400.0, help="converge: seconds per form")
     ap.add_argument("--workload", default="c5",
                     choices=["c5", "hb", "vjp", "c5vjp", "converge", "bump", "hbconverge"],
@@ -612,7 +630,7 @@ def main():
               "supported (multi-GPU sharding is designed, not built)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "hb":
-        print(json.dumps(bench_hb(args.steps, args.warmup)), flush=True)
+        print(json.dumps(bench_hb(args.steps, args.warmup, args.frame)), flush=True)
         return
     if args.workload == "hbconverge":
         print(json.dumps(bench_hb_converge(max(args.steps, 25), time_limit=args.time_limit)), flush=True)
