@@ -360,8 +360,7 @@ def bench_hb(steps, warmup, frame="none"):
     hipGraph-captured steps); the FP64 form (GMRES(30), CGS) beside it.
     frame: "phase-lag" makes the cyclic pair phase-lagged (inter-blade
     phase angle 2 pi/9, cylCoords); "mrf" lets the whole sector rotate about
-    its axis at 300 rad/s (every physical patch non-rotating; full-block
-    Jacobian storage)."""
+    its axis at 300 rad/s (every physical patch non-rotating)."""
     from icsfoam_amd import cases
     from icsfoam_amd.mesh import prepare_case
     from icsfoam_amd.solver import hbCase
@@ -398,7 +397,7 @@ def bench_hb(steps, warmup, frame="none"):
         del g
     nC = insts[0].mesh.n_cells
     fdesc = {"none": "", "phase-lag": ", phase-lagged cyclic pair (IBPA 2 pi/9, cylCoords)",
-             "mrf": ", whole sector rotating at 300 rad/s (MRF, full-block Jacobian)"}[frame]
+             "mrf": ", whole sector rotating at 300 rad/s (MRF)"}[frame]
     return {"metric": "harmonic-balance pseudo-steps/s (7 instants)", "value": round(rate["mixed"], 4),
             "unit": "pseudo-steps/s", "n_gpus": 1, "steps": steps, "warmup": max(warmup, 3),
             "ms_per_step": round(1e3 / rate["mixed"], 3), "higher_is_better": True, "scaling": "weak",

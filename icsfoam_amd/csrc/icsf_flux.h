@@ -220,15 +220,17 @@ DF void fluxJacobianT(C gI, C ux, C uy, C uz, C E, C nx, C ny, C nz, C A[25]) {
 }
 
 // Entries of one off-diagonal block from its generators (genOff layout):
-// h A(U, E, n), diagonal entries then minus v and minus sv, as the
-// Jacobian kernel forms the stored block.
+// h A(U, E, n), diagonal entries then minus the frame divergence term mo
+// (hasMo: a frame zone), minus v and minus sv, as the Jacobian kernel forms
+// the stored block.
 template <typename C>
-DF void blockFromGen(const C g[10], C gamma, C b[25]) {
+DF void blockFromGen(const C g[10], C gamma, C b[25], C mo = (C)0, bool hasMo = false) {
     fluxJacobianT<C>(gamma, g[0], g[1], g[2], g[3], g[4], g[5], g[6], b);
 #pragma unroll
     for (int e = 0; e < 25; ++e) {
         C x = g[7] * b[e];
         if (e == 0 || e == 6 || e == 12 || e == 18 || e == 24) {
+            if (hasMo) x = x - mo;
             x = x - g[8];
             x = x - g[9];
         }

@@ -51,7 +51,10 @@ __device__ __forceinline__ void slotBlock(const MeshView& m, const TB* __restric
 #pragma unroll
         for (int e = 0; e < NGEN; ++e)
             g[e] = NT ? __builtin_nontemporal_load(blk + gb + (size_t)e * 64) : blk[gb + (size_t)e * 64];
-        blockFromGen<TB>(g, (TB)m.gamma, t);
+        const TB* gm = (const TB*)m.genMo;
+        TB mo = (TB)0;
+        if (gm) mo = NT ? __builtin_nontemporal_load(gm + sk * 64 + lane) : gm[sk * 64 + lane];
+        blockFromGen<TB>(g, (TB)m.gamma, t, mo, gm != nullptr);
 #pragma unroll
         for (int e = 0; e < NB; ++e) b[e] = (A)t[e];
     } else {

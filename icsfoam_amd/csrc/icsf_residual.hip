@@ -859,6 +859,8 @@ __device__ __forceinline__ void cellJacobianBody(const MeshView& m, const Phys& 
             const double gv[NGEN] = {t[bs], t[bs + 1], t[bs + 2], t[bs + 3], nrm.x, nrm.y, nrm.z, hb, v, sv};
 #pragma unroll
             for (int e = 0; e < NGEN; ++e) stnt(blk + gb + (size_t)e * 64, (TB)gv[e]);
+            // the frame divergence term beside them (a frame zone)
+            if (mrf && m.genMo) stnt((TB*)m.genMo + (rowSlot0 + slot) * 64 + lane, (TB)mo);
         } else {
             fluxJacobian(g, V3{t[bs], t[bs + 1], t[bs + 2]}, t[bs + 3], nrm, A);
             const size_t base = ((rowSlot0 + slot) * NB) * 64 + lane;

@@ -188,6 +188,7 @@ struct icsf_case {
     std::vector<char> mrfIncl;
     std::vector<double> mrfUb;
     double *d_umrf = nullptr, *d_omC = nullptr;
+    void* d_genMo = nullptr;  // the frame divergence term per slot (generator storage)
     MeshView mv{};
     StateView sv{};
     // residual adjoint (allocated on first use)
@@ -1910,8 +1911,8 @@ FramePart rotCombined(const std::vector<FramePart>& zones) {
 // of the translating one, the moving faces' U (the translating zone's
 // after the rotating ones', outerLoop.H:6-7) written into their fixed
 // values; Omega from the rotating zones.  The frame divergence adds a
-// diagonal term to every off-diagonal block that the generator form does
-// not carry, so a case with generators switches to full blocks.
+// diagonal term to every off-diagonal block: with generator storage it is
+// kept as one more value per slot (MeshView::genMo).
 int frameCompose(icsf_case* c) {
     const HostMesh& h = c->mesh->h;
     const int nF = h.nFaces, nI = h.nInternal, nB = nF - nI;
@@ -1929,18 +1930,16 @@ int frameCompose(icsf_case* c) {
     HIPCHK(hipStreamSynchronize(c->st));
     if (!c->d_umrf) RC(c->buf.alloc(c->d_umrf, (size_t)nF));
     HIPCHK(hipMemcpy(c->d_umrf, um.data(), (size_t)nF * sizeof(double), hipMemcpyHostToDevice));
-    if (c->blkGen) {
+    if (c->blkGen && !c->d_genMo) {
+        // generator storage keeps its ten generators per slot; the frame
+        // divergence's diagonal term goes beside them, one value per slot
         const size_t slots = (size_t)c->mesh->sell.slicePtr.back() * SLICE;
-        const size_t nBlk = slots * NB + (size_t)c->mesh->view.nCyc * NB;
         const size_t eb = c->blkFp32 ? 4 : 8;
         char* p;
-        RC(c->buf.alloc(p, nBlk * eb));
-        HIPCHK(hipMemset(p, 0, nBlk * eb));
-        c->buf.release(c->blk);
-        c->blk = p;
-        c->blkGen = false;
-        c->mv.blkGen = 0;
-        c->mv.cycBase = slots * NB;
+        RC(c->buf.alloc(p, slots * eb));
+        HIPCHK(hipMemset(p, 0, slots * eb));
+        c->d_genMo = p;
+        c->mv.genMo = p;
     }
     c->mv.umrf = c->d_umrf;
     c->mv.omC = R.on ? c->d_omC : nullptr;

@@ -95,6 +95,12 @@ struct MeshView {
     // harmonic balance with phase-lagged cyclic patches: the lagged side of
     // every cyclic face (-1: a plain cyclic face); null without any
     const int* plSide = nullptr;  // [nCyc]
+    // rotating or translating zone with generator storage: the frame
+    // divergence's diagonal term of every off-diagonal block (one value per
+    // sliced-ELL slot, in the generators' precision, slot * 64 + lane);
+    // the same for every harmonic-balance instance (geometry and frame
+    // only); null without a zone
+    void* genMo = nullptr;
 };
 
 // Lagged far-side record of a phase-lagged cyclic face (PLW doubles per

@@ -317,8 +317,8 @@ int icsf_get_tuning(icsf_case* c, int* params);
  * their U condition must be fixedValue and its value becomes
  * Omega ^ (Cf - origin).  Call after the boundary conditions and before the
  * fields.  The frame divergence puts a diagonal term on every off-diagonal
- * block, which the generator form does not hold: a case with
- * jacobianStorage generators switches to full blocks here.  Each call adds
+ * block; with jacobianStorage generators it is stored as one more value
+ * per slot beside the ten generators.  Each call adds
  * the zone list's next entry: as MRFCoupledZoneList (MRFCoupledZoneList.C:
  * 55-142), every zone in call order assigns its faces' um, its cells' Omega
  * and its rotating faces' U, so a later zone overrides an earlier one where
@@ -331,7 +331,7 @@ int icsf_set_mrf(icsf_case* c, int nZoneCells, const int* zoneCells, const doubl
  * the rotating zone; its faces add frameVelocity . Sf/|Sf| to the face
  * velocity the schemes see, its moving patches (fixedValue U) take the
  * frame velocity (after the rotating zones'), no Coriolis term.  One per
- * case, beside any rotating zones; the same storage switch and VJP refusal
+ * case, beside any rotating zones; the same storage and VJP refusal
  * as icsf_set_mrf. */
 int icsf_set_translating_frame(icsf_case* c, int nZoneCells, const int* zoneCells, const double* velocity,
                                int nNonMoving, const int* nonMovingPatches);

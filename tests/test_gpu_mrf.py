@@ -14,8 +14,8 @@ second order):
 * 5 pseudo-steps: first-step residuals 1e-12, histories and fields 1e-8
   (FP64 form); the mixed form within 1e-3 of the FP64 oracle; a replayed
   hipGraph step equals the eager step bit for bit;
-* a case asking for generator storage switches to full blocks and gives
-  the same bits;
+* a case with generator storage (the frame divergence stored per slot
+  beside the ten generators) gives the same bits as full-block storage;
 * set-up errors: a rotating patch without a fixedValue U condition, the
   residual VJP.
 """
@@ -100,6 +100,8 @@ CASES = {
     "box_translating": lambda: _translating(False),
     "box_translating_and_rotating": lambda: _translating(True),
     "box_roe_viscous": lambda: _box("ROE", rotating_walls=False),
+    "box_roe_viscous_generators": lambda: dataclasses.replace(_box("ROE", rotating_walls=False),
+                                                              fvSolution=_sol(jacobianStorage="generators")),
     "box_ausm_rotating_walls": lambda: _box("AUSMPlusUp", rotating_walls=True),
     "box_hllc_upwind": lambda: _box("HLLC", rotating_walls=True, recon="upwind", mu=0.0),
     "sector_roe": _sector,
@@ -165,8 +167,8 @@ def test_mrf_graph_replay_and_mixed_form():
 def test_mrf_set_up_errors():
     c = _box("ROE", rotating_walls=False)
     print("[mrf errors]", flush=True)
-    # generator storage cannot carry the frame terms: the case switches to
-    # full blocks, with the same results
+    # generator storage carries the frame divergence per slot: the same
+    # results as full blocks
     gg = dbnsCase.from_case(dataclasses.replace(c, fvSolution=_sol(jacobianStorage="generators")))
     gb = dbnsCase.from_case(c)
     np.testing.assert_array_equal(gg.pseudo_steps(3), gb.pseudo_steps(3))
