@@ -352,7 +352,7 @@ def bench_bump(steps=50):
                              "sample": f"oracle (fast build) on the same case, the same {steps} steps"}}
 
 
-def bench_hb(steps, warmup, frame="none"):
+def bench_hb(steps, warmup, frame="none", mu=0.0):
     """Harmonic-balance pseudo-steps/s on a 64x64x32 annular sector with
     rotational cyclic theta patches and 7 instants (3 harmonics of 100 Hz),
     Roe inviscid, LU-SGS.  value: the mixed form (outer FGMRES(1) on the
@@ -360,13 +360,14 @@ def bench_hb(steps, warmup, frame="none"):
     hipGraph-captured steps); the FP64 form (GMRES(30), CGS) beside it.
     frame: "phase-lag" makes the cyclic pair phase-lagged (inter-blade
     phase angle 2 pi/9, cylCoords); "mrf" lets the whole sector rotate about
-    its axis at 300 rad/s (every physical patch non-rotating)."""
+    its axis at 300 rad/s (every physical patch non-rotating).  mu > 0:
+    viscous (the viscous terms across the cyclic faces included)."""
     from icsfoam_amd import cases
     from icsfoam_amd.mesh import prepare_case
     from icsfoam_amd.solver import hbCase
 
     t0 = time.time()
-    insts, D, t = cases.case_hb(n=(64, 64, 32), n_harmonics=3, flux="ROE", orthogonalisation="CGS")
+    insts, D, t = cases.case_hb(n=(64, 64, 32), n_harmonics=3, flux="ROE", orthogonalisation="CGS", mu=mu)
     m = insts[0].mesh
     if frame == "phase-lag":
         lo = next(i for i, p in enumerate(m.patches) if p.kind == cases.PATCH_CYCLIC)
@@ -404,7 +405,8 @@ def bench_hb(steps, warmup, frame="none"):
             "vs_baseline": None, "dtype": "f64 residual/outer + f32 Jacobian/inner",
             "data": "synthetic (seeded annular sector, periodic inflow)",
             "config": {"workload": f"64x64x32 annular sector with rotational cyclic patches ({nC} cells) x 7 "
-                                   f"instants ({7 * nC} block rows), Roe inviscid, LU-SGS, hipGraph steps{fdesc}",
+                                   f"instants ({7 * nC} block rows), Roe "
+                                   f"{'viscous (mu %g)' % mu if mu > 0 else 'inviscid'}, LU-SGS, hipGraph steps{fdesc}",
                        "mixed": "FGMRES(1) FP64 outer + FP32 GMRES(30, CGS) inner",
                        "fp64": "GMRES(30, CGS)", "fp64_steps_per_s": round(rate["fp64"], 4),
                        "mixed_over_fp64": round(rate["mixed"] / rate["fp64"], 3)}}
@@ -616,6 +618,7 @@ def main():
     ap.add_argument("--forms", default="fp64,mixed", help="converge: forms to run, in order")
     ap.add_argument("--frame", default="none", choices=["none", "phase-lag", "mrf"],
                     help="hb: a phase-lagged cyclic pair or a rotating sector")
+    ap.add_argument("--mu", type=float, default=0.0, help="hb: viscosity (0: inviscid)")
     ap.add_argument("--time-limit", type=float, default=400.0, help="converge: seconds per form")
     ap.add_argument("--workload", default="c5",
                     choices=["c5", "hb", "vjp", "c5vjp", "converge", "bump", "hbconverge"],
@@ -629,7 +632,7 @@ def main():
               "supported (multi-GPU sharding is designed, not built)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "hb":
-        print(json.dumps(bench_hb(args.steps, args.warmup, args.frame)), flush=True)
+        print(json.dumps(bench_hb(args.steps, args.warmup, args.frame, args.mu)), flush=True)
         return
     if args.workload == "hbconverge":
         print(json.dumps(bench_hb_converge(max(args.steps, 25), time_limit=args.time_limit)), flush=True)
