@@ -412,7 +412,7 @@ def bench_hb(steps, warmup, frame="none", mu=0.0):
                        "mixed_over_fp64": round(rate["mixed"] / rate["fp64"], 3)}}
 
 
-def bench_vjp(calls):
+def bench_vjp(calls, flux="AUSMPlusUp"):
     """Residual VJP (inviscid AUSM+up, 2nd order) at 100^3: ms per call with
     device-resident W and v (torch tensors)."""
     import torch
@@ -421,7 +421,7 @@ def bench_vjp(calls):
     from icsfoam_amd.adjoint import residual_device, residual_vjp_device
     from icsfoam_amd.mesh import prepare_case
 
-    c = cases.case_box(n=(100, 100, 100), lengths=(1.0, 1.0, 1.0), jitter=0.1, seed=7, flux="AUSMPlusUp", mu=0.0)
+    c = cases.case_box(n=(100, 100, 100), lengths=(1.0, 1.0, 1.0), jitter=0.1, seed=7, flux=flux, mu=0.0)
     c = prepare_case(c, permute_seed=2024)
     g = dbnsCase.from_case(c)
     W = torch.tensor(g.fields()["W"], device="cuda")
@@ -442,7 +442,7 @@ def bench_vjp(calls):
             "unit": "calls/s", "n_gpus": 1, "steps": calls, "warmup": 3, "ms_per_step": round(1e3 * dt, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (seeded jittered 100^3 box)",
-            "config": {"workload": "100^3 jittered box (1000000 cells), inviscid AUSM+up, linearUpwind + "
+            "config": {"workload": f"100^3 jittered box (1000000 cells), inviscid {flux}, linearUpwind + "
                                    "cellLimited<Venkatakrishnan>; gW, gSf, gCf per call",
                        "residual_ms": round(1e3 * dr, 3)}}
 
@@ -619,6 +619,8 @@ def main():
     ap.add_argument("--frame", default="none", choices=["none", "phase-lag", "mrf"],
                     help="hb: a phase-lagged cyclic pair or a rotating sector")
     ap.add_argument("--mu", type=float, default=0.0, help="hb: viscosity (0: inviscid)")
+    ap.add_argument("--flux", default="AUSMPlusUp", choices=["AUSMPlusUp", "ROE", "HLLC"],
+                    help="vjp: the flux scheme")
     ap.add_argument("--time-limit", type=float, default=400.0, help="converge: seconds per form")
     ap.add_argument("--workload", default="c5",
                     choices=["c5", "hb", "vjp", "c5vjp", "converge", "bump", "hbconverge"],
@@ -641,7 +643,7 @@ def main():
         print(json.dumps(bench_bump(max(args.steps, 50))), flush=True)
         return
     if args.workload == "vjp":
-        print(json.dumps(bench_vjp(max(args.steps, 10))), flush=True)
+        print(json.dumps(bench_vjp(max(args.steps, 10), args.flux)), flush=True)
         return
     if args.workload == "c5vjp":
         print(json.dumps(bench_c5_vjp(args.n, 8000, time_limit=args.time_limit)), flush=True)
